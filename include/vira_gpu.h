@@ -1,0 +1,225 @@
+/* vira_gpu.h -- C-ABI of the MI355X (gfx950) ray-tracing hot path.
+ *
+ * This is the drop-in boundary below vira's renderer API.  In the reference
+ * (crgnam/vira v0.8.3) the hot path is reached through
+ *   CPUPathTracer::render(Camera&, Scene&)      include/vira/rendering/cpu_path_tracer.hpp:50
+ * and, below it, the polymorphic acceleration interface
+ *   BLAS::intersect / TLAS::intersect / TLAS::init
+ *                                               include/vira/rendering/acceleration/blas.hpp:20-37
+ *                                               include/vira/rendering/acceleration/tlas.hpp:19-31
+ * whose working implementation is Embree (embree_blas.ipp:14-71,
+ * embree_tlas.ipp:77-173).  Every entry point below names the reference
+ * interface it replaces.  Plain pointers and sizes only; no exceptions cross
+ * this boundary: each call returns a status and vira_gpu_last_error() gives
+ * the message (thread-local).  The C++ wrapper (include/vira_gpu.hpp) turns
+ * non-OK statuses into std::runtime_error, as the reference throws
+ * (vira_tlas.ipp:33, cpu_path_tracer.ipp:121).
+ *
+ * Pointers: arguments named *_host are host memory; arguments named *_dev are
+ * device memory (hipMalloc / torch CUDA tensors) on the context's device.
+ * Every call is enqueued on the context's stream and is synchronous unless
+ * stated otherwise.
+ */
+#ifndef VIRA_GPU_H
+#define VIRA_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VIRA_GPU_NUM_BINS 8 /* SpectralData<8, UNIFORM_RANGE, 400, 1000> */
+#define VIRA_GPU_MAX_LIGHTS 8
+
+typedef enum {
+    VIRA_GPU_OK = 0,
+    VIRA_GPU_ERR_INVALID = 1,     /* bad argument (null pointer, size, id)      */
+    VIRA_GPU_ERR_HIP = 2,         /* HIP runtime error (launch, fault, copy)    */
+    VIRA_GPU_ERR_OOM = 3,         /* device allocation failed                   */
+    VIRA_GPU_ERR_STATE = 4,       /* call order (e.g. render before BLAS build) */
+    VIRA_GPU_ERR_UNSUPPORTED = 5  /* valid request this build does not handle   */
+} vira_gpu_status;
+
+typedef struct vira_gpu_ctx vira_gpu_ctx;
+
+typedef struct {
+    int device;   /* HIP device ordinal */
+    void* stream; /* hipStream_t to enqueue on; NULL = a stream owned by the context */
+} vira_gpu_opts;
+
+/* DEM grid -> world projection.  kind 0: flat grid x = (i-cx)s, y = (cy-j)s,
+ * z = h (the non-OGR branch of DEMProjection::transformToWorld,
+ * dem_projection.ipp:130-141).  kind 1: gnomonic tile on a sphere of the
+ * given radius centred on +x of the body frame.  Positions are formed in FP64
+ * and `offset` (the rebasing origin, e.g. the camera position) is subtracted
+ * before the FP32 store. */
+typedef struct {
+    int kind;
+    double spacing;
+    double radius;
+    double center[3];
+    double offset[3];
+} vira_gpu_projection;
+
+typedef struct {
+    int max_leaf;   /* triangles per leaf, 1..15 (default 4)  */
+    int bins;       /* SAH bins per node, 4..32 (default 16)  */
+} vira_gpu_bvh_opts;
+
+typedef struct {
+    uint64_t num_nodes;      /* internal nodes of the binary tree      */
+    uint64_t num_leaves;
+    uint32_t max_depth;
+    uint32_t max_leaf_size;
+    double sah_cost;         /* sum over internal nodes of area/root area, plus leaf tris*area/root area */
+    double build_ms;         /* device time of the build, HIP events   */
+    uint64_t bytes;          /* device bytes of nodes + leaf triangles */
+} vira_gpu_bvh_stats;
+
+/* Light: kind 0 = PointLight (point_light.ipp:10-38), value = spectral
+ * intensity P/(4 pi); kind 1 = SphereLight (sphere_light.ipp:16-125), value =
+ * spectral radiance.  Position in the rebased frame (FP64). */
+typedef struct {
+    int kind;
+    double position[3];
+    float radius;
+    float value[VIRA_GPU_NUM_BINS];
+} vira_gpu_light;
+
+/* Lambertian material (lambertian.ipp:9-15).  Albedo = vertex albedo x
+ * albedo[k] x grey texture (bilinear, Image::sampleUVs, image.ipp:589-670). */
+typedef struct {
+    float albedo[VIRA_GPU_NUM_BINS];
+    const float* texture_dev; /* row-major i + j*W grey texture, or NULL */
+    int tex_width;
+    int tex_height;
+    int smooth_shading;       /* Mesh::smoothShading (mesh.hpp:144) */
+} vira_gpu_material;
+
+/* Pinhole camera: kinv is GLM mat3x2 K^-1 stored [col][row] (camera.ipp:1526-1568),
+ * rot row-major with columns = camera x, y, z axes (camera.ipp:1455-1500).
+ * position is in the rebased frame; z_dir = +1 (or -1 for Blender frames). */
+typedef struct {
+    int width;
+    int height;
+    float kinv[6];
+    float rot[9];
+    double position[3];
+    float z_dir;
+} vira_gpu_camera;
+
+/* CPUPathTracerOptions (cpu_path_tracer.hpp:30-43) plus the new controls. */
+typedef struct {
+    int samples;         /* spp */
+    int bounces;
+    int russian_roulette;
+    int rr_start;        /* first bounce whose continuation is subject to RR */
+    uint32_t seed;
+} vira_gpu_render_opts;
+
+/* Render passes (passes.hpp:51-95); every pointer optional, device memory,
+ * pixel index i + j*W. */
+typedef struct {
+    float* radiance_dev;   /* W*H*8 total radiance            */
+    float* direct_dev;     /* W*H*8 bounce-0 radiance         */
+    float* indirect_dev;   /* W*H*8 total - direct            */
+    float* albedo_dev;     /* W*H*8 mean first-hit albedo     */
+    float* normal_dev;     /* W*H*3 normalised first-hit normal */
+    float* depth_dev;      /* W*H mean first-hit distance (inf = no hit) */
+    float* alpha_dev;      /* W*H hits / spp                  */
+    uint32_t* tri_id_dev;  /* W*H first-hit triangle (0xffffffff = none) */
+} vira_gpu_outputs;
+
+typedef struct {
+    uint64_t primary_rays;
+    uint64_t shadow_rays;
+    uint64_t bounce_rays;
+    double total_ms;         /* device time of the whole render (HIP events) */
+    double trace_closest_ms; /* sum over closest-hit traversal launches */
+    double trace_shadow_ms;  /* sum over any-hit traversal launches */
+    double shade_ms;         /* sum over shading + accumulation launches */
+    uint32_t trace_closest_launches;
+    uint32_t trace_shadow_launches;
+} vira_gpu_render_stats;
+
+/* --- context ------------------------------------------------------------ */
+vira_gpu_status vira_gpu_create(vira_gpu_ctx** ctx, const vira_gpu_opts* opts);
+vira_gpu_status vira_gpu_destroy(vira_gpu_ctx* ctx);
+vira_gpu_status vira_gpu_set_stream(vira_gpu_ctx* ctx, void* stream);
+vira_gpu_status vira_gpu_synchronize(vira_gpu_ctx* ctx);
+const char* vira_gpu_last_error(void);
+const char* vira_gpu_version(void);
+
+/* --- synthetic inputs (value-noise fBm DEM, per-vertex albedo) ------------ */
+vira_gpu_status vira_gpu_synth_fbm(vira_gpu_ctx* ctx, int width, int height, uint32_t seed,
+                                   double amplitude, int octaves, double base_cells, float* heights_dev);
+vira_gpu_status vira_gpu_synth_vertex_albedo(vira_gpu_ctx* ctx, size_t num_vertices, uint32_t seed,
+                                             float* albedo_dev);
+
+/* --- meshes (a1) ---------------------------------------------------------- */
+/* DEM::makeMesh (dem.ipp:138-211) + imageToIndexBuffer (image_utils.ipp:364-412)
+ * + Mesh::calculateNormals (mesh.ipp:199-233), on the GPU.  heights_dev is
+ * W*H row-major (i + j*W); grey_albedo_dev is W*H or NULL (albedo 1).
+ * UVs are (i/(W-1), j/(H-1)). */
+vira_gpu_status vira_gpu_dem_mesh(vira_gpu_ctx* ctx, const float* heights_dev, int width, int height,
+                                  const vira_gpu_projection* proj, const float* grey_albedo_dev,
+                                  int32_t* mesh_id);
+/* Mesh(vertexBuffer, indexBuffer) (mesh.ipp:25-83) from host buffers.
+ * albedo_host: nv*8 or NULL (1); uv_host: nv*2 or NULL (0); normals_host:
+ * nv*3 or NULL (computed as Mesh::calculateNormals). */
+vira_gpu_status vira_gpu_mesh_upload(vira_gpu_ctx* ctx, const float* positions_host, size_t num_vertices,
+                                     const uint32_t* indices_host, size_t num_triangles,
+                                     const float* albedo_host, const float* uv_host,
+                                     const float* normals_host, int32_t* mesh_id);
+vira_gpu_status vira_gpu_mesh_info(vira_gpu_ctx* ctx, int32_t mesh_id, size_t* num_vertices,
+                                   size_t* num_triangles);
+/* Copies the mesh back (positions nv*3, indices nt*3, normals nv*3); any may be NULL. */
+vira_gpu_status vira_gpu_mesh_download(vira_gpu_ctx* ctx, int32_t mesh_id, float* positions_host,
+                                       uint32_t* indices_host, float* normals_host);
+vira_gpu_status vira_gpu_mesh_release(vira_gpu_ctx* ctx, int32_t mesh_id);
+
+/* --- acceleration (a2, a3) ------------------------------------------------ */
+/* Mesh::buildBVH -> EmbreeBLAS (mesh.ipp:396-418, embree_blas.ipp:14-71):
+ * binned-SAH build on the GPU. opts may be NULL. */
+vira_gpu_status vira_gpu_blas_build(vira_gpu_ctx* ctx, int32_t mesh_id, const vira_gpu_bvh_opts* opts);
+vira_gpu_status vira_gpu_blas_stats(vira_gpu_ctx* ctx, int32_t mesh_id, vira_gpu_bvh_stats* stats);
+
+typedef struct {
+    int32_t mesh_id;
+    double transform[16]; /* column-major 4x4 model matrix, FP64 */
+} vira_gpu_instance;
+
+/* Scene::buildTLAS -> EmbreeTLAS::newInstance/build (scene.ipp:712-812,
+ * embree_tlas.ipp:77-115).  This build renders one instance with an identity
+ * transform; other transforms return VIRA_GPU_ERR_UNSUPPORTED. */
+vira_gpu_status vira_gpu_tlas_build(vira_gpu_ctx* ctx, const vira_gpu_instance* instances_host, size_t n);
+
+/* --- shading state (a7) --------------------------------------------------- */
+/* Scene::light_cache_ (scene.hpp:263-285). */
+vira_gpu_status vira_gpu_lights_set(vira_gpu_ctx* ctx, const vira_gpu_light* lights_host, int n);
+/* Mesh::material_cache_ (mesh.hpp:93). */
+vira_gpu_status vira_gpu_material_set(vira_gpu_ctx* ctx, int32_t mesh_id, const vira_gpu_material* mat);
+
+/* --- rendering (a4-a9) ---------------------------------------------------- */
+/* CPUPathTracer::render (cpu_path_tracer.ipp:29-161) over the current TLAS. */
+vira_gpu_status vira_gpu_render(vira_gpu_ctx* ctx, const vira_gpu_camera* cam, const vira_gpu_render_opts* opts,
+                                const vira_gpu_outputs* out, vira_gpu_render_stats* stats);
+
+/* Parity hook: Scene::intersect -> TLAS::intersect (scene.hpp:230-233,
+ * embree_tlas.ipp:128-173) for a batch of rays against one mesh's BLAS.
+ * rays_dev: n x 8 floats (ox oy oz 0 dx dy dz tmax); any_hit != 0 -> occlusion
+ * within [0, tmax] (t = 0 where occluded, +inf otherwise).  Outputs n floats t
+ * (+inf = miss), n uint32 triangle ids (0xffffffff = miss), n floats b1, b2
+ * (Embree barycentrics u, v); any output may be NULL.  kernel_ms (may be NULL)
+ * receives the traversal kernel's device time. */
+vira_gpu_status vira_gpu_trace_rays(vira_gpu_ctx* ctx, int32_t mesh_id, const float* rays_dev, size_t n,
+                                    int any_hit, float* t_dev, uint32_t* tri_id_dev, float* b1_dev,
+                                    float* b2_dev, double* kernel_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VIRA_GPU_H */

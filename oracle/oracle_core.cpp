@@ -1,0 +1,519 @@
+// oracle_core.cpp -- BVH build and integrator of the CPU oracle.
+// TEST INFRASTRUCTURE ONLY (see oracle_core.hpp).
+#include "oracle_core.hpp"
+
+#include <atomic>
+#include <thread>
+
+namespace vo {
+
+// ============================================================== BVH build
+// ViraBLAS::subdivide / findBestSplitPlane (vira_blas.ipp:132-254).  Subtrees
+// below the top levels are built on worker threads into private node arrays
+// and spliced afterwards; the tree is identical to the sequential build.
+namespace {
+
+struct Builder {
+    const Mesh& m;
+    std::vector<double> tlo, thi, cen;  // per-triangle bounds and centroid (FP64)
+    std::vector<uint32_t>& triIdx;
+    Builder(const Mesh& mesh, std::vector<uint32_t>& ti) : m(mesh), triIdx(ti) {
+        tlo.resize(3 * m.nt); thi.resize(3 * m.nt); cen.resize(3 * m.nt);
+        for (uint32_t t = 0; t < (uint32_t)m.nt; ++t) bvh_tri_bounds(m, t, &tlo[3 * t], &thi[3 * t], &cen[3 * t]);
+    }
+
+    void bounds(BNode& n) const {  // updateNodeBounds (vira_blas.ipp:117-129)
+        for (int k = 0; k < 3; ++k) { n.lo[k] = INFINITY; n.hi[k] = -INFINITY; }
+        for (uint32_t i = 0; i < n.triCount; ++i) {
+            uint32_t t = triIdx[n.leftFirst + i];
+            for (int k = 0; k < 3; ++k) {
+                n.lo[k] = std::min(n.lo[k], tlo[3 * t + k]);
+                n.hi[k] = std::max(n.hi[k], thi[3 * t + k]);
+            }
+        }
+    }
+
+    static double area(const double lo[3], const double hi[3]) {  // aabb.ipp:34-38
+        double ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+        return ex * ey + ey * ez + ez * ex;
+    }
+
+    // findBestSplitPlane (vira_blas.ipp:185-254)
+    double best_split(const BNode& n, int& axis, double& splitPos) const {
+        double bestCost = std::numeric_limits<double>::max();
+        int a = 0;
+        double ext[3] = {n.hi[0] - n.lo[0], n.hi[1] - n.lo[1], n.hi[2] - n.lo[2]};
+        if (ext[1] > ext[0]) a = 1;
+        if (ext[2] > ext[a]) a = 2;
+        double bmin = INFINITY, bmax = -INFINITY;
+        for (uint32_t i = 0; i < n.triCount; ++i) {
+            uint32_t t = triIdx[n.leftFirst + i];
+            bmin = std::min(bmin, cen[3 * t + a]);
+            bmax = std::max(bmax, cen[3 * t + a]);
+        }
+        constexpr int BINS = 8;
+        struct Bin { double lo[3], hi[3]; uint32_t cnt; } bin[BINS];
+        for (auto& b : bin) { for (int k = 0; k < 3; ++k) { b.lo[k] = INFINITY; b.hi[k] = -INFINITY; } b.cnt = 0; }
+        double scale = BINS / (bmax - bmin);
+        for (uint32_t i = 0; i < n.triCount; ++i) {
+            uint32_t t = triIdx[n.leftFirst + i];
+            double f = (cen[3 * t + a] - bmin) * scale;
+            size_t bi = (f >= (double)BINS || !(f == f)) ? BINS - 1 : std::min<size_t>(BINS - 1, (size_t)f);
+            bin[bi].cnt++;
+            for (int k = 0; k < 3; ++k) {
+                bin[bi].lo[k] = std::min(bin[bi].lo[k], tlo[3 * t + k]);
+                bin[bi].hi[k] = std::max(bin[bi].hi[k], thi[3 * t + k]);
+            }
+        }
+        double leftArea[BINS - 1], rightArea[BINS - 1];
+        size_t leftCount[BINS - 1], rightCount[BINS - 1];
+        double llo[3] = {INFINITY, INFINITY, INFINITY}, lhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        double rlo[3] = {INFINITY, INFINITY, INFINITY}, rhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        size_t ls = 0, rs = 0;
+        for (int i = 0; i < BINS - 1; ++i) {
+            ls += bin[i].cnt;
+            leftCount[i] = ls;
+            if (!std::isinf(bin[i].lo[0]))
+                for (int k = 0; k < 3; ++k) { llo[k] = std::min(llo[k], bin[i].lo[k]); lhi[k] = std::max(lhi[k], bin[i].hi[k]); }
+            leftArea[i] = area(llo, lhi);
+            rs += bin[BINS - 1 - i].cnt;
+            rightCount[BINS - 2 - i] = rs;
+            const Bin& rb = bin[BINS - 1 - i];
+            if (!std::isinf(rb.lo[0]))
+                for (int k = 0; k < 3; ++k) { rlo[k] = std::min(rlo[k], rb.lo[k]); rhi[k] = std::max(rhi[k], rb.hi[k]); }
+            rightArea[BINS - 2 - i] = area(rlo, rhi);
+        }
+        scale = (bmax - bmin) / BINS;
+        for (int i = 0; i < BINS - 1; ++i) {
+            double cost = (double)leftCount[i] * leftArea[i] + (double)rightCount[i] * rightArea[i];
+            if (cost < bestCost) {
+                axis = a;
+                splitPos = bmin + scale * (double)(i + 1);
+                bestCost = cost;
+            }
+        }
+        return bestCost;
+    }
+
+    // Builds the subtree rooted at local node 0 of `out` (already holding the
+    // root with leftFirst/triCount/bounds set).  Child pairs are adjacent.
+    void subdivide(std::vector<BNode>& out, uint32_t ni, int depth, int par_depth) {
+        BNode n = out[ni];
+        if (n.triCount <= 2) return;
+        int axis = 0;
+        double splitPos = 0;
+        double cost = best_split(n, axis, splitPos);
+        double ex = n.hi[0] - n.lo[0], ey = n.hi[1] - n.lo[1], ez = n.hi[2] - n.lo[2];
+        double nodeCost = (double)n.triCount * (ex * ey + ey * ez + ez * ex);  // vira_blas.ipp:287-292
+        if (cost >= nodeCost) return;
+        // in-place partition (vira_blas.ipp:149-159)
+        int64_t i = n.leftFirst, j = i + n.triCount - 1;
+        while (i <= j) {
+            if (cen[3 * triIdx[i] + axis] < splitPos) i++;
+            else std::swap(triIdx[i], triIdx[j--]);
+        }
+        uint32_t leftCount = (uint32_t)(i - n.leftFirst);
+        if (leftCount == 0 || leftCount == n.triCount) return;
+        uint32_t l = (uint32_t)out.size();
+        BNode L{}, R{};
+        L.leftFirst = n.leftFirst; L.triCount = leftCount;
+        R.leftFirst = (uint32_t)i; R.triCount = n.triCount - leftCount;
+        bounds(L); bounds(R);
+        out.push_back(L); out.push_back(R);
+        out[ni].leftFirst = l;
+        out[ni].triCount = 0;
+        if (depth < par_depth) {
+            // build both children as independent subtrees, the right one on a thread
+            std::vector<BNode> lsub{out[l]}, rsub{out[l + 1]};
+            std::thread th([&] { subdivide(rsub, 0, depth + 1, par_depth); });
+            subdivide(lsub, 0, depth + 1, par_depth);
+            th.join();
+            splice(out, l, lsub);
+            splice(out, l + 1, rsub);
+        } else {
+            subdivide(out, l, depth + 1, par_depth);
+            subdivide(out, l + 1, depth + 1, par_depth);
+        }
+    }
+
+    // Copies a subtree built in a private array (root at index 0) into `out`,
+    // with its root replacing out[at].
+    static void splice(std::vector<BNode>& out, uint32_t at, const std::vector<BNode>& sub) {
+        uint32_t base = (uint32_t)out.size();
+        // sub[0] -> out[at]; sub[k>0] -> out[base + k - 1]
+        auto map = [&](uint32_t k) { return k == 0 ? at : base + k - 1; };
+        out.resize(base + sub.size() - 1);
+        for (uint32_t k = 0; k < sub.size(); ++k) {
+            BNode c = sub[k];
+            if (c.triCount == 0) c.leftFirst = map(c.leftFirst);
+            out[map(k)] = c;
+        }
+    }
+};
+
+}  // namespace
+
+BVH bvh_build(const Mesh& m, int nthreads) {
+    BVH b;
+    b.triIdx.resize(m.nt);
+    for (uint32_t i = 0; i < (uint32_t)m.nt; ++i) b.triIdx[i] = i;
+    if (m.nt == 0) return b;
+    Builder bd(m, b.triIdx);
+    b.nodes.reserve(2 * m.nt);
+    BNode root{};
+    root.leftFirst = 0;
+    root.triCount = (uint32_t)m.nt;
+    bd.bounds(root);
+    b.nodes.push_back(root);
+    int par_depth = 0;
+    while ((1 << par_depth) < nthreads && par_depth < 6) par_depth++;
+    if (m.nt < 100000) par_depth = 0;
+    bd.subdivide(b.nodes, 0, 0, par_depth);
+    b.flo.resize(3 * b.nodes.size());
+    b.fhi.resize(3 * b.nodes.size());
+    for (size_t n = 0; n < b.nodes.size(); ++n)
+        for (int k = 0; k < 3; ++k) {
+            b.flo[3 * n + k] = (float)b.nodes[n].lo[k];  // exact: bounds of FP32 positions
+            b.fhi[3 * n + k] = (float)b.nodes[n].hi[k];
+        }
+    return b;
+}
+
+// ============================================================== integrator
+namespace {
+
+template <class R> struct Consts;
+template <> struct Consts<float> {
+    static constexpr float pi = 3.14159265358979323846f;
+    static constexpr float inv_pi = 0.318309886183790671538f;
+};
+template <> struct Consts<double> {
+    static constexpr double pi = 3.14159265358979323846;
+    static constexpr double inv_pi = 0.318309886183790671538;
+};
+
+// offsetIntersection (rendering/ray.ipp:49-139): integer-ULP push along N.
+inline V3<float> offset_point(V3<float> p, V3<float> n) {
+    auto f2i = [](float v) { int32_t o; std::memcpy(&o, &v, 4); return o; };
+    auto i2f = [](int32_t v) { float o; std::memcpy(&o, &v, 4); return o; };
+    int32_t oi[3] = {(int32_t)(256.0f * n.x), (int32_t)(256.0f * n.y), (int32_t)(256.0f * n.z)};
+    float pc[3] = {p.x, p.y, p.z}, nc[3] = {n.x, n.y, n.z}, r[3];
+    for (int k = 0; k < 3; ++k) {
+        float pi = i2f(f2i(pc[k]) + ((pc[k] < 0) ? -oi[k] : oi[k]));
+        r[k] = (std::fabs(pc[k]) < (1.0f / 32.0f)) ? pc[k] + (1.0f / 65536.0f) * nc[k] : pi;
+    }
+    return {r[0], r[1], r[2]};
+}
+inline V3<double> offset_point(V3<double> p, V3<double> n) {
+    auto f2i = [](double v) { int64_t o; std::memcpy(&o, &v, 8); return o; };
+    auto i2f = [](int64_t v) { double o; std::memcpy(&o, &v, 8); return o; };
+    int64_t oi[3] = {(int64_t)(65536.0 * n.x), (int64_t)(65536.0 * n.y), (int64_t)(65536.0 * n.z)};
+    double pc[3] = {p.x, p.y, p.z}, nc[3] = {n.x, n.y, n.z}, r[3];
+    for (int k = 0; k < 3; ++k) {
+        double pi = i2f(f2i(pc[k]) + ((pc[k] < 0) ? -oi[k] : oi[k]));
+        r[k] = (std::fabs(pc[k]) < (1.0 / 64.0)) ? pc[k] + (1.0 / 4294967296.0) * nc[k] : pi;
+    }
+    return {r[0], r[1], r[2]};
+}
+
+template <class R>
+inline R power_heuristic(R fPdf, R gPdf) {  // cpu_path_tracer.ipp:472-478
+    R f = fPdf, g = gPdf;
+    return (f * f) / (f * f + g * g);
+}
+
+// Image::sampleUVs + interpolatePixel (image.ipp:589-670), grey texture.
+template <class R>
+inline R tex_bilinear(const Material& mat, R u, R v) {
+    R rx = (R)mat.tex_w, ry = (R)mat.tex_h;
+    R px = std::fmod(u * rx, rx), py = std::fmod(v * ry, ry);
+    if (px < 0) px += rx;
+    if (py < 0) py += ry;
+    int x0 = (int)std::floor(px), y0 = (int)std::floor(py);
+    x0 = std::min(x0, mat.tex_w - 1);
+    y0 = std::min(y0, mat.tex_h - 1);
+    int x1 = std::min(x0 + 1, mat.tex_w - 1), y1 = std::min(y0 + 1, mat.tex_h - 1);
+    R p00 = mat.tex[x0 + (size_t)y0 * mat.tex_w], p10 = mat.tex[x1 + (size_t)y0 * mat.tex_w];
+    R p01 = mat.tex[x0 + (size_t)y1 * mat.tex_w], p11 = mat.tex[x1 + (size_t)y1 * mat.tex_w];
+    R dx = px - (R)x0, dy = py - (R)y0;
+    return ((((R)1 - dx) * ((R)1 - dy)) * p00 + (dx * ((R)1 - dy)) * p10) + (((R)1 - dx) * dy) * p01 +
+           (dx * dy) * p11;
+}
+
+// Light::sample (point_light.ipp:18-38, sphere_light.ipp:31-125).
+template <class R>
+inline bool sample_light(const Light& L, const V3<R>& p, uint32_t seed, uint32_t pix, uint32_t s,
+                         uint32_t dim0, V3<R>& dir, R& distance, R& pdf, R rad[NB]) {
+    V3<R> c{(R)L.pos[0], (R)L.pos[1], (R)L.pos[2]};
+    V3<R> toL = c - p;
+    distance = length(toL);
+    R dInv = (R)1 / distance;
+    if (L.kind == 0) {
+        R dSq = distance * distance;
+        pdf = 1;
+        dir = dInv * toL;
+        for (int k = 0; k < NB; ++k) rad[k] = (R)L.value[k] / dSq;
+        return true;
+    }
+    R radius = (R)L.radius;
+    if (distance <= radius) {  // inside the sphere: sphere_light.ipp:40-57
+        pdf = (R)1 / ((R)4 * Consts<R>::pi * radius * radius);
+        if (distance < (R)1e-3) dir = {1, 0, 0};
+        else dir = dInv * toL;
+        distance = radius;
+        for (int k = 0; k < NB; ++k) rad[k] = (R)L.value[k];
+        return true;
+    }
+    R r1 = (R)rng_u01(seed, pix, s, dim0), r2 = (R)rng_u01(seed, pix, s, dim0 + 1);
+    V3<R> W = dInv * toL;
+    V3<R> N = (W.x < W.y) ? V3<R>{1, 0, 0} : V3<R>{0, 1, 0};
+    V3<R> Vv = normalize(cross(W, N));
+    V3<R> U = normalize(cross(W, Vv));
+    R phi = (R)2 * Consts<R>::pi * r1;
+    R zeta = r2;
+    R cc = radius / distance;
+    R theta = std::acos(((R)1 - zeta) + zeta * std::sqrt((R)1 - cc * cc));
+    R sinT = std::sin(theta), sinP = std::sin(phi), cosT = std::cos(theta), cosP = std::cos(phi);
+    R a = cosP * sinT, bb = cosT, cz = sinP * sinT;
+    dir = {(a * U.x + bb * W.x) + cz * Vv.x, (a * U.y + bb * W.y) + cz * Vv.y, (a * U.z + bb * W.z) + cz * Vv.z};
+    R r2s = radius * radius;
+    R Omega = (R)2 * Consts<R>::pi * ((R)1 - std::sqrt(distance * distance - r2s) / distance);
+    pdf = (R)1 / Omega;
+    for (int k = 0; k < NB; ++k) rad[k] = (R)L.value[k];
+    return true;
+}
+
+// Light::getPDF (point_light.ipp:40-57, sphere_light.ipp:127-153).
+template <class R>
+inline R light_pdf(const Light& L, const V3<R>& p, const V3<R>& dir) {
+    V3<R> c{(R)L.pos[0], (R)L.pos[1], (R)L.pos[2]};
+    if (L.kind == 0) {
+        V3<R> toL = normalize(c - p);
+        V3<R> q = normalize(dir);
+        R d = dot(toL, q);
+        return (d > (R)1 - (R)1e-6) ? (R)1 : (R)0;
+    }
+    V3<R> toC = c - p;
+    R dc = length(toC);
+    R radius = (R)L.radius;
+    if (dc <= radius) return (R)1 / ((R)4 * Consts<R>::pi * radius * radius);
+    V3<R> tn = toC * ((R)1 / dc);
+    R cosMax = std::sqrt((R)1 - (radius * radius) / (dc * dc));
+    R ca = dot(dir, tn);
+    if (ca < cosMax) return 0;
+    R Omega = (R)2 * Consts<R>::pi * ((R)1 - std::sqrt(dc * dc - radius * radius) / dc);
+    return (R)1 / Omega;
+}
+
+template <class R>
+struct PixelAcc {
+    Kahan<R> rad[NB], dir[NB];
+    R alb[NB] = {0};
+    V3<R> nrm{0, 0, 0};
+    R depth = 0;
+    uint32_t count = 0;
+    uint32_t tri = 0xffffffffu;
+};
+
+template <class R>
+void render_impl(const Mesh& m, const BVH& b, const Camera& cam, const Light* lights, int nl,
+                 const Material& mat, const Options& opt, Outputs& out, int nthreads,
+                 const int* pixel_list, size_t npix) {
+    const size_t total = pixel_list ? npix : (size_t)cam.W * cam.H;
+    std::atomic<size_t> next{0};
+    std::atomic<uint64_t> g_cnt[7];
+    for (auto& c : g_cnt) c = 0;
+    auto vpos = [&](uint32_t v) { return mesh_pos<R>(m, v); };
+    auto vnrm = [&](uint32_t v) { return V3<R>{(R)m.nrm[3 * v], (R)m.nrm[3 * v + 1], (R)m.nrm[3 * v + 2]}; };
+
+    auto worker = [&]() {
+        uint64_t cnt[7] = {0};
+        TravCount tc_closest, tc_any;
+        while (true) {
+            size_t k0 = next.fetch_add(64);
+            if (k0 >= total) break;
+            size_t k1 = std::min(total, k0 + 64);
+            for (size_t kk = k0; kk < k1; ++kk) {
+                int pix = pixel_list ? pixel_list[kk] : (int)kk;
+                int pi = pix % cam.W, pj = pix / cam.W;
+                PixelAcc<R> acc;
+                for (int s = 0; s < opt.spp; ++s) {
+                    // ---- camera ray (cpu_path_tracer.ipp:178-192, camera.ipp:1126-1163,1738-1749)
+                    R X = (R)pi, Y = (R)pj;
+                    if (s > 0) {
+                        X = X + (R)rng_u01(opt.seed, (uint32_t)pix, (uint32_t)s, SLOT_JITTER_X);
+                        Y = Y + (R)rng_u01(opt.seed, (uint32_t)pix, (uint32_t)s, SLOT_JITTER_Y);
+                    }
+                    R hx = ((R)cam.kinv[0] * X + (R)cam.kinv[2] * Y) + (R)cam.kinv[4];
+                    R hy = ((R)cam.kinv[1] * X + (R)cam.kinv[3] * Y) + (R)cam.kinv[5];
+                    V3<R> dl{hx, (R)cam.zdir * hy, (R)cam.zdir};
+                    const float* Rm = cam.rot;
+                    V3<R> dg{((R)Rm[0] * dl.x + (R)Rm[1] * dl.y) + (R)Rm[2] * dl.z,
+                             ((R)Rm[3] * dl.x + (R)Rm[4] * dl.y) + (R)Rm[5] * dl.z,
+                             ((R)Rm[6] * dl.x + (R)Rm[7] * dl.y) + (R)Rm[8] * dl.z};
+                    RayT<R> ray{{(R)cam.pos[0], (R)cam.pos[1], (R)cam.pos[2]}, normalize(dg)};
+                    cnt[0]++;
+                    R beta[NB];
+                    for (int k = 0; k < NB; ++k) beta[k] = 1;
+                    R path[NB] = {0}, direct[NB] = {0};
+                    for (int bounce = 0; bounce <= opt.bounces; ++bounce) {
+                        Hit<R> h = bvh_trace<R>(m, b, ray, false, (R)INFINITY, &tc_closest);
+                        if (h.id == 0xffffffffu) break;
+                        // ---- processIntersection (cpu_path_tracer.ipp:279-469)
+                        uint32_t i0 = m.idx[3 * h.id], i1 = m.idx[3 * h.id + 1], i2 = m.idx[3 * h.id + 2];
+                        V3<R> p0 = vpos(i0), p1 = vpos(i1), p2 = vpos(i2);
+                        R w0 = h.b1, w1 = h.b2, w2 = ((R)1 - h.b1) - h.b2;
+                        R uvu = 0, uvv = 0;
+                        if (!m.uv.empty()) {
+                            uvu = (w0 * (R)m.uv[2 * i1] + w1 * (R)m.uv[2 * i2]) + w2 * (R)m.uv[2 * i0];
+                            uvv = (w0 * (R)m.uv[2 * i1 + 1] + w1 * (R)m.uv[2 * i2 + 1]) + w2 * (R)m.uv[2 * i0 + 1];
+                        }
+                        V3<R> fn = normalize(cross(p1 - p0, p2 - p0));
+                        V3<R> n0 = vnrm(i0), n1 = vnrm(i1), n2 = vnrm(i2);
+                        V3<R> Nl = fn;
+                        if (mat.smooth) Nl = normalize((w0 * n1 + w1 * n2) + w2 * n0);
+                        V3<R> N = Nl;
+                        V3<R> arb = (std::fabs(N.x) < (R)0.9) ? V3<R>{1, 0, 0} : V3<R>{0, 1, 0};
+                        V3<R> tg = normalize(cross(arb, N));
+                        V3<R> bt = cross(N, tg);
+                        R texv = 1;
+                        if (mat.tex) texv = tex_bilinear<R>(mat, uvu, uvv);
+                        R alb[NB];
+                        for (int k = 0; k < NB; ++k) {
+                            R av = (w0 * (R)m.alb[NB * i1 + k] + w1 * (R)m.alb[NB * i2 + k]) + w2 * (R)m.alb[NB * i0 + k];
+                            R mv = mat.tex ? texv * (R)mat.albedo[k] : (R)mat.albedo[k];
+                            alb[k] = av * mv;
+                        }
+                        V3<R> P = (w0 * p1 + w1 * p2) + w2 * p0;
+                        {  // computeShadingPoint (cpu_path_tracer.ipp:481-511)
+                            V3<R> q0 = P - dot(P - p1, n1) * n1;
+                            V3<R> q1 = P - dot(P - p2, n2) * n2;
+                            V3<R> q2 = P - dot(P - p0, n0) * n0;
+                            V3<R> sp = (q0 * w0 + q1 * w1) + q2 * w2;
+                            V3<R> diff = sp - P;
+                            if (dot(diff, Nl) > (R)0) P = sp;
+                        }
+                        if (bounce == 0) {
+                            if (acc.tri == 0xffffffffu) acc.tri = h.id;
+                            for (int k = 0; k < NB; ++k) acc.alb[k] = acc.alb[k] + alb[k];
+                            acc.nrm = acc.nrm + N;
+                            acc.depth = acc.depth + h.t;
+                            acc.count++;
+                        }
+                        V3<R> org = offset_point(P, fn);
+                        V3<R> V = {-ray.d.x, -ray.d.y, -ray.d.z};
+                        (void)V;
+                        R vrad[NB] = {0};
+                        for (int l = 0; l < nl; ++l) {
+                            V3<R> ld;
+                            R dist, lpdf, lrad[NB];
+                            sample_light<R>(lights[l], org, opt.seed, (uint32_t)pix, (uint32_t)s,
+                                            ((uint32_t)bounce << 8) | (SLOT_LIGHT0 + 2 * l), ld, dist, lpdf, lrad);
+                            if (!(lpdf > 0)) continue;
+                            cnt[1]++;
+                            RayT<R> sray{org, ld};
+                            Hit<R> sh = bvh_trace<R>(m, b, sray, true, dist, &tc_any);
+                            if (sh.id != 0xffffffffu) continue;  // occluded
+                            R ndl = dot(N, ld);
+                            R mx = std::max(ndl, (R)0);
+                            R cth = std::max((R)0, dot(ld, N));
+                            R mpdf = (ndl <= (R)0) ? (R)0 : ndl * Consts<R>::inv_pi;
+                            R wgt = power_heuristic(lpdf, mpdf);
+                            for (int k = 0; k < NB; ++k) {
+                                R bsdf = (alb[k] * Consts<R>::inv_pi) * mx;
+                                vrad[k] = vrad[k] + ((((beta[k] * bsdf) * lrad[k]) * cth) * wgt) / lpdf;
+                            }
+                        }
+                        for (int k = 0; k < NB; ++k) path[k] = path[k] + vrad[k];
+                        if (bounce == 0)
+                            for (int k = 0; k < NB; ++k) direct[k] = direct[k] + vrad[k];
+                        if (bounce >= opt.bounces) break;
+                        // ---- BSDF sampling (material.ipp:8-33, material_sampling.hpp:15-30, sampling.ipp:53-84)
+                        uint32_t db = (uint32_t)bounce << 8;
+                        R u1 = (R)rng_u01(opt.seed, (uint32_t)pix, (uint32_t)s, db | SLOT_BSDF_U);
+                        R u2 = (R)rng_u01(opt.seed, (uint32_t)pix, (uint32_t)s, db | SLOT_BSDF_V);
+                        R ox = (R)2 * u1 - (R)1, oy = (R)2 * u2 - (R)1;
+                        R dx = 0, dy = 0;
+                        if (!(std::fabs(ox) < (R)1e-6 && std::fabs(oy) < (R)1e-6)) {
+                            R r, th;
+                            if (std::fabs(ox) > std::fabs(oy)) { r = ox; th = (Consts<R>::pi / (R)4) * (oy / ox); }
+                            else { r = oy; th = (Consts<R>::pi / (R)2) - (Consts<R>::pi / (R)4) * (ox / oy); }
+                            dx = r * std::cos(th);
+                            dy = r * std::sin(th);
+                        }
+                        R lz = std::sqrt(std::max((R)0, ((R)1 - dx * dx) - dy * dy));
+                        R mpdf = lz * Consts<R>::inv_pi;
+                        if (!(mpdf > (R)0)) break;
+                        V3<R> nd{(tg.x * dx + bt.x * dy) + N.x * lz, (tg.y * dx + bt.y * dy) + N.y * lz,
+                                 (tg.z * dx + bt.z * dy) + N.z * lz};
+                        R ndd = dot(N, nd);
+                        R mx = std::max(ndd, (R)0);
+                        R cth = std::max((R)0, dot(nd, N));
+                        R lp = 0;
+                        for (int l = 0; l < nl; ++l) lp = lp + light_pdf<R>(lights[l], org, nd);
+                        if (nl > 0) lp = lp / (R)nl;
+                        R wgt = power_heuristic(mpdf, lp);
+                        R qmax = 0;
+                        for (int k = 0; k < NB; ++k) {
+                            R bsdf = (alb[k] * Consts<R>::inv_pi) * mx;
+                            beta[k] = beta[k] * (((bsdf * cth) * wgt) / mpdf);
+                            qmax = std::max(qmax, beta[k]);
+                        }
+                        if (opt.rr && bounce + 1 >= opt.rr_start) {
+                            R q = std::min((R)1, qmax);
+                            R ur = (R)rng_u01(opt.seed, (uint32_t)pix, (uint32_t)s, db | SLOT_RR);
+                            if (ur >= q) break;
+                            for (int k = 0; k < NB; ++k) beta[k] = beta[k] / q;
+                        }
+                        ray = RayT<R>{org, nd};
+                        cnt[2]++;
+                    }
+                    for (int k = 0; k < NB; ++k) { acc.rad[k].add(path[k]); acc.dir[k].add(direct[k]); }
+                }
+                // ---- averaging and passes (cpu_path_tracer.ipp:229-239, passes.ipp:81-127)
+                R sf = (R)opt.spp;
+                size_t o = (size_t)pix;
+                for (int k = 0; k < NB; ++k) {
+                    R rad = acc.rad[k].s / sf, dr = acc.dir[k].s / sf;
+                    if (out.radiance) out.radiance[NB * o + k] = (float)rad;
+                    if (out.direct) out.direct[NB * o + k] = (float)dr;
+                    if (out.indirect) out.indirect[NB * o + k] = (float)(rad - dr);
+                    if (out.albedo) out.albedo[NB * o + k] = (float)(acc.alb[k] / sf);
+                }
+                if (out.normal) {
+                    V3<R> n = acc.nrm;
+                    if (acc.count > 0) n = normalize(n);
+                    out.normal[3 * o] = (float)n.x; out.normal[3 * o + 1] = (float)n.y; out.normal[3 * o + 2] = (float)n.z;
+                }
+                if (out.depth) out.depth[o] = acc.count ? (float)(acc.depth / (R)acc.count) : INFINITY;
+                if (out.alpha) out.alpha[o] = (float)acc.count / (float)opt.spp;
+                if (out.tri_id) out.tri_id[o] = acc.tri;
+            }
+        }
+        cnt[3] = tc_closest.internal; cnt[4] = tc_closest.tris;
+        cnt[5] = tc_any.internal; cnt[6] = tc_any.tris;
+        for (int k = 0; k < 7; ++k) g_cnt[k] += cnt[k];
+    };
+    int nt = std::max(1, nthreads);
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(worker);
+    worker();
+    for (auto& t : th) t.join();
+    if (out.counts)
+        for (int k = 0; k < 7; ++k) out.counts[k] = g_cnt[k];
+}
+
+}  // namespace
+
+void render_f32(const Mesh& m, const BVH& b, const Camera& cam, const Light* lights, int nl,
+                const Material& mat, const Options& opt, Outputs& out, int nthreads,
+                const int* pixel_list, size_t npix) {
+    render_impl<float>(m, b, cam, lights, nl, mat, opt, out, nthreads, pixel_list, npix);
+}
+void render_f64(const Mesh& m, const BVH& b, const Camera& cam, const Light* lights, int nl,
+                const Material& mat, const Options& opt, Outputs& out, int nthreads,
+                const int* pixel_list, size_t npix) {
+    render_impl<double>(m, b, cam, lights, nl, mat, opt, out, nthreads, pixel_list, npix);
+}
+
+}  // namespace vo

@@ -1,0 +1,198 @@
+"""ctypes binding of the C-ABI in include/vira_gpu.h.
+
+The shared library is the only compute path: if it is missing this module
+raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+NUM_BINS = 8
+MAX_LIGHTS = 8
+
+_LIB_PATH = Path(__file__).resolve().parent / "lib" / "libvira_gpu.so"
+
+STATUS = {
+    0: "VIRA_GPU_OK",
+    1: "VIRA_GPU_ERR_INVALID",
+    2: "VIRA_GPU_ERR_HIP",
+    3: "VIRA_GPU_ERR_OOM",
+    4: "VIRA_GPU_ERR_STATE",
+    5: "VIRA_GPU_ERR_UNSUPPORTED",
+}
+
+
+class ViraGPUError(RuntimeError):
+    """A non-OK status from the C-ABI (the reference throws std::runtime_error)."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+class Opts(C.Structure):
+    _fields_ = [("device", C.c_int), ("stream", C.c_void_p)]
+
+
+class Projection(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int),
+        ("spacing", C.c_double),
+        ("radius", C.c_double),
+        ("center", C.c_double * 3),
+        ("offset", C.c_double * 3),
+    ]
+
+
+class BVHOpts(C.Structure):
+    _fields_ = [("max_leaf", C.c_int), ("bins", C.c_int)]
+
+
+class BVHStats(C.Structure):
+    _fields_ = [
+        ("num_nodes", C.c_uint64),
+        ("num_leaves", C.c_uint64),
+        ("max_depth", C.c_uint32),
+        ("max_leaf_size", C.c_uint32),
+        ("sah_cost", C.c_double),
+        ("build_ms", C.c_double),
+        ("bytes", C.c_uint64),
+    ]
+
+
+class Light(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int),
+        ("position", C.c_double * 3),
+        ("radius", C.c_float),
+        ("value", C.c_float * NUM_BINS),
+    ]
+
+
+class Material(C.Structure):
+    _fields_ = [
+        ("albedo", C.c_float * NUM_BINS),
+        ("texture_dev", C.c_void_p),
+        ("tex_width", C.c_int),
+        ("tex_height", C.c_int),
+        ("smooth_shading", C.c_int),
+    ]
+
+
+class Camera(C.Structure):
+    _fields_ = [
+        ("width", C.c_int),
+        ("height", C.c_int),
+        ("kinv", C.c_float * 6),
+        ("rot", C.c_float * 9),
+        ("position", C.c_double * 3),
+        ("z_dir", C.c_float),
+    ]
+
+
+class RenderOpts(C.Structure):
+    _fields_ = [
+        ("samples", C.c_int),
+        ("bounces", C.c_int),
+        ("russian_roulette", C.c_int),
+        ("rr_start", C.c_int),
+        ("seed", C.c_uint32),
+    ]
+
+
+class Outputs(C.Structure):
+    _fields_ = [
+        ("radiance_dev", C.c_void_p),
+        ("direct_dev", C.c_void_p),
+        ("indirect_dev", C.c_void_p),
+        ("albedo_dev", C.c_void_p),
+        ("normal_dev", C.c_void_p),
+        ("depth_dev", C.c_void_p),
+        ("alpha_dev", C.c_void_p),
+        ("tri_id_dev", C.c_void_p),
+    ]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [
+        ("primary_rays", C.c_uint64),
+        ("shadow_rays", C.c_uint64),
+        ("bounce_rays", C.c_uint64),
+        ("total_ms", C.c_double),
+        ("trace_closest_ms", C.c_double),
+        ("trace_shadow_ms", C.c_double),
+        ("shade_ms", C.c_double),
+        ("trace_closest_launches", C.c_uint32),
+        ("trace_shadow_launches", C.c_uint32),
+    ]
+
+
+class Instance(C.Structure):
+    _fields_ = [("mesh_id", C.c_int32), ("transform", C.c_double * 16)]
+
+
+# (name, restype, argtypes) for every entry point of include/vira_gpu.h
+_P = C.c_void_p
+_S = C.c_int
+SIGNATURES = [
+    ("vira_gpu_create", _S, [C.POINTER(_P), C.POINTER(Opts)]),
+    ("vira_gpu_destroy", _S, [_P]),
+    ("vira_gpu_set_stream", _S, [_P, _P]),
+    ("vira_gpu_synchronize", _S, [_P]),
+    ("vira_gpu_last_error", C.c_char_p, []),
+    ("vira_gpu_version", C.c_char_p, []),
+    ("vira_gpu_synth_fbm", _S, [_P, C.c_int, C.c_int, C.c_uint32, C.c_double, C.c_int, C.c_double, _P]),
+    ("vira_gpu_synth_vertex_albedo", _S, [_P, C.c_size_t, C.c_uint32, _P]),
+    ("vira_gpu_dem_mesh", _S, [_P, _P, C.c_int, C.c_int, C.POINTER(Projection), _P, C.POINTER(C.c_int32)]),
+    ("vira_gpu_mesh_upload", _S, [_P, _P, C.c_size_t, _P, C.c_size_t, _P, _P, _P, C.POINTER(C.c_int32)]),
+    ("vira_gpu_mesh_info", _S, [_P, C.c_int32, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    ("vira_gpu_mesh_download", _S, [_P, C.c_int32, _P, _P, _P]),
+    ("vira_gpu_mesh_release", _S, [_P, C.c_int32]),
+    ("vira_gpu_blas_build", _S, [_P, C.c_int32, C.POINTER(BVHOpts)]),
+    ("vira_gpu_blas_stats", _S, [_P, C.c_int32, C.POINTER(BVHStats)]),
+    ("vira_gpu_tlas_build", _S, [_P, C.POINTER(Instance), C.c_size_t]),
+    ("vira_gpu_lights_set", _S, [_P, C.POINTER(Light), C.c_int]),
+    ("vira_gpu_material_set", _S, [_P, C.c_int32, C.POINTER(Material)]),
+    ("vira_gpu_render", _S, [_P, C.POINTER(Camera), C.POINTER(RenderOpts), C.POINTER(Outputs), C.POINTER(RenderStats)]),
+    ("vira_gpu_trace_rays", _S, [_P, C.c_int32, _P, C.c_size_t, C.c_int, _P, _P, _P, _P, C.POINTER(C.c_double)]),
+]
+
+_lib = None
+
+
+def lib_path() -> Path:
+    return _LIB_PATH
+
+
+def load() -> C.CDLL:
+    """Loads libvira_gpu.so (built by vira_amd/build.py); raises if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not _LIB_PATH.exists():
+        raise ImportError(
+            f"{_LIB_PATH} is missing: build it with `python -m vira_amd.build` "
+            "(the HIP library is the only compute path)"
+        )
+    lib = C.CDLL(str(_LIB_PATH))
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status: int) -> None:
+    if status != 0:
+        msg = load().vira_gpu_last_error().decode(errors="replace")
+        raise ViraGPUError(status, msg)
+
+
+def ptr(t) -> int | None:
+    """Device pointer of a torch tensor (or None)."""
+    if t is None:
+        return None
+    return t.data_ptr()

@@ -1,0 +1,747 @@
+// bvh.hip -- binned-SAH BVH build on the GPU (row a2).
+//
+// Replaces Mesh::buildBVH -> EmbreeBLAS (mesh.ipp:396-418,
+// embree_blas.ipp:14-71) and restates the reference's own binned SAH
+// (ViraBLAS, vira_blas.ipp:132-254) for a GPU:
+//   * large nodes (> kSmall triangles) are split level by level: each node is
+//     cut into 4096-slot chunks, one workgroup bins a chunk in LDS and merges
+//     it into the node's global bins; the host sweeps the bins (SAH), and a
+//     stable chunked partition (block scan) moves the triangle ids.  Every
+//     step is deterministic.
+//   * small nodes are finished by one thread each (binned SAH, in-place
+//     partition), the per-thread "subdivide" of the reference.
+//   * the binary tree is then rewritten into 64-byte nodes that hold both
+//     children's boxes, and the triangles into leaf order (48 B each).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "context.hpp"
+
+namespace vg {
+
+void mesh_upload_pack(Ctx& c, MeshG& m, const float* pos, const float* alb, const float* uv, const float* nrm);
+
+namespace {
+
+constexpr uint32_t kChunk = 4096;      // slots per binning / partition workgroup
+constexpr int kChunkThreads = 256;
+constexpr int kChunkItems = kChunk / kChunkThreads;
+constexpr uint32_t kSmall = 512;       // nodes at or below this size are built per thread
+constexpr int kMaxBins = 32;
+constexpr int kSmallBins = 8;
+constexpr int kBinVals = 13;           // count, box lo/hi (6), centroid lo/hi (6)
+constexpr int kMedianDepth = 48;       // deeper nodes are split at the object median
+
+struct BNodeG {  // binary build node: children of an internal node are adjacent
+    float lo[3], hi[3];
+    uint32_t left;   // first child (internal) or first slot (leaf)
+    uint32_t count;  // 0 = internal
+};
+
+struct PrimBox {
+    float4 lo;  // xyz = box min
+    float4 hi;  // xyz = box max
+    float4 c;   // xyz = box centre
+};
+
+struct Chunk {
+    uint32_t task, begin, end, pad;
+};
+struct TaskParam {
+    int axis;
+    float cmin;
+    float scale;
+    int split;  // bins <= split go left
+};
+struct ChunkBase {
+    uint32_t left, right;
+};
+struct SmallTask {
+    uint32_t node, begin, count, depth;
+};
+
+__device__ __forceinline__ int bin_of(float c, float cmin, float scale, int nb) {
+    float f = (c - cmin) * scale;
+    int b = (int)f;
+    b = b < 0 ? 0 : b;
+    return b >= nb ? nb - 1 : b;
+}
+
+__global__ void k_prim_boxes(size_t nt, const uint32_t* idx, const GVertex* V, PrimBox* pb, uint32_t* prims) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const GVertex& a = V[idx[3 * t]];
+    const GVertex& b = V[idx[3 * t + 1]];
+    const GVertex& c = V[idx[3 * t + 2]];
+    PrimBox o;
+    o.lo = make_float4(fminf(fminf(a.px, b.px), c.px), fminf(fminf(a.py, b.py), c.py), fminf(fminf(a.pz, b.pz), c.pz), 0.f);
+    o.hi = make_float4(fmaxf(fmaxf(a.px, b.px), c.px), fmaxf(fmaxf(a.py, b.py), c.py), fmaxf(fmaxf(a.pz, b.pz), c.pz), 0.f);
+    o.c = make_float4(0.5f * (o.lo.x + o.hi.x), 0.5f * (o.lo.y + o.hi.y), 0.5f * (o.lo.z + o.hi.z), 0.f);
+    pb[t] = o;
+    prims[t] = (uint32_t)t;
+}
+
+// Root box and centroid box: LDS reduction, then one set of global atomics per block.
+__global__ void __launch_bounds__(256) k_reduce_bounds(size_t n, const PrimBox* pb, int* keys) {
+    __shared__ int s[12];
+    if (threadIdx.x < 12) s[threadIdx.x] = (threadIdx.x % 6) < 3 ? INT_MAX : INT_MIN;
+    __syncthreads();
+    int l[12];
+    for (int k = 0; k < 12; ++k) l[k] = (k % 6) < 3 ? INT_MAX : INT_MIN;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const PrimBox& p = pb[i];
+        float v[12] = {p.lo.x, p.lo.y, p.lo.z, p.hi.x, p.hi.y, p.hi.z, p.c.x, p.c.y, p.c.z, p.c.x, p.c.y, p.c.z};
+        for (int k = 0; k < 12; ++k) {
+            int key = f2key(v[k]);
+            l[k] = (k % 6) < 3 ? min(l[k], key) : max(l[k], key);
+        }
+    }
+    for (int k = 0; k < 12; ++k) {
+        if ((k % 6) < 3) atomicMin(&s[k], l[k]);
+        else atomicMax(&s[k], l[k]);
+    }
+    __syncthreads();
+    if (threadIdx.x < 12) {
+        int k = threadIdx.x;
+        if ((k % 6) < 3) atomicMin(&keys[k], s[k]);
+        else atomicMax(&keys[k], s[k]);
+    }
+}
+
+__global__ void k_init_bins(int* bins, size_t nvals) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nvals) return;
+    int v = (int)(i % kBinVals);
+    bins[i] = v == 0 ? 0 : (((v - 1) % 6) < 3 ? INT_MAX : INT_MIN);
+}
+
+// One workgroup per chunk of a large node: bin into LDS, merge into the node's bins.
+__global__ void __launch_bounds__(kChunkThreads) k_bin_large(const Chunk* chunks, const TaskParam* params, int nb,
+                                                             const uint32_t* prims, const PrimBox* pb, int* bins,
+                                                             uint32_t* chunk_counts) {
+    __shared__ int s[kMaxBins * kBinVals];
+    for (int i = threadIdx.x; i < nb * kBinVals; i += blockDim.x) {
+        int v = i % kBinVals;
+        s[i] = v == 0 ? 0 : (((v - 1) % 6) < 3 ? INT_MAX : INT_MIN);
+    }
+    __syncthreads();
+    Chunk ch = chunks[blockIdx.x];
+    TaskParam tp = params[ch.task];
+    for (uint32_t slot = ch.begin + threadIdx.x; slot < ch.end; slot += blockDim.x) {
+        const PrimBox& p = pb[prims[slot]];
+        float cc = tp.axis == 0 ? p.c.x : (tp.axis == 1 ? p.c.y : p.c.z);
+        int b = bin_of(cc, tp.cmin, tp.scale, nb);
+        int* sb = &s[b * kBinVals];
+        atomicAdd(&sb[0], 1);
+        atomicMin(&sb[1], f2key(p.lo.x));
+        atomicMin(&sb[2], f2key(p.lo.y));
+        atomicMin(&sb[3], f2key(p.lo.z));
+        atomicMax(&sb[4], f2key(p.hi.x));
+        atomicMax(&sb[5], f2key(p.hi.y));
+        atomicMax(&sb[6], f2key(p.hi.z));
+        atomicMin(&sb[7], f2key(p.c.x));
+        atomicMin(&sb[8], f2key(p.c.y));
+        atomicMin(&sb[9], f2key(p.c.z));
+        atomicMax(&sb[10], f2key(p.c.x));
+        atomicMax(&sb[11], f2key(p.c.y));
+        atomicMax(&sb[12], f2key(p.c.z));
+    }
+    __syncthreads();
+    if (threadIdx.x < (unsigned)nb) {
+        int b = threadIdx.x;
+        const int* sb = &s[b * kBinVals];
+        chunk_counts[(size_t)blockIdx.x * nb + b] = (uint32_t)sb[0];
+        if (sb[0] > 0) {
+            int* gb = &bins[((size_t)ch.task * nb + b) * kBinVals];
+            atomicAdd(&gb[0], sb[0]);
+            for (int k = 1; k < kBinVals; ++k) {
+                if (((k - 1) % 6) < 3) atomicMin(&gb[k], sb[k]);
+                else atomicMax(&gb[k], sb[k]);
+            }
+        }
+    }
+}
+
+// Stable partition of one chunk into tmp (left part, then right part).
+__global__ void __launch_bounds__(kChunkThreads) k_partition_large(const Chunk* chunks, const TaskParam* params,
+                                                                   int nb, const ChunkBase* bases,
+                                                                   const uint32_t* prims, const PrimBox* pb,
+                                                                   uint32_t* tmp) {
+    __shared__ uint32_t lds[kChunkThreads / 64 + 1];
+    Chunk ch = chunks[blockIdx.x];
+    TaskParam tp = params[ch.task];
+    ChunkBase cb = bases[blockIdx.x];
+    uint32_t ids[kChunkItems];
+    uint32_t flags = 0, nl = 0;
+    for (int k = 0; k < kChunkItems; ++k) {
+        uint32_t slot = ch.begin + threadIdx.x * kChunkItems + k;
+        ids[k] = 0;
+        if (slot < ch.end) {
+            uint32_t p = prims[slot];
+            ids[k] = p;
+            const PrimBox& b = pb[p];
+            float cc = tp.axis == 0 ? b.c.x : (tp.axis == 1 ? b.c.y : b.c.z);
+            if (bin_of(cc, tp.cmin, tp.scale, nb) <= tp.split) {
+                flags |= 1u << k;
+                nl++;
+            }
+        }
+    }
+    // block exclusive scan of nl
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t x = nl;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) lds[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (int w = 0; w < kChunkThreads / 64; ++w) {
+            uint32_t t = lds[w];
+            lds[w] = s;
+            s += t;
+        }
+    }
+    __syncthreads();
+    uint32_t lbefore = x - nl + lds[wid];
+    uint32_t pos = threadIdx.x * kChunkItems;  // chunk-local index of the first item
+    for (int k = 0; k < kChunkItems; ++k) {
+        uint32_t slot = ch.begin + pos + k;
+        if (slot >= ch.end) break;
+        uint32_t li = pos + k;
+        if (flags & (1u << k)) {
+            tmp[cb.left + lbefore] = ids[k];
+            lbefore++;
+        } else {
+            uint32_t rbefore = li - lbefore;
+            tmp[cb.right + rbefore] = ids[k];
+        }
+    }
+}
+
+__global__ void k_copy_back(const Chunk* chunks, const uint32_t* tmp, uint32_t* prims) {
+    Chunk ch = chunks[blockIdx.x];
+    for (uint32_t s = ch.begin + threadIdx.x; s < ch.end; s += blockDim.x) prims[s] = tmp[s];
+}
+
+__device__ __forceinline__ float half_area(const float lo[3], const float hi[3]) {
+    float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+    return ex * ey + ey * ez + ez * ex;
+}
+
+// One thread builds the whole subtree of a small node.
+__global__ void k_build_small(const SmallTask* tasks, uint32_t ntasks, uint32_t* prims, const PrimBox* pb,
+                              BNodeG* nodes, uint32_t* node_counter, uint32_t* max_depth, int max_leaf) {
+    uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ti >= ntasks) return;
+    struct Ent {
+        uint32_t node, begin, count, depth;
+    };
+    Ent stack[64];
+    int sp = 0;
+    stack[sp++] = Ent{tasks[ti].node, tasks[ti].begin, tasks[ti].count, tasks[ti].depth};
+    uint32_t md = 0;
+    while (sp > 0) {
+        Ent e = stack[--sp];
+        md = max(md, e.depth);
+        BNodeG nd = nodes[e.node];
+        uint32_t n = e.count;
+        bool leaf = n <= 1;
+        int axis = 0, split = -1;
+        float cmin = 0.f, scale = 0.f;
+        float blo[kSmallBins][3], bhi[kSmallBins][3];
+        uint32_t bcnt[kSmallBins];
+        if (!leaf) {
+            float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (uint32_t i = 0; i < n; ++i) {
+                float4 c = pb[prims[e.begin + i]].c;
+                clo[0] = fminf(clo[0], c.x); clo[1] = fminf(clo[1], c.y); clo[2] = fminf(clo[2], c.z);
+                chi[0] = fmaxf(chi[0], c.x); chi[1] = fmaxf(chi[1], c.y); chi[2] = fmaxf(chi[2], c.z);
+            }
+            float ext[3] = {chi[0] - clo[0], chi[1] - clo[1], chi[2] - clo[2]};
+            axis = (ext[1] > ext[0]) ? 1 : 0;
+            if (ext[2] > ext[axis]) axis = 2;
+            cmin = clo[axis];
+            scale = (float)kSmallBins / ext[axis];
+            bool degenerate = !(ext[axis] > 0.f) || !isfinite(scale) || e.depth >= kMedianDepth;
+            if (degenerate) {
+                leaf = n <= (uint32_t)max_leaf;
+            } else {
+                for (int b = 0; b < kSmallBins; ++b) {
+                    bcnt[b] = 0;
+                    blo[b][0] = blo[b][1] = blo[b][2] = INFINITY;
+                    bhi[b][0] = bhi[b][1] = bhi[b][2] = -INFINITY;
+                }
+                for (uint32_t i = 0; i < n; ++i) {
+                    const PrimBox& p = pb[prims[e.begin + i]];
+                    float cc = axis == 0 ? p.c.x : (axis == 1 ? p.c.y : p.c.z);
+                    int b = bin_of(cc, cmin, scale, kSmallBins);
+                    bcnt[b]++;
+                    blo[b][0] = fminf(blo[b][0], p.lo.x); blo[b][1] = fminf(blo[b][1], p.lo.y); blo[b][2] = fminf(blo[b][2], p.lo.z);
+                    bhi[b][0] = fmaxf(bhi[b][0], p.hi.x); bhi[b][1] = fmaxf(bhi[b][1], p.hi.y); bhi[b][2] = fmaxf(bhi[b][2], p.hi.z);
+                }
+                // SAH sweep: right-to-left areas, then left-to-right costs
+                float rarea[kSmallBins];
+                uint32_t rcnt[kSmallBins];
+                float rl[3] = {INFINITY, INFINITY, INFINITY}, rh[3] = {-INFINITY, -INFINITY, -INFINITY};
+                uint32_t rc = 0;
+                for (int b = kSmallBins - 1; b > 0; --b) {
+                    rc += bcnt[b];
+                    for (int k = 0; k < 3; ++k) { rl[k] = fminf(rl[k], blo[b][k]); rh[k] = fmaxf(rh[k], bhi[b][k]); }
+                    rarea[b] = rc ? half_area(rl, rh) : 0.f;
+                    rcnt[b] = rc;
+                }
+                float ll[3] = {INFINITY, INFINITY, INFINITY}, lh[3] = {-INFINITY, -INFINITY, -INFINITY};
+                uint32_t lc = 0;
+                float best = INFINITY;
+                for (int b = 0; b < kSmallBins - 1; ++b) {
+                    lc += bcnt[b];
+                    for (int k = 0; k < 3; ++k) { ll[k] = fminf(ll[k], blo[b][k]); lh[k] = fmaxf(lh[k], bhi[b][k]); }
+                    if (lc == 0 || rcnt[b + 1] == 0) continue;
+                    float cost = (float)lc * half_area(ll, lh) + (float)rcnt[b + 1] * rarea[b + 1];
+                    if (cost < best) { best = cost; split = b; }
+                }
+                float A = half_area(nd.lo, nd.hi);
+                float leaf_cost = (float)n * A;
+                if (split < 0) leaf = n <= (uint32_t)max_leaf;
+                else if (n <= (uint32_t)max_leaf && leaf_cost <= A + best) leaf = true;
+            }
+        }
+        if (leaf) {
+            nd.left = e.begin;
+            nd.count = n;
+            nodes[e.node] = nd;
+            continue;
+        }
+        uint32_t nl;
+        BNodeG L, R;
+        if (split >= 0) {
+            // in-place partition (vira_blas.ipp:149-159)
+            int64_t i = e.begin, j = (int64_t)e.begin + n - 1;
+            while (i <= j) {
+                const PrimBox& p = pb[prims[i]];
+                float cc = axis == 0 ? p.c.x : (axis == 1 ? p.c.y : p.c.z);
+                if (bin_of(cc, cmin, scale, kSmallBins) <= split) {
+                    i++;
+                } else {
+                    uint32_t t = prims[i];
+                    prims[i] = prims[j];
+                    prims[j] = t;
+                    j--;
+                }
+            }
+            nl = (uint32_t)(i - e.begin);
+            for (int k = 0; k < 3; ++k) { L.lo[k] = INFINITY; L.hi[k] = -INFINITY; R.lo[k] = INFINITY; R.hi[k] = -INFINITY; }
+            for (int b = 0; b < kSmallBins; ++b) {
+                BNodeG& D = (b <= split) ? L : R;
+                for (int k = 0; k < 3; ++k) { D.lo[k] = fminf(D.lo[k], blo[b][k]); D.hi[k] = fmaxf(D.hi[k], bhi[b][k]); }
+            }
+        } else {
+            // object median; children keep the parent box (conservative)
+            nl = n / 2;
+            for (int k = 0; k < 3; ++k) { L.lo[k] = R.lo[k] = nd.lo[k]; L.hi[k] = R.hi[k] = nd.hi[k]; }
+        }
+        uint32_t c = atomicAdd(node_counter, 2u);
+        L.left = e.begin; L.count = nl;
+        R.left = e.begin + nl; R.count = n - nl;
+        nodes[c] = L;
+        nodes[c + 1] = R;
+        nd.left = c;
+        nd.count = 0;
+        nodes[e.node] = nd;
+        // push right then left (left is built first)
+        if (sp + 2 <= 64) {
+            stack[sp++] = Ent{c + 1, e.begin + nl, n - nl, e.depth + 1};
+            stack[sp++] = Ent{c, e.begin, nl, e.depth + 1};
+        }
+    }
+    atomicMax(max_depth, md);
+}
+
+// Binary tree -> 64-byte child-box nodes.  internal_index[n] is the exclusive
+// scan of "node n is internal".
+__global__ void k_flag_internal(const BNodeG* nodes, uint32_t nn, uint32_t* flag) {
+    uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn) return;
+    flag[n] = nodes[n].count == 0 ? 1u : 0u;
+}
+
+__device__ __forceinline__ uint32_t child_ref(const BNodeG* nodes, const uint32_t* iidx, uint32_t c) {
+    const BNodeG& ch = nodes[c];
+    if (ch.count == 0) return iidx[c];
+    return make_leaf(ch.left, ch.count);
+}
+
+__global__ void k_emit_nodes(const BNodeG* nodes, uint32_t nn, const uint32_t* iidx, Node2* out) {
+    uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn) return;
+    const BNodeG& nd = nodes[n];
+    if (nd.count != 0) return;
+    const BNodeG& a = nodes[nd.left];
+    const BNodeG& b = nodes[nd.left + 1];
+    Node2 o;
+    o.x = make_float4(a.lo[0], a.hi[0], b.lo[0], b.hi[0]);
+    o.y = make_float4(a.lo[1], a.hi[1], b.lo[1], b.hi[1]);
+    o.z = make_float4(a.lo[2], a.hi[2], b.lo[2], b.hi[2]);
+    o.ref = make_uint4(child_ref(nodes, iidx, nd.left), child_ref(nodes, iidx, nd.left + 1), 0u, 0u);
+    out[iidx[n]] = o;
+}
+
+// Root that is a leaf: one node with the leaf as child 0 and an empty child 1.
+__global__ void k_emit_leaf_root(const BNodeG* nodes, Node2* out) {
+    const BNodeG& a = nodes[0];
+    Node2 o;
+    o.x = make_float4(a.lo[0], a.hi[0], 0.f, 0.f);
+    o.y = make_float4(a.lo[1], a.hi[1], 0.f, 0.f);
+    o.z = make_float4(a.lo[2], a.hi[2], 0.f, 0.f);
+    o.ref = make_uint4(make_leaf(a.left, a.count), kEmptyRef, 0u, 0u);
+    out[0] = o;
+}
+
+__global__ void k_emit_tris(size_t nt, const uint32_t* prims, const uint32_t* idx, const GVertex* V, float4* tris) {
+    size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nt) return;
+    uint32_t p = prims[s];
+    const GVertex& a = V[idx[3 * (size_t)p]];
+    const GVertex& b = V[idx[3 * (size_t)p + 1]];
+    const GVertex& c = V[idx[3 * (size_t)p + 2]];
+    tris[3 * s] = make_float4(a.px, a.py, a.pz, __uint_as_float(p));
+    tris[3 * s + 1] = make_float4(b.px, b.py, b.pz, 0.f);
+    tris[3 * s + 2] = make_float4(c.px, c.py, c.pz, 0.f);
+}
+
+__global__ void k_tree_stats(const BNodeG* nodes, uint32_t nn, float inv_root_area, unsigned long long* cnt,
+                             uint32_t* max_leaf, double* sah) {
+    uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn) return;
+    const BNodeG& nd = nodes[n];
+    float a = half_area(nd.lo, nd.hi) * inv_root_area;
+    if (nd.count == 0) {
+        atomicAdd(&cnt[0], 1ull);
+        atomicAdd(sah, (double)a);
+    } else {
+        atomicAdd(&cnt[1], 1ull);
+        atomicMax(max_leaf, nd.count);
+        atomicAdd(sah, (double)a * (double)nd.count);
+    }
+}
+
+float host_area(const float lo[3], const float hi[3]) {
+    float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+    return ex * ey + ey * ez + ez * ex;
+}
+float key2f_h(int k) {
+    int i = k >= 0 ? k : (k ^ 0x7fffffff);
+    float f;
+    std::memcpy(&f, &i, 4);
+    return f;
+}
+
+struct HostTask {
+    uint32_t node, begin, count, depth;
+    float clo[3], chi[3];
+};
+
+}  // namespace
+
+void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
+    const int max_leaf = std::min(15, std::max(1, opt.max_leaf > 0 ? opt.max_leaf : 4));
+    const int nb = std::min(kMaxBins, std::max(4, opt.bins > 0 ? opt.bins : 16));
+    const size_t N = m.nt;
+    Blas& B = m.blas;
+    B.built = false;
+    VG_CHECK(N < (1u << 27), VIRA_GPU_ERR_INVALID, "BLAS limited to 2^27 triangles");
+    hipEvent_t e0, e1;
+    VG_HIP(hipEventCreate(&e0));
+    VG_HIP(hipEventCreate(&e1));
+    VG_HIP(hipEventRecord(e0, c.stream));
+    if (N == 0) {
+        B.nodes.alloc(sizeof(Node2));
+        Node2 empty{};
+        empty.ref = make_uint4(kEmptyRef, kEmptyRef, 0, 0);
+        VG_HIP(hipMemcpyAsync(B.nodes.p, &empty, sizeof(Node2), hipMemcpyHostToDevice, c.stream));
+        B.tris.alloc(16);
+        B.num_nodes = 1;
+        VG_HIP(hipStreamSynchronize(c.stream));
+        B.stats = vira_gpu_bvh_stats{};
+        B.stats.num_nodes = 1;
+        B.built = true;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return;
+    }
+    DBuf pbuf, prims, tmp, nodes, keys;
+    pbuf.alloc(N * sizeof(PrimBox));
+    prims.alloc(N * 4);
+    tmp.alloc(N * 4);
+    nodes.alloc(2 * N * sizeof(BNodeG));
+    keys.alloc(12 * 4);
+    k_prim_boxes<<<blocks_for(N, 256), 256, 0, c.stream>>>(N, m.idx.as<uint32_t>(), m.verts.as<GVertex>(),
+                                                            pbuf.as<PrimBox>(), prims.as<uint32_t>());
+    VG_HIP(hipGetLastError());
+    int hk[12];
+    for (int k = 0; k < 12; ++k) hk[k] = (k % 6) < 3 ? INT_MAX : INT_MIN;
+    VG_HIP(hipMemcpyAsync(keys.p, hk, sizeof(hk), hipMemcpyHostToDevice, c.stream));
+    k_reduce_bounds<<<std::min<unsigned>(1024, blocks_for(N, 256)), 256, 0, c.stream>>>(N, pbuf.as<PrimBox>(),
+                                                                                      keys.as<int>());
+    VG_HIP(hipGetLastError());
+    VG_HIP(hipMemcpyAsync(hk, keys.p, sizeof(hk), hipMemcpyDeviceToHost, c.stream));
+    VG_HIP(hipStreamSynchronize(c.stream));
+
+    std::vector<BNodeG> hnodes;  // nodes created by the large-node levels
+    hnodes.reserve(1 << 16);
+    BNodeG root{};
+    for (int k = 0; k < 3; ++k) { root.lo[k] = key2f_h(hk[k]); root.hi[k] = key2f_h(hk[3 + k]); }
+    root.left = 0;
+    root.count = (uint32_t)N;
+    hnodes.push_back(root);
+    const float root_area = host_area(root.lo, root.hi);
+
+    std::vector<HostTask> large, small_h;
+    HostTask rt{0, 0, (uint32_t)N, 0, {key2f_h(hk[6]), key2f_h(hk[7]), key2f_h(hk[8])},
+                {key2f_h(hk[9]), key2f_h(hk[10]), key2f_h(hk[11])}};
+    if (N > kSmall) large.push_back(rt);
+    else small_h.push_back(rt);
+
+    DBuf d_chunks, d_params, d_bins, d_counts, d_bases;
+    std::vector<Chunk> chunks;
+    std::vector<TaskParam> params;
+    std::vector<int> hbins;
+    std::vector<uint32_t> hcounts;
+    std::vector<ChunkBase> bases;
+    uint32_t max_depth = 0;
+    while (!large.empty()) {
+        std::vector<HostTask> next;
+        // tasks whose centroids coincide (or that are too deep) split at the median without binning
+        std::vector<HostTask> binned;
+        for (const HostTask& t : large) {
+            max_depth = std::max(max_depth, t.depth);
+            float ext[3] = {t.chi[0] - t.clo[0], t.chi[1] - t.clo[1], t.chi[2] - t.clo[2]};
+            int axis = (ext[1] > ext[0]) ? 1 : 0;
+            if (ext[2] > ext[axis]) axis = 2;
+            float scale = (float)nb / ext[axis];
+            if (!(ext[axis] > 0.f) || !std::isfinite(scale) || t.depth >= (uint32_t)kMedianDepth) {
+                uint32_t nl = t.count / 2;
+                uint32_t ci = (uint32_t)hnodes.size();
+                BNodeG par = hnodes[t.node];
+                BNodeG L = par, R = par;
+                L.left = t.begin; L.count = nl;
+                R.left = t.begin + nl; R.count = t.count - nl;
+                hnodes.push_back(L);
+                hnodes.push_back(R);
+                hnodes[t.node].left = ci;
+                hnodes[t.node].count = 0;
+                HostTask lt = t, rtk = t;
+                lt.node = ci; lt.count = nl; lt.depth = t.depth + 1;
+                rtk.node = ci + 1; rtk.begin = t.begin + nl; rtk.count = t.count - nl; rtk.depth = t.depth + 1;
+                (lt.count > kSmall ? next : small_h).push_back(lt);
+                (rtk.count > kSmall ? next : small_h).push_back(rtk);
+                continue;
+            }
+            binned.push_back(t);
+        }
+        if (!binned.empty()) {
+            chunks.clear();
+            params.clear();
+            for (uint32_t ti = 0; ti < binned.size(); ++ti) {
+                const HostTask& t = binned[ti];
+                float ext[3] = {t.chi[0] - t.clo[0], t.chi[1] - t.clo[1], t.chi[2] - t.clo[2]};
+                int axis = (ext[1] > ext[0]) ? 1 : 0;
+                if (ext[2] > ext[axis]) axis = 2;
+                params.push_back(TaskParam{axis, t.clo[axis], (float)nb / ext[axis], -1});
+                for (uint32_t b = t.begin; b < t.begin + t.count; b += kChunk)
+                    chunks.push_back(Chunk{ti, b, std::min(b + kChunk, t.begin + t.count), 0});
+            }
+            size_t nbv = binned.size() * nb * kBinVals;
+            d_chunks.alloc(chunks.size() * sizeof(Chunk));
+            d_params.alloc(params.size() * sizeof(TaskParam));
+            d_bins.alloc(nbv * 4);
+            d_counts.alloc(chunks.size() * nb * 4);
+            VG_HIP(hipMemcpyAsync(d_chunks.p, chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice, c.stream));
+            VG_HIP(hipMemcpyAsync(d_params.p, params.data(), params.size() * sizeof(TaskParam), hipMemcpyHostToDevice, c.stream));
+            k_init_bins<<<blocks_for(nbv, 256), 256, 0, c.stream>>>(d_bins.as<int>(), nbv);
+            VG_HIP(hipGetLastError());
+            k_bin_large<<<(unsigned)chunks.size(), kChunkThreads, 0, c.stream>>>(
+                d_chunks.as<Chunk>(), d_params.as<TaskParam>(), nb, prims.as<uint32_t>(), pbuf.as<PrimBox>(),
+                d_bins.as<int>(), d_counts.as<uint32_t>());
+            VG_HIP(hipGetLastError());
+            hbins.resize(nbv);
+            hcounts.resize(chunks.size() * nb);
+            VG_HIP(hipMemcpyAsync(hbins.data(), d_bins.p, nbv * 4, hipMemcpyDeviceToHost, c.stream));
+            VG_HIP(hipMemcpyAsync(hcounts.data(), d_counts.p, hcounts.size() * 4, hipMemcpyDeviceToHost, c.stream));
+            VG_HIP(hipStreamSynchronize(c.stream));
+            // SAH sweep per task on the host
+            std::vector<uint32_t> nleft(binned.size());
+            for (uint32_t ti = 0; ti < binned.size(); ++ti) {
+                const HostTask& t = binned[ti];
+                const int* tb = &hbins[(size_t)ti * nb * kBinVals];
+                std::vector<float> rarea(nb), rclo(3 * nb), rchi(3 * nb), rlo(3 * nb), rhi(3 * nb);
+                std::vector<uint32_t> rcnt(nb);
+                float rl[3] = {INFINITY, INFINITY, INFINITY}, rh[3] = {-INFINITY, -INFINITY, -INFINITY};
+                uint32_t rc = 0;
+                for (int b = nb - 1; b > 0; --b) {
+                    const int* bb = tb + b * kBinVals;
+                    rc += (uint32_t)bb[0];
+                    if (bb[0] > 0)
+                        for (int k = 0; k < 3; ++k) { rl[k] = std::min(rl[k], key2f_h(bb[1 + k])); rh[k] = std::max(rh[k], key2f_h(bb[4 + k])); }
+                    rarea[b] = rc ? host_area(rl, rh) : 0.f;
+                    rcnt[b] = rc;
+                }
+                float ll[3] = {INFINITY, INFINITY, INFINITY}, lh[3] = {-INFINITY, -INFINITY, -INFINITY};
+                uint32_t lc = 0;
+                float best = INFINITY;
+                int split = -1;
+                for (int b = 0; b < nb - 1; ++b) {
+                    const int* bb = tb + b * kBinVals;
+                    lc += (uint32_t)bb[0];
+                    if (bb[0] > 0)
+                        for (int k = 0; k < 3; ++k) { ll[k] = std::min(ll[k], key2f_h(bb[1 + k])); lh[k] = std::max(lh[k], key2f_h(bb[4 + k])); }
+                    if (lc == 0 || rcnt[b + 1] == 0) continue;
+                    float cost = (float)lc * host_area(ll, lh) + (float)rcnt[b + 1] * rarea[b + 1];
+                    if (cost < best) { best = cost; split = b; }
+                }
+                VG_CHECK(split >= 0, VIRA_GPU_ERR_HIP, "BVH build: no valid split for a binned node");
+                params[ti].split = split;
+                // children from the bins
+                HostTask lt{}, rtk{};
+                BNodeG L{}, R{};
+                for (int k = 0; k < 3; ++k) {
+                    L.lo[k] = R.lo[k] = lt.clo[k] = rtk.clo[k] = INFINITY;
+                    L.hi[k] = R.hi[k] = lt.chi[k] = rtk.chi[k] = -INFINITY;
+                }
+                uint32_t nl = 0;
+                for (int b = 0; b < nb; ++b) {
+                    const int* bb = tb + b * kBinVals;
+                    if (bb[0] == 0) continue;
+                    bool left = b <= split;
+                    BNodeG& D = left ? L : R;
+                    HostTask& T = left ? lt : rtk;
+                    if (left) nl += (uint32_t)bb[0];
+                    for (int k = 0; k < 3; ++k) {
+                        D.lo[k] = std::min(D.lo[k], key2f_h(bb[1 + k]));
+                        D.hi[k] = std::max(D.hi[k], key2f_h(bb[4 + k]));
+                        T.clo[k] = std::min(T.clo[k], key2f_h(bb[7 + k]));
+                        T.chi[k] = std::max(T.chi[k], key2f_h(bb[10 + k]));
+                    }
+                }
+                nleft[ti] = nl;
+                uint32_t ci = (uint32_t)hnodes.size();
+                L.left = t.begin; L.count = nl;
+                R.left = t.begin + nl; R.count = t.count - nl;
+                hnodes.push_back(L);
+                hnodes.push_back(R);
+                hnodes[t.node].left = ci;
+                hnodes[t.node].count = 0;
+                lt.node = ci; lt.begin = t.begin; lt.count = nl; lt.depth = t.depth + 1;
+                rtk.node = ci + 1; rtk.begin = t.begin + nl; rtk.count = t.count - nl; rtk.depth = t.depth + 1;
+                (lt.count > kSmall ? next : small_h).push_back(lt);
+                (rtk.count > kSmall ? next : small_h).push_back(rtk);
+            }
+            // chunk bases for the stable partition
+            bases.resize(chunks.size());
+            std::vector<uint32_t> lrun(binned.size(), 0), rrun(binned.size(), 0);
+            for (size_t ci = 0; ci < chunks.size(); ++ci) {
+                uint32_t ti = chunks[ci].task;
+                uint32_t cl = 0, ctot = chunks[ci].end - chunks[ci].begin;
+                for (int b = 0; b <= params[ti].split; ++b) cl += hcounts[ci * nb + b];
+                bases[ci].left = binned[ti].begin + lrun[ti];
+                bases[ci].right = binned[ti].begin + nleft[ti] + rrun[ti];
+                lrun[ti] += cl;
+                rrun[ti] += ctot - cl;
+            }
+            d_bases.alloc(bases.size() * sizeof(ChunkBase));
+            VG_HIP(hipMemcpyAsync(d_bases.p, bases.data(), bases.size() * sizeof(ChunkBase), hipMemcpyHostToDevice, c.stream));
+            VG_HIP(hipMemcpyAsync(d_params.p, params.data(), params.size() * sizeof(TaskParam), hipMemcpyHostToDevice, c.stream));
+            k_partition_large<<<(unsigned)chunks.size(), kChunkThreads, 0, c.stream>>>(
+                d_chunks.as<Chunk>(), d_params.as<TaskParam>(), nb, d_bases.as<ChunkBase>(), prims.as<uint32_t>(),
+                pbuf.as<PrimBox>(), tmp.as<uint32_t>());
+            VG_HIP(hipGetLastError());
+            k_copy_back<<<(unsigned)chunks.size(), 256, 0, c.stream>>>(d_chunks.as<Chunk>(), tmp.as<uint32_t>(),
+                                                                       prims.as<uint32_t>());
+            VG_HIP(hipGetLastError());
+            VG_HIP(hipStreamSynchronize(c.stream));
+        }
+        large.swap(next);
+    }
+    // upload the large-level nodes, then finish the small subtrees on the GPU
+    VG_HIP(hipMemcpyAsync(nodes.p, hnodes.data(), hnodes.size() * sizeof(BNodeG), hipMemcpyHostToDevice, c.stream));
+    DBuf d_small, d_counter;
+    d_counter.alloc(8);
+    uint32_t hc[2] = {(uint32_t)hnodes.size(), max_depth};
+    VG_HIP(hipMemcpyAsync(d_counter.p, hc, 8, hipMemcpyHostToDevice, c.stream));
+    if (!small_h.empty()) {
+        std::vector<SmallTask> st(small_h.size());
+        for (size_t i = 0; i < small_h.size(); ++i)
+            st[i] = SmallTask{small_h[i].node, small_h[i].begin, small_h[i].count, small_h[i].depth};
+        d_small.alloc(st.size() * sizeof(SmallTask));
+        VG_HIP(hipMemcpyAsync(d_small.p, st.data(), st.size() * sizeof(SmallTask), hipMemcpyHostToDevice, c.stream));
+        k_build_small<<<blocks_for(st.size(), 64), 64, 0, c.stream>>>(
+            d_small.as<SmallTask>(), (uint32_t)st.size(), prims.as<uint32_t>(), pbuf.as<PrimBox>(),
+            nodes.as<BNodeG>(), d_counter.as<uint32_t>(), d_counter.as<uint32_t>() + 1, max_leaf);
+        VG_HIP(hipGetLastError());
+    }
+    VG_HIP(hipMemcpyAsync(hc, d_counter.p, 8, hipMemcpyDeviceToHost, c.stream));
+    VG_HIP(hipStreamSynchronize(c.stream));
+    const uint32_t nn = hc[0];
+    max_depth = hc[1];
+    VG_CHECK(max_depth < 63, VIRA_GPU_ERR_HIP, "BVH deeper than the traversal stack");
+    VG_CHECK(nn <= 2 * N, VIRA_GPU_ERR_HIP, "BVH node count overflow");
+
+    // layout: internal nodes -> Node2, triangles -> leaf order
+    DBuf flag, iidx;
+    flag.alloc((size_t)nn * 4);
+    iidx.alloc((size_t)nn * 4);
+    k_flag_internal<<<blocks_for(nn, 256), 256, 0, c.stream>>>(nodes.as<BNodeG>(), nn, flag.as<uint32_t>());
+    VG_HIP(hipGetLastError());
+    uint32_t ninternal = scan_exclusive_u32(c, flag.as<uint32_t>(), iidx.as<uint32_t>(), nn);
+    B.num_nodes = std::max<uint32_t>(1, ninternal);
+    B.nodes.alloc(B.num_nodes * sizeof(Node2));
+    B.tris.alloc(N * 3 * sizeof(float4));
+    if (ninternal == 0) {
+        k_emit_leaf_root<<<1, 1, 0, c.stream>>>(nodes.as<BNodeG>(), B.nodes.as<Node2>());
+    } else {
+        k_emit_nodes<<<blocks_for(nn, 256), 256, 0, c.stream>>>(nodes.as<BNodeG>(), nn, iidx.as<uint32_t>(),
+                                                                B.nodes.as<Node2>());
+    }
+    VG_HIP(hipGetLastError());
+    k_emit_tris<<<blocks_for(N, 256), 256, 0, c.stream>>>(N, prims.as<uint32_t>(), m.idx.as<uint32_t>(),
+                                                           m.verts.as<GVertex>(), B.tris.as<float4>());
+    VG_HIP(hipGetLastError());
+    VG_HIP(hipEventRecord(e1, c.stream));
+    // statistics (outside the timed region)
+    DBuf sbuf;
+    sbuf.alloc(32);
+    VG_HIP(hipMemsetAsync(sbuf.p, 0, 32, c.stream));
+    k_tree_stats<<<blocks_for(nn, 256), 256, 0, c.stream>>>(
+        nodes.as<BNodeG>(), nn, root_area > 0.f ? 1.0f / root_area : 0.f, sbuf.as<unsigned long long>(),
+        reinterpret_cast<uint32_t*>(sbuf.as<char>() + 16), reinterpret_cast<double*>(sbuf.as<char>() + 24));
+    VG_HIP(hipGetLastError());
+    unsigned char hs[32];
+    VG_HIP(hipMemcpyAsync(hs, sbuf.p, 32, hipMemcpyDeviceToHost, c.stream));
+    VG_HIP(hipStreamSynchronize(c.stream));
+    float ms = 0.f;
+    VG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    unsigned long long cnts[2];
+    uint32_t mleaf;
+    double sah;
+    std::memcpy(cnts, hs, 16);
+    std::memcpy(&mleaf, hs + 16, 4);
+    std::memcpy(&sah, hs + 24, 8);
+    B.stats = vira_gpu_bvh_stats{};
+    B.stats.num_nodes = cnts[0];
+    B.stats.num_leaves = cnts[1];
+    B.stats.max_depth = max_depth + 1;
+    B.stats.max_leaf_size = mleaf;
+    B.stats.sah_cost = sah;
+    B.stats.build_ms = ms;
+    B.stats.bytes = B.num_nodes * sizeof(Node2) + N * 3 * sizeof(float4);
+    B.built = true;
+}
+
+}  // namespace vg

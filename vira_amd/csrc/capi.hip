@@ -1,0 +1,308 @@
+// capi.hip -- the extern "C" entry points of include/vira_gpu.h.
+// Every entry point catches library errors and returns a status; the message
+// is kept per thread for vira_gpu_last_error().
+#include <cstring>
+#include <string>
+
+#include "context.hpp"
+
+namespace vg {
+void mesh_upload_pack(Ctx& c, MeshG& m, const float* pos, const float* alb, const float* uv, const float* nrm);
+}
+
+namespace {
+thread_local std::string g_last_error;
+
+template <class F>
+vira_gpu_status guarded(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return VIRA_GPU_OK;
+    } catch (const vg::Error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return VIRA_GPU_ERR_OOM;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return VIRA_GPU_ERR_HIP;
+    }
+}
+
+vg::Ctx& C(vira_gpu_ctx* c) {
+    VG_CHECK(c != nullptr, VIRA_GPU_ERR_INVALID, "null context");
+    return *reinterpret_cast<vg::Ctx*>(c);
+}
+
+int32_t new_mesh(vg::Ctx& c) {
+    c.meshes.push_back(std::make_unique<vg::MeshG>());
+    c.meshes.back()->alive = true;
+    vira_gpu_material& m = c.meshes.back()->mat;
+    for (int k = 0; k < VIRA_GPU_NUM_BINS; ++k) m.albedo[k] = 1.0f;
+    m.texture_dev = nullptr;
+    m.tex_width = m.tex_height = 0;
+    m.smooth_shading = 0;
+    return (int32_t)c.meshes.size() - 1;
+}
+
+void set_device(vg::Ctx& c) { VG_HIP(hipSetDevice(c.device)); }
+}  // namespace
+
+extern "C" {
+
+const char* vira_gpu_last_error(void) { return g_last_error.c_str(); }
+const char* vira_gpu_version(void) { return "vira_gpu 0.1 (gfx950)"; }
+
+vira_gpu_status vira_gpu_create(vira_gpu_ctx** out, const vira_gpu_opts* opts) {
+    return guarded([&] {
+        VG_CHECK(out != nullptr, VIRA_GPU_ERR_INVALID, "null output pointer");
+        auto c = std::make_unique<vg::Ctx>();
+        c->device = opts ? opts->device : 0;
+        int n = 0;
+        VG_HIP(hipGetDeviceCount(&n));
+        VG_CHECK(c->device >= 0 && c->device < n, VIRA_GPU_ERR_INVALID, "invalid device ordinal");
+        VG_HIP(hipSetDevice(c->device));
+        if (opts && opts->stream) {
+            c->stream = static_cast<hipStream_t>(opts->stream);
+        } else {
+            VG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+            c->own_stream = true;
+        }
+        *out = reinterpret_cast<vira_gpu_ctx*>(c.release());
+    });
+}
+
+vira_gpu_status vira_gpu_destroy(vira_gpu_ctx* ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        vg::Ctx* c = reinterpret_cast<vg::Ctx*>(ctx);
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        if (c->own_stream) (void)hipStreamDestroy(c->stream);
+        delete c;
+    });
+}
+
+vira_gpu_status vira_gpu_set_stream(vira_gpu_ctx* ctx, void* stream) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        set_device(c);
+        VG_HIP(hipStreamSynchronize(c.stream));
+        if (c.own_stream) VG_HIP(hipStreamDestroy(c.stream));
+        if (stream) {
+            c.stream = static_cast<hipStream_t>(stream);
+            c.own_stream = false;
+        } else {
+            VG_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+            c.own_stream = true;
+        }
+    });
+}
+
+vira_gpu_status vira_gpu_synchronize(vira_gpu_ctx* ctx) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        set_device(c);
+        VG_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+vira_gpu_status vira_gpu_synth_fbm(vira_gpu_ctx* ctx, int width, int height, uint32_t seed, double amplitude,
+                                   int octaves, double base_cells, float* heights_dev) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        set_device(c);
+        VG_CHECK(width > 0 && height > 0 && heights_dev, VIRA_GPU_ERR_INVALID, "synth_fbm: bad arguments");
+        VG_CHECK(octaves >= 1 && octaves <= 30 && base_cells > 0, VIRA_GPU_ERR_INVALID, "synth_fbm: bad octaves/base");
+        vg::launch_synth_fbm(c.stream, width, height, seed, amplitude, octaves, base_cells, heights_dev);
+        VG_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+vira_gpu_status vira_gpu_synth_vertex_albedo(vira_gpu_ctx* ctx, size_t nv, uint32_t seed, float* albedo_dev) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        set_device(c);
+        VG_CHECK(albedo_dev || nv == 0, VIRA_GPU_ERR_INVALID, "synth_vertex_albedo: null output");
+        if (nv == 0) return;
+        vg::launch_synth_albedo(c.stream, nv, seed, albedo_dev);
+        VG_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+vira_gpu_status vira_gpu_dem_mesh(vira_gpu_ctx* ctx, const float* heights_dev, int width, int height,
+                                  const vira_gpu_projection* proj, const float* grey_albedo_dev, int32_t* mesh_id) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        set_device(c);
+        VG_CHECK(heights_dev && proj && mesh_id, VIRA_GPU_ERR_INVALID, "dem_mesh: null argument");
+        VG_CHECK(proj->kind == 0 || proj->kind == 1, VIRA_GPU_ERR_INVALID, "dem_mesh: unknown projection kind");
+        VG_CHECK(proj->spacing > 0 && (proj->kind == 0 || proj->radius > 0), VIRA_GPU_ERR_INVALID,
+                 "dem_mesh: spacing/radius must be positive");
+        int32_t id = new_mesh(c);
+        try {
+            vg::dem_build_mesh(c, *c.meshes[id], heights_dev, width, height, *proj, grey_albedo_dev);
+        } catch (...) {
+            c.meshes[id].reset();
+            throw;
+        }
+        *mesh_id = id;
+    });
+}
+
+vira_gpu_status vira_gpu_mesh_upload(vira_gpu_ctx* ctx, const float* pos, size_t nv, const uint32_t* idx, size_t nt,
+                                     const float* alb, const float* uv, const float* nrm, int32_t* mesh_id) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        set_device(c);
+        VG_CHECK(mesh_id && (pos || nv == 0) && (idx || nt == 0), VIRA_GPU_ERR_INVALID, "mesh_upload: null argument");
+        VG_CHECK(nv < (1ull << 31) && nt < (1ull << 27), VIRA_GPU_ERR_INVALID, "mesh_upload: mesh too large");
+        for (size_t i = 0; i < 3 * nt; ++i)
+            VG_CHECK(idx[i] < nv, VIRA_GPU_ERR_INVALID, "mesh_upload: index out of range");
+        int32_t id = new_mesh(c);
+        vg::MeshG& m = *c.meshes[id];
+        try {
+            m.nv = nv;
+            m.nt = nt;
+            m.verts.alloc(std::max<size_t>(1, nv) * sizeof(vg::GVertex));
+            m.idx.alloc(std::max<size_t>(1, nt) * 12);
+            vg::DBuf dpos, dalb, duv, dnrm;
+            dpos.alloc(std::max<size_t>(1, nv) * 12);
+            if (nv) VG_HIP(hipMemcpyAsync(dpos.p, pos, nv * 12, hipMemcpyHostToDevice, c.stream));
+            if (alb) { dalb.alloc(nv * 32); VG_HIP(hipMemcpyAsync(dalb.p, alb, nv * 32, hipMemcpyHostToDevice, c.stream)); }
+            if (uv) { duv.alloc(nv * 8); VG_HIP(hipMemcpyAsync(duv.p, uv, nv * 8, hipMemcpyHostToDevice, c.stream)); }
+            if (nrm) { dnrm.alloc(nv * 12); VG_HIP(hipMemcpyAsync(dnrm.p, nrm, nv * 12, hipMemcpyHostToDevice, c.stream)); }
+            if (nt) VG_HIP(hipMemcpyAsync(m.idx.p, idx, nt * 12, hipMemcpyHostToDevice, c.stream));
+            if (nv) vg::mesh_upload_pack(c, m, dpos.as<float>(), dalb.as<float>(), duv.as<float>(), dnrm.as<float>());
+            VG_HIP(hipStreamSynchronize(c.stream));
+            // Mesh::init computes normals when the first vertex normal is zero (mesh.ipp:62-64)
+            bool need = !nrm || (nv > 0 && nrm[0] == 0.f && nrm[1] == 0.f && nrm[2] == 0.f);
+            if (need) vg::mesh_compute_normals(c, m);
+        } catch (...) {
+            c.meshes[id].reset();
+            throw;
+        }
+        *mesh_id = id;
+    });
+}
+
+vira_gpu_status vira_gpu_mesh_info(vira_gpu_ctx* ctx, int32_t mesh_id, size_t* nv, size_t* nt) {
+    return guarded([&] {
+        vg::MeshG& m = C(ctx).mesh(mesh_id);
+        if (nv) *nv = m.nv;
+        if (nt) *nt = m.nt;
+    });
+}
+
+vira_gpu_status vira_gpu_mesh_download(vira_gpu_ctx* ctx, int32_t mesh_id, float* pos, uint32_t* idx, float* nrm) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        set_device(c);
+        vg::MeshG& m = c.mesh(mesh_id);
+        VG_HIP(hipStreamSynchronize(c.stream));
+        if (idx && m.nt) VG_HIP(hipMemcpy(idx, m.idx.p, m.nt * 12, hipMemcpyDeviceToHost));
+        if ((pos || nrm) && m.nv) {
+            std::vector<vg::GVertex> v(m.nv);
+            VG_HIP(hipMemcpy(v.data(), m.verts.p, m.nv * sizeof(vg::GVertex), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < m.nv; ++i) {
+                if (pos) { pos[3 * i] = v[i].px; pos[3 * i + 1] = v[i].py; pos[3 * i + 2] = v[i].pz; }
+                if (nrm) { nrm[3 * i] = v[i].nx; nrm[3 * i + 1] = v[i].ny; nrm[3 * i + 2] = v[i].nz; }
+            }
+        }
+    });
+}
+
+vira_gpu_status vira_gpu_mesh_release(vira_gpu_ctx* ctx, int32_t mesh_id) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        set_device(c);
+        c.mesh(mesh_id);
+        VG_HIP(hipStreamSynchronize(c.stream));
+        c.meshes[mesh_id].reset();
+        if (c.scene_mesh == mesh_id) c.scene_mesh = -1;
+    });
+}
+
+vira_gpu_status vira_gpu_blas_build(vira_gpu_ctx* ctx, int32_t mesh_id, const vira_gpu_bvh_opts* opts) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        set_device(c);
+        vira_gpu_bvh_opts o{4, 16};
+        if (opts) o = *opts;
+        VG_CHECK(o.max_leaf >= 0 && o.max_leaf <= 15 && o.bins >= 0 && o.bins <= 32, VIRA_GPU_ERR_INVALID,
+                 "blas_build: max_leaf must be 1..15 and bins 4..32");
+        vg::blas_build(c, c.mesh(mesh_id), o);
+    });
+}
+
+vira_gpu_status vira_gpu_blas_stats(vira_gpu_ctx* ctx, int32_t mesh_id, vira_gpu_bvh_stats* stats) {
+    return guarded([&] {
+        vg::MeshG& m = C(ctx).mesh(mesh_id);
+        VG_CHECK(stats, VIRA_GPU_ERR_INVALID, "blas_stats: null output");
+        VG_CHECK(m.blas.built, VIRA_GPU_ERR_STATE, "blas_stats: BLAS not built");
+        *stats = m.blas.stats;
+    });
+}
+
+vira_gpu_status vira_gpu_tlas_build(vira_gpu_ctx* ctx, const vira_gpu_instance* inst, size_t n) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        VG_CHECK(inst || n == 0, VIRA_GPU_ERR_INVALID, "tlas_build: null instances");
+        VG_CHECK(n != 0, VIRA_GPU_ERR_INVALID, "tlas_build: empty TLAS");  // vira_tlas.ipp:33
+        VG_CHECK(n == 1, VIRA_GPU_ERR_UNSUPPORTED, "tlas_build: one instance per scene in this build");
+        static const double I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+        VG_CHECK(std::memcmp(inst[0].transform, I, sizeof(I)) == 0, VIRA_GPU_ERR_UNSUPPORTED,
+                 "tlas_build: non-identity instance transforms are not supported in this build");
+        vg::MeshG& m = c.mesh(inst[0].mesh_id);
+        VG_CHECK(m.blas.built, VIRA_GPU_ERR_STATE, "tlas_build: BLAS not built");
+        c.scene_mesh = inst[0].mesh_id;
+    });
+}
+
+vira_gpu_status vira_gpu_lights_set(vira_gpu_ctx* ctx, const vira_gpu_light* lights, int n) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        VG_CHECK(n >= 0 && n <= VIRA_GPU_MAX_LIGHTS, VIRA_GPU_ERR_INVALID, "lights_set: 0..8 lights");
+        VG_CHECK(lights || n == 0, VIRA_GPU_ERR_INVALID, "lights_set: null lights");
+        for (int l = 0; l < n; ++l) {
+            VG_CHECK(lights[l].kind == 0 || lights[l].kind == 1, VIRA_GPU_ERR_INVALID, "lights_set: unknown kind");
+            VG_CHECK(lights[l].kind == 0 || lights[l].radius > 0, VIRA_GPU_ERR_INVALID,
+                     "lights_set: SphereLight Radius must be positive");  // sphere_light.ipp:20
+        }
+        c.lights.assign(lights, lights + n);
+    });
+}
+
+vira_gpu_status vira_gpu_material_set(vira_gpu_ctx* ctx, int32_t mesh_id, const vira_gpu_material* mat) {
+    return guarded([&] {
+        vg::MeshG& m = C(ctx).mesh(mesh_id);
+        VG_CHECK(mat, VIRA_GPU_ERR_INVALID, "material_set: null material");
+        VG_CHECK(!mat->texture_dev || (mat->tex_width > 0 && mat->tex_height > 0), VIRA_GPU_ERR_INVALID,
+                 "material_set: texture needs positive size");
+        m.mat = *mat;
+    });
+}
+
+vira_gpu_status vira_gpu_render(vira_gpu_ctx* ctx, const vira_gpu_camera* cam, const vira_gpu_render_opts* opts,
+                                const vira_gpu_outputs* out, vira_gpu_render_stats* stats) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        set_device(c);
+        VG_CHECK(cam && opts && out, VIRA_GPU_ERR_INVALID, "render: null argument");
+        vg::render(c, *cam, *opts, *out, stats);
+    });
+}
+
+vira_gpu_status vira_gpu_trace_rays(vira_gpu_ctx* ctx, int32_t mesh_id, const float* rays, size_t n, int any_hit,
+                                    float* t, uint32_t* id, float* b1, float* b2, double* ms) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        set_device(c);
+        VG_CHECK(rays || n == 0, VIRA_GPU_ERR_INVALID, "trace_rays: null rays");
+        vg::trace_rays(c, c.mesh(mesh_id), rays, n, any_hit, t, id, b1, b2, ms);
+    });
+}
+
+}  // extern "C"

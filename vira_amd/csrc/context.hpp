@@ -1,0 +1,122 @@
+// context.hpp -- host-side state behind the vira_gpu C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/vira_gpu.h"
+#include "device_math.hpp"
+
+namespace vg {
+
+// Error raised inside the library and turned into a status at the C-ABI.
+struct Error : std::runtime_error {
+    vira_gpu_status code;
+    Error(vira_gpu_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define VG_HIP(call)                                                                                  \
+    do {                                                                                              \
+        hipError_t e_ = (call);                                                                       \
+        if (e_ != hipSuccess)                                                                         \
+            throw ::vg::Error(e_ == hipErrorOutOfMemory ? VIRA_GPU_ERR_OOM : VIRA_GPU_ERR_HIP,        \
+                              std::string(#call) + ": " + hipGetErrorString(e_));                    \
+    } while (0)
+
+#define VG_CHECK(cond, code, msg)                          \
+    do {                                                   \
+        if (!(cond)) throw ::vg::Error((code), (msg));     \
+    } while (0)
+
+// Owning device buffer.
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t n) {
+        if (n <= bytes && p) return;
+        release();
+        if (n == 0) return;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw Error(e == hipErrorOutOfMemory ? VIRA_GPU_ERR_OOM : VIRA_GPU_ERR_HIP,
+                        "hipMalloc(" + std::to_string(n) + " B): " + hipGetErrorString(e));
+        }
+        bytes = n;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct Blas {
+    bool built = false;
+    DBuf nodes;   // Node2[num_nodes]
+    DBuf tris;    // float4[3 * nt] in leaf order: v0.xyz+id, v1.xyz, v2.xyz
+    uint64_t num_nodes = 0;
+    vira_gpu_bvh_stats stats{};
+};
+
+struct MeshG {
+    bool alive = false;
+    size_t nv = 0, nt = 0;
+    DBuf verts;   // GVertex[nv]
+    DBuf idx;     // uint32[3 * nt]
+    Blas blas;
+    vira_gpu_material mat{};
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::vector<std::unique_ptr<MeshG>> meshes;
+    int32_t scene_mesh = -1;  // the one instance of the current TLAS
+    std::vector<vira_gpu_light> lights;
+    // render scratch, grown on demand
+    DBuf r_ray_o, r_ray_d, r_hit, r_queue[2], r_beta, r_path_rad, r_path_dir, r_srays_o, r_srays_d,
+        r_scontrib, r_svis, r_sinfo, r_acc, r_counters;
+    hipEvent_t ev[8] = {};
+
+    MeshG& mesh(int32_t id) {
+        VG_CHECK(id >= 0 && (size_t)id < meshes.size() && meshes[id] && meshes[id]->alive, VIRA_GPU_ERR_INVALID,
+                 "invalid mesh id " + std::to_string(id));
+        return *meshes[id];
+    }
+};
+
+// Launch helpers ------------------------------------------------------------
+inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// Kernel entry points implemented in the .hip translation units.
+void launch_synth_fbm(hipStream_t s, int W, int H, uint32_t seed, double amp, int oct, double base, float* out);
+void launch_synth_albedo(hipStream_t s, size_t nv, uint32_t seed, float* out);
+void dem_build_mesh(Ctx& c, MeshG& m, const float* heights, int W, int H, const vira_gpu_projection& p,
+                    const float* grey);
+void mesh_compute_normals(Ctx& c, MeshG& m);
+void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& o);
+void trace_rays(Ctx& c, MeshG& m, const float* rays, size_t n, int any_hit, float* t, uint32_t* id, float* b1,
+                float* b2, double* ms);
+void render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, const vira_gpu_outputs& out,
+            vira_gpu_render_stats* st);
+
+// Exclusive scan of n uint32 values (in place allowed); returns the total.
+uint32_t scan_exclusive_u32(Ctx& c, const uint32_t* in, uint32_t* out, size_t n);
+
+}  // namespace vg

@@ -1,0 +1,198 @@
+// device_math.hpp -- gfx950 device helpers shared by the vira_gpu kernels.
+//
+// All code is compiled with -ffp-contract=off: every +, -, * and / below is a
+// separately rounded IEEE operation in the order written, which is what makes
+// primary-hit ids and deterministic radiance reproducible bit for bit against
+// a CPU restatement of the same formulas.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vg {
+
+constexpr int NB = 8;  // spectral bins
+
+struct f3 {
+    float x, y, z;
+};
+__device__ __forceinline__ f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 add(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 sub(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 mul(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 mul(float s, f3 a) { return f3{s * a.x, s * a.y, s * a.z}; }
+// GLM order: (x*x' + y*y') + z*z'
+__device__ __forceinline__ float dot(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+// GLM cross (glm/detail/func_geometric.inl)
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+    return f3{a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+}
+__device__ __forceinline__ float length(f3 a) { return __fsqrt_rn(dot(a, a)); }
+// GLM normalize: v * (1 / sqrt(dot(v, v)))
+__device__ __forceinline__ f3 normalize(f3 a) {
+    float inv = __fdiv_rn(1.0f, __fsqrt_rn(dot(a, a)));
+    return mul(a, inv);
+}
+__device__ __forceinline__ float comp(f3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+// ---------------------------------------------------------------- hashing / RNG
+__device__ __forceinline__ uint32_t lowbias32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+    x *= 0x846ca68bU;
+    x ^= x >> 16;
+    return x;
+}
+__device__ __forceinline__ float rng_u01(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t dim) {
+    uint32_t h = lowbias32(seed ^ 0x9e3779b9U);
+    h = lowbias32(h ^ pixel);
+    h = lowbias32(h ^ sample);
+    h = lowbias32(h ^ dim);
+    return (float)(h >> 8) * (1.0f / 16777216.0f);
+}
+enum : uint32_t {
+    SLOT_JITTER_X = 0,
+    SLOT_JITTER_Y = 1,
+    SLOT_BSDF_U = 2,
+    SLOT_BSDF_V = 3,
+    SLOT_RR = 4,
+    SLOT_LIGHT0 = 16,
+};
+
+// ---------------------------------------------------------------- order-preserving float keys
+__device__ __forceinline__ int f2key(float f) {
+    int i = __float_as_int(f);
+    return i >= 0 ? i : (i ^ 0x7fffffff);
+}
+__device__ __forceinline__ float key2f(int k) { return __int_as_float(k >= 0 ? k : (k ^ 0x7fffffff)); }
+
+// ---------------------------------------------------------------- watertight ray/triangle
+// Woop, Benthin, Wald, "Watertight Ray/Triangle Intersection", JCGT 2(1), 2013.
+struct WT {
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+};
+
+__device__ __forceinline__ WT wt_setup(f3 d) {
+    WT w;
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int kz = 0;
+    if (ay > ax) kz = 1;
+    if (az > (kz == 0 ? ax : ay)) kz = 2;
+    int kx = kz + 1;
+    if (kx == 3) kx = 0;
+    int ky = kx + 1;
+    if (ky == 3) ky = 0;
+    if (comp(d, kz) < 0.0f) {
+        int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    w.kx = kx;
+    w.ky = ky;
+    w.kz = kz;
+    float dz = comp(d, kz);
+    w.Sx = __fdiv_rn(comp(d, kx), dz);
+    w.Sy = __fdiv_rn(comp(d, ky), dz);
+    w.Sz = __fdiv_rn(1.0f, dz);
+    return w;
+}
+
+// Returns true with t >= 0 and Embree barycentrics (b1, b2) on a hit.
+__device__ __forceinline__ bool wt_intersect(f3 o, const WT& w, f3 v0, f3 v1, f3 v2, float& t_out,
+                                             float& b1_out, float& b2_out) {
+    f3 A = sub(v0, o), B = sub(v1, o), C = sub(v2, o);
+    float Akx = comp(A, w.kx), Aky = comp(A, w.ky), Akz = comp(A, w.kz);
+    float Bkx = comp(B, w.kx), Bky = comp(B, w.ky), Bkz = comp(B, w.kz);
+    float Ckx = comp(C, w.kx), Cky = comp(C, w.ky), Ckz = comp(C, w.kz);
+    float Ax = Akx - w.Sx * Akz;
+    float Ay = Aky - w.Sy * Akz;
+    float Bx = Bkx - w.Sx * Bkz;
+    float By = Bky - w.Sy * Bkz;
+    float Cx = Ckx - w.Sx * Ckz;
+    float Cy = Cky - w.Sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
+        U = (float)(CxBy - CyBx);
+        double AxCy = (double)Ax * (double)Cy, AyCx = (double)Ay * (double)Cx;
+        V = (float)(AxCy - AyCx);
+        double BxAy = (double)Bx * (double)Ay, ByAx = (double)By * (double)Ax;
+        W = (float)(BxAy - ByAx);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    float det = U + V + W;
+    if (det == 0.0f) return false;
+    float Az = w.Sz * Akz;
+    float Bz = w.Sz * Bkz;
+    float Cz = w.Sz * Ckz;
+    float T = (U * Az + V * Bz) + W * Cz;
+    if ((det < 0.0f && T > 0.0f) || (det > 0.0f && T < 0.0f)) return false;
+    float rcp = __fdiv_rn(1.0f, det);
+    float t = T * rcp;
+    if (!(t >= 0.0f) || isinf(t)) return false;
+    t_out = t;
+    b1_out = V * rcp;
+    b2_out = W * rcp;
+    return true;
+}
+
+// ---------------------------------------------------------------- conservative slab test
+constexpr float kSlabGrow = 1.0000005f;  // 1 + 2*gamma(3) rounded up (Ize 2013)
+
+__device__ __forceinline__ float safe_rcp(float c) {
+    if (c == 0.0f) c = signbit(c) ? -1e-20f : 1e-20f;
+    return __fdiv_rn(1.0f, c);
+}
+
+// Entry distance of [lo, hi] or +inf.  Uses the same min/max/compare sequence
+// as the CPU restatement.
+__device__ __forceinline__ float slab(float lox, float loy, float loz, float hix, float hiy, float hiz, f3 o,
+                                      f3 inv, float best) {
+    float tx1 = (lox - o.x) * inv.x, tx2 = (hix - o.x) * inv.x;
+    float tmin = fminf(tx1, tx2), tmax = fmaxf(tx1, tx2);
+    float ty1 = (loy - o.y) * inv.y, ty2 = (hiy - o.y) * inv.y;
+    tmin = fmaxf(tmin, fminf(ty1, ty2));
+    tmax = fminf(tmax, fmaxf(ty1, ty2));
+    float tz1 = (loz - o.z) * inv.z, tz2 = (hiz - o.z) * inv.z;
+    tmin = fmaxf(tmin, fminf(tz1, tz2));
+    tmax = fminf(tmax, fmaxf(tz1, tz2));
+    tmin = fmaxf(tmin, 0.0f);
+    tmax = tmax * kSlabGrow;
+    float lim = best * kSlabGrow;
+    return (tmin <= tmax && tmin <= lim) ? tmin : __int_as_float(0x7f800000);
+}
+
+// ---------------------------------------------------------------- BVH layout
+// Binary node holding its two children's boxes (64 B, one cache line).
+// Child reference: bit 31 clear -> internal node index; bit 31 set -> leaf
+// with first triangle slot (ref >> 4) & 0x7ffffff and count (ref & 15) + 1;
+// 0xffffffff -> empty child.
+struct __attribute__((aligned(16))) Node2 {
+    float4 x;  // c0.lo.x c0.hi.x c1.lo.x c1.hi.x
+    float4 y;
+    float4 z;
+    uint4 ref; // ref0 ref1 - -
+};
+constexpr uint32_t kEmptyRef = 0xffffffffu;
+__device__ __host__ __forceinline__ bool ref_is_leaf(uint32_t r) { return (r & 0x80000000u) != 0; }
+__device__ __host__ __forceinline__ uint32_t leaf_first(uint32_t r) { return (r >> 4) & 0x7ffffffu; }
+__device__ __host__ __forceinline__ uint32_t leaf_count(uint32_t r) { return (r & 15u) + 1u; }
+__device__ __host__ __forceinline__ uint32_t make_leaf(uint32_t first, uint32_t count) {
+    return 0x80000000u | (first << 4) | (count - 1u);
+}
+
+// Vertex record: 64 B, the Vertex<TSpectral=8 bins, float> of vertex.hpp:18-48
+// (position, spectral albedo, normal, uv) so one hit fetches 3 cache lines.
+struct __attribute__((aligned(64))) GVertex {
+    float px, py, pz;
+    float alb[NB];
+    float nx, ny, nz;
+    float u, v;
+};
+static_assert(sizeof(GVertex) == 64, "vertex record is one cache line");
+
+}  // namespace vg
