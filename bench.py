@@ -1,0 +1,182 @@
+"""Benchmark: Mrays/s (primary + shadow) on the 33.5M-triangle DEM at 2048^2.
+
+Workload (one "step" = one full frame): 4097^2 value-noise fBm DEM (seed 1)
+draped on a 1737.4 km sphere with FP64 camera-relative rebasing, camera at
+100 km, 2048^2 pixels, 16 spp, 3 bounces with Russian roulette, 8 spectral
+bins, newSun at 1 AU and 30 deg elevation, 4096^2 log-normal albedo texture.
+Inputs (mesh, BVH, texture) are resident in HBM before the timed region.
+
+Prints one JSON line (see the field notes at the bottom of this file).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+COUNTS_FIXTURE = ROOT / "tests" / "golden" / "c1_traversal_counts.json"
+
+
+def algorithmic_bytes(v_int: float, v_tri: float) -> float:
+    """Bytes per traced ray: ray in (32) + hit out (16) + 64 B per BVH2 internal
+    node visited + 36 B per triangle tested, with the visit counts of the
+    reference binned-SAH BVH2 traversal on the same rays."""
+    return 48.0 + 64.0 * v_int + 36.0 * v_tri
+
+
+def cpu_baseline(built, ctx, stride: int, threads: int):
+    """Times the oracle (FP32 CPU restatement of the reference algorithm) on a
+    strided pixel sample of the same frame; returns (dict, traversal counts)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as O  # noqa: E402  (checker only, never the measured GPU path)
+    from vira_amd import workloads as W
+
+    sc = built.scene
+    p = W.make_projection(sc)
+    heights = built.heights.cpu().numpy()
+    t0 = time.time()
+    om = O.dem_mesh(heights, O.make_projection(p.kind, p.spacing, p.radius, tuple(p.center), tuple(p.offset)),
+                    None if built.vertex_albedo is None else built.vertex_albedo.cpu().numpy())
+    build_s = time.time() - t0
+    cam = O.camera_from(built.camera.to_c())
+    lights = [O.light_from(L.to_c()) for L in built.lights]
+    tex = None if built.texture is None else built.texture.cpu().numpy()
+    mat = O.material(1.0, tex, sc.smooth_shading)
+    opt = O.options(sc.spp, sc.bounces, sc.russian_roulette, 1, sc.render_seed)
+    ii, jj = np.meshgrid(np.arange(0, sc.image, stride), np.arange(0, sc.image, stride))
+    pix = (ii + jj * sc.image).astype(np.int32).reshape(-1)
+    t0 = time.time()
+    out = om.render(cam, lights, mat, opt, precision=32, pixels=pix, threads=threads)
+    dt = time.time() - t0
+    c = out["counts"]
+    ps = c["primary"] + c["shadow"]
+    res = {
+        "value": ps / dt / 1e6,
+        "unit": "Mrays/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{pix.size} pixels (every {stride}th row and column) of" if stride > 1 else "all pixels of")
+                  + f" the same frame x {sc.spp} spp x {sc.bounces} bounces, {ps} primary+shadow rays in {dt:.2f} s "
+                  f"(reference-algorithm FP32 CPU restatement, binned-SAH BVH2, std::thread); mesh + BVH2 build "
+                  f"{build_s:.1f} s not timed",
+    }
+    counts = {
+        "closest_rays": c["primary"] + c["bounce"],
+        "closest_internal": c["closest_internal"],
+        "closest_tris": c["closest_tris"],
+        "shadow_rays": c["shadow"],
+        "any_internal": c["any_internal"],
+        "any_tris": c["any_tris"],
+        "sample_pixels": int(pix.size),
+    }
+    return res, counts
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scene", default="c1", choices=["c0", "c1"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-stride", type=int, default=1, help="time every n-th pixel row/column on the CPU")
+    ap.add_argument("--counts-out", default=None, help="write the oracle traversal counts (fixture) to this path")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU); "
+              "run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import torch
+    from vira_amd import scene as S
+    from vira_amd import workloads as W
+
+    sc = W.CONFIG1 if args.scene == "c1" else W.CONFIG0
+    ctx = S.Context(0)
+    t0 = time.time()
+    built = W.build(ctx, sc)
+    torch.cuda.synchronize()
+    setup_s = time.time() - t0
+    print(f"# scene {sc.name}: {sc.triangles} triangles, BVH build {built.bvh['build_ms']:.1f} ms (device), "
+          f"setup {setup_s:.1f} s", file=sys.stderr, flush=True)
+    passes = S.RenderPasses(sc.image, sc.image)
+    for _ in range(args.warmup):
+        ctx.render(built.camera, built.options, passes)
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stats.append(ctx.render(built.camera, built.options, passes))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    prim = sum(s["primary_rays"] for s in stats)
+    shad = sum(s["shadow_rays"] for s in stats)
+    bnc = sum(s["bounce_rays"] for s in stats)
+    value = (prim + shad) / elapsed / 1e6
+    print(f"# {args.steps} frames in {elapsed:.3f} s: primary {prim}, shadow {shad}, bounce {bnc}; "
+          f"closest-hit kernels {sum(s['trace_closest_ms'] for s in stats):.1f} ms, any-hit "
+          f"{sum(s['trace_shadow_ms'] for s in stats):.1f} ms, shading {sum(s['shade_ms'] for s in stats):.1f} ms",
+          file=sys.stderr, flush=True)
+
+    cpu = None
+    counts = None
+    if not args.no_cpu_baseline:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        cpu, counts = cpu_baseline(built, ctx, args.cpu_stride, threads)
+        if args.counts_out:
+            Path(args.counts_out).parent.mkdir(parents=True, exist_ok=True)
+            Path(args.counts_out).write_text(json.dumps({"scene": sc.name, **counts}, indent=1) + "\n")
+    if counts is None and COUNTS_FIXTURE.exists() and args.scene == "c1":
+        counts = json.loads(COUNTS_FIXTURE.read_text())
+    roofline = None
+    if counts is not None and counts.get("closest_rays", 0) > 0:
+        vi = counts["closest_internal"] / counts["closest_rays"]
+        vt = counts["closest_tris"] / counts["closest_rays"]
+        a_closest = algorithmic_bytes(vi, vt)
+        closest_rays = prim + bnc
+        closest_ms = sum(s["trace_closest_ms"] for s in stats)
+        achieved = closest_rays * a_closest / (closest_ms / 1e3) / 1e9
+        roofline = {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_trace_closest",
+            "bytes_per_ray": round(a_closest, 1), "v_int": round(vi, 2), "v_tri": round(vt, 2),
+            "launches": sum(s["trace_closest_launches"] for s in stats),
+            "avg_launch_ms": round(closest_ms / max(1, sum(s["trace_closest_launches"] for s in stats)), 4),
+        }
+    line = {
+        "metric": "Mrays/s (primary+shadow), 33.5M-tri DEM @2048^2, 16 spp, 3 bounces",
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded fBm DEM, log-normal albedo texture)",
+        "config": {"workload": sc.name, "dem": sc.dem, "triangles": sc.triangles, "image": sc.image,
+                   "spp": sc.spp, "bounces": sc.bounces, "russian_roulette": sc.russian_roulette,
+                   "bins": 8, "bounce_rays_per_frame": bnc // args.steps,
+                   "bvh_build_ms": round(built.bvh["build_ms"], 1)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -198,6 +198,27 @@ int orc_intersect_tri(const float* ray6, const float* tri9, float* t, float* b1,
     return wt_intersect(ray, w, v0, v1, v2, *t, *b1, *b2) ? 1 : 0;
 }
 
+// Node records: lohi 6 doubles per node, leftFirst, triCount.
+void orc_bvh_nodes(void* h, double* lohi, uint32_t* left, uint32_t* count) {
+    auto* o = static_cast<OMesh*>(h);
+    for (size_t n = 0; n < o->b.nodes.size(); ++n) {
+        const BNode& b = o->b.nodes[n];
+        for (int k = 0; k < 3; ++k) { lohi[6 * n + k] = b.lo[k]; lohi[6 * n + 3 + k] = b.hi[k]; }
+        left[n] = b.leftFirst;
+        count[n] = b.triCount;
+    }
+}
+
+void orc_bvh_tri_index(void* h, uint32_t* out) {
+    auto* o = static_cast<OMesh*>(h);
+    std::copy(o->b.triIdx.begin(), o->b.triIdx.end(), out);
+}
+
+void orc_offset_point(const float* p, const float* n, float* out) {
+    V3<float> r = offset_point_f32({p[0], p[1], p[2]}, {n[0], n[1], n[2]});
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
 uint32_t orc_lowbias32(uint32_t x) { return lowbias32(x); }
 float orc_rng_u01(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t dim) {
     return rng_u01(seed, pixel, sample, dim);

@@ -216,6 +216,12 @@ inline V3<double> offset_point(V3<double> p, V3<double> n) {
     return {r[0], r[1], r[2]};
 }
 
+}  // namespace
+
+V3<float> offset_point_f32(V3<float> p, V3<float> n) { return offset_point(p, n); }
+
+namespace {
+
 template <class R>
 inline R power_heuristic(R fPdf, R gPdf) {  // cpu_path_tracer.ipp:472-478
     R f = fPdf, g = gPdf;
@@ -229,6 +235,8 @@ inline R tex_bilinear(const Material& mat, R u, R v) {
     R px = std::fmod(u * rx, rx), py = std::fmod(v * ry, ry);
     if (px < 0) px += rx;
     if (py < 0) py += ry;
+    if (!(px >= 0)) px = 0;  // non-finite uv: keep the fetch in range
+    if (!(py >= 0)) py = 0;
     int x0 = (int)std::floor(px), y0 = (int)std::floor(py);
     x0 = std::min(x0, mat.tex_w - 1);
     y0 = std::min(y0, mat.tex_h - 1);

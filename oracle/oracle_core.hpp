@@ -350,19 +350,23 @@ struct TravCount {
     uint64_t tris = 0;      // ray/triangle tests
 };
 
-// Conservative slab test: returns entry distance or +inf.
-constexpr float kSlabGrowF = 1.0000005f;   // 1 + 2*gamma(3) rounded up (Ize 2013)
+// Conservative slab test (Ize 2013): returns the entry distance or +inf.
+// e[k] is 0 for ordinary direction components; for |d_k| < 1e-30 it is
+// FLT_MIN and inv_k = 1/d_k is +-inf, which makes a ray lying in a box face
+// count as inside the closed box (an exact 0 * inf would give NaN).
+constexpr float kSlabGrowF = 1.0000005f;   // 1 + 2*gamma(3) rounded up
 constexpr double kSlabGrowD = 1.0 + 1e-15;
+constexpr float kTinyDir = 1e-30f;
 
 template <class R>
-inline R slab(const R lo[3], const R hi[3], const V3<R>& o, const V3<R>& inv, R best) {
+inline R slab(const R lo[3], const R hi[3], const V3<R>& o, const V3<R>& inv, const V3<R>& e, R best) {
     R grow = std::is_same<R, float>::value ? (R)kSlabGrowF : (R)kSlabGrowD;
-    R tx1 = (lo[0] - o.x) * inv.x, tx2 = (hi[0] - o.x) * inv.x;
+    R tx1 = ((lo[0] - o.x) - e.x) * inv.x, tx2 = ((hi[0] - o.x) + e.x) * inv.x;
     R tmin = std::min(tx1, tx2), tmax = std::max(tx1, tx2);
-    R ty1 = (lo[1] - o.y) * inv.y, ty2 = (hi[1] - o.y) * inv.y;
+    R ty1 = ((lo[1] - o.y) - e.y) * inv.y, ty2 = ((hi[1] - o.y) + e.y) * inv.y;
     tmin = std::max(tmin, std::min(ty1, ty2));
     tmax = std::min(tmax, std::max(ty1, ty2));
-    R tz1 = (lo[2] - o.z) * inv.z, tz2 = (hi[2] - o.z) * inv.z;
+    R tz1 = ((lo[2] - o.z) - e.z) * inv.z, tz2 = ((hi[2] - o.z) + e.z) * inv.z;
     tmin = std::max(tmin, std::min(tz1, tz2));
     tmax = std::min(tmax, std::max(tz1, tz2));
     tmin = std::max(tmin, R(0));
@@ -372,15 +376,15 @@ inline R slab(const R lo[3], const R hi[3], const V3<R>& o, const V3<R>& inv, R 
     return (R)INFINITY;
 }
 
-// Reciprocal direction with exact zeros replaced by +-1e-20 (sign kept) so the
-// slab test treats a ray lying in a box face as inside the closed box.
+template <class R>
+inline V3<R> slab_eps(const V3<R>& d) {
+    auto f = [](R c) -> R { return std::fabs(c) < (R)kTinyDir ? (R)std::numeric_limits<float>::min() : R(0); };
+    return {f(d.x), f(d.y), f(d.z)};
+}
+
 template <class R>
 inline V3<R> safe_inv(const V3<R>& d) {
-    auto f = [](R c) -> R {
-        if (c == R(0)) c = std::signbit(c) ? R(-1e-20) : R(1e-20);
-        return R(1) / c;
-    };
-    return {f(d.x), f(d.y), f(d.z)};
+    return {R(1) / d.x, R(1) / d.y, R(1) / d.z};
 }
 
 template <class R>
@@ -401,6 +405,7 @@ inline Hit<R> bvh_trace(const Mesh& m, const BVH& b, const RayT<R>& ray, bool an
     R best = anyhit ? tmax : (R)INFINITY;
     WTRay<R> w = wt_setup(ray.d);
     V3<R> inv = safe_inv(ray.d);
+    V3<R> eps = slab_eps(ray.d);
     auto box = [&](uint32_t n, R lo[3], R hi[3]) {
         for (int k = 0; k < 3; ++k) {
             if constexpr (std::is_same<R, float>::value) { lo[k] = b.flo[3 * n + k]; hi[k] = b.fhi[3 * n + k]; }
@@ -424,8 +429,9 @@ inline Hit<R> bvh_trace(const Mesh& m, const BVH& b, const RayT<R>& ray, bool an
         return false;
     };
     R lo[3], hi[3];
+    if (b.nodes.empty()) return h;
     box(0, lo, hi);
-    if (b.nodes.empty() || std::isinf(slab(lo, hi, ray.o, inv, best))) return h;
+    if (std::isinf(slab(lo, hi, ray.o, inv, eps, best))) return h;
     uint32_t stack[128];
     int sp = 0;
     uint32_t node = 0;
@@ -440,9 +446,9 @@ inline Hit<R> bvh_trace(const Mesh& m, const BVH& b, const RayT<R>& ray, bool an
         if (cnt) cnt->internal++;
         uint32_t c1 = nd.leftFirst, c2 = nd.leftFirst + 1;
         box(c1, lo, hi);
-        R d1 = slab(lo, hi, ray.o, inv, best);
+        R d1 = slab(lo, hi, ray.o, inv, eps, best);
         box(c2, lo, hi);
-        R d2 = slab(lo, hi, ray.o, inv, best);
+        R d2 = slab(lo, hi, ray.o, inv, eps, best);
         if (d1 > d2) { std::swap(d1, d2); std::swap(c1, c2); }
         if (std::isinf(d1)) {
             if (sp == 0) break;
@@ -527,6 +533,8 @@ struct Kahan {
         s = t;
     }
 };
+
+V3<float> offset_point_f32(V3<float> p, V3<float> n);
 
 void render_f32(const Mesh& m, const BVH& b, const Camera& cam, const Light* lights, int nl,
                 const Material& mat, const Options& opt, Outputs& out, int nthreads,

@@ -1,0 +1,298 @@
+"""ctypes wrapper of the CPU oracle (oracle/build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker, never as the measured or
+shipped path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "build" / "liboracle.so"
+NB = 8
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+class Projection(C.Structure):
+    _fields_ = [("kind", C.c_int), ("spacing", C.c_double), ("radius", C.c_double),
+                ("center", C.c_double * 3), ("offset", C.c_double * 3)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("W", C.c_int), ("H", C.c_int), ("kinv", C.c_float * 6), ("rot", C.c_float * 9),
+                ("pos", C.c_double * 3), ("zdir", C.c_float)]
+
+
+class Light(C.Structure):
+    _fields_ = [("kind", C.c_int), ("pos", C.c_double * 3), ("radius", C.c_float), ("value", C.c_float * NB)]
+
+
+class Material(C.Structure):
+    _fields_ = [("albedo", C.c_float * NB), ("tex_w", C.c_int), ("tex_h", C.c_int),
+                ("tex", C.c_void_p), ("smooth", C.c_int)]
+
+
+class Options(C.Structure):
+    _fields_ = [("spp", C.c_int), ("bounces", C.c_int), ("rr", C.c_int), ("rr_start", C.c_int),
+                ("seed", C.c_uint32)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB.exists():
+        build()
+    L = C.CDLL(str(LIB))
+    P = C.c_void_p
+    sig = {
+        "orc_fbm_heights": (None, [C.c_int, C.c_int, C.c_uint32, C.c_double, C.c_int, C.c_double, P, C.c_int]),
+        "orc_vertex_albedo": (None, [C.c_size_t, C.c_uint32, P]),
+        "orc_dem_indices": (C.c_int64, [P, C.c_int, C.c_int, P]),
+        "orc_dem_positions": (None, [P, C.c_int, C.c_int, C.POINTER(Projection), P, P]),
+        "orc_vertex_normals": (None, [P, C.c_size_t, P, C.c_size_t, P]),
+        "orc_mesh_create": (P, [P, P, C.c_size_t, P, C.c_size_t, P, P, P, C.c_int]),
+        "orc_mesh_destroy": (None, [P]),
+        "orc_mesh_build_ms": (C.c_double, [P]),
+        "orc_bvh_num_nodes": (C.c_size_t, [P]),
+        "orc_bvh_check": (C.c_int, [P, P, P]),
+        "orc_trace": (None, [P, P, C.c_size_t, C.c_int, P, C.c_int, P, P, P, P, P, C.c_int]),
+        "orc_trace64": (None, [P, P, C.c_size_t, C.c_int, P, P, P, P, P, C.c_int]),
+        "orc_intersect_tri": (C.c_int, [P, P, P, P, P]),
+        "orc_lowbias32": (C.c_uint32, [C.c_uint32]),
+        "orc_bvh_nodes": (None, [P, P, P, P]),
+        "orc_bvh_tri_index": (None, [P, P]),
+        "orc_offset_point": (None, [P, P, P]),
+        "orc_rng_u01": (C.c_float, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
+        "orc_render": (None, [P, C.c_int, C.POINTER(Camera), C.POINTER(Light), C.c_int, C.POINTER(Material),
+                              C.POINTER(Options), P, P, P, P, P, P, P, P, P, C.c_int, P, C.c_size_t]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def nthreads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def fbm_heights(W, H, seed, amplitude=500.0, octaves=6, base_cells=64.0) -> np.ndarray:
+    out = np.empty((H, W), np.float32)
+    lib().orc_fbm_heights(W, H, seed, amplitude, octaves, base_cells, out.ctypes.data, nthreads())
+    return out
+
+
+def vertex_albedo(nv, seed) -> np.ndarray:
+    out = np.empty(nv, np.float32)
+    lib().orc_vertex_albedo(nv, seed, out.ctypes.data)
+    return out
+
+
+def dem_indices(heights: np.ndarray) -> np.ndarray:
+    h = np.ascontiguousarray(heights, np.float32)
+    H, W = h.shape
+    out = np.empty((2 * (W - 1) * (H - 1), 3), np.uint32)
+    nt = lib().orc_dem_indices(h.ctypes.data, W, H, out.ctypes.data)
+    return out[:nt].copy()
+
+
+def make_projection(kind=0, spacing=30.0, radius=0.0, center=(0, 0, 0), offset=(0, 0, 0)) -> Projection:
+    p = Projection()
+    p.kind, p.spacing, p.radius = kind, spacing, radius
+    for k in range(3):
+        p.center[k], p.offset[k] = center[k], offset[k]
+    return p
+
+
+def dem_positions(heights: np.ndarray, proj: Projection):
+    h = np.ascontiguousarray(heights, np.float32)
+    H, W = h.shape
+    pos = np.empty((H * W, 3), np.float32)
+    pos64 = np.empty((H * W, 3), np.float64)
+    lib().orc_dem_positions(h.ctypes.data, W, H, C.byref(proj), pos.ctypes.data, pos64.ctypes.data)
+    return pos, pos64
+
+
+def dem_uv(W, H) -> np.ndarray:
+    i = np.arange(W, dtype=np.float64)[None, :].repeat(H, 0)
+    j = np.arange(H, dtype=np.float64)[:, None].repeat(W, 1)
+    return np.stack([(i / (W - 1)).astype(np.float32), (j / (H - 1)).astype(np.float32)], -1).reshape(-1, 2)
+
+
+def vertex_normals(pos: np.ndarray, idx: np.ndarray) -> np.ndarray:
+    pos = np.ascontiguousarray(pos, np.float32)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    out = np.empty_like(pos)
+    lib().orc_vertex_normals(pos.ctypes.data, pos.shape[0], idx.ctypes.data, idx.size // 3, out.ctypes.data)
+    return out
+
+
+class Mesh:
+    """Mesh + reference binned-SAH BVH2 on the CPU."""
+
+    def __init__(self, pos, idx, nrm=None, alb=None, uv=None, pos64=None):
+        self.pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        self.idx = np.ascontiguousarray(idx, np.uint32).reshape(-1, 3)
+        self.nrm = None if nrm is None else np.ascontiguousarray(nrm, np.float32)
+        self.alb = None if alb is None else np.ascontiguousarray(alb, np.float32)
+        self.uv = None if uv is None else np.ascontiguousarray(uv, np.float32)
+        self.pos64 = None if pos64 is None else np.ascontiguousarray(pos64, np.float64)
+        self.h = lib().orc_mesh_create(_p(self.pos), _p(self.pos64), self.pos.shape[0], _p(self.idx),
+                                       self.idx.shape[0], _p(self.nrm), _p(self.alb), _p(self.uv), nthreads())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_mesh_destroy(self.h)
+            self.h = None
+
+    @property
+    def build_ms(self):
+        return lib().orc_mesh_build_ms(self.h)
+
+    def check(self):
+        md, ml = C.c_uint32(), C.c_uint32()
+        code = lib().orc_bvh_check(self.h, C.byref(md), C.byref(ml))
+        return code, md.value, ml.value
+
+    def nodes(self):
+        n = lib().orc_bvh_num_nodes(self.h)
+        lohi = np.empty((n, 6), np.float64)
+        left = np.empty(n, np.uint32)
+        cnt = np.empty(n, np.uint32)
+        lib().orc_bvh_nodes(self.h, lohi.ctypes.data, left.ctypes.data, cnt.ctypes.data)
+        tri = np.empty(self.idx.shape[0], np.uint32)
+        lib().orc_bvh_tri_index(self.h, tri.ctypes.data)
+        return lohi, left, cnt, tri
+
+    def trace(self, rays: np.ndarray, any_hit=False, tmax=None, brute=False):
+        """rays: (n, 6) float32.  Returns t, id, b1, b2, (internal visits, tri tests)."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        t = np.empty(n, np.float32)
+        ids = np.empty(n, np.uint32)
+        b1 = np.empty(n, np.float32)
+        b2 = np.empty(n, np.float32)
+        cnt = np.zeros(2, np.uint64)
+        tm = None if tmax is None else np.ascontiguousarray(tmax, np.float32)
+        lib().orc_trace(self.h, rays.ctypes.data, n, int(any_hit), _p(tm), int(brute), t.ctypes.data,
+                        ids.ctypes.data, b1.ctypes.data, b2.ctypes.data, cnt.ctypes.data, nthreads())
+        return t, ids, b1, b2, (int(cnt[0]), int(cnt[1]))
+
+    def trace64(self, rays: np.ndarray, any_hit=False, tmax=None):
+        rays = np.ascontiguousarray(rays, np.float64)
+        n = rays.shape[0]
+        t = np.empty(n, np.float64)
+        ids = np.empty(n, np.uint32)
+        b1 = np.empty(n, np.float64)
+        b2 = np.empty(n, np.float64)
+        tm = None if tmax is None else np.ascontiguousarray(tmax, np.float64)
+        lib().orc_trace64(self.h, rays.ctypes.data, n, int(any_hit), _p(tm), t.ctypes.data, ids.ctypes.data,
+                          b1.ctypes.data, b2.ctypes.data, nthreads())
+        return t, ids, b1, b2
+
+    def render(self, cam: Camera, lights, mat: Material, opt: Options, precision=32, pixels=None,
+               threads=None):
+        W, H = cam.W, cam.H
+        n = W * H
+        out = {
+            "radiance": np.zeros((n, NB), np.float32), "direct": np.zeros((n, NB), np.float32),
+            "indirect": np.zeros((n, NB), np.float32), "albedo": np.zeros((n, NB), np.float32),
+            "normal": np.zeros((n, 3), np.float32), "depth": np.zeros(n, np.float32),
+            "alpha": np.zeros(n, np.float32), "tri_id": np.full(n, 0xFFFFFFFF, np.uint32),
+        }
+        cnt = np.zeros(7, np.uint64)
+        larr = (Light * max(1, len(lights)))(*lights) if lights else (Light * 1)()
+        pl = None if pixels is None else np.ascontiguousarray(pixels, np.int32)
+        lib().orc_render(self.h, precision, C.byref(cam), larr, len(lights), C.byref(mat), C.byref(opt),
+                         *(out[k].ctypes.data for k in ("radiance", "direct", "indirect", "albedo", "normal",
+                                                        "depth", "alpha", "tri_id")),
+                         cnt.ctypes.data, threads or nthreads(), _p(pl), 0 if pl is None else pl.size)
+        out["counts"] = {"primary": int(cnt[0]), "shadow": int(cnt[1]), "bounce": int(cnt[2]),
+                         "closest_internal": int(cnt[3]), "closest_tris": int(cnt[4]),
+                         "any_internal": int(cnt[5]), "any_tris": int(cnt[6])}
+        return out
+
+
+def offset_point(p, n):
+    p = np.ascontiguousarray(p, np.float32)
+    n = np.ascontiguousarray(n, np.float32)
+    out = np.empty(3, np.float32)
+    lib().orc_offset_point(p.ctypes.data, n.ctypes.data, out.ctypes.data)
+    return out
+
+
+def camera_from(c) -> Camera:
+    """Copies a vira_amd camera record (same numbers) into the oracle's struct."""
+    o = Camera()
+    o.W, o.H = c.width, c.height
+    for k in range(6):
+        o.kinv[k] = c.kinv[k]
+    for k in range(9):
+        o.rot[k] = c.rot[k]
+    for k in range(3):
+        o.pos[k] = c.position[k]
+    o.zdir = c.z_dir
+    return o
+
+
+def light_from(L) -> Light:
+    o = Light()
+    o.kind = L.kind
+    for k in range(3):
+        o.pos[k] = L.position[k]
+    o.radius = L.radius
+    for k in range(NB):
+        o.value[k] = L.value[k]
+    return o
+
+
+def material(albedo=1.0, texture: np.ndarray | None = None, smooth=False) -> Material:
+    m = Material()
+    a = np.broadcast_to(np.asarray(albedo, np.float32), (NB,))
+    for k in range(NB):
+        m.albedo[k] = float(a[k])
+    if texture is not None:
+        texture = np.ascontiguousarray(texture, np.float32)
+        m.tex_h, m.tex_w = texture.shape
+        m.tex = texture.ctypes.data
+        m._keep = texture
+    m.smooth = int(smooth)
+    return m
+
+
+def options(spp=1, bounces=0, rr=False, rr_start=1, seed=0) -> Options:
+    o = Options()
+    o.spp, o.bounces, o.rr, o.rr_start, o.seed = spp, bounces, int(rr), rr_start, seed
+    return o
+
+
+def dem_mesh(heights: np.ndarray, proj: Projection, grey_albedo: np.ndarray | None = None,
+             with_pos64=False) -> Mesh:
+    """DEM -> mesh exactly as the oracle restates dem.ipp / image_utils.ipp / mesh.ipp."""
+    H, W = heights.shape
+    pos, pos64 = dem_positions(heights, proj)
+    idx = dem_indices(heights)
+    nrm = vertex_normals(pos, idx)
+    a = np.ones(H * W, np.float32) if grey_albedo is None else np.asarray(grey_albedo, np.float32)
+    alb = np.repeat(a[:, None], NB, axis=1)
+    return Mesh(pos, idx, nrm, alb, dem_uv(W, H), pos64 if with_pos64 else None)

@@ -1,0 +1,190 @@
+"""GPU parity against the CPU oracle, through the C-ABI (rows a1-a10).
+
+Bar: bit-exact for integer/index work (triangle ids, tessellation) and for the
+FP32 twin wherever no transcendental function is involved (positions, normals,
+hit distances, barycentrics, direct-light radiance of a point light);
+<= 1e-5 relative per bin where sin/cos/acos enter (sphere-light and bounce
+sampling use the device and host libm, which may differ in the last ulp).
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from vira_amd import workloads as W
+from vira_amd import scene as S
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_projection(p):
+    return O.make_projection(p.kind, p.spacing, p.radius, tuple(p.center), tuple(p.offset))
+
+
+def oracle_scene(built):
+    sc = built.scene
+    heights = built.heights.cpu().numpy()
+    valb = None if built.vertex_albedo is None else built.vertex_albedo.cpu().numpy()
+    om = O.dem_mesh(heights, oracle_projection(W.make_projection(sc)), valb)
+    cam = O.camera_from(built.camera.to_c())
+    lights = [O.light_from(L.to_c()) for L in built.lights]
+    tex = None if built.texture is None else built.texture.cpu().numpy()
+    mat = O.material(1.0, tex, sc.smooth_shading)
+    opt = O.options(sc.spp, sc.bounces, sc.russian_roulette, 1, sc.render_seed)
+    return om, cam, lights, mat, opt
+
+
+def test_synth_fbm_bit_exact(gpu_ctx):
+    for (w, h, seed) in [(257, 257, 0), (300, 129, 7)]:
+        g = gpu_ctx.synth_fbm(w, h, seed).cpu().numpy()
+        o = O.fbm_heights(w, h, seed)
+        assert np.array_equal(g.view(np.uint32), o.view(np.uint32)), (w, h, seed)
+    a = gpu_ctx.synth_vertex_albedo(1000, 3).cpu().numpy()
+    assert np.array_equal(a, O.vertex_albedo(1000, 3))
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_dem_mesh_bit_exact(gpu_ctx, kind):
+    import torch
+    h = gpu_ctx.synth_fbm(65, 49, 5)
+    if kind == 1:
+        proj = S.projection(1, 30.0, W.MOON_RADIUS, (0, 0, 0), (W.MOON_RADIUS + 100e3, 0, 0))
+    else:
+        proj = S.projection(0, 30.0)
+    mid = gpu_ctx.dem_mesh(h, proj)
+    pos, idx, nrm = gpu_ctx.mesh_download(mid)
+    hn = h.cpu().numpy()
+    opos, _ = O.dem_positions(hn, oracle_projection(proj))
+    oidx = O.dem_indices(hn)
+    onrm = O.vertex_normals(opos, oidx)
+    assert np.array_equal(idx, oidx)
+    assert np.array_equal(pos.view(np.uint32), opos.view(np.uint32))
+    assert np.array_equal(nrm.view(np.uint32), onrm.view(np.uint32))
+
+
+def test_dem_mesh_holes_bit_exact(gpu_ctx):
+    import torch
+    h = gpu_ctx.synth_fbm(40, 33, 9)
+    h[5, 7] = float("inf")
+    h[20, 3] = float("nan")
+    h[0, 0] = float("-inf")
+    h[32, 39] = float("inf")
+    proj = S.projection(0, 30.0)
+    mid = gpu_ctx.dem_mesh(h, proj)
+    pos, idx, nrm = gpu_ctx.mesh_download(mid)
+    hn = h.cpu().numpy()
+    oidx = O.dem_indices(hn)
+    assert idx.shape == oidx.shape and oidx.shape[0] < 2 * 39 * 32
+    assert np.array_equal(idx, oidx)
+    opos, _ = O.dem_positions(hn, oracle_projection(proj))
+    onrm = O.vertex_normals(opos, oidx)
+    fin = np.isfinite(opos).all(1)
+    assert np.array_equal(pos[fin].view(np.uint32), opos[fin].view(np.uint32))
+    assert np.array_equal(nrm.view(np.uint32), onrm.view(np.uint32))
+
+
+def _rays_over_dem(n, extent, rng):
+    o = np.zeros((n, 3), np.float32)
+    o[:, 0] = rng.uniform(-extent, extent, n)
+    o[:, 1] = rng.uniform(-extent, extent, n)
+    o[:, 2] = rng.uniform(1000, 20000, n)
+    tgt = np.zeros((n, 3), np.float64)
+    tgt[:, 0] = rng.uniform(-extent, extent, n)
+    tgt[:, 1] = rng.uniform(-extent, extent, n)
+    d = (tgt - o.astype(np.float64))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return np.concatenate([o, d.astype(np.float32)], 1)
+
+
+def test_trace_rays_bit_exact(gpu_ctx):
+    import torch
+    sc = W.CONFIG0
+    built = W.build(gpu_ctx, sc)
+    om, *_ = oracle_scene(built)
+    rng = np.random.default_rng(1)
+    rays6 = _rays_over_dem(200000, 4500.0, rng)
+    # axis-aligned and vertex-aimed rays: exact zeros in the direction, origins on slab planes
+    grid = np.array([[x, y, 20000.0, 0.0, 0.0, -1.0] for x in np.arange(-3840, 3841, 30.0 * 17)
+                     for y in np.arange(-3840, 3841, 30.0 * 13)], np.float32)
+    rays6 = np.concatenate([rays6, grid], 0)
+    n = rays6.shape[0]
+    rays8 = np.zeros((n, 8), np.float32)
+    rays8[:, 0:3] = rays6[:, 0:3]
+    rays8[:, 4:7] = rays6[:, 3:6]
+    rays8[:, 7] = np.inf
+    t, ids, b1, b2, ms = gpu_ctx.trace_rays(built.mesh, torch.from_numpy(rays8).cuda())
+    ot, oid, ob1, ob2, _ = om.trace(rays6)
+    ids = ids.cpu().numpy().view(np.uint32)
+    assert (oid != 0xFFFFFFFF).mean() > 0.5
+    assert np.array_equal(ids, oid)
+    assert np.array_equal(t.cpu().numpy().view(np.uint32), ot.view(np.uint32))
+    assert np.array_equal(b1.cpu().numpy().view(np.uint32), ob1.view(np.uint32))
+    assert np.array_equal(b2.cpu().numpy().view(np.uint32), ob2.view(np.uint32))
+    # any-hit within tmax = 0.9 * closest t (never occluded by the closest hit itself) and 1.1 * t
+    for f in (0.9, 1.1):
+        tm = np.where(np.isfinite(ot), ot * f, 1e30).astype(np.float32)
+        rays8[:, 7] = tm
+        ta, ida, _, _, _ = gpu_ctx.trace_rays(built.mesh, torch.from_numpy(rays8).cuda(), any_hit=True)
+        _, oida, _, _, _ = om.trace(rays6, any_hit=True, tmax=tm)
+        occ_g = ida.cpu().numpy().view(np.uint32) != 0xFFFFFFFF
+        occ_o = oida != 0xFFFFFFFF
+        assert np.array_equal(occ_g, occ_o), f
+        assert np.array_equal(np.isfinite(ta.cpu().numpy()), ~occ_g)
+
+
+def test_render_config0_parity(gpu_ctx):
+    sc = W.CONFIG0
+    built = W.build(gpu_ctx, sc)
+    rp = S.RenderPasses(sc.image, sc.image)
+    st = gpu_ctx.render(built.camera, built.options, rp)
+    om, cam, lights, mat, opt = oracle_scene(built)
+    o = om.render(cam, lights, mat, opt, precision=32)
+    tri = rp.triangle_id.cpu().numpy().reshape(-1).view(np.uint32)
+    assert np.array_equal(tri, o["tri_id"])
+    assert (tri != 0xFFFFFFFF).sum() > 0.2 * tri.size
+    assert st["primary_rays"] == sc.image * sc.image == o["counts"]["primary"]
+    assert st["shadow_rays"] == o["counts"]["shadow"]
+    dep = rp.depth.cpu().numpy().reshape(-1)
+    assert np.array_equal(dep.view(np.uint32), o["depth"].view(np.uint32))
+    rad = rp.total_radiance.cpu().numpy().reshape(-1, 8)
+    assert np.array_equal(rad.view(np.uint32), o["radiance"].view(np.uint32))
+    alb = rp.albedo.cpu().numpy().reshape(-1, 8)
+    assert np.array_equal(alb.view(np.uint32), o["albedo"].view(np.uint32))
+    nrm = rp.normal_global.cpu().numpy().reshape(-1, 3)
+    assert np.array_equal(nrm.view(np.uint32), o["normal"].view(np.uint32))
+    # FP64 oracle: radiance error of the FP32 path
+    o64 = om.render(cam, lights, mat, opt, precision=64)
+    hit = o["tri_id"] != 0xFFFFFFFF
+    same = o64["tri_id"] == o["tri_id"]
+    ref = o64["radiance"][hit & same]
+    err = np.abs(rad[hit & same] - ref) / np.maximum(np.abs(ref), 1e-30)
+    lit = ref > 0
+    assert np.median(err[lit]) < 1e-5
+
+
+def test_render_bounces_sphere_light_small(gpu_ctx):
+    sc = W.scaled(W.CONFIG1, dem=129, image=64, spp=4, texture=64)
+    built = W.build(gpu_ctx, sc)
+    rp = S.RenderPasses(sc.image, sc.image)
+    st = gpu_ctx.render(built.camera, built.options, rp)
+    om, cam, lights, mat, opt = oracle_scene(built)
+    o = om.render(cam, lights, mat, opt, precision=32)
+    tri = rp.triangle_id.cpu().numpy().reshape(-1).view(np.uint32)
+    assert np.array_equal(tri, o["tri_id"])  # sample 0 is deterministic: primary hits are exact
+    rad = rp.total_radiance.cpu().numpy().reshape(-1, 8).astype(np.float64)
+    ref = o["radiance"].astype(np.float64)
+    rel = np.abs(rad - ref) / np.maximum(np.abs(ref), 1e-30)
+    lit = ref > 0
+    assert lit.mean() > 0.3
+    assert np.mean(rel[lit] <= 1e-5) > 0.97
+    assert abs(rad.sum() / ref.sum() - 1) < 1e-3
+    assert st["bounce_rays"] > 0
+
+
+def test_errors_are_status_codes(gpu_ctx):
+    from vira_amd import ViraGPUError
+    with pytest.raises(ViraGPUError) as e:
+        gpu_ctx.blas_build(12345)
+    assert e.value.status == 1
+    with pytest.raises(ViraGPUError) as e:
+        gpu_ctx.tlas_build([])
+    assert e.value.status == 1

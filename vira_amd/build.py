@@ -29,6 +29,7 @@ FLAGS = [
     "-fPIC",
     "-ffp-contract=off",
     "-fno-fast-math",
+    "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall",
     "-Wno-unused-function",

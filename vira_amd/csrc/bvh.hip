@@ -370,13 +370,18 @@ __global__ void k_flag_internal(const BNodeG* nodes, uint32_t nn, uint32_t* flag
     flag[n] = nodes[n].count == 0 ? 1u : 0u;
 }
 
-__device__ __forceinline__ uint32_t child_ref(const BNodeG* nodes, const uint32_t* iidx, uint32_t c) {
+__device__ __forceinline__ uint32_t child_ref(const BNodeG* nodes, const uint32_t* iidx, uint32_t c,
+                                              uint32_t* err) {
     const BNodeG& ch = nodes[c];
     if (ch.count == 0) return iidx[c];
+    if (ch.count > 16) {  // not encodable: report instead of emitting a bad reference
+        atomicOr(err, 1u);
+        return kEmptyRef;
+    }
     return make_leaf(ch.left, ch.count);
 }
 
-__global__ void k_emit_nodes(const BNodeG* nodes, uint32_t nn, const uint32_t* iidx, Node2* out) {
+__global__ void k_emit_nodes(const BNodeG* nodes, uint32_t nn, const uint32_t* iidx, Node2* out, uint32_t* err) {
     uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= nn) return;
     const BNodeG& nd = nodes[n];
@@ -387,7 +392,7 @@ __global__ void k_emit_nodes(const BNodeG* nodes, uint32_t nn, const uint32_t* i
     o.x = make_float4(a.lo[0], a.hi[0], b.lo[0], b.hi[0]);
     o.y = make_float4(a.lo[1], a.hi[1], b.lo[1], b.hi[1]);
     o.z = make_float4(a.lo[2], a.hi[2], b.lo[2], b.hi[2]);
-    o.ref = make_uint4(child_ref(nodes, iidx, nd.left), child_ref(nodes, iidx, nd.left + 1), 0u, 0u);
+    o.ref = make_uint4(child_ref(nodes, iidx, nd.left, err), child_ref(nodes, iidx, nd.left + 1, err), 0u, 0u);
     out[iidx[n]] = o;
 }
 
@@ -691,7 +696,8 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     VG_CHECK(max_depth < 63, VIRA_GPU_ERR_HIP, "BVH deeper than the traversal stack");
     VG_CHECK(nn <= 2 * N, VIRA_GPU_ERR_HIP, "BVH node count overflow");
 
-    // layout: internal nodes -> Node2, triangles -> leaf order
+    // layout: internal nodes -> Node2, triangles -> leaf order (counter word 1 becomes the error flag)
+    VG_HIP(hipMemsetAsync(d_counter.as<char>() + 4, 0, 4, c.stream));
     DBuf flag, iidx;
     flag.alloc((size_t)nn * 4);
     iidx.alloc((size_t)nn * 4);
@@ -705,7 +711,8 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
         k_emit_leaf_root<<<1, 1, 0, c.stream>>>(nodes.as<BNodeG>(), B.nodes.as<Node2>());
     } else {
         k_emit_nodes<<<blocks_for(nn, 256), 256, 0, c.stream>>>(nodes.as<BNodeG>(), nn, iidx.as<uint32_t>(),
-                                                                B.nodes.as<Node2>());
+                                                                B.nodes.as<Node2>(),
+                                                                reinterpret_cast<uint32_t*>(d_counter.as<char>() + 4));
     }
     VG_HIP(hipGetLastError());
     k_emit_tris<<<blocks_for(N, 256), 256, 0, c.stream>>>(N, prims.as<uint32_t>(), m.idx.as<uint32_t>(),
@@ -722,7 +729,10 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     VG_HIP(hipGetLastError());
     unsigned char hs[32];
     VG_HIP(hipMemcpyAsync(hs, sbuf.p, 32, hipMemcpyDeviceToHost, c.stream));
+    uint32_t emit_err = 0;
+    VG_HIP(hipMemcpyAsync(&emit_err, d_counter.as<char>() + 4, 4, hipMemcpyDeviceToHost, c.stream));
     VG_HIP(hipStreamSynchronize(c.stream));
+    VG_CHECK(emit_err == 0, VIRA_GPU_ERR_HIP, "BVH build produced a leaf with more than 16 triangles");
     float ms = 0.f;
     VG_HIP(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);

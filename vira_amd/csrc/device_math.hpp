@@ -1,6 +1,7 @@
 // device_math.hpp -- gfx950 device helpers shared by the vira_gpu kernels.
 //
-// All code is compiled with -ffp-contract=off: every +, -, * and / below is a
+// All code is compiled with -ffp-contract=off and correctly rounded FP32
+// division and square root: every +, -, *, / and sqrt below is a
 // separately rounded IEEE operation in the order written, which is what makes
 // primary-hit ids and deterministic radiance reproducible bit for bit against
 // a CPU restatement of the same formulas.
@@ -27,10 +28,10 @@ __device__ __forceinline__ float dot(f3 a, f3 b) { return (a.x * b.x + a.y * b.y
 __device__ __forceinline__ f3 cross(f3 a, f3 b) {
     return f3{a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
 }
-__device__ __forceinline__ float length(f3 a) { return __fsqrt_rn(dot(a, a)); }
+__device__ __forceinline__ float length(f3 a) { return __builtin_sqrtf(dot(a, a)); }
 // GLM normalize: v * (1 / sqrt(dot(v, v)))
 __device__ __forceinline__ f3 normalize(f3 a) {
-    float inv = __fdiv_rn(1.0f, __fsqrt_rn(dot(a, a)));
+    float inv = __fdiv_rn(1.0f, __builtin_sqrtf(dot(a, a)));
     return mul(a, inv);
 }
 __device__ __forceinline__ float comp(f3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
@@ -141,23 +142,31 @@ __device__ __forceinline__ bool wt_intersect(f3 o, const WT& w, f3 v0, f3 v1, f3
 }
 
 // ---------------------------------------------------------------- conservative slab test
-constexpr float kSlabGrow = 1.0000005f;  // 1 + 2*gamma(3) rounded up (Ize 2013)
+// Ize, "Robust BVH Ray Traversal", JCGT 2(2), 2013: the far distance is grown
+// by 1 + 2*gamma(3) so rounding never culls a box the ray touches.
+constexpr float kSlabGrow = 1.0000005f;
+constexpr float kTinyDir = 1e-30f;
 
-__device__ __forceinline__ float safe_rcp(float c) {
-    if (c == 0.0f) c = signbit(c) ? -1e-20f : 1e-20f;
-    return __fdiv_rn(1.0f, c);
-}
-
-// Entry distance of [lo, hi] or +inf.  Uses the same min/max/compare sequence
-// as the CPU restatement.
+// inv = 1/d (IEEE: +-inf for a zero component).  e_k is FLT_MIN where
+// |d_k| < 1e-30, else 0: with SAFE, the bounds are moved out by e_k before the
+// multiply, so a ray lying in a box face counts as inside the closed box
+// instead of producing 0 * inf = NaN.  SAFE is only needed for rays with such
+// a component; without it the e terms are left out.
+template <bool SAFE>
 __device__ __forceinline__ float slab(float lox, float loy, float loz, float hix, float hiy, float hiz, f3 o,
-                                      f3 inv, float best) {
-    float tx1 = (lox - o.x) * inv.x, tx2 = (hix - o.x) * inv.x;
+                                      f3 inv, f3 e, float best) {
+    float ax = lox - o.x, bx = hix - o.x, ay = loy - o.y, by = hiy - o.y, az = loz - o.z, bz = hiz - o.z;
+    if (SAFE) {
+        ax = ax - e.x; bx = bx + e.x;
+        ay = ay - e.y; by = by + e.y;
+        az = az - e.z; bz = bz + e.z;
+    }
+    float tx1 = ax * inv.x, tx2 = bx * inv.x;
     float tmin = fminf(tx1, tx2), tmax = fmaxf(tx1, tx2);
-    float ty1 = (loy - o.y) * inv.y, ty2 = (hiy - o.y) * inv.y;
+    float ty1 = ay * inv.y, ty2 = by * inv.y;
     tmin = fmaxf(tmin, fminf(ty1, ty2));
     tmax = fminf(tmax, fmaxf(ty1, ty2));
-    float tz1 = (loz - o.z) * inv.z, tz2 = (hiz - o.z) * inv.z;
+    float tz1 = az * inv.z, tz2 = bz * inv.z;
     tmin = fmaxf(tmin, fminf(tz1, tz2));
     tmax = fminf(tmax, fmaxf(tz1, tz2));
     tmin = fmaxf(tmin, 0.0f);
@@ -165,6 +174,8 @@ __device__ __forceinline__ float slab(float lox, float loy, float loz, float hix
     float lim = best * kSlabGrow;
     return (tmin <= tmax && tmin <= lim) ? tmin : __int_as_float(0x7f800000);
 }
+
+__device__ __forceinline__ float slab_eps(float c) { return fabsf(c) < kTinyDir ? 1.17549435e-38f : 0.0f; }
 
 // ---------------------------------------------------------------- BVH layout
 // Binary node holding its two children's boxes (64 B, one cache line).

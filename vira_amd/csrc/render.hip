@@ -246,12 +246,12 @@ __device__ __forceinline__ void sample_light(const LightF& L, f3 p, uint32_t see
     float phi = 2.0f * kPi * r1;
     float zeta = r2;
     float cc = __fdiv_rn(radius, distance);
-    float theta = acosf((1.0f - zeta) + zeta * __fsqrt_rn(1.0f - cc * cc));
+    float theta = acosf((1.0f - zeta) + zeta * __builtin_sqrtf(1.0f - cc * cc));
     float sinT = sinf(theta), sinP = sinf(phi), cosT = cosf(theta), cosP = cosf(phi);
     float a = cosP * sinT, bb = cosT, cz = sinP * sinT;
     dir = mk3((a * U.x + bb * W.x) + cz * Vv.x, (a * U.y + bb * W.y) + cz * Vv.y, (a * U.z + bb * W.z) + cz * Vv.z);
     float r2s = radius * radius;
-    float Omega = 2.0f * kPi * (1.0f - __fdiv_rn(__fsqrt_rn(distance * distance - r2s), distance));
+    float Omega = 2.0f * kPi * (1.0f - __fdiv_rn(__builtin_sqrtf(distance * distance - r2s), distance));
     pdf = __fdiv_rn(1.0f, Omega);
 #pragma unroll
     for (int k = 0; k < NB; ++k) rad[k] = L.value[k];
@@ -271,10 +271,10 @@ __device__ __forceinline__ float light_pdf(const LightF& L, f3 p, f3 dir) {
     float radius = L.radius;
     if (dc <= radius) return __fdiv_rn(1.0f, (4.0f * kPi * radius * radius));
     f3 tn = mul(toC, __fdiv_rn(1.0f, dc));
-    float cosMax = __fsqrt_rn(1.0f - __fdiv_rn(radius * radius, dc * dc));
+    float cosMax = __builtin_sqrtf(1.0f - __fdiv_rn(radius * radius, dc * dc));
     float ca = dot(dir, tn);
     if (ca < cosMax) return 0.0f;
-    float Omega = 2.0f * kPi * (1.0f - __fdiv_rn(__fsqrt_rn(dc * dc - radius * radius), dc));
+    float Omega = 2.0f * kPi * (1.0f - __fdiv_rn(__builtin_sqrtf(dc * dc - radius * radius), dc));
     return __fdiv_rn(1.0f, Omega);
 }
 
@@ -390,7 +390,7 @@ __global__ void __launch_bounds__(128) k_shade(
                 dx = r * cosf(th);
                 dy = r * sinf(th);
             }
-            float lz = __fsqrt_rn(fmaxf(0.0f, (1.0f - dx * dx) - dy * dy));
+            float lz = __builtin_sqrtf(fmaxf(0.0f, (1.0f - dx * dx) - dy * dy));
             float mpdf = lz * kInvPi;
             if (mpdf > 0.0f) {
                 nd = mk3((tg.x * dx + bt.x * dy) + N.x * lz, (tg.y * dx + bt.y * dy) + N.y * lz,

@@ -21,9 +21,10 @@ struct HitRec {
 };
 
 // lds_ref/lds_dist point at this lane's column: entry k is at [k * stride].
-template <bool ANY>
-__device__ __forceinline__ HitRec traverse(const Node2* __restrict__ nodes, const float4* __restrict__ tris, f3 o,
-                                           f3 d, float tmax, uint32_t* lds_ref, float* lds_dist, int stride) {
+template <bool ANY, bool SAFE>
+__device__ __forceinline__ HitRec traverse_impl(const Node2* __restrict__ nodes, const float4* __restrict__ tris,
+                                                f3 o, f3 d, float tmax, uint32_t* lds_ref, float* lds_dist,
+                                                int stride) {
     HitRec h;
     h.t = __int_as_float(0x7f800000);
     h.id = 0xffffffffu;
@@ -31,7 +32,8 @@ __device__ __forceinline__ HitRec traverse(const Node2* __restrict__ nodes, cons
     h.b2 = 0.f;
     float best = ANY ? tmax : __int_as_float(0x7f800000);
     const WT w = wt_setup(d);
-    const f3 inv = mk3(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
+    const f3 inv = mk3(__fdiv_rn(1.0f, d.x), __fdiv_rn(1.0f, d.y), __fdiv_rn(1.0f, d.z));
+    const f3 e = mk3(slab_eps(d.x), slab_eps(d.y), slab_eps(d.z));
     uint32_t spill_ref[64 - kStackLds];
     float spill_dist[64 - kStackLds];
     int sp = 0;
@@ -40,10 +42,12 @@ __device__ __forceinline__ HitRec traverse(const Node2* __restrict__ nodes, cons
         // ---- descend through internal nodes until a leaf (or nothing) is reached
         while (!ref_is_leaf(cur)) {
             const Node2 nd = nodes[cur];
-            float d0 = nd.ref.x != kEmptyRef ? slab(nd.x.x, nd.y.x, nd.z.x, nd.x.y, nd.y.y, nd.z.y, o, inv, best)
-                                             : __int_as_float(0x7f800000);
-            float d1 = nd.ref.y != kEmptyRef ? slab(nd.x.z, nd.y.z, nd.z.z, nd.x.w, nd.y.w, nd.z.w, o, inv, best)
-                                             : __int_as_float(0x7f800000);
+            float d0 = nd.ref.x != kEmptyRef
+                           ? slab<SAFE>(nd.x.x, nd.y.x, nd.z.x, nd.x.y, nd.y.y, nd.z.y, o, inv, e, best)
+                           : __int_as_float(0x7f800000);
+            float d1 = nd.ref.y != kEmptyRef
+                           ? slab<SAFE>(nd.x.z, nd.y.z, nd.z.z, nd.x.w, nd.y.w, nd.z.w, o, inv, e, best)
+                           : __int_as_float(0x7f800000);
             bool h0 = d0 < __int_as_float(0x7f800000), h1 = d1 < __int_as_float(0x7f800000);
             if (h0 && h1) {
                 uint32_t nearr = nd.ref.x, farr = nd.ref.y;
@@ -120,6 +124,15 @@ __device__ __forceinline__ HitRec traverse(const Node2* __restrict__ nodes, cons
         if (!found) break;
     }
     return h;
+}
+
+// Rays with a (near-)zero direction component take the SAFE slab test.
+template <bool ANY>
+__device__ __forceinline__ HitRec traverse(const Node2* __restrict__ nodes, const float4* __restrict__ tris, f3 o,
+                                           f3 d, float tmax, uint32_t* lds_ref, float* lds_dist, int stride) {
+    if (fabsf(d.x) < kTinyDir || fabsf(d.y) < kTinyDir || fabsf(d.z) < kTinyDir)
+        return traverse_impl<ANY, true>(nodes, tris, o, d, tmax, lds_ref, lds_dist, stride);
+    return traverse_impl<ANY, false>(nodes, tris, o, d, tmax, lds_ref, lds_dist, stride);
 }
 
 }  // namespace vg

@@ -1,0 +1,366 @@
+"""Host-side scene description and the GPU renderer, mirroring vira's API.
+
+The reference's user-facing objects are ``Camera`` (include/vira/cameras/
+camera.hpp), the lights (include/vira/lights), ``Scene`` and
+``CPUPathTracer`` with ``options`` / ``renderPasses``
+(include/vira/rendering/cpu_path_tracer.hpp:30-71).  This module provides the
+same objects over the C-ABI of include/vira_gpu.h; all compute happens in the
+HIP library.  Device memory is held in torch tensors (plumbing only).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _capi as capi
+
+NUM_BINS = capi.NUM_BINS
+SOLAR_RADIUS = 696000000.0  # include/vira/math.hpp:55
+AU = 1.495978707e11
+
+# SpectralData<8, UNIFORM_RANGE, 400, 1000>: 8 uniform bands (spectral_data.ipp:64-132)
+BAND_EDGES_NM = np.linspace(400.0, 1000.0, NUM_BINS + 1)
+
+
+# ---------------------------------------------------------------- camera (a4)
+def intrinsics_inverse(width: int, height: int, focal_length: float, sensor_w: float, sensor_h: float,
+                       z_dir: float = 1.0) -> np.ndarray:
+    """K^-1 as in Camera::initializeIntrinsicMatrix (camera.ipp:1526-1568), FP32.
+
+    Returns the 6 entries [col][row] = (k00, k01, k10, k11, k20, k21).
+    """
+    f32 = np.float32
+    kx = f32(width) / f32(sensor_w)
+    ky_ = f32(height) / f32(sensor_h)
+    px_ = f32(width) / f32(2)
+    py_ = f32(height) / f32(2)
+    zd = f32(z_dir)
+    ky = zd * ky_
+    px = zd * px_
+    py = zd * py_
+    kxy = zd * f32(0)
+    kyx = zd * f32(0)
+    fx = f32(focal_length) * kx
+    fy = f32(focal_length) * ky
+    det = (fx * fy) - (kxy * kyx)
+    k00 = fy / det
+    k10 = -kxy / det
+    k20 = ((kxy * py) - (fy * px)) / det
+    k01 = -kyx / det
+    k11 = fx / det
+    k21 = ((kyx * px) - (fx * py)) / det
+    return np.array([k00, k01, k10, k11, k20, k21], dtype=np.float32)
+
+
+def look_in_direction(direction, up=(0.0, 0.0, 1.0), blender_frame: bool = False) -> np.ndarray:
+    """Camera::lookInDirection (camera.ipp:1455-1500); returns R (3x3, columns = axes)."""
+    def nrm(v):
+        v = np.asarray(v, dtype=np.float64)
+        return v / np.sqrt(np.dot(v, v))
+    up = nrm(up)
+    d = nrm(direction)
+    if abs(np.dot(d, up)) > 0.999:
+        for cand in ((0.0, 1.0, 0.0), (0.0, 0.0, 1.0), (1.0, 0.0, 0.0)):
+            if abs(np.dot(d, cand)) < 0.999:
+                up = np.asarray(cand)
+                break
+    if blender_frame:
+        y_axis, z_axis = up, -d
+    else:
+        y_axis, z_axis = -up, d
+    x_axis = nrm(np.cross(y_axis, z_axis))
+    y_axis = nrm(np.cross(z_axis, x_axis))
+    return np.stack([x_axis, y_axis, z_axis], axis=1)
+
+
+@dataclass
+class Camera:
+    """Pinhole camera: resolution, square pixels, horizontal field of view."""
+    width: int
+    height: int
+    fov_deg: float = 40.0
+    position: tuple = (0.0, 0.0, 0.0)
+    rotation: np.ndarray = field(default_factory=lambda: np.eye(3))
+    focal_length: float = 0.05
+    z_dir: float = 1.0
+
+    def look_in_direction(self, direction, up=(0.0, 0.0, 1.0)):
+        self.rotation = look_in_direction(direction, up)
+        return self
+
+    def sensor_size(self):
+        sw = 2.0 * self.focal_length * math.tan(math.radians(self.fov_deg) / 2.0)
+        return sw, sw * self.height / self.width
+
+    def to_c(self) -> capi.Camera:
+        sw, sh = self.sensor_size()
+        kinv = intrinsics_inverse(self.width, self.height, self.focal_length, sw, sh, self.z_dir)
+        c = capi.Camera()
+        c.width, c.height = self.width, self.height
+        for k in range(6):
+            c.kinv[k] = float(kinv[k])
+        R = np.asarray(self.rotation, dtype=np.float64).astype(np.float32)
+        for r in range(3):
+            for k in range(3):
+                c.rot[3 * r + k] = float(R[r, k])
+        for k in range(3):
+            c.position[k] = float(self.position[k])
+        c.z_dir = float(self.z_dir)
+        return c
+
+
+# ---------------------------------------------------------------- lights (a7)
+def planck_band_radiance(temperature: float, steps: int = 1000) -> np.ndarray:
+    """blackBodyRadiance (spectral_data.ipp:613-629): trapezoid of Planck's law per band."""
+    h, c, kb = 6.62607015e-34, 299792458.0, 1.380649e-23
+    out = np.zeros(NUM_BINS)
+    for i in range(NUM_BINS):
+        lo, hi = BAND_EDGES_NM[i] * 1e-9, BAND_EDGES_NM[i + 1] * 1e-9
+        lam = np.linspace(lo, hi, steps)
+        rad = (2 * h * c * c / lam**5) / (np.exp(h * c / (lam * kb * temperature)) - 1.0)
+        out[i] = np.sum(0.5 * (rad[1:] + rad[:-1]) * np.diff(lam))
+    return out.astype(np.float32)
+
+
+@dataclass
+class PointLight:
+    """PointLight (point_light.ipp:10-38): spectral power P, intensity P / 4 pi."""
+    position: tuple
+    intensity: np.ndarray  # P / (4 pi) per band
+
+    def to_c(self) -> capi.Light:
+        L = capi.Light()
+        L.kind = 0
+        for k in range(3):
+            L.position[k] = float(self.position[k])
+        L.radius = 0.0
+        for k in range(NUM_BINS):
+            L.value[k] = float(self.intensity[k])
+        return L
+
+
+@dataclass
+class SphereLight:
+    """SphereLight (sphere_light.ipp:16-125) with spectral radiance."""
+    position: tuple
+    radius: float
+    radiance: np.ndarray
+
+    def to_c(self) -> capi.Light:
+        L = capi.Light()
+        L.kind = 1
+        for k in range(3):
+            L.position[k] = float(self.position[k])
+        L.radius = float(self.radius)
+        for k in range(NUM_BINS):
+            L.value[k] = float(self.radiance[k])
+        return L
+
+
+def directional_sun(elevation_deg: float, azimuth_deg: float, irradiance, distance: float = AU,
+                    origin=(0.0, 0.0, 0.0), up=(0.0, 0.0, 1.0), east=(1.0, 0.0, 0.0)) -> PointLight:
+    """A distant PointLight giving the stated irradiance (W/m^2 per band) at `origin`."""
+    up = np.asarray(up, dtype=np.float64)
+    east = np.asarray(east, dtype=np.float64)
+    north = np.cross(up, east)
+    el, az = math.radians(elevation_deg), math.radians(azimuth_deg)
+    d = math.cos(el) * (math.cos(az) * east + math.sin(az) * north) + math.sin(el) * up
+    pos = np.asarray(origin, dtype=np.float64) + distance * d
+    e = np.broadcast_to(np.asarray(irradiance, dtype=np.float64), (NUM_BINS,))
+    return PointLight(tuple(pos), (e * distance * distance).astype(np.float32))
+
+
+def new_sun(position) -> SphereLight:
+    """Group::newSun (group.ipp:585-591): SphereLight, R_sun, 5800 K blackbody radiance."""
+    return SphereLight(tuple(position), SOLAR_RADIUS, planck_band_radiance(5800.0, 1000))
+
+
+# ---------------------------------------------------------------- options / passes
+@dataclass
+class PathTracerOptions:
+    """CPUPathTracerOptions (cpu_path_tracer.hpp:30-43) + the GPU additions."""
+    samples: int = 1
+    bounces: int = 0
+    russian_roulette: bool = False
+    rr_start: int = 1
+    seed: int = 0
+
+    def to_c(self) -> capi.RenderOpts:
+        o = capi.RenderOpts()
+        o.samples, o.bounces = int(self.samples), int(self.bounces)
+        o.russian_roulette, o.rr_start, o.seed = int(self.russian_roulette), int(self.rr_start), int(self.seed)
+        return o
+
+
+class RenderPasses:
+    """RenderPasses (passes.hpp:51-95) as device tensors, pixel index i + j*W."""
+
+    def __init__(self, width: int, height: int, device="cuda"):
+        import torch
+        n = width * height
+        self.width, self.height = width, height
+        self.total_radiance = torch.zeros((height, width, NUM_BINS), dtype=torch.float32, device=device)
+        self.direct_radiance = torch.zeros_like(self.total_radiance)
+        self.indirect_radiance = torch.zeros_like(self.total_radiance)
+        self.albedo = torch.zeros_like(self.total_radiance)
+        self.normal_global = torch.zeros((height, width, 3), dtype=torch.float32, device=device)
+        self.depth = torch.zeros((height, width), dtype=torch.float32, device=device)
+        self.alpha = torch.zeros((height, width), dtype=torch.float32, device=device)
+        self.triangle_id = torch.zeros((height, width), dtype=torch.int32, device=device)
+        assert n == self.depth.numel()
+
+    def to_c(self) -> capi.Outputs:
+        o = capi.Outputs()
+        o.radiance_dev = self.total_radiance.data_ptr()
+        o.direct_dev = self.direct_radiance.data_ptr()
+        o.indirect_dev = self.indirect_radiance.data_ptr()
+        o.albedo_dev = self.albedo.data_ptr()
+        o.normal_dev = self.normal_global.data_ptr()
+        o.depth_dev = self.depth.data_ptr()
+        o.alpha_dev = self.alpha.data_ptr()
+        o.tri_id_dev = self.triangle_id.data_ptr()
+        return o
+
+
+# ---------------------------------------------------------------- context
+class Context:
+    """One GPU, one HIP stream (torch's current stream by default)."""
+
+    def __init__(self, device: int = 0, use_torch_stream: bool = True):
+        import torch
+        self.lib = capi.load()
+        self.torch = torch
+        stream = torch.cuda.current_stream(device).cuda_stream if use_torch_stream else None
+        opts = capi.Opts(device, stream)
+        h = C.c_void_p()
+        capi.check(self.lib.vira_gpu_create(C.byref(h), C.byref(opts)))
+        self.h = h
+        self.device = device
+        self.last_stats = None
+
+    def close(self):
+        if self.h:
+            capi.check(self.lib.vira_gpu_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- synthetic inputs
+    def synth_fbm(self, width: int, height: int, seed: int, amplitude: float = 500.0, octaves: int = 6,
+                  base_cells: float = 64.0):
+        t = self.torch.empty((height, width), dtype=self.torch.float32, device=f"cuda:{self.device}")
+        capi.check(self.lib.vira_gpu_synth_fbm(self.h, width, height, seed, amplitude, octaves, base_cells,
+                                               t.data_ptr()))
+        return t
+
+    def synth_vertex_albedo(self, nv: int, seed: int):
+        t = self.torch.empty((nv,), dtype=self.torch.float32, device=f"cuda:{self.device}")
+        capi.check(self.lib.vira_gpu_synth_vertex_albedo(self.h, nv, seed, t.data_ptr()))
+        return t
+
+    # ---- meshes
+    def dem_mesh(self, heights, proj: capi.Projection, grey_albedo=None) -> int:
+        H, W = heights.shape
+        mid = C.c_int32()
+        capi.check(self.lib.vira_gpu_dem_mesh(self.h, heights.data_ptr(), W, H, C.byref(proj),
+                                              None if grey_albedo is None else grey_albedo.data_ptr(),
+                                              C.byref(mid)))
+        return mid.value
+
+    def mesh_upload(self, pos, idx, albedo=None, uv=None, normals=None) -> int:
+        pos = np.ascontiguousarray(pos, dtype=np.float32)
+        idx = np.ascontiguousarray(idx, dtype=np.uint32)
+        arr = [None if a is None else np.ascontiguousarray(a, dtype=np.float32) for a in (albedo, uv, normals)]
+        p = lambda a: None if a is None else a.ctypes.data
+        mid = C.c_int32()
+        capi.check(self.lib.vira_gpu_mesh_upload(self.h, p(pos), pos.size // 3, p(idx), idx.size // 3,
+                                                 p(arr[0]), p(arr[1]), p(arr[2]), C.byref(mid)))
+        return mid.value
+
+    def mesh_info(self, mid: int):
+        nv, nt = C.c_size_t(), C.c_size_t()
+        capi.check(self.lib.vira_gpu_mesh_info(self.h, mid, C.byref(nv), C.byref(nt)))
+        return nv.value, nt.value
+
+    def mesh_download(self, mid: int):
+        nv, nt = self.mesh_info(mid)
+        pos = np.empty((nv, 3), np.float32)
+        idx = np.empty((nt, 3), np.uint32)
+        nrm = np.empty((nv, 3), np.float32)
+        capi.check(self.lib.vira_gpu_mesh_download(self.h, mid, pos.ctypes.data, idx.ctypes.data, nrm.ctypes.data))
+        return pos, idx, nrm
+
+    def blas_build(self, mid: int, max_leaf: int = 4, bins: int = 16):
+        o = capi.BVHOpts(max_leaf, bins)
+        capi.check(self.lib.vira_gpu_blas_build(self.h, mid, C.byref(o)))
+        s = capi.BVHStats()
+        capi.check(self.lib.vira_gpu_blas_stats(self.h, mid, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in capi.BVHStats._fields_}
+
+    def tlas_build(self, mesh_ids):
+        arr = (capi.Instance * len(mesh_ids))()
+        for i, m in enumerate(mesh_ids):
+            arr[i].mesh_id = m
+            for k in range(16):
+                arr[i].transform[k] = 1.0 if k % 5 == 0 else 0.0
+        capi.check(self.lib.vira_gpu_tlas_build(self.h, arr, len(mesh_ids)))
+
+    def lights_set(self, lights):
+        arr = (capi.Light * max(1, len(lights)))()
+        for i, L in enumerate(lights):
+            arr[i] = L.to_c()
+        capi.check(self.lib.vira_gpu_lights_set(self.h, arr, len(lights)))
+
+    def material_set(self, mid: int, albedo=None, texture=None, smooth_shading: bool = False):
+        m = capi.Material()
+        a = np.ones(NUM_BINS, np.float32) if albedo is None else np.broadcast_to(np.asarray(albedo, np.float32), (NUM_BINS,))
+        for k in range(NUM_BINS):
+            m.albedo[k] = float(a[k])
+        if texture is not None:
+            m.texture_dev = texture.data_ptr()
+            m.tex_height, m.tex_width = texture.shape
+        m.smooth_shading = int(smooth_shading)
+        self._texture = texture  # keep alive
+        capi.check(self.lib.vira_gpu_material_set(self.h, mid, C.byref(m)))
+
+    def render(self, camera: Camera, options: PathTracerOptions, passes: RenderPasses):
+        st = capi.RenderStats()
+        cam = camera.to_c()
+        opt = options.to_c()
+        out = passes.to_c()
+        capi.check(self.lib.vira_gpu_render(self.h, C.byref(cam), C.byref(opt), C.byref(out), C.byref(st)))
+        self.last_stats = {f: getattr(st, f) for f, _ in capi.RenderStats._fields_}
+        return self.last_stats
+
+    def trace_rays(self, mid: int, rays, any_hit: bool = False):
+        """rays: (n, 8) float32 CUDA tensor (ox oy oz 0 dx dy dz tmax)."""
+        n = rays.shape[0]
+        dev = rays.device
+        t = self.torch.empty(n, dtype=self.torch.float32, device=dev)
+        ids = self.torch.empty(n, dtype=self.torch.int32, device=dev)
+        b1 = self.torch.empty(n, dtype=self.torch.float32, device=dev)
+        b2 = self.torch.empty(n, dtype=self.torch.float32, device=dev)
+        ms = C.c_double()
+        capi.check(self.lib.vira_gpu_trace_rays(self.h, mid, rays.data_ptr(), n, int(any_hit), t.data_ptr(),
+                                                ids.data_ptr(), b1.data_ptr(), b2.data_ptr(), C.byref(ms)))
+        return t, ids, b1, b2, ms.value
+
+    def synchronize(self):
+        capi.check(self.lib.vira_gpu_synchronize(self.h))
+
+
+def projection(kind: int = 0, spacing: float = 30.0, radius: float = 0.0, center=(0.0, 0.0, 0.0),
+               offset=(0.0, 0.0, 0.0)) -> capi.Projection:
+    p = capi.Projection()
+    p.kind, p.spacing, p.radius = kind, spacing, radius
+    for k in range(3):
+        p.center[k] = float(center[k])
+        p.offset[k] = float(offset[k])
+    return p

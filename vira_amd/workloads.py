@@ -1,0 +1,151 @@
+"""The benchmark scenes: seeded fBm DEMs, camera, sun and render options.
+
+Each scene is plain data plus a builder that creates it on the GPU through
+the C-ABI.  Synthetic inputs are generated on the device (value-noise fBm,
+hashed per-vertex albedo, a seeded log-normal albedo texture).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, replace
+
+import numpy as np
+
+from . import scene as S
+
+MOON_RADIUS = 1737.4e3
+
+
+@dataclass(frozen=True)
+class DemScene:
+    name: str
+    dem: int                 # DEM samples per side
+    seed: int
+    amplitude: float = 500.0
+    spacing: float = 30.0
+    octaves: int = 6
+    base_cells: float = 64.0
+    sphere_radius: float = 0.0   # > 0: gnomonic drape on a sphere + camera-relative FP64 rebasing
+    altitude: float = 20000.0
+    image: int = 512
+    fov_deg: float = 40.0
+    spp: int = 1
+    bounces: int = 0
+    russian_roulette: bool = False
+    sun_elevation_deg: float = 30.0
+    sun_kind: str = "point"      # "point": distant PointLight, flat 1 W/m^2/band; "sphere": newSun at 1 AU
+    albedo: str = "vertex"       # "vertex": 0.12 +- 0.02 per vertex; "texture": log-normal texture
+    texture: int = 0             # texture side
+    smooth_shading: bool = False
+    render_seed: int = 0
+
+    @property
+    def triangles(self) -> int:
+        return 2 * (self.dem - 1) ** 2
+
+
+# Config 0: 257^2 DEM, 512^2, 1 spp, primary + shadow, deterministic.
+CONFIG0 = DemScene("dem257_512px_1spp_direct", dem=257, seed=0)
+# Config 1: 4097^2 DEM (33.5M triangles) on a 1737 km sphere, 2048^2, 16 spp, 3 bounces + RR.
+CONFIG1 = DemScene("dem4097_2048px_16spp_3b", dem=4097, seed=1, sphere_radius=MOON_RADIUS, altitude=100e3,
+                   image=2048, spp=16, bounces=3, russian_roulette=True, sun_kind="sphere",
+                   albedo="texture", texture=4096)
+
+
+def scaled(sc: DemScene, dem: int | None = None, image: int | None = None, spp: int | None = None,
+           bounces: int | None = None, texture: int | None = None) -> DemScene:
+    """A smaller copy of a scene (same generator and geometry rules)."""
+    kw = {}
+    if dem is not None:
+        kw["dem"] = dem
+    if image is not None:
+        kw["image"] = image
+    if spp is not None:
+        kw["spp"] = spp
+    if bounces is not None:
+        kw["bounces"] = bounces
+    if texture is not None:
+        kw["texture"] = texture
+    return replace(sc, **kw)
+
+
+def camera_position(sc: DemScene) -> tuple:
+    if sc.sphere_radius > 0:
+        return (sc.sphere_radius + sc.altitude, 0.0, 0.0)
+    return (0.0, 0.0, sc.altitude)
+
+
+def rebase_offset(sc: DemScene) -> tuple:
+    """FP64 origin subtracted from every position before the FP32 store."""
+    return camera_position(sc) if sc.sphere_radius > 0 else (0.0, 0.0, 0.0)
+
+
+def make_projection(sc: DemScene):
+    if sc.sphere_radius > 0:
+        return S.projection(1, sc.spacing, sc.sphere_radius, (0.0, 0.0, 0.0), rebase_offset(sc))
+    return S.projection(0, sc.spacing)
+
+
+def make_camera(sc: DemScene) -> S.Camera:
+    pos = np.asarray(camera_position(sc)) - np.asarray(rebase_offset(sc))
+    cam = S.Camera(sc.image, sc.image, sc.fov_deg, tuple(pos))
+    if sc.sphere_radius > 0:
+        cam.look_in_direction((-1.0, 0.0, 0.0), up=(0.0, 0.0, 1.0))
+    else:
+        cam.look_in_direction((0.0, 0.0, -1.0))
+    return cam
+
+
+def make_lights(sc: DemScene):
+    off = np.asarray(rebase_offset(sc))
+    if sc.sphere_radius > 0:
+        up, east = (1.0, 0.0, 0.0), (0.0, 1.0, 0.0)
+        ground = np.array([sc.sphere_radius, 0.0, 0.0])
+    else:
+        up, east = (0.0, 0.0, 1.0), (1.0, 0.0, 0.0)
+        ground = np.zeros(3)
+    origin = ground - off
+    if sc.sun_kind == "point":
+        return [S.directional_sun(sc.sun_elevation_deg, 0.0, 1.0, origin=origin, up=up, east=east)]
+    pl = S.directional_sun(sc.sun_elevation_deg, 0.0, 1.0, origin=origin, up=up, east=east)
+    return [S.new_sun(pl.position)]
+
+
+def make_options(sc: DemScene) -> S.PathTracerOptions:
+    return S.PathTracerOptions(samples=sc.spp, bounces=sc.bounces, russian_roulette=sc.russian_roulette,
+                               rr_start=1, seed=sc.render_seed)
+
+
+def make_texture(ctx: S.Context, sc: DemScene):
+    """Seeded log-normal grey albedo texture (median 0.11, sigma 0.25)."""
+    import torch
+    g = torch.Generator(device=f"cuda:{ctx.device}")
+    g.manual_seed(1000 + sc.seed)
+    z = torch.randn((sc.texture, sc.texture), generator=g, device=f"cuda:{ctx.device}", dtype=torch.float32)
+    return (0.11 * torch.exp(0.25 * z)).contiguous()
+
+
+@dataclass
+class BuiltScene:
+    scene: DemScene
+    mesh: int
+    heights: object
+    vertex_albedo: object
+    texture: object
+    camera: S.Camera
+    lights: list
+    options: S.PathTracerOptions
+    bvh: dict
+
+
+def build(ctx: S.Context, sc: DemScene, bvh_max_leaf: int = 4, bvh_bins: int = 16) -> BuiltScene:
+    heights = ctx.synth_fbm(sc.dem, sc.dem, sc.seed, sc.amplitude, sc.octaves, sc.base_cells)
+    valb = ctx.synth_vertex_albedo(sc.dem * sc.dem, sc.seed) if sc.albedo == "vertex" else None
+    mesh = ctx.dem_mesh(heights, make_projection(sc), valb)
+    bvh = ctx.blas_build(mesh, bvh_max_leaf, bvh_bins)
+    ctx.tlas_build([mesh])
+    lights = make_lights(sc)
+    ctx.lights_set(lights)
+    tex = make_texture(ctx, sc) if sc.albedo == "texture" else None
+    ctx.material_set(mesh, albedo=1.0, texture=tex, smooth_shading=sc.smooth_shading)
+    return BuiltScene(sc, mesh, heights, valb, tex, make_camera(sc), lights, make_options(sc), bvh)
