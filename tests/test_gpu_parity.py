@@ -128,7 +128,7 @@ def test_trace_rays_bit_exact(gpu_ctx):
         occ_g = ida.cpu().numpy().view(np.uint32) != 0xFFFFFFFF
         occ_o = oida != 0xFFFFFFFF
         assert np.array_equal(occ_g, occ_o), f
-        assert np.array_equal(np.isfinite(ta.cpu().numpy()), ~occ_g)
+        assert np.array_equal(np.isfinite(ta.cpu().numpy()), occ_g)
 
 
 def test_render_config0_parity(gpu_ctx):

@@ -54,10 +54,12 @@ CONFIG1 = DemScene("dem4097_2048px_16spp_3b", dem=4097, seed=1, sphere_radius=MO
 
 def scaled(sc: DemScene, dem: int | None = None, image: int | None = None, spp: int | None = None,
            bounces: int | None = None, texture: int | None = None) -> DemScene:
-    """A smaller copy of a scene (same generator and geometry rules)."""
+    """A smaller copy of a scene (same generator and geometry rules); the camera
+    altitude shrinks with the DEM so the DEM covers the same part of the image."""
     kw = {}
     if dem is not None:
         kw["dem"] = dem
+        kw["altitude"] = sc.altitude * (dem - 1) / (sc.dem - 1)
     if image is not None:
         kw["image"] = image
     if spp is not None:
