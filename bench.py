@@ -90,6 +90,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=1, help="time every n-th pixel row/column on the CPU")
     ap.add_argument("--counts-out", default=None, help="write the oracle traversal counts (fixture) to this path")
+    ap.add_argument("--integrator", default="auto", choices=["auto", "fused", "wavefront"])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU); "
@@ -109,6 +110,7 @@ def main() -> int:
     print(f"# scene {sc.name}: {sc.triangles} triangles, BVH build {built.bvh['build_ms']:.1f} ms (device), "
           f"setup {setup_s:.1f} s", file=sys.stderr, flush=True)
     passes = S.RenderPasses(sc.image, sc.image)
+    built.options.integrator = args.integrator
     for _ in range(args.warmup):
         ctx.render(built.camera, built.options, passes)
     torch.cuda.synchronize()
@@ -122,8 +124,10 @@ def main() -> int:
     shad = sum(s["shadow_rays"] for s in stats)
     bnc = sum(s["bounce_rays"] for s in stats)
     value = (prim + shad) / elapsed / 1e6
-    print(f"# {args.steps} frames in {elapsed:.3f} s: primary {prim}, shadow {shad}, bounce {bnc}; "
-          f"closest-hit kernels {sum(s['trace_closest_ms'] for s in stats):.1f} ms, any-hit "
+    fused = stats[0]["integrator"] == 1
+    print(f"# {args.steps} frames in {elapsed:.3f} s ({'fused' if fused else 'wavefront'}): primary {prim}, "
+          f"shadow {shad}, bounce {bnc}; fused kernel {sum(s['fused_ms'] for s in stats):.1f} ms, closest-hit "
+          f"kernels {sum(s['trace_closest_ms'] for s in stats):.1f} ms, any-hit "
           f"{sum(s['trace_shadow_ms'] for s in stats):.1f} ms, shading {sum(s['shade_ms'] for s in stats):.1f} ms",
           file=sys.stderr, flush=True)
 
@@ -141,17 +145,28 @@ def main() -> int:
     if counts is not None and counts.get("closest_rays", 0) > 0:
         vi = counts["closest_internal"] / counts["closest_rays"]
         vt = counts["closest_tris"] / counts["closest_rays"]
+        si = counts["any_internal"] / max(1, counts["shadow_rays"])
+        stt = counts["any_tris"] / max(1, counts["shadow_rays"])
         a_closest = algorithmic_bytes(vi, vt)
+        a_shadow = algorithmic_bytes(si, stt)
         closest_rays = prim + bnc
-        closest_ms = sum(s["trace_closest_ms"] for s in stats)
-        achieved = closest_rays * a_closest / (closest_ms / 1e3) / 1e9
+        if fused:
+            # one launch traces camera, bounce and shadow rays
+            kernel, launches = "k_render_fused", sum(s["fused_launches"] for s in stats)
+            ms = sum(s["fused_ms"] for s in stats)
+            nbytes = closest_rays * a_closest + shad * a_shadow
+        else:
+            kernel, launches = "k_trace_queue (closest-hit launches)", sum(s["trace_closest_launches"] for s in stats)
+            ms = sum(s["trace_closest_ms"] for s in stats)
+            nbytes = closest_rays * a_closest
+        achieved = nbytes / (ms / 1e3) / 1e9
         roofline = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_trace_closest",
-            "bytes_per_ray": round(a_closest, 1), "v_int": round(vi, 2), "v_tri": round(vt, 2),
-            "launches": sum(s["trace_closest_launches"] for s in stats),
-            "avg_launch_ms": round(closest_ms / max(1, sum(s["trace_closest_launches"] for s in stats)), 4),
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
+            "bytes_per_closest_ray": round(a_closest, 1), "bytes_per_shadow_ray": round(a_shadow, 1),
+            "v_int_closest": round(vi, 2), "v_tri_closest": round(vt, 2),
+            "v_int_shadow": round(si, 2), "v_tri_shadow": round(stt, 2),
+            "launches": launches, "avg_launch_ms": round(ms / max(1, launches), 4),
         }
     line = {
         "metric": "Mrays/s (primary+shadow), 33.5M-tri DEM @2048^2, 16 spp, 3 bounces",
@@ -166,7 +181,8 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded fBm DEM, log-normal albedo texture)",
-        "config": {"workload": sc.name, "dem": sc.dem, "triangles": sc.triangles, "image": sc.image,
+        "config": {"workload": sc.name, "integrator": "fused" if fused else "wavefront",
+                   "dem": sc.dem, "triangles": sc.triangles, "image": sc.image,
                    "spp": sc.spp, "bounces": sc.bounces, "russian_roulette": sc.russian_roulette,
                    "bins": 8, "bounce_rays_per_frame": bnc // args.steps,
                    "bvh_build_ms": round(built.bvh["build_ms"], 1)},

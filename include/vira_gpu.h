@@ -117,6 +117,7 @@ typedef struct {
     int russian_roulette;
     int rr_start;        /* first bounce whose continuation is subject to RR */
     uint32_t seed;
+    int integrator;      /* 0 auto, 1 fused per-pixel (<= 1 light), 2 wavefront queues */
 } vira_gpu_render_opts;
 
 /* Render passes (passes.hpp:51-95); every pointer optional, device memory,
@@ -142,6 +143,9 @@ typedef struct {
     double shade_ms;         /* sum over shading + accumulation launches */
     uint32_t trace_closest_launches;
     uint32_t trace_shadow_launches;
+    int integrator;          /* 1 fused, 2 wavefront */
+    double fused_ms;         /* device time of the fused renderer launch(es) */
+    uint32_t fused_launches;
 } vira_gpu_render_stats;
 
 /* --- context ------------------------------------------------------------ */

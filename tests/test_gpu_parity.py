@@ -131,11 +131,14 @@ def test_trace_rays_bit_exact(gpu_ctx):
         assert np.array_equal(np.isfinite(ta.cpu().numpy()), occ_g)
 
 
-def test_render_config0_parity(gpu_ctx):
+@pytest.mark.parametrize("integrator", ["fused", "wavefront"])
+def test_render_config0_parity(gpu_ctx, integrator):
     sc = W.CONFIG0
     built = W.build(gpu_ctx, sc)
     rp = S.RenderPasses(sc.image, sc.image)
+    built.options.integrator = integrator
     st = gpu_ctx.render(built.camera, built.options, rp)
+    assert st["integrator"] == (1 if integrator == "fused" else 2)
     om, cam, lights, mat, opt = oracle_scene(built)
     o = om.render(cam, lights, mat, opt, precision=32)
     tri = rp.triangle_id.cpu().numpy().reshape(-1).view(np.uint32)
@@ -161,10 +164,12 @@ def test_render_config0_parity(gpu_ctx):
     assert np.median(err[lit]) < 1e-5
 
 
-def test_render_bounces_sphere_light_small(gpu_ctx):
+@pytest.mark.parametrize("integrator", ["fused", "wavefront"])
+def test_render_bounces_sphere_light_small(gpu_ctx, integrator):
     sc = W.scaled(W.CONFIG1, dem=129, image=64, spp=4, texture=64)
     built = W.build(gpu_ctx, sc)
     rp = S.RenderPasses(sc.image, sc.image)
+    built.options.integrator = integrator
     st = gpu_ctx.render(built.camera, built.options, rp)
     om, cam, lights, mat, opt = oracle_scene(built)
     o = om.render(cam, lights, mat, opt, precision=32)
@@ -178,6 +183,52 @@ def test_render_bounces_sphere_light_small(gpu_ctx):
     assert np.mean(rel[lit] <= 1e-5) > 0.97
     assert abs(rad.sum() / ref.sum() - 1) < 1e-3
     assert st["bounce_rays"] > 0
+
+
+def test_render_two_point_lights_wavefront(gpu_ctx):
+    """Several lights: shadow rays per light, contributions summed in light order."""
+    sc = W.scaled(W.CONFIG0, dem=97, image=96, spp=2, bounces=1)
+    built = W.build(gpu_ctx, sc)
+    lights = built.lights + [S.directional_sun(55.0, 140.0, [0.5, 1, 1.5, 2, 2.5, 3, 3.5, 4])]
+    gpu_ctx.lights_set(lights)
+    rp = S.RenderPasses(sc.image, sc.image)
+    st = gpu_ctx.render(built.camera, built.options, rp)
+    assert st["integrator"] == 2
+    om, cam, _, mat, opt = oracle_scene(built)
+    o = om.render(cam, [O.light_from(L.to_c()) for L in lights], mat, opt)
+    assert np.array_equal(rp.triangle_id.cpu().numpy().reshape(-1).view(np.uint32), o["tri_id"])
+    rad = rp.total_radiance.cpu().numpy().reshape(-1, 8).astype(np.float64)
+    dr = rp.direct_radiance.cpu().numpy().reshape(-1, 8)
+    ref = o["radiance"].astype(np.float64)
+    lit = ref > 0
+    rel = np.abs(rad - ref) / np.maximum(ref, 1e-30)
+    assert np.mean(rel[lit] <= 1e-5) > 0.97
+    # bounce-0 light from point lights has no transcendental step: exact
+    assert np.array_equal(dr.view(np.uint32), o["direct"].view(np.uint32))
+    from vira_amd import ViraGPUError
+    built.options.integrator = "fused"
+    with pytest.raises(ViraGPUError) as e:
+        gpu_ctx.render(built.camera, built.options, rp)
+    assert e.value.status == 5
+    gpu_ctx.lights_set(built.lights)
+
+
+def test_fused_equals_wavefront_bounces(gpu_ctx):
+    """The two schedules run the same formulas: identical images."""
+    sc = W.scaled(W.CONFIG1, dem=257, image=128, spp=3, texture=256)
+    built = W.build(gpu_ctx, sc)
+    out = {}
+    for integ in ("fused", "wavefront"):
+        rp = S.RenderPasses(sc.image, sc.image)
+        built.options.integrator = integ
+        st = gpu_ctx.render(built.camera, built.options, rp)
+        out[integ] = (rp, st)
+    a, b = out["fused"][0], out["wavefront"][0]
+    for name in ("total_radiance", "direct_radiance", "albedo", "normal_global", "depth", "alpha", "triangle_id"):
+        x, y = getattr(a, name).cpu().numpy(), getattr(b, name).cpu().numpy()
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32)), name
+    sa, sb = out["fused"][1], out["wavefront"][1]
+    assert (sa["primary_rays"], sa["shadow_rays"], sa["bounce_rays"]) == (sb["primary_rays"], sb["shadow_rays"], sb["bounce_rays"])
 
 
 def test_errors_are_status_codes(gpu_ctx):

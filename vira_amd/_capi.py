@@ -99,6 +99,7 @@ class RenderOpts(C.Structure):
         ("russian_roulette", C.c_int),
         ("rr_start", C.c_int),
         ("seed", C.c_uint32),
+        ("integrator", C.c_int),
     ]
 
 
@@ -126,6 +127,9 @@ class RenderStats(C.Structure):
         ("shade_ms", C.c_double),
         ("trace_closest_launches", C.c_uint32),
         ("trace_shadow_launches", C.c_uint32),
+        ("integrator", C.c_int),
+        ("fused_ms", C.c_double),
+        ("fused_launches", C.c_uint32),
     ]
 
 

@@ -91,7 +91,7 @@ struct Ctx {
     std::vector<vira_gpu_light> lights;
     // render scratch, grown on demand
     DBuf r_ray_o, r_ray_d, r_hit, r_queue[2], r_beta, r_path_rad, r_path_dir, r_srays_o, r_srays_d,
-        r_scontrib, r_svis, r_sinfo, r_acc, r_counters;
+        r_scontrib, r_svis, r_sinfo, r_acc, r_counters, r_lights;
     hipEvent_t ev[8] = {};
 
     MeshG& mesh(int32_t id) {

@@ -1,0 +1,308 @@
+// shading.hpp -- surface interaction, light sampling and BSDF sampling (rows a6, a7, a10).
+//
+// CPUPathTracer::processIntersection (cpu_path_tracer.ipp:279-469) split into
+// three steps shared by both integrators (fused per-pixel and wavefront), so
+// the two compute the same numbers:
+//   surface_at()     interpolation, shading frame, albedo, shading point,
+//                    integer-ULP offset of the new origin;
+//   light_contrib()  Light::sample + the MIS-weighted Lambertian term of one
+//                    light (the shadow ray decides whether it is added);
+//   sample_bounce()  cosine-weighted direction, throughput update and the
+//                    Russian-roulette decision.
+// Operation order follows the reference expression by expression.
+#pragma once
+
+#include "device_math.hpp"
+#include "../../include/vira_gpu.h"
+
+namespace vg {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kInvPi = 0.318309886183790671538f;
+constexpr float kInf = __builtin_huge_valf();
+
+struct LightF {
+    int kind;
+    float px, py, pz;
+    float radius;
+    float value[NB];
+};
+// Lights live in device memory (a dynamically indexed by-value kernel argument
+// would be copied into registers).
+struct Lights {
+    int n;
+    const LightF* __restrict__ l;
+};
+struct CamF {
+    int W, H;
+    float kinv[6];
+    float rot[9];
+    float ox, oy, oz;
+    float zdir;
+};
+struct MatF {
+    float albedo[NB];
+    const float* tex;
+    int tw, th;
+    int smooth;
+};
+
+// Pixel tiles of 8x8: queue/accumulator index q = tile * 64 + (y & 7) * 8 + (x & 7).
+__device__ __forceinline__ void tile_xy(uint32_t q, uint32_t tiles_x, uint32_t& x, uint32_t& y) {
+    uint32_t tile = q >> 6, lane = q & 63u;
+    x = (tile % tiles_x) * 8u + (lane & 7u);
+    y = (tile / tiles_x) * 8u + (lane >> 3);
+}
+__device__ __forceinline__ uint32_t tile_index(uint32_t x, uint32_t y, uint32_t tiles_x) {
+    return (((y >> 3) * tiles_x + (x >> 3)) << 6) + ((y & 7u) << 3) + (x & 7u);
+}
+
+// Camera::pixelToRay for sample s of pixel (x, y): sample 0 through the pixel
+// corner, later samples jittered (cpu_path_tracer.ipp:178-192,
+// camera.ipp:1126-1163,1738-1749).
+__device__ __forceinline__ f3 camera_dir(const CamF& cam, uint32_t x, uint32_t y, uint32_t pix, uint32_t seed,
+                                         uint32_t s) {
+    float X = (float)x, Y = (float)y;
+    if (s > 0) {
+        X = X + rng_u01(seed, pix, s, SLOT_JITTER_X);
+        Y = Y + rng_u01(seed, pix, s, SLOT_JITTER_Y);
+    }
+    float hx = (cam.kinv[0] * X + cam.kinv[2] * Y) + cam.kinv[4];
+    float hy = (cam.kinv[1] * X + cam.kinv[3] * Y) + cam.kinv[5];
+    f3 dl = mk3(hx, cam.zdir * hy, cam.zdir);
+    f3 dg = mk3((cam.rot[0] * dl.x + cam.rot[1] * dl.y) + cam.rot[2] * dl.z,
+                (cam.rot[3] * dl.x + cam.rot[4] * dl.y) + cam.rot[5] * dl.z,
+                (cam.rot[6] * dl.x + cam.rot[7] * dl.y) + cam.rot[8] * dl.z);
+    return normalize(dg);
+}
+
+// offsetIntersection (rendering/ray.ipp:107-139)
+__device__ __forceinline__ f3 offset_point(f3 p, f3 n) {
+    int32_t oi[3] = {(int32_t)(256.0f * n.x), (int32_t)(256.0f * n.y), (int32_t)(256.0f * n.z)};
+    float pc[3] = {p.x, p.y, p.z}, nc[3] = {n.x, n.y, n.z}, r[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float pi = __int_as_float(__float_as_int(pc[k]) + ((pc[k] < 0.0f) ? -oi[k] : oi[k]));
+        r[k] = (fabsf(pc[k]) < (1.0f / 32.0f)) ? pc[k] + (1.0f / 65536.0f) * nc[k] : pi;
+    }
+    return mk3(r[0], r[1], r[2]);
+}
+
+// PowerHeuristic (cpu_path_tracer.ipp:472-478), one sample each
+__device__ __forceinline__ float power_heuristic(float f, float g) { return (f * f) / (f * f + g * g); }
+
+// Image::sampleUVs + interpolatePixel (image.ipp:589-670) on a grey texture
+__device__ __forceinline__ float tex_bilinear(const MatF& m, float u, float v) {
+    float rx = (float)m.tw, ry = (float)m.th;
+    float px = fmodf(u * rx, rx), py = fmodf(v * ry, ry);
+    if (px < 0.f) px += rx;
+    if (py < 0.f) py += ry;
+    if (!(px >= 0.f)) px = 0.f;  // non-finite uv: keep the fetch in range
+    if (!(py >= 0.f)) py = 0.f;
+    int x0 = (int)floorf(px), y0 = (int)floorf(py);
+    x0 = min(x0, m.tw - 1);
+    y0 = min(y0, m.th - 1);
+    int x1 = min(x0 + 1, m.tw - 1), y1 = min(y0 + 1, m.th - 1);
+    float p00 = m.tex[x0 + (size_t)y0 * m.tw], p10 = m.tex[x1 + (size_t)y0 * m.tw];
+    float p01 = m.tex[x0 + (size_t)y1 * m.tw], p11 = m.tex[x1 + (size_t)y1 * m.tw];
+    float dx = px - (float)x0, dy = py - (float)y0;
+    return ((((1.0f - dx) * (1.0f - dy)) * p00 + (dx * (1.0f - dy)) * p10) + ((1.0f - dx) * dy) * p01) +
+           (dx * dy) * p11;
+}
+
+// Light::sample (point_light.ipp:18-38, sphere_light.ipp:31-125)
+__device__ __forceinline__ void sample_light(const LightF& L, f3 p, uint32_t seed, uint32_t pix, uint32_t s,
+                                             uint32_t dim0, f3& dir, float& distance, float& pdf, float rad[NB]) {
+    f3 c = mk3(L.px, L.py, L.pz);
+    f3 toL = sub(c, p);
+    distance = length(toL);
+    float dInv = __fdiv_rn(1.0f, distance);
+    if (L.kind == 0) {
+        float dSq = distance * distance;
+        pdf = 1.0f;
+        dir = mul(dInv, toL);
+#pragma unroll
+        for (int k = 0; k < NB; ++k) rad[k] = __fdiv_rn(L.value[k], dSq);
+        return;
+    }
+    float radius = L.radius;
+    if (distance <= radius) {
+        pdf = __fdiv_rn(1.0f, (4.0f * kPi * radius * radius));
+        dir = distance < 1e-3f ? mk3(1.f, 0.f, 0.f) : mul(dInv, toL);
+        distance = radius;
+#pragma unroll
+        for (int k = 0; k < NB; ++k) rad[k] = L.value[k];
+        return;
+    }
+    float r1 = rng_u01(seed, pix, s, dim0), r2 = rng_u01(seed, pix, s, dim0 + 1);
+    f3 W = mul(dInv, toL);
+    f3 N = (W.x < W.y) ? mk3(1.f, 0.f, 0.f) : mk3(0.f, 1.f, 0.f);
+    f3 Vv = normalize(cross(W, N));
+    f3 U = normalize(cross(W, Vv));
+    float phi = 2.0f * kPi * r1;
+    float zeta = r2;
+    float cc = __fdiv_rn(radius, distance);
+    float theta = acosf((1.0f - zeta) + zeta * __builtin_sqrtf(1.0f - cc * cc));
+    float sinT = sinf(theta), sinP = sinf(phi), cosT = cosf(theta), cosP = cosf(phi);
+    float a = cosP * sinT, bb = cosT, cz = sinP * sinT;
+    dir = mk3((a * U.x + bb * W.x) + cz * Vv.x, (a * U.y + bb * W.y) + cz * Vv.y, (a * U.z + bb * W.z) + cz * Vv.z);
+    float r2s = radius * radius;
+    float Omega = 2.0f * kPi * (1.0f - __fdiv_rn(__builtin_sqrtf(distance * distance - r2s), distance));
+    pdf = __fdiv_rn(1.0f, Omega);
+#pragma unroll
+    for (int k = 0; k < NB; ++k) rad[k] = L.value[k];
+}
+
+// Light::getPDF (point_light.ipp:40-57, sphere_light.ipp:127-153)
+__device__ __forceinline__ float light_pdf(const LightF& L, f3 p, f3 dir) {
+    f3 c = mk3(L.px, L.py, L.pz);
+    if (L.kind == 0) {
+        f3 toL = normalize(sub(c, p));
+        f3 q = normalize(dir);
+        float d = dot(toL, q);
+        return (d > 1.0f - 1e-6f) ? 1.0f : 0.0f;
+    }
+    f3 toC = sub(c, p);
+    float dc = length(toC);
+    float radius = L.radius;
+    if (dc <= radius) return __fdiv_rn(1.0f, (4.0f * kPi * radius * radius));
+    f3 tn = mul(toC, __fdiv_rn(1.0f, dc));
+    float cosMax = __builtin_sqrtf(1.0f - __fdiv_rn(radius * radius, dc * dc));
+    float ca = dot(dir, tn);
+    if (ca < cosMax) return 0.0f;
+    float Omega = 2.0f * kPi * (1.0f - __fdiv_rn(__builtin_sqrtf(dc * dc - radius * radius), dc));
+    return __fdiv_rn(1.0f, Omega);
+}
+
+struct Surface {
+    f3 org;         // offset shading point (origin of shadow and bounce rays)
+    f3 N, tg, bt;   // shading normal and tangent frame
+    float alb[NB];  // vertex albedo x material albedo x texture
+};
+
+// Interpolation, frame, albedo, shading point and offset for a hit
+// (cpu_path_tracer.ipp:284-349,386).
+__device__ __forceinline__ Surface surface_at(const GVertex* __restrict__ verts, const uint32_t* __restrict__ idx,
+                                              const MatF& mat, uint32_t id, float b1, float b2) {
+    Surface S;
+    const uint32_t i0 = idx[3 * (size_t)id], i1 = idx[3 * (size_t)id + 1], i2 = idx[3 * (size_t)id + 2];
+    const GVertex V0 = verts[i0], V1 = verts[i1], V2 = verts[i2];
+    const f3 p0 = mk3(V0.px, V0.py, V0.pz), p1 = mk3(V1.px, V1.py, V1.pz), p2 = mk3(V2.px, V2.py, V2.pz);
+    // Embree barycentrics: w = (u, v, 1-u-v) paired with (v1, v2, v0) (embree_tlas.ipp:160-166)
+    const float w0 = b1, w1 = b2, w2 = (1.0f - b1) - b2;
+    const f3 fn = normalize(cross(sub(p1, p0), sub(p2, p0)));  // == -normalize(cross(v0-v1, v2-v0))
+    const f3 n0 = mk3(V0.nx, V0.ny, V0.nz), n1 = mk3(V1.nx, V1.ny, V1.nz), n2 = mk3(V2.nx, V2.ny, V2.nz);
+    f3 Nl = fn;
+    if (mat.smooth) Nl = normalize(add(add(mul(w0, n1), mul(w1, n2)), mul(w2, n0)));
+    S.N = Nl;
+    const f3 arb = (fabsf(Nl.x) < 0.9f) ? mk3(1.f, 0.f, 0.f) : mk3(0.f, 1.f, 0.f);
+    S.tg = normalize(cross(arb, Nl));
+    S.bt = cross(Nl, S.tg);
+    float texv = 1.0f;
+    if (mat.tex) {
+        const float uvu = (w0 * V1.u + w1 * V2.u) + w2 * V0.u;
+        const float uvv = (w0 * V1.v + w1 * V2.v) + w2 * V0.v;
+        texv = tex_bilinear(mat, uvu, uvv);
+    }
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        float av = (w0 * V1.alb[k] + w1 * V2.alb[k]) + w2 * V0.alb[k];
+        float mv = mat.tex ? texv * mat.albedo[k] : mat.albedo[k];
+        S.alb[k] = av * mv;
+    }
+    f3 P = add(add(mul(w0, p1), mul(w1, p2)), mul(w2, p0));
+    {  // computeShadingPoint (cpu_path_tracer.ipp:481-511)
+        f3 q0 = sub(P, mul(dot(sub(P, p1), n1), n1));
+        f3 q1 = sub(P, mul(dot(sub(P, p2), n2), n2));
+        f3 q2 = sub(P, mul(dot(sub(P, p0), n0), n0));
+        f3 sp = add(add(mul(q0, w0), mul(q1, w1)), mul(q2, w2));
+        f3 diff = sub(sp, P);
+        if (dot(diff, Nl) > 0.0f) P = sp;
+    }
+    S.org = offset_point(P, fn);
+    return S;
+}
+
+// One light's direct-lighting term (cpu_path_tracer.ipp:399-428): returns
+// false when the light cannot be sampled (pdf == 0); otherwise the shadow ray
+// (S.org, dir, tmax = dist) and the contribution added if it is unoccluded.
+__device__ __forceinline__ bool light_contrib(const Surface& S, const LightF& L, const float beta[NB], uint32_t seed,
+                                              uint32_t pix, uint32_t s, uint32_t dim0, f3& dir, float& dist,
+                                              float contrib[NB]) {
+    float lpdf, lrad[NB];
+    sample_light(L, S.org, seed, pix, s, dim0, dir, dist, lpdf, lrad);
+    if (!(lpdf > 0.0f)) return false;
+    float ndl = dot(S.N, dir);
+    float mx = fmaxf(ndl, 0.0f);
+    float cth = fmaxf(0.0f, dot(dir, S.N));
+    float mpdf = (ndl <= 0.0f) ? 0.0f : ndl * kInvPi;
+    float wgt = power_heuristic(lpdf, mpdf);
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        float bsdf = (S.alb[k] * kInvPi) * mx;
+        contrib[k] = __fdiv_rn((((beta[k] * bsdf) * lrad[k]) * cth) * wgt, lpdf);
+    }
+    return true;
+}
+
+// BSDF sampling of the next bounce (cpu_path_tracer.ipp:431-461,
+// material.ipp:8-33, sampling.ipp:53-84) and Russian roulette.  Updates beta
+// and returns whether the path continues along nd.
+__device__ __forceinline__ bool sample_bounce(const Surface& S, const Lights& lights, float beta[NB], uint32_t seed,
+                                              uint32_t pix, uint32_t s, uint32_t bounce, int rr, int rr_start,
+                                              f3& nd) {
+    uint32_t db = bounce << 8;
+    float u1 = rng_u01(seed, pix, s, db | SLOT_BSDF_U);
+    float u2 = rng_u01(seed, pix, s, db | SLOT_BSDF_V);
+    float ox = 2.0f * u1 - 1.0f, oy = 2.0f * u2 - 1.0f;
+    float dx = 0.f, dy = 0.f;
+    if (!(fabsf(ox) < 1e-6f && fabsf(oy) < 1e-6f)) {
+        float r, th;
+        if (fabsf(ox) > fabsf(oy)) {
+            r = ox;
+            th = (kPi / 4.0f) * __fdiv_rn(oy, ox);
+        } else {
+            r = oy;
+            th = (kPi / 2.0f) - (kPi / 4.0f) * __fdiv_rn(ox, oy);
+        }
+        dx = r * cosf(th);
+        dy = r * sinf(th);
+    }
+    float lz = __builtin_sqrtf(fmaxf(0.0f, (1.0f - dx * dx) - dy * dy));
+    float mpdf = lz * kInvPi;
+    if (!(mpdf > 0.0f)) return false;
+    nd = mk3((S.tg.x * dx + S.bt.x * dy) + S.N.x * lz, (S.tg.y * dx + S.bt.y * dy) + S.N.y * lz,
+             (S.tg.z * dx + S.bt.z * dy) + S.N.z * lz);
+    float ndd = dot(S.N, nd);
+    float mx = fmaxf(ndd, 0.0f);
+    float cth = fmaxf(0.0f, dot(nd, S.N));
+    float lp = 0.0f;
+    for (int l = 0; l < lights.n; ++l) lp = lp + light_pdf(lights.l[l], S.org, nd);
+    if (lights.n > 0) lp = __fdiv_rn(lp, (float)lights.n);
+    float wgt = power_heuristic(mpdf, lp);
+    float qmax = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        float bsdf = (S.alb[k] * kInvPi) * mx;
+        beta[k] = beta[k] * __fdiv_rn((bsdf * cth) * wgt, mpdf);
+        qmax = fmaxf(qmax, beta[k]);
+    }
+    if (rr && (int)bounce + 1 >= rr_start) {
+        float qq = fminf(1.0f, qmax);
+        float ur = rng_u01(seed, pix, s, db | SLOT_RR);
+        if (ur >= qq) return false;
+#pragma unroll
+        for (int k = 0; k < NB; ++k) beta[k] = __fdiv_rn(beta[k], qq);
+    }
+    return true;
+}
+
+// Kahan-compensated add of x into (s, c).
+__device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
+    float y = x - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+}  // namespace vg

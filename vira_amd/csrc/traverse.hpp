@@ -21,16 +21,18 @@ struct HitRec {
 };
 
 // lds_ref/lds_dist point at this lane's column: entry k is at [k * stride].
-template <bool ANY, bool SAFE>
+// any != 0: occlusion query within [0, tmax] (returns on the first hit);
+// otherwise closest hit within [0, tmax], ties on t resolved to the lowest id.
+template <bool SAFE>
 __device__ __forceinline__ HitRec traverse_impl(const Node2* __restrict__ nodes, const float4* __restrict__ tris,
-                                                f3 o, f3 d, float tmax, uint32_t* lds_ref, float* lds_dist,
-                                                int stride) {
+                                                f3 o, f3 d, float tmax, bool any, uint32_t* lds_ref,
+                                                float* lds_dist, int stride) {
     HitRec h;
     h.t = __int_as_float(0x7f800000);
     h.id = 0xffffffffu;
     h.b1 = 0.f;
     h.b2 = 0.f;
-    float best = ANY ? tmax : __int_as_float(0x7f800000);
+    float best = tmax;
     const WT w = wt_setup(d);
     const f3 inv = mk3(__fdiv_rn(1.0f, d.x), __fdiv_rn(1.0f, d.y), __fdiv_rn(1.0f, d.z));
     const f3 e = mk3(slab_eps(d.x), slab_eps(d.y), slab_eps(d.z));
@@ -87,7 +89,7 @@ __device__ __forceinline__ HitRec traverse_impl(const Node2* __restrict__ nodes,
                 if (!wt_intersect(o, w, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), t, b1, b2))
                     continue;
                 uint32_t id = __float_as_uint(a.w);
-                if (ANY) {
+                if (any) {
                     if (t <= tmax) {
                         h.t = t;
                         h.id = id;
@@ -95,7 +97,7 @@ __device__ __forceinline__ HitRec traverse_impl(const Node2* __restrict__ nodes,
                         h.b2 = b2;
                         return h;
                     }
-                } else if (t < h.t || (t == h.t && id < h.id)) {
+                } else if (t <= tmax && (t < h.t || (t == h.t && id < h.id))) {
                     h.t = t;
                     h.id = id;
                     h.b1 = b1;
@@ -127,12 +129,12 @@ __device__ __forceinline__ HitRec traverse_impl(const Node2* __restrict__ nodes,
 }
 
 // Rays with a (near-)zero direction component take the SAFE slab test.
-template <bool ANY>
 __device__ __forceinline__ HitRec traverse(const Node2* __restrict__ nodes, const float4* __restrict__ tris, f3 o,
-                                           f3 d, float tmax, uint32_t* lds_ref, float* lds_dist, int stride) {
+                                           f3 d, float tmax, bool any, uint32_t* lds_ref, float* lds_dist,
+                                           int stride) {
     if (fabsf(d.x) < kTinyDir || fabsf(d.y) < kTinyDir || fabsf(d.z) < kTinyDir)
-        return traverse_impl<ANY, true>(nodes, tris, o, d, tmax, lds_ref, lds_dist, stride);
-    return traverse_impl<ANY, false>(nodes, tris, o, d, tmax, lds_ref, lds_dist, stride);
+        return traverse_impl<true>(nodes, tris, o, d, tmax, any, lds_ref, lds_dist, stride);
+    return traverse_impl<false>(nodes, tris, o, d, tmax, any, lds_ref, lds_dist, stride);
 }
 
 }  // namespace vg

@@ -187,11 +187,13 @@ class PathTracerOptions:
     russian_roulette: bool = False
     rr_start: int = 1
     seed: int = 0
+    integrator: str = "auto"  # "auto" | "fused" | "wavefront"
 
     def to_c(self) -> capi.RenderOpts:
         o = capi.RenderOpts()
         o.samples, o.bounces = int(self.samples), int(self.bounces)
         o.russian_roulette, o.rr_start, o.seed = int(self.russian_roulette), int(self.rr_start), int(self.seed)
+        o.integrator = {"auto": 0, "fused": 1, "wavefront": 2}[self.integrator]
         return o
 
 
