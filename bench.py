@@ -91,6 +91,9 @@ def main() -> int:
     ap.add_argument("--cpu-stride", type=int, default=1, help="time every n-th pixel row/column on the CPU")
     ap.add_argument("--counts-out", default=None, help="write the oracle traversal counts (fixture) to this path")
     ap.add_argument("--integrator", default="auto", choices=["auto", "fused", "wavefront"])
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (a known-size kernel for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU); "
@@ -102,6 +105,16 @@ def main() -> int:
     from vira_amd import workloads as W
 
     sc = W.CONFIG1 if args.scene == "c1" else W.CONFIG0
+    if args.copy_calibration_mib > 0:
+        n = args.copy_calibration_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"# calibration copy: {args.copy_calibration_mib} MiB read + {args.copy_calibration_mib} MiB written",
+              file=sys.stderr, flush=True)
+        del src, dst
     ctx = S.Context(0)
     t0 = time.time()
     built = W.build(ctx, sc)

@@ -66,10 +66,14 @@ typedef struct {
 typedef struct {
     int max_leaf;   /* triangles per leaf, 1..15 (default 4)  */
     int bins;       /* SAH bins per node, 4..32 (default 16)  */
+    int width;      /* 2 (binary, 64 B nodes) or 4 (collapsed, 128 B nodes; default) */
 } vira_gpu_bvh_opts;
 
 typedef struct {
     uint64_t num_nodes;      /* internal nodes of the binary tree      */
+    uint64_t num_wide_nodes; /* nodes of the traversal layout (Node2 or Node4) */
+    uint32_t width;          /* 2 or 4 */
+    uint32_t wide_depth;     /* depth of the traversal layout */
     uint64_t num_leaves;
     uint32_t max_depth;
     uint32_t max_leaf_size;

@@ -95,10 +95,12 @@ def _rays_over_dem(n, extent, rng):
     return np.concatenate([o, d.astype(np.float32)], 1)
 
 
-def test_trace_rays_bit_exact(gpu_ctx):
+@pytest.mark.parametrize("width", [2, 4])
+def test_trace_rays_bit_exact(gpu_ctx, width):
     import torch
     sc = W.CONFIG0
-    built = W.build(gpu_ctx, sc)
+    built = W.build(gpu_ctx, sc, bvh_width=width)
+    assert built.bvh["width"] == width and built.bvh["num_wide_nodes"] > 0
     om, *_ = oracle_scene(built)
     rng = np.random.default_rng(1)
     rays6 = _rays_over_dem(200000, 4500.0, rng)
@@ -131,10 +133,10 @@ def test_trace_rays_bit_exact(gpu_ctx):
         assert np.array_equal(np.isfinite(ta.cpu().numpy()), occ_g)
 
 
-@pytest.mark.parametrize("integrator", ["fused", "wavefront"])
-def test_render_config0_parity(gpu_ctx, integrator):
+@pytest.mark.parametrize("integrator,width", [("fused", 4), ("wavefront", 4), ("fused", 2)])
+def test_render_config0_parity(gpu_ctx, integrator, width):
     sc = W.CONFIG0
-    built = W.build(gpu_ctx, sc)
+    built = W.build(gpu_ctx, sc, bvh_width=width)
     rp = S.RenderPasses(sc.image, sc.image)
     built.options.integrator = integrator
     st = gpu_ctx.render(built.camera, built.options, rp)
@@ -229,6 +231,32 @@ def test_fused_equals_wavefront_bounces(gpu_ctx):
         assert np.array_equal(x.view(np.uint32), y.view(np.uint32)), name
     sa, sb = out["fused"][1], out["wavefront"][1]
     assert (sa["primary_rays"], sa["shadow_rays"], sa["bounce_rays"]) == (sb["primary_rays"], sb["shadow_rays"], sb["bounce_rays"])
+
+
+@pytest.mark.parametrize("max_leaf", [1, 4, 15])
+def test_bvh_leaf_sizes_and_small_meshes(gpu_ctx, max_leaf):
+    """Leaf-size limits, meshes smaller than a leaf, and a single triangle all trace exactly."""
+    import torch
+    rng = np.random.default_rng(max_leaf)
+    for ntri in (1, 3, 50, 2000):
+        c = rng.uniform(-10, 10, (ntri, 1, 3))
+        pos = (c + rng.normal(0, 0.7, (ntri, 3, 3))).reshape(-1, 3).astype(np.float32)
+        idx = np.arange(3 * ntri, dtype=np.uint32).reshape(-1, 3)
+        mid = gpu_ctx.mesh_upload(pos, idx)
+        st = gpu_ctx.blas_build(mid, max_leaf=max_leaf, width=4)
+        assert st["max_leaf_size"] <= max_leaf
+        o = rng.uniform(-15, 15, (4000, 3))
+        d = rng.normal(size=(4000, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        rays6 = np.concatenate([o, d], 1).astype(np.float32)
+        rays8 = np.zeros((4000, 8), np.float32)
+        rays8[:, :3], rays8[:, 4:7], rays8[:, 7] = rays6[:, :3], rays6[:, 3:], np.inf
+        t, ids, _, _, _ = gpu_ctx.trace_rays(mid, torch.from_numpy(rays8).cuda())
+        om = O.Mesh(pos, idx)
+        ot, oid, _, _, _ = om.trace(rays6, brute=True)
+        assert np.array_equal(ids.cpu().numpy().view(np.uint32), oid), ntri
+        assert np.array_equal(t.cpu().numpy().view(np.uint32), ot.view(np.uint32)), ntri
+        gpu_ctx.mesh_release(mid)
 
 
 def test_errors_are_status_codes(gpu_ctx):

@@ -47,12 +47,15 @@ class Projection(C.Structure):
 
 
 class BVHOpts(C.Structure):
-    _fields_ = [("max_leaf", C.c_int), ("bins", C.c_int)]
+    _fields_ = [("max_leaf", C.c_int), ("bins", C.c_int), ("width", C.c_int)]
 
 
 class BVHStats(C.Structure):
     _fields_ = [
         ("num_nodes", C.c_uint64),
+        ("num_wide_nodes", C.c_uint64),
+        ("width", C.c_uint32),
+        ("wide_depth", C.c_uint32),
         ("num_leaves", C.c_uint64),
         ("max_depth", C.c_uint32),
         ("max_leaf_size", C.c_uint32),

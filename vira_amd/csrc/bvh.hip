@@ -407,6 +407,93 @@ __global__ void k_emit_leaf_root(const BNodeG* nodes, Node2* out) {
     out[0] = o;
 }
 
+// Binary tree -> four-wide nodes, one level of the four-wide tree per launch.
+// Each item turns binary internal node `bnode` into Node4 `n4`: starting from
+// its two children, the internal child of largest surface area is replaced by
+// its own two children until there are four (or only leaves remain).
+struct CItem {
+    uint32_t bnode, n4;
+};
+
+__global__ void k_collapse4(const BNodeG* __restrict__ nodes, const CItem* __restrict__ in, uint32_t nin,
+                            Node4* __restrict__ out, uint32_t* __restrict__ n4_counter, CItem* __restrict__ qout,
+                            uint32_t* __restrict__ qout_count, uint32_t* __restrict__ err) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nin) return;
+    CItem it = in[i];
+    const BNodeG& bn = nodes[it.bnode];
+    uint32_t ch[4] = {bn.left, bn.left + 1, 0u, 0u};
+    int nc = 2;
+    while (nc < 4) {
+        int best = -1;
+        float bestA = -1.0f;
+        for (int k = 0; k < nc; ++k) {
+            const BNodeG& c = nodes[ch[k]];
+            if (c.count != 0) continue;
+            float A = half_area(c.lo, c.hi);
+            if (A > bestA) {
+                bestA = A;
+                best = k;
+            }
+        }
+        if (best < 0) break;
+        uint32_t l = nodes[ch[best]].left;
+        ch[best] = l;
+        ch[nc++] = l + 1;
+    }
+    float lo[3][4], hi[3][4];
+    uint32_t ref[4];
+    for (int k = 0; k < 4; ++k) {
+        if (k >= nc) {
+            for (int a = 0; a < 3; ++a) lo[a][k] = hi[a][k] = 0.0f;
+            ref[k] = kEmptyRef;
+            continue;
+        }
+        const BNodeG& c = nodes[ch[k]];
+        for (int a = 0; a < 3; ++a) {
+            lo[a][k] = c.lo[a];
+            hi[a][k] = c.hi[a];
+        }
+        if (c.count != 0) {
+            if (c.count > 16) {
+                atomicOr(err, 1u);
+                ref[k] = kEmptyRef;
+            } else {
+                ref[k] = make_leaf(c.left, c.count);
+            }
+        } else {
+            uint32_t n = atomicAdd(n4_counter, 1u);
+            uint32_t slot = atomicAdd(qout_count, 1u);
+            qout[slot] = CItem{ch[k], n};
+            ref[k] = n;
+        }
+    }
+    Node4 o;
+    o.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+    o.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+    o.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+    o.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+    o.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+    o.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+    o.ref = make_uint4(ref[0], ref[1], ref[2], ref[3]);
+    o.pad = make_uint4(0u, 0u, 0u, 0u);
+    out[it.n4] = o;
+}
+
+__global__ void k_emit_leaf_root4(const BNodeG* nodes, Node4* out) {
+    const BNodeG& a = nodes[0];
+    Node4 o;
+    o.lox = make_float4(a.lo[0], 0.f, 0.f, 0.f);
+    o.hix = make_float4(a.hi[0], 0.f, 0.f, 0.f);
+    o.loy = make_float4(a.lo[1], 0.f, 0.f, 0.f);
+    o.hiy = make_float4(a.hi[1], 0.f, 0.f, 0.f);
+    o.loz = make_float4(a.lo[2], 0.f, 0.f, 0.f);
+    o.hiz = make_float4(a.hi[2], 0.f, 0.f, 0.f);
+    o.ref = make_uint4(make_leaf(a.left, a.count), kEmptyRef, kEmptyRef, kEmptyRef);
+    o.pad = make_uint4(0u, 0u, 0u, 0u);
+    out[0] = o;
+}
+
 __global__ void k_emit_tris(size_t nt, const uint32_t* prims, const uint32_t* idx, const GVertex* V, float4* tris) {
     size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nt) return;
@@ -465,10 +552,13 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     VG_HIP(hipEventCreate(&e1));
     VG_HIP(hipEventRecord(e0, c.stream));
     if (N == 0) {
-        B.nodes.alloc(sizeof(Node2));
-        Node2 empty{};
-        empty.ref = make_uint4(kEmptyRef, kEmptyRef, 0, 0);
-        VG_HIP(hipMemcpyAsync(B.nodes.p, &empty, sizeof(Node2), hipMemcpyHostToDevice, c.stream));
+        // one root with only empty children, valid as a Node2 (refs at word 12) and as a Node4 (refs at word 24)
+        B.width = opt.width == 2 ? 2 : 4;
+        B.nodes.alloc(sizeof(Node4));
+        uint32_t empty[32] = {};
+        empty[12] = empty[13] = kEmptyRef;
+        for (int k = 24; k < 28; ++k) empty[k] = kEmptyRef;
+        VG_HIP(hipMemcpyAsync(B.nodes.p, empty, sizeof(empty), hipMemcpyHostToDevice, c.stream));
         B.tris.alloc(16);
         B.num_nodes = 1;
         VG_HIP(hipStreamSynchronize(c.stream));
@@ -696,25 +786,65 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     VG_CHECK(max_depth < 63, VIRA_GPU_ERR_HIP, "BVH deeper than the traversal stack");
     VG_CHECK(nn <= 2 * N, VIRA_GPU_ERR_HIP, "BVH node count overflow");
 
-    // layout: internal nodes -> Node2, triangles -> leaf order (counter word 1 becomes the error flag)
+    // layout: internal nodes -> Node2 / Node4, triangles -> leaf order (counter word 1 becomes the error flag)
     VG_HIP(hipMemsetAsync(d_counter.as<char>() + 4, 0, 4, c.stream));
-    DBuf flag, iidx;
-    flag.alloc((size_t)nn * 4);
-    iidx.alloc((size_t)nn * 4);
-    k_flag_internal<<<blocks_for(nn, 256), 256, 0, c.stream>>>(nodes.as<BNodeG>(), nn, flag.as<uint32_t>());
-    VG_HIP(hipGetLastError());
-    uint32_t ninternal = scan_exclusive_u32(c, flag.as<uint32_t>(), iidx.as<uint32_t>(), nn);
-    B.num_nodes = std::max<uint32_t>(1, ninternal);
-    B.nodes.alloc(B.num_nodes * sizeof(Node2));
-    B.tris.alloc(N * 3 * sizeof(float4));
-    if (ninternal == 0) {
-        k_emit_leaf_root<<<1, 1, 0, c.stream>>>(nodes.as<BNodeG>(), B.nodes.as<Node2>());
+    uint32_t* d_err = reinterpret_cast<uint32_t*>(d_counter.as<char>() + 4);
+    B.width = opt.width == 2 ? 2 : 4;
+    uint32_t wide_depth = 0;
+    if (B.width == 2) {
+        DBuf flag, iidx;
+        flag.alloc((size_t)nn * 4);
+        iidx.alloc((size_t)nn * 4);
+        k_flag_internal<<<blocks_for(nn, 256), 256, 0, c.stream>>>(nodes.as<BNodeG>(), nn, flag.as<uint32_t>());
+        VG_HIP(hipGetLastError());
+        uint32_t ninternal = scan_exclusive_u32(c, flag.as<uint32_t>(), iidx.as<uint32_t>(), nn);
+        B.num_nodes = std::max<uint32_t>(1, ninternal);
+        B.nodes.alloc(B.num_nodes * sizeof(Node2));
+        if (ninternal == 0) {
+            k_emit_leaf_root<<<1, 1, 0, c.stream>>>(nodes.as<BNodeG>(), B.nodes.as<Node2>());
+        } else {
+            k_emit_nodes<<<blocks_for(nn, 256), 256, 0, c.stream>>>(nodes.as<BNodeG>(), nn, iidx.as<uint32_t>(),
+                                                                    B.nodes.as<Node2>(), d_err);
+        }
+        VG_HIP(hipGetLastError());
+        wide_depth = max_depth;
     } else {
-        k_emit_nodes<<<blocks_for(nn, 256), 256, 0, c.stream>>>(nodes.as<BNodeG>(), nn, iidx.as<uint32_t>(),
-                                                                B.nodes.as<Node2>(),
-                                                                reinterpret_cast<uint32_t*>(d_counter.as<char>() + 4));
+        // a full binary tree of nn nodes has (nn - 1) / 2 internal nodes, each at most one Node4
+        const size_t cap = std::max<size_t>(1, (nn - 1) / 2);
+        B.nodes.alloc(cap * sizeof(Node4));
+        if (nn == 1) {
+            k_emit_leaf_root4<<<1, 1, 0, c.stream>>>(nodes.as<BNodeG>(), B.nodes.as<Node4>());
+            VG_HIP(hipGetLastError());
+            B.num_nodes = 1;
+        } else {
+            DBuf qa, qb, qc;
+            qa.alloc(cap * sizeof(CItem));
+            qb.alloc(cap * sizeof(CItem));
+            qc.alloc(8);
+            CItem root{0u, 0u};
+            VG_HIP(hipMemcpyAsync(qa.p, &root, sizeof(root), hipMemcpyHostToDevice, c.stream));
+            uint32_t hq[2] = {1u, 0u};  // node4 counter, next-queue length
+            uint32_t nin = 1;
+            CItem* qin = qa.as<CItem>();
+            CItem* qout = qb.as<CItem>();
+            while (nin > 0) {
+                hq[1] = 0;
+                VG_HIP(hipMemcpyAsync(qc.p, hq, 8, hipMemcpyHostToDevice, c.stream));
+                k_collapse4<<<blocks_for(nin, 256), 256, 0, c.stream>>>(nodes.as<BNodeG>(), qin, nin,
+                                                                        B.nodes.as<Node4>(), qc.as<uint32_t>(), qout,
+                                                                        qc.as<uint32_t>() + 1, d_err);
+                VG_HIP(hipGetLastError());
+                VG_HIP(hipMemcpyAsync(hq, qc.p, 8, hipMemcpyDeviceToHost, c.stream));
+                VG_HIP(hipStreamSynchronize(c.stream));
+                VG_CHECK(hq[0] <= cap, VIRA_GPU_ERR_HIP, "BVH4 collapse overflow");
+                nin = hq[1];
+                std::swap(qin, qout);
+                wide_depth++;
+            }
+            B.num_nodes = hq[0];
+        }
     }
-    VG_HIP(hipGetLastError());
+    B.tris.alloc(N * 3 * sizeof(float4));
     k_emit_tris<<<blocks_for(N, 256), 256, 0, c.stream>>>(N, prims.as<uint32_t>(), m.idx.as<uint32_t>(),
                                                            m.verts.as<GVertex>(), B.tris.as<float4>());
     VG_HIP(hipGetLastError());
@@ -745,12 +875,15 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     std::memcpy(&sah, hs + 24, 8);
     B.stats = vira_gpu_bvh_stats{};
     B.stats.num_nodes = cnts[0];
+    B.stats.num_wide_nodes = B.num_nodes;
+    B.stats.width = (uint32_t)B.width;
+    B.stats.wide_depth = wide_depth;
     B.stats.num_leaves = cnts[1];
     B.stats.max_depth = max_depth + 1;
     B.stats.max_leaf_size = mleaf;
     B.stats.sah_cost = sah;
     B.stats.build_ms = ms;
-    B.stats.bytes = B.num_nodes * sizeof(Node2) + N * 3 * sizeof(float4);
+    B.stats.bytes = B.num_nodes * (B.width == 2 ? sizeof(Node2) : sizeof(Node4)) + N * 3 * sizeof(float4);
     B.built = true;
 }
 

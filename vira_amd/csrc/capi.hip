@@ -229,10 +229,12 @@ vira_gpu_status vira_gpu_blas_build(vira_gpu_ctx* ctx, int32_t mesh_id, const vi
     return guarded([&] {
         vg::Ctx& c = C(ctx);
         set_device(c);
-        vira_gpu_bvh_opts o{4, 16};
+        vira_gpu_bvh_opts o{4, 16, 4};
         if (opts) o = *opts;
+        if (o.width == 0) o.width = 4;
         VG_CHECK(o.max_leaf >= 0 && o.max_leaf <= 15 && o.bins >= 0 && o.bins <= 32, VIRA_GPU_ERR_INVALID,
                  "blas_build: max_leaf must be 1..15 and bins 4..32");
+        VG_CHECK(o.width == 2 || o.width == 4, VIRA_GPU_ERR_INVALID, "blas_build: width must be 2 or 4");
         vg::blas_build(c, c.mesh(mesh_id), o);
     });
 }

@@ -67,7 +67,8 @@ struct DBuf {
 
 struct Blas {
     bool built = false;
-    DBuf nodes;   // Node2[num_nodes]
+    int width = 2;
+    DBuf nodes;   // Node2[num_nodes] or Node4[num_nodes]
     DBuf tris;    // float4[3 * nt] in leaf order: v0.xyz+id, v1.xyz, v2.xyz
     uint64_t num_nodes = 0;
     vira_gpu_bvh_stats stats{};

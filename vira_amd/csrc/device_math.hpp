@@ -189,6 +189,16 @@ struct __attribute__((aligned(16))) Node2 {
     uint4 ref; // ref0 ref1 - -
 };
 constexpr uint32_t kEmptyRef = 0xffffffffu;
+
+// Four-wide node (128 B, two cache lines): per axis one float4 of the four
+// children's lower bounds and one of their upper bounds, then the four child
+// references (same encoding as Node2).
+struct __attribute__((aligned(16))) Node4 {
+    float4 lox, hix, loy, hiy, loz, hiz;
+    uint4 ref;
+    uint4 pad;
+};
+static_assert(sizeof(Node4) == 128, "Node4 is two cache lines");
 __device__ __host__ __forceinline__ bool ref_is_leaf(uint32_t r) { return (r & 0x80000000u) != 0; }
 __device__ __host__ __forceinline__ uint32_t leaf_first(uint32_t r) { return (r >> 4) & 0x7ffffffu; }
 __device__ __host__ __forceinline__ uint32_t leaf_count(uint32_t r) { return (r & 15u) + 1u; }

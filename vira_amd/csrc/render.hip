@@ -153,7 +153,8 @@ __device__ __forceinline__ void acc_clear(float* __restrict__ a, uint32_t lane) 
 }
 
 // ------------------------------------------------------------ trace_rays hook
-__global__ void __launch_bounds__(kTraceBS) k_trace_rays(const Node2* __restrict__ nodes, const float4* __restrict__ tris,
+template <int W>
+__global__ void __launch_bounds__(kTraceBS) k_trace_rays(const void* __restrict__ nodes, const float4* __restrict__ tris,
                                                          const float* __restrict__ rays, size_t n, int any,
                                                          float* t_out, uint32_t* id_out, float* b1_out, float* b2_out) {
     __shared__ uint32_t s_ref[kStackLds * kTraceBS];
@@ -162,7 +163,7 @@ __global__ void __launch_bounds__(kTraceBS) k_trace_rays(const Node2* __restrict
     if (r >= n) return;
     const float4 a = reinterpret_cast<const float4*>(rays)[2 * r];
     const float4 b = reinterpret_cast<const float4*>(rays)[2 * r + 1];
-    HitRec h = traverse(nodes, tris, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), b.w, any != 0, s_ref + threadIdx.x,
+    HitRec h = traverse<W>(nodes, tris, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), b.w, any != 0, s_ref + threadIdx.x,
                         s_dist + threadIdx.x, kTraceBS);
     if (t_out) t_out[r] = any ? (h.id != 0xffffffffu ? 0.0f : kInf) : h.t;
     if (id_out) id_out[r] = h.id;
@@ -173,8 +174,9 @@ __global__ void __launch_bounds__(kTraceBS) k_trace_rays(const Node2* __restrict
 // ============================================================ fused schedule
 enum : int { M_NEW = 0, M_CLOSEST = 1, M_SHADOW = 2, M_DONE = 3 };
 
+template <int W>
 __global__ void __launch_bounds__(kFusedBS) k_render_fused(
-    const Node2* __restrict__ nodes, const float4* __restrict__ tris, const GVertex* __restrict__ verts,
+    const void* __restrict__ nodes, const float4* __restrict__ tris, const GVertex* __restrict__ verts,
     const uint32_t* __restrict__ idx, MatF mat, Lights lights, CamF cam, uint32_t seed, int spp, int max_bounces,
     int rr, int rr_start, uint32_t tiles_x, float* __restrict__ acc, OutPtrs out,
     unsigned long long* __restrict__ counters) {
@@ -223,7 +225,7 @@ __global__ void __launch_bounds__(kFusedBS) k_render_fused(
         h.t = kInf;
         h.b1 = h.b2 = 0.f;
         if (mode != M_DONE)
-            h = traverse(nodes, tris, o, d, tmax, mode == M_SHADOW, s_ref + lane, s_dist + lane, kFusedBS);
+            h = traverse<W>(nodes, tris, o, d, tmax, mode == M_SHADOW, s_ref + lane, s_dist + lane, kFusedBS);
         bool vertex_done = false, lit = false;
         if (mode == M_CLOSEST) {
             if (h.id == 0xffffffffu) {
@@ -313,7 +315,8 @@ __global__ void k_camera(CamF cam, uint32_t seed, uint32_t s, uint32_t nq, uint3
     }
 }
 
-__global__ void __launch_bounds__(kTraceBS) k_trace_queue(const Node2* __restrict__ nodes,
+template <int W>
+__global__ void __launch_bounds__(kTraceBS) k_trace_queue(const void* __restrict__ nodes,
                                                           const float4* __restrict__ tris,
                                                           const float4* __restrict__ ray_o,
                                                           const float4* __restrict__ ray_d,
@@ -327,7 +330,7 @@ __global__ void __launch_bounds__(kTraceBS) k_trace_queue(const Node2* __restric
     float4 o = ray_o[e];
     if (__float_as_uint(o.w) == kNoPath) return;
     float4 d = ray_d[e];
-    HitRec h = traverse(nodes, tris, mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z), d.w, any != 0, s_ref + threadIdx.x,
+    HitRec h = traverse<W>(nodes, tris, mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z), d.w, any != 0, s_ref + threadIdx.x,
                         s_dist + threadIdx.x, kTraceBS);
     hit[e] = make_float4(h.t, h.b1, h.b2, __uint_as_float(h.id));
 }
@@ -471,8 +474,12 @@ void trace_rays(Ctx& c, MeshG& m, const float* rays, size_t n, int any_hit, floa
     VG_HIP(hipEventCreate(&e0));
     VG_HIP(hipEventCreate(&e1));
     VG_HIP(hipEventRecord(e0, c.stream));
-    k_trace_rays<<<blocks_for(n, kTraceBS), kTraceBS, 0, c.stream>>>(m.blas.nodes.as<Node2>(), m.blas.tris.as<float4>(),
-                                                                      rays, n, any_hit, t, id, b1, b2);
+    if (m.blas.width == 4)
+        k_trace_rays<4><<<blocks_for(n, kTraceBS), kTraceBS, 0, c.stream>>>(m.blas.nodes.p, m.blas.tris.as<float4>(),
+                                                                            rays, n, any_hit, t, id, b1, b2);
+    else
+        k_trace_rays<2><<<blocks_for(n, kTraceBS), kTraceBS, 0, c.stream>>>(m.blas.nodes.p, m.blas.tris.as<float4>(),
+                                                                            rays, n, any_hit, t, id, b1, b2);
     VG_HIP(hipGetLastError());
     VG_HIP(hipEventRecord(e1, c.stream));
     VG_HIP(hipEventSynchronize(e1));
@@ -538,7 +545,8 @@ void render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, c
     uint32_t* qcount = reinterpret_cast<uint32_t*>(c.r_counters.as<char>() + 64);  // uint32 queue lengths per bounce
     const int maxq = 64;
     VG_HIP(hipMemsetAsync(c.r_counters.p, 0, 64 * 8, c.stream));
-    const Node2* nodes = m.blas.nodes.as<Node2>();
+    const void* nodes = m.blas.nodes.p;
+    const bool w4 = m.blas.width == 4;
     const float4* tris = m.blas.tris.as<float4>();
 
     std::vector<EvPair> evs;
@@ -558,9 +566,10 @@ void render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, c
 
     if (fused) {
         ev_begin(3);
-        k_render_fused<<<tiles, kFusedBS, 0, c.stream>>>(nodes, tris, m.verts.as<GVertex>(), m.idx.as<uint32_t>(), mf,
-                                                         lf, cf, o.seed, o.samples, o.bounces, o.russian_roulette,
-                                                         o.rr_start, tiles_x, acc, op, cnt);
+        auto kern = w4 ? k_render_fused<4> : k_render_fused<2>;
+        kern<<<tiles, kFusedBS, 0, c.stream>>>(nodes, tris, m.verts.as<GVertex>(), m.idx.as<uint32_t>(), mf, lf, cf,
+                                               o.seed, o.samples, o.bounces, o.russian_roulette, o.rr_start, tiles_x,
+                                               acc, op, cnt);
         VG_HIP(hipGetLastError());
         ev_end();
     } else {
@@ -587,8 +596,9 @@ void render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, c
                 const int cur = b & 1, nxt = cur ^ 1;
                 const uint32_t* nqp = b == 0 ? nullptr : &qcount[b];
                 ev_begin(0);
-                k_trace_queue<<<blocks_for(nq, kTraceBS), kTraceBS, 0, c.stream>>>(nodes, tris, ro[cur], rd[cur], nqp,
-                                                                                   nq, 1u, 0, c.r_hit.as<float4>());
+                auto tq = w4 ? k_trace_queue<4> : k_trace_queue<2>;
+                tq<<<blocks_for(nq, kTraceBS), kTraceBS, 0, c.stream>>>(nodes, tris, ro[cur], rd[cur], nqp, nq, 1u, 0,
+                                                                        c.r_hit.as<float4>());
                 VG_HIP(hipGetLastError());
                 ev_end();
                 ev_begin(2);
@@ -602,7 +612,7 @@ void render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, c
                 if (nl > 0) {
                     // the slot array holds nl entries per queue entry of this bounce (unused ones are marked)
                     ev_begin(1);
-                    k_trace_queue<<<blocks_for(nslots, kTraceBS), kTraceBS, 0, c.stream>>>(
+                    tq<<<blocks_for(nslots, kTraceBS), kTraceBS, 0, c.stream>>>(
                         nodes, tris, c.r_srays_o.as<float4>(), c.r_srays_d.as<float4>(), nqp, nq,
                         (uint32_t)nl, 1, c.r_svis.as<float4>());
                     VG_HIP(hipGetLastError());

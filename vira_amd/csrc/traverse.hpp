@@ -1,11 +1,13 @@
 // traverse.hpp -- BVH traversal with watertight intersection (row a5, a6).
 //
-// Replaces rtcIntersect1 (embree_tlas.ipp:128-173) and restates the order of
-// ViraBLAS::intersect (vira_blas.ipp:58-108): both child boxes are tested,
-// the nearer child is entered and the farther one pushed.  The stack's top 16
-// entries live in LDS (one column per lane, so a wave's pushes and pops hit 64
-// consecutive dwords); deeper entries spill to private memory, which the
-// tree depth (< 63) bounds.
+// Replaces rtcIntersect1 (embree_tlas.ipp:128-173) and follows the order of
+// ViraBLAS::intersect (vira_blas.ipp:58-108): all child boxes of a node are
+// tested, the nearest child is entered and the others are pushed far-to-near.
+// Two node layouts: Node2 (binary, 64 B) and Node4 (four-wide, 128 B).
+// The stack's top 16 entries live in LDS (one column per lane, so a wave's
+// pushes and pops hit 64 consecutive dwords); deeper entries spill to private
+// memory.  Capacity: a binary tree of depth < 63 (the builder's bound) needs
+// at most 62 entries; its four-wide collapse at most 3 per level, 186.
 #pragma once
 
 #include "device_math.hpp"
@@ -13,6 +15,7 @@
 namespace vg {
 
 constexpr int kStackLds = 16;
+constexpr int kStackSpill = 176;
 
 struct HitRec {
     float t;
@@ -20,15 +23,56 @@ struct HitRec {
     float b1, b2;
 };
 
-// lds_ref/lds_dist point at this lane's column: entry k is at [k * stride].
-// any != 0: occlusion query within [0, tmax] (returns on the first hit);
+struct TravStack {
+    uint32_t* lds_ref;
+    float* lds_dist;
+    int stride;
+    int sp;
+    uint32_t spill_ref[kStackSpill];
+    float spill_dist[kStackSpill];
+
+    __device__ __forceinline__ void push(uint32_t r, float d) {
+        int slot = sp & (kStackLds - 1);
+        if (sp >= kStackLds) {  // the ring slot holds entry sp - 16: move it to private memory
+            spill_ref[sp - kStackLds] = lds_ref[slot * stride];
+            spill_dist[sp - kStackLds] = lds_dist[slot * stride];
+        }
+        lds_ref[slot * stride] = r;
+        lds_dist[slot * stride] = d;
+        sp++;
+    }
+    __device__ __forceinline__ void pop(uint32_t& r, float& d) {
+        sp--;
+        int slot = sp & (kStackLds - 1);
+        r = lds_ref[slot * stride];
+        d = lds_dist[slot * stride];
+        if (sp >= kStackLds) {  // bring entry sp - 16 back into the freed ring slot
+            lds_ref[slot * stride] = spill_ref[sp - kStackLds];
+            lds_dist[slot * stride] = spill_dist[sp - kStackLds];
+        }
+    }
+};
+
+__device__ __forceinline__ void cswap(float& da, uint32_t& ra, float& db, uint32_t& rb) {
+    if (db < da) {
+        float t = da;
+        da = db;
+        db = t;
+        uint32_t u = ra;
+        ra = rb;
+        rb = u;
+    }
+}
+
+// any: occlusion query within [0, tmax] (returns on the first hit);
 // otherwise closest hit within [0, tmax], ties on t resolved to the lowest id.
-template <bool SAFE>
-__device__ __forceinline__ HitRec traverse_impl(const Node2* __restrict__ nodes, const float4* __restrict__ tris,
-                                                f3 o, f3 d, float tmax, bool any, uint32_t* lds_ref,
-                                                float* lds_dist, int stride) {
+template <int W, bool SAFE>
+__device__ __forceinline__ HitRec traverse_impl(const void* __restrict__ nodes_v, const float4* __restrict__ tris,
+                                                f3 o, f3 d, float tmax, bool any, uint32_t* lds_ref, float* lds_dist,
+                                                int stride) {
+    constexpr float kInfF = __builtin_huge_valf();
     HitRec h;
-    h.t = __int_as_float(0x7f800000);
+    h.t = kInfF;
     h.id = 0xffffffffu;
     h.b1 = 0.f;
     h.b2 = 0.f;
@@ -36,46 +80,71 @@ __device__ __forceinline__ HitRec traverse_impl(const Node2* __restrict__ nodes,
     const WT w = wt_setup(d);
     const f3 inv = mk3(__fdiv_rn(1.0f, d.x), __fdiv_rn(1.0f, d.y), __fdiv_rn(1.0f, d.z));
     const f3 e = mk3(slab_eps(d.x), slab_eps(d.y), slab_eps(d.z));
-    uint32_t spill_ref[64 - kStackLds];
-    float spill_dist[64 - kStackLds];
-    int sp = 0;
+    TravStack st;
+    st.lds_ref = lds_ref;
+    st.lds_dist = lds_dist;
+    st.stride = stride;
+    st.sp = 0;
     uint32_t cur = 0;  // root node
     while (true) {
         // ---- descend through internal nodes until a leaf (or nothing) is reached
         while (!ref_is_leaf(cur)) {
-            const Node2 nd = nodes[cur];
-            float d0 = nd.ref.x != kEmptyRef
-                           ? slab<SAFE>(nd.x.x, nd.y.x, nd.z.x, nd.x.y, nd.y.y, nd.z.y, o, inv, e, best)
-                           : __int_as_float(0x7f800000);
-            float d1 = nd.ref.y != kEmptyRef
-                           ? slab<SAFE>(nd.x.z, nd.y.z, nd.z.z, nd.x.w, nd.y.w, nd.z.w, o, inv, e, best)
-                           : __int_as_float(0x7f800000);
-            bool h0 = d0 < __int_as_float(0x7f800000), h1 = d1 < __int_as_float(0x7f800000);
-            if (h0 && h1) {
-                uint32_t nearr = nd.ref.x, farr = nd.ref.y;
-                float fard = d1;
-                if (d1 < d0) {
-                    nearr = nd.ref.y;
-                    farr = nd.ref.x;
-                    fard = d0;
+            if (W == 2) {
+                const Node2 nd = reinterpret_cast<const Node2*>(nodes_v)[cur];
+                float d0 = nd.ref.x != kEmptyRef
+                               ? slab<SAFE>(nd.x.x, nd.y.x, nd.z.x, nd.x.y, nd.y.y, nd.z.y, o, inv, e, best)
+                               : kInfF;
+                float d1 = nd.ref.y != kEmptyRef
+                               ? slab<SAFE>(nd.x.z, nd.y.z, nd.z.z, nd.x.w, nd.y.w, nd.z.w, o, inv, e, best)
+                               : kInfF;
+                bool h0 = d0 < kInfF, h1 = d1 < kInfF;
+                if (h0 && h1) {
+                    uint32_t nearr = nd.ref.x, farr = nd.ref.y;
+                    float fard = d1;
+                    if (d1 < d0) {
+                        nearr = nd.ref.y;
+                        farr = nd.ref.x;
+                        fard = d0;
+                    }
+                    st.push(farr, fard);
+                    cur = nearr;
+                } else if (h0) {
+                    cur = nd.ref.x;
+                } else if (h1) {
+                    cur = nd.ref.y;
+                } else {
+                    cur = kEmptyRef;
+                    break;
                 }
-                // push far
-                int slot = sp & (kStackLds - 1);
-                if (sp >= kStackLds) {
-                    spill_ref[sp - kStackLds] = lds_ref[slot * stride];
-                    spill_dist[sp - kStackLds] = lds_dist[slot * stride];
-                }
-                lds_ref[slot * stride] = farr;
-                lds_dist[slot * stride] = fard;
-                sp++;
-                cur = nearr;
-            } else if (h0) {
-                cur = nd.ref.x;
-            } else if (h1) {
-                cur = nd.ref.y;
             } else {
-                cur = kEmptyRef;
-                break;
+                const Node4 nd = reinterpret_cast<const Node4*>(nodes_v)[cur];
+                uint32_t r0 = nd.ref.x, r1 = nd.ref.y, r2 = nd.ref.z, r3 = nd.ref.w;
+                float d0 = r0 != kEmptyRef
+                               ? slab<SAFE>(nd.lox.x, nd.loy.x, nd.loz.x, nd.hix.x, nd.hiy.x, nd.hiz.x, o, inv, e, best)
+                               : kInfF;
+                float d1 = r1 != kEmptyRef
+                               ? slab<SAFE>(nd.lox.y, nd.loy.y, nd.loz.y, nd.hix.y, nd.hiy.y, nd.hiz.y, o, inv, e, best)
+                               : kInfF;
+                float d2 = r2 != kEmptyRef
+                               ? slab<SAFE>(nd.lox.z, nd.loy.z, nd.loz.z, nd.hix.z, nd.hiy.z, nd.hiz.z, o, inv, e, best)
+                               : kInfF;
+                float d3 = r3 != kEmptyRef
+                               ? slab<SAFE>(nd.lox.w, nd.loy.w, nd.loz.w, nd.hix.w, nd.hiy.w, nd.hiz.w, o, inv, e, best)
+                               : kInfF;
+                // sorting network: d0 <= d1 <= d2 <= d3
+                cswap(d0, r0, d1, r1);
+                cswap(d2, r2, d3, r3);
+                cswap(d0, r0, d2, r2);
+                cswap(d1, r1, d3, r3);
+                cswap(d1, r1, d2, r2);
+                if (!(d0 < kInfF)) {
+                    cur = kEmptyRef;
+                    break;
+                }
+                if (d3 < kInfF) st.push(r3, d3);
+                if (d2 < kInfF) st.push(r2, d2);
+                if (d1 < kInfF) st.push(r1, d1);
+                cur = r0;
             }
         }
         // ---- leaf
@@ -108,15 +177,10 @@ __device__ __forceinline__ HitRec traverse_impl(const Node2* __restrict__ nodes,
         }
         // ---- pop the next entry that can still hold a closer hit
         bool found = false;
-        while (sp > 0) {
-            sp--;
-            int slot = sp & (kStackLds - 1);
-            uint32_t r = lds_ref[slot * stride];
-            float dist = lds_dist[slot * stride];
-            if (sp >= kStackLds) {
-                lds_ref[slot * stride] = spill_ref[sp - kStackLds];
-                lds_dist[slot * stride] = spill_dist[sp - kStackLds];
-            }
+        while (st.sp > 0) {
+            uint32_t r;
+            float dist;
+            st.pop(r, dist);
             if (dist <= best * kSlabGrow) {
                 cur = r;
                 found = true;
@@ -129,12 +193,12 @@ __device__ __forceinline__ HitRec traverse_impl(const Node2* __restrict__ nodes,
 }
 
 // Rays with a (near-)zero direction component take the SAFE slab test.
-__device__ __forceinline__ HitRec traverse(const Node2* __restrict__ nodes, const float4* __restrict__ tris, f3 o,
-                                           f3 d, float tmax, bool any, uint32_t* lds_ref, float* lds_dist,
-                                           int stride) {
+template <int W>
+__device__ __forceinline__ HitRec traverse(const void* __restrict__ nodes, const float4* __restrict__ tris, f3 o, f3 d,
+                                           float tmax, bool any, uint32_t* lds_ref, float* lds_dist, int stride) {
     if (fabsf(d.x) < kTinyDir || fabsf(d.y) < kTinyDir || fabsf(d.z) < kTinyDir)
-        return traverse_impl<true>(nodes, tris, o, d, tmax, any, lds_ref, lds_dist, stride);
-    return traverse_impl<false>(nodes, tris, o, d, tmax, any, lds_ref, lds_dist, stride);
+        return traverse_impl<W, true>(nodes, tris, o, d, tmax, any, lds_ref, lds_dist, stride);
+    return traverse_impl<W, false>(nodes, tris, o, d, tmax, any, lds_ref, lds_dist, stride);
 }
 
 }  // namespace vg

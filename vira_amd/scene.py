@@ -299,8 +299,11 @@ class Context:
         capi.check(self.lib.vira_gpu_mesh_download(self.h, mid, pos.ctypes.data, idx.ctypes.data, nrm.ctypes.data))
         return pos, idx, nrm
 
-    def blas_build(self, mid: int, max_leaf: int = 4, bins: int = 16):
-        o = capi.BVHOpts(max_leaf, bins)
+    def mesh_release(self, mid: int):
+        capi.check(self.lib.vira_gpu_mesh_release(self.h, mid))
+
+    def blas_build(self, mid: int, max_leaf: int = 4, bins: int = 16, width: int = 4):
+        o = capi.BVHOpts(max_leaf, bins, width)
         capi.check(self.lib.vira_gpu_blas_build(self.h, mid, C.byref(o)))
         s = capi.BVHStats()
         capi.check(self.lib.vira_gpu_blas_stats(self.h, mid, C.byref(s)))
