@@ -24,6 +24,10 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 COUNTS_FIXTURE = ROOT / "tests" / "golden" / "c1_traversal_counts.json"
+# HBM bytes per launch of the fused renderer from rocprofv3 FETCH_SIZE / WRITE_SIZE
+# runs, scaled by the same counters on a copy of known size
+TRAFFIC_PROFILE = ROOT / "profiles" / "k_render_fused_dem4097_2048px_traffic.json"
+METRIC = "Mrays/s, 33M-tri DEM @2048^2: batched vs plain GPU; adjoint/forward time ratio"
 
 
 def algorithmic_bytes(v_int: float, v_tri: float) -> float:
@@ -31,6 +35,16 @@ def algorithmic_bytes(v_int: float, v_tri: float) -> float:
     node visited + 36 B per triangle tested, with the visit counts of the
     reference binned-SAH BVH2 traversal on the same rays."""
     return 48.0 + 64.0 * v_int + 36.0 * v_tri
+
+
+def traffic_bytes(kernel: str, workload: str):
+    """Measured HBM bytes per launch (read + write) of this kernel on this workload, or None."""
+    if not TRAFFIC_PROFILE.exists():
+        return None
+    t = json.loads(TRAFFIC_PROFILE.read_text())
+    if not t["kernel"].startswith(kernel) or t["workload"] != workload:
+        return None
+    return int(t["traffic_bytes"])
 
 
 def cpu_baseline(built, ctx, stride: int, threads: int):
@@ -175,14 +189,14 @@ def main() -> int:
         achieved = nbytes / (ms / 1e3) / 1e9
         roofline = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_bytes(kernel, sc.name), "kernel": kernel,
             "bytes_per_closest_ray": round(a_closest, 1), "bytes_per_shadow_ray": round(a_shadow, 1),
             "v_int_closest": round(vi, 2), "v_tri_closest": round(vt, 2),
             "v_int_shadow": round(si, 2), "v_tri_shadow": round(stt, 2),
             "launches": launches, "avg_launch_ms": round(ms / max(1, launches), 4),
         }
     line = {
-        "metric": "Mrays/s (primary+shadow), 33.5M-tri DEM @2048^2, 16 spp, 3 bounces",
+        "metric": METRIC,
         "value": round(value, 2),
         "unit": "Mrays/s",
         "n_gpus": 1,
