@@ -95,6 +95,57 @@ def cpu_baseline(built, ctx, stride: int, threads: int):
     return res, counts
 
 
+def adjoint_leg(ctx, steps: int, warmup: int) -> dict:
+    """Config 2: 1025^2 DEM (2,097,152 triangles), 1024^2 image, fixed-visibility
+    direct illumination.  Times the forward render (keeping hit records) and the
+    adjoint (dL/dimage of the L2 loss -> dL/dheights for 1,050,625 nodes and
+    dL/dtexels for 1024^2 texels), each a synchronous call, and the adjoint that
+    re-traces the records instead (record budget 0)."""
+    import torch
+    from vira_amd import scene as S
+    from vira_amd import workloads as W
+
+    sc = W.CONFIG2
+    d = W.build_diff(ctx, sc)
+    b = d.built
+    gh, gt = torch.empty_like(b.heights), torch.empty_like(b.texture)
+
+    def timed(fn, n):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n * 1e3
+
+    res = {}
+    for label, budget in (("kept", 1 << 30), ("retraced", 0)):
+        opts = S.DIOptions(samples=1, seed=0, record_budget=budget)
+        img = ctx.di_render(b.camera, opts)
+        grad = 2.0 * (img - d.target)
+        for _ in range(warmup):
+            ctx.di_render(b.camera, opts)
+            ctx.adjoint(b.camera, opts, grad, gh, gt)
+        fwd = timed(lambda: ctx.di_render(b.camera, opts), steps)
+        st = {}
+
+        def adj():
+            st.update(ctx.adjoint(b.camera, opts, grad, gh, gt))
+        adj_ms = timed(adj, steps)
+        assert st["used_records"] == (1 if budget else 0)
+        res[label] = (fwd, adj_ms)
+    update_ms = timed(lambda: ctx.dem_update(b.mesh, b.heights), max(1, steps))
+    fwd, adj_ms = res["kept"]
+    print(f"# config 2 ({sc.name}): forward {fwd:.3f} ms, adjoint {adj_ms:.3f} ms (records kept), "
+          f"adjoint {res['retraced'][1]:.3f} ms (re-traced); mesh update + BVH rebuild {update_ms:.1f} ms",
+          file=sys.stderr, flush=True)
+    return {"workload": sc.name, "forward_ms": round(fwd, 3), "adjoint_ms": round(adj_ms, 3),
+            "ratio": round(adj_ms / fwd, 3),
+            "adjoint_retraced_ms": round(res["retraced"][1], 3),
+            "ratio_retraced": round(res["retraced"][1] / res["retraced"][0], 3),
+            "mesh_update_bvh_rebuild_ms": round(update_ms, 2)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +156,7 @@ def main() -> int:
     ap.add_argument("--cpu-stride", type=int, default=1, help="time every n-th pixel row/column on the CPU")
     ap.add_argument("--counts-out", default=None, help="write the oracle traversal counts (fixture) to this path")
     ap.add_argument("--integrator", default="auto", choices=["auto", "fused", "wavefront"])
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the config-2 adjoint/forward leg")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -157,6 +209,8 @@ def main() -> int:
           f"kernels {sum(s['trace_closest_ms'] for s in stats):.1f} ms, any-hit "
           f"{sum(s['trace_shadow_ms'] for s in stats):.1f} ms, shading {sum(s['shade_ms'] for s in stats):.1f} ms",
           file=sys.stderr, flush=True)
+
+    adjoint = None if args.no_adjoint else adjoint_leg(ctx, max(3, args.steps), max(1, args.warmup))
 
     cpu = None
     counts = None
@@ -215,6 +269,8 @@ def main() -> int:
                    "bvh_build_ms": round(built.bvh["build_ms"], 1)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "adjoint_forward_ratio": None if adjoint is None else adjoint["ratio"],
+        "adjoint": adjoint,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

@@ -152,6 +152,25 @@ typedef struct {
     uint32_t fused_launches;
 } vira_gpu_render_stats;
 
+/* Fixed-visibility direct illumination (the differentiable render).  New: the
+ * reference has no differentiable renderer; the forward model is the bounce-0
+ * direct-light term of processIntersection (cpu_path_tracer.ipp:284-428). */
+typedef struct {
+    int samples;            /* spp >= 1; sample 0 through the pixel corner */
+    uint32_t seed;
+    uint64_t record_budget; /* bytes of hit records (16 per pixel and sample) the forward pass may keep
+                               for the adjoint; above it the adjoint re-traces them */
+} vira_gpu_di_opts;
+
+typedef struct {
+    uint64_t primary_rays;
+    uint64_t shadow_rays;
+    double ms;              /* device time of the call's kernels (HIP events) */
+    double trace_ms;        /* adjoint: the part that re-traced hit records (0 when they were kept) */
+    int32_t used_records;   /* adjoint: 1 = read the forward pass's records, 0 = re-traced */
+    uint64_t record_bytes;  /* forward: bytes of records kept (0 = over budget) */
+} vira_gpu_di_stats;
+
 /* --- context ------------------------------------------------------------ */
 vira_gpu_status vira_gpu_create(vira_gpu_ctx** ctx, const vira_gpu_opts* opts);
 vira_gpu_status vira_gpu_destroy(vira_gpu_ctx* ctx);
@@ -187,6 +206,11 @@ vira_gpu_status vira_gpu_mesh_info(vira_gpu_ctx* ctx, int32_t mesh_id, size_t* n
 vira_gpu_status vira_gpu_mesh_download(vira_gpu_ctx* ctx, int32_t mesh_id, float* positions_host,
                                        uint32_t* indices_host, float* normals_host);
 vira_gpu_status vira_gpu_mesh_release(vira_gpu_ctx* ctx, int32_t mesh_id);
+/* New heights for a mesh made by vira_gpu_dem_mesh (DEM::makeMesh,
+ * dem.ipp:165-211, re-run on the same grid): positions and normals are
+ * recomputed in place and a built BLAS is rebuilt with its last options.  The
+ * set of non-finite heights (holes) must not change. */
+vira_gpu_status vira_gpu_dem_update(vira_gpu_ctx* ctx, int32_t mesh_id, const float* heights_dev);
 
 /* --- acceleration (a2, a3) ------------------------------------------------ */
 /* Mesh::buildBVH -> EmbreeBLAS (mesh.ipp:396-418, embree_blas.ipp:14-71):
@@ -225,6 +249,25 @@ vira_gpu_status vira_gpu_render(vira_gpu_ctx* ctx, const vira_gpu_camera* cam, c
 vira_gpu_status vira_gpu_trace_rays(vira_gpu_ctx* ctx, int32_t mesh_id, const float* rays_dev, size_t n,
                                     int any_hit, float* t_dev, uint32_t* tri_id_dev, float* b1_dev,
                                     float* b2_dev, double* kernel_ms);
+
+/* --- differentiable direct illumination (new) ----------------------------- */
+/* Forward: radiance (W*H*8, as vira_gpu_outputs.radiance_dev) of the scene
+ * mesh lit by point lights, one bounce-0 direct-light evaluation per sample
+ * (identical to vira_gpu_render with bounces = 0).  Keeps the hit records
+ * (triangle, barycentrics, per-light visibility) for vira_gpu_adjoint when
+ * they fit in opts->record_budget. */
+vira_gpu_status vira_gpu_di_render(vira_gpu_ctx* ctx, const vira_gpu_camera* cam, const vira_gpu_di_opts* opts,
+                                   float* radiance_dev, vira_gpu_di_stats* stats);
+/* Adjoint of vira_gpu_di_render with visibility held fixed: from dL/dimage
+ * (W*H*8) to dL/dheights (one per grid node of the scene mesh, which must come
+ * from vira_gpu_dem_mesh; 0 at holes) and dL/dtexels (tex_width*tex_height of
+ * the scene material); either output may be NULL and both are overwritten.
+ * Uses the records of the preceding vira_gpu_di_render with the same camera,
+ * samples and seed and an unchanged scene; re-traces them otherwise.  Point
+ * lights only (VIRA_GPU_ERR_UNSUPPORTED for sphere lights). */
+vira_gpu_status vira_gpu_adjoint(vira_gpu_ctx* ctx, const vira_gpu_camera* cam, const vira_gpu_di_opts* opts,
+                                 const float* dL_dimage_dev, float* dL_dheights_dev, float* dL_dtexture_dev,
+                                 vira_gpu_di_stats* stats);
 
 #ifdef __cplusplus
 }

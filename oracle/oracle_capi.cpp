@@ -5,6 +5,7 @@
 #include <thread>
 
 #include "oracle_core.hpp"
+#include "oracle_diff.hpp"
 
 using namespace vo;
 
@@ -228,13 +229,50 @@ float orc_rng_u01(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t dim) 
 void orc_render(void* h, int precision, const Camera* cam, const Light* lights, int nl, const Material* mat,
                 const Options* opt, float* radiance, float* direct, float* indirect, float* albedo,
                 float* normal, float* depth, float* alpha, uint32_t* tri_id, uint64_t* counts,
-                int nthreads, const int* pixel_list, size_t npix) {
+                uint32_t* records, int nthreads, const int* pixel_list, size_t npix) {
     auto* o = static_cast<OMesh*>(h);
     Outputs out;
     out.radiance = radiance; out.direct = direct; out.indirect = indirect; out.albedo = albedo;
     out.normal = normal; out.depth = depth; out.alpha = alpha; out.tri_id = tri_id; out.counts = counts;
+    out.records = records;
     if (precision == 64) render_f64(o->m, o->b, *cam, lights, nl, *mat, *opt, out, nthreads, pixel_list, npix);
     else render_f32(o->m, o->b, *cam, lights, nl, *mat, *opt, out, nthreads, pixel_list, npix);
+}
+
+// ---- fixed-visibility direct illumination, FP64 forward / adjoint / finite differences
+void* orc_dem_diff_create(int W, int H, const Projection* proj, const float* heights, const float* grey) {
+    return new DemDiff(dem_diff_make(W, H, *proj, heights, grey));
+}
+void orc_dem_diff_destroy(void* d) { delete static_cast<DemDiff*>(d); }
+
+static DiScene di_scene(void* d, const uint32_t* rec, size_t npix, int spp, const Light* lights, int nl,
+                        const Material* mat) {
+    DiScene s;
+    s.dd = static_cast<DemDiff*>(d);
+    s.rec = reinterpret_cast<const DiRecord*>(rec);
+    s.npix = npix;
+    s.spp = spp;
+    s.lights = lights;
+    s.nl = nl;
+    s.mat = mat;
+    return s;
+}
+
+void orc_di_forward64(void* d, const double* h, const double* tex, const uint32_t* rec, size_t npix, int spp,
+                      const Light* lights, int nl, const Material* mat, double* img, int nthreads) {
+    di_forward64(di_scene(d, rec, npix, spp, lights, nl, mat), h, tex, img, nthreads);
+}
+
+void orc_di_adjoint64(void* d, const double* h, const double* tex, const uint32_t* rec, size_t npix, int spp,
+                      const Light* lights, int nl, const Material* mat, const double* g, double* dh,
+                      double* dtex) {
+    di_adjoint64(di_scene(d, rec, npix, spp, lights, nl, mat), h, tex, g, dh, dtex);
+}
+
+void orc_di_fd64(void* d, const double* h, const double* tex, const uint32_t* rec, size_t npix, int spp,
+                 const Light* lights, int nl, const Material* mat, const double* target, const int64_t* nodes,
+                 size_t n, double step, double* out, int nthreads) {
+    di_fd_heights64(di_scene(d, rec, npix, spp, lights, nl, mat), h, tex, target, nodes, n, step, out, nthreads);
 }
 
 }  // extern "C"

@@ -364,9 +364,18 @@ void render_impl(const Mesh& m, const BVH& b, const Camera& cam, const Light* li
                     R beta[NB];
                     for (int k = 0; k < NB; ++k) beta[k] = 1;
                     R path[NB] = {0}, direct[NB] = {0};
+                    uint32_t* rec = out.records ? out.records + 4 * ((size_t)s * cam.W * cam.H + (size_t)pix)
+                                                : nullptr;
+                    if (rec) { rec[0] = 0xffffffffu; rec[1] = rec[2] = rec[3] = 0; }
                     for (int bounce = 0; bounce <= opt.bounces; ++bounce) {
                         Hit<R> h = bvh_trace<R>(m, b, ray, false, (R)INFINITY, &tc_closest);
                         if (h.id == 0xffffffffu) break;
+                        if (rec && bounce == 0) {
+                            float fb1 = (float)h.b1, fb2 = (float)h.b2;
+                            rec[0] = h.id;
+                            std::memcpy(&rec[1], &fb1, 4);
+                            std::memcpy(&rec[2], &fb2, 4);
+                        }
                         // ---- processIntersection (cpu_path_tracer.ipp:279-469)
                         uint32_t i0 = m.idx[3 * h.id], i1 = m.idx[3 * h.id + 1], i2 = m.idx[3 * h.id + 2];
                         V3<R> p0 = vpos(i0), p1 = vpos(i1), p2 = vpos(i2);
@@ -422,6 +431,7 @@ void render_impl(const Mesh& m, const BVH& b, const Camera& cam, const Light* li
                             RayT<R> sray{org, ld};
                             Hit<R> sh = bvh_trace<R>(m, b, sray, true, dist, &tc_any);
                             if (sh.id != 0xffffffffu) continue;  // occluded
+                            if (rec && bounce == 0) rec[3] |= 1u << l;
                             R ndl = dot(N, ld);
                             R mx = std::max(ndl, (R)0);
                             R cth = std::max((R)0, dot(ld, N));

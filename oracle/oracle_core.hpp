@@ -521,6 +521,8 @@ struct Outputs {  // all optional (nullptr = skip); pixel index i + j*W
     uint32_t* tri_id = nullptr;
     uint64_t* counts = nullptr;  // [0] primary rays, [1] shadow rays, [2] bounce rays,
                                  // [3..4] internal/tris closest, [5..6] internal/tris any-hit
+    uint32_t* records = nullptr; // 4 per (sample, pixel) at [s * W*H + pixel]: first-hit id,
+                                 // b1 and b2 bits, bit l = light l sampled and unoccluded
 };
 
 template <class R>

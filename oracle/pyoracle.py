@@ -78,7 +78,15 @@ def lib() -> C.CDLL:
         "orc_offset_point": (None, [P, P, P]),
         "orc_rng_u01": (C.c_float, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
         "orc_render": (None, [P, C.c_int, C.POINTER(Camera), C.POINTER(Light), C.c_int, C.POINTER(Material),
-                              C.POINTER(Options), P, P, P, P, P, P, P, P, P, C.c_int, P, C.c_size_t]),
+                              C.POINTER(Options), P, P, P, P, P, P, P, P, P, P, C.c_int, P, C.c_size_t]),
+        "orc_dem_diff_create": (P, [C.c_int, C.c_int, C.POINTER(Projection), P, P]),
+        "orc_dem_diff_destroy": (None, [P]),
+        "orc_di_forward64": (None, [P, P, P, P, C.c_size_t, C.c_int, C.POINTER(Light), C.c_int,
+                                    C.POINTER(Material), P, C.c_int]),
+        "orc_di_adjoint64": (None, [P, P, P, P, C.c_size_t, C.c_int, C.POINTER(Light), C.c_int,
+                                    C.POINTER(Material), P, P, P]),
+        "orc_di_fd64": (None, [P, P, P, P, C.c_size_t, C.c_int, C.POINTER(Light), C.c_int, C.POINTER(Material),
+                               P, P, C.c_size_t, C.c_double, P, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -211,7 +219,7 @@ class Mesh:
         return t, ids, b1, b2
 
     def render(self, cam: Camera, lights, mat: Material, opt: Options, precision=32, pixels=None,
-               threads=None):
+               threads=None, records=False):
         W, H = cam.W, cam.H
         n = W * H
         out = {
@@ -221,12 +229,15 @@ class Mesh:
             "alpha": np.zeros(n, np.float32), "tri_id": np.full(n, 0xFFFFFFFF, np.uint32),
         }
         cnt = np.zeros(7, np.uint64)
+        rec = np.zeros((opt.spp, n, 4), np.uint32) if records else None
         larr = (Light * max(1, len(lights)))(*lights) if lights else (Light * 1)()
         pl = None if pixels is None else np.ascontiguousarray(pixels, np.int32)
         lib().orc_render(self.h, precision, C.byref(cam), larr, len(lights), C.byref(mat), C.byref(opt),
                          *(out[k].ctypes.data for k in ("radiance", "direct", "indirect", "albedo", "normal",
                                                         "depth", "alpha", "tri_id")),
-                         cnt.ctypes.data, threads or nthreads(), _p(pl), 0 if pl is None else pl.size)
+                         cnt.ctypes.data, _p(rec), threads or nthreads(), _p(pl), 0 if pl is None else pl.size)
+        if rec is not None:
+            out["records"] = rec
         out["counts"] = {"primary": int(cnt[0]), "shadow": int(cnt[1]), "bounce": int(cnt[2]),
                          "closest_internal": int(cnt[3]), "closest_tris": int(cnt[4]),
                          "any_internal": int(cnt[5]), "any_tris": int(cnt[6])}
@@ -296,3 +307,57 @@ def dem_mesh(heights: np.ndarray, proj: Projection, grey_albedo: np.ndarray | No
     a = np.ones(H * W, np.float32) if grey_albedo is None else np.asarray(grey_albedo, np.float32)
     alb = np.repeat(a[:, None], NB, axis=1)
     return Mesh(pos, idx, nrm, alb, dem_uv(W, H), pos64 if with_pos64 else None)
+
+
+class DemDiff:
+    """FP64 fixed-visibility direct-illumination model of a DEM (oracle_diff.hpp):
+    forward image, adjoint and central finite differences over hit records
+    (Mesh.render(..., records=True)["records"], shape (spp, W*H, 4))."""
+
+    def __init__(self, heights: np.ndarray, proj: Projection, grey_albedo: np.ndarray | None = None):
+        self.shape = heights.shape
+        h = np.ascontiguousarray(heights, np.float32)
+        g = None if grey_albedo is None else np.ascontiguousarray(grey_albedo, np.float32)
+        self.h = lib().orc_dem_diff_create(h.shape[1], h.shape[0], C.byref(proj), h.ctypes.data, _p(g))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_dem_diff_destroy(self.h)
+            self.h = None
+
+    @staticmethod
+    def _args(records, lights, mat):
+        rec = np.ascontiguousarray(records, np.uint32)
+        larr = (Light * max(1, len(lights)))(*lights)
+        return rec, rec.shape[0], rec.shape[1], larr
+
+    def forward(self, heights, texture, records, lights, mat: Material) -> np.ndarray:
+        rec, spp, npix, larr = self._args(records, lights, mat)
+        h = np.ascontiguousarray(heights, np.float64).reshape(-1)
+        t = None if texture is None else np.ascontiguousarray(texture, np.float64).reshape(-1)
+        img = np.zeros((npix, NB), np.float64)
+        lib().orc_di_forward64(self.h, h.ctypes.data, _p(t), rec.ctypes.data, npix, spp, larr, len(lights),
+                               C.byref(mat), img.ctypes.data, nthreads())
+        return img
+
+    def adjoint(self, heights, texture, records, lights, mat: Material, grad_image):
+        rec, spp, npix, larr = self._args(records, lights, mat)
+        h = np.ascontiguousarray(heights, np.float64).reshape(-1)
+        t = None if texture is None else np.ascontiguousarray(texture, np.float64).reshape(-1)
+        g = np.ascontiguousarray(grad_image, np.float64).reshape(npix, NB)
+        dh = np.zeros(h.size, np.float64)
+        dt = None if t is None else np.zeros(t.size, np.float64)
+        lib().orc_di_adjoint64(self.h, h.ctypes.data, _p(t), rec.ctypes.data, npix, spp, larr, len(lights),
+                               C.byref(mat), g.ctypes.data, dh.ctypes.data, _p(dt))
+        return dh, dt
+
+    def fd_heights(self, heights, texture, records, lights, mat: Material, target, nodes, step=1e-3):
+        rec, spp, npix, larr = self._args(records, lights, mat)
+        h = np.ascontiguousarray(heights, np.float64).reshape(-1)
+        t = None if texture is None else np.ascontiguousarray(texture, np.float64).reshape(-1)
+        tg = np.ascontiguousarray(target, np.float64).reshape(npix, NB)
+        nd = np.ascontiguousarray(nodes, np.int64)
+        out = np.zeros(nd.size, np.float64)
+        lib().orc_di_fd64(self.h, h.ctypes.data, _p(t), rec.ctypes.data, npix, spp, larr, len(lights),
+                          C.byref(mat), tg.ctypes.data, nd.ctypes.data, nd.size, step, out.ctypes.data, nthreads())
+        return out

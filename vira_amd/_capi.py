@@ -136,6 +136,21 @@ class RenderStats(C.Structure):
     ]
 
 
+class DIOpts(C.Structure):
+    _fields_ = [("samples", C.c_int), ("seed", C.c_uint32), ("record_budget", C.c_uint64)]
+
+
+class DIStats(C.Structure):
+    _fields_ = [
+        ("primary_rays", C.c_uint64),
+        ("shadow_rays", C.c_uint64),
+        ("ms", C.c_double),
+        ("trace_ms", C.c_double),
+        ("used_records", C.c_int32),
+        ("record_bytes", C.c_uint64),
+    ]
+
+
 class Instance(C.Structure):
     _fields_ = [("mesh_id", C.c_int32), ("transform", C.c_double * 16)]
 
@@ -164,6 +179,9 @@ SIGNATURES = [
     ("vira_gpu_material_set", _S, [_P, C.c_int32, C.POINTER(Material)]),
     ("vira_gpu_render", _S, [_P, C.POINTER(Camera), C.POINTER(RenderOpts), C.POINTER(Outputs), C.POINTER(RenderStats)]),
     ("vira_gpu_trace_rays", _S, [_P, C.c_int32, _P, C.c_size_t, C.c_int, _P, _P, _P, _P, C.POINTER(C.c_double)]),
+    ("vira_gpu_dem_update", _S, [_P, C.c_int32, _P]),
+    ("vira_gpu_di_render", _S, [_P, C.POINTER(Camera), C.POINTER(DIOpts), _P, C.POINTER(DIStats)]),
+    ("vira_gpu_adjoint", _S, [_P, C.POINTER(Camera), C.POINTER(DIOpts), _P, _P, _P, C.POINTER(DIStats)]),
 ]
 
 _lib = None

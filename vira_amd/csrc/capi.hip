@@ -144,6 +144,7 @@ vira_gpu_status vira_gpu_dem_mesh(vira_gpu_ctx* ctx, const float* heights_dev, i
         int32_t id = new_mesh(c);
         try {
             vg::dem_build_mesh(c, *c.meshes[id], heights_dev, width, height, *proj, grey_albedo_dev);
+            c.scene_gen++;
         } catch (...) {
             c.meshes[id].reset();
             throw;
@@ -180,6 +181,7 @@ vira_gpu_status vira_gpu_mesh_upload(vira_gpu_ctx* ctx, const float* pos, size_t
             // Mesh::init computes normals when the first vertex normal is zero (mesh.ipp:62-64)
             bool need = !nrm || (nv > 0 && nrm[0] == 0.f && nrm[1] == 0.f && nrm[2] == 0.f);
             if (need) vg::mesh_compute_normals(c, m);
+            c.scene_gen++;
         } catch (...) {
             c.meshes[id].reset();
             throw;
@@ -222,6 +224,7 @@ vira_gpu_status vira_gpu_mesh_release(vira_gpu_ctx* ctx, int32_t mesh_id) {
         VG_HIP(hipStreamSynchronize(c.stream));
         c.meshes[mesh_id].reset();
         if (c.scene_mesh == mesh_id) c.scene_mesh = -1;
+        c.scene_gen++;
     });
 }
 
@@ -235,7 +238,10 @@ vira_gpu_status vira_gpu_blas_build(vira_gpu_ctx* ctx, int32_t mesh_id, const vi
         VG_CHECK(o.max_leaf >= 0 && o.max_leaf <= 15 && o.bins >= 0 && o.bins <= 32, VIRA_GPU_ERR_INVALID,
                  "blas_build: max_leaf must be 1..15 and bins 4..32");
         VG_CHECK(o.width == 2 || o.width == 4, VIRA_GPU_ERR_INVALID, "blas_build: width must be 2 or 4");
-        vg::blas_build(c, c.mesh(mesh_id), o);
+        vg::MeshG& m = c.mesh(mesh_id);
+        c.scene_gen++;
+        vg::blas_build(c, m, o);
+        m.blas.opts = o;
     });
 }
 
@@ -260,6 +266,7 @@ vira_gpu_status vira_gpu_tlas_build(vira_gpu_ctx* ctx, const vira_gpu_instance* 
         vg::MeshG& m = c.mesh(inst[0].mesh_id);
         VG_CHECK(m.blas.built, VIRA_GPU_ERR_STATE, "tlas_build: BLAS not built");
         c.scene_mesh = inst[0].mesh_id;
+        c.scene_gen++;
     });
 }
 
@@ -274,12 +281,15 @@ vira_gpu_status vira_gpu_lights_set(vira_gpu_ctx* ctx, const vira_gpu_light* lig
                      "lights_set: SphereLight Radius must be positive");  // sphere_light.ipp:20
         }
         c.lights.assign(lights, lights + n);
+        c.scene_gen++;
     });
 }
 
 vira_gpu_status vira_gpu_material_set(vira_gpu_ctx* ctx, int32_t mesh_id, const vira_gpu_material* mat) {
     return guarded([&] {
-        vg::MeshG& m = C(ctx).mesh(mesh_id);
+        vg::Ctx& c = C(ctx);
+        vg::MeshG& m = c.mesh(mesh_id);
+        c.scene_gen++;
         VG_CHECK(mat, VIRA_GPU_ERR_INVALID, "material_set: null material");
         VG_CHECK(!mat->texture_dev || (mat->tex_width > 0 && mat->tex_height > 0), VIRA_GPU_ERR_INVALID,
                  "material_set: texture needs positive size");
@@ -304,6 +314,39 @@ vira_gpu_status vira_gpu_trace_rays(vira_gpu_ctx* ctx, int32_t mesh_id, const fl
         set_device(c);
         VG_CHECK(rays || n == 0, VIRA_GPU_ERR_INVALID, "trace_rays: null rays");
         vg::trace_rays(c, c.mesh(mesh_id), rays, n, any_hit, t, id, b1, b2, ms);
+    });
+}
+
+vira_gpu_status vira_gpu_dem_update(vira_gpu_ctx* ctx, int32_t mesh_id, const float* heights_dev) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        set_device(c);
+        VG_CHECK(heights_dev, VIRA_GPU_ERR_INVALID, "dem_update: null heights");
+        vg::MeshG& m = c.mesh(mesh_id);
+        c.scene_gen++;
+        vg::dem_update(c, m, heights_dev);
+        if (m.blas.built) vg::blas_build(c, m, m.blas.opts);
+    });
+}
+
+vira_gpu_status vira_gpu_di_render(vira_gpu_ctx* ctx, const vira_gpu_camera* cam, const vira_gpu_di_opts* opts,
+                                   float* radiance_dev, vira_gpu_di_stats* stats) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        set_device(c);
+        VG_CHECK(cam && opts, VIRA_GPU_ERR_INVALID, "di_render: null argument");
+        vg::di_render(c, *cam, *opts, radiance_dev, stats);
+    });
+}
+
+vira_gpu_status vira_gpu_adjoint(vira_gpu_ctx* ctx, const vira_gpu_camera* cam, const vira_gpu_di_opts* opts,
+                                 const float* dL_dimage_dev, float* dL_dheights_dev, float* dL_dtexture_dev,
+                                 vira_gpu_di_stats* stats) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        set_device(c);
+        VG_CHECK(cam && opts, VIRA_GPU_ERR_INVALID, "adjoint: null argument");
+        vg::di_adjoint(c, *cam, *opts, dL_dimage_dev, dL_dheights_dev, dL_dtexture_dev, stats);
     });
 }
 

@@ -72,6 +72,7 @@ struct Blas {
     DBuf tris;    // float4[3 * nt] in leaf order: v0.xyz+id, v1.xyz, v2.xyz
     uint64_t num_nodes = 0;
     vira_gpu_bvh_stats stats{};
+    vira_gpu_bvh_opts opts{4, 16, 4};  // options of the last build (dem_update rebuilds with them)
 };
 
 struct MeshG {
@@ -81,6 +82,13 @@ struct MeshG {
     DBuf idx;     // uint32[3 * nt]
     Blas blas;
     vira_gpu_material mat{};
+    // DEM provenance (dem_mesh): node v is grid sample (v % dem_w, v / dem_w)
+    bool is_dem = false;
+    int dem_w = 0, dem_h = 0;
+    vira_gpu_projection dem_proj{};
+    // vertex -> incident triangles, sorted by id (CSR), built on first use
+    bool adj_built = false;
+    DBuf adj_offs, adj_count, adj;
 };
 
 struct Ctx {
@@ -94,6 +102,15 @@ struct Ctx {
     DBuf r_ray_o, r_ray_d, r_hit, r_queue[2], r_beta, r_path_rad, r_path_dir, r_srays_o, r_srays_d,
         r_scontrib, r_svis, r_sinfo, r_acc, r_counters, r_lights;
     hipEvent_t ev[8] = {};
+    // bumped by every call that changes geometry, BVHs, lights or materials
+    uint64_t scene_gen = 1;
+    // fixed-visibility direct illumination: hit records of the last forward pass
+    DBuf di_rec, di_gpos, di_gnrm, di_gs;
+    bool di_rec_valid = false;
+    uint64_t di_gen = 0;
+    vira_gpu_camera di_cam{};
+    int32_t di_spp = 0;
+    uint32_t di_seed = 0;
 
     MeshG& mesh(int32_t id) {
         VG_CHECK(id >= 0 && (size_t)id < meshes.size() && meshes[id] && meshes[id]->alive, VIRA_GPU_ERR_INVALID,
@@ -111,6 +128,12 @@ void launch_synth_albedo(hipStream_t s, size_t nv, uint32_t seed, float* out);
 void dem_build_mesh(Ctx& c, MeshG& m, const float* heights, int W, int H, const vira_gpu_projection& p,
                     const float* grey);
 void mesh_compute_normals(Ctx& c, MeshG& m);
+void mesh_adjacency(Ctx& c, MeshG& m);
+void dem_update(Ctx& c, MeshG& m, const float* heights);
+void di_render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_di_opts& o, float* radiance,
+               vira_gpu_di_stats* st);
+void di_adjoint(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_di_opts& o, const float* grad_image,
+                float* grad_heights, float* grad_texture, vira_gpu_di_stats* st);
 void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& o);
 void trace_rays(Ctx& c, MeshG& m, const float* rays, size_t n, int any_hit, float* t, uint32_t* id, float* b1,
                 float* b2, double* ms);

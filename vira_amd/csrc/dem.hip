@@ -136,17 +136,6 @@ __global__ void k_dem_vertices(const float* heights, int W, int H, vira_gpu_proj
     verts[v] = out;
 }
 
-__device__ __forceinline__ f3 vpos(const GVertex* V, uint32_t i) { return f3{V[i].px, V[i].py, V[i].pz}; }
-
-// Unit face normal normalize(cross(p1 - p0, p2 - p0)) of mesh.ipp:211-214;
-// returns false where the reference skips the triangle.
-__device__ __forceinline__ bool face_normal(const GVertex* V, uint32_t i0, uint32_t i1, uint32_t i2, f3& fn) {
-    f3 p0 = vpos(V, i0), p1 = vpos(V, i1), p2 = vpos(V, i2);
-    if (isinf(p0.x) || isinf(p1.x) || isinf(p2.x)) return false;
-    fn = normalize(cross(sub(p1, p0), sub(p2, p0)));
-    return !(isnan(fn.x) || isnan(fn.y) || isnan(fn.z));
-}
-
 __device__ __forceinline__ void finish_normal(GVertex& out, f3 n) {
     if (length(n) != 0.0f) n = normalize(n);
     out.nx = n.x;
@@ -209,10 +198,8 @@ __global__ void k_adj_fill(size_t nt, const uint32_t* idx, const uint32_t* offs,
     }
 }
 
-// Sorts each vertex's incident triangles (insertion sort: lists are short) and
-// sums their face normals in triangle order.
-__global__ void k_adj_normals(size_t nv, const uint32_t* offs, const uint32_t* count, uint32_t* adj,
-                              const uint32_t* idx, GVertex* verts) {
+// Sorts each vertex's incident triangles by id (insertion sort: lists are short).
+__global__ void k_adj_sort(size_t nv, const uint32_t* offs, const uint32_t* count, uint32_t* adj) {
     size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nv) return;
     uint32_t b = offs[v], n = count[v];
@@ -225,6 +212,14 @@ __global__ void k_adj_normals(size_t nv, const uint32_t* offs, const uint32_t* c
         }
         adj[b + k] = key;
     }
+}
+
+// Sums each vertex's face normals in triangle order (sorted adjacency).
+__global__ void k_adj_normals(size_t nv, const uint32_t* offs, const uint32_t* count, const uint32_t* adj,
+                              const uint32_t* idx, GVertex* verts) {
+    size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    uint32_t b = offs[v], n = count[v];
     f3 acc = mk3(0.0f, 0.0f, 0.0f);
     for (uint32_t a = 0; a < n; ++a) {
         uint32_t t = adj[b + a];
@@ -233,6 +228,27 @@ __global__ void k_adj_normals(size_t nv, const uint32_t* offs, const uint32_t* c
         acc = add(acc, fn);
     }
     finish_normal(verts[v], acc);
+}
+
+// DEM node positions for new heights (uv, albedo and normals untouched).
+__global__ void k_dem_positions(const float* heights, int W, int H, vira_gpu_projection proj, GVertex* verts) {
+    size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= (size_t)W * H) return;
+    float h = heights[v];
+    if (!isfinite(h)) return;
+    double p[3];
+    dem_world(proj, W, H, (int)(v % W), (int)(v / W), (double)h, p);
+    verts[v].px = (float)p[0];
+    verts[v].py = (float)p[1];
+    verts[v].pz = (float)p[2];
+}
+
+// Counts nodes whose height changed between finite and non-finite (the
+// triangle set of a DEM mesh is fixed once built).
+__global__ void k_dem_mask_diff(const float* heights, size_t nv, const GVertex* verts, uint32_t* count) {
+    size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    if ((bool)isfinite(heights[v]) == (bool)isinf(verts[v].px)) atomicAdd(count, 1u);
 }
 
 }  // namespace
@@ -276,6 +292,35 @@ void dem_build_mesh(Ctx& c, MeshG& m, const float* heights, int W, int H, const 
                                                               m.verts.as<GVertex>());
     VG_HIP(hipGetLastError());
     VG_HIP(hipStreamSynchronize(c.stream));
+    m.is_dem = true;
+    m.dem_w = W;
+    m.dem_h = H;
+    m.dem_proj = p;
+}
+
+void dem_update(Ctx& c, MeshG& m, const float* heights) {
+    VG_CHECK(m.is_dem, VIRA_GPU_ERR_INVALID, "dem_update: mesh was not made by dem_mesh");
+    const size_t nv = m.nv;
+    DBuf cnt;
+    cnt.alloc(4);
+    VG_HIP(hipMemsetAsync(cnt.p, 0, 4, c.stream));
+    k_dem_mask_diff<<<blocks_for(nv, 256), 256, 0, c.stream>>>(heights, nv, m.verts.as<GVertex>(),
+                                                                cnt.as<uint32_t>());
+    VG_HIP(hipGetLastError());
+    uint32_t bad = 0;
+    VG_HIP(hipMemcpyAsync(&bad, cnt.p, 4, hipMemcpyDeviceToHost, c.stream));
+    VG_HIP(hipStreamSynchronize(c.stream));
+    VG_CHECK(bad == 0, VIRA_GPU_ERR_INVALID,
+             "dem_update: " + std::to_string(bad) + " heights changed between finite and non-finite");
+    k_dem_positions<<<blocks_for(nv, 256), 256, 0, c.stream>>>(heights, m.dem_w, m.dem_h, m.dem_proj,
+                                                                m.verts.as<GVertex>());
+    VG_HIP(hipGetLastError());
+    mesh_adjacency(c, m);
+    k_adj_normals<<<blocks_for(nv, 256), 256, 0, c.stream>>>(nv, m.adj_offs.as<uint32_t>(), m.adj_count.as<uint32_t>(),
+                                                              m.adj.as<uint32_t>(), m.idx.as<uint32_t>(),
+                                                              m.verts.as<GVertex>());
+    VG_HIP(hipGetLastError());
+    VG_HIP(hipStreamSynchronize(c.stream));
 }
 
 void mesh_upload_pack(Ctx& c, MeshG& m, const float* pos, const float* alb, const float* uv, const float* nrm) {
@@ -283,28 +328,39 @@ void mesh_upload_pack(Ctx& c, MeshG& m, const float* pos, const float* alb, cons
     VG_HIP(hipGetLastError());
 }
 
-void mesh_compute_normals(Ctx& c, MeshG& m) {
-    if (m.nv == 0) return;
-    DBuf count, offs, cursor, adj;
-    count.alloc(m.nv * 4);
-    offs.alloc(m.nv * 4);
+void mesh_adjacency(Ctx& c, MeshG& m) {
+    if (m.adj_built || m.nv == 0) return;
+    m.adj_count.alloc(m.nv * 4);
+    m.adj_offs.alloc(m.nv * 4);
+    m.adj.alloc(std::max<size_t>(1, m.nt * 3 * 4));
+    DBuf cursor;
     cursor.alloc(m.nv * 4);
-    adj.alloc(std::max<size_t>(1, m.nt * 3 * 4));
-    VG_HIP(hipMemsetAsync(count.p, 0, m.nv * 4, c.stream));
+    VG_HIP(hipMemsetAsync(m.adj_count.p, 0, m.nv * 4, c.stream));
     VG_HIP(hipMemsetAsync(cursor.p, 0, m.nv * 4, c.stream));
     if (m.nt) {
-        k_adj_count<<<blocks_for(m.nt, 256), 256, 0, c.stream>>>(m.nt, m.idx.as<uint32_t>(), count.as<uint32_t>());
+        k_adj_count<<<blocks_for(m.nt, 256), 256, 0, c.stream>>>(m.nt, m.idx.as<uint32_t>(),
+                                                                  m.adj_count.as<uint32_t>());
         VG_HIP(hipGetLastError());
     }
-    scan_exclusive_u32(c, count.as<uint32_t>(), offs.as<uint32_t>(), m.nv);
+    scan_exclusive_u32(c, m.adj_count.as<uint32_t>(), m.adj_offs.as<uint32_t>(), m.nv);
     if (m.nt) {
-        k_adj_fill<<<blocks_for(m.nt, 256), 256, 0, c.stream>>>(m.nt, m.idx.as<uint32_t>(), offs.as<uint32_t>(),
-                                                                 cursor.as<uint32_t>(), adj.as<uint32_t>());
+        k_adj_fill<<<blocks_for(m.nt, 256), 256, 0, c.stream>>>(m.nt, m.idx.as<uint32_t>(), m.adj_offs.as<uint32_t>(),
+                                                                 cursor.as<uint32_t>(), m.adj.as<uint32_t>());
         VG_HIP(hipGetLastError());
     }
-    k_adj_normals<<<blocks_for(m.nv, 256), 256, 0, c.stream>>>(m.nv, offs.as<uint32_t>(), count.as<uint32_t>(),
-                                                                adj.as<uint32_t>(), m.idx.as<uint32_t>(),
-                                                                m.verts.as<GVertex>());
+    k_adj_sort<<<blocks_for(m.nv, 256), 256, 0, c.stream>>>(m.nv, m.adj_offs.as<uint32_t>(),
+                                                             m.adj_count.as<uint32_t>(), m.adj.as<uint32_t>());
+    VG_HIP(hipGetLastError());
+    VG_HIP(hipStreamSynchronize(c.stream));
+    m.adj_built = true;
+}
+
+void mesh_compute_normals(Ctx& c, MeshG& m) {
+    if (m.nv == 0) return;
+    mesh_adjacency(c, m);
+    k_adj_normals<<<blocks_for(m.nv, 256), 256, 0, c.stream>>>(m.nv, m.adj_offs.as<uint32_t>(),
+                                                                m.adj_count.as<uint32_t>(), m.adj.as<uint32_t>(),
+                                                                m.idx.as<uint32_t>(), m.verts.as<GVertex>());
     VG_HIP(hipGetLastError());
     VG_HIP(hipStreamSynchronize(c.stream));
 }

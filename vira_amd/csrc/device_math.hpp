@@ -216,4 +216,29 @@ struct __attribute__((aligned(64))) GVertex {
 };
 static_assert(sizeof(GVertex) == 64, "vertex record is one cache line");
 
+// Adds v over the wave's active lanes into *counter with one atomic.
+__device__ __forceinline__ void wave_add(unsigned long long* counter, uint32_t v) {
+    // one atomic per wave: reduce over the active lanes
+    unsigned long long active = __ballot(1);
+    uint32_t sum = v;
+    for (int off = 32; off > 0; off >>= 1) {
+        uint32_t o = __shfl_xor(sum, off, 64);
+        // lanes that exited contribute nothing: their shuffled value is read from an inactive lane
+        if (active & (1ull << (__lane_id() ^ off))) sum += o;
+    }
+    if (__lane_id() == (unsigned)(__ffsll((long long)active) - 1)) atomicAdd(counter, (unsigned long long)sum);
+}
+
+__device__ __forceinline__ f3 vpos(const GVertex* V, uint32_t i) { return f3{V[i].px, V[i].py, V[i].pz}; }
+
+// Unit face normal normalize(cross(p1 - p0, p2 - p0)) of mesh.ipp:211-214;
+// returns false where the reference skips the triangle (a corner at infinity
+// or a degenerate triangle).
+__device__ __forceinline__ bool face_normal(const GVertex* V, uint32_t i0, uint32_t i1, uint32_t i2, f3& fn) {
+    f3 p0 = vpos(V, i0), p1 = vpos(V, i1), p2 = vpos(V, i2);
+    if (isinf(p0.x) || isinf(p1.x) || isinf(p2.x)) return false;
+    fn = normalize(cross(sub(p1, p0), sub(p2, p0)));
+    return !(isnan(fn.x) || isnan(fn.y) || isnan(fn.z));
+}
+
 }  // namespace vg

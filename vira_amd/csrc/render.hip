@@ -20,6 +20,7 @@
 #include <cmath>
 
 #include "context.hpp"
+#include "scene_params.hpp"
 #include "shading.hpp"
 #include "traverse.hpp"
 
@@ -48,18 +49,6 @@ enum : int {
 };
 // Per-path fields of the wavefront schedule (one path per pixel per sample pass).
 enum : int { P_BETA = 0, P_RAD = 8, P_DIR = 16, P_FIELDS = 24 };
-
-__device__ __forceinline__ void wave_add(unsigned long long* counter, uint32_t v) {
-    // one atomic per wave: reduce over the active lanes
-    unsigned long long active = __ballot(1);
-    uint32_t sum = v;
-    for (int off = 32; off > 0; off >>= 1) {
-        uint32_t o = __shfl_xor(sum, off, 64);
-        // lanes that exited contribute nothing: their shuffled value is read from an inactive lane
-        if (active & (1ull << (__lane_id() ^ off))) sum += o;
-    }
-    if (__lane_id() == (unsigned)(__ffsll((long long)active) - 1)) atomicAdd(counter, (unsigned long long)sum);
-}
 
 // Compacted append: returns this lane's slot (only meaningful where pred).
 __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
@@ -507,34 +496,9 @@ void render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, c
     const bool fused = o.integrator == 1 || (o.integrator == 0 && nl <= 1);
     VG_CHECK(!(fused && nl > 1), VIRA_GPU_ERR_UNSUPPORTED, "render: the fused integrator takes at most one light");
 
-    CamF cf;
-    cf.W = cam.width;
-    cf.H = cam.height;
-    for (int k = 0; k < 6; ++k) cf.kinv[k] = cam.kinv[k];
-    for (int k = 0; k < 9; ++k) cf.rot[k] = cam.rot[k];
-    cf.ox = (float)cam.position[0];
-    cf.oy = (float)cam.position[1];
-    cf.oz = (float)cam.position[2];
-    cf.zdir = cam.z_dir;
-    LightF hl[VIRA_GPU_MAX_LIGHTS];
-    for (int l = 0; l < nl; ++l) {
-        const vira_gpu_light& L = c.lights[l];
-        hl[l].kind = L.kind;
-        hl[l].px = (float)L.position[0];
-        hl[l].py = (float)L.position[1];
-        hl[l].pz = (float)L.position[2];
-        hl[l].radius = L.radius;
-        for (int k = 0; k < NB; ++k) hl[l].value[k] = L.value[k];
-    }
-    c.r_lights.alloc(sizeof(hl));
-    VG_HIP(hipMemcpyAsync(c.r_lights.p, hl, sizeof(LightF) * std::max(1, nl), hipMemcpyHostToDevice, c.stream));
-    Lights lf{nl, c.r_lights.as<LightF>()};
-    MatF mf;
-    for (int k = 0; k < NB; ++k) mf.albedo[k] = m.mat.albedo[k];
-    mf.tex = m.mat.texture_dev;
-    mf.tw = m.mat.tex_width;
-    mf.th = m.mat.tex_height;
-    mf.smooth = m.mat.smooth_shading;
+    const CamF cf = cam_params(cam);
+    const Lights lf = upload_lights(c, c.r_lights);
+    const MatF mf = mat_params(m);
     OutPtrs op{out.radiance_dev, out.direct_dev, out.indirect_dev, out.albedo_dev, out.normal_dev,
                out.depth_dev, out.alpha_dev, out.tri_id_dev};
 

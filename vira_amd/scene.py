@@ -197,6 +197,17 @@ class PathTracerOptions:
         return o
 
 
+@dataclass
+class DIOptions:
+    """Fixed-visibility direct-illumination render (forward + adjoint)."""
+    samples: int = 1
+    seed: int = 0
+    record_budget: int = 1 << 30   # bytes of hit records kept for the adjoint
+
+    def to_c(self) -> capi.DIOpts:
+        return capi.DIOpts(self.samples, self.seed, self.record_budget)
+
+
 class RenderPasses:
     """RenderPasses (passes.hpp:51-95) as device tensors, pixel index i + j*W."""
 
@@ -359,6 +370,37 @@ class Context:
 
     def synchronize(self):
         capi.check(self.lib.vira_gpu_synchronize(self.h))
+
+    # ---- differentiable direct illumination
+    def dem_update(self, mid: int, heights):
+        """New heights (H, W) for a DEM mesh; rebuilds its BLAS if one was built."""
+        capi.check(self.lib.vira_gpu_dem_update(self.h, mid, heights.contiguous().data_ptr()))
+
+    def di_render(self, camera: Camera, options: DIOptions, radiance=None):
+        """Radiance (H, W, 8) of the scene mesh under point lights, bounce 0 only."""
+        if radiance is None:
+            radiance = self.torch.empty((camera.height, camera.width, NUM_BINS), dtype=self.torch.float32,
+                                        device=f"cuda:{self.device}")
+        st = capi.DIStats()
+        cam = camera.to_c()
+        opt = options.to_c()
+        capi.check(self.lib.vira_gpu_di_render(self.h, C.byref(cam), C.byref(opt), radiance.data_ptr(),
+                                               C.byref(st)))
+        self.last_stats = {f: getattr(st, f) for f, _ in capi.DIStats._fields_}
+        return radiance
+
+    def adjoint(self, camera: Camera, options: DIOptions, grad_image, grad_heights=None, grad_texture=None):
+        """dL/dheights and dL/dtexels from dL/dimage (H, W, 8); pass output tensors to fill (or None)."""
+        st = capi.DIStats()
+        cam = camera.to_c()
+        opt = options.to_c()
+        g = grad_image.contiguous()
+        capi.check(self.lib.vira_gpu_adjoint(self.h, C.byref(cam), C.byref(opt), g.data_ptr(),
+                                             None if grad_heights is None else grad_heights.data_ptr(),
+                                             None if grad_texture is None else grad_texture.data_ptr(),
+                                             C.byref(st)))
+        self.last_stats = {f: getattr(st, f) for f, _ in capi.DIStats._fields_}
+        return self.last_stats
 
 
 def projection(kind: int = 0, spacing: float = 30.0, radius: float = 0.0, center=(0.0, 0.0, 0.0),

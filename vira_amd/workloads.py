@@ -51,6 +51,12 @@ CONFIG1 = DemScene("dem4097_2048px_16spp_3b", dem=4097, seed=1, sphere_radius=MO
                    image=2048, spp=16, bounces=3, russian_roulette=True, sun_kind="sphere",
                    albedo="texture", texture=4096)
 
+# Config 2: 1025^2 DEM (2.1M triangles), 1024^2, direct illumination with fixed
+# visibility; per-texel albedo (1024^2 log-normal texture) and the heights are
+# the differentiable parameters.  40 km altitude: the DEM fills the image.
+CONFIG2 = DemScene("dem1025_1024px_di_adjoint", dem=1025, seed=2, altitude=40000.0, image=1024,
+                   albedo="texture", texture=1024)
+
 
 def scaled(sc: DemScene, dem: int | None = None, image: int | None = None, spp: int | None = None,
            bounces: int | None = None, texture: int | None = None) -> DemScene:
@@ -151,3 +157,42 @@ def build(ctx: S.Context, sc: DemScene, bvh_max_leaf: int = 4, bvh_bins: int = 1
     tex = make_texture(ctx, sc) if sc.albedo == "texture" else None
     ctx.material_set(mesh, albedo=1.0, texture=tex, smooth_shading=sc.smooth_shading)
     return BuiltScene(sc, mesh, heights, valb, tex, make_camera(sc), lights, make_options(sc), bvh)
+
+
+@dataclass
+class DiffScene:
+    built: BuiltScene
+    dem: object               # autograd.DifferentiableDem
+    target: object            # (H, W, 8) image of the perturbed DEM
+    target_heights: object
+    target_texture: object
+
+
+def perturbed_inputs(ctx: S.Context, sc: DemScene, heights, texture, height_sigma: float = 2.0,
+                     albedo_sigma: float = 0.05, seed: int | None = None):
+    """Target parameters: heights + N(0, height_sigma) and texture x (1 + N(0, albedo_sigma)),
+    from a torch generator seeded with 2000 + scene seed."""
+    import torch
+    g = torch.Generator(device=f"cuda:{ctx.device}")
+    g.manual_seed(2000 + sc.seed if seed is None else seed)
+    dh = torch.randn(heights.shape, generator=g, device=heights.device, dtype=torch.float32) * height_sigma
+    th = heights + torch.where(torch.isfinite(heights), dh, torch.zeros_like(dh))
+    tt = None
+    if texture is not None:
+        dt = torch.randn(texture.shape, generator=g, device=texture.device, dtype=torch.float32) * albedo_sigma
+        tt = texture * (1.0 + dt)
+    return th.contiguous(), None if tt is None else tt.contiguous()
+
+
+def build_diff(ctx: S.Context, sc: DemScene, options: S.DIOptions | None = None, bvh_width: int = 4) -> DiffScene:
+    """Config-2 style differentiable scene: the base DEM on the GPU plus the
+    target image rendered from the perturbed heights and texture."""
+    from .autograd import DifferentiableDem
+    b = build(ctx, sc, bvh_width=bvh_width)
+    dem = DifferentiableDem(ctx, b.mesh, b.camera, options or S.DIOptions(), smooth_shading=sc.smooth_shading)
+    th, tt = perturbed_inputs(ctx, sc, b.heights, b.texture)
+    import torch
+    with torch.no_grad():
+        target = dem(th, tt).clone()
+    dem._apply(b.heights, b.texture)
+    return DiffScene(b, dem, target, th, tt)
