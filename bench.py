@@ -146,6 +146,53 @@ def adjoint_leg(ctx, steps: int, warmup: int) -> dict:
             "mesh_update_bvh_rebuild_ms": round(update_ms, 2)}
 
 
+def batch_leg(ctx, c1_built, steps: int, warmup: int) -> dict:
+    """Config 3: 64 requests (image side uniform over {128, 256, 512, 1024,
+    2048}, spp uniform over {1, 4, 16, 64}, 3 bounces + RR, one of four meshes:
+    the DEMs of configs 0-2 and a 200k-triangle small body; random camera and
+    sun; seed 3).  Times the per-request form (one render call each) and the
+    batched form (one persistent graph launch, configuration from the
+    autotuner), after checking they write identical passes."""
+    import torch
+    from vira_amd import workloads as W
+
+    meshes = W.build_batch_meshes(ctx, c1_built)
+    reqs = W.make_batch(ctx, meshes, n=64, seed=3)
+    t0 = time.perf_counter()
+    cfg, tune_ms = ctx.batch_autotune(reqs, sample_stride=16)
+    tune_s = time.perf_counter() - t0
+    plain_reqs = [r.__class__(r.mesh, r.camera, r.options, r.light,
+                              r.passes.__class__(r.passes.width, r.passes.height, which=("radiance", "depth", "tri_id")))
+                  for r in reqs]
+
+    def timed(fn, n):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / n
+
+    for _ in range(warmup):
+        W.render_plain(ctx, plain_reqs)
+        ctx.batch_submit(reqs)
+    identical = all(torch.equal(a.passes.total_radiance, b.passes.total_radiance) and
+                    torch.equal(a.passes.triangle_id, b.passes.triangle_id) for a, b in zip(reqs, plain_reqs))
+    plain_s = timed(lambda: W.render_plain(ctx, plain_reqs), steps)
+    st = {}
+    batch_s = timed(lambda: st.update(ctx.batch_submit(reqs)), steps)
+    rays = st["primary_rays"] + st["shadow_rays"]
+    print(f"# config 3: 64 requests, {st['tiles']} tiles, {rays} primary+shadow rays; per-request {plain_s * 1e3:.1f} ms, "
+          f"batched {batch_s * 1e3:.1f} ms (kernel {st['ms']:.1f} ms, graph reused {st['graph_reused']}); "
+          f"autotune {tune_s:.1f} s -> {cfg}; identical passes: {identical}", file=sys.stderr, flush=True)
+    return {"workload": "batch64_dem_smallbody_seed3", "plain_ms": round(plain_s * 1e3, 2),
+            "batched_ms": round(batch_s * 1e3, 2), "speedup": round(plain_s / batch_s, 3),
+            "batched_mrays_s": round(rays / batch_s / 1e6, 1), "identical_passes": bool(identical),
+            "config": {"bvh_width": cfg.bvh_width, "waves_per_block": cfg.waves_per_block,
+                       "waves_per_cu": cfg.waves_per_cu, "tiles_per_grab": cfg.tiles_per_grab},
+            "autotune_s": round(tune_s, 2)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,6 +204,7 @@ def main() -> int:
     ap.add_argument("--counts-out", default=None, help="write the oracle traversal counts (fixture) to this path")
     ap.add_argument("--integrator", default="auto", choices=["auto", "fused", "wavefront"])
     ap.add_argument("--no-adjoint", action="store_true", help="skip the config-2 adjoint/forward leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the config-3 batched/per-request leg")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -211,6 +259,7 @@ def main() -> int:
           file=sys.stderr, flush=True)
 
     adjoint = None if args.no_adjoint else adjoint_leg(ctx, max(3, args.steps), max(1, args.warmup))
+    batch = None if (args.no_batch or args.scene != "c1") else batch_leg(ctx, built, max(1, min(3, args.steps)), 1)
 
     cpu = None
     counts = None
@@ -269,7 +318,9 @@ def main() -> int:
                    "bvh_build_ms": round(built.bvh["build_ms"], 1)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched_vs_plain": None if batch is None else batch["speedup"],
         "adjoint_forward_ratio": None if adjoint is None else adjoint["ratio"],
+        "batch": batch,
         "adjoint": adjoint,
     }
     print(json.dumps(line), flush=True)

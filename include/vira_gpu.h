@@ -171,6 +171,34 @@ typedef struct {
     uint64_t record_bytes;  /* forward: bytes of records kept (0 = over budget) */
 } vira_gpu_di_stats;
 
+/* Batched rendering (new): many render requests of uneven image size, spp and
+ * mesh in one persistent launch over a shared queue of 8x8 pixel tiles. */
+typedef struct {
+    int32_t mesh_id;             /* rendered mesh: its BLAS and material */
+    vira_gpu_camera camera;
+    vira_gpu_render_opts opts;   /* integrator is ignored: the per-pixel renderer runs every request */
+    int32_t num_lights;          /* 0 or 1 */
+    vira_gpu_light light;
+    vira_gpu_outputs out;        /* caller-owned device passes, as for vira_gpu_render */
+} vira_gpu_request;
+
+typedef struct {
+    int32_t bvh_width;        /* 2 or 4; every request's mesh must have a BLAS of this width */
+    int32_t waves_per_block;  /* 1, 2 or 4 */
+    int32_t waves_per_cu;     /* persistent waves per compute unit: the depth of the shared queue's consumers */
+    int32_t tiles_per_grab;   /* tiles taken from the shared queue per atomic, 1..16 */
+    int32_t order;            /* 0 = requests with the most rays per tile first, 1 = submission order */
+} vira_gpu_batch_config;
+
+typedef struct {
+    uint64_t primary_rays;
+    uint64_t shadow_rays;
+    uint64_t bounce_rays;
+    uint64_t tiles;
+    double ms;                /* device time of the launch (HIP events) */
+    int32_t graph_reused;     /* 1 = replayed the captured hipGraph of an identical batch layout */
+} vira_gpu_batch_stats;
+
 /* --- context ------------------------------------------------------------ */
 vira_gpu_status vira_gpu_create(vira_gpu_ctx** ctx, const vira_gpu_opts* opts);
 vira_gpu_status vira_gpu_destroy(vira_gpu_ctx* ctx);
@@ -268,6 +296,21 @@ vira_gpu_status vira_gpu_di_render(vira_gpu_ctx* ctx, const vira_gpu_camera* cam
 vira_gpu_status vira_gpu_adjoint(vira_gpu_ctx* ctx, const vira_gpu_camera* cam, const vira_gpu_di_opts* opts,
                                  const float* dL_dimage_dev, float* dL_dheights_dev, float* dL_dtexture_dev,
                                  vira_gpu_di_stats* stats);
+
+/* --- batched rendering (new) --------------------------------------------- */
+/* Renders n requests (CPUPathTracer::render semantics per request,
+ * cpu_path_tracer.ipp:29-161) in one persistent, hipGraph-captured launch;
+ * each request's passes equal what vira_gpu_render writes for it (same
+ * integrator, same sample order).  cfg NULL = the configuration chosen by
+ * the last vira_gpu_batch_autotune, else a default. */
+vira_gpu_status vira_gpu_batch_submit(vira_gpu_ctx* ctx, const vira_gpu_request* requests_host, size_t n,
+                                      const vira_gpu_batch_config* cfg, vira_gpu_batch_stats* stats);
+/* Times candidate configurations (BVH width, waves per block, waves per CU,
+ * tiles per grab) on every sample_stride-th tile of each request, keeps the
+ * fastest for later submits and returns it (rebuilding the requests' BLASes
+ * at its BVH width). */
+vira_gpu_status vira_gpu_batch_autotune(vira_gpu_ctx* ctx, const vira_gpu_request* requests_host, size_t n,
+                                        int sample_stride, vira_gpu_batch_config* best, double* best_ms);
 
 #ifdef __cplusplus
 }

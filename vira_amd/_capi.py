@@ -151,6 +151,33 @@ class DIStats(C.Structure):
     ]
 
 
+class Request(C.Structure):
+    _fields_ = [
+        ("mesh_id", C.c_int32),
+        ("camera", Camera),
+        ("opts", RenderOpts),
+        ("num_lights", C.c_int32),
+        ("light", Light),
+        ("out", Outputs),
+    ]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("bvh_width", C.c_int32), ("waves_per_block", C.c_int32), ("waves_per_cu", C.c_int32),
+                ("tiles_per_grab", C.c_int32), ("order", C.c_int32)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [
+        ("primary_rays", C.c_uint64),
+        ("shadow_rays", C.c_uint64),
+        ("bounce_rays", C.c_uint64),
+        ("tiles", C.c_uint64),
+        ("ms", C.c_double),
+        ("graph_reused", C.c_int32),
+    ]
+
+
 class Instance(C.Structure):
     _fields_ = [("mesh_id", C.c_int32), ("transform", C.c_double * 16)]
 
@@ -182,6 +209,10 @@ SIGNATURES = [
     ("vira_gpu_dem_update", _S, [_P, C.c_int32, _P]),
     ("vira_gpu_di_render", _S, [_P, C.POINTER(Camera), C.POINTER(DIOpts), _P, C.POINTER(DIStats)]),
     ("vira_gpu_adjoint", _S, [_P, C.POINTER(Camera), C.POINTER(DIOpts), _P, _P, _P, C.POINTER(DIStats)]),
+    ("vira_gpu_batch_submit", _S, [_P, C.POINTER(Request), C.c_size_t, C.POINTER(BatchConfig),
+                                   C.POINTER(BatchStats)]),
+    ("vira_gpu_batch_autotune", _S, [_P, C.POINTER(Request), C.c_size_t, C.c_int, C.POINTER(BatchConfig),
+                                     C.POINTER(C.c_double)]),
 ]
 
 _lib = None

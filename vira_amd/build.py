@@ -37,7 +37,7 @@ FLAGS = [
     "-Wno-unused-but-set-variable",
     f"-I{ROOT / 'include'}",
 ]
-SOURCES = ["capi.hip", "dem.hip", "bvh.hip", "render.hip", "adjoint.hip", "scan.hip"]
+SOURCES = ["capi.hip", "dem.hip", "bvh.hip", "render.hip", "adjoint.hip", "batch.hip", "scan.hip"]
 
 
 def _compile(src: Path, obj: Path) -> None:

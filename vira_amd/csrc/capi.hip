@@ -350,4 +350,31 @@ vira_gpu_status vira_gpu_adjoint(vira_gpu_ctx* ctx, const vira_gpu_camera* cam, 
     });
 }
 
+vira_gpu_status vira_gpu_batch_submit(vira_gpu_ctx* ctx, const vira_gpu_request* requests, size_t n,
+                                      const vira_gpu_batch_config* cfg, vira_gpu_batch_stats* stats) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        set_device(c);
+        VG_CHECK(requests || n == 0, VIRA_GPU_ERR_INVALID, "batch_submit: null requests");
+        if (n == 0) {
+            if (stats) *stats = vira_gpu_batch_stats{};
+            return;
+        }
+        vira_gpu_batch_config g = c.b_cfg;  // the last autotuned configuration, or the default
+        if (cfg) g = *cfg;
+        else g.bvh_width = c.mesh(requests[0].mesh_id).blas.width;
+        vg::batch_submit(c, requests, n, g, 1, stats);
+    });
+}
+
+vira_gpu_status vira_gpu_batch_autotune(vira_gpu_ctx* ctx, const vira_gpu_request* requests, size_t n,
+                                        int sample_stride, vira_gpu_batch_config* best, double* best_ms) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        set_device(c);
+        VG_CHECK(requests, VIRA_GPU_ERR_INVALID, "batch_autotune: null requests");
+        vg::batch_autotune(c, requests, n, sample_stride, best, best_ms);
+    });
+}
+
 }  // extern "C"

@@ -111,6 +111,15 @@ struct Ctx {
     vira_gpu_camera di_cam{};
     int32_t di_spp = 0;
     uint32_t di_seed = 0;
+    // batched rendering: descriptors, queue state and the captured launch
+    DBuf b_reqs, b_firsts, b_state, b_acc;
+    hipGraphExec_t b_exec = nullptr;
+    std::vector<uint64_t> b_key;
+    bool b_tuned = false;
+    vira_gpu_batch_config b_cfg{4, 1, 12, 1, 0};
+    ~Ctx() {
+        if (b_exec) (void)hipGraphExecDestroy(b_exec);
+    }
 
     MeshG& mesh(int32_t id) {
         VG_CHECK(id >= 0 && (size_t)id < meshes.size() && meshes[id] && meshes[id]->alive, VIRA_GPU_ERR_INVALID,
@@ -134,6 +143,10 @@ void di_render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_di_opts& o, fl
                vira_gpu_di_stats* st);
 void di_adjoint(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_di_opts& o, const float* grad_image,
                 float* grad_heights, float* grad_texture, vira_gpu_di_stats* st);
+void batch_submit(Ctx& c, const vira_gpu_request* reqs, size_t n, const vira_gpu_batch_config& cfg, int stride,
+                  vira_gpu_batch_stats* st);
+void batch_autotune(Ctx& c, const vira_gpu_request* reqs, size_t n, int stride, vira_gpu_batch_config* best,
+                    double* best_ms);
 void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& o);
 void trace_rays(Ctx& c, MeshG& m, const float* rays, size_t n, int any_hit, float* t, uint32_t* id, float* b1,
                 float* b2, double* ms);

@@ -198,6 +198,40 @@ class PathTracerOptions:
 
 
 @dataclass
+class RenderRequest:
+    """One request of a batch: mesh, camera, options, at most one light, passes."""
+    mesh: int
+    camera: "Camera"
+    options: PathTracerOptions
+    light: object  # PointLight | SphereLight | None
+    passes: "RenderPasses"
+
+    def to_c(self) -> capi.Request:
+        r = capi.Request()
+        r.mesh_id = self.mesh
+        r.camera = self.camera.to_c()
+        r.opts = self.options.to_c()
+        r.num_lights = 0 if self.light is None else 1
+        if self.light is not None:
+            r.light = self.light.to_c()
+        r.out = self.passes.to_c()
+        return r
+
+
+@dataclass
+class BatchConfig:
+    bvh_width: int = 4
+    waves_per_block: int = 1
+    waves_per_cu: int = 12
+    tiles_per_grab: int = 1
+    order: int = 0
+
+    def to_c(self) -> capi.BatchConfig:
+        return capi.BatchConfig(self.bvh_width, self.waves_per_block, self.waves_per_cu, self.tiles_per_grab,
+                                self.order)
+
+
+@dataclass
 class DIOptions:
     """Fixed-visibility direct-illumination render (forward + adjoint)."""
     samples: int = 1
@@ -209,36 +243,41 @@ class DIOptions:
 
 
 class RenderPasses:
-    """RenderPasses (passes.hpp:51-95) as device tensors, pixel index i + j*W."""
+    """RenderPasses (passes.hpp:51-95) as device tensors, pixel index i + j*W.
+    `which` limits the passes allocated (the others are not written)."""
 
-    def __init__(self, width: int, height: int, device="cuda"):
+    ALL = ("radiance", "direct", "indirect", "albedo", "normal", "depth", "alpha", "tri_id")
+
+    def __init__(self, width: int, height: int, device="cuda", which=ALL):
         import torch
-        n = width * height
         self.width, self.height = width, height
-        self.total_radiance = torch.zeros((height, width, NUM_BINS), dtype=torch.float32, device=device)
-        self.direct_radiance = torch.zeros_like(self.total_radiance)
-        self.indirect_radiance = torch.zeros_like(self.total_radiance)
-        self.albedo = torch.zeros_like(self.total_radiance)
-        self.normal_global = torch.zeros((height, width, 3), dtype=torch.float32, device=device)
-        self.depth = torch.zeros((height, width), dtype=torch.float32, device=device)
-        self.alpha = torch.zeros((height, width), dtype=torch.float32, device=device)
-        self.triangle_id = torch.zeros((height, width), dtype=torch.int32, device=device)
-        assert n == self.depth.numel()
+        w = set(which)
+        spec = lambda: torch.zeros((height, width, NUM_BINS), dtype=torch.float32, device=device)
+        self.total_radiance = spec() if "radiance" in w else None
+        self.direct_radiance = spec() if "direct" in w else None
+        self.indirect_radiance = spec() if "indirect" in w else None
+        self.albedo = spec() if "albedo" in w else None
+        self.normal_global = (torch.zeros((height, width, 3), dtype=torch.float32, device=device)
+                              if "normal" in w else None)
+        self.depth = torch.zeros((height, width), dtype=torch.float32, device=device) if "depth" in w else None
+        self.alpha = torch.zeros((height, width), dtype=torch.float32, device=device) if "alpha" in w else None
+        self.triangle_id = (torch.zeros((height, width), dtype=torch.int32, device=device)
+                            if "tri_id" in w else None)
 
     def to_c(self) -> capi.Outputs:
+        p = lambda t: None if t is None else t.data_ptr()
         o = capi.Outputs()
-        o.radiance_dev = self.total_radiance.data_ptr()
-        o.direct_dev = self.direct_radiance.data_ptr()
-        o.indirect_dev = self.indirect_radiance.data_ptr()
-        o.albedo_dev = self.albedo.data_ptr()
-        o.normal_dev = self.normal_global.data_ptr()
-        o.depth_dev = self.depth.data_ptr()
-        o.alpha_dev = self.alpha.data_ptr()
-        o.tri_id_dev = self.triangle_id.data_ptr()
+        o.radiance_dev = p(self.total_radiance)
+        o.direct_dev = p(self.direct_radiance)
+        o.indirect_dev = p(self.indirect_radiance)
+        o.albedo_dev = p(self.albedo)
+        o.normal_dev = p(self.normal_global)
+        o.depth_dev = p(self.depth)
+        o.alpha_dev = p(self.alpha)
+        o.tri_id_dev = p(self.triangle_id)
         return o
 
 
-# ---------------------------------------------------------------- context
 class Context:
     """One GPU, one HIP stream (torch's current stream by default)."""
 
@@ -253,6 +292,13 @@ class Context:
         self.h = h
         self.device = device
         self.last_stats = None
+
+    def _call(self, fn, *args):
+        """Calls into the library after the work queued on torch's current stream
+        has finished (the library runs on its own stream and returns with its
+        work complete, so outputs are ready for any stream afterwards)."""
+        self.torch.cuda.current_stream(self.device).synchronize()
+        capi.check(fn(*args))
 
     def close(self):
         if self.h:
@@ -269,22 +315,22 @@ class Context:
     def synth_fbm(self, width: int, height: int, seed: int, amplitude: float = 500.0, octaves: int = 6,
                   base_cells: float = 64.0):
         t = self.torch.empty((height, width), dtype=self.torch.float32, device=f"cuda:{self.device}")
-        capi.check(self.lib.vira_gpu_synth_fbm(self.h, width, height, seed, amplitude, octaves, base_cells,
-                                               t.data_ptr()))
+        self._call(self.lib.vira_gpu_synth_fbm, self.h, width, height, seed, amplitude, octaves, base_cells,
+                                               t.data_ptr())
         return t
 
     def synth_vertex_albedo(self, nv: int, seed: int):
         t = self.torch.empty((nv,), dtype=self.torch.float32, device=f"cuda:{self.device}")
-        capi.check(self.lib.vira_gpu_synth_vertex_albedo(self.h, nv, seed, t.data_ptr()))
+        self._call(self.lib.vira_gpu_synth_vertex_albedo, self.h, nv, seed, t.data_ptr())
         return t
 
     # ---- meshes
     def dem_mesh(self, heights, proj: capi.Projection, grey_albedo=None) -> int:
         H, W = heights.shape
         mid = C.c_int32()
-        capi.check(self.lib.vira_gpu_dem_mesh(self.h, heights.data_ptr(), W, H, C.byref(proj),
+        self._call(self.lib.vira_gpu_dem_mesh, self.h, heights.data_ptr(), W, H, C.byref(proj),
                                               None if grey_albedo is None else grey_albedo.data_ptr(),
-                                              C.byref(mid)))
+                                              C.byref(mid))
         return mid.value
 
     def mesh_upload(self, pos, idx, albedo=None, uv=None, normals=None) -> int:
@@ -293,13 +339,13 @@ class Context:
         arr = [None if a is None else np.ascontiguousarray(a, dtype=np.float32) for a in (albedo, uv, normals)]
         p = lambda a: None if a is None else a.ctypes.data
         mid = C.c_int32()
-        capi.check(self.lib.vira_gpu_mesh_upload(self.h, p(pos), pos.size // 3, p(idx), idx.size // 3,
-                                                 p(arr[0]), p(arr[1]), p(arr[2]), C.byref(mid)))
+        self._call(self.lib.vira_gpu_mesh_upload, self.h, p(pos), pos.size // 3, p(idx), idx.size // 3,
+                                                 p(arr[0]), p(arr[1]), p(arr[2]), C.byref(mid))
         return mid.value
 
     def mesh_info(self, mid: int):
         nv, nt = C.c_size_t(), C.c_size_t()
-        capi.check(self.lib.vira_gpu_mesh_info(self.h, mid, C.byref(nv), C.byref(nt)))
+        self._call(self.lib.vira_gpu_mesh_info, self.h, mid, C.byref(nv), C.byref(nt))
         return nv.value, nt.value
 
     def mesh_download(self, mid: int):
@@ -307,17 +353,17 @@ class Context:
         pos = np.empty((nv, 3), np.float32)
         idx = np.empty((nt, 3), np.uint32)
         nrm = np.empty((nv, 3), np.float32)
-        capi.check(self.lib.vira_gpu_mesh_download(self.h, mid, pos.ctypes.data, idx.ctypes.data, nrm.ctypes.data))
+        self._call(self.lib.vira_gpu_mesh_download, self.h, mid, pos.ctypes.data, idx.ctypes.data, nrm.ctypes.data)
         return pos, idx, nrm
 
     def mesh_release(self, mid: int):
-        capi.check(self.lib.vira_gpu_mesh_release(self.h, mid))
+        self._call(self.lib.vira_gpu_mesh_release, self.h, mid)
 
     def blas_build(self, mid: int, max_leaf: int = 4, bins: int = 16, width: int = 4):
         o = capi.BVHOpts(max_leaf, bins, width)
-        capi.check(self.lib.vira_gpu_blas_build(self.h, mid, C.byref(o)))
+        self._call(self.lib.vira_gpu_blas_build, self.h, mid, C.byref(o))
         s = capi.BVHStats()
-        capi.check(self.lib.vira_gpu_blas_stats(self.h, mid, C.byref(s)))
+        self._call(self.lib.vira_gpu_blas_stats, self.h, mid, C.byref(s))
         return {f: getattr(s, f) for f, _ in capi.BVHStats._fields_}
 
     def tlas_build(self, mesh_ids):
@@ -326,13 +372,13 @@ class Context:
             arr[i].mesh_id = m
             for k in range(16):
                 arr[i].transform[k] = 1.0 if k % 5 == 0 else 0.0
-        capi.check(self.lib.vira_gpu_tlas_build(self.h, arr, len(mesh_ids)))
+        self._call(self.lib.vira_gpu_tlas_build, self.h, arr, len(mesh_ids))
 
     def lights_set(self, lights):
         arr = (capi.Light * max(1, len(lights)))()
         for i, L in enumerate(lights):
             arr[i] = L.to_c()
-        capi.check(self.lib.vira_gpu_lights_set(self.h, arr, len(lights)))
+        self._call(self.lib.vira_gpu_lights_set, self.h, arr, len(lights))
 
     def material_set(self, mid: int, albedo=None, texture=None, smooth_shading: bool = False):
         m = capi.Material()
@@ -344,14 +390,14 @@ class Context:
             m.tex_height, m.tex_width = texture.shape
         m.smooth_shading = int(smooth_shading)
         self._texture = texture  # keep alive
-        capi.check(self.lib.vira_gpu_material_set(self.h, mid, C.byref(m)))
+        self._call(self.lib.vira_gpu_material_set, self.h, mid, C.byref(m))
 
     def render(self, camera: Camera, options: PathTracerOptions, passes: RenderPasses):
         st = capi.RenderStats()
         cam = camera.to_c()
         opt = options.to_c()
         out = passes.to_c()
-        capi.check(self.lib.vira_gpu_render(self.h, C.byref(cam), C.byref(opt), C.byref(out), C.byref(st)))
+        self._call(self.lib.vira_gpu_render, self.h, C.byref(cam), C.byref(opt), C.byref(out), C.byref(st))
         self.last_stats = {f: getattr(st, f) for f, _ in capi.RenderStats._fields_}
         return self.last_stats
 
@@ -364,17 +410,44 @@ class Context:
         b1 = self.torch.empty(n, dtype=self.torch.float32, device=dev)
         b2 = self.torch.empty(n, dtype=self.torch.float32, device=dev)
         ms = C.c_double()
-        capi.check(self.lib.vira_gpu_trace_rays(self.h, mid, rays.data_ptr(), n, int(any_hit), t.data_ptr(),
-                                                ids.data_ptr(), b1.data_ptr(), b2.data_ptr(), C.byref(ms)))
+        self._call(self.lib.vira_gpu_trace_rays, self.h, mid, rays.data_ptr(), n, int(any_hit), t.data_ptr(),
+                                                ids.data_ptr(), b1.data_ptr(), b2.data_ptr(), C.byref(ms))
         return t, ids, b1, b2, ms.value
 
     def synchronize(self):
-        capi.check(self.lib.vira_gpu_synchronize(self.h))
+        self._call(self.lib.vira_gpu_synchronize, self.h)
+
+    # ---- batched rendering
+    @staticmethod
+    def _requests(requests):
+        arr = (capi.Request * len(requests))()
+        for i, r in enumerate(requests):
+            arr[i] = r.to_c()
+        return arr
+
+    def batch_submit(self, requests, config: "BatchConfig | None" = None):
+        """Renders every RenderRequest in one persistent launch; returns the launch stats."""
+        arr = self._requests(requests)
+        st = capi.BatchStats()
+        cfg = None if config is None else C.byref(config.to_c())
+        self._call(self.lib.vira_gpu_batch_submit, self.h, arr, len(requests), cfg, C.byref(st))
+        self.last_stats = {f: getattr(st, f) for f, _ in capi.BatchStats._fields_}
+        return self.last_stats
+
+    def batch_autotune(self, requests, sample_stride: int = 16):
+        """Picks and keeps the fastest batch configuration for these requests."""
+        arr = self._requests(requests)
+        best = capi.BatchConfig()
+        ms = C.c_double()
+        self._call(self.lib.vira_gpu_batch_autotune, self.h, arr, len(requests), sample_stride, C.byref(best),
+                   C.byref(ms))
+        return BatchConfig(best.bvh_width, best.waves_per_block, best.waves_per_cu, best.tiles_per_grab,
+                           best.order), ms.value
 
     # ---- differentiable direct illumination
     def dem_update(self, mid: int, heights):
         """New heights (H, W) for a DEM mesh; rebuilds its BLAS if one was built."""
-        capi.check(self.lib.vira_gpu_dem_update(self.h, mid, heights.contiguous().data_ptr()))
+        self._call(self.lib.vira_gpu_dem_update, self.h, mid, heights.contiguous().data_ptr())
 
     def di_render(self, camera: Camera, options: DIOptions, radiance=None):
         """Radiance (H, W, 8) of the scene mesh under point lights, bounce 0 only."""
@@ -384,8 +457,8 @@ class Context:
         st = capi.DIStats()
         cam = camera.to_c()
         opt = options.to_c()
-        capi.check(self.lib.vira_gpu_di_render(self.h, C.byref(cam), C.byref(opt), radiance.data_ptr(),
-                                               C.byref(st)))
+        self._call(self.lib.vira_gpu_di_render, self.h, C.byref(cam), C.byref(opt), radiance.data_ptr(),
+                                               C.byref(st))
         self.last_stats = {f: getattr(st, f) for f, _ in capi.DIStats._fields_}
         return radiance
 
@@ -395,10 +468,10 @@ class Context:
         cam = camera.to_c()
         opt = options.to_c()
         g = grad_image.contiguous()
-        capi.check(self.lib.vira_gpu_adjoint(self.h, C.byref(cam), C.byref(opt), g.data_ptr(),
+        self._call(self.lib.vira_gpu_adjoint, self.h, C.byref(cam), C.byref(opt), g.data_ptr(),
                                              None if grad_heights is None else grad_heights.data_ptr(),
                                              None if grad_texture is None else grad_texture.data_ptr(),
-                                             C.byref(st)))
+                                             C.byref(st))
         self.last_stats = {f: getattr(st, f) for f, _ in capi.DIStats._fields_}
         return self.last_stats
 

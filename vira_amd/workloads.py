@@ -196,3 +196,171 @@ def build_diff(ctx: S.Context, sc: DemScene, options: S.DIOptions | None = None,
         target = dem(th, tt).clone()
     dem._apply(b.heights, b.texture)
     return DiffScene(b, dem, target, th, tt)
+
+
+# ---------------------------------------------------------------- config 3: batched mixed workload
+def _lowbias32(x):
+    x = np.asarray(x, dtype=np.uint64) & 0xFFFFFFFF
+    x ^= x >> 16
+    x = (x * 0x7FEB352D) & 0xFFFFFFFF
+    x ^= x >> 15
+    x = (x * 0x846CA68B) & 0xFFFFFFFF
+    x ^= x >> 16
+    return x
+
+
+def _lattice3(ix, iy, iz, octave, seed):
+    h = _lowbias32(np.uint64(seed) ^ np.uint64(0x9E3779B9))
+    h = _lowbias32(h ^ np.uint64(octave))
+    for c in (ix, iy, iz):
+        h = _lowbias32(h ^ (np.asarray(c, np.int64).astype(np.uint64) & 0xFFFFFFFF))
+    return h.astype(np.float64) / 4294967295.0 * 2.0 - 1.0
+
+
+def fbm3(p: np.ndarray, seed: int, octaves: int = 6) -> np.ndarray:
+    """3-D value-noise fBm in [-1, 1] at points p (n, 3): lacunarity 2, gain 0.5,
+    quintic fade, lattice values hashed with lowbias32 (seed, octave, ix, iy, iz)."""
+    total = np.zeros(p.shape[0])
+    amp, norm, freq = 1.0, 0.0, 1.0
+    for o in range(octaves):
+        q = p * freq
+        i0 = np.floor(q).astype(np.int64)
+        f = q - i0
+        w = f * f * f * (f * (f * 6.0 - 15.0) + 10.0)
+        acc = np.zeros(p.shape[0])
+        for dx in (0, 1):
+            for dy in (0, 1):
+                for dz in (0, 1):
+                    v = _lattice3(i0[:, 0] + dx, i0[:, 1] + dy, i0[:, 2] + dz, o, seed)
+                    wt = ((w[:, 0] if dx else 1 - w[:, 0]) * (w[:, 1] if dy else 1 - w[:, 1]) *
+                          (w[:, 2] if dz else 1 - w[:, 2]))
+                    acc += wt * v
+        total += amp * acc
+        norm += amp
+        amp *= 0.5
+        freq *= 2.0
+    return total / norm
+
+
+def small_body_mesh(seed: int = 3, axes=(500.0, 350.0, 300.0), nlat: int = 251, nlon: int = 400,
+                    relief: float = 0.08):
+    """Seeded small-body shape model: a latitude/longitude tessellated ellipsoid
+    (semi-axes `axes`, metres) with radial fBm relief; 2 * nlon * (nlat - 1)
+    triangles (200,000 by default), outward winding.  Returns (pos, idx)."""
+    th = np.pi * np.arange(1, nlat) / nlat
+    ph = 2.0 * np.pi * np.arange(nlon) / nlon
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    ring = np.stack([np.sin(T) * np.cos(P), np.sin(T) * np.sin(P), np.cos(T)], -1).reshape(-1, 3)
+    dirs = np.concatenate([[[0.0, 0.0, 1.0]], ring, [[0.0, 0.0, -1.0]]])
+    a, b, c = axes
+    re = 1.0 / np.sqrt((dirs[:, 0] / a) ** 2 + (dirs[:, 1] / b) ** 2 + (dirs[:, 2] / c) ** 2)
+    r = re * (1.0 + relief * fbm3(dirs * 2.0 + 7.0, seed))
+    pos = dirs * r[:, None]
+    N, S = 0, 1 + (nlat - 1) * nlon
+    vid = lambda i, j: 1 + i * nlon + (j % nlon)
+    tris = []
+    for j in range(nlon):
+        tris.append((N, vid(0, j), vid(0, j + 1)))
+    for i in range(nlat - 2):
+        for j in range(nlon):
+            tris.append((vid(i, j), vid(i + 1, j), vid(i + 1, j + 1)))
+            tris.append((vid(i, j), vid(i + 1, j + 1), vid(i, j + 1)))
+    for j in range(nlon):
+        tris.append((S, vid(nlat - 2, j + 1), vid(nlat - 2, j)))
+    idx = np.asarray(tris, np.uint32)
+    p0, p1, p2 = pos[idx[:, 0]], pos[idx[:, 1]], pos[idx[:, 2]]
+    fn = np.cross(p1 - p0, p2 - p0)
+    if np.mean(np.sum(fn * (p0 + p1 + p2), 1)) < 0:
+        idx = idx[:, [0, 2, 1]].copy()
+    return pos.astype(np.float32), idx
+
+
+BATCH_SIDES = (128, 256, 512, 1024, 2048)
+BATCH_SPP = (1, 4, 16, 64)
+
+
+@dataclass
+class BatchMeshes:
+    ids: list          # mesh ids: config-0 DEM, config-1 DEM, config-2 DEM, small body
+    scenes: list       # DemScene (or None for the small body)
+    keep: list         # tensors the meshes' materials point at
+
+
+def build_batch_meshes(ctx: S.Context, c1_built: "BuiltScene | None" = None, bvh_width: int = 4) -> BatchMeshes:
+    """The four meshes of the batched workload: the DEMs of configs 0, 1 and 2
+    (with their albedo) and the 200k-triangle small body (albedo 0.12)."""
+    ids, scenes, keep = [], [], []
+    for sc in (CONFIG0, CONFIG1, CONFIG2):
+        b = c1_built if (sc is CONFIG1 and c1_built is not None) else build(ctx, sc, bvh_width=bvh_width)
+        ids.append(b.mesh)
+        scenes.append(sc)
+        keep.append(b.texture)
+        keep.append(b.heights)
+    pos, idx = small_body_mesh()
+    mid = ctx.mesh_upload(pos, idx)
+    ctx.blas_build(mid, width=bvh_width)
+    ctx.material_set(mid, albedo=0.12)
+    ids.append(mid)
+    scenes.append(None)
+    # the config-1 material must point at its texture again (build() of other scenes does not touch it)
+    return BatchMeshes(ids, scenes, keep)
+
+
+def _unit(v):
+    v = np.asarray(v, np.float64)
+    return v / np.linalg.norm(v)
+
+
+def make_batch(ctx: S.Context, meshes: BatchMeshes, n: int = 64, seed: int = 3, bounces: int = 3,
+               passes=("radiance", "depth", "tri_id"), sides=BATCH_SIDES, spps=BATCH_SPP):
+    """Config 3: n requests, image side uniform over `sides`, spp uniform over
+    `spps`, mesh uniform over the four meshes, random camera and sun per request
+    (numpy Generator seeded with `seed`)."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(n):
+        side = int(rng.choice(sides))
+        spp = int(rng.choice(spps))
+        k = int(rng.integers(0, len(meshes.ids)))
+        sc = meshes.scenes[k]
+        tilt = np.radians(rng.uniform(0.0, 20.0))
+        az = rng.uniform(0.0, 2.0 * np.pi)
+        elev = rng.uniform(20.0, 60.0)
+        sun_az = rng.uniform(0.0, 360.0)
+        if sc is None:  # small body: camera 1.5-3 km away looking at its centre
+            d = _unit(rng.standard_normal(3))
+            pos = d * rng.uniform(1500.0, 3000.0)
+            cam = S.Camera(side, side, 30.0, tuple(pos))
+            up = (0.0, 0.0, 1.0) if abs(d[2]) < 0.9 else (1.0, 0.0, 0.0)
+            cam.look_in_direction(tuple(-d), up=up)
+            sd = _unit(rng.standard_normal(3))
+            e = _unit(np.cross(sd, [0.0, 0.0, 1.0] if abs(sd[2]) < 0.9 else [1.0, 0.0, 0.0]))
+            light = S.directional_sun(90.0, 0.0, 1.0, up=tuple(sd), east=tuple(e))
+        elif sc.sphere_radius > 0:  # config-1 DEM: rebased frame, ground 100 km below along -x
+            ext = (sc.dem - 1) * sc.spacing
+            pos = (rng.uniform(-0.3, 0.3) * sc.altitude, rng.uniform(-0.15, 0.15) * ext,
+                   rng.uniform(-0.15, 0.15) * ext)
+            cam = S.Camera(side, side, sc.fov_deg, pos)
+            cam.look_in_direction((-np.cos(tilt), np.sin(tilt) * np.cos(az), np.sin(tilt) * np.sin(az)),
+                                  up=(0.0, 0.0, 1.0))
+            off = np.asarray(rebase_offset(sc))
+            ground = np.array([sc.sphere_radius, 0.0, 0.0]) - off
+            light = S.directional_sun(elev, sun_az, 1.0, origin=ground, up=(1.0, 0.0, 0.0), east=(0.0, 1.0, 0.0))
+        else:  # flat DEMs of configs 0 and 2
+            ext = (sc.dem - 1) * sc.spacing
+            pos = (rng.uniform(-0.15, 0.15) * ext, rng.uniform(-0.15, 0.15) * ext,
+                   sc.altitude * rng.uniform(0.7, 1.3))
+            cam = S.Camera(side, side, sc.fov_deg, pos)
+            cam.look_in_direction((np.sin(tilt) * np.cos(az), np.sin(tilt) * np.sin(az), -np.cos(tilt)))
+            light = S.directional_sun(elev, sun_az, 1.0)
+        opts = S.PathTracerOptions(samples=spp, bounces=bounces, russian_roulette=True, rr_start=1, seed=1000 + i)
+        reqs.append(S.RenderRequest(meshes.ids[k], cam, opts, light, S.RenderPasses(side, side, which=passes)))
+    return reqs
+
+
+def render_plain(ctx: S.Context, reqs) -> None:
+    """The per-request form: one synchronous render call per request."""
+    for r in reqs:
+        ctx.tlas_build([r.mesh])
+        ctx.lights_set([] if r.light is None else [r.light])
+        ctx.render(r.camera, r.options, r.passes)
