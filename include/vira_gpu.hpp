@@ -1,0 +1,152 @@
+// vira_gpu.hpp -- C++ wrapper of the vira_gpu C-ABI (include/vira_gpu.h).
+//
+// RAII ownership of the context, and every non-OK status rethrown as
+// vira_gpu::Error (a std::runtime_error), as the reference throws
+// (vira_tlas.ipp:33, cpu_path_tracer.ipp:121).  PathTracer mirrors the public
+// surface of vira::rendering::CPUPathTracer (cpu_path_tracer.hpp:46-71):
+// `options`, `renderPasses` and `render(camera)` over the context's scene.
+// No HIP types appear here: device buffers are plain pointers the caller owns.
+#ifndef VIRA_GPU_HPP
+#define VIRA_GPU_HPP
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "vira_gpu.h"
+
+namespace vira_gpu {
+
+class Error : public std::runtime_error {
+public:
+    Error(vira_gpu_status s, const std::string& msg) : std::runtime_error(msg), status(s) {}
+    vira_gpu_status status;
+};
+
+inline void check(vira_gpu_status s) {
+    if (s != VIRA_GPU_OK) throw Error(s, vira_gpu_last_error());
+}
+
+inline std::string version() { return vira_gpu_version(); }
+
+// One GPU, one stream; owns meshes, BVHs, lights and materials (the parts of
+// vira::Scene the renderer reads, scene.hpp:212-302).
+class Context {
+public:
+    explicit Context(int device = 0, void* stream = nullptr) {
+        vira_gpu_opts o{device, stream};
+        check(vira_gpu_create(&h_, &o));
+    }
+    ~Context() {
+        if (h_) (void)vira_gpu_destroy(h_);
+    }
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+    Context(Context&& o) noexcept : h_(std::exchange(o.h_, nullptr)) {}
+    Context& operator=(Context&& o) noexcept {
+        if (this != &o) {
+            if (h_) (void)vira_gpu_destroy(h_);
+            h_ = std::exchange(o.h_, nullptr);
+        }
+        return *this;
+    }
+    vira_gpu_ctx* get() const { return h_; }
+
+    void synthFbm(int w, int h, uint32_t seed, double amplitude, int octaves, double base_cells, float* heights_dev) {
+        check(vira_gpu_synth_fbm(h_, w, h, seed, amplitude, octaves, base_cells, heights_dev));
+    }
+    // DEM::makeMesh (dem.ipp:138-211)
+    int32_t demMesh(const float* heights_dev, int w, int h, const vira_gpu_projection& p,
+                    const float* grey_albedo_dev = nullptr) {
+        int32_t id = -1;
+        check(vira_gpu_dem_mesh(h_, heights_dev, w, h, &p, grey_albedo_dev, &id));
+        return id;
+    }
+    // Mesh(vertexBuffer, indexBuffer) (mesh.ipp:25-83)
+    int32_t meshUpload(const std::vector<float>& positions, const std::vector<uint32_t>& indices,
+                       const float* albedo = nullptr, const float* uv = nullptr, const float* normals = nullptr) {
+        int32_t id = -1;
+        check(vira_gpu_mesh_upload(h_, positions.data(), positions.size() / 3, indices.data(), indices.size() / 3,
+                                   albedo, uv, normals, &id));
+        return id;
+    }
+    void releaseMesh(int32_t mesh) { check(vira_gpu_mesh_release(h_, mesh)); }
+    void demUpdate(int32_t mesh, const float* heights_dev) { check(vira_gpu_dem_update(h_, mesh, heights_dev)); }
+    // Mesh::buildBVH (mesh.ipp:396-418)
+    vira_gpu_bvh_stats buildBLAS(int32_t mesh, vira_gpu_bvh_opts o = {4, 16, 4}) {
+        check(vira_gpu_blas_build(h_, mesh, &o));
+        vira_gpu_bvh_stats s{};
+        check(vira_gpu_blas_stats(h_, mesh, &s));
+        return s;
+    }
+    // Scene::buildTLAS (scene.ipp:712-812)
+    void buildTLAS(const std::vector<vira_gpu_instance>& instances) {
+        check(vira_gpu_tlas_build(h_, instances.data(), instances.size()));
+    }
+    void setLights(const std::vector<vira_gpu_light>& lights) {
+        check(vira_gpu_lights_set(h_, lights.data(), (int)lights.size()));
+    }
+    void setMaterial(int32_t mesh, const vira_gpu_material& m) { check(vira_gpu_material_set(h_, mesh, &m)); }
+    void synchronize() { check(vira_gpu_synchronize(h_)); }
+
+private:
+    vira_gpu_ctx* h_ = nullptr;
+};
+
+// CPUPathTracerOptions (cpu_path_tracer.hpp:30-43) and the GPU additions.
+struct PathTracerOptions {
+    int samples = 1;
+    int bounces = 0;
+    bool russianRoulette = false;
+    int rrStart = 1;
+    uint32_t seed = 0;
+    int integrator = 0;  // 0 auto, 1 per-pixel, 2 wavefront
+};
+
+// GPUPathTracer: CPUPathTracer's render(camera, scene) with the scene held by
+// the context; renderPasses are caller-owned device buffers (W*H*8 floats per
+// spectral pass, as RenderPasses::updateImages fills them, passes.ipp:81-127).
+class PathTracer {
+public:
+    explicit PathTracer(Context& ctx) : ctx_(&ctx) {}
+    PathTracerOptions options;
+    vira_gpu_outputs renderPasses{};
+    vira_gpu_render_stats lastStats{};
+
+    void render(const vira_gpu_camera& camera) {
+        vira_gpu_render_opts o{options.samples, options.bounces, options.russianRoulette ? 1 : 0, options.rrStart,
+                               options.seed, options.integrator};
+        check(vira_gpu_render(ctx_->get(), &camera, &o, &renderPasses, &lastStats));
+    }
+
+private:
+    Context* ctx_;
+};
+
+// Batched requests in one persistent launch (vira_gpu_batch_submit).
+inline vira_gpu_batch_stats renderBatch(Context& ctx, const std::vector<vira_gpu_request>& requests,
+                                        const vira_gpu_batch_config* cfg = nullptr) {
+    vira_gpu_batch_stats s{};
+    check(vira_gpu_batch_submit(ctx.get(), requests.data(), requests.size(), cfg, &s));
+    return s;
+}
+
+// Differentiable direct illumination: forward, then the adjoint.
+inline vira_gpu_di_stats renderDirect(Context& ctx, const vira_gpu_camera& cam, const vira_gpu_di_opts& o,
+                                      float* radiance_dev) {
+    vira_gpu_di_stats s{};
+    check(vira_gpu_di_render(ctx.get(), &cam, &o, radiance_dev, &s));
+    return s;
+}
+inline vira_gpu_di_stats adjoint(Context& ctx, const vira_gpu_camera& cam, const vira_gpu_di_opts& o,
+                                 const float* dL_dimage_dev, float* dL_dheights_dev, float* dL_dtexture_dev) {
+    vira_gpu_di_stats s{};
+    check(vira_gpu_adjoint(ctx.get(), &cam, &o, dL_dimage_dev, dL_dheights_dev, dL_dtexture_dev, &s));
+    return s;
+}
+
+}  // namespace vira_gpu
+
+#endif  // VIRA_GPU_HPP

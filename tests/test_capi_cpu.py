@@ -39,3 +39,19 @@ def test_version_and_error_without_gpu():
     st = lib.vira_gpu_synchronize(None)
     assert st == 1
     assert b"null context" in lib.vira_gpu_last_error()
+
+
+def test_cpp_wrapper_reports_status_without_gpu():
+    """The C++ wrapper example links against the library and turns a failed
+    call into vira_gpu::Error (no GPU in this container: device query fails)."""
+    import json
+    import subprocess
+    import torch
+    exe = ROOT / "vira_amd" / "lib" / "render_dem"
+    if not exe.exists():
+        pytest.skip("C++ example not built (run __graft_entry__.build())")
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present: covered by tests/test_gpu_cpp.py")
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    j = json.loads(r.stdout.strip().splitlines()[-1])
+    assert r.returncode == 3 and j["status"] in (1, 2) and j["error"]

@@ -38,6 +38,8 @@ FLAGS = [
     f"-I{ROOT / 'include'}",
 ]
 SOURCES = ["capi.hip", "dem.hip", "bvh.hip", "render.hip", "adjoint.hip", "batch.hip", "scan.hip"]
+# C++ programs over include/vira_gpu.hpp, linked against the library (host code only)
+EXAMPLES = ["render_dem.cpp"]
 
 
 def _compile(src: Path, obj: Path) -> None:
@@ -71,6 +73,13 @@ def build(verbose: bool = True) -> Path:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB} ({LIB.stat().st_size // 1024} KiB)")
+    for ex_src in EXAMPLES:
+        exe = OUT_DIR / Path(ex_src).stem
+        cmd = [HIPCC, "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(PKG / "examples" / ex_src), "-o",
+               str(exe), f"-L{OUT_DIR}", "-lvira_gpu", "-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"building {ex_src} failed:\n{r.stdout}\n{r.stderr}")
     return LIB
 
 
