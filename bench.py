@@ -193,6 +193,38 @@ def batch_leg(ctx, c1_built, steps: int, warmup: int) -> dict:
             "autotune_s": round(tune_s, 2)}
 
 
+def instanced_leg(ctx, c1_built, steps: int, warmup: int) -> dict:
+    """Config 4: two-level scene (the config-1 tile + 64k sphere for the rest of
+    the Moon, 200k small body, 50k spacecraft) with FP64 model matrices at
+    ~1.5e8 km, camera on the spacecraft; 4096^2, 8 spp, 2 bounces, sun as a
+    sphere light at the origin.  Rebasing into the camera frame runs on the GPU
+    in FP64 at every render."""
+    import torch
+    from vira_amd import scene as S
+    from vira_amd import workloads as W
+
+    isc = W.build_instanced(ctx, c1_built)
+    passes = S.RenderPasses(4096, 4096, which=("radiance", "depth", "tri_id", "instance_id"))
+    for _ in range(warmup):
+        ctx.render(isc.camera, isc.options, passes)
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        stats.append(ctx.render(isc.camera, isc.options, passes))
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    rays = stats[0]["primary_rays"] + stats[0]["shadow_rays"]
+    inst = passes.instance_id.reshape(-1)
+    seen = sorted(int(v) for v in torch.unique(inst[inst >= 0]).cpu())
+    print(f"# config 4: {dt * 1e3:.1f} ms/frame, {rays / dt / 1e6:.1f} Mrays/s (primary+shadow), instances in view "
+          f"{seen}", file=sys.stderr, flush=True)
+    ctx.tlas_build([c1_built.mesh])  # back to the config-1 scene
+    ctx.lights_set(c1_built.lights)
+    return {"workload": "instanced_moon_smallbody_spacecraft_4096px_8spp_2b", "ms_per_frame": round(dt * 1e3, 2),
+            "mrays_s": round(rays / dt / 1e6, 1), "bounce_rays": stats[0]["bounce_rays"], "instances_in_view": seen}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,6 +237,7 @@ def main() -> int:
     ap.add_argument("--integrator", default="auto", choices=["auto", "fused", "wavefront"])
     ap.add_argument("--no-adjoint", action="store_true", help="skip the config-2 adjoint/forward leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the config-3 batched/per-request leg")
+    ap.add_argument("--no-instanced", action="store_true", help="skip the config-4 instanced-scene leg")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -260,6 +293,7 @@ def main() -> int:
 
     adjoint = None if args.no_adjoint else adjoint_leg(ctx, max(3, args.steps), max(1, args.warmup))
     batch = None if (args.no_batch or args.scene != "c1") else batch_leg(ctx, built, max(1, min(3, args.steps)), 1)
+    instanced = None if (args.no_instanced or args.scene != "c1") else instanced_leg(ctx, built, max(1, min(3, args.steps)), 1)
 
     cpu = None
     counts = None
@@ -322,6 +356,7 @@ def main() -> int:
         "adjoint_forward_ratio": None if adjoint is None else adjoint["ratio"],
         "batch": batch,
         "adjoint": adjoint,
+        "instanced": instanced,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

@@ -135,6 +135,7 @@ typedef struct {
     float* depth_dev;      /* W*H mean first-hit distance (inf = no hit) */
     float* alpha_dev;      /* W*H hits / spp                  */
     uint32_t* tri_id_dev;  /* W*H first-hit triangle (0xffffffff = none) */
+    uint32_t* instance_id_dev; /* W*H first-hit instance: index into the TLAS (0xffffffff = none) */
 } vira_gpu_outputs;
 
 typedef struct {
@@ -252,8 +253,15 @@ typedef struct {
 } vira_gpu_instance;
 
 /* Scene::buildTLAS -> EmbreeTLAS::newInstance/build (scene.ipp:712-812,
- * embree_tlas.ipp:77-115).  This build renders one instance with an identity
- * transform; other transforms return VIRA_GPU_ERR_UNSUPPORTED. */
+ * embree_tlas.ipp:77-115).  1..16 instances, each a built BLAS (all of one
+ * BVH width) placed by an FP64 column-major model matrix (invertible affine).
+ * One instance with the identity matrix renders in the caller's frame as
+ * given.  Any other TLAS is rendered camera-relative: at each render a kernel
+ * subtracts the FP64 camera position from every model matrix and light
+ * position and rounds the results (and the inverses and normal matrices) to
+ * FP32, so positions may be planetary- or AU-scale.  Instanced scenes use the
+ * per-pixel renderer (at most one light); the direct-illumination adjoint
+ * takes a single-instance TLAS. */
 vira_gpu_status vira_gpu_tlas_build(vira_gpu_ctx* ctx, const vira_gpu_instance* instances_host, size_t n);
 
 /* --- shading state (a7) --------------------------------------------------- */

@@ -239,6 +239,28 @@ void orc_render(void* h, int precision, const Camera* cam, const Light* lights, 
     else render_f32(o->m, o->b, *cam, lights, nl, *mat, *opt, out, nthreads, pixel_list, npix);
 }
 
+// Two-level scene of n instances (mesh handles, FP64 column-major model
+// matrices, materials); outputs as orc_render plus the first-hit instance.
+void orc_render_scene(void* const* meshes, const double* transforms, const Material* mats, int n, int precision,
+                      const Camera* cam, const Light* lights, int nl, const Options* opt, float* radiance,
+                      float* direct, float* indirect, float* albedo, float* normal, float* depth, float* alpha,
+                      uint32_t* tri_id, uint32_t* instance_id, uint64_t* counts, int nthreads, const int* pixel_list,
+                      size_t npix) {
+    std::vector<InstanceO> inst(n);
+    for (int i = 0; i < n; ++i) {
+        auto* o = static_cast<OMesh*>(meshes[i]);
+        inst[i].m = &o->m;
+        inst[i].b = &o->b;
+        inst[i].mat = mats[i];
+        for (int k = 0; k < 16; ++k) inst[i].M[k] = transforms[16 * i + k];
+    }
+    Outputs out;
+    out.radiance = radiance; out.direct = direct; out.indirect = indirect; out.albedo = albedo;
+    out.normal = normal; out.depth = depth; out.alpha = alpha; out.tri_id = tri_id; out.counts = counts;
+    out.instance_id = instance_id;
+    render_scene(precision, inst.data(), n, *cam, lights, nl, *opt, out, nthreads, pixel_list, npix);
+}
+
 // ---- fixed-visibility direct illumination, FP64 forward / adjoint / finite differences
 void* orc_dem_diff_create(int W, int H, const Projection* proj, const float* heights, const float* grey) {
     return new DemDiff(dem_diff_make(W, H, *proj, heights, grey));

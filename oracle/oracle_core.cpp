@@ -323,16 +323,97 @@ struct PixelAcc {
     uint32_t tri = 0xffffffffu;
 };
 
+// Camera-relative placement of an instance: the FP64 model matrix minus the
+// camera position, its inverse and normal matrix, rounded to R (the same FP64
+// operations as the GPU's rebase kernel).
 template <class R>
-void render_impl(const Mesh& m, const BVH& b, const Camera& cam, const Light* lights, int nl,
-                 const Material& mat, const Options& opt, Outputs& out, int nthreads,
-                 const int* pixel_list, size_t npix) {
-    const size_t total = pixel_list ? npix : (size_t)cam.W * cam.H;
+struct Placement {
+    R l2w[12], w2l[12], nm[9];
+};
+
+template <class R>
+Placement<R> rebase_instance(const double* M, const double c[3]) {
+    double A[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 3; ++k) A[r][k] = M[k * 4 + r];
+    const double t[3] = {M[12] - c[0], M[13] - c[1], M[14] - c[2]};
+    const double c00 = A[1][1] * A[2][2] - A[1][2] * A[2][1];
+    const double c01 = A[1][2] * A[2][0] - A[1][0] * A[2][2];
+    const double c02 = A[1][0] * A[2][1] - A[1][1] * A[2][0];
+    const double det = (A[0][0] * c00 + A[0][1] * c01) + A[0][2] * c02;
+    const double id = 1.0 / det;
+    double inv[3][3];
+    inv[0][0] = c00 * id;
+    inv[1][0] = c01 * id;
+    inv[2][0] = c02 * id;
+    inv[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) * id;
+    inv[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) * id;
+    inv[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) * id;
+    inv[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) * id;
+    inv[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) * id;
+    inv[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) * id;
+    Placement<R> P;
+    for (int r = 0; r < 3; ++r) {
+        for (int k = 0; k < 3; ++k) {
+            P.l2w[r * 4 + k] = (R)A[r][k];
+            P.w2l[r * 4 + k] = (R)inv[r][k];
+            P.nm[r * 3 + k] = (R)inv[k][r];
+        }
+        P.l2w[r * 4 + 3] = (R)t[r];
+        P.w2l[r * 4 + 3] = (R)(-((inv[r][0] * t[0] + inv[r][1] * t[1]) + inv[r][2] * t[2]));
+    }
+    return P;
+}
+
+template <class R>
+inline V3<R> xform_point(const R* m, const V3<R>& p) {
+    return {((m[0] * p.x + m[1] * p.y) + m[2] * p.z) + m[3], ((m[4] * p.x + m[5] * p.y) + m[6] * p.z) + m[7],
+            ((m[8] * p.x + m[9] * p.y) + m[10] * p.z) + m[11]};
+}
+template <class R>
+inline V3<R> xform_dir(const R* m, const V3<R>& d) {
+    return {(m[0] * d.x + m[1] * d.y) + m[2] * d.z, (m[4] * d.x + m[5] * d.y) + m[6] * d.z,
+            (m[8] * d.x + m[9] * d.y) + m[10] * d.z};
+}
+template <class R>
+inline V3<R> xform_normal(const R* n, const V3<R>& v) {
+    return {(n[0] * v.x + n[1] * v.y) + n[2] * v.z, (n[3] * v.x + n[4] * v.y) + n[5] * v.z,
+            (n[6] * v.x + n[7] * v.y) + n[8] * v.z};
+}
+
+template <class R>
+void render_impl(const InstanceO* inst, int ninst, const Camera& cam_in, const Light* lights_in, int nl,
+                 const Options& opt, Outputs& out, int nthreads, const int* pixel_list, size_t npix) {
+    const size_t total = pixel_list ? npix : (size_t)cam_in.W * cam_in.H;
     std::atomic<size_t> next{0};
     std::atomic<uint64_t> g_cnt[7];
     for (auto& c : g_cnt) c = 0;
-    auto vpos = [&](uint32_t v) { return mesh_pos<R>(m, v); };
-    auto vnrm = [&](uint32_t v) { return V3<R>{(R)m.nrm[3 * v], (R)m.nrm[3 * v + 1], (R)m.nrm[3 * v + 2]}; };
+    static const double kIdentity[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    const bool rebased = !(ninst == 1 && std::memcmp(inst[0].M, kIdentity, sizeof(kIdentity)) == 0);
+    // camera-relative frame for instanced scenes: camera at the origin, lights moved in FP64
+    Camera cam = cam_in;
+    std::vector<Light> lights(lights_in, lights_in + nl);
+    std::vector<Placement<R>> place(ninst);
+    if (rebased) {
+        for (int i = 0; i < ninst; ++i) place[i] = rebase_instance<R>(inst[i].M, cam_in.pos);
+        for (auto& L : lights)
+            for (int k = 0; k < 3; ++k) L.pos[k] = L.pos[k] - cam_in.pos[k];
+        cam.pos[0] = cam.pos[1] = cam.pos[2] = 0.0;
+    }
+    // closest hit over the instances (ties keep the lower instance), or occlusion
+    auto trace = [&](const RayT<R>& ray, bool any, R tmax, TravCount* tc, uint32_t& hi) {
+        Hit<R> best{(R)INFINITY, 0xffffffffu, 0, 0};
+        hi = 0;
+        for (int i = 0; i < ninst; ++i) {
+            RayT<R> r = ray;
+            if (rebased) r = RayT<R>{xform_point(place[i].w2l, ray.o), xform_dir(place[i].w2l, ray.d)};
+            Hit<R> h = bvh_trace<R>(*inst[i].m, *inst[i].b, r, any, tmax, tc);
+            if (h.id == 0xffffffffu) continue;
+            if (best.id == 0xffffffffu || h.t < best.t) { best = h; hi = (uint32_t)i; }
+            if (any) break;
+        }
+        return best;
+    };
 
     auto worker = [&]() {
         uint64_t cnt[7] = {0};
@@ -368,8 +449,15 @@ void render_impl(const Mesh& m, const BVH& b, const Camera& cam, const Light* li
                                                 : nullptr;
                     if (rec) { rec[0] = 0xffffffffu; rec[1] = rec[2] = rec[3] = 0; }
                     for (int bounce = 0; bounce <= opt.bounces; ++bounce) {
-                        Hit<R> h = bvh_trace<R>(m, b, ray, false, (R)INFINITY, &tc_closest);
+                        uint32_t hinst = 0;
+                        Hit<R> h = trace(ray, false, (R)INFINITY, &tc_closest, hinst);
                         if (h.id == 0xffffffffu) break;
+                        const Mesh& m = *inst[hinst].m;
+                        const Material& mat = inst[hinst].mat;
+                        auto vpos = [&](uint32_t v) { return mesh_pos<R>(m, v); };
+                        auto vnrm = [&](uint32_t v) {
+                            return V3<R>{(R)m.nrm[3 * v], (R)m.nrm[3 * v + 1], (R)m.nrm[3 * v + 2]};
+                        };
                         if (rec && bounce == 0) {
                             float fb1 = (float)h.b1, fb2 = (float)h.b2;
                             rec[0] = h.id;
@@ -389,7 +477,9 @@ void render_impl(const Mesh& m, const BVH& b, const Camera& cam, const Light* li
                         V3<R> n0 = vnrm(i0), n1 = vnrm(i1), n2 = vnrm(i2);
                         V3<R> Nl = fn;
                         if (mat.smooth) Nl = normalize((w0 * n1 + w1 * n2) + w2 * n0);
-                        V3<R> N = Nl;
+                        // instances: normal matrix on the normals (not renormalised, cpu_path_tracer.ipp:310)
+                        const V3<R> N = rebased ? xform_normal(place[hinst].nm, Nl) : Nl;
+                        const V3<R> fng = rebased ? xform_normal(place[hinst].nm, fn) : fn;
                         V3<R> arb = (std::fabs(N.x) < (R)0.9) ? V3<R>{1, 0, 0} : V3<R>{0, 1, 0};
                         V3<R> tg = normalize(cross(arb, N));
                         V3<R> bt = cross(N, tg);
@@ -410,14 +500,15 @@ void render_impl(const Mesh& m, const BVH& b, const Camera& cam, const Light* li
                             V3<R> diff = sp - P;
                             if (dot(diff, Nl) > (R)0) P = sp;
                         }
+                        if (rebased) P = xform_point(place[hinst].l2w, P);  // model matrix (cpu_path_tracer.ipp:344)
                         if (bounce == 0) {
-                            if (acc.tri == 0xffffffffu) acc.tri = h.id;
+                            if (acc.tri == 0xffffffffu) acc.tri = h.id | (hinst << 27);
                             for (int k = 0; k < NB; ++k) acc.alb[k] = acc.alb[k] + alb[k];
                             acc.nrm = acc.nrm + N;
                             acc.depth = acc.depth + h.t;
                             acc.count++;
                         }
-                        V3<R> org = offset_point(P, fn);
+                        V3<R> org = offset_point(P, fng);
                         V3<R> V = {-ray.d.x, -ray.d.y, -ray.d.z};
                         (void)V;
                         R vrad[NB] = {0};
@@ -429,7 +520,8 @@ void render_impl(const Mesh& m, const BVH& b, const Camera& cam, const Light* li
                             if (!(lpdf > 0)) continue;
                             cnt[1]++;
                             RayT<R> sray{org, ld};
-                            Hit<R> sh = bvh_trace<R>(m, b, sray, true, dist, &tc_any);
+                            uint32_t si = 0;
+                            Hit<R> sh = trace(sray, true, dist, &tc_any, si);
                             if (sh.id != 0xffffffffu) continue;  // occluded
                             if (rec && bounce == 0) rec[3] |= 1u << l;
                             R ndl = dot(N, ld);
@@ -505,7 +597,8 @@ void render_impl(const Mesh& m, const BVH& b, const Camera& cam, const Light* li
                 }
                 if (out.depth) out.depth[o] = acc.count ? (float)(acc.depth / (R)acc.count) : INFINITY;
                 if (out.alpha) out.alpha[o] = (float)acc.count / (float)opt.spp;
-                if (out.tri_id) out.tri_id[o] = acc.tri;
+                if (out.tri_id) out.tri_id[o] = acc.tri == 0xffffffffu ? acc.tri : (acc.tri & ((1u << 27) - 1u));
+                if (out.instance_id) out.instance_id[o] = acc.tri == 0xffffffffu ? acc.tri : (acc.tri >> 27);
             }
         }
         cnt[3] = tc_closest.internal; cnt[4] = tc_closest.tris;
@@ -523,15 +616,27 @@ void render_impl(const Mesh& m, const BVH& b, const Camera& cam, const Light* li
 
 }  // namespace
 
+static InstanceO identity_instance(const Mesh& m, const BVH& b, const Material& mat) {
+    InstanceO I{&m, &b, mat, {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1}};
+    return I;
+}
+
 void render_f32(const Mesh& m, const BVH& b, const Camera& cam, const Light* lights, int nl,
                 const Material& mat, const Options& opt, Outputs& out, int nthreads,
                 const int* pixel_list, size_t npix) {
-    render_impl<float>(m, b, cam, lights, nl, mat, opt, out, nthreads, pixel_list, npix);
+    InstanceO I = identity_instance(m, b, mat);
+    render_impl<float>(&I, 1, cam, lights, nl, opt, out, nthreads, pixel_list, npix);
 }
 void render_f64(const Mesh& m, const BVH& b, const Camera& cam, const Light* lights, int nl,
                 const Material& mat, const Options& opt, Outputs& out, int nthreads,
                 const int* pixel_list, size_t npix) {
-    render_impl<double>(m, b, cam, lights, nl, mat, opt, out, nthreads, pixel_list, npix);
+    InstanceO I = identity_instance(m, b, mat);
+    render_impl<double>(&I, 1, cam, lights, nl, opt, out, nthreads, pixel_list, npix);
+}
+void render_scene(int precision, const InstanceO* inst, int n, const Camera& cam, const Light* lights, int nl,
+                  const Options& opt, Outputs& out, int nthreads, const int* pixel_list, size_t npix) {
+    if (precision == 64) render_impl<double>(inst, n, cam, lights, nl, opt, out, nthreads, pixel_list, npix);
+    else render_impl<float>(inst, n, cam, lights, nl, opt, out, nthreads, pixel_list, npix);
 }
 
 }  // namespace vo

@@ -523,6 +523,16 @@ struct Outputs {  // all optional (nullptr = skip); pixel index i + j*W
                                  // [3..4] internal/tris closest, [5..6] internal/tris any-hit
     uint32_t* records = nullptr; // 4 per (sample, pixel) at [s * W*H + pixel]: first-hit id,
                                  // b1 and b2 bits, bit l = light l sampled and unoccluded
+    uint32_t* instance_id = nullptr;
+};
+
+// One instance of a two-level scene (EmbreeTLAS::newInstance, embree_tlas.ipp:77-115):
+// a mesh with its BVH and material placed by an FP64 column-major model matrix.
+struct InstanceO {
+    const Mesh* m;
+    const BVH* b;
+    Material mat;
+    double M[16];
 };
 
 template <class R>
@@ -544,5 +554,11 @@ void render_f32(const Mesh& m, const BVH& b, const Camera& cam, const Light* lig
 void render_f64(const Mesh& m, const BVH& b, const Camera& cam, const Light* lights, int nl,
                 const Material& mat, const Options& opt, Outputs& out, int nthreads,
                 const int* pixel_list, size_t npix);
+// Two-level scene.  One instance with the identity matrix renders in the
+// caller's frame; anything else renders camera-relative: model matrices and
+// light positions minus the FP64 camera position, rounded to the working
+// precision as the GPU does (FP32) or kept (FP64).
+void render_scene(int precision, const InstanceO* inst, int n, const Camera& cam, const Light* lights, int nl,
+                  const Options& opt, Outputs& out, int nthreads, const int* pixel_list, size_t npix);
 
 }  // namespace vo

@@ -79,6 +79,8 @@ def lib() -> C.CDLL:
         "orc_rng_u01": (C.c_float, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
         "orc_render": (None, [P, C.c_int, C.POINTER(Camera), C.POINTER(Light), C.c_int, C.POINTER(Material),
                               C.POINTER(Options), P, P, P, P, P, P, P, P, P, P, C.c_int, P, C.c_size_t]),
+        "orc_render_scene": (None, [P, P, P, C.c_int, C.c_int, C.POINTER(Camera), C.POINTER(Light), C.c_int,
+                                    C.POINTER(Options), P, P, P, P, P, P, P, P, P, P, C.c_int, P, C.c_size_t]),
         "orc_dem_diff_create": (P, [C.c_int, C.c_int, C.POINTER(Projection), P, P]),
         "orc_dem_diff_destroy": (None, [P]),
         "orc_di_forward64": (None, [P, P, P, P, C.c_size_t, C.c_int, C.POINTER(Light), C.c_int,
@@ -307,6 +309,34 @@ def dem_mesh(heights: np.ndarray, proj: Projection, grey_albedo: np.ndarray | No
     a = np.ones(H * W, np.float32) if grey_albedo is None else np.asarray(grey_albedo, np.float32)
     alb = np.repeat(a[:, None], NB, axis=1)
     return Mesh(pos, idx, nrm, alb, dem_uv(W, H), pos64 if with_pos64 else None)
+
+
+def render_scene(meshes, transforms, mats, cam: Camera, lights, opt: Options, precision=32, pixels=None,
+                 threads=None):
+    """Two-level scene: meshes (Mesh), transforms (4x4 FP64, row-major numpy), mats (Material)."""
+    n = len(meshes)
+    W, H = cam.W, cam.H
+    npx = W * H
+    out = {
+        "radiance": np.zeros((npx, NB), np.float32), "direct": np.zeros((npx, NB), np.float32),
+        "indirect": np.zeros((npx, NB), np.float32), "albedo": np.zeros((npx, NB), np.float32),
+        "normal": np.zeros((npx, 3), np.float32), "depth": np.zeros(npx, np.float32),
+        "alpha": np.zeros(npx, np.float32), "tri_id": np.full(npx, 0xFFFFFFFF, np.uint32),
+        "instance_id": np.full(npx, 0xFFFFFFFF, np.uint32),
+    }
+    cnt = np.zeros(7, np.uint64)
+    handles = (C.c_void_p * n)(*[m.h for m in meshes])
+    tr = np.ascontiguousarray(np.stack([np.asarray(t, np.float64).T.reshape(-1) for t in transforms]))  # column-major
+    marr = (Material * n)(*mats)
+    larr = (Light * max(1, len(lights)))(*lights) if lights else (Light * 1)()
+    pl = None if pixels is None else np.ascontiguousarray(pixels, np.int32)
+    lib().orc_render_scene(handles, tr.ctypes.data, marr, n, precision, C.byref(cam), larr, len(lights),
+                           C.byref(opt), *(out[k].ctypes.data for k in ("radiance", "direct", "indirect", "albedo",
+                                                                        "normal", "depth", "alpha", "tri_id",
+                                                                        "instance_id")),
+                           cnt.ctypes.data, threads or nthreads(), _p(pl), 0 if pl is None else pl.size)
+    out["counts"] = {"primary": int(cnt[0]), "shadow": int(cnt[1]), "bounce": int(cnt[2])}
+    return out
 
 
 class DemDiff:

@@ -116,6 +116,7 @@ class Outputs(C.Structure):
         ("depth_dev", C.c_void_p),
         ("alpha_dev", C.c_void_p),
         ("tri_id_dev", C.c_void_p),
+        ("instance_id_dev", C.c_void_p),
     ]
 
 

@@ -408,6 +408,8 @@ bool same_camera(const vira_gpu_camera& a, const vira_gpu_camera& b) {
 
 MeshG& di_scene_mesh(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_di_opts& o, const char* who) {
     VG_CHECK(c.scene_mesh >= 0, VIRA_GPU_ERR_STATE, std::string(who) + ": no TLAS built");
+    VG_CHECK(!c.instanced, VIRA_GPU_ERR_UNSUPPORTED,
+             std::string(who) + ": direct illumination takes a TLAS of one identity instance");
     MeshG& m = c.mesh(c.scene_mesh);
     VG_CHECK(m.blas.built, VIRA_GPU_ERR_STATE, std::string(who) + ": BLAS not built");
     VG_CHECK(cam.width > 0 && cam.height > 0, VIRA_GPU_ERR_INVALID, std::string(who) + ": empty image");

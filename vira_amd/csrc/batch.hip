@@ -165,7 +165,10 @@ void batch_submit(Ctx& c, const vira_gpu_request* reqs, size_t n, const vira_gpu
         d.job.rr_start = r.opts.rr_start;
         d.job.tiles_x = g.tiles_x;
         d.job.out = OutPtrs{r.out.radiance_dev, r.out.direct_dev, r.out.indirect_dev, r.out.albedo_dev,
-                            r.out.normal_dev, r.out.depth_dev, r.out.alpha_dev, r.out.tri_id_dev};
+                            r.out.normal_dev, r.out.depth_dev, r.out.alpha_dev, r.out.tri_id_dev,
+                            r.out.instance_id_dev};
+        d.job.inst = nullptr;
+        d.job.ninst = 0;
         d.first = (uint32_t)items;
         d.tile_stride = (uint32_t)stride;
         const uint64_t nt = (g.tiles + (uint32_t)stride - 1) / (uint32_t)stride;

@@ -1,10 +1,12 @@
 // capi.hip -- the extern "C" entry points of include/vira_gpu.h.
 // Every entry point catches library errors and returns a status; the message
 // is kept per thread for vira_gpu_last_error().
+#include <cmath>
 #include <cstring>
 #include <string>
 
 #include "context.hpp"
+#include "instance.hpp"
 
 namespace vg {
 void mesh_upload_pack(Ctx& c, MeshG& m, const float* pos, const float* alb, const float* uv, const float* nrm);
@@ -259,12 +261,26 @@ vira_gpu_status vira_gpu_tlas_build(vira_gpu_ctx* ctx, const vira_gpu_instance* 
         vg::Ctx& c = C(ctx);
         VG_CHECK(inst || n == 0, VIRA_GPU_ERR_INVALID, "tlas_build: null instances");
         VG_CHECK(n != 0, VIRA_GPU_ERR_INVALID, "tlas_build: empty TLAS");  // vira_tlas.ipp:33
-        VG_CHECK(n == 1, VIRA_GPU_ERR_UNSUPPORTED, "tlas_build: one instance per scene in this build");
+        VG_CHECK(n <= (size_t)vg::kMaxInstances, VIRA_GPU_ERR_UNSUPPORTED, "tlas_build: at most 16 instances");
         static const double I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
-        VG_CHECK(std::memcmp(inst[0].transform, I, sizeof(I)) == 0, VIRA_GPU_ERR_UNSUPPORTED,
-                 "tlas_build: non-identity instance transforms are not supported in this build");
-        vg::MeshG& m = c.mesh(inst[0].mesh_id);
-        VG_CHECK(m.blas.built, VIRA_GPU_ERR_STATE, "tlas_build: BLAS not built");
+        int width = 0;
+        for (size_t i = 0; i < n; ++i) {
+            vg::MeshG& m = c.mesh(inst[i].mesh_id);
+            VG_CHECK(m.blas.built, VIRA_GPU_ERR_STATE, "tlas_build: BLAS not built");
+            VG_CHECK(width == 0 || m.blas.width == width, VIRA_GPU_ERR_INVALID,
+                     "tlas_build: all instances need BLASes of one BVH width");
+            width = m.blas.width;
+            const double* M = inst[i].transform;
+            VG_CHECK(M[3] == 0.0 && M[7] == 0.0 && M[11] == 0.0 && M[15] == 1.0, VIRA_GPU_ERR_INVALID,
+                     "tlas_build: model matrices must be affine (last row 0 0 0 1)");
+            for (int k = 0; k < 16; ++k)
+                VG_CHECK(std::isfinite(M[k]), VIRA_GPU_ERR_INVALID, "tlas_build: non-finite model matrix");
+            const double det = M[0] * (M[5] * M[10] - M[9] * M[6]) - M[4] * (M[1] * M[10] - M[9] * M[2]) +
+                               M[8] * (M[1] * M[6] - M[5] * M[2]);
+            VG_CHECK(det != 0.0 && std::isfinite(det), VIRA_GPU_ERR_INVALID, "tlas_build: singular model matrix");
+        }
+        c.tlas.assign(inst, inst + n);
+        c.instanced = !(n == 1 && std::memcmp(inst[0].transform, I, sizeof(I)) == 0);
         c.scene_mesh = inst[0].mesh_id;
         c.scene_gen++;
     });

@@ -96,11 +96,13 @@ struct Ctx {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     std::vector<std::unique_ptr<MeshG>> meshes;
-    int32_t scene_mesh = -1;  // the one instance of the current TLAS
+    int32_t scene_mesh = -1;  // the (first) instance's mesh of the current TLAS
+    std::vector<vira_gpu_instance> tlas;
+    bool instanced = false;   // TLAS other than one identity instance: rendered camera-relative
     std::vector<vira_gpu_light> lights;
     // render scratch, grown on demand
     DBuf r_ray_o, r_ray_d, r_hit, r_queue[2], r_beta, r_path_rad, r_path_dir, r_srays_o, r_srays_d,
-        r_scontrib, r_svis, r_sinfo, r_acc, r_counters, r_lights;
+        r_scontrib, r_svis, r_sinfo, r_acc, r_counters, r_lights, r_inst, r_inst_m, r_light_pos;
     hipEvent_t ev[8] = {};
     // bumped by every call that changes geometry, BVHs, lights or materials
     uint64_t scene_gen = 1;

@@ -11,6 +11,7 @@
 // registers; the per-pixel sums live in a tile-blocked accumulator block.
 #pragma once
 
+#include "instance.hpp"
 #include "shading.hpp"
 #include "traverse.hpp"
 
@@ -70,7 +71,11 @@ __device__ __forceinline__ void accumulate_first_hit(float* __restrict__ a, uint
 struct OutPtrs {
     float *radiance, *direct, *indirect, *albedo, *normal, *depth, *alpha;
     uint32_t* tri;
+    uint32_t* inst;
 };
+
+// The first-hit id field holds instance << 27 | triangle (triangles < 2^27).
+constexpr uint32_t kTriBits = 27;
 
 // Averages and passes (cpu_path_tracer.ipp:229-239, passes.ipp:81-127).
 __device__ __forceinline__ void write_pixel(const OutPtrs& o, size_t p, int spp, const float* __restrict__ a,
@@ -95,7 +100,9 @@ __device__ __forceinline__ void write_pixel(const OutPtrs& o, size_t p, int spp,
     }
     if (o.depth) o.depth[p] = cnt > 0.0f ? __fdiv_rn(ACC(a, A_DEPTH, lane), cnt) : kInf;
     if (o.alpha) o.alpha[p] = __fdiv_rn(cnt, sf);
-    if (o.tri) o.tri[p] = __float_as_uint(ACC(a, A_TRI, lane));
+    const uint32_t first = __float_as_uint(ACC(a, A_TRI, lane));
+    if (o.tri) o.tri[p] = first == 0xffffffffu ? first : (first & ((1u << kTriBits) - 1u));
+    if (o.inst) o.inst[p] = first == 0xffffffffu ? first : (first >> kTriBits);
 }
 
 // Kahan-compensated add of v[] (or zeros) into the (sum, comp) fields fs, fc.
@@ -130,12 +137,14 @@ struct TileJob {
     int spp, max_bounces, rr, rr_start;
     uint32_t tiles_x;
     OutPtrs out;
+    const InstG* inst;  // instanced scenes: the camera-relative instance table
+    int ninst;
 };
 
 // One wave renders the 8x8 tile `tile` of job J: lane = pixel; accumulators at
 // a (this wave's A_FIELDS x 64 block); LDS slices of this wave.  Adds the
 // wave's ray counts to n[0..2] (per lane).
-template <int W>
+template <int W, bool INST = false>
 __device__ __forceinline__ void render_tile(const TileJob& J, uint32_t tile, float* __restrict__ a,
                                             uint32_t* s_ref, float* s_dist, float* s_contrib, uint32_t lane,
                                             uint32_t& n_prim, uint32_t& n_shadow, uint32_t& n_bounce) {
@@ -187,8 +196,14 @@ __device__ __forceinline__ void render_tile(const TileJob& J, uint32_t tile, flo
         h.id = 0xffffffffu;
         h.t = kInf;
         h.b1 = h.b2 = 0.f;
-        if (mode != M_DONE)
-            h = traverse<W>(nodes, tris, o, d, tmax, mode == M_SHADOW, s_ref + lane, s_dist + lane, kFusedBS);
+        uint32_t hinst = 0;
+        if (mode != M_DONE) {
+            if (INST)
+                h = trace_instances<W>(J.inst, J.ninst, o, d, tmax, mode == M_SHADOW, s_ref + lane, s_dist + lane,
+                                       kFusedBS, hinst);
+            else
+                h = traverse<W>(nodes, tris, o, d, tmax, mode == M_SHADOW, s_ref + lane, s_dist + lane, kFusedBS);
+        }
         bool vertex_done = false, lit = false;
         if (mode == M_CLOSEST) {
             if (h.id == 0xffffffffu) {
@@ -198,8 +213,9 @@ __device__ __forceinline__ void render_tile(const TileJob& J, uint32_t tile, flo
                 s++;
                 mode = M_NEW;
             } else {
-                Surface S = surface_at(verts, idx, mat, h.id, h.b1, h.b2);
-                if (b == 0) accumulate_first_hit(a, lane, h.id, h.t, S);
+                Surface S = INST ? surface_instance(J.inst[hinst], h.id, h.b1, h.b2)
+                                 : surface_at(verts, idx, mat, h.id, h.b1, h.b2);
+                if (b == 0) accumulate_first_hit(a, lane, h.id | (hinst << kTriBits), h.t, S);
                 f3 ld = mk3(0.f, 0.f, 1.f);
                 float dist = 0.f, c[NB];
                 bool sh = one_light &&

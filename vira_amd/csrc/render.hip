@@ -66,7 +66,7 @@ __global__ void __launch_bounds__(kTraceBS) k_trace_rays(const void* __restrict_
 
 // ============================================================ fused schedule
 // Per-request launch: block b renders tile b.
-template <int W>
+template <int W, bool INST>
 __global__ void __launch_bounds__(kFusedBS) k_render_fused(TileJob J, float* __restrict__ acc,
                                                          unsigned long long* __restrict__ counters) {
     __shared__ uint32_t s_ref[kStackLds * kFusedBS];
@@ -74,11 +74,24 @@ __global__ void __launch_bounds__(kFusedBS) k_render_fused(TileJob J, float* __r
     __shared__ float s_contrib[NB * kFusedBS];  // staged light contribution while the shadow ray is traced
     const uint32_t lane = threadIdx.x;
     uint32_t n_prim = 0, n_shadow = 0, n_bounce = 0;
-    render_tile<W>(J, blockIdx.x, acc + (size_t)blockIdx.x * (A_FIELDS * 64), s_ref, s_dist, s_contrib, lane, n_prim,
-                   n_shadow, n_bounce);
+    render_tile<W, INST>(J, blockIdx.x, acc + (size_t)blockIdx.x * (A_FIELDS * 64), s_ref, s_dist, s_contrib, lane,
+                         n_prim, n_shadow, n_bounce);
     wave_add(&counters[0], n_prim);
     wave_add(&counters[1], n_shadow);
     wave_add(&counters[2], n_bounce);
+}
+
+// Camera-relative FP32 placement of every instance and light, computed in
+// FP64 from the FP64 model matrices and positions (one thread per item).
+__global__ void k_rebase(const double* __restrict__ M, int n, const double* __restrict__ lpos, int nl, double cx,
+                         double cy, double cz, InstG* __restrict__ inst, LightF* __restrict__ lights) {
+    const int i = threadIdx.x;
+    if (i < n) rebase_instance(M + 16 * i, cx, cy, cz, inst[i]);
+    if (i < nl) {
+        lights[i].px = (float)(lpos[3 * i] - cx);
+        lights[i].py = (float)(lpos[3 * i + 1] - cy);
+        lights[i].pz = (float)(lpos[3 * i + 2] - cz);
+    }
 }
 
 // ============================================================ wavefront schedule
@@ -287,6 +300,13 @@ void render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, c
     VG_CHECK(c.scene_mesh >= 0, VIRA_GPU_ERR_STATE, "render: no TLAS built");
     MeshG& m = c.mesh(c.scene_mesh);
     VG_CHECK(m.blas.built, VIRA_GPU_ERR_STATE, "render: BLAS not built");
+    const bool inst = c.instanced;
+    const int ninst = inst ? (int)c.tlas.size() : 0;
+    for (int i = 0; i < ninst; ++i) {
+        const MeshG& mi = c.mesh(c.tlas[i].mesh_id);
+        VG_CHECK(mi.blas.built && mi.blas.width == m.blas.width, VIRA_GPU_ERR_STATE,
+                 "render: every instance needs a built BLAS of one width");
+    }
     VG_CHECK(cam.width > 0 && cam.height > 0, VIRA_GPU_ERR_INVALID, "render: empty image");
     VG_CHECK(o.samples > 0 && o.bounces >= 0 && o.bounces < 64, VIRA_GPU_ERR_INVALID, "render: bad samples/bounces");
     VG_CHECK(o.integrator >= 0 && o.integrator <= 2, VIRA_GPU_ERR_INVALID, "render: unknown integrator");
@@ -298,12 +318,42 @@ void render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, c
     const int nl = (int)c.lights.size();
     const bool fused = o.integrator == 1 || (o.integrator == 0 && nl <= 1);
     VG_CHECK(!(fused && nl > 1), VIRA_GPU_ERR_UNSUPPORTED, "render: the fused integrator takes at most one light");
+    VG_CHECK(!inst || fused, VIRA_GPU_ERR_UNSUPPORTED,
+             "render: instanced scenes use the per-pixel integrator (at most one light)");
 
-    const CamF cf = cam_params(cam);
+    CamF cf = cam_params(cam);
     const Lights lf = upload_lights(c, c.r_lights);
     const MatF mf = mat_params(m);
+    if (inst) {
+        // instance table (pointers, materials) and FP64 inputs; k_rebase fills the FP32 placement
+        std::vector<InstG> hi(ninst);
+        std::vector<double> hm(16 * (size_t)ninst), hl(3 * (size_t)std::max(1, nl));
+        for (int i = 0; i < ninst; ++i) {
+            const MeshG& mi = c.mesh(c.tlas[i].mesh_id);
+            hi[i] = InstG{};
+            hi[i].nodes = mi.blas.nodes.p;
+            hi[i].tris = mi.blas.tris.as<float4>();
+            hi[i].verts = mi.verts.as<GVertex>();
+            hi[i].idx = mi.idx.as<uint32_t>();
+            hi[i].mat = mat_params(mi);
+            for (int k = 0; k < 16; ++k) hm[16 * i + k] = c.tlas[i].transform[k];
+        }
+        for (int l = 0; l < nl; ++l)
+            for (int k = 0; k < 3; ++k) hl[3 * l + k] = c.lights[l].position[k];
+        c.r_inst.alloc(sizeof(InstG) * ninst);
+        c.r_inst_m.alloc(hm.size() * 8);
+        c.r_light_pos.alloc(hl.size() * 8);
+        VG_HIP(hipMemcpyAsync(c.r_inst.p, hi.data(), sizeof(InstG) * ninst, hipMemcpyHostToDevice, c.stream));
+        VG_HIP(hipMemcpyAsync(c.r_inst_m.p, hm.data(), hm.size() * 8, hipMemcpyHostToDevice, c.stream));
+        VG_HIP(hipMemcpyAsync(c.r_light_pos.p, hl.data(), hl.size() * 8, hipMemcpyHostToDevice, c.stream));
+        k_rebase<<<1, 64, 0, c.stream>>>(c.r_inst_m.as<double>(), ninst, c.r_light_pos.as<double>(), nl,
+                                         cam.position[0], cam.position[1], cam.position[2], c.r_inst.as<InstG>(),
+                                         c.r_lights.as<LightF>());
+        VG_HIP(hipGetLastError());
+        cf.ox = cf.oy = cf.oz = 0.0f;  // the camera is the origin of the render frame
+    }
     OutPtrs op{out.radiance_dev, out.direct_dev, out.indirect_dev, out.albedo_dev, out.normal_dev,
-               out.depth_dev, out.alpha_dev, out.tri_id_dev};
+               out.depth_dev, out.alpha_dev, out.tri_id_dev, out.instance_id_dev};
 
     c.r_acc.alloc((size_t)A_FIELDS * nq * 4);
     c.r_counters.alloc(64 * 8);
@@ -334,8 +384,10 @@ void render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, c
     if (fused) {
         ev_begin(3);
         const TileJob J{nodes, tris, m.verts.as<GVertex>(), m.idx.as<uint32_t>(), mf, lf, cf, o.seed, o.samples,
-                        o.bounces, o.russian_roulette, o.rr_start, tiles_x, op};
-        auto kern = w4 ? k_render_fused<4> : k_render_fused<2>;
+                        o.bounces, o.russian_roulette, o.rr_start, tiles_x, op,
+                        inst ? c.r_inst.as<InstG>() : nullptr, ninst};
+        auto kern = inst ? (w4 ? k_render_fused<4, true> : k_render_fused<2, true>)
+                         : (w4 ? k_render_fused<4, false> : k_render_fused<2, false>);
         kern<<<tiles, kFusedBS, 0, c.stream>>>(J, acc, cnt);
         VG_HIP(hipGetLastError());
         ev_end();

@@ -180,24 +180,26 @@ struct Surface {
     float alb[NB];  // vertex albedo x material albedo x texture
 };
 
-// Interpolation, frame, albedo, shading point and offset for a hit
-// (cpu_path_tracer.ipp:284-349,386).
-__device__ __forceinline__ Surface surface_at(const GVertex* __restrict__ verts, const uint32_t* __restrict__ idx,
-                                              const MatF& mat, uint32_t id, float b1, float b2) {
-    Surface S;
+// Mesh-local part of a hit (cpu_path_tracer.ipp:284-349): shading normal,
+// face normal, albedo and the shading point, all in the mesh's frame.
+struct LocalHit {
+    f3 P;    // computeShadingPoint of the barycentric point
+    f3 Nl;   // shading normal (face normal, or interpolated vertex normals when smooth)
+    f3 fn;   // unit face normal
+    float alb[NB];
+};
+
+__device__ __forceinline__ void local_hit(const GVertex* __restrict__ verts, const uint32_t* __restrict__ idx,
+                                          const MatF& mat, uint32_t id, float b1, float b2, LocalHit& L) {
     const uint32_t i0 = idx[3 * (size_t)id], i1 = idx[3 * (size_t)id + 1], i2 = idx[3 * (size_t)id + 2];
     const GVertex V0 = verts[i0], V1 = verts[i1], V2 = verts[i2];
     const f3 p0 = mk3(V0.px, V0.py, V0.pz), p1 = mk3(V1.px, V1.py, V1.pz), p2 = mk3(V2.px, V2.py, V2.pz);
     // Embree barycentrics: w = (u, v, 1-u-v) paired with (v1, v2, v0) (embree_tlas.ipp:160-166)
     const float w0 = b1, w1 = b2, w2 = (1.0f - b1) - b2;
-    const f3 fn = normalize(cross(sub(p1, p0), sub(p2, p0)));  // == -normalize(cross(v0-v1, v2-v0))
+    L.fn = normalize(cross(sub(p1, p0), sub(p2, p0)));  // == -normalize(cross(v0-v1, v2-v0))
     const f3 n0 = mk3(V0.nx, V0.ny, V0.nz), n1 = mk3(V1.nx, V1.ny, V1.nz), n2 = mk3(V2.nx, V2.ny, V2.nz);
-    f3 Nl = fn;
-    if (mat.smooth) Nl = normalize(add(add(mul(w0, n1), mul(w1, n2)), mul(w2, n0)));
-    S.N = Nl;
-    const f3 arb = (fabsf(Nl.x) < 0.9f) ? mk3(1.f, 0.f, 0.f) : mk3(0.f, 1.f, 0.f);
-    S.tg = normalize(cross(arb, Nl));
-    S.bt = cross(Nl, S.tg);
+    L.Nl = L.fn;
+    if (mat.smooth) L.Nl = normalize(add(add(mul(w0, n1), mul(w1, n2)), mul(w2, n0)));
     float texv = 1.0f;
     if (mat.tex) {
         const float uvu = (w0 * V1.u + w1 * V2.u) + w2 * V0.u;
@@ -208,7 +210,7 @@ __device__ __forceinline__ Surface surface_at(const GVertex* __restrict__ verts,
     for (int k = 0; k < NB; ++k) {
         float av = (w0 * V1.alb[k] + w1 * V2.alb[k]) + w2 * V0.alb[k];
         float mv = mat.tex ? texv * mat.albedo[k] : mat.albedo[k];
-        S.alb[k] = av * mv;
+        L.alb[k] = av * mv;
     }
     f3 P = add(add(mul(w0, p1), mul(w1, p2)), mul(w2, p0));
     {  // computeShadingPoint (cpu_path_tracer.ipp:481-511)
@@ -217,10 +219,32 @@ __device__ __forceinline__ Surface surface_at(const GVertex* __restrict__ verts,
         f3 q2 = sub(P, mul(dot(sub(P, p0), n0), n0));
         f3 sp = add(add(mul(q0, w0), mul(q1, w1)), mul(q2, w2));
         f3 diff = sub(sp, P);
-        if (dot(diff, Nl) > 0.0f) P = sp;
+        if (dot(diff, L.Nl) > 0.0f) P = sp;
     }
-    S.org = offset_point(P, fn);
+    L.P = P;
+}
+
+// Tangent frame, albedo and offset origin from global-frame quantities
+// (cpu_path_tracer.ipp:307-318,386): N is the global shading normal, fng the
+// global face normal the offset follows, P the global shading point.
+__device__ __forceinline__ Surface finish_surface(f3 P, f3 N, f3 fng, const float alb[NB]) {
+    Surface S;
+    S.N = N;
+    const f3 arb = (fabsf(N.x) < 0.9f) ? mk3(1.f, 0.f, 0.f) : mk3(0.f, 1.f, 0.f);
+    S.tg = normalize(cross(arb, N));
+    S.bt = cross(N, S.tg);
+#pragma unroll
+    for (int k = 0; k < NB; ++k) S.alb[k] = alb[k];
+    S.org = offset_point(P, fng);
     return S;
+}
+
+// A hit on a mesh whose frame is the render frame (identity instance).
+__device__ __forceinline__ Surface surface_at(const GVertex* __restrict__ verts, const uint32_t* __restrict__ idx,
+                                              const MatF& mat, uint32_t id, float b1, float b2) {
+    LocalHit L;
+    local_hit(verts, idx, mat, id, b1, b2, L);
+    return finish_surface(L.P, L.Nl, L.fn, L.alb);
 }
 
 // One light's direct-lighting term (cpu_path_tracer.ipp:399-428): returns

@@ -246,7 +246,7 @@ class RenderPasses:
     """RenderPasses (passes.hpp:51-95) as device tensors, pixel index i + j*W.
     `which` limits the passes allocated (the others are not written)."""
 
-    ALL = ("radiance", "direct", "indirect", "albedo", "normal", "depth", "alpha", "tri_id")
+    ALL = ("radiance", "direct", "indirect", "albedo", "normal", "depth", "alpha", "tri_id", "instance_id")
 
     def __init__(self, width: int, height: int, device="cuda", which=ALL):
         import torch
@@ -263,6 +263,8 @@ class RenderPasses:
         self.alpha = torch.zeros((height, width), dtype=torch.float32, device=device) if "alpha" in w else None
         self.triangle_id = (torch.zeros((height, width), dtype=torch.int32, device=device)
                             if "tri_id" in w else None)
+        self.instance_id = (torch.zeros((height, width), dtype=torch.int32, device=device)
+                            if "instance_id" in w else None)
 
     def to_c(self) -> capi.Outputs:
         p = lambda t: None if t is None else t.data_ptr()
@@ -275,6 +277,7 @@ class RenderPasses:
         o.depth_dev = p(self.depth)
         o.alpha_dev = p(self.alpha)
         o.tri_id_dev = p(self.triangle_id)
+        o.instance_id_dev = p(self.instance_id)
         return o
 
 
@@ -366,12 +369,16 @@ class Context:
         self._call(self.lib.vira_gpu_blas_stats, self.h, mid, C.byref(s))
         return {f: getattr(s, f) for f, _ in capi.BVHStats._fields_}
 
-    def tlas_build(self, mesh_ids):
+    def tlas_build(self, mesh_ids, transforms=None):
+        """Instances of built BLASes; transforms: FP64 4x4 model matrices (row-major numpy
+        arrays, identity if None)."""
         arr = (capi.Instance * len(mesh_ids))()
         for i, m in enumerate(mesh_ids):
             arr[i].mesh_id = m
-            for k in range(16):
-                arr[i].transform[k] = 1.0 if k % 5 == 0 else 0.0
+            M = np.eye(4) if transforms is None or transforms[i] is None else np.asarray(transforms[i], np.float64)
+            for c in range(4):
+                for r in range(4):
+                    arr[i].transform[c * 4 + r] = float(M[r, c])  # column-major
         self._call(self.lib.vira_gpu_tlas_build, self.h, arr, len(mesh_ids))
 
     def lights_set(self, lights):

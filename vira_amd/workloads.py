@@ -364,3 +364,136 @@ def render_plain(ctx: S.Context, reqs) -> None:
         ctx.tlas_build([r.mesh])
         ctx.lights_set([] if r.light is None else [r.light])
         ctx.render(r.camera, r.options, r.passes)
+
+
+# ---------------------------------------------------------------- config 4: instanced multi-body scene
+AU = S.AU
+
+
+def box_mesh(size, center=(0.0, 0.0, 0.0), n: int = 8):
+    """Axis-aligned box (size = full extents) with each face split into an n x n
+    grid of quads (12 n^2 triangles), outward winding, per-face vertices."""
+    sx, sy, sz = (0.5 * float(v) for v in size)
+    c = np.asarray(center, np.float64)
+    g = np.linspace(-1.0, 1.0, n + 1)
+    U, V = np.meshgrid(g, g, indexing="ij")
+    pos, idx = [], []
+    faces = [  # (axis, sign)
+        (0, 1.0), (0, -1.0), (1, 1.0), (1, -1.0), (2, 1.0), (2, -1.0)]
+    half = np.array([sx, sy, sz])
+    for axis, sign in faces:
+        a1, a2 = [k for k in range(3) if k != axis]
+        P = np.zeros(((n + 1) ** 2, 3))
+        P[:, axis] = sign * half[axis]
+        P[:, a1] = U.reshape(-1) * half[a1]
+        P[:, a2] = V.reshape(-1) * half[a2]
+        base = sum(len(p) for p in pos)
+        q = np.arange(n)
+        i, j = np.meshgrid(q, q, indexing="ij")
+        v00 = (i * (n + 1) + j).reshape(-1)
+        v10, v01, v11 = v00 + (n + 1), v00 + 1, v00 + (n + 1) + 1
+        t = np.concatenate([np.stack([v00, v10, v11], 1), np.stack([v00, v11, v01], 1)]) + base
+        # outward: cross(e1, e2) along +axis*sign
+        p0, p1, p2 = P[t[0, 0] - base], P[t[0, 1] - base], P[t[0, 2] - base]
+        if np.dot(np.cross(p1 - p0, p2 - p0), np.eye(3)[axis] * sign) < 0:
+            t = t[:, [0, 2, 1]]
+        pos.append(P + c)
+        idx.append(t)
+    return np.concatenate(pos).astype(np.float32), np.concatenate(idx).astype(np.uint32)
+
+
+def spacecraft_mesh():
+    """Box-and-panel spacecraft (50,496 triangles): a 3 x 2 x 2 m bus, two
+    6 x 2 m solar panels and a 3 m boom along -x (local frame: bus centre)."""
+    parts = [box_mesh((3.0, 2.0, 2.0), (0.0, 0.0, 0.0), 40),
+             box_mesh((0.05, 6.0, 2.0), (0.0, 4.0, 0.0), 36),
+             box_mesh((0.05, 6.0, 2.0), (0.0, -4.0, 0.0), 36),
+             box_mesh((3.0, 0.1, 0.1), (-3.0, 0.6, 0.5), 4)]
+    pos, idx, base = [], [], 0
+    for p, t in parts:
+        pos.append(p)
+        idx.append(t + base)
+        base += len(p)
+    return np.concatenate(pos), np.concatenate(idx)
+
+
+def _rot_z(deg):
+    a = np.radians(deg)
+    R = np.eye(4)
+    R[0, 0], R[0, 1], R[1, 0], R[1, 1] = np.cos(a), -np.sin(a), np.sin(a), np.cos(a)
+    return R
+
+
+def _rot_axis(axis, deg):
+    k = np.asarray(axis, np.float64) / np.linalg.norm(axis)
+    a = np.radians(deg)
+    K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+    R = np.eye(4)
+    R[:3, :3] = np.eye(3) + np.sin(a) * K + (1 - np.cos(a)) * (K @ K)
+    return R
+
+
+def _trans(t):
+    T = np.eye(4)
+    T[:3, 3] = t
+    return T
+
+
+@dataclass
+class InstancedScene:
+    meshes: list        # mesh ids: DEM tile, sphere, small body, spacecraft
+    transforms: list    # FP64 4x4 model matrices (row-major)
+    camera: S.Camera
+    lights: list
+    options: S.PathTracerOptions
+    keep: list
+    local: dict         # host copies of the generic meshes (for the oracle)
+
+
+def build_instanced(ctx: S.Context, dem_built: "BuiltScene | None" = None, dem: int = 4097, image: int = 4096,
+                    spp: int = 8, bounces: int = 2, sphere=(161, 200), body=(251, 400), sun: str = "sphere",
+                    bvh_width: int = 4) -> InstancedScene:
+    """Config 4: the Moon (a 4097^2 DEM tile + a 64k-triangle sphere for the
+    rest) at 1 AU from the Sun, a 200k-triangle small body 30 km from the
+    camera and a 50,496-triangle spacecraft carrying the camera, every body
+    placed by an FP64 model matrix at ~1.5e8 km from the origin; camera 100 km
+    above the tile (scaled with the DEM), 4096^2, 8 spp, 2 bounces."""
+    sc = CONFIG1 if dem == CONFIG1.dem else scaled(CONFIG1, dem=dem, texture=min(CONFIG1.texture, 4 * (dem - 1)))
+    b = dem_built if dem_built is not None else build(ctx, sc, bvh_width=bvh_width)
+    R = sc.sphere_radius
+    alt = sc.altitude
+    P_moon = np.array([AU, 0.0, 0.0])
+    Rz = _rot_z(120.0)                                   # tile centre direction u = Rz (1, 0, 0)
+    u = Rz[:3, :3] @ np.array([1.0, 0.0, 0.0])
+    v = Rz[:3, :3] @ np.array([0.0, 1.0, 0.0])
+    P_cam = P_moon + (R + alt) * u                       # the DEM mesh's local origin
+    M_tile = _trans(P_cam) @ Rz
+    sp_pos, sp_idx = small_body_mesh(seed=0, axes=(R - 1000.0,) * 3, nlat=sphere[0], nlon=sphere[1], relief=0.0)
+    sphere_id = ctx.mesh_upload(sp_pos, sp_idx)
+    ctx.blas_build(sphere_id, width=bvh_width)
+    ctx.material_set(sphere_id, albedo=0.11)
+    M_sphere = _trans(P_moon)
+    sb_pos, sb_idx = small_body_mesh(nlat=body[0], nlon=body[1])
+    sb_id = ctx.mesh_upload(sb_pos, sb_idx)
+    ctx.blas_build(sb_id, width=bvh_width)
+    ctx.material_set(sb_id, albedo=0.12)
+    d_sb = 0.3 * alt
+    M_sb = _trans(P_cam - d_sb * u + 0.04 * d_sb / 0.3 * v) @ _rot_axis((1.0, 1.0, 1.0), 40.0)
+    sc_pos, sc_idx = spacecraft_mesh()
+    sc_id = ctx.mesh_upload(sc_pos, sc_idx)
+    ctx.blas_build(sc_id, width=bvh_width)
+    ctx.material_set(sc_id, albedo=0.5)
+    M_sc = _trans(P_cam + 2.0 * u) @ Rz
+    meshes = [b.mesh, sphere_id, sb_id, sc_id]
+    transforms = [M_tile, M_sphere, M_sb, M_sc]
+    ctx.tlas_build(meshes, transforms)
+    cam = S.Camera(image, image, 40.0, tuple(P_cam))
+    cam.look_in_direction(tuple(-u), up=(0.0, 0.0, 1.0))
+    if sun == "sphere":
+        lights = [S.new_sun((0.0, 0.0, 0.0))]
+    else:  # a point sun at the origin giving 1 W/m^2 per band at 1 AU
+        lights = [S.PointLight((0.0, 0.0, 0.0), np.full(8, AU * AU, np.float32))]
+    ctx.lights_set(lights)
+    opts = S.PathTracerOptions(samples=spp, bounces=bounces, seed=4)
+    local = {"sphere": (sp_pos, sp_idx), "body": (sb_pos, sb_idx), "craft": (sc_pos, sc_idx)}
+    return InstancedScene(meshes, transforms, cam, lights, opts, [b.heights, b.texture], local)
