@@ -200,6 +200,12 @@ typedef struct {
     int32_t graph_reused;     /* 1 = replayed the captured hipGraph of an identical batch layout */
 } vira_gpu_batch_stats;
 
+/* Sensor radiometry of Camera::calculateReceivedPower (camera.ipp:1175-1178). */
+typedef struct {
+    float optical_efficiency[VIRA_GPU_NUM_BINS]; /* eta per band (default 1) */
+    float aperture_area;                         /* pi (f/N / 2)^2, m^2 (aperture.ipp:14-16, camera.ipp:82) */
+} vira_gpu_sensor;
+
 /* --- context ------------------------------------------------------------ */
 vira_gpu_status vira_gpu_create(vira_gpu_ctx** ctx, const vira_gpu_opts* opts);
 vira_gpu_status vira_gpu_destroy(vira_gpu_ctx* ctx);
@@ -304,6 +310,14 @@ vira_gpu_status vira_gpu_di_render(vira_gpu_ctx* ctx, const vira_gpu_camera* cam
 vira_gpu_status vira_gpu_adjoint(vira_gpu_ctx* ctx, const vira_gpu_camera* cam, const vira_gpu_di_opts* opts,
                                  const float* dL_dimage_dev, float* dL_dheights_dev, float* dL_dtexture_dev,
                                  vira_gpu_di_stats* stats);
+
+/* Received power P = eta * A * Omega_ij * L per pixel and band
+ * (cpu_path_tracer.ipp:137-146 -> camera.ipp:1175-1178), Omega_ij the pixel's
+ * solid angle (camera.ipp:1594-1666, FP64 on the device, stored FP32).
+ * radiance_dev / power_dev: W*H*8; solid_angle_dev: W*H; power or solid angle
+ * output may be NULL. */
+vira_gpu_status vira_gpu_received_power(vira_gpu_ctx* ctx, const vira_gpu_camera* cam, const vira_gpu_sensor* sensor,
+                                        const float* radiance_dev, float* power_dev, float* solid_angle_dev);
 
 /* --- batched rendering (new) --------------------------------------------- */
 /* Renders n requests (CPUPathTracer::render semantics per request,

@@ -261,6 +261,10 @@ void orc_render_scene(void* const* meshes, const double* transforms, const Mater
     render_scene(precision, inst.data(), n, *cam, lights, nl, *opt, out, nthreads, pixel_list, npix);
 }
 
+void orc_pixel_solid_angles(const float* kinv, float zdir, int W, int H, float* out) {
+    pixel_solid_angles(kinv, zdir, W, H, out);
+}
+
 // ---- fixed-visibility direct illumination, FP64 forward / adjoint / finite differences
 void* orc_dem_diff_create(int W, int H, const Projection* proj, const float* heights, const float* grey) {
     return new DemDiff(dem_diff_make(W, H, *proj, heights, grey));

@@ -616,6 +616,29 @@ void render_impl(const InstanceO* inst, int ninst, const Camera& cam_in, const L
 
 }  // namespace
 
+void pixel_solid_angles(const float kinv[6], float zdir, int W, int H, float* out) {
+    using D = V3<double>;
+    const double k[6] = {kinv[0], kinv[1], kinv[2], kinv[3], kinv[4], kinv[5]};
+    const double zd = zdir;
+    auto corner = [&](double x, double y) {
+        const double hx = (k[0] * x + k[2] * y) + k[4];
+        const double hy = (k[1] * x + k[3] * y) + k[5];
+        return normalize(D{hx, zd * hy, zd});
+    };
+    auto tangent = [](const D& p0, const D& p1) { return normalize(cross(cross(p0, p1 - p0), p0)); };
+    auto tri = [&](const D& c0, const D& c1, const D& c2) {
+        const double a0 = std::acos(dot(tangent(c0, c1), tangent(c0, c2)));
+        const double a1 = std::acos(dot(tangent(c1, c0), tangent(c1, c2)));
+        const double a2 = std::acos(dot(tangent(c2, c0), tangent(c2, c1)));
+        return ((a0 + a1) + a2) - 3.14159265358979323846;
+    };
+    for (int j = 0; j < H; ++j)
+        for (int i = 0; i < W; ++i) {
+            const D c0 = corner(i, j), c1 = corner(i + 1, j), c2 = corner(i + 1, j + 1), c3 = corner(i, j + 1);
+            out[(size_t)j * W + i] = (float)(tri(c0, c1, c2) + tri(c0, c2, c3));
+        }
+}
+
 static InstanceO identity_instance(const Mesh& m, const BVH& b, const Material& mat) {
     InstanceO I{&m, &b, mat, {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1}};
     return I;

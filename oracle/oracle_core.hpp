@@ -554,6 +554,11 @@ void render_f32(const Mesh& m, const BVH& b, const Camera& cam, const Light* lig
 void render_f64(const Mesh& m, const BVH& b, const Camera& cam, const Light* lights, int nl,
                 const Material& mat, const Options& opt, Outputs& out, int nthreads,
                 const int* pixel_list, size_t npix);
+// Pixel solid angles of Camera::initializePixelSolidAngle (camera.ipp:1594-1666):
+// FP64 corner directions K^-1 (x, y, 1) (the FP32 K^-1 widened), two spherical
+// triangles by Girard's theorem (camera.ipp:1578-1592), stored FP32.
+void pixel_solid_angles(const float kinv[6], float zdir, int W, int H, float* out);
+
 // Two-level scene.  One instance with the identity matrix renders in the
 // caller's frame; anything else renders camera-relative: model matrices and
 // light positions minus the FP64 camera position, rounded to the working

@@ -81,6 +81,7 @@ def lib() -> C.CDLL:
                               C.POINTER(Options), P, P, P, P, P, P, P, P, P, P, C.c_int, P, C.c_size_t]),
         "orc_render_scene": (None, [P, P, P, C.c_int, C.c_int, C.POINTER(Camera), C.POINTER(Light), C.c_int,
                                     C.POINTER(Options), P, P, P, P, P, P, P, P, P, P, C.c_int, P, C.c_size_t]),
+        "orc_pixel_solid_angles": (None, [P, C.c_float, C.c_int, C.c_int, P]),
         "orc_dem_diff_create": (P, [C.c_int, C.c_int, C.POINTER(Projection), P, P]),
         "orc_dem_diff_destroy": (None, [P]),
         "orc_di_forward64": (None, [P, P, P, P, C.c_size_t, C.c_int, C.POINTER(Light), C.c_int,
@@ -309,6 +310,14 @@ def dem_mesh(heights: np.ndarray, proj: Projection, grey_albedo: np.ndarray | No
     a = np.ones(H * W, np.float32) if grey_albedo is None else np.asarray(grey_albedo, np.float32)
     alb = np.repeat(a[:, None], NB, axis=1)
     return Mesh(pos, idx, nrm, alb, dem_uv(W, H), pos64 if with_pos64 else None)
+
+
+def pixel_solid_angles(cam: Camera) -> np.ndarray:
+    """(H, W) FP32 solid angle of every pixel (camera.ipp:1594-1666)."""
+    k = np.array([cam.kinv[i] for i in range(6)], np.float32)
+    out = np.empty((cam.H, cam.W), np.float32)
+    lib().orc_pixel_solid_angles(k.ctypes.data, cam.zdir, cam.W, cam.H, out.ctypes.data)
+    return out
 
 
 def render_scene(meshes, transforms, mats, cam: Camera, lights, opt: Options, precision=32, pixels=None,

@@ -384,3 +384,28 @@ def test_rng_uniform_range_and_independence():
     u = np.array([O.lib().orc_rng_u01(0, p, s, d) for p in range(40) for s in range(5) for d in range(5)])
     assert u.min() >= 0 and u.max() < 1 and abs(u.mean() - 0.5) < 0.05
     assert len(np.unique(u)) == len(u)
+
+
+def _rect_solid_angle(x0, x1, y0, y1):
+    """Solid angle of the rectangle [x0,x1] x [y0,y1] on the plane z = 1 seen from
+    the origin: F(x1,y1) - F(x0,y1) - F(x1,y0) + F(x0,y0), F = atan(xy / sqrt(1+x^2+y^2))."""
+    F = lambda x, y: np.arctan(x * y / np.sqrt(1.0 + x * x + y * y))
+    return F(x1, y1) - F(x0, y1) - F(x1, y0) + F(x0, y0)
+
+
+@pytest.mark.parametrize("W,H,fov", [(64, 48, 40.0), (31, 31, 90.0)])
+def test_pixel_solid_angle_against_closed_form(W, H, fov):
+    cam = S.Camera(W, H, fov, (0.0, 0.0, 0.0))
+    c = O.camera_from(cam.to_c())
+    om = O.pixel_solid_angles(c).astype(np.float64)
+    k = [np.float64(c.kinv[i]) for i in range(6)]
+    i = np.arange(W + 1, dtype=np.float64)
+    j = np.arange(H + 1, dtype=np.float64)
+    xs = k[0] * i + k[4]                 # K^-1 has no skew: x depends on the column only
+    ys = k[3] * j + k[5]
+    ref = _rect_solid_angle(xs[None, :-1], xs[None, 1:], ys[:-1, None], ys[1:, None])
+    ref = np.abs(ref)
+    assert np.max(np.abs(om - ref) / ref) < 2e-6
+    # the pixels tile the sensor: their sum is the solid angle of the whole sensor
+    whole = abs(_rect_solid_angle(xs[0], xs[-1], ys[0], ys[-1]))
+    assert abs(om.sum() - whole) / whole < 1e-5

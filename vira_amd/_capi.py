@@ -179,6 +179,10 @@ class BatchStats(C.Structure):
     ]
 
 
+class Sensor(C.Structure):
+    _fields_ = [("optical_efficiency", C.c_float * NUM_BINS), ("aperture_area", C.c_float)]
+
+
 class Instance(C.Structure):
     _fields_ = [("mesh_id", C.c_int32), ("transform", C.c_double * 16)]
 
@@ -210,6 +214,7 @@ SIGNATURES = [
     ("vira_gpu_dem_update", _S, [_P, C.c_int32, _P]),
     ("vira_gpu_di_render", _S, [_P, C.POINTER(Camera), C.POINTER(DIOpts), _P, C.POINTER(DIStats)]),
     ("vira_gpu_adjoint", _S, [_P, C.POINTER(Camera), C.POINTER(DIOpts), _P, _P, _P, C.POINTER(DIStats)]),
+    ("vira_gpu_received_power", _S, [_P, C.POINTER(Camera), C.POINTER(Sensor), _P, _P, _P]),
     ("vira_gpu_batch_submit", _S, [_P, C.POINTER(Request), C.c_size_t, C.POINTER(BatchConfig),
                                    C.POINTER(BatchStats)]),
     ("vira_gpu_batch_autotune", _S, [_P, C.POINTER(Request), C.c_size_t, C.c_int, C.POINTER(BatchConfig),

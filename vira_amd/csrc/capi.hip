@@ -393,4 +393,14 @@ vira_gpu_status vira_gpu_batch_autotune(vira_gpu_ctx* ctx, const vira_gpu_reques
     });
 }
 
+vira_gpu_status vira_gpu_received_power(vira_gpu_ctx* ctx, const vira_gpu_camera* cam, const vira_gpu_sensor* sensor,
+                                        const float* radiance_dev, float* power_dev, float* solid_angle_dev) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        set_device(c);
+        VG_CHECK(cam && sensor, VIRA_GPU_ERR_INVALID, "received_power: null argument");
+        vg::received_power(c, *cam, *sensor, radiance_dev, power_dev, solid_angle_dev);
+    });
+}
+
 }  // extern "C"

@@ -145,6 +145,8 @@ void di_render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_di_opts& o, fl
                vira_gpu_di_stats* st);
 void di_adjoint(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_di_opts& o, const float* grad_image,
                 float* grad_heights, float* grad_texture, vira_gpu_di_stats* st);
+void received_power(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_sensor& s, const float* radiance,
+                    float* power, float* omega);
 void batch_submit(Ctx& c, const vira_gpu_request* reqs, size_t n, const vira_gpu_batch_config& cfg, int stride,
                   vira_gpu_batch_stats* st);
 void batch_autotune(Ctx& c, const vira_gpu_request* reqs, size_t n, int stride, vira_gpu_batch_config* best,

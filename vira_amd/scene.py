@@ -86,6 +86,23 @@ class Camera:
     rotation: np.ndarray = field(default_factory=lambda: np.eye(3))
     focal_length: float = 0.05
     z_dir: float = 1.0
+    f_stop: float = 2.8            # camera.hpp:235
+    optical_efficiency: float = 1.0
+
+    def aperture_area(self) -> float:
+        """CircularAperture of diameter f/N (camera.ipp:82, aperture.ipp:14-16), FP32 as the reference."""
+        f32 = np.float32
+        d = f32(self.focal_length) / f32(self.f_stop)
+        r = d / f32(2)
+        return float(f32(np.pi) * r * r)
+
+    def sensor(self) -> capi.Sensor:
+        s = capi.Sensor()
+        eta = np.broadcast_to(np.asarray(self.optical_efficiency, np.float32), (NUM_BINS,))
+        for k in range(NUM_BINS):
+            s.optical_efficiency[k] = float(eta[k])
+        s.aperture_area = self.aperture_area()
+        return s
 
     def look_in_direction(self, direction, up=(0.0, 0.0, 1.0)):
         self.rotation = look_in_direction(direction, up)
@@ -423,6 +440,19 @@ class Context:
 
     def synchronize(self):
         self._call(self.lib.vira_gpu_synchronize, self.h)
+
+    def received_power(self, camera: Camera, radiance=None, solid_angle: bool = False):
+        """Received power (H, W, 8) from a radiance image (Camera::calculateReceivedPower) and,
+        if asked, the per-pixel solid angles (H, W)."""
+        power = None if radiance is None else self.torch.empty_like(radiance)
+        omega = (self.torch.empty((camera.height, camera.width), dtype=self.torch.float32,
+                                  device=f"cuda:{self.device}") if solid_angle else None)
+        cam = camera.to_c()
+        sen = camera.sensor()
+        self._call(self.lib.vira_gpu_received_power, self.h, C.byref(cam), C.byref(sen),
+                   None if radiance is None else radiance.contiguous().data_ptr(),
+                   None if power is None else power.data_ptr(), None if omega is None else omega.data_ptr())
+        return power, omega
 
     # ---- batched rendering
     @staticmethod
