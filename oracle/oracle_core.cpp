@@ -279,13 +279,18 @@ inline bool sample_light(const Light& L, const V3<R>& p, uint32_t seed, uint32_t
     V3<R> U = normalize(cross(W, Vv));
     R phi = (R)2 * Consts<R>::pi * r1;
     R zeta = r2;
+    // cos(theta) = (1 - zeta) + zeta cos(theta_max); 1 - cos(theta_max) as
+    // x / (1 + sqrt(1 - x)), x = (r/d)^2 -- algebraically the reference's
+    // 1 - sqrt(d^2 - r^2)/d, without its FP32 cancellation for a sun at 1 AU
     R cc = radius / distance;
-    R theta = std::acos(((R)1 - zeta) + zeta * std::sqrt((R)1 - cc * cc));
-    R sinT = std::sin(theta), sinP = std::sin(phi), cosT = std::cos(theta), cosP = std::cos(phi);
+    R x = cc * cc;
+    R q = x / ((R)1 + std::sqrt((R)1 - x));
+    R t = zeta * q;
+    R cosT = (R)1 - t, sinT = std::sqrt(t * ((R)2 - t));
+    R sinP = std::sin(phi), cosP = std::cos(phi);
     R a = cosP * sinT, bb = cosT, cz = sinP * sinT;
     dir = {(a * U.x + bb * W.x) + cz * Vv.x, (a * U.y + bb * W.y) + cz * Vv.y, (a * U.z + bb * W.z) + cz * Vv.z};
-    R r2s = radius * radius;
-    R Omega = (R)2 * Consts<R>::pi * ((R)1 - std::sqrt(distance * distance - r2s) / distance);
+    R Omega = ((R)2 * Consts<R>::pi) * q;
     pdf = (R)1 / Omega;
     for (int k = 0; k < NB; ++k) rad[k] = (R)L.value[k];
     return true;
@@ -306,10 +311,13 @@ inline R light_pdf(const Light& L, const V3<R>& p, const V3<R>& dir) {
     R radius = (R)L.radius;
     if (dc <= radius) return (R)1 / ((R)4 * Consts<R>::pi * radius * radius);
     V3<R> tn = toC * ((R)1 / dc);
-    R cosMax = std::sqrt((R)1 - (radius * radius) / (dc * dc));
+    R cc = radius / dc;
+    R x = cc * cc;
+    R q = x / ((R)1 + std::sqrt((R)1 - x));
+    R cosMax = (R)1 - q;
     R ca = dot(dir, tn);
     if (ca < cosMax) return 0;
-    R Omega = (R)2 * Consts<R>::pi * ((R)1 - std::sqrt(dc * dc - radius * radius) / dc);
+    R Omega = ((R)2 * Consts<R>::pi) * q;
     return (R)1 / Omega;
 }
 

@@ -34,7 +34,12 @@
 //   * Kahan-compensated FP32 accumulation over samples (the reference sums
 //     naively, cpu_path_tracer.ipp:195);
 //   * a bounce whose cosine sample has pdf == 0 ends the path (the reference
-//     re-traces the old ray, cpu_path_tracer.ipp:437-460).
+//     re-traces the old ray, cpu_path_tracer.ipp:437-460);
+//   * the sphere light's cone (sphere_light.ipp:31-153) uses
+//     1 - cos(theta_max) = x / (1 + sqrt(1 - x)), x = (r/d)^2, and samples
+//     cos(theta) = 1 - zeta (1 - cos(theta_max)), sin(theta) from it: the same
+//     quantities as the reference's 1 - sqrt(d^2 - r^2)/d and acos/sin/cos,
+//     without the FP32 cancellation that biases a 1 AU sun by ~0.3 %.
 #pragma once
 
 #include <algorithm>

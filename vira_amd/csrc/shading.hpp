@@ -141,13 +141,19 @@ __device__ __forceinline__ void sample_light(const LightF& L, f3 p, uint32_t see
     f3 U = normalize(cross(W, Vv));
     float phi = 2.0f * kPi * r1;
     float zeta = r2;
+    // cos(theta) = (1 - zeta) + zeta cos(theta_max), with 1 - cos(theta_max)
+    // = 1 - sqrt(1 - (r/d)^2) written as x / (1 + sqrt(1 - x)), x = (r/d)^2,
+    // which keeps its precision in FP32 for a sun at 1 AU (the reference's
+    // 1 - sqrt(d^2 - r^2)/d loses about 0.3 % there)
     float cc = __fdiv_rn(radius, distance);
-    float theta = acosf((1.0f - zeta) + zeta * __builtin_sqrtf(1.0f - cc * cc));
-    float sinT = sinf(theta), sinP = sinf(phi), cosT = cosf(theta), cosP = cosf(phi);
+    float x = cc * cc;
+    float q = __fdiv_rn(x, 1.0f + __builtin_sqrtf(1.0f - x));  // 1 - cos(theta_max)
+    float t = zeta * q;                                          // 1 - cos(theta)
+    float cosT = 1.0f - t, sinT = __builtin_sqrtf(t * (2.0f - t));
+    float sinP = sinf(phi), cosP = cosf(phi);
     float a = cosP * sinT, bb = cosT, cz = sinP * sinT;
     dir = mk3((a * U.x + bb * W.x) + cz * Vv.x, (a * U.y + bb * W.y) + cz * Vv.y, (a * U.z + bb * W.z) + cz * Vv.z);
-    float r2s = radius * radius;
-    float Omega = 2.0f * kPi * (1.0f - __fdiv_rn(__builtin_sqrtf(distance * distance - r2s), distance));
+    float Omega = (2.0f * kPi) * q;  // 2 pi (1 - cos(theta_max))
     pdf = __fdiv_rn(1.0f, Omega);
 #pragma unroll
     for (int k = 0; k < NB; ++k) rad[k] = L.value[k];
@@ -167,10 +173,13 @@ __device__ __forceinline__ float light_pdf(const LightF& L, f3 p, f3 dir) {
     float radius = L.radius;
     if (dc <= radius) return __fdiv_rn(1.0f, (4.0f * kPi * radius * radius));
     f3 tn = mul(toC, __fdiv_rn(1.0f, dc));
-    float cosMax = __builtin_sqrtf(1.0f - __fdiv_rn(radius * radius, dc * dc));
+    float cc = __fdiv_rn(radius, dc);
+    float x = cc * cc;
+    float q = __fdiv_rn(x, 1.0f + __builtin_sqrtf(1.0f - x));  // 1 - cos(theta_max), as in sample_light
+    float cosMax = 1.0f - q;
     float ca = dot(dir, tn);
     if (ca < cosMax) return 0.0f;
-    float Omega = 2.0f * kPi * (1.0f - __fdiv_rn(__builtin_sqrtf(dc * dc - radius * radius), dc));
+    float Omega = (2.0f * kPi) * q;
     return __fdiv_rn(1.0f, Omega);
 }
 
