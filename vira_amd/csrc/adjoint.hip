@@ -74,7 +74,7 @@ __device__ __forceinline__ uint4 di_sample(const void* __restrict__ nodes, const
 }
 
 template <int W>
-__global__ void __launch_bounds__(kDiBS) k_di_forward(const void* __restrict__ nodes, const float4* __restrict__ tris,
+__global__ void __launch_bounds__(kDiBS, 4) k_di_forward(const void* __restrict__ nodes, const float4* __restrict__ tris,
                                                       const GVertex* __restrict__ verts,
                                                       const uint32_t* __restrict__ idx, MatF mat, Lights lights,
                                                       CamF cam, uint32_t seed, int spp, uint32_t tiles_x, uint32_t nq,
@@ -263,7 +263,7 @@ __device__ __forceinline__ void di_backward(const GVertex* __restrict__ verts, c
 }
 
 template <int W, bool RETRACE>
-__global__ void __launch_bounds__(kDiBS) k_di_adjoint(const void* __restrict__ nodes, const float4* __restrict__ tris,
+__global__ void __launch_bounds__(kDiBS, 4) k_di_adjoint(const void* __restrict__ nodes, const float4* __restrict__ tris,
                                                       const GVertex* __restrict__ verts,
                                                       const uint32_t* __restrict__ idx, MatF mat, Lights lights,
                                                       CamF cam, uint32_t seed, int spp, uint32_t tiles_x, uint32_t nq,

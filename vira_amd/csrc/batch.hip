@@ -37,7 +37,7 @@ __global__ void k_batch_reset(unsigned long long* state) {
 }
 
 template <int W, int WPB>
-__global__ void __launch_bounds__(kFusedBS* WPB) k_render_batch(const BatchReq* __restrict__ reqs, int nreq,
+__global__ void __launch_bounds__(kFusedBS* WPB, kFusedWavesPerSimd) k_render_batch(const BatchReq* __restrict__ reqs, int nreq,
                                                                 uint32_t nitems, uint32_t grab,
                                                                 float* __restrict__ acc,
                                                                 unsigned long long* __restrict__ state) {
@@ -263,12 +263,12 @@ void batch_autotune(Ctx& c, const vira_gpu_request* reqs, size_t n, int stride, 
     };
     const int w0 = c.mesh(meshes[0]).blas.width;
     set_width(w0);
-    vira_gpu_batch_config bc{w0, 1, 12, 1, 0};
+    vira_gpu_batch_config bc{w0, 1, 16, 1, 0};
     double bms = 1e300;
     for (int w : {w0, w0 == 4 ? 2 : 4}) {
         set_width(w);
         for (int wpb : {1, 2})
-            for (int wpcu : {8, 12})
+            for (int wpcu : {8, 12, 16})
                 for (int grab : {1, 4}) {
                     vira_gpu_batch_config g{w, wpb, wpcu, grab, 0};
                     vira_gpu_batch_stats s{};

@@ -118,7 +118,7 @@ struct Ctx {
     hipGraphExec_t b_exec = nullptr;
     std::vector<uint64_t> b_key;
     bool b_tuned = false;
-    vira_gpu_batch_config b_cfg{4, 1, 12, 1, 0};
+    vira_gpu_batch_config b_cfg{4, 1, 16, 1, 0};
     ~Ctx() {
         if (b_exec) (void)hipGraphExecDestroy(b_exec);
     }

@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(kTraceBS) k_trace_rays(const void* __restrict_
 // ============================================================ fused schedule
 // Per-request launch: block b renders tile b.
 template <int W, bool INST>
-__global__ void __launch_bounds__(kFusedBS) k_render_fused(TileJob J, float* __restrict__ acc,
+__global__ void __launch_bounds__(kFusedBS, kFusedWavesPerSimd) k_render_fused(TileJob J, float* __restrict__ acc,
                                                          unsigned long long* __restrict__ counters) {
     __shared__ uint32_t s_ref[kStackLds * kFusedBS];
     __shared__ float s_dist[kStackLds * kFusedBS];

@@ -239,7 +239,7 @@ class RenderRequest:
 class BatchConfig:
     bvh_width: int = 4
     waves_per_block: int = 1
-    waves_per_cu: int = 12
+    waves_per_cu: int = 16
     tiles_per_grab: int = 1
     order: int = 0
 
