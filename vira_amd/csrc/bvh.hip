@@ -506,19 +506,49 @@ __global__ void k_emit_tris(size_t nt, const uint32_t* prims, const uint32_t* id
     tris[3 * s + 2] = make_float4(c.px, c.py, c.pz, 0.f);
 }
 
-__global__ void k_tree_stats(const BNodeG* nodes, uint32_t nn, float inv_root_area, unsigned long long* cnt,
-                             uint32_t* max_leaf, double* sah) {
-    uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nn) return;
-    const BNodeG& nd = nodes[n];
-    float a = half_area(nd.lo, nd.hi) * inv_root_area;
-    if (nd.count == 0) {
-        atomicAdd(&cnt[0], 1ull);
-        atomicAdd(sah, (double)a);
-    } else {
-        atomicAdd(&cnt[1], 1ull);
-        atomicMax(max_leaf, nd.count);
-        atomicAdd(sah, (double)a * (double)nd.count);
+// Tree statistics (internal / leaf counts, largest leaf, SAH cost): each block
+// reduces in LDS and adds once, instead of one contended atomic per node.
+__global__ void __launch_bounds__(256) k_tree_stats(const BNodeG* nodes, uint32_t nn, float inv_root_area,
+                                                    unsigned long long* cnt, uint32_t* max_leaf, double* sah) {
+    __shared__ unsigned long long s_int[256], s_leaf[256];
+    __shared__ uint32_t s_max[256];
+    __shared__ double s_sah[256];
+    const uint32_t t = threadIdx.x;
+    const uint32_t n = blockIdx.x * blockDim.x + t;
+    unsigned long long ni = 0, nl = 0;
+    uint32_t mx = 0;
+    double sa = 0.0;
+    if (n < nn) {
+        const BNodeG& nd = nodes[n];
+        const double a = (double)(half_area(nd.lo, nd.hi) * inv_root_area);
+        if (nd.count == 0) {
+            ni = 1;
+            sa = a;
+        } else {
+            nl = 1;
+            mx = nd.count;
+            sa = a * (double)nd.count;
+        }
+    }
+    s_int[t] = ni;
+    s_leaf[t] = nl;
+    s_max[t] = mx;
+    s_sah[t] = sa;
+    __syncthreads();
+    for (uint32_t off = 128; off > 0; off >>= 1) {
+        if (t < off) {
+            s_int[t] += s_int[t + off];
+            s_leaf[t] += s_leaf[t + off];
+            s_max[t] = max(s_max[t], s_max[t + off]);
+            s_sah[t] += s_sah[t + off];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        atomicAdd(&cnt[0], s_int[0]);
+        atomicAdd(&cnt[1], s_leaf[0]);
+        atomicMax(max_leaf, s_max[0]);
+        atomicAdd(sah, s_sah[0]);
     }
 }
 
