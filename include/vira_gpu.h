@@ -206,6 +206,39 @@ typedef struct {
     float aperture_area;                         /* pi (f/N / 2)^2, m^2 (aperture.ipp:14-16, camera.ipp:82) */
 } vira_gpu_sensor;
 
+/* Point-spread function (PointSpreadFunction::makeKernel, psf.ipp): kind 0 =
+ * GaussianPSF (gaussian_psf.ipp; sigmas in pixels per band, rotation angle in
+ * degrees), kind 1 = AiryDiskPSF (airy_disk.ipp; focal length, aperture
+ * diameter and pixel pitch in metres, band centre wavelengths in metres). */
+typedef struct {
+    int kind;
+    float sigma_x[VIRA_GPU_NUM_BINS];
+    float sigma_y[VIRA_GPU_NUM_BINS];
+    float angle_deg;
+    float focal_length;
+    float aperture_diameter;
+    float pixel_size[2];
+    float wavelength[VIRA_GPU_NUM_BINS];
+    int supersample;  /* sub-samples per kernel-pixel side (initKernels uses 20) */
+} vira_gpu_psf;
+
+/* Sensor tail (Camera::simulateSensor / getPhotonCounts, camera.ipp:1221-1340;
+ * Photosite::exposePixel, photosite.ipp; NoiseModel::simulate, noise_model.ipp). */
+typedef struct {
+    float exposure_time;                          /* s */
+    float photon_energy[VIRA_GPU_NUM_BINS];       /* J per photon at each band centre */
+    float quantum_efficiency[VIRA_GPU_NUM_BINS];
+    float gain;                                   /* ADU per electron */
+    float max_adu;                                /* linear_scale_factor * (2^bit_depth - 1) */
+    int32_t photon_noise;                         /* Poisson photon noise when >= 1 photon in the pixel */
+    int32_t noise_model;                          /* dark current, readout noise, fixed pattern */
+    float dark_current;                           /* electrons / s (Poisson rate) */
+    float readout_noise_mean, readout_noise_std;
+    float horizontal_scale, vertical_scale;
+    int32_t horizontal_pattern_period, vertical_pattern_period;
+    uint32_t seed;                                /* counter-based RNG key */
+} vira_gpu_sensor_sim;
+
 /* --- context ------------------------------------------------------------ */
 vira_gpu_status vira_gpu_create(vira_gpu_ctx** ctx, const vira_gpu_opts* opts);
 vira_gpu_status vira_gpu_destroy(vira_gpu_ctx* ctx);
@@ -318,6 +351,19 @@ vira_gpu_status vira_gpu_adjoint(vira_gpu_ctx* ctx, const vira_gpu_camera* cam, 
  * output may be NULL. */
 vira_gpu_status vira_gpu_received_power(vira_gpu_ctx* ctx, const vira_gpu_camera* cam, const vira_gpu_sensor* sensor,
                                         const float* radiance_dev, float* power_dev, float* solid_angle_dev);
+
+/* PointSpreadFunction::makeKernel (psf.ipp): size x size x 8 kernel, value of
+ * kernel pixel (x, y) band k at kernel_dev[(x + y*size)*8 + k]; size odd >= 3. */
+vira_gpu_status vira_gpu_psf_kernel(vira_gpu_ctx* ctx, const vira_gpu_psf* psf, int size, float* kernel_dev);
+/* Image::convolve (image.ipp:993-1065, 1267-1330): per band, zero-padded,
+ * out(x, y) = sum_{kx, ky} in(x + kx - K/2, y + ky - K/2) * kernel(kx, ky).
+ * image_dev / out_dev: W*H*8 (out must not alias image). */
+vira_gpu_status vira_gpu_convolve(vira_gpu_ctx* ctx, const float* image_dev, int width, int height,
+                                  const float* kernel_dev, int ksize, float* out_dev);
+/* Camera::simulateSensor: received power (W*H*8, watts) -> photon counts
+ * (W*H*8, optional) and the normalised digital image (W*H, 0..1). */
+vira_gpu_status vira_gpu_simulate_sensor(vira_gpu_ctx* ctx, const vira_gpu_sensor_sim* cfg, const float* power_dev,
+                                         int width, int height, float* photons_dev, float* image_dev);
 
 /* --- batched rendering (new) --------------------------------------------- */
 /* Renders n requests (CPUPathTracer::render semantics per request,

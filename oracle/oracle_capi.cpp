@@ -265,6 +265,11 @@ void orc_pixel_solid_angles(const float* kinv, float zdir, int W, int H, float* 
     pixel_solid_angles(kinv, zdir, W, H, out);
 }
 
+void orc_psf_kernel(const PSFO* p, int K, float* out) { psf_kernel(*p, K, out); }
+void orc_convolve(const float* in, int W, int H, const float* kern, int K, float* out) {
+    convolve_spatial(in, W, H, kern, K, out);
+}
+
 // ---- fixed-visibility direct illumination, FP64 forward / adjoint / finite differences
 void* orc_dem_diff_create(int W, int H, const Projection* proj, const float* heights, const float* grey) {
     return new DemDiff(dem_diff_make(W, H, *proj, heights, grey));

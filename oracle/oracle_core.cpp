@@ -647,6 +647,70 @@ void pixel_solid_angles(const float kinv[6], float zdir, int W, int H, float* ou
         }
 }
 
+void psf_kernel(const PSFO& P, int K, float* out) {
+    const int S = P.supersample;
+    const float center = ((float)K - 1.0f) / 2.0f;
+    const float ang = (3.14159265358979323846f / 180.0f) * P.angle_deg;
+    const float cr = std::cos(ang), sr = std::sin(ang);
+    auto eval = [&](float px, float py, float v[NB]) {
+        if (P.kind == 0) {
+            const float rx = cr * px + sr * py, ry = -sr * px + cr * py;
+            for (int k = 0; k < NB; ++k) {
+                const float xn = rx / P.sigma_x[k], yn = ry / P.sigma_y[k];
+                const float ex = -0.5f * ((xn * xn) + (yn * yn));
+                const float coef = 1.0f / (((2.0f * 3.14159265358979323846f) * P.sigma_x[k]) * P.sigma_y[k]);
+                v[k] = coef * std::exp(ex);
+            }
+            return;
+        }
+        const float qx = px * P.pixel_size[0], qy = py * P.pixel_size[1];
+        const double radius = (double)std::sqrt(qx * qx + qy * qy);
+        if (radius < 1e-20) {
+            for (int k = 0; k < NB; ++k) v[k] = 1.0f;
+            return;
+        }
+        const double coefficient = 3.141592653589793 * (double)(P.aperture_diameter / P.focal_length);
+        for (int k = 0; k < NB; ++k) {
+            const double x = coefficient * radius / (double)P.wavelength[k];
+            const float a = (float)(2.0 * ::j1(x) / x);
+            v[k] = a * a;
+        }
+    };
+    for (int j = 0; j < K; ++j)
+        for (int i = 0; i < K; ++i) {
+            float acc[NB] = {0};
+            for (int si = 0; si < S; ++si)
+                for (int sj = 0; sj < S; ++sj) {
+                    const float ox = ((float)si + 0.5f) / (float)S, oy = ((float)sj + 0.5f) / (float)S;
+                    const float spx = ((float)i + ox) - 0.5f, spy = ((float)j + oy) - 0.5f;
+                    float v[NB];
+                    eval(spx - center, spy - center, v);
+                    for (int k = 0; k < NB; ++k) acc[k] = acc[k] + v[k];
+                }
+            for (int k = 0; k < NB; ++k) out[((size_t)i + (size_t)j * K) * NB + k] = acc[k] / (float)(S * S);
+        }
+}
+
+void convolve_spatial(const float* in, int W, int H, const float* kern, int K, float* out) {
+    const int h = K / 2;
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            float sum[NB] = {0};
+            for (int ky = 0; ky < K; ++ky) {
+                const int iy = y + ky - h;
+                if (iy < 0 || iy >= H) continue;
+                for (int kx = 0; kx < K; ++kx) {
+                    const int ix = x + kx - h;
+                    if (ix < 0 || ix >= W) continue;
+                    const float* src = in + ((size_t)iy * W + ix) * NB;
+                    const float* kv = kern + ((size_t)ky * K + kx) * NB;
+                    for (int k = 0; k < NB; ++k) sum[k] = sum[k] + src[k] * kv[k];
+                }
+            }
+            for (int k = 0; k < NB; ++k) out[((size_t)y * W + x) * NB + k] = sum[k];
+        }
+}
+
 static InstanceO identity_instance(const Mesh& m, const BVH& b, const Material& mat) {
     InstanceO I{&m, &b, mat, {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1}};
     return I;

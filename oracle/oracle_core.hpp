@@ -564,6 +564,22 @@ void render_f64(const Mesh& m, const BVH& b, const Camera& cam, const Light* lig
 // triangles by Girard's theorem (camera.ipp:1578-1592), stored FP32.
 void pixel_solid_angles(const float kinv[6], float zdir, int W, int H, float* out);
 
+// Point-spread functions (same field layout as the library's vira_gpu_psf):
+// GaussianPSF::evaluate (gaussian_psf.ipp), AiryDiskPSF::evaluate
+// (airy_disk.ipp), kernels by PointSpreadFunction::makeKernel (psf.ipp).
+struct PSFO {
+    int kind;  // 0 Gaussian, 1 Airy disk
+    float sigma_x[NB], sigma_y[NB];
+    float angle_deg;
+    float focal_length, aperture_diameter;
+    float pixel_size[2];
+    float wavelength[NB];
+    int supersample;
+};
+void psf_kernel(const PSFO& p, int K, float* out);  // out[(x + y*K)*NB + k]
+// Image::convolveSpatial (image.ipp:1267-1330), FP32, zero padding, ky outer / kx inner.
+void convolve_spatial(const float* in, int W, int H, const float* kern, int K, float* out);
+
 // Two-level scene.  One instance with the identity matrix renders in the
 // caller's frame; anything else renders camera-relative: model matrices and
 // light positions minus the FP64 camera position, rounded to the working

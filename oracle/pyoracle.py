@@ -42,6 +42,12 @@ class Material(C.Structure):
                 ("tex", C.c_void_p), ("smooth", C.c_int)]
 
 
+class PSF(C.Structure):
+    _fields_ = [("kind", C.c_int), ("sigma_x", C.c_float * NB), ("sigma_y", C.c_float * NB),
+                ("angle_deg", C.c_float), ("focal_length", C.c_float), ("aperture_diameter", C.c_float),
+                ("pixel_size", C.c_float * 2), ("wavelength", C.c_float * NB), ("supersample", C.c_int)]
+
+
 class Options(C.Structure):
     _fields_ = [("spp", C.c_int), ("bounces", C.c_int), ("rr", C.c_int), ("rr_start", C.c_int),
                 ("seed", C.c_uint32)]
@@ -82,6 +88,8 @@ def lib() -> C.CDLL:
         "orc_render_scene": (None, [P, P, P, C.c_int, C.c_int, C.POINTER(Camera), C.POINTER(Light), C.c_int,
                                     C.POINTER(Options), P, P, P, P, P, P, P, P, P, P, C.c_int, P, C.c_size_t]),
         "orc_pixel_solid_angles": (None, [P, C.c_float, C.c_int, C.c_int, P]),
+        "orc_psf_kernel": (None, [C.POINTER(PSF), C.c_int, P]),
+        "orc_convolve": (None, [P, C.c_int, C.c_int, P, C.c_int, P]),
         "orc_dem_diff_create": (P, [C.c_int, C.c_int, C.POINTER(Projection), P, P]),
         "orc_dem_diff_destroy": (None, [P]),
         "orc_di_forward64": (None, [P, P, P, P, C.c_size_t, C.c_int, C.POINTER(Light), C.c_int,
@@ -317,6 +325,28 @@ def pixel_solid_angles(cam: Camera) -> np.ndarray:
     k = np.array([cam.kinv[i] for i in range(6)], np.float32)
     out = np.empty((cam.H, cam.W), np.float32)
     lib().orc_pixel_solid_angles(k.ctypes.data, cam.zdir, cam.W, cam.H, out.ctypes.data)
+    return out
+
+
+def psf_from(c) -> PSF:
+    """Copy of a library PSF struct (same field layout)."""
+    p = PSF()
+    C.memmove(C.byref(p), C.byref(c), C.sizeof(PSF))
+    return p
+
+
+def psf_kernel(p: PSF, K: int) -> np.ndarray:
+    out = np.empty((K, K, NB), np.float32)
+    lib().orc_psf_kernel(C.byref(p), K, out.ctypes.data)
+    return out
+
+
+def convolve(img: np.ndarray, kern: np.ndarray) -> np.ndarray:
+    img = np.ascontiguousarray(img, np.float32)
+    kern = np.ascontiguousarray(kern, np.float32)
+    H, W = img.shape[:2]
+    out = np.empty_like(img)
+    lib().orc_convolve(img.ctypes.data, W, H, kern.ctypes.data, kern.shape[0], out.ctypes.data)
     return out
 
 

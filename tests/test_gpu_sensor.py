@@ -83,3 +83,95 @@ def test_monte_carlo_mean_within_3_sigma(gpu_ctx):
         else:
             assert np.mean(z <= 3.0) >= 0.97
             assert abs(bias) <= 2e-3
+
+
+# ---------------------------------------------------------------- sensor tail
+from vira_amd import sensor as SN  # noqa: E402
+
+
+@pytest.mark.parametrize("psf", [SN.GaussianPSF(1.3, 0.7, 25.0, 20),
+                                 SN.AiryDiskPSF(0.05, 0.05 / 2.8, (4.8e-6, 4.8e-6), 10)])
+def test_psf_kernel_against_oracle(gpu_ctx, psf):
+    K = 27
+    g = gpu_ctx.psf_kernel(psf, K).cpu().numpy().astype(np.float64)
+    o = O.psf_kernel(O.psf_from(psf.to_c()), K).astype(np.float64)
+    big = o > 1e-6 * o.max()
+    rel = np.abs(g - o)[big] / o[big]
+    assert rel.max() <= 1e-5, rel.max()  # libm exp / j1 of device and host
+
+
+def test_convolution_bit_exact_against_oracle(gpu_ctx):
+    import torch
+    rng = np.random.default_rng(1)
+    img = rng.random((48, 64, 8)).astype(np.float32)
+    for K in (9, 27, 301):
+        kern = gpu_ctx.psf_kernel(SN.GaussianPSF(1.5 + K / 20.0, supersample=4), K)
+        got = gpu_ctx.convolve(torch.from_numpy(img).cuda(), kern).cpu().numpy()
+        ref = O.convolve(img, kern.cpu().numpy())
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), K
+
+
+def test_sensor_deterministic_bit_exact(gpu_ctx):
+    import torch
+    rng = np.random.default_rng(2)
+    P = (rng.random((32, 40, 8)) * 1e-15).astype(np.float32)
+    cfg = SN.SensorConfig(exposure_time=0.01, gain=0.5, bit_depth=12)
+    img, ph = gpu_ctx.simulate_sensor(cfg, torch.from_numpy(P).cuda(), photons=True)
+    c = cfg.to_c()
+    E = np.array([c.photon_energy[k] for k in range(8)], np.float32)
+    qe = np.array([c.quantum_efficiency[k] for k in range(8)], np.float32)
+    cnt = (np.float32(c.exposure_time) * P) / E
+    assert np.array_equal(ph.cpu().numpy().view(np.uint32), cnt.view(np.uint32))
+    e = np.zeros(cnt.shape[:2], np.float32)
+    for k in range(8):
+        e = e + qe[k] * cnt[:, :, k]
+    e = e + np.float32(0)
+    ref = np.minimum(np.float32(c.gain) * e, np.float32(c.max_adu)) / np.float32(c.max_adu)
+    assert np.array_equal(img.cpu().numpy().view(np.uint32), ref.astype(np.float32).view(np.uint32))
+    assert 0 < ref.max() <= 1.0
+
+
+def test_sensor_noise_statistics(gpu_ctx):
+    """Poisson photon noise (multiplication method below 10, PTRS above), dark
+    current (Poisson) and readout noise (normal): sample means and variances
+    over 2^17 pixels within 5 standard errors of the parameters."""
+    import torch
+    n = 1 << 17
+    c0 = SN.SensorConfig()
+    E = np.array([c0.to_c().photon_energy[k] for k in range(8)], np.float32)
+    lams = np.array([0.5, 3.0, 9.9, 10.5, 60.0, 5000.0, 2.0, 25.0], np.float32)
+    P = np.broadcast_to(lams * E, (n, 1, 8)).astype(np.float32).copy()  # exposure 1 s: photons = P / E
+    cfg = SN.SensorConfig(exposure_time=1.0, photon_noise=True, seed=5)
+    _, ph = gpu_ctx.simulate_sensor(cfg, torch.from_numpy(P).cuda(), photons=True)
+    ph = ph.cpu().numpy().reshape(n, 8).astype(np.float64)
+    lam = (np.float32(1.0) * P[0, 0] / E).astype(np.float64)
+    for k in range(8):
+        m, v = ph[:, k].mean(), ph[:, k].var()
+        assert np.all(ph[:, k] == np.round(ph[:, k])) and ph[:, k].min() >= 0
+        assert abs(m - lam[k]) <= 5 * np.sqrt(lam[k] / n), (k, m, lam[k])
+        assert abs(v - lam[k]) <= 5 * lam[k] * np.sqrt(2.0 / n) + 1e-3, (k, v, lam[k])
+    # dark current + readout noise, no light: image = gain (fp x (dark t + N(mu, sd))) / max_adu
+    cfg = SN.SensorConfig(exposure_time=2.0, noise_model=True, dark_current=7.0, readout_noise_mean=20.0,
+                          readout_noise_std=3.0, gain=1.0, bit_depth=16, seed=9)
+    img, _ = gpu_ctx.simulate_sensor(cfg, torch.zeros((n, 1, 8), device="cuda"))
+    e = img.cpu().numpy().reshape(-1).astype(np.float64) * cfg.max_adu()
+    mean, var = 7.0 * 2.0 + 20.0, 7.0 * 4.0 + 9.0
+    assert abs(e.mean() - mean) <= 5 * np.sqrt(var / n)
+    assert abs(e.var() - var) <= 5 * var * np.sqrt(2.0 / n) * 2
+
+
+def test_sensor_fixed_pattern(gpu_ctx):
+    import torch
+    cfg = SN.SensorConfig(noise_model=True, readout_noise_mean=10.0, horizontal_scale=0.2, vertical_scale=0.1,
+                          horizontal_pattern_period=8, vertical_pattern_period=5, gain=1.0, bit_depth=16)
+    H, W = 12, 20
+    img, _ = gpu_ctx.simulate_sensor(cfg, torch.zeros((H, W, 8), device="cuda"))
+    f32 = np.float32
+    i = np.arange(W, dtype=np.float32)[None, :]
+    j = np.arange(H, dtype=np.float32)[:, None]
+    two_pi = f32(2.0) * f32(np.pi)
+    sh = (f32(1) - f32(0.2)) + f32(0.2) * (f32(1) + np.sin((two_pi * i) / (f32(2) * f32(8))))
+    sv = (f32(1) - f32(0.1)) + f32(0.1) * (f32(1) + np.sin((two_pi * j) / (f32(2) * f32(5))))
+    ref = (sh * sv) * f32(10.0)
+    got = img.cpu().numpy() * np.float32(cfg.max_adu())
+    assert np.max(np.abs(got - ref) / ref) < 1e-5

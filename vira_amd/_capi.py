@@ -183,6 +183,32 @@ class Sensor(C.Structure):
     _fields_ = [("optical_efficiency", C.c_float * NUM_BINS), ("aperture_area", C.c_float)]
 
 
+class PSF(C.Structure):
+    _fields_ = [("kind", C.c_int), ("sigma_x", C.c_float * NUM_BINS), ("sigma_y", C.c_float * NUM_BINS),
+                ("angle_deg", C.c_float), ("focal_length", C.c_float), ("aperture_diameter", C.c_float),
+                ("pixel_size", C.c_float * 2), ("wavelength", C.c_float * NUM_BINS), ("supersample", C.c_int)]
+
+
+class SensorSim(C.Structure):
+    _fields_ = [
+        ("exposure_time", C.c_float),
+        ("photon_energy", C.c_float * NUM_BINS),
+        ("quantum_efficiency", C.c_float * NUM_BINS),
+        ("gain", C.c_float),
+        ("max_adu", C.c_float),
+        ("photon_noise", C.c_int32),
+        ("noise_model", C.c_int32),
+        ("dark_current", C.c_float),
+        ("readout_noise_mean", C.c_float),
+        ("readout_noise_std", C.c_float),
+        ("horizontal_scale", C.c_float),
+        ("vertical_scale", C.c_float),
+        ("horizontal_pattern_period", C.c_int32),
+        ("vertical_pattern_period", C.c_int32),
+        ("seed", C.c_uint32),
+    ]
+
+
 class Instance(C.Structure):
     _fields_ = [("mesh_id", C.c_int32), ("transform", C.c_double * 16)]
 
@@ -215,6 +241,9 @@ SIGNATURES = [
     ("vira_gpu_di_render", _S, [_P, C.POINTER(Camera), C.POINTER(DIOpts), _P, C.POINTER(DIStats)]),
     ("vira_gpu_adjoint", _S, [_P, C.POINTER(Camera), C.POINTER(DIOpts), _P, _P, _P, C.POINTER(DIStats)]),
     ("vira_gpu_received_power", _S, [_P, C.POINTER(Camera), C.POINTER(Sensor), _P, _P, _P]),
+    ("vira_gpu_psf_kernel", _S, [_P, C.POINTER(PSF), C.c_int, _P]),
+    ("vira_gpu_convolve", _S, [_P, _P, C.c_int, C.c_int, _P, C.c_int, _P]),
+    ("vira_gpu_simulate_sensor", _S, [_P, C.POINTER(SensorSim), _P, C.c_int, C.c_int, _P, _P]),
     ("vira_gpu_batch_submit", _S, [_P, C.POINTER(Request), C.c_size_t, C.POINTER(BatchConfig),
                                    C.POINTER(BatchStats)]),
     ("vira_gpu_batch_autotune", _S, [_P, C.POINTER(Request), C.c_size_t, C.c_int, C.POINTER(BatchConfig),

@@ -403,4 +403,32 @@ vira_gpu_status vira_gpu_received_power(vira_gpu_ctx* ctx, const vira_gpu_camera
     });
 }
 
+vira_gpu_status vira_gpu_psf_kernel(vira_gpu_ctx* ctx, const vira_gpu_psf* psf, int size, float* kernel_dev) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        set_device(c);
+        VG_CHECK(psf, VIRA_GPU_ERR_INVALID, "psf_kernel: null psf");
+        vg::psf_kernel(c, *psf, size, kernel_dev);
+    });
+}
+
+vira_gpu_status vira_gpu_convolve(vira_gpu_ctx* ctx, const float* image_dev, int width, int height,
+                                  const float* kernel_dev, int ksize, float* out_dev) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        set_device(c);
+        vg::convolve(c, image_dev, width, height, kernel_dev, ksize, out_dev);
+    });
+}
+
+vira_gpu_status vira_gpu_simulate_sensor(vira_gpu_ctx* ctx, const vira_gpu_sensor_sim* cfg, const float* power_dev,
+                                         int width, int height, float* photons_dev, float* image_dev) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        set_device(c);
+        VG_CHECK(cfg, VIRA_GPU_ERR_INVALID, "simulate_sensor: null config");
+        vg::simulate_sensor(c, *cfg, power_dev, width, height, photons_dev, image_dev);
+    });
+}
+
 }  // extern "C"

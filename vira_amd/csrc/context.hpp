@@ -147,6 +147,10 @@ void di_adjoint(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_di_opts& o, c
                 float* grad_heights, float* grad_texture, vira_gpu_di_stats* st);
 void received_power(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_sensor& s, const float* radiance,
                     float* power, float* omega);
+void psf_kernel(Ctx& c, const vira_gpu_psf& P, int K, float* out);
+void convolve(Ctx& c, const float* in, int W, int H, const float* kern, int K, float* out);
+void simulate_sensor(Ctx& c, const vira_gpu_sensor_sim& cfg, const float* power, int W, int H, float* photons,
+                     float* image);
 void batch_submit(Ctx& c, const vira_gpu_request* reqs, size_t n, const vira_gpu_batch_config& cfg, int stride,
                   vira_gpu_batch_stats* st);
 void batch_autotune(Ctx& c, const vira_gpu_request* reqs, size_t n, int stride, vira_gpu_batch_config* best,

@@ -59,7 +59,233 @@ __global__ void k_received_power(int W, int H, CamF cam, vira_gpu_sensor sensor,
     }
 }
 
+// ------------------------------------------------------------ point-spread functions
+// GaussianPSF::evaluate (gaussian_psf.ipp) / AiryDiskPSF::evaluate (airy_disk.ipp)
+// at a kernel-plane offset (pixels) from the kernel centre.
+__device__ __forceinline__ void psf_eval(const vira_gpu_psf& P, float cr, float sr, float px, float py,
+                                         float v[NB]) {
+    if (P.kind == 0) {
+        const float rx = cr * px + sr * py;  // rotation_ * point, rotation_ = [[c, -s], [s, c]] (column-major)
+        const float ry = -sr * px + cr * py;
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            const float xn = __fdiv_rn(rx, P.sigma_x[k]), yn = __fdiv_rn(ry, P.sigma_y[k]);
+            const float ex = -0.5f * ((xn * xn) + (yn * yn));
+            const float coef = __fdiv_rn(1.0f, ((2.0f * kPi) * P.sigma_x[k]) * P.sigma_y[k]);
+            v[k] = coef * expf(ex);
+        }
+        return;
+    }
+    const float qx = px * P.pixel_size[0], qy = py * P.pixel_size[1];
+    const double radius = (double)__builtin_sqrtf(qx * qx + qy * qy);  // length() of the FP32 vector
+    if (radius < 1e-20) {
+#pragma unroll
+        for (int k = 0; k < NB; ++k) v[k] = 1.0f;
+        return;
+    }
+    const double coefficient = 3.141592653589793 * (double)__fdiv_rn(P.aperture_diameter, P.focal_length);
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        const double x = coefficient * radius / (double)P.wavelength[k];
+        const float a = (float)(2.0 * j1(x) / x);
+        v[k] = a * a;
+    }
+}
+
+// PointSpreadFunction::makeKernel (psf.ipp): kernel pixel (x, y) is the mean of
+// S x S sub-pixel evaluations, sample_i outer and sample_j inner.
+__global__ void k_psf_kernel(vira_gpu_psf P, int K, float* __restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= K * K) return;
+    const int i = t % K, j = t / K;
+    const int S = P.supersample;
+    const float center = ((float)K - 1.0f) / 2.0f;
+    const float ang = (3.14159265358979323846f / 180.0f) * P.angle_deg;
+    const float cr = cosf(ang), sr = sinf(ang);
+    float acc[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) acc[k] = 0.0f;
+    for (int si = 0; si < S; ++si) {
+        for (int sj = 0; sj < S; ++sj) {
+            const float ox = __fdiv_rn((float)si + 0.5f, (float)S), oy = __fdiv_rn((float)sj + 0.5f, (float)S);
+            const float spx = ((float)i + ox) - 0.5f, spy = ((float)j + oy) - 0.5f;
+            float v[NB];
+            psf_eval(P, cr, sr, spx - center, spy - center, v);
+#pragma unroll
+            for (int k = 0; k < NB; ++k) acc[k] = acc[k] + v[k];
+        }
+    }
+    const float n = (float)(S * S);
+#pragma unroll
+    for (int k = 0; k < NB; ++k) out[(size_t)t * NB + k] = __fdiv_rn(acc[k], n);
+}
+
+// Image::convolveSpatial (image.ipp:1267-1330): zero padding, per-band sum in
+// kernel order ky outer, kx inner.  The kernel row being summed is staged in
+// LDS for the block.
+constexpr int kConvBS = 256;
+__global__ void __launch_bounds__(kConvBS) k_convolve(const float* __restrict__ in, int W, int H,
+                                                     const float* __restrict__ kern, int K, float* __restrict__ out) {
+    __shared__ float s_row[256 * NB];
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = p < (size_t)W * H;
+    const int x = valid ? (int)(p % (size_t)W) : 0, y = valid ? (int)(p / (size_t)W) : 0;
+    const int h = K / 2;
+    float sum[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) sum[k] = 0.0f;
+    for (int ky = 0; ky < K; ++ky) {
+        for (int kx0 = 0; kx0 < K; kx0 += 256) {
+            const int n = min(256, K - kx0);
+            __syncthreads();
+            for (int e = threadIdx.x; e < n * NB; e += kConvBS) s_row[e] = kern[((size_t)ky * K + kx0) * NB + e];
+            __syncthreads();
+            const int iy = y + ky - h;
+            if (!valid || iy < 0 || iy >= H) continue;
+            for (int kx = 0; kx < n; ++kx) {
+                const int ix = x + kx0 + kx - h;
+                if (ix < 0 || ix >= W) continue;
+                const float* src = in + ((size_t)iy * W + ix) * NB;
+#pragma unroll
+                for (int k = 0; k < NB; ++k) sum[k] = sum[k] + src[k] * s_row[kx * NB + k];
+            }
+        }
+    }
+    if (!valid) return;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) out[p * NB + k] = sum[k];
+}
+
+// ------------------------------------------------------------ sensor
+// Counter-based draws: stream (seed, pixel, iteration, dim).
+constexpr uint32_t kDimPhoton = 64, kDimDark = 80, kDimReadout = 96;
+
+// Poisson(lambda): multiplication method below 10, PTRS (Hoermann 1993,
+// "The transformed rejection method for generating Poisson random
+// variables") at and above; at most 64 proposals (the mean is returned in
+// the vanishingly rare case that none is accepted).
+__device__ __forceinline__ float poisson(float lambda, uint32_t seed, uint32_t pix, uint32_t dim) {
+    if (!(lambda > 0.0f)) return 0.0f;
+    if (lambda < 10.0f) {
+        const float L = expf(-lambda);
+        float prod = 1.0f;
+        for (uint32_t k = 0; k < 256u; ++k) {
+            prod = prod * rng_u01(seed, pix, k, dim);
+            if (prod <= L) return (float)k;
+        }
+        return lambda;
+    }
+    const float slam = sqrtf(lambda), loglam = logf(lambda);
+    const float b = 0.931f + 2.53f * slam;
+    const float a = -0.059f + 0.02483f * b;
+    const float invalpha = 1.1239f + __fdiv_rn(1.1328f, b - 3.4f);
+    const float vr = 0.9277f - __fdiv_rn(3.6224f, b - 2.0f);
+    for (uint32_t it = 0; it < 64u; ++it) {
+        const float U = rng_u01(seed, pix, 2u * it, dim) - 0.5f;
+        const float V = rng_u01(seed, pix, 2u * it + 1u, dim);
+        const float us = 0.5f - fabsf(U);
+        const float k = floorf((2.0f * __fdiv_rn(a, us) + b) * U + lambda + 0.43f);
+        if (us >= 0.07f && V <= vr) return k;
+        if (k < 0.0f || (us < 0.013f && V > us)) continue;
+        if ((logf(V) + logf(invalpha) - logf(__fdiv_rn(a, us * us) + b)) <= (-lambda + k * loglam - lgammaf(k + 1.0f)))
+            return k;
+    }
+    return lambda;
+}
+
+// Normal(mean, std) by Box-Muller on two counter draws.
+__device__ __forceinline__ float normal(float mean, float sd, uint32_t seed, uint32_t pix, uint32_t dim) {
+    const float u1 = 1.0f - rng_u01(seed, pix, 0u, dim);  // (0, 1]
+    const float u2 = rng_u01(seed, pix, 1u, dim);
+    return mean + sd * (sqrtf(-2.0f * logf(u1)) * cosf((2.0f * kPi) * u2));
+}
+
+__global__ void k_simulate_sensor(vira_gpu_sensor_sim c, const float* __restrict__ power, int W, int H,
+                                  float* __restrict__ photons, float* __restrict__ image) {
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (size_t)W * H) return;
+    const uint32_t pix = (uint32_t)p;
+    // getPhotonCounts (camera.ipp:1281-1330): energy = t P, photons = energy / E_photon
+    float cnt[NB];
+    float total = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        const float energy = c.exposure_time * power[p * NB + k];
+        cnt[k] = __fdiv_rn(energy, c.photon_energy[k]);
+        total = total + cnt[k];
+    }
+    if (c.photon_noise && total >= 1.0f) {
+#pragma unroll
+        for (int k = 0; k < NB; ++k) cnt[k] = poisson(cnt[k], c.seed, pix, kDimPhoton + (uint32_t)k);
+    }
+    if (photons) {
+#pragma unroll
+        for (int k = 0; k < NB; ++k) photons[p * NB + k] = cnt[k];
+    }
+    // NoiseModel::simulate (noise_model.ipp): fixed pattern x (dark + readout)
+    float noise = 0.0f;
+    if (c.noise_model) {
+        const int i = (int)(p % (size_t)W), j = (int)(p / (size_t)W);
+        const float dark = poisson(c.dark_current, c.seed, pix, kDimDark) * c.exposure_time;
+        const float readout = normal(c.readout_noise_mean, c.readout_noise_std, c.seed, pix, kDimReadout);
+        float scale = 1.0f;
+        if (c.horizontal_pattern_period != 0)
+            scale = scale * ((1.0f - c.horizontal_scale) +
+                             c.horizontal_scale * (1.0f + sinf(__fdiv_rn((2.0f * kPi) * (float)i,
+                                                                          2.0f * (float)c.horizontal_pattern_period))));
+        if (c.vertical_pattern_period != 0)
+            scale = scale * ((1.0f - c.vertical_scale) +
+                             c.vertical_scale * (1.0f + sinf(__fdiv_rn((2.0f * kPi) * (float)j,
+                                                                        2.0f * (float)c.vertical_pattern_period))));
+        noise = scale * (dark + readout);
+    }
+    // Photosite::exposePixel (photosite.ipp): electrons = sum QE photons + noise
+    float e = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) e = e + c.quantum_efficiency[k] * cnt[k];
+    e = e + noise;
+    const float amp = fminf(c.gain * e, c.max_adu);
+    if (image) image[p] = __fdiv_rn(amp, c.max_adu);
+}
+
 }  // namespace
+
+void psf_kernel(Ctx& c, const vira_gpu_psf& P, int K, float* out) {
+    VG_CHECK(K >= 3 && K % 2 == 1 && K <= 1023, VIRA_GPU_ERR_INVALID, "psf_kernel: size must be odd, 3..1023");
+    VG_CHECK(P.kind == 0 || P.kind == 1, VIRA_GPU_ERR_INVALID, "psf_kernel: kind 0 (Gaussian) or 1 (Airy disk)");
+    VG_CHECK(P.supersample >= 1 && P.supersample <= 64, VIRA_GPU_ERR_INVALID, "psf_kernel: supersample 1..64");
+    VG_CHECK(out, VIRA_GPU_ERR_INVALID, "psf_kernel: null output");
+    if (P.kind == 0)
+        for (int k = 0; k < NB; ++k)
+            VG_CHECK(P.sigma_x[k] > 0 && P.sigma_y[k] > 0, VIRA_GPU_ERR_INVALID, "psf_kernel: sigmas must be positive");
+    else
+        VG_CHECK(P.focal_length > 0 && P.aperture_diameter > 0 && P.pixel_size[0] > 0 && P.pixel_size[1] > 0,
+                 VIRA_GPU_ERR_INVALID, "psf_kernel: Airy disk needs focal length, aperture and pixel size");
+    k_psf_kernel<<<blocks_for((size_t)K * K, 64), 64, 0, c.stream>>>(P, K, out);
+    VG_HIP(hipGetLastError());
+    VG_HIP(hipStreamSynchronize(c.stream));
+}
+
+void convolve(Ctx& c, const float* in, int W, int H, const float* kern, int K, float* out) {
+    VG_CHECK(in && kern && out && in != out, VIRA_GPU_ERR_INVALID, "convolve: null or aliased buffers");
+    VG_CHECK(W > 0 && H > 0 && K >= 1 && K % 2 == 1, VIRA_GPU_ERR_INVALID, "convolve: bad sizes");
+    const size_t n = (size_t)W * H;
+    k_convolve<<<blocks_for(n, kConvBS), kConvBS, 0, c.stream>>>(in, W, H, kern, K, out);
+    VG_HIP(hipGetLastError());
+    VG_HIP(hipStreamSynchronize(c.stream));
+}
+
+void simulate_sensor(Ctx& c, const vira_gpu_sensor_sim& cfg, const float* power, int W, int H, float* photons,
+                     float* image) {
+    VG_CHECK(power && W > 0 && H > 0, VIRA_GPU_ERR_INVALID, "simulate_sensor: bad arguments");
+    VG_CHECK(cfg.max_adu > 0 && cfg.exposure_time >= 0, VIRA_GPU_ERR_INVALID, "simulate_sensor: bad sensor config");
+    for (int k = 0; k < NB; ++k)
+        VG_CHECK(cfg.photon_energy[k] > 0, VIRA_GPU_ERR_INVALID, "simulate_sensor: photon energies must be positive");
+    const size_t n = (size_t)W * H;
+    k_simulate_sensor<<<blocks_for(n, 256), 256, 0, c.stream>>>(cfg, power, W, H, photons, image);
+    VG_HIP(hipGetLastError());
+    VG_HIP(hipStreamSynchronize(c.stream));
+}
 
 void received_power(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_sensor& s, const float* radiance,
                     float* power, float* omega) {

@@ -454,6 +454,35 @@ class Context:
                    None if power is None else power.data_ptr(), None if omega is None else omega.data_ptr())
         return power, omega
 
+    # ---- sensor tail
+    def psf_kernel(self, psf, size: int):
+        """(size, size, 8) kernel of a GaussianPSF / AiryDiskPSF (PointSpreadFunction::makeKernel)."""
+        k = self.torch.empty((size, size, NUM_BINS), dtype=self.torch.float32, device=f"cuda:{self.device}")
+        c = psf.to_c()
+        self._call(self.lib.vira_gpu_psf_kernel, self.h, C.byref(c), size, k.data_ptr())
+        return k
+
+    def convolve(self, image, kernel):
+        """Image::convolve of an (H, W, 8) image with a (K, K, 8) kernel, zero-padded."""
+        img = image.contiguous()
+        out = self.torch.empty_like(img)
+        H, W = img.shape[:2]
+        self._call(self.lib.vira_gpu_convolve, self.h, img.data_ptr(), W, H, kernel.contiguous().data_ptr(),
+                   kernel.shape[0], out.data_ptr())
+        return out
+
+    def simulate_sensor(self, config, power, photons: bool = False):
+        """Camera::simulateSensor: (H, W, 8) received power -> (H, W) normalised image
+        (and the (H, W, 8) photon counts if asked)."""
+        p = power.contiguous()
+        H, W = p.shape[:2]
+        img = self.torch.empty((H, W), dtype=self.torch.float32, device=p.device)
+        ph = self.torch.empty_like(p) if photons else None
+        c = config.to_c()
+        self._call(self.lib.vira_gpu_simulate_sensor, self.h, C.byref(c), p.data_ptr(), W, H,
+                   None if ph is None else ph.data_ptr(), img.data_ptr())
+        return img, ph
+
     # ---- batched rendering
     @staticmethod
     def _requests(requests):

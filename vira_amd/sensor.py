@@ -1,0 +1,126 @@
+"""Camera sensor tail: spectral band table, point-spread functions and the
+photosite/noise configuration, mirroring vira's camera components.
+
+* band table and photon energies: ``initializeSpectralBands`` /
+  ``PhotonEnergy`` (spectral_data.ipp:64-132, math.ipp:11-14), FP32 as there;
+* ``GaussianPSF`` (gaussian_psf.ipp), ``AiryDiskPSF`` (airy_disk.ipp),
+  kernels by ``PointSpreadFunction::makeKernel`` (psf.ipp);
+* ``PhotositeConfig`` (photosite.hpp:19-42, photosite.ipp) and
+  ``NoiseModelConfig`` (noise_model.hpp:16-25).
+All compute runs in the HIP library (``Context.psf_kernel``, ``convolve``,
+``simulate_sensor``).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _capi as capi
+
+NUM_BINS = capi.NUM_BINS
+PLANCK = np.float32(6.62607015e-34)      # math.hpp:35
+LIGHT_SPEED = np.float32(299792458.0)    # math.hpp:32
+
+
+def band_table(lo_nm: float = 400.0, hi_nm: float = 1000.0, n: int = NUM_BINS):
+    """(min, max, centre) wavelengths in metres and photon energies in joules,
+    computed in FP32 as initializeSpectralBands<N, UNIFORM_RANGE, lo, hi>."""
+    f = np.float32
+    step = (f(hi_nm) - f(lo_nm)) / f(n)
+    bounds = np.array([f(lo_nm) + f(i) * step for i in range(n + 1)], np.float32)
+    lo = bounds[:-1] * f(1e-9)
+    hi = bounds[1:] * f(1e-9)
+    centre = (lo + hi) / f(2)
+    energy = (PLANCK * LIGHT_SPEED) / centre
+    return lo, hi, centre.astype(np.float32), energy.astype(np.float32)
+
+
+def default_quantum_efficiency() -> np.ndarray:
+    """PhotositeConfig::defaultQE (photosite.ipp): 0.5 x E_k / mean(E)."""
+    _, _, _, E = band_table()
+    total = np.float32(0)
+    for e in E:
+        total = np.float32(total + e / np.float32(NUM_BINS))
+    return (np.float32(0.5) * E / total).astype(np.float32)
+
+
+@dataclass
+class GaussianPSF:
+    sigma_x: object = 1.0   # pixels, scalar or per band
+    sigma_y: object = None
+    angle_deg: float = 0.0
+    supersample: int = 20
+
+    def to_c(self) -> capi.PSF:
+        p = capi.PSF()
+        p.kind = 0
+        sx = np.broadcast_to(np.asarray(self.sigma_x, np.float32), (NUM_BINS,))
+        sy = sx if self.sigma_y is None else np.broadcast_to(np.asarray(self.sigma_y, np.float32), (NUM_BINS,))
+        for k in range(NUM_BINS):
+            p.sigma_x[k], p.sigma_y[k] = float(sx[k]), float(sy[k])
+        p.angle_deg = float(self.angle_deg)
+        p.supersample = int(self.supersample)
+        return p
+
+
+@dataclass
+class AiryDiskPSF:
+    focal_length: float        # metres
+    aperture_diameter: float   # metres
+    pixel_size: tuple          # metres (x, y)
+    supersample: int = 20
+
+    def to_c(self) -> capi.PSF:
+        p = capi.PSF()
+        p.kind = 1
+        p.focal_length = float(self.focal_length)
+        p.aperture_diameter = float(self.aperture_diameter)
+        p.pixel_size[0], p.pixel_size[1] = float(self.pixel_size[0]), float(self.pixel_size[1])
+        _, _, centre, _ = band_table()
+        for k in range(NUM_BINS):
+            p.wavelength[k] = float(centre[k])
+        p.supersample = int(self.supersample)
+        return p
+
+
+@dataclass
+class SensorConfig:
+    """Photosite + noise model + exposure (defaults as the reference)."""
+    exposure_time: float = 1.0
+    quantum_efficiency: np.ndarray = field(default_factory=default_quantum_efficiency)
+    gain: float = 0.333333         # ADU / electron (photosite.hpp:25)
+    bit_depth: int = 8             # photosite.hpp:22
+    linear_scale_factor: float = 1.0
+    photon_noise: bool = False
+    noise_model: bool = False
+    dark_current: float = 0.0
+    readout_noise_mean: float = 0.0
+    readout_noise_std: float = 0.0
+    horizontal_scale: float = 0.0
+    vertical_scale: float = 0.0
+    horizontal_pattern_period: int = 0
+    vertical_pattern_period: int = 0
+    seed: int = 0
+
+    def max_adu(self) -> float:
+        f = np.float32
+        return float(f(self.linear_scale_factor) * (f(2.0) ** f(self.bit_depth) - f(1.0)))
+
+    def to_c(self) -> capi.SensorSim:
+        c = capi.SensorSim()
+        _, _, _, E = band_table()
+        qe = np.broadcast_to(np.asarray(self.quantum_efficiency, np.float32), (NUM_BINS,))
+        c.exposure_time = float(self.exposure_time)
+        for k in range(NUM_BINS):
+            c.photon_energy[k] = float(E[k])
+            c.quantum_efficiency[k] = float(qe[k])
+        c.gain, c.max_adu = float(self.gain), self.max_adu()
+        c.photon_noise, c.noise_model = int(self.photon_noise), int(self.noise_model)
+        c.dark_current = float(self.dark_current)
+        c.readout_noise_mean, c.readout_noise_std = float(self.readout_noise_mean), float(self.readout_noise_std)
+        c.horizontal_scale, c.vertical_scale = float(self.horizontal_scale), float(self.vertical_scale)
+        c.horizontal_pattern_period = int(self.horizontal_pattern_period)
+        c.vertical_pattern_period = int(self.vertical_pattern_period)
+        c.seed = int(self.seed)
+        return c
