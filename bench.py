@@ -221,8 +221,65 @@ def instanced_leg(ctx, c1_built, steps: int, warmup: int) -> dict:
           f"{seen}", file=sys.stderr, flush=True)
     ctx.tlas_build([c1_built.mesh])  # back to the config-1 scene
     ctx.lights_set(c1_built.lights)
+    stars = star_leg(ctx, steps)
     return {"workload": "instanced_moon_smallbody_spacecraft_4096px_8spp_2b", "ms_per_frame": round(dt * 1e3, 2),
-            "mrays_s": round(rays / dt / 1e6, 1), "bounce_rays": stats[0]["bounce_rays"], "instances_in_view": seen}
+            "mrays_s": round(rays / dt / 1e6, 1), "bounce_rays": stats[0]["bounce_rays"], "instances_in_view": seen,
+            "stars": stars}
+
+
+def star_leg(ctx, steps: int) -> dict:
+    """Config 4's star field: 100,000 stars (V magnitudes exponential on 0-10,
+    uniform on the sky) through the unresolved renderer into the 4096^2 camera
+    with its Airy-disk PSF (kernels 3/9/27/81 chosen per star)."""
+    import torch
+    from vira_amd import workloads as W
+
+    cam, psf = W.config4_star_camera()
+    sp = ctx.star_psf(psf, cam)
+    dirs, irr, _ = W.make_star_field()
+    d = torch.from_numpy(dirs).cuda()
+    e = torch.from_numpy(irr).cuda()
+    out = torch.zeros((cam.height, cam.width, 8), dtype=torch.float32, device="cuda")
+    ctx.render_stars(cam, sp, d, e, out=out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.render_stars(cam, sp, d, e, out=out)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    lit = int((out[:, :, 0] > 0).sum().item())
+    print(f"# config 4 stars: 100,000 sources in {dt * 1e3:.2f} ms ({lit} pixels lit)", file=sys.stderr, flush=True)
+    return {"sources": int(dirs.shape[0]), "ms": round(dt * 1e3, 3), "pixels_lit": lit}
+
+
+def sensor_leg(ctx, built, passes, steps: int) -> dict:
+    """The sensor tail after a config-1 frame: received power (pixel solid
+    angles by Girard's theorem in FP64), convolution with an 81x81 Gaussian PSF
+    (the largest of PointSpreadFunction's kernels, used for resolved renders)
+    and the photosite/noise simulation, on the 2048^2 x 8-band image."""
+    import torch
+    from vira_amd import sensor as SN
+
+    cam = built.camera
+    psf = ctx.psf_kernel(SN.GaussianPSF(1.5), 81)
+    cfg = SN.SensorConfig(exposure_time=0.01, photon_noise=True, noise_model=True, dark_current=5.0,
+                          readout_noise_std=2.0, bit_depth=12)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            out = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps * 1e3, out
+
+    rp_ms, (power, _) = timed(lambda: ctx.received_power(cam, passes.total_radiance))
+    cv_ms, blurred = timed(lambda: ctx.convolve(power, psf))
+    ss_ms, _ = timed(lambda: ctx.simulate_sensor(cfg, blurred))
+    print(f"# sensor tail (2048^2 x 8): received power {rp_ms:.2f} ms, 81x81 PSF convolution {cv_ms:.2f} ms, "
+          f"photosite + noise {ss_ms:.2f} ms", file=sys.stderr, flush=True)
+    return {"received_power_ms": round(rp_ms, 3), "psf81_convolution_ms": round(cv_ms, 3),
+            "sensor_ms": round(ss_ms, 3)}
 
 
 def main() -> int:
@@ -291,6 +348,7 @@ def main() -> int:
           f"{sum(s['trace_shadow_ms'] for s in stats):.1f} ms, shading {sum(s['shade_ms'] for s in stats):.1f} ms",
           file=sys.stderr, flush=True)
 
+    sensor = sensor_leg(ctx, built, passes, max(1, min(3, args.steps))) if args.scene == "c1" else None
     adjoint = None if args.no_adjoint else adjoint_leg(ctx, max(3, args.steps), max(1, args.warmup))
     batch = None if (args.no_batch or args.scene != "c1") else batch_leg(ctx, built, max(1, min(3, args.steps)), 1)
     instanced = None if (args.no_instanced or args.scene != "c1") else instanced_leg(ctx, built, max(1, min(3, args.steps)), 1)
@@ -357,6 +415,7 @@ def main() -> int:
         "batch": batch,
         "adjoint": adjoint,
         "instanced": instanced,
+        "sensor_tail": sensor,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

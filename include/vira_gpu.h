@@ -239,6 +239,22 @@ typedef struct {
     uint32_t seed;                                /* counter-based RNG key */
 } vira_gpu_sensor_sim;
 
+/* Point sources for the unresolved renderer (CPUUnresolvedRenderer::render,
+ * cpu_unresolved_renderer.ipp:25-200): PSF kernels of ascending odd sizes
+ * (PointSpreadFunction::initKernels, psf.ipp), the edge maxima that choose
+ * between them (getKernel), the minimum power below which a source is
+ * skipped, and the camera's intrinsic matrix K as GLM mat3x2 [col][row]
+ * (camera.ipp:1555-1557) for Camera::projectCameraPoint (camera.ipp:1062-1081). */
+typedef struct {
+    int32_t count;                                 /* 0 (no PSF) .. 4 kernels */
+    int32_t size[4];
+    const float* kernel_dev[4];                    /* size*size*8 each, as vira_gpu_psf_kernel writes */
+    float edge_max[4][VIRA_GPU_NUM_BINS];
+    float minimum_power;                           /* W */
+    float intrinsic[6];                            /* K: fx kyx | kxy fy | px py */
+    int32_t cull_behind;                           /* 0: as the reference, 1: skip camera-frame z <= 0 */
+} vira_gpu_star_psf;
+
 /* --- context ------------------------------------------------------------ */
 vira_gpu_status vira_gpu_create(vira_gpu_ctx** ctx, const vira_gpu_opts* opts);
 vira_gpu_status vira_gpu_destroy(vira_gpu_ctx* ctx);
@@ -364,6 +380,22 @@ vira_gpu_status vira_gpu_convolve(vira_gpu_ctx* ctx, const float* image_dev, int
  * (W*H*8, optional) and the normalised digital image (W*H, 0..1). */
 vira_gpu_status vira_gpu_simulate_sensor(vira_gpu_ctx* ctx, const vira_gpu_sensor_sim* cfg, const float* power_dev,
                                          int width, int height, float* photons_dev, float* image_dev);
+
+/* CPUUnresolvedRenderer (cpu_unresolved_renderer.ipp:25-200): n point sources
+ * with global unit directions (n*3) and spectral irradiance (n*8, W/m^2 per
+ * band) are turned to the camera frame (R^T d), projected with K, given the
+ * received power eta * A * E (camera.ipp:1187-1191) and splatted with the
+ * bilinear-sampled PSF response (Image::addImage, image.ipp:1074-1150) into
+ * power_dev (W*H*8, accumulated: clear it first).  With no PSF (count 0)
+ * a source's power replaces the value of pixel ((int) x, (int) y), the
+ * last source in order winning (processRegion, :191-196).  Each pixel receives its
+ * sources' responses in source order, as the reference's sequential loop
+ * adds them, so the result is deterministic.  The reference projects every
+ * source, so one behind the camera lands where its antipode would
+ * (cull_behind = 0); cull_behind = 1 skips camera-frame z <= 0. */
+vira_gpu_status vira_gpu_render_stars(vira_gpu_ctx* ctx, const vira_gpu_camera* cam, const vira_gpu_sensor* sensor,
+                                      const vira_gpu_star_psf* psf, const float* directions_dev,
+                                      const float* irradiance_dev, size_t n, float* power_dev);
 
 /* --- batched rendering (new) --------------------------------------------- */
 /* Renders n requests (CPUPathTracer::render semantics per request,

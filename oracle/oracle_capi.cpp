@@ -266,6 +266,20 @@ void orc_pixel_solid_angles(const float* kinv, float zdir, int W, int H, float* 
 }
 
 void orc_psf_kernel(const PSFO* p, int K, float* out) { psf_kernel(*p, K, out); }
+void orc_render_stars(const float* rot, int W, int H, const float* eta, float area, int count, const int* sizes,
+                      const float* const* kernels, float minimum_power, const float* intrinsic, int cull_behind,
+                      const float* dirs, const float* irr, size_t n, float* img) {
+    StarPSFO p{};
+    p.count = count;
+    for (int i = 0; i < count; ++i) {
+        p.size[i] = sizes[i];
+        p.kernel[i] = kernels[i];
+    }
+    p.minimum_power = minimum_power;
+    for (int i = 0; i < 6; ++i) p.intrinsic[i] = intrinsic[i];
+    p.cull_behind = cull_behind;
+    render_stars(rot, W, H, eta, area, p, dirs, irr, n, img);
+}
 void orc_convolve(const float* in, int W, int H, const float* kern, int K, float* out) {
     convolve_spatial(in, W, H, kern, K, out);
 }

@@ -711,6 +711,123 @@ void convolve_spatial(const float* in, int W, int H, const float* kern, int K, f
         }
 }
 
+static float magnitude_f(const float* v) {
+    float q = 0.0f;
+    for (int k = 0; k < NB; ++k) q += v[k] * v[k];
+    return std::sqrt(q);
+}
+
+void render_stars(const float rot[9], int W, int H, const float eta[NB], float area, const StarPSFO& psf,
+                  const float* dirs, const float* irr, size_t n, float* img) {
+    // initKernels: the border pixel of largest magnitude, kernel(0, x),
+    // kernel(size-1, x), then kernel(y, 0), kernel(y, size-1); std::max keeps
+    // the first of equal magnitudes.
+    float edge[4][NB];
+    for (int i = 0; i < psf.count; ++i) {
+        const int K = psf.size[i];
+        const float* kern = psf.kernel[i];
+        auto at = [&](int a, int b) { return kern + ((size_t)a + (size_t)b * K) * NB; };
+        float best[NB] = {0};
+        auto consider = [&](const float* v) {
+            if (magnitude_f(best) < magnitude_f(v))
+                for (int k = 0; k < NB; ++k) best[k] = v[k];
+        };
+        for (int x = 0; x < K; ++x) {
+            consider(at(0, x));
+            consider(at(K - 1, x));
+        }
+        for (int y = 1; y < K - 1; ++y) {
+            consider(at(y, 0));
+            consider(at(y, K - 1));
+        }
+        for (int k = 0; k < NB; ++k) edge[i][k] = best[k];
+    }
+    // camera-frame vectors (cpu_unresolved_renderer.ipp:81-126)
+    std::vector<float> v(3 * n);
+    for (size_t s = 0; s < n; ++s) {
+        const float* d = dirs + 3 * s;
+        v[3 * s] = (rot[0] * d[0] + rot[3] * d[1]) + rot[6] * d[2];
+        v[3 * s + 1] = (rot[1] * d[0] + rot[4] * d[1]) + rot[7] * d[2];
+        v[3 * s + 2] = (rot[2] * d[0] + rot[5] * d[1]) + rot[8] * d[2];
+    }
+    const float* Km = psf.intrinsic;
+    const int tile = 512;
+    for (int ty = 0; ty < (H + tile - 1) / tile; ++ty)
+        for (int tx = 0; tx < (W + tile - 1) / tile; ++tx) {
+            const int rx0 = tx * tile, ry0 = ty * tile;
+            const int rx1 = std::min(W, (tx + 1) * tile), ry1 = std::min(H, (ty + 1) * tile);
+            for (size_t s = 0; s < n; ++s) {
+                const float* c = &v[3 * s];
+                if (psf.cull_behind && !(c[2] > 0.0f)) continue;
+                const float hx = c[0] / c[2], hy = c[1] / c[2];
+                const float X = (Km[0] * hx + Km[2] * hy) + Km[4];
+                const float Y = (Km[1] * hx + Km[3] * hy) + Km[5];
+                const float radius = 41;
+                if (!((X + radius) >= (float)rx0 && (X - radius) < (float)rx1 && (Y + radius) >= (float)ry0 &&
+                      (Y - radius) < (float)ry1))
+                    continue;
+                float P[NB];
+                for (int k = 0; k < NB; ++k) P[k] = (eta[k] * area) * irr[NB * s + k];
+                if (!(magnitude_f(P) > psf.minimum_power)) continue;
+                if (psf.count == 0) {  // no PSF: processRegion assigns the power to one pixel
+                    const int px = (int)X, py = (int)Y;
+                    if (px >= rx0 && px < rx1 && py >= ry0 && py < ry1)
+                        for (int k = 0; k < NB; ++k) img[((size_t)py * W + px) * NB + k] = P[k];
+                    continue;
+                }
+                // getKernel (psf.ipp:89-110)
+                int sel = psf.count - 1;
+                if (psf.minimum_power != 0.0f) {
+                    for (int i = 0; i < psf.count; ++i) {
+                        sel = i;
+                        float e[NB];
+                        for (int k = 0; k < NB; ++k) e[k] = edge[i][k] * P[k];
+                        if (magnitude_f(e) < psf.minimum_power) break;
+                    }
+                }
+                const int K = psf.size[sel];
+                // getResponse: received_power * kernel
+                std::vector<float> resp((size_t)K * K * NB);
+                for (size_t q = 0; q < (size_t)K * K; ++q)
+                    for (int k = 0; k < NB; ++k) resp[q * NB + k] = P[k] * psf.kernel[sel][q * NB + k];
+                auto R = [&](int a, int b, int k) { return resp[((size_t)a + (size_t)b * K) * NB + k]; };
+                // addImage within the ROI
+                const int ix = (int)std::floor(X), iy = (int)std::floor(Y);
+                const float rxf = (float)K / 2.0f, ryf = (float)K / 2.0f;
+                const int sx = std::max(rx0, ix - (int)rxf), sy = std::max(ry0, iy - (int)ryf);
+                const int ex = std::min(rx1, ix + (int)rxf + 1), ey = std::min(ry1, iy + (int)ryf + 1);
+                for (int i = sx; i < ex; ++i)
+                    for (int j = sy; j < ey; ++j) {
+                        const float kx = (float)i - (X - rxf), ky = (float)j - (Y - ryf);
+                        const int kx0 = (int)std::floor(kx), ky0 = (int)std::floor(ky);
+                        const int kx1 = kx0 + 1, ky1 = ky0 + 1;
+                        const float wx1 = kx - (float)kx0, wy1 = ky - (float)ky0;
+                        const float wx0 = 1.0f - wx1, wy0 = 1.0f - wy1;
+                        if (!(kx0 >= 0 && kx0 < K && ky0 >= 0 && ky0 < K)) continue;
+                        float* dst = img + ((size_t)j * W + i) * NB;
+                        for (int k = 0; k < NB; ++k) {
+                            float val = 0.0f;
+                            if (kx1 < K && ky1 < K) {
+                                val += R(kx0, ky0, k) * wx0 * wy0;
+                                val += R(kx1, ky0, k) * wx1 * wy0;
+                                val += R(kx0, ky1, k) * wx0 * wy1;
+                                val += R(kx1, ky1, k) * wx1 * wy1;
+                            } else if (kx1 < K) {
+                                val += R(kx0, ky0, k) * wx0;
+                                val += R(kx1, ky0, k) * wx1;
+                            } else if (ky1 < K) {
+                                val += R(kx0, ky0, k) * wy0;
+                                val += R(kx0, ky1, k) * wy1;
+                            } else {
+                                val = R(kx0, ky0, k);
+                            }
+                            dst[k] += val;
+                        }
+                    }
+            }
+        }
+}
+
 static InstanceO identity_instance(const Mesh& m, const BVH& b, const Material& mat) {
     InstanceO I{&m, &b, mat, {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1}};
     return I;

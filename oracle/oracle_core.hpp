@@ -580,6 +580,25 @@ void psf_kernel(const PSFO& p, int K, float* out);  // out[(x + y*K)*NB + k]
 // Image::convolveSpatial (image.ipp:1267-1330), FP32, zero padding, ky outer / kx inner.
 void convolve_spatial(const float* in, int W, int H, const float* kern, int K, float* out);
 
+// Unresolved point sources (CPUUnresolvedRenderer::render / findPointsInRegion /
+// processRegion, cpu_unresolved_renderer.ipp:25-200): camera-frame direction
+// R^T d (rot row-major), projection K (x/z, y/z, 1) (camera.ipp:1062-1081),
+// power (eta*A)*E (camera.ipp:1187-1191), 512-pixel ROIs each taking the
+// sources within 41 px in source order, kernel choice by the edge maxima
+// that PointSpreadFunction::initKernels finds (psf.ipp:134-162, computed here
+// from the kernels), and Image::addImage (image.ipp:1074-1150) into img
+// (W*H*NB, accumulated).  cull_behind = 1 skips camera-frame z <= 0.
+struct StarPSFO {
+    int count;
+    int size[4];
+    const float* kernel[4];  // [(x + y*size)*NB + k]
+    float minimum_power;
+    float intrinsic[6];      // GLM mat3x2 [col][row]
+    int cull_behind;
+};
+void render_stars(const float rot[9], int W, int H, const float eta[NB], float area, const StarPSFO& psf,
+                  const float* dirs, const float* irr, size_t n, float* img);
+
 // Two-level scene.  One instance with the identity matrix renders in the
 // caller's frame; anything else renders camera-relative: model matrices and
 // light positions minus the FP64 camera position, rounded to the working

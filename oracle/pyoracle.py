@@ -90,6 +90,8 @@ def lib() -> C.CDLL:
         "orc_pixel_solid_angles": (None, [P, C.c_float, C.c_int, C.c_int, P]),
         "orc_psf_kernel": (None, [C.POINTER(PSF), C.c_int, P]),
         "orc_convolve": (None, [P, C.c_int, C.c_int, P, C.c_int, P]),
+        "orc_render_stars": (None, [P, C.c_int, C.c_int, P, C.c_float, C.c_int, P, P, C.c_float, P, C.c_int,
+                                    P, P, C.c_size_t, P]),
         "orc_dem_diff_create": (P, [C.c_int, C.c_int, C.POINTER(Projection), P, P]),
         "orc_dem_diff_destroy": (None, [P]),
         "orc_di_forward64": (None, [P, P, P, P, C.c_size_t, C.c_int, C.POINTER(Light), C.c_int,
@@ -347,6 +349,26 @@ def convolve(img: np.ndarray, kern: np.ndarray) -> np.ndarray:
     H, W = img.shape[:2]
     out = np.empty_like(img)
     lib().orc_convolve(img.ctypes.data, W, H, kern.ctypes.data, kern.shape[0], out.ctypes.data)
+    return out
+
+
+def render_stars(rot, width: int, height: int, eta, area: float, kernels, minimum_power: float, intrinsic,
+                 directions, irradiance, cull_behind: bool = False, img=None) -> np.ndarray:
+    """Unresolved point sources: rot (3x3, columns = camera axes), kernels a list of
+    (K, K, NB) arrays indexed [y][x] in ascending size, intrinsic the 6 entries of K
+    ([col][row]), directions (n, 3), irradiance (n, NB).  Returns (H, W, NB)."""
+    rot = np.ascontiguousarray(np.asarray(rot, np.float32).reshape(3, 3))
+    eta = np.ascontiguousarray(np.broadcast_to(np.asarray(eta, np.float32), (NB,)))
+    ks = [np.ascontiguousarray(k, np.float32) for k in kernels]
+    sizes = np.array([k.shape[0] for k in ks], np.int32)
+    ptrs = (C.c_void_p * len(ks))(*[k.ctypes.data for k in ks])
+    intr = np.ascontiguousarray(intrinsic, np.float32)
+    d = np.ascontiguousarray(directions, np.float32)
+    e = np.ascontiguousarray(irradiance, np.float32)
+    out = np.zeros((height, width, NB), np.float32) if img is None else np.ascontiguousarray(img, np.float32).copy()
+    lib().orc_render_stars(rot.ctypes.data, width, height, eta.ctypes.data, float(area), len(ks),
+                           sizes.ctypes.data, C.cast(ptrs, C.c_void_p), float(minimum_power), intr.ctypes.data,
+                           int(bool(cull_behind)), d.ctypes.data, e.ctypes.data, d.shape[0], out.ctypes.data)
     return out
 
 

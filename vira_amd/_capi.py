@@ -209,6 +209,12 @@ class SensorSim(C.Structure):
     ]
 
 
+class StarPSF(C.Structure):
+    _fields_ = [("count", C.c_int32), ("size", C.c_int32 * 4), ("kernel_dev", C.c_void_p * 4),
+                ("edge_max", (C.c_float * NUM_BINS) * 4), ("minimum_power", C.c_float),
+                ("intrinsic", C.c_float * 6), ("cull_behind", C.c_int32)]
+
+
 class Instance(C.Structure):
     _fields_ = [("mesh_id", C.c_int32), ("transform", C.c_double * 16)]
 
@@ -244,6 +250,8 @@ SIGNATURES = [
     ("vira_gpu_psf_kernel", _S, [_P, C.POINTER(PSF), C.c_int, _P]),
     ("vira_gpu_convolve", _S, [_P, _P, C.c_int, C.c_int, _P, C.c_int, _P]),
     ("vira_gpu_simulate_sensor", _S, [_P, C.POINTER(SensorSim), _P, C.c_int, C.c_int, _P, _P]),
+    ("vira_gpu_render_stars", _S, [_P, C.POINTER(Camera), C.POINTER(Sensor), C.POINTER(StarPSF), _P, _P,
+                                   C.c_size_t, _P]),
     ("vira_gpu_batch_submit", _S, [_P, C.POINTER(Request), C.c_size_t, C.POINTER(BatchConfig),
                                    C.POINTER(BatchStats)]),
     ("vira_gpu_batch_autotune", _S, [_P, C.POINTER(Request), C.c_size_t, C.c_int, C.POINTER(BatchConfig),

@@ -431,4 +431,15 @@ vira_gpu_status vira_gpu_simulate_sensor(vira_gpu_ctx* ctx, const vira_gpu_senso
     });
 }
 
+vira_gpu_status vira_gpu_render_stars(vira_gpu_ctx* ctx, const vira_gpu_camera* cam, const vira_gpu_sensor* sensor,
+                                      const vira_gpu_star_psf* psf, const float* directions_dev,
+                                      const float* irradiance_dev, size_t n, float* power_dev) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        set_device(c);
+        VG_CHECK(cam && sensor && psf, VIRA_GPU_ERR_INVALID, "render_stars: null argument");
+        vg::render_stars(c, *cam, *sensor, *psf, directions_dev, irradiance_dev, n, power_dev);
+    });
+}
+
 }  // extern "C"

@@ -119,6 +119,8 @@ struct Ctx {
     std::vector<uint64_t> b_key;
     bool b_tuned = false;
     vira_gpu_batch_config b_cfg{4, 1, 16, 1, 0};
+    // unresolved sources: per-source records, per-tile counts/offsets/fill, tile lists
+    DBuf st_rec, st_count, st_start, st_fill, st_list;
     ~Ctx() {
         if (b_exec) (void)hipGraphExecDestroy(b_exec);
     }
@@ -147,6 +149,8 @@ void di_adjoint(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_di_opts& o, c
                 float* grad_heights, float* grad_texture, vira_gpu_di_stats* st);
 void received_power(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_sensor& s, const float* radiance,
                     float* power, float* omega);
+void render_stars(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_sensor& sen, const vira_gpu_star_psf& psf,
+                  const float* dirs, const float* irr, size_t n, float* img);
 void psf_kernel(Ctx& c, const vira_gpu_psf& P, int K, float* out);
 void convolve(Ctx& c, const float* in, int W, int H, const float* kern, int K, float* out);
 void simulate_sensor(Ctx& c, const vira_gpu_sensor_sim& cfg, const float* power, int W, int H, float* photons,

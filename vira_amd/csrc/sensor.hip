@@ -7,6 +7,7 @@
 // two spherical triangles' areas by Girard's theorem (camera.ipp:1578-1592),
 // rounded to FP32.
 #include "context.hpp"
+#include "scene_params.hpp"
 #include "shading.hpp"
 
 namespace vg {
@@ -248,7 +249,281 @@ __global__ void k_simulate_sensor(vira_gpu_sensor_sim c, const float* __restrict
     if (image) image[p] = __fdiv_rn(amp, c.max_adu);
 }
 
+// ------------------------------------------------------------ unresolved point sources
+// Deterministic splat: the reference adds each source's response to the image
+// in source order (cpu_unresolved_renderer.ipp:176-199), so a pixel's value is
+// ((0 + v_s1) + v_s2) + ... over its sources s1 < s2 < ...  Here the sources
+// are projected once (k_star_project), binned to 32x32-pixel tiles
+// (k_star_bin), and each tile sorts its list by source index in LDS and sums
+// per pixel in that order (k_star_splat): the same additions in the same order.
+constexpr int kStarTile = 32;
+constexpr int kStarSplatBS = 256;
+constexpr int kStarListCap = 4096;  // sources per tile sorted in LDS; busier tiles scan every source
+constexpr int kStarAssign = 4;      // StarRec::sel of a source assigned to one pixel (camera without PSF)
+
+struct StarRec {
+    float X, Y;          // projected position (pixels)
+    int32_t sel;         // chosen kernel, kStarAssign without PSF, -1 if the source adds nothing
+    int32_t x0, y0, x1, y1;  // footprint clipped to the image, [x0, x1) x [y0, y1)
+    int32_t pad;
+    float P[NB];         // received power
+};
+static_assert(sizeof(StarRec) == 64, "StarRec is one 64-byte record");
+
+__device__ __forceinline__ float magnitude8(const float v[NB]) {
+    float q = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) q = q + v[k] * v[k];
+    return __fsqrt_rn(q);
+}
+
+// One thread per source: camera frame, projection, received power, kernel
+// choice and footprint; counts the 32x32 tiles the footprint touches.
+__global__ void __launch_bounds__(256) k_star_project(CamF cam, vira_gpu_sensor sen, vira_gpu_star_psf psf,
+                                                      const float* __restrict__ dirs, const float* __restrict__ irr,
+                                                      uint32_t n, int tiles_x, StarRec* __restrict__ recs,
+                                                      uint32_t* __restrict__ tile_count) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    StarRec r;
+    r.X = r.Y = 0.0f;
+    r.sel = -1;
+    r.x0 = r.y0 = r.x1 = r.y1 = 0;
+    r.pad = 0;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) r.P[k] = 0.0f;
+    const float dx = dirs[3 * (size_t)s], dy = dirs[3 * (size_t)s + 1], dz = dirs[3 * (size_t)s + 2];
+    // camera_view_normal * direction (cpu_unresolved_renderer.ipp:104): R^T d
+    const float cx = (cam.rot[0] * dx + cam.rot[3] * dy) + cam.rot[6] * dz;
+    const float cy = (cam.rot[1] * dx + cam.rot[4] * dy) + cam.rot[7] * dz;
+    const float cz = (cam.rot[2] * dx + cam.rot[5] * dy) + cam.rot[8] * dz;
+    bool live = !(psf.cull_behind && !(cz > 0.0f));
+    // Camera::projectCameraPoint (camera.ipp:1062-1081): K * (x/z, y/z, 1)
+    const float hx = __fdiv_rn(cx, cz), hy = __fdiv_rn(cy, cz);
+    const float X = (psf.intrinsic[0] * hx + psf.intrinsic[2] * hy) + psf.intrinsic[4];
+    const float Y = (psf.intrinsic[1] * hx + psf.intrinsic[3] * hy) + psf.intrinsic[5];
+    const float radius = 41.0f;  // findPointsInRegion (cpu_unresolved_renderer.ipp:165)
+    live = live && (X + radius) >= 0.0f && (X - radius) < (float)cam.W && (Y + radius) >= 0.0f &&
+           (Y - radius) < (float)cam.H;
+    if (live) {
+        float P[NB];
+#pragma unroll
+        for (int k = 0; k < NB; ++k) P[k] = (sen.optical_efficiency[k] * sen.aperture_area) * irr[(size_t)NB * s + k];
+        live = magnitude8(P) > psf.minimum_power;
+        if (live) {
+            // no PSF: the source's power is assigned to the pixel (int) X, (int) Y
+            // (processRegion, cpu_unresolved_renderer.ipp:191-196); else the first
+            // kernel whose edge response is below the minimum (PointSpreadFunction::
+            // getKernel, psf.ipp:89-110) and Image::addImage's footprint
+            // (image.ipp:1082-1098)
+            int sel = kStarAssign, x0, y0, x1, y1;
+            if (psf.count == 0) {
+                x0 = (int)X;
+                y0 = (int)Y;
+                x1 = x0 + 1;
+                y1 = y0 + 1;
+                live = x0 >= 0 && x0 < cam.W && y0 >= 0 && y0 < cam.H;
+            } else {
+                sel = 0;
+                for (int i = 0; i < psf.count; ++i) {
+                    sel = i;
+                    float e[NB];
+#pragma unroll
+                    for (int k = 0; k < NB; ++k) e[k] = psf.edge_max[i][k] * P[k];
+                    if (magnitude8(e) < psf.minimum_power) break;
+                }
+                const int K = psf.size[sel];
+                const int ix = (int)floorf(X), iy = (int)floorf(Y);
+                const int h = (int)((float)K / 2.0f);
+                x0 = max(0, ix - h);
+                y0 = max(0, iy - h);
+                x1 = min(cam.W, ix + h + 1);
+                y1 = min(cam.H, iy + h + 1);
+                live = x0 < x1 && y0 < y1;
+            }
+            if (live) {
+                r.X = X;
+                r.Y = Y;
+                r.sel = sel;
+                r.x0 = x0;
+                r.y0 = y0;
+                r.x1 = x1;
+                r.y1 = y1;
+#pragma unroll
+                for (int k = 0; k < NB; ++k) r.P[k] = P[k];
+                for (int ty = y0 / kStarTile; ty <= (y1 - 1) / kStarTile; ++ty)
+                    for (int tx = x0 / kStarTile; tx <= (x1 - 1) / kStarTile; ++tx)
+                        atomicAdd(tile_count + (size_t)ty * tiles_x + tx, 1u);
+            }
+        }
+    }
+    recs[s] = r;
+}
+
+// One thread per source: its index into the lists of the tiles it touches
+// (order within a tile is arbitrary here and fixed by the splat's sort).
+__global__ void __launch_bounds__(256) k_star_bin(const StarRec* __restrict__ recs, uint32_t n, int tiles_x,
+                                                  const uint32_t* __restrict__ tile_start,
+                                                  uint32_t* __restrict__ tile_fill, uint32_t* __restrict__ list) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const StarRec& r = recs[s];
+    if (r.sel < 0) return;
+    for (int ty = r.y0 / kStarTile; ty <= (r.y1 - 1) / kStarTile; ++ty)
+        for (int tx = r.x0 / kStarTile; tx <= (r.x1 - 1) / kStarTile; ++tx) {
+            const size_t t = (size_t)ty * tiles_x + tx;
+            const uint32_t slot = atomicAdd(tile_fill + t, 1u);
+            list[tile_start[t] + slot] = s;
+        }
+}
+
+// The bilinear-sampled response of one source at pixel (i, j), added to acc
+// (Image::addImage, image.ipp:1099-1150, with received_power * kernel formed
+// per kernel pixel as PointSpreadFunction::getResponse does).
+__device__ __forceinline__ void star_add(const StarRec& r, const vira_gpu_star_psf& psf, int i, int j,
+                                         float acc[NB]) {
+    const int K = psf.size[r.sel];
+    const float* __restrict__ kern = psf.kernel_dev[r.sel];
+    const float rx = (float)K / 2.0f, ry = (float)K / 2.0f;
+    const float kx = (float)i - (r.X - rx), ky = (float)j - (r.Y - ry);
+    const int kx0 = (int)floorf(kx), ky0 = (int)floorf(ky);
+    if (!(kx0 >= 0 && kx0 < K && ky0 >= 0 && ky0 < K)) return;
+    const int kx1 = kx0 + 1, ky1 = ky0 + 1;
+    const float wx1 = kx - (float)kx0, wy1 = ky - (float)ky0;
+    const float wx0 = 1.0f - wx1, wy0 = 1.0f - wy1;
+    const float* k00 = kern + ((size_t)kx0 + (size_t)ky0 * K) * NB;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        const float P = r.P[k];
+        float v = 0.0f;
+        if (kx1 < K && ky1 < K) {
+            v = v + ((P * k00[k]) * wx0) * wy0;
+            v = v + ((P * k00[NB + k]) * wx1) * wy0;
+            v = v + ((P * k00[(size_t)K * NB + k]) * wx0) * wy1;
+            v = v + ((P * k00[(size_t)(K + 1) * NB + k]) * wx1) * wy1;
+        } else if (kx1 < K) {
+            v = v + (P * k00[k]) * wx0;
+            v = v + (P * k00[NB + k]) * wx1;
+        } else if (ky1 < K) {
+            v = v + (P * k00[k]) * wy0;
+            v = v + (P * k00[(size_t)K * NB + k]) * wy1;
+        } else {
+            v = P * k00[k];
+        }
+        acc[k] = acc[k] + v;
+    }
+}
+
+__device__ __forceinline__ bool star_covers(const StarRec& r, int i, int j) {
+    return r.sel >= 0 && i >= r.x0 && i < r.x1 && j >= r.y0 && j < r.y1;
+}
+
+// One block per 32x32 tile: sort the tile's sources by index (bitonic, in
+// LDS), then each pixel adds its sources' responses in that order.
+__global__ void __launch_bounds__(kStarSplatBS) k_star_splat(vira_gpu_star_psf psf, const StarRec* __restrict__ recs,
+                                                             uint32_t n, int W, int H, int tiles_x,
+                                                             const uint32_t* __restrict__ tile_count,
+                                                             const uint32_t* __restrict__ tile_start,
+                                                             const uint32_t* __restrict__ list,
+                                                             float* __restrict__ img) {
+    __shared__ uint32_t keys[kStarListCap];
+    const uint32_t t = blockIdx.x;
+    const uint32_t cnt = tile_count[t];
+    if (cnt == 0) return;
+    const int tx = (int)(t % (uint32_t)tiles_x), ty = (int)(t / (uint32_t)tiles_x);
+    const bool sorted = cnt <= (uint32_t)kStarListCap;
+    if (sorted) {
+        uint32_t m = 1;
+        while (m < cnt) m <<= 1;
+        for (uint32_t q = threadIdx.x; q < m; q += kStarSplatBS)
+            keys[q] = q < cnt ? list[tile_start[t] + q] : 0xffffffffu;
+        __syncthreads();
+        for (uint32_t size = 2; size <= m; size <<= 1)
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                for (uint32_t q = threadIdx.x; q < m; q += kStarSplatBS) {
+                    const uint32_t o = q ^ stride;
+                    if (o > q) {
+                        const uint32_t a = keys[q], b = keys[o];
+                        const bool up = (q & size) == 0;
+                        if ((a > b) == up) {
+                            keys[q] = b;
+                            keys[o] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+    }
+    for (int lp = threadIdx.x; lp < kStarTile * kStarTile; lp += kStarSplatBS) {
+        const int i = tx * kStarTile + (lp & (kStarTile - 1)), j = ty * kStarTile + lp / kStarTile;
+        if (i >= W || j >= H) continue;
+        float acc[NB];
+        bool touched = false;
+        float* dst = img + ((size_t)j * W + i) * NB;
+        const uint32_t m = sorted ? cnt : n;
+        for (uint32_t e = 0; e < m; ++e) {
+            const uint32_t s = sorted ? keys[e] : e;
+            const StarRec& r = recs[s];
+            if (!star_covers(r, i, j)) continue;
+            if (!touched) {
+#pragma unroll
+                for (int k = 0; k < NB; ++k) acc[k] = dst[k];
+                touched = true;
+            }
+            if (r.sel == kStarAssign) {
+#pragma unroll
+                for (int k = 0; k < NB; ++k) acc[k] = r.P[k];
+            } else {
+                star_add(r, psf, i, j, acc);
+            }
+        }
+        if (touched) {
+#pragma unroll
+            for (int k = 0; k < NB; ++k) dst[k] = acc[k];
+        }
+    }
+}
+
 }  // namespace
+
+void render_stars(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_sensor& sen, const vira_gpu_star_psf& psf,
+                  const float* dirs, const float* irr, size_t n, float* img) {
+    VG_CHECK(cam.width > 0 && cam.height > 0, VIRA_GPU_ERR_INVALID, "render_stars: empty image");
+    VG_CHECK(img && (n == 0 || (dirs && irr)), VIRA_GPU_ERR_INVALID, "render_stars: null buffers");
+    VG_CHECK(psf.count >= 0 && psf.count <= 4, VIRA_GPU_ERR_INVALID, "render_stars: 0..4 PSF kernels");
+    for (int i = 0; i < psf.count; ++i)
+        VG_CHECK(psf.size[i] >= 1 && psf.size[i] % 2 == 1 && psf.kernel_dev[i], VIRA_GPU_ERR_INVALID,
+                 "render_stars: kernels need odd sizes and device data");
+    VG_CHECK(psf.cull_behind == 0 || psf.cull_behind == 1, VIRA_GPU_ERR_INVALID, "render_stars: cull_behind 0 or 1");
+    VG_CHECK(n < (1u << 30), VIRA_GPU_ERR_INVALID, "render_stars: too many sources");
+    if (n == 0) return;
+    const CamF cf = cam_params(cam);
+    const int tiles_x = (cam.width + kStarTile - 1) / kStarTile, tiles_y = (cam.height + kStarTile - 1) / kStarTile;
+    const size_t tiles = (size_t)tiles_x * tiles_y;
+    c.st_rec.alloc(n * sizeof(StarRec));
+    c.st_count.alloc(tiles * sizeof(uint32_t));
+    c.st_start.alloc(tiles * sizeof(uint32_t));
+    c.st_fill.alloc(tiles * sizeof(uint32_t));
+    auto* recs = (StarRec*)c.st_rec.p;
+    auto* cnt = (uint32_t*)c.st_count.p;
+    auto* start = (uint32_t*)c.st_start.p;
+    auto* fill = (uint32_t*)c.st_fill.p;
+    VG_HIP(hipMemsetAsync(cnt, 0, tiles * sizeof(uint32_t), c.stream));
+    VG_HIP(hipMemsetAsync(fill, 0, tiles * sizeof(uint32_t), c.stream));
+    k_star_project<<<blocks_for(n, 256), 256, 0, c.stream>>>(cf, sen, psf, dirs, irr, (uint32_t)n, tiles_x, recs,
+                                                             cnt);
+    VG_HIP(hipGetLastError());
+    const uint32_t total = scan_exclusive_u32(c, cnt, start, tiles);
+    if (total == 0) return;
+    c.st_list.alloc((size_t)total * sizeof(uint32_t));
+    auto* list = (uint32_t*)c.st_list.p;
+    k_star_bin<<<blocks_for(n, 256), 256, 0, c.stream>>>(recs, (uint32_t)n, tiles_x, start, fill, list);
+    VG_HIP(hipGetLastError());
+    k_star_splat<<<(unsigned)tiles, kStarSplatBS, 0, c.stream>>>(psf, recs, (uint32_t)n, cam.width, cam.height,
+                                                                 tiles_x, cnt, start, list, img);
+    VG_HIP(hipGetLastError());
+    VG_HIP(hipStreamSynchronize(c.stream));
+}
 
 void psf_kernel(Ctx& c, const vira_gpu_psf& P, int K, float* out) {
     VG_CHECK(K >= 3 && K % 2 == 1 && K <= 1023, VIRA_GPU_ERR_INVALID, "psf_kernel: size must be odd, 3..1023");

@@ -87,6 +87,7 @@ class Camera:
     focal_length: float = 0.05
     z_dir: float = 1.0
     f_stop: float = 2.8            # camera.hpp:235
+    exposure_time: float = 0.01    # camera.hpp:233
     optical_efficiency: float = 1.0
 
     def aperture_area(self) -> float:
@@ -482,6 +483,55 @@ class Context:
         self._call(self.lib.vira_gpu_simulate_sensor, self.h, C.byref(c), p.data_ptr(), W, H,
                    None if ph is None else ph.data_ptr(), img.data_ptr())
         return img, ph
+
+    def star_psf(self, psf, camera: "Camera", sizes=(3, 9, 27, 81), cull_behind: bool = False):
+        """PSF kernels of ascending sizes (PointSpreadFunction::initKernels default,
+        psf.ipp) with their edge maxima, the minimum detectable power and K, for
+        render_stars.  cull_behind=False projects sources behind the camera as
+        the reference does.  psf=None: a camera without PSF, each source's power
+        assigned to one pixel."""
+        from . import sensor as SN
+        sp = capi.StarPSF()
+        if psf is None:
+            sizes = ()
+        sp.count = len(sizes)
+        kernels = []
+        for i, K in enumerate(sizes):
+            k = self.psf_kernel(psf, K)
+            kernels.append(k)
+            sp.size[i] = K
+            sp.kernel_dev[i] = k.data_ptr()
+            em = SN.edge_max(k.cpu().numpy())
+            for b in range(NUM_BINS):
+                sp.edge_max[i][b] = float(em[b])
+        sp.minimum_power = SN.minimum_detectable_power(camera.aperture_area(), camera.exposure_time)
+        sp.cull_behind = int(bool(cull_behind))
+        K = SN.intrinsic_matrix(camera)
+        for i in range(6):
+            sp.intrinsic[i] = float(K[i])
+        sp._keep = kernels
+        return sp
+
+    def render_stars(self, camera: "Camera", star_psf, directions, irradiance, out=None):
+        """Unresolved point sources (unit directions (n, 3), irradiance (n, 8)) added into
+        an (H, W, 8) received-power image (zeros if out is None)."""
+        if out is None:
+            out = self.torch.zeros((camera.height, camera.width, NUM_BINS), dtype=self.torch.float32,
+                                   device=f"cuda:{self.device}")
+        d = directions.contiguous()
+        e = irradiance.contiguous()
+        n = d.shape[0]
+        for name, t, shape in (("directions", d, (n, 3)), ("irradiance", e, (n, NUM_BINS)),
+                               ("out", out, (camera.height, camera.width, NUM_BINS))):
+            if tuple(t.shape) != shape or t.dtype != self.torch.float32 or not t.is_cuda:
+                raise ValueError(f"render_stars: {name} must be a float32 device tensor of shape {shape}")
+        if not out.is_contiguous():
+            raise ValueError("render_stars: out must be contiguous")
+        cam = camera.to_c()
+        sen = camera.sensor()
+        self._call(self.lib.vira_gpu_render_stars, self.h, C.byref(cam), C.byref(sen), C.byref(star_psf),
+                   d.data_ptr(), e.data_ptr(), n, out.data_ptr())
+        return out
 
     # ---- batched rendering
     @staticmethod

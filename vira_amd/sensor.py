@@ -124,3 +124,63 @@ class SensorConfig:
         c.vertical_pattern_period = int(self.vertical_pattern_period)
         c.seed = int(self.seed)
         return c
+
+
+def intrinsic_matrix(camera) -> np.ndarray:
+    """K as Camera::initializeIntrinsicMatrix builds it (camera.ipp:1526-1557), FP32,
+    GLM mat3x2 [col][row]: (fx, kyx, kxy, fy, px, py)."""
+    f = np.float32
+    sw, sh = camera.sensor_size()
+    kx = f(camera.width) / f(sw)
+    ky_ = f(camera.height) / f(sh)
+    zd = f(camera.z_dir)
+    ky = zd * ky_
+    px = zd * (f(camera.width) / f(2))
+    py = zd * (f(camera.height) / f(2))
+    kxy = zd * f(0)
+    kyx = zd * f(0)
+    fx = f(camera.focal_length) * kx
+    fy = f(camera.focal_length) * ky
+    return np.array([fx, kyx, kxy, fy, px, py], np.float32)
+
+
+def edge_max(kernel: np.ndarray) -> np.ndarray:
+    """PointSpreadFunction::initKernels edge maximum (psf.ipp:143-159) of a kernel
+    indexed [y][x]: the border pixel of largest spectral magnitude, visited as
+    kernel(0, x), kernel(K-1, x), then kernel(y, 0), kernel(y, K-1) (Image(i, j)
+    is pixel x = i, y = j); std::max keeps the first of equal magnitudes."""
+    k = np.asarray(kernel, np.float32)
+    K = k.shape[0]
+
+    def mag(v):
+        q = np.float32(0)
+        for x in v:
+            q = np.float32(q + np.float32(x * x))
+        return np.float32(np.sqrt(q))
+
+    best = np.zeros(NUM_BINS, np.float32)
+    for x in range(K):
+        for v in (k[x, 0], k[x, K - 1]):
+            if mag(best) < mag(v):
+                best = v.copy()
+    for y in range(1, K - 1):
+        for v in (k[0, y], k[K - 1, y]):
+            if mag(best) < mag(v):
+                best = v.copy()
+    return best
+
+
+def minimum_detectable_power(aperture_area: float, exposure_time: float = 0.01,
+                             minimum_electrons: float = 10.0) -> float:
+    """The unresolved renderer's cut (cpu_unresolved_renderer.ipp:56-57):
+    Camera::computeMinimumDetectableIrradiance (camera.ipp:1201-1211), 10 electrons
+    x the photon energies integrated over the bands / (area x exposure), times the
+    exposure again, in FP32 as the reference."""
+    f = np.float32
+    lo, hi, _, E = band_table()
+    total = f(0)
+    for k in range(NUM_BINS):
+        total = f(total + f(f(hi[k] - lo[k]) * E[k]))
+    energy = f(f(minimum_electrons) * total)
+    irradiance = f(energy / f(f(aperture_area) * f(exposure_time)))
+    return float(f(irradiance * f(exposure_time)))

@@ -497,3 +497,67 @@ def build_instanced(ctx: S.Context, dem_built: "BuiltScene | None" = None, dem: 
     opts = S.PathTracerOptions(samples=spp, bounces=bounces, seed=4)
     local = {"sphere": (sp_pos, sp_idx), "body": (sb_pos, sb_idx), "craft": (sc_pos, sc_idx)}
     return InstancedScene(meshes, transforms, cam, lights, opts, [b.heights, b.texture], local)
+
+
+# ---------------------------------------------------------------- config 4: the star field
+def johnson_v_band(n: int = 1000):
+    """johnsonVBandApproximation (magnitudes.ipp): skew-normal V-band response on 450-725 nm (FP64)."""
+    from scipy.special import erfc
+    lam = np.linspace(450e-9, 725e-9, n)
+    sigma, mu, alpha, scale = 63e-9, 505e-9, 6e7, 5.4729e6
+    x = lam - mu
+    s2 = sigma * sigma
+    a = 1.0 / np.sqrt(2 * np.pi * s2)
+    b = np.exp(-(x * x) / (2 * s2))
+    cdf = 0.5 * erfc(-(alpha * x) * (1.0 / np.sqrt(2.0)))
+    return lam, a * b * cdf / scale
+
+
+def star_irradiance(vmag, bv: float = 0.3):
+    """Spectral irradiance (n, 8) of stars of V magnitude vmag and colour index bv
+    (Tycho2Interface::initIrradiance, tycho2_interface.ipp:112-150; Star::getIrradiance,
+    star.ipp): T from B-V, solid angle from the V-band photon flux, times the
+    black-body band radiance."""
+    c, h, kb = 299792458.0, 6.62607015e-34, 1.380649e-23
+    T = 4600.0 * (1.0 / (0.92 * bv + 1.7) + 1.0 / (0.92 * bv + 0.62))
+    # Band(550 nm, fwhm 0.16, 3640 Jy).fluxFromMagnitude (magnitudes.hpp)
+    freq = c / 550.0
+    jy2photon = (1e-23 * 10000 * freq) / ((h * freq) / 1e-7)
+    flux = np.power(10.0, np.asarray(vmag, np.float64) / -2.5) * 3640.0 * jy2photon * 0.16
+    lam, v = johnson_v_band()
+    planck = 2 * h * c * c / lam ** 5 / (np.exp(h * c / (lam * kb * T)) - 1.0)
+    counts = v * planck / (h * c / lam)
+    flux2 = np.sum(0.5 * (counts[1:] + counts[:-1]) * np.diff(lam))
+    omega = flux / flux2
+    band = S.planck_band_radiance(T, steps=100)
+    return (omega[:, None] * band[None, :]).astype(np.float32)
+
+
+def make_star_field(n: int = 100_000, seed: int = 4, mag_max: float = 10.0):
+    """n stars: V magnitudes from an exponential law on [0, mag_max] (density
+    proportional to e^m, faint stars most numerous), directions uniform on the
+    sky; numpy Generator(seed).  Returns (directions (n, 3), irradiance (n, 8),
+    magnitudes)."""
+    rng = np.random.default_rng(seed)
+    u = rng.random(n)
+    mag = np.log1p(u * np.expm1(mag_max))
+    z = rng.uniform(-1.0, 1.0, n)
+    phi = rng.uniform(0.0, 2.0 * np.pi, n)
+    r = np.sqrt(1.0 - z * z)
+    dirs = np.stack([r * np.cos(phi), r * np.sin(phi), z], 1).astype(np.float32)
+    return dirs, star_irradiance(mag), mag
+
+
+def config4_star_camera(image: int = 4096, direction=(0.6, -0.48, 0.64)):
+    """Config 4's camera (4096^2, 40 degree field, f = 50 mm, f/2.8) for its star
+    field, with the Airy-disk PSF of those optics (camera.ipp:90-97: focal
+    length, aperture diameter f/N, pixel size = sensor size / resolution)."""
+    from . import sensor as SN
+    cam = S.Camera(image, image, 40.0)
+    cam.look_in_direction(direction, up=(0.0, 0.0, 1.0))
+    sw, sh = cam.sensor_size()
+    f32 = np.float32
+    psf = SN.AiryDiskPSF(focal_length=cam.focal_length,
+                         aperture_diameter=float(f32(cam.focal_length) / f32(cam.f_stop)),
+                         pixel_size=(float(f32(sw) / f32(image)), float(f32(sh) / f32(image))))
+    return cam, psf
