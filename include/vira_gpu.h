@@ -406,9 +406,10 @@ vira_gpu_status vira_gpu_render_stars(vira_gpu_ctx* ctx, const vira_gpu_camera* 
 vira_gpu_status vira_gpu_batch_submit(vira_gpu_ctx* ctx, const vira_gpu_request* requests_host, size_t n,
                                       const vira_gpu_batch_config* cfg, vira_gpu_batch_stats* stats);
 /* Times candidate configurations (BVH width, waves per block, waves per CU,
- * tiles per grab) on every sample_stride-th tile of each request, keeps the
- * fastest for later submits and returns it (rebuilding the requests' BLASes
- * at its BVH width). */
+ * tiles per grab) on every sample_stride-th tile of each request (best of
+ * two runs), re-times the three fastest on all tiles when sample_stride > 1,
+ * keeps the fastest of those for later submits and returns it (rebuilding
+ * the requests' BLASes at its BVH width). */
 vira_gpu_status vira_gpu_batch_autotune(vira_gpu_ctx* ctx, const vira_gpu_request* requests_host, size_t n,
                                         int sample_stride, vira_gpu_batch_config* best, double* best_ms);
 
