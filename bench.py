@@ -276,10 +276,16 @@ def sensor_leg(ctx, built, passes, steps: int) -> dict:
     rp_ms, (power, _) = timed(lambda: ctx.received_power(cam, passes.total_radiance))
     cv_ms, blurred = timed(lambda: ctx.convolve(power, psf))
     ss_ms, _ = timed(lambda: ctx.simulate_sensor(cfg, blurred))
+    # EATWT denoiser (5 + 5 levels) on copies of the frame's passes
+    from vira_amd import scene as S
+    work = S.RenderPasses(passes.width, passes.height)
+    for name in ("total_radiance", "direct_radiance", "indirect_radiance", "albedo", "normal_global", "depth"):
+        getattr(work, name).copy_(getattr(passes, name))
+    dn_ms, _ = timed(lambda: ctx.denoise(work))
     print(f"# sensor tail (2048^2 x 8): received power {rp_ms:.2f} ms, 81x81 PSF convolution {cv_ms:.2f} ms, "
-          f"photosite + noise {ss_ms:.2f} ms", file=sys.stderr, flush=True)
+          f"photosite + noise {ss_ms:.2f} ms; EATWT denoiser {dn_ms:.2f} ms", file=sys.stderr, flush=True)
     return {"received_power_ms": round(rp_ms, 3), "psf81_convolution_ms": round(cv_ms, 3),
-            "sensor_ms": round(ss_ms, 3)}
+            "sensor_ms": round(ss_ms, 3), "denoise_eatwt_ms": round(dn_ms, 3)}
 
 
 def main() -> int:

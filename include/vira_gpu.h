@@ -255,6 +255,16 @@ typedef struct {
     int32_t cull_behind;                           /* 0: as the reference, 1: skip camera-frame z <= 0 */
 } vira_gpu_star_psf;
 
+/* EATWTOptions (cpu_denoise.hpp:9-18); TILE_SIZE has no counterpart. */
+typedef struct {
+    int32_t max_levels_direct;     /* 5 */
+    int32_t max_levels_indirect;   /* 5 */
+    float depth_threshold;         /* 0.2 */
+    float normal_threshold;        /* 0.5 */
+    float min_normal_weight;       /* 0.3 */
+    float epsilon;                 /* 1e-6 */
+} vira_gpu_denoise_opts;
+
 /* --- context ------------------------------------------------------------ */
 vira_gpu_status vira_gpu_create(vira_gpu_ctx** ctx, const vira_gpu_opts* opts);
 vira_gpu_status vira_gpu_destroy(vira_gpu_ctx* ctx);
@@ -396,6 +406,18 @@ vira_gpu_status vira_gpu_simulate_sensor(vira_gpu_ctx* ctx, const vira_gpu_senso
 vira_gpu_status vira_gpu_render_stars(vira_gpu_ctx* ctx, const vira_gpu_camera* cam, const vira_gpu_sensor* sensor,
                                       const vira_gpu_star_psf* psf, const float* directions_dev,
                                       const float* irradiance_dev, size_t n, float* power_dev);
+
+/* denoiseSpectralRadianceEATWT (cpu_denoise.ipp:10-187), as
+ * CPUPathTracer::render applies it when options.denoise is set
+ * (cpu_path_tracer.ipp:124-134): direct_dev and indirect_dev (W*H*8) are
+ * denoised in place, edge-stopped by depth_dev (W*H) and normal_dev (W*H*3,
+ * the global first-hit normal), demodulated by albedo_dev (W*H*8); total_dev
+ * (optional) receives direct + indirect.  The first pass of each level runs
+ * along the image's j axis, the second along i (the reference's order on
+ * square images).  Spectral images 16-byte aligned. */
+vira_gpu_status vira_gpu_denoise(vira_gpu_ctx* ctx, const vira_gpu_denoise_opts* opts, int width, int height,
+                                 float* direct_dev, float* indirect_dev, const float* albedo_dev,
+                                 const float* depth_dev, const float* normal_dev, float* total_dev);
 
 /* --- batched rendering (new) --------------------------------------------- */
 /* Renders n requests (CPUPathTracer::render semantics per request,

@@ -103,7 +103,11 @@ struct PathTracerOptions {
     int rrStart = 1;
     uint32_t seed = 0;
     int integrator = 0;  // 0 auto, 1 per-pixel, 2 wavefront
+    bool denoise = false;  // CPUPathTracerOptions::denoise (cpu_path_tracer.hpp:35)
 };
+
+// EATWTOptions (cpu_denoise.hpp:9-18).
+inline vira_gpu_denoise_opts defaultDenoiserOptions() { return vira_gpu_denoise_opts{5, 5, 0.2f, 0.5f, 0.3f, 1e-6f}; }
 
 // GPUPathTracer: CPUPathTracer's render(camera, scene) with the scene held by
 // the context; renderPasses are caller-owned device buffers (W*H*8 floats per
@@ -112,13 +116,20 @@ class PathTracer {
 public:
     explicit PathTracer(Context& ctx) : ctx_(&ctx) {}
     PathTracerOptions options;
+    vira_gpu_denoise_opts denoiserOptions = defaultDenoiserOptions();
     vira_gpu_outputs renderPasses{};
     vira_gpu_render_stats lastStats{};
 
+    // With options.denoise the direct and indirect passes are denoised after the
+    // render and the total pass becomes their sum (cpu_path_tracer.ipp:124-134).
     void render(const vira_gpu_camera& camera) {
         vira_gpu_render_opts o{options.samples, options.bounces, options.russianRoulette ? 1 : 0, options.rrStart,
                                options.seed, options.integrator};
         check(vira_gpu_render(ctx_->get(), &camera, &o, &renderPasses, &lastStats));
+        if (options.denoise)
+            check(vira_gpu_denoise(ctx_->get(), &denoiserOptions, camera.width, camera.height,
+                                   renderPasses.direct_dev, renderPasses.indirect_dev, renderPasses.albedo_dev,
+                                   renderPasses.depth_dev, renderPasses.normal_dev, renderPasses.radiance_dev));
     }
 
 private:

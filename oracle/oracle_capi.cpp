@@ -266,6 +266,10 @@ void orc_pixel_solid_angles(const float* kinv, float zdir, int W, int H, float* 
 }
 
 void orc_psf_kernel(const PSFO* p, int K, float* out) { psf_kernel(*p, K, out); }
+void orc_denoise(const DenoiseO* o, int W, int H, float* direct, float* indirect, const float* albedo,
+                 const float* depth, const float* normal, int literal) {
+    denoise_eatwt(*o, W, H, direct, indirect, albedo, depth, normal, literal != 0);
+}
 void orc_render_stars(const float* rot, int W, int H, const float* eta, float area, int count, const int* sizes,
                       const float* const* kernels, float minimum_power, const float* intrinsic, int cull_behind,
                       const float* dirs, const float* irr, size_t n, float* img) {

@@ -828,6 +828,76 @@ void render_stars(const float rot[9], int W, int H, const float eta[NB], float a
         }
 }
 
+// One level pass of the reference's filterPixelSeparable over a whole image.
+// pix(a, b) is the linear pixel of loop coordinates (a, b); the pass moves b
+// (first pass) or a (second pass), clamped to [0, nb-1] / [0, na-1].
+template <class Pix>
+static void eatwt_pass(const DenoiseO& o, int na, int nb, int step, bool move_b, Pix pix, const float* in,
+                       float* out, const float* depth, const float* normal) {
+    static const float K1[5] = {1.0f / 16.0f, 4.0f / 16.0f, 6.0f / 16.0f, 4.0f / 16.0f, 1.0f / 16.0f};
+    for (int a = 0; a < na; ++a)
+        for (int b = 0; b < nb; ++b) {
+            const size_t c = pix(a, b);
+            float sum[NB] = {0};
+            float ws = 0.0f;
+            const float cd = depth[c];
+            const float* cn = normal + 3 * c;
+            for (int k = -2; k <= 2; ++k) {
+                int sa = move_b ? a : a + k * step;
+                int sb = move_b ? b + k * step : b;
+                sa = std::clamp(sa, 0, na - 1);
+                sb = std::clamp(sb, 0, nb - 1);
+                const size_t s = pix(sa, sb);
+                const float sd = depth[s];
+                const float ratio = std::max(cd, sd) / (std::min(cd, sd) + o.eps);
+                const float dw = ratio < (1.0f + o.depth_threshold) ? 1.0f
+                                                                     : std::exp(-(ratio - 1.0f) / o.depth_threshold);
+                const float* sn = normal + 3 * s;
+                const float dot = cn[0] * sn[0] + cn[1] * sn[1] + cn[2] * sn[2];
+                const float sim = std::max(0.0f, dot);
+                const float nw = std::max(o.min_normal_weight,
+                                          std::exp(-std::pow(1.0f - sim, 2.0f) / o.normal_threshold));
+                const float w = K1[k + 2] * dw * nw;
+                for (int q = 0; q < NB; ++q) sum[q] += in[s * NB + q] * w;
+                ws += w;
+            }
+            if (ws > o.eps)
+                for (int q = 0; q < NB; ++q) out[c * NB + q] = sum[q] / ws;
+            else
+                for (int q = 0; q < NB; ++q) out[c * NB + q] = in[c * NB + q];
+        }
+}
+
+void denoise_eatwt(const DenoiseO& o, int W, int H, float* direct, float* indirect, const float* albedo,
+                   const float* depth, const float* normal, bool literal) {
+    const size_t n = (size_t)W * H * NB;
+    std::vector<float> illum[2] = {std::vector<float>(n), std::vector<float>(n)};
+    float* img[2] = {direct, indirect};
+    for (int m = 0; m < 2; ++m)
+        for (size_t q = 0; q < n; ++q) illum[m][q] = img[m][q] / std::max(albedo[q], o.eps);
+    const int levels[2] = {o.levels_direct, o.levels_indirect};
+    for (int m = 0; m < 2; ++m) {
+        std::vector<float> tmp(n);
+        float* cur = illum[m].data();
+        float* nxt = tmp.data();
+        for (int level = 0; level < levels[m]; ++level) {
+            const int step = 1 << level;
+            for (int half = 0; half < 2; ++half) {
+                if (literal)  // loops y over H, x over W; image(y, x) = data[y + x*W]; "horizontal" moves x first
+                    eatwt_pass(o, H, W, step, half == 0, [W](int y, int x) { return (size_t)y + (size_t)x * W; },
+                               cur, nxt, depth, normal);
+                else          // loops i over W, j over H; the first pass moves j, the second i
+                    eatwt_pass(o, W, H, step, half == 0, [W](int i, int j) { return (size_t)i + (size_t)j * W; },
+                               cur, nxt, depth, normal);
+                std::swap(cur, nxt);
+            }
+        }
+        if (cur != illum[m].data()) std::copy(cur, cur + n, illum[m].data());
+    }
+    for (int m = 0; m < 2; ++m)
+        for (size_t q = 0; q < n; ++q) img[m][q] = illum[m][q] * albedo[q];
+}
+
 static InstanceO identity_instance(const Mesh& m, const BVH& b, const Material& mat) {
     InstanceO I{&m, &b, mat, {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1}};
     return I;

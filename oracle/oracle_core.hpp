@@ -599,6 +599,19 @@ struct StarPSFO {
 void render_stars(const float rot[9], int W, int H, const float eta[NB], float area, const StarPSFO& psf,
                   const float* dirs, const float* irr, size_t n, float* img);
 
+// Edge-avoiding a-trous wavelet denoiser (denoiseSpectralRadianceEATWT,
+// cpu_denoise.ipp:10-187; options cpu_denoise.hpp:9-18).  literal = true
+// follows the reference's image(y, x) = data[y + x*W] indexing (square images
+// only); literal = false filters first along j (clamped to H-1), then along i
+// (clamped to W-1), which is the same thing for W == H and defined for any
+// W x H.  direct and indirect (W*H*NB) are denoised in place.
+struct DenoiseO {
+    int levels_direct, levels_indirect;
+    float depth_threshold, normal_threshold, min_normal_weight, eps;
+};
+void denoise_eatwt(const DenoiseO& o, int W, int H, float* direct, float* indirect, const float* albedo,
+                   const float* depth, const float* normal, bool literal);
+
 // Two-level scene.  One instance with the identity matrix renders in the
 // caller's frame; anything else renders camera-relative: model matrices and
 // light positions minus the FP64 camera position, rounded to the working

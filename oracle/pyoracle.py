@@ -48,6 +48,11 @@ class PSF(C.Structure):
                 ("pixel_size", C.c_float * 2), ("wavelength", C.c_float * NB), ("supersample", C.c_int)]
 
 
+class DenoiseOpts(C.Structure):
+    _fields_ = [("levels_direct", C.c_int), ("levels_indirect", C.c_int), ("depth_threshold", C.c_float),
+                ("normal_threshold", C.c_float), ("min_normal_weight", C.c_float), ("eps", C.c_float)]
+
+
 class Options(C.Structure):
     _fields_ = [("spp", C.c_int), ("bounces", C.c_int), ("rr", C.c_int), ("rr_start", C.c_int),
                 ("seed", C.c_uint32)]
@@ -90,6 +95,7 @@ def lib() -> C.CDLL:
         "orc_pixel_solid_angles": (None, [P, C.c_float, C.c_int, C.c_int, P]),
         "orc_psf_kernel": (None, [C.POINTER(PSF), C.c_int, P]),
         "orc_convolve": (None, [P, C.c_int, C.c_int, P, C.c_int, P]),
+        "orc_denoise": (None, [C.POINTER(DenoiseOpts), C.c_int, C.c_int, P, P, P, P, P, C.c_int]),
         "orc_render_stars": (None, [P, C.c_int, C.c_int, P, C.c_float, C.c_int, P, P, C.c_float, P, C.c_int,
                                     P, P, C.c_size_t, P]),
         "orc_dem_diff_create": (P, [C.c_int, C.c_int, C.POINTER(Projection), P, P]),
@@ -350,6 +356,24 @@ def convolve(img: np.ndarray, kern: np.ndarray) -> np.ndarray:
     out = np.empty_like(img)
     lib().orc_convolve(img.ctypes.data, W, H, kern.ctypes.data, kern.shape[0], out.ctypes.data)
     return out
+
+
+def denoise(direct, indirect, albedo, depth, normal, levels=(5, 5), depth_threshold=0.2, normal_threshold=0.5,
+            min_normal_weight=0.3, eps=1e-6, literal=False):
+    """EATWT denoiser on (H, W, NB) direct/indirect/albedo, (H, W) depth, (H, W, 3) normal.
+    Returns (direct, indirect) denoised (new arrays)."""
+    d = np.ascontiguousarray(direct, np.float32).copy()
+    i = np.ascontiguousarray(indirect, np.float32).copy()
+    a = np.ascontiguousarray(albedo, np.float32)
+    z = np.ascontiguousarray(depth, np.float32)
+    n = np.ascontiguousarray(normal, np.float32)
+    H, W = d.shape[:2]
+    o = DenoiseOpts(int(levels[0]), int(levels[1]), depth_threshold, normal_threshold, min_normal_weight, eps)
+    if literal and W != H:
+        raise ValueError("the reference's indexing is defined for square images only")
+    lib().orc_denoise(C.byref(o), W, H, d.ctypes.data, i.ctypes.data, a.ctypes.data, z.ctypes.data, n.ctypes.data,
+                      int(bool(literal)))
+    return d, i
 
 
 def render_stars(rot, width: int, height: int, eta, area: float, kernels, minimum_power: float, intrinsic,

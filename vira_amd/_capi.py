@@ -189,6 +189,12 @@ class PSF(C.Structure):
                 ("pixel_size", C.c_float * 2), ("wavelength", C.c_float * NUM_BINS), ("supersample", C.c_int)]
 
 
+class DenoiseOpts(C.Structure):
+    _fields_ = [("max_levels_direct", C.c_int32), ("max_levels_indirect", C.c_int32),
+                ("depth_threshold", C.c_float), ("normal_threshold", C.c_float),
+                ("min_normal_weight", C.c_float), ("epsilon", C.c_float)]
+
+
 class SensorSim(C.Structure):
     _fields_ = [
         ("exposure_time", C.c_float),
@@ -250,6 +256,7 @@ SIGNATURES = [
     ("vira_gpu_psf_kernel", _S, [_P, C.POINTER(PSF), C.c_int, _P]),
     ("vira_gpu_convolve", _S, [_P, _P, C.c_int, C.c_int, _P, C.c_int, _P]),
     ("vira_gpu_simulate_sensor", _S, [_P, C.POINTER(SensorSim), _P, C.c_int, C.c_int, _P, _P]),
+    ("vira_gpu_denoise", _S, [_P, C.POINTER(DenoiseOpts), C.c_int, C.c_int, _P, _P, _P, _P, _P, _P]),
     ("vira_gpu_render_stars", _S, [_P, C.POINTER(Camera), C.POINTER(Sensor), C.POINTER(StarPSF), _P, _P,
                                    C.c_size_t, _P]),
     ("vira_gpu_batch_submit", _S, [_P, C.POINTER(Request), C.c_size_t, C.POINTER(BatchConfig),

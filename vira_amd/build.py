@@ -37,7 +37,8 @@ FLAGS = [
     "-Wno-unused-but-set-variable",
     f"-I{ROOT / 'include'}",
 ]
-SOURCES = ["capi.hip", "dem.hip", "bvh.hip", "render.hip", "adjoint.hip", "batch.hip", "sensor.hip", "scan.hip"]
+SOURCES = ["capi.hip", "dem.hip", "bvh.hip", "render.hip", "adjoint.hip", "batch.hip", "sensor.hip", "denoise.hip",
+           "scan.hip"]
 # C++ programs over include/vira_gpu.hpp, linked against the library (host code only)
 EXAMPLES = ["render_dem.cpp"]
 

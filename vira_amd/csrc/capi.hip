@@ -442,4 +442,15 @@ vira_gpu_status vira_gpu_render_stars(vira_gpu_ctx* ctx, const vira_gpu_camera* 
     });
 }
 
+vira_gpu_status vira_gpu_denoise(vira_gpu_ctx* ctx, const vira_gpu_denoise_opts* opts, int width, int height,
+                                 float* direct_dev, float* indirect_dev, const float* albedo_dev,
+                                 const float* depth_dev, const float* normal_dev, float* total_dev) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        set_device(c);
+        VG_CHECK(opts, VIRA_GPU_ERR_INVALID, "denoise: null options");
+        vg::denoise(c, *opts, width, height, direct_dev, indirect_dev, albedo_dev, depth_dev, normal_dev, total_dev);
+    });
+}
+
 }  // extern "C"

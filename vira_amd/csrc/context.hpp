@@ -121,6 +121,8 @@ struct Ctx {
     vira_gpu_batch_config b_cfg{4, 1, 16, 1, 0};
     // unresolved sources: per-source records, per-tile counts/offsets/fill, tile lists
     DBuf st_rec, st_count, st_start, st_fill, st_list;
+    // denoiser work images: two per pass (direct, indirect) in ping-pong
+    DBuf dn_work[4];
     ~Ctx() {
         if (b_exec) (void)hipGraphExecDestroy(b_exec);
     }
@@ -152,6 +154,8 @@ void received_power(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_sensor& s
 void render_stars(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_sensor& sen, const vira_gpu_star_psf& psf,
                   const float* dirs, const float* irr, size_t n, float* img);
 void psf_kernel(Ctx& c, const vira_gpu_psf& P, int K, float* out);
+void denoise(Ctx& c, const vira_gpu_denoise_opts& o, int W, int H, float* direct, float* indirect,
+             const float* albedo, const float* depth, const float* normal, float* total);
 void convolve(Ctx& c, const float* in, int W, int H, const float* kern, int K, float* out);
 void simulate_sensor(Ctx& c, const vira_gpu_sensor_sim& cfg, const float* power, int W, int H, float* photons,
                      float* image);

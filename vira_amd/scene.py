@@ -198,14 +198,34 @@ def new_sun(position) -> SphereLight:
 
 # ---------------------------------------------------------------- options / passes
 @dataclass
+class DenoiseOptions:
+    """EATWTOptions (cpu_denoise.hpp:9-18)."""
+    max_levels_direct: int = 5
+    max_levels_indirect: int = 5
+    depth_threshold: float = 0.2
+    normal_threshold: float = 0.5
+    min_normal_weight: float = 0.3
+    epsilon: float = 1e-6
+
+    def to_c(self) -> capi.DenoiseOpts:
+        return capi.DenoiseOpts(int(self.max_levels_direct), int(self.max_levels_indirect),
+                                float(self.depth_threshold), float(self.normal_threshold),
+                                float(self.min_normal_weight), float(self.epsilon))
+
+
+@dataclass
 class PathTracerOptions:
-    """CPUPathTracerOptions (cpu_path_tracer.hpp:30-43) + the GPU additions."""
+    """CPUPathTracerOptions (cpu_path_tracer.hpp:30-43) + the GPU additions.
+    denoise / denoiser: CPUPathTracer::options.denoise and denoiserOptions
+    (cpu_path_tracer.hpp:35,57), applied after the render."""
     samples: int = 1
     bounces: int = 0
     russian_roulette: bool = False
     rr_start: int = 1
     seed: int = 0
     integrator: str = "auto"  # "auto" | "fused" | "wavefront"
+    denoise: bool = False
+    denoiser: DenoiseOptions = field(default_factory=DenoiseOptions)
 
     def to_c(self) -> capi.RenderOpts:
         o = capi.RenderOpts()
@@ -422,9 +442,27 @@ class Context:
         cam = camera.to_c()
         opt = options.to_c()
         out = passes.to_c()
+        if options.denoise and any(t is None for t in (passes.direct_radiance, passes.indirect_radiance,
+                                                        passes.albedo, passes.depth, passes.normal_global)):
+            raise ValueError("denoise needs the direct, indirect, albedo, depth and normal passes")
         self._call(self.lib.vira_gpu_render, self.h, C.byref(cam), C.byref(opt), C.byref(out), C.byref(st))
         self.last_stats = {f: getattr(st, f) for f, _ in capi.RenderStats._fields_}
+        if options.denoise:  # cpu_path_tracer.ipp:124-134
+            self.denoise(passes, options.denoiser)
         return self.last_stats
+
+    def denoise(self, passes: RenderPasses, options: "DenoiseOptions" = None):
+        """denoiseSpectralRadianceEATWT on the passes' direct and indirect radiance (in
+        place), then total = direct + indirect when the passes hold a total."""
+        o = (options or DenoiseOptions()).to_c()
+        need = (passes.direct_radiance, passes.indirect_radiance, passes.albedo, passes.depth, passes.normal_global)
+        if any(t is None or not t.is_contiguous() for t in need):
+            raise ValueError("denoise needs contiguous direct, indirect, albedo, depth and normal passes")
+        tot = passes.total_radiance
+        self._call(self.lib.vira_gpu_denoise, self.h, C.byref(o), passes.width, passes.height,
+                   passes.direct_radiance.data_ptr(), passes.indirect_radiance.data_ptr(), passes.albedo.data_ptr(),
+                   passes.depth.data_ptr(), passes.normal_global.data_ptr(),
+                   None if tot is None else tot.data_ptr())
 
     def trace_rays(self, mid: int, rays, any_hit: bool = False):
         """rays: (n, 8) float32 CUDA tensor (ox oy oz 0 dx dy dz tmax)."""
@@ -542,12 +580,16 @@ class Context:
         return arr
 
     def batch_submit(self, requests, config: "BatchConfig | None" = None):
-        """Renders every RenderRequest in one persistent launch; returns the launch stats."""
+        """Renders every RenderRequest in one persistent launch; returns the launch stats.
+        Requests whose options ask for denoising are denoised afterwards, one by one."""
         arr = self._requests(requests)
         st = capi.BatchStats()
         cfg = None if config is None else C.byref(config.to_c())
         self._call(self.lib.vira_gpu_batch_submit, self.h, arr, len(requests), cfg, C.byref(st))
         self.last_stats = {f: getattr(st, f) for f, _ in capi.BatchStats._fields_}
+        for r in requests:
+            if r.options.denoise:
+                self.denoise(r.passes, r.options.denoiser)
         return self.last_stats
 
     def batch_autotune(self, requests, sample_stride: int = 16):
