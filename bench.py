@@ -288,6 +288,50 @@ def sensor_leg(ctx, built, passes, steps: int) -> dict:
             "sensor_ms": round(ss_ms, 3), "denoise_eatwt_ms": round(dn_ms, 3)}
 
 
+def quipu_leg(ctx, built) -> dict:
+    """Config 1's DEM (4097^2) as a quipu pyramid (levels 4097^2, 2049^2,
+    1025^2; 16-bit heights in LZ4 blocks) written to a temporary file, then read
+    back: file parse + LZ4 decode on the host, dequantisation and mesh on the
+    GPU, and the BVH build of the finest level."""
+    import tempfile
+    import torch
+    from vira_amd import quipu as Q
+
+    h = built.heights.cpu().numpy()
+    sp = float(built.scene.spacing)
+    levels = []
+    for k in range(3):
+        hk = np.ascontiguousarray(h[::1 << k, ::1 << k])
+        p = Q.DEMProjection(resolution=(hk.shape[1], hk.shape[0]), pixel_scale=(sp, sp), dx=float(1 << k),
+                            dy=float(1 << k))
+        levels.append(Q.DEMLevel(hk, p))
+    with tempfile.TemporaryDirectory() as tmp:
+        path = f"{tmp}/c1.qld"
+        t0 = time.perf_counter()
+        Q.write_pyramid(path, levels, compress=True)
+        write_s = time.perf_counter() - t0
+        size = Path(path).stat().st_size
+        d = Q.QuipuDEM(path)
+        mid, _, _, info = d.read_level(ctx, 0)          # warm (file cache, allocations)
+        ctx.mesh_release(mid)
+        t0 = time.perf_counter()
+        mid, _, _, info = d.read_level(ctx, 0)
+        torch.cuda.synchronize()
+        read_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ctx.blas_build(mid)
+        torch.cuda.synchronize()
+        bvh_s = time.perf_counter() - t0
+        ctx.mesh_release(mid)
+        d.close()
+    print(f"# quipu: 4097^2 level ({size / 2**20:.1f} MiB file, 16-bit LZ4) read + mesh {read_s * 1e3:.1f} ms "
+          f"(parse {info['read_ms']:.1f} ms of which LZ4 {info['decode_ms']:.1f} ms, device {info['build_ms']:.1f} ms), "
+          f"BVH {bvh_s * 1e3:.1f} ms; write {write_s:.1f} s", file=sys.stderr, flush=True)
+    return {"file_mib": round(size / 2**20, 1), "read_mesh_ms": round(read_s * 1e3, 2),
+            "parse_ms": round(info["read_ms"], 2), "lz4_ms": round(info["decode_ms"], 2),
+            "device_ms": round(info["build_ms"], 2), "bvh_ms": round(bvh_s * 1e3, 2)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -301,6 +345,7 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true", help="skip the config-2 adjoint/forward leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the config-3 batched/per-request leg")
     ap.add_argument("--no-instanced", action="store_true", help="skip the config-4 instanced-scene leg")
+    ap.add_argument("--no-quipu", action="store_true", help="skip the quipu pyramid read leg")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -355,6 +400,7 @@ def main() -> int:
           file=sys.stderr, flush=True)
 
     sensor = sensor_leg(ctx, built, passes, max(1, min(3, args.steps))) if args.scene == "c1" else None
+    quipu = quipu_leg(ctx, built) if (args.scene == "c1" and not args.no_quipu) else None
     adjoint = None if args.no_adjoint else adjoint_leg(ctx, max(3, args.steps), max(1, args.warmup))
     batch = None if (args.no_batch or args.scene != "c1") else batch_leg(ctx, built, max(1, min(3, args.steps)), 1)
     instanced = None if (args.no_instanced or args.scene != "c1") else instanced_leg(ctx, built, max(1, min(3, args.steps)), 1)
@@ -422,6 +468,7 @@ def main() -> int:
         "adjoint": adjoint,
         "instanced": instanced,
         "sensor_tail": sensor,
+        "quipu": quipu,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

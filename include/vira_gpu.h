@@ -419,6 +419,56 @@ vira_gpu_status vira_gpu_denoise(vira_gpu_ctx* ctx, const vira_gpu_denoise_opts*
                                  float* direct_dev, float* indirect_dev, const float* albedo_dev,
                                  const float* depth_dev, const float* normal_dev, float* total_dev);
 
+/* --- quipu DEM pyramids (.qld) -------------------------------------------- */
+/* DEMQuipu (dem_quipu.hpp:34-81, dem_quipu.ipp:29-236): a file of DEM levels
+ * of decreasing resolution, each with its GSD.  The reader parses the file
+ * and decodes its LZ4 blocks on the host and builds the level's mesh on the
+ * GPU, ready for vira_gpu_blas_build. */
+typedef struct vira_gpu_qld vira_gpu_qld;
+typedef struct {
+    int32_t levels;
+    int32_t has_albedos;
+    double transformation[16];   /* model matrix, column-major (readTransformation) */
+    float normal[3];             /* normal cone of the coarsest level */
+    float cone_angle;
+} vira_gpu_qld_info;
+typedef struct {
+    int32_t read_albedos;        /* DEMQuipuReaderOptions::readAlbedos */
+    float default_albedo;        /* used when the file has no albedo or read_albedos = 0 (0.06) */
+    float albedo_profile[VIRA_GPU_NUM_BINS];  /* grey albedo images are multiplied by this (1) */
+    double offset[3];            /* subtracted from OGR (map-projected) positions, as makeVertexBuffer does */
+} vira_gpu_qld_read_opts;
+typedef struct {
+    int32_t level;
+    double gsd;
+    int32_t width, height;
+    int32_t height_bits;         /* 8, 16 (fixed point) or 32 (float) */
+    int32_t compressed;          /* heights stored as an LZ4 block */
+    int32_t albedo_type;         /* -1 none, 0 constant, 1 grey image, 2 eight channels */
+    int32_t is_ogr;
+    double read_ms, decode_ms, build_ms;  /* file read + parse, of which LZ4 decode; device work */
+} vira_gpu_qld_level;
+/* The LZ4 block decoder the reader uses (LZ4_decompress_safe's contract,
+ * quipu_io.ipp:219-235): returns the decoded size, or -1 for a malformed or
+ * truncated block or one that does not fit in dst_capacity.  Host memory. */
+int64_t vira_gpu_lz4_decode_block(const void* src, size_t src_size, void* dst, size_t dst_capacity);
+/* DEMQuipu::DEMQuipu (dem_quipu.ipp:29-56): header and level table. */
+vira_gpu_status vira_gpu_qld_open(const char* path, vira_gpu_qld** qld, vira_gpu_qld_info* info);
+vira_gpu_status vira_gpu_qld_close(vira_gpu_qld* qld);
+vira_gpu_status vira_gpu_qld_level_gsd(const vira_gpu_qld* qld, int32_t level, double* gsd);
+/* readBuffers' choice (dem_quipu.ipp:137-151): the coarsest level whose GSD
+ * is <= required_gsd, else level 0. */
+vira_gpu_status vira_gpu_qld_select(const vira_gpu_qld* qld, double required_gsd, int32_t* level);
+/* DEMQuipu::readBuffers (dem_quipu.ipp:131-236) for one level: the mesh
+ * (DEM::makeVertexBuffer / makeIndexBuffer, dem.ipp:138-211, with the level's
+ * DEMProjection: planar, or OGR equirectangular) in a new mesh id; heights_dev
+ * (W*H, optional) and albedo_dev (W*H*8, optional) receive the decoded
+ * samples; mesh_id may be NULL to decode only.  OGR projections other than
+ * equirectangular return VIRA_GPU_ERR_UNSUPPORTED. */
+vira_gpu_status vira_gpu_qld_read(vira_gpu_ctx* ctx, vira_gpu_qld* qld, int32_t level,
+                                  const vira_gpu_qld_read_opts* opts, int32_t* mesh_id, float* heights_dev,
+                                  float* albedo_dev, vira_gpu_qld_level* info);
+
 /* --- batched rendering (new) --------------------------------------------- */
 /* Renders n requests (CPUPathTracer::render semantics per request,
  * cpu_path_tracer.ipp:29-161) in one persistent, hipGraph-captured launch;

@@ -10,7 +10,7 @@ ROOT = Path(__file__).resolve().parent.parent
 
 def declared_entry_points():
     text = (ROOT / "include" / "vira_gpu.h").read_text()
-    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_]+\*?\s+\**(vira_gpu_[a-z0-9_]+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\*?\s+\**(vira_gpu_[a-z0-9_]+)\s*\(", text, re.M)))
 
 
 def test_header_declares_entry_points():

@@ -1,0 +1,196 @@
+"""quipu (.qld) reader on the host: the library's LZ4 block decoder against
+the system liblz4 (the codec the reference links, quipu_io.ipp:176-235), the
+header and level table written by vira_amd.quipu.write_pyramid
+(DEMQuipu::write, dem_quipu.ipp:238-333) and read back by vira_gpu_qld_open
+(DEMQuipu::DEMQuipu, dem_quipu.ipp:29-56), the field sizes the reference's own
+quipu I/O test fixes (tests/quipu/test_quipu_io.cpp:12-70), readBuffers' level
+choice and the level-of-detail rule.  No GPU is needed."""
+import ctypes as C
+import math
+import struct
+
+import numpy as np
+import pytest
+
+from vira_amd import quipu as Q
+from vira_amd import scene as S
+
+
+def _liblz4():
+    try:
+        return Q._lz4()
+    except RuntimeError:
+        pytest.skip("system liblz4 not present")
+
+
+def _lz4_decompress_ref(block: bytes, size: int) -> bytes:
+    lib = _liblz4()
+    lib.LZ4_decompress_safe.restype = C.c_int
+    lib.LZ4_decompress_safe.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int]
+    out = C.create_string_buffer(max(1, size))
+    n = lib.LZ4_decompress_safe(block, out, len(block), size)
+    return out.raw[:n] if n >= 0 else None
+
+
+def _payloads():
+    rng = np.random.default_rng(0)
+    yield b""
+    yield b"a"
+    yield b"abcd" * 3
+    yield bytes(rng.integers(0, 256, 100_000, dtype=np.uint8))           # incompressible
+    yield b"\x07" * 300_000                                               # offset-1 runs, long match lengths
+    yield (b"0123456789" * 50_000)[:499_999]
+    h = (np.cumsum(rng.integers(-3, 4, 1 << 18)) & 0xFFFF).astype(np.uint16)
+    yield h.tobytes()                                                     # a smooth 16-bit height profile
+    words = [b"crater ", b"regolith ", b"mare ", b"highland "]
+    yield b"".join(words[k] for k in rng.integers(0, 4, 40_000))
+
+
+@pytest.mark.parametrize("k", range(8))
+def test_lz4_decoder_matches_liblz4(k):
+    data = list(_payloads())[k]
+    block = Q.lz4_compress_block(data) if data else Q.lz4_compress_block(b"")
+    assert Q.lz4_decode_block(block, len(data)) == data
+    if data:
+        assert _lz4_decompress_ref(block, len(data)) == data
+
+
+def test_lz4_decoder_rejects_malformed_blocks():
+    from vira_amd import _capi
+    lib = _capi.load()
+    data = (b"lunar south pole " * 400) + bytes(range(256))
+    block = Q.lz4_compress_block(data)
+
+    def dec(b, cap):
+        out = C.create_string_buffer(max(1, cap))
+        return lib.vira_gpu_lz4_decode_block(b, len(b), out, cap)
+
+    assert dec(block, len(data)) == len(data)
+    # every truncation fails cleanly (or, if it ends on a sequence boundary, is short)
+    for n in range(0, len(block)):
+        r = dec(block[:n], len(data))
+        assert r == -1 or r < len(data)
+    assert dec(block, len(data) - 1) == -1           # output does not fit
+    # a match offset of zero, or reaching before the start of the output
+    assert dec(bytes([0x10, 0x41, 0x00, 0x00]), 64) == -1
+    assert dec(bytes([0x10, 0x41, 0x05, 0x00]), 64) == -1
+    # a literal run longer than the input
+    assert dec(bytes([0xF0, 0xFF, 0x10]), 1 << 20) == -1
+
+
+def test_field_sizes_as_the_reference_io_test():
+    """writeVec: class id (2 B) + length (1 B) + N values; writeMat: class id +
+    two sizes + N*M values (tests/quipu/test_quipu_io.cpp:15, 46)."""
+    for n in (1, 2, 3, 4):
+        w = Q._W()
+        assert w.vec([2.3 * i + 4.5 for i in range(n)]) == 2 + 1 + 8 * n
+        assert w.vec([2.3 * i + 4.5 for i in range(n)], double=False) == 2 + 1 + 4 * n
+    w = Q._W()
+    assert w.mat3(np.eye(3)) == 2 + 2 + 9 * 8
+
+
+def _levels(n=3, base=33, scale=30.0, albedo=None, seed=0):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        W = (base - 1) // (1 << k) + 1
+        h = (100.0 * rng.random((W + 2, W))).astype(np.float32)
+        p = Q.DEMProjection(resolution=(W, W + 2), pixel_scale=(scale, scale), dx=float(1 << k), dy=float(1 << k))
+        out.append(Q.DEMLevel(h, p, albedo))
+    return out
+
+
+def _rotation(ax, ang):
+    ax = np.asarray(ax, float) / np.linalg.norm(ax)
+    K = np.array([[0, -ax[2], ax[1]], [ax[2], 0, -ax[0]], [-ax[1], ax[0], 0]])
+    return np.eye(3) + math.sin(ang) * K + (1 - math.cos(ang)) * K @ K
+
+
+def test_header_and_level_table_round_trip(tmp_path):
+    M = np.eye(4)
+    M[:3, :3] = _rotation((1, 2, 3), 0.7) @ np.diag([2.0, 3.0, 0.5])
+    M[:3, 3] = (1.5e11, -2.0e10, 3.25e9)
+    lv = _levels()
+    path = tmp_path / "p.qld"
+    Q.write_pyramid(str(path), lv, transformation=M, compress=False, normal=(0.0, 0.6, 0.8), cone_angle=0.25)
+    d = Q.QuipuDEM(str(path))
+    assert d.levels == 3 and d.has_albedos
+    assert d.gsds == [30.0, 60.0, 120.0]
+    assert np.allclose(d.transformation, M, rtol=1e-12, atol=1e-3)
+    assert np.allclose(d.normal, (0.0, 0.6, 0.8)) and abs(d.cone_angle - 0.25) < 1e-7
+    # the table's offsets point at each level's projection record
+    raw = path.read_bytes()
+    toc_at = raw.index(struct.pack("<d", 30.0))
+    for k in range(3):
+        gsd, off = struct.unpack_from("<dQ", raw, toc_at + 16 * k)
+        W, H = struct.unpack_from("<II", raw, off)
+        assert gsd == d.gsds[k] and (W, H) == (lv[k].heights.shape[1], lv[k].heights.shape[0])
+    d.close()
+
+
+def test_open_rejects_bad_files(tmp_path):
+    from vira_amd._capi import ViraGPUError
+    bad = tmp_path / "bad.qld"
+    bad.write_bytes(b"NOTQUIPU")
+    with pytest.raises(ViraGPUError, match="not a valid Quipu"):
+        Q.QuipuDEM(str(bad))
+    good = tmp_path / "good.qld"
+    Q.write_pyramid(str(good), _levels(1), compress=False)
+    raw = good.read_bytes()
+    for cut in (3, 7, 40, 90):
+        (tmp_path / "cut.qld").write_bytes(raw[:cut])
+        with pytest.raises(ViraGPUError):
+            Q.QuipuDEM(str(tmp_path / "cut.qld"))
+    other = bytearray(raw)
+    other[5:7] = struct.pack("<H", 2001)  # a single DEM, not a pyramid
+    (tmp_path / "cls.qld").write_bytes(bytes(other))
+    with pytest.raises(ViraGPUError, match="2002"):
+        Q.QuipuDEM(str(tmp_path / "cls.qld"))
+    with pytest.raises(ViraGPUError, match="cannot open"):
+        Q.QuipuDEM(str(tmp_path / "missing.qld"))
+
+
+def test_level_choice_as_read_buffers(tmp_path):
+    def pyramid(gsds):
+        lv = []
+        for g in gsds:
+            p = Q.DEMProjection(resolution=(3, 3), pixel_scale=(g, g))
+            lv.append(Q.DEMLevel(np.zeros((3, 3), np.float32), p))
+        path = tmp_path / f"g{len(gsds)}_{gsds[-1]}.qld"
+        Q.write_pyramid(str(path), lv, normal=(0, 0, 1), cone_angle=0.0)
+        return Q.QuipuDEM(str(path))
+
+    d = pyramid([1.0, 2.0, 4.0, 8.0])
+    for req, want in ((0.5, 0), (1.0, 0), (1.99, 0), (2.0, 1), (3.0, 1), (8.0, 3), (math.inf, 3)):
+        assert d.select(req) == want, (req, want)
+    # the scan stops at the first level coarser than asked
+    d = pyramid([1.0, 4.0, 2.0])
+    assert d.select(3.0) == 0 and d.select(5.0) == 2
+
+
+def test_lod_rule():
+    cam = S.Camera(1000, 800, fov_deg=30.0, focal_length=0.1)
+    sw, sh = cam.sensor_size()
+    d = 2.0e5
+    assert math.isclose(Q.calculate_gsd(cam, d), min(d * sw / (0.1 * 1000), d * sh / (0.1 * 800)))
+    assert math.isclose(Q.target_gsd(cam, d, 2.0), 2.0 * Q.calculate_gsd(cam, d), rel_tol=1e-6)
+    assert Q.lod_update(100.0, 50.0) == (50.0, True)      # too coarse: reload finer
+    assert Q.lod_update(40.0, 50.0) == (40.0, False)      # finer but within 2x: keep
+    assert Q.lod_update(20.0, 50.0) == (50.0, True)       # more than 2x finer: reload coarser
+    assert Q.lod_update(50.0, 50.0) == (50.0, False)
+
+
+def test_fixed_point_codec_is_the_reference_formula():
+    """floatToFixed then fixedToFloat (image_utils.ipp:484-527) on the host:
+    the reader's dequantisation of a value round-trips within one step (the
+    reference's +1 offset on write is not undone on read, so values come back
+    one quantum high), and no-data maps to 0 and back to +inf."""
+    h = np.array([[-12.5, 0.0, 7.25, np.inf], [100.0, 55.5, -12.5, 3.0]], np.float32)
+    mm = Q._valid_minmax(h)
+    assert mm == (-12.5, 100.0)
+    q = Q.float_to_fixed(h, mm, np.uint16)
+    assert q[0, 3] == 0 and q[0, 0] == 1 and q[1, 0] == 65535
+    back = (q.astype(np.float32) / np.float32(65534)) * np.float32(mm[1] - mm[0]) + np.float32(mm[0])
+    step = (mm[1] - mm[0]) / 65534
+    fin = np.isfinite(h)
+    assert np.all(np.abs(back[fin] - h[fin] - step) <= step)

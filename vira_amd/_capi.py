@@ -221,6 +221,22 @@ class StarPSF(C.Structure):
                 ("intrinsic", C.c_float * 6), ("cull_behind", C.c_int32)]
 
 
+class QldInfo(C.Structure):
+    _fields_ = [("levels", C.c_int32), ("has_albedos", C.c_int32), ("transformation", C.c_double * 16),
+                ("normal", C.c_float * 3), ("cone_angle", C.c_float)]
+
+
+class QldReadOpts(C.Structure):
+    _fields_ = [("read_albedos", C.c_int32), ("default_albedo", C.c_float), ("albedo_profile", C.c_float * NUM_BINS),
+                ("offset", C.c_double * 3)]
+
+
+class QldLevel(C.Structure):
+    _fields_ = [("level", C.c_int32), ("gsd", C.c_double), ("width", C.c_int32), ("height", C.c_int32),
+                ("height_bits", C.c_int32), ("compressed", C.c_int32), ("albedo_type", C.c_int32),
+                ("is_ogr", C.c_int32), ("read_ms", C.c_double), ("decode_ms", C.c_double), ("build_ms", C.c_double)]
+
+
 class Instance(C.Structure):
     _fields_ = [("mesh_id", C.c_int32), ("transform", C.c_double * 16)]
 
@@ -256,6 +272,13 @@ SIGNATURES = [
     ("vira_gpu_psf_kernel", _S, [_P, C.POINTER(PSF), C.c_int, _P]),
     ("vira_gpu_convolve", _S, [_P, _P, C.c_int, C.c_int, _P, C.c_int, _P]),
     ("vira_gpu_simulate_sensor", _S, [_P, C.POINTER(SensorSim), _P, C.c_int, C.c_int, _P, _P]),
+    ("vira_gpu_lz4_decode_block", C.c_int64, [_P, C.c_size_t, _P, C.c_size_t]),
+    ("vira_gpu_qld_open", _S, [C.c_char_p, C.POINTER(_P), C.POINTER(QldInfo)]),
+    ("vira_gpu_qld_close", _S, [_P]),
+    ("vira_gpu_qld_level_gsd", _S, [_P, C.c_int32, C.POINTER(C.c_double)]),
+    ("vira_gpu_qld_select", _S, [_P, C.c_double, C.POINTER(C.c_int32)]),
+    ("vira_gpu_qld_read", _S, [_P, _P, C.c_int32, C.POINTER(QldReadOpts), C.POINTER(C.c_int32), _P, _P,
+                               C.POINTER(QldLevel)]),
     ("vira_gpu_denoise", _S, [_P, C.POINTER(DenoiseOpts), C.c_int, C.c_int, _P, _P, _P, _P, _P, _P]),
     ("vira_gpu_render_stars", _S, [_P, C.POINTER(Camera), C.POINTER(Sensor), C.POINTER(StarPSF), _P, _P,
                                    C.c_size_t, _P]),

@@ -6,6 +6,7 @@
 #include <string>
 
 #include "context.hpp"
+#include "lz4_block.hpp"
 #include "instance.hpp"
 
 namespace vg {
@@ -439,6 +440,61 @@ vira_gpu_status vira_gpu_render_stars(vira_gpu_ctx* ctx, const vira_gpu_camera* 
         set_device(c);
         VG_CHECK(cam && sensor && psf, VIRA_GPU_ERR_INVALID, "render_stars: null argument");
         vg::render_stars(c, *cam, *sensor, *psf, directions_dev, irradiance_dev, n, power_dev);
+    });
+}
+
+int64_t vira_gpu_lz4_decode_block(const void* src, size_t src_size, void* dst, size_t dst_capacity) {
+    if (!src || (!dst && dst_capacity)) return -1;
+    return vg::lz4_decode_block(static_cast<const uint8_t*>(src), src_size, static_cast<uint8_t*>(dst), dst_capacity);
+}
+
+vira_gpu_status vira_gpu_qld_open(const char* path, vira_gpu_qld** qld, vira_gpu_qld_info* info) {
+    return guarded([&] {
+        VG_CHECK(qld, VIRA_GPU_ERR_INVALID, "qld_open: null handle pointer");
+        *qld = vg::qld_open(path, info);
+    });
+}
+
+vira_gpu_status vira_gpu_qld_close(vira_gpu_qld* qld) {
+    return guarded([&] { delete qld; });
+}
+
+vira_gpu_status vira_gpu_qld_level_gsd(const vira_gpu_qld* qld, int32_t level, double* gsd) {
+    return guarded([&] {
+        VG_CHECK(qld && gsd, VIRA_GPU_ERR_INVALID, "qld_level_gsd: null argument");
+        VG_CHECK(level >= 0 && level < qld->levels, VIRA_GPU_ERR_INVALID, "qld_level_gsd: level out of range");
+        *gsd = qld->gsd[level];
+    });
+}
+
+vira_gpu_status vira_gpu_qld_select(const vira_gpu_qld* qld, double required_gsd, int32_t* level) {
+    return guarded([&] {
+        VG_CHECK(qld && level, VIRA_GPU_ERR_INVALID, "qld_select: null argument");
+        *level = vg::qld_select(*qld, required_gsd);
+    });
+}
+
+vira_gpu_status vira_gpu_qld_read(vira_gpu_ctx* ctx, vira_gpu_qld* qld, int32_t level,
+                                  const vira_gpu_qld_read_opts* opts, int32_t* mesh_id, float* heights_dev,
+                                  float* albedo_dev, vira_gpu_qld_level* info) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        set_device(c);
+        VG_CHECK(qld && opts, VIRA_GPU_ERR_INVALID, "qld_read: null argument");
+        int32_t made = -1;
+        try {
+            vg::qld_read(c, *qld, level, *opts, heights_dev, albedo_dev, mesh_id ? &made : nullptr, info, [&] {
+                made = new_mesh(c);
+                return made;
+            });
+        } catch (...) {
+            if (made >= 0) c.meshes[made].reset();
+            throw;
+        }
+        if (mesh_id) {
+            c.scene_gen++;
+            *mesh_id = made;
+        }
     });
 }
 

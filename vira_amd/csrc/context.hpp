@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -10,6 +11,20 @@
 
 #include "../../include/vira_gpu.h"
 #include "device_math.hpp"
+
+// An open quipu DEM pyramid: its header and level table (the levels are read
+// on demand).
+struct vira_gpu_qld {
+    std::string path;
+    int32_t levels = 0;
+    bool has_albedos = false;
+    double M[16] = {};
+    float normal[3] = {};
+    float cone = 0.0f;
+    std::vector<double> gsd;
+    std::vector<uint64_t> offset;
+    uint64_t file_size = 0;
+};
 
 namespace vg {
 
@@ -142,6 +157,11 @@ void launch_synth_fbm(hipStream_t s, int W, int H, uint32_t seed, double amp, in
 void launch_synth_albedo(hipStream_t s, size_t nv, uint32_t seed, float* out);
 void dem_build_mesh(Ctx& c, MeshG& m, const float* heights, int W, int H, const vira_gpu_projection& p,
                     const float* grey);
+// The DEM grid's triangles (imageToIndexBuffer order, holes skipped) and vertex
+// normals around vertices that write_vertices fills (GVertex[W*H]; normals are
+// overwritten).
+void dem_grid_mesh(Ctx& c, MeshG& m, const float* heights, int W, int H,
+                   const std::function<void(GVertex*)>& write_vertices);
 void mesh_compute_normals(Ctx& c, MeshG& m);
 void mesh_adjacency(Ctx& c, MeshG& m);
 void dem_update(Ctx& c, MeshG& m, const float* heights);
@@ -154,6 +174,11 @@ void received_power(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_sensor& s
 void render_stars(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_sensor& sen, const vira_gpu_star_psf& psf,
                   const float* dirs, const float* irr, size_t n, float* img);
 void psf_kernel(Ctx& c, const vira_gpu_psf& P, int K, float* out);
+vira_gpu_qld* qld_open(const char* path, vira_gpu_qld_info* info);
+int qld_select(const vira_gpu_qld& q, double required_gsd);
+void qld_read(Ctx& c, const vira_gpu_qld& q, int level, const vira_gpu_qld_read_opts& o, float* heights_dev,
+              float* albedo_dev, int32_t* mesh_id_out, vira_gpu_qld_level* info,
+              const std::function<int32_t()>& new_mesh);
 void denoise(Ctx& c, const vira_gpu_denoise_opts& o, int W, int H, float* direct, float* indirect,
              const float* albedo, const float* depth, const float* normal, float* total);
 void convolve(Ctx& c, const float* in, int W, int H, const float* kern, int K, float* out);

@@ -5,6 +5,8 @@
 // restated as gather kernels: every output element is produced by one thread
 // from its inputs in the reference's summation order, so the results do not
 // depend on scheduling.
+#include <functional>
+
 #include "context.hpp"
 
 namespace vg {
@@ -264,8 +266,8 @@ void launch_synth_albedo(hipStream_t s, size_t nv, uint32_t seed, float* out) {
     VG_HIP(hipGetLastError());
 }
 
-void dem_build_mesh(Ctx& c, MeshG& m, const float* heights, int W, int H, const vira_gpu_projection& p,
-                    const float* grey) {
+void dem_grid_mesh(Ctx& c, MeshG& m, const float* heights, int W, int H,
+                   const std::function<void(GVertex*)>& write_vertices) {
     VG_CHECK(W >= 2 && H >= 2, VIRA_GPU_ERR_INVALID, "DEM needs at least 2x2 samples");
     VG_CHECK((size_t)W * H < (1ull << 31), VIRA_GPU_ERR_INVALID, "DEM too large for 32-bit vertex ids");
     size_t nv = (size_t)W * H;
@@ -286,12 +288,19 @@ void dem_build_mesh(Ctx& c, MeshG& m, const float* heights, int W, int H, const 
                                                                 ids.as<uint32_t>(), m.idx.as<uint32_t>(),
                                                                 c2t.as<uint32_t>());
     VG_HIP(hipGetLastError());
-    k_dem_vertices<<<blocks_for(nv, 256), 256, 0, c.stream>>>(heights, W, H, p, grey, m.verts.as<GVertex>());
-    VG_HIP(hipGetLastError());
+    write_vertices(m.verts.as<GVertex>());
     k_dem_normals<<<blocks_for(nv, 256), 256, 0, c.stream>>>(W, H, c2t.as<uint32_t>(), m.idx.as<uint32_t>(),
                                                               m.verts.as<GVertex>());
     VG_HIP(hipGetLastError());
     VG_HIP(hipStreamSynchronize(c.stream));
+}
+
+void dem_build_mesh(Ctx& c, MeshG& m, const float* heights, int W, int H, const vira_gpu_projection& p,
+                    const float* grey) {
+    dem_grid_mesh(c, m, heights, W, H, [&](GVertex* verts) {
+        k_dem_vertices<<<blocks_for((size_t)W * H, 256), 256, 0, c.stream>>>(heights, W, H, p, grey, verts);
+        VG_HIP(hipGetLastError());
+    });
     m.is_dem = true;
     m.dem_w = W;
     m.dem_h = H;
