@@ -101,14 +101,43 @@ def test_psf_kernel_against_oracle(gpu_ctx, psf):
 
 
 def test_convolution_bit_exact_against_oracle(gpu_ctx):
+    """Kernels below 256 pixels take the reference's spatial branch
+    (image.ipp:1009-1012): same sums in the same order, bit-exact."""
     import torch
     rng = np.random.default_rng(1)
     img = rng.random((48, 64, 8)).astype(np.float32)
-    for K in (9, 27, 301):
-        kern = gpu_ctx.psf_kernel(SN.GaussianPSF(1.5 + K / 20.0, supersample=4), K)
+    for K in (1, 3, 9, 15):
+        kern = gpu_ctx.psf_kernel(SN.GaussianPSF(1.5 + K / 20.0, supersample=4), K) if K >= 3 else \
+            torch.full((1, 1, 8), 0.5, device="cuda")
         got = gpu_ctx.convolve(torch.from_numpy(img).cuda(), kern).cpu().numpy()
         ref = O.convolve(img, kern.cpu().numpy())
         assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), K
+
+
+@pytest.mark.parametrize("shape,K", [((48, 64), 17), ((100, 130), 81), ((37, 29), 51), ((70, 90), 301)])
+def test_convolution_large_kernels(gpu_ctx, shape, K):
+    """Kernels of 256 pixels or more (the reference's FFT branch, image.ipp:1014-1065):
+    the same zero-padded correlation.  Against the exact (FP64) correlation the
+    GPU's FMA sums are within 2e-5 relative where the result is above 1e-3 of
+    the maximum and 1e-5 of the maximum everywhere -- the error of an FP32 sum
+    of K^2 non-negative terms (the oracle's sequential FP32 sum is checked
+    against the same bound)."""
+    import torch
+    from scipy.signal import correlate
+    rng = np.random.default_rng(K)
+    img = rng.random(shape + (8,)).astype(np.float32)
+    img[rng.random(shape) < 0.3] = 0.0
+    kern = gpu_ctx.psf_kernel(SN.GaussianPSF(np.linspace(1.0, K / 6.0, 8), angle_deg=15.0, supersample=2), K)
+    kh = kern.cpu().numpy()
+    got = gpu_ctx.convolve(torch.from_numpy(img).cuda(), kern).cpu().numpy().astype(np.float64)
+    exact = np.stack([correlate(img[:, :, b].astype(np.float64), kh[:, :, b].astype(np.float64), mode="same")
+                      for b in range(8)], 2)
+    ora = O.convolve(img, kh).astype(np.float64)
+    scale = np.abs(exact).max()
+    big = np.abs(exact) > 1e-3 * scale
+    for res in (got, ora):
+        assert np.abs(res - exact).max() <= 1e-5 * scale
+        assert (np.abs(res - exact)[big] / np.abs(exact[big])).max() <= 2e-5
 
 
 def test_sensor_deterministic_bit_exact(gpu_ctx):

@@ -157,6 +157,90 @@ __global__ void __launch_bounds__(kConvBS) k_convolve(const float* __restrict__ 
     for (int k = 0; k < NB; ++k) out[p * NB + k] = sum[k];
 }
 
+// Kernels of 256 pixels or more: the reference switches to an FFT there
+// (Image::convolve, image.ipp:1006-1065), which is the same zero-padded
+// correlation up to the transform's rounding.  Here it is computed directly
+// with fused multiply-adds, tiled for the CU:
+//   * one block per 64 x 32 output tile of one band (grid z = band);
+//   * the kernel is walked in sub-blocks of at most 48 x 48 taps; for each,
+//     the block stages the input window (64 + kb - 1) x (32 + kb - 1) of its
+//     band in LDS (zeros outside the image) and the sub-block's taps;
+//   * a thread owns 8 outputs of one column; it walks the taps column by
+//     column and, down a column, keeps the 8 input values it needs in a
+//     ring of registers, so each LDS read feeds 8 FMAs and a wave's reads
+//     are 64 consecutive floats (no bank conflicts).
+constexpr int kCvTW = 64, kCvTH = 32, kCvR = 8, kCvKB = 48;
+constexpr int kCvRows = kCvTH / kCvR;  // 4 thread rows: 256 threads
+constexpr int kCvSW = kCvTW + kCvKB - 1, kCvSH = kCvTH + kCvKB - 1;
+
+__global__ void __launch_bounds__(kCvTW* kCvRows) k_convolve_tiled(const float* __restrict__ in, int W, int H,
+                                                                  const float* __restrict__ kern, int K,
+                                                                  float* __restrict__ out) {
+    __shared__ float tile[kCvSH * kCvSW];
+    __shared__ float taps[kCvKB * kCvKB];
+    const int band = blockIdx.z;
+    const int x0 = blockIdx.x * kCvTW, y0 = blockIdx.y * kCvTH;
+    const int tx = threadIdx.x % kCvTW, ty = threadIdx.x / kCvTW;
+    const int h = K / 2;
+    float acc[kCvR];
+#pragma unroll
+    for (int r = 0; r < kCvR; ++r) acc[r] = 0.0f;
+    for (int by = 0; by < K; by += kCvKB) {
+        const int kbh = min(kCvKB, K - by);
+        for (int bx = 0; bx < K; bx += kCvKB) {
+            const int kbw = min(kCvKB, K - bx);
+            const int sw = kCvTW + kbw - 1, sh = kCvTH + kbh - 1;
+            __syncthreads();
+            // input window: rows y0 + by - h .., columns x0 + bx - h ..
+            for (int e = threadIdx.x; e < sw * sh; e += kCvTW * kCvRows) {
+                const int r = e / sw, q = e - r * sw;
+                const int iy = y0 + by - h + r, ix = x0 + bx - h + q;
+                float v = 0.0f;
+                if (iy >= 0 && iy < H && ix >= 0 && ix < W) v = in[((size_t)iy * W + ix) * NB + band];
+                tile[r * kCvSW + q] = v;
+            }
+            for (int e = threadIdx.x; e < kbw * kbh; e += kCvTW * kCvRows) {
+                const int r = e / kbw, q = e - r * kbw;
+                taps[q * kCvKB + r] = kern[((size_t)(by + r) * K + (bx + q)) * NB + band];  // column-major
+            }
+            __syncthreads();
+            const int row0 = ty * kCvR;
+            for (int kx = 0; kx < kbw; ++kx) {
+                const float* col = tile + tx + kx;
+                const float* tcol = taps + kx * kCvKB;
+                float ring[kCvR];
+#pragma unroll
+                for (int r = 0; r < kCvR; ++r) ring[r] = col[(row0 + r) * kCvSW];
+                int ky = 0;
+                for (; ky + kCvR <= kbh; ky += kCvR) {
+#pragma unroll
+                    for (int s = 0; s < kCvR; ++s) {
+                        const float kv = tcol[ky + s];
+#pragma unroll
+                        for (int r = 0; r < kCvR; ++r) acc[r] = fmaf(ring[(r + s) % kCvR], kv, acc[r]);
+                        // slot s held row row0 + ky + s, which only r = 0 needed: refill it with the row
+                        // r = 7 needs at the next tap (none after the column's last tap)
+                        if (ky + s + 1 < kbh) ring[s] = col[(row0 + kCvR + ky + s) * kCvSW];
+                    }
+                }
+                // the last kbh % 8 taps of the column (the ring is rotated by ky % 8 == 0 here)
+                for (int s = 0; ky + s < kbh; ++s) {
+                    const float kv = tcol[ky + s];
+#pragma unroll
+                    for (int r = 0; r < kCvR; ++r) acc[r] = fmaf(col[(row0 + r + ky + s) * kCvSW], kv, acc[r]);
+                }
+            }
+        }
+    }
+    const int ox = x0 + tx;
+    if (ox >= W) return;
+#pragma unroll
+    for (int r = 0; r < kCvR; ++r) {
+        const int oy = y0 + ty * kCvR + r;
+        if (oy < H) out[((size_t)oy * W + ox) * NB + band] = acc[r];
+    }
+}
+
 // ------------------------------------------------------------ sensor
 // Counter-based draws: stream (seed, pixel, iteration, dim).
 constexpr uint32_t kDimPhoton = 64, kDimDark = 80, kDimReadout = 96;
@@ -545,7 +629,13 @@ void convolve(Ctx& c, const float* in, int W, int H, const float* kern, int K, f
     VG_CHECK(in && kern && out && in != out, VIRA_GPU_ERR_INVALID, "convolve: null or aliased buffers");
     VG_CHECK(W > 0 && H > 0 && K >= 1 && K % 2 == 1, VIRA_GPU_ERR_INVALID, "convolve: bad sizes");
     const size_t n = (size_t)W * H;
-    k_convolve<<<blocks_for(n, kConvBS), kConvBS, 0, c.stream>>>(in, W, H, kern, K, out);
+    if ((size_t)K * K < 256) {
+        // the reference's spatial branch: per band, ky outer, kx inner, separate multiply and add
+        k_convolve<<<blocks_for(n, kConvBS), kConvBS, 0, c.stream>>>(in, W, H, kern, K, out);
+    } else {
+        const dim3 grid((W + kCvTW - 1) / kCvTW, (H + kCvTH - 1) / kCvTH, NB);
+        k_convolve_tiled<<<grid, kCvTW * kCvRows, 0, c.stream>>>(in, W, H, kern, K, out);
+    }
     VG_HIP(hipGetLastError());
     VG_HIP(hipStreamSynchronize(c.stream));
 }
