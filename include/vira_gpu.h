@@ -67,6 +67,7 @@ typedef struct {
     int max_leaf;   /* triangles per leaf, 1..15 (default 4)  */
     int bins;       /* SAH bins per node, 4..32 (default 16)  */
     int width;      /* 2 (binary, 64 B nodes) or 4 (collapsed, 128 B nodes; default) */
+    float traversal_cost; /* SAH cost of a node visit relative to a triangle test (0 = 1) */
 } vira_gpu_bvh_opts;
 
 typedef struct {

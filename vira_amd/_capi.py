@@ -47,7 +47,7 @@ class Projection(C.Structure):
 
 
 class BVHOpts(C.Structure):
-    _fields_ = [("max_leaf", C.c_int), ("bins", C.c_int), ("width", C.c_int)]
+    _fields_ = [("max_leaf", C.c_int), ("bins", C.c_int), ("width", C.c_int), ("traversal_cost", C.c_float)]
 
 
 class BVHStats(C.Structure):

@@ -235,7 +235,8 @@ __device__ __forceinline__ float half_area(const float lo[3], const float hi[3])
 
 // One thread builds the whole subtree of a small node.
 __global__ void k_build_small(const SmallTask* tasks, uint32_t ntasks, uint32_t* prims, const PrimBox* pb,
-                              BNodeG* nodes, uint32_t* node_counter, uint32_t* max_depth, int max_leaf) {
+                              BNodeG* nodes, uint32_t* node_counter, uint32_t* max_depth, int max_leaf,
+                              float trav_cost) {
     uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x;
     if (ti >= ntasks) return;
     struct Ent {
@@ -308,7 +309,7 @@ __global__ void k_build_small(const SmallTask* tasks, uint32_t ntasks, uint32_t*
                 float A = half_area(nd.lo, nd.hi);
                 float leaf_cost = (float)n * A;
                 if (split < 0) leaf = n <= (uint32_t)max_leaf;
-                else if (n <= (uint32_t)max_leaf && leaf_cost <= A + best) leaf = true;
+                else if (n <= (uint32_t)max_leaf && leaf_cost <= trav_cost * A + best) leaf = true;
             }
         }
         if (leaf) {
@@ -806,7 +807,8 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
         VG_HIP(hipMemcpyAsync(d_small.p, st.data(), st.size() * sizeof(SmallTask), hipMemcpyHostToDevice, c.stream));
         k_build_small<<<blocks_for(st.size(), 64), 64, 0, c.stream>>>(
             d_small.as<SmallTask>(), (uint32_t)st.size(), prims.as<uint32_t>(), pbuf.as<PrimBox>(),
-            nodes.as<BNodeG>(), d_counter.as<uint32_t>(), d_counter.as<uint32_t>() + 1, max_leaf);
+            nodes.as<BNodeG>(), d_counter.as<uint32_t>(), d_counter.as<uint32_t>() + 1, max_leaf,
+            opt.traversal_cost > 0.0f ? opt.traversal_cost : 1.0f);
         VG_HIP(hipGetLastError());
     }
     VG_HIP(hipMemcpyAsync(hc, d_counter.p, 8, hipMemcpyDeviceToHost, c.stream));
