@@ -28,7 +28,7 @@ namespace {
 constexpr uint32_t kChunk = 4096;      // slots per binning / partition workgroup
 constexpr int kChunkThreads = 256;
 constexpr int kChunkItems = kChunk / kChunkThreads;
-constexpr uint32_t kSmall = 512;       // nodes at or below this size are built per thread
+constexpr uint32_t kSmall = 256;       // nodes at or below this size are built per thread
 constexpr int kMaxBins = 32;
 constexpr int kSmallBins = 8;
 constexpr int kBinVals = 13;           // count, box lo/hi (6), centroid lo/hi (6)
@@ -705,8 +705,8 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
             for (uint32_t ti = 0; ti < binned.size(); ++ti) {
                 const HostTask& t = binned[ti];
                 const int* tb = &hbins[(size_t)ti * nb * kBinVals];
-                std::vector<float> rarea(nb), rclo(3 * nb), rchi(3 * nb), rlo(3 * nb), rhi(3 * nb);
-                std::vector<uint32_t> rcnt(nb);
+                float rarea[kMaxBins];
+                uint32_t rcnt[kMaxBins];
                 float rl[3] = {INFINITY, INFINITY, INFINITY}, rh[3] = {-INFINITY, -INFINITY, -INFINITY};
                 uint32_t rc = 0;
                 for (int b = nb - 1; b > 0; --b) {
