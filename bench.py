@@ -23,6 +23,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# FP32 FMA per second without packed FP32 (MI355X: 157.3 TFLOP/s counts packed FMA as 4 flops per lane)
+FP32_FMA_PEAK = 39.3e12
 COUNTS_FIXTURE = ROOT / "tests" / "golden" / "c1_traversal_counts.json"
 # HBM bytes per launch of the fused renderer from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # runs, scaled by the same counters on a copy of known size
@@ -93,6 +95,55 @@ def cpu_baseline(built, ctx, stride: int, threads: int):
         "sample_pixels": int(pix.size),
     }
     return res, counts
+
+
+def cpu_baseline_rows(built, passes) -> dict:
+    """The oracle's CPU restatements of the sensor tail, denoiser and star
+    renderer timed on bounded samples of the same inputs (one host thread each)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as O  # noqa: E402  (checker only, never the measured GPU path)
+    from vira_amd import sensor as SN
+    from vira_amd import workloads as W
+
+    out = {}
+    # pixel solid angles (FP64 Girard) of the full 2048^2 camera
+    cam = built.camera
+    t0 = time.perf_counter()
+    O.pixel_solid_angles(O.camera_from(cam.to_c()))
+    dt = time.perf_counter() - t0
+    out["received_power"] = {"value": round(cam.width * cam.height / dt / 1e6, 2), "unit": "Mpixel/s", "cores": 1,
+                             "kind": "port", "sample": f"{cam.width}x{cam.height} pixel solid angles in {dt:.2f} s"}
+    # 81x81 convolution of a 128x128 crop of the frame's radiance
+    psf = SN.GaussianPSF(1.5)
+    k81 = O.psf_kernel(O.psf_from(psf.to_c()), 81)
+    crop = passes.total_radiance[:128, :128].cpu().numpy()
+    t0 = time.perf_counter()
+    O.convolve(crop, k81)
+    dt = time.perf_counter() - t0
+    out["psf81_convolution"] = {"value": round(128 * 128 / dt / 1e6, 4), "unit": "Mpixel/s", "cores": 1,
+                                "kind": "port", "sample": f"128x128 crop x 8 bands, 81x81 kernel in {dt:.2f} s"}
+    # EATWT denoiser (5 + 5 levels) on a 512x512 crop of the passes
+    sl = (slice(0, 512), slice(0, 512))
+    arr = [t[sl].cpu().numpy() for t in (passes.direct_radiance, passes.indirect_radiance, passes.albedo,
+                                         passes.depth, passes.normal_global)]
+    t0 = time.perf_counter()
+    O.denoise(*arr)
+    dt = time.perf_counter() - t0
+    out["denoise_eatwt"] = {"value": round(512 * 512 / dt / 1e6, 3), "unit": "Mpixel/s", "cores": 1, "kind": "port",
+                            "sample": f"512x512 crop of the frame's passes, 5 + 5 levels in {dt:.2f} s"}
+    # config 4's 100,000 stars (the whole field)
+    scam, spsf = W.config4_star_camera()
+    ks = [O.psf_kernel(O.psf_from(spsf.to_c()), K) for K in (3, 9, 27, 81)]
+    dirs, irr, _ = W.make_star_field()
+    pmin = SN.minimum_detectable_power(scam.aperture_area(), scam.exposure_time)
+    R = np.asarray(scam.rotation, np.float64).astype(np.float32)
+    t0 = time.perf_counter()
+    O.render_stars(R, scam.width, scam.height, 1.0, scam.aperture_area(), ks, pmin, SN.intrinsic_matrix(scam),
+                   dirs, irr)
+    dt = time.perf_counter() - t0
+    out["stars"] = {"value": round(dirs.shape[0] / dt / 1e6, 3), "unit": "Msources/s", "cores": 1, "kind": "port",
+                    "sample": f"{dirs.shape[0]} sources into 4096^2 in {dt:.2f} s (kernels computed beforehand)"}
+    return out
 
 
 def adjoint_leg(ctx, steps: int, warmup: int) -> dict:
@@ -284,8 +335,25 @@ def sensor_leg(ctx, built, passes, steps: int) -> dict:
     dn_ms, _ = timed(lambda: ctx.denoise(work))
     print(f"# sensor tail (2048^2 x 8): received power {rp_ms:.2f} ms, 81x81 PSF convolution {cv_ms:.2f} ms, "
           f"photosite + noise {ss_ms:.2f} ms; EATWT denoiser {dn_ms:.2f} ms", file=sys.stderr, flush=True)
+    npx = passes.width * passes.height
+
+    def hbm(nbytes, ms):
+        a = nbytes / (ms * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": round(a, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(a / HBM_PEAK_GBS, 4)}
+
+    conv_fma = npx * 8 * 81 * 81
+    conv_rate = conv_fma / (cv_ms * 1e-3)
     return {"received_power_ms": round(rp_ms, 3), "psf81_convolution_ms": round(cv_ms, 3),
-            "sensor_ms": round(ss_ms, 3), "denoise_eatwt_ms": round(dn_ms, 3)}
+            "sensor_ms": round(ss_ms, 3), "denoise_eatwt_ms": round(dn_ms, 3),
+            # algorithmic bytes / FMA per call over the synchronous call's wall time
+            "roofline": {
+                "received_power": hbm(npx * (32 + 32 + 4), rp_ms),
+                "psf81_convolution": {"bound": "valu", "achieved": round(conv_rate / 1e12, 2), "peak": 39.3,
+                                      "unit": "TFMA/s (FP32, unpacked)", "frac": round(conv_rate / FP32_FMA_PEAK, 4)},
+                "sensor": hbm(npx * (32 + 4), ss_ms),
+                "denoise_eatwt": hbm(npx * 144 * 10, dn_ms),
+            }}
 
 
 def quipu_leg(ctx, built) -> dict:
@@ -410,6 +478,8 @@ def main() -> int:
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         cpu, counts = cpu_baseline(built, ctx, args.cpu_stride, threads)
+        if args.scene == "c1":
+            cpu["rows"] = cpu_baseline_rows(built, passes)
         if args.counts_out:
             Path(args.counts_out).parent.mkdir(parents=True, exist_ok=True)
             Path(args.counts_out).write_text(json.dumps({"scene": sc.name, **counts}, indent=1) + "\n")
