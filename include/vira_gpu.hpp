@@ -91,6 +91,30 @@ public:
     void setMaterial(int32_t mesh, const vira_gpu_material& m) { check(vira_gpu_material_set(h_, mesh, &m)); }
     void synchronize() { check(vira_gpu_synchronize(h_)); }
 
+    // --- camera sensor tail (camera.ipp:1175-1340) on W*H*8 device images
+    void receivedPower(const vira_gpu_camera& cam, const vira_gpu_sensor& sensor, const float* radiance_dev,
+                       float* power_dev, float* solid_angle_dev = nullptr) {
+        check(vira_gpu_received_power(h_, &cam, &sensor, radiance_dev, power_dev, solid_angle_dev));
+    }
+    // PointSpreadFunction::makeKernel (psf.ipp): size*size*8 floats
+    void psfKernel(const vira_gpu_psf& psf, int size, float* kernel_dev) {
+        check(vira_gpu_psf_kernel(h_, &psf, size, kernel_dev));
+    }
+    // Image::convolve (image.ipp:993-1065)
+    void convolve(const float* image_dev, int w, int h, const float* kernel_dev, int ksize, float* out_dev) {
+        check(vira_gpu_convolve(h_, image_dev, w, h, kernel_dev, ksize, out_dev));
+    }
+    // Camera::simulateSensor: photons (optional) and the normalised image
+    void simulateSensor(const vira_gpu_sensor_sim& cfg, const float* power_dev, int w, int h, float* photons_dev,
+                        float* image_dev) {
+        check(vira_gpu_simulate_sensor(h_, &cfg, power_dev, w, h, photons_dev, image_dev));
+    }
+    // denoiseSpectralRadianceEATWT (cpu_denoise.ipp:10-187), in place
+    void denoise(const vira_gpu_denoise_opts& o, int w, int h, float* direct_dev, float* indirect_dev,
+                 const float* albedo_dev, const float* depth_dev, const float* normal_dev, float* total_dev) {
+        check(vira_gpu_denoise(h_, &o, w, h, direct_dev, indirect_dev, albedo_dev, depth_dev, normal_dev, total_dev));
+    }
+
 private:
     vira_gpu_ctx* h_ = nullptr;
 };
@@ -134,6 +158,70 @@ public:
 
 private:
     Context* ctx_;
+};
+
+// CPUUnresolvedRenderer (cpu_unresolved_renderer.ipp:25-200): point sources
+// splatted into a received-power image; `psf` carries the kernels (device
+// pointers the caller owns), their edge maxima, the minimum power and K.
+class UnresolvedRenderer {
+public:
+    explicit UnresolvedRenderer(Context& ctx) : ctx_(&ctx) {}
+    vira_gpu_star_psf psf{};
+    void render(const vira_gpu_camera& camera, const vira_gpu_sensor& sensor, const float* directions_dev,
+                const float* irradiance_dev, size_t n, float* power_dev) {
+        check(vira_gpu_render_stars(ctx_->get(), &camera, &sensor, &psf, directions_dev, irradiance_dev, n,
+                                    power_dev));
+    }
+
+private:
+    Context* ctx_;
+};
+
+// DEMQuipu (dem_quipu.hpp:34-81): a .qld DEM pyramid.  readBuffers builds the
+// chosen level's mesh on the GPU and keeps it while the choice does not
+// change (dem_quipu.ipp:131-163); the mesh belongs to the context.
+class QuipuDEM {
+public:
+    explicit QuipuDEM(const std::string& path) { check(vira_gpu_qld_open(path.c_str(), &h_, &info_)); }
+    ~QuipuDEM() {
+        if (h_) (void)vira_gpu_qld_close(h_);
+    }
+    QuipuDEM(const QuipuDEM&) = delete;
+    QuipuDEM& operator=(const QuipuDEM&) = delete;
+    const vira_gpu_qld_info& info() const { return info_; }
+    int levels() const { return info_.levels; }
+    double gsd(int level) const {
+        double g = 0;
+        check(vira_gpu_qld_level_gsd(h_, level, &g));
+        return g;
+    }
+    vira_gpu_qld_read_opts options = defaultOptions();
+    static vira_gpu_qld_read_opts defaultOptions() {
+        vira_gpu_qld_read_opts o{};
+        o.read_albedos = 1;
+        o.default_albedo = 0.06f;
+        for (int k = 0; k < VIRA_GPU_NUM_BINS; ++k) o.albedo_profile[k] = 1.0f;
+        return o;
+    }
+    // required_gsd: the coarsest level whose GSD is <= it (infinity: the coarsest)
+    int32_t readBuffers(Context& ctx, double required_gsd) {
+        int32_t level = 0;
+        check(vira_gpu_qld_select(h_, required_gsd, &level));
+        if (level == level_ && mesh_ >= 0) return mesh_;
+        int32_t mesh = -1;
+        check(vira_gpu_qld_read(ctx.get(), h_, level, &options, &mesh, nullptr, nullptr, &last_));
+        if (mesh_ >= 0) ctx.releaseMesh(mesh_);
+        mesh_ = mesh;
+        level_ = level;
+        return mesh_;
+    }
+    const vira_gpu_qld_level& lastLevel() const { return last_; }
+
+private:
+    vira_gpu_qld* h_ = nullptr;
+    vira_gpu_qld_info info_{};
+    vira_gpu_qld_level last_{};
+    int32_t level_ = -1, mesh_ = -1;
 };
 
 // Batched requests in one persistent launch (vira_gpu_batch_submit).

@@ -40,7 +40,7 @@ FLAGS = [
 SOURCES = ["capi.hip", "dem.hip", "bvh.hip", "render.hip", "adjoint.hip", "batch.hip", "sensor.hip", "denoise.hip",
            "quipu.hip", "scan.hip"]
 # C++ programs over include/vira_gpu.hpp, linked against the library (host code only)
-EXAMPLES = ["render_dem.cpp"]
+EXAMPLES = ["render_dem.cpp", "quipu_scene.cpp"]
 
 
 def _compile(src: Path, obj: Path) -> None:
