@@ -595,6 +595,7 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
         VG_HIP(hipStreamSynchronize(c.stream));
         B.stats = vira_gpu_bvh_stats{};
         B.stats.num_nodes = 1;
+        for (int k = 0; k < 3; ++k) { B.lo[k] = 1.f; B.hi[k] = -1.f; }  // empty box: never entered
         B.built = true;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
@@ -622,6 +623,7 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     hnodes.reserve(1 << 16);
     BNodeG root{};
     for (int k = 0; k < 3; ++k) { root.lo[k] = key2f_h(hk[k]); root.hi[k] = key2f_h(hk[3 + k]); }
+    for (int k = 0; k < 3; ++k) { B.lo[k] = root.lo[k]; B.hi[k] = root.hi[k]; }
     root.left = 0;
     root.count = (uint32_t)N;
     hnodes.push_back(root);

@@ -88,6 +88,7 @@ struct Blas {
     uint64_t num_nodes = 0;
     vira_gpu_bvh_stats stats{};
     vira_gpu_bvh_opts opts{4, 16, 4};  // options of the last build (dem_update rebuilds with them)
+    float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {0.f, 0.f, 0.f};  // box of all triangles (empty: lo > hi)
 };
 
 struct MeshG {

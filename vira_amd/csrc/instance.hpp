@@ -27,6 +27,8 @@ struct InstG {
     float l2w[12];  // row-major 3x4: mesh frame -> camera-relative frame
     float w2l[12];  // its inverse
     float nm[9];    // normal matrix: transpose(inverse(linear part)), row-major
+    float blo[3], bhi[3];  // the BLAS's box in the mesh frame (lo > hi: empty)
+    float wlo[3], whi[3];  // that box placed in the camera-relative frame, rounded outward and padded
 };
 
 // GLM order: ((m0 x + m1 y) + m2 z) + m3 per row.
@@ -74,6 +76,30 @@ __device__ __forceinline__ void rebase_instance(const double* M, double cx, doub
         I.l2w[r * 4 + 3] = (float)t[r];
         I.w2l[r * 4 + 3] = (float)(-((inv[r][0] * t[0] + inv[r][1] * t[1]) + inv[r][2] * t[2]));
     }
+    // World box of the mesh box (FP64 corners), padded by 1e-5 of its size and
+    // position so that the FP32 ray transforms of the traversal can never
+    // reach a triangle the box test has already rejected.
+    if (!(I.blo[0] <= I.bhi[0])) {
+        for (int k = 0; k < 3; ++k) { I.wlo[k] = 1.f; I.whi[k] = -1.f; }
+        return;
+    }
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int c = 0; c < 8; ++c) {
+        const double p[3] = {(double)((c & 1) ? I.bhi[0] : I.blo[0]), (double)((c & 2) ? I.bhi[1] : I.blo[1]),
+                             (double)((c & 4) ? I.bhi[2] : I.blo[2])};
+        for (int r = 0; r < 3; ++r) {
+            const double w = ((A[r][0] * p[0] + A[r][1] * p[1]) + A[r][2] * p[2]) + t[r];
+            lo[r] = fmin(lo[r], w);
+            hi[r] = fmax(hi[r], w);
+        }
+    }
+    double pad = 0.0;
+    for (int r = 0; r < 3; ++r) pad = fmax(pad, fmax(fabs(lo[r]), fabs(hi[r])) + (hi[r] - lo[r]));
+    pad *= 1e-5;
+    for (int r = 0; r < 3; ++r) {
+        I.wlo[r] = __double2float_rd(lo[r] - pad);
+        I.whi[r] = __double2float_ru(hi[r] + pad);
+    }
 }
 
 // Closest hit (or occlusion) over all instances: the ray is carried into each
@@ -89,8 +115,14 @@ __device__ __forceinline__ HitRec trace_instances(const InstG* __restrict__ inst
     best.b1 = best.b2 = 0.f;
     hit_inst = 0;
     float bt = tmax;
+    // instances whose placed box the ray misses, or enters beyond the best hit, are skipped
+    const f3 inv = mk3(__fdiv_rn(1.0f, d.x), __fdiv_rn(1.0f, d.y), __fdiv_rn(1.0f, d.z));
+    const f3 e = mk3(slab_eps(d.x), slab_eps(d.y), slab_eps(d.z));
     for (int i = 0; i < n; ++i) {
         const InstG& I = inst[i];
+        if (!(slab<true>(I.wlo[0], I.wlo[1], I.wlo[2], I.whi[0], I.whi[1], I.whi[2], o, inv, e, bt) <
+              __builtin_huge_valf()))
+            continue;
         const f3 ol = xform_point(I.w2l, o), dl = xform_dir(I.w2l, d);
         const HitRec h = traverse<W>(I.nodes, I.tris, ol, dl, bt, any, lds_ref, lds_dist, stride);
         if (h.id == 0xffffffffu) continue;

@@ -336,6 +336,10 @@ void render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, c
             hi[i].verts = mi.verts.as<GVertex>();
             hi[i].idx = mi.idx.as<uint32_t>();
             hi[i].mat = mat_params(mi);
+            for (int k = 0; k < 3; ++k) {
+                hi[i].blo[k] = mi.blas.lo[k];
+                hi[i].bhi[k] = mi.blas.hi[k];
+            }
             for (int k = 0; k < 16; ++k) hm[16 * i + k] = c.tlas[i].transform[k];
         }
         for (int l = 0; l < nl; ++l)
