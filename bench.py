@@ -25,7 +25,7 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 # FP32 FMA per second without packed FP32 (MI355X: 157.3 TFLOP/s counts packed FMA as 4 flops per lane)
 FP32_FMA_PEAK = 39.3e12
-COUNTS_FIXTURE = ROOT / "tests" / "golden" / "c1_traversal_counts.json"
+COUNTS_FIXTURE = ROOT / "tests" / "golden" / "dem4097_2048px_traversal_counts.json"
 # HBM bytes per launch of the fused renderer from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # runs, scaled by the same counters on a copy of known size
 TRAFFIC_PROFILE = ROOT / "profiles" / "k_render_fused_dem4097_2048px_traffic.json"
@@ -131,8 +131,8 @@ def cpu_baseline_rows(built, passes) -> dict:
     dt = time.perf_counter() - t0
     out["denoise_eatwt"] = {"value": round(512 * 512 / dt / 1e6, 3), "unit": "Mpixel/s", "cores": 1, "kind": "port",
                             "sample": f"512x512 crop of the frame's passes, 5 + 5 levels in {dt:.2f} s"}
-    # config 4's 100,000 stars (the whole field)
-    scam, spsf = W.config4_star_camera()
+    # the 100,000-star field (the whole field)
+    scam, spsf = W.star_field_camera()
     ks = [O.psf_kernel(O.psf_from(spsf.to_c()), K) for K in (3, 9, 27, 81)]
     dirs, irr, _ = W.make_star_field()
     pmin = SN.minimum_detectable_power(scam.aperture_area(), scam.exposure_time)
@@ -147,7 +147,7 @@ def cpu_baseline_rows(built, passes) -> dict:
 
 
 def adjoint_leg(ctx, steps: int, warmup: int) -> dict:
-    """Config 2: 1025^2 DEM (2,097,152 triangles), 1024^2 image, fixed-visibility
+    """Differentiable DI: 1025^2 DEM (2,097,152 triangles), 1024^2 image, fixed-visibility
     direct illumination.  Times the forward render (keeping hit records) and the
     adjoint (dL/dimage of the L2 loss -> dL/dheights for 1,050,625 nodes and
     dL/dtexels for 1024^2 texels), each a synchronous call, and the adjoint that
@@ -156,7 +156,7 @@ def adjoint_leg(ctx, steps: int, warmup: int) -> dict:
     from vira_amd import scene as S
     from vira_amd import workloads as W
 
-    sc = W.CONFIG2
+    sc = W.DEM1025_DI
     d = W.build_diff(ctx, sc)
     b = d.built
     gh, gt = torch.empty_like(b.heights), torch.empty_like(b.texture)
@@ -187,7 +187,7 @@ def adjoint_leg(ctx, steps: int, warmup: int) -> dict:
         res[label] = (fwd, adj_ms)
     update_ms = timed(lambda: ctx.dem_update(b.mesh, b.heights), max(1, steps))
     fwd, adj_ms = res["kept"]
-    print(f"# config 2 ({sc.name}): forward {fwd:.3f} ms, adjoint {adj_ms:.3f} ms (records kept), "
+    print(f"# DI adjoint ({sc.name}): forward {fwd:.3f} ms, adjoint {adj_ms:.3f} ms (records kept), "
           f"adjoint {res['retraced'][1]:.3f} ms (re-traced); mesh update + BVH rebuild {update_ms:.1f} ms",
           file=sys.stderr, flush=True)
     return {"workload": sc.name, "forward_ms": round(fwd, 3), "adjoint_ms": round(adj_ms, 3),
@@ -197,8 +197,8 @@ def adjoint_leg(ctx, steps: int, warmup: int) -> dict:
             "mesh_update_bvh_rebuild_ms": round(update_ms, 2)}
 
 
-def batch_leg(ctx, c1_built, steps: int, warmup: int) -> dict:
-    """Config 3: 64 requests (image side uniform over {128, 256, 512, 1024,
+def batch_leg(ctx, dem_built, steps: int, warmup: int) -> dict:
+    """Mixed batch: 64 requests (image side uniform over {128, 256, 512, 1024,
     2048}, spp uniform over {1, 4, 16, 64}, 3 bounces + RR, one of four meshes:
     the DEMs of configs 0-2 and a 200k-triangle small body; random camera and
     sun; seed 3).  Times the per-request form (one render call each) and the
@@ -207,7 +207,7 @@ def batch_leg(ctx, c1_built, steps: int, warmup: int) -> dict:
     import torch
     from vira_amd import workloads as W
 
-    meshes = W.build_batch_meshes(ctx, c1_built)
+    meshes = W.build_batch_meshes(ctx, dem_built)
     reqs = W.make_batch(ctx, meshes, n=64, seed=3)
     t0 = time.perf_counter()
     cfg, tune_ms = ctx.batch_autotune(reqs, sample_stride=16)
@@ -233,7 +233,7 @@ def batch_leg(ctx, c1_built, steps: int, warmup: int) -> dict:
     st = {}
     batch_s = timed(lambda: st.update(ctx.batch_submit(reqs)), steps)
     rays = st["primary_rays"] + st["shadow_rays"]
-    print(f"# config 3: 64 requests, {st['tiles']} tiles, {rays} primary+shadow rays; per-request {plain_s * 1e3:.1f} ms, "
+    print(f"# mixed batch: 64 requests, {st['tiles']} tiles, {rays} primary+shadow rays; per-request {plain_s * 1e3:.1f} ms, "
           f"batched {batch_s * 1e3:.1f} ms (kernel {st['ms']:.1f} ms, graph reused {st['graph_reused']}); "
           f"autotune {tune_s:.1f} s -> {cfg}; identical passes: {identical}", file=sys.stderr, flush=True)
     return {"workload": "batch64_dem_smallbody_seed3", "plain_ms": round(plain_s * 1e3, 2),
@@ -244,8 +244,8 @@ def batch_leg(ctx, c1_built, steps: int, warmup: int) -> dict:
             "autotune_s": round(tune_s, 2)}
 
 
-def instanced_leg(ctx, c1_built, steps: int, warmup: int) -> dict:
-    """Config 4: two-level scene (the config-1 tile + 64k sphere for the rest of
+def instanced_leg(ctx, dem_built, steps: int, warmup: int) -> dict:
+    """Instanced scene: two-level scene (the 4097^2 DEM tile + 64k sphere for the rest of
     the Moon, 200k small body, 50k spacecraft) with FP64 model matrices at
     ~1.5e8 km, camera on the spacecraft; 4096^2, 8 spp, 2 bounces, sun as a
     sphere light at the origin.  Rebasing into the camera frame runs on the GPU
@@ -254,7 +254,7 @@ def instanced_leg(ctx, c1_built, steps: int, warmup: int) -> dict:
     from vira_amd import scene as S
     from vira_amd import workloads as W
 
-    isc = W.build_instanced(ctx, c1_built)
+    isc = W.build_instanced(ctx, dem_built)
     passes = S.RenderPasses(4096, 4096, which=("radiance", "depth", "tri_id", "instance_id"))
     for _ in range(warmup):
         ctx.render(isc.camera, isc.options, passes)
@@ -268,10 +268,10 @@ def instanced_leg(ctx, c1_built, steps: int, warmup: int) -> dict:
     rays = stats[0]["primary_rays"] + stats[0]["shadow_rays"]
     inst = passes.instance_id.reshape(-1)
     seen = sorted(int(v) for v in torch.unique(inst[inst >= 0]).cpu())
-    print(f"# config 4: {dt * 1e3:.1f} ms/frame, {rays / dt / 1e6:.1f} Mrays/s (primary+shadow), instances in view "
+    print(f"# instanced scene: {dt * 1e3:.1f} ms/frame, {rays / dt / 1e6:.1f} Mrays/s (primary+shadow), instances in view "
           f"{seen}", file=sys.stderr, flush=True)
-    ctx.tlas_build([c1_built.mesh])  # back to the config-1 scene
-    ctx.lights_set(c1_built.lights)
+    ctx.tlas_build([dem_built.mesh])  # back to the 4097^2 DEM scene
+    ctx.lights_set(dem_built.lights)
     stars = star_leg(ctx, steps)
     return {"workload": "instanced_moon_smallbody_spacecraft_4096px_8spp_2b", "ms_per_frame": round(dt * 1e3, 2),
             "mrays_s": round(rays / dt / 1e6, 1), "bounce_rays": stats[0]["bounce_rays"], "instances_in_view": seen,
@@ -279,13 +279,13 @@ def instanced_leg(ctx, c1_built, steps: int, warmup: int) -> dict:
 
 
 def star_leg(ctx, steps: int) -> dict:
-    """Config 4's star field: 100,000 stars (V magnitudes exponential on 0-10,
+    """The star field: 100,000 stars (V magnitudes exponential on 0-10,
     uniform on the sky) through the unresolved renderer into the 4096^2 camera
     with its Airy-disk PSF (kernels 3/9/27/81 chosen per star)."""
     import torch
     from vira_amd import workloads as W
 
-    cam, psf = W.config4_star_camera()
+    cam, psf = W.star_field_camera()
     sp = ctx.star_psf(psf, cam)
     dirs, irr, _ = W.make_star_field()
     d = torch.from_numpy(dirs).cuda()
@@ -299,12 +299,12 @@ def star_leg(ctx, steps: int) -> dict:
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
     lit = int((out[:, :, 0] > 0).sum().item())
-    print(f"# config 4 stars: 100,000 sources in {dt * 1e3:.2f} ms ({lit} pixels lit)", file=sys.stderr, flush=True)
+    print(f"# stars: 100,000 sources in {dt * 1e3:.2f} ms ({lit} pixels lit)", file=sys.stderr, flush=True)
     return {"sources": int(dirs.shape[0]), "ms": round(dt * 1e3, 3), "pixels_lit": lit}
 
 
 def sensor_leg(ctx, built, passes, steps: int) -> dict:
-    """The sensor tail after a config-1 frame: received power (pixel solid
+    """The sensor tail after a 4097^2 DEM frame: received power (pixel solid
     angles by Girard's theorem in FP64), convolution with an 81x81 Gaussian PSF
     (the largest of PointSpreadFunction's kernels, used for resolved renders)
     and the photosite/noise simulation, on the 2048^2 x 8-band image."""
@@ -357,7 +357,7 @@ def sensor_leg(ctx, built, passes, steps: int) -> dict:
 
 
 def quipu_leg(ctx, built) -> dict:
-    """Config 1's DEM (4097^2) as a quipu pyramid (levels 4097^2, 2049^2,
+    """The 4097^2 DEM as a quipu pyramid (levels 4097^2, 2049^2,
     1025^2; 16-bit heights in LZ4 blocks) written to a temporary file, then read
     back: file parse + LZ4 decode on the host, dequantisation and mesh on the
     GPU, and the BVH build of the finest level."""
@@ -374,7 +374,7 @@ def quipu_leg(ctx, built) -> dict:
                             dy=float(1 << k))
         levels.append(Q.DEMLevel(hk, p))
     with tempfile.TemporaryDirectory() as tmp:
-        path = f"{tmp}/c1.qld"
+        path = f"{tmp}/dem4097.qld"
         t0 = time.perf_counter()
         Q.write_pyramid(path, levels, compress=True)
         write_s = time.perf_counter() - t0
@@ -405,14 +405,14 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--scene", default="c1", choices=["c0", "c1"])
+    ap.add_argument("--scene", default="dem4097", choices=["dem257", "dem4097"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=1, help="time every n-th pixel row/column on the CPU")
     ap.add_argument("--counts-out", default=None, help="write the oracle traversal counts (fixture) to this path")
     ap.add_argument("--integrator", default="auto", choices=["auto", "fused", "wavefront"])
-    ap.add_argument("--no-adjoint", action="store_true", help="skip the config-2 adjoint/forward leg")
-    ap.add_argument("--no-batch", action="store_true", help="skip the config-3 batched/per-request leg")
-    ap.add_argument("--no-instanced", action="store_true", help="skip the config-4 instanced-scene leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable DI adjoint/forward leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the mixed-batch batched/per-request leg")
+    ap.add_argument("--no-instanced", action="store_true", help="skip the instanced-scene leg")
     ap.add_argument("--no-quipu", action="store_true", help="skip the quipu pyramid read leg")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size kernel for scaling "
@@ -427,7 +427,7 @@ def main() -> int:
     from vira_amd import scene as S
     from vira_amd import workloads as W
 
-    sc = W.CONFIG1 if args.scene == "c1" else W.CONFIG0
+    sc = W.DEM4097 if args.scene == "dem4097" else W.DEM257
     if args.copy_calibration_mib > 0:
         n = args.copy_calibration_mib * (1 << 20) // 4
         src = torch.ones(n, dtype=torch.float32, device="cuda")
@@ -467,23 +467,23 @@ def main() -> int:
           f"{sum(s['trace_shadow_ms'] for s in stats):.1f} ms, shading {sum(s['shade_ms'] for s in stats):.1f} ms",
           file=sys.stderr, flush=True)
 
-    sensor = sensor_leg(ctx, built, passes, max(1, min(3, args.steps))) if args.scene == "c1" else None
-    quipu = quipu_leg(ctx, built) if (args.scene == "c1" and not args.no_quipu) else None
+    sensor = sensor_leg(ctx, built, passes, max(1, min(3, args.steps))) if args.scene == "dem4097" else None
+    quipu = quipu_leg(ctx, built) if (args.scene == "dem4097" and not args.no_quipu) else None
     adjoint = None if args.no_adjoint else adjoint_leg(ctx, max(3, args.steps), max(1, args.warmup))
-    batch = None if (args.no_batch or args.scene != "c1") else batch_leg(ctx, built, max(1, min(3, args.steps)), 1)
-    instanced = None if (args.no_instanced or args.scene != "c1") else instanced_leg(ctx, built, max(1, min(3, args.steps)), 1)
+    batch = None if (args.no_batch or args.scene != "dem4097") else batch_leg(ctx, built, max(1, min(3, args.steps)), 1)
+    instanced = None if (args.no_instanced or args.scene != "dem4097") else instanced_leg(ctx, built, max(1, min(3, args.steps)), 1)
 
     cpu = None
     counts = None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         cpu, counts = cpu_baseline(built, ctx, args.cpu_stride, threads)
-        if args.scene == "c1":
+        if args.scene == "dem4097":
             cpu["rows"] = cpu_baseline_rows(built, passes)
         if args.counts_out:
             Path(args.counts_out).parent.mkdir(parents=True, exist_ok=True)
             Path(args.counts_out).write_text(json.dumps({"scene": sc.name, **counts}, indent=1) + "\n")
-    if counts is None and COUNTS_FIXTURE.exists() and args.scene == "c1":
+    if counts is None and COUNTS_FIXTURE.exists() and args.scene == "dem4097":
         counts = json.loads(COUNTS_FIXTURE.read_text())
     roofline = None
     if counts is not None and counts.get("closest_rays", 0) > 0:

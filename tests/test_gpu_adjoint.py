@@ -1,4 +1,4 @@
-"""GPU differentiable direct illumination (config 2 and scaled copies) against
+"""GPU differentiable direct illumination (the 1025^2 DEM scene and scaled copies) against
 the FP64 oracle model (oracle/oracle_diff.hpp), through the C-ABI.
 
 Bar:
@@ -38,7 +38,7 @@ def _oracle(built):
 
 
 def _small(smooth=False, spp=1):
-    sc = W.scaled(W.CONFIG2, dem=65, image=96, texture=64)
+    sc = W.scaled(W.DEM1025_DI, dem=65, image=96, texture=64)
     return replace(sc, smooth_shading=smooth, spp=spp)
 
 
@@ -221,10 +221,10 @@ def test_autograd_function_and_descent(gpu_ctx):
     assert losses[-1] < 0.8 * losses[0], losses
 
 
-def test_config2_full_size_adjoint(gpu_ctx):
-    """Config 2 at full size: 1025^2 DEM (2,097,152 triangles), 1024^2 image."""
+def test_dem1025_full_size_adjoint(gpu_ctx):
+    """Full size: 1025^2 DEM (2,097,152 triangles), 1024^2 image."""
     import torch
-    sc = W.CONFIG2
+    sc = W.DEM1025_DI
     b = W.build(gpu_ctx, sc)
     opts = S.DIOptions(samples=1, seed=0)
     img = gpu_ctx.di_render(b.camera, opts)
@@ -233,13 +233,13 @@ def test_config2_full_size_adjoint(gpu_ctx):
     g = rng.standard_normal(img.shape).astype(np.float32)
     gh, gt = torch.empty_like(b.heights), torch.empty_like(b.texture)
     st = gpu_ctx.adjoint(b.camera, opts, torch.from_numpy(g).cuda(), gh, gt)
-    print(f"config 2: forward {fwd['ms']:.2f} ms, adjoint {st['ms']:.2f} ms (records kept: {st['used_records']})")
+    print(f"1025^2 DI: forward {fwd['ms']:.2f} ms, adjoint {st['ms']:.2f} ms (records kept: {st['used_records']})")
     om, cam, lights, mat, dd, heights, tex = _oracle(b)
     out = om.render(cam, lights, mat, O.options(1, 0, False, 1, 0), precision=32, records=True)
     assert np.array_equal(img.cpu().numpy().reshape(-1, 8).view(np.uint32), out["radiance"].view(np.uint32))
     gh64, gt64 = dd.adjoint(heights, tex, out["records"], lights, mat, g.reshape(-1, 8).astype(np.float64))
     eh = _rel_l2(gh.cpu().numpy().reshape(-1).astype(np.float64), gh64)
     et = _rel_l2(gt.cpu().numpy().reshape(-1).astype(np.float64), gt64)
-    print(f"config 2: rel L2 error heights {eh:.2e}, texels {et:.2e}")
+    print(f"1025^2 DI: rel L2 error heights {eh:.2e}, texels {et:.2e}")
     assert eh <= 1e-3 and et <= 1e-3
     _fd_check(gpu_ctx, b, opts, 1000, 12)

@@ -1,4 +1,4 @@
-"""Batched rendering (config 3 and scaled copies): one persistent,
+"""Batched rendering (the 64-request mixed batch and scaled copies): one persistent,
 graph-captured launch over many requests must write, for every request, the
 passes the per-request render writes -- bit for bit -- and the small-body mesh
 must render as the oracle does."""
@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 @pytest.fixture(scope="module")
 def small_meshes(gpu_ctx):
     ids, scenes, keep = [], [], []
-    for sc, dem in ((W.CONFIG0, 65), (W.CONFIG1, 129), (W.CONFIG2, 65)):
+    for sc, dem in ((W.DEM257, 65), (W.DEM4097, 129), (W.DEM1025_DI, 65)):
         s2 = W.scaled(sc, dem=dem, texture=64 if sc.texture else None)
         b = W.build(gpu_ctx, s2)
         ids.append(b.mesh)

@@ -64,7 +64,7 @@ def _random(H, Wd, seed):
 
 def test_denoise_render_passes(gpu_ctx):
     import torch
-    b = W.build(gpu_ctx, W.scaled(W.CONFIG0, image=256))
+    b = W.build(gpu_ctx, W.scaled(W.DEM257, image=256))
     passes = S.RenderPasses(256, 256)
     gpu_ctx.render(b.camera, S.PathTracerOptions(samples=4, bounces=2, seed=7), passes)
     torch.cuda.synchronize()
@@ -101,7 +101,7 @@ def test_denoise_non_square_levels(gpu_ctx, levels):
 
 def test_render_with_denoise_option(gpu_ctx):
     import torch
-    b = W.build(gpu_ctx, W.scaled(W.CONFIG0, image=128))
+    b = W.build(gpu_ctx, W.scaled(W.DEM257, image=128))
     opts = S.PathTracerOptions(samples=2, bounces=1, seed=3)
     p1 = S.RenderPasses(128, 128)
     gpu_ctx.render(b.camera, opts, p1)

@@ -1,4 +1,4 @@
-"""Two-level instanced scenes (config 4 and scaled copies): four bodies placed by
+"""Two-level instanced scenes (the 1 AU Moon scene and scaled copies): four bodies placed by
 FP64 model matrices about 1.5e8 km from the origin, rendered camera-relative.
 
 Bar: the oracle's FP32 twin (oracle render_scene, the same FP64 rebasing
@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _oracle_meshes(ctx, isc, dem):
-    sc = W.scaled(W.CONFIG1, dem=dem, texture=4 * (dem - 1)) if dem != W.CONFIG1.dem else W.CONFIG1
+    sc = W.scaled(W.DEM4097, dem=dem, texture=4 * (dem - 1)) if dem != W.DEM4097.dem else W.DEM4097
     heights = isc.keep[0].cpu().numpy()
     tex = isc.keep[1].cpu().numpy()
     p = W.make_projection(sc)
@@ -110,8 +110,8 @@ def test_tlas_validation(gpu_ctx):
     assert "identity" in str(e.value)
 
 
-def test_config4_full_size_sample(gpu_ctx):
-    """Config 4 at full size (33.5M-triangle tile, 4096^2, 8 spp, 2 bounces):
+def test_instanced_full_size_sample(gpu_ctx):
+    """The instanced scene at full size (33.5M-triangle tile, 4096^2, 8 spp, 2 bounces):
     2,048 random pixels against the oracle's FP32 twin."""
     isc = W.build_instanced(gpu_ctx)
     passes, st = _render(gpu_ctx, isc)
@@ -128,6 +128,6 @@ def test_config4_full_size_sample(gpu_ctx):
     ref = out["radiance"][pix].astype(np.float64)
     lit = ref > 1e-6 * ref.max()
     rel = np.abs(rad - ref)[lit] / ref[lit]
-    print(f"config 4: {st['primary_rays'] + st['shadow_rays']} primary+shadow rays in {st['total_ms']:.1f} ms; "
+    print(f"instanced scene: {st['primary_rays'] + st['shadow_rays']} primary+shadow rays in {st['total_ms']:.1f} ms; "
           f"sample radiance rel err median {np.median(rel):.2e}")
     assert np.mean(rel <= 1e-5) >= 0.97

@@ -98,7 +98,7 @@ def _rays_over_dem(n, extent, rng):
 @pytest.mark.parametrize("width", [2, 4])
 def test_trace_rays_bit_exact(gpu_ctx, width):
     import torch
-    sc = W.CONFIG0
+    sc = W.DEM257
     built = W.build(gpu_ctx, sc, bvh_width=width)
     assert built.bvh["width"] == width and built.bvh["num_wide_nodes"] > 0
     om, *_ = oracle_scene(built)
@@ -134,8 +134,8 @@ def test_trace_rays_bit_exact(gpu_ctx, width):
 
 
 @pytest.mark.parametrize("integrator,width", [("fused", 4), ("wavefront", 4), ("fused", 2)])
-def test_render_config0_parity(gpu_ctx, integrator, width):
-    sc = W.CONFIG0
+def test_render_dem257_parity(gpu_ctx, integrator, width):
+    sc = W.DEM257
     built = W.build(gpu_ctx, sc, bvh_width=width)
     rp = S.RenderPasses(sc.image, sc.image)
     built.options.integrator = integrator
@@ -168,7 +168,7 @@ def test_render_config0_parity(gpu_ctx, integrator, width):
 
 @pytest.mark.parametrize("integrator", ["fused", "wavefront"])
 def test_render_bounces_sphere_light_small(gpu_ctx, integrator):
-    sc = W.scaled(W.CONFIG1, dem=129, image=64, spp=4, texture=64)
+    sc = W.scaled(W.DEM4097, dem=129, image=64, spp=4, texture=64)
     built = W.build(gpu_ctx, sc)
     rp = S.RenderPasses(sc.image, sc.image)
     built.options.integrator = integrator
@@ -189,7 +189,7 @@ def test_render_bounces_sphere_light_small(gpu_ctx, integrator):
 
 def test_render_two_point_lights_wavefront(gpu_ctx):
     """Several lights: shadow rays per light, contributions summed in light order."""
-    sc = W.scaled(W.CONFIG0, dem=97, image=96, spp=2, bounces=1)
+    sc = W.scaled(W.DEM257, dem=97, image=96, spp=2, bounces=1)
     built = W.build(gpu_ctx, sc)
     lights = built.lights + [S.directional_sun(55.0, 140.0, [0.5, 1, 1.5, 2, 2.5, 3, 3.5, 4])]
     gpu_ctx.lights_set(lights)
@@ -217,7 +217,7 @@ def test_render_two_point_lights_wavefront(gpu_ctx):
 
 def test_fused_equals_wavefront_bounces(gpu_ctx):
     """The two schedules run the same formulas: identical images."""
-    sc = W.scaled(W.CONFIG1, dem=257, image=128, spp=3, texture=256)
+    sc = W.scaled(W.DEM4097, dem=257, image=128, spp=3, texture=256)
     built = W.build(gpu_ctx, sc)
     out = {}
     for integ in ("fused", "wavefront"):

@@ -54,7 +54,7 @@ def test_monte_carlo_mean_within_3_sigma(gpu_ctx):
       terminators slightly, a systematic effect the FP64 oracle does not have.
     """
     import torch
-    sc = W.scaled(W.CONFIG1, dem=65, image=32, spp=4, bounces=2, texture=64)
+    sc = W.scaled(W.DEM4097, dem=65, image=32, spp=4, bounces=2, texture=64)
     b = W.build(gpu_ctx, sc)
     passes = S.RenderPasses(32, 32)
     gpu = []

@@ -7,7 +7,7 @@ the reference's sequential loop does, so the FP32 sums are the same.  Cases:
 a random sky with overlaps, a cluster of 6,000 sources on one 32x32 tile (more
 than the 4,096 the per-tile sort holds: the scan-every-source path), a camera
 without PSF (assignment, last source wins), culling behind the camera,
-accumulation into a non-zero image, and config 4's 100,000-star field at
+accumulation into a non-zero image, and the 100,000-star field at
 4096^2."""
 import numpy as np
 import pytest
@@ -127,10 +127,10 @@ def test_empty_and_invalid(gpu_ctx):
     assert "odd" in str(e.value)
 
 
-def test_config4_star_field_full_size(gpu_ctx):
+def test_star_field_full_size(gpu_ctx):
     """100,000 stars (magnitudes exponential on 0-10, uniform on the sky) into
-    config 4's 4096^2 camera with the Airy-disk PSF of its optics."""
-    cam, psf = W.config4_star_camera()
+    the instanced scene's 4096^2 camera with the Airy-disk PSF of its optics."""
+    cam, psf = W.star_field_camera()
     sp = gpu_ctx.star_psf(psf, cam)
     dirs, irr, _ = W.make_star_field()
     got = _run(gpu_ctx, cam, sp, dirs, irr)

@@ -360,8 +360,8 @@ def test_point_light_closed_form(precision):
     assert np.allclose(out["albedo"], alb) and np.allclose(out["normal"], [0, 0, 1])
 
 
-def test_config0_fp32_vs_fp64():
-    sc = W.CONFIG0
+def test_dem257_fp32_vs_fp64():
+    sc = W.DEM257
     h = O.fbm_heights(sc.dem, sc.dem, sc.seed)
     p = W.make_projection(sc)
     proj = O.make_projection(p.kind, p.spacing, p.radius, tuple(p.center), tuple(p.offset))

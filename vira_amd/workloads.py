@@ -44,17 +44,17 @@ class DemScene:
         return 2 * (self.dem - 1) ** 2
 
 
-# Config 0: 257^2 DEM, 512^2, 1 spp, primary + shadow, deterministic.
-CONFIG0 = DemScene("dem257_512px_1spp_direct", dem=257, seed=0)
-# Config 1: 4097^2 DEM (33.5M triangles) on a 1737 km sphere, 2048^2, 16 spp, 3 bounces + RR.
-CONFIG1 = DemScene("dem4097_2048px_16spp_3b", dem=4097, seed=1, sphere_radius=MOON_RADIUS, altitude=100e3,
+# 257^2 DEM, 512^2, 1 spp, primary + shadow, deterministic.
+DEM257 = DemScene("dem257_512px_1spp_direct", dem=257, seed=0)
+# 4097^2 DEM (33.5M triangles) on a 1737 km sphere, 2048^2, 16 spp, 3 bounces + RR.
+DEM4097 = DemScene("dem4097_2048px_16spp_3b", dem=4097, seed=1, sphere_radius=MOON_RADIUS, altitude=100e3,
                    image=2048, spp=16, bounces=3, russian_roulette=True, sun_kind="sphere",
                    albedo="texture", texture=4096)
 
-# Config 2: 1025^2 DEM (2.1M triangles), 1024^2, direct illumination with fixed
+# 1025^2 DEM (2.1M triangles), 1024^2, direct illumination with fixed
 # visibility; per-texel albedo (1024^2 log-normal texture) and the heights are
 # the differentiable parameters.  40 km altitude: the DEM fills the image.
-CONFIG2 = DemScene("dem1025_1024px_di_adjoint", dem=1025, seed=2, altitude=40000.0, image=1024,
+DEM1025_DI = DemScene("dem1025_1024px_di_adjoint", dem=1025, seed=2, altitude=40000.0, image=1024,
                    albedo="texture", texture=1024)
 
 
@@ -186,7 +186,7 @@ def perturbed_inputs(ctx: S.Context, sc: DemScene, heights, texture, height_sigm
 
 
 def build_diff(ctx: S.Context, sc: DemScene, options: S.DIOptions | None = None, bvh_width: int = 4) -> DiffScene:
-    """Config-2 style differentiable scene: the base DEM on the GPU plus the
+    """Differentiable scene of the 1025^2 kind: the base DEM on the GPU plus the
     target image rendered from the perturbed heights and texture."""
     from .autograd import DifferentiableDem
     b = build(ctx, sc, bvh_width=bvh_width)
@@ -199,7 +199,7 @@ def build_diff(ctx: S.Context, sc: DemScene, options: S.DIOptions | None = None,
     return DiffScene(b, dem, target, th, tt)
 
 
-# ---------------------------------------------------------------- config 3: batched mixed workload
+# ---------------------------------------------------------------- batched mixed workload (64 requests)
 def _lowbias32(x):
     x = np.asarray(x, dtype=np.uint64) & 0xFFFFFFFF
     x ^= x >> 16
@@ -282,17 +282,17 @@ BATCH_SPP = (1, 4, 16, 64)
 
 @dataclass
 class BatchMeshes:
-    ids: list          # mesh ids: config-0 DEM, config-1 DEM, config-2 DEM, small body
+    ids: list          # mesh ids: 257^2 DEM, 4097^2 DEM, 1025^2 DEM, small body
     scenes: list       # DemScene (or None for the small body)
     keep: list         # tensors the meshes' materials point at
 
 
-def build_batch_meshes(ctx: S.Context, c1_built: "BuiltScene | None" = None, bvh_width: int = 4) -> BatchMeshes:
+def build_batch_meshes(ctx: S.Context, dem_built: "BuiltScene | None" = None, bvh_width: int = 4) -> BatchMeshes:
     """The four meshes of the batched workload: the DEMs of configs 0, 1 and 2
     (with their albedo) and the 200k-triangle small body (albedo 0.12)."""
     ids, scenes, keep = [], [], []
-    for sc in (CONFIG0, CONFIG1, CONFIG2):
-        b = c1_built if (sc is CONFIG1 and c1_built is not None) else build(ctx, sc, bvh_width=bvh_width)
+    for sc in (DEM257, DEM4097, DEM1025_DI):
+        b = dem_built if (sc is DEM4097 and dem_built is not None) else build(ctx, sc, bvh_width=bvh_width)
         ids.append(b.mesh)
         scenes.append(sc)
         keep.append(b.texture)
@@ -303,7 +303,7 @@ def build_batch_meshes(ctx: S.Context, c1_built: "BuiltScene | None" = None, bvh
     ctx.material_set(mid, albedo=0.12)
     ids.append(mid)
     scenes.append(None)
-    # the config-1 material must point at its texture again (build() of other scenes does not touch it)
+    # the 4097^2 DEM's material must point at its texture again (build() of other scenes does not touch it)
     return BatchMeshes(ids, scenes, keep)
 
 
@@ -314,7 +314,7 @@ def _unit(v):
 
 def make_batch(ctx: S.Context, meshes: BatchMeshes, n: int = 64, seed: int = 3, bounces: int = 3,
                passes=("radiance", "depth", "tri_id"), sides=BATCH_SIDES, spps=BATCH_SPP):
-    """Config 3: n requests, image side uniform over `sides`, spp uniform over
+    """Mixed batch: n requests, image side uniform over `sides`, spp uniform over
     `spps`, mesh uniform over the four meshes, random camera and sun per request
     (numpy Generator seeded with `seed`)."""
     rng = np.random.default_rng(seed)
@@ -337,7 +337,7 @@ def make_batch(ctx: S.Context, meshes: BatchMeshes, n: int = 64, seed: int = 3, 
             sd = _unit(rng.standard_normal(3))
             e = _unit(np.cross(sd, [0.0, 0.0, 1.0] if abs(sd[2]) < 0.9 else [1.0, 0.0, 0.0]))
             light = S.directional_sun(90.0, 0.0, 1.0, up=tuple(sd), east=tuple(e))
-        elif sc.sphere_radius > 0:  # config-1 DEM: rebased frame, ground 100 km below along -x
+        elif sc.sphere_radius > 0:  # 4097^2 DEM: rebased frame, ground 100 km below along -x
             ext = (sc.dem - 1) * sc.spacing
             pos = (rng.uniform(-0.3, 0.3) * sc.altitude, rng.uniform(-0.15, 0.15) * ext,
                    rng.uniform(-0.15, 0.15) * ext)
@@ -367,7 +367,7 @@ def render_plain(ctx: S.Context, reqs) -> None:
         ctx.render(r.camera, r.options, r.passes)
 
 
-# ---------------------------------------------------------------- config 4: instanced multi-body scene
+# ---------------------------------------------------------------- instanced multi-body scene at 1 AU
 AU = S.AU
 
 
@@ -454,12 +454,12 @@ class InstancedScene:
 def build_instanced(ctx: S.Context, dem_built: "BuiltScene | None" = None, dem: int = 4097, image: int = 4096,
                     spp: int = 8, bounces: int = 2, sphere=(161, 200), body=(251, 400), sun: str = "sphere",
                     bvh_width: int = 4) -> InstancedScene:
-    """Config 4: the Moon (a 4097^2 DEM tile + a 64k-triangle sphere for the
+    """Instanced scene: the Moon (a 4097^2 DEM tile + a 64k-triangle sphere for the
     rest) at 1 AU from the Sun, a 200k-triangle small body 30 km from the
     camera and a 50,496-triangle spacecraft carrying the camera, every body
     placed by an FP64 model matrix at ~1.5e8 km from the origin; camera 100 km
     above the tile (scaled with the DEM), 4096^2, 8 spp, 2 bounces."""
-    sc = CONFIG1 if dem == CONFIG1.dem else scaled(CONFIG1, dem=dem, texture=min(CONFIG1.texture, 4 * (dem - 1)))
+    sc = DEM4097 if dem == DEM4097.dem else scaled(DEM4097, dem=dem, texture=min(DEM4097.texture, 4 * (dem - 1)))
     b = dem_built if dem_built is not None else build(ctx, sc, bvh_width=bvh_width)
     R = sc.sphere_radius
     alt = sc.altitude
@@ -500,7 +500,7 @@ def build_instanced(ctx: S.Context, dem_built: "BuiltScene | None" = None, dem: 
     return InstancedScene(meshes, transforms, cam, lights, opts, [b.heights, b.texture], local)
 
 
-# ---------------------------------------------------------------- config 4: the star field
+# ---------------------------------------------------------------- the 100,000-star field
 def johnson_v_band(n: int = 1000):
     """johnsonVBandApproximation (magnitudes.ipp): skew-normal V-band response on 450-725 nm (FP64)."""
     from scipy.special import erfc
@@ -549,8 +549,8 @@ def make_star_field(n: int = 100_000, seed: int = 4, mag_max: float = 10.0):
     return dirs, star_irradiance(mag), mag
 
 
-def config4_star_camera(image: int = 4096, direction=(0.6, -0.48, 0.64)):
-    """Config 4's camera (4096^2, 40 degree field, f = 50 mm, f/2.8) for its star
+def star_field_camera(image: int = 4096, direction=(0.6, -0.48, 0.64)):
+    """The instanced scene's camera (4096^2, 40 degree field, f = 50 mm, f/2.8) for its star
     field, with the Airy-disk PSF of those optics (camera.ipp:90-97: focal
     length, aperture diameter f/N, pixel size = sensor size / resolution)."""
     from . import sensor as SN
