@@ -39,6 +39,16 @@ def algorithmic_bytes(v_int: float, v_tri: float) -> float:
     return 48.0 + 64.0 * v_int + 36.0 * v_tri
 
 
+# Per shaded surface point: three vertices' albedo (32 B), normal (12 B) and uv
+# (8 B), plus the 4 texels of one bilinear fetch of a one-channel FP32 albedo
+# texture when the scene has one.
+BYTES_PER_HIT = 156.0
+BYTES_PER_HIT_TEXEL = 4 * 4.0
+# Per camera sample: the pixel's global Kahan accumulators (value and
+# compensation, read and write)
+BYTES_PER_SAMPLE_ACC = 128.0
+
+
 def traffic_bytes(kernel: str, workload: str):
     """Measured HBM bytes per launch (read + write) of this kernel on this workload, or None."""
     if not TRAFFIC_PROFILE.exists():
@@ -494,11 +504,14 @@ def main() -> int:
         a_closest = algorithmic_bytes(vi, vt)
         a_shadow = algorithmic_bytes(si, stt)
         closest_rays = prim + bnc
+        hits, a_hit = None, None
         if fused:
             # one launch traces camera, bounce and shadow rays
             kernel, launches = "k_render_fused", sum(s["fused_launches"] for s in stats)
             ms = sum(s["fused_ms"] for s in stats)
-            nbytes = closest_rays * a_closest + shad * a_shadow
+            hits = sum(s["closest_hits"] for s in stats)
+            a_hit = BYTES_PER_HIT + (BYTES_PER_HIT_TEXEL if sc.albedo == "texture" else 0.0)
+            nbytes = closest_rays * a_closest + shad * a_shadow + hits * a_hit + prim * BYTES_PER_SAMPLE_ACC
         else:
             kernel, launches = "k_trace_queue (closest-hit launches)", sum(s["trace_closest_launches"] for s in stats)
             ms = sum(s["trace_closest_ms"] for s in stats)
@@ -510,6 +523,8 @@ def main() -> int:
             "bytes_per_closest_ray": round(a_closest, 1), "bytes_per_shadow_ray": round(a_shadow, 1),
             "v_int_closest": round(vi, 2), "v_tri_closest": round(vt, 2),
             "v_int_shadow": round(si, 2), "v_tri_shadow": round(stt, 2),
+            "bytes_per_hit": None if a_hit is None else round(a_hit, 1), "closest_hits": hits,
+            "bytes_per_sample_acc": BYTES_PER_SAMPLE_ACC if fused else None,
             "launches": launches, "avg_launch_ms": round(ms / max(1, launches), 4),
         }
     line = {

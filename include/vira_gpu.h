@@ -152,6 +152,7 @@ typedef struct {
     int integrator;          /* 1 fused, 2 wavefront */
     double fused_ms;         /* device time of the fused renderer launch(es) */
     uint32_t fused_launches;
+    uint64_t closest_hits;   /* camera and bounce rays that hit (shaded surface points; per-pixel renderer) */
 } vira_gpu_render_stats;
 
 /* Fixed-visibility direct illumination (the differentiable render).  New: the
@@ -199,6 +200,7 @@ typedef struct {
     uint64_t tiles;
     double ms;                /* device time of the launch (HIP events) */
     int32_t graph_reused;     /* 1 = replayed the captured hipGraph of an identical batch layout */
+    uint64_t closest_hits;    /* camera and bounce rays that hit (shaded surface points) */
 } vira_gpu_batch_stats;
 
 /* Sensor radiometry of Camera::calculateReceivedPower (camera.ipp:1175-1178). */

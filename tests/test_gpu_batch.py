@@ -55,8 +55,13 @@ def test_batch_equals_per_request_renders(gpu_ctx, small_meshes):
     reqs = W.make_batch(gpu_ctx, small_meshes, n=16, seed=5, bounces=2, passes=S.RenderPasses.ALL,
                         sides=(16, 24, 40, 64), spps=(1, 2, 5))
     plain = _clone(reqs)
-    W.render_plain(gpu_ctx, plain)
+    hits = 0
+    for r in plain:
+        gpu_ctx.tlas_build([r.mesh])
+        gpu_ctx.lights_set([] if r.light is None else [r.light])
+        hits += gpu_ctx.render(r.camera, r.options, r.passes)["closest_hits"]
     st = gpu_ctx.batch_submit(reqs)
+    assert st["closest_hits"] == hits > 0
     assert st["graph_reused"] == 0
     for i, (a, b) in enumerate(zip(reqs, plain)):
         bad = _passes_equal(a.passes, b.passes)

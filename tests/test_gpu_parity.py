@@ -148,6 +148,9 @@ def test_render_dem257_parity(gpu_ctx, integrator, width):
     assert (tri != 0xFFFFFFFF).sum() > 0.2 * tri.size
     assert st["primary_rays"] == sc.image * sc.image == o["counts"]["primary"]
     assert st["shadow_rays"] == o["counts"]["shadow"]
+    if integrator == "fused":
+        # one camera sample per pixel and no bounces: shaded points = pixels with a hit
+        assert st["closest_hits"] == int((tri != 0xFFFFFFFF).sum())
     dep = rp.depth.cpu().numpy().reshape(-1)
     assert np.array_equal(dep.view(np.uint32), o["depth"].view(np.uint32))
     rad = rp.total_radiance.cpu().numpy().reshape(-1, 8)

@@ -134,6 +134,7 @@ class RenderStats(C.Structure):
         ("integrator", C.c_int),
         ("fused_ms", C.c_double),
         ("fused_launches", C.c_uint32),
+        ("closest_hits", C.c_uint64),
     ]
 
 
@@ -176,6 +177,7 @@ class BatchStats(C.Structure):
         ("tiles", C.c_uint64),
         ("ms", C.c_double),
         ("graph_reused", C.c_int32),
+        ("closest_hits", C.c_uint64),
     ]
 
 

@@ -50,7 +50,7 @@ __global__ void __launch_bounds__(kFusedBS* WPB, kFusedWavesPerSimd) k_render_ba
     uint32_t* s_ref_w = s_ref + wave * (kStackLds * kFusedBS);
     float* s_dist_w = s_dist + wave * (kStackLds * kFusedBS);
     float* s_contrib_w = s_contrib + wave * (NB * kFusedBS);
-    uint32_t n_prim = 0, n_shadow = 0, n_bounce = 0;
+    uint32_t n_prim = 0, n_shadow = 0, n_bounce = 0, n_hit = 0;
     uint32_t* head = reinterpret_cast<uint32_t*>(state);
     while (true) {
         uint32_t base = 0;
@@ -68,12 +68,13 @@ __global__ void __launch_bounds__(kFusedBS* WPB, kFusedWavesPerSimd) k_render_ba
             }
             const BatchReq& R = reqs[lo];
             const uint32_t tile = (k - R.first) * R.tile_stride;
-            render_tile<W>(R.job, tile, a, s_ref_w, s_dist_w, s_contrib_w, lane, n_prim, n_shadow, n_bounce);
+            render_tile<W>(R.job, tile, a, s_ref_w, s_dist_w, s_contrib_w, lane, n_prim, n_shadow, n_bounce, n_hit);
         }
     }
     wave_add(&state[2], n_prim);
     wave_add(&state[3], n_shadow);
     wave_add(&state[4], n_bounce);
+    wave_add(&state[5], n_hit);
 }
 
 using BatchKernel = void (*)(const BatchReq*, int, uint32_t, uint32_t, float*, unsigned long long*);
@@ -232,6 +233,7 @@ void batch_submit(Ctx& c, const vira_gpu_request* reqs, size_t n, const vira_gpu
         st->primary_rays = hc[2];
         st->shadow_rays = hc[3];
         st->bounce_rays = hc[4];
+        st->closest_hits = hc[5];
         st->tiles = tiles_total;
         st->ms = ms;
         st->graph_reused = reused ? 1 : 0;

@@ -151,7 +151,8 @@ struct TileJob {
 template <int W, bool INST = false>
 __device__ __forceinline__ void render_tile(const TileJob& J, uint32_t tile, float* __restrict__ a,
                                             uint32_t* s_ref, float* s_dist, float* s_contrib, uint32_t lane,
-                                            uint32_t& n_prim, uint32_t& n_shadow, uint32_t& n_bounce) {
+                                            uint32_t& n_prim, uint32_t& n_shadow, uint32_t& n_bounce,
+                                            uint32_t& n_hit) {
     const void* __restrict__ nodes = J.nodes;
     const float4* __restrict__ tris = J.tris;
     const GVertex* __restrict__ verts = J.verts;
@@ -219,6 +220,7 @@ __device__ __forceinline__ void render_tile(const TileJob& J, uint32_t tile, flo
             } else {
                 Surface S = INST ? surface_instance(J.inst[hinst], h.id, h.b1, h.b2)
                                  : surface_at(verts, idx, mat, h.id, h.b1, h.b2);
+                n_hit++;
                 if (b == 0) accumulate_first_hit(a, lane, h.id | (hinst << kTriBits), h.t, S);
                 f3 ld = mk3(0.f, 0.f, 1.f);
                 float dist = 0.f, c[NB];

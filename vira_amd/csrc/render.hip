@@ -73,12 +73,13 @@ __global__ void __launch_bounds__(kFusedBS, kFusedWavesPerSimd) k_render_fused(T
     __shared__ float s_dist[kStackLds * kFusedBS];
     __shared__ float s_contrib[NB * kFusedBS];  // staged light contribution while the shadow ray is traced
     const uint32_t lane = threadIdx.x;
-    uint32_t n_prim = 0, n_shadow = 0, n_bounce = 0;
+    uint32_t n_prim = 0, n_shadow = 0, n_bounce = 0, n_hit = 0;
     render_tile<W, INST>(J, blockIdx.x, acc + (size_t)blockIdx.x * (A_FIELDS * 64), s_ref, s_dist, s_contrib, lane,
-                         n_prim, n_shadow, n_bounce);
+                         n_prim, n_shadow, n_bounce, n_hit);
     wave_add(&counters[0], n_prim);
     wave_add(&counters[1], n_shadow);
     wave_add(&counters[2], n_bounce);
+    wave_add(&counters[3], n_hit);
 }
 
 // Camera-relative FP32 placement of every instance and light, computed in
@@ -455,7 +456,7 @@ void render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, c
         VG_HIP(hipGetLastError());
     }
     VG_HIP(hipEventRecord(t1, c.stream));
-    unsigned long long hc[3];
+    unsigned long long hc[4];
     VG_HIP(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, c.stream));
     VG_HIP(hipStreamSynchronize(c.stream));
     if (st) {
@@ -463,6 +464,7 @@ void render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, c
         st->primary_rays = hc[0];
         st->shadow_rays = hc[1];
         st->bounce_rays = hc[2];
+        st->closest_hits = hc[3];
         float f = 0.f;
         VG_HIP(hipEventElapsedTime(&f, t0, t1));
         st->total_ms = f;
