@@ -114,30 +114,75 @@ def test_convolution_bit_exact_against_oracle(gpu_ctx):
         assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), K
 
 
-@pytest.mark.parametrize("shape,K", [((48, 64), 17), ((100, 130), 81), ((37, 29), 51), ((70, 90), 301)])
-def test_convolution_large_kernels(gpu_ctx, shape, K):
-    """Kernels of 256 pixels or more (the reference's FFT branch, image.ipp:1014-1065):
-    the same zero-padded correlation.  Against the exact (FP64) correlation the
-    GPU's FMA sums are within 2e-5 relative where the result is above 1e-3 of
-    the maximum and 1e-5 of the maximum everywhere -- the error of an FP32 sum
-    of K^2 non-negative terms (the oracle's sequential FP32 sum is checked
-    against the same bound)."""
-    import torch
+def _exact_correlation(img, kh):
     from scipy.signal import correlate
+    return np.stack([correlate(img[:, :, b].astype(np.float64), kh[:, :, b].astype(np.float64), mode="same")
+                     for b in range(8)], 2)
+
+
+@pytest.mark.parametrize("shape,K", [((48, 64), 17), ((100, 130), 81), ((37, 29), 51), ((70, 90), 301),
+                                     ((1, 5), 17), ((300, 17), 27)])
+def test_convolution_large_kernels(gpu_ctx, shape, K):
+    """Kernels of 256 pixels or more take the reference's FFT branch
+    (image.ipp:1014-1065): the same zero-padded correlation through 2-D FFTs.
+    An FP32 FFT's error is spread over the image rather than relative to each
+    pixel, so the bar is the exact (FP64) correlation to within 2e-6 of its
+    maximum everywhere.  The oracle's sequential FP32 sum (the spatial
+    branch's arithmetic) is within 1e-5 of the maximum of the same exact
+    result."""
+    import torch
     rng = np.random.default_rng(K)
     img = rng.random(shape + (8,)).astype(np.float32)
     img[rng.random(shape) < 0.3] = 0.0
     kern = gpu_ctx.psf_kernel(SN.GaussianPSF(np.linspace(1.0, K / 6.0, 8), angle_deg=15.0, supersample=2), K)
     kh = kern.cpu().numpy()
     got = gpu_ctx.convolve(torch.from_numpy(img).cuda(), kern).cpu().numpy().astype(np.float64)
-    exact = np.stack([correlate(img[:, :, b].astype(np.float64), kh[:, :, b].astype(np.float64), mode="same")
-                      for b in range(8)], 2)
+    exact = _exact_correlation(img, kh)
     ora = O.convolve(img, kh).astype(np.float64)
     scale = np.abs(exact).max()
+    err = np.abs(got - exact).max() / scale
+    print(f"{shape} K={K}: max |fft - exact| / max = {err:.2e}")
+    assert err <= 2e-6
+    assert np.abs(ora - exact).max() <= 1e-5 * scale
+
+
+def test_convolution_fft_kernel_cache(gpu_ctx):
+    """The kernel's spectrum is kept between calls: a new kernel of the same
+    size, then the first kernel again, each give their own correlation."""
+    import torch
+    rng = np.random.default_rng(5)
+    img = rng.random((64, 96, 8)).astype(np.float32)
+    k1 = gpu_ctx.psf_kernel(SN.GaussianPSF(2.0), 21)
+    k2 = gpu_ctx.psf_kernel(SN.GaussianPSF(3.5, angle_deg=30.0, sigma_y=1.5), 21)
+    d = torch.from_numpy(img).cuda()
+    r1 = gpu_ctx.convolve(d, k1).cpu().numpy()
+    r2 = gpu_ctx.convolve(d, k2).cpu().numpy()
+    r1b = gpu_ctx.convolve(d, k1).cpu().numpy()
+    assert np.array_equal(r1, r1b)
+    for r, k in ((r1, k1), (r2, k2)):
+        exact = _exact_correlation(img, k.cpu().numpy())
+        assert np.abs(r - exact).max() <= 2e-6 * np.abs(exact).max()
+    # a kernel edited in place (same buffer, same size) is seen
+    k1.mul_(0.5)
+    r3 = gpu_ctx.convolve(d, k1).cpu().numpy()
+    exact = _exact_correlation(img, k1.cpu().numpy())
+    assert np.abs(r3 - exact).max() <= 2e-6 * np.abs(exact).max()
+
+
+def test_convolution_past_longest_transform(gpu_ctx):
+    """An image wider than the longest transform (8192 minus half the kernel)
+    is summed directly, the FMA sums within 2e-5 relative of the exact
+    correlation where it is above 1e-3 of its maximum."""
+    import torch
+    rng = np.random.default_rng(8)
+    img = rng.random((3, 8200, 8)).astype(np.float32)
+    kern = gpu_ctx.psf_kernel(SN.GaussianPSF(2.5), 17)
+    got = gpu_ctx.convolve(torch.from_numpy(img).cuda(), kern).cpu().numpy().astype(np.float64)
+    exact = _exact_correlation(img, kern.cpu().numpy())
+    scale = np.abs(exact).max()
     big = np.abs(exact) > 1e-3 * scale
-    for res in (got, ora):
-        assert np.abs(res - exact).max() <= 1e-5 * scale
-        assert (np.abs(res - exact)[big] / np.abs(exact[big])).max() <= 2e-5
+    assert np.abs(got - exact).max() <= 1e-5 * scale
+    assert (np.abs(got - exact)[big] / np.abs(exact[big])).max() <= 2e-5
 
 
 def test_sensor_deterministic_bit_exact(gpu_ctx):

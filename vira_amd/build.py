@@ -37,7 +37,7 @@ FLAGS = [
     "-Wno-unused-but-set-variable",
     f"-I{ROOT / 'include'}",
 ]
-SOURCES = ["capi.hip", "dem.hip", "bvh.hip", "render.hip", "adjoint.hip", "batch.hip", "sensor.hip", "denoise.hip",
+SOURCES = ["capi.hip", "dem.hip", "bvh.hip", "render.hip", "adjoint.hip", "batch.hip", "sensor.hip", "fftconv.hip", "denoise.hip",
            "quipu.hip", "scan.hip"]
 # C++ programs over include/vira_gpu.hpp, linked against the library (host code only)
 EXAMPLES = ["render_dem.cpp", "quipu_scene.cpp"]

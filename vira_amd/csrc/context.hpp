@@ -139,6 +139,12 @@ struct Ctx {
     DBuf st_rec, st_count, st_start, st_fill, st_list;
     // denoiser work images: two per pass (direct, indirect) in ping-pong
     DBuf dn_work[4];
+    // FFT convolution: twiddle table, row/column spectra, the kernel's spectrum
+    // and the kernel values and padded sizes it was computed for
+    DBuf fc_tw, fc_spec, fc_kspec;
+    std::vector<float> fc_kern;
+    int fc_K = 0, fc_logLx = 0, fc_logLy = 0;
+    bool fc_kvalid = false, fc_attrs = false;
     ~Ctx() {
         if (b_exec) (void)hipGraphExecDestroy(b_exec);
     }
@@ -183,6 +189,9 @@ void qld_read(Ctx& c, const vira_gpu_qld& q, int level, const vira_gpu_qld_read_
 void denoise(Ctx& c, const vira_gpu_denoise_opts& o, int W, int H, float* direct, float* indirect,
              const float* albedo, const float* depth, const float* normal, float* total);
 void convolve(Ctx& c, const float* in, int W, int H, const float* kern, int K, float* out);
+// FFT correlation for kernels of 256 taps or more; fits = both padded sizes <= 8192
+bool fft_convolve_fits(int W, int H, int K);
+void fft_convolve(Ctx& c, const float* in, int W, int H, const float* kern, int K, float* out);
 void simulate_sensor(Ctx& c, const vira_gpu_sensor_sim& cfg, const float* power, int W, int H, float* photons,
                      float* image);
 void batch_submit(Ctx& c, const vira_gpu_request* reqs, size_t n, const vira_gpu_batch_config& cfg, int stride,

@@ -1,0 +1,368 @@
+// fftconv.hip -- zero-padded correlation of an 8-band image with a K x K
+// kernel through 2-D FFTs: the branch Image::convolve takes for kernels of
+// 256 pixels or more (image.ipp:1006-1065, convolveChannelFFT :1160-1264),
+//   out(x, y) = sum_{kx, ky} in(x + kx - K/2, y + ky - K/2) * kernel(kx, ky),
+// the same result as the spatial branch up to the transforms' rounding.
+//
+// Sizes: Lx = 2^a >= max(W + K/2, K), Ly = 2^b >= max(H + K/2, K).  A tap
+// that reaches past the image then lands in the zero padding and never wraps
+// onto a pixel, and no two taps share a residue, so the circular correlation
+// of size Lx x Ly restricted to the W x H pixels is the zero-padded one.
+//
+// Three launches per image, each a batch of in-LDS Stockham FFTs (radix 16,
+// then one radix 2/4/8 stage; twiddles from an FP64-computed table):
+//   rows:    per image row, the two bands 2p, 2p+1 packed as one complex row
+//            (re, im), forward FFT of length Lx, split into the two bands'
+//            half spectra (fx = 0 .. Lx/2; a real row's spectrum is
+//            Hermitian), written to the spectrum buffer in strips of CW
+//            columns: spec[band][strip][row][c];
+//   columns: per (band, strip), the CW columns of length Ly (rows past the
+//            image are zero), forward FFT, times the kernel's spectrum,
+//            inverse FFT, the H image rows written back in place;
+//   inverse: per image row, the band pairs' half spectra extended by
+//            Hermitian symmetry, packed (a + i b), inverse FFT, real and
+//            imaginary parts written to the two bands.
+// The kernel's spectrum comes from the first two steps run on the kernel
+// placed at offsets (kx - K/2, ky - K/2) mod (Lx, Ly), conjugated (a
+// correlation) and scaled by 1 / (4 Lx Ly) (the inverse transforms are
+// unnormalised and both operands come out of the row split doubled).  It is
+// kept on the context and recomputed only when the kernel's values or the
+// padded sizes change.
+#include "context.hpp"
+
+namespace vg {
+namespace {
+
+constexpr int kFcLogMax = 13;
+constexpr int kFcMax = 1 << kFcLogMax;  // longest transform (8192)
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 mul_negi(float2 a) { return make_float2(a.y, -a.x); }  // a * (-i)
+
+// cos and sin of 2 pi m / 16
+constexpr double kC1 = 0.92387953251128674, kS1 = 0.38268343236508977, kR2 = 0.70710678118654752;
+constexpr double kCos16[16] = {1, kC1, kR2, kS1, 0, -kS1, -kR2, -kC1, -1, -kC1, -kR2, -kS1, 0, kS1, kR2, kC1};
+constexpr double kSin16[16] = {0, kS1, kR2, kC1, 1, kC1, kR2, kS1, 0, -kS1, -kR2, -kC1, -1, -kC1, -kR2, -kS1};
+
+// v * exp(-2 pi i m / 16), m known at compile time
+template <int M>
+__device__ __forceinline__ float2 w16(float2 v) {
+    constexpr int m = M & 15;
+    if constexpr (m == 0) return v;
+    else if constexpr (m == 4) return mul_negi(v);
+    else if constexpr (m == 8) return make_float2(-v.x, -v.y);
+    else if constexpr (m == 12) return make_float2(-v.y, v.x);
+    else return cmul(v, make_float2((float)kCos16[m], (float)-kSin16[m]));
+}
+
+__device__ __forceinline__ void dft4(float2& v0, float2& v1, float2& v2, float2& v3) {
+    const float2 a0 = cadd(v0, v2), a1 = csub(v0, v2), a2 = cadd(v1, v3), a3 = mul_negi(csub(v1, v3));
+    v0 = cadd(a0, a2);
+    v1 = cadd(a1, a3);
+    v2 = csub(a0, a2);
+    v3 = csub(a1, a3);
+}
+
+// In-register DFT of R points (R = 2, 4, 8, 16): R = 4 * N2 splits into N2
+// 4-point DFTs over stride-N2 inputs, twiddles w_R^(n2 k1), then 4 N2-point
+// DFTs; outputs in natural order.
+template <int R>
+__device__ __forceinline__ void dft(float2* v) {
+    if constexpr (R == 2) {
+        const float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 4) {
+        dft4(v[0], v[1], v[2], v[3]);
+    } else {
+        constexpr int N2 = R / 4;
+        float2 y[N2][4];
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) {
+            y[n2][0] = v[n2];
+            y[n2][1] = v[N2 + n2];
+            y[n2][2] = v[2 * N2 + n2];
+            y[n2][3] = v[3 * N2 + n2];
+            dft4(y[n2][0], y[n2][1], y[n2][2], y[n2][3]);
+        }
+        // w_R^(n2 k1) = w_16^(n2 k1 16 / R)
+        if constexpr (N2 >= 2) {
+            y[1][1] = w16<1 * 16 / R>(y[1][1]);
+            y[1][2] = w16<2 * 16 / R>(y[1][2]);
+            y[1][3] = w16<3 * 16 / R>(y[1][3]);
+        }
+        if constexpr (N2 == 4) {
+            y[2][1] = w16<2>(y[2][1]);
+            y[2][2] = w16<4>(y[2][2]);
+            y[2][3] = w16<6>(y[2][3]);
+            y[3][1] = w16<3>(y[3][1]);
+            y[3][2] = w16<6>(y[3][2]);
+            y[3][3] = w16<9>(y[3][3]);
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            float2 t[N2];
+#pragma unroll
+            for (int n2 = 0; n2 < N2; ++n2) t[n2] = y[n2][k1];
+            dft<N2>(t);
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) v[k1 + 4 * k2] = t[k2];
+        }
+    }
+}
+
+// One radix-R Stockham stage over nb sequences of length L = 2^logL in LDS
+// (sequence q at z + q * stride), after stages whose radices multiply to Ns:
+// butterfly j of a sequence reads z[j + r L/R], twiddles by
+// exp(-2 pi i r (j mod Ns) / (Ns R)), and writes its DFT to
+// z[(j - j mod Ns) R + j mod Ns + r Ns].  A thread holds at most 16 values
+// (nb L / blockDim <= 16), read before the barrier and written after it.
+template <int R>
+__device__ __forceinline__ void fft_stage(float2* z, int logL, int nb, int stride, int logNs, const float2* tw) {
+    constexpr int kQ = 16 / R;
+    constexpr int logR = R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : 4;
+    const int logB = logL - logR, B = 1 << logB;  // butterflies per sequence
+    const int Ns = 1 << logNs, total = nb * B;
+    const int twshift = kFcLogMax - (logNs + logR);
+    float2 v[kQ][R];
+    int dst[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        const int jj = (int)threadIdx.x + q * (int)blockDim.x;
+        dst[q] = -1;
+        if (jj < total) {
+            const int s = jj >> logB, j = jj & (B - 1), k = j & (Ns - 1);
+            const float2* zs = z + s * stride;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q][r] = zs[j + r * B];
+            if (logNs > 0) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], tw[(r * k) << twshift]);
+            }
+            dft<R>(v[q]);
+            dst[q] = s * stride + ((j - k) << logR) + k;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        if (dst[q] >= 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) z[dst[q] + r * Ns] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+// Forward FFT (unnormalised, natural order in and out) of nb sequences of
+// length 2^logL in LDS; the block must be synchronised on entry and is on exit.
+__device__ void fft_lds(float2* z, int logL, int nb, int stride, const float2* tw) {
+    int logNs = 0;
+    for (; logL - logNs >= 4; logNs += 4) fft_stage<16>(z, logL, nb, stride, logNs, tw);
+    const int rem = logL - logNs;
+    if (rem == 3) fft_stage<8>(z, logL, nb, stride, logNs, tw);
+    else if (rem == 2) fft_stage<4>(z, logL, nb, stride, logNs, tw);
+    else if (rem == 1) fft_stage<2>(z, logL, nb, stride, logNs, tw);
+}
+
+struct FcGeom {
+    int logLx, logLy;
+    int cw, ns;  // strip width (columns) and strips per band
+};
+
+__host__ __device__ inline size_t spec_at(const FcGeom& g, int nrows, int band, int fx, int row) {
+    return (((size_t)band * g.ns + fx / g.cw) * nrows + row) * g.cw + fx % g.cw;
+}
+
+// Row transform: row `row` of an AoS [rows][width][8] image, each band pair
+// packed as one complex row placed at x - shift (mod Lx), forward FFT, the
+// two half spectra (doubled) into spec (nrows rows per strip).
+__global__ void __launch_bounds__(1024) k_fc_rows(const float* __restrict__ src, int width, int shift, FcGeom g,
+                                                  int nrows, float2* __restrict__ spec, const float2* __restrict__ tw) {
+    extern __shared__ float2 fc_lds[];
+    float2* z = fc_lds;
+    const int row = blockIdx.x, Lx = 1 << g.logLx, T = blockDim.x;
+    const float* in = src + (size_t)row * width * 8;
+    for (int p = 0; p < 4; ++p) {
+        for (int i = threadIdx.x; i < Lx; i += T) {
+            const int x = (i + shift) & (Lx - 1);
+            float2 v = make_float2(0.f, 0.f);
+            if (x < width) v = make_float2(in[(size_t)x * 8 + 2 * p], in[(size_t)x * 8 + 2 * p + 1]);
+            z[i] = v;
+        }
+        __syncthreads();
+        fft_lds(z, g.logLx, 1, Lx, tw);
+        // Z = A + iB with A, B the spectra of the real rows: 2A = Z(f) + conj Z(-f), 2B = -i (Z(f) - conj Z(-f))
+        const int ncols = g.ns * g.cw;
+        for (int f = threadIdx.x; f < ncols; f += T) {
+            float2 a = make_float2(0.f, 0.f), b = a;
+            if (f <= Lx / 2) {
+                const float2 zk = z[f], zm = cconj(z[(Lx - f) & (Lx - 1)]);
+                a = cadd(zk, zm);
+                b = mul_negi(csub(zk, zm));
+            }
+            spec[spec_at(g, nrows, 2 * p, f, row)] = a;
+            spec[spec_at(g, nrows, 2 * p + 1, f, row)] = b;
+        }
+        __syncthreads();
+    }
+}
+
+// Column transform of one (band, strip): SPECTRUM = the kernel's rows (nrows
+// = K, row ky at (ky - shift) mod Ly), forward FFT, stored conjugated and
+// scaled as all Ly rows of the kernel spectrum; otherwise the image rows
+// (nrows = H from row 0), forward FFT, times the kernel spectrum, inverse
+// FFT, the H rows written back.
+template <bool SPECTRUM>
+__global__ void __launch_bounds__(1024) k_fc_cols(float2* __restrict__ spec, int nrows, int shift, FcGeom g,
+                                                  float2* __restrict__ kspec, float scale,
+                                                  const float2* __restrict__ tw) {
+    extern __shared__ float2 fc_lds[];
+    const int strip = blockIdx.x, band = blockIdx.y, T = blockDim.x;
+    const int Ly = 1 << g.logLy, cw = g.cw, stride = Ly + 1;  // one pad entry per column
+    float2* z = fc_lds;
+    const size_t sbase = ((size_t)band * g.ns + strip) * (size_t)nrows * cw;
+    const size_t kbase = ((size_t)band * g.ns + strip) * (size_t)Ly * cw;
+    for (int e = threadIdx.x; e < Ly * cw; e += T) {
+        const int r = e / cw, c = e - r * cw;
+        const int src = SPECTRUM ? ((r + shift) & (Ly - 1)) : r;
+        z[c * stride + r] = src < nrows ? spec[sbase + (size_t)src * cw + c] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    fft_lds(z, g.logLy, cw, stride, tw);
+    if (SPECTRUM) {
+        for (int e = threadIdx.x; e < Ly * cw; e += T) {
+            const int r = e / cw, c = e - r * cw;
+            const float2 v = z[c * stride + r];
+            kspec[kbase + e] = make_float2(v.x * scale, -v.y * scale);
+        }
+        return;
+    }
+    // inverse through the forward transform: ifft(X) = conj(fft(conj X)) (unnormalised)
+    for (int e = threadIdx.x; e < Ly * cw; e += T) {
+        const int r = e / cw, c = e - r * cw;
+        z[c * stride + r] = cconj(cmul(z[c * stride + r], kspec[kbase + e]));
+    }
+    __syncthreads();
+    fft_lds(z, g.logLy, cw, stride, tw);
+    for (int e = threadIdx.x; e < nrows * cw; e += T) {
+        const int r = e / cw, c = e - r * cw;
+        spec[sbase + e] = cconj(z[c * stride + r]);
+    }
+}
+
+// Inverse row transform: row `row`, per band pair the half spectra extended by
+// Hermitian symmetry and packed a + i b, inverse FFT, written to the bands.
+__global__ void __launch_bounds__(1024) k_fc_rows_inv(const float2* __restrict__ spec, int width, FcGeom g,
+                                                      int nrows, float* __restrict__ out,
+                                                      const float2* __restrict__ tw) {
+    extern __shared__ float2 fc_lds[];
+    float2* z = fc_lds;
+    const int row = blockIdx.x, Lx = 1 << g.logLx, T = blockDim.x;
+    float* o = out + (size_t)row * width * 8;
+    for (int p = 0; p < 4; ++p) {
+        for (int f = threadIdx.x; f < Lx; f += T) {
+            const bool lo = f <= Lx / 2;
+            const int m = lo ? f : Lx - f;
+            float2 a = spec[spec_at(g, nrows, 2 * p, m, row)], b = spec[spec_at(g, nrows, 2 * p + 1, m, row)];
+            if (!lo) {
+                a = cconj(a);
+                b = cconj(b);
+            }
+            z[f] = cconj(make_float2(a.x - b.y, a.y + b.x));  // conj(a + i b): inverse via the forward FFT
+        }
+        __syncthreads();
+        fft_lds(z, g.logLx, 1, Lx, tw);
+        for (int x = threadIdx.x; x < width; x += T) {
+            const float2 v = z[x];  // conj of (a + i b)(x)
+            o[(size_t)x * 8 + 2 * p] = v.x;
+            o[(size_t)x * 8 + 2 * p + 1] = -v.y;
+        }
+        __syncthreads();
+    }
+}
+
+int log2_ceil(int n) {
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return l;
+}
+
+// threads for nb transforms of length 2^logL: at most 16 values per thread
+unsigned fc_threads(int logL, int nb) {
+    const long n = (long)nb << logL;
+    return (unsigned)std::min<long>(1024, std::max<long>(64, n / 16));
+}
+
+}  // namespace
+
+bool fft_convolve_fits(int W, int H, int K) {
+    const int h = K / 2;
+    return std::max(W + h, K) <= kFcMax && std::max(H + h, K) <= kFcMax;
+}
+
+void fft_convolve(Ctx& c, const float* in, int W, int H, const float* kern, int K, float* out) {
+    VG_CHECK(fft_convolve_fits(W, H, K), VIRA_GPU_ERR_INVALID, "fft_convolve: image too large for one transform");
+    const int h = K / 2;
+    FcGeom g;
+    g.logLx = std::max(1, log2_ceil(std::max(W + h, K)));
+    g.logLy = std::max(1, log2_ceil(std::max(H + h, K)));
+    g.cw = g.logLy <= 12 ? 4 : 2;  // a (band, strip) block's columns fill at most 128 KiB of LDS
+    const int Lx = 1 << g.logLx, Ly = 1 << g.logLy;
+    g.ns = (Lx / 2 + 1 + g.cw - 1) / g.cw;
+    const size_t row_lds = (size_t)Lx * sizeof(float2), col_lds = (size_t)g.cw * (Ly + 1) * sizeof(float2);
+    if (!c.fc_attrs) {  // the column blocks take up to 128 KiB of LDS
+        for (const void* f : {(const void*)k_fc_rows, (const void*)k_fc_rows_inv, (const void*)k_fc_cols<true>,
+                              (const void*)k_fc_cols<false>})
+            VG_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        c.fc_attrs = true;
+    }
+    if (!c.fc_tw.p) {
+        std::vector<float2> t(kFcMax);
+        for (int m = 0; m < kFcMax; ++m) {
+            const double a = -2.0 * 3.14159265358979323846 * (double)m / (double)kFcMax;
+            t[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+        }
+        c.fc_tw.alloc(t.size() * sizeof(float2));
+        VG_HIP(hipMemcpy(c.fc_tw.p, t.data(), t.size() * sizeof(float2), hipMemcpyHostToDevice));
+    }
+    const float2* tw = c.fc_tw.as<float2>();
+    // the kernel's spectrum, unless the cached one is of the same values and sizes
+    std::vector<float> kh((size_t)K * K * 8);
+    VG_HIP(hipMemcpyAsync(kh.data(), kern, kh.size() * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+    VG_HIP(hipStreamSynchronize(c.stream));
+    const bool cached = c.fc_kvalid && c.fc_K == K && c.fc_logLx == g.logLx && c.fc_logLy == g.logLy &&
+                        c.fc_kern == kh;
+    if (!cached) {
+        c.fc_kvalid = false;
+        c.fc_kspec.alloc((size_t)8 * g.ns * Ly * g.cw * sizeof(float2));
+        c.fc_spec.alloc((size_t)8 * g.ns * std::max(H, K) * g.cw * sizeof(float2));
+        k_fc_rows<<<K, fc_threads(g.logLx, 1), row_lds, c.stream>>>(kern, K, h, g, K, c.fc_spec.as<float2>(), tw);
+        VG_HIP(hipGetLastError());
+        k_fc_cols<true><<<dim3(g.ns, 8), fc_threads(g.logLy, g.cw), col_lds, c.stream>>>(
+            c.fc_spec.as<float2>(), K, h, g, c.fc_kspec.as<float2>(), 1.0f / (4.0f * (float)Lx * (float)Ly), tw);
+        VG_HIP(hipGetLastError());
+        c.fc_kern = std::move(kh);
+        c.fc_K = K;
+        c.fc_logLx = g.logLx;
+        c.fc_logLy = g.logLy;
+        c.fc_kvalid = true;
+    }
+    c.fc_spec.alloc((size_t)8 * g.ns * H * g.cw * sizeof(float2));
+    float2* spec = c.fc_spec.as<float2>();
+    k_fc_rows<<<H, fc_threads(g.logLx, 1), row_lds, c.stream>>>(in, W, 0, g, H, spec, tw);
+    VG_HIP(hipGetLastError());
+    k_fc_cols<false><<<dim3(g.ns, 8), fc_threads(g.logLy, g.cw), col_lds, c.stream>>>(
+        spec, H, 0, g, c.fc_kspec.as<float2>(), 0.0f, tw);
+    VG_HIP(hipGetLastError());
+    k_fc_rows_inv<<<H, fc_threads(g.logLx, 1), row_lds, c.stream>>>(spec, W, g, H, out, tw);
+    VG_HIP(hipGetLastError());
+    VG_HIP(hipStreamSynchronize(c.stream));
+}
+
+}  // namespace vg

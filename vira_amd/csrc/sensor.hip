@@ -632,7 +632,11 @@ void convolve(Ctx& c, const float* in, int W, int H, const float* kern, int K, f
     if ((size_t)K * K < 256) {
         // the reference's spatial branch: per band, ky outer, kx inner, separate multiply and add
         k_convolve<<<blocks_for(n, kConvBS), kConvBS, 0, c.stream>>>(in, W, H, kern, K, out);
+    } else if (fft_convolve_fits(W, H, K)) {
+        fft_convolve(c, in, W, H, kern, K, out);  // the reference's FFT branch (image.ipp:1014-1065)
+        return;
     } else {
+        // images past the longest transform: the direct sums
         const dim3 grid((W + kCvTW - 1) / kCvTW, (H + kCvTH - 1) / kCvTH, NB);
         k_convolve_tiled<<<grid, kCvTW * kCvRows, 0, c.stream>>>(in, W, H, kern, K, out);
     }
