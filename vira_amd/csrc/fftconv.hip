@@ -116,19 +116,47 @@ __device__ __forceinline__ void dft(float2* v) {
     }
 }
 
+// LDS slot of logical element i: for transforms of 256 points or more, one
+// unused slot after every 16, so that the first stage's writes (16
+// consecutive values per thread) and the other stages' strided accesses fall
+// in distinct banks.  The strides the stages step by are then multiples of
+// 16, so slot(i + m) = slot(i) + m + m/16 for them.
+__device__ __forceinline__ int lp(int i, bool pad) { return pad ? i + (i >> 4) : i; }
+
+// Twiddles exp(-2 pi i m / 8192) as the product of two correctly rounded
+// table entries, exp(-2 pi i 64 (m >> 6) / 8192) (128 coarse) and
+// exp(-2 pi i (m & 63) / 8192) (64 fine): 192 entries, copied to LDS by every
+// block, instead of one 8192-entry table read through the caches.
+constexpr int kFcTwCoarse = 128, kFcTwFine = 64, kFcTw = kFcTwCoarse + kFcTwFine;
+__device__ __forceinline__ float2 twiddle(const float2* twl, unsigned m) {
+    return cmul(twl[m >> 6], twl[kFcTwCoarse + (m & 63)]);
+}
+// LDS of a block: the twiddle tables, then the transforms' slots
+__device__ __forceinline__ float2* load_twiddles(float2* lds, const float2* __restrict__ tw) {
+    for (int i = threadIdx.x; i < kFcTw; i += blockDim.x) lds[i] = tw[i];
+    return lds + kFcTw;
+}
+__host__ inline size_t lds_bytes(size_t n) { return (kFcTw + n + n / 16 + 1) * sizeof(float2); }
+
 // One radix-R Stockham stage over nb sequences of length L = 2^logL in LDS
-// (sequence q at z + q * stride), after stages whose radices multiply to Ns:
+// (logical element i of sequence q at slot lp(q L + i, L >= 256)), after stages whose
+// radices multiply to Ns:
 // butterfly j of a sequence reads z[j + r L/R], twiddles by
 // exp(-2 pi i r (j mod Ns) / (Ns R)), and writes its DFT to
 // z[(j - j mod Ns) R + j mod Ns + r Ns].  A thread holds at most 16 values
 // (nb L / blockDim <= 16), read before the barrier and written after it.
 template <int R>
-__device__ __forceinline__ void fft_stage(float2* z, int logL, int nb, int stride, int logNs, const float2* tw) {
+__device__ __forceinline__ void fft_stage(float2* z, int logL, int nb, int logNs, const float2* twl) {
     constexpr int kQ = 16 / R;
     constexpr int logR = R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : 4;
     const int logB = logL - logR, B = 1 << logB;  // butterflies per sequence
     const int Ns = 1 << logNs, total = nb * B;
     const int twshift = kFcLogMax - (logNs + logR);
+    const bool pad = logL >= 8;
+    // slot steps between a butterfly's reads (B apart) and writes (Ns apart;
+    // the first stage's 16 writes are consecutive and start on a multiple of 16)
+    const int rstep = pad ? B + (B >> 4) : B;
+    const int wstep = pad ? (Ns >= 16 ? Ns + (Ns >> 4) : 1) : Ns;
     float2 v[kQ][R];
     int dst[kQ];
 #pragma unroll
@@ -137,15 +165,15 @@ __device__ __forceinline__ void fft_stage(float2* z, int logL, int nb, int strid
         dst[q] = -1;
         if (jj < total) {
             const int s = jj >> logB, j = jj & (B - 1), k = j & (Ns - 1);
-            const float2* zs = z + s * stride;
+            const int base = lp((s << logL) + j, pad);
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[q][r] = zs[j + r * B];
+            for (int r = 0; r < R; ++r) v[q][r] = z[base + r * rstep];
             if (logNs > 0) {
 #pragma unroll
-                for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], tw[(r * k) << twshift]);
+                for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], twiddle(twl, (unsigned)(r * k) << twshift));
             }
             dft<R>(v[q]);
-            dst[q] = s * stride + ((j - k) << logR) + k;
+            dst[q] = lp((s << logL) + ((j - k) << logR) + k, pad);
         }
     }
     __syncthreads();
@@ -153,27 +181,38 @@ __device__ __forceinline__ void fft_stage(float2* z, int logL, int nb, int strid
     for (int q = 0; q < kQ; ++q) {
         if (dst[q] >= 0) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) z[dst[q] + r * Ns] = v[q][r];
+            for (int r = 0; r < R; ++r) z[dst[q] + r * wstep] = v[q][r];
         }
     }
     __syncthreads();
 }
 
 // Forward FFT (unnormalised, natural order in and out) of nb sequences of
-// length 2^logL in LDS; the block must be synchronised on entry and is on exit.
-__device__ void fft_lds(float2* z, int logL, int nb, int stride, const float2* tw) {
+// length 2^logL in LDS (slots lp(q L + i, L >= 256)); the block must be synchronised on
+// entry and is on exit.
+__device__ void fft_lds(float2* z, int logL, int nb, const float2* twl) {
     int logNs = 0;
-    for (; logL - logNs >= 4; logNs += 4) fft_stage<16>(z, logL, nb, stride, logNs, tw);
+    for (; logL - logNs >= 4; logNs += 4) fft_stage<16>(z, logL, nb, logNs, twl);
     const int rem = logL - logNs;
-    if (rem == 3) fft_stage<8>(z, logL, nb, stride, logNs, tw);
-    else if (rem == 2) fft_stage<4>(z, logL, nb, stride, logNs, tw);
-    else if (rem == 1) fft_stage<2>(z, logL, nb, stride, logNs, tw);
+    if (rem == 3) fft_stage<8>(z, logL, nb, logNs, twl);
+    else if (rem == 2) fft_stage<4>(z, logL, nb, logNs, twl);
+    else if (rem == 1) fft_stage<2>(z, logL, nb, logNs, twl);
 }
 
 struct FcGeom {
     int logLx, logLy;
     int cw, ns;  // strip width (columns) and strips per band
+    int nc;      // columns per column-transform block (1 or 2)
 };
+
+// Workgroups reach the 8 XCDs round robin (block b on XCD b mod 8).  The
+// logical index of block b numbers each XCD's blocks consecutively, so
+// neighbouring logical blocks (rows or half-strips sharing cache lines) run
+// on one XCD, through one L2, at about the same time.
+__device__ __forceinline__ int xcd_block(int b, int n) {
+    const int q = n >> 3, rem = n & 7, x = b & 7, i = b >> 3;
+    return x < rem ? x * (q + 1) + i : rem * (q + 1) + (x - rem) * q + i;
+}
 
 __host__ __device__ inline size_t spec_at(const FcGeom& g, int nrows, int band, int fx, int row) {
     return (((size_t)band * g.ns + fx / g.cw) * nrows + row) * g.cw + fx % g.cw;
@@ -185,24 +224,26 @@ __host__ __device__ inline size_t spec_at(const FcGeom& g, int nrows, int band, 
 __global__ void __launch_bounds__(1024) k_fc_rows(const float* __restrict__ src, int width, int shift, FcGeom g,
                                                   int nrows, float2* __restrict__ spec, const float2* __restrict__ tw) {
     extern __shared__ float2 fc_lds[];
-    float2* z = fc_lds;
-    const int row = blockIdx.x, Lx = 1 << g.logLx, T = blockDim.x;
+    float2* z = load_twiddles(fc_lds, tw);
+    const float2* twl = fc_lds;
+    const int row = xcd_block(blockIdx.x, gridDim.x), Lx = 1 << g.logLx, T = blockDim.x;
+    const bool pad = g.logLx >= 8;
     const float* in = src + (size_t)row * width * 8;
     for (int p = 0; p < 4; ++p) {
         for (int i = threadIdx.x; i < Lx; i += T) {
             const int x = (i + shift) & (Lx - 1);
             float2 v = make_float2(0.f, 0.f);
             if (x < width) v = make_float2(in[(size_t)x * 8 + 2 * p], in[(size_t)x * 8 + 2 * p + 1]);
-            z[i] = v;
+            z[lp(i, pad)] = v;
         }
         __syncthreads();
-        fft_lds(z, g.logLx, 1, Lx, tw);
+        fft_lds(z, g.logLx, 1, twl);
         // Z = A + iB with A, B the spectra of the real rows: 2A = Z(f) + conj Z(-f), 2B = -i (Z(f) - conj Z(-f))
         const int ncols = g.ns * g.cw;
         for (int f = threadIdx.x; f < ncols; f += T) {
             float2 a = make_float2(0.f, 0.f), b = a;
             if (f <= Lx / 2) {
-                const float2 zk = z[f], zm = cconj(z[(Lx - f) & (Lx - 1)]);
+                const float2 zk = z[lp(f, pad)], zm = cconj(z[lp((Lx - f) & (Lx - 1), pad)]);
                 a = cadd(zk, zm);
                 b = mul_negi(csub(zk, zm));
             }
@@ -213,46 +254,53 @@ __global__ void __launch_bounds__(1024) k_fc_rows(const float* __restrict__ src,
     }
 }
 
-// Column transform of one (band, strip): SPECTRUM = the kernel's rows (nrows
-// = K, row ky at (ky - shift) mod Ly), forward FFT, stored conjugated and
-// scaled as all Ly rows of the kernel spectrum; otherwise the image rows
-// (nrows = H from row 0), forward FFT, times the kernel spectrum, inverse
-// FFT, the H rows written back.
+// Column transform of nc columns of one (band, strip): SPECTRUM = the kernel's
+// rows (nrows = K, row ky at (ky - shift) mod Ly), forward FFT, stored
+// conjugated and scaled as all Ly rows of the kernel spectrum; otherwise the
+// image rows (nrows = H from row 0), forward FFT, times the kernel spectrum,
+// inverse FFT, the H rows written back.  A strip's column blocks are
+// consecutive after xcd_block, so they read the strip's lines through one L2.
 template <bool SPECTRUM>
 __global__ void __launch_bounds__(1024) k_fc_cols(float2* __restrict__ spec, int nrows, int shift, FcGeom g,
                                                   float2* __restrict__ kspec, float scale,
                                                   const float2* __restrict__ tw) {
     extern __shared__ float2 fc_lds[];
-    const int strip = blockIdx.x, band = blockIdx.y, T = blockDim.x;
-    const int Ly = 1 << g.logLy, cw = g.cw, stride = Ly + 1;  // one pad entry per column
-    float2* z = fc_lds;
-    const size_t sbase = ((size_t)band * g.ns + strip) * (size_t)nrows * cw;
-    const size_t kbase = ((size_t)band * g.ns + strip) * (size_t)Ly * cw;
-    for (int e = threadIdx.x; e < Ly * cw; e += T) {
-        const int r = e / cw, c = e - r * cw;
+    const int parts = g.cw / g.nc, lognc = g.nc == 2 ? 1 : 0;
+    const int L = xcd_block(blockIdx.x, gridDim.x);
+    const int band = L / (g.ns * parts), rest = L - band * g.ns * parts;
+    const int strip = rest / parts, c0 = (rest - strip * parts) * g.nc;
+    const int Ly = 1 << g.logLy, cw = g.cw, nc = g.nc, T = blockDim.x;
+    const bool pad = g.logLy >= 8;
+    float2* z = load_twiddles(fc_lds, tw);
+    const float2* twl = fc_lds;
+    const size_t sbase = ((size_t)band * g.ns + strip) * (size_t)nrows * cw + c0;
+    const size_t kbase = ((size_t)band * g.ns + strip) * (size_t)Ly * cw + c0;
+    for (int e = threadIdx.x; e < Ly * nc; e += T) {
+        const int r = e >> lognc, c = e & (nc - 1);
         const int src = SPECTRUM ? ((r + shift) & (Ly - 1)) : r;
-        z[c * stride + r] = src < nrows ? spec[sbase + (size_t)src * cw + c] : make_float2(0.f, 0.f);
+        z[lp(c * Ly + r, pad)] = src < nrows ? spec[sbase + (size_t)src * cw + c] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    fft_lds(z, g.logLy, cw, stride, tw);
+    fft_lds(z, g.logLy, nc, twl);
     if (SPECTRUM) {
-        for (int e = threadIdx.x; e < Ly * cw; e += T) {
-            const int r = e / cw, c = e - r * cw;
-            const float2 v = z[c * stride + r];
-            kspec[kbase + e] = make_float2(v.x * scale, -v.y * scale);
+        for (int e = threadIdx.x; e < Ly * nc; e += T) {
+            const int r = e >> lognc, c = e & (nc - 1);
+            const float2 v = z[lp(c * Ly + r, pad)];
+            kspec[kbase + (size_t)r * cw + c] = make_float2(v.x * scale, -v.y * scale);
         }
         return;
     }
     // inverse through the forward transform: ifft(X) = conj(fft(conj X)) (unnormalised)
-    for (int e = threadIdx.x; e < Ly * cw; e += T) {
-        const int r = e / cw, c = e - r * cw;
-        z[c * stride + r] = cconj(cmul(z[c * stride + r], kspec[kbase + e]));
+    for (int e = threadIdx.x; e < Ly * nc; e += T) {
+        const int r = e >> lognc, c = e & (nc - 1);
+        const int i = lp(c * Ly + r, pad);
+        z[i] = cconj(cmul(z[i], kspec[kbase + (size_t)r * cw + c]));
     }
     __syncthreads();
-    fft_lds(z, g.logLy, cw, stride, tw);
-    for (int e = threadIdx.x; e < nrows * cw; e += T) {
-        const int r = e / cw, c = e - r * cw;
-        spec[sbase + e] = cconj(z[c * stride + r]);
+    fft_lds(z, g.logLy, nc, twl);
+    for (int e = threadIdx.x; e < nrows * nc; e += T) {
+        const int r = e >> lognc, c = e & (nc - 1);
+        spec[sbase + (size_t)r * cw + c] = cconj(z[lp(c * Ly + r, pad)]);
     }
 }
 
@@ -262,8 +310,10 @@ __global__ void __launch_bounds__(1024) k_fc_rows_inv(const float2* __restrict__
                                                       int nrows, float* __restrict__ out,
                                                       const float2* __restrict__ tw) {
     extern __shared__ float2 fc_lds[];
-    float2* z = fc_lds;
-    const int row = blockIdx.x, Lx = 1 << g.logLx, T = blockDim.x;
+    float2* z = load_twiddles(fc_lds, tw);
+    const float2* twl = fc_lds;
+    const int row = xcd_block(blockIdx.x, gridDim.x), Lx = 1 << g.logLx, T = blockDim.x;
+    const bool pad = g.logLx >= 8;
     float* o = out + (size_t)row * width * 8;
     for (int p = 0; p < 4; ++p) {
         for (int f = threadIdx.x; f < Lx; f += T) {
@@ -274,12 +324,12 @@ __global__ void __launch_bounds__(1024) k_fc_rows_inv(const float2* __restrict__
                 a = cconj(a);
                 b = cconj(b);
             }
-            z[f] = cconj(make_float2(a.x - b.y, a.y + b.x));  // conj(a + i b): inverse via the forward FFT
+            z[lp(f, pad)] = cconj(make_float2(a.x - b.y, a.y + b.x));  // conj(a + i b): inverse via the forward FFT
         }
         __syncthreads();
-        fft_lds(z, g.logLx, 1, Lx, tw);
+        fft_lds(z, g.logLx, 1, twl);
         for (int x = threadIdx.x; x < width; x += T) {
-            const float2 v = z[x];  // conj of (a + i b)(x)
+            const float2 v = z[lp(x, pad)];  // conj of (a + i b)(x)
             o[(size_t)x * 8 + 2 * p] = v.x;
             o[(size_t)x * 8 + 2 * p + 1] = -v.y;
         }
@@ -312,21 +362,24 @@ void fft_convolve(Ctx& c, const float* in, int W, int H, const float* kern, int 
     FcGeom g;
     g.logLx = std::max(1, log2_ceil(std::max(W + h, K)));
     g.logLy = std::max(1, log2_ceil(std::max(H + h, K)));
-    g.cw = g.logLy <= 12 ? 4 : 2;  // a (band, strip) block's columns fill at most 128 KiB of LDS
+    g.cw = g.logLy <= 12 ? 4 : 2;  // strip width: a half strip of Ly rows fills at most 70 KiB of LDS
     const int Lx = 1 << g.logLx, Ly = 1 << g.logLy;
     g.ns = (Lx / 2 + 1 + g.cw - 1) / g.cw;
-    const size_t row_lds = (size_t)Lx * sizeof(float2), col_lds = (size_t)g.cw * (Ly + 1) * sizeof(float2);
-    if (!c.fc_attrs) {  // the column blocks take up to 128 KiB of LDS
+    g.nc = g.cw / 2;  // two column blocks per strip: 70 KiB of LDS each at most, two blocks per CU
+    const size_t row_lds = lds_bytes(Lx), col_lds = lds_bytes((size_t)g.nc * Ly);
+    const unsigned col_blocks = 8u * g.ns * (g.cw / g.nc);
+    if (!c.fc_attrs) {  // the blocks take up to 70 KiB of LDS
         for (const void* f : {(const void*)k_fc_rows, (const void*)k_fc_rows_inv, (const void*)k_fc_cols<true>,
                               (const void*)k_fc_cols<false>})
             VG_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         c.fc_attrs = true;
     }
     if (!c.fc_tw.p) {
-        std::vector<float2> t(kFcMax);
-        for (int m = 0; m < kFcMax; ++m) {
+        std::vector<float2> t(kFcTw);
+        for (int i = 0; i < kFcTw; ++i) {
+            const int m = i < kFcTwCoarse ? 64 * i : i - kFcTwCoarse;
             const double a = -2.0 * 3.14159265358979323846 * (double)m / (double)kFcMax;
-            t[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+            t[i] = make_float2((float)std::cos(a), (float)std::sin(a));
         }
         c.fc_tw.alloc(t.size() * sizeof(float2));
         VG_HIP(hipMemcpy(c.fc_tw.p, t.data(), t.size() * sizeof(float2), hipMemcpyHostToDevice));
@@ -344,7 +397,7 @@ void fft_convolve(Ctx& c, const float* in, int W, int H, const float* kern, int 
         c.fc_spec.alloc((size_t)8 * g.ns * std::max(H, K) * g.cw * sizeof(float2));
         k_fc_rows<<<K, fc_threads(g.logLx, 1), row_lds, c.stream>>>(kern, K, h, g, K, c.fc_spec.as<float2>(), tw);
         VG_HIP(hipGetLastError());
-        k_fc_cols<true><<<dim3(g.ns, 8), fc_threads(g.logLy, g.cw), col_lds, c.stream>>>(
+        k_fc_cols<true><<<col_blocks, fc_threads(g.logLy, g.nc), col_lds, c.stream>>>(
             c.fc_spec.as<float2>(), K, h, g, c.fc_kspec.as<float2>(), 1.0f / (4.0f * (float)Lx * (float)Ly), tw);
         VG_HIP(hipGetLastError());
         c.fc_kern = std::move(kh);
@@ -357,7 +410,7 @@ void fft_convolve(Ctx& c, const float* in, int W, int H, const float* kern, int 
     float2* spec = c.fc_spec.as<float2>();
     k_fc_rows<<<H, fc_threads(g.logLx, 1), row_lds, c.stream>>>(in, W, 0, g, H, spec, tw);
     VG_HIP(hipGetLastError());
-    k_fc_cols<false><<<dim3(g.ns, 8), fc_threads(g.logLy, g.cw), col_lds, c.stream>>>(
+    k_fc_cols<false><<<col_blocks, fc_threads(g.logLy, g.nc), col_lds, c.stream>>>(
         spec, H, 0, g, c.fc_kspec.as<float2>(), 0.0f, tw);
     VG_HIP(hipGetLastError());
     k_fc_rows_inv<<<H, fc_threads(g.logLx, 1), row_lds, c.stream>>>(spec, W, g, H, out, tw);
