@@ -386,7 +386,12 @@ vira_gpu_status vira_gpu_received_power(vira_gpu_ctx* ctx, const vira_gpu_camera
 vira_gpu_status vira_gpu_psf_kernel(vira_gpu_ctx* ctx, const vira_gpu_psf* psf, int size, float* kernel_dev);
 /* Image::convolve (image.ipp:993-1065, 1267-1330): per band, zero-padded,
  * out(x, y) = sum_{kx, ky} in(x + kx - K/2, y + ky - K/2) * kernel(kx, ky).
- * image_dev / out_dev: W*H*8 (out must not alias image). */
+ * image_dev / out_dev: W*H*8 (out must not alias image).  Kernels below 256
+ * taps are summed directly (the reference's spatial branch, same order);
+ * larger ones go through 2-D FFTs (its FFT branch) when W and H plus K/2 fit
+ * 8192, with the three buffers 8-byte aligned, and are summed directly for
+ * larger images.  The kernel's spectrum is kept on the context for the next
+ * call with the same kernel values and padded sizes. */
 vira_gpu_status vira_gpu_convolve(vira_gpu_ctx* ctx, const float* image_dev, int width, int height,
                                   const float* kernel_dev, int ksize, float* out_dev);
 /* Camera::simulateSensor: received power (W*H*8, watts) -> photon counts

@@ -139,10 +139,10 @@ struct Ctx {
     DBuf st_rec, st_count, st_start, st_fill, st_list;
     // denoiser work images: two per pass (direct, indirect) in ping-pong
     DBuf dn_work[4];
-    // FFT convolution: twiddle table, row/column spectra, the kernel's spectrum
-    // and the kernel values and padded sizes it was computed for
-    DBuf fc_tw, fc_spec, fc_kspec;
-    std::vector<float> fc_kern;
+    // FFT convolution: twiddle tables, row/column spectra, the kernel's spectrum,
+    // the kernel values (on the device) and padded sizes it was computed for,
+    // and the device flag saying whether it was recomputed
+    DBuf fc_tw, fc_spec, fc_kspec, fc_kept, fc_flag;
     int fc_K = 0, fc_logLx = 0, fc_logLy = 0;
     bool fc_kvalid = false, fc_attrs = false;
     ~Ctx() {

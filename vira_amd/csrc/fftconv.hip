@@ -169,8 +169,19 @@ __device__ __forceinline__ void fft_stage(float2* z, int logL, int nb, int logNs
 #pragma unroll
             for (int r = 0; r < R; ++r) v[q][r] = z[base + r * rstep];
             if (logNs > 0) {
+                // w^r for w = exp(-2 pi i k / (Ns R)): w, w^2, w^4, w^8 from the tables, the other
+                // powers as one product of two of those (at most three roundings deep)
+                float2 w[R];
+                const unsigned t = (unsigned)k << twshift;
 #pragma unroll
-                for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], twiddle(twl, (unsigned)(r * k) << twshift));
+                for (int r = 1; r < R; r <<= 1) w[r] = twiddle(twl, t * r);
+#pragma unroll
+                for (int r = 3; r < R; ++r) {
+                    const int hb = r >= 8 ? 8 : r >= 4 ? 4 : 2;
+                    if (r != hb) w[r] = cmul(w[hb], w[r - hb]);
+                }
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], w[r]);
             }
             dft<R>(v[q]);
             dst[q] = lp((s << logL) + ((j - k) << logR) + k, pad);
@@ -203,6 +214,7 @@ struct FcGeom {
     int logLx, logLy;
     int cw, ns;  // strip width (columns) and strips per band
     int nc;      // columns per column-transform block (1 or 2)
+    int logcw;   // log2(cw)
 };
 
 // Workgroups reach the 8 XCDs round robin (block b on XCD b mod 8).  The
@@ -215,14 +227,16 @@ __device__ __forceinline__ int xcd_block(int b, int n) {
 }
 
 __host__ __device__ inline size_t spec_at(const FcGeom& g, int nrows, int band, int fx, int row) {
-    return (((size_t)band * g.ns + fx / g.cw) * nrows + row) * g.cw + fx % g.cw;
+    return ((((size_t)band * g.ns + (fx >> g.logcw)) * nrows + row) << g.logcw) + (fx & (g.cw - 1));
 }
 
 // Row transform: row `row` of an AoS [rows][width][8] image, each band pair
 // packed as one complex row placed at x - shift (mod Lx), forward FFT, the
 // two half spectra (doubled) into spec (nrows rows per strip).
 __global__ void __launch_bounds__(1024) k_fc_rows(const float* __restrict__ src, int width, int shift, FcGeom g,
-                                                  int nrows, float2* __restrict__ spec, const float2* __restrict__ tw) {
+                                                  int nrows, float2* __restrict__ spec, const float2* __restrict__ tw,
+                                                  const uint32_t* __restrict__ only_if) {
+    if (only_if && *only_if == 0u) return;
     extern __shared__ float2 fc_lds[];
     float2* z = load_twiddles(fc_lds, tw);
     const float2* twl = fc_lds;
@@ -233,7 +247,7 @@ __global__ void __launch_bounds__(1024) k_fc_rows(const float* __restrict__ src,
         for (int i = threadIdx.x; i < Lx; i += T) {
             const int x = (i + shift) & (Lx - 1);
             float2 v = make_float2(0.f, 0.f);
-            if (x < width) v = make_float2(in[(size_t)x * 8 + 2 * p], in[(size_t)x * 8 + 2 * p + 1]);
+            if (x < width) v = *reinterpret_cast<const float2*>(in + (size_t)x * 8 + 2 * p);
             z[lp(i, pad)] = v;
         }
         __syncthreads();
@@ -263,7 +277,8 @@ __global__ void __launch_bounds__(1024) k_fc_rows(const float* __restrict__ src,
 template <bool SPECTRUM>
 __global__ void __launch_bounds__(1024) k_fc_cols(float2* __restrict__ spec, int nrows, int shift, FcGeom g,
                                                   float2* __restrict__ kspec, float scale,
-                                                  const float2* __restrict__ tw) {
+                                                  const float2* __restrict__ tw, const uint32_t* __restrict__ only_if) {
+    if (only_if && *only_if == 0u) return;
     extern __shared__ float2 fc_lds[];
     const int parts = g.cw / g.nc, lognc = g.nc == 2 ? 1 : 0;
     const int L = xcd_block(blockIdx.x, gridDim.x);
@@ -316,25 +331,48 @@ __global__ void __launch_bounds__(1024) k_fc_rows_inv(const float2* __restrict__
     const bool pad = g.logLx >= 8;
     float* o = out + (size_t)row * width * 8;
     for (int p = 0; p < 4; ++p) {
-        for (int f = threadIdx.x; f < Lx; f += T) {
-            const bool lo = f <= Lx / 2;
-            const int m = lo ? f : Lx - f;
-            float2 a = spec[spec_at(g, nrows, 2 * p, m, row)], b = spec[spec_at(g, nrows, 2 * p + 1, m, row)];
-            if (!lo) {
-                a = cconj(a);
-                b = cconj(b);
-            }
-            z[lp(f, pad)] = cconj(make_float2(a.x - b.y, a.y + b.x));  // conj(a + i b): inverse via the forward FFT
+        // each stored frequency m fills slots m and Lx - m (A(-m) = conj A(m) for both bands); the
+        // slot holds conj(a + i b), so the forward FFT computes the inverse
+        for (int m = threadIdx.x; m <= Lx / 2; m += T) {
+            const float2 a = spec[spec_at(g, nrows, 2 * p, m, row)], b = spec[spec_at(g, nrows, 2 * p + 1, m, row)];
+            z[lp(m, pad)] = make_float2(a.x - b.y, -(a.y + b.x));
+            if (m > 0 && m < Lx / 2) z[lp(Lx - m, pad)] = make_float2(a.x + b.y, a.y - b.x);
         }
         __syncthreads();
         fft_lds(z, g.logLx, 1, twl);
         for (int x = threadIdx.x; x < width; x += T) {
             const float2 v = z[lp(x, pad)];  // conj of (a + i b)(x)
-            o[(size_t)x * 8 + 2 * p] = v.x;
-            o[(size_t)x * 8 + 2 * p + 1] = -v.y;
+            *reinterpret_cast<float2*>(o + (size_t)x * 8 + 2 * p) = make_float2(v.x, -v.y);
         }
         __syncthreads();
     }
+}
+
+// One block: *changed = whether the kernel's bits differ from the copy kept
+// with its spectrum (or force); if so the copy is replaced.
+__global__ void __launch_bounds__(1024) k_fc_kernel_changed(const float* __restrict__ kern, float* __restrict__ kept,
+                                                            int n, int force, uint32_t* __restrict__ changed) {
+    __shared__ uint32_t diff;
+    if (threadIdx.x == 0) diff = force ? 1u : 0u;
+    __syncthreads();
+    // n is a multiple of 8 and both buffers are 8-byte aligned: compare and copy in pairs
+    const uint2* a = reinterpret_cast<const uint2*>(kern);
+    const uint2* b = reinterpret_cast<const uint2*>(kept);
+    const int n2 = n / 2;
+    uint32_t d = 0u;
+    if (!force) {
+#pragma unroll 4
+        for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+            const uint2 x = a[i], y = b[i];
+            d |= (x.x ^ y.x) | (x.y ^ y.y);
+        }
+    }
+    if (d) atomicOr(&diff, 1u);
+    __syncthreads();
+    const bool replace = diff != 0u;
+    if (replace)
+        for (int i = threadIdx.x; i < n2; i += blockDim.x) reinterpret_cast<uint2*>(kept)[i] = a[i];
+    if (threadIdx.x == 0) *changed = replace ? 1u : 0u;
 }
 
 int log2_ceil(int n) {
@@ -358,11 +396,14 @@ bool fft_convolve_fits(int W, int H, int K) {
 
 void fft_convolve(Ctx& c, const float* in, int W, int H, const float* kern, int K, float* out) {
     VG_CHECK(fft_convolve_fits(W, H, K), VIRA_GPU_ERR_INVALID, "fft_convolve: image too large for one transform");
+    VG_CHECK((((uintptr_t)in | (uintptr_t)out | (uintptr_t)kern) & 7) == 0, VIRA_GPU_ERR_INVALID,
+             "convolve: image, kernel and output must be 8-byte aligned for kernels of 256 taps or more");
     const int h = K / 2;
     FcGeom g;
     g.logLx = std::max(1, log2_ceil(std::max(W + h, K)));
     g.logLy = std::max(1, log2_ceil(std::max(H + h, K)));
     g.cw = g.logLy <= 12 ? 4 : 2;  // strip width: a half strip of Ly rows fills at most 70 KiB of LDS
+    g.logcw = g.cw == 4 ? 2 : 1;
     const int Lx = 1 << g.logLx, Ly = 1 << g.logLy;
     g.ns = (Lx / 2 + 1 + g.cw - 1) / g.cw;
     g.nc = g.cw / 2;  // two column blocks per strip: 70 KiB of LDS each at most, two blocks per CU
@@ -385,33 +426,34 @@ void fft_convolve(Ctx& c, const float* in, int W, int H, const float* kern, int 
         VG_HIP(hipMemcpy(c.fc_tw.p, t.data(), t.size() * sizeof(float2), hipMemcpyHostToDevice));
     }
     const float2* tw = c.fc_tw.as<float2>();
-    // the kernel's spectrum, unless the cached one is of the same values and sizes
-    std::vector<float> kh((size_t)K * K * 8);
-    VG_HIP(hipMemcpyAsync(kh.data(), kern, kh.size() * sizeof(float), hipMemcpyDeviceToHost, c.stream));
-    VG_HIP(hipStreamSynchronize(c.stream));
-    const bool cached = c.fc_kvalid && c.fc_K == K && c.fc_logLx == g.logLx && c.fc_logLy == g.logLy &&
-                        c.fc_kern == kh;
-    if (!cached) {
-        c.fc_kvalid = false;
-        c.fc_kspec.alloc((size_t)8 * g.ns * Ly * g.cw * sizeof(float2));
-        c.fc_spec.alloc((size_t)8 * g.ns * std::max(H, K) * g.cw * sizeof(float2));
-        k_fc_rows<<<K, fc_threads(g.logLx, 1), row_lds, c.stream>>>(kern, K, h, g, K, c.fc_spec.as<float2>(), tw);
-        VG_HIP(hipGetLastError());
-        k_fc_cols<true><<<col_blocks, fc_threads(g.logLy, g.nc), col_lds, c.stream>>>(
-            c.fc_spec.as<float2>(), K, h, g, c.fc_kspec.as<float2>(), 1.0f / (4.0f * (float)Lx * (float)Ly), tw);
-        VG_HIP(hipGetLastError());
-        c.fc_kern = std::move(kh);
-        c.fc_K = K;
-        c.fc_logLx = g.logLx;
-        c.fc_logLy = g.logLy;
-        c.fc_kvalid = true;
-    }
-    c.fc_spec.alloc((size_t)8 * g.ns * H * g.cw * sizeof(float2));
+    // The kernel's spectrum: recomputed when the sizes changed or, decided on the
+    // device without a round trip, when the kernel's values differ from the
+    // copy kept with it (the spectrum launches then return at once).
+    const bool same_sizes = c.fc_kvalid && c.fc_K == K && c.fc_logLx == g.logLx && c.fc_logLy == g.logLy;
+    const size_t nk = (size_t)K * K * 8;
+    c.fc_kvalid = false;
+    c.fc_kept.alloc(nk * sizeof(float));
+    c.fc_flag.alloc(sizeof(uint32_t));
+    c.fc_kspec.alloc((size_t)8 * g.ns * Ly * g.cw * sizeof(float2));
+    c.fc_spec.alloc((size_t)8 * g.ns * std::max(H, K) * g.cw * sizeof(float2));
     float2* spec = c.fc_spec.as<float2>();
-    k_fc_rows<<<H, fc_threads(g.logLx, 1), row_lds, c.stream>>>(in, W, 0, g, H, spec, tw);
+    const uint32_t* changed = c.fc_flag.as<uint32_t>();
+    k_fc_kernel_changed<<<1, 1024, 0, c.stream>>>(kern, c.fc_kept.as<float>(), (int)nk, same_sizes ? 0 : 1,
+                                                  c.fc_flag.as<uint32_t>());
+    VG_HIP(hipGetLastError());
+    k_fc_rows<<<K, fc_threads(g.logLx, 1), row_lds, c.stream>>>(kern, K, h, g, K, spec, tw, changed);
+    VG_HIP(hipGetLastError());
+    k_fc_cols<true><<<col_blocks, fc_threads(g.logLy, g.nc), col_lds, c.stream>>>(
+        spec, K, h, g, c.fc_kspec.as<float2>(), 1.0f / (4.0f * (float)Lx * (float)Ly), tw, changed);
+    VG_HIP(hipGetLastError());
+    c.fc_K = K;
+    c.fc_logLx = g.logLx;
+    c.fc_logLy = g.logLy;
+    c.fc_kvalid = true;
+    k_fc_rows<<<H, fc_threads(g.logLx, 1), row_lds, c.stream>>>(in, W, 0, g, H, spec, tw, nullptr);
     VG_HIP(hipGetLastError());
     k_fc_cols<false><<<col_blocks, fc_threads(g.logLy, g.nc), col_lds, c.stream>>>(
-        spec, H, 0, g, c.fc_kspec.as<float2>(), 0.0f, tw);
+        spec, H, 0, g, c.fc_kspec.as<float2>(), 0.0f, tw, nullptr);
     VG_HIP(hipGetLastError());
     k_fc_rows_inv<<<H, fc_threads(g.logLx, 1), row_lds, c.stream>>>(spec, W, g, H, out, tw);
     VG_HIP(hipGetLastError());
