@@ -53,13 +53,27 @@ struct TaskParam {
     int axis;
     float cmin;
     float scale;
-    int split;  // bins <= split go left
+    int split;        // bins <= split go left (set by k_sweep_large)
+    uint32_t begin;   // first slot of the node
+    uint32_t chunk0;  // first chunk of the node
+    uint32_t nchunk;  // chunks of the node
+    uint32_t pad;
+};
+// The SAH decision of one binned node: split bin, left count, the children's
+// boxes and centroid bounds (0 = left, 1 = right).
+struct SweepOut {
+    int split;  // -1: no bin boundary separates the centroids
+    uint32_t nleft;
+    float lo[2][3], hi[2][3];
+    float clo[2][3], chi[2][3];
 };
 struct ChunkBase {
     uint32_t left, right;
 };
 struct SmallTask {
     uint32_t node, begin, count, depth;
+    uint32_t base;  // first node of the task's scratch region (2 (count - 1) nodes)
+    uint32_t pad;
 };
 
 __device__ __forceinline__ int bin_of(float c, float cmin, float scale, int nb) {
@@ -233,134 +247,253 @@ __device__ __forceinline__ float half_area(const float lo[3], const float hi[3])
     return ex * ey + ey * ez + ez * ex;
 }
 
-// One thread builds the whole subtree of a small node.
-__global__ void k_build_small(const SmallTask* tasks, uint32_t ntasks, uint32_t* prims, const PrimBox* pb,
-                              BNodeG* nodes, uint32_t* node_counter, uint32_t* max_depth, int max_leaf,
-                              float trav_cost) {
-    uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ float key2f(int k) { return __int_as_float(k >= 0 ? k : (k ^ 0x7fffffff)); }
+
+// SAH sweep of one binned node's bins (vira_blas.ipp:185-254 with bins):
+// right-to-left areas, then the cheapest left-to-right boundary; the split is
+// written to the node's TaskParam for the partition, and the children's boxes
+// and centroid bounds are gathered from the bins.
+__global__ void k_sweep_large(const int* __restrict__ bins, int nb, uint32_t ntasks, TaskParam* params,
+                              SweepOut* out) {
+    const uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x;
     if (ti >= ntasks) return;
-    struct Ent {
-        uint32_t node, begin, count, depth;
-    };
-    Ent stack[64];
-    int sp = 0;
-    stack[sp++] = Ent{tasks[ti].node, tasks[ti].begin, tasks[ti].count, tasks[ti].depth};
-    uint32_t md = 0;
-    while (sp > 0) {
-        Ent e = stack[--sp];
-        md = max(md, e.depth);
-        BNodeG nd = nodes[e.node];
-        uint32_t n = e.count;
-        bool leaf = n <= 1;
-        int axis = 0, split = -1;
-        float cmin = 0.f, scale = 0.f;
-        float blo[kSmallBins][3], bhi[kSmallBins][3];
-        uint32_t bcnt[kSmallBins];
-        if (!leaf) {
-            float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
-            for (uint32_t i = 0; i < n; ++i) {
-                float4 c = pb[prims[e.begin + i]].c;
-                clo[0] = fminf(clo[0], c.x); clo[1] = fminf(clo[1], c.y); clo[2] = fminf(clo[2], c.z);
-                chi[0] = fmaxf(chi[0], c.x); chi[1] = fmaxf(chi[1], c.y); chi[2] = fmaxf(chi[2], c.z);
-            }
-            float ext[3] = {chi[0] - clo[0], chi[1] - clo[1], chi[2] - clo[2]};
-            axis = (ext[1] > ext[0]) ? 1 : 0;
-            if (ext[2] > ext[axis]) axis = 2;
-            cmin = clo[axis];
-            scale = (float)kSmallBins / ext[axis];
-            bool degenerate = !(ext[axis] > 0.f) || !isfinite(scale) || e.depth >= kMedianDepth;
-            if (degenerate) {
-                leaf = n <= (uint32_t)max_leaf;
-            } else {
-                for (int b = 0; b < kSmallBins; ++b) {
-                    bcnt[b] = 0;
-                    blo[b][0] = blo[b][1] = blo[b][2] = INFINITY;
-                    bhi[b][0] = bhi[b][1] = bhi[b][2] = -INFINITY;
-                }
-                for (uint32_t i = 0; i < n; ++i) {
-                    const PrimBox& p = pb[prims[e.begin + i]];
-                    float cc = axis == 0 ? p.c.x : (axis == 1 ? p.c.y : p.c.z);
-                    int b = bin_of(cc, cmin, scale, kSmallBins);
-                    bcnt[b]++;
-                    blo[b][0] = fminf(blo[b][0], p.lo.x); blo[b][1] = fminf(blo[b][1], p.lo.y); blo[b][2] = fminf(blo[b][2], p.lo.z);
-                    bhi[b][0] = fmaxf(bhi[b][0], p.hi.x); bhi[b][1] = fmaxf(bhi[b][1], p.hi.y); bhi[b][2] = fmaxf(bhi[b][2], p.hi.z);
-                }
-                // SAH sweep: right-to-left areas, then left-to-right costs
-                float rarea[kSmallBins];
-                uint32_t rcnt[kSmallBins];
-                float rl[3] = {INFINITY, INFINITY, INFINITY}, rh[3] = {-INFINITY, -INFINITY, -INFINITY};
-                uint32_t rc = 0;
-                for (int b = kSmallBins - 1; b > 0; --b) {
-                    rc += bcnt[b];
-                    for (int k = 0; k < 3; ++k) { rl[k] = fminf(rl[k], blo[b][k]); rh[k] = fmaxf(rh[k], bhi[b][k]); }
-                    rarea[b] = rc ? half_area(rl, rh) : 0.f;
-                    rcnt[b] = rc;
-                }
-                float ll[3] = {INFINITY, INFINITY, INFINITY}, lh[3] = {-INFINITY, -INFINITY, -INFINITY};
-                uint32_t lc = 0;
-                float best = INFINITY;
-                for (int b = 0; b < kSmallBins - 1; ++b) {
-                    lc += bcnt[b];
-                    for (int k = 0; k < 3; ++k) { ll[k] = fminf(ll[k], blo[b][k]); lh[k] = fmaxf(lh[k], bhi[b][k]); }
-                    if (lc == 0 || rcnt[b + 1] == 0) continue;
-                    float cost = (float)lc * half_area(ll, lh) + (float)rcnt[b + 1] * rarea[b + 1];
-                    if (cost < best) { best = cost; split = b; }
-                }
-                float A = half_area(nd.lo, nd.hi);
-                float leaf_cost = (float)n * A;
-                if (split < 0) leaf = n <= (uint32_t)max_leaf;
-                else if (n <= (uint32_t)max_leaf && leaf_cost <= trav_cost * A + best) leaf = true;
-            }
+    const int* tb = bins + (size_t)ti * nb * kBinVals;
+    float rarea[kMaxBins];
+    uint32_t rcnt[kMaxBins];
+    float rl[3] = {INFINITY, INFINITY, INFINITY}, rh[3] = {-INFINITY, -INFINITY, -INFINITY};
+    uint32_t rc = 0;
+    for (int b = nb - 1; b > 0; --b) {
+        const int* bb = tb + b * kBinVals;
+        rc += (uint32_t)bb[0];
+        if (bb[0] > 0)
+            for (int k = 0; k < 3; ++k) { rl[k] = fminf(rl[k], key2f(bb[1 + k])); rh[k] = fmaxf(rh[k], key2f(bb[4 + k])); }
+        rarea[b] = rc ? half_area(rl, rh) : 0.f;
+        rcnt[b] = rc;
+    }
+    float ll[3] = {INFINITY, INFINITY, INFINITY}, lh[3] = {-INFINITY, -INFINITY, -INFINITY};
+    uint32_t lc = 0;
+    float best = INFINITY;
+    int split = -1;
+    for (int b = 0; b < nb - 1; ++b) {
+        const int* bb = tb + b * kBinVals;
+        lc += (uint32_t)bb[0];
+        if (bb[0] > 0)
+            for (int k = 0; k < 3; ++k) { ll[k] = fminf(ll[k], key2f(bb[1 + k])); lh[k] = fmaxf(lh[k], key2f(bb[4 + k])); }
+        if (lc == 0 || rcnt[b + 1] == 0) continue;
+        const float cost = (float)lc * half_area(ll, lh) + (float)rcnt[b + 1] * rarea[b + 1];
+        if (cost < best) { best = cost; split = b; }
+    }
+    params[ti].split = split;
+    SweepOut o;
+    for (int h = 0; h < 2; ++h)
+        for (int k = 0; k < 3; ++k) {
+            o.lo[h][k] = o.clo[h][k] = INFINITY;
+            o.hi[h][k] = o.chi[h][k] = -INFINITY;
         }
-        if (leaf) {
-            nd.left = e.begin;
-            nd.count = n;
-            nodes[e.node] = nd;
-            continue;
-        }
-        uint32_t nl;
-        BNodeG L, R;
-        if (split >= 0) {
-            // in-place partition (vira_blas.ipp:149-159)
-            int64_t i = e.begin, j = (int64_t)e.begin + n - 1;
-            while (i <= j) {
-                const PrimBox& p = pb[prims[i]];
-                float cc = axis == 0 ? p.c.x : (axis == 1 ? p.c.y : p.c.z);
-                if (bin_of(cc, cmin, scale, kSmallBins) <= split) {
-                    i++;
-                } else {
-                    uint32_t t = prims[i];
-                    prims[i] = prims[j];
-                    prims[j] = t;
-                    j--;
-                }
-            }
-            nl = (uint32_t)(i - e.begin);
-            for (int k = 0; k < 3; ++k) { L.lo[k] = INFINITY; L.hi[k] = -INFINITY; R.lo[k] = INFINITY; R.hi[k] = -INFINITY; }
-            for (int b = 0; b < kSmallBins; ++b) {
-                BNodeG& D = (b <= split) ? L : R;
-                for (int k = 0; k < 3; ++k) { D.lo[k] = fminf(D.lo[k], blo[b][k]); D.hi[k] = fmaxf(D.hi[k], bhi[b][k]); }
-            }
-        } else {
-            // object median; children keep the parent box (conservative)
-            nl = n / 2;
-            for (int k = 0; k < 3; ++k) { L.lo[k] = R.lo[k] = nd.lo[k]; L.hi[k] = R.hi[k] = nd.hi[k]; }
-        }
-        uint32_t c = atomicAdd(node_counter, 2u);
-        L.left = e.begin; L.count = nl;
-        R.left = e.begin + nl; R.count = n - nl;
-        nodes[c] = L;
-        nodes[c + 1] = R;
-        nd.left = c;
-        nd.count = 0;
-        nodes[e.node] = nd;
-        // push right then left (left is built first)
-        if (sp + 2 <= 64) {
-            stack[sp++] = Ent{c + 1, e.begin + nl, n - nl, e.depth + 1};
-            stack[sp++] = Ent{c, e.begin, nl, e.depth + 1};
+    uint32_t nl = 0;
+    for (int b = 0; b < nb; ++b) {
+        const int* bb = tb + b * kBinVals;
+        if (bb[0] == 0) continue;
+        const int h = b <= split ? 0 : 1;
+        if (h == 0) nl += (uint32_t)bb[0];
+        for (int k = 0; k < 3; ++k) {
+            o.lo[h][k] = fminf(o.lo[h][k], key2f(bb[1 + k]));
+            o.hi[h][k] = fmaxf(o.hi[h][k], key2f(bb[4 + k]));
+            o.clo[h][k] = fminf(o.clo[h][k], key2f(bb[7 + k]));
+            o.chi[h][k] = fmaxf(o.chi[h][k], key2f(bb[10 + k]));
         }
     }
-    atomicMax(max_depth, md);
+    o.split = split;
+    o.nleft = nl;
+    out[ti] = o;
+}
+
+// Where each chunk of a binned node writes its left and right triangles for
+// the stable partition: the node's chunks in order, lefts from the node's
+// first slot, rights after all its lefts.
+__global__ void k_chunk_bases(const Chunk* __restrict__ chunks, const TaskParam* __restrict__ params,
+                              const SweepOut* __restrict__ sw, const uint32_t* __restrict__ counts, int nb,
+                              uint32_t ntasks, ChunkBase* bases) {
+    const uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ti >= ntasks) return;
+    const TaskParam tp = params[ti];
+    const uint32_t nleft = sw[ti].nleft;
+    uint32_t lrun = 0, rrun = 0;
+    for (uint32_t ci = tp.chunk0; ci < tp.chunk0 + tp.nchunk; ++ci) {
+        uint32_t cl = 0;
+        for (int b = 0; b <= tp.split; ++b) cl += counts[(size_t)ci * nb + b];
+        bases[ci] = ChunkBase{tp.begin + lrun, tp.begin + nleft + rrun};
+        lrun += cl;
+        rrun += (chunks[ci].end - chunks[ci].begin) - cl;
+    }
+}
+
+// One thread builds the whole subtree of a small node (the per-thread
+// "subdivide" of vira_blas.ipp:132-182).  Its nodes go to the task's own
+// scratch region (at most 2 (count - 1) of them, numbered from 0 within the
+// task), so no counter is shared between threads; k_compact_small then moves
+// every task's nodes behind the large-level nodes.  The smaller child is built
+// first and the larger one stacked, so the stack never holds more than
+// log2(count) + 1 entries.
+constexpr uint32_t kTaskRoot = 0xffffffffu;
+constexpr int kSmallStack = 12;  // > log2(kSmall) + 1
+
+__global__ void k_build_small(const SmallTask* tasks, uint32_t ntasks, uint32_t* prims, const PrimBox* pb,
+                              BNodeG* nodes, BNodeG* snodes, uint32_t* used, uint32_t* max_depth, uint32_t* err,
+                              int max_leaf, float trav_cost) {
+    const uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t md = 0;
+    if (ti < ntasks) {
+        const SmallTask task = tasks[ti];
+        BNodeG* local = snodes + task.base;
+        struct Ent {
+            uint32_t node, begin, count, depth;
+        };
+        Ent stack[kSmallStack];
+        int sp = 0;
+        uint32_t next = 0;  // scratch nodes used
+        Ent e{kTaskRoot, task.begin, task.count, task.depth};
+        bool have = true;
+        while (have) {
+            md = max(md, e.depth);
+            BNodeG* slot = e.node == kTaskRoot ? &nodes[task.node] : &local[e.node];
+            BNodeG nd = *slot;
+            uint32_t n = e.count;
+            bool leaf = n <= 1;
+            int axis = 0, split = -1;
+            float cmin = 0.f, scale = 0.f;
+            float blo[kSmallBins][3], bhi[kSmallBins][3];
+            uint32_t bcnt[kSmallBins];
+            if (!leaf) {
+                float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+                for (uint32_t i = 0; i < n; ++i) {
+                    float4 c = pb[prims[e.begin + i]].c;
+                    clo[0] = fminf(clo[0], c.x); clo[1] = fminf(clo[1], c.y); clo[2] = fminf(clo[2], c.z);
+                    chi[0] = fmaxf(chi[0], c.x); chi[1] = fmaxf(chi[1], c.y); chi[2] = fmaxf(chi[2], c.z);
+                }
+                float ext[3] = {chi[0] - clo[0], chi[1] - clo[1], chi[2] - clo[2]};
+                axis = (ext[1] > ext[0]) ? 1 : 0;
+                if (ext[2] > ext[axis]) axis = 2;
+                cmin = clo[axis];
+                scale = (float)kSmallBins / ext[axis];
+                bool degenerate = !(ext[axis] > 0.f) || !isfinite(scale) || e.depth >= kMedianDepth;
+                if (degenerate) {
+                    leaf = n <= (uint32_t)max_leaf;
+                } else {
+                    for (int b = 0; b < kSmallBins; ++b) {
+                        bcnt[b] = 0;
+                        blo[b][0] = blo[b][1] = blo[b][2] = INFINITY;
+                        bhi[b][0] = bhi[b][1] = bhi[b][2] = -INFINITY;
+                    }
+                    for (uint32_t i = 0; i < n; ++i) {
+                        const PrimBox& p = pb[prims[e.begin + i]];
+                        float cc = axis == 0 ? p.c.x : (axis == 1 ? p.c.y : p.c.z);
+                        int b = bin_of(cc, cmin, scale, kSmallBins);
+                        bcnt[b]++;
+                        blo[b][0] = fminf(blo[b][0], p.lo.x); blo[b][1] = fminf(blo[b][1], p.lo.y); blo[b][2] = fminf(blo[b][2], p.lo.z);
+                        bhi[b][0] = fmaxf(bhi[b][0], p.hi.x); bhi[b][1] = fmaxf(bhi[b][1], p.hi.y); bhi[b][2] = fmaxf(bhi[b][2], p.hi.z);
+                    }
+                    // SAH sweep: right-to-left areas, then left-to-right costs
+                    float rarea[kSmallBins];
+                    uint32_t rcnt[kSmallBins];
+                    float rl[3] = {INFINITY, INFINITY, INFINITY}, rh[3] = {-INFINITY, -INFINITY, -INFINITY};
+                    uint32_t rc = 0;
+                    for (int b = kSmallBins - 1; b > 0; --b) {
+                        rc += bcnt[b];
+                        for (int k = 0; k < 3; ++k) { rl[k] = fminf(rl[k], blo[b][k]); rh[k] = fmaxf(rh[k], bhi[b][k]); }
+                        rarea[b] = rc ? half_area(rl, rh) : 0.f;
+                        rcnt[b] = rc;
+                    }
+                    float ll[3] = {INFINITY, INFINITY, INFINITY}, lh[3] = {-INFINITY, -INFINITY, -INFINITY};
+                    uint32_t lc = 0;
+                    float best = INFINITY;
+                    for (int b = 0; b < kSmallBins - 1; ++b) {
+                        lc += bcnt[b];
+                        for (int k = 0; k < 3; ++k) { ll[k] = fminf(ll[k], blo[b][k]); lh[k] = fmaxf(lh[k], bhi[b][k]); }
+                        if (lc == 0 || rcnt[b + 1] == 0) continue;
+                        float cost = (float)lc * half_area(ll, lh) + (float)rcnt[b + 1] * rarea[b + 1];
+                        if (cost < best) { best = cost; split = b; }
+                    }
+                    float A = half_area(nd.lo, nd.hi);
+                    float leaf_cost = (float)n * A;
+                    if (split < 0) leaf = n <= (uint32_t)max_leaf;
+                    else if (n <= (uint32_t)max_leaf && leaf_cost <= trav_cost * A + best) leaf = true;
+                }
+            }
+            if (leaf) {
+                nd.left = e.begin;
+                nd.count = n;
+                *slot = nd;
+            } else {
+                uint32_t nl;
+                BNodeG L, R;
+                if (split >= 0) {
+                    // in-place partition (vira_blas.ipp:149-159)
+                    int64_t i = e.begin, j = (int64_t)e.begin + n - 1;
+                    while (i <= j) {
+                        const PrimBox& p = pb[prims[i]];
+                        float cc = axis == 0 ? p.c.x : (axis == 1 ? p.c.y : p.c.z);
+                        if (bin_of(cc, cmin, scale, kSmallBins) <= split) {
+                            i++;
+                        } else {
+                            uint32_t t = prims[i];
+                            prims[i] = prims[j];
+                            prims[j] = t;
+                            j--;
+                        }
+                    }
+                    nl = (uint32_t)(i - e.begin);
+                    for (int k = 0; k < 3; ++k) { L.lo[k] = INFINITY; L.hi[k] = -INFINITY; R.lo[k] = INFINITY; R.hi[k] = -INFINITY; }
+                    for (int b = 0; b < kSmallBins; ++b) {
+                        BNodeG& D = (b <= split) ? L : R;
+                        for (int k = 0; k < 3; ++k) { D.lo[k] = fminf(D.lo[k], blo[b][k]); D.hi[k] = fmaxf(D.hi[k], bhi[b][k]); }
+                    }
+                } else {
+                    // object median; children keep the parent box (conservative)
+                    nl = n / 2;
+                    for (int k = 0; k < 3; ++k) { L.lo[k] = R.lo[k] = nd.lo[k]; L.hi[k] = R.hi[k] = nd.hi[k]; }
+                }
+                const uint32_t c = next;  // at most 2 (count - 1) children in a binary tree of count leaves
+                next += 2;
+                L.left = e.begin; L.count = nl;
+                R.left = e.begin + nl; R.count = n - nl;
+                local[c] = L;
+                local[c + 1] = R;
+                nd.left = c;  // task-local until k_compact_small
+                nd.count = 0;
+                *slot = nd;
+                const Ent le{c, e.begin, nl, e.depth + 1}, re{c + 1, e.begin + nl, n - nl, e.depth + 1};
+                const bool left_small = nl <= n - nl;
+                if (sp < kSmallStack) stack[sp++] = left_small ? re : le;
+                else atomicOr(err, 2u);  // cannot happen: the stacked child is never the smaller one
+                e = left_small ? le : re;
+                continue;
+            }
+            have = sp > 0;
+            if (have) e = stack[--sp];
+        }
+        used[ti] = next;
+    }
+    // one atomic per wave for the depth
+    for (int off = 32; off > 0; off >>= 1) md = max(md, (uint32_t)__shfl_xor((int)md, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(max_depth, md);
+}
+
+// Moves each task's scratch nodes to their final place (first = the task's
+// exclusive-scan offset behind the large-level nodes) and turns task-local
+// child numbers into global ones, the task root's included.
+__global__ void k_compact_small(const SmallTask* tasks, uint32_t ntasks, const uint32_t* used,
+                                const uint32_t* first, uint32_t offset, const BNodeG* snodes, BNodeG* nodes) {
+    const uint32_t ti = blockIdx.x;
+    if (ti >= ntasks) return;
+    const SmallTask task = tasks[ti];
+    const uint32_t n = used[ti], dst = offset + first[ti];
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        BNodeG nd = snodes[task.base + i];
+        if (nd.count == 0) nd.left += dst;
+        nodes[dst + i] = nd;
+    }
+    if (threadIdx.x == 0 && nodes[task.node].count == 0) nodes[task.node].left += dst;
 }
 
 // Binary tree -> 64-byte child-box nodes.  internal_index[n] is the exclusive
@@ -507,50 +640,64 @@ __global__ void k_emit_tris(size_t nt, const uint32_t* prims, const uint32_t* id
     tris[3 * s + 2] = make_float4(c.px, c.py, c.pz, 0.f);
 }
 
-// Tree statistics (internal / leaf counts, largest leaf, SAH cost): each block
-// reduces in LDS and adds once, instead of one contended atomic per node.
-__global__ void __launch_bounds__(256) k_tree_stats(const BNodeG* nodes, uint32_t nn, float inv_root_area,
-                                                    unsigned long long* cnt, uint32_t* max_leaf, double* sah) {
-    __shared__ unsigned long long s_int[256], s_leaf[256];
-    __shared__ uint32_t s_max[256];
-    __shared__ double s_sah[256];
+// Tree statistics (internal / leaf counts, largest leaf, SAH cost): a fixed
+// grid strides over the nodes and reduces in LDS to one partial per block,
+// then one block adds the partials in a fixed order (no contended atomics,
+// the same sum on every run).
+constexpr int kStatsBlocks = 1024;
+struct StatPart {
+    unsigned long long internal, leaves;
+    uint32_t max_leaf, pad;
+    double sah;
+};
+
+__device__ void stats_reduce(StatPart* sp, StatPart v) {
     const uint32_t t = threadIdx.x;
-    const uint32_t n = blockIdx.x * blockDim.x + t;
-    unsigned long long ni = 0, nl = 0;
-    uint32_t mx = 0;
-    double sa = 0.0;
-    if (n < nn) {
-        const BNodeG& nd = nodes[n];
-        const double a = (double)(half_area(nd.lo, nd.hi) * inv_root_area);
-        if (nd.count == 0) {
-            ni = 1;
-            sa = a;
-        } else {
-            nl = 1;
-            mx = nd.count;
-            sa = a * (double)nd.count;
-        }
-    }
-    s_int[t] = ni;
-    s_leaf[t] = nl;
-    s_max[t] = mx;
-    s_sah[t] = sa;
+    sp[t] = v;
     __syncthreads();
-    for (uint32_t off = 128; off > 0; off >>= 1) {
+    for (uint32_t off = blockDim.x / 2; off > 0; off >>= 1) {
         if (t < off) {
-            s_int[t] += s_int[t + off];
-            s_leaf[t] += s_leaf[t + off];
-            s_max[t] = max(s_max[t], s_max[t + off]);
-            s_sah[t] += s_sah[t + off];
+            sp[t].internal += sp[t + off].internal;
+            sp[t].leaves += sp[t + off].leaves;
+            sp[t].max_leaf = max(sp[t].max_leaf, sp[t + off].max_leaf);
+            sp[t].sah += sp[t + off].sah;
         }
         __syncthreads();
     }
-    if (t == 0) {
-        atomicAdd(&cnt[0], s_int[0]);
-        atomicAdd(&cnt[1], s_leaf[0]);
-        atomicMax(max_leaf, s_max[0]);
-        atomicAdd(sah, s_sah[0]);
+}
+
+__global__ void __launch_bounds__(256) k_tree_stats(const BNodeG* nodes, uint32_t nn, float inv_root_area,
+                                                    StatPart* parts) {
+    __shared__ StatPart sp[256];
+    StatPart v{0ull, 0ull, 0u, 0u, 0.0};
+    for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < nn; n += gridDim.x * blockDim.x) {
+        const BNodeG& nd = nodes[n];
+        const double a = (double)(half_area(nd.lo, nd.hi) * inv_root_area);
+        if (nd.count == 0) {
+            v.internal++;
+            v.sah += a;
+        } else {
+            v.leaves++;
+            v.max_leaf = max(v.max_leaf, nd.count);
+            v.sah += a * (double)nd.count;
+        }
     }
+    stats_reduce(sp, v);
+    if (threadIdx.x == 0) parts[blockIdx.x] = sp[0];
+}
+
+__global__ void __launch_bounds__(256) k_tree_stats_sum(const StatPart* parts, uint32_t nparts, StatPart* out) {
+    __shared__ StatPart sp[256];
+    StatPart v{0ull, 0ull, 0u, 0u, 0.0};
+    for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x) {
+        const StatPart p = parts[i];
+        v.internal += p.internal;
+        v.leaves += p.leaves;
+        v.max_leaf = max(v.max_leaf, p.max_leaf);
+        v.sah += p.sah;
+    }
+    stats_reduce(sp, v);
+    if (threadIdx.x == 0) *out = sp[0];
 }
 
 float host_area(const float lo[3], const float hi[3]) {
@@ -635,12 +782,10 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     if (N > kSmall) large.push_back(rt);
     else small_h.push_back(rt);
 
-    DBuf d_chunks, d_params, d_bins, d_counts, d_bases;
+    DBuf d_chunks, d_params, d_bins, d_counts, d_bases, d_sweep;
     std::vector<Chunk> chunks;
     std::vector<TaskParam> params;
-    std::vector<int> hbins;
-    std::vector<uint32_t> hcounts;
-    std::vector<ChunkBase> bases;
+    std::vector<SweepOut> hsweep;
     uint32_t max_depth = 0;
     while (!large.empty()) {
         std::vector<HostTask> next;
@@ -680,15 +825,20 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
                 float ext[3] = {t.chi[0] - t.clo[0], t.chi[1] - t.clo[1], t.chi[2] - t.clo[2]};
                 int axis = (ext[1] > ext[0]) ? 1 : 0;
                 if (ext[2] > ext[axis]) axis = 2;
-                params.push_back(TaskParam{axis, t.clo[axis], (float)nb / ext[axis], -1});
+                const uint32_t chunk0 = (uint32_t)chunks.size();
                 for (uint32_t b = t.begin; b < t.begin + t.count; b += kChunk)
                     chunks.push_back(Chunk{ti, b, std::min(b + kChunk, t.begin + t.count), 0});
+                params.push_back(TaskParam{axis, t.clo[axis], (float)nb / ext[axis], -1, t.begin, chunk0,
+                                           (uint32_t)chunks.size() - chunk0, 0u});
             }
-            size_t nbv = binned.size() * nb * kBinVals;
+            const uint32_t nt = (uint32_t)binned.size();
+            size_t nbv = (size_t)nt * nb * kBinVals;
             d_chunks.alloc(chunks.size() * sizeof(Chunk));
             d_params.alloc(params.size() * sizeof(TaskParam));
             d_bins.alloc(nbv * 4);
             d_counts.alloc(chunks.size() * nb * 4);
+            d_sweep.alloc((size_t)nt * sizeof(SweepOut));
+            d_bases.alloc(chunks.size() * sizeof(ChunkBase));
             VG_HIP(hipMemcpyAsync(d_chunks.p, chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice, c.stream));
             VG_HIP(hipMemcpyAsync(d_params.p, params.data(), params.size() * sizeof(TaskParam), hipMemcpyHostToDevice, c.stream));
             k_init_bins<<<blocks_for(nbv, 256), 256, 0, c.stream>>>(d_bins.as<int>(), nbv);
@@ -697,66 +847,40 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
                 d_chunks.as<Chunk>(), d_params.as<TaskParam>(), nb, prims.as<uint32_t>(), pbuf.as<PrimBox>(),
                 d_bins.as<int>(), d_counts.as<uint32_t>());
             VG_HIP(hipGetLastError());
-            hbins.resize(nbv);
-            hcounts.resize(chunks.size() * nb);
-            VG_HIP(hipMemcpyAsync(hbins.data(), d_bins.p, nbv * 4, hipMemcpyDeviceToHost, c.stream));
-            VG_HIP(hipMemcpyAsync(hcounts.data(), d_counts.p, hcounts.size() * 4, hipMemcpyDeviceToHost, c.stream));
+            k_sweep_large<<<blocks_for(nt, 64), 64, 0, c.stream>>>(d_bins.as<int>(), nb, nt, d_params.as<TaskParam>(),
+                                                                   d_sweep.as<SweepOut>());
+            VG_HIP(hipGetLastError());
+            k_chunk_bases<<<blocks_for(nt, 64), 64, 0, c.stream>>>(d_chunks.as<Chunk>(), d_params.as<TaskParam>(),
+                                                                   d_sweep.as<SweepOut>(), d_counts.as<uint32_t>(),
+                                                                   nb, nt, d_bases.as<ChunkBase>());
+            VG_HIP(hipGetLastError());
+            // a node without a valid split leaves its triangles where they are (split -1: all go right at
+            // offset nleft = 0, in order); the build is rejected below before they are used
+            k_partition_large<<<(unsigned)chunks.size(), kChunkThreads, 0, c.stream>>>(
+                d_chunks.as<Chunk>(), d_params.as<TaskParam>(), nb, d_bases.as<ChunkBase>(), prims.as<uint32_t>(),
+                pbuf.as<PrimBox>(), tmp.as<uint32_t>());
+            VG_HIP(hipGetLastError());
+            k_copy_back<<<(unsigned)chunks.size(), 256, 0, c.stream>>>(d_chunks.as<Chunk>(), tmp.as<uint32_t>(),
+                                                                       prims.as<uint32_t>());
+            VG_HIP(hipGetLastError());
+            hsweep.resize(nt);
+            VG_HIP(hipMemcpyAsync(hsweep.data(), d_sweep.p, (size_t)nt * sizeof(SweepOut), hipMemcpyDeviceToHost,
+                                  c.stream));
             VG_HIP(hipStreamSynchronize(c.stream));
-            // SAH sweep per task on the host
-            std::vector<uint32_t> nleft(binned.size());
-            for (uint32_t ti = 0; ti < binned.size(); ++ti) {
+            // children of each binned node from its sweep
+            for (uint32_t ti = 0; ti < nt; ++ti) {
                 const HostTask& t = binned[ti];
-                const int* tb = &hbins[(size_t)ti * nb * kBinVals];
-                float rarea[kMaxBins];
-                uint32_t rcnt[kMaxBins];
-                float rl[3] = {INFINITY, INFINITY, INFINITY}, rh[3] = {-INFINITY, -INFINITY, -INFINITY};
-                uint32_t rc = 0;
-                for (int b = nb - 1; b > 0; --b) {
-                    const int* bb = tb + b * kBinVals;
-                    rc += (uint32_t)bb[0];
-                    if (bb[0] > 0)
-                        for (int k = 0; k < 3; ++k) { rl[k] = std::min(rl[k], key2f_h(bb[1 + k])); rh[k] = std::max(rh[k], key2f_h(bb[4 + k])); }
-                    rarea[b] = rc ? host_area(rl, rh) : 0.f;
-                    rcnt[b] = rc;
-                }
-                float ll[3] = {INFINITY, INFINITY, INFINITY}, lh[3] = {-INFINITY, -INFINITY, -INFINITY};
-                uint32_t lc = 0;
-                float best = INFINITY;
-                int split = -1;
-                for (int b = 0; b < nb - 1; ++b) {
-                    const int* bb = tb + b * kBinVals;
-                    lc += (uint32_t)bb[0];
-                    if (bb[0] > 0)
-                        for (int k = 0; k < 3; ++k) { ll[k] = std::min(ll[k], key2f_h(bb[1 + k])); lh[k] = std::max(lh[k], key2f_h(bb[4 + k])); }
-                    if (lc == 0 || rcnt[b + 1] == 0) continue;
-                    float cost = (float)lc * host_area(ll, lh) + (float)rcnt[b + 1] * rarea[b + 1];
-                    if (cost < best) { best = cost; split = b; }
-                }
-                VG_CHECK(split >= 0, VIRA_GPU_ERR_HIP, "BVH build: no valid split for a binned node");
-                params[ti].split = split;
-                // children from the bins
+                const SweepOut& o = hsweep[ti];
+                VG_CHECK(o.split >= 0, VIRA_GPU_ERR_HIP, "BVH build: no valid split for a binned node");
                 HostTask lt{}, rtk{};
                 BNodeG L{}, R{};
                 for (int k = 0; k < 3; ++k) {
-                    L.lo[k] = R.lo[k] = lt.clo[k] = rtk.clo[k] = INFINITY;
-                    L.hi[k] = R.hi[k] = lt.chi[k] = rtk.chi[k] = -INFINITY;
+                    L.lo[k] = o.lo[0][k]; L.hi[k] = o.hi[0][k];
+                    R.lo[k] = o.lo[1][k]; R.hi[k] = o.hi[1][k];
+                    lt.clo[k] = o.clo[0][k]; lt.chi[k] = o.chi[0][k];
+                    rtk.clo[k] = o.clo[1][k]; rtk.chi[k] = o.chi[1][k];
                 }
-                uint32_t nl = 0;
-                for (int b = 0; b < nb; ++b) {
-                    const int* bb = tb + b * kBinVals;
-                    if (bb[0] == 0) continue;
-                    bool left = b <= split;
-                    BNodeG& D = left ? L : R;
-                    HostTask& T = left ? lt : rtk;
-                    if (left) nl += (uint32_t)bb[0];
-                    for (int k = 0; k < 3; ++k) {
-                        D.lo[k] = std::min(D.lo[k], key2f_h(bb[1 + k]));
-                        D.hi[k] = std::max(D.hi[k], key2f_h(bb[4 + k]));
-                        T.clo[k] = std::min(T.clo[k], key2f_h(bb[7 + k]));
-                        T.chi[k] = std::max(T.chi[k], key2f_h(bb[10 + k]));
-                    }
-                }
-                nleft[ti] = nl;
+                const uint32_t nl = o.nleft;
                 uint32_t ci = (uint32_t)hnodes.size();
                 L.left = t.begin; L.count = nl;
                 R.left = t.begin + nl; R.count = t.count - nl;
@@ -769,53 +893,48 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
                 (lt.count > kSmall ? next : small_h).push_back(lt);
                 (rtk.count > kSmall ? next : small_h).push_back(rtk);
             }
-            // chunk bases for the stable partition
-            bases.resize(chunks.size());
-            std::vector<uint32_t> lrun(binned.size(), 0), rrun(binned.size(), 0);
-            for (size_t ci = 0; ci < chunks.size(); ++ci) {
-                uint32_t ti = chunks[ci].task;
-                uint32_t cl = 0, ctot = chunks[ci].end - chunks[ci].begin;
-                for (int b = 0; b <= params[ti].split; ++b) cl += hcounts[ci * nb + b];
-                bases[ci].left = binned[ti].begin + lrun[ti];
-                bases[ci].right = binned[ti].begin + nleft[ti] + rrun[ti];
-                lrun[ti] += cl;
-                rrun[ti] += ctot - cl;
-            }
-            d_bases.alloc(bases.size() * sizeof(ChunkBase));
-            VG_HIP(hipMemcpyAsync(d_bases.p, bases.data(), bases.size() * sizeof(ChunkBase), hipMemcpyHostToDevice, c.stream));
-            VG_HIP(hipMemcpyAsync(d_params.p, params.data(), params.size() * sizeof(TaskParam), hipMemcpyHostToDevice, c.stream));
-            k_partition_large<<<(unsigned)chunks.size(), kChunkThreads, 0, c.stream>>>(
-                d_chunks.as<Chunk>(), d_params.as<TaskParam>(), nb, d_bases.as<ChunkBase>(), prims.as<uint32_t>(),
-                pbuf.as<PrimBox>(), tmp.as<uint32_t>());
-            VG_HIP(hipGetLastError());
-            k_copy_back<<<(unsigned)chunks.size(), 256, 0, c.stream>>>(d_chunks.as<Chunk>(), tmp.as<uint32_t>(),
-                                                                       prims.as<uint32_t>());
-            VG_HIP(hipGetLastError());
-            VG_HIP(hipStreamSynchronize(c.stream));
         }
         large.swap(next);
     }
     // upload the large-level nodes, then finish the small subtrees on the GPU
     VG_HIP(hipMemcpyAsync(nodes.p, hnodes.data(), hnodes.size() * sizeof(BNodeG), hipMemcpyHostToDevice, c.stream));
-    DBuf d_small, d_counter;
-    d_counter.alloc(8);
-    uint32_t hc[2] = {(uint32_t)hnodes.size(), max_depth};
-    VG_HIP(hipMemcpyAsync(d_counter.p, hc, 8, hipMemcpyHostToDevice, c.stream));
+    // counter words: 0 unused, 1 max depth, 2 small-builder error flag (word 1 later becomes the emit error flag)
+    DBuf d_small, d_counter, snodes, d_used, d_first;
+    d_counter.alloc(12);
+    uint32_t hc[3] = {0u, max_depth, 0u};
+    VG_HIP(hipMemcpyAsync(d_counter.p, hc, 12, hipMemcpyHostToDevice, c.stream));
+    uint32_t nn = (uint32_t)hnodes.size();
     if (!small_h.empty()) {
-        std::vector<SmallTask> st(small_h.size());
-        for (size_t i = 0; i < small_h.size(); ++i)
-            st[i] = SmallTask{small_h[i].node, small_h[i].begin, small_h[i].count, small_h[i].depth};
+        const uint32_t ntasks = (uint32_t)small_h.size();
+        std::vector<SmallTask> st(ntasks);
+        size_t scratch = 0;
+        for (uint32_t i = 0; i < ntasks; ++i) {
+            st[i] = SmallTask{small_h[i].node, small_h[i].begin, small_h[i].count, small_h[i].depth,
+                              (uint32_t)scratch, 0u};
+            scratch += 2 * (size_t)(small_h[i].count - 1);
+        }
+        VG_CHECK(scratch < (1ull << 32), VIRA_GPU_ERR_INVALID, "BVH build: small-subtree scratch overflow");
         d_small.alloc(st.size() * sizeof(SmallTask));
+        snodes.alloc(std::max<size_t>(1, scratch) * sizeof(BNodeG));
+        d_used.alloc((size_t)ntasks * 4);
+        d_first.alloc((size_t)ntasks * 4);
         VG_HIP(hipMemcpyAsync(d_small.p, st.data(), st.size() * sizeof(SmallTask), hipMemcpyHostToDevice, c.stream));
-        k_build_small<<<blocks_for(st.size(), 64), 64, 0, c.stream>>>(
-            d_small.as<SmallTask>(), (uint32_t)st.size(), prims.as<uint32_t>(), pbuf.as<PrimBox>(),
-            nodes.as<BNodeG>(), d_counter.as<uint32_t>(), d_counter.as<uint32_t>() + 1, max_leaf,
-            opt.traversal_cost > 0.0f ? opt.traversal_cost : 1.0f);
+        k_build_small<<<blocks_for(ntasks, 64), 64, 0, c.stream>>>(
+            d_small.as<SmallTask>(), ntasks, prims.as<uint32_t>(), pbuf.as<PrimBox>(), nodes.as<BNodeG>(),
+            snodes.as<BNodeG>(), d_used.as<uint32_t>(), d_counter.as<uint32_t>() + 1, d_counter.as<uint32_t>() + 2,
+            max_leaf, opt.traversal_cost > 0.0f ? opt.traversal_cost : 1.0f);
         VG_HIP(hipGetLastError());
+        const uint32_t small_nodes = scan_exclusive_u32(c, d_used.as<uint32_t>(), d_first.as<uint32_t>(), ntasks);
+        VG_CHECK((size_t)nn + small_nodes <= 2 * N, VIRA_GPU_ERR_HIP, "BVH node count overflow");
+        k_compact_small<<<ntasks, 64, 0, c.stream>>>(d_small.as<SmallTask>(), ntasks, d_used.as<uint32_t>(),
+                                                     d_first.as<uint32_t>(), nn, snodes.as<BNodeG>(),
+                                                     nodes.as<BNodeG>());
+        VG_HIP(hipGetLastError());
+        nn += small_nodes;
     }
-    VG_HIP(hipMemcpyAsync(hc, d_counter.p, 8, hipMemcpyDeviceToHost, c.stream));
+    VG_HIP(hipMemcpyAsync(hc, d_counter.p, 12, hipMemcpyDeviceToHost, c.stream));
     VG_HIP(hipStreamSynchronize(c.stream));
-    const uint32_t nn = hc[0];
+    VG_CHECK(hc[2] == 0, VIRA_GPU_ERR_HIP, "BVH build: small-subtree stack overflow");
     max_depth = hc[1];
     VG_CHECK(max_depth < 63, VIRA_GPU_ERR_HIP, "BVH deeper than the traversal stack");
     VG_CHECK(nn <= 2 * N, VIRA_GPU_ERR_HIP, "BVH node count overflow");
@@ -884,15 +1003,17 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     VG_HIP(hipGetLastError());
     VG_HIP(hipEventRecord(e1, c.stream));
     // statistics (outside the timed region)
+    const uint32_t sblocks = std::min<uint32_t>(kStatsBlocks, blocks_for(nn, 256));
     DBuf sbuf;
-    sbuf.alloc(32);
-    VG_HIP(hipMemsetAsync(sbuf.p, 0, 32, c.stream));
-    k_tree_stats<<<blocks_for(nn, 256), 256, 0, c.stream>>>(
-        nodes.as<BNodeG>(), nn, root_area > 0.f ? 1.0f / root_area : 0.f, sbuf.as<unsigned long long>(),
-        reinterpret_cast<uint32_t*>(sbuf.as<char>() + 16), reinterpret_cast<double*>(sbuf.as<char>() + 24));
+    sbuf.alloc((sblocks + 1) * sizeof(StatPart));
+    StatPart* parts = sbuf.as<StatPart>();
+    k_tree_stats<<<sblocks, 256, 0, c.stream>>>(nodes.as<BNodeG>(), nn, root_area > 0.f ? 1.0f / root_area : 0.f,
+                                               parts);
     VG_HIP(hipGetLastError());
-    unsigned char hs[32];
-    VG_HIP(hipMemcpyAsync(hs, sbuf.p, 32, hipMemcpyDeviceToHost, c.stream));
+    k_tree_stats_sum<<<1, 256, 0, c.stream>>>(parts, sblocks, parts + sblocks);
+    VG_HIP(hipGetLastError());
+    StatPart hs;
+    VG_HIP(hipMemcpyAsync(&hs, parts + sblocks, sizeof(hs), hipMemcpyDeviceToHost, c.stream));
     uint32_t emit_err = 0;
     VG_HIP(hipMemcpyAsync(&emit_err, d_counter.as<char>() + 4, 4, hipMemcpyDeviceToHost, c.stream));
     VG_HIP(hipStreamSynchronize(c.stream));
@@ -901,21 +1022,15 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     VG_HIP(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    unsigned long long cnts[2];
-    uint32_t mleaf;
-    double sah;
-    std::memcpy(cnts, hs, 16);
-    std::memcpy(&mleaf, hs + 16, 4);
-    std::memcpy(&sah, hs + 24, 8);
     B.stats = vira_gpu_bvh_stats{};
-    B.stats.num_nodes = cnts[0];
+    B.stats.num_nodes = hs.internal;
     B.stats.num_wide_nodes = B.num_nodes;
     B.stats.width = (uint32_t)B.width;
     B.stats.wide_depth = wide_depth;
-    B.stats.num_leaves = cnts[1];
+    B.stats.num_leaves = hs.leaves;
     B.stats.max_depth = max_depth + 1;
-    B.stats.max_leaf_size = mleaf;
-    B.stats.sah_cost = sah;
+    B.stats.max_leaf_size = hs.max_leaf;
+    B.stats.sah_cost = hs.sah;
     B.stats.build_ms = ms;
     B.stats.bytes = B.num_nodes * (B.width == 2 ? sizeof(Node2) : sizeof(Node4)) + N * 3 * sizeof(float4);
     B.built = true;
