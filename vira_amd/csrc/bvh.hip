@@ -132,9 +132,11 @@ __global__ void k_init_bins(int* bins, size_t nvals) {
 }
 
 // One workgroup per chunk of a large node: bin into LDS, merge into the node's bins.
+// Bins one chunk of a node in LDS and merges the chunk's bins into the
+// node's; the bin of every slot is kept for the partition.
 __global__ void __launch_bounds__(kChunkThreads) k_bin_large(const Chunk* chunks, const TaskParam* params, int nb,
                                                              const uint32_t* prims, const PrimBox* pb, int* bins,
-                                                             uint32_t* chunk_counts) {
+                                                             uint32_t* chunk_counts, uint8_t* slot_bin) {
     __shared__ int s[kMaxBins * kBinVals];
     for (int i = threadIdx.x; i < nb * kBinVals; i += blockDim.x) {
         int v = i % kBinVals;
@@ -147,6 +149,7 @@ __global__ void __launch_bounds__(kChunkThreads) k_bin_large(const Chunk* chunks
         const PrimBox& p = pb[prims[slot]];
         float cc = tp.axis == 0 ? p.c.x : (tp.axis == 1 ? p.c.y : p.c.z);
         int b = bin_of(cc, tp.cmin, tp.scale, nb);
+        slot_bin[slot] = (uint8_t)b;
         int* sb = &s[b * kBinVals];
         atomicAdd(&sb[0], 1);
         atomicMin(&sb[1], f2key(p.lo.x));
@@ -178,11 +181,9 @@ __global__ void __launch_bounds__(kChunkThreads) k_bin_large(const Chunk* chunks
     }
 }
 
-// Stable partition of one chunk into tmp (left part, then right part).
 __global__ void __launch_bounds__(kChunkThreads) k_partition_large(const Chunk* chunks, const TaskParam* params,
-                                                                   int nb, const ChunkBase* bases,
-                                                                   const uint32_t* prims, const PrimBox* pb,
-                                                                   uint32_t* tmp) {
+                                                                   const ChunkBase* bases, const uint32_t* prims,
+                                                                   const uint8_t* slot_bin, uint32_t* tmp) {
     __shared__ uint32_t lds[kChunkThreads / 64 + 1];
     Chunk ch = chunks[blockIdx.x];
     TaskParam tp = params[ch.task];
@@ -193,11 +194,8 @@ __global__ void __launch_bounds__(kChunkThreads) k_partition_large(const Chunk* 
         uint32_t slot = ch.begin + threadIdx.x * kChunkItems + k;
         ids[k] = 0;
         if (slot < ch.end) {
-            uint32_t p = prims[slot];
-            ids[k] = p;
-            const PrimBox& b = pb[p];
-            float cc = tp.axis == 0 ? b.c.x : (tp.axis == 1 ? b.c.y : b.c.z);
-            if (bin_of(cc, tp.cmin, tp.scale, nb) <= tp.split) {
+            ids[k] = prims[slot];
+            if ((int)slot_bin[slot] <= tp.split) {
                 flags |= 1u << k;
                 nl++;
             }
@@ -308,24 +306,33 @@ __global__ void k_sweep_large(const int* __restrict__ bins, int nb, uint32_t nta
     out[ti] = o;
 }
 
-// Where each chunk of a binned node writes its left and right triangles for
-// the stable partition: the node's chunks in order, lefts from the node's
-// first slot, rights after all its lefts.
+// Per chunk: how many of its triangles go left (bins <= the node's split).
+__global__ void k_chunk_left(const Chunk* __restrict__ chunks, const TaskParam* __restrict__ params,
+                             const uint32_t* __restrict__ counts, int nb, uint32_t nchunks, uint32_t* left) {
+    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= nchunks) return;
+    const int split = params[chunks[ci].task].split;
+    uint32_t cl = 0;
+    for (int b = 0; b <= split; ++b) cl += counts[(size_t)ci * nb + b];
+    left[ci] = cl;
+}
+
+// Where each chunk writes its left and right triangles in the stable
+// partition: the node's chunks in order, lefts from the node's first slot,
+// rights after all its lefts.  lscan = exclusive scan of k_chunk_left over all
+// chunks; a node's chunks are consecutive, so the lefts before chunk ci within
+// its node are lscan[ci] - lscan[first chunk], and the rights the remaining
+// slots before it.
 __global__ void k_chunk_bases(const Chunk* __restrict__ chunks, const TaskParam* __restrict__ params,
-                              const SweepOut* __restrict__ sw, const uint32_t* __restrict__ counts, int nb,
-                              uint32_t ntasks, ChunkBase* bases) {
-    const uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ti >= ntasks) return;
-    const TaskParam tp = params[ti];
-    const uint32_t nleft = sw[ti].nleft;
-    uint32_t lrun = 0, rrun = 0;
-    for (uint32_t ci = tp.chunk0; ci < tp.chunk0 + tp.nchunk; ++ci) {
-        uint32_t cl = 0;
-        for (int b = 0; b <= tp.split; ++b) cl += counts[(size_t)ci * nb + b];
-        bases[ci] = ChunkBase{tp.begin + lrun, tp.begin + nleft + rrun};
-        lrun += cl;
-        rrun += (chunks[ci].end - chunks[ci].begin) - cl;
-    }
+                              const SweepOut* __restrict__ sw, const uint32_t* __restrict__ lscan, uint32_t nchunks,
+                              ChunkBase* bases) {
+    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= nchunks) return;
+    const Chunk ch = chunks[ci];
+    const TaskParam tp = params[ch.task];
+    const uint32_t lrun = lscan[ci] - lscan[tp.chunk0];
+    const uint32_t rrun = (ch.begin - tp.begin) - lrun;
+    bases[ci] = ChunkBase{tp.begin + lrun, tp.begin + sw[ch.task].nleft + rrun};
 }
 
 // One thread builds the whole subtree of a small node (the per-thread
@@ -748,8 +755,9 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
         (void)hipEventDestroy(e1);
         return;
     }
-    DBuf pbuf, prims, tmp, nodes, keys;
+    DBuf pbuf, prims, tmp, nodes, keys, slot_bin;
     pbuf.alloc(N * sizeof(PrimBox));
+    slot_bin.alloc(N);
     prims.alloc(N * 4);
     tmp.alloc(N * 4);
     nodes.alloc(2 * N * sizeof(BNodeG));
@@ -782,7 +790,7 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     if (N > kSmall) large.push_back(rt);
     else small_h.push_back(rt);
 
-    DBuf d_chunks, d_params, d_bins, d_counts, d_bases, d_sweep;
+    DBuf d_chunks, d_params, d_bins, d_counts, d_bases, d_sweep, d_left, d_lscan;
     std::vector<Chunk> chunks;
     std::vector<TaskParam> params;
     std::vector<SweepOut> hsweep;
@@ -839,26 +847,34 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
             d_counts.alloc(chunks.size() * nb * 4);
             d_sweep.alloc((size_t)nt * sizeof(SweepOut));
             d_bases.alloc(chunks.size() * sizeof(ChunkBase));
+            d_left.alloc(chunks.size() * 4);
+            d_lscan.alloc(chunks.size() * 4);
             VG_HIP(hipMemcpyAsync(d_chunks.p, chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice, c.stream));
             VG_HIP(hipMemcpyAsync(d_params.p, params.data(), params.size() * sizeof(TaskParam), hipMemcpyHostToDevice, c.stream));
             k_init_bins<<<blocks_for(nbv, 256), 256, 0, c.stream>>>(d_bins.as<int>(), nbv);
             VG_HIP(hipGetLastError());
             k_bin_large<<<(unsigned)chunks.size(), kChunkThreads, 0, c.stream>>>(
                 d_chunks.as<Chunk>(), d_params.as<TaskParam>(), nb, prims.as<uint32_t>(), pbuf.as<PrimBox>(),
-                d_bins.as<int>(), d_counts.as<uint32_t>());
+                d_bins.as<int>(), d_counts.as<uint32_t>(), slot_bin.as<uint8_t>());
             VG_HIP(hipGetLastError());
             k_sweep_large<<<blocks_for(nt, 64), 64, 0, c.stream>>>(d_bins.as<int>(), nb, nt, d_params.as<TaskParam>(),
                                                                    d_sweep.as<SweepOut>());
             VG_HIP(hipGetLastError());
-            k_chunk_bases<<<blocks_for(nt, 64), 64, 0, c.stream>>>(d_chunks.as<Chunk>(), d_params.as<TaskParam>(),
-                                                                   d_sweep.as<SweepOut>(), d_counts.as<uint32_t>(),
-                                                                   nb, nt, d_bases.as<ChunkBase>());
+            const uint32_t nch = (uint32_t)chunks.size();
+            k_chunk_left<<<blocks_for(nch, 256), 256, 0, c.stream>>>(d_chunks.as<Chunk>(), d_params.as<TaskParam>(),
+                                                                     d_counts.as<uint32_t>(), nb, nch,
+                                                                     d_left.as<uint32_t>());
+            VG_HIP(hipGetLastError());
+            scan_exclusive_u32(c, d_left.as<uint32_t>(), d_lscan.as<uint32_t>(), nch);
+            k_chunk_bases<<<blocks_for(nch, 256), 256, 0, c.stream>>>(d_chunks.as<Chunk>(), d_params.as<TaskParam>(),
+                                                                      d_sweep.as<SweepOut>(), d_lscan.as<uint32_t>(),
+                                                                      nch, d_bases.as<ChunkBase>());
             VG_HIP(hipGetLastError());
             // a node without a valid split leaves its triangles where they are (split -1: all go right at
             // offset nleft = 0, in order); the build is rejected below before they are used
             k_partition_large<<<(unsigned)chunks.size(), kChunkThreads, 0, c.stream>>>(
-                d_chunks.as<Chunk>(), d_params.as<TaskParam>(), nb, d_bases.as<ChunkBase>(), prims.as<uint32_t>(),
-                pbuf.as<PrimBox>(), tmp.as<uint32_t>());
+                d_chunks.as<Chunk>(), d_params.as<TaskParam>(), d_bases.as<ChunkBase>(), prims.as<uint32_t>(),
+                slot_bin.as<uint8_t>(), tmp.as<uint32_t>());
             VG_HIP(hipGetLastError());
             k_copy_back<<<(unsigned)chunks.size(), 256, 0, c.stream>>>(d_chunks.as<Chunk>(), tmp.as<uint32_t>(),
                                                                        prims.as<uint32_t>());
