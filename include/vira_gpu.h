@@ -308,12 +308,22 @@ vira_gpu_status vira_gpu_mesh_release(vira_gpu_ctx* ctx, int32_t mesh_id);
  * recomputed in place and a built BLAS is rebuilt with its last options.  The
  * set of non-finite heights (holes) must not change. */
 vira_gpu_status vira_gpu_dem_update(vira_gpu_ctx* ctx, int32_t mesh_id, const float* heights_dev);
+/* vira_gpu_dem_update with the BLAS refitted (vira_gpu_blas_refit) instead of
+ * rebuilt: for the small height changes of an optimisation loop. */
+vira_gpu_status vira_gpu_dem_update_refit(vira_gpu_ctx* ctx, int32_t mesh_id, const float* heights_dev);
 
 /* --- acceleration (a2, a3) ------------------------------------------------ */
 /* Mesh::buildBVH -> EmbreeBLAS (mesh.ipp:396-418, embree_blas.ipp:14-71):
  * binned-SAH build on the GPU. opts may be NULL. */
 vira_gpu_status vira_gpu_blas_build(vira_gpu_ctx* ctx, int32_t mesh_id, const vira_gpu_bvh_opts* opts);
 vira_gpu_status vira_gpu_blas_stats(vira_gpu_ctx* ctx, int32_t mesh_id, vira_gpu_bvh_stats* stats);
+/* New (the reference rebuilds, embree_blas.ipp:14-71): a built BLAS refitted
+ * to its mesh's current vertex positions, same triangles.  The tree is kept;
+ * the leaf triangles and every box are recomputed bottom up, so hits stay
+ * exact while traversal cost drifts with the deformation.  A 2-wide BLAS is
+ * rebuilt with its last options.  stats.build_ms becomes the refit's time;
+ * the other statistics describe the tree as built. */
+vira_gpu_status vira_gpu_blas_refit(vira_gpu_ctx* ctx, int32_t mesh_id);
 
 typedef struct {
     int32_t mesh_id;

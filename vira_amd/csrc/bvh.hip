@@ -556,14 +556,13 @@ struct CItem {
     uint32_t bnode, n4;
 };
 
-__global__ void k_collapse4(const BNodeG* __restrict__ nodes, const CItem* __restrict__ in, uint32_t nin,
-                            Node4* __restrict__ out, uint32_t* __restrict__ n4_counter, CItem* __restrict__ qout,
-                            uint32_t* __restrict__ qout_count, uint32_t* __restrict__ err) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nin) return;
-    CItem it = in[i];
-    const BNodeG& bn = nodes[it.bnode];
-    uint32_t ch[4] = {bn.left, bn.left + 1, 0u, 0u};
+// The up-to-4 descendants a Node4 takes from binary node bnode: its two
+// children, then repeatedly the internal one of largest surface area opened
+// into its two children.
+__device__ int collapse_children(const BNodeG* __restrict__ nodes, uint32_t bnode, uint32_t ch[4]) {
+    const BNodeG& bn = nodes[bnode];
+    ch[0] = bn.left;
+    ch[1] = bn.left + 1;
     int nc = 2;
     while (nc < 4) {
         int best = -1;
@@ -582,8 +581,36 @@ __global__ void k_collapse4(const BNodeG* __restrict__ nodes, const CItem* __res
         ch[best] = l;
         ch[nc++] = l + 1;
     }
+    return nc;
+}
+
+// Binary tree -> Node4, one level of Node4s per pair of launches: the count
+// pass gives each item its number of internal children, an exclusive scan
+// numbers them, and the emit pass writes the item's Node4 and queues those
+// children (Node4 index = first index of the next level + scan offset), so the
+// layout is the same on every run and each level's Node4s are consecutive.
+__global__ void k_collapse4_count(const BNodeG* __restrict__ nodes, const CItem* __restrict__ in, uint32_t nin,
+                                  uint32_t* __restrict__ cnt) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nin) return;
+    uint32_t ch[4];
+    const int nc = collapse_children(nodes, in[i].bnode, ch);
+    uint32_t n = 0;
+    for (int k = 0; k < nc; ++k) n += nodes[ch[k]].count == 0 ? 1u : 0u;
+    cnt[i] = n;
+}
+
+__global__ void k_collapse4_emit(const BNodeG* __restrict__ nodes, const CItem* __restrict__ in, uint32_t nin,
+                                 const uint32_t* __restrict__ off, uint32_t next_base, Node4* __restrict__ out,
+                                 CItem* __restrict__ qout, uint32_t* __restrict__ err) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nin) return;
+    const CItem it = in[i];
+    uint32_t ch[4];
+    const int nc = collapse_children(nodes, it.bnode, ch);
     float lo[3][4], hi[3][4];
     uint32_t ref[4];
+    uint32_t q = off[i];
     for (int k = 0; k < 4; ++k) {
         if (k >= nc) {
             for (int a = 0; a < 3; ++a) lo[a][k] = hi[a][k] = 0.0f;
@@ -603,9 +630,8 @@ __global__ void k_collapse4(const BNodeG* __restrict__ nodes, const CItem* __res
                 ref[k] = make_leaf(c.left, c.count);
             }
         } else {
-            uint32_t n = atomicAdd(n4_counter, 1u);
-            uint32_t slot = atomicAdd(qout_count, 1u);
-            qout[slot] = CItem{ch[k], n};
+            const uint32_t n = next_base + q;
+            qout[q++] = CItem{ch[k], n};
             ref[k] = n;
         }
     }
@@ -619,6 +645,78 @@ __global__ void k_collapse4(const BNodeG* __restrict__ nodes, const CItem* __res
     o.ref = make_uint4(ref[0], ref[1], ref[2], ref[3]);
     o.pad = make_uint4(0u, 0u, 0u, 0u);
     out[it.n4] = o;
+}
+
+// Refit (the tree kept, boxes recomputed from the current vertices): the
+// leaf-order triangles are rewritten from their ids, then each level of
+// Node4s, deepest first, takes every child's box from its triangles (leaf)
+// or from the union of the child Node4's boxes.
+__global__ void k_refit_tris(size_t nt, const uint32_t* __restrict__ idx, const GVertex* __restrict__ V,
+                             float4* __restrict__ tris) {
+    const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nt) return;
+    const float4 t0 = tris[3 * s];
+    const size_t p = __float_as_uint(t0.w);
+    const GVertex& a = V[idx[3 * p]];
+    const GVertex& b = V[idx[3 * p + 1]];
+    const GVertex& c = V[idx[3 * p + 2]];
+    tris[3 * s] = make_float4(a.px, a.py, a.pz, t0.w);
+    tris[3 * s + 1] = make_float4(b.px, b.py, b.pz, 0.f);
+    tris[3 * s + 2] = make_float4(c.px, c.py, c.pz, 0.f);
+}
+
+__device__ __forceinline__ float f4_get(const float4& v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
+__device__ __forceinline__ uint32_t u4_get(const uint4& v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
+
+__global__ void k_refit4(Node4* __restrict__ nodes, uint32_t first, uint32_t count, const float4* __restrict__ tris) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    Node4 nd = nodes[first + i];
+    float lo[3][4], hi[3][4];
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t r = u4_get(nd.ref, k);
+        float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+        if (r == kEmptyRef) {
+            for (int a = 0; a < 3; ++a) l[a] = h[a] = 0.0f;
+        } else if (ref_is_leaf(r)) {
+            const uint32_t f = leaf_first(r), n = leaf_count(r);
+            for (uint32_t t = f; t < f + n; ++t)
+                for (int v = 0; v < 3; ++v) {
+                    const float4 p = tris[3 * (size_t)t + v];
+                    l[0] = fminf(l[0], p.x); l[1] = fminf(l[1], p.y); l[2] = fminf(l[2], p.z);
+                    h[0] = fmaxf(h[0], p.x); h[1] = fmaxf(h[1], p.y); h[2] = fmaxf(h[2], p.z);
+                }
+        } else {
+            const Node4 cn = nodes[r];
+            for (int q = 0; q < 4; ++q) {
+                if (u4_get(cn.ref, q) == kEmptyRef) continue;
+                l[0] = fminf(l[0], f4_get(cn.lox, q)); h[0] = fmaxf(h[0], f4_get(cn.hix, q));
+                l[1] = fminf(l[1], f4_get(cn.loy, q)); h[1] = fmaxf(h[1], f4_get(cn.hiy, q));
+                l[2] = fminf(l[2], f4_get(cn.loz, q)); h[2] = fmaxf(h[2], f4_get(cn.hiz, q));
+            }
+        }
+        for (int a = 0; a < 3; ++a) { lo[a][k] = l[a]; hi[a][k] = h[a]; }
+    }
+    nd.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+    nd.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+    nd.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+    nd.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+    nd.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+    nd.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+    nodes[first + i] = nd;
+}
+
+// The root's box (the union of its children) for the BLAS box kept on the host.
+__global__ void k_root_box4(const Node4* __restrict__ nodes, float* __restrict__ box) {
+    const Node4 nd = nodes[0];
+    float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int q = 0; q < 4; ++q) {
+        if (u4_get(nd.ref, q) == kEmptyRef) continue;
+        l[0] = fminf(l[0], f4_get(nd.lox, q)); h[0] = fmaxf(h[0], f4_get(nd.hix, q));
+        l[1] = fminf(l[1], f4_get(nd.loy, q)); h[1] = fmaxf(h[1], f4_get(nd.hiy, q));
+        l[2] = fminf(l[2], f4_get(nd.loz, q)); h[2] = fmaxf(h[2], f4_get(nd.hiz, q));
+    }
+    for (int a = 0; a < 3; ++a) { box[a] = l[a]; box[3 + a] = h[a]; }
 }
 
 __global__ void k_emit_leaf_root4(const BNodeG* nodes, Node4* out) {
@@ -725,12 +823,51 @@ struct HostTask {
 
 }  // namespace
 
+void blas_refit(Ctx& c, MeshG& m) {
+    Blas& B = m.blas;
+    VG_CHECK(B.built, VIRA_GPU_ERR_INVALID, "blas_refit: the mesh has no built BLAS");
+    if (m.nt == 0) return;
+    if (B.width != 4 || B.level_start.size() < 2) {
+        blas_build(c, m, B.opts);
+        return;
+    }
+    hipEvent_t e0, e1;
+    VG_HIP(hipEventCreate(&e0));
+    VG_HIP(hipEventCreate(&e1));
+    VG_HIP(hipEventRecord(e0, c.stream));
+    k_refit_tris<<<blocks_for(m.nt, 256), 256, 0, c.stream>>>(m.nt, m.idx.as<uint32_t>(), m.verts.as<GVertex>(),
+                                                              B.tris.as<float4>());
+    VG_HIP(hipGetLastError());
+    for (size_t l = B.level_start.size() - 1; l-- > 0;) {
+        const uint32_t first = B.level_start[l], count = B.level_start[l + 1] - first;
+        if (count == 0) continue;
+        k_refit4<<<blocks_for(count, 256), 256, 0, c.stream>>>(B.nodes.as<Node4>(), first, count,
+                                                               B.tris.as<float4>());
+        VG_HIP(hipGetLastError());
+    }
+    DBuf box;
+    box.alloc(6 * sizeof(float));
+    k_root_box4<<<1, 1, 0, c.stream>>>(B.nodes.as<Node4>(), box.as<float>());
+    VG_HIP(hipGetLastError());
+    VG_HIP(hipEventRecord(e1, c.stream));
+    float hb[6];
+    VG_HIP(hipMemcpyAsync(hb, box.p, sizeof(hb), hipMemcpyDeviceToHost, c.stream));
+    VG_HIP(hipStreamSynchronize(c.stream));
+    for (int k = 0; k < 3; ++k) { B.lo[k] = hb[k]; B.hi[k] = hb[3 + k]; }
+    float ms = 0.f;
+    VG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    B.stats.build_ms = ms;
+}
+
 void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     const int max_leaf = std::min(15, std::max(1, opt.max_leaf > 0 ? opt.max_leaf : 4));
     const int nb = std::min(kMaxBins, std::max(4, opt.bins > 0 ? opt.bins : 16));
     const size_t N = m.nt;
     Blas& B = m.blas;
     B.built = false;
+    B.level_start.clear();
     VG_CHECK(N < (1u << 27), VIRA_GPU_ERR_INVALID, "BLAS limited to 2^27 triangles");
     hipEvent_t e0, e1;
     VG_HIP(hipEventCreate(&e0));
@@ -985,32 +1122,36 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
             k_emit_leaf_root4<<<1, 1, 0, c.stream>>>(nodes.as<BNodeG>(), B.nodes.as<Node4>());
             VG_HIP(hipGetLastError());
             B.num_nodes = 1;
+            B.level_start = {0u, 1u};
         } else {
-            DBuf qa, qb, qc;
+            DBuf qa, qb, qcnt, qoff;
             qa.alloc(cap * sizeof(CItem));
             qb.alloc(cap * sizeof(CItem));
-            qc.alloc(8);
+            qcnt.alloc(cap * 4);
+            qoff.alloc(cap * 4);
             CItem root{0u, 0u};
             VG_HIP(hipMemcpyAsync(qa.p, &root, sizeof(root), hipMemcpyHostToDevice, c.stream));
-            uint32_t hq[2] = {1u, 0u};  // node4 counter, next-queue length
-            uint32_t nin = 1;
+            uint32_t total = 1, nin = 1;
+            B.level_start = {0u, 1u};  // level 0 = the root; each pass below appends the end of the next level
             CItem* qin = qa.as<CItem>();
             CItem* qout = qb.as<CItem>();
             while (nin > 0) {
-                hq[1] = 0;
-                VG_HIP(hipMemcpyAsync(qc.p, hq, 8, hipMemcpyHostToDevice, c.stream));
-                k_collapse4<<<blocks_for(nin, 256), 256, 0, c.stream>>>(nodes.as<BNodeG>(), qin, nin,
-                                                                        B.nodes.as<Node4>(), qc.as<uint32_t>(), qout,
-                                                                        qc.as<uint32_t>() + 1, d_err);
+                k_collapse4_count<<<blocks_for(nin, 256), 256, 0, c.stream>>>(nodes.as<BNodeG>(), qin, nin,
+                                                                              qcnt.as<uint32_t>());
                 VG_HIP(hipGetLastError());
-                VG_HIP(hipMemcpyAsync(hq, qc.p, 8, hipMemcpyDeviceToHost, c.stream));
-                VG_HIP(hipStreamSynchronize(c.stream));
-                VG_CHECK(hq[0] <= cap, VIRA_GPU_ERR_HIP, "BVH4 collapse overflow");
-                nin = hq[1];
+                const uint32_t nnext = scan_exclusive_u32(c, qcnt.as<uint32_t>(), qoff.as<uint32_t>(), nin);
+                VG_CHECK((size_t)total + nnext <= cap, VIRA_GPU_ERR_HIP, "BVH4 collapse overflow");
+                k_collapse4_emit<<<blocks_for(nin, 256), 256, 0, c.stream>>>(nodes.as<BNodeG>(), qin, nin,
+                                                                             qoff.as<uint32_t>(), total,
+                                                                             B.nodes.as<Node4>(), qout, d_err);
+                VG_HIP(hipGetLastError());
+                total += nnext;
+                B.level_start.push_back(total);
+                nin = nnext;
                 std::swap(qin, qout);
                 wide_depth++;
             }
-            B.num_nodes = hq[0];
+            B.num_nodes = total;
         }
     }
     B.tris.alloc(N * 3 * sizeof(float4));

@@ -348,6 +348,28 @@ vira_gpu_status vira_gpu_dem_update(vira_gpu_ctx* ctx, int32_t mesh_id, const fl
     });
 }
 
+vira_gpu_status vira_gpu_dem_update_refit(vira_gpu_ctx* ctx, int32_t mesh_id, const float* heights_dev) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        set_device(c);
+        VG_CHECK(heights_dev, VIRA_GPU_ERR_INVALID, "dem_update_refit: null heights");
+        vg::MeshG& m = c.mesh(mesh_id);
+        c.scene_gen++;
+        vg::dem_update(c, m, heights_dev);
+        if (m.blas.built) vg::blas_refit(c, m);
+    });
+}
+
+vira_gpu_status vira_gpu_blas_refit(vira_gpu_ctx* ctx, int32_t mesh_id) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        set_device(c);
+        vg::MeshG& m = c.mesh(mesh_id);
+        c.scene_gen++;
+        vg::blas_refit(c, m);
+    });
+}
+
 vira_gpu_status vira_gpu_di_render(vira_gpu_ctx* ctx, const vira_gpu_camera* cam, const vira_gpu_di_opts* opts,
                                    float* radiance_dev, vira_gpu_di_stats* stats) {
     return guarded([&] {

@@ -89,6 +89,7 @@ struct Blas {
     vira_gpu_bvh_stats stats{};
     vira_gpu_bvh_opts opts{4, 16, 4};  // options of the last build (dem_update rebuilds with them)
     float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {0.f, 0.f, 0.f};  // box of all triangles (empty: lo > hi)
+    std::vector<uint32_t> level_start;  // Node4 levels: level l is [level_start[l], level_start[l + 1])
 };
 
 struct MeshG {
@@ -199,6 +200,9 @@ void batch_submit(Ctx& c, const vira_gpu_request* reqs, size_t n, const vira_gpu
 void batch_autotune(Ctx& c, const vira_gpu_request* reqs, size_t n, int stride, vira_gpu_batch_config* best,
                     double* best_ms);
 void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& o);
+// New boxes for a built BLAS after its mesh's vertices moved (same triangles);
+// 4-wide BLASes are refitted in place, 2-wide ones rebuilt.
+void blas_refit(Ctx& c, MeshG& m);
 void trace_rays(Ctx& c, MeshG& m, const float* rays, size_t n, int any_hit, float* t, uint32_t* id, float* b1,
                 float* b2, double* ms);
 void render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, const vira_gpu_outputs& out,

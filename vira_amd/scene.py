@@ -603,9 +603,19 @@ class Context:
                            best.order), ms.value
 
     # ---- differentiable direct illumination
-    def dem_update(self, mid: int, heights):
-        """New heights (H, W) for a DEM mesh; rebuilds its BLAS if one was built."""
-        self._call(self.lib.vira_gpu_dem_update, self.h, mid, heights.contiguous().data_ptr())
+    def dem_update(self, mid: int, heights, refit: bool = False):
+        """New heights (H, W) for a DEM mesh; rebuilds its BLAS if one was built,
+        or refits it (same tree, new boxes) with refit=True."""
+        fn = self.lib.vira_gpu_dem_update_refit if refit else self.lib.vira_gpu_dem_update
+        self._call(fn, self.h, mid, heights.contiguous().data_ptr())
+
+    def blas_refit(self, mid: int):
+        """Refits a built BLAS to the mesh's current vertices (same tree, new
+        boxes); returns the BLAS statistics (build_ms = the refit's time)."""
+        self._call(self.lib.vira_gpu_blas_refit, self.h, mid)
+        s = capi.BVHStats()
+        self._call(self.lib.vira_gpu_blas_stats, self.h, mid, C.byref(s))
+        return {f: getattr(s, f) for f, _ in capi.BVHStats._fields_}
 
     def di_render(self, camera: Camera, options: DIOptions, radiance=None):
         """Radiance (H, W, 8) of the scene mesh under point lights, bounce 0 only."""
