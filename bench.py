@@ -209,7 +209,7 @@ def adjoint_leg(ctx, steps: int, warmup: int) -> dict:
 def batch_leg(ctx, dem_built, steps: int, warmup: int) -> dict:
     """Mixed batch: 64 requests (image side uniform over {128, 256, 512, 1024,
     2048}, spp uniform over {1, 4, 16, 64}, 3 bounces + RR, one of four meshes:
-    the DEMs of configs 0-2 and a 200k-triangle small body; random camera and
+    the 257^2, 4097^2 and 1025^2 DEMs and a 200k-triangle small body; random camera and
     sun; seed 3).  Times the per-request form (one render call each) and the
     batched form (one persistent graph launch, configuration from the
     autotuner), after checking they write identical passes."""

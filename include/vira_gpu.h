@@ -282,7 +282,7 @@ vira_gpu_status vira_gpu_synth_fbm(vira_gpu_ctx* ctx, int width, int height, uin
 vira_gpu_status vira_gpu_synth_vertex_albedo(vira_gpu_ctx* ctx, size_t num_vertices, uint32_t seed,
                                              float* albedo_dev);
 
-/* --- meshes (a1) ---------------------------------------------------------- */
+/* --- meshes ---------------------------------------------------------------- */
 /* DEM::makeMesh (dem.ipp:138-211) + imageToIndexBuffer (image_utils.ipp:364-412)
  * + Mesh::calculateNormals (mesh.ipp:199-233), on the GPU.  heights_dev is
  * W*H row-major (i + j*W); grey_albedo_dev is W*H or NULL (albedo 1).
@@ -312,7 +312,7 @@ vira_gpu_status vira_gpu_dem_update(vira_gpu_ctx* ctx, int32_t mesh_id, const fl
  * rebuilt: for the small height changes of an optimisation loop. */
 vira_gpu_status vira_gpu_dem_update_refit(vira_gpu_ctx* ctx, int32_t mesh_id, const float* heights_dev);
 
-/* --- acceleration (a2, a3) ------------------------------------------------ */
+/* --- acceleration ---------------------------------------------------------- */
 /* Mesh::buildBVH -> EmbreeBLAS (mesh.ipp:396-418, embree_blas.ipp:14-71):
  * binned-SAH build on the GPU. opts may be NULL. */
 vira_gpu_status vira_gpu_blas_build(vira_gpu_ctx* ctx, int32_t mesh_id, const vira_gpu_bvh_opts* opts);
@@ -342,13 +342,13 @@ typedef struct {
  * takes a single-instance TLAS. */
 vira_gpu_status vira_gpu_tlas_build(vira_gpu_ctx* ctx, const vira_gpu_instance* instances_host, size_t n);
 
-/* --- shading state (a7) --------------------------------------------------- */
+/* --- shading state -------------------------------------------------------- */
 /* Scene::light_cache_ (scene.hpp:263-285). */
 vira_gpu_status vira_gpu_lights_set(vira_gpu_ctx* ctx, const vira_gpu_light* lights_host, int n);
 /* Mesh::material_cache_ (mesh.hpp:93). */
 vira_gpu_status vira_gpu_material_set(vira_gpu_ctx* ctx, int32_t mesh_id, const vira_gpu_material* mat);
 
-/* --- rendering (a4-a9) ---------------------------------------------------- */
+/* --- rendering --------------------------------------------------------- */
 /* CPUPathTracer::render (cpu_path_tracer.ipp:29-161) over the current TLAS. */
 vira_gpu_status vira_gpu_render(vira_gpu_ctx* ctx, const vira_gpu_camera* cam, const vira_gpu_render_opts* opts,
                                 const vira_gpu_outputs* out, vira_gpu_render_stats* stats);

@@ -155,7 +155,7 @@ template <class T> inline V3<T> normalize(V3<T> a) {
     return a * inv;
 }
 
-// ---------------------------------------------------------------- tessellation (a1)
+// ---------------------------------------------------------------- tessellation
 struct Projection {
     int kind;           // 0 = flat grid, 1 = gnomonic drape on a sphere
     double spacing;     // metres per DEM cell
@@ -225,7 +225,7 @@ inline void vertex_normals(const float* pos, size_t nv, const uint32_t* idx, siz
     }
 }
 
-// ---------------------------------------------------------------- watertight test (a5)
+// ---------------------------------------------------------------- watertight test
 template <class R>
 struct RayT {
     V3<R> o, d;
@@ -303,7 +303,7 @@ inline bool wt_intersect(const RayT<R>& ray, const WTRay<R>& w, const V3<R>& v0,
     return true;
 }
 
-// ---------------------------------------------------------------- BVH2 (a2)
+// ---------------------------------------------------------------- BVH2
 // Binned SAH exactly as ViraBLAS (vira_blas.ipp:44-55,132-254,287-292):
 // longest axis of the node box, 8 bins over the centroid bounds on that axis,
 // cost nL*AL + nR*AR, split iff cost < N*SA, leaf when N <= 2, in-place

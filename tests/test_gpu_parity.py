@@ -1,4 +1,4 @@
-"""GPU parity against the CPU oracle, through the C-ABI (rows a1-a10).
+"""GPU parity against the CPU oracle, through the C-ABI.
 
 Bar: bit-exact for integer/index work (triangle ids, tessellation) and for the
 FP32 twin wherever no transcendental function is involved (positions, normals,

@@ -87,7 +87,7 @@ def test_vertex_albedo_range():
     assert a.min() >= 0.10 and a.max() <= 0.14 and abs(a.mean() - 0.12) < 1e-3
 
 
-# ---------------------------------------------------------------- tessellation (a1)
+# ---------------------------------------------------------------- tessellation
 def test_index_buffer_closed_form():
     h = np.zeros((4, 5), np.float32)  # H=4 rows (j), W=5 columns (i)
     idx = O.dem_indices(h)
@@ -136,7 +136,7 @@ def test_sphere_drape_radius():
     assert np.allclose(p64[4 * 9 + 4], [R + 100.0, 0, 0])
 
 
-# ---------------------------------------------------------------- watertight intersection (a5)
+# ---------------------------------------------------------------- watertight intersection
 def _hit(ray, tri):
     t, b1, b2 = np.zeros(1, np.float32), np.zeros(1, np.float32), np.zeros(1, np.float32)
     r = np.ascontiguousarray(ray, np.float32)
@@ -270,7 +270,7 @@ def test_empty_mesh():
     assert i[0] == 0xFFFFFFFF and np.isinf(t[0])
 
 
-# ---------------------------------------------------------------- offset (a6)
+# ---------------------------------------------------------------- offset
 def test_offset_intersection_integer_ulp():
     p = np.array([1.0, -2.0, 0.01], np.float32)
     n = np.array([0.6, -0.8, 0.0], np.float32)
@@ -282,7 +282,7 @@ def test_offset_intersection_integer_ulp():
     assert r[2] == np.float32(0.01) + np.float32(1 / 65536.0) * np.float32(0.0)
 
 
-# ---------------------------------------------------------------- camera (a4)
+# ---------------------------------------------------------------- camera
 def test_intrinsics_closed_form():
     cam = S.Camera(512, 384, 40.0)
     sw, sh = cam.sensor_size()
@@ -303,7 +303,7 @@ def test_look_in_direction_nadir():
     assert np.allclose(R[:, 2], [-1, 0, 0]) and np.allclose(R[:, 1], [0, 0, -1])
 
 
-# ---------------------------------------------------------------- lights (a7)
+# ---------------------------------------------------------------- lights
 def test_planck_bands_against_quadrature():
     from scipy.integrate import quad
     h, c, kb = 6.62607015e-34, 299792458.0, 1.380649e-23
@@ -315,7 +315,7 @@ def test_planck_bands_against_quadrature():
         assert abs(rad[b] / want - 1) < 1e-6
 
 
-# ---------------------------------------------------------------- shading closed form (a7, a9)
+# ---------------------------------------------------------------- shading closed form
 def _flat_scene(n=17, spacing=10.0, alb=0.3):
     h = np.zeros((n, n), np.float32)
     pos, pos64 = O.dem_positions(h, O.make_projection(0, spacing))

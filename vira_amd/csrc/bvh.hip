@@ -1,4 +1,4 @@
-// bvh.hip -- binned-SAH BVH build on the GPU (row a2).
+// bvh.hip -- binned-SAH BVH build on the GPU.
 //
 // Replaces Mesh::buildBVH -> EmbreeBLAS (mesh.ipp:396-418,
 // embree_blas.ipp:14-71) and restates the reference's own binned SAH

@@ -1,4 +1,4 @@
-// dem.hip -- synthetic DEM inputs and mesh construction on the GPU (row a1).
+// dem.hip -- synthetic DEM inputs and mesh construction on the GPU.
 //
 // DEM::makeVertexBuffer (dem.ipp:165-211), imageToIndexBuffer
 // (image_utils.ipp:364-412) and Mesh::calculateNormals (mesh.ipp:199-233)

@@ -1,4 +1,4 @@
-// render.hip -- the path tracer (rows a4-a10) and the trace_rays hook.
+// render.hip -- the path tracer and the trace_rays hook.
 //
 // CPUPathTracer::render / unidirectional / simulatePath / processIntersection
 // (cpu_path_tracer.ipp:29-511) on the GPU, in two schedules that share every

@@ -1,4 +1,4 @@
-// sensor.hip -- received power per pixel (row a9, the last step of the render):
+// sensor.hip -- received power per pixel (the last step of the render):
 // Camera::calculateReceivedPower (camera.ipp:1175-1178)
 //   P_k(i, j) = ((eta_k * A) * Omega_ij) * L_k(i, j)
 // with Omega_ij the pixel's solid angle (Camera::initializePixelSolidAngle,

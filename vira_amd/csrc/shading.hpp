@@ -1,4 +1,4 @@
-// shading.hpp -- surface interaction, light sampling and BSDF sampling (rows a6, a7, a10).
+// shading.hpp -- surface interaction, light sampling and BSDF sampling.
 //
 // CPUPathTracer::processIntersection (cpu_path_tracer.ipp:279-469) split into
 // three steps shared by both integrators (fused per-pixel and wavefront), so

@@ -1,4 +1,4 @@
-// traverse.hpp -- BVH traversal with watertight intersection (row a5, a6).
+// traverse.hpp -- BVH traversal with watertight intersection.
 //
 // Replaces rtcIntersect1 (embree_tlas.ipp:128-173) and follows the order of
 // ViraBLAS::intersect (vira_blas.ipp:58-108): all child boxes of a node are

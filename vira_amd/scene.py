@@ -25,7 +25,7 @@ AU = 1.495978707e11
 BAND_EDGES_NM = np.linspace(400.0, 1000.0, NUM_BINS + 1)
 
 
-# ---------------------------------------------------------------- camera (a4)
+# ---------------------------------------------------------------- camera
 def intrinsics_inverse(width: int, height: int, focal_length: float, sensor_w: float, sensor_h: float,
                        z_dir: float = 1.0) -> np.ndarray:
     """K^-1 as in Camera::initializeIntrinsicMatrix (camera.ipp:1526-1568), FP32.
@@ -130,7 +130,7 @@ class Camera:
         return c
 
 
-# ---------------------------------------------------------------- lights (a7)
+# ---------------------------------------------------------------- lights
 def planck_band_radiance(temperature: float, steps: int = 1000) -> np.ndarray:
     """blackBodyRadiance (spectral_data.ipp:613-629): trapezoid of Planck's law per band."""
     h, c, kb = 6.62607015e-34, 299792458.0, 1.380649e-23

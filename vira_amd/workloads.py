@@ -288,8 +288,8 @@ class BatchMeshes:
 
 
 def build_batch_meshes(ctx: S.Context, dem_built: "BuiltScene | None" = None, bvh_width: int = 4) -> BatchMeshes:
-    """The four meshes of the batched workload: the DEMs of configs 0, 1 and 2
-    (with their albedo) and the 200k-triangle small body (albedo 0.12)."""
+    """The four meshes of the batched workload: the 257^2, 4097^2 and 1025^2
+    DEMs (with their albedo) and the 200k-triangle small body (albedo 0.12)."""
     ids, scenes, keep = [], [], []
     for sc in (DEM257, DEM4097, DEM1025_DI):
         b = dem_built if (sc is DEM4097 and dem_built is not None) else build(ctx, sc, bvh_width=bvh_width)
@@ -347,7 +347,7 @@ def make_batch(ctx: S.Context, meshes: BatchMeshes, n: int = 64, seed: int = 3, 
             off = np.asarray(rebase_offset(sc))
             ground = np.array([sc.sphere_radius, 0.0, 0.0]) - off
             light = S.directional_sun(elev, sun_az, 1.0, origin=ground, up=(1.0, 0.0, 0.0), east=(0.0, 1.0, 0.0))
-        else:  # flat DEMs of configs 0 and 2
+        else:  # the flat 257^2 and 1025^2 DEMs
             ext = (sc.dem - 1) * sc.spacing
             pos = (rng.uniform(-0.15, 0.15) * ext, rng.uniform(-0.15, 0.15) * ext,
                    sc.altitude * rng.uniform(0.7, 1.3))
