@@ -73,10 +73,20 @@ public:
         return id;
     }
     void releaseMesh(int32_t mesh) { check(vira_gpu_mesh_release(h_, mesh)); }
-    void demUpdate(int32_t mesh, const float* heights_dev) { check(vira_gpu_dem_update(h_, mesh, heights_dev)); }
+    // refit = true keeps the BLAS's tree and recomputes its boxes instead of rebuilding it
+    void demUpdate(int32_t mesh, const float* heights_dev, bool refit = false) {
+        check(refit ? vira_gpu_dem_update_refit(h_, mesh, heights_dev) : vira_gpu_dem_update(h_, mesh, heights_dev));
+    }
     // Mesh::buildBVH (mesh.ipp:396-418)
     vira_gpu_bvh_stats buildBLAS(int32_t mesh, vira_gpu_bvh_opts o = {4, 16, 4}) {
         check(vira_gpu_blas_build(h_, mesh, &o));
+        vira_gpu_bvh_stats s{};
+        check(vira_gpu_blas_stats(h_, mesh, &s));
+        return s;
+    }
+    // New: the built BLAS refitted to the mesh's current vertices (tree kept, boxes recomputed)
+    vira_gpu_bvh_stats refitBLAS(int32_t mesh) {
+        check(vira_gpu_blas_refit(h_, mesh));
         vira_gpu_bvh_stats s{};
         check(vira_gpu_blas_stats(h_, mesh, &s));
         return s;
