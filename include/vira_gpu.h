@@ -387,7 +387,9 @@ vira_gpu_status vira_gpu_adjoint(vira_gpu_ctx* ctx, const vira_gpu_camera* cam, 
  * (cpu_path_tracer.ipp:137-146 -> camera.ipp:1175-1178), Omega_ij the pixel's
  * solid angle (camera.ipp:1594-1666, FP64 on the device, stored FP32).
  * radiance_dev / power_dev: W*H*8; solid_angle_dev: W*H; power or solid angle
- * output may be NULL. */
+ * output may be NULL.  The solid angles are computed once per camera geometry
+ * (size, K^-1, z sign) and kept on the context, as the reference computes them
+ * when the camera is initialised. */
 vira_gpu_status vira_gpu_received_power(vira_gpu_ctx* ctx, const vira_gpu_camera* cam, const vira_gpu_sensor* sensor,
                                         const float* radiance_dev, float* power_dev, float* solid_angle_dev);
 

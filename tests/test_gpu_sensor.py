@@ -28,6 +28,20 @@ def test_received_power_and_solid_angles(gpu_ctx):
     A = np.float32(s.aperture_area)
     ref = ((eta * A)[None, None, :] * om_g[:, :, None]) * rad.cpu().numpy()
     assert np.array_equal(P.cpu().numpy().view(np.uint32), ref.astype(np.float32).view(np.uint32))
+    # the solid angles are kept per camera geometry: another field of view, then the first again
+    cam2 = S.Camera(64, 48, 25.0, (0.0, 0.0, 0.0))
+    _, om2 = gpu_ctx.received_power(cam2, rad, solid_angle=True)
+    om2_o = O.pixel_solid_angles(O.camera_from(cam2.to_c()))
+    assert np.abs(om2.cpu().numpy().view(np.int32).astype(np.int64) - om2_o.view(np.int32).astype(np.int64)).max() <= 1
+    P3, om3 = gpu_ctx.received_power(cam, rad, solid_angle=True)
+    assert torch.equal(om3, om) and torch.equal(P3, P)
+    # a radiance image 4 bytes off a 16-byte boundary takes the per-band path, same values
+    buf = torch.empty(48 * 64 * 8 + 1, device="cuda")
+    rad_odd = buf[1:].view(48, 64, 8)
+    rad_odd.copy_(rad)
+    assert rad_odd.data_ptr() % 16 != 0
+    P4, _ = gpu_ctx.received_power(cam, rad_odd)
+    assert torch.equal(P4, P)
 
 
 def _z_scores(g, r):

@@ -140,6 +140,9 @@ struct Ctx {
     DBuf st_rec, st_count, st_start, st_fill, st_list;
     // denoiser work images: two per pass (direct, indirect) in ping-pong
     DBuf dn_work[4];
+    // pixel solid angles of the last camera geometry (width, height, z sign, K^-1)
+    DBuf rp_omega;
+    std::vector<float> rp_key;
     // FFT convolution: twiddle tables, row/column spectra, the kernel's spectrum,
     // the kernel values (on the device) and padded sizes it was computed for,
     // and the device flag saying whether it was recomputed

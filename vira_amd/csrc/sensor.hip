@@ -41,8 +41,10 @@ __device__ __forceinline__ D3 corner(const double k[6], double zdir, double x, d
     return dnormalize(D3{hx, zdir * hy, zdir});
 }
 
-__global__ void k_received_power(int W, int H, CamF cam, vira_gpu_sensor sensor, const float* __restrict__ radiance,
-                                 float* __restrict__ power, float* __restrict__ omega_out) {
+// Pixel solid angles of a camera (Camera::initializePixelSolidAngle): computed
+// once per camera geometry and kept on the context, as the reference keeps
+// them on the camera after initialisation.
+__global__ void k_pixel_solid_angle(int W, int H, CamF cam, float* __restrict__ omega_out) {
     const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= (size_t)W * H) return;
     const int i = (int)(p % (size_t)W), j = (int)(p / (size_t)W);
@@ -51,12 +53,36 @@ __global__ void k_received_power(int W, int H, CamF cam, vira_gpu_sensor sensor,
     const double zd = (double)cam.zdir;
     const D3 c0 = corner(k, zd, i, j), c1 = corner(k, zd, i + 1, j);
     const D3 c2 = corner(k, zd, i + 1, j + 1), c3 = corner(k, zd, i, j + 1);
-    const float omega = (float)(triangle_solid_angle(c0, c1, c2) + triangle_solid_angle(c0, c2, c3));
-    if (omega_out) omega_out[p] = omega;
-    if (power) {
+    omega_out[p] = (float)(triangle_solid_angle(c0, c1, c2) + triangle_solid_angle(c0, c2, c3));
+}
+
+// P = ((eta * A) * Omega) * L per band (camera.ipp:1175-1178), a thread per
+// pixel; VEC: the 8 bands as two float4 loads and stores (16-byte aligned images).
+template <bool VEC>
+__global__ void k_received_power(size_t n, vira_gpu_sensor sensor, const float* __restrict__ omega,
+                                 const float* __restrict__ radiance, float* __restrict__ power) {
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const float om = omega[p];
+    float r[NB];
+    if (VEC) {
+        const float4 r0 = reinterpret_cast<const float4*>(radiance)[2 * p];
+        const float4 r1 = reinterpret_cast<const float4*>(radiance)[2 * p + 1];
+        r[0] = r0.x; r[1] = r0.y; r[2] = r0.z; r[3] = r0.w;
+        r[4] = r1.x; r[5] = r1.y; r[6] = r1.z; r[7] = r1.w;
+    } else {
 #pragma unroll
-        for (int b = 0; b < NB; ++b)
-            power[NB * p + b] = ((sensor.optical_efficiency[b] * sensor.aperture_area) * omega) * radiance[NB * p + b];
+        for (int b = 0; b < NB; ++b) r[b] = radiance[NB * p + b];
+    }
+    float o[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) o[b] = ((sensor.optical_efficiency[b] * sensor.aperture_area) * om) * r[b];
+    if (VEC) {
+        reinterpret_cast<float4*>(power)[2 * p] = make_float4(o[0], o[1], o[2], o[3]);
+        reinterpret_cast<float4*>(power)[2 * p + 1] = make_float4(o[4], o[5], o[6], o[7]);
+    } else {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) power[NB * p + b] = o[b];
     }
 }
 
@@ -666,8 +692,28 @@ void received_power(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_sensor& s
     cf.H = cam.height;
     for (int k = 0; k < 6; ++k) cf.kinv[k] = cam.kinv[k];
     cf.zdir = cam.z_dir;
-    k_received_power<<<blocks_for(n, 256), 256, 0, c.stream>>>(cam.width, cam.height, cf, s, radiance, power, omega);
-    VG_HIP(hipGetLastError());
+    // the solid angles depend only on the size, K^-1 and the z sign: recomputed when those change
+    std::vector<float> key = {(float)cam.width, (float)cam.height, cam.z_dir};
+    key.insert(key.end(), cam.kinv, cam.kinv + 6);
+    if (!c.rp_omega.p || c.rp_key != key) {
+        c.rp_key.clear();
+        c.rp_omega.alloc(n * sizeof(float));
+        k_pixel_solid_angle<<<blocks_for(n, 256), 256, 0, c.stream>>>(cam.width, cam.height, cf,
+                                                                        c.rp_omega.as<float>());
+        VG_HIP(hipGetLastError());
+        c.rp_key = key;
+    }
+    if (omega)
+        VG_HIP(hipMemcpyAsync(omega, c.rp_omega.p, n * sizeof(float), hipMemcpyDeviceToDevice, c.stream));
+    if (power) {
+        if ((((uintptr_t)power | (uintptr_t)radiance) & 15) == 0)
+            k_received_power<true><<<blocks_for(n, 256), 256, 0, c.stream>>>(n, s, c.rp_omega.as<float>(), radiance,
+                                                                             power);
+        else
+            k_received_power<false><<<blocks_for(n, 256), 256, 0, c.stream>>>(n, s, c.rp_omega.as<float>(),
+                                                                              radiance, power);
+        VG_HIP(hipGetLastError());
+    }
     VG_HIP(hipStreamSynchronize(c.stream));
 }
 
