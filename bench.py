@@ -57,6 +57,18 @@ def traffic_bytes(kernel: str, workload: str):
     return int(t["traffic_bytes"])
 
 
+def bvh_build_row(stats: dict, triangles: int) -> dict:
+    """The BVH build against HBM: ~36 B per triangle (its box and id) read per
+    level of the binary tree, over the build's device time."""
+    levels = int(stats["max_depth"])
+    nbytes = 36.0 * triangles * levels
+    a = nbytes / (stats["build_ms"] * 1e-3) / 1e9
+    return {"ms": round(stats["build_ms"], 1), "triangles": triangles, "levels": levels,
+            "roofline": {"bound": "hbm", "achieved": round(a, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(a / HBM_PEAK_GBS, 4), "algorithmic_bytes": nbytes},
+            "sah_cost": round(stats["sah_cost"], 3), "wide_nodes": stats["num_wide_nodes"]}
+
+
 def cpu_baseline(built, ctx, stride: int, threads: int):
     """Times the oracle (FP32 CPU restatement of the reference algorithm) on a
     strided pixel sample of the same frame; returns (dict, traversal counts)."""
@@ -92,6 +104,8 @@ def cpu_baseline(built, ctx, stride: int, threads: int):
                   + f" the same frame x {sc.spp} spp x {sc.bounces} bounces, {ps} primary+shadow rays in {dt:.2f} s "
                   f"(reference-algorithm FP32 CPU restatement, binned-SAH BVH2, std::thread); mesh + BVH2 build "
                   f"{build_s:.1f} s not timed",
+        "bvh_build_ms": om.build_ms,
+        "bvh_build_threads": O.nthreads(),
     }
     counts = {
         "closest_rays": c["primary"] + c["bounce"],
@@ -506,8 +520,15 @@ def main() -> int:
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         cpu, counts = cpu_baseline(built, ctx, args.cpu_stride, threads)
+        cpu_build_ms, cpu_build_threads = cpu.pop("bvh_build_ms"), cpu.pop("bvh_build_threads")
         if args.scene == "dem4097":
             cpu["rows"] = cpu_baseline_rows(built, passes)
+            cpu["rows"]["bvh_build"] = {
+                "value": round(sc.triangles / (cpu_build_ms * 1e-3) / 1e6, 2), "unit": "Mtriangles/s",
+                "cores": cpu_build_threads, "kind": "port",
+                "sample": f"the oracle's binned-SAH BVH2 build (8 bins, leaves of <= 2) of the same "
+                          f"{sc.triangles}-triangle mesh in {cpu_build_ms / 1e3:.2f} s, subtrees below the top "
+                          f"levels on worker threads"}
         if args.counts_out:
             Path(args.counts_out).parent.mkdir(parents=True, exist_ok=True)
             Path(args.counts_out).write_text(json.dumps({"scene": sc.name, **counts}, indent=1) + "\n")
@@ -570,6 +591,7 @@ def main() -> int:
         "batch": batch,
         "adjoint": adjoint,
         "instanced": instanced,
+        "bvh_build": bvh_build_row(built.bvh, sc.triangles),
         "sensor_tail": sensor,
         "quipu": quipu,
     }
