@@ -40,11 +40,12 @@ struct BNodeG {  // binary build node: children of an internal node are adjacent
     uint32_t count;  // 0 = internal
 };
 
-struct PrimBox {
-    float4 lo;  // xyz = box min
-    float4 hi;  // xyz = box max
-    float4 c;   // xyz = box centre
+// A triangle's box (24 B, read at random by the binning passes); its centroid
+// is recomputed where needed as 0.5 (lo + hi), the same value every time.
+struct __attribute__((aligned(8))) PrimBox {
+    float lo[3], hi[3];
 };
+__device__ __forceinline__ float centroid(const PrimBox& b, int a) { return 0.5f * (b.lo[a] + b.hi[a]); }
 
 struct Chunk {
     uint32_t task, begin, end, pad;
@@ -90,9 +91,8 @@ __global__ void k_prim_boxes(size_t nt, const uint32_t* idx, const GVertex* V, P
     const GVertex& b = V[idx[3 * t + 1]];
     const GVertex& c = V[idx[3 * t + 2]];
     PrimBox o;
-    o.lo = make_float4(fminf(fminf(a.px, b.px), c.px), fminf(fminf(a.py, b.py), c.py), fminf(fminf(a.pz, b.pz), c.pz), 0.f);
-    o.hi = make_float4(fmaxf(fmaxf(a.px, b.px), c.px), fmaxf(fmaxf(a.py, b.py), c.py), fmaxf(fmaxf(a.pz, b.pz), c.pz), 0.f);
-    o.c = make_float4(0.5f * (o.lo.x + o.hi.x), 0.5f * (o.lo.y + o.hi.y), 0.5f * (o.lo.z + o.hi.z), 0.f);
+    o.lo[0] = fminf(fminf(a.px, b.px), c.px); o.lo[1] = fminf(fminf(a.py, b.py), c.py); o.lo[2] = fminf(fminf(a.pz, b.pz), c.pz);
+    o.hi[0] = fmaxf(fmaxf(a.px, b.px), c.px); o.hi[1] = fmaxf(fmaxf(a.py, b.py), c.py); o.hi[2] = fmaxf(fmaxf(a.pz, b.pz), c.pz);
     pb[t] = o;
     prims[t] = (uint32_t)t;
 }
@@ -105,8 +105,9 @@ __global__ void __launch_bounds__(256) k_reduce_bounds(size_t n, const PrimBox* 
     int l[12];
     for (int k = 0; k < 12; ++k) l[k] = (k % 6) < 3 ? INT_MAX : INT_MIN;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const PrimBox& p = pb[i];
-        float v[12] = {p.lo.x, p.lo.y, p.lo.z, p.hi.x, p.hi.y, p.hi.z, p.c.x, p.c.y, p.c.z, p.c.x, p.c.y, p.c.z};
+        const PrimBox p = pb[i];
+        const float cx = centroid(p, 0), cy = centroid(p, 1), cz = centroid(p, 2);
+        float v[12] = {p.lo[0], p.lo[1], p.lo[2], p.hi[0], p.hi[1], p.hi[2], cx, cy, cz, cx, cy, cz};
         for (int k = 0; k < 12; ++k) {
             int key = f2key(v[k]);
             l[k] = (k % 6) < 3 ? min(l[k], key) : max(l[k], key);
@@ -146,24 +147,25 @@ __global__ void __launch_bounds__(kChunkThreads) k_bin_large(const Chunk* chunks
     Chunk ch = chunks[blockIdx.x];
     TaskParam tp = params[ch.task];
     for (uint32_t slot = ch.begin + threadIdx.x; slot < ch.end; slot += blockDim.x) {
-        const PrimBox& p = pb[prims[slot]];
-        float cc = tp.axis == 0 ? p.c.x : (tp.axis == 1 ? p.c.y : p.c.z);
+        const PrimBox p = pb[prims[slot]];
+        const float c[3] = {centroid(p, 0), centroid(p, 1), centroid(p, 2)};
+        float cc = c[tp.axis];
         int b = bin_of(cc, tp.cmin, tp.scale, nb);
         slot_bin[slot] = (uint8_t)b;
         int* sb = &s[b * kBinVals];
         atomicAdd(&sb[0], 1);
-        atomicMin(&sb[1], f2key(p.lo.x));
-        atomicMin(&sb[2], f2key(p.lo.y));
-        atomicMin(&sb[3], f2key(p.lo.z));
-        atomicMax(&sb[4], f2key(p.hi.x));
-        atomicMax(&sb[5], f2key(p.hi.y));
-        atomicMax(&sb[6], f2key(p.hi.z));
-        atomicMin(&sb[7], f2key(p.c.x));
-        atomicMin(&sb[8], f2key(p.c.y));
-        atomicMin(&sb[9], f2key(p.c.z));
-        atomicMax(&sb[10], f2key(p.c.x));
-        atomicMax(&sb[11], f2key(p.c.y));
-        atomicMax(&sb[12], f2key(p.c.z));
+        atomicMin(&sb[1], f2key(p.lo[0]));
+        atomicMin(&sb[2], f2key(p.lo[1]));
+        atomicMin(&sb[3], f2key(p.lo[2]));
+        atomicMax(&sb[4], f2key(p.hi[0]));
+        atomicMax(&sb[5], f2key(p.hi[1]));
+        atomicMax(&sb[6], f2key(p.hi[2]));
+        atomicMin(&sb[7], f2key(c[0]));
+        atomicMin(&sb[8], f2key(c[1]));
+        atomicMin(&sb[9], f2key(c[2]));
+        atomicMax(&sb[10], f2key(c[0]));
+        atomicMax(&sb[11], f2key(c[1]));
+        atomicMax(&sb[12], f2key(c[2]));
     }
     __syncthreads();
     if (threadIdx.x < (unsigned)nb) {
@@ -374,9 +376,12 @@ __global__ void k_build_small(const SmallTask* tasks, uint32_t ntasks, uint32_t*
             if (!leaf) {
                 float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
                 for (uint32_t i = 0; i < n; ++i) {
-                    float4 c = pb[prims[e.begin + i]].c;
-                    clo[0] = fminf(clo[0], c.x); clo[1] = fminf(clo[1], c.y); clo[2] = fminf(clo[2], c.z);
-                    chi[0] = fmaxf(chi[0], c.x); chi[1] = fmaxf(chi[1], c.y); chi[2] = fmaxf(chi[2], c.z);
+                    const PrimBox p = pb[prims[e.begin + i]];
+                    for (int a = 0; a < 3; ++a) {
+                        const float c = centroid(p, a);
+                        clo[a] = fminf(clo[a], c);
+                        chi[a] = fmaxf(chi[a], c);
+                    }
                 }
                 float ext[3] = {chi[0] - clo[0], chi[1] - clo[1], chi[2] - clo[2]};
                 axis = (ext[1] > ext[0]) ? 1 : 0;
@@ -393,12 +398,14 @@ __global__ void k_build_small(const SmallTask* tasks, uint32_t ntasks, uint32_t*
                         bhi[b][0] = bhi[b][1] = bhi[b][2] = -INFINITY;
                     }
                     for (uint32_t i = 0; i < n; ++i) {
-                        const PrimBox& p = pb[prims[e.begin + i]];
-                        float cc = axis == 0 ? p.c.x : (axis == 1 ? p.c.y : p.c.z);
+                        const PrimBox p = pb[prims[e.begin + i]];
+                        float cc = centroid(p, axis);
                         int b = bin_of(cc, cmin, scale, kSmallBins);
                         bcnt[b]++;
-                        blo[b][0] = fminf(blo[b][0], p.lo.x); blo[b][1] = fminf(blo[b][1], p.lo.y); blo[b][2] = fminf(blo[b][2], p.lo.z);
-                        bhi[b][0] = fmaxf(bhi[b][0], p.hi.x); bhi[b][1] = fmaxf(bhi[b][1], p.hi.y); bhi[b][2] = fmaxf(bhi[b][2], p.hi.z);
+                        for (int a = 0; a < 3; ++a) {
+                            blo[b][a] = fminf(blo[b][a], p.lo[a]);
+                            bhi[b][a] = fmaxf(bhi[b][a], p.hi[a]);
+                        }
                     }
                     // SAH sweep: right-to-left areas, then left-to-right costs
                     float rarea[kSmallBins];
@@ -438,8 +445,7 @@ __global__ void k_build_small(const SmallTask* tasks, uint32_t ntasks, uint32_t*
                     // in-place partition (vira_blas.ipp:149-159)
                     int64_t i = e.begin, j = (int64_t)e.begin + n - 1;
                     while (i <= j) {
-                        const PrimBox& p = pb[prims[i]];
-                        float cc = axis == 0 ? p.c.x : (axis == 1 ? p.c.y : p.c.z);
+                        float cc = centroid(pb[prims[i]], axis);
                         if (bin_of(cc, cmin, scale, kSmallBins) <= split) {
                             i++;
                         } else {
