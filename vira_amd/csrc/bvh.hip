@@ -46,6 +46,11 @@ struct __attribute__((aligned(8))) PrimBox {
     float lo[3], hi[3];
 };
 __device__ __forceinline__ float centroid(const PrimBox& b, int a) { return 0.5f * (b.lo[a] + b.hi[a]); }
+// the same for a run-time axis, by selection (no indexing into registers)
+__device__ __forceinline__ float centroid_on(const PrimBox& b, int a) {
+    return a == 0 ? centroid(b, 0) : (a == 1 ? centroid(b, 1) : centroid(b, 2));
+}
+__device__ __forceinline__ float pick3(const float v[3], int a) { return a == 0 ? v[0] : (a == 1 ? v[1] : v[2]); }
 
 struct Chunk {
     uint32_t task, begin, end, pad;
@@ -347,7 +352,7 @@ __global__ void k_chunk_bases(const Chunk* __restrict__ chunks, const TaskParam*
 constexpr uint32_t kTaskRoot = 0xffffffffu;
 constexpr int kSmallStack = 12;  // > log2(kSmall) + 1
 
-__global__ void k_build_small(const SmallTask* tasks, uint32_t ntasks, uint32_t* prims, const PrimBox* pb,
+__global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, uint32_t ntasks, uint32_t* prims, const PrimBox* pb,
                               BNodeG* nodes, BNodeG* snodes, uint32_t* used, uint32_t* max_depth, uint32_t* err,
                               int max_leaf, float trav_cost) {
     const uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x;
@@ -385,13 +390,17 @@ __global__ void k_build_small(const SmallTask* tasks, uint32_t ntasks, uint32_t*
                 }
                 float ext[3] = {chi[0] - clo[0], chi[1] - clo[1], chi[2] - clo[2]};
                 axis = (ext[1] > ext[0]) ? 1 : 0;
-                if (ext[2] > ext[axis]) axis = 2;
-                cmin = clo[axis];
-                scale = (float)kSmallBins / ext[axis];
-                bool degenerate = !(ext[axis] > 0.f) || !isfinite(scale) || e.depth >= kMedianDepth;
+                if (ext[2] > pick3(ext, axis)) axis = 2;
+                cmin = pick3(clo, axis);
+                const float eaxis = pick3(ext, axis);
+                scale = (float)kSmallBins / eaxis;
+                bool degenerate = !(eaxis > 0.f) || !isfinite(scale) || e.depth >= kMedianDepth;
                 if (degenerate) {
                     leaf = n <= (uint32_t)max_leaf;
                 } else {
+                    // the bins stay in registers: every bin is updated under a compare with its
+                    // compile-time index instead of being indexed by the run-time bin
+#pragma unroll
                     for (int b = 0; b < kSmallBins; ++b) {
                         bcnt[b] = 0;
                         blo[b][0] = blo[b][1] = blo[b][2] = INFINITY;
@@ -399,12 +408,16 @@ __global__ void k_build_small(const SmallTask* tasks, uint32_t ntasks, uint32_t*
                     }
                     for (uint32_t i = 0; i < n; ++i) {
                         const PrimBox p = pb[prims[e.begin + i]];
-                        float cc = centroid(p, axis);
-                        int b = bin_of(cc, cmin, scale, kSmallBins);
-                        bcnt[b]++;
-                        for (int a = 0; a < 3; ++a) {
-                            blo[b][a] = fminf(blo[b][a], p.lo[a]);
-                            bhi[b][a] = fmaxf(bhi[b][a], p.hi[a]);
+                        const int bi = bin_of(centroid_on(p, axis), cmin, scale, kSmallBins);
+#pragma unroll
+                        for (int b = 0; b < kSmallBins; ++b) {
+                            const bool m = bi == b;
+                            bcnt[b] += m ? 1u : 0u;
+#pragma unroll
+                            for (int a = 0; a < 3; ++a) {
+                                blo[b][a] = m ? fminf(blo[b][a], p.lo[a]) : blo[b][a];
+                                bhi[b][a] = m ? fmaxf(bhi[b][a], p.hi[a]) : bhi[b][a];
+                            }
                         }
                     }
                     // SAH sweep: right-to-left areas, then left-to-right costs
@@ -412,6 +425,7 @@ __global__ void k_build_small(const SmallTask* tasks, uint32_t ntasks, uint32_t*
                     uint32_t rcnt[kSmallBins];
                     float rl[3] = {INFINITY, INFINITY, INFINITY}, rh[3] = {-INFINITY, -INFINITY, -INFINITY};
                     uint32_t rc = 0;
+#pragma unroll
                     for (int b = kSmallBins - 1; b > 0; --b) {
                         rc += bcnt[b];
                         for (int k = 0; k < 3; ++k) { rl[k] = fminf(rl[k], blo[b][k]); rh[k] = fmaxf(rh[k], bhi[b][k]); }
@@ -421,6 +435,7 @@ __global__ void k_build_small(const SmallTask* tasks, uint32_t ntasks, uint32_t*
                     float ll[3] = {INFINITY, INFINITY, INFINITY}, lh[3] = {-INFINITY, -INFINITY, -INFINITY};
                     uint32_t lc = 0;
                     float best = INFINITY;
+#pragma unroll
                     for (int b = 0; b < kSmallBins - 1; ++b) {
                         lc += bcnt[b];
                         for (int k = 0; k < 3; ++k) { ll[k] = fminf(ll[k], blo[b][k]); lh[k] = fmaxf(lh[k], bhi[b][k]); }
@@ -445,7 +460,7 @@ __global__ void k_build_small(const SmallTask* tasks, uint32_t ntasks, uint32_t*
                     // in-place partition (vira_blas.ipp:149-159)
                     int64_t i = e.begin, j = (int64_t)e.begin + n - 1;
                     while (i <= j) {
-                        float cc = centroid(pb[prims[i]], axis);
+                        float cc = centroid_on(pb[prims[i]], axis);
                         if (bin_of(cc, cmin, scale, kSmallBins) <= split) {
                             i++;
                         } else {
@@ -457,9 +472,16 @@ __global__ void k_build_small(const SmallTask* tasks, uint32_t ntasks, uint32_t*
                     }
                     nl = (uint32_t)(i - e.begin);
                     for (int k = 0; k < 3; ++k) { L.lo[k] = INFINITY; L.hi[k] = -INFINITY; R.lo[k] = INFINITY; R.hi[k] = -INFINITY; }
+#pragma unroll
                     for (int b = 0; b < kSmallBins; ++b) {
-                        BNodeG& D = (b <= split) ? L : R;
-                        for (int k = 0; k < 3; ++k) { D.lo[k] = fminf(D.lo[k], blo[b][k]); D.hi[k] = fmaxf(D.hi[k], bhi[b][k]); }
+                        const bool left = b <= split;
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            L.lo[k] = left ? fminf(L.lo[k], blo[b][k]) : L.lo[k];
+                            L.hi[k] = left ? fmaxf(L.hi[k], bhi[b][k]) : L.hi[k];
+                            R.lo[k] = left ? R.lo[k] : fminf(R.lo[k], blo[b][k]);
+                            R.hi[k] = left ? R.hi[k] : fmaxf(R.hi[k], bhi[b][k]);
+                        }
                     }
                 } else {
                     // object median; children keep the parent box (conservative)
