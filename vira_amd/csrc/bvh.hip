@@ -355,6 +355,8 @@ constexpr int kSmallStack = 12;  // > log2(kSmall) + 1
 __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, uint32_t ntasks, uint32_t* prims, const PrimBox* pb,
                               BNodeG* nodes, BNodeG* snodes, uint32_t* used, uint32_t* max_depth, uint32_t* err,
                               int max_leaf, float trav_cost) {
+    // each thread's stack in its own LDS column (not in scratch memory)
+    __shared__ uint4 s_stack[kSmallStack][64];
     const uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t md = 0;
     if (ti < ntasks) {
@@ -363,7 +365,6 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, uint
         struct Ent {
             uint32_t node, begin, count, depth;
         };
-        Ent stack[kSmallStack];
         int sp = 0;
         uint32_t next = 0;  // scratch nodes used
         Ent e{kTaskRoot, task.begin, task.count, task.depth};
@@ -499,13 +500,17 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, uint
                 *slot = nd;
                 const Ent le{c, e.begin, nl, e.depth + 1}, re{c + 1, e.begin + nl, n - nl, e.depth + 1};
                 const bool left_small = nl <= n - nl;
-                if (sp < kSmallStack) stack[sp++] = left_small ? re : le;
+                const Ent& big = left_small ? re : le;
+                if (sp < kSmallStack) s_stack[sp++][threadIdx.x] = make_uint4(big.node, big.begin, big.count, big.depth);
                 else atomicOr(err, 2u);  // cannot happen: the stacked child is never the smaller one
                 e = left_small ? le : re;
                 continue;
             }
             have = sp > 0;
-            if (have) e = stack[--sp];
+            if (have) {
+                const uint4 t = s_stack[--sp][threadIdx.x];
+                e = Ent{t.x, t.y, t.z, t.w};
+            }
         }
         used[ti] = next;
     }
