@@ -960,10 +960,29 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     if (N > kSmall) large.push_back(rt);
     else small_h.push_back(rt);
 
+    // level buffers sized once for the largest level: a binned node holds more than kSmall
+    // triangles, and its chunks number at most count / kChunk + 1
+    const size_t tasks_cap = N / (kSmall + 1) + 1, chunks_cap = N / kChunk + tasks_cap + 1;
     DBuf d_chunks, d_params, d_bins, d_counts, d_bases, d_sweep, d_left, d_lscan;
-    std::vector<Chunk> chunks;
-    std::vector<TaskParam> params;
-    std::vector<SweepOut> hsweep;
+    d_chunks.alloc(chunks_cap * sizeof(Chunk));
+    d_params.alloc(tasks_cap * sizeof(TaskParam));
+    d_bins.alloc(tasks_cap * nb * kBinVals * 4);
+    d_counts.alloc(chunks_cap * nb * 4);
+    d_sweep.alloc(tasks_cap * sizeof(SweepOut));
+    d_bases.alloc(chunks_cap * sizeof(ChunkBase));
+    d_left.alloc(chunks_cap * 4);
+    d_lscan.alloc(chunks_cap * 4);
+    c.bvh_up_chunks.alloc(chunks_cap * sizeof(Chunk));
+    c.bvh_up_params.alloc(tasks_cap * sizeof(TaskParam));
+    c.bvh_down.alloc(tasks_cap * sizeof(SweepOut));
+    struct Event {  // destroyed on every path out of the build
+        hipEvent_t e = nullptr;
+        ~Event() { if (e) (void)hipEventDestroy(e); }
+    } sweep_done;
+    VG_HIP(hipEventCreateWithFlags(&sweep_done.e, hipEventDisableTiming));
+    Chunk* chunks = c.bvh_up_chunks.as<Chunk>();
+    TaskParam* params = c.bvh_up_params.as<TaskParam>();
+    const SweepOut* hsweep = c.bvh_down.as<SweepOut>();
     uint32_t max_depth = 0;
     while (!large.empty()) {
         std::vector<HostTask> next;
@@ -996,63 +1015,56 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
             binned.push_back(t);
         }
         if (!binned.empty()) {
-            chunks.clear();
-            params.clear();
-            for (uint32_t ti = 0; ti < binned.size(); ++ti) {
+            const uint32_t nt = (uint32_t)binned.size();
+            VG_CHECK(nt <= tasks_cap, VIRA_GPU_ERR_HIP, "BVH build: level task overflow");
+            uint32_t nch = 0;
+            for (uint32_t ti = 0; ti < nt; ++ti) {
                 const HostTask& t = binned[ti];
                 float ext[3] = {t.chi[0] - t.clo[0], t.chi[1] - t.clo[1], t.chi[2] - t.clo[2]};
                 int axis = (ext[1] > ext[0]) ? 1 : 0;
                 if (ext[2] > ext[axis]) axis = 2;
-                const uint32_t chunk0 = (uint32_t)chunks.size();
-                for (uint32_t b = t.begin; b < t.begin + t.count; b += kChunk)
-                    chunks.push_back(Chunk{ti, b, std::min(b + kChunk, t.begin + t.count), 0});
-                params.push_back(TaskParam{axis, t.clo[axis], (float)nb / ext[axis], -1, t.begin, chunk0,
-                                           (uint32_t)chunks.size() - chunk0, 0u});
+                const uint32_t chunk0 = nch;
+                for (uint32_t b = t.begin; b < t.begin + t.count; b += kChunk) {
+                    VG_CHECK(nch < chunks_cap, VIRA_GPU_ERR_HIP, "BVH build: level chunk overflow");
+                    chunks[nch++] = Chunk{ti, b, std::min(b + kChunk, t.begin + t.count), 0};
+                }
+                params[ti] = TaskParam{axis, t.clo[axis], (float)nb / ext[axis], -1, t.begin, chunk0, nch - chunk0, 0u};
             }
-            const uint32_t nt = (uint32_t)binned.size();
             size_t nbv = (size_t)nt * nb * kBinVals;
-            d_chunks.alloc(chunks.size() * sizeof(Chunk));
-            d_params.alloc(params.size() * sizeof(TaskParam));
-            d_bins.alloc(nbv * 4);
-            d_counts.alloc(chunks.size() * nb * 4);
-            d_sweep.alloc((size_t)nt * sizeof(SweepOut));
-            d_bases.alloc(chunks.size() * sizeof(ChunkBase));
-            d_left.alloc(chunks.size() * 4);
-            d_lscan.alloc(chunks.size() * 4);
-            VG_HIP(hipMemcpyAsync(d_chunks.p, chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice, c.stream));
-            VG_HIP(hipMemcpyAsync(d_params.p, params.data(), params.size() * sizeof(TaskParam), hipMemcpyHostToDevice, c.stream));
+            VG_HIP(hipMemcpyAsync(d_chunks.p, chunks, nch * sizeof(Chunk), hipMemcpyHostToDevice, c.stream));
+            VG_HIP(hipMemcpyAsync(d_params.p, params, nt * sizeof(TaskParam), hipMemcpyHostToDevice, c.stream));
             k_init_bins<<<blocks_for(nbv, 256), 256, 0, c.stream>>>(d_bins.as<int>(), nbv);
             VG_HIP(hipGetLastError());
-            k_bin_large<<<(unsigned)chunks.size(), kChunkThreads, 0, c.stream>>>(
+            k_bin_large<<<nch, kChunkThreads, 0, c.stream>>>(
                 d_chunks.as<Chunk>(), d_params.as<TaskParam>(), nb, prims.as<uint32_t>(), pbuf.as<PrimBox>(),
                 d_bins.as<int>(), d_counts.as<uint32_t>(), slot_bin.as<uint8_t>());
             VG_HIP(hipGetLastError());
             k_sweep_large<<<blocks_for(nt, 64), 64, 0, c.stream>>>(d_bins.as<int>(), nb, nt, d_params.as<TaskParam>(),
                                                                    d_sweep.as<SweepOut>());
             VG_HIP(hipGetLastError());
-            const uint32_t nch = (uint32_t)chunks.size();
+            // the decisions go down now; the host builds the next level while the GPU partitions
+            VG_HIP(hipMemcpyAsync(c.bvh_down.p, d_sweep.p, (size_t)nt * sizeof(SweepOut), hipMemcpyDeviceToHost,
+                                  c.stream));
+            VG_HIP(hipEventRecord(sweep_done.e, c.stream));
             k_chunk_left<<<blocks_for(nch, 256), 256, 0, c.stream>>>(d_chunks.as<Chunk>(), d_params.as<TaskParam>(),
                                                                      d_counts.as<uint32_t>(), nb, nch,
                                                                      d_left.as<uint32_t>());
             VG_HIP(hipGetLastError());
-            scan_exclusive_u32(c, d_left.as<uint32_t>(), d_lscan.as<uint32_t>(), nch);
+            scan_exclusive_u32(c, d_left.as<uint32_t>(), d_lscan.as<uint32_t>(), nch, false);
             k_chunk_bases<<<blocks_for(nch, 256), 256, 0, c.stream>>>(d_chunks.as<Chunk>(), d_params.as<TaskParam>(),
                                                                       d_sweep.as<SweepOut>(), d_lscan.as<uint32_t>(),
                                                                       nch, d_bases.as<ChunkBase>());
             VG_HIP(hipGetLastError());
             // a node without a valid split leaves its triangles where they are (split -1: all go right at
             // offset nleft = 0, in order); the build is rejected below before they are used
-            k_partition_large<<<(unsigned)chunks.size(), kChunkThreads, 0, c.stream>>>(
+            k_partition_large<<<nch, kChunkThreads, 0, c.stream>>>(
                 d_chunks.as<Chunk>(), d_params.as<TaskParam>(), d_bases.as<ChunkBase>(), prims.as<uint32_t>(),
                 slot_bin.as<uint8_t>(), tmp.as<uint32_t>());
             VG_HIP(hipGetLastError());
-            k_copy_back<<<(unsigned)chunks.size(), 256, 0, c.stream>>>(d_chunks.as<Chunk>(), tmp.as<uint32_t>(),
+            k_copy_back<<<nch, 256, 0, c.stream>>>(d_chunks.as<Chunk>(), tmp.as<uint32_t>(),
                                                                        prims.as<uint32_t>());
             VG_HIP(hipGetLastError());
-            hsweep.resize(nt);
-            VG_HIP(hipMemcpyAsync(hsweep.data(), d_sweep.p, (size_t)nt * sizeof(SweepOut), hipMemcpyDeviceToHost,
-                                  c.stream));
-            VG_HIP(hipStreamSynchronize(c.stream));
+            VG_HIP(hipEventSynchronize(sweep_done.e));
             // children of each binned node from its sweep
             for (uint32_t ti = 0; ti < nt; ++ti) {
                 const HostTask& t = binned[ti];

@@ -80,6 +80,35 @@ struct DBuf {
     template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
+// Owning page-locked host buffer (hipHostMalloc), for transfers that must not
+// stage through pageable memory.
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() { release(); }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t n) {
+        if (n <= bytes && p) return;
+        release();
+        if (n == 0) return;
+        hipError_t e = hipHostMalloc(&p, n, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw Error(e == hipErrorOutOfMemory ? VIRA_GPU_ERR_OOM : VIRA_GPU_ERR_HIP,
+                        "hipHostMalloc(" + std::to_string(n) + " B): " + hipGetErrorString(e));
+        }
+        bytes = n;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
 struct Blas {
     bool built = false;
     int width = 2;
@@ -140,6 +169,10 @@ struct Ctx {
     DBuf st_rec, st_count, st_start, st_fill, st_list;
     // denoiser work images: two per pass (direct, indirect) in ping-pong
     DBuf dn_work[4];
+    // scan scratch (block sums) and its pinned total; BVH build staging: the
+    // level's chunks and node parameters (up), the nodes' SAH decisions (down)
+    DBuf scan_sums;
+    PinnedBuf scan_total, bvh_up_chunks, bvh_up_params, bvh_down;
     // pixel solid angles of the last camera geometry (width, height, z sign, K^-1)
     DBuf rp_omega;
     std::vector<float> rp_key;
@@ -211,7 +244,8 @@ void trace_rays(Ctx& c, MeshG& m, const float* rays, size_t n, int any_hit, floa
 void render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, const vira_gpu_outputs& out,
             vira_gpu_render_stats* st);
 
-// Exclusive scan of n uint32 values (in place allowed); returns the total.
-uint32_t scan_exclusive_u32(Ctx& c, const uint32_t* in, uint32_t* out, size_t n);
+// Exclusive scan of n uint32 values (in place allowed); returns the total after
+// a stream synchronisation, or 0 without waiting when total_wanted is false.
+uint32_t scan_exclusive_u32(Ctx& c, const uint32_t* in, uint32_t* out, size_t n, bool total_wanted = true);
 
 }  // namespace vg

@@ -86,12 +86,12 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_apply(const uint32_t* in, u
 
 }  // namespace
 
-uint32_t scan_exclusive_u32(Ctx& c, const uint32_t* in, uint32_t* out, size_t n) {
+uint32_t scan_exclusive_u32(Ctx& c, const uint32_t* in, uint32_t* out, size_t n, bool total_wanted) {
     if (n == 0) return 0;
     size_t nb = (n + kScanTile - 1) / kScanTile;
-    DBuf sums;
-    sums.alloc((nb + 1) * sizeof(uint32_t));
-    uint32_t* s = sums.as<uint32_t>();
+    c.scan_sums.alloc((nb + 1) * sizeof(uint32_t));
+    c.scan_total.alloc(sizeof(uint32_t));
+    uint32_t* s = c.scan_sums.as<uint32_t>();
     // k_scan_reduce sums each tile's elements in a strided order and
     // k_scan_apply scans them contiguously per thread: both cover the same
     // kScanTile elements of the tile, so the block totals agree.
@@ -101,10 +101,11 @@ uint32_t scan_exclusive_u32(Ctx& c, const uint32_t* in, uint32_t* out, size_t n)
     VG_HIP(hipGetLastError());
     k_scan_apply<<<(unsigned)nb, kScanBlock, 0, c.stream>>>(in, out, n, s);
     VG_HIP(hipGetLastError());
-    uint32_t total = 0;
-    VG_HIP(hipMemcpyAsync(&total, s + nb, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+    if (!total_wanted) return 0;  // the scan stays queued on the stream
+    uint32_t* total = c.scan_total.as<uint32_t>();
+    VG_HIP(hipMemcpyAsync(total, s + nb, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
     VG_HIP(hipStreamSynchronize(c.stream));
-    return total;
+    return *total;
 }
 
 }  // namespace vg
