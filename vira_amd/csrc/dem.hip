@@ -147,10 +147,14 @@ __device__ __forceinline__ void finish_normal(GVertex& out, f3 n) {
 
 // Grid normals: the (up to) six triangles around vertex (x, y) in increasing
 // triangle-id order: B(x-1,y-1), A(x-1,y), B(x-1,y), A(x,y-1), B(x,y-1), A(x,y).
+// Threads walk the vertices column by column (y fastest), the order of the
+// quad table and of the triangle ids, so neighbouring threads read
+// neighbouring entries of both.
 __global__ void k_dem_normals(int W, int H, const uint32_t* cand_to_tri, const uint32_t* idx, GVertex* verts) {
-    size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= (size_t)W * H) return;
-    int x = (int)(v % W), y = (int)(v / W);
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)W * H) return;
+    const int y = (int)(t % H), x = (int)(t / H);
+    const size_t v = (size_t)y * W + x;
     const int dqx[6] = {-1, -1, -1, 0, 0, 0};
     const int dqy[6] = {-1, 0, 0, -1, -1, 0};
     const int ab[6] = {1, 0, 1, 0, 1, 0};
