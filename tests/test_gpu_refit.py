@@ -137,3 +137,27 @@ def test_refit_requires_built_blas(gpu_ctx):
     # a DEM update on a mesh without a BLAS only moves the vertices
     gpu_ctx.dem_update(mesh, (h + 1.0).contiguous(), refit=True)
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("side", [2, 3, 17])
+def test_refit_tiny_dems(gpu_ctx, side):
+    """A 2x2 DEM (two triangles, one leaf under the root), 3x3 and 17x17: the
+    refit hits equal a rebuild's after the heights change."""
+    import torch
+    proj = S.projection(0, 30.0)
+    h = torch.linspace(0.0, 50.0, side * side, device="cuda").reshape(side, side).contiguous()
+    mesh = gpu_ctx.dem_mesh(h, proj, None)
+    gpu_ctx.blas_build(mesh)
+    ext = 0.5 * (side - 1) * 30.0
+    rng = np.random.default_rng(side)
+    n = 4000
+    o = np.stack([rng.uniform(-ext, ext, n), rng.uniform(-ext, ext, n), np.full(n, 500.0)], 1)
+    d = np.stack([rng.uniform(-0.1, 0.1, n), rng.uniform(-0.1, 0.1, n), -np.ones(n)], 1)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([o, np.zeros((n, 1)), d, np.full((n, 1), np.inf)], 1).astype(np.float32)
+    h2 = (h * -1.5 + 120.0).contiguous()
+    gpu_ctx.dem_update(mesh, h2, refit=True)
+    refit = _trace(gpu_ctx, mesh, rays)
+    gpu_ctx.blas_build(mesh)
+    _same_hits(gpu_ctx, mesh, rays, refit)
+    assert (refit[1] != 0xFFFFFFFF).any()
