@@ -427,6 +427,33 @@ vira_gpu_status vira_gpu_render_stars(vira_gpu_ctx* ctx, const vira_gpu_camera* 
                                       const vira_gpu_star_psf* psf, const float* directions_dev,
                                       const float* irradiance_dev, size_t n, float* power_dev);
 
+/* Star (star.hpp:10-40) as a star quipu stores it (StarQuipu::write,
+ * star_quipu.ipp:31-41: the raw 64-byte record): the initialised flag and
+ * padding, catalogue right ascension and declination (radians), proper
+ * motions (radians per year), V magnitude, temperature (K) and angular size
+ * (sr). */
+typedef struct {
+    uint8_t initialized;
+    uint8_t pad[7];
+    double ra, de, pm_ra, pm_de, vmag, temperature, omega;
+} vira_gpu_star;
+
+/* StarCatalogue::makeStarLight (star_catalogue.ipp:55-78) for n stars in
+ * device memory: the unit vector at ephemeris time et, seconds
+ * (Star::getUnitVector, star.ipp:19-32, FP32 from FP64 angles), and the
+ * irradiance omega * blackBodyRadiance(T, 100) per band (Star::getIrradiance,
+ * star.ipp:13-16; spectral_data.ipp:613-629: a 100-point trapezoid of Planck's
+ * law in FP64 over each band).  As in the reference, pm_ra (rad/yr) moves the
+ * declination and pm_de the right ascension.  stars_dev 8-byte aligned;
+ * directions_dev n*3, irradiance_dev n*8: the inputs of vira_gpu_render_stars. */
+vira_gpu_status vira_gpu_star_light(vira_gpu_ctx* ctx, const vira_gpu_star* stars_dev, size_t n, double et,
+                                    float* directions_dev, float* irradiance_dev);
+/* StarQuipu::read (star_quipu.ipp:20-29): the stars of a .qsc file into host
+ * memory.  With stars_host NULL only *count is set; otherwise up to capacity
+ * records are copied and *count is the number in the file.  Host only: no
+ * context or GPU is touched. */
+vira_gpu_status vira_gpu_qsc_read(const char* path, vira_gpu_star* stars_host, size_t capacity, size_t* count);
+
 /* denoiseSpectralRadianceEATWT (cpu_denoise.ipp:10-187), as
  * CPUPathTracer::render applies it when options.denoise is set
  * (cpu_path_tracer.ipp:124-134): direct_dev and indirect_dev (W*H*8) are

@@ -187,6 +187,22 @@ private:
     Context* ctx_;
 };
 
+// StarQuipu::read (star_quipu.ipp:20-29): a .qsc star catalogue into host records.
+inline std::vector<vira_gpu_star> readStarQuipu(const std::string& path) {
+    size_t n = 0;
+    check(vira_gpu_qsc_read(path.c_str(), nullptr, 0, &n));
+    std::vector<vira_gpu_star> stars(n);
+    if (n) check(vira_gpu_qsc_read(path.c_str(), stars.data(), n, &n));
+    return stars;
+}
+
+// StarCatalogue::makeStarLight (star_catalogue.ipp:55-68): n device records to
+// unit directions (3 floats each) and band irradiance (8 floats each) at et.
+inline void makeStarLight(Context& ctx, const vira_gpu_star* stars_dev, size_t n, double et, float* directions_dev,
+                          float* irradiance_dev) {
+    check(vira_gpu_star_light(ctx.get(), stars_dev, n, et, directions_dev, irradiance_dev));
+}
+
 // DEMQuipu (dem_quipu.hpp:34-81): a .qld DEM pyramid.  readBuffers builds the
 // chosen level's mesh on the GPU and keeps it while the choice does not
 // change (dem_quipu.ipp:131-163); the mesh belongs to the context.

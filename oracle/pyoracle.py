@@ -376,6 +376,58 @@ def denoise(direct, indirect, albedo, depth, normal, levels=(5, 5), depth_thresh
     return d, i
 
 
+SECONDS_PER_YEAR = 365.25 * 86400.0   # math.hpp:53
+
+
+def band_edges_m() -> np.ndarray:
+    """The 8 bands' [min, max] wavelengths in metres as the spectrum holds them:
+    FP32 nm bounds 400 + 75 i scaled by 1e-9f (spectral_data.ipp:64-125), read
+    back as FP64 by blackBodyRadiance (spectral_data.ipp:617-619)."""
+    nm = np.float32(400.0) + np.arange(NB + 1, dtype=np.float32) * np.float32(75.0)
+    return (nm * np.float32(1e-9)).astype(np.float64)
+
+
+def planck(temperature, lam):
+    """plancksLaw (math.ipp:45-59), FP64, spectral radiance per metre."""
+    c, h, kB = 299792458.0, 6.62607015e-34, 1.380649e-23
+    coeff = 2 * h * c * c / lam ** 5.0
+    return coeff * 1.0 / (np.exp(h * c / (lam * kB * temperature)) - 1.0)
+
+
+def black_body_bands(temperature, steps: int = 100) -> np.ndarray:
+    """blackBodyRadiance (spectral_data.ipp:613-629) for an array of temperatures:
+    per band, Planck at linspace(min, max, steps) (math.ipp:33-42) integrated by
+    trapezoids summed in order (math.ipp:131-172), rounded to FP32.  (n, 8)."""
+    T = np.atleast_1d(np.asarray(temperature, np.float64))[:, None]
+    e = band_edges_m()
+    out = np.zeros((T.shape[0], NB), np.float32)
+    for k in range(NB):
+        lo, hi = e[k], e[k + 1]
+        step = (hi - lo) / float(steps - 1)
+        x = lo + np.arange(steps, dtype=np.float64) * step
+        y = planck(T, x[None, :])
+        area = np.zeros(T.shape[0])
+        for i in range(steps - 1):
+            area = area + (x[i + 1] - x[i]) * (y[:, i] + y[:, i + 1]) / 2.0
+        out[:, k] = area.astype(np.float32)
+    return out
+
+
+def star_light(stars, et: float):
+    """StarCatalogue::makeStarLight (star_catalogue.ipp:55-68) over STAR_DTYPE
+    records: unit vectors per Star::getUnitVector (star.ipp:19-32; FP32, the
+    right-ascension rate added to the declination and the declination rate to
+    the right ascension, as the reference does) and irradiance per
+    Star::getIrradiance (star.ipp:13-16; (float)omega times the FP32 band
+    radiance).  Returns (directions (n, 3), irradiance (n, 8)), float32."""
+    dt = float(et) / SECONDS_PER_YEAR
+    delta = (stars["de"] + stars["pm_ra"] * dt).astype(np.float32)
+    alpha = (stars["ra"] + stars["pm_de"] * dt).astype(np.float32)
+    d = np.stack([np.cos(delta) * np.cos(alpha), np.cos(delta) * np.sin(alpha), np.sin(delta)], axis=1)
+    irr = stars["omega"].astype(np.float32)[:, None] * black_body_bands(stars["temperature"], 100)
+    return d.astype(np.float32), irr.astype(np.float32)
+
+
 def render_stars(rot, width: int, height: int, eta, area: float, kernels, minimum_power: float, intrinsic,
                  directions, irradiance, cull_behind: bool = False, img=None) -> np.ndarray:
     """Unresolved point sources: rot (3x3, columns = camera axes), kernels a list of

@@ -137,3 +137,63 @@ def test_star_field_full_size(gpu_ctx):
     want = _oracle(gpu_ctx, cam, sp, [k.cpu().numpy() for k in sp._keep], dirs, irr)
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
     assert (got[:, :, 0] > 0).sum() > 1000
+
+
+# ---- StarCatalogue::makeStarLight on the device (k_star_light)
+# Bar: directions within 4e-7 (a few FP32 ulps: the device's cosf/sinf and
+# numpy's differ by an ulp), irradiance within 1e-6 relative (FP64 pow/exp of
+# the device library against the host's, rounded to FP32).
+
+def _star_light(ctx, cat, et):
+    import torch
+    t = torch.from_numpy(cat.view(np.uint8).reshape(-1, 64).copy()).cuda()
+    d, e = ctx.star_light(t, et)
+    return d.cpu().numpy(), e.cpu().numpy()
+
+
+@pytest.mark.parametrize("n,et", [(1, 0.0), (129, 0.0), (1000, 3.2e8), (1000, -7.5e8)])
+def test_star_light_matches_oracle(gpu_ctx, n, et):
+    from vira_amd import quipu as Q
+    rng = np.random.default_rng(n)
+    cat = Q.make_stars(rng.uniform(0, 2 * np.pi, n), rng.uniform(-1.5, 1.5, n), rng.uniform(-1, 12, n),
+                       rng.uniform(2000, 45000, n), 10 ** rng.uniform(-18, -10, n),
+                       pm_ra=rng.normal(0, 1e-7, n), pm_de=rng.normal(0, 1e-7, n))
+    d, e = _star_light(gpu_ctx, cat, et)
+    dw, ew = O.star_light(cat, et)
+    assert d.shape == (n, 3) and e.shape == (n, 8)
+    np.testing.assert_allclose(d, dw, rtol=0, atol=4e-7)
+    np.testing.assert_allclose(e, ew, rtol=1e-6, atol=0)
+
+
+def test_star_light_closed_forms_empty_and_invalid(gpu_ctx):
+    import torch
+    from vira_amd import quipu as Q
+    cat = Q.make_stars([0.0, np.pi / 2, 1.0], [0.0, 0.0, np.pi / 2], 1.0, 5772.0, 1e-10)
+    d, _ = _star_light(gpu_ctx, cat, 0.0)
+    np.testing.assert_allclose(d, [[1, 0, 0], [0, 1, 0], [0, 0, 1]], atol=1e-7)
+    d, e = gpu_ctx.star_light(torch.zeros((0, 64), dtype=torch.uint8, device="cuda"))
+    assert d.shape == (0, 3) and e.shape == (0, 8)
+    with pytest.raises(ValueError):
+        gpu_ctx.star_light(torch.zeros((4, 63), dtype=torch.uint8, device="cuda"))
+
+
+def test_star_catalogue_full_size_into_render_stars(gpu_ctx, tmp_path):
+    """The 100,000-star catalogue: written as a .qsc, read back through the
+    C-ABI, lit on the device, then rendered; the lit sources match the oracle
+    to the bar above and the rendered image is the oracle's for those sources."""
+    import torch
+    from vira_amd import quipu as Q
+    cat = W.make_star_catalogue()
+    back = Q.read_star_quipu(Q.write_star_quipu(tmp_path / "field", cat))
+    assert back.tobytes() == cat.tobytes()
+    et = 6.3e8
+    d, e = _star_light(gpu_ctx, back, et)
+    dw, ew = O.star_light(cat, et)
+    np.testing.assert_allclose(d, dw, rtol=0, atol=4e-7)
+    np.testing.assert_allclose(e, ew, rtol=1e-6, atol=0)
+    cam, psf = W.star_field_camera()
+    sp = gpu_ctx.star_psf(psf, cam)
+    got = _run(gpu_ctx, cam, sp, d, e)
+    want = _oracle(gpu_ctx, cam, sp, [k.cpu().numpy() for k in sp._keep], d, e)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert (got[:, :, 0] > 0).sum() > 1000

@@ -175,3 +175,85 @@ def test_intrinsic_matrix_inverts_kinv():
     Ki = np.array([[kinv[0], kinv[2], kinv[4]], [kinv[1], kinv[3], kinv[5]], [0, 0, 1]])
     assert np.allclose(Km @ Ki, np.eye(3), atol=1e-3)  # FP32 entries up to 320 px
     assert abs(K[0] - cam.focal_length * cam.width / sw) <= 1e-4 * K[0]
+
+
+# ---- star catalogue light (StarCatalogue::makeStarLight)
+
+def _band_integral_series(T, lo, hi, terms=400):
+    """Planck's law integrated over [lo, hi] metres in closed form: with
+    x = hc / (lambda kB T), the integral is 2 kB^4 T^4 / (h^3 c^2) times
+    G(x(hi)) - G(x(lo)), G(a) = sum_n e^(-n a) (a^3/n + 3a^2/n^2 + 6a/n^3 + 6/n^4)
+    (the blackbody fractional-emission series)."""
+    c, h, kB = 299792458.0, 6.62607015e-34, 1.380649e-23
+
+    def G(a):
+        n = np.arange(1, terms + 1, dtype=np.float64)
+        return float(np.sum(np.exp(-n * a) * (a ** 3 / n + 3 * a ** 2 / n ** 2 + 6 * a / n ** 3 + 6 / n ** 4)))
+
+    x = lambda lam: h * c / (lam * kB * T)
+    return 2 * kB ** 4 * T ** 4 / (h ** 3 * c ** 2) * (G(x(hi)) - G(x(lo)))
+
+
+def test_series_matches_stefan_boltzmann_at_the_limits():
+    # G(0) = pi^4 / 15: the whole spectrum gives sigma T^4 / pi
+    T = 5772.0
+    sigma = 5.670374419e-8
+    whole = _band_integral_series(T, 1e-12, 1e3, terms=4000)
+    assert whole == pytest.approx(sigma * T ** 4 / np.pi, rel=1e-6)
+
+
+@pytest.mark.parametrize("T", [2500.0, 5772.0, 9600.0, 30000.0])
+def test_black_body_bands_match_closed_form_band_integrals(T):
+    bands = O.black_body_bands(T)[0]
+    fine = O.black_body_bands(T, steps=1000)[0]
+    e = O.band_edges_m()
+    for k in range(O.NB):
+        exact = _band_integral_series(T, e[k], e[k + 1])
+        # trapezoid error (b - a) h^2 f''/12: below 1e-4 at 100 points over a
+        # 75 nm band (the cool star's Wien tail is the steepest), and O(h^2), so
+        # 1000 points land within FP32 rounding of the closed form
+        assert bands[k] == pytest.approx(exact, rel=1e-4), (T, k)
+        assert fine[k] == pytest.approx(exact, rel=1e-6), (T, k)
+
+
+def test_band_edges_are_the_fp32_spectrum_bounds():
+    e = O.band_edges_m()
+    assert e.dtype == np.float64 and e.shape == (O.NB + 1,)
+    nm = np.array([400, 475, 550, 625, 700, 775, 850, 925, 1000], np.float32)
+    assert np.array_equal(e, (nm * np.float32(1e-9)).astype(np.float64))
+    assert e[0] != 400e-9   # FP32 edges, widened: not the FP64 decimal
+
+
+def test_star_directions_closed_forms_and_proper_motion():
+    from vira_amd import quipu as Q
+    st = Q.make_stars([0.0, np.pi / 2, 1.0, 0.3], [0.0, 0.0, np.pi / 2, -0.4], 1.0, 5772.0, 1e-10)
+    d, _ = O.star_light(st, 0.0)
+    np.testing.assert_allclose(d[0], [1, 0, 0], atol=1e-7)
+    np.testing.assert_allclose(d[1], [0, 1, 0], atol=1e-7)
+    np.testing.assert_allclose(d[2], [0, 0, 1], atol=1e-7)
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-6)
+    # RA 0.3, DE -0.4: z = sin(DE), atan2(y, x) = RA
+    assert d[3, 2] == pytest.approx(np.sin(-0.4), abs=1e-7)
+    assert np.arctan2(d[3, 1], d[3, 0]) == pytest.approx(0.3, abs=1e-6)
+    # proper motion in rad/yr over et seconds: the reference adds the RA rate to
+    # the declination and the DE rate to the right ascension (star.ipp:27-28)
+    year = 365.25 * 86400.0
+    moving = Q.make_stars([0.2, 0.2], [0.1, 0.1], 1.0, 5772.0, 1e-10, pm_ra=[0.05, 0.0], pm_de=[0.0, 0.05])
+    d2, _ = O.star_light(moving, 2 * year)
+    d_de, _ = O.star_light(Q.make_stars(0.2, 0.1 + 0.1, 1.0, 5772.0, 1e-10), 0.0)
+    d_ra, _ = O.star_light(Q.make_stars(0.2 + 0.1, 0.1, 1.0, 5772.0, 1e-10), 0.0)
+    np.testing.assert_array_equal(d2[0], d_de[0])
+    np.testing.assert_array_equal(d2[1], d_ra[0])
+
+
+def test_star_irradiance_is_fp32_omega_times_band_radiance():
+    from vira_amd import quipu as Q
+    om = np.array([1e-12, 3.3e-9, 2.0])
+    st = Q.make_stars([0.0] * 3, [0.0] * 3, 1.0, [3000.0, 6000.0, 12000.0], om)
+    _, irr = O.star_light(st, 0.0)
+    assert irr.dtype == np.float32
+    for i in range(3):
+        expect = np.float32(om[i]) * O.black_body_bands(st["temperature"][i])[0]
+        np.testing.assert_array_equal(irr[i], expect)
+    # hotter stars peak bluer: 12000 K brightest in the first band, 3000 K in the last
+    assert np.argmax(irr[2]) == 0 and np.argmax(irr[0]) == O.NB - 1

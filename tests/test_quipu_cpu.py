@@ -194,3 +194,40 @@ def test_fixed_point_codec_is_the_reference_formula():
     step = (mm[1] - mm[0]) / 65534
     fin = np.isfinite(h)
     assert np.all(np.abs(back[fin] - h[fin] - step) <= step)
+
+
+# ---- star quipu (.qsc): StarQuipu::write / StarQuipu::read
+
+def test_star_quipu_round_trip_through_the_c_abi(tmp_path):
+    from vira_amd import quipu as Q
+    rng = np.random.default_rng(5)
+    n = 1000
+    st = Q.make_stars(rng.uniform(0, 2 * np.pi, n), rng.uniform(-1.5, 1.5, n), rng.uniform(-1, 12, n),
+                      rng.uniform(2500, 40000, n), 10 ** rng.uniform(-18, -14, n),
+                      pm_ra=rng.normal(0, 1e-8, n), pm_de=rng.normal(0, 1e-8, n))
+    p = Q.write_star_quipu(tmp_path / "cat.bin", st)
+    assert p.endswith(".qsc")
+    raw = open(p, "rb").read()
+    assert raw[:5] == b"QUIPU" and len(raw) == 5 + 8 + 64 * n
+    assert np.frombuffer(raw[5:13], "<u4").tolist() == [n, 64 * n]
+    back = Q.read_star_quipu(p)
+    assert back.dtype == Q.STAR_DTYPE and back.tobytes() == st.tobytes()
+
+
+def test_star_quipu_empty_and_malformed(tmp_path):
+    from vira_amd import quipu as Q
+    from vira_amd import _capi as capi
+    p = Q.write_star_quipu(tmp_path / "empty", Q.make_stars([], [], [], [], []))
+    assert Q.read_star_quipu(p).shape == (0,)
+    bad = tmp_path / "bad.qsc"
+    bad.write_bytes(b"QUIPX" + np.array([1, 64], "<u4").tobytes() + bytes(64))
+    with pytest.raises(capi.ViraGPUError, match="not a valid Quipu"):
+        Q.read_star_quipu(bad)
+    bad.write_bytes(b"QUIPU" + np.array([2, 64], "<u4").tobytes() + bytes(128))
+    with pytest.raises(capi.ViraGPUError, match="does not hold"):
+        Q.read_star_quipu(bad)
+    bad.write_bytes(b"QUIPU" + np.array([2, 128], "<u4").tobytes() + bytes(100))
+    with pytest.raises(capi.ViraGPUError, match="truncated"):
+        Q.read_star_quipu(bad)
+    with pytest.raises(capi.ViraGPUError, match="cannot open"):
+        Q.read_star_quipu(tmp_path / "missing.qsc")

@@ -223,6 +223,15 @@ class StarPSF(C.Structure):
                 ("intrinsic", C.c_float * 6), ("cull_behind", C.c_int32)]
 
 
+class Star(C.Structure):
+    """One star quipu record: the reference's Star object as it lies in memory
+    (star.hpp:27-38): initialized flag, padding, then RA, DE, the two proper
+    motions (rad/yr), V magnitude, temperature (K) and solid angle (sr)."""
+    _fields_ = [("initialized", C.c_uint8), ("pad", C.c_uint8 * 7), ("ra", C.c_double), ("de", C.c_double),
+                ("pm_ra", C.c_double), ("pm_de", C.c_double), ("vmag", C.c_double),
+                ("temperature", C.c_double), ("omega", C.c_double)]
+
+
 class QldInfo(C.Structure):
     _fields_ = [("levels", C.c_int32), ("has_albedos", C.c_int32), ("transformation", C.c_double * 16),
                 ("normal", C.c_float * 3), ("cone_angle", C.c_float)]
@@ -286,6 +295,8 @@ SIGNATURES = [
     ("vira_gpu_denoise", _S, [_P, C.POINTER(DenoiseOpts), C.c_int, C.c_int, _P, _P, _P, _P, _P, _P]),
     ("vira_gpu_render_stars", _S, [_P, C.POINTER(Camera), C.POINTER(Sensor), C.POINTER(StarPSF), _P, _P,
                                    C.c_size_t, _P]),
+    ("vira_gpu_star_light", _S, [_P, _P, C.c_size_t, C.c_double, _P, _P]),
+    ("vira_gpu_qsc_read", _S, [C.c_char_p, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     ("vira_gpu_batch_submit", _S, [_P, C.POINTER(Request), C.c_size_t, C.POINTER(BatchConfig),
                                    C.POINTER(BatchStats)]),
     ("vira_gpu_batch_autotune", _S, [_P, C.POINTER(Request), C.c_size_t, C.c_int, C.POINTER(BatchConfig),

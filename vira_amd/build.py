@@ -38,7 +38,7 @@ FLAGS = [
     f"-I{ROOT / 'include'}",
 ]
 SOURCES = ["capi.hip", "dem.hip", "bvh.hip", "render.hip", "adjoint.hip", "batch.hip", "sensor.hip", "fftconv.hip", "denoise.hip",
-           "quipu.hip", "scan.hip"]
+           "quipu.hip", "scan.hip", "catalogue.hip"]
 # C++ programs over include/vira_gpu.hpp, linked against the library (host code only)
 EXAMPLES = ["render_dem.cpp", "quipu_scene.cpp"]
 

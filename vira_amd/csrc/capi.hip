@@ -467,6 +467,19 @@ vira_gpu_status vira_gpu_render_stars(vira_gpu_ctx* ctx, const vira_gpu_camera* 
     });
 }
 
+vira_gpu_status vira_gpu_star_light(vira_gpu_ctx* ctx, const vira_gpu_star* stars_dev, size_t n, double et,
+                                    float* directions_dev, float* irradiance_dev) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        set_device(c);
+        vg::star_light(c, stars_dev, n, et, directions_dev, irradiance_dev);
+    });
+}
+
+vira_gpu_status vira_gpu_qsc_read(const char* path, vira_gpu_star* stars_host, size_t capacity, size_t* count) {
+    return guarded([&] { vg::qsc_read(path, stars_host, capacity, count); });
+}
+
 int64_t vira_gpu_lz4_decode_block(const void* src, size_t src_size, void* dst, size_t dst_capacity) {
     if (!src || (!dst && dst_capacity)) return -1;
     return vg::lz4_decode_block(static_cast<const uint8_t*>(src), src_size, static_cast<uint8_t*>(dst), dst_capacity);

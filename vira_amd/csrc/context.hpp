@@ -218,6 +218,8 @@ void received_power(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_sensor& s
 void render_stars(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_sensor& sen, const vira_gpu_star_psf& psf,
                   const float* dirs, const float* irr, size_t n, float* img);
 void psf_kernel(Ctx& c, const vira_gpu_psf& P, int K, float* out);
+void star_light(Ctx& c, const vira_gpu_star* stars, size_t n, double et, float* dirs, float* irr);
+void qsc_read(const char* path, vira_gpu_star* out, size_t capacity, size_t* count);
 vira_gpu_qld* qld_open(const char* path, vira_gpu_qld_info* info);
 int qld_select(const vira_gpu_qld& q, double required_gsd);
 void qld_read(Ctx& c, const vira_gpu_qld& q, int level, const vira_gpu_qld_read_opts& o, float* heights_dev,

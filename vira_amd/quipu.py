@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 import struct
 from dataclasses import dataclass, field
 
@@ -326,6 +327,46 @@ def write_pyramid(path: str, levels, transformation=np.eye(4), compress: bool = 
 
 
 # ----------------------------------------------------------------- reader
+# The star quipu record (star.hpp:27-38): the Star object's bytes, 64 per star.
+STAR_DTYPE = np.dtype([("initialized", "u1"), ("pad", "u1", 7), ("ra", "<f8"), ("de", "<f8"), ("pm_ra", "<f8"),
+                       ("pm_de", "<f8"), ("vmag", "<f8"), ("temperature", "<f8"), ("omega", "<f8")])
+
+
+def make_stars(ra, de, vmag, temperature, omega, pm_ra=0.0, pm_de=0.0) -> np.ndarray:
+    """A star catalogue as STAR_DTYPE records (the Star constructors, star.ipp:34-46)."""
+    ra = np.atleast_1d(np.asarray(ra, np.float64))
+    out = np.zeros(ra.shape[0], STAR_DTYPE)
+    out["initialized"] = 1
+    for name, v in (("ra", ra), ("de", de), ("pm_ra", pm_ra), ("pm_de", pm_de), ("vmag", vmag),
+                    ("temperature", temperature), ("omega", omega)):
+        out[name] = v
+    return out
+
+
+def write_star_quipu(path, stars: np.ndarray) -> str:
+    """StarQuipu::write (star_quipu.ipp:31-41): the extension becomes .qsc, then
+    "QUIPU", uint32 count, uint32 byte size and the raw records (writeVector,
+    quipu_io.ipp:106-116)."""
+    stars = np.ascontiguousarray(stars, STAR_DTYPE)
+    p = os.path.splitext(str(path))[0] + ".qsc"
+    os.makedirs(os.path.dirname(os.path.abspath(p)), exist_ok=True)
+    with open(p, "wb") as f:
+        f.write(b"QUIPU" + struct.pack("<II", stars.shape[0], stars.nbytes) + stars.tobytes())
+    return p
+
+
+def read_star_quipu(path) -> np.ndarray:
+    """StarQuipu::read (star_quipu.ipp:20-29) through vira_gpu_qsc_read: the
+    catalogue as STAR_DTYPE records."""
+    lib = capi.load()
+    n = C.c_size_t(0)
+    capi.check(lib.vira_gpu_qsc_read(str(path).encode(), None, 0, C.byref(n)))
+    out = np.zeros(n.value, STAR_DTYPE)
+    capi.check(lib.vira_gpu_qsc_read(str(path).encode(), out.ctypes.data if n.value else None, n.value,
+                                     C.byref(n)))
+    return out
+
+
 @dataclass
 class QuipuReaderOptions:
     """DEMQuipuReaderOptions (dem_quipu.hpp:24-30)."""

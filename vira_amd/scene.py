@@ -571,6 +571,21 @@ class Context:
                    d.data_ptr(), e.data_ptr(), n, out.data_ptr())
         return out
 
+    def star_light(self, stars, et: float = 0.0):
+        """StarCatalogue::makeStarLight (star_catalogue.ipp:55-68) on the device:
+        stars a uint8 device tensor holding n 64-byte star quipu records (n x 64),
+        et seconds past J2000.  Returns (directions (n, 3), irradiance (n, 8)),
+        float32 device tensors for render_stars."""
+        if stars.dtype != self.torch.uint8 or not stars.is_cuda or stars.dim() != 2 or stars.shape[1] != 64 \
+                or not stars.is_contiguous():
+            raise ValueError("star_light: stars must be a contiguous (n, 64) uint8 device tensor of star records")
+        n = stars.shape[0]
+        dev = f"cuda:{self.device}"
+        d = self.torch.empty((n, 3), dtype=self.torch.float32, device=dev)
+        e = self.torch.empty((n, NUM_BINS), dtype=self.torch.float32, device=dev)
+        self._call(self.lib.vira_gpu_star_light, self.h, stars.data_ptr(), n, float(et), d.data_ptr(), e.data_ptr())
+        return d, e
+
     # ---- batched rendering
     @staticmethod
     def _requests(requests):

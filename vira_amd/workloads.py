@@ -514,11 +514,10 @@ def johnson_v_band(n: int = 1000):
     return lam, a * b * cdf / scale
 
 
-def star_irradiance(vmag, bv: float = 0.3):
-    """Spectral irradiance (n, 8) of stars of V magnitude vmag and colour index bv
-    (Tycho2Interface::initIrradiance, tycho2_interface.ipp:112-150; Star::getIrradiance,
-    star.ipp): T from B-V, solid angle from the V-band photon flux, times the
-    black-body band radiance."""
+def star_temperature_omega(vmag, bv: float = 0.3):
+    """Temperature (K) and solid angle (sr) of stars of V magnitude vmag and colour
+    index bv (Tycho2Interface::initIrradiance, tycho2_interface.ipp:112-150): T
+    from B-V, solid angle from the V-band photon flux over the black body's."""
     c, h, kb = 299792458.0, 6.62607015e-34, 1.380649e-23
     T = 4600.0 * (1.0 / (0.92 * bv + 1.7) + 1.0 / (0.92 * bv + 0.62))
     # Band(550 nm, fwhm 0.16, 3640 Jy).fluxFromMagnitude (magnitudes.hpp)
@@ -529,7 +528,14 @@ def star_irradiance(vmag, bv: float = 0.3):
     planck = 2 * h * c * c / lam ** 5 / (np.exp(h * c / (lam * kb * T)) - 1.0)
     counts = v * planck / (h * c / lam)
     flux2 = np.sum(0.5 * (counts[1:] + counts[:-1]) * np.diff(lam))
-    omega = flux / flux2
+    return T, flux / flux2
+
+
+def star_irradiance(vmag, bv: float = 0.3):
+    """Spectral irradiance (n, 8) of stars of V magnitude vmag and colour index bv
+    (Star::getIrradiance, star.ipp:13-16): solid angle times the black-body band
+    radiance."""
+    T, omega = star_temperature_omega(vmag, bv)
     band = S.planck_band_radiance(T, steps=100)
     return (omega[:, None] * band[None, :]).astype(np.float32)
 
@@ -547,6 +553,20 @@ def make_star_field(n: int = 100_000, seed: int = 4, mag_max: float = 10.0):
     r = np.sqrt(1.0 - z * z)
     dirs = np.stack([r * np.cos(phi), r * np.sin(phi), z], 1).astype(np.float32)
     return dirs, star_irradiance(mag), mag
+
+
+def make_star_catalogue(n: int = 100_000, seed: int = 4, mag_max: float = 10.0, pm_sigma: float = 1e-8):
+    """The star field of make_star_field(n, seed, mag_max) as a star catalogue:
+    64-byte star quipu records (RA = atan2(y, x), DE = asin(z) of the same
+    directions, temperature and solid angle from the V magnitude at B-V 0.3) with
+    proper motions N(0, pm_sigma) rad/yr from numpy Generator(seed + 1)."""
+    from . import quipu as Q
+    dirs, _, mag = make_star_field(n, seed, mag_max)
+    d = dirs.astype(np.float64)
+    T, omega = star_temperature_omega(mag)
+    pm = np.random.default_rng(seed + 1).normal(0.0, pm_sigma, (2, n))
+    return Q.make_stars(np.arctan2(d[:, 1], d[:, 0]), np.arcsin(np.clip(d[:, 2], -1, 1)), mag, T, omega,
+                        pm_ra=pm[0], pm_de=pm[1])
 
 
 def star_field_camera(image: int = 4096, direction=(0.6, -0.48, 0.64)):
