@@ -508,10 +508,11 @@ vira_gpu_status vira_gpu_qld_level_gsd(const vira_gpu_qld* qld, int32_t level, d
 vira_gpu_status vira_gpu_qld_select(const vira_gpu_qld* qld, double required_gsd, int32_t* level);
 /* DEMQuipu::readBuffers (dem_quipu.ipp:131-236) for one level: the mesh
  * (DEM::makeVertexBuffer / makeIndexBuffer, dem.ipp:138-211, with the level's
- * DEMProjection: planar, or OGR equirectangular) in a new mesh id; heights_dev
+ * DEMProjection: planar, or OGR equirectangular, or OGR polar stereographic on
+ * a sphere) in a new mesh id; heights_dev
  * (W*H, optional) and albedo_dev (W*H*8, optional) receive the decoded
- * samples; mesh_id may be NULL to decode only.  OGR projections other than
- * equirectangular return VIRA_GPU_ERR_UNSUPPORTED. */
+ * samples; mesh_id may be NULL to decode only.  Other OGR projections, and
+ * polar stereographic on an ellipsoid, return VIRA_GPU_ERR_UNSUPPORTED. */
 vira_gpu_status vira_gpu_qld_read(vira_gpu_ctx* ctx, vira_gpu_qld* qld, int32_t level,
                                   const vira_gpu_qld_read_opts* opts, int32_t* mesh_id, float* heights_dev,
                                   float* albedo_dev, vira_gpu_qld_level* info);

@@ -172,12 +172,44 @@ def test_equirectangular_level(gpu_ctx, tmp_path):
     assert abs(lon - 30.0) < 1e-4 and abs(c[2]) < 1.0
     # other map projections are refused with their name
     p2 = Q.DEMProjection(resolution=(65, 65), pixel_scale=(ps, -ps), is_ogr=True,
-                         proj_ref=Q.equirectangular_wkt(R).replace("Equirectangular\"]", "Polar_Stereographic\"]"))
-    Q.write_pyramid(str(tmp_path / "ps.qld"), [Q.DEMLevel(h, p2)], compress=False)
-    d2 = Q.QuipuDEM(str(tmp_path / "ps.qld"))
-    with pytest.raises(_capi.ViraGPUError, match="polar_stereographic"):
+                         proj_ref=Q.equirectangular_wkt(R).replace("Equirectangular\"]", "Lambert_Conformal_Conic_2SP\"]"))
+    Q.write_pyramid(str(tmp_path / "lcc.qld"), [Q.DEMLevel(h, p2)], compress=False)
+    d2 = Q.QuipuDEM(str(tmp_path / "lcc.qld"))
+    with pytest.raises(_capi.ViraGPUError, match="lambert_conformal_conic_2sp"):
         d2.read_level(gpu_ctx, 0)
     d.close()
+    d2.close()
+
+
+@pytest.mark.parametrize("south,lat,lon0", [(False, None, 0.0), (True, None, 60.0), (False, 75.0, -20.0)])
+def test_polar_stereographic_level(gpu_ctx, tmp_path, south, lat, lon0):
+    """A 65^2 level around the pole (20 km pixels), read through the C-ABI:
+    positions within one FP32 ulp of the FP64 inverse projection."""
+    R = 1737400.0
+    h = gpu_ctx.synth_fbm(65, 65, 5, 2000.0).cpu().numpy()
+    ps = 20000.0
+    p = Q.DEMProjection(resolution=(65, 65), pixel_scale=(ps, -ps), is_ogr=True,
+                        proj_ref=Q.polar_stereographic_wkt(R, south=south, central_meridian=lon0,
+                                                           latitude_of_origin=lat),
+                        model_tie_point=(-32.5 * ps, 32.5 * ps))
+    path = tmp_path / "ps.qld"
+    Q.write_pyramid(str(path), [Q.DEMLevel(h, p)], compress=True)
+    d = Q.QuipuDEM(str(path))
+    mid, hd, _, info = d.read_level(gpu_ctx, 0, samples=True)
+    pos, _, _ = gpu_ctx.mesh_download(mid)
+    want = Q.level_positions(Q.DEMLevel(hd.cpu().numpy(), p)).reshape(-1, 3).astype(np.float32)
+    ulp = np.spacing(np.abs(want))
+    assert np.all(np.abs(pos - want) <= ulp), np.abs(pos - want).max()
+    # the centre sample is the pole
+    c = pos.reshape(65, 65, 3)[32, 32].astype(np.float64)
+    assert np.sign(c[2]) == (-1 if south else 1) and np.hypot(c[0], c[1]) < 2 * ps
+    d.close()
+    # polar stereographic on an ellipsoid is refused
+    p.proj_ref = p.proj_ref.replace(f"{R!r},0.0]", f"{R!r},298.257]")
+    Q.write_pyramid(str(tmp_path / "pse.qld"), [Q.DEMLevel(h, p)], compress=False)
+    d2 = Q.QuipuDEM(str(tmp_path / "pse.qld"))
+    with pytest.raises(_capi.ViraGPUError, match="sphere only"):
+        d2.read_level(gpu_ctx, 0)
     d2.close()
 
 

@@ -231,3 +231,77 @@ def test_star_quipu_empty_and_malformed(tmp_path):
         Q.read_star_quipu(bad)
     with pytest.raises(capi.ViraGPUError, match="cannot open"):
         Q.read_star_quipu(tmp_path / "missing.qsc")
+
+
+# ---- polar stereographic levels: the inverse of level_positions against the
+# geometric projection (from the far pole onto the plane at the near pole)
+
+def _ps_level(R, n, px, south=False, lat=None, k=1.0, lon0=0.0):
+    p = Q.DEMProjection(resolution=(n, n), pixel_scale=(px, -px), is_ogr=True,
+                        proj_ref=Q.polar_stereographic_wkt(R, south=south, central_meridian=lon0,
+                                                           latitude_of_origin=lat, scale_factor=k),
+                        model_tie_point=(-(n / 2) * px, (n / 2) * px))
+    return Q.DEMLevel(np.zeros((n, n), np.float32), p)
+
+
+def _grid_xy(lv):
+    p = lv.projection
+    n = lv.heights.shape[0]
+    c = np.arange(n) + 0.5
+    X = p.model_tie_point[0] + c[None, :] * p.pixel_scale[0]
+    Y = p.model_tie_point[1] + c[:, None] * p.pixel_scale[1]
+    return np.broadcast_arrays(X, Y)
+
+
+@pytest.mark.parametrize("south,k,lon0", [(False, 1.0, 0.0), (True, 1.0, 0.0), (False, 0.994, 0.0),
+                                          (True, 1.0, 45.0)])
+def test_polar_stereographic_matches_geometric_projection(south, k, lon0):
+    R = 1737400.0
+    lv = _ps_level(R, 33, 20000.0, south=south, k=k, lon0=lon0)
+    P = Q.level_positions(lv)
+    u = P / np.linalg.norm(P, axis=2, keepdims=True)
+    assert np.allclose(np.linalg.norm(P, axis=2), R, rtol=1e-14)
+    # rotate the central meridian onto +x, then project from the far pole
+    t = np.radians(lon0)
+    ux = np.cos(t) * u[..., 0] + np.sin(t) * u[..., 1]
+    uy = -np.sin(t) * u[..., 0] + np.cos(t) * u[..., 1]
+    s = -1.0 if south else 1.0
+    q = 2.0 * k * R / (1.0 + s * u[..., 2])
+    X, Y = _grid_xy(lv)
+    np.testing.assert_allclose(q * uy, X, atol=1e-6)
+    np.testing.assert_allclose(-s * q * ux, Y, atol=1e-6)
+
+
+def test_polar_stereographic_known_points():
+    R = 1000.0
+    # 2 R k0 from the pole is the equator (k0 = 1); the pole is the origin
+    lv = _ps_level(R, 2, 2 * R)      # pixel centres at (+-R, +-R): rho = R sqrt 2
+    P = Q.level_positions(lv)
+    colat = np.degrees(np.arccos(P[..., 2] / R))
+    np.testing.assert_allclose(colat, np.degrees(2 * np.arctan(np.sqrt(2) / 2)), rtol=1e-12)
+    lv = _ps_level(R, 2, 4 * R / np.sqrt(2))   # rho = 2 R: on the equator
+    assert np.allclose(Q.level_positions(lv)[..., 2], 0.0, atol=1e-9)
+    lv = Q.DEMLevel(np.zeros((1, 1), np.float32),
+                    Q.DEMProjection(resolution=(1, 1), pixel_scale=(2.0, -2.0), is_ogr=True,
+                                    proj_ref=Q.polar_stereographic_wkt(R, south=True),
+                                    model_tie_point=(-1.0, 1.0)))
+    np.testing.assert_allclose(Q.level_positions(lv)[0, 0], [0, 0, -R], atol=1e-9)
+
+
+def test_polar_stereographic_true_scale_latitude():
+    # a latitude of origin other than +-90 is the latitude of true scale: there,
+    # one pixel along the parallel spans one pixel's length on the sphere
+    R, px = 1737400.0, 10.0
+    for lat in (71.0, -80.0):
+        lv = _ps_level(R, 2, px, south=lat < 0, lat=lat)
+        pole, akm1 = Q._polar_params(lv.projection.proj_ref)
+        assert pole == np.sign(lat) and akm1 == pytest.approx(1 + np.sin(np.radians(abs(lat))), rel=1e-14)
+        # shift the grid to the true-scale parallel on the central meridian
+        rho = R * akm1 * np.tan(np.radians(90 - abs(lat)) / 2)
+        p = lv.projection
+        p.model_tie_point = (-px, (-rho if lat > 0 else rho) + px)
+        P = Q.level_positions(lv)
+        lat_got = np.degrees(np.arcsin(P[..., 2] / R))
+        assert abs(abs(lat_got).mean() - abs(lat)) < 1e-3
+        ds = np.linalg.norm(P[0, 1] - P[0, 0])
+        assert ds == pytest.approx(px, rel=1e-4)

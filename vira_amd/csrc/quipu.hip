@@ -203,12 +203,13 @@ Img read_image(Rd& r, uint16_t expect_id, int elem, double& decode_ms) {
 
 // ------------------------------------------------------------- device side
 struct QProj {
-    int kind;  // 0 planar (non-OGR), 1 equirectangular (OGR)
+    int kind;  // 0 planar (non-OGR), 1 equirectangular (OGR), 2 polar stereographic (OGR, sphere)
     double pixel_scale[2], xoff, yoff, dx, dy;
     double start_x, start_y, scale_x, scale_y;  // DEMProjection::initToWorld
     double rot[2], tie[2];
     double a, b;                                // ellipsoid semi-axes
     double lon0, lat_ts, lat0, fe, fn;          // equirectangular parameters (degrees, metres)
+    double pole, akm1;                          // polar stereographic: +1 north / -1 south, 2 k0
     double offset[3];
 };
 
@@ -247,8 +248,24 @@ __device__ void q_world(const QProj& p, int i, int j, double z, double out[3]) {
     const double X = p.tie[0] + (Ix * p.pixel_scale[0]) + (Iy * p.rot[0]);
     const double Y = p.tie[1] + (Ix * p.rot[1]) + (Iy * p.pixel_scale[1]);
     const double d2r = 3.14159265358979323846 / 180.0;
-    const double lon = p.lon0 + __ddiv_rn(X - p.fe, p.a * cos(p.lat_ts * d2r)) / d2r;
-    const double lat = p.lat0 + __ddiv_rn(Y - p.fn, p.a) / d2r;
+    double lon, lat;
+    if (p.kind == 2) {
+        // PROJ's spherical polar stere inverse: (x, y) / a, rh = hypot,
+        // c = 2 atan(rh / 2k0), phi = +-asin(cos c), lam = atan2(x, -+y)
+        const double x = __ddiv_rn(X - p.fe, p.a), y = __ddiv_rn(Y - p.fn, p.a);
+        const double rh = hypot(x, y);
+        const double cosc = cos(2.0 * atan(__ddiv_rn(rh, p.akm1)));
+        if (rh <= 1e-10) {
+            lat = 90.0 * p.pole;
+            lon = p.lon0;
+        } else {
+            lat = asin(p.pole * cosc) / d2r;
+            lon = p.lon0 + atan2(x, -p.pole * y) / d2r;
+        }
+    } else {
+        lon = p.lon0 + __ddiv_rn(X - p.fe, p.a * cos(p.lat_ts * d2r)) / d2r;
+        lat = p.lat0 + __ddiv_rn(Y - p.fn, p.a) / d2r;
+    }
     const double ee2 = __ddiv_rn(p.a * p.a - p.b * p.b, p.a * p.a);
     const double clat = cos(lat * d2r), slat = sin(lat * d2r), clon = cos(lon * d2r), slon = sin(lon * d2r);
     const double V = __ddiv_rn(p.a, __dsqrt_rn(1.0 - ee2 * (clat * clat) * (slon * slon)));
@@ -332,9 +349,11 @@ QProj make_qproj(const Proj& P, const double offset[3]) {
     VG_CHECK(pj != std::string::npos, VIRA_GPU_ERR_UNSUPPORTED, "quipu: projection string has no PROJECTION");
     const size_t e = lw.find('"', pj + 12);
     const std::string name = lw.substr(pj + 12, e - (pj + 12));
-    VG_CHECK(name == "equirectangular" || name == "equidistant_cylindrical" || name == "plate_carree",
-             VIRA_GPU_ERR_UNSUPPORTED,
-             "quipu: OGR projection '" + name + "' is not supported (equirectangular only)");
+    const bool eqc = name == "equirectangular" || name == "equidistant_cylindrical" || name == "plate_carree";
+    const bool ps_n = name == "stereographic_north_pole", ps_s = name == "stereographic_south_pole";
+    const bool ps = name == "polar_stereographic" || ps_n || ps_s;
+    VG_CHECK(eqc || ps, VIRA_GPU_ERR_UNSUPPORTED,
+             "quipu: OGR projection '" + name + "' is not supported (equirectangular and polar stereographic only)");
     const size_t sp = lw.find("spheroid[\"");
     VG_CHECK(sp != std::string::npos, VIRA_GPU_ERR_INVALID, "quipu: projection string has no SPHEROID");
     const size_t c1 = P.wkt.find(',', sp);
@@ -351,6 +370,31 @@ QProj make_qproj(const Proj& P, const double offset[3]) {
     wkt_param(P.wkt, "latitude_of_origin", q.lat0) || wkt_param(P.wkt, "latitude_of_center", q.lat0);
     wkt_param(P.wkt, "false_easting", q.fe);
     wkt_param(P.wkt, "false_northing", q.fn);
+    q.pole = 1.0;
+    q.akm1 = 2.0;
+    if (ps) {
+        // GDAL's reading of WKT1 polar stereographic: a latitude of origin of
+        // +-90 is the pole with scale_factor k0; any other value is the
+        // latitude of true scale (the ESRI names give it as
+        // standard_parallel_1), the pole on its side.  PROJ's stere then uses
+        // akm1 = 2 k0, or cos(phits) / tan(pi/4 - phits/2) for |phits| < 90.
+        VG_CHECK(q.a == q.b, VIRA_GPU_ERR_UNSUPPORTED,
+                 "quipu: polar stereographic is supported on a sphere only (inverse flattening 0)");
+        double lat_o = ps_s ? -90.0 : 90.0, k0 = 1.0;
+        wkt_param(P.wkt, "latitude_of_origin", lat_o) || wkt_param(P.wkt, "standard_parallel_1", lat_o);
+        wkt_param(P.wkt, "scale_factor", k0);
+        wkt_param(P.wkt, "central_meridian", q.lon0) ||
+            wkt_param(P.wkt, "straight_vertical_longitude_from_pole", q.lon0) ||
+            wkt_param(P.wkt, "longitude_of_origin", q.lon0);
+        q.pole = ps_n ? 1.0 : ps_s ? -1.0 : (lat_o < 0.0 ? -1.0 : 1.0);
+        const double phits = std::fabs(lat_o) * (3.14159265358979323846 / 180.0);
+        VG_CHECK(std::fabs(lat_o) <= 90.0 && k0 > 0.0, VIRA_GPU_ERR_INVALID,
+                 "quipu: polar stereographic latitude or scale factor out of range");
+        q.akm1 = std::fabs(phits - 3.14159265358979323846 / 2) >= 1e-10
+                     ? std::cos(phits) / std::tan(3.14159265358979323846 / 4 - 0.5 * phits)
+                     : 2.0 * k0;
+        q.kind = 2;
+    }
     return q;
 }
 

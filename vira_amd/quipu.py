@@ -75,6 +75,20 @@ def equirectangular_wkt(radius: float, central_meridian: float = 0.0, standard_p
             f'{standard_parallel!r}],UNIT["Meter",1.0]]')
 
 
+def polar_stereographic_wkt(radius: float, south: bool = False, central_meridian: float = 0.0,
+                            latitude_of_origin: float | None = None, scale_factor: float = 1.0,
+                            name: str = "Moon") -> str:
+    """A WKT1 polar stereographic projection on a sphere, as polar lunar DEM
+    products carry: latitude_of_origin +-90 (the pole, with scale_factor) or
+    the latitude of true scale."""
+    lat = (-90.0 if south else 90.0) if latitude_of_origin is None else latitude_of_origin
+    return (f'PROJCS["{name}_Polar_Stereographic",GEOGCS["GCS_{name}",DATUM["D_{name}",SPHEROID["{name}",'
+            f'{radius!r},0.0]],PRIMEM["Reference_Meridian",0.0],UNIT["Degree",0.0174532925199433]],'
+            f'PROJECTION["Polar_Stereographic"],PARAMETER["latitude_of_origin",{lat!r}],'
+            f'PARAMETER["central_meridian",{central_meridian!r}],PARAMETER["scale_factor",{scale_factor!r}],'
+            f'PARAMETER["false_easting",0.0],PARAMETER["false_northing",0.0],UNIT["Meter",1.0]]')
+
+
 # ----------------------------------------------------------------- writer
 def _lz4():
     try:
@@ -194,7 +208,8 @@ def _grid_normals(P: np.ndarray, valid: np.ndarray) -> np.ndarray:
 
 def level_positions(level: DEMLevel) -> np.ndarray:
     """DEMProjection::transformToWorld (dem_projection.ipp:102-144) in FP64 for
-    every sample (planar, or OGR equirectangular on a sphere/ellipsoid)."""
+    every sample (planar, or OGR equirectangular or polar stereographic on a
+    sphere)."""
     h = np.asarray(level.heights, np.float64)
     H, W = h.shape
     p = level.projection
@@ -212,8 +227,18 @@ def level_positions(level: DEMLevel) -> np.ndarray:
     Iy = p.dy * j + p.yoff + 0.5
     X = p.model_tie_point[0] + Ix * p.pixel_scale[0] + Iy * p.row_col_rotation[0]
     Y = p.model_tie_point[1] + Ix * p.row_col_rotation[1] + Iy * p.pixel_scale[1]
-    lon = np.radians(lon0) + X / (a * math.cos(math.radians(lat_ts)))
-    lat = Y / a
+    ps = _polar_params(p.proj_ref)
+    if ps is not None:
+        # spherical polar stereographic inverse (PROJ stere): c = 2 atan(rh / 2k0)
+        pole, akm1 = ps
+        x, y = X / a, Y / a
+        rh = np.hypot(x, y)
+        lat = np.where(rh <= 1e-10, pole * np.pi / 2, np.arcsin(pole * np.cos(2.0 * np.arctan(rh / akm1))))
+        lon = np.radians(lon0) + np.where(rh <= 1e-10, 0.0, np.arctan2(x, -pole * y))
+        lat, lon = np.broadcast_arrays(lat, lon)
+    else:
+        lon = np.radians(lon0) + X / (a * math.cos(math.radians(lat_ts)))
+        lat = Y / a
     r = a + h
     return np.stack([r * np.cos(lon) * np.cos(lat), r * np.sin(lon) * np.cos(lat), r * np.sin(lat)], axis=2)
 
@@ -225,6 +250,24 @@ def _eqc_params(wkt: str):
     cm = re.search(r'parameter\["central_meridian",\s*([-+0-9.eE]+)', lw)
     sp = re.search(r'parameter\["standard_parallel_1",\s*([-+0-9.eE]+)', lw)
     return a, float(cm.group(1)) if cm else 0.0, float(sp.group(1)) if sp else 0.0
+
+
+def _polar_params(wkt: str):
+    """(pole +-1, 2 k0) of a polar stereographic WKT, None for other projections."""
+    import re
+    lw = wkt.lower()
+    m = re.search(r'projection\["([^"]*)"', lw)
+    name = m.group(1) if m else ""
+    if name not in ("polar_stereographic", "stereographic_north_pole", "stereographic_south_pole"):
+        return None
+    lo = re.search(r'parameter\["(?:latitude_of_origin|standard_parallel_1)",\s*([-+0-9.eE]+)', lw)
+    k = re.search(r'parameter\["scale_factor",\s*([-+0-9.eE]+)', lw)
+    lat = float(lo.group(1)) if lo else (-90.0 if name.endswith("south_pole") else 90.0)
+    pole = 1.0 if name.endswith("north_pole") else -1.0 if name.endswith("south_pole") else (-1.0 if lat < 0 else 1.0)
+    phits = math.radians(abs(lat))
+    if abs(phits - math.pi / 2) >= 1e-10:
+        return pole, math.cos(phits) / math.tan(math.pi / 4 - 0.5 * phits)
+    return pole, 2.0 * (float(k.group(1)) if k else 1.0)
 
 
 def normal_cone(level: DEMLevel):
