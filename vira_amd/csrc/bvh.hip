@@ -31,6 +31,7 @@ constexpr int kChunkItems = kChunk / kChunkThreads;
 constexpr uint32_t kSmall = 256;       // nodes at or below this size are built per thread
 constexpr int kMaxBins = 32;
 constexpr int kSmallBins = 8;
+constexpr uint32_t kReduceMinChunks = 16;  // nodes of >= 16 chunks bin by wave reductions
 constexpr int kBinVals = 13;           // count, box lo/hi (6), centroid lo/hi (6)
 constexpr int kMedianDepth = 48;       // deeper nodes are split at the object median
 
@@ -151,26 +152,75 @@ __global__ void __launch_bounds__(kChunkThreads) k_bin_large(const Chunk* chunks
     __syncthreads();
     Chunk ch = chunks[blockIdx.x];
     TaskParam tp = params[ch.task];
-    for (uint32_t slot = ch.begin + threadIdx.x; slot < ch.end; slot += blockDim.x) {
-        const PrimBox p = pb[prims[slot]];
-        const float c[3] = {centroid(p, 0), centroid(p, 1), centroid(p, 2)};
-        float cc = c[tp.axis];
-        int b = bin_of(cc, tp.cmin, tp.scale, nb);
-        slot_bin[slot] = (uint8_t)b;
-        int* sb = &s[b * kBinVals];
-        atomicAdd(&sb[0], 1);
-        atomicMin(&sb[1], f2key(p.lo[0]));
-        atomicMin(&sb[2], f2key(p.lo[1]));
-        atomicMin(&sb[3], f2key(p.lo[2]));
-        atomicMax(&sb[4], f2key(p.hi[0]));
-        atomicMax(&sb[5], f2key(p.hi[1]));
-        atomicMax(&sb[6], f2key(p.hi[2]));
-        atomicMin(&sb[7], f2key(c[0]));
-        atomicMin(&sb[8], f2key(c[1]));
-        atomicMin(&sb[9], f2key(c[2]));
-        atomicMax(&sb[10], f2key(c[0]));
-        atomicMax(&sb[11], f2key(c[1]));
-        atomicMax(&sb[12], f2key(c[2]));
+    const int lane = threadIdx.x & 63;
+    // In a node of many chunks a bin spans a wide slab, neighbouring slots
+    // mostly fall into the same bin, and LDS atomics from a wave would mostly
+    // hit one address.  There the wave reduces the lanes of one bin at a time
+    // (the first pending lane's) with cross-lane min/max/add and one lane adds
+    // the result: as many rounds as distinct bins in the wave.  Small nodes'
+    // bins are narrow, their waves span many bins, and per-lane atomics win.
+    if (tp.nchunk < kReduceMinChunks) {
+        for (uint32_t slot = ch.begin + threadIdx.x; slot < ch.end; slot += blockDim.x) {
+            const PrimBox p = pb[prims[slot]];
+            const float c[3] = {centroid(p, 0), centroid(p, 1), centroid(p, 2)};
+            const int b = bin_of(c[tp.axis], tp.cmin, tp.scale, nb);
+            slot_bin[slot] = (uint8_t)b;
+            int* sb = &s[b * kBinVals];
+            atomicAdd(&sb[0], 1);
+            for (int k = 0; k < 3; ++k) {
+                atomicMin(&sb[1 + k], f2key(p.lo[k]));
+                atomicMax(&sb[4 + k], f2key(p.hi[k]));
+                atomicMin(&sb[7 + k], f2key(c[k]));
+                atomicMax(&sb[10 + k], f2key(c[k]));
+            }
+        }
+    } else for (uint32_t base = ch.begin; base < ch.end; base += blockDim.x) {
+        const uint32_t slot = base + threadIdx.x;
+        const bool valid = slot < ch.end;
+        int b = -1;
+        int v[kBinVals];
+        if (valid) {
+            const PrimBox p = pb[prims[slot]];
+            const float c[3] = {centroid(p, 0), centroid(p, 1), centroid(p, 2)};
+            b = bin_of(c[tp.axis], tp.cmin, tp.scale, nb);
+            slot_bin[slot] = (uint8_t)b;
+            v[0] = 1;
+            for (int k = 0; k < 3; ++k) {
+                v[1 + k] = f2key(p.lo[k]);
+                v[4 + k] = f2key(p.hi[k]);
+                v[7 + k] = f2key(c[k]);
+                v[10 + k] = f2key(c[k]);
+            }
+        }
+        uint64_t pending = __ballot(valid);
+        while (pending) {
+            const int lead = __builtin_ctzll(pending);
+            const int b0 = __shfl(b, lead, 64);
+            const bool mine = b == b0;
+            pending &= ~__ballot(mine);
+            int r[kBinVals];
+            r[0] = mine ? 1 : 0;
+#pragma unroll
+            for (int k = 1; k < kBinVals; ++k) r[k] = mine ? v[k] : (((k - 1) % 6) < 3 ? INT_MAX : INT_MIN);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                r[0] += __shfl_xor(r[0], o, 64);
+#pragma unroll
+                for (int k = 1; k < kBinVals; ++k) {
+                    const int y = __shfl_xor(r[k], o, 64);
+                    r[k] = ((k - 1) % 6) < 3 ? min(r[k], y) : max(r[k], y);
+                }
+            }
+            if (lane == lead) {
+                int* sb = &s[b0 * kBinVals];
+                atomicAdd(&sb[0], r[0]);
+#pragma unroll
+                for (int k = 1; k < kBinVals; ++k) {
+                    if (((k - 1) % 6) < 3) atomicMin(&sb[k], r[k]);
+                    else atomicMax(&sb[k], r[k]);
+                }
+            }
+        }
     }
     __syncthreads();
     if (threadIdx.x < (unsigned)nb) {
