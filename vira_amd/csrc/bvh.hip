@@ -402,14 +402,17 @@ __global__ void k_chunk_bases(const Chunk* __restrict__ chunks, const TaskParam*
 constexpr uint32_t kTaskRoot = 0xffffffffu;
 constexpr int kSmallStack = 12;  // > log2(kSmall) + 1
 
-__global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, uint32_t ntasks, uint32_t* prims, const PrimBox* pb,
+__global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, const uint32_t* order, uint32_t ntasks, uint32_t* prims, const PrimBox* pb,
                               BNodeG* nodes, BNodeG* snodes, uint32_t* used, uint32_t* max_depth, uint32_t* err,
                               int max_leaf, float trav_cost) {
     // each thread's stack in its own LDS column (not in scratch memory)
     __shared__ uint4 s_stack[kSmallStack][64];
-    const uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t md = 0;
-    if (ti < ntasks) {
+    if (gid < ntasks) {
+        // threads take the tasks largest first, so a wave's lanes build
+        // subtrees of about the same size; results stay indexed by task
+        const uint32_t ti = order[gid];
         const SmallTask task = tasks[ti];
         BNodeG* local = snodes + task.base;
         struct Ent {
@@ -1147,7 +1150,7 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     // upload the large-level nodes, then finish the small subtrees on the GPU
     VG_HIP(hipMemcpyAsync(nodes.p, hnodes.data(), hnodes.size() * sizeof(BNodeG), hipMemcpyHostToDevice, c.stream));
     // counter words: 0 unused, 1 max depth, 2 small-builder error flag (word 1 later becomes the emit error flag)
-    DBuf d_small, d_counter, snodes, d_used, d_first;
+    DBuf d_small, d_order, d_counter, snodes, d_used, d_first;
     d_counter.alloc(12);
     uint32_t hc[3] = {0u, max_depth, 0u};
     VG_HIP(hipMemcpyAsync(d_counter.p, hc, 12, hipMemcpyHostToDevice, c.stream));
@@ -1162,13 +1165,20 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
             scratch += 2 * (size_t)(small_h[i].count - 1);
         }
         VG_CHECK(scratch < (1ull << 32), VIRA_GPU_ERR_INVALID, "BVH build: small-subtree scratch overflow");
+        // counting sort of the tasks by size, largest first (sizes 1..kSmall)
+        std::vector<uint32_t> start(kSmall + 2, 0u), order(ntasks);
+        for (uint32_t i = 0; i < ntasks; ++i) start[kSmall - std::min(st[i].count, kSmall) + 1]++;
+        for (uint32_t k = 1; k < kSmall + 2; ++k) start[k] += start[k - 1];
+        for (uint32_t i = 0; i < ntasks; ++i) order[start[kSmall - std::min(st[i].count, kSmall)]++] = i;
+        d_order.alloc((size_t)ntasks * 4);
+        VG_HIP(hipMemcpyAsync(d_order.p, order.data(), (size_t)ntasks * 4, hipMemcpyHostToDevice, c.stream));
         d_small.alloc(st.size() * sizeof(SmallTask));
         snodes.alloc(std::max<size_t>(1, scratch) * sizeof(BNodeG));
         d_used.alloc((size_t)ntasks * 4);
         d_first.alloc((size_t)ntasks * 4);
         VG_HIP(hipMemcpyAsync(d_small.p, st.data(), st.size() * sizeof(SmallTask), hipMemcpyHostToDevice, c.stream));
         k_build_small<<<blocks_for(ntasks, 64), 64, 0, c.stream>>>(
-            d_small.as<SmallTask>(), ntasks, prims.as<uint32_t>(), pbuf.as<PrimBox>(), nodes.as<BNodeG>(),
+            d_small.as<SmallTask>(), d_order.as<uint32_t>(), ntasks, prims.as<uint32_t>(), pbuf.as<PrimBox>(), nodes.as<BNodeG>(),
             snodes.as<BNodeG>(), d_used.as<uint32_t>(), d_counter.as<uint32_t>() + 1, d_counter.as<uint32_t>() + 2,
             max_leaf, opt.traversal_cost > 0.0f ? opt.traversal_cost : 1.0f);
         VG_HIP(hipGetLastError());
