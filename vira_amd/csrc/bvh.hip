@@ -31,7 +31,7 @@ constexpr int kChunkItems = kChunk / kChunkThreads;
 constexpr uint32_t kSmall = 256;       // nodes at or below this size are built per thread
 constexpr int kMaxBins = 32;
 constexpr int kSmallBins = 8;
-constexpr uint32_t kReduceMinChunks = 16;  // nodes of >= 16 chunks bin by wave reductions
+constexpr uint32_t kReduceMinChunks = 4;  // nodes of >= 4 chunks bin by wave reductions
 constexpr int kBinVals = 13;           // count, box lo/hi (6), centroid lo/hi (6)
 constexpr int kMedianDepth = 48;       // deeper nodes are split at the object median
 
@@ -202,15 +202,11 @@ __global__ void __launch_bounds__(kChunkThreads) k_bin_large(const Chunk* chunks
             r[0] = mine ? 1 : 0;
 #pragma unroll
             for (int k = 1; k < kBinVals; ++k) r[k] = mine ? v[k] : (((k - 1) % 6) < 3 ? INT_MAX : INT_MIN);
+            // whole-wave reductions (DPP-based device library routines)
+            r[0] = __ockl_wfred_add_i32(r[0]);
 #pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                r[0] += __shfl_xor(r[0], o, 64);
-#pragma unroll
-                for (int k = 1; k < kBinVals; ++k) {
-                    const int y = __shfl_xor(r[k], o, 64);
-                    r[k] = ((k - 1) % 6) < 3 ? min(r[k], y) : max(r[k], y);
-                }
-            }
+            for (int k = 1; k < kBinVals; ++k)
+                r[k] = ((k - 1) % 6) < 3 ? __ockl_wfred_min_i32(r[k]) : __ockl_wfred_max_i32(r[k]);
             if (lane == lead) {
                 int* sb = &s[b0 * kBinVals];
                 atomicAdd(&sb[0], r[0]);
