@@ -175,6 +175,13 @@ def test_star_light_closed_forms_empty_and_invalid(gpu_ctx):
     assert d.shape == (0, 3) and e.shape == (0, 8)
     with pytest.raises(ValueError):
         gpu_ctx.star_light(torch.zeros((4, 63), dtype=torch.uint8, device="cuda"))
+    # the C-ABI refuses misaligned records
+    from vira_amd import _capi
+    buf = torch.zeros((3 * 64 + 8,), dtype=torch.uint8, device="cuda")
+    out = torch.empty((3, 11), dtype=torch.float32, device="cuda")
+    with pytest.raises(_capi.ViraGPUError, match="aligned"):
+        _capi.check(gpu_ctx.lib.vira_gpu_star_light(gpu_ctx.h, buf.data_ptr() + 4, 3, 0.0, out.data_ptr(),
+                                                     out.data_ptr() + 36))
 
 
 def test_star_catalogue_full_size_into_render_stars(gpu_ctx, tmp_path):

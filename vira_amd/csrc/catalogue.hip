@@ -86,6 +86,9 @@ __global__ __launch_bounds__(kBlock) void k_star_light(const vira_gpu_star* __re
 void star_light(Ctx& c, const vira_gpu_star* stars, size_t n, double et, float* dirs, float* irr) {
     VG_CHECK(n == 0 || (stars && dirs && irr), VIRA_GPU_ERR_INVALID, "star_light: null buffers");
     if (n == 0) return;
+    VG_CHECK(((uintptr_t)stars & 7) == 0 && ((uintptr_t)dirs & 3) == 0 && ((uintptr_t)irr & 3) == 0,
+             VIRA_GPU_ERR_INVALID, "star_light: star records must be 8-byte aligned, outputs 4-byte aligned");
+    VG_CHECK(n <= ((size_t)1 << 36), VIRA_GPU_ERR_INVALID, "star_light: too many stars for one launch");
     k_star_light<<<blocks_for(n * NB, kBlock), kBlock, 0, c.stream>>>(stars, n, et, dirs, irr);
     VG_HIP(hipGetLastError());
     VG_HIP(hipStreamSynchronize(c.stream));
