@@ -6,6 +6,7 @@
 
 #include "oracle_core.hpp"
 #include "oracle_diff.hpp"
+#include "../vira_amd/csrc/fp_sincos.hpp"
 
 using namespace vo;
 
@@ -322,6 +323,11 @@ void orc_di_fd64(void* d, const double* h, const double* tex, const uint32_t* re
                  const Light* lights, int nl, const Material* mat, const double* target, const int64_t* nodes,
                  size_t n, double step, double* out, int nthreads) {
     di_fd_heights64(di_scene(d, rec, npix, spp, lights, nl, mat), h, tex, target, nodes, n, step, out, nthreads);
+}
+
+// sin and cos of the kernels' FP32 routine (fp_sincos.hpp), for the tests that pin it.
+void orc_sincos_f32(const float* x, size_t n, float* s, float* c) {
+    for (size_t i = 0; i < n; ++i) vgm::sincos_f32(x[i], s[i], c[i]);
 }
 
 }  // extern "C"

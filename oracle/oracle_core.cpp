@@ -2,6 +2,8 @@
 // TEST INFRASTRUCTURE ONLY (see oracle_core.hpp).
 #include "oracle_core.hpp"
 
+#include "../vira_amd/csrc/fp_sincos.hpp"
+
 #include <atomic>
 #include <thread>
 
@@ -222,6 +224,12 @@ V3<float> offset_point_f32(V3<float> p, V3<float> n) { return offset_point(p, n)
 
 namespace {
 
+// sin and cos of the sampling angles: the FP32 twin uses the kernels' own
+// routine (fp_sincos.hpp, Cody-Waite reduction + fitted polynomials, pinned
+// against FP64 sin/cos by the tests); the FP64 reference answer uses libm.
+inline void sincos_r(float x, float& s, float& c) { vgm::sincos_f32(x, s, c); }
+inline void sincos_r(double x, double& s, double& c) { s = std::sin(x); c = std::cos(x); }
+
 template <class R>
 inline R power_heuristic(R fPdf, R gPdf) {  // cpu_path_tracer.ipp:472-478
     R f = fPdf, g = gPdf;
@@ -287,7 +295,8 @@ inline bool sample_light(const Light& L, const V3<R>& p, uint32_t seed, uint32_t
     R q = x / ((R)1 + std::sqrt((R)1 - x));
     R t = zeta * q;
     R cosT = (R)1 - t, sinT = std::sqrt(t * ((R)2 - t));
-    R sinP = std::sin(phi), cosP = std::cos(phi);
+    R sinP, cosP;
+    sincos_r(phi, sinP, cosP);
     R a = cosP * sinT, bb = cosT, cz = sinP * sinT;
     dir = {(a * U.x + bb * W.x) + cz * Vv.x, (a * U.y + bb * W.y) + cz * Vv.y, (a * U.z + bb * W.z) + cz * Vv.z};
     R Omega = ((R)2 * Consts<R>::pi) * q;
@@ -556,8 +565,10 @@ void render_impl(const InstanceO* inst, int ninst, const Camera& cam_in, const L
                             R r, th;
                             if (std::fabs(ox) > std::fabs(oy)) { r = ox; th = (Consts<R>::pi / (R)4) * (oy / ox); }
                             else { r = oy; th = (Consts<R>::pi / (R)2) - (Consts<R>::pi / (R)4) * (ox / oy); }
-                            dx = r * std::cos(th);
-                            dy = r * std::sin(th);
+                            R sn, cs;
+                            sincos_r(th, sn, cs);
+                            dx = r * cs;
+                            dy = r * sn;
                         }
                         R lz = std::sqrt(std::max((R)0, ((R)1 - dx * dx) - dy * dy));
                         R mpdf = lz * Consts<R>::inv_pi;

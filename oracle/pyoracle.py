@@ -104,6 +104,7 @@ def lib() -> C.CDLL:
                                     C.POINTER(Material), P, C.c_int]),
         "orc_di_adjoint64": (None, [P, P, P, P, C.c_size_t, C.c_int, C.POINTER(Light), C.c_int,
                                     C.POINTER(Material), P, P, P]),
+        "orc_sincos_f32": (None, [P, C.c_size_t, P, P]),
         "orc_di_fd64": (None, [P, P, P, P, C.c_size_t, C.c_int, C.POINTER(Light), C.c_int, C.POINTER(Material),
                                P, P, C.c_size_t, C.c_double, P, C.c_int]),
     }
@@ -261,6 +262,15 @@ class Mesh:
                          "closest_internal": int(cnt[3]), "closest_tris": int(cnt[4]),
                          "any_internal": int(cnt[5]), "any_tris": int(cnt[6])}
         return out
+
+
+def sincos_f32(x):
+    """The kernels' FP32 sin/cos routine (vira_amd/csrc/fp_sincos.hpp) on an array."""
+    x = np.ascontiguousarray(x, np.float32)
+    s = np.empty_like(x)
+    c = np.empty_like(x)
+    lib().orc_sincos_f32(x.ctypes.data, x.size, s.ctypes.data, c.ctypes.data)
+    return s, c
 
 
 def offset_point(p, n):

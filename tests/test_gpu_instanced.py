@@ -3,10 +3,9 @@ FP64 model matrices about 1.5e8 km from the origin, rendered camera-relative.
 
 Bar: the oracle's FP32 twin (oracle render_scene, the same FP64 rebasing
 rounded to FP32) gives the same first-hit triangle and instance ids and, for
-a point sun, the same radiance bit for bit; with the sphere sun and bounces,
-ids exact and >= 97 % of lit pixel-bins within 1e-5 (libm ulps of sin/cos/acos
-differ between host and device).  The error against the FP64 oracle (every
-matrix and position kept in FP64) is stated."""
+a point or sphere sun, with or without bounces, the same radiance bit for bit
+(sampling angles go through the shared sin/cos of fp_sincos.hpp).  The error
+against the FP64 oracle (every matrix and position kept in FP64) is stated."""
 import numpy as np
 import pytest
 
@@ -81,11 +80,9 @@ def test_instanced_sphere_sun_bounces(gpu_ctx):
     tri = passes.triangle_id.cpu().numpy().reshape(-1).view(np.uint32)
     assert np.array_equal(tri, out["tri_id"])
     assert np.array_equal(passes.instance_id.cpu().numpy().reshape(-1).view(np.uint32), out["instance_id"])
-    rad = passes.total_radiance.cpu().numpy().reshape(-1, 8).astype(np.float64)
-    ref = out["radiance"].astype(np.float64)
-    lit = ref > 1e-6 * ref.max()
-    rel = np.abs(rad - ref)[lit] / ref[lit]
-    assert np.mean(rel <= 1e-5) >= 0.97, np.percentile(rel, [50, 97, 99])
+    rad = passes.total_radiance.cpu().numpy().reshape(-1, 8)
+    assert (out["radiance"].max(1) > 0).mean() > 0.1
+    assert np.array_equal(rad.view(np.uint32), out["radiance"].view(np.uint32))
 
 
 def test_tlas_validation(gpu_ctx):
@@ -124,10 +121,8 @@ def test_instanced_full_size_sample(gpu_ctx):
     tri = passes.triangle_id.cpu().numpy().reshape(-1).view(np.uint32)[pix]
     inst = passes.instance_id.cpu().numpy().reshape(-1).view(np.uint32)[pix]
     assert np.array_equal(tri, out["tri_id"][pix]) and np.array_equal(inst, out["instance_id"][pix])
-    rad = passes.total_radiance.cpu().numpy().reshape(-1, 8)[pix].astype(np.float64)
-    ref = out["radiance"][pix].astype(np.float64)
-    lit = ref > 1e-6 * ref.max()
-    rel = np.abs(rad - ref)[lit] / ref[lit]
-    print(f"instanced scene: {st['primary_rays'] + st['shadow_rays']} primary+shadow rays in {st['total_ms']:.1f} ms; "
-          f"sample radiance rel err median {np.median(rel):.2e}")
-    assert np.mean(rel <= 1e-5) >= 0.97
+    rad = passes.total_radiance.cpu().numpy().reshape(-1, 8)[pix]
+    ref = out["radiance"][pix]
+    print(f"instanced scene: {st['primary_rays'] + st['shadow_rays']} primary+shadow rays in {st['total_ms']:.1f} ms")
+    assert (ref.max(1) > 0).sum() > 100
+    assert np.array_equal(rad.view(np.uint32), ref.view(np.uint32))

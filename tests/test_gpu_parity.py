@@ -1,10 +1,10 @@
 """GPU parity against the CPU oracle, through the C-ABI.
 
-Bar: bit-exact for integer/index work (triangle ids, tessellation) and for the
-FP32 twin wherever no transcendental function is involved (positions, normals,
-hit distances, barycentrics, direct-light radiance of a point light);
-<= 1e-5 relative per bin where sin/cos/acos enter (sphere-light and bounce
-sampling use the device and host libm, which may differ in the last ulp).
+Bar: bit-exact against the oracle's FP32 twin for integer/index work (triangle
+ids, tessellation) and for every floating-point pass: positions, normals, hit
+distances, barycentrics and radiance, including sphere-light and bounce
+sampling (both sides take sin/cos from the one routine in
+vira_amd/csrc/fp_sincos.hpp, so no libm ulp separates them).
 """
 import numpy as np
 import pytest
@@ -166,7 +166,11 @@ def test_render_dem257_parity(gpu_ctx, integrator, width):
     ref = o64["radiance"][hit & same]
     err = np.abs(rad[hit & same] - ref) / np.maximum(np.abs(ref), 1e-30)
     lit = ref > 0
-    assert np.median(err[lit]) < 1e-5
+    # measured on the CPU twin: 99.92 % of lit bins within 1e-5, 99.9th percentile 8.7e-6; the rest
+    # sit on shadow terminators, where cos(theta) -> 0 amplifies the FP32 rounding of the hit point
+    print(f"257^2 DEM vs FP64: {np.mean(err[lit] <= 1e-5):.4%} of lit bins within 1e-5, "
+          f"99.9th pct {np.quantile(err[lit], 0.999):.2e}")
+    assert np.mean(err[lit] <= 1e-5) >= 0.999 and np.quantile(err[lit], 0.999) <= 1e-5
 
 
 @pytest.mark.parametrize("integrator", ["fused", "wavefront"])
@@ -180,14 +184,16 @@ def test_render_bounces_sphere_light_small(gpu_ctx, integrator):
     o = om.render(cam, lights, mat, opt, precision=32)
     tri = rp.triangle_id.cpu().numpy().reshape(-1).view(np.uint32)
     assert np.array_equal(tri, o["tri_id"])  # sample 0 is deterministic: primary hits are exact
-    rad = rp.total_radiance.cpu().numpy().reshape(-1, 8).astype(np.float64)
-    ref = o["radiance"].astype(np.float64)
-    rel = np.abs(rad - ref) / np.maximum(np.abs(ref), 1e-30)
-    lit = ref > 0
-    assert lit.mean() > 0.3
-    assert np.mean(rel[lit] <= 1e-5) > 0.97
-    assert abs(rad.sum() / ref.sum() - 1) < 1e-3
-    assert st["bounce_rays"] > 0
+    rad = rp.total_radiance.cpu().numpy().reshape(-1, 8)
+    ref = o["radiance"]
+    assert (ref > 0).mean() > 0.3
+    assert np.array_equal(rad.view(np.uint32), ref.view(np.uint32))
+    for name, key in (("direct_radiance", "direct"), ("indirect_radiance", "indirect"), ("albedo", "albedo"),
+                      ("depth", "depth")):
+        x = getattr(rp, name).cpu().numpy().reshape(o[key].shape)
+        assert np.array_equal(x.view(np.uint32), o[key].view(np.uint32)), name
+    assert st["bounce_rays"] > 0 and st["bounce_rays"] == o["counts"]["bounce"]
+    assert st["shadow_rays"] == o["counts"]["shadow"]
 
 
 def test_render_two_point_lights_wavefront(gpu_ctx):
@@ -202,13 +208,10 @@ def test_render_two_point_lights_wavefront(gpu_ctx):
     om, cam, _, mat, opt = oracle_scene(built)
     o = om.render(cam, [O.light_from(L.to_c()) for L in lights], mat, opt)
     assert np.array_equal(rp.triangle_id.cpu().numpy().reshape(-1).view(np.uint32), o["tri_id"])
-    rad = rp.total_radiance.cpu().numpy().reshape(-1, 8).astype(np.float64)
+    rad = rp.total_radiance.cpu().numpy().reshape(-1, 8)
     dr = rp.direct_radiance.cpu().numpy().reshape(-1, 8)
-    ref = o["radiance"].astype(np.float64)
-    lit = ref > 0
-    rel = np.abs(rad - ref) / np.maximum(ref, 1e-30)
-    assert np.mean(rel[lit] <= 1e-5) > 0.97
-    # bounce-0 light from point lights has no transcendental step: exact
+    assert (o["radiance"] > 0).mean() > 0.2
+    assert np.array_equal(rad.view(np.uint32), o["radiance"].view(np.uint32))
     assert np.array_equal(dr.view(np.uint32), o["direct"].view(np.uint32))
     from vira_amd import ViraGPUError
     built.options.integrator = "fused"

@@ -13,6 +13,7 @@
 #pragma once
 
 #include "device_math.hpp"
+#include "fp_sincos.hpp"
 #include "../../include/vira_gpu.h"
 
 namespace vg {
@@ -150,7 +151,8 @@ __device__ __forceinline__ void sample_light(const LightF& L, f3 p, uint32_t see
     float q = __fdiv_rn(x, 1.0f + __builtin_sqrtf(1.0f - x));  // 1 - cos(theta_max)
     float t = zeta * q;                                          // 1 - cos(theta)
     float cosT = 1.0f - t, sinT = __builtin_sqrtf(t * (2.0f - t));
-    float sinP = sinf(phi), cosP = cosf(phi);
+    float sinP, cosP;
+    vgm::sincos_f32(phi, sinP, cosP);
     float a = cosP * sinT, bb = cosT, cz = sinP * sinT;
     dir = mk3((a * U.x + bb * W.x) + cz * Vv.x, (a * U.y + bb * W.y) + cz * Vv.y, (a * U.z + bb * W.z) + cz * Vv.z);
     float Omega = (2.0f * kPi) * q;  // 2 pi (1 - cos(theta_max))
@@ -298,8 +300,10 @@ __device__ __forceinline__ bool sample_bounce(const Surface& S, const Lights& li
             r = oy;
             th = (kPi / 2.0f) - (kPi / 4.0f) * __fdiv_rn(ox, oy);
         }
-        dx = r * cosf(th);
-        dy = r * sinf(th);
+        float sn, cs;
+        vgm::sincos_f32(th, sn, cs);
+        dx = r * cs;
+        dy = r * sn;
     }
     float lz = __builtin_sqrtf(fmaxf(0.0f, (1.0f - dx * dx) - dy * dy));
     float mpdf = lz * kInvPi;
