@@ -276,6 +276,8 @@ SIGNATURES = [
     ("vira_gpu_lights_set", _S, [_P, C.POINTER(Light), C.c_int]),
     ("vira_gpu_material_set", _S, [_P, C.c_int32, C.POINTER(Material)]),
     ("vira_gpu_render", _S, [_P, C.POINTER(Camera), C.POINTER(RenderOpts), C.POINTER(Outputs), C.POINTER(RenderStats)]),
+    ("vira_gpu_render_noalloc", _S, [_P, C.POINTER(Camera), C.POINTER(RenderOpts), C.POINTER(Outputs),
+                                     C.POINTER(RenderStats)]),
     ("vira_gpu_trace_rays", _S, [_P, C.c_int32, _P, C.c_size_t, C.c_int, _P, _P, _P, _P, C.POINTER(C.c_double)]),
     ("vira_gpu_dem_update", _S, [_P, C.c_int32, _P]),
     ("vira_gpu_dem_update_refit", _S, [_P, C.c_int32, _P]),

@@ -67,6 +67,10 @@ vira_gpu_status vira_gpu_create(vira_gpu_ctx** out, const vira_gpu_opts* opts) {
         VG_HIP(hipGetDeviceCount(&n));
         VG_CHECK(c->device >= 0 && c->device < n, VIRA_GPU_ERR_INVALID, "invalid device ordinal");
         VG_HIP(hipSetDevice(c->device));
+        // the allocation-free render's scratch address on this device and its timing events
+        vg::noalloc_init(c->device);
+        VG_HIP(hipEventCreate(&c->na_ev[0]));
+        VG_HIP(hipEventCreate(&c->na_ev[1]));
         if (opts && opts->stream) {
             c->stream = static_cast<hipStream_t>(opts->stream);
         } else {
@@ -323,6 +327,17 @@ vira_gpu_status vira_gpu_render(vira_gpu_ctx* ctx, const vira_gpu_camera* cam, c
         set_device(c);
         VG_CHECK(cam && opts && out, VIRA_GPU_ERR_INVALID, "render: null argument");
         vg::render(c, *cam, *opts, *out, stats);
+    });
+}
+
+vira_gpu_status vira_gpu_render_noalloc(vira_gpu_ctx* ctx, const vira_gpu_camera* cam,
+                                        const vira_gpu_render_opts* opts, const vira_gpu_outputs* out,
+                                        vira_gpu_render_stats* stats) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        set_device(c);
+        VG_CHECK(cam && opts && out, VIRA_GPU_ERR_INVALID, "render_noalloc: null argument");
+        vg::render_noalloc(c, *cam, *opts, *out, stats);
     });
 }
 

@@ -109,6 +109,26 @@ struct PinnedBuf {
     template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
+// Two HIP events owned for the duration of one call (released on every path).
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    EventPair() {
+        VG_HIP(hipEventCreate(&a));
+        hipError_t e = hipEventCreate(&b);
+        if (e != hipSuccess) {
+            (void)hipEventDestroy(a);
+            a = nullptr;
+            VG_HIP(e);
+        }
+    }
+    EventPair(const EventPair&) = delete;
+    EventPair& operator=(const EventPair&) = delete;
+    ~EventPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
 struct Blas {
     bool built = false;
     int width = 2;
@@ -149,7 +169,8 @@ struct Ctx {
     // render scratch, grown on demand
     DBuf r_ray_o, r_ray_d, r_hit, r_queue[2], r_beta, r_path_rad, r_path_dir, r_srays_o, r_srays_d,
         r_scontrib, r_svis, r_sinfo, r_acc, r_counters, r_lights, r_inst, r_inst_m, r_light_pos;
-    hipEvent_t ev[8] = {};
+    // timing events of vira_gpu_render_noalloc, created with the context
+    hipEvent_t na_ev[2] = {};
     // bumped by every call that changes geometry, BVHs, lights or materials
     uint64_t scene_gen = 1;
     // fixed-visibility direct illumination: hit records of the last forward pass
@@ -184,6 +205,8 @@ struct Ctx {
     bool fc_kvalid = false, fc_attrs = false;
     ~Ctx() {
         if (b_exec) (void)hipGraphExecDestroy(b_exec);
+        for (hipEvent_t& e : na_ev)
+            if (e) (void)hipEventDestroy(e);
     }
 
     MeshG& mesh(int32_t id) {
@@ -245,6 +268,13 @@ void trace_rays(Ctx& c, MeshG& m, const float* rays, size_t n, int any_hit, floa
                 float* b2, double* ms);
 void render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, const vira_gpu_outputs& out,
             vira_gpu_render_stats* st);
+// vira_gpu_render_noalloc: the per-pixel render with every scratch buffer in
+// the library's fixed per-device array; noalloc_init finds that array's
+// device address once per device (with the device current).
+void render_noalloc(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, const vira_gpu_outputs& out,
+                    vira_gpu_render_stats* st);
+void noalloc_init(int device);
+size_t noalloc_max_tiles();
 
 // Exclusive scan of n uint32 values (in place allowed); returns the total after
 // a stream synchronisation, or 0 without waiting when total_wanted is false.

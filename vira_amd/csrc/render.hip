@@ -18,6 +18,9 @@
 // Both fold samples into Kahan-compensated FP32 sums in sample order, so the
 // image does not depend on scheduling.
 #include <cmath>
+#include <memory>
+#include <mutex>
+#include <string>
 
 #include "context.hpp"
 #include "fused.hpp"
@@ -86,7 +89,7 @@ __global__ void __launch_bounds__(kFusedBS, kFusedWavesPerSimd) k_render_fused(T
 // FP64 from the FP64 model matrices and positions (one thread per item).
 __global__ void k_rebase(const double* __restrict__ M, int n, const double* __restrict__ lpos, int nl, double cx,
                          double cy, double cz, InstG* __restrict__ inst, LightF* __restrict__ lights) {
-    const int i = threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) rebase_instance(M + 16 * i, cx, cy, cz, inst[i]);
     if (i < nl) {
         lights[i].px = (float)(lpos[3 * i] - cx);
@@ -262,11 +265,6 @@ __global__ void k_finalize(uint32_t nq, uint32_t tiles_x, int W, int H, int spp,
     write_pixel(out, x + (size_t)y * W, spp, acc_tile(acc, q), q & 63u);
 }
 
-struct EvPair {
-    hipEvent_t a, b;
-    int kind;  // 0 closest, 1 shadow, 2 shade, 3 fused
-};
-
 }  // namespace
 
 void trace_rays(Ctx& c, MeshG& m, const float* rays, size_t n, int any_hit, float* t, uint32_t* id, float* b1,
@@ -276,10 +274,8 @@ void trace_rays(Ctx& c, MeshG& m, const float* rays, size_t n, int any_hit, floa
         if (ms) *ms = 0;
         return;
     }
-    hipEvent_t e0, e1;
-    VG_HIP(hipEventCreate(&e0));
-    VG_HIP(hipEventCreate(&e1));
-    VG_HIP(hipEventRecord(e0, c.stream));
+    EventPair ev;
+    VG_HIP(hipEventRecord(ev.a, c.stream));
     if (m.blas.width == 4)
         k_trace_rays<4><<<blocks_for(n, kTraceBS), kTraceBS, 0, c.stream>>>(m.blas.nodes.p, m.blas.tris.as<float4>(),
                                                                             rays, n, any_hit, t, id, b1, b2);
@@ -287,22 +283,64 @@ void trace_rays(Ctx& c, MeshG& m, const float* rays, size_t n, int any_hit, floa
         k_trace_rays<2><<<blocks_for(n, kTraceBS), kTraceBS, 0, c.stream>>>(m.blas.nodes.p, m.blas.tris.as<float4>(),
                                                                             rays, n, any_hit, t, id, b1, b2);
     VG_HIP(hipGetLastError());
-    VG_HIP(hipEventRecord(e1, c.stream));
-    VG_HIP(hipEventSynchronize(e1));
+    VG_HIP(hipEventRecord(ev.b, c.stream));
+    VG_HIP(hipEventSynchronize(ev.b));
     float f = 0.f;
-    VG_HIP(hipEventElapsedTime(&f, e0, e1));
+    VG_HIP(hipEventElapsedTime(&f, ev.a, ev.b));
     if (ms) *ms = f;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
 }
 
-void render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, const vira_gpu_outputs& out,
-            vira_gpu_render_stats* st) {
+namespace {
+
+// Scratch of one fused (per-pixel) render: the tile accumulator blocks, the
+// ray counters, the light table and, for instanced scenes, the instance table
+// with its FP64 inputs.  vira_gpu_render takes these from buffers the context
+// grows on demand; vira_gpu_render_noalloc from the file-scope array below.
+struct FusedScratch {
+    float* acc;                    // A_FIELDS x 64 floats per tile
+    unsigned long long* counters;  // kCounterWords
+    LightF* lights;                // VIRA_GPU_MAX_LIGHTS
+    InstG* inst;                   // ninst
+    double* inst_m;                // 16 per instance
+    double* light_pos;             // 3 per light
+};
+constexpr int kCounterWords = 64;
+
+__global__ void k_zero_u64(unsigned long long* __restrict__ p, int n) {
+    const int i = threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// ---------------------------------------------------------------- allocation-free render
+// vira_gpu_render_noalloc's scratch: one array per GPU, living as long as the
+// process.  Accumulator blocks of up to kNoAllocMaxTiles 8x8 tiles, then the
+// tables.  Host code never names the array: it uses the device address
+// hipGetSymbolAddress returns (kept per device by noalloc_init).
+constexpr uint32_t kNoAllocMaxTiles = 4096;                                     // e.g. 512 x 512 pixels
+constexpr size_t kNoAllocAccBytes = (size_t)kNoAllocMaxTiles * A_FIELDS * 64 * 4;  // 46 MiB
+constexpr int kNoAllocMaxInstances = 64;
+constexpr size_t kNoAllocCounterOff = kNoAllocAccBytes;
+constexpr size_t kNoAllocLightOff = kNoAllocCounterOff + kCounterWords * 8;
+constexpr size_t kNoAllocLightPosOff = kNoAllocLightOff + 1024;
+constexpr size_t kNoAllocInstOff = kNoAllocLightPosOff + 256;
+constexpr size_t kNoAllocInstMOff = kNoAllocInstOff + sizeof(InstG) * kNoAllocMaxInstances;
+constexpr size_t kNoAllocBytes = kNoAllocInstMOff + 16 * 8 * kNoAllocMaxInstances;
+static_assert(sizeof(LightF) * VIRA_GPU_MAX_LIGHTS <= 1024, "light table fits its slot");
+static_assert(3 * 8 * VIRA_GPU_MAX_LIGHTS <= 256, "light positions fit their slot");
+static_assert(kNoAllocInstOff % 16 == 0 && kNoAllocInstMOff % 16 == 0, "aligned tables");
+
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kMaxDevices] = {};
+
+// Validation shared by both render entry points; returns the light count.
+int render_checks(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o) {
     VG_CHECK(c.scene_mesh >= 0, VIRA_GPU_ERR_STATE, "render: no TLAS built");
     MeshG& m = c.mesh(c.scene_mesh);
     VG_CHECK(m.blas.built, VIRA_GPU_ERR_STATE, "render: BLAS not built");
-    const bool inst = c.instanced;
-    const int ninst = inst ? (int)c.tlas.size() : 0;
+    const int ninst = c.instanced ? (int)c.tlas.size() : 0;
     for (int i = 0; i < ninst; ++i) {
         const MeshG& mi = c.mesh(c.tlas[i].mesh_id);
         VG_CHECK(mi.blas.built && mi.blas.width == m.blas.width, VIRA_GPU_ERR_STATE,
@@ -313,90 +351,170 @@ void render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, c
     VG_CHECK(o.integrator >= 0 && o.integrator <= 2, VIRA_GPU_ERR_INVALID, "render: unknown integrator");
     const size_t npix = (size_t)cam.width * cam.height;
     VG_CHECK(npix < (1ull << 30), VIRA_GPU_ERR_INVALID, "render: image too large");
+    return (int)c.lights.size();
+}
+
+// The fused (per-pixel) render into scratch S: one wave per 8x8 tile.  The
+// caller has checked the request (render_checks) and that the fused schedule
+// applies.  Timing through ev (two events owned by the caller).
+void run_fused(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, const vira_gpu_outputs& out,
+               vira_gpu_render_stats* st, const FusedScratch& S, hipEvent_t ev0, hipEvent_t ev1) {
+    MeshG& m = c.mesh(c.scene_mesh);
+    const bool inst = c.instanced;
+    const int ninst = inst ? (int)c.tlas.size() : 0;
+    const int nl = (int)c.lights.size();
     const uint32_t tiles_x = (uint32_t)((cam.width + 7) / 8);
     const uint32_t tiles = tiles_x * (uint32_t)((cam.height + 7) / 8);
-    const uint32_t nq = tiles * 64u;
-    const int nl = (int)c.lights.size();
-    const bool fused = o.integrator == 1 || (o.integrator == 0 && nl <= 1);
-    VG_CHECK(!(fused && nl > 1), VIRA_GPU_ERR_UNSUPPORTED, "render: the fused integrator takes at most one light");
-    VG_CHECK(!inst || fused, VIRA_GPU_ERR_UNSUPPORTED,
-             "render: instanced scenes use the per-pixel integrator (at most one light)");
-
     CamF cf = cam_params(cam);
-    const Lights lf = upload_lights(c, c.r_lights);
+    {
+        LightF hl[VIRA_GPU_MAX_LIGHTS];
+        for (int l = 0; l < nl; ++l) hl[l] = light_params(c.lights[l]);
+        if (nl > 0)
+            VG_HIP(hipMemcpyAsync(S.lights, hl, sizeof(LightF) * nl, hipMemcpyHostToDevice, c.stream));
+    }
+    const Lights lf{nl, S.lights};
     const MatF mf = mat_params(m);
     if (inst) {
-        // instance table (pointers, materials) and FP64 inputs; k_rebase fills the FP32 placement
-        std::vector<InstG> hi(ninst);
-        std::vector<double> hm(16 * (size_t)ninst), hl(3 * (size_t)std::max(1, nl));
-        for (int i = 0; i < ninst; ++i) {
-            const MeshG& mi = c.mesh(c.tlas[i].mesh_id);
-            hi[i] = InstG{};
-            hi[i].nodes = mi.blas.nodes.p;
-            hi[i].tris = mi.blas.tris.as<float4>();
-            hi[i].verts = mi.verts.as<GVertex>();
-            hi[i].idx = mi.idx.as<uint32_t>();
-            hi[i].mat = mat_params(mi);
-            for (int k = 0; k < 3; ++k) {
-                hi[i].blo[k] = mi.blas.lo[k];
-                hi[i].bhi[k] = mi.blas.hi[k];
+        // instance table (pointers, materials) and FP64 inputs; k_rebase fills the FP32 placement.
+        // Staged in blocks of kStage instances so no host memory is allocated.
+        constexpr int kStage = 16;
+        InstG hi[kStage];
+        double hm[16 * kStage];
+        for (int i0 = 0; i0 < ninst; i0 += kStage) {
+            const int n = std::min(kStage, ninst - i0);
+            for (int i = 0; i < n; ++i) {
+                const MeshG& mi = c.mesh(c.tlas[i0 + i].mesh_id);
+                hi[i] = InstG{};
+                hi[i].nodes = mi.blas.nodes.p;
+                hi[i].tris = mi.blas.tris.as<float4>();
+                hi[i].verts = mi.verts.as<GVertex>();
+                hi[i].idx = mi.idx.as<uint32_t>();
+                hi[i].mat = mat_params(mi);
+                for (int k = 0; k < 3; ++k) {
+                    hi[i].blo[k] = mi.blas.lo[k];
+                    hi[i].bhi[k] = mi.blas.hi[k];
+                }
+                for (int k = 0; k < 16; ++k) hm[16 * i + k] = c.tlas[i0 + i].transform[k];
             }
-            for (int k = 0; k < 16; ++k) hm[16 * i + k] = c.tlas[i].transform[k];
+            VG_HIP(hipMemcpyAsync(S.inst + i0, hi, sizeof(InstG) * n, hipMemcpyHostToDevice, c.stream));
+            VG_HIP(hipMemcpyAsync(S.inst_m + 16 * (size_t)i0, hm, 16 * 8 * n, hipMemcpyHostToDevice, c.stream));
+            // pageable sources: wait before the staging arrays are refilled
+            if (i0 + kStage < ninst) VG_HIP(hipStreamSynchronize(c.stream));
         }
+        double hl[3 * VIRA_GPU_MAX_LIGHTS];
         for (int l = 0; l < nl; ++l)
             for (int k = 0; k < 3; ++k) hl[3 * l + k] = c.lights[l].position[k];
-        c.r_inst.alloc(sizeof(InstG) * ninst);
-        c.r_inst_m.alloc(hm.size() * 8);
-        c.r_light_pos.alloc(hl.size() * 8);
-        VG_HIP(hipMemcpyAsync(c.r_inst.p, hi.data(), sizeof(InstG) * ninst, hipMemcpyHostToDevice, c.stream));
-        VG_HIP(hipMemcpyAsync(c.r_inst_m.p, hm.data(), hm.size() * 8, hipMemcpyHostToDevice, c.stream));
-        VG_HIP(hipMemcpyAsync(c.r_light_pos.p, hl.data(), hl.size() * 8, hipMemcpyHostToDevice, c.stream));
-        k_rebase<<<1, 64, 0, c.stream>>>(c.r_inst_m.as<double>(), ninst, c.r_light_pos.as<double>(), nl,
-                                         cam.position[0], cam.position[1], cam.position[2], c.r_inst.as<InstG>(),
-                                         c.r_lights.as<LightF>());
+        if (nl > 0) VG_HIP(hipMemcpyAsync(S.light_pos, hl, 3 * 8 * nl, hipMemcpyHostToDevice, c.stream));
+        k_rebase<<<blocks_for(std::max(ninst, nl), 64), 64, 0, c.stream>>>(
+            S.inst_m, ninst, S.light_pos, nl, cam.position[0], cam.position[1], cam.position[2], S.inst, S.lights);
         VG_HIP(hipGetLastError());
         cf.ox = cf.oy = cf.oz = 0.0f;  // the camera is the origin of the render frame
     }
     OutPtrs op{out.radiance_dev, out.direct_dev, out.indirect_dev, out.albedo_dev, out.normal_dev,
                out.depth_dev, out.alpha_dev, out.tri_id_dev, out.instance_id_dev};
+    k_zero_u64<<<1, kCounterWords, 0, c.stream>>>(S.counters, kCounterWords);
+    VG_HIP(hipGetLastError());
+    VG_HIP(hipEventRecord(ev0, c.stream));
+    const bool w4 = m.blas.width == 4;
+    const TileJob J{m.blas.nodes.p, m.blas.tris.as<float4>(), m.verts.as<GVertex>(), m.idx.as<uint32_t>(), mf, lf, cf,
+                    o.seed, o.samples, o.bounces, o.russian_roulette, o.rr_start, tiles_x, op,
+                    inst ? S.inst : nullptr, ninst};
+    auto kern = inst ? (w4 ? k_render_fused<4, true> : k_render_fused<2, true>)
+                     : (w4 ? k_render_fused<4, false> : k_render_fused<2, false>);
+    kern<<<tiles, kFusedBS, 0, c.stream>>>(J, S.acc, S.counters);
+    VG_HIP(hipGetLastError());
+    VG_HIP(hipEventRecord(ev1, c.stream));
+    unsigned long long hc[4];
+    VG_HIP(hipMemcpyAsync(hc, S.counters, sizeof(hc), hipMemcpyDeviceToHost, c.stream));
+    VG_HIP(hipStreamSynchronize(c.stream));
+    if (st) {
+        *st = vira_gpu_render_stats{};
+        st->primary_rays = hc[0];
+        st->shadow_rays = hc[1];
+        st->bounce_rays = hc[2];
+        st->closest_hits = hc[3];
+        float f = 0.f;
+        VG_HIP(hipEventElapsedTime(&f, ev0, ev1));
+        st->total_ms = f;
+        st->fused_ms = f;
+        st->fused_launches = 1;
+        st->integrator = 1;
+    }
+}
+
+}  // namespace
+
+void noalloc_init(int device) {
+    VG_CHECK(device >= 0 && device < kMaxDevices, VIRA_GPU_ERR_INVALID, "noalloc_init: device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return;
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    VG_CHECK(e == hipSuccess && ptr != nullptr, VIRA_GPU_ERR_HIP,
+             std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_noalloc_ptr[device] = ptr;
+}
+
+size_t noalloc_max_tiles() { return kNoAllocMaxTiles; }
+
+void render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, const vira_gpu_outputs& out,
+            vira_gpu_render_stats* st) {
+    const int nl = render_checks(c, cam, o);
+    MeshG& m = c.mesh(c.scene_mesh);
+    const bool inst = c.instanced;
+    const int ninst = inst ? (int)c.tlas.size() : 0;
+    const uint32_t tiles_x = (uint32_t)((cam.width + 7) / 8);
+    const uint32_t tiles = tiles_x * (uint32_t)((cam.height + 7) / 8);
+    const uint32_t nq = tiles * 64u;
+    const bool fused = o.integrator == 1 || (o.integrator == 0 && nl <= 1);
+    VG_CHECK(!(fused && nl > 1), VIRA_GPU_ERR_UNSUPPORTED, "render: the fused integrator takes at most one light");
+    VG_CHECK(!inst || fused, VIRA_GPU_ERR_UNSUPPORTED,
+             "render: instanced scenes use the per-pixel integrator (at most one light)");
 
     c.r_acc.alloc((size_t)A_FIELDS * nq * 4);
-    c.r_counters.alloc(64 * 8);
+    c.r_counters.alloc(kCounterWords * 8);
+    c.r_lights.alloc(sizeof(LightF) * VIRA_GPU_MAX_LIGHTS);
+    if (fused) {
+        if (inst) {
+            c.r_inst.alloc(sizeof(InstG) * ninst);
+            c.r_inst_m.alloc((size_t)16 * 8 * ninst);
+            c.r_light_pos.alloc((size_t)3 * 8 * VIRA_GPU_MAX_LIGHTS);
+        }
+        const FusedScratch S{c.r_acc.as<float>(), c.r_counters.as<unsigned long long>(), c.r_lights.as<LightF>(),
+                             c.r_inst.as<InstG>(), c.r_inst_m.as<double>(), c.r_light_pos.as<double>()};
+        EventPair ev;
+        run_fused(c, cam, o, out, st, S, ev.a, ev.b);
+        return;
+    }
+
+    // ---- wavefront schedule
+    CamF cf = cam_params(cam);
+    const Lights lf = upload_lights(c, c.r_lights);
+    const MatF mf = mat_params(m);
+    OutPtrs op{out.radiance_dev, out.direct_dev, out.indirect_dev, out.albedo_dev, out.normal_dev,
+               out.depth_dev, out.alpha_dev, out.tri_id_dev, out.instance_id_dev};
     float* acc = c.r_acc.as<float>();
     unsigned long long* cnt = c.r_counters.as<unsigned long long>();
     uint32_t* qcount = reinterpret_cast<uint32_t*>(c.r_counters.as<char>() + 64);  // uint32 queue lengths per bounce
     const int maxq = 64;
-    VG_HIP(hipMemsetAsync(c.r_counters.p, 0, 64 * 8, c.stream));
+    k_zero_u64<<<1, kCounterWords, 0, c.stream>>>(cnt, kCounterWords);
+    VG_HIP(hipGetLastError());
     const void* nodes = m.blas.nodes.p;
     const bool w4 = m.blas.width == 4;
     const float4* tris = m.blas.tris.as<float4>();
 
-    std::vector<EvPair> evs;
+    // one event pair per launch, owned here and released on every path
+    std::vector<std::unique_ptr<EventPair>> evs;
+    std::vector<int> kinds;
     auto ev_begin = [&](int kind) {
-        EvPair p;
-        VG_HIP(hipEventCreate(&p.a));
-        VG_HIP(hipEventCreate(&p.b));
-        p.kind = kind;
-        VG_HIP(hipEventRecord(p.a, c.stream));
-        evs.push_back(p);
+        evs.push_back(std::make_unique<EventPair>());
+        kinds.push_back(kind);
+        VG_HIP(hipEventRecord(evs.back()->a, c.stream));
     };
-    auto ev_end = [&]() { VG_HIP(hipEventRecord(evs.back().b, c.stream)); };
-    hipEvent_t t0, t1;
-    VG_HIP(hipEventCreate(&t0));
-    VG_HIP(hipEventCreate(&t1));
-    VG_HIP(hipEventRecord(t0, c.stream));
-
-    if (fused) {
-        ev_begin(3);
-        const TileJob J{nodes, tris, m.verts.as<GVertex>(), m.idx.as<uint32_t>(), mf, lf, cf, o.seed, o.samples,
-                        o.bounces, o.russian_roulette, o.rr_start, tiles_x, op,
-                        inst ? c.r_inst.as<InstG>() : nullptr, ninst};
-        auto kern = inst ? (w4 ? k_render_fused<4, true> : k_render_fused<2, true>)
-                         : (w4 ? k_render_fused<4, false> : k_render_fused<2, false>);
-        kern<<<tiles, kFusedBS, 0, c.stream>>>(J, acc, cnt);
-        VG_HIP(hipGetLastError());
-        ev_end();
-    } else {
+    auto ev_end = [&]() { VG_HIP(hipEventRecord(evs.back()->b, c.stream)); };
+    EventPair total;
+    VG_HIP(hipEventRecord(total.a, c.stream));
+    {
         const uint32_t nslots = nq * (uint32_t)std::max(1, nl);
         c.r_ray_o.alloc((size_t)nq * 16 * 2);
         c.r_ray_d.alloc((size_t)nq * 16 * 2);
@@ -455,7 +573,7 @@ void render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, c
         k_finalize<<<blocks_for(nq, 256), 256, 0, c.stream>>>(nq, tiles_x, cam.width, cam.height, o.samples, acc, op);
         VG_HIP(hipGetLastError());
     }
-    VG_HIP(hipEventRecord(t1, c.stream));
+    VG_HIP(hipEventRecord(total.b, c.stream));
     unsigned long long hc[4];
     VG_HIP(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, c.stream));
     VG_HIP(hipStreamSynchronize(c.stream));
@@ -466,24 +584,43 @@ void render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, c
         st->bounce_rays = hc[2];
         st->closest_hits = hc[3];
         float f = 0.f;
-        VG_HIP(hipEventElapsedTime(&f, t0, t1));
+        VG_HIP(hipEventElapsedTime(&f, total.a, total.b));
         st->total_ms = f;
-        st->integrator = fused ? 1 : 2;
-        for (const EvPair& p : evs) {
+        st->integrator = 2;
+        for (size_t i = 0; i < evs.size(); ++i) {
             float g = 0.f;
-            VG_HIP(hipEventElapsedTime(&g, p.a, p.b));
-            if (p.kind == 0) { st->trace_closest_ms += g; st->trace_closest_launches++; }
-            else if (p.kind == 1) { st->trace_shadow_ms += g; st->trace_shadow_launches++; }
-            else if (p.kind == 2) st->shade_ms += g;
-            else { st->fused_ms += g; st->fused_launches++; }
+            VG_HIP(hipEventElapsedTime(&g, evs[i]->a, evs[i]->b));
+            if (kinds[i] == 0) { st->trace_closest_ms += g; st->trace_closest_launches++; }
+            else if (kinds[i] == 1) { st->trace_shadow_ms += g; st->trace_shadow_launches++; }
+            else st->shade_ms += g;
         }
     }
-    for (const EvPair& p : evs) {
-        (void)hipEventDestroy(p.a);
-        (void)hipEventDestroy(p.b);
+}
+
+void render_noalloc(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, const vira_gpu_outputs& out,
+                    vira_gpu_render_stats* st) {
+    const int nl = render_checks(c, cam, o);
+    VG_CHECK(o.integrator != 2, VIRA_GPU_ERR_UNSUPPORTED,
+             "render_noalloc: only the per-pixel (fused) integrator runs without allocation");
+    VG_CHECK(nl <= 1, VIRA_GPU_ERR_UNSUPPORTED, "render_noalloc: the per-pixel integrator takes at most one light");
+    const uint32_t tiles = (uint32_t)((cam.width + 7) / 8) * (uint32_t)((cam.height + 7) / 8);
+    VG_CHECK(tiles <= kNoAllocMaxTiles, VIRA_GPU_ERR_INVALID,
+             "render_noalloc: " + std::to_string(tiles) + " 8x8 tiles exceed the scratch maximum of " +
+                 std::to_string(kNoAllocMaxTiles) + " (e.g. 512 x 512 pixels)");
+    const int ninst = c.instanced ? (int)c.tlas.size() : 0;
+    VG_CHECK(ninst <= kNoAllocMaxInstances, VIRA_GPU_ERR_INVALID, "render_noalloc: too many instances");
+    VG_CHECK(c.device >= 0 && c.device < kMaxDevices, VIRA_GPU_ERR_STATE, "render_noalloc: device out of range");
+    unsigned char* base;
+    {
+        std::lock_guard<std::mutex> lk(g_noalloc_mu);
+        base = static_cast<unsigned char*>(g_noalloc_ptr[c.device]);
     }
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
+    VG_CHECK(base != nullptr, VIRA_GPU_ERR_STATE, "render_noalloc: scratch not set up on this device");
+    const FusedScratch S{reinterpret_cast<float*>(base), reinterpret_cast<unsigned long long*>(base + kNoAllocCounterOff),
+                         reinterpret_cast<LightF*>(base + kNoAllocLightOff), reinterpret_cast<InstG*>(base + kNoAllocInstOff),
+                         reinterpret_cast<double*>(base + kNoAllocInstMOff),
+                         reinterpret_cast<double*>(base + kNoAllocLightPosOff)};
+    run_fused(c, cam, o, out, st, S, c.na_ev[0], c.na_ev[1]);
 }
 
 }  // namespace vg

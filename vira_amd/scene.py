@@ -451,6 +451,18 @@ class Context:
             self.denoise(passes, options.denoiser)
         return self.last_stats
 
+    def render_noalloc(self, camera: Camera, options: PathTracerOptions, passes: RenderPasses):
+        """vira_gpu_render_noalloc: the per-pixel render without any device
+        allocation (scratch in the library's fixed per-GPU array; at most 4096
+        8x8 tiles, one light).  Same passes as render(); no denoise step."""
+        st = capi.RenderStats()
+        cam = camera.to_c()
+        opt = options.to_c()
+        out = passes.to_c()
+        self._call(self.lib.vira_gpu_render_noalloc, self.h, C.byref(cam), C.byref(opt), C.byref(out), C.byref(st))
+        self.last_stats = {f: getattr(st, f) for f, _ in capi.RenderStats._fields_}
+        return self.last_stats
+
     def denoise(self, passes: RenderPasses, options: "DenoiseOptions" = None):
         """denoiseSpectralRadianceEATWT on the passes' direct and indirect radiance (in
         place), then total = direct + indirect when the passes hold a total."""
