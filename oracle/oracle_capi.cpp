@@ -325,6 +325,22 @@ void orc_di_fd64(void* d, const double* h, const double* tex, const uint32_t* re
     di_fd_heights64(di_scene(d, rec, npix, spp, lights, nl, mat), h, tex, target, nodes, n, step, out, nthreads);
 }
 
+void orc_light_sample(int precision, const Light* L, const double* p, uint32_t seed, uint32_t pix, uint32_t dim0,
+                      size_t n, double* dirs, double* dist, double* pdf, double* rad) {
+    hook_light_sample(precision, *L, p, seed, pix, dim0, n, dirs, dist, pdf, rad);
+}
+void orc_light_pdf(int precision, const Light* L, const double* p, const double* dirs, size_t n, double* out) {
+    hook_light_pdf(precision, *L, p, dirs, n, out);
+}
+void orc_bounce_dirs(int precision, const double* N, uint32_t seed, uint32_t pix, uint32_t bounce, size_t n,
+                     double* dirs, double* pdf) {
+    hook_bounce_dirs(precision, N, seed, pix, bounce, n, dirs, pdf);
+}
+void orc_tex_bilinear(int precision, const Material* m, const double* uv, size_t n, double* out) {
+    hook_tex_bilinear(precision, *m, uv, n, out);
+}
+float orc_kahan_sum_f32(const float* x, size_t n) { return hook_kahan_sum_f32(x, n); }
+
 // sin and cos of the kernels' FP32 routine (fp_sincos.hpp), for the tests that pin it.
 void orc_sincos_f32(const float* x, size_t n, float* s, float* c) {
     for (size_t i = 0; i < n; ++i) vgm::sincos_f32(x[i], s[i], c[i]);

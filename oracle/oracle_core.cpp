@@ -330,6 +330,44 @@ inline R light_pdf(const Light& L, const V3<R>& p, const V3<R>& dir) {
     return (R)1 / Omega;
 }
 
+// Shading frame of processIntersection (cpu_path_tracer.ipp:313-324).
+template <class R>
+inline void tangent_frame(const V3<R>& N, V3<R>& tg, V3<R>& bt) {
+    V3<R> arb = (std::fabs(N.x) < (R)0.9) ? V3<R>{1, 0, 0} : V3<R>{0, 1, 0};
+    tg = normalize(cross(arb, N));
+    bt = cross(N, tg);
+}
+
+// Material::sampleDirection for the Lambertian's COSINE_WEIGHTED strategy
+// (material.ipp:8-33, material_sampling.hpp:15-21, sampling.ipp:53-84):
+// Shirley-Chiu concentric map of (u1, u2) to the disk, lifted to the
+// hemisphere, into the frame [tg, bt, N]; pdf = cos(theta) / pi.  Returns
+// false where the pdf is 0 (the path then ends).
+template <class R>
+inline bool cosine_sample(uint32_t seed, uint32_t pix, uint32_t s, uint32_t bounce, const V3<R>& N, const V3<R>& tg,
+                          const V3<R>& bt, V3<R>& nd, R& mpdf) {
+    const uint32_t db = bounce << 8;
+    R u1 = (R)rng_u01(seed, pix, s, db | SLOT_BSDF_U);
+    R u2 = (R)rng_u01(seed, pix, s, db | SLOT_BSDF_V);
+    R ox = (R)2 * u1 - (R)1, oy = (R)2 * u2 - (R)1;
+    R dx = 0, dy = 0;
+    if (!(std::fabs(ox) < (R)1e-6 && std::fabs(oy) < (R)1e-6)) {
+        R r, th;
+        if (std::fabs(ox) > std::fabs(oy)) { r = ox; th = (Consts<R>::pi / (R)4) * (oy / ox); }
+        else { r = oy; th = (Consts<R>::pi / (R)2) - (Consts<R>::pi / (R)4) * (ox / oy); }
+        R sn, cs;
+        sincos_r(th, sn, cs);
+        dx = r * cs;
+        dy = r * sn;
+    }
+    R lz = std::sqrt(std::max((R)0, ((R)1 - dx * dx) - dy * dy));
+    mpdf = lz * Consts<R>::inv_pi;
+    if (!(mpdf > (R)0)) return false;
+    nd = V3<R>{(tg.x * dx + bt.x * dy) + N.x * lz, (tg.y * dx + bt.y * dy) + N.y * lz,
+               (tg.z * dx + bt.z * dy) + N.z * lz};
+    return true;
+}
+
 template <class R>
 struct PixelAcc {
     Kahan<R> rad[NB], dir[NB];
@@ -497,9 +535,8 @@ void render_impl(const InstanceO* inst, int ninst, const Camera& cam_in, const L
                         // instances: normal matrix on the normals (not renormalised, cpu_path_tracer.ipp:310)
                         const V3<R> N = rebased ? xform_normal(place[hinst].nm, Nl) : Nl;
                         const V3<R> fng = rebased ? xform_normal(place[hinst].nm, fn) : fn;
-                        V3<R> arb = (std::fabs(N.x) < (R)0.9) ? V3<R>{1, 0, 0} : V3<R>{0, 1, 0};
-                        V3<R> tg = normalize(cross(arb, N));
-                        V3<R> bt = cross(N, tg);
+                        V3<R> tg, bt;
+                        tangent_frame(N, tg, bt);
                         R texv = 1;
                         if (mat.tex) texv = tex_bilinear<R>(mat, uvu, uvv);
                         R alb[NB];
@@ -557,24 +594,10 @@ void render_impl(const InstanceO* inst, int ninst, const Camera& cam_in, const L
                         if (bounce >= opt.bounces) break;
                         // ---- BSDF sampling (material.ipp:8-33, material_sampling.hpp:15-30, sampling.ipp:53-84)
                         uint32_t db = (uint32_t)bounce << 8;
-                        R u1 = (R)rng_u01(opt.seed, (uint32_t)pix, (uint32_t)s, db | SLOT_BSDF_U);
-                        R u2 = (R)rng_u01(opt.seed, (uint32_t)pix, (uint32_t)s, db | SLOT_BSDF_V);
-                        R ox = (R)2 * u1 - (R)1, oy = (R)2 * u2 - (R)1;
-                        R dx = 0, dy = 0;
-                        if (!(std::fabs(ox) < (R)1e-6 && std::fabs(oy) < (R)1e-6)) {
-                            R r, th;
-                            if (std::fabs(ox) > std::fabs(oy)) { r = ox; th = (Consts<R>::pi / (R)4) * (oy / ox); }
-                            else { r = oy; th = (Consts<R>::pi / (R)2) - (Consts<R>::pi / (R)4) * (ox / oy); }
-                            R sn, cs;
-                            sincos_r(th, sn, cs);
-                            dx = r * cs;
-                            dy = r * sn;
-                        }
-                        R lz = std::sqrt(std::max((R)0, ((R)1 - dx * dx) - dy * dy));
-                        R mpdf = lz * Consts<R>::inv_pi;
-                        if (!(mpdf > (R)0)) break;
-                        V3<R> nd{(tg.x * dx + bt.x * dy) + N.x * lz, (tg.y * dx + bt.y * dy) + N.y * lz,
-                                 (tg.z * dx + bt.z * dy) + N.z * lz};
+                        V3<R> nd;
+                        R mpdf;
+                        if (!cosine_sample(opt.seed, (uint32_t)pix, (uint32_t)s, (uint32_t)bounce, N, tg, bt, nd, mpdf))
+                            break;
                         R ndd = dot(N, nd);
                         R mx = std::max(ndd, (R)0);
                         R cth = std::max((R)0, dot(nd, N));
@@ -930,6 +953,73 @@ void render_scene(int precision, const InstanceO* inst, int n, const Camera& cam
                   const Options& opt, Outputs& out, int nthreads, const int* pixel_list, size_t npix) {
     if (precision == 64) render_impl<double>(inst, n, cam, lights, nl, opt, out, nthreads, pixel_list, npix);
     else render_impl<float>(inst, n, cam, lights, nl, opt, out, nthreads, pixel_list, npix);
+}
+
+// ---------------------------------------------------------------- single-function hooks
+// The path's sampling functions one at a time, for the tests' closed-form and
+// distribution pins (the renders call the very same templates).
+namespace {
+template <class R>
+void light_sample_n(const Light& L, const double p[3], uint32_t seed, uint32_t pix, uint32_t dim0, size_t n,
+                    double* dirs, double* dist, double* pdf, double* rad) {
+    const V3<R> P{(R)p[0], (R)p[1], (R)p[2]};
+    for (size_t i = 0; i < n; ++i) {
+        V3<R> d;
+        R ds, pd, rd[NB];
+        sample_light<R>(L, P, seed, pix, (uint32_t)i, dim0, d, ds, pd, rd);
+        dirs[3 * i] = d.x; dirs[3 * i + 1] = d.y; dirs[3 * i + 2] = d.z;
+        dist[i] = ds;
+        pdf[i] = pd;
+        for (int k = 0; k < NB; ++k) rad[NB * i + k] = rd[k];
+    }
+}
+template <class R>
+void light_pdf_n(const Light& L, const double p[3], const double* dirs, size_t n, double* out) {
+    const V3<R> P{(R)p[0], (R)p[1], (R)p[2]};
+    for (size_t i = 0; i < n; ++i) out[i] = light_pdf<R>(L, P, V3<R>{(R)dirs[3 * i], (R)dirs[3 * i + 1], (R)dirs[3 * i + 2]});
+}
+template <class R>
+void bounce_dirs_n(const double N_[3], uint32_t seed, uint32_t pix, uint32_t bounce, size_t n, double* dirs,
+                   double* pdf) {
+    const V3<R> N{(R)N_[0], (R)N_[1], (R)N_[2]};
+    V3<R> tg, bt;
+    tangent_frame(N, tg, bt);
+    for (size_t i = 0; i < n; ++i) {
+        V3<R> d{0, 0, 0};
+        R mp = 0;
+        if (!cosine_sample(seed, pix, (uint32_t)i, bounce, N, tg, bt, d, mp)) mp = 0;
+        dirs[3 * i] = d.x; dirs[3 * i + 1] = d.y; dirs[3 * i + 2] = d.z;
+        pdf[i] = mp;
+    }
+}
+template <class R>
+void tex_n(const Material& m, const double* uv, size_t n, double* out) {
+    for (size_t i = 0; i < n; ++i) out[i] = tex_bilinear<R>(m, (R)uv[2 * i], (R)uv[2 * i + 1]);
+}
+}  // namespace
+
+void hook_light_sample(int precision, const Light& L, const double p[3], uint32_t seed, uint32_t pix, uint32_t dim0,
+                       size_t n, double* dirs, double* dist, double* pdf, double* rad) {
+    if (precision == 64) light_sample_n<double>(L, p, seed, pix, dim0, n, dirs, dist, pdf, rad);
+    else light_sample_n<float>(L, p, seed, pix, dim0, n, dirs, dist, pdf, rad);
+}
+void hook_light_pdf(int precision, const Light& L, const double p[3], const double* dirs, size_t n, double* out) {
+    if (precision == 64) light_pdf_n<double>(L, p, dirs, n, out);
+    else light_pdf_n<float>(L, p, dirs, n, out);
+}
+void hook_bounce_dirs(int precision, const double N[3], uint32_t seed, uint32_t pix, uint32_t bounce, size_t n,
+                      double* dirs, double* pdf) {
+    if (precision == 64) bounce_dirs_n<double>(N, seed, pix, bounce, n, dirs, pdf);
+    else bounce_dirs_n<float>(N, seed, pix, bounce, n, dirs, pdf);
+}
+void hook_tex_bilinear(int precision, const Material& m, const double* uv, size_t n, double* out) {
+    if (precision == 64) tex_n<double>(m, uv, n, out);
+    else tex_n<float>(m, uv, n, out);
+}
+float hook_kahan_sum_f32(const float* x, size_t n) {
+    Kahan<float> k;
+    for (size_t i = 0; i < n; ++i) k.add(x[i]);
+    return k.s;
 }
 
 }  // namespace vo

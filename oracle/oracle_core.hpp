@@ -619,4 +619,17 @@ void denoise_eatwt(const DenoiseO& o, int W, int H, float* direct, float* indire
 void render_scene(int precision, const InstanceO* inst, int n, const Camera& cam, const Light* lights, int nl,
                   const Options& opt, Outputs& out, int nthreads, const int* pixel_list, size_t npix);
 
+// Single-function hooks for the tests' pins (precision 32 = FP32 twin, 64 = FP64):
+// Light::sample for samples s = 0..n-1 at point p (RNG dimensions dim0, dim0+1);
+// Light::getPDF of n directions; the cosine-weighted bounce direction and pdf in the
+// shading frame of normal N for samples 0..n-1; the bilinear albedo lookup at n uvs;
+// the Kahan sum the pixel accumulators use.
+void hook_light_sample(int precision, const Light& L, const double p[3], uint32_t seed, uint32_t pix, uint32_t dim0,
+                       size_t n, double* dirs, double* dist, double* pdf, double* rad);
+void hook_light_pdf(int precision, const Light& L, const double p[3], const double* dirs, size_t n, double* out);
+void hook_bounce_dirs(int precision, const double N[3], uint32_t seed, uint32_t pix, uint32_t bounce, size_t n,
+                      double* dirs, double* pdf);
+void hook_tex_bilinear(int precision, const Material& m, const double* uv, size_t n, double* out);
+float hook_kahan_sum_f32(const float* x, size_t n);
+
 }  // namespace vo

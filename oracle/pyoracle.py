@@ -105,6 +105,12 @@ def lib() -> C.CDLL:
         "orc_di_adjoint64": (None, [P, P, P, P, C.c_size_t, C.c_int, C.POINTER(Light), C.c_int,
                                     C.POINTER(Material), P, P, P]),
         "orc_sincos_f32": (None, [P, C.c_size_t, P, P]),
+        "orc_light_sample": (None, [C.c_int, C.POINTER(Light), P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_size_t,
+                                    P, P, P, P]),
+        "orc_light_pdf": (None, [C.c_int, C.POINTER(Light), P, P, C.c_size_t, P]),
+        "orc_bounce_dirs": (None, [C.c_int, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_size_t, P, P]),
+        "orc_tex_bilinear": (None, [C.c_int, C.POINTER(Material), P, C.c_size_t, P]),
+        "orc_kahan_sum_f32": (C.c_float, [P, C.c_size_t]),
         "orc_di_fd64": (None, [P, P, P, P, C.c_size_t, C.c_int, C.POINTER(Light), C.c_int, C.POINTER(Material),
                                P, P, C.c_size_t, C.c_double, P, C.c_int]),
     }
@@ -271,6 +277,45 @@ def sincos_f32(x):
     c = np.empty_like(x)
     lib().orc_sincos_f32(x.ctypes.data, x.size, s.ctypes.data, c.ctypes.data)
     return s, c
+
+
+def light_sample(L: Light, p, n: int, seed=0, pix=0, dim0=16, precision=64):
+    """Light::sample at point p for samples 0..n-1: (dirs (n,3), distance (n), pdf (n), radiance (n,8))."""
+    pp = np.ascontiguousarray(p, np.float64)
+    dirs = np.empty((n, 3)); dist = np.empty(n); pdf = np.empty(n); rad = np.empty((n, NB))
+    lib().orc_light_sample(precision, C.byref(L), pp.ctypes.data, seed, pix, dim0, n, dirs.ctypes.data,
+                           dist.ctypes.data, pdf.ctypes.data, rad.ctypes.data)
+    return dirs, dist, pdf, rad
+
+
+def light_pdf(L: Light, p, dirs, precision=64):
+    pp = np.ascontiguousarray(p, np.float64)
+    d = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+    out = np.empty(d.shape[0])
+    lib().orc_light_pdf(precision, C.byref(L), pp.ctypes.data, d.ctypes.data, d.shape[0], out.ctypes.data)
+    return out
+
+
+def bounce_dirs(N, n: int, seed=0, pix=0, bounce=0, precision=64):
+    """Cosine-weighted bounce directions (n,3) and pdfs (n) around normal N, samples 0..n-1."""
+    NN = np.ascontiguousarray(N, np.float64)
+    dirs = np.empty((n, 3)); pdf = np.empty(n)
+    lib().orc_bounce_dirs(precision, NN.ctypes.data, seed, pix, bounce, n, dirs.ctypes.data, pdf.ctypes.data)
+    return dirs, pdf
+
+
+def tex_bilinear(texture: np.ndarray, uv, precision=32):
+    """Image::sampleUVs of a grey texture (H, W) at uv (n, 2)."""
+    m = material(1.0, texture)
+    u = np.ascontiguousarray(uv, np.float64).reshape(-1, 2)
+    out = np.empty(u.shape[0])
+    lib().orc_tex_bilinear(precision, C.byref(m), u.ctypes.data, u.shape[0], out.ctypes.data)
+    return out
+
+
+def kahan_sum_f32(x) -> float:
+    x = np.ascontiguousarray(x, np.float32)
+    return float(lib().orc_kahan_sum_f32(x.ctypes.data, x.size))
 
 
 def offset_point(p, n):
