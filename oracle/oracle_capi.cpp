@@ -6,7 +6,7 @@
 
 #include "oracle_core.hpp"
 #include "oracle_diff.hpp"
-#include "../vira_amd/csrc/fp_sincos.hpp"
+#include "../vira_amd/csrc/fp_trig.hpp"
 
 using namespace vo;
 
@@ -227,15 +227,19 @@ float orc_rng_u01(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t dim) 
 }
 
 // precision: 32 = FP32 twin, 64 = FP64 reference.
+void orc_mesh_set_id(void* h, uint32_t id) { static_cast<OMesh*>(h)->m.id = id; }
+
 void orc_render(void* h, int precision, const Camera* cam, const Light* lights, int nl, const Material* mat,
                 const Options* opt, float* radiance, float* direct, float* indirect, float* albedo,
                 float* normal, float* depth, float* alpha, uint32_t* tri_id, uint64_t* counts,
-                uint32_t* records, int nthreads, const int* pixel_list, size_t npix) {
+                uint32_t* records, int nthreads, const int* pixel_list, size_t npix, float* normal_camera,
+                uint32_t* mesh_id, uint32_t* material_id) {
     auto* o = static_cast<OMesh*>(h);
     Outputs out;
     out.radiance = radiance; out.direct = direct; out.indirect = indirect; out.albedo = albedo;
     out.normal = normal; out.depth = depth; out.alpha = alpha; out.tri_id = tri_id; out.counts = counts;
     out.records = records;
+    out.normal_camera = normal_camera; out.mesh_id = mesh_id; out.material_id = material_id;
     if (precision == 64) render_f64(o->m, o->b, *cam, lights, nl, *mat, *opt, out, nthreads, pixel_list, npix);
     else render_f32(o->m, o->b, *cam, lights, nl, *mat, *opt, out, nthreads, pixel_list, npix);
 }
@@ -246,7 +250,7 @@ void orc_render_scene(void* const* meshes, const double* transforms, const Mater
                       const Camera* cam, const Light* lights, int nl, const Options* opt, float* radiance,
                       float* direct, float* indirect, float* albedo, float* normal, float* depth, float* alpha,
                       uint32_t* tri_id, uint32_t* instance_id, uint64_t* counts, int nthreads, const int* pixel_list,
-                      size_t npix) {
+                      size_t npix, float* normal_camera, uint32_t* mesh_id, uint32_t* material_id) {
     std::vector<InstanceO> inst(n);
     for (int i = 0; i < n; ++i) {
         auto* o = static_cast<OMesh*>(meshes[i]);
@@ -259,6 +263,7 @@ void orc_render_scene(void* const* meshes, const double* transforms, const Mater
     out.radiance = radiance; out.direct = direct; out.indirect = indirect; out.albedo = albedo;
     out.normal = normal; out.depth = depth; out.alpha = alpha; out.tri_id = tri_id; out.counts = counts;
     out.instance_id = instance_id;
+    out.normal_camera = normal_camera; out.mesh_id = mesh_id; out.material_id = material_id;
     render_scene(precision, inst.data(), n, *cam, lights, nl, *opt, out, nthreads, pixel_list, npix);
 }
 
@@ -341,7 +346,11 @@ void orc_tex_bilinear(int precision, const Material* m, const double* uv, size_t
 }
 float orc_kahan_sum_f32(const float* x, size_t n) { return hook_kahan_sum_f32(x, n); }
 
-// sin and cos of the kernels' FP32 routine (fp_sincos.hpp), for the tests that pin it.
+void orc_atan2_f32(const float* y, const float* x, size_t n, float* out) {
+    for (size_t i = 0; i < n; ++i) out[i] = vgm::atan2_f32(y[i], x[i]);
+}
+
+// sin and cos of the kernels' FP32 routine (fp_trig.hpp), for the tests that pin it.
 void orc_sincos_f32(const float* x, size_t n, float* s, float* c) {
     for (size_t i = 0; i < n; ++i) vgm::sincos_f32(x[i], s[i], c[i]);
 }

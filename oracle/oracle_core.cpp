@@ -2,7 +2,7 @@
 // TEST INFRASTRUCTURE ONLY (see oracle_core.hpp).
 #include "oracle_core.hpp"
 
-#include "../vira_amd/csrc/fp_sincos.hpp"
+#include "../vira_amd/csrc/fp_trig.hpp"
 
 #include <atomic>
 #include <thread>
@@ -225,7 +225,7 @@ V3<float> offset_point_f32(V3<float> p, V3<float> n) { return offset_point(p, n)
 namespace {
 
 // sin and cos of the sampling angles: the FP32 twin uses the kernels' own
-// routine (fp_sincos.hpp, Cody-Waite reduction + fitted polynomials, pinned
+// routine (fp_trig.hpp, Cody-Waite reduction + fitted polynomials, pinned
 // against FP64 sin/cos by the tests); the FP64 reference answer uses libm.
 inline void sincos_r(float x, float& s, float& c) { vgm::sincos_f32(x, s, c); }
 inline void sincos_r(double x, double& s, double& c) { s = std::sin(x); c = std::cos(x); }
@@ -254,6 +254,46 @@ inline R tex_bilinear(const Material& mat, R u, R v) {
     R dx = px - (R)x0, dy = py - (R)y0;
     return ((((R)1 - dx) * ((R)1 - dy)) * p00 + (dx * ((R)1 - dy)) * p10) + (((R)1 - dx) * dy) * p01 +
            (dx * dy) * p11;
+}
+
+// Image::sampleUVs + interpolatePixel (image.ipp:589-670) of a C-channel image
+// (row-major, C floats per texel): the same texel choice and weights as
+// tex_bilinear, per channel ((w00 p00 + w10 p10) + w01 p01) + w11 p11.
+template <class R>
+inline void image_bilinear(const float* img, int w, int h, int C, R u, R v, R* out) {
+    R rx = (R)w, ry = (R)h;
+    R px = std::fmod(u * rx, rx), py = std::fmod(v * ry, ry);
+    if (px < 0) px += rx;
+    if (py < 0) py += ry;
+    if (!(px >= 0)) px = 0;
+    if (!(py >= 0)) py = 0;
+    int x0 = (int)std::floor(px), y0 = (int)std::floor(py);
+    x0 = std::min(x0, w - 1);
+    y0 = std::min(y0, h - 1);
+    int x1 = std::min(x0 + 1, w - 1), y1 = std::min(y0 + 1, h - 1);
+    R dx = px - (R)x0, dy = py - (R)y0;
+    R w00 = ((R)1 - dx) * ((R)1 - dy), w10 = dx * ((R)1 - dy), w01 = ((R)1 - dx) * dy, w11 = dx * dy;
+    const float* p00 = img + ((size_t)x0 + (size_t)y0 * w) * C;
+    const float* p10 = img + ((size_t)x1 + (size_t)y0 * w) * C;
+    const float* p01 = img + ((size_t)x0 + (size_t)y1 * w) * C;
+    const float* p11 = img + ((size_t)x1 + (size_t)y1 * w) * C;
+    for (int c = 0; c < C; ++c) out[c] = ((w00 * (R)p00[c] + w10 * (R)p10[c]) + w01 * (R)p01[c]) + w11 * (R)p11[c];
+}
+
+inline float atan2_r(float y, float x) { return vgm::atan2_f32(y, x); }
+inline double atan2_r(double y, double x) { return std::atan2(y, x); }
+
+// Background radiance seen along direction d (cpu_path_tracer.ipp:257-270):
+// uv = (atan2(y, x) / 2 pi, atan2(sqrt(x^2 + y^2), z) / pi), then sampleUVs.
+template <class R>
+inline void background_radiance(const Options& opt, const V3<R>& d, R bg[NB]) {
+    if (!opt.background) {
+        for (int k = 0; k < NB; ++k) bg[k] = 0;
+        return;
+    }
+    const R u = atan2_r(d.y, d.x) / ((R)2 * Consts<R>::pi);
+    const R v = atan2_r(std::sqrt(d.x * d.x + d.y * d.y), d.z) / Consts<R>::pi;
+    image_bilinear<R>(opt.background, opt.bg_w, opt.bg_h, NB, u, v, bg);
 }
 
 // Light::sample (point_light.ipp:18-38, sphere_light.ipp:31-125).
@@ -376,6 +416,7 @@ struct PixelAcc {
     R depth = 0;
     uint32_t count = 0;
     uint32_t tri = 0xffffffffu;
+    uint32_t mesh = 0xffffffffu, material = 0xffffffffu;
 };
 
 // Camera-relative placement of an instance: the FP64 model matrix minus the
@@ -470,6 +511,10 @@ void render_impl(const InstanceO* inst, int ninst, const Camera& cam_in, const L
         return best;
     };
 
+    // adaptive-sampling settings: 0 stands for the reference's defaults (cpu_path_tracer.hpp:39-42)
+    const int batch_n = opt.samples_per_batch > 0 ? opt.samples_per_batch : 30;
+    const int miss_n = opt.samples_to_detect_miss > 0 ? opt.samples_to_detect_miss : 30;
+    const float tol = opt.sampling_tolerance > 0.0f ? opt.sampling_tolerance : 0.05f;
     auto worker = [&]() {
         uint64_t cnt[7] = {0};
         TravCount tc_closest, tc_any;
@@ -481,6 +526,8 @@ void render_impl(const InstanceO* inst, int ninst, const Camera& cam_in, const L
                 int pix = pixel_list ? pixel_list[kk] : (int)kk;
                 int pi = pix % cam.W, pj = pix / cam.W;
                 PixelAcc<R> acc;
+                R s1 = 0, s2 = 0;   // adaptive sampling sums (cpu_path_tracer.ipp:198-227)
+                int nsamp = opt.spp;
                 for (int s = 0; s < opt.spp; ++s) {
                     // ---- camera ray (cpu_path_tracer.ipp:178-192, camera.ipp:1126-1163,1738-1749)
                     R X = (R)pi, Y = (R)pj;
@@ -506,7 +553,15 @@ void render_impl(const InstanceO* inst, int ninst, const Camera& cam_in, const L
                     for (int bounce = 0; bounce <= opt.bounces; ++bounce) {
                         uint32_t hinst = 0;
                         Hit<R> h = trace(ray, false, (R)INFINITY, &tc_closest, hinst);
-                        if (h.id == 0xffffffffu) break;
+                        if (h.id == 0xffffffffu) {
+                            // miss: background radiance when shown (cpu_path_tracer.ipp:257-270)
+                            if (opt.show_background) {
+                                R bg[NB];
+                                background_radiance<R>(opt, ray.d, bg);
+                                for (int k = 0; k < NB; ++k) path[k] = path[k] + beta[k] * (bg[k] * (R)1);
+                            }
+                            break;
+                        }
                         const Mesh& m = *inst[hinst].m;
                         const Material& mat = inst[hinst].mat;
                         auto vpos = [&](uint32_t v) { return mesh_pos<R>(m, v); };
@@ -533,17 +588,31 @@ void render_impl(const InstanceO* inst, int ninst, const Camera& cam_in, const L
                         V3<R> Nl = fn;
                         if (mat.smooth) Nl = normalize((w0 * n1 + w1 * n2) + w2 * n0);
                         // instances: normal matrix on the normals (not renormalised, cpu_path_tracer.ipp:310)
-                        const V3<R> N = rebased ? xform_normal(place[hinst].nm, Nl) : Nl;
+                        const V3<R> N_frame = rebased ? xform_normal(place[hinst].nm, Nl) : Nl;
                         const V3<R> fng = rebased ? xform_normal(place[hinst].nm, fn) : fn;
                         V3<R> tg, bt;
-                        tangent_frame(N, tg, bt);
-                        R texv = 1;
-                        if (mat.tex) texv = tex_bilinear<R>(mat, uvu, uvv);
+                        tangent_frame(N_frame, tg, bt);
+                        // Material::getNormal (material.ipp:129-143): tangent-space map through [tg bt N]
+                        V3<R> N = N_frame;
+                        if (mat.nmap) {
+                            R nt[3];
+                            image_bilinear<R>(mat.nmap, mat.nmap_w, mat.nmap_h, 3, uvu, uvv, nt);
+                            N = normalize(V3<R>{(tg.x * nt[0] + bt.x * nt[1]) + N_frame.x * nt[2],
+                                                (tg.y * nt[0] + bt.y * nt[1]) + N_frame.y * nt[2],
+                                                (tg.z * nt[0] + bt.z * nt[1]) + N_frame.z * nt[2]});
+                        }
+                        // albedo: vertex albedo x Material::getAlbedo (material.ipp:114-116)
+                        R mvs[NB];
+                        if (mat.tex && mat.tex_channels == 8) {
+                            image_bilinear<R>(mat.tex, mat.tex_w, mat.tex_h, NB, uvu, uvv, mvs);
+                        } else {
+                            R texv = mat.tex ? tex_bilinear<R>(mat, uvu, uvv) : (R)1;
+                            for (int k = 0; k < NB; ++k) mvs[k] = mat.tex ? texv * (R)mat.albedo[k] : (R)mat.albedo[k];
+                        }
                         R alb[NB];
                         for (int k = 0; k < NB; ++k) {
                             R av = (w0 * (R)m.alb[NB * i1 + k] + w1 * (R)m.alb[NB * i2 + k]) + w2 * (R)m.alb[NB * i0 + k];
-                            R mv = mat.tex ? texv * (R)mat.albedo[k] : (R)mat.albedo[k];
-                            alb[k] = av * mv;
+                            alb[k] = av * mvs[k];
                         }
                         V3<R> P = (w0 * p1 + w1 * p2) + w2 * p0;
                         {  // computeShadingPoint (cpu_path_tracer.ipp:481-511)
@@ -556,7 +625,11 @@ void render_impl(const InstanceO* inst, int ninst, const Camera& cam_in, const L
                         }
                         if (rebased) P = xform_point(place[hinst].l2w, P);  // model matrix (cpu_path_tracer.ipp:344)
                         if (bounce == 0) {
-                            if (acc.tri == 0xffffffffu) acc.tri = h.id | (hinst << 27);
+                            if (acc.tri == 0xffffffffu) {
+                                acc.tri = h.id | (hinst << 27);
+                                acc.mesh = m.id;
+                                acc.material = mat.material_id;
+                            }
                             for (int k = 0; k < NB; ++k) acc.alb[k] = acc.alb[k] + alb[k];
                             acc.nrm = acc.nrm + N;
                             acc.depth = acc.depth + h.t;
@@ -566,6 +639,9 @@ void render_impl(const InstanceO* inst, int ninst, const Camera& cam_in, const L
                         V3<R> V = {-ray.d.x, -ray.d.y, -ray.d.z};
                         (void)V;
                         R vrad[NB] = {0};
+                        if (bounce == 0 && opt.has_ambient) {  // cpu_path_tracer.ipp:389-396, material.ipp:67
+                            for (int k = 0; k < NB; ++k) vrad[k] = vrad[k] + beta[k] * ((R)opt.ambient[k] * alb[k]);
+                        }
                         for (int l = 0; l < nl; ++l) {
                             V3<R> ld;
                             R dist, lpdf, lrad[NB];
@@ -596,7 +672,7 @@ void render_impl(const InstanceO* inst, int ninst, const Camera& cam_in, const L
                         uint32_t db = (uint32_t)bounce << 8;
                         V3<R> nd;
                         R mpdf;
-                        if (!cosine_sample(opt.seed, (uint32_t)pix, (uint32_t)s, (uint32_t)bounce, N, tg, bt, nd, mpdf))
+                        if (!cosine_sample(opt.seed, (uint32_t)pix, (uint32_t)s, (uint32_t)bounce, N_frame, tg, bt, nd, mpdf))
                             break;
                         R ndd = dot(N, nd);
                         R mx = std::max(ndd, (R)0);
@@ -621,9 +697,34 @@ void render_impl(const InstanceO* inst, int ninst, const Camera& cam_in, const L
                         cnt[2]++;
                     }
                     for (int k = 0; k < NB; ++k) { acc.rad[k].add(path[k]); acc.dir[k].add(direct[k]); }
+                    if (opt.adaptive) {
+                        // cpu_path_tracer.ipp:198-227; a stop at sample s leaves s as the divisor
+                        R sq = 0;
+                        for (int k = 0; k < NB; ++k) sq = sq + path[k] * path[k];
+                        const R illum = std::sqrt(sq);
+                        s1 = s1 + illum;
+                        s2 = s2 + illum * illum;
+                        bool stop = false;
+                        if (s % miss_n == 0 && s != 0 && acc.depth == (R)0) stop = true;
+                        if (!stop && s % batch_n == 0 && s != 0) {
+                            if (s1 == (R)0) {
+                                stop = true;
+                            } else {
+                                const R n = (R)s + (R)1;
+                                const R mu = s1 / n;
+                                const R var = ((R)1 / (n - (R)1)) * (s2 - (s1 * s1) / n);
+                                const R I = (std::is_same<R, float>::value ? (R)1.96f : (R)1.96) * std::sqrt(var / n);
+                                if (I <= (R)tol * mu) stop = true;
+                            }
+                        }
+                        if (stop) {
+                            nsamp = s;
+                            break;
+                        }
+                    }
                 }
                 // ---- averaging and passes (cpu_path_tracer.ipp:229-239, passes.ipp:81-127)
-                R sf = (R)opt.spp;
+                R sf = (R)nsamp;
                 size_t o = (size_t)pix;
                 for (int k = 0; k < NB; ++k) {
                     R rad = acc.rad[k].s / sf, dr = acc.dir[k].s / sf;
@@ -632,13 +733,33 @@ void render_impl(const InstanceO* inst, int ninst, const Camera& cam_in, const L
                     if (out.indirect) out.indirect[NB * o + k] = (float)(rad - dr);
                     if (out.albedo) out.albedo[NB * o + k] = (float)(acc.alb[k] / sf);
                 }
-                if (out.normal) {
+                {
                     V3<R> n = acc.nrm;
                     if (acc.count > 0) n = normalize(n);
-                    out.normal[3 * o] = (float)n.x; out.normal[3 * o + 1] = (float)n.y; out.normal[3 * o + 2] = (float)n.z;
+                    if (out.normal) {
+                        out.normal[3 * o] = (float)n.x; out.normal[3 * o + 1] = (float)n.y; out.normal[3 * o + 2] = (float)n.z;
+                    }
+                    if (out.normal_camera) {  // normalize(getViewNormalMatrix() * normal_global) (cpu_path_tracer.ipp:237-239)
+                        V3<R> c{0, 0, 0};
+                        if (acc.count > 0) {
+                            const float* vn = cam_in.view_normal;
+                            float t[9];
+                            bool given = false;
+                            for (int k = 0; k < 9; ++k) given = given || vn[k] != 0.0f;
+                            for (int r = 0; r < 3; ++r)
+                                for (int q = 0; q < 3; ++q) t[r * 3 + q] = given ? vn[r * 3 + q] : cam_in.rot[q * 3 + r];
+                            c = normalize(V3<R>{((R)t[0] * n.x + (R)t[1] * n.y) + (R)t[2] * n.z,
+                                                ((R)t[3] * n.x + (R)t[4] * n.y) + (R)t[5] * n.z,
+                                                ((R)t[6] * n.x + (R)t[7] * n.y) + (R)t[8] * n.z});
+                        }
+                        out.normal_camera[3 * o] = (float)c.x; out.normal_camera[3 * o + 1] = (float)c.y;
+                        out.normal_camera[3 * o + 2] = (float)c.z;
+                    }
                 }
                 if (out.depth) out.depth[o] = acc.count ? (float)(acc.depth / (R)acc.count) : INFINITY;
-                if (out.alpha) out.alpha[o] = (float)acc.count / (float)opt.spp;
+                if (out.alpha) out.alpha[o] = (float)acc.count / (float)nsamp;
+                if (out.mesh_id) out.mesh_id[o] = acc.tri == 0xffffffffu ? 0xffffffffu : acc.mesh;
+                if (out.material_id) out.material_id[o] = acc.tri == 0xffffffffu ? 0xffffffffu : acc.material;
                 if (out.tri_id) out.tri_id[o] = acc.tri == 0xffffffffu ? acc.tri : (acc.tri & ((1u << 27) - 1u));
                 if (out.instance_id) out.instance_id[o] = acc.tri == 0xffffffffu ? acc.tri : (acc.tri >> 27);
             }

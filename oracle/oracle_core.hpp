@@ -323,6 +323,7 @@ struct Mesh {
     std::vector<float> alb;       // NB*nv
     std::vector<float> uv;        // 2*nv (may be empty)
     size_t nv = 0, nt = 0;
+    uint32_t id = 0;              // Mesh::getID, written to the mesh_id pass
 };
 
 struct BVH {
@@ -491,6 +492,7 @@ struct Camera {
     float rot[9];    // row-major R (columns = camera x, y, z axes in world)
     double pos[3];   // camera position in the rebased world frame
     float zdir;      // +1 (computer-vision frame) or -1 (Blender frame)
+    float view_normal[9];  // Camera::getViewNormalMatrix, row-major; all zero = transpose(rot)
 };
 
 struct Light {
@@ -505,6 +507,10 @@ struct Material {
     int tex_w = 0, tex_h = 0;  // optional grey albedo texture, row-major i + j*W
     const float* tex = nullptr;
     int smooth = 0;            // Mesh::smoothShading (default false, mesh.hpp:144)
+    int tex_channels = 1;      // 1: grey texture x albedo[k]; 8: spectral albedo map (Image<TSpectral>)
+    const float* nmap = nullptr;  // tangent-space normal map, 3 floats per texel (Material::getNormal)
+    int nmap_w = 0, nmap_h = 0;
+    uint32_t material_id = 0;  // Material::getID, written to the material_id pass
 };
 
 struct Options {
@@ -513,6 +519,17 @@ struct Options {
     int rr = 0;          // Russian roulette on bounce rays
     int rr_start = 1;    // first bounce index whose continuation is subject to RR
     uint32_t seed = 0;
+    // CPUPathTracerOptions (cpu_path_tracer.hpp:30-43)
+    int show_background = 0;
+    int adaptive = 0;
+    int samples_per_batch = 30;
+    float sampling_tolerance = 0.05f;
+    int samples_to_detect_miss = 30;
+    // Scene::getAmbient / hasAmbient, getBackgroundEmission (scene.hpp:100-113)
+    float ambient[NB] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int has_ambient = 0;
+    const float* background = nullptr;  // W*H*NB spectral image, or null (zero background)
+    int bg_w = 0, bg_h = 0;
 };
 
 struct Outputs {  // all optional (nullptr = skip); pixel index i + j*W
@@ -529,6 +546,9 @@ struct Outputs {  // all optional (nullptr = skip); pixel index i + j*W
     uint32_t* records = nullptr; // 4 per (sample, pixel) at [s * W*H + pixel]: first-hit id,
                                  // b1 and b2 bits, bit l = light l sampled and unoccluded
     uint32_t* instance_id = nullptr;
+    float* normal_camera = nullptr;  // 3
+    uint32_t* mesh_id = nullptr;
+    uint32_t* material_id = nullptr;
 };
 
 // One instance of a two-level scene (EmbreeTLAS::newInstance, embree_tlas.ipp:77-115):

@@ -30,7 +30,7 @@ class Projection(C.Structure):
 
 class Camera(C.Structure):
     _fields_ = [("W", C.c_int), ("H", C.c_int), ("kinv", C.c_float * 6), ("rot", C.c_float * 9),
-                ("pos", C.c_double * 3), ("zdir", C.c_float)]
+                ("pos", C.c_double * 3), ("zdir", C.c_float), ("view_normal", C.c_float * 9)]
 
 
 class Light(C.Structure):
@@ -39,7 +39,8 @@ class Light(C.Structure):
 
 class Material(C.Structure):
     _fields_ = [("albedo", C.c_float * NB), ("tex_w", C.c_int), ("tex_h", C.c_int),
-                ("tex", C.c_void_p), ("smooth", C.c_int)]
+                ("tex", C.c_void_p), ("smooth", C.c_int), ("tex_channels", C.c_int), ("nmap", C.c_void_p),
+                ("nmap_w", C.c_int), ("nmap_h", C.c_int), ("material_id", C.c_uint32)]
 
 
 class PSF(C.Structure):
@@ -55,7 +56,10 @@ class DenoiseOpts(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("spp", C.c_int), ("bounces", C.c_int), ("rr", C.c_int), ("rr_start", C.c_int),
-                ("seed", C.c_uint32)]
+                ("seed", C.c_uint32), ("show_background", C.c_int), ("adaptive", C.c_int),
+                ("samples_per_batch", C.c_int), ("sampling_tolerance", C.c_float),
+                ("samples_to_detect_miss", C.c_int), ("ambient", C.c_float * NB), ("has_ambient", C.c_int),
+                ("background", C.c_void_p), ("bg_w", C.c_int), ("bg_h", C.c_int)]
 
 
 _lib = None
@@ -89,9 +93,12 @@ def lib() -> C.CDLL:
         "orc_offset_point": (None, [P, P, P]),
         "orc_rng_u01": (C.c_float, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
         "orc_render": (None, [P, C.c_int, C.POINTER(Camera), C.POINTER(Light), C.c_int, C.POINTER(Material),
-                              C.POINTER(Options), P, P, P, P, P, P, P, P, P, P, C.c_int, P, C.c_size_t]),
+                              C.POINTER(Options), P, P, P, P, P, P, P, P, P, P, C.c_int, P, C.c_size_t, P, P, P]),
         "orc_render_scene": (None, [P, P, P, C.c_int, C.c_int, C.POINTER(Camera), C.POINTER(Light), C.c_int,
-                                    C.POINTER(Options), P, P, P, P, P, P, P, P, P, P, C.c_int, P, C.c_size_t]),
+                                    C.POINTER(Options), P, P, P, P, P, P, P, P, P, P, C.c_int, P, C.c_size_t,
+                                    P, P, P]),
+        "orc_mesh_set_id": (None, [P, C.c_uint32]),
+        "orc_atan2_f32": (None, [P, P, C.c_size_t, P]),
         "orc_pixel_solid_angles": (None, [P, C.c_float, C.c_int, C.c_int, P]),
         "orc_psf_kernel": (None, [C.POINTER(PSF), C.c_int, P]),
         "orc_convolve": (None, [P, C.c_int, C.c_int, P, C.c_int, P]),
@@ -184,7 +191,7 @@ def vertex_normals(pos: np.ndarray, idx: np.ndarray) -> np.ndarray:
 class Mesh:
     """Mesh + reference binned-SAH BVH2 on the CPU."""
 
-    def __init__(self, pos, idx, nrm=None, alb=None, uv=None, pos64=None):
+    def __init__(self, pos, idx, nrm=None, alb=None, uv=None, pos64=None, mesh_id=0):
         self.pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
         self.idx = np.ascontiguousarray(idx, np.uint32).reshape(-1, 3)
         self.nrm = None if nrm is None else np.ascontiguousarray(nrm, np.float32)
@@ -193,6 +200,7 @@ class Mesh:
         self.pos64 = None if pos64 is None else np.ascontiguousarray(pos64, np.float64)
         self.h = lib().orc_mesh_create(_p(self.pos), _p(self.pos64), self.pos.shape[0], _p(self.idx),
                                        self.idx.shape[0], _p(self.nrm), _p(self.alb), _p(self.uv), nthreads())
+        lib().orc_mesh_set_id(self.h, int(mesh_id))
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -253,6 +261,8 @@ class Mesh:
             "indirect": np.zeros((n, NB), np.float32), "albedo": np.zeros((n, NB), np.float32),
             "normal": np.zeros((n, 3), np.float32), "depth": np.zeros(n, np.float32),
             "alpha": np.zeros(n, np.float32), "tri_id": np.full(n, 0xFFFFFFFF, np.uint32),
+            "normal_camera": np.zeros((n, 3), np.float32), "mesh_id": np.zeros(n, np.uint32),
+            "material_id": np.zeros(n, np.uint32),
         }
         cnt = np.zeros(7, np.uint64)
         rec = np.zeros((opt.spp, n, 4), np.uint32) if records else None
@@ -261,7 +271,8 @@ class Mesh:
         lib().orc_render(self.h, precision, C.byref(cam), larr, len(lights), C.byref(mat), C.byref(opt),
                          *(out[k].ctypes.data for k in ("radiance", "direct", "indirect", "albedo", "normal",
                                                         "depth", "alpha", "tri_id")),
-                         cnt.ctypes.data, _p(rec), threads or nthreads(), _p(pl), 0 if pl is None else pl.size)
+                         cnt.ctypes.data, _p(rec), threads or nthreads(), _p(pl), 0 if pl is None else pl.size,
+                         *(out[k].ctypes.data for k in ("normal_camera", "mesh_id", "material_id")))
         if rec is not None:
             out["records"] = rec
         out["counts"] = {"primary": int(cnt[0]), "shadow": int(cnt[1]), "bounce": int(cnt[2]),
@@ -270,8 +281,17 @@ class Mesh:
         return out
 
 
+def atan2_f32(y, x):
+    """The kernels' FP32 atan2 (vira_amd/csrc/fp_trig.hpp)."""
+    y = np.ascontiguousarray(y, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty_like(y)
+    lib().orc_atan2_f32(y.ctypes.data, x.ctypes.data, y.size, out.ctypes.data)
+    return out
+
+
 def sincos_f32(x):
-    """The kernels' FP32 sin/cos routine (vira_amd/csrc/fp_sincos.hpp) on an array."""
+    """The kernels' FP32 sin/cos routine (vira_amd/csrc/fp_trig.hpp) on an array."""
     x = np.ascontiguousarray(x, np.float32)
     s = np.empty_like(x)
     c = np.empty_like(x)
@@ -337,6 +357,10 @@ def camera_from(c) -> Camera:
     for k in range(3):
         o.pos[k] = c.position[k]
     o.zdir = c.z_dir
+    vn = getattr(c, "view_normal", None)
+    if vn is not None:
+        for k in range(9):
+            o.view_normal[k] = vn[k]
     return o
 
 
@@ -351,23 +375,51 @@ def light_from(L) -> Light:
     return o
 
 
-def material(albedo=1.0, texture: np.ndarray | None = None, smooth=False) -> Material:
+def material(albedo=1.0, texture: np.ndarray | None = None, smooth=False, normal_map: np.ndarray | None = None,
+             material_id: int = 0) -> Material:
+    """texture (H, W) grey x albedo, or (H, W, 8) spectral albedo map; normal_map (H, W, 3) tangent space."""
     m = Material()
     a = np.broadcast_to(np.asarray(albedo, np.float32), (NB,))
     for k in range(NB):
         m.albedo[k] = float(a[k])
+    keep = []
+    m.tex_channels = 1
     if texture is not None:
         texture = np.ascontiguousarray(texture, np.float32)
-        m.tex_h, m.tex_w = texture.shape
+        m.tex_h, m.tex_w = texture.shape[:2]
+        m.tex_channels = 1 if texture.ndim == 2 else texture.shape[2]
         m.tex = texture.ctypes.data
-        m._keep = texture
+        keep.append(texture)
+    if normal_map is not None:
+        normal_map = np.ascontiguousarray(normal_map, np.float32)
+        m.nmap_h, m.nmap_w = normal_map.shape[:2]
+        m.nmap = normal_map.ctypes.data
+        keep.append(normal_map)
+    m._keep = keep
     m.smooth = int(smooth)
+    m.material_id = int(material_id)
     return m
 
 
-def options(spp=1, bounces=0, rr=False, rr_start=1, seed=0) -> Options:
+def options(spp=1, bounces=0, rr=False, rr_start=1, seed=0, show_background=False, adaptive=False,
+            samples_per_batch=30, sampling_tolerance=0.05, samples_to_detect_miss=30, ambient=None,
+            background: np.ndarray | None = None) -> Options:
+    """PathTracer options plus the scene's ambient light and background image ((H, W, 8))."""
     o = Options()
     o.spp, o.bounces, o.rr, o.rr_start, o.seed = spp, bounces, int(rr), rr_start, seed
+    o.show_background, o.adaptive = int(show_background), int(adaptive)
+    o.samples_per_batch, o.sampling_tolerance, o.samples_to_detect_miss = samples_per_batch, sampling_tolerance, \
+        samples_to_detect_miss
+    if ambient is not None:
+        a = np.broadcast_to(np.asarray(ambient, np.float32), (NB,))
+        for k in range(NB):
+            o.ambient[k] = float(a[k])
+        o.has_ambient = int(np.any(a != 0))
+    if background is not None:
+        bg = np.ascontiguousarray(background, np.float32)
+        o.bg_h, o.bg_w = bg.shape[:2]
+        o.background = bg.ctypes.data
+        o._keep = bg
     return o
 
 
@@ -515,6 +567,8 @@ def render_scene(meshes, transforms, mats, cam: Camera, lights, opt: Options, pr
         "normal": np.zeros((npx, 3), np.float32), "depth": np.zeros(npx, np.float32),
         "alpha": np.zeros(npx, np.float32), "tri_id": np.full(npx, 0xFFFFFFFF, np.uint32),
         "instance_id": np.full(npx, 0xFFFFFFFF, np.uint32),
+        "normal_camera": np.zeros((npx, 3), np.float32), "mesh_id": np.zeros(npx, np.uint32),
+        "material_id": np.zeros(npx, np.uint32),
     }
     cnt = np.zeros(7, np.uint64)
     handles = (C.c_void_p * n)(*[m.h for m in meshes])
@@ -526,7 +580,8 @@ def render_scene(meshes, transforms, mats, cam: Camera, lights, opt: Options, pr
                            C.byref(opt), *(out[k].ctypes.data for k in ("radiance", "direct", "indirect", "albedo",
                                                                         "normal", "depth", "alpha", "tri_id",
                                                                         "instance_id")),
-                           cnt.ctypes.data, threads or nthreads(), _p(pl), 0 if pl is None else pl.size)
+                           cnt.ctypes.data, threads or nthreads(), _p(pl), 0 if pl is None else pl.size,
+                           *(out[k].ctypes.data for k in ("normal_camera", "mesh_id", "material_id")))
     out["counts"] = {"primary": int(cnt[0]), "shadow": int(cnt[1]), "bounce": int(cnt[2])}
     return out
 

@@ -18,7 +18,7 @@ def _ulp_err(got, ref):
 
 
 def test_sincos_f32_against_fp64():
-    """fp_sincos.hpp: Cody-Waite reduction + fitted polynomials, at most 2 ulp from
+    """fp_trig.hpp: Cody-Waite reduction + fitted polynomials, at most 2 ulp from
     FP64 sin/cos over the arguments the path takes (phi in [0, 2 pi), the
     concentric-disk angle in [-pi/4, 3 pi/4]) and well beyond; next to a zero of
     the function (results below 1e-6) the error is bounded absolutely by 2e-14,

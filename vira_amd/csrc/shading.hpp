@@ -13,7 +13,7 @@
 #pragma once
 
 #include "device_math.hpp"
-#include "fp_sincos.hpp"
+#include "fp_trig.hpp"
 #include "../../include/vira_gpu.h"
 
 namespace vg {
