@@ -93,14 +93,25 @@ typedef struct {
     float value[VIRA_GPU_NUM_BINS];
 } vira_gpu_light;
 
-/* Lambertian material (lambertian.ipp:9-15).  Albedo = vertex albedo x
- * albedo[k] x grey texture (bilinear, Image::sampleUVs, image.ipp:589-670). */
+/* Lambertian material (lambertian.ipp:9-15).  Albedo = vertex albedo x the
+ * material's albedo map sampled bilinearly at the hit's uv (Image::sampleUVs,
+ * image.ipp:589-670; Material::getAlbedo, material.ipp:114-116): either a
+ * grey texture times albedo[k] (tex_channels 1) or a spectral albedo map of 8
+ * bands per texel (tex_channels 8, Image<TSpectral>; albedo[] unused), or
+ * albedo[k] alone without a texture.  An optional tangent-space normal map
+ * (Material::getNormal, material.ipp:129-143) replaces the shading normal by
+ * normalize([tangent bitangent N] * map(uv)). */
 typedef struct {
     float albedo[VIRA_GPU_NUM_BINS];
-    const float* texture_dev; /* row-major i + j*W grey texture, or NULL */
+    const float* texture_dev; /* row-major i + j*W, tex_channels floats per texel, or NULL */
     int tex_width;
     int tex_height;
     int smooth_shading;       /* Mesh::smoothShading (mesh.hpp:144) */
+    int tex_channels;         /* 1 (or 0) grey, 8 spectral */
+    const float* normal_map_dev; /* row-major, 3 floats (x, y, z in the tangent frame) per texel, or NULL */
+    int nmap_width;
+    int nmap_height;
+    uint32_t material_id;     /* Material::getID, written to the material_id pass */
 } vira_gpu_material;
 
 /* Pinhole camera: kinv is GLM mat3x2 K^-1 stored [col][row] (camera.ipp:1526-1568),
@@ -113,6 +124,8 @@ typedef struct {
     float rot[9];
     double position[3];
     float z_dir;
+    float view_normal[9]; /* Camera::getViewNormalMatrix (camera.hpp:190), row-major, for the
+                             normal_camera pass; all zero = the transpose of rot */
 } vira_gpu_camera;
 
 /* CPUPathTracerOptions (cpu_path_tracer.hpp:30-43) plus the new controls. */
@@ -122,7 +135,12 @@ typedef struct {
     int russian_roulette;
     int rr_start;        /* first bounce whose continuation is subject to RR */
     uint32_t seed;
-    int integrator;      /* 0 auto, 1 fused per-pixel (<= 1 light), 2 wavefront queues */
+    int integrator;      /* 0 auto (= 1), 1 fused per-pixel, 2 wavefront queues */
+    int show_background; /* misses add throughput x background (cpu_path_tracer.ipp:257-270) */
+    int adaptive_sampling;       /* cpu_path_tracer.ipp:198-227 (per-pixel integrator) */
+    int samples_per_batch;       /* 0 = 30 */
+    float sampling_tolerance;    /* 0 = 0.05 */
+    int samples_to_detect_miss;  /* 0 = 30 */
 } vira_gpu_render_opts;
 
 /* Render passes (passes.hpp:51-95); every pointer optional, device memory,
@@ -137,6 +155,9 @@ typedef struct {
     float* alpha_dev;      /* W*H hits / spp                  */
     uint32_t* tri_id_dev;  /* W*H first-hit triangle (0xffffffff = none) */
     uint32_t* instance_id_dev; /* W*H first-hit instance: index into the TLAS (0xffffffff = none) */
+    float* normal_camera_dev;  /* W*H*3 normalize(view normal matrix x normal) (0 where no hit) */
+    uint32_t* mesh_id_dev;     /* W*H first-hit mesh id (0xffffffff = none) */
+    uint32_t* material_id_dev; /* W*H first-hit material id (vira_gpu_material.material_id) */
 } vira_gpu_outputs;
 
 typedef struct {
@@ -180,8 +201,8 @@ typedef struct {
     int32_t mesh_id;             /* rendered mesh: its BLAS and material */
     vira_gpu_camera camera;
     vira_gpu_render_opts opts;   /* integrator is ignored: the per-pixel renderer runs every request */
-    int32_t num_lights;          /* 0 or 1 */
-    vira_gpu_light light;
+    int32_t num_lights;          /* 0 .. VIRA_GPU_MAX_LIGHTS */
+    vira_gpu_light lights[VIRA_GPU_MAX_LIGHTS];
     vira_gpu_outputs out;        /* caller-owned device passes, as for vira_gpu_render */
 } vira_gpu_request;
 
@@ -347,6 +368,14 @@ vira_gpu_status vira_gpu_tlas_build(vira_gpu_ctx* ctx, const vira_gpu_instance* 
 vira_gpu_status vira_gpu_lights_set(vira_gpu_ctx* ctx, const vira_gpu_light* lights_host, int n);
 /* Mesh::material_cache_ (mesh.hpp:93). */
 vira_gpu_status vira_gpu_material_set(vira_gpu_ctx* ctx, int32_t mesh_id, const vira_gpu_material* mat);
+/* Scene::setAmbient / setBackgroundEmission (scene.hpp:90-113): ambient
+ * spectral radiance (8 values, host; NULL or all zero = no ambient light),
+ * added at the first hit as throughput x ambient x albedo
+ * (cpu_path_tracer.ipp:389-396), and the background image (W*H*8 on the
+ * device, kept by reference; NULL = zero) seen by rays that miss when
+ * show_background is set.  Both are used by the per-pixel integrator. */
+vira_gpu_status vira_gpu_environment_set(vira_gpu_ctx* ctx, const float* ambient_host, const float* background_dev,
+                                         int bg_width, int bg_height);
 
 /* --- rendering --------------------------------------------------------- */
 /* CPUPathTracer::render (cpu_path_tracer.ipp:29-161) over the current TLAS. */
@@ -359,11 +388,10 @@ vira_gpu_status vira_gpu_render(vira_gpu_ctx* ctx, const vira_gpu_camera* cam, c
  * integrator, as vira_gpu_render does for scenes of at most one light, and
  * writes the same passes bit for bit; its scratch (tile accumulators, ray
  * counters, light and instance tables) is one fixed array per GPU that the
- * library holds for the life of the process (about 46 MiB), so no hipMalloc,
+ * library holds for the life of the process (about 48 MiB), so no hipMalloc,
  * hipFree or other allocating call is made.  Limits: at most 4096 8x8 pixel
- * tiles (e.g. 512 x 512 pixels; larger images return VIRA_GPU_ERR_INVALID),
- * at most one light and 64 instances (VIRA_GPU_ERR_UNSUPPORTED for the
- * wavefront integrator or more lights).  Every call on one GPU shares that
+ * tiles (e.g. 512 x 512 pixels; larger images return VIRA_GPU_ERR_INVALID)
+ * and 64 instances (VIRA_GPU_ERR_UNSUPPORTED for the wavefront integrator).  Every call on one GPU shares that
  * array: calls that overlap on one GPU (two contexts, two threads) race. */
 vira_gpu_status vira_gpu_render_noalloc(vira_gpu_ctx* ctx, const vira_gpu_camera* cam,
                                         const vira_gpu_render_opts* opts, const vira_gpu_outputs* out,

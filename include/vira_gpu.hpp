@@ -99,6 +99,12 @@ public:
         check(vira_gpu_lights_set(h_, lights.data(), (int)lights.size()));
     }
     void setMaterial(int32_t mesh, const vira_gpu_material& m) { check(vira_gpu_material_set(h_, mesh, &m)); }
+    // Scene::setAmbient / setBackgroundEmission (scene.hpp:90-113): 8 ambient values (or
+    // nullptr) and a W*H*8 device background image (or nullptr)
+    void setEnvironment(const float* ambient, const float* background_dev = nullptr, int bg_width = 0,
+                        int bg_height = 0) {
+        check(vira_gpu_environment_set(h_, ambient, background_dev, bg_width, bg_height));
+    }
     void synchronize() { check(vira_gpu_synchronize(h_)); }
 
     // --- camera sensor tail (camera.ipp:1175-1340) on W*H*8 device images
@@ -138,6 +144,11 @@ struct PathTracerOptions {
     uint32_t seed = 0;
     int integrator = 0;  // 0 auto, 1 per-pixel, 2 wavefront
     bool denoise = false;  // CPUPathTracerOptions::denoise (cpu_path_tracer.hpp:35)
+    bool showBackground = false;          // show_background
+    bool adaptiveSampling = false;        // adaptive_sampling
+    int samplesPerBatch = 30;             // samples_per_batch
+    float samplingTolerance = 0.05f;      // sampling_tolerance
+    int samplesToDetectMiss = 30;         // samples_to_detect_miss
 };
 
 // EATWTOptions (cpu_denoise.hpp:9-18).
@@ -157,8 +168,18 @@ public:
     // With options.denoise the direct and indirect passes are denoised after the
     // render and the total pass becomes their sum (cpu_path_tracer.ipp:124-134).
     void render(const vira_gpu_camera& camera) {
-        vira_gpu_render_opts o{options.samples, options.bounces, options.russianRoulette ? 1 : 0, options.rrStart,
-                               options.seed, options.integrator};
+        vira_gpu_render_opts o{};
+        o.samples = options.samples;
+        o.bounces = options.bounces;
+        o.russian_roulette = options.russianRoulette ? 1 : 0;
+        o.rr_start = options.rrStart;
+        o.seed = options.seed;
+        o.integrator = options.integrator;
+        o.show_background = options.showBackground ? 1 : 0;
+        o.adaptive_sampling = options.adaptiveSampling ? 1 : 0;
+        o.samples_per_batch = options.samplesPerBatch;
+        o.sampling_tolerance = options.samplingTolerance;
+        o.samples_to_detect_miss = options.samplesToDetectMiss;
         check(vira_gpu_render(ctx_->get(), &camera, &o, &renderPasses, &lastStats));
         if (options.denoise)
             check(vira_gpu_denoise(ctx_->get(), &denoiserOptions, camera.width, camera.height,

@@ -197,12 +197,14 @@ def test_render_bounces_sphere_light_small(gpu_ctx, integrator):
 
 
 def test_render_two_point_lights_wavefront(gpu_ctx):
-    """Several lights: shadow rays per light, contributions summed in light order."""
+    """Several lights: shadow rays per light, contributions summed in light order
+    (the wavefront schedule, then the per-pixel one)."""
     sc = W.scaled(W.DEM257, dem=97, image=96, spp=2, bounces=1)
     built = W.build(gpu_ctx, sc)
     lights = built.lights + [S.directional_sun(55.0, 140.0, [0.5, 1, 1.5, 2, 2.5, 3, 3.5, 4])]
     gpu_ctx.lights_set(lights)
     rp = S.RenderPasses(sc.image, sc.image)
+    built.options.integrator = "wavefront"
     st = gpu_ctx.render(built.camera, built.options, rp)
     assert st["integrator"] == 2
     om, cam, _, mat, opt = oracle_scene(built)
@@ -213,11 +215,13 @@ def test_render_two_point_lights_wavefront(gpu_ctx):
     assert (o["radiance"] > 0).mean() > 0.2
     assert np.array_equal(rad.view(np.uint32), o["radiance"].view(np.uint32))
     assert np.array_equal(dr.view(np.uint32), o["direct"].view(np.uint32))
-    from vira_amd import ViraGPUError
+    # the per-pixel integrator takes several lights too (its LDS-held vertex variant): same image
     built.options.integrator = "fused"
-    with pytest.raises(ViraGPUError) as e:
-        gpu_ctx.render(built.camera, built.options, rp)
-    assert e.value.status == 5
+    rp2 = S.RenderPasses(sc.image, sc.image)
+    st2 = gpu_ctx.render(built.camera, built.options, rp2)
+    assert st2["integrator"] == 1 and st2["shadow_rays"] == st["shadow_rays"]
+    assert np.array_equal(rp2.total_radiance.cpu().numpy().reshape(-1, 8).view(np.uint32),
+                          o["radiance"].view(np.uint32))
     gpu_ctx.lights_set(built.lights)
 
 

@@ -13,7 +13,7 @@ from vira_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 PASSES = ("total_radiance", "direct_radiance", "indirect_radiance", "albedo", "normal_global", "depth", "alpha",
-          "triangle_id", "instance_id")
+          "triangle_id", "instance_id", "normal_camera", "mesh_id", "material_id")
 
 
 def _both(ctx, built, image):
@@ -68,8 +68,21 @@ def test_noalloc_limits(gpu_ctx):
     with pytest.raises(ViraGPUError) as e:
         gpu_ctx.render_noalloc(built.camera, built.options, S.RenderPasses(520, 520))
     assert e.value.status == 1 and "4096" in str(e.value)
-    gpu_ctx.lights_set(built.lights + built.lights)
+    built.options.integrator = "wavefront"
     with pytest.raises(ViraGPUError) as e:
-        gpu_ctx.render_noalloc(built.camera, built.options, S.RenderPasses(520, 520))
+        gpu_ctx.render_noalloc(built.camera, built.options, S.RenderPasses(64, 64))
     assert e.value.status == 5
-    gpu_ctx.lights_set(built.lights)
+    built.options.integrator = "auto"
+
+
+def test_noalloc_several_lights_matches_render(gpu_ctx):
+    """The several-light variant of the per-pixel renderer through the fixed scratch."""
+    sc = W.scaled(W.DEM257, dem=65, image=80, spp=2, bounces=1)
+    built = W.build(gpu_ctx, sc)
+    gpu_ctx.lights_set(built.lights + [S.directional_sun(50.0, 200.0, 1.5, distance=1e6)])
+    try:
+        a, sa, b, sb = _both(gpu_ctx, built, sc.image)
+        _same(a, b)
+        assert sa["shadow_rays"] == sb["shadow_rays"] > 0
+    finally:
+        gpu_ctx.lights_set(built.lights)

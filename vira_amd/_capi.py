@@ -81,6 +81,11 @@ class Material(C.Structure):
         ("tex_width", C.c_int),
         ("tex_height", C.c_int),
         ("smooth_shading", C.c_int),
+        ("tex_channels", C.c_int),
+        ("normal_map_dev", C.c_void_p),
+        ("nmap_width", C.c_int),
+        ("nmap_height", C.c_int),
+        ("material_id", C.c_uint32),
     ]
 
 
@@ -92,6 +97,7 @@ class Camera(C.Structure):
         ("rot", C.c_float * 9),
         ("position", C.c_double * 3),
         ("z_dir", C.c_float),
+        ("view_normal", C.c_float * 9),
     ]
 
 
@@ -103,6 +109,11 @@ class RenderOpts(C.Structure):
         ("rr_start", C.c_int),
         ("seed", C.c_uint32),
         ("integrator", C.c_int),
+        ("show_background", C.c_int),
+        ("adaptive_sampling", C.c_int),
+        ("samples_per_batch", C.c_int),
+        ("sampling_tolerance", C.c_float),
+        ("samples_to_detect_miss", C.c_int),
     ]
 
 
@@ -117,6 +128,9 @@ class Outputs(C.Structure):
         ("alpha_dev", C.c_void_p),
         ("tri_id_dev", C.c_void_p),
         ("instance_id_dev", C.c_void_p),
+        ("normal_camera_dev", C.c_void_p),
+        ("mesh_id_dev", C.c_void_p),
+        ("material_id_dev", C.c_void_p),
     ]
 
 
@@ -159,7 +173,7 @@ class Request(C.Structure):
         ("camera", Camera),
         ("opts", RenderOpts),
         ("num_lights", C.c_int32),
-        ("light", Light),
+        ("lights", Light * MAX_LIGHTS),
         ("out", Outputs),
     ]
 
@@ -275,6 +289,7 @@ SIGNATURES = [
     ("vira_gpu_tlas_build", _S, [_P, C.POINTER(Instance), C.c_size_t]),
     ("vira_gpu_lights_set", _S, [_P, C.POINTER(Light), C.c_int]),
     ("vira_gpu_material_set", _S, [_P, C.c_int32, C.POINTER(Material)]),
+    ("vira_gpu_environment_set", _S, [_P, C.POINTER(C.c_float), _P, C.c_int, C.c_int]),
     ("vira_gpu_render", _S, [_P, C.POINTER(Camera), C.POINTER(RenderOpts), C.POINTER(Outputs), C.POINTER(RenderStats)]),
     ("vira_gpu_render_noalloc", _S, [_P, C.POINTER(Camera), C.POINTER(RenderOpts), C.POINTER(Outputs),
                                      C.POINTER(RenderStats)]),

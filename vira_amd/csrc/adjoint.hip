@@ -416,6 +416,9 @@ MeshG& di_scene_mesh(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_di_opts&
     VG_CHECK((size_t)cam.width * cam.height < (1ull << 30), VIRA_GPU_ERR_INVALID,
              std::string(who) + ": image too large");
     VG_CHECK(o.samples >= 1 && o.samples <= 65536, VIRA_GPU_ERR_INVALID, std::string(who) + ": samples 1..65536");
+    VG_CHECK(!m.mat.normal_map_dev && m.mat.tex_channels <= 1, VIRA_GPU_ERR_UNSUPPORTED,
+             std::string(who) + ": the differentiable model takes no normal map and a grey albedo texture");
+    VG_CHECK(!c.env_has_ambient, VIRA_GPU_ERR_UNSUPPORTED, std::string(who) + ": no ambient light in the model");
     return m;
 }
 

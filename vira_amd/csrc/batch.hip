@@ -1,7 +1,8 @@
 // batch.hip -- many render requests in one persistent, hipGraph-captured launch.
 //
 // Each request is CPUPathTracer::render (cpu_path_tracer.ipp:29-161) of one
-// mesh with one camera, options and light.  The work unit is the per-pixel
+// mesh with one camera, options and up to VIRA_GPU_MAX_LIGHTS lights, in the
+// context's light environment (ambient light, background).  The work unit is the per-pixel
 // renderer's 8x8 tile (fused.hpp: render_tile), so a request's passes are the
 // ones the per-request launch writes, bit for bit.  All requests' tiles form
 // one queue, ordered by request (longest rays-per-tile first by default); a
@@ -26,7 +27,8 @@ namespace {
 
 struct BatchReq {
     TileJob job;
-    LightF light;
+    JobExt ext;
+    LightF lights[VIRA_GPU_MAX_LIGHTS];
     uint32_t first;        // first queue item of this request
     uint32_t tile_stride;  // queue item i of the request renders tile i * tile_stride
 };
@@ -36,20 +38,21 @@ __global__ void k_batch_reset(unsigned long long* state) {
     if (threadIdx.x < 8) state[threadIdx.x] = 0ull;
 }
 
-template <int W, int WPB>
+template <int W, int WPB, bool MULTI, bool FEAT>
 __global__ void __launch_bounds__(kFusedBS* WPB, kFusedWavesPerSimd) k_render_batch(const BatchReq* __restrict__ reqs, int nreq,
                                                                 uint32_t nitems, uint32_t grab,
                                                                 float* __restrict__ acc,
                                                                 unsigned long long* __restrict__ state) {
+    constexpr int kL = fused_lds_floats<MULTI>();
     __shared__ uint32_t s_ref[WPB * kStackLds * kFusedBS];
     __shared__ float s_dist[WPB * kStackLds * kFusedBS];
-    __shared__ float s_contrib[WPB * NB * kFusedBS];
+    __shared__ float s_contrib[WPB * kL * kFusedBS];
     const uint32_t wave = threadIdx.x / kFusedBS, lane = threadIdx.x % kFusedBS;
     const uint32_t gw = blockIdx.x * WPB + wave;
     float* __restrict__ a = acc + (size_t)gw * (A_FIELDS * 64);
     uint32_t* s_ref_w = s_ref + wave * (kStackLds * kFusedBS);
     float* s_dist_w = s_dist + wave * (kStackLds * kFusedBS);
-    float* s_contrib_w = s_contrib + wave * (NB * kFusedBS);
+    float* s_contrib_w = s_contrib + wave * (kL * kFusedBS);
     uint32_t n_prim = 0, n_shadow = 0, n_bounce = 0, n_hit = 0;
     uint32_t* head = reinterpret_cast<uint32_t*>(state);
     while (true) {
@@ -68,7 +71,8 @@ __global__ void __launch_bounds__(kFusedBS* WPB, kFusedWavesPerSimd) k_render_ba
             }
             const BatchReq& R = reqs[lo];
             const uint32_t tile = (k - R.first) * R.tile_stride;
-            render_tile<W>(R.job, tile, a, s_ref_w, s_dist_w, s_contrib_w, lane, n_prim, n_shadow, n_bounce, n_hit);
+            render_tile<W, false, MULTI, FEAT>(R.job, tile, a, s_ref_w, s_dist_w, s_contrib_w, lane, n_prim, n_shadow,
+                                         n_bounce, n_hit);
         }
     }
     wave_add(&state[2], n_prim);
@@ -79,15 +83,24 @@ __global__ void __launch_bounds__(kFusedBS* WPB, kFusedWavesPerSimd) k_render_ba
 
 using BatchKernel = void (*)(const BatchReq*, int, uint32_t, uint32_t, float*, unsigned long long*);
 
-BatchKernel pick_kernel(int width, int wpb) {
+template <bool MULTI, bool FEAT>
+BatchKernel pick_kernel_t(int width, int wpb) {
     if (width == 4) {
-        if (wpb == 1) return k_render_batch<4, 1>;
-        if (wpb == 2) return k_render_batch<4, 2>;
-        return k_render_batch<4, 4>;
+        if (wpb == 1) return k_render_batch<4, 1, MULTI, FEAT>;
+        if (wpb == 2) return k_render_batch<4, 2, MULTI, FEAT>;
+        return k_render_batch<4, 4, MULTI, FEAT>;
     }
-    if (wpb == 1) return k_render_batch<2, 1>;
-    if (wpb == 2) return k_render_batch<2, 2>;
-    return k_render_batch<2, 4>;
+    if (wpb == 1) return k_render_batch<2, 1, MULTI, FEAT>;
+    if (wpb == 2) return k_render_batch<2, 2, MULTI, FEAT>;
+    return k_render_batch<2, 4, MULTI, FEAT>;
+}
+// Several lights or ambient light in any request: the variant that keeps the
+// vertex in LDS (it gives single-light requests the same numbers); a shown
+// background, adaptive sampling or a normal/spectral map in any request: the
+// feature variant (likewise the same numbers for the others).
+BatchKernel pick_kernel(int width, int wpb, bool multi, bool feat) {
+    if (multi) return feat ? pick_kernel_t<true, true>(width, wpb) : pick_kernel_t<true, false>(width, wpb);
+    return feat ? pick_kernel_t<false, true>(width, wpb) : pick_kernel_t<false, false>(width, wpb);
 }
 
 void check_config(const vira_gpu_batch_config& g) {
@@ -116,6 +129,7 @@ void batch_submit(Ctx& c, const vira_gpu_request* reqs, size_t n, const vira_gpu
     std::vector<uint32_t> order(n);
     std::iota(order.begin(), order.end(), 0u);
     std::vector<double> cost(n);
+    bool multi = c.env_has_ambient, feat = false;
     for (size_t i = 0; i < n; ++i) {
         const vira_gpu_request& r = reqs[i];
         MeshG& m = c.mesh(r.mesh_id);
@@ -128,12 +142,16 @@ void batch_submit(Ctx& c, const vira_gpu_request* reqs, size_t n, const vira_gpu
                  VIRA_GPU_ERR_INVALID, "batch: request " + std::to_string(i) + ": bad image size");
         VG_CHECK(r.opts.samples > 0 && r.opts.bounces >= 0 && r.opts.bounces < 64, VIRA_GPU_ERR_INVALID,
                  "batch: request " + std::to_string(i) + ": bad samples/bounces");
-        VG_CHECK(r.num_lights == 0 || r.num_lights == 1, VIRA_GPU_ERR_UNSUPPORTED,
-                 "batch: request " + std::to_string(i) + ": 0 or 1 light per request");
-        VG_CHECK(r.num_lights == 0 || r.light.kind == 0 || r.light.kind == 1, VIRA_GPU_ERR_INVALID,
-                 "batch: request " + std::to_string(i) + ": unknown light kind");
-        VG_CHECK(r.num_lights == 0 || r.light.kind == 0 || r.light.radius > 0, VIRA_GPU_ERR_INVALID,
-                 "batch: request " + std::to_string(i) + ": SphereLight radius must be positive");
+        VG_CHECK(r.num_lights >= 0 && r.num_lights <= VIRA_GPU_MAX_LIGHTS, VIRA_GPU_ERR_INVALID,
+                 "batch: request " + std::to_string(i) + ": 0.." + std::to_string(VIRA_GPU_MAX_LIGHTS) + " lights");
+        for (int l = 0; l < r.num_lights; ++l) {
+            VG_CHECK(r.lights[l].kind == 0 || r.lights[l].kind == 1, VIRA_GPU_ERR_INVALID,
+                     "batch: request " + std::to_string(i) + ": unknown light kind");
+            VG_CHECK(r.lights[l].kind == 0 || r.lights[l].radius > 0, VIRA_GPU_ERR_INVALID,
+                     "batch: request " + std::to_string(i) + ": SphereLight radius must be positive");
+        }
+        if (r.num_lights > 1) multi = true;
+        if ((r.opts.show_background && c.env_bg) || r.opts.adaptive_sampling || material_feat(m.mat)) feat = true;
         cost[i] = (double)r.opts.samples * (1.0 + r.opts.bounces);
     }
     if (cfg.order == 0)
@@ -149,15 +167,15 @@ void batch_submit(Ctx& c, const vira_gpu_request* reqs, size_t n, const vira_gpu
         MeshG& m = c.mesh(r.mesh_id);
         const TileGrid g(r.camera);
         BatchReq& d = hr[qi];
-        d.light = r.num_lights ? light_params(r.light) : LightF{};
+        for (int l = 0; l < VIRA_GPU_MAX_LIGHTS; ++l) d.lights[l] = l < r.num_lights ? light_params(r.lights[l]) : LightF{};
         d.job.nodes = m.blas.nodes.p;
         d.job.tris = m.blas.tris.as<float4>();
         d.job.verts = m.verts.as<GVertex>();
         d.job.idx = m.idx.as<uint32_t>();
         d.job.mat = mat_params(m);
-        // the light lives in this descriptor's device copy
-        d.job.lights = Lights{r.num_lights, reinterpret_cast<const LightF*>(
-                                                reinterpret_cast<const char*>(dev_base + qi) + offsetof(BatchReq, light))};
+        // the lights live in this descriptor's device copy
+        d.job.lights = Lights{r.num_lights, reinterpret_cast<const LightF*>(reinterpret_cast<const char*>(dev_base + qi) +
+                                                                            offsetof(BatchReq, lights))};
         d.job.cam = cam_params(r.camera);
         d.job.seed = r.opts.seed;
         d.job.spp = r.opts.samples;
@@ -165,11 +183,14 @@ void batch_submit(Ctx& c, const vira_gpu_request* reqs, size_t n, const vira_gpu
         d.job.rr = r.opts.russian_roulette;
         d.job.rr_start = r.opts.rr_start;
         d.job.tiles_x = g.tiles_x;
-        d.job.out = OutPtrs{r.out.radiance_dev, r.out.direct_dev, r.out.indirect_dev, r.out.albedo_dev,
-                            r.out.normal_dev, r.out.depth_dev, r.out.alpha_dev, r.out.tri_id_dev,
-                            r.out.instance_id_dev};
         d.job.inst = nullptr;
         d.job.ninst = 0;
+        d.ext.out = out_params(r.out);
+        d.ext.env = env_params(c, r.opts);
+        d.ext.adapt = adaptive_params(r.opts);
+        d.ext.pass = pass_params(r.camera, r.mesh_id, m);
+        d.job.ext = reinterpret_cast<const JobExt*>(reinterpret_cast<const char*>(dev_base + qi) +
+                                                    offsetof(BatchReq, ext));
         d.first = (uint32_t)items;
         d.tile_stride = (uint32_t)stride;
         const uint64_t nt = (g.tiles + (uint32_t)stride - 1) / (uint32_t)stride;
@@ -189,7 +210,8 @@ void batch_submit(Ctx& c, const vira_gpu_request* reqs, size_t n, const vira_gpu
     const std::vector<uint64_t> key = {(uint64_t)(uintptr_t)c.b_reqs.p, (uint64_t)(uintptr_t)c.b_acc.p,
                                        (uint64_t)(uintptr_t)c.b_state.p, (uint64_t)n, items,
                                        (uint64_t)cfg.bvh_width, (uint64_t)cfg.waves_per_block,
-                                       (uint64_t)cfg.waves_per_cu, (uint64_t)cfg.tiles_per_grab, (uint64_t)blocks};
+                                       (uint64_t)cfg.waves_per_cu, (uint64_t)cfg.tiles_per_grab, (uint64_t)blocks,
+                                       (uint64_t)multi, (uint64_t)feat};
     bool reused = c.b_exec && c.b_key == key;
     if (!reused) {
         if (c.b_exec) {
@@ -201,7 +223,7 @@ void batch_submit(Ctx& c, const vira_gpu_request* reqs, size_t n, const vira_gpu
         hipError_t e1 = hipSuccess, e2 = hipSuccess;
         k_batch_reset<<<1, 64, 0, c.stream>>>(c.b_state.as<unsigned long long>());
         e1 = hipGetLastError();
-        BatchKernel kern = pick_kernel(cfg.bvh_width, wpb);
+        BatchKernel kern = pick_kernel(cfg.bvh_width, wpb, multi, feat);
         kern<<<blocks, kFusedBS * wpb, 0, c.stream>>>(c.b_reqs.as<BatchReq>(), (int)n, (uint32_t)items,
                                                       (uint32_t)cfg.tiles_per_grab, c.b_acc.as<float>(),
                                                       c.b_state.as<unsigned long long>());
@@ -215,19 +237,15 @@ void batch_submit(Ctx& c, const vira_gpu_request* reqs, size_t n, const vira_gpu
         VG_HIP(ei);
         c.b_key = key;
     }
-    hipEvent_t e0, e1;
-    VG_HIP(hipEventCreate(&e0));
-    VG_HIP(hipEventCreate(&e1));
-    VG_HIP(hipEventRecord(e0, c.stream));
+    EventPair ev;
+    VG_HIP(hipEventRecord(ev.a, c.stream));
     VG_HIP(hipGraphLaunch(c.b_exec, c.stream));
-    VG_HIP(hipEventRecord(e1, c.stream));
+    VG_HIP(hipEventRecord(ev.b, c.stream));
     unsigned long long hc[8];
     VG_HIP(hipMemcpyAsync(hc, c.b_state.p, sizeof(hc), hipMemcpyDeviceToHost, c.stream));
     VG_HIP(hipStreamSynchronize(c.stream));
     float ms = 0.f;
-    VG_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    VG_HIP(hipEventElapsedTime(&ms, ev.a, ev.b));
     if (st) {
         *st = vira_gpu_batch_stats{};
         st->primary_rays = hc[2];

@@ -47,6 +47,10 @@ int32_t new_mesh(vg::Ctx& c) {
     m.texture_dev = nullptr;
     m.tex_width = m.tex_height = 0;
     m.smooth_shading = 0;
+    m.tex_channels = 1;
+    m.normal_map_dev = nullptr;
+    m.nmap_width = m.nmap_height = 0;
+    m.material_id = 0;
     return (int32_t)c.meshes.size() - 1;
 }
 
@@ -316,7 +320,31 @@ vira_gpu_status vira_gpu_material_set(vira_gpu_ctx* ctx, int32_t mesh_id, const 
         VG_CHECK(mat, VIRA_GPU_ERR_INVALID, "material_set: null material");
         VG_CHECK(!mat->texture_dev || (mat->tex_width > 0 && mat->tex_height > 0), VIRA_GPU_ERR_INVALID,
                  "material_set: texture needs positive size");
+        VG_CHECK(mat->tex_channels == 0 || mat->tex_channels == 1 || mat->tex_channels == VIRA_GPU_NUM_BINS,
+                 VIRA_GPU_ERR_INVALID, "material_set: tex_channels must be 1 (grey) or 8 (spectral)");
+        VG_CHECK(!mat->normal_map_dev || (mat->nmap_width > 0 && mat->nmap_height > 0), VIRA_GPU_ERR_INVALID,
+                 "material_set: normal map needs positive size");
         m.mat = *mat;
+        if (m.mat.tex_channels == 0) m.mat.tex_channels = 1;
+    });
+}
+
+vira_gpu_status vira_gpu_environment_set(vira_gpu_ctx* ctx, const float* ambient, const float* background,
+                                         int bg_width, int bg_height) {
+    return guarded([&] {
+        vg::Ctx& c = C(ctx);
+        VG_CHECK(!background || (bg_width > 0 && bg_height > 0), VIRA_GPU_ERR_INVALID,
+                 "environment_set: background needs positive size");
+        c.scene_gen++;
+        bool any = false;
+        for (int k = 0; k < VIRA_GPU_NUM_BINS; ++k) {
+            c.env_ambient[k] = ambient ? ambient[k] : 0.0f;
+            any = any || c.env_ambient[k] != 0.0f;
+        }
+        c.env_has_ambient = any;  // Scene::setAmbient: hasAmbient when the magnitude is not 0
+        c.env_bg = background;
+        c.env_bw = background ? bg_width : 0;
+        c.env_bh = background ? bg_height : 0;
     });
 }
 

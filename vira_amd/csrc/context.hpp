@@ -166,9 +166,14 @@ struct Ctx {
     std::vector<vira_gpu_instance> tlas;
     bool instanced = false;   // TLAS other than one identity instance: rendered camera-relative
     std::vector<vira_gpu_light> lights;
+    // Scene::getAmbient / hasAmbient and the background image (getBackgroundEmission)
+    float env_ambient[VIRA_GPU_NUM_BINS] = {};
+    bool env_has_ambient = false;
+    const float* env_bg = nullptr;
+    int env_bw = 0, env_bh = 0;
     // render scratch, grown on demand
     DBuf r_ray_o, r_ray_d, r_hit, r_queue[2], r_beta, r_path_rad, r_path_dir, r_srays_o, r_srays_d,
-        r_scontrib, r_svis, r_sinfo, r_acc, r_counters, r_lights, r_inst, r_inst_m, r_light_pos;
+        r_scontrib, r_svis, r_sinfo, r_acc, r_counters, r_lights, r_inst, r_inst_m, r_light_pos, r_ext;
     // timing events of vira_gpu_render_noalloc, created with the context
     hipEvent_t na_ev[2] = {};
     // bumped by every call that changes geometry, BVHs, lights or materials

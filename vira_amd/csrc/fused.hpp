@@ -37,16 +37,17 @@ enum : int {
     A_DEPTH = 43,
     A_COUNT = 44,
     A_TRI = 45,
-    A_FIELDS = 46
+    A_S1 = 46,  // adaptive sampling: sum of the samples' radiance magnitudes
+    A_S2 = 47,  //                    and of their squares
+    A_FIELDS = 48
 };
-// Per-path fields of the wavefront schedule (one path per pixel per sample pass).
 // Accumulator of pixel q: base of its tile block; field f is base[f * 64 + (q & 63)].
 __device__ __forceinline__ float* acc_tile(float* acc, uint32_t q) { return acc + (size_t)(q >> 6) * (A_FIELDS * 64); }
 __device__ __forceinline__ const float* acc_tile(const float* acc, uint32_t q) {
     return acc + (size_t)(q >> 6) * (A_FIELDS * 64);
 }
 // The empty asm makes the lane offset opaque at each use, so the compiler does
-// not hoist 46 per-field addresses out of the render loop into live registers.
+// not hoist 48 per-field addresses out of the render loop into live registers.
 __device__ __forceinline__ float& acc_ref(float* base, int f, uint32_t lane) {
     uint32_t l = lane;
     asm volatile("" : "+v"(l));
@@ -76,15 +77,27 @@ struct OutPtrs {
     float *radiance, *direct, *indirect, *albedo, *normal, *depth, *alpha;
     uint32_t* tri;
     uint32_t* inst;
+    float* normal_camera;
+    uint32_t* mesh_id;
+    uint32_t* material_id;
+};
+
+// Per-request values the passes need besides the sums: the view normal
+// matrix (row-major) and the mesh/material ids of an uninstanced scene.
+struct PassInfo {
+    float vn[9];
+    uint32_t mesh_id, material_id;
 };
 
 // The first-hit id field holds instance << 27 | triangle (triangles < 2^27).
 constexpr uint32_t kTriBits = 27;
 
-// Averages and passes (cpu_path_tracer.ipp:229-239, passes.ipp:81-127).
-__device__ __forceinline__ void write_pixel(const OutPtrs& o, size_t p, int spp, const float* __restrict__ a,
-                                            uint32_t lane) {
-    float sf = (float)spp;
+// Averages and passes (cpu_path_tracer.ipp:229-239, passes.ipp:81-127); ns is
+// the sample count the reference divides by (the samples taken, or the index
+// of the sample at which adaptive sampling stopped).
+__device__ __forceinline__ void write_pixel(const OutPtrs& o, size_t p, int ns, const float* __restrict__ a,
+                                            uint32_t lane, const PassInfo& pi, const void* inst_table) {
+    float sf = (float)ns;
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
         float rad = __fdiv_rn(ACC(a, A_RAD_S + k, lane), sf);
@@ -95,18 +108,41 @@ __device__ __forceinline__ void write_pixel(const OutPtrs& o, size_t p, int spp,
         if (o.albedo) o.albedo[NB * p + k] = __fdiv_rn(ACC(a, A_ALB + k, lane), sf);
     }
     float cnt = ACC(a, A_COUNT, lane);
-    if (o.normal) {
+    if (o.normal || o.normal_camera) {
         f3 n = mk3(ACC(a, A_NRM, lane), ACC(a, A_NRM + 1, lane), ACC(a, A_NRM + 2, lane));
         if (cnt > 0.0f) n = normalize(n);
-        o.normal[3 * p] = n.x;
-        o.normal[3 * p + 1] = n.y;
-        o.normal[3 * p + 2] = n.z;
+        if (o.normal) {
+            o.normal[3 * p] = n.x;
+            o.normal[3 * p + 1] = n.y;
+            o.normal[3 * p + 2] = n.z;
+        }
+        if (o.normal_camera) {  // normalize(getViewNormalMatrix() * normal_global) (cpu_path_tracer.ipp:237-239)
+            f3 c = mk3(0.f, 0.f, 0.f);
+            if (cnt > 0.0f)
+                c = normalize(mk3((pi.vn[0] * n.x + pi.vn[1] * n.y) + pi.vn[2] * n.z,
+                                  (pi.vn[3] * n.x + pi.vn[4] * n.y) + pi.vn[5] * n.z,
+                                  (pi.vn[6] * n.x + pi.vn[7] * n.y) + pi.vn[8] * n.z));
+            o.normal_camera[3 * p] = c.x;
+            o.normal_camera[3 * p + 1] = c.y;
+            o.normal_camera[3 * p + 2] = c.z;
+        }
     }
     if (o.depth) o.depth[p] = cnt > 0.0f ? __fdiv_rn(ACC(a, A_DEPTH, lane), cnt) : kInf;
     if (o.alpha) o.alpha[p] = __fdiv_rn(cnt, sf);
     const uint32_t first = __float_as_uint(ACC(a, A_TRI, lane));
-    if (o.tri) o.tri[p] = first == 0xffffffffu ? first : (first & ((1u << kTriBits) - 1u));
-    if (o.inst) o.inst[p] = first == 0xffffffffu ? first : (first >> kTriBits);
+    const bool none = first == 0xffffffffu;
+    if (o.tri) o.tri[p] = none ? first : (first & ((1u << kTriBits) - 1u));
+    if (o.inst) o.inst[p] = none ? first : (first >> kTriBits);
+    if (o.mesh_id || o.material_id) {
+        uint32_t mid = pi.mesh_id, tid = pi.material_id;
+        if (inst_table && !none) {
+            const InstG& I = static_cast<const InstG*>(inst_table)[first >> kTriBits];
+            mid = I.mesh_id;
+            tid = I.material_id;
+        }
+        if (o.mesh_id) o.mesh_id[p] = none ? 0xffffffffu : mid;
+        if (o.material_id) o.material_id[p] = none ? 0xffffffffu : tid;
+    }
 }
 
 // Kahan-compensated add of v[] (or zeros) into the (sum, comp) fields fs, fc.
@@ -127,6 +163,26 @@ __device__ __forceinline__ void acc_clear(float* __restrict__ a, uint32_t lane) 
 
 enum : int { M_NEW = 0, M_CLOSEST = 1, M_SHADOW = 2, M_DONE = 3 };
 
+// Adaptive sampling (CPUPathTracerOptions::adaptive_sampling and its three
+// parameters, cpu_path_tracer.hpp:39-42); adaptive = 0 takes every sample.
+struct Adaptive {
+    int on;
+    int batch;       // samples_per_batch
+    int miss;        // samples_to_detect_miss
+    float tol;       // sampling_tolerance
+};
+
+// What a render request needs besides the render loop's own inputs: the
+// output passes, the light environment, adaptive sampling and the pass
+// values.  It lives in device memory and the loop reaches it by one pointer,
+// so the kernel's uniform registers hold only what the traversal uses.
+struct JobExt {
+    OutPtrs out;
+    EnvF env;
+    Adaptive adapt;
+    PassInfo pass;
+};
+
 // Everything one render request needs per tile (kernel argument of the
 // per-request launch; device-memory descriptor of the batched launch).
 struct TileJob {
@@ -140,17 +196,72 @@ struct TileJob {
     uint32_t seed;
     int spp, max_bounces, rr, rr_start;
     uint32_t tiles_x;
-    OutPtrs out;
     const InstG* inst;  // instanced scenes: the camera-relative instance table
     int ninst;
+    const JobExt* ext;  // device memory
 };
 
+// Per-lane LDS of the several-light renderer: the vertex's throughput before
+// the bounce update, shading normal, albedo and its radiance so far.
+enum : int { ML_BETA = 0, ML_N = 8, ML_ALB = 11, ML_VRAD = 19, ML_FIELDS = 27 };
+// Per-lane LDS floats of the per-pixel renderers besides the stack.
+template <bool MULTI>
+constexpr int fused_lds_floats() { return MULTI ? ML_FIELDS : NB; }
+
+// End of sample s of a lane's pixel: fold the path radiance into the Kahan
+// sums and, with adaptive sampling, apply the reference's stopping rules
+// (cpu_path_tracer.ipp:198-227).  Returns true when the pixel is finished
+// (s is then the divisor, as dataPayload.sample is in the reference).
+template <bool FEAT>
+__device__ __forceinline__ bool end_sample(float* __restrict__ a, uint32_t lane, const float path[NB],
+                                           const Adaptive& ad, uint32_t& s) {
+    kahan_acc(a, lane, A_RAD_S, A_RAD_C, path, false);
+    if (FEAT && ad.on) {
+        float sq = 0.0f;
+#pragma unroll
+        for (int k = 0; k < NB; ++k) sq = sq + path[k] * path[k];
+        const float illum = __builtin_sqrtf(sq);
+        const float s1 = ACC(a, A_S1, lane) + illum;
+        const float s2 = ACC(a, A_S2, lane) + illum * illum;
+        ACC(a, A_S1, lane) = s1;
+        ACC(a, A_S2, lane) = s2;
+        bool stop = false;
+        if (s % (uint32_t)ad.miss == 0 && s != 0 && ACC(a, A_DEPTH, lane) == 0.0f) stop = true;
+        if (!stop && s % (uint32_t)ad.batch == 0 && s != 0) {
+            if (s1 == 0.0f) {
+                stop = true;
+            } else {
+                const float n = (float)s + 1.0f;
+                const float mu = __fdiv_rn(s1, n);
+                const float var = __fdiv_rn(1.0f, n - 1.0f) * (s2 - __fdiv_rn(s1 * s1, n));
+                const float I = 1.96f * __builtin_sqrtf(__fdiv_rn(var, n));
+                if (I <= ad.tol * mu) stop = true;
+            }
+        }
+        if (stop) return true;
+    }
+    s++;
+    return false;
+}
+
 // One wave renders the 8x8 tile `tile` of job J: lane = pixel; accumulators at
-// a (this wave's A_FIELDS x 64 block); LDS slices of this wave.  Adds the
-// wave's ray counts to n[0..2] (per lane).
-template <int W, bool INST = false>
+// a (this wave's A_FIELDS x 64 block); LDS slices of this wave (s_lds holds
+// fused_lds_floats<MULTI>() x 64 floats).  Adds the wave's ray counts to the
+// per-lane counters.
+//
+// FEAT = false compiles out background, adaptive sampling, normal maps and
+// spectral albedo maps (the caller picks FEAT = true when any is in use).
+// MULTI = false: at most one light and no ambient light; the light's
+// contribution is staged in LDS while its shadow ray is traced.
+// MULTI = true: any number of lights and ambient light; the vertex's
+// throughput, normal, albedo and radiance stay in LDS while the shadow rays of
+// the lights are traced one after the other, and each light's (deterministic)
+// sample is drawn again when its shadow ray returns unoccluded.  Both give the
+// numbers of the reference's loop (cpu_path_tracer.ipp:384-428): radiance =
+// 0 (+ ambient at bounce 0) + the lit lights' terms in light order.
+template <int W, bool INST = false, bool MULTI = false, bool FEAT = false>
 __device__ __forceinline__ void render_tile(const TileJob& J, uint32_t tile, float* __restrict__ a,
-                                            uint32_t* s_ref, float* s_dist, float* s_contrib, uint32_t lane,
+                                            uint32_t* s_ref, float* s_dist, float* s_lds, uint32_t lane,
                                             uint32_t& n_prim, uint32_t& n_shadow, uint32_t& n_bounce,
                                             uint32_t& n_hit) {
     const void* __restrict__ nodes = J.nodes;
@@ -177,7 +288,28 @@ __device__ __forceinline__ void render_tile(const TileJob& J, uint32_t tile, flo
     uint32_t s = 0, b = 0;
     int mode = valid ? M_NEW : M_DONE;
     bool has_next = false;
+    int li = 0;  // MULTI: the light whose shadow ray is in flight
     const bool one_light = lights.n > 0;
+    float* const sm = s_lds + lane;  // this lane's LDS column (stride kFusedBS)
+    // MULTI: from light l0 on, the first light that can be sampled gets its
+    // shadow ray; false when none is left
+    auto next_light = [&](int l0) -> bool {
+        for (int l = l0; l < lights.n; ++l) {
+            float lpdf, lrad[NB], dist;
+            f3 ld;
+            sample_light(lights.l[l], o, seed, pix, s, (b << 8) | (SLOT_LIGHT0 + 2u * (uint32_t)l), ld, dist, lpdf,
+                         lrad);
+            if (lpdf > 0.0f) {
+                d = ld;
+                tmax = dist;
+                li = l;
+                mode = M_SHADOW;
+                n_shadow++;
+                return true;
+            }
+        }
+        return false;
+    };
     while (true) {
         if (mode == M_NEW) {
             if ((int)s < spp) {
@@ -212,46 +344,89 @@ __device__ __forceinline__ void render_tile(const TileJob& J, uint32_t tile, flo
         bool vertex_done = false, lit = false;
         if (mode == M_CLOSEST) {
             if (h.id == 0xffffffffu) {
-                // miss: the path ends (no background, cpu_path_tracer.ipp:256-270)
+                // miss: background radiance when shown (cpu_path_tracer.ipp:256-270)
+                if (FEAT && J.ext->env.show_bg) {
+                    float bg[NB];
+                    background_radiance(J.ext->env, d, bg);
+#pragma unroll
+                    for (int k = 0; k < NB; ++k) path[k] = path[k] + beta[k] * (bg[k] * 1.0f);
+                }
                 if (b == 0) kahan_acc(a, lane, A_DIR_S, A_DIR_C, path, true);
-                kahan_acc(a, lane, A_RAD_S, A_RAD_C, path, false);
-                s++;
-                mode = M_NEW;
+                mode = end_sample<FEAT>(a, lane, path, J.ext->adapt, s) ? M_DONE : M_NEW;
             } else {
-                Surface S = INST ? surface_instance(J.inst[hinst], h.id, h.b1, h.b2)
-                                 : surface_at(verts, idx, mat, h.id, h.b1, h.b2);
+                Surface S = INST ? surface_instance<FEAT>(J.inst[hinst], h.id, h.b1, h.b2)
+                                 : surface_at<FEAT>(verts, idx, mat, h.id, h.b1, h.b2);
                 n_hit++;
                 if (b == 0) accumulate_first_hit(a, lane, h.id | (hinst << kTriBits), h.t, S);
-                f3 ld = mk3(0.f, 0.f, 1.f);
-                float dist = 0.f, c[NB];
-                bool sh = one_light &&
-                          light_contrib(S, lights.l[0], beta, seed, pix, s, (b << 8) | SLOT_LIGHT0, ld, dist, c);
-                if (sh) {
-#pragma unroll
-                    for (int k = 0; k < NB; ++k) s_contrib[k * kFusedBS + lane] = c[k];
-                }
-                has_next = (int)b < max_bounces && sample_bounce(S, lights, beta, seed, pix, s, b, rr, rr_start, nd);
                 o = S.org;
-                if (sh) {
-                    d = ld;
-                    tmax = dist;
-                    mode = M_SHADOW;
-                    n_shadow++;
+                if (MULTI) {
+                    // vertex radiance starts at 0 (+ ambient x albedo at bounce 0, cpu_path_tracer.ipp:389-396)
+#pragma unroll
+                    for (int k = 0; k < NB; ++k) {
+                        float vr = 0.0f;
+                        if (b == 0 && J.ext->env.has_ambient) vr = vr + beta[k] * (J.ext->env.ambient[k] * S.alb[k]);
+                        sm[(ML_VRAD + k) * kFusedBS] = vr;
+                        sm[(ML_BETA + k) * kFusedBS] = beta[k];
+                        sm[(ML_ALB + k) * kFusedBS] = S.alb[k];
+                    }
+                    sm[(ML_N + 0) * kFusedBS] = S.N.x;
+                    sm[(ML_N + 1) * kFusedBS] = S.N.y;
+                    sm[(ML_N + 2) * kFusedBS] = S.N.z;
+                    has_next =
+                        (int)b < max_bounces && sample_bounce(S, lights, beta, seed, pix, s, b, rr, rr_start, nd);
+                    if (!next_light(0)) vertex_done = true;
                 } else {
-                    vertex_done = true;
+                    f3 ld = mk3(0.f, 0.f, 1.f);
+                    float dist = 0.f, c[NB];
+                    bool sh = one_light && light_contrib(S, lights.l[0], beta, seed, pix, s,
+                                                         (b << 8) | SLOT_LIGHT0, ld, dist, c);
+                    if (sh) {
+#pragma unroll
+                        for (int k = 0; k < NB; ++k) sm[k * kFusedBS] = c[k];
+                    }
+                    has_next =
+                        (int)b < max_bounces && sample_bounce(S, lights, beta, seed, pix, s, b, rr, rr_start, nd);
+                    if (sh) {
+                        d = ld;
+                        tmax = dist;
+                        mode = M_SHADOW;
+                        n_shadow++;
+                    } else {
+                        vertex_done = true;
+                    }
                 }
             }
         } else if (mode == M_SHADOW) {
             lit = h.id == 0xffffffffu;
-            vertex_done = true;
+            if (MULTI) {
+                if (lit) {
+                    // this light's term, drawn again from the kept surface values
+                    float bo[NB], al[NB], c[NB], dist;
+                    f3 ld;
+#pragma unroll
+                    for (int k = 0; k < NB; ++k) {
+                        bo[k] = sm[(ML_BETA + k) * kFusedBS];
+                        al[k] = sm[(ML_ALB + k) * kFusedBS];
+                    }
+                    const f3 Nv = mk3(sm[ML_N * kFusedBS], sm[(ML_N + 1) * kFusedBS], sm[(ML_N + 2) * kFusedBS]);
+                    light_contrib_at(o, Nv, al, lights.l[li], bo, seed, pix, s,
+                                     (b << 8) | (SLOT_LIGHT0 + 2u * (uint32_t)li), ld, dist, c);
+#pragma unroll
+                    for (int k = 0; k < NB; ++k) sm[(ML_VRAD + k) * kFusedBS] = sm[(ML_VRAD + k) * kFusedBS] + c[k];
+                }
+                if (!next_light(li + 1)) vertex_done = true;
+            } else {
+                vertex_done = true;
+            }
         }
         if (vertex_done) {
-            // radiance = 0 + lit contribution; pathRadiance += radiance (cpu_path_tracer.ipp:272,384-428);
-            // the direct pass is the bounce-0 radiance (:463-465)
+            // pathRadiance += radiance (cpu_path_tracer.ipp:272,384-428); the direct pass is the
+            // bounce-0 radiance (:463-465)
             float v[NB];
 #pragma unroll
             for (int k = 0; k < NB; ++k) {
-                v[k] = 0.0f + (lit ? s_contrib[k * kFusedBS + lane] : 0.0f);
+                if (MULTI) v[k] = sm[(ML_VRAD + k) * kFusedBS];
+                else v[k] = 0.0f + (lit ? sm[k * kFusedBS] : 0.0f);
                 path[k] = path[k] + v[k];
             }
             if (b == 0) kahan_acc(a, lane, A_DIR_S, A_DIR_C, v, false);
@@ -262,13 +437,11 @@ __device__ __forceinline__ void render_tile(const TileJob& J, uint32_t tile, flo
                 mode = M_CLOSEST;
                 n_bounce++;
             } else {
-                kahan_acc(a, lane, A_RAD_S, A_RAD_C, path, false);
-                s++;
-                mode = M_NEW;
+                mode = end_sample<FEAT>(a, lane, path, J.ext->adapt, s) ? M_DONE : M_NEW;
             }
         }
     }
-    if (valid) write_pixel(J.out, pix, spp, a, lane);
+    if (valid) write_pixel(J.ext->out, pix, (int)s, a, lane, J.ext->pass, INST ? (const void*)J.inst : nullptr);
 }
 
 }  // namespace vg

@@ -29,6 +29,7 @@ struct InstG {
     float nm[9];    // normal matrix: transpose(inverse(linear part)), row-major
     float blo[3], bhi[3];  // the BLAS's box in the mesh frame (lo > hi: empty)
     float wlo[3], whi[3];  // that box placed in the camera-relative frame, rounded outward and padded
+    uint32_t mesh_id, material_id;  // for the mesh_id and material_id passes
 };
 
 // GLM order: ((m0 x + m1 y) + m2 z) + m3 per row.
@@ -138,10 +139,12 @@ __device__ __forceinline__ HitRec trace_instances(const InstG* __restrict__ inst
 
 // A hit on instance I: local hit, then model matrix for the point and normal
 // matrix for the normals (not renormalised, as the reference).
+template <bool FEAT = true>
 __device__ __forceinline__ Surface surface_instance(const InstG& I, uint32_t id, float b1, float b2) {
     LocalHit L;
-    local_hit(I.verts, I.idx, I.mat, id, b1, b2, L);
-    return finish_surface(xform_point(I.l2w, L.P), xform_normal(I.nm, L.Nl), xform_normal(I.nm, L.fn), L.alb);
+    local_hit<FEAT>(I.verts, I.idx, I.mat, id, b1, b2, L);
+    return finish_surface<FEAT>(xform_point(I.l2w, L.P), xform_normal(I.nm, L.Nl), xform_normal(I.nm, L.fn), L.alb,
+                                I.mat, L.u, L.v);
 }
 
 }  // namespace vg

@@ -69,16 +69,17 @@ __global__ void __launch_bounds__(kTraceBS) k_trace_rays(const void* __restrict_
 
 // ============================================================ fused schedule
 // Per-request launch: block b renders tile b.
-template <int W, bool INST>
+template <int W, bool INST, bool MULTI, bool FEAT>
 __global__ void __launch_bounds__(kFusedBS, kFusedWavesPerSimd) k_render_fused(TileJob J, float* __restrict__ acc,
                                                          unsigned long long* __restrict__ counters) {
     __shared__ uint32_t s_ref[kStackLds * kFusedBS];
     __shared__ float s_dist[kStackLds * kFusedBS];
-    __shared__ float s_contrib[NB * kFusedBS];  // staged light contribution while the shadow ray is traced
+    // staged light contribution (one light) or the vertex state (several lights)
+    __shared__ float s_lds[fused_lds_floats<MULTI>() * kFusedBS];
     const uint32_t lane = threadIdx.x;
     uint32_t n_prim = 0, n_shadow = 0, n_bounce = 0, n_hit = 0;
-    render_tile<W, INST>(J, blockIdx.x, acc + (size_t)blockIdx.x * (A_FIELDS * 64), s_ref, s_dist, s_contrib, lane,
-                         n_prim, n_shadow, n_bounce, n_hit);
+    render_tile<W, INST, MULTI, FEAT>(J, blockIdx.x, acc + (size_t)blockIdx.x * (A_FIELDS * 64), s_ref, s_dist, s_lds,
+                                lane, n_prim, n_shadow, n_bounce, n_hit);
     wave_add(&counters[0], n_prim);
     wave_add(&counters[1], n_shadow);
     wave_add(&counters[2], n_bounce);
@@ -256,13 +257,13 @@ __global__ void k_acc_init(uint32_t nq, float* acc) {
 }
 
 __global__ void k_finalize(uint32_t nq, uint32_t tiles_x, int W, int H, int spp, const float* __restrict__ acc,
-                           OutPtrs out) {
+                           OutPtrs out, PassInfo pi) {
     uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nq) return;
     uint32_t x, y;
     tile_xy(q, tiles_x, x, y);
     if (x >= (uint32_t)W || y >= (uint32_t)H) return;
-    write_pixel(out, x + (size_t)y * W, spp, acc_tile(acc, q), q & 63u);
+    write_pixel(out, x + (size_t)y * W, spp, acc_tile(acc, q), q & 63u, pi, nullptr);
 }
 
 }  // namespace
@@ -303,6 +304,7 @@ struct FusedScratch {
     InstG* inst;                   // ninst
     double* inst_m;                // 16 per instance
     double* light_pos;             // 3 per light
+    JobExt* ext;                   // one
 };
 constexpr int kCounterWords = 64;
 
@@ -324,10 +326,12 @@ constexpr size_t kNoAllocLightOff = kNoAllocCounterOff + kCounterWords * 8;
 constexpr size_t kNoAllocLightPosOff = kNoAllocLightOff + 1024;
 constexpr size_t kNoAllocInstOff = kNoAllocLightPosOff + 256;
 constexpr size_t kNoAllocInstMOff = kNoAllocInstOff + sizeof(InstG) * kNoAllocMaxInstances;
-constexpr size_t kNoAllocBytes = kNoAllocInstMOff + 16 * 8 * kNoAllocMaxInstances;
+constexpr size_t kNoAllocExtOff = kNoAllocInstMOff + 16 * 8 * kNoAllocMaxInstances;
+constexpr size_t kNoAllocBytes = kNoAllocExtOff + ((sizeof(JobExt) + 255) / 256) * 256;
 static_assert(sizeof(LightF) * VIRA_GPU_MAX_LIGHTS <= 1024, "light table fits its slot");
 static_assert(3 * 8 * VIRA_GPU_MAX_LIGHTS <= 256, "light positions fit their slot");
-static_assert(kNoAllocInstOff % 16 == 0 && kNoAllocInstMOff % 16 == 0, "aligned tables");
+static_assert(kNoAllocInstOff % 16 == 0 && kNoAllocInstMOff % 16 == 0 && kNoAllocExtOff % 16 == 0,
+              "aligned tables");
 
 __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
@@ -390,6 +394,8 @@ void run_fused(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o
                 hi[i].verts = mi.verts.as<GVertex>();
                 hi[i].idx = mi.idx.as<uint32_t>();
                 hi[i].mat = mat_params(mi);
+                hi[i].mesh_id = (uint32_t)c.tlas[i0 + i].mesh_id;
+                hi[i].material_id = mi.mat.material_id;
                 for (int k = 0; k < 3; ++k) {
                     hi[i].blo[k] = mi.blas.lo[k];
                     hi[i].bhi[k] = mi.blas.hi[k];
@@ -410,17 +416,46 @@ void run_fused(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o
         VG_HIP(hipGetLastError());
         cf.ox = cf.oy = cf.oz = 0.0f;  // the camera is the origin of the render frame
     }
-    OutPtrs op{out.radiance_dev, out.direct_dev, out.indirect_dev, out.albedo_dev, out.normal_dev,
-               out.depth_dev, out.alpha_dev, out.tri_id_dev, out.instance_id_dev};
+    {
+        JobExt X{};
+        X.out = out_params(out);
+        X.env = env_params(c, o);
+        X.adapt = adaptive_params(o);
+        X.pass = pass_params(cam, c.scene_mesh, m);
+        VG_HIP(hipMemcpyAsync(S.ext, &X, sizeof(X), hipMemcpyHostToDevice, c.stream));
+    }
     k_zero_u64<<<1, kCounterWords, 0, c.stream>>>(S.counters, kCounterWords);
     VG_HIP(hipGetLastError());
     VG_HIP(hipEventRecord(ev0, c.stream));
     const bool w4 = m.blas.width == 4;
-    const TileJob J{m.blas.nodes.p, m.blas.tris.as<float4>(), m.verts.as<GVertex>(), m.idx.as<uint32_t>(), mf, lf, cf,
-                    o.seed, o.samples, o.bounces, o.russian_roulette, o.rr_start, tiles_x, op,
-                    inst ? S.inst : nullptr, ninst};
-    auto kern = inst ? (w4 ? k_render_fused<4, true> : k_render_fused<2, true>)
-                     : (w4 ? k_render_fused<4, false> : k_render_fused<2, false>);
+    TileJob J{};
+    J.nodes = m.blas.nodes.p;
+    J.tris = m.blas.tris.as<float4>();
+    J.verts = m.verts.as<GVertex>();
+    J.idx = m.idx.as<uint32_t>();
+    J.mat = mf;
+    J.lights = lf;
+    J.cam = cf;
+    J.seed = o.seed;
+    J.spp = o.samples;
+    J.max_bounces = o.bounces;
+    J.rr = o.russian_roulette;
+    J.rr_start = o.rr_start;
+    J.tiles_x = tiles_x;
+    J.inst = inst ? S.inst : nullptr;
+    J.ninst = ninst;
+    J.ext = S.ext;
+    const bool multi = fused_multi(c);
+    const bool feat = fused_feat(c, o);
+    using K = void (*)(TileJob, float*, unsigned long long*);
+#define VG_FUSED_PAIR(W_, I_, M_) {k_render_fused<W_, I_, M_, false>, k_render_fused<W_, I_, M_, true>}
+    const K table[2][2][2][2] = {
+        {{VG_FUSED_PAIR(2, false, false), VG_FUSED_PAIR(2, false, true)},
+         {VG_FUSED_PAIR(2, true, false), VG_FUSED_PAIR(2, true, true)}},
+        {{VG_FUSED_PAIR(4, false, false), VG_FUSED_PAIR(4, false, true)},
+         {VG_FUSED_PAIR(4, true, false), VG_FUSED_PAIR(4, true, true)}}};
+#undef VG_FUSED_PAIR
+    const K kern = table[w4 ? 1 : 0][inst ? 1 : 0][multi ? 1 : 0][feat ? 1 : 0];
     kern<<<tiles, kFusedBS, 0, c.stream>>>(J, S.acc, S.counters);
     VG_HIP(hipGetLastError());
     VG_HIP(hipEventRecord(ev1, c.stream));
@@ -466,10 +501,16 @@ void render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, c
     const uint32_t tiles_x = (uint32_t)((cam.width + 7) / 8);
     const uint32_t tiles = tiles_x * (uint32_t)((cam.height + 7) / 8);
     const uint32_t nq = tiles * 64u;
-    const bool fused = o.integrator == 1 || (o.integrator == 0 && nl <= 1);
-    VG_CHECK(!(fused && nl > 1), VIRA_GPU_ERR_UNSUPPORTED, "render: the fused integrator takes at most one light");
-    VG_CHECK(!inst || fused, VIRA_GPU_ERR_UNSUPPORTED,
-             "render: instanced scenes use the per-pixel integrator (at most one light)");
+    const bool fused = o.integrator != 2;
+    VG_CHECK(!inst || fused, VIRA_GPU_ERR_UNSUPPORTED, "render: instanced scenes use the per-pixel integrator");
+    if (!fused) {
+        VG_CHECK(!o.adaptive_sampling, VIRA_GPU_ERR_UNSUPPORTED,
+                 "render: adaptive sampling runs in the per-pixel integrator only");
+        VG_CHECK(!c.env_has_ambient, VIRA_GPU_ERR_UNSUPPORTED,
+                 "render: ambient light runs in the per-pixel integrator only");
+        VG_CHECK(!(o.show_background && c.env_bg), VIRA_GPU_ERR_UNSUPPORTED,
+                 "render: a shown background runs in the per-pixel integrator only");
+    }
 
     c.r_acc.alloc((size_t)A_FIELDS * nq * 4);
     c.r_counters.alloc(kCounterWords * 8);
@@ -480,8 +521,10 @@ void render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, c
             c.r_inst_m.alloc((size_t)16 * 8 * ninst);
             c.r_light_pos.alloc((size_t)3 * 8 * VIRA_GPU_MAX_LIGHTS);
         }
+        c.r_ext.alloc(sizeof(JobExt));
         const FusedScratch S{c.r_acc.as<float>(), c.r_counters.as<unsigned long long>(), c.r_lights.as<LightF>(),
-                             c.r_inst.as<InstG>(), c.r_inst_m.as<double>(), c.r_light_pos.as<double>()};
+                             c.r_inst.as<InstG>(), c.r_inst_m.as<double>(), c.r_light_pos.as<double>(),
+                             c.r_ext.as<JobExt>()};
         EventPair ev;
         run_fused(c, cam, o, out, st, S, ev.a, ev.b);
         return;
@@ -491,8 +534,7 @@ void render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, c
     CamF cf = cam_params(cam);
     const Lights lf = upload_lights(c, c.r_lights);
     const MatF mf = mat_params(m);
-    OutPtrs op{out.radiance_dev, out.direct_dev, out.indirect_dev, out.albedo_dev, out.normal_dev,
-               out.depth_dev, out.alpha_dev, out.tri_id_dev, out.instance_id_dev};
+    const OutPtrs op = out_params(out);
     float* acc = c.r_acc.as<float>();
     unsigned long long* cnt = c.r_counters.as<unsigned long long>();
     uint32_t* qcount = reinterpret_cast<uint32_t*>(c.r_counters.as<char>() + 64);  // uint32 queue lengths per bounce
@@ -570,7 +612,8 @@ void render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, c
             k_resolve<<<blocks_for(nq, 256), 256, 0, c.stream>>>(nq, path, acc);
             VG_HIP(hipGetLastError());
         }
-        k_finalize<<<blocks_for(nq, 256), 256, 0, c.stream>>>(nq, tiles_x, cam.width, cam.height, o.samples, acc, op);
+        k_finalize<<<blocks_for(nq, 256), 256, 0, c.stream>>>(nq, tiles_x, cam.width, cam.height, o.samples, acc, op,
+                                                              pass_params(cam, c.scene_mesh, m));
         VG_HIP(hipGetLastError());
     }
     VG_HIP(hipEventRecord(total.b, c.stream));
@@ -599,10 +642,9 @@ void render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, c
 
 void render_noalloc(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, const vira_gpu_outputs& out,
                     vira_gpu_render_stats* st) {
-    const int nl = render_checks(c, cam, o);
+    render_checks(c, cam, o);
     VG_CHECK(o.integrator != 2, VIRA_GPU_ERR_UNSUPPORTED,
              "render_noalloc: only the per-pixel (fused) integrator runs without allocation");
-    VG_CHECK(nl <= 1, VIRA_GPU_ERR_UNSUPPORTED, "render_noalloc: the per-pixel integrator takes at most one light");
     const uint32_t tiles = (uint32_t)((cam.width + 7) / 8) * (uint32_t)((cam.height + 7) / 8);
     VG_CHECK(tiles <= kNoAllocMaxTiles, VIRA_GPU_ERR_INVALID,
              "render_noalloc: " + std::to_string(tiles) + " 8x8 tiles exceed the scratch maximum of " +
@@ -619,7 +661,8 @@ void render_noalloc(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_op
     const FusedScratch S{reinterpret_cast<float*>(base), reinterpret_cast<unsigned long long*>(base + kNoAllocCounterOff),
                          reinterpret_cast<LightF*>(base + kNoAllocLightOff), reinterpret_cast<InstG*>(base + kNoAllocInstOff),
                          reinterpret_cast<double*>(base + kNoAllocInstMOff),
-                         reinterpret_cast<double*>(base + kNoAllocLightPosOff)};
+                         reinterpret_cast<double*>(base + kNoAllocLightPosOff),
+                         reinterpret_cast<JobExt*>(base + kNoAllocExtOff)};
     run_fused(c, cam, o, out, st, S, c.na_ev[0], c.na_ev[1]);
 }
 

@@ -43,9 +43,21 @@ struct CamF {
 };
 struct MatF {
     float albedo[NB];
-    const float* tex;
+    const float* tex;   // albedo map: tch floats per texel
     int tw, th;
     int smooth;
+    int tch;            // 1: grey texture x albedo[k]; 8: spectral albedo map (Image<TSpectral>)
+    const float* nmap;  // tangent-space normal map, 3 floats per texel (Material::getNormal), or null
+    int nw, nh;
+};
+// The scene's light environment: ambient light (Scene::getAmbient/hasAmbient)
+// and the background image shown on misses (Scene::getBackgroundEmission).
+struct EnvF {
+    float ambient[NB];
+    int has_ambient;
+    int show_bg;        // CPUPathTracerOptions::show_background
+    const float* bg;    // W*H*8 spectral image, or null (zero background)
+    int bw, bh;
 };
 
 // Pixel tiles of 8x8: queue/accumulator index q = tile * 64 + (y & 7) * 8 + (x & 7).
@@ -92,23 +104,55 @@ __device__ __forceinline__ f3 offset_point(f3 p, f3 n) {
 // PowerHeuristic (cpu_path_tracer.ipp:472-478), one sample each
 __device__ __forceinline__ float power_heuristic(float f, float g) { return (f * f) / (f * f + g * g); }
 
-// Image::sampleUVs + interpolatePixel (image.ipp:589-670) on a grey texture
-__device__ __forceinline__ float tex_bilinear(const MatF& m, float u, float v) {
-    float rx = (float)m.tw, ry = (float)m.th;
+// Image::sampleUVs + interpolatePixel (image.ipp:589-670): uv * resolution
+// modulo the resolution (negative wrapped up), the floor texel and its +1
+// neighbours clamped to the last texel, bilinear weights.
+struct Taps {
+    size_t o00, o10, o01, o11;  // texel indices
+    float w00, w10, w01, w11;
+};
+__device__ __forceinline__ Taps bilinear_taps(int w, int h, float u, float v) {
+    const float rx = (float)w, ry = (float)h;
     float px = fmodf(u * rx, rx), py = fmodf(v * ry, ry);
     if (px < 0.f) px += rx;
     if (py < 0.f) py += ry;
     if (!(px >= 0.f)) px = 0.f;  // non-finite uv: keep the fetch in range
     if (!(py >= 0.f)) py = 0.f;
     int x0 = (int)floorf(px), y0 = (int)floorf(py);
-    x0 = min(x0, m.tw - 1);
-    y0 = min(y0, m.th - 1);
-    int x1 = min(x0 + 1, m.tw - 1), y1 = min(y0 + 1, m.th - 1);
-    float p00 = m.tex[x0 + (size_t)y0 * m.tw], p10 = m.tex[x1 + (size_t)y0 * m.tw];
-    float p01 = m.tex[x0 + (size_t)y1 * m.tw], p11 = m.tex[x1 + (size_t)y1 * m.tw];
-    float dx = px - (float)x0, dy = py - (float)y0;
-    return ((((1.0f - dx) * (1.0f - dy)) * p00 + (dx * (1.0f - dy)) * p10) + ((1.0f - dx) * dy) * p01) +
-           (dx * dy) * p11;
+    x0 = min(x0, w - 1);
+    y0 = min(y0, h - 1);
+    const int x1 = min(x0 + 1, w - 1), y1 = min(y0 + 1, h - 1);
+    const float dx = px - (float)x0, dy = py - (float)y0;
+    Taps T;
+    T.o00 = x0 + (size_t)y0 * w;
+    T.o10 = x1 + (size_t)y0 * w;
+    T.o01 = x0 + (size_t)y1 * w;
+    T.o11 = x1 + (size_t)y1 * w;
+    T.w00 = (1.0f - dx) * (1.0f - dy);
+    T.w10 = dx * (1.0f - dy);
+    T.w01 = (1.0f - dx) * dy;
+    T.w11 = dx * dy;
+    return T;
+}
+// Channel c of a C-channel image at the taps.
+__device__ __forceinline__ float taps_eval(const Taps& T, const float* __restrict__ img, int C, int c) {
+    return ((T.w00 * img[T.o00 * C + c] + T.w10 * img[T.o10 * C + c]) + T.w01 * img[T.o01 * C + c]) +
+           T.w11 * img[T.o11 * C + c];
+}
+
+// Background radiance along direction d (cpu_path_tracer.ipp:257-270):
+// uv = (atan2(y, x) / 2 pi, atan2(sqrt(x^2 + y^2), z) / pi), sampleUVs.
+__device__ __forceinline__ void background_radiance(const EnvF& E, f3 d, float bg[NB]) {
+    if (!E.bg) {
+#pragma unroll
+        for (int k = 0; k < NB; ++k) bg[k] = 0.0f;
+        return;
+    }
+    const float u = __fdiv_rn(vgm::atan2_f32(d.y, d.x), 2.0f * kPi);
+    const float v = __fdiv_rn(vgm::atan2_f32(__builtin_sqrtf(d.x * d.x + d.y * d.y), d.z), kPi);
+    const Taps T = bilinear_taps(E.bw, E.bh, u, v);
+#pragma unroll
+    for (int k = 0; k < NB; ++k) bg[k] = taps_eval(T, E.bg, NB, k);
 }
 
 // Light::sample (point_light.ipp:18-38, sphere_light.ipp:31-125)
@@ -187,8 +231,9 @@ __device__ __forceinline__ float light_pdf(const LightF& L, f3 p, f3 dir) {
 
 struct Surface {
     f3 org;         // offset shading point (origin of shadow and bounce rays)
-    f3 N, tg, bt;   // shading normal and tangent frame
-    float alb[NB];  // vertex albedo x material albedo x texture
+    f3 N;           // shading normal (after the normal map)
+    f3 Nf, tg, bt;  // tangent frame [tg bt Nf] of the normal before the map (BSDF sampling uses it)
+    float alb[NB];  // vertex albedo x material albedo map
 };
 
 // Mesh-local part of a hit (cpu_path_tracer.ipp:284-349): shading normal,
@@ -197,9 +242,12 @@ struct LocalHit {
     f3 P;    // computeShadingPoint of the barycentric point
     f3 Nl;   // shading normal (face normal, or interpolated vertex normals when smooth)
     f3 fn;   // unit face normal
+    float u, v;  // interpolated uv
     float alb[NB];
 };
 
+// FEAT = false takes the grey-texture branch only (no spectral albedo map).
+template <bool FEAT = true>
 __device__ __forceinline__ void local_hit(const GVertex* __restrict__ verts, const uint32_t* __restrict__ idx,
                                           const MatF& mat, uint32_t id, float b1, float b2, LocalHit& L) {
     const uint32_t i0 = idx[3 * (size_t)id], i1 = idx[3 * (size_t)id + 1], i2 = idx[3 * (size_t)id + 2];
@@ -211,17 +259,28 @@ __device__ __forceinline__ void local_hit(const GVertex* __restrict__ verts, con
     const f3 n0 = mk3(V0.nx, V0.ny, V0.nz), n1 = mk3(V1.nx, V1.ny, V1.nz), n2 = mk3(V2.nx, V2.ny, V2.nz);
     L.Nl = L.fn;
     if (mat.smooth) L.Nl = normalize(add(add(mul(w0, n1), mul(w1, n2)), mul(w2, n0)));
-    float texv = 1.0f;
+    L.u = (w0 * V1.u + w1 * V2.u) + w2 * V0.u;
+    L.v = (w0 * V1.v + w1 * V2.v) + w2 * V0.v;
+    // Material::getAlbedo (material.ipp:114-116): a spectral map per band, or a grey map x albedo
+    float mv[NB];
     if (mat.tex) {
-        const float uvu = (w0 * V1.u + w1 * V2.u) + w2 * V0.u;
-        const float uvv = (w0 * V1.v + w1 * V2.v) + w2 * V0.v;
-        texv = tex_bilinear(mat, uvu, uvv);
+        const Taps T = bilinear_taps(mat.tw, mat.th, L.u, L.v);
+        if (FEAT && mat.tch == NB) {
+#pragma unroll
+            for (int k = 0; k < NB; ++k) mv[k] = taps_eval(T, mat.tex, NB, k);
+        } else {
+            const float texv = taps_eval(T, mat.tex, 1, 0);
+#pragma unroll
+            for (int k = 0; k < NB; ++k) mv[k] = texv * mat.albedo[k];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NB; ++k) mv[k] = mat.albedo[k];
     }
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
         float av = (w0 * V1.alb[k] + w1 * V2.alb[k]) + w2 * V0.alb[k];
-        float mv = mat.tex ? texv * mat.albedo[k] : mat.albedo[k];
-        L.alb[k] = av * mv;
+        L.alb[k] = av * mv[k];
     }
     f3 P = add(add(mul(w0, p1), mul(w1, p2)), mul(w2, p0));
     {  // computeShadingPoint (cpu_path_tracer.ipp:481-511)
@@ -235,15 +294,28 @@ __device__ __forceinline__ void local_hit(const GVertex* __restrict__ verts, con
     L.P = P;
 }
 
-// Tangent frame, albedo and offset origin from global-frame quantities
-// (cpu_path_tracer.ipp:307-318,386): N is the global shading normal, fng the
-// global face normal the offset follows, P the global shading point.
-__device__ __forceinline__ Surface finish_surface(f3 P, f3 N, f3 fng, const float alb[NB]) {
+// Tangent frame, normal map, albedo and offset origin from global-frame
+// quantities (cpu_path_tracer.ipp:307-330,386): N is the global shading normal,
+// fng the global face normal the offset follows, P the global shading point,
+// (u, v) the hit's uv for the normal map (Material::getNormal,
+// material.ipp:129-143: the tangent-space normal through [tg bt N], normalised).
+// FEAT = false leaves the normal map out.
+template <bool FEAT = true>
+__device__ __forceinline__ Surface finish_surface(f3 P, f3 N, f3 fng, const float alb[NB], const MatF& mat, float u,
+                                                  float v) {
     Surface S;
-    S.N = N;
+    S.Nf = N;
     const f3 arb = (fabsf(N.x) < 0.9f) ? mk3(1.f, 0.f, 0.f) : mk3(0.f, 1.f, 0.f);
     S.tg = normalize(cross(arb, N));
     S.bt = cross(N, S.tg);
+    S.N = N;
+    if (FEAT && mat.nmap) {
+        const Taps T = bilinear_taps(mat.nw, mat.nh, u, v);
+        const float nx = taps_eval(T, mat.nmap, 3, 0), ny = taps_eval(T, mat.nmap, 3, 1),
+                    nz = taps_eval(T, mat.nmap, 3, 2);
+        S.N = normalize(mk3((S.tg.x * nx + S.bt.x * ny) + N.x * nz, (S.tg.y * nx + S.bt.y * ny) + N.y * nz,
+                            (S.tg.z * nx + S.bt.z * ny) + N.z * nz));
+    }
 #pragma unroll
     for (int k = 0; k < NB; ++k) S.alb[k] = alb[k];
     S.org = offset_point(P, fng);
@@ -251,33 +323,41 @@ __device__ __forceinline__ Surface finish_surface(f3 P, f3 N, f3 fng, const floa
 }
 
 // A hit on a mesh whose frame is the render frame (identity instance).
+template <bool FEAT = true>
 __device__ __forceinline__ Surface surface_at(const GVertex* __restrict__ verts, const uint32_t* __restrict__ idx,
                                               const MatF& mat, uint32_t id, float b1, float b2) {
     LocalHit L;
-    local_hit(verts, idx, mat, id, b1, b2, L);
-    return finish_surface(L.P, L.Nl, L.fn, L.alb);
+    local_hit<FEAT>(verts, idx, mat, id, b1, b2, L);
+    return finish_surface<FEAT>(L.P, L.Nl, L.fn, L.alb, mat, L.u, L.v);
 }
 
 // One light's direct-lighting term (cpu_path_tracer.ipp:399-428): returns
 // false when the light cannot be sampled (pdf == 0); otherwise the shadow ray
 // (S.org, dir, tmax = dist) and the contribution added if it is unoccluded.
-__device__ __forceinline__ bool light_contrib(const Surface& S, const LightF& L, const float beta[NB], uint32_t seed,
-                                              uint32_t pix, uint32_t s, uint32_t dim0, f3& dir, float& dist,
-                                              float contrib[NB]) {
+// The same from the surface's origin, shading normal and albedo alone (the
+// several-light renderer keeps only these while it traces the shadow rays).
+__device__ __forceinline__ bool light_contrib_at(f3 org, f3 N, const float alb[NB], const LightF& L,
+                                                 const float beta[NB], uint32_t seed, uint32_t pix, uint32_t s,
+                                                 uint32_t dim0, f3& dir, float& dist, float contrib[NB]) {
     float lpdf, lrad[NB];
-    sample_light(L, S.org, seed, pix, s, dim0, dir, dist, lpdf, lrad);
+    sample_light(L, org, seed, pix, s, dim0, dir, dist, lpdf, lrad);
     if (!(lpdf > 0.0f)) return false;
-    float ndl = dot(S.N, dir);
+    float ndl = dot(N, dir);
     float mx = fmaxf(ndl, 0.0f);
-    float cth = fmaxf(0.0f, dot(dir, S.N));
+    float cth = fmaxf(0.0f, dot(dir, N));
     float mpdf = (ndl <= 0.0f) ? 0.0f : ndl * kInvPi;
     float wgt = power_heuristic(lpdf, mpdf);
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-        float bsdf = (S.alb[k] * kInvPi) * mx;
+        float bsdf = (alb[k] * kInvPi) * mx;
         contrib[k] = __fdiv_rn((((beta[k] * bsdf) * lrad[k]) * cth) * wgt, lpdf);
     }
     return true;
+}
+__device__ __forceinline__ bool light_contrib(const Surface& S, const LightF& L, const float beta[NB], uint32_t seed,
+                                              uint32_t pix, uint32_t s, uint32_t dim0, f3& dir, float& dist,
+                                              float contrib[NB]) {
+    return light_contrib_at(S.org, S.N, S.alb, L, beta, seed, pix, s, dim0, dir, dist, contrib);
 }
 
 // BSDF sampling of the next bounce (cpu_path_tracer.ipp:431-461,
@@ -308,8 +388,10 @@ __device__ __forceinline__ bool sample_bounce(const Surface& S, const Lights& li
     float lz = __builtin_sqrtf(fmaxf(0.0f, (1.0f - dx * dx) - dy * dy));
     float mpdf = lz * kInvPi;
     if (!(mpdf > 0.0f)) return false;
-    nd = mk3((S.tg.x * dx + S.bt.x * dy) + S.N.x * lz, (S.tg.y * dx + S.bt.y * dy) + S.N.y * lz,
-             (S.tg.z * dx + S.bt.z * dy) + S.N.z * lz);
+    // the frame of the normal before the normal map (material_sampling.hpp:15-21);
+    // the weights below use the mapped normal (material.ipp:8-67)
+    nd = mk3((S.tg.x * dx + S.bt.x * dy) + S.Nf.x * lz, (S.tg.y * dx + S.bt.y * dy) + S.Nf.y * lz,
+             (S.tg.z * dx + S.bt.z * dy) + S.Nf.z * lz);
     float ndd = dot(S.N, nd);
     float mx = fmaxf(ndd, 0.0f);
     float cth = fmaxf(0.0f, dot(nd, S.N));

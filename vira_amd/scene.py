@@ -89,6 +89,7 @@ class Camera:
     f_stop: float = 2.8            # camera.hpp:235
     exposure_time: float = 0.01    # camera.hpp:233
     optical_efficiency: float = 1.0
+    view_normal: object = None     # Camera::getViewNormalMatrix (3x3); None = transpose of rotation
 
     def aperture_area(self) -> float:
         """CircularAperture of diameter f/N (camera.ipp:82, aperture.ipp:14-16), FP32 as the reference."""
@@ -127,6 +128,10 @@ class Camera:
         for k in range(3):
             c.position[k] = float(self.position[k])
         c.z_dir = float(self.z_dir)
+        if self.view_normal is not None:
+            vn = np.asarray(self.view_normal, np.float32).reshape(9)
+            for k in range(9):
+                c.view_normal[k] = float(vn[k])
         return c
 
 
@@ -226,22 +231,30 @@ class PathTracerOptions:
     integrator: str = "auto"  # "auto" | "fused" | "wavefront"
     denoise: bool = False
     denoiser: DenoiseOptions = field(default_factory=DenoiseOptions)
+    show_background: bool = False
+    adaptive_sampling: bool = False
+    samples_per_batch: int = 30
+    sampling_tolerance: float = 0.05
+    samples_to_detect_miss: int = 30
 
     def to_c(self) -> capi.RenderOpts:
         o = capi.RenderOpts()
         o.samples, o.bounces = int(self.samples), int(self.bounces)
         o.russian_roulette, o.rr_start, o.seed = int(self.russian_roulette), int(self.rr_start), int(self.seed)
         o.integrator = {"auto": 0, "fused": 1, "wavefront": 2}[self.integrator]
+        o.show_background, o.adaptive_sampling = int(self.show_background), int(self.adaptive_sampling)
+        o.samples_per_batch, o.sampling_tolerance = int(self.samples_per_batch), float(self.sampling_tolerance)
+        o.samples_to_detect_miss = int(self.samples_to_detect_miss)
         return o
 
 
 @dataclass
 class RenderRequest:
-    """One request of a batch: mesh, camera, options, at most one light, passes."""
+    """One request of a batch: mesh, camera, options, lights, passes."""
     mesh: int
     camera: "Camera"
     options: PathTracerOptions
-    light: object  # PointLight | SphereLight | None
+    light: object  # PointLight | SphereLight | a list of them (<= 8) | None
     passes: "RenderPasses"
 
     def to_c(self) -> capi.Request:
@@ -249,9 +262,12 @@ class RenderRequest:
         r.mesh_id = self.mesh
         r.camera = self.camera.to_c()
         r.opts = self.options.to_c()
-        r.num_lights = 0 if self.light is None else 1
-        if self.light is not None:
-            r.light = self.light.to_c()
+        ls = [] if self.light is None else (list(self.light) if isinstance(self.light, (list, tuple)) else [self.light])
+        if len(ls) > capi.MAX_LIGHTS:
+            raise ValueError(f"at most {capi.MAX_LIGHTS} lights per request")
+        r.num_lights = len(ls)
+        for i, L in enumerate(ls):
+            r.lights[i] = L.to_c()
         r.out = self.passes.to_c()
         return r
 
@@ -284,7 +300,8 @@ class RenderPasses:
     """RenderPasses (passes.hpp:51-95) as device tensors, pixel index i + j*W.
     `which` limits the passes allocated (the others are not written)."""
 
-    ALL = ("radiance", "direct", "indirect", "albedo", "normal", "depth", "alpha", "tri_id", "instance_id")
+    ALL = ("radiance", "direct", "indirect", "albedo", "normal", "depth", "alpha", "tri_id", "instance_id",
+           "normal_camera", "mesh_id", "material_id")
 
     def __init__(self, width: int, height: int, device="cuda", which=ALL):
         import torch
@@ -303,6 +320,11 @@ class RenderPasses:
                             if "tri_id" in w else None)
         self.instance_id = (torch.zeros((height, width), dtype=torch.int32, device=device)
                             if "instance_id" in w else None)
+        self.normal_camera = (torch.zeros((height, width, 3), dtype=torch.float32, device=device)
+                              if "normal_camera" in w else None)
+        self.mesh_id = torch.zeros((height, width), dtype=torch.int32, device=device) if "mesh_id" in w else None
+        self.material_id = (torch.zeros((height, width), dtype=torch.int32, device=device)
+                            if "material_id" in w else None)
 
     def to_c(self) -> capi.Outputs:
         p = lambda t: None if t is None else t.data_ptr()
@@ -316,6 +338,9 @@ class RenderPasses:
         o.alpha_dev = p(self.alpha)
         o.tri_id_dev = p(self.triangle_id)
         o.instance_id_dev = p(self.instance_id)
+        o.normal_camera_dev = p(self.normal_camera)
+        o.mesh_id_dev = p(self.mesh_id)
+        o.material_id_dev = p(self.material_id)
         return o
 
 
@@ -425,17 +450,50 @@ class Context:
             arr[i] = L.to_c()
         self._call(self.lib.vira_gpu_lights_set, self.h, arr, len(lights))
 
-    def material_set(self, mid: int, albedo=None, texture=None, smooth_shading: bool = False):
+    def material_set(self, mid: int, albedo=None, texture=None, smooth_shading: bool = False, normal_map=None,
+                     material_id: int = 0):
+        """Material of mesh mid: constant albedo (8 bands), texture (H, W) grey x albedo or
+        (H, W, 8) spectral albedo map, normal_map (H, W, 3) tangent-space normals (device
+        tensors, kept alive here), material_id for the material_id pass."""
         m = capi.Material()
         a = np.ones(NUM_BINS, np.float32) if albedo is None else np.broadcast_to(np.asarray(albedo, np.float32), (NUM_BINS,))
         for k in range(NUM_BINS):
             m.albedo[k] = float(a[k])
+        m.tex_channels = 1
         if texture is not None:
+            if not texture.is_contiguous() or texture.dtype != self.torch.float32:
+                raise ValueError("texture: contiguous float32 device tensor")
             m.texture_dev = texture.data_ptr()
-            m.tex_height, m.tex_width = texture.shape
+            m.tex_height, m.tex_width = texture.shape[:2]
+            m.tex_channels = 1 if texture.dim() == 2 else int(texture.shape[2])
+        if normal_map is not None:
+            if not normal_map.is_contiguous() or normal_map.dim() != 3 or normal_map.shape[2] != 3:
+                raise ValueError("normal_map: contiguous (H, W, 3) float32 device tensor")
+            m.normal_map_dev = normal_map.data_ptr()
+            m.nmap_height, m.nmap_width = normal_map.shape[:2]
         m.smooth_shading = int(smooth_shading)
-        self._texture = texture  # keep alive
+        m.material_id = int(material_id)
+        if not hasattr(self, "_materials"):
+            self._materials = {}
+        self._materials[mid] = (texture, normal_map)  # keep alive
+        self._texture = texture
         self._call(self.lib.vira_gpu_material_set, self.h, mid, C.byref(m))
+
+    def environment_set(self, ambient=None, background=None):
+        """Scene::setAmbient (8 bands or a scalar; None = none) and setBackgroundEmission
+        ((H, W, 8) float32 device tensor, kept alive here; None = zero)."""
+        amb = None
+        if ambient is not None:
+            a = np.broadcast_to(np.asarray(ambient, np.float32), (NUM_BINS,))
+            amb = (C.c_float * NUM_BINS)(*[float(v) for v in a])
+        bw = bh = 0
+        if background is not None:
+            if not background.is_contiguous() or background.dim() != 3 or background.shape[2] != NUM_BINS:
+                raise ValueError("background: contiguous (H, W, 8) float32 device tensor")
+            bh, bw = background.shape[:2]
+        self._background = background
+        self._call(self.lib.vira_gpu_environment_set, self.h, amb,
+                   None if background is None else background.data_ptr(), bw, bh)
 
     def render(self, camera: Camera, options: PathTracerOptions, passes: RenderPasses):
         st = capi.RenderStats()
