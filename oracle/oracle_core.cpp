@@ -417,6 +417,7 @@ struct PixelAcc {
     uint32_t count = 0;
     uint32_t tri = 0xffffffffu;
     uint32_t mesh = 0xffffffffu, material = 0xffffffffu;
+    uint32_t inst = 0xffffffffu;
 };
 
 // Camera-relative placement of an instance: the FP64 model matrix minus the
@@ -626,7 +627,8 @@ void render_impl(const InstanceO* inst, int ninst, const Camera& cam_in, const L
                         if (rebased) P = xform_point(place[hinst].l2w, P);  // model matrix (cpu_path_tracer.ipp:344)
                         if (bounce == 0) {
                             if (acc.tri == 0xffffffffu) {
-                                acc.tri = h.id | (hinst << 27);
+                                acc.tri = h.id;
+                                acc.inst = hinst;
                                 acc.mesh = m.id;
                                 acc.material = mat.material_id;
                             }
@@ -760,8 +762,8 @@ void render_impl(const InstanceO* inst, int ninst, const Camera& cam_in, const L
                 if (out.alpha) out.alpha[o] = (float)acc.count / (float)nsamp;
                 if (out.mesh_id) out.mesh_id[o] = acc.tri == 0xffffffffu ? 0xffffffffu : acc.mesh;
                 if (out.material_id) out.material_id[o] = acc.tri == 0xffffffffu ? 0xffffffffu : acc.material;
-                if (out.tri_id) out.tri_id[o] = acc.tri == 0xffffffffu ? acc.tri : (acc.tri & ((1u << 27) - 1u));
-                if (out.instance_id) out.instance_id[o] = acc.tri == 0xffffffffu ? acc.tri : (acc.tri >> 27);
+                if (out.tri_id) out.tri_id[o] = acc.tri;
+                if (out.instance_id) out.instance_id[o] = acc.tri == 0xffffffffu ? acc.tri : acc.inst;
             }
         }
         cnt[3] = tc_closest.internal; cnt[4] = tc_closest.tris;

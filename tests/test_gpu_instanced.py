@@ -98,8 +98,9 @@ def test_tlas_validation(gpu_ctx):
     with pytest.raises(Exception) as e:
         gpu_ctx.tlas_build(isc.meshes[:1], [proj])
     assert "affine" in str(e.value)
-    with pytest.raises(Exception):
-        gpu_ctx.tlas_build(isc.meshes * 5, None)  # 20 > 16 instances
+    with pytest.raises(Exception) as e:
+        gpu_ctx.tlas_build(isc.meshes[:1] * 65537, None)  # beyond the TLAS's 65536 instances
+    assert "65536" in str(e.value)
     # the direct-illumination path takes one identity instance
     gpu_ctx.tlas_build(isc.meshes, isc.transforms)
     with pytest.raises(Exception) as e:
@@ -126,3 +127,62 @@ def test_instanced_full_size_sample(gpu_ctx):
     print(f"instanced scene: {st['primary_rays'] + st['shadow_rays']} primary+shadow rays in {st['total_ms']:.1f} ms")
     assert (ref.max(1) > 0).sum() > 100
     assert np.array_equal(rad.view(np.uint32), ref.view(np.uint32))
+
+
+def test_tlas_256_instances_bit_exact(gpu_ctx):
+    """A TLAS of 256 instances of two meshes (a small DEM and the small body), each
+    rotated and placed on a jittered grid 1e7 m from the origin, seen camera-
+    relative with a point sun and one bounce: triangle ids, instance ids, mesh ids
+    and radiance equal the oracle's (which tests every instance in index order,
+    ties to the lower instance), whatever order the TLAS visits them in."""
+    import math
+    rng = np.random.default_rng(8)
+    sc = W.scaled(W.DEM257, dem=33, image=8)
+    built = W.build(gpu_ctx, sc)
+    pos, idx = W.small_body_mesh(3, nlat=21, nlon=32)
+    body = gpu_ctx.mesh_upload(pos, idx)
+    gpu_ctx.blas_build(body)
+    gpu_ctx.material_set(body, albedo=0.2, material_id=7)
+    meshes, transforms = [], []
+    base = np.array([1.0e7, -2.0e6, 3.0e6])
+    for i in range(256):
+        a, b = rng.uniform(0, 2 * math.pi, 2)
+        Rz = np.array([[math.cos(a), -math.sin(a), 0], [math.sin(a), math.cos(a), 0], [0, 0, 1]])
+        Rx = np.array([[1, 0, 0], [0, math.cos(b), -math.sin(b)], [0, math.sin(b), math.cos(b)]])
+        M = np.eye(4)
+        M[:3, :3] = Rz @ Rx
+        M[:3, 3] = base + np.array([(i % 16) * 2500.0, (i // 16) * 2500.0, 0.0]) + rng.uniform(-300, 300, 3)
+        meshes.append(built.mesh if i % 3 else body)
+        transforms.append(M)
+    gpu_ctx.tlas_build(meshes, transforms)
+    centre = base + np.array([7.5 * 2500, 7.5 * 2500, 0.0])
+    cam = S.Camera(96, 96, 55.0, tuple(centre + np.array([0.0, -30000.0, 25000.0]))).look_in_direction(
+        tuple(np.array([0.0, 30000.0, -25000.0])))
+    sun = S.directional_sun(50.0, 30.0, 2.0, distance=1e10, origin=tuple(centre))
+    gpu_ctx.lights_set([sun])
+    opts = S.PathTracerOptions(samples=2, bounces=1, seed=5)
+    passes = S.RenderPasses(96, 96)
+    gpu_ctx.render(cam, opts, passes)
+    heights = built.heights.cpu().numpy()
+    p = W.make_projection(sc)
+    tile = O.dem_mesh(heights, O.make_projection(p.kind, p.spacing, p.radius, tuple(p.center), tuple(p.offset)),
+                      built.vertex_albedo.cpu().numpy() if built.vertex_albedo is not None else None)
+    om_body = O.Mesh(pos, idx, O.vertex_normals(pos, idx), None, None)
+    ms = [tile if m == built.mesh else om_body for m in meshes]
+    tex = None if built.texture is None else built.texture.cpu().numpy()
+    mats = [O.material(1.0, tex, sc.smooth_shading) if m == built.mesh else O.material(0.2, material_id=7)
+            for m in meshes]
+    out = O.render_scene(ms, transforms, mats, O.camera_from(cam.to_c()), [O.light_from(sun.to_c())],
+                         O.options(2, 1, False, 1, 5), precision=32)
+    inst = passes.instance_id.cpu().numpy().reshape(-1).view(np.uint32)
+    hit = out["tri_id"] != 0xFFFFFFFF
+    assert hit.mean() > 0.1 and len(np.unique(out["instance_id"][hit])) > 200  # 13 % of pixels, 224 instances
+    assert np.array_equal(passes.triangle_id.cpu().numpy().reshape(-1).view(np.uint32), out["tri_id"])
+    assert np.array_equal(inst, out["instance_id"])
+    rad = passes.total_radiance.cpu().numpy().reshape(-1, 8)
+    assert np.array_equal(rad.view(np.uint32), out["radiance"].view(np.uint32))
+    mid = passes.mesh_id.cpu().numpy().reshape(-1).view(np.uint32)
+    want = np.where(hit, np.array(meshes, np.uint32)[np.where(hit, out["instance_id"], 0)], 0xFFFFFFFF)
+    assert np.array_equal(mid, want)
+    matid = passes.material_id.cpu().numpy().reshape(-1).view(np.uint32)
+    assert np.all(matid[hit & (mid == body)] == 7)

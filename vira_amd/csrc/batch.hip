@@ -184,7 +184,8 @@ void batch_submit(Ctx& c, const vira_gpu_request* reqs, size_t n, const vira_gpu
         d.job.rr_start = r.opts.rr_start;
         d.job.tiles_x = g.tiles_x;
         d.job.inst = nullptr;
-        d.job.ninst = 0;
+        d.job.tlas = nullptr;
+        d.job.ntlas = 0;
         d.ext.out = out_params(r.out);
         d.ext.env = env_params(c, r.opts);
         d.ext.adapt = adaptive_params(r.opts);

@@ -1,7 +1,9 @@
 // capi.hip -- the extern "C" entry points of include/vira_gpu.h.
 // Every entry point catches library errors and returns a status; the message
 // is kept per thread for vira_gpu_last_error().
+#include <algorithm>
 #include <cmath>
+#include <functional>
 #include <cstring>
 #include <string>
 
@@ -272,7 +274,7 @@ vira_gpu_status vira_gpu_tlas_build(vira_gpu_ctx* ctx, const vira_gpu_instance* 
         vg::Ctx& c = C(ctx);
         VG_CHECK(inst || n == 0, VIRA_GPU_ERR_INVALID, "tlas_build: null instances");
         VG_CHECK(n != 0, VIRA_GPU_ERR_INVALID, "tlas_build: empty TLAS");  // vira_tlas.ipp:33
-        VG_CHECK(n <= (size_t)vg::kMaxInstances, VIRA_GPU_ERR_UNSUPPORTED, "tlas_build: at most 16 instances");
+        VG_CHECK(n <= (size_t)vg::kMaxInstances, VIRA_GPU_ERR_UNSUPPORTED, "tlas_build: at most 65536 instances");
         static const double I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
         int width = 0;
         for (size_t i = 0; i < n; ++i) {
@@ -292,6 +294,66 @@ vira_gpu_status vira_gpu_tlas_build(vira_gpu_ctx* ctx, const vira_gpu_instance* 
         }
         c.tlas.assign(inst, inst + n);
         c.instanced = !(n == 1 && std::memcmp(inst[0].transform, I, sizeof(I)) == 0);
+        // TLAS topology (EmbreeTLAS::build, embree_tlas.ipp:77-115; ViraTLAS clusters, vira_tlas.ipp:29-79):
+        // a binary tree over the instances' placed boxes, split at the median centroid of the longest axis
+        std::vector<double> box(6 * n), cen(3 * n);
+        for (size_t i = 0; i < n; ++i) {
+            const vg::MeshG& m = c.mesh(inst[i].mesh_id);
+            const double* M = inst[i].transform;
+            double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+            if (m.blas.lo[0] <= m.blas.hi[0]) {
+                for (int q = 0; q < 8; ++q) {
+                    const double p[3] = {(q & 1) ? m.blas.hi[0] : m.blas.lo[0], (q & 2) ? m.blas.hi[1] : m.blas.lo[1],
+                                         (q & 4) ? m.blas.hi[2] : m.blas.lo[2]};
+                    for (int r = 0; r < 3; ++r) {
+                        const double w = M[r] * p[0] + M[4 + r] * p[1] + M[8 + r] * p[2] + M[12 + r];
+                        lo[r] = std::min(lo[r], w);
+                        hi[r] = std::max(hi[r], w);
+                    }
+                }
+            } else {
+                for (int r = 0; r < 3; ++r) lo[r] = hi[r] = M[12 + r];
+            }
+            for (int r = 0; r < 3; ++r) {
+                box[6 * i + r] = lo[r];
+                box[6 * i + 3 + r] = hi[r];
+                cen[3 * i + r] = 0.5 * (lo[r] + hi[r]);
+            }
+        }
+        std::vector<uint32_t> ids(n);
+        for (size_t i = 0; i < n; ++i) ids[i] = (uint32_t)i;
+        std::vector<int32_t> topo;  // (inst, skip) pairs
+        topo.reserve(4 * n);
+        std::function<void(size_t, size_t)> build = [&](size_t b, size_t e) {
+            const size_t node = topo.size() / 2;
+            topo.push_back(-1);
+            topo.push_back(0);
+            if (e - b == 1) {
+                topo[2 * node] = (int32_t)ids[b];
+                topo[2 * node + 1] = (int32_t)node + 1;
+                return;
+            }
+            double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+            for (size_t k = b; k < e; ++k)
+                for (int r = 0; r < 3; ++r) {
+                    lo[r] = std::min(lo[r], cen[3 * ids[k] + r]);
+                    hi[r] = std::max(hi[r], cen[3 * ids[k] + r]);
+                }
+            int ax = 0;
+            for (int r = 1; r < 3; ++r)
+                if (hi[r] - lo[r] > hi[ax] - lo[ax]) ax = r;
+            const size_t mid = b + (e - b) / 2;
+            std::nth_element(ids.begin() + b, ids.begin() + mid, ids.begin() + e, [&](uint32_t x, uint32_t y) {
+                return cen[3 * x + ax] < cen[3 * y + ax] || (cen[3 * x + ax] == cen[3 * y + ax] && x < y);
+            });
+            build(b, mid);
+            build(mid, e);
+            topo[2 * node + 1] = (int32_t)(topo.size() / 2);
+        };
+        build(0, n);
+        c.tlas_nodes = (int32_t)(topo.size() / 2);
+        c.tlas_topo.alloc(topo.size() * sizeof(int32_t));
+        VG_HIP(hipMemcpy(c.tlas_topo.p, topo.data(), topo.size() * sizeof(int32_t), hipMemcpyHostToDevice));
         c.scene_mesh = inst[0].mesh_id;
         c.scene_gen++;
     });

@@ -164,6 +164,10 @@ struct Ctx {
     std::vector<std::unique_ptr<MeshG>> meshes;
     int32_t scene_mesh = -1;  // the (first) instance's mesh of the current TLAS
     std::vector<vira_gpu_instance> tlas;
+    // TLAS topology over the instances' placed boxes: (instance or -1, skip) per
+    // node in depth-first order, on the device; boxes are made per render
+    int32_t tlas_nodes = 0;
+    DBuf tlas_topo;
     bool instanced = false;   // TLAS other than one identity instance: rendered camera-relative
     std::vector<vira_gpu_light> lights;
     // Scene::getAmbient / hasAmbient and the background image (getBackgroundEmission)
@@ -173,7 +177,7 @@ struct Ctx {
     int env_bw = 0, env_bh = 0;
     // render scratch, grown on demand
     DBuf r_ray_o, r_ray_d, r_hit, r_queue[2], r_beta, r_path_rad, r_path_dir, r_srays_o, r_srays_d,
-        r_scontrib, r_svis, r_sinfo, r_acc, r_counters, r_lights, r_inst, r_inst_m, r_light_pos, r_ext;
+        r_scontrib, r_svis, r_sinfo, r_acc, r_counters, r_lights, r_inst, r_inst_m, r_light_pos, r_ext, r_tnodes;
     // timing events of vira_gpu_render_noalloc, created with the context
     hipEvent_t na_ev[2] = {};
     // bumped by every call that changes geometry, BVHs, lights or materials

@@ -39,7 +39,8 @@ enum : int {
     A_TRI = 45,
     A_S1 = 46,  // adaptive sampling: sum of the samples' radiance magnitudes
     A_S2 = 47,  //                    and of their squares
-    A_FIELDS = 48
+    A_INST = 48,  // first-hit instance
+    A_FIELDS = 49
 };
 // Accumulator of pixel q: base of its tile block; field f is base[f * 64 + (q & 63)].
 __device__ __forceinline__ float* acc_tile(float* acc, uint32_t q) { return acc + (size_t)(q >> 6) * (A_FIELDS * 64); }
@@ -61,9 +62,12 @@ __device__ __forceinline__ const float& acc_ref(const float* base, int f, uint32
 #define ACC(base, f, lane) acc_ref((base), (f), (lane))
 
 // First-hit passes at bounce 0 (cpu_path_tracer.ipp:352-381).
-__device__ __forceinline__ void accumulate_first_hit(float* __restrict__ a, uint32_t lane, uint32_t id, float t,
-                                                     const Surface& S) {
-    if (__float_as_uint(ACC(a, A_TRI, lane)) == 0xffffffffu) ACC(a, A_TRI, lane) = __uint_as_float(id);
+__device__ __forceinline__ void accumulate_first_hit(float* __restrict__ a, uint32_t lane, uint32_t id,
+                                                     uint32_t inst, float t, const Surface& S) {
+    if (__float_as_uint(ACC(a, A_TRI, lane)) == 0xffffffffu) {
+        ACC(a, A_TRI, lane) = __uint_as_float(id);
+        ACC(a, A_INST, lane) = __uint_as_float(inst);
+    }
 #pragma unroll
     for (int k = 0; k < NB; ++k) ACC(a, A_ALB + k, lane) = ACC(a, A_ALB + k, lane) + S.alb[k];
     ACC(a, A_NRM + 0, lane) = ACC(a, A_NRM + 0, lane) + S.N.x;
@@ -88,9 +92,6 @@ struct PassInfo {
     float vn[9];
     uint32_t mesh_id, material_id;
 };
-
-// The first-hit id field holds instance << 27 | triangle (triangles < 2^27).
-constexpr uint32_t kTriBits = 27;
 
 // Averages and passes (cpu_path_tracer.ipp:229-239, passes.ipp:81-127); ns is
 // the sample count the reference divides by (the samples taken, or the index
@@ -130,13 +131,14 @@ __device__ __forceinline__ void write_pixel(const OutPtrs& o, size_t p, int ns, 
     if (o.depth) o.depth[p] = cnt > 0.0f ? __fdiv_rn(ACC(a, A_DEPTH, lane), cnt) : kInf;
     if (o.alpha) o.alpha[p] = __fdiv_rn(cnt, sf);
     const uint32_t first = __float_as_uint(ACC(a, A_TRI, lane));
+    const uint32_t finst = __float_as_uint(ACC(a, A_INST, lane));
     const bool none = first == 0xffffffffu;
-    if (o.tri) o.tri[p] = none ? first : (first & ((1u << kTriBits) - 1u));
-    if (o.inst) o.inst[p] = none ? first : (first >> kTriBits);
+    if (o.tri) o.tri[p] = first;
+    if (o.inst) o.inst[p] = none ? first : finst;
     if (o.mesh_id || o.material_id) {
         uint32_t mid = pi.mesh_id, tid = pi.material_id;
         if (inst_table && !none) {
-            const InstG& I = static_cast<const InstG*>(inst_table)[first >> kTriBits];
+            const InstG& I = static_cast<const InstG*>(inst_table)[finst];
             mid = I.mesh_id;
             tid = I.material_id;
         }
@@ -197,7 +199,8 @@ struct TileJob {
     int spp, max_bounces, rr, rr_start;
     uint32_t tiles_x;
     const InstG* inst;  // instanced scenes: the camera-relative instance table
-    int ninst;
+    const TNode* tlas;  // and its TLAS nodes
+    int ntlas;
     const JobExt* ext;  // device memory
 };
 
@@ -336,8 +339,8 @@ __device__ __forceinline__ void render_tile(const TileJob& J, uint32_t tile, flo
         uint32_t hinst = 0;
         if (mode != M_DONE) {
             if (INST)
-                h = trace_instances<W>(J.inst, J.ninst, o, d, tmax, mode == M_SHADOW, s_ref + lane, s_dist + lane,
-                                       kFusedBS, hinst);
+                h = trace_instances<W>(J.inst, J.tlas, J.ntlas, o, d, tmax, mode == M_SHADOW, s_ref + lane,
+                                       s_dist + lane, kFusedBS, hinst);
             else
                 h = traverse<W>(nodes, tris, o, d, tmax, mode == M_SHADOW, s_ref + lane, s_dist + lane, kFusedBS);
         }
@@ -357,7 +360,7 @@ __device__ __forceinline__ void render_tile(const TileJob& J, uint32_t tile, flo
                 Surface S = INST ? surface_instance<FEAT>(J.inst[hinst], h.id, h.b1, h.b2)
                                  : surface_at<FEAT>(verts, idx, mat, h.id, h.b1, h.b2);
                 n_hit++;
-                if (b == 0) accumulate_first_hit(a, lane, h.id | (hinst << kTriBits), h.t, S);
+                if (b == 0) accumulate_first_hit(a, lane, h.id, hinst, h.t, S);
                 o = S.org;
                 if (MULTI) {
                     // vertex radiance starts at 0 (+ ambient x albedo at bounce 0, cpu_path_tracer.ipp:389-396)

@@ -16,7 +16,18 @@
 
 namespace vg {
 
-constexpr int kMaxInstances = 16;
+constexpr int kMaxInstances = 1 << 16;
+
+// TLAS node in the camera-relative frame, in depth-first order with skip
+// links (a stackless walk): inst >= 0 marks a leaf holding that instance,
+// whose box is the instance's placed box; an internal node's box is the union
+// of the leaves in [index + 1, skip), and its first child follows it.
+struct TNode {
+    float lo[3], hi[3];
+    int32_t inst;
+    int32_t skip;  // the node after this subtree
+};
+static_assert(sizeof(TNode) == 32, "TLAS node is 32 bytes");
 
 struct InstG {
     const void* nodes;
@@ -103,36 +114,46 @@ __device__ __forceinline__ void rebase_instance(const double* M, double cx, doub
     }
 }
 
-// Closest hit (or occlusion) over all instances: the ray is carried into each
-// instance's frame (t is the same in both frames) and its BLAS is traversed
-// with the best t so far; ties keep the lower instance.  h.inst = instance.
+// Closest hit (or occlusion) over the instances through the TLAS: the ray is
+// carried into each reached instance's frame (t is the same in both frames)
+// and its BLAS is traversed with the best t so far.  Subtrees whose box the
+// ray misses, or enters beyond the best hit, are skipped.  Ties in t go to the
+// lower instance, so the hit does not depend on the visiting order.
 template <int W>
-__device__ __forceinline__ HitRec trace_instances(const InstG* __restrict__ inst, int n, f3 o, f3 d, float tmax,
-                                                  bool any, uint32_t* lds_ref, float* lds_dist, int stride,
-                                                  uint32_t& hit_inst) {
+__device__ __forceinline__ HitRec trace_instances(const InstG* __restrict__ inst, const TNode* __restrict__ tn,
+                                                  int nn, f3 o, f3 d, float tmax, bool any, uint32_t* lds_ref,
+                                                  float* lds_dist, int stride, uint32_t& hit_inst) {
     HitRec best;
     best.t = __builtin_huge_valf();
     best.id = 0xffffffffu;
     best.b1 = best.b2 = 0.f;
     hit_inst = 0;
     float bt = tmax;
-    // instances whose placed box the ray misses, or enters beyond the best hit, are skipped
     const f3 inv = mk3(__fdiv_rn(1.0f, d.x), __fdiv_rn(1.0f, d.y), __fdiv_rn(1.0f, d.z));
     const f3 e = mk3(slab_eps(d.x), slab_eps(d.y), slab_eps(d.z));
-    for (int i = 0; i < n; ++i) {
-        const InstG& I = inst[i];
-        if (!(slab<true>(I.wlo[0], I.wlo[1], I.wlo[2], I.whi[0], I.whi[1], I.whi[2], o, inv, e, bt) <
-              __builtin_huge_valf()))
+    int i = 0;
+    while (i < nn) {
+        const TNode N = tn[i];
+        if (!(slab<true>(N.lo[0], N.lo[1], N.lo[2], N.hi[0], N.hi[1], N.hi[2], o, inv, e, bt) <
+              __builtin_huge_valf())) {
+            i = N.skip;
             continue;
+        }
+        if (N.inst < 0) {
+            i++;
+            continue;
+        }
+        const InstG& I = inst[N.inst];
         const f3 ol = xform_point(I.w2l, o), dl = xform_dir(I.w2l, d);
         const HitRec h = traverse<W>(I.nodes, I.tris, ol, dl, bt, any, lds_ref, lds_dist, stride);
-        if (h.id == 0xffffffffu) continue;
-        if (best.id == 0xffffffffu || h.t < best.t) {
+        if (h.id != 0xffffffffu &&
+            (best.id == 0xffffffffu || h.t < best.t || (h.t == best.t && (uint32_t)N.inst < hit_inst))) {
             best = h;
-            hit_inst = (uint32_t)i;
+            hit_inst = (uint32_t)N.inst;
             bt = h.t;
+            if (any) break;
         }
-        if (any) break;
+        i = N.skip;
     }
     return best;
 }

@@ -99,6 +99,33 @@ __global__ void k_rebase(const double* __restrict__ M, int n, const double* __re
     }
 }
 
+// Camera-relative TLAS node boxes: a leaf takes its instance's placed box, an
+// internal node the union of the leaves of its subtree ([i + 1, skip) in
+// depth-first order), so every box holds what its subtree holds, exactly.
+__global__ void k_tlas_boxes(const int2* __restrict__ topo, int nn, const InstG* __restrict__ inst,
+                             TNode* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    const int2 t = topo[i];
+    TNode N;
+    N.inst = t.x;
+    N.skip = t.y;
+    for (int k = 0; k < 3; ++k) {
+        N.lo[k] = __builtin_huge_valf();
+        N.hi[k] = -__builtin_huge_valf();
+    }
+    for (int j = i; j < t.y; ++j) {
+        const int leaf = topo[j].x;
+        if (leaf < 0) continue;
+        const InstG& I = inst[leaf];
+        for (int k = 0; k < 3; ++k) {
+            N.lo[k] = fminf(N.lo[k], I.wlo[k]);
+            N.hi[k] = fmaxf(N.hi[k], I.whi[k]);
+        }
+    }
+    out[i] = N;
+}
+
 // ============================================================ wavefront schedule
 // Queue entries carry the tile-ordered path index q in origin.w (kNoPath = none).
 __global__ void k_camera(CamF cam, uint32_t seed, uint32_t s, uint32_t nq, uint32_t tiles_x, float4* ray_o,
@@ -170,7 +197,7 @@ __global__ void __launch_bounds__(128) k_shade(
         tile_xy(q, tiles_x, x, y);
         uint32_t pix = x + y * (uint32_t)W;
         Surface S = surface_at(verts, idx, mat, id, h4.y, h4.z);
-        if (bounce == 0) accumulate_first_hit(acc_tile(acc, q), q & 63u, id, h4.x, S);
+        if (bounce == 0) accumulate_first_hit(acc_tile(acc, q), q & 63u, id, 0u, h4.x, S);
         org = S.org;
         float beta[NB];
 #pragma unroll
@@ -305,6 +332,8 @@ struct FusedScratch {
     double* inst_m;                // 16 per instance
     double* light_pos;             // 3 per light
     JobExt* ext;                   // one
+    TNode* tnodes;                 // TLAS nodes of the instanced scene
+    bool host_staging;             // the instance table may be staged through host memory the call allocates
 };
 constexpr int kCounterWords = 64;
 
@@ -327,7 +356,8 @@ constexpr size_t kNoAllocLightPosOff = kNoAllocLightOff + 1024;
 constexpr size_t kNoAllocInstOff = kNoAllocLightPosOff + 256;
 constexpr size_t kNoAllocInstMOff = kNoAllocInstOff + sizeof(InstG) * kNoAllocMaxInstances;
 constexpr size_t kNoAllocExtOff = kNoAllocInstMOff + 16 * 8 * kNoAllocMaxInstances;
-constexpr size_t kNoAllocBytes = kNoAllocExtOff + ((sizeof(JobExt) + 255) / 256) * 256;
+constexpr size_t kNoAllocTNodeOff = kNoAllocExtOff + ((sizeof(JobExt) + 255) / 256) * 256;
+constexpr size_t kNoAllocBytes = kNoAllocTNodeOff + sizeof(TNode) * (2 * kNoAllocMaxInstances);
 static_assert(sizeof(LightF) * VIRA_GPU_MAX_LIGHTS <= 1024, "light table fits its slot");
 static_assert(3 * 8 * VIRA_GPU_MAX_LIGHTS <= 256, "light positions fit their slot");
 static_assert(kNoAllocInstOff % 16 == 0 && kNoAllocInstMOff % 16 == 0 && kNoAllocExtOff % 16 == 0,
@@ -380,12 +410,22 @@ void run_fused(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o
     const MatF mf = mat_params(m);
     if (inst) {
         // instance table (pointers, materials) and FP64 inputs; k_rebase fills the FP32 placement.
-        // Staged in blocks of kStage instances so no host memory is allocated.
+        // The allocation-free call stages blocks of kStage instances on the stack; others take the
+        // whole table in one host buffer.
         constexpr int kStage = 16;
-        InstG hi[kStage];
-        double hm[16 * kStage];
-        for (int i0 = 0; i0 < ninst; i0 += kStage) {
-            const int n = std::min(kStage, ninst - i0);
+        InstG hi_stack[kStage];
+        double hm_stack[16 * kStage];
+        std::vector<InstG> hi_vec;
+        std::vector<double> hm_vec;
+        const int stage = S.host_staging ? std::max(ninst, 1) : kStage;
+        if (S.host_staging) {
+            hi_vec.resize(stage);
+            hm_vec.resize(16 * (size_t)stage);
+        }
+        InstG* hi = S.host_staging ? hi_vec.data() : hi_stack;
+        double* hm = S.host_staging ? hm_vec.data() : hm_stack;
+        for (int i0 = 0; i0 < ninst; i0 += stage) {
+            const int n = std::min(stage, ninst - i0);
             for (int i = 0; i < n; ++i) {
                 const MeshG& mi = c.mesh(c.tlas[i0 + i].mesh_id);
                 hi[i] = InstG{};
@@ -405,7 +445,7 @@ void run_fused(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o
             VG_HIP(hipMemcpyAsync(S.inst + i0, hi, sizeof(InstG) * n, hipMemcpyHostToDevice, c.stream));
             VG_HIP(hipMemcpyAsync(S.inst_m + 16 * (size_t)i0, hm, 16 * 8 * n, hipMemcpyHostToDevice, c.stream));
             // pageable sources: wait before the staging arrays are refilled
-            if (i0 + kStage < ninst) VG_HIP(hipStreamSynchronize(c.stream));
+            if (i0 + stage < ninst) VG_HIP(hipStreamSynchronize(c.stream));
         }
         double hl[3 * VIRA_GPU_MAX_LIGHTS];
         for (int l = 0; l < nl; ++l)
@@ -413,6 +453,9 @@ void run_fused(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o
         if (nl > 0) VG_HIP(hipMemcpyAsync(S.light_pos, hl, 3 * 8 * nl, hipMemcpyHostToDevice, c.stream));
         k_rebase<<<blocks_for(std::max(ninst, nl), 64), 64, 0, c.stream>>>(
             S.inst_m, ninst, S.light_pos, nl, cam.position[0], cam.position[1], cam.position[2], S.inst, S.lights);
+        VG_HIP(hipGetLastError());
+        k_tlas_boxes<<<blocks_for(c.tlas_nodes, 64), 64, 0, c.stream>>>(c.tlas_topo.as<int2>(), c.tlas_nodes, S.inst,
+                                                                        S.tnodes);
         VG_HIP(hipGetLastError());
         cf.ox = cf.oy = cf.oz = 0.0f;  // the camera is the origin of the render frame
     }
@@ -443,7 +486,8 @@ void run_fused(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o
     J.rr_start = o.rr_start;
     J.tiles_x = tiles_x;
     J.inst = inst ? S.inst : nullptr;
-    J.ninst = ninst;
+    J.tlas = inst ? S.tnodes : nullptr;
+    J.ntlas = inst ? c.tlas_nodes : 0;
     J.ext = S.ext;
     const bool multi = fused_multi(c);
     const bool feat = fused_feat(c, o);
@@ -519,12 +563,13 @@ void render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, c
         if (inst) {
             c.r_inst.alloc(sizeof(InstG) * ninst);
             c.r_inst_m.alloc((size_t)16 * 8 * ninst);
+            c.r_tnodes.alloc(sizeof(TNode) * (size_t)c.tlas_nodes);
             c.r_light_pos.alloc((size_t)3 * 8 * VIRA_GPU_MAX_LIGHTS);
         }
         c.r_ext.alloc(sizeof(JobExt));
         const FusedScratch S{c.r_acc.as<float>(), c.r_counters.as<unsigned long long>(), c.r_lights.as<LightF>(),
                              c.r_inst.as<InstG>(), c.r_inst_m.as<double>(), c.r_light_pos.as<double>(),
-                             c.r_ext.as<JobExt>()};
+                             c.r_ext.as<JobExt>(), c.r_tnodes.as<TNode>(), true};
         EventPair ev;
         run_fused(c, cam, o, out, st, S, ev.a, ev.b);
         return;
@@ -662,7 +707,8 @@ void render_noalloc(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_op
                          reinterpret_cast<LightF*>(base + kNoAllocLightOff), reinterpret_cast<InstG*>(base + kNoAllocInstOff),
                          reinterpret_cast<double*>(base + kNoAllocInstMOff),
                          reinterpret_cast<double*>(base + kNoAllocLightPosOff),
-                         reinterpret_cast<JobExt*>(base + kNoAllocExtOff)};
+                         reinterpret_cast<JobExt*>(base + kNoAllocExtOff),
+                         reinterpret_cast<TNode*>(base + kNoAllocTNodeOff), false};
     run_fused(c, cam, o, out, st, S, c.na_ev[0], c.na_ev[1]);
 }
 
