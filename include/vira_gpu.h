@@ -261,6 +261,8 @@ typedef struct {
     float horizontal_scale, vertical_scale;
     int32_t horizontal_pattern_period, vertical_pattern_period;
     uint32_t seed;                                /* counter-based RNG key */
+    const float* filter_mosaic_dev;  /* W*H*8 colour filter array applied to the received energy
+                                        (Camera::hasFilterMosaic, camera.ipp:1293-1306), or NULL */
 } vira_gpu_sensor_sim;
 
 /* Point sources for the unresolved renderer (CPUUnresolvedRenderer::render,

@@ -226,3 +226,32 @@ def test_read_buffers_keeps_the_loaded_level(gpu_ctx, tmp_path):
     nv, nt = gpu_ctx.mesh_info(m1)
     assert nv == 65 * 65 and nt == 2 * 64 * 64
     d.close()
+
+
+def test_highly_compressible_level_reads(gpu_ctx, tmp_path):
+    """A nearly flat 513^2 DEM and a near-constant grey albedo image compress to a file
+    far smaller than the decoded samples; the reader bounds a compressed image by its
+    LZ4 block (at most ~255-fold expansion), not by the file size, so it reads them."""
+    n = 513
+    h = np.zeros((n, n), np.float32)
+    h[0, 0] = 1.0                      # min != max, so the fixed-point range is defined
+    alb = np.full((n, n), 0.12, np.float32)
+    alb[0, 0] = 0.13
+    p = Q.DEMProjection(resolution=(n, n), pixel_scale=(20.0, 20.0))
+    lv = [Q.DEMLevel(h, p, albedo=alb)]
+    path = tmp_path / "flat.qld"
+    Q.write_pyramid(str(path), lv, compress=True)
+    size = path.stat().st_size
+    assert size < n * n  # smaller than even the 8-bit samples
+    d = Q.QuipuDEM(str(path))
+    mid, got_h, got_a, info = d.read_level(gpu_ctx, 0, mesh=True, samples=True)
+    assert info["compressed"] == 1 and info["albedo_type"] == 1
+    mm = Q._valid_minmax(h)
+    bits = info["height_bits"]
+    want = _fixed_to_float(Q.float_to_fixed(h, mm, np.uint8 if bits == 8 else np.uint16), mm) if bits < 32 else h
+    assert np.array_equal(got_h.cpu().numpy().view(np.uint32), want.view(np.uint32))
+    am = Q._valid_minmax(alb)
+    wa = _fixed_to_float(Q.float_to_fixed(alb, am, np.uint8), am)
+    assert np.array_equal(got_a.cpu().numpy()[..., 0], wa)
+    assert gpu_ctx.mesh_info(mid)[1] == 2 * (n - 1) ** 2
+    d.close()

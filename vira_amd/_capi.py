@@ -228,6 +228,7 @@ class SensorSim(C.Structure):
         ("horizontal_pattern_period", C.c_int32),
         ("vertical_pattern_period", C.c_int32),
         ("seed", C.c_uint32),
+        ("filter_mosaic_dev", C.c_void_p),
     ]
 
 

@@ -913,9 +913,8 @@ void blas_refit(Ctx& c, MeshG& m) {
         blas_build(c, m, B.opts);
         return;
     }
-    hipEvent_t e0, e1;
-    VG_HIP(hipEventCreate(&e0));
-    VG_HIP(hipEventCreate(&e1));
+    EventPair ev;  // released on every path
+    hipEvent_t e0 = ev.a, e1 = ev.b;
     VG_HIP(hipEventRecord(e0, c.stream));
     k_refit_tris<<<blocks_for(m.nt, 256), 256, 0, c.stream>>>(m.nt, m.idx.as<uint32_t>(), m.verts.as<GVertex>(),
                                                               B.tris.as<float4>());
@@ -938,8 +937,6 @@ void blas_refit(Ctx& c, MeshG& m) {
     for (int k = 0; k < 3; ++k) { B.lo[k] = hb[k]; B.hi[k] = hb[3 + k]; }
     float ms = 0.f;
     VG_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     B.stats.build_ms = ms;
 }
 
@@ -951,9 +948,8 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     B.built = false;
     B.level_start.clear();
     VG_CHECK(N < (1u << 27), VIRA_GPU_ERR_INVALID, "BLAS limited to 2^27 triangles");
-    hipEvent_t e0, e1;
-    VG_HIP(hipEventCreate(&e0));
-    VG_HIP(hipEventCreate(&e1));
+    EventPair ev;  // released on every path
+    hipEvent_t e0 = ev.a, e1 = ev.b;
     VG_HIP(hipEventRecord(e0, c.stream));
     if (N == 0) {
         // one root with only empty children, valid as a Node2 (refs at word 12) and as a Node4 (refs at word 24)
@@ -970,8 +966,6 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
         B.stats.num_nodes = 1;
         for (int k = 0; k < 3; ++k) { B.lo[k] = 1.f; B.hi[k] = -1.f; }  // empty box: never entered
         B.built = true;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         return;
     }
     DBuf pbuf, prims, tmp, nodes, keys, slot_bin;
@@ -1278,8 +1272,6 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     VG_CHECK(emit_err == 0, VIRA_GPU_ERR_HIP, "BVH build produced a leaf with more than 16 triangles");
     float ms = 0.f;
     VG_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     B.stats = vira_gpu_bvh_stats{};
     B.stats.num_nodes = hs.internal;
     B.stats.num_wide_nodes = B.num_nodes;

@@ -176,14 +176,17 @@ Img read_image(Rd& r, uint16_t expect_id, int elem, double& decode_ms) {
     im.h = r.get<uint32_t>();
     im.elem = elem;
     im.compressed = r.get<uint8_t>() != 0;
+    VG_CHECK((uint64_t)im.w * im.h < (1ull << 31), VIRA_GPU_ERR_INVALID, "quipu: image of 2^31 pixels or more");
     const uint64_t n = (uint64_t)im.w * im.h * (uint64_t)elem;
-    VG_CHECK(n <= r.size, VIRA_GPU_ERR_INVALID, "quipu: image larger than its file");
-    im.raw.resize(n);
+    const uint64_t left = r.size - std::min(r.size, r.pos());
     if (im.compressed) {
         const uint64_t stated = r.get<uint64_t>();
         VG_CHECK(stated == n, VIRA_GPU_ERR_INVALID, "quipu: compressed image size disagrees with its resolution");
         const uint64_t cn = r.get<uint64_t>();
-        VG_CHECK(cn <= r.size - std::min(r.size, r.pos()), VIRA_GPU_ERR_INVALID, "quipu: truncated LZ4 block");
+        VG_CHECK(cn <= left, VIRA_GPU_ERR_INVALID, "quipu: truncated LZ4 block");
+        // an LZ4 block expands at most ~255-fold (a match length byte of 255 per input byte)
+        VG_CHECK(n <= 255 * cn + 16, VIRA_GPU_ERR_INVALID, "quipu: image larger than its LZ4 block can hold");
+        im.raw.resize(n);
         std::vector<uint8_t> comp(cn);
         r.bytes(comp.data(), cn);
         const auto t0 = std::chrono::steady_clock::now();
@@ -191,6 +194,8 @@ Img read_image(Rd& r, uint16_t expect_id, int elem, double& decode_ms) {
         decode_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         VG_CHECK(got == (long long)n, VIRA_GPU_ERR_INVALID, "quipu: LZ4 block failed to decompress");
     } else {
+        VG_CHECK(n <= left, VIRA_GPU_ERR_INVALID, "quipu: image larger than the rest of its file");
+        im.raw.resize(n);
         // stored with i outer, j inner: transpose into memory order
         std::vector<uint8_t> tmp(n);
         r.bytes(tmp.data(), n);

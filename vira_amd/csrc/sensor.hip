@@ -273,8 +273,9 @@ constexpr uint32_t kDimPhoton = 64, kDimDark = 80, kDimReadout = 96;
 
 // Poisson(lambda): multiplication method below 10, PTRS (Hoermann 1993,
 // "The transformed rejection method for generating Poisson random
-// variables") at and above; at most 64 proposals (the mean is returned in
-// the vanishingly rare case that none is accepted).
+// variables") at and above, in FP64 as std::poisson_distribution works
+// (camera.ipp:1306-1316); at most 64 proposals (the mean is returned in the
+// vanishingly rare case that none is accepted).
 __device__ __forceinline__ float poisson(float lambda, uint32_t seed, uint32_t pix, uint32_t dim) {
     if (!(lambda > 0.0f)) return 0.0f;
     if (lambda < 10.0f) {
@@ -286,20 +287,23 @@ __device__ __forceinline__ float poisson(float lambda, uint32_t seed, uint32_t p
         }
         return lambda;
     }
-    const float slam = sqrtf(lambda), loglam = logf(lambda);
-    const float b = 0.931f + 2.53f * slam;
-    const float a = -0.059f + 0.02483f * b;
-    const float invalpha = 1.1239f + __fdiv_rn(1.1328f, b - 3.4f);
-    const float vr = 0.9277f - __fdiv_rn(3.6224f, b - 2.0f);
+    // PTRS in FP64: at large means the log-acceptance terms reach ~1e7, where
+    // FP32's spacing (~1) would distort the accepted distribution
+    const double lam = (double)lambda;
+    const double slam = sqrt(lam), loglam = log(lam);
+    const double b = 0.931 + 2.53 * slam;
+    const double a = -0.059 + 0.02483 * b;
+    const double invalpha = 1.1239 + 1.1328 / (b - 3.4);
+    const double vr = 0.9277 - 3.6224 / (b - 2.0);
     for (uint32_t it = 0; it < 64u; ++it) {
-        const float U = rng_u01(seed, pix, 2u * it, dim) - 0.5f;
-        const float V = rng_u01(seed, pix, 2u * it + 1u, dim);
-        const float us = 0.5f - fabsf(U);
-        const float k = floorf((2.0f * __fdiv_rn(a, us) + b) * U + lambda + 0.43f);
-        if (us >= 0.07f && V <= vr) return k;
-        if (k < 0.0f || (us < 0.013f && V > us)) continue;
-        if ((logf(V) + logf(invalpha) - logf(__fdiv_rn(a, us * us) + b)) <= (-lambda + k * loglam - lgammaf(k + 1.0f)))
-            return k;
+        const double U = (double)rng_u01(seed, pix, 2u * it, dim) - 0.5;
+        const double V = (double)rng_u01(seed, pix, 2u * it + 1u, dim);
+        const double us = 0.5 - fabs(U);
+        const double k = floor((2.0 * a / us + b) * U + lam + 0.43);
+        if (us >= 0.07 && V <= vr) return (float)k;
+        if (k < 0.0 || (us < 0.013 && V > us)) continue;
+        if ((log(V) + log(invalpha) - log(a / (us * us) + b)) <= (-lam + k * loglam - lgamma(k + 1.0)))
+            return (float)k;
     }
     return lambda;
 }
@@ -316,12 +320,13 @@ __global__ void k_simulate_sensor(vira_gpu_sensor_sim c, const float* __restrict
     const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= (size_t)W * H) return;
     const uint32_t pix = (uint32_t)p;
-    // getPhotonCounts (camera.ipp:1281-1330): energy = t P, photons = energy / E_photon
+    // getPhotonCounts (camera.ipp:1281-1330): energy = t P (x the filter mosaic), photons = energy / E_photon
     float cnt[NB];
     float total = 0.0f;
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-        const float energy = c.exposure_time * power[p * NB + k];
+        float energy = c.exposure_time * power[p * NB + k];
+        if (c.filter_mosaic_dev) energy = energy * c.filter_mosaic_dev[p * NB + k];  // camera.ipp:1296
         cnt[k] = __fdiv_rn(energy, c.photon_energy[k]);
         total = total + cnt[k];
     }

@@ -580,14 +580,19 @@ class Context:
                    kernel.shape[0], out.data_ptr())
         return out
 
-    def simulate_sensor(self, config, power, photons: bool = False):
+    def simulate_sensor(self, config, power, photons: bool = False, filter_mosaic=None):
         """Camera::simulateSensor: (H, W, 8) received power -> (H, W) normalised image
-        (and the (H, W, 8) photon counts if asked)."""
+        (and the (H, W, 8) photon counts if asked); filter_mosaic: an optional (H, W, 8)
+        colour filter array on the device (e.g. sensor.bayer_mosaic)."""
         p = power.contiguous()
         H, W = p.shape[:2]
         img = self.torch.empty((H, W), dtype=self.torch.float32, device=p.device)
         ph = self.torch.empty_like(p) if photons else None
         c = config.to_c()
+        if filter_mosaic is not None:
+            if tuple(filter_mosaic.shape) != (H, W, NUM_BINS) or not filter_mosaic.is_contiguous():
+                raise ValueError("filter_mosaic: contiguous (H, W, 8) float32 device tensor")
+            c.filter_mosaic_dev = filter_mosaic.data_ptr()
         self._call(self.lib.vira_gpu_simulate_sensor, self.h, C.byref(c), p.data_ptr(), W, H,
                    None if ph is None else ph.data_ptr(), img.data_ptr())
         return img, ph

@@ -184,3 +184,16 @@ def minimum_detectable_power(aperture_area: float, exposure_time: float = 0.01,
     energy = f(f(minimum_electrons) * total)
     irradiance = f(energy / f(f(aperture_area) * f(exposure_time)))
     return float(f(irradiance * f(exposure_time)))
+
+
+def bayer_mosaic(width: int, height: int, red, green, blue) -> np.ndarray:
+    """bayerFilter (filter_arrays.ipp:22-50): (H, W, 8) float32 colour filter array;
+    pixel (i, j) (column i, row j) takes red where i and j are even, blue where both
+    are odd and green elsewhere.  red/green/blue: 8-band transmissions."""
+    r, g, b = (np.broadcast_to(np.asarray(x, np.float32), (NUM_BINS,)) for x in (red, green, blue))
+    i = np.arange(width)[None, :]
+    j = np.arange(height)[:, None]
+    even_i, even_j = (i % 2) == 0, (j % 2) == 0
+    out = np.where((even_i & even_j)[..., None], r, np.where((~even_i & ~even_j)[..., None], b, g))
+    return np.ascontiguousarray(out, np.float32)
+
