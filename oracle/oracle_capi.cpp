@@ -68,8 +68,9 @@ void orc_dem_positions(const float* heights, int W, int H, const Projection* pro
         }
 }
 
-void orc_vertex_normals(const float* pos, size_t nv, const uint32_t* idx, size_t nt, float* nrm) {
-    vertex_normals(pos, nv, idx, nt, nrm);
+void orc_vertex_normals(const float* pos, const double* pos64, size_t nv, const uint32_t* idx, size_t nt,
+                        float* nrm) {
+    vertex_normals(pos, pos64, nv, idx, nt, nrm);
 }
 
 void* orc_mesh_create(const float* pos, const double* pos64, size_t nv, const uint32_t* idx, size_t nt,
@@ -85,6 +86,8 @@ void* orc_mesh_create(const float* pos, const double* pos64, size_t nv, const ui
     if (alb) o->m.alb.assign(alb, alb + (size_t)NB * nv);
     else o->m.alb.assign((size_t)NB * nv, 1.0f);
     if (uv) o->m.uv.assign(uv, uv + 2 * nv);
+    o->m.fn.resize(4 * nt);
+    face_normals(o->m.pos.data(), pos64, o->m.idx.data(), nt, o->m.fn.data());
     auto t0 = std::chrono::steady_clock::now();
     o->b = bvh_build(o->m, nthreads);
     o->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

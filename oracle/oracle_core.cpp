@@ -584,7 +584,13 @@ void render_impl(const InstanceO* inst, int ninst, const Camera& cam_in, const L
                             uvu = (w0 * (R)m.uv[2 * i1] + w1 * (R)m.uv[2 * i2]) + w2 * (R)m.uv[2 * i0];
                             uvv = (w0 * (R)m.uv[2 * i1 + 1] + w1 * (R)m.uv[2 * i2 + 1]) + w2 * (R)m.uv[2 * i0 + 1];
                         }
-                        V3<R> fn = normalize(cross(p1 - p0, p2 - p0));
+                        // FP32 twin: the mesh's stored face normal (FP64-derived, as the kernels read it);
+                        // FP64: from the FP64 positions directly
+                        V3<R> fn;
+                        if constexpr (std::is_same<R, float>::value)
+                            fn = V3<R>{m.fn[4 * h.id], m.fn[4 * h.id + 1], m.fn[4 * h.id + 2]};
+                        else
+                            fn = normalize(cross(p1 - p0, p2 - p0));
                         V3<R> n0 = vnrm(i0), n1 = vnrm(i1), n2 = vnrm(i2);
                         V3<R> Nl = fn;
                         if (mat.smooth) Nl = normalize((w0 * n1 + w1 * n2) + w2 * n0);

@@ -198,22 +198,56 @@ inline void dem_indices(const float* heights, int W, int H, std::vector<uint32_t
     }
 }
 
-// Vertex normals as Mesh::calculateNormals (mesh.ipp:199-233): sum of unit face
-// normals normalize(cross(p1-p0, p2-p0)) in triangle order, then normalised.
-inline void vertex_normals(const float* pos, size_t nv, const uint32_t* idx, size_t nt, float* nrm) {
-    std::fill(nrm, nrm + 3 * nv, 0.0f);
+// Unit face normal normalize(cross(q1 - q0, q2 - q0)) (triangle.ipp:36-44) in
+// FP64 from the FP64 positions the mesh was built from (pos64, or the FP32
+// positions widened), rounded to FP32 -- the face normal the kernels store per
+// triangle.  Returns false where the reference skips the triangle (a corner at
+// infinity, or a degenerate triangle: NaN normal, mesh.ipp:211-216).
+inline bool face_normal64(const double* q0, const double* q1, const double* q2, float out[3]) {
+    if (std::isinf(q0[0]) || std::isinf(q1[0]) || std::isinf(q2[0])) return false;
+    const double ax = q1[0] - q0[0], ay = q1[1] - q0[1], az = q1[2] - q0[2];
+    const double bx = q2[0] - q0[0], by = q2[1] - q0[1], bz = q2[2] - q0[2];
+    const double cx = ay * bz - by * az, cy = az * bx - bz * ax, cz = ax * by - bx * ay;
+    const double inv = 1.0 / std::sqrt((cx * cx + cy * cy) + cz * cz);
+    const double nx = cx * inv, ny = cy * inv, nz = cz * inv;
+    if (std::isnan(nx) || std::isnan(ny) || std::isnan(nz)) return false;
+    out[0] = (float)nx;
+    out[1] = (float)ny;
+    out[2] = (float)nz;
+    return true;
+}
+
+// Face normals of every triangle: fn[4t..4t+2] the normal, fn[4t+3] = 1 (or 0 where skipped).
+inline void face_normals(const float* pos, const double* pos64, const uint32_t* idx, size_t nt, float* fn) {
     for (size_t t = 0; t < nt; ++t) {
-        uint32_t i0 = idx[3 * t], i1 = idx[3 * t + 1], i2 = idx[3 * t + 2];
-        V3<float> p0{pos[3 * i0], pos[3 * i0 + 1], pos[3 * i0 + 2]};
-        V3<float> p1{pos[3 * i1], pos[3 * i1 + 1], pos[3 * i1 + 2]};
-        V3<float> p2{pos[3 * i2], pos[3 * i2 + 1], pos[3 * i2 + 2]};
-        if (std::isinf(p0.x) || std::isinf(p1.x) || std::isinf(p2.x)) continue;
-        V3<float> fn = normalize(cross(p1 - p0, p2 - p0));
-        if (std::isnan(fn.x) || std::isnan(fn.y) || std::isnan(fn.z)) continue;
-        for (uint32_t v : {i0, i1, i2}) {
-            nrm[3 * v] = nrm[3 * v] + fn.x;
-            nrm[3 * v + 1] = nrm[3 * v + 1] + fn.y;
-            nrm[3 * v + 2] = nrm[3 * v + 2] + fn.z;
+        double q[3][3];
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t v = idx[3 * t + k];
+            for (int r = 0; r < 3; ++r) q[k][r] = pos64 ? pos64[3 * v + r] : (double)pos[3 * v + r];
+        }
+        float n[3] = {0.0f, 0.0f, 0.0f};
+        const bool ok = face_normal64(q[0], q[1], q[2], n);
+        fn[4 * t] = n[0];
+        fn[4 * t + 1] = n[1];
+        fn[4 * t + 2] = n[2];
+        fn[4 * t + 3] = ok ? 1.0f : 0.0f;
+    }
+}
+
+// Vertex normals as Mesh::calculateNormals (mesh.ipp:199-233): the sum of the
+// unit face normals (face_normals) in triangle order, in FP32, then normalised.
+inline void vertex_normals(const float* pos, const double* pos64, size_t nv, const uint32_t* idx, size_t nt,
+                           float* nrm) {
+    std::fill(nrm, nrm + 3 * nv, 0.0f);
+    std::vector<float> fn(4 * nt);
+    face_normals(pos, pos64, idx, nt, fn.data());
+    for (size_t t = 0; t < nt; ++t) {
+        if (fn[4 * t + 3] == 0.0f) continue;
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t v = idx[3 * t + k];
+            nrm[3 * v] = nrm[3 * v] + fn[4 * t];
+            nrm[3 * v + 1] = nrm[3 * v + 1] + fn[4 * t + 1];
+            nrm[3 * v + 2] = nrm[3 * v + 2] + fn[4 * t + 2];
         }
     }
     for (size_t v = 0; v < nv; ++v) {
@@ -322,6 +356,7 @@ struct Mesh {
     std::vector<float> nrm;       // 3*nv
     std::vector<float> alb;       // NB*nv
     std::vector<float> uv;        // 2*nv (may be empty)
+    std::vector<float> fn;        // 4*nt face normals from the FP64 positions (face_normals)
     size_t nv = 0, nt = 0;
     uint32_t id = 0;              // Mesh::getID, written to the mesh_id pass
 };

@@ -78,7 +78,7 @@ def lib() -> C.CDLL:
         "orc_vertex_albedo": (None, [C.c_size_t, C.c_uint32, P]),
         "orc_dem_indices": (C.c_int64, [P, C.c_int, C.c_int, P]),
         "orc_dem_positions": (None, [P, C.c_int, C.c_int, C.POINTER(Projection), P, P]),
-        "orc_vertex_normals": (None, [P, C.c_size_t, P, C.c_size_t, P]),
+        "orc_vertex_normals": (None, [P, P, C.c_size_t, P, C.c_size_t, P]),
         "orc_mesh_create": (P, [P, P, C.c_size_t, P, C.c_size_t, P, P, P, C.c_int]),
         "orc_mesh_destroy": (None, [P]),
         "orc_mesh_build_ms": (C.c_double, [P]),
@@ -180,11 +180,14 @@ def dem_uv(W, H) -> np.ndarray:
     return np.stack([(i / (W - 1)).astype(np.float32), (j / (H - 1)).astype(np.float32)], -1).reshape(-1, 2)
 
 
-def vertex_normals(pos: np.ndarray, idx: np.ndarray) -> np.ndarray:
+def vertex_normals(pos: np.ndarray, idx: np.ndarray, pos64: np.ndarray | None = None) -> np.ndarray:
+    """Mesh::calculateNormals over face normals evaluated in FP64 from pos64 (the
+    FP64 positions the mesh was built from) or, without it, from pos widened."""
     pos = np.ascontiguousarray(pos, np.float32)
     idx = np.ascontiguousarray(idx, np.uint32)
+    p64 = None if pos64 is None else np.ascontiguousarray(pos64, np.float64)
     out = np.empty_like(pos)
-    lib().orc_vertex_normals(pos.ctypes.data, pos.shape[0], idx.ctypes.data, idx.size // 3, out.ctypes.data)
+    lib().orc_vertex_normals(pos.ctypes.data, _p(p64), pos.shape[0], idx.ctypes.data, idx.size // 3, out.ctypes.data)
     return out
 
 
@@ -424,15 +427,17 @@ def options(spp=1, bounces=0, rr=False, rr_start=1, seed=0, show_background=Fals
 
 
 def dem_mesh(heights: np.ndarray, proj: Projection, grey_albedo: np.ndarray | None = None,
-             with_pos64=False) -> Mesh:
-    """DEM -> mesh exactly as the oracle restates dem.ipp / image_utils.ipp / mesh.ipp."""
+             with_pos64=True) -> Mesh:
+    """DEM -> mesh exactly as the oracle restates dem.ipp / image_utils.ipp / mesh.ipp.
+    The mesh keeps its FP64 positions (face normals are evaluated from them, and
+    FP64 renders trace them); with_pos64 is accepted for older callers."""
     H, W = heights.shape
     pos, pos64 = dem_positions(heights, proj)
     idx = dem_indices(heights)
-    nrm = vertex_normals(pos, idx)
+    nrm = vertex_normals(pos, idx, pos64)
     a = np.ones(H * W, np.float32) if grey_albedo is None else np.asarray(grey_albedo, np.float32)
     alb = np.repeat(a[:, None], NB, axis=1)
-    return Mesh(pos, idx, nrm, alb, dem_uv(W, H), pos64 if with_pos64 else None)
+    return Mesh(pos, idx, nrm, alb, dem_uv(W, H), pos64)
 
 
 def pixel_solid_angles(cam: Camera) -> np.ndarray:

@@ -158,8 +158,8 @@ def test_dem_update_bit_exact_and_mask_rule(gpu_ctx):
     proj = W.make_projection(sc)
     oproj = O.make_projection(proj.kind, proj.spacing, proj.radius, tuple(proj.center), tuple(proj.offset))
     hn = h2.cpu().numpy()
-    opos, _ = O.dem_positions(hn, oproj)
-    onrm = O.vertex_normals(opos, O.dem_indices(hn))
+    opos, opos64 = O.dem_positions(hn, oproj)
+    onrm = O.vertex_normals(opos, O.dem_indices(hn), opos64)
     assert np.array_equal(pos.view(np.uint32), opos.view(np.uint32))
     assert np.array_equal(nrm.view(np.uint32), onrm.view(np.uint32))
     # the BLAS was rebuilt: closest hits equal the oracle's on the new geometry

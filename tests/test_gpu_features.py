@@ -26,11 +26,11 @@ def _oracle_mesh(built, mesh_id):
     proj = O.make_projection(p.kind, p.spacing, p.radius, tuple(p.center), tuple(p.offset))
     valb = None if built.vertex_albedo is None else built.vertex_albedo.cpu().numpy()
     h = built.heights.cpu().numpy()
-    pos, _ = O.dem_positions(h, proj)
+    pos, pos64 = O.dem_positions(h, proj)
     idx = O.dem_indices(h)
-    nrm = O.vertex_normals(pos, idx)
+    nrm = O.vertex_normals(pos, idx, pos64)
     a = np.ones(h.size, np.float32) if valb is None else valb
-    return O.Mesh(pos, idx, nrm, np.repeat(a[:, None], 8, 1), O.dem_uv(h.shape[1], h.shape[0]), None,
+    return O.Mesh(pos, idx, nrm, np.repeat(a[:, None], 8, 1), O.dem_uv(h.shape[1], h.shape[0]), pos64,
                   mesh_id=mesh_id)
 
 

@@ -53,9 +53,9 @@ def test_dem_mesh_bit_exact(gpu_ctx, kind):
     mid = gpu_ctx.dem_mesh(h, proj)
     pos, idx, nrm = gpu_ctx.mesh_download(mid)
     hn = h.cpu().numpy()
-    opos, _ = O.dem_positions(hn, oracle_projection(proj))
+    opos, opos64 = O.dem_positions(hn, oracle_projection(proj))
     oidx = O.dem_indices(hn)
-    onrm = O.vertex_normals(opos, oidx)
+    onrm = O.vertex_normals(opos, oidx, opos64)
     assert np.array_equal(idx, oidx)
     assert np.array_equal(pos.view(np.uint32), opos.view(np.uint32))
     assert np.array_equal(nrm.view(np.uint32), onrm.view(np.uint32))
@@ -75,8 +75,8 @@ def test_dem_mesh_holes_bit_exact(gpu_ctx):
     oidx = O.dem_indices(hn)
     assert idx.shape == oidx.shape and oidx.shape[0] < 2 * 39 * 32
     assert np.array_equal(idx, oidx)
-    opos, _ = O.dem_positions(hn, oracle_projection(proj))
-    onrm = O.vertex_normals(opos, oidx)
+    opos, opos64 = O.dem_positions(hn, oracle_projection(proj))
+    onrm = O.vertex_normals(opos, oidx, opos64)
     fin = np.isfinite(opos).all(1)
     assert np.array_equal(pos[fin].view(np.uint32), opos[fin].view(np.uint32))
     assert np.array_equal(nrm.view(np.uint32), onrm.view(np.uint32))

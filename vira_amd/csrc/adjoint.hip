@@ -310,9 +310,8 @@ __global__ void __launch_bounds__(kDiBS, 4) k_di_adjoint(const void* __restrict_
 
 // dL/dn_v -> dL/ds_v for n_v = s_v / |s_v|, s_v = sum of unit face normals.
 __global__ void k_adj_ngrad(size_t nv, const uint32_t* __restrict__ offs, const uint32_t* __restrict__ count,
-                            const uint32_t* __restrict__ adj, const uint32_t* __restrict__ idx,
-                            const GVertex* __restrict__ verts, const float4* __restrict__ gnrm,
-                            float4* __restrict__ gs) {
+                            const uint32_t* __restrict__ adj, const float4* __restrict__ fnrm,
+                            const float4* __restrict__ gnrm, float4* __restrict__ gs) {
     size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nv) return;
     const float4 g4 = gnrm[v];
@@ -322,10 +321,8 @@ __global__ void k_adj_ngrad(size_t nv, const uint32_t* __restrict__ offs, const 
         f3 s = mk3(0.f, 0.f, 0.f);
         const uint32_t b = offs[v], n = count[v];
         for (uint32_t a = 0; a < n; ++a) {
-            const uint32_t t = adj[b + a];
-            f3 fn;
-            if (face_normal(verts, idx[3 * (size_t)t], idx[3 * (size_t)t + 1], idx[3 * (size_t)t + 2], fn))
-                s = add(s, fn);
+            const float4 fn = fnrm[adj[b + a]];
+            if (fn.w != 0.0f) s = add(s, mk3(fn.x, fn.y, fn.z));
         }
         const float ls = length(s);
         if (ls > 0.0f) {
@@ -528,8 +525,8 @@ void di_adjoint(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_di_opts& o, c
     if (geo && nv) {
         k_adj_ngrad<<<blocks_for(nv, 256), 256, 0, c.stream>>>(nv, m.adj_offs.as<uint32_t>(),
                                                                 m.adj_count.as<uint32_t>(), m.adj.as<uint32_t>(),
-                                                                m.idx.as<uint32_t>(), m.verts.as<GVertex>(),
-                                                                c.di_gnrm.as<float4>(), c.di_gs.as<float4>());
+                                                                m.fnrm.as<float4>(), c.di_gnrm.as<float4>(),
+                                                                c.di_gs.as<float4>());
         VG_HIP(hipGetLastError());
         k_adj_heights<<<blocks_for(nv, 256), 256, 0, c.stream>>>(
             nv, m.dem_w, m.dem_h, m.dem_proj, m.adj_offs.as<uint32_t>(), m.adj_count.as<uint32_t>(),

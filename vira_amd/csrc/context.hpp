@@ -146,6 +146,7 @@ struct MeshG {
     size_t nv = 0, nt = 0;
     DBuf verts;   // GVertex[nv]
     DBuf idx;     // uint32[3 * nt]
+    DBuf fnrm;    // float4[nt]: unit face normal from the FP64 positions, w = 0 where skipped
     Blas blas;
     vira_gpu_material mat{};
     // DEM provenance (dem_mesh): node v is grid sample (v % dem_w, v / dem_w)
@@ -233,11 +234,12 @@ void launch_synth_fbm(hipStream_t s, int W, int H, uint32_t seed, double amp, in
 void launch_synth_albedo(hipStream_t s, size_t nv, uint32_t seed, float* out);
 void dem_build_mesh(Ctx& c, MeshG& m, const float* heights, int W, int H, const vira_gpu_projection& p,
                     const float* grey);
-// The DEM grid's triangles (imageToIndexBuffer order, holes skipped) and vertex
-// normals around vertices that write_vertices fills (GVertex[W*H]; normals are
-// overwritten).
+// The DEM grid's triangles (imageToIndexBuffer order, holes skipped), face
+// normals and vertex normals around vertices that write_vertices fills
+// (GVertex[W*H], and the FP64 positions, 3 per vertex, +inf x for holes;
+// normals are overwritten).
 void dem_grid_mesh(Ctx& c, MeshG& m, const float* heights, int W, int H,
-                   const std::function<void(GVertex*)>& write_vertices);
+                   const std::function<void(GVertex*, double*)>& write_vertices);
 void mesh_compute_normals(Ctx& c, MeshG& m);
 void mesh_adjacency(Ctx& c, MeshG& m);
 void dem_update(Ctx& c, MeshG& m, const float* heights);

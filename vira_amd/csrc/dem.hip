@@ -115,21 +115,24 @@ __device__ __forceinline__ void dem_world(const vira_gpu_projection& p, int W, i
 }
 
 __global__ void k_dem_vertices(const float* heights, int W, int H, vira_gpu_projection proj, const float* grey,
-                               GVertex* verts) {
+                               GVertex* verts, double* pos64) {
     size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= (size_t)W * H) return;
     int i = (int)(v % W), j = (int)(v / W);
     float h = heights[v];
     GVertex out;
+    double p[3] = {__longlong_as_double(0x7ff0000000000000ll), 0.0, 0.0};
     if (!isfinite(h)) {
         out.px = out.py = out.pz = __int_as_float(0x7f800000);
     } else {
-        double p[3];
         dem_world(proj, W, H, i, j, (double)h, p);
         out.px = (float)p[0];
         out.py = (float)p[1];
         out.pz = (float)p[2];
     }
+    pos64[3 * v] = p[0];
+    pos64[3 * v + 1] = p[1];
+    pos64[3 * v + 2] = p[2];
     float a = grey ? grey[v] : 1.0f;
     for (int k = 0; k < NB; ++k) out.alb[k] = a;
     out.nx = out.ny = out.nz = 0.0f;
@@ -150,7 +153,7 @@ __device__ __forceinline__ void finish_normal(GVertex& out, f3 n) {
 // Threads walk the vertices column by column (y fastest), the order of the
 // quad table and of the triangle ids, so neighbouring threads read
 // neighbouring entries of both.
-__global__ void k_dem_normals(int W, int H, const uint32_t* cand_to_tri, const uint32_t* idx, GVertex* verts) {
+__global__ void k_dem_normals(int W, int H, const uint32_t* cand_to_tri, const float4* fnrm, GVertex* verts) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)W * H) return;
     const int y = (int)(t % H), x = (int)(t / H);
@@ -165,9 +168,9 @@ __global__ void k_dem_normals(int W, int H, const uint32_t* cand_to_tri, const u
         size_t c = 2 * ((size_t)qi * (size_t)(H - 1) + (size_t)qj) + (size_t)ab[k];
         uint32_t t = cand_to_tri[c];
         if (t == 0xffffffffu) continue;
-        f3 fn;
-        if (!face_normal(verts, idx[3 * (size_t)t], idx[3 * (size_t)t + 1], idx[3 * (size_t)t + 2], fn)) continue;
-        n = add(n, fn);
+        const float4 fn = fnrm[t];
+        if (fn.w == 0.0f) continue;
+        n = add(n, mk3(fn.x, fn.y, fn.z));
     }
     finish_normal(verts[v], n);
 }
@@ -222,31 +225,60 @@ __global__ void k_adj_sort(size_t nv, const uint32_t* offs, const uint32_t* coun
 
 // Sums each vertex's face normals in triangle order (sorted adjacency).
 __global__ void k_adj_normals(size_t nv, const uint32_t* offs, const uint32_t* count, const uint32_t* adj,
-                              const uint32_t* idx, GVertex* verts) {
+                              const float4* fnrm, GVertex* verts) {
     size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nv) return;
     uint32_t b = offs[v], n = count[v];
     f3 acc = mk3(0.0f, 0.0f, 0.0f);
     for (uint32_t a = 0; a < n; ++a) {
-        uint32_t t = adj[b + a];
-        f3 fn;
-        if (!face_normal(verts, idx[3 * (size_t)t], idx[3 * (size_t)t + 1], idx[3 * (size_t)t + 2], fn)) continue;
-        acc = add(acc, fn);
+        const float4 fn = fnrm[adj[b + a]];
+        if (fn.w == 0.0f) continue;
+        acc = add(acc, mk3(fn.x, fn.y, fn.z));
     }
     finish_normal(verts[v], acc);
 }
 
-// DEM node positions for new heights (uv, albedo and normals untouched).
-__global__ void k_dem_positions(const float* heights, int W, int H, vira_gpu_projection proj, GVertex* verts) {
+// Face normals from FP64 positions (pos64, 3 per vertex).
+__global__ void k_face_normals64(size_t nt, const uint32_t* __restrict__ idx, const double* __restrict__ pos64,
+                                 float4* __restrict__ fnrm) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const uint32_t i0 = idx[3 * t], i1 = idx[3 * t + 1], i2 = idx[3 * t + 2];
+    fnrm[t] = face_normal64(pos64 + 3 * (size_t)i0, pos64 + 3 * (size_t)i1, pos64 + 3 * (size_t)i2);
+}
+
+// Face normals of a mesh given in FP32 (its positions widened to FP64).
+__global__ void k_face_normals_f32(size_t nt, const uint32_t* __restrict__ idx, const GVertex* __restrict__ verts,
+                                   float4* __restrict__ fnrm) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    double q[3][3];
+    for (int k = 0; k < 3; ++k) {
+        const GVertex& V = verts[idx[3 * t + k]];
+        q[k][0] = V.px;
+        q[k][1] = V.py;
+        q[k][2] = V.pz;
+    }
+    fnrm[t] = face_normal64(q[0], q[1], q[2]);
+}
+
+// DEM node positions for new heights (uv, albedo and normals untouched), and
+// their FP64 values for the face normals.
+__global__ void k_dem_positions(const float* heights, int W, int H, vira_gpu_projection proj, GVertex* verts,
+                                double* pos64) {
     size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= (size_t)W * H) return;
     float h = heights[v];
-    if (!isfinite(h)) return;
-    double p[3];
-    dem_world(proj, W, H, (int)(v % W), (int)(v / W), (double)h, p);
-    verts[v].px = (float)p[0];
-    verts[v].py = (float)p[1];
-    verts[v].pz = (float)p[2];
+    double p[3] = {__longlong_as_double(0x7ff0000000000000ll), 0.0, 0.0};
+    if (isfinite(h)) {
+        dem_world(proj, W, H, (int)(v % W), (int)(v / W), (double)h, p);
+        verts[v].px = (float)p[0];
+        verts[v].py = (float)p[1];
+        verts[v].pz = (float)p[2];
+    }
+    pos64[3 * v] = p[0];
+    pos64[3 * v + 1] = p[1];
+    pos64[3 * v + 2] = p[2];
 }
 
 // Counts nodes whose height changed between finite and non-finite (the
@@ -271,7 +303,7 @@ void launch_synth_albedo(hipStream_t s, size_t nv, uint32_t seed, float* out) {
 }
 
 void dem_grid_mesh(Ctx& c, MeshG& m, const float* heights, int W, int H,
-                   const std::function<void(GVertex*)>& write_vertices) {
+                   const std::function<void(GVertex*, double*)>& write_vertices) {
     VG_CHECK(W >= 2 && H >= 2, VIRA_GPU_ERR_INVALID, "DEM needs at least 2x2 samples");
     VG_CHECK((size_t)W * H < (1ull << 31), VIRA_GPU_ERR_INVALID, "DEM too large for 32-bit vertex ids");
     size_t nv = (size_t)W * H;
@@ -292,8 +324,16 @@ void dem_grid_mesh(Ctx& c, MeshG& m, const float* heights, int W, int H,
                                                                 ids.as<uint32_t>(), m.idx.as<uint32_t>(),
                                                                 c2t.as<uint32_t>());
     VG_HIP(hipGetLastError());
-    write_vertices(m.verts.as<GVertex>());
-    k_dem_normals<<<blocks_for(nv, 256), 256, 0, c.stream>>>(W, H, c2t.as<uint32_t>(), m.idx.as<uint32_t>(),
+    DBuf pos64;
+    pos64.alloc(nv * 3 * sizeof(double));
+    write_vertices(m.verts.as<GVertex>(), pos64.as<double>());
+    m.fnrm.alloc(std::max<size_t>(1, (size_t)nt) * sizeof(float4));
+    if (nt) {
+        k_face_normals64<<<blocks_for(nt, 256), 256, 0, c.stream>>>(nt, m.idx.as<uint32_t>(), pos64.as<double>(),
+                                                                    m.fnrm.as<float4>());
+        VG_HIP(hipGetLastError());
+    }
+    k_dem_normals<<<blocks_for(nv, 256), 256, 0, c.stream>>>(W, H, c2t.as<uint32_t>(), m.fnrm.as<float4>(),
                                                               m.verts.as<GVertex>());
     VG_HIP(hipGetLastError());
     VG_HIP(hipStreamSynchronize(c.stream));
@@ -301,8 +341,8 @@ void dem_grid_mesh(Ctx& c, MeshG& m, const float* heights, int W, int H,
 
 void dem_build_mesh(Ctx& c, MeshG& m, const float* heights, int W, int H, const vira_gpu_projection& p,
                     const float* grey) {
-    dem_grid_mesh(c, m, heights, W, H, [&](GVertex* verts) {
-        k_dem_vertices<<<blocks_for((size_t)W * H, 256), 256, 0, c.stream>>>(heights, W, H, p, grey, verts);
+    dem_grid_mesh(c, m, heights, W, H, [&](GVertex* verts, double* pos64) {
+        k_dem_vertices<<<blocks_for((size_t)W * H, 256), 256, 0, c.stream>>>(heights, W, H, p, grey, verts, pos64);
         VG_HIP(hipGetLastError());
     });
     m.is_dem = true;
@@ -325,12 +365,19 @@ void dem_update(Ctx& c, MeshG& m, const float* heights) {
     VG_HIP(hipStreamSynchronize(c.stream));
     VG_CHECK(bad == 0, VIRA_GPU_ERR_INVALID,
              "dem_update: " + std::to_string(bad) + " heights changed between finite and non-finite");
+    DBuf pos64;
+    pos64.alloc(nv * 3 * sizeof(double));
     k_dem_positions<<<blocks_for(nv, 256), 256, 0, c.stream>>>(heights, m.dem_w, m.dem_h, m.dem_proj,
-                                                                m.verts.as<GVertex>());
+                                                                m.verts.as<GVertex>(), pos64.as<double>());
     VG_HIP(hipGetLastError());
+    if (m.nt) {
+        k_face_normals64<<<blocks_for(m.nt, 256), 256, 0, c.stream>>>(m.nt, m.idx.as<uint32_t>(), pos64.as<double>(),
+                                                                      m.fnrm.as<float4>());
+        VG_HIP(hipGetLastError());
+    }
     mesh_adjacency(c, m);
     k_adj_normals<<<blocks_for(nv, 256), 256, 0, c.stream>>>(nv, m.adj_offs.as<uint32_t>(), m.adj_count.as<uint32_t>(),
-                                                              m.adj.as<uint32_t>(), m.idx.as<uint32_t>(),
+                                                              m.adj.as<uint32_t>(), m.fnrm.as<float4>(),
                                                               m.verts.as<GVertex>());
     VG_HIP(hipGetLastError());
     VG_HIP(hipStreamSynchronize(c.stream));
@@ -339,6 +386,12 @@ void dem_update(Ctx& c, MeshG& m, const float* heights) {
 void mesh_upload_pack(Ctx& c, MeshG& m, const float* pos, const float* alb, const float* uv, const float* nrm) {
     k_pack_vertices<<<blocks_for(m.nv, 256), 256, 0, c.stream>>>(m.nv, pos, alb, uv, nrm, m.verts.as<GVertex>());
     VG_HIP(hipGetLastError());
+    m.fnrm.alloc(std::max<size_t>(1, m.nt) * sizeof(float4));
+    if (m.nt) {
+        k_face_normals_f32<<<blocks_for(m.nt, 256), 256, 0, c.stream>>>(m.nt, m.idx.as<uint32_t>(),
+                                                                        m.verts.as<GVertex>(), m.fnrm.as<float4>());
+        VG_HIP(hipGetLastError());
+    }
 }
 
 void mesh_adjacency(Ctx& c, MeshG& m) {
@@ -373,7 +426,7 @@ void mesh_compute_normals(Ctx& c, MeshG& m) {
     mesh_adjacency(c, m);
     k_adj_normals<<<blocks_for(m.nv, 256), 256, 0, c.stream>>>(m.nv, m.adj_offs.as<uint32_t>(),
                                                                 m.adj_count.as<uint32_t>(), m.adj.as<uint32_t>(),
-                                                                m.idx.as<uint32_t>(), m.verts.as<GVertex>());
+                                                                m.fnrm.as<float4>(), m.verts.as<GVertex>());
     VG_HIP(hipGetLastError());
     VG_HIP(hipStreamSynchronize(c.stream));
 }

@@ -231,14 +231,22 @@ __device__ __forceinline__ void wave_add(unsigned long long* counter, uint32_t v
 
 __device__ __forceinline__ f3 vpos(const GVertex* V, uint32_t i) { return f3{V[i].px, V[i].py, V[i].pz}; }
 
-// Unit face normal normalize(cross(p1 - p0, p2 - p0)) of mesh.ipp:211-214;
-// returns false where the reference skips the triangle (a corner at infinity
-// or a degenerate triangle).
-__device__ __forceinline__ bool face_normal(const GVertex* V, uint32_t i0, uint32_t i1, uint32_t i2, f3& fn) {
-    f3 p0 = vpos(V, i0), p1 = vpos(V, i1), p2 = vpos(V, i2);
-    if (isinf(p0.x) || isinf(p1.x) || isinf(p2.x)) return false;
-    fn = normalize(cross(sub(p1, p0), sub(p2, p0)));
-    return !(isnan(fn.x) || isnan(fn.y) || isnan(fn.z));
+// Unit face normal normalize(cross(q1 - q0, q2 - q0)) (triangle.ipp:36-44,
+// mesh.ipp:211-214) evaluated in FP64 from the FP64 positions the mesh was
+// built from, stored FP32: x, y, z and w = 1, or w = 0 where the reference
+// skips the triangle (a corner at infinity or a degenerate triangle).  FP32
+// positions tens of kilometres from the origin are quantised to millimetres,
+// which would tilt the normals of 30 m facets by ~1e-4 rad; from FP64 they
+// keep FP32 precision.
+__device__ __forceinline__ float4 face_normal64(const double* q0, const double* q1, const double* q2) {
+    if (isinf(q0[0]) || isinf(q1[0]) || isinf(q2[0])) return make_float4(0.f, 0.f, 0.f, 0.f);
+    const double ax = q1[0] - q0[0], ay = q1[1] - q0[1], az = q1[2] - q0[2];
+    const double bx = q2[0] - q0[0], by = q2[1] - q0[1], bz = q2[2] - q0[2];
+    const double cx = ay * bz - by * az, cy = az * bx - bz * ax, cz = ax * by - bx * ay;
+    const double inv = __ddiv_rn(1.0, __dsqrt_rn((cx * cx + cy * cy) + cz * cz));
+    const double nx = cx * inv, ny = cy * inv, nz = cz * inv;
+    if (isnan(nx) || isnan(ny) || isnan(nz)) return make_float4(0.f, 0.f, 0.f, 0.f);
+    return make_float4((float)nx, (float)ny, (float)nz, 1.0f);
 }
 
 }  // namespace vg

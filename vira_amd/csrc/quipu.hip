@@ -282,17 +282,18 @@ __device__ void q_world(const QProj& p, int i, int j, double z, double out[3]) {
 // albedo: 0 none (default value), 1 constant spectrum, 2 grey image x
 // profile, 3 eight grey channels
 __global__ void k_qld_vertices(const float* __restrict__ heights, int W, int H, QProj p, int albedo_kind,
-                               const float* __restrict__ albedo, vira_gpu_qld_read_opts o, GVertex* __restrict__ verts) {
+                               const float* __restrict__ albedo, vira_gpu_qld_read_opts o, GVertex* __restrict__ verts,
+                               double* __restrict__ pos64) {
     const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= (size_t)W * H) return;
     const int i = (int)(v % (size_t)W), j = (int)(v / (size_t)W);
     const float h = heights[v];
     GVertex out;
     for (int k = 0; k < NB; ++k) out.alb[k] = 0.0f;
+    double w[3] = {__longlong_as_double(0x7ff0000000000000ll), 0.0, 0.0};
     if (!isfinite(h)) {
         out.px = out.py = out.pz = __int_as_float(0x7f800000);
     } else {
-        double w[3];
         q_world(p, i, j, (double)h, w);
         out.px = (float)w[0];
         out.py = (float)w[1];
@@ -309,6 +310,9 @@ __global__ void k_qld_vertices(const float* __restrict__ heights, int W, int H, 
     out.u = (float)__ddiv_rn((double)i, (double)(W - 1));
     out.v = (float)__ddiv_rn((double)j, (double)(H - 1));
     verts[v] = out;
+    pos64[3 * v] = w[0];
+    pos64[3 * v + 1] = w[1];
+    pos64[3 * v + 2] = w[2];
 }
 
 // A tolerant scan of a WKT1 projection string: PROJECTION["name"] and
@@ -560,11 +564,12 @@ void qld_read(Ctx& c, const vira_gpu_qld& q, int level, const vira_gpu_qld_read_
     if (albedo_dev) {
         // W*H*8 per-vertex albedo as the mesh would carry it (k_qld_vertices' rule), for callers that
         // want the decoded data without a mesh
-        DBuf tmpv;
+        DBuf tmpv, tmp64;
         tmpv.alloc(n * sizeof(GVertex));
+        tmp64.alloc(n * 3 * sizeof(double));
         QProj qp = make_qproj(L.proj, o.offset);
         k_qld_vertices<<<blocks_for(n, 256), 256, 0, c.stream>>>(heights, W, H, qp, kind, alb, o,
-                                                                 (GVertex*)tmpv.p);
+                                                                 (GVertex*)tmpv.p, tmp64.as<double>());
         VG_HIP(hipGetLastError());
         VG_HIP(hipMemcpy2DAsync(albedo_dev, NB * sizeof(float), (const char*)tmpv.p + offsetof(GVertex, alb),
                                 sizeof(GVertex), NB * sizeof(float), n, hipMemcpyDeviceToDevice, c.stream));
@@ -575,8 +580,8 @@ void qld_read(Ctx& c, const vira_gpu_qld& q, int level, const vira_gpu_qld_read_
         id = new_mesh();
         MeshG& m = c.mesh(id);
         const QProj qp = make_qproj(L.proj, o.offset);
-        dem_grid_mesh(c, m, heights, W, H, [&](GVertex* verts) {
-            k_qld_vertices<<<blocks_for(n, 256), 256, 0, c.stream>>>(heights, W, H, qp, kind, alb, o, verts);
+        dem_grid_mesh(c, m, heights, W, H, [&](GVertex* verts, double* pos64) {
+            k_qld_vertices<<<blocks_for(n, 256), 256, 0, c.stream>>>(heights, W, H, qp, kind, alb, o, verts, pos64);
             VG_HIP(hipGetLastError());
         });
         *mesh_id_out = id;

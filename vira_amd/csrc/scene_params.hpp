@@ -23,6 +23,7 @@ inline CamF cam_params(const vira_gpu_camera& cam) {
 
 inline MatF mat_params(const MeshG& m) {
     MatF mf;
+    mf.fnrm = m.fnrm.as<float4>();
     for (int k = 0; k < NB; ++k) mf.albedo[k] = m.mat.albedo[k];
     mf.tex = m.mat.texture_dev;
     mf.tw = m.mat.tex_width;

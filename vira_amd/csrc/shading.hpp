@@ -42,6 +42,7 @@ struct CamF {
     float zdir;
 };
 struct MatF {
+    const float4* fnrm; // the mesh's unit face normals (FP64-derived, FP32), per triangle
     float albedo[NB];
     const float* tex;   // albedo map: tch floats per texel
     int tw, th;
@@ -255,7 +256,9 @@ __device__ __forceinline__ void local_hit(const GVertex* __restrict__ verts, con
     const f3 p0 = mk3(V0.px, V0.py, V0.pz), p1 = mk3(V1.px, V1.py, V1.pz), p2 = mk3(V2.px, V2.py, V2.pz);
     // Embree barycentrics: w = (u, v, 1-u-v) paired with (v1, v2, v0) (embree_tlas.ipp:160-166)
     const float w0 = b1, w1 = b2, w2 = (1.0f - b1) - b2;
-    L.fn = normalize(cross(sub(p1, p0), sub(p2, p0)));  // == -normalize(cross(v0-v1, v2-v0))
+    // normalize(cross(p1 - p0, p2 - p0)) of the mesh's FP64 positions (triangle.ipp:36-44)
+    const float4 fn4 = mat.fnrm[id];
+    L.fn = mk3(fn4.x, fn4.y, fn4.z);
     const f3 n0 = mk3(V0.nx, V0.ny, V0.nz), n1 = mk3(V1.nx, V1.ny, V1.nz), n2 = mk3(V2.nx, V2.ny, V2.nz);
     L.Nl = L.fn;
     if (mat.smooth) L.Nl = normalize(add(add(mul(w0, n1), mul(w1, n2)), mul(w2, n0)));
