@@ -175,6 +175,22 @@ def cpu_baseline_rows(built, passes) -> dict:
     return out
 
 
+def visits_leg(ctx, built, passes) -> dict:
+    """One untimed render of the same frame by the visit-counting kernel: BVH
+    nodes, leaves and triangles fetched per ray on the GPU's own tree, pushes
+    past the 16 LDS stack entries, and the deepest stack."""
+    from dataclasses import replace
+    st = ctx.render(built.camera, replace(built.options, count_visits=True), passes)
+    n_closest = st["primary_rays"] + st["bounce_rays"]
+    n_shadow = max(1, st["shadow_rays"])
+    out = {"kernel_ms": round(st["fused_ms"], 2)}
+    for kind, v, n in (("closest", st["closest_visits"], n_closest), ("shadow", st["shadow_visits"], n_shadow)):
+        out[kind] = {"rays": n, "nodes_per_ray": round(v["nodes"] / n, 3), "leaves_per_ray": round(v["leaves"] / n, 3),
+                     "triangles_per_ray": round(v["triangles"] / n, 3),
+                     "spills_per_ray": round(v["stack_spills"] / n, 4), "max_stack": v["max_stack"]}
+    return out
+
+
 def adjoint_leg(ctx, steps: int, warmup: int) -> dict:
     """Differentiable DI: 1025^2 DEM (2,097,152 triangles), 1024^2 image, fixed-visibility
     direct illumination.  Times the forward render (keeping hit records) and the
@@ -470,6 +486,7 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the mixed-batch batched/per-request leg")
     ap.add_argument("--no-instanced", action="store_true", help="skip the instanced-scene leg")
     ap.add_argument("--no-quipu", action="store_true", help="skip the quipu pyramid read leg")
+    ap.add_argument("--no-visits", action="store_true", help="skip the BVH visit-counting render")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -523,6 +540,9 @@ def main() -> int:
           f"{sum(s['trace_shadow_ms'] for s in stats):.1f} ms, shading {sum(s['shade_ms'] for s in stats):.1f} ms",
           file=sys.stderr, flush=True)
 
+    visits = visits_leg(ctx, built, passes) if (fused and not args.no_visits) else None
+    if visits is not None:
+        print(f"# GPU visits (BVH{built.bvh.get('width', 4)}): {json.dumps(visits)}", file=sys.stderr, flush=True)
     sensor = sensor_leg(ctx, built, passes, max(1, min(3, args.steps))) if args.scene == "dem4097" else None
     quipu = quipu_leg(ctx, built) if (args.scene == "dem4097" and not args.no_quipu) else None
     adjoint = None if args.no_adjoint else adjoint_leg(ctx, max(3, args.steps), max(1, args.warmup))
@@ -605,6 +625,7 @@ def main() -> int:
         "batch": batch,
         "adjoint": adjoint,
         "instanced": instanced,
+        "gpu_visits": visits,
         "bvh_build": bvh_build_row(built.bvh, sc.triangles),
         "sensor_tail": sensor,
         "quipu": quipu,

@@ -141,6 +141,9 @@ typedef struct {
     int samples_per_batch;       /* 0 = 30 */
     float sampling_tolerance;    /* 0 = 0.05 */
     int samples_to_detect_miss;  /* 0 = 30 */
+    int count_visits;    /* per-pixel integrator: count BVH visits into the stats (a separate kernel;
+                            one light or none, no ambient light, background, adaptive sampling or
+                            material maps; the image is the same) */
 } vira_gpu_render_opts;
 
 /* Render passes (passes.hpp:51-95); every pointer optional, device memory,
@@ -160,6 +163,18 @@ typedef struct {
     uint32_t* material_id_dev; /* W*H first-hit material id (vira_gpu_material.material_id) */
 } vira_gpu_outputs;
 
+/* Traversal work of one kind of ray, summed over the render (opt-in visit
+ * counters; the reference's Vulkan path tracer keeps debug counters of the same
+ * kind, ray_generation.glsl:274-276). */
+typedef struct {
+    uint64_t nodes;         /* internal BVH nodes fetched (each tests all its child boxes) */
+    uint64_t leaves;        /* leaves entered */
+    uint64_t triangles;     /* ray-triangle tests */
+    uint64_t stack_spills;  /* pushes past the 16 stack entries kept in LDS (entry moved to private memory) */
+    uint64_t tlas_nodes;    /* TLAS nodes whose box was tested (instanced scenes) */
+    uint32_t max_stack;     /* deepest traversal stack of any ray */
+} vira_gpu_visit_counts;
+
 typedef struct {
     uint64_t primary_rays;
     uint64_t shadow_rays;
@@ -174,6 +189,9 @@ typedef struct {
     double fused_ms;         /* device time of the fused renderer launch(es) */
     uint32_t fused_launches;
     uint64_t closest_hits;   /* camera and bounce rays that hit (shaded surface points; per-pixel renderer) */
+    int visits_counted;      /* 1 when opts.count_visits filled the two fields below */
+    vira_gpu_visit_counts closest_visits;  /* camera and bounce rays (closest hit) */
+    vira_gpu_visit_counts shadow_visits;   /* shadow rays (any hit) */
 } vira_gpu_render_stats;
 
 /* Fixed-visibility direct illumination (the differentiable render).  New: the

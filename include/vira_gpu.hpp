@@ -149,6 +149,7 @@ struct PathTracerOptions {
     int samplesPerBatch = 30;             // samples_per_batch
     float samplingTolerance = 0.05f;      // sampling_tolerance
     int samplesToDetectMiss = 30;         // samples_to_detect_miss
+    bool countVisits = false;             // BVH visit counters into lastStats (per-pixel integrator)
 };
 
 // EATWTOptions (cpu_denoise.hpp:9-18).
@@ -180,6 +181,7 @@ public:
         o.samples_per_batch = options.samplesPerBatch;
         o.sampling_tolerance = options.samplingTolerance;
         o.samples_to_detect_miss = options.samplesToDetectMiss;
+        o.count_visits = options.countVisits ? 1 : 0;
         check(vira_gpu_render(ctx_->get(), &camera, &o, &renderPasses, &lastStats));
         if (options.denoise)
             check(vira_gpu_denoise(ctx_->get(), &denoiserOptions, camera.width, camera.height,

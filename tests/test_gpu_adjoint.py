@@ -4,10 +4,14 @@ the FP64 oracle model (oracle/oracle_diff.hpp), through the C-ABI.
 Bar:
 * forward image bit-exact with the path tracer at bounces = 0 and with the
   oracle's FP32 twin (same hit records);
-* dL/dheights and dL/dtexels within 1e-3 relative L2 error of the FP64 oracle
-  adjoint on the same records (FP32 arithmetic and float atomics on the GPU);
+* dL/dheights and dL/dtexels against the FP64 oracle adjoint on the same
+  records (FP32 arithmetic and float atomics in run-to-run order on the GPU),
+  relative L2 error: small scenes <= 1e-5 (heights) and 2e-5 (texels), measured
+  5e-7 and 2.6e-6; the 1025^2 scene <= 5e-5 and 2.5e-4, measured 1.0e-5 and
+  5.8e-5 (a texel sums the contributions of ~16 pixels' samples);
 * dL/dheights against central finite differences of the FP64 L2 loss
-  (h = 1e-3 m) on random nodes: |g - fd| <= 2e-3 max|fd| + 1e-2 |fd|;
+  (h = 1e-3 m) on random nodes: |g - fd| <= 2e-4 max|fd| + 1e-3 |fd| (measured
+  worst node 5e-5 max|fd| on the 1025^2 scene, 2e-6 on the small ones);
 * hit records kept by the forward pass and re-traced ones give the same
   gradients (up to the order of the float atomics).
 """
@@ -86,7 +90,7 @@ def test_adjoint_matches_fp64_oracle(gpu_ctx, smooth):
     et = _rel_l2(gt.cpu().numpy().reshape(-1).astype(np.float64), gt64)
     print(f"smooth={smooth}: rel L2 error heights {eh:.2e}, texels {et:.2e}")
     assert np.count_nonzero(gh64) > 100 and np.count_nonzero(gt64) > 100
-    assert eh <= 1e-3 and et <= 1e-3
+    assert eh <= 1e-5 and et <= 2e-5
 
 
 def _fd_check(ctx, b, opts, nodes_n, seed):
@@ -111,7 +115,7 @@ def _fd_check(ctx, b, opts, nodes_n, seed):
                        target.cpu().numpy().reshape(-1, 8).astype(np.float64), nodes, 1e-3)
     g = gh.cpu().numpy().reshape(-1)[nodes].astype(np.float64)
     err = np.abs(g - fd)
-    tol = 2e-3 * np.abs(fd).max() + 1e-2 * np.abs(fd)
+    tol = 2e-4 * np.abs(fd).max() + 1e-3 * np.abs(fd)
     print(f"FD check on {nodes_n} nodes: max |g - fd| = {err.max():.3e}, max |fd| = {np.abs(fd).max():.3e}, "
           f"rel L2 {_rel_l2(g, fd):.2e}, nonzero {np.count_nonzero(fd)}")
     assert np.count_nonzero(fd) > nodes_n // 2
@@ -241,5 +245,5 @@ def test_dem1025_full_size_adjoint(gpu_ctx):
     eh = _rel_l2(gh.cpu().numpy().reshape(-1).astype(np.float64), gh64)
     et = _rel_l2(gt.cpu().numpy().reshape(-1).astype(np.float64), gt64)
     print(f"1025^2 DI: rel L2 error heights {eh:.2e}, texels {et:.2e}")
-    assert eh <= 1e-3 and et <= 1e-3
+    assert eh <= 5e-5 and et <= 2.5e-4
     _fd_check(gpu_ctx, b, opts, 1000, 12)

@@ -68,6 +68,7 @@ def test_instanced_point_sun_bit_exact(gpu_ctx, width):
     print(f"BVH{width}: ids equal to FP64 oracle on {same:.4%} of pixels; radiance vs FP64 on shared hits: "
           f"median {np.median(rel[lit]):.2e}, 99th pct {np.percentile(rel[lit], 99):.2e}")
     assert same >= 0.99
+    assert np.median(rel[lit]) <= 1e-6 and np.percentile(rel[lit], 99) <= 5e-5
 
 
 def test_instanced_sphere_sun_bounces(gpu_ctx):

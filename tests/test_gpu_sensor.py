@@ -45,32 +45,37 @@ def test_received_power_and_solid_angles(gpu_ctx):
 
 
 def _z_scores(g, r):
+    """Per pixel-bin z-scores of the two means, and the z-score of the image-wide
+    sums (standard error from the per-seed image sums)."""
     n = g.shape[0]
     mg, mr = g.mean(0), r.mean(0)
     se = np.sqrt(g.var(0, ddof=1) / n + r.var(0, ddof=1) / n)
     lit = mr > 1e-6 * mr.max()
     m = lit & (se > 0)
     z = np.abs(mg - mr)[m] / se[m]
-    bias = (mg[lit].sum() - mr[lit].sum()) / mr[lit].sum()
-    return z, bias, lit, mg, mr, se
+    gs, rs = g.reshape(n, -1).sum(1), r.reshape(n, -1).sum(1)
+    z_img = (gs.mean() - rs.mean()) / np.sqrt(gs.var(ddof=1) / n + rs.var(ddof=1) / n)
+    return z, z_img, (gs.mean() - rs.mean()) / rs.mean()
 
 
 def test_monte_carlo_mean_within_3_sigma(gpu_ctx):
     """32 GPU renders (seeds 0..31, sphere sun at 1 AU, 2 bounces + RR) against
-    32 oracle renders with other seeds (1000..1031).
+    32 oracle renders with other seeds (1000..1031), in FP32 and in FP64: on
+    >= 99 % of lit pixel-bins the two means agree within 3 standard errors, the
+    mean squared z-score is about 1 (the estimator is unbiased and its variance
+    is as sampled), and the image-wide sums agree within 4 standard errors.
 
-    * Against the oracle's FP32 twin the two means must agree within 3 standard
-      errors on >= 99 % of lit pixel-bins, with mean squared z-score about 1:
-      the estimator is unbiased and its variance is as sampled.
-    * Against the FP64 oracle >= 97 % of pixel-bins lie within 3 standard
-      errors and the image-wide mean differs by <= 2e-3: FP32 geometry (the
-      reference's integer-ULP shadow-ray offset, about 256 ulps) moves shadow
-      terminators slightly, a systematic effect the FP64 oracle does not have.
+    The image is 40^2 over the 64-cell DEM so that pixel centres do not fall on
+    mesh edges: sample 0 of every pixel is the pixel centre, the same ray for
+    every seed, and where it crosses a shared edge exactly (a 32^2 image of this
+    DEM looking straight down), FP32 and FP64 rounding pick different flat-shaded
+    faces; that fixed difference never averages out (test_oracle_pins.py pins
+    it on the CPU).
     """
     import torch
-    sc = W.scaled(W.DEM4097, dem=65, image=32, spp=4, bounces=2, texture=64)
+    sc = W.scaled(W.DEM4097, dem=65, image=40, spp=4, bounces=2, texture=64)
     b = W.build(gpu_ctx, sc)
-    passes = S.RenderPasses(32, 32)
+    passes = S.RenderPasses(40, 40)
     gpu = []
     for seed in range(32):
         opts = S.PathTracerOptions(samples=4, bounces=2, russian_roulette=True, rr_start=1, seed=seed)
@@ -87,16 +92,13 @@ def test_monte_carlo_mean_within_3_sigma(gpu_ctx):
     for prec in (32, 64):
         ref = np.stack([om.render(cam, lights, mat, O.options(4, 2, True, 1, 1000 + s), precision=prec)["radiance"]
                         .astype(np.float64) for s in range(32)])
-        z, bias, lit, mg, mr, se = _z_scores(g, ref)
+        z, z_img, rel = _z_scores(g, ref)
         print(f"MC vs FP{prec} oracle: {z.size} pixel-bins, within 3 sigma {np.mean(z <= 3):.4f}, "
-              f"mean z^2 {np.mean(z * z):.3f}, image-wide relative difference {bias:+.2e}")
+              f"mean z^2 {np.mean(z * z):.3f}, image-wide relative difference {rel:+.2e} (z {z_img:+.2f})")
         assert z.size > 1000
-        if prec == 32:
-            assert np.mean(z <= 3.0) >= 0.99
-            assert 0.6 <= np.mean(z * z) <= 1.5
-        else:
-            assert np.mean(z <= 3.0) >= 0.97
-            assert abs(bias) <= 2e-3
+        assert np.mean(z <= 3.0) >= 0.99
+        assert 0.6 <= np.mean(z * z) <= 1.5
+        assert abs(z_img) <= 4.0
 
 
 # ---------------------------------------------------------------- sensor tail

@@ -308,3 +308,36 @@ def test_instanced_fp64_equals_pretransformed_fp64():
     lit = r2 > 1e-6 * r2.max()
     assert np.quantile(np.abs(r1 - r2)[lit] / r2[lit], 0.99) < 1e-6
     assert np.allclose(inst["depth"][both], flat["depth"][both], rtol=1e-7)
+
+
+def test_centre_rays_on_shared_edges_are_the_fp32_fp64_id_differences():
+    """Sample 0 of every pixel is the pixel centre, the same ray for every seed.
+    With a 32^2 image over the 64-cell sphere-draped DEM looking straight down,
+    centre rays cross shared triangle edges exactly, and FP32 and FP64 rounding
+    pick different sides: every differing first hit is the pair of triangles
+    sharing that edge, at the same depth. With a 40^2 image no centre ray sits on
+    an edge and the ids agree everywhere. This is why the Monte Carlo comparison
+    against FP64 uses an image that does not align with the mesh: a flat-shaded
+    face chosen by rounding is a fixed difference that seeds never average away."""
+    from vira_amd import workloads as W
+    for image, expect_ties in ((32, True), (40, False)):
+        sc = W.scaled(W.DEM4097, dem=65, image=image)
+        h = O.fbm_heights(sc.dem, sc.dem, sc.seed, sc.amplitude, sc.octaves, sc.base_cells)
+        p = W.make_projection(sc)
+        om = O.dem_mesh(h, O.make_projection(p.kind, p.spacing, p.radius, tuple(p.center), tuple(p.offset)), None)
+        cam = O.camera_from(W.make_camera(sc).to_c())
+        lights = [O.light_from(L.to_c()) for L in W.make_lights(sc)]
+        o = {prec: om.render(cam, lights, O.material(0.11), O.options(1, 0, False, 1, 0), precision=prec)
+             for prec in (32, 64)}
+        a, b = o[32]["tri_id"], o[64]["tri_id"]
+        assert np.all(a != 0xFFFFFFFF) and np.all(b != 0xFFFFFFFF)
+        diff = np.nonzero(a != b)[0]
+        if not expect_ties:
+            assert diff.size == 0
+            continue
+        assert 0.01 < diff.size / a.size < 0.2
+        idx = O.dem_indices(h).reshape(-1, 3)
+        for i in diff:
+            assert len(set(idx[a[i]]) & set(idx[b[i]])) == 2, i  # neighbours across one edge
+        d32, d64 = o[32]["depth"][diff].astype(np.float64), o[64]["depth"][diff].astype(np.float64)
+        assert np.max(np.abs(d32 - d64) / d64) < 1e-6

@@ -114,6 +114,7 @@ class RenderOpts(C.Structure):
         ("samples_per_batch", C.c_int),
         ("sampling_tolerance", C.c_float),
         ("samples_to_detect_miss", C.c_int),
+        ("count_visits", C.c_int),
     ]
 
 
@@ -134,6 +135,17 @@ class Outputs(C.Structure):
     ]
 
 
+class VisitCounts(C.Structure):
+    _fields_ = [
+        ("nodes", C.c_uint64),
+        ("leaves", C.c_uint64),
+        ("triangles", C.c_uint64),
+        ("stack_spills", C.c_uint64),
+        ("tlas_nodes", C.c_uint64),
+        ("max_stack", C.c_uint32),
+    ]
+
+
 class RenderStats(C.Structure):
     _fields_ = [
         ("primary_rays", C.c_uint64),
@@ -149,7 +161,19 @@ class RenderStats(C.Structure):
         ("fused_ms", C.c_double),
         ("fused_launches", C.c_uint32),
         ("closest_hits", C.c_uint64),
+        ("visits_counted", C.c_int),
+        ("closest_visits", VisitCounts),
+        ("shadow_visits", VisitCounts),
     ]
+
+
+def struct_dict(st) -> dict:
+    """Fields of a ctypes structure as a dict (nested structures as dicts)."""
+    out = {}
+    for f, _ in st._fields_:
+        v = getattr(st, f)
+        out[f] = struct_dict(v) if isinstance(v, C.Structure) else v
+    return out
 
 
 class DIOpts(C.Structure):

@@ -142,6 +142,8 @@ void batch_submit(Ctx& c, const vira_gpu_request* reqs, size_t n, const vira_gpu
                  VIRA_GPU_ERR_INVALID, "batch: request " + std::to_string(i) + ": bad image size");
         VG_CHECK(r.opts.samples > 0 && r.opts.bounces >= 0 && r.opts.bounces < 64, VIRA_GPU_ERR_INVALID,
                  "batch: request " + std::to_string(i) + ": bad samples/bounces");
+        VG_CHECK(!r.opts.count_visits, VIRA_GPU_ERR_UNSUPPORTED,
+                 "batch: request " + std::to_string(i) + ": visit counting runs in vira_gpu_render only");
         VG_CHECK(r.num_lights >= 0 && r.num_lights <= VIRA_GPU_MAX_LIGHTS, VIRA_GPU_ERR_INVALID,
                  "batch: request " + std::to_string(i) + ": 0.." + std::to_string(VIRA_GPU_MAX_LIGHTS) + " lights");
         for (int l = 0; l < r.num_lights; ++l) {

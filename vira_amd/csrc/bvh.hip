@@ -5,9 +5,10 @@
 // (ViraBLAS, vira_blas.ipp:132-254) for a GPU:
 //   * large nodes (> kSmall triangles) are split level by level: each node is
 //     cut into 4096-slot chunks, one workgroup bins a chunk in LDS and merges
-//     it into the node's global bins; the host sweeps the bins (SAH), and a
-//     stable chunked partition (block scan) moves the triangle ids.  Every
-//     step is deterministic.
+//     it into the node's global bins; one block per node sweeps the bins on
+//     the GPU (SAH) and the host reads back only each node's decision (104 B)
+//     to lay out the next level; a stable chunked partition (block scan)
+//     moves the triangle ids.  Every step is deterministic.
 //   * small nodes are finished by one thread each (binned SAH, in-place
 //     partition), the per-thread "subdivide" of the reference.
 //   * the binary tree is then rewritten into 64-byte nodes that hold both

@@ -229,6 +229,28 @@ __device__ __forceinline__ void wave_add(unsigned long long* counter, uint32_t v
     if (__lane_id() == (unsigned)(__ffsll((long long)active) - 1)) atomicAdd(counter, (unsigned long long)sum);
 }
 
+// The same with 64-bit sums (counts that can pass 2^32 within one wave).
+__device__ __forceinline__ void wave_add64(unsigned long long* counter, unsigned long long v) {
+    unsigned long long active = __ballot(1);
+    unsigned long long sum = v;
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(sum, off, 64);
+        if (active & (1ull << (__lane_id() ^ off))) sum += o;
+    }
+    if (__lane_id() == (unsigned)(__ffsll((long long)active) - 1)) atomicAdd(counter, sum);
+}
+
+// Maximum over the wave's active lanes, one atomic per wave.
+__device__ __forceinline__ void wave_max(unsigned long long* counter, uint32_t v) {
+    unsigned long long active = __ballot(1);
+    uint32_t m = v;
+    for (int off = 32; off > 0; off >>= 1) {
+        uint32_t o = __shfl_xor(m, off, 64);
+        if ((active & (1ull << (__lane_id() ^ off))) && o > m) m = o;
+    }
+    if (__lane_id() == (unsigned)(__ffsll((long long)active) - 1)) atomicMax(counter, (unsigned long long)m);
+}
+
 __device__ __forceinline__ f3 vpos(const GVertex* V, uint32_t i) { return f3{V[i].px, V[i].py, V[i].pz}; }
 
 // Unit face normal normalize(cross(q1 - q0, q2 - q0)) (triangle.ipp:36-44,

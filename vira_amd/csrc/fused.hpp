@@ -262,11 +262,14 @@ __device__ __forceinline__ bool end_sample(float* __restrict__ a, uint32_t lane,
 // sample is drawn again when its shadow ray returns unoccluded.  Both give the
 // numbers of the reference's loop (cpu_path_tracer.ipp:384-428): radiance =
 // 0 (+ ambient at bounce 0) + the lit lights' terms in light order.
-template <int W, bool INST = false, bool MULTI = false, bool FEAT = false>
+//
+// STATS = true adds each traversal's visits to tcount[0] (camera and bounce
+// rays) or tcount[1] (shadow rays); maxsp keeps the deepest stack.
+template <int W, bool INST = false, bool MULTI = false, bool FEAT = false, bool STATS = false>
 __device__ __forceinline__ void render_tile(const TileJob& J, uint32_t tile, float* __restrict__ a,
                                             uint32_t* s_ref, float* s_dist, float* s_lds, uint32_t lane,
                                             uint32_t& n_prim, uint32_t& n_shadow, uint32_t& n_bounce,
-                                            uint32_t& n_hit) {
+                                            uint32_t& n_hit, TravCount* tcount = nullptr) {
     const void* __restrict__ nodes = J.nodes;
     const float4* __restrict__ tris = J.tris;
     const GVertex* __restrict__ verts = J.verts;
@@ -338,11 +341,25 @@ __device__ __forceinline__ void render_tile(const TileJob& J, uint32_t tile, flo
         h.b1 = h.b2 = 0.f;
         uint32_t hinst = 0;
         if (mode != M_DONE) {
+            TravCount tc{};
             if (INST)
-                h = trace_instances<W>(J.inst, J.tlas, J.ntlas, o, d, tmax, mode == M_SHADOW, s_ref + lane,
-                                       s_dist + lane, kFusedBS, hinst);
+                h = trace_instances<W, STATS>(J.inst, J.tlas, J.ntlas, o, d, tmax, mode == M_SHADOW, s_ref + lane,
+                                              s_dist + lane, kFusedBS, hinst, &tc);
             else
-                h = traverse<W>(nodes, tris, o, d, tmax, mode == M_SHADOW, s_ref + lane, s_dist + lane, kFusedBS);
+                h = traverse<W, STATS>(nodes, tris, o, d, tmax, mode == M_SHADOW, s_ref + lane, s_dist + lane,
+                                       kFusedBS, &tc);
+            if (STATS) {
+                auto add = [&](TravCount& to) {
+                    to.nodes += tc.nodes;
+                    to.leaves += tc.leaves;
+                    to.tris += tc.tris;
+                    to.spills += tc.spills;
+                    to.tlas += tc.tlas;
+                    to.maxsp = to.maxsp > tc.maxsp ? to.maxsp : tc.maxsp;
+                };
+                if (mode == M_SHADOW) add(tcount[1]);
+                else add(tcount[0]);
+            }
         }
         bool vertex_done = false, lit = false;
         if (mode == M_CLOSEST) {

@@ -119,10 +119,11 @@ __device__ __forceinline__ void rebase_instance(const double* M, double cx, doub
 // and its BLAS is traversed with the best t so far.  Subtrees whose box the
 // ray misses, or enters beyond the best hit, are skipped.  Ties in t go to the
 // lower instance, so the hit does not depend on the visiting order.
-template <int W>
+template <int W, bool STATS = false>
 __device__ __forceinline__ HitRec trace_instances(const InstG* __restrict__ inst, const TNode* __restrict__ tn,
                                                   int nn, f3 o, f3 d, float tmax, bool any, uint32_t* lds_ref,
-                                                  float* lds_dist, int stride, uint32_t& hit_inst) {
+                                                  float* lds_dist, int stride, uint32_t& hit_inst,
+                                                  TravCount* tc = nullptr) {
     HitRec best;
     best.t = __builtin_huge_valf();
     best.id = 0xffffffffu;
@@ -134,6 +135,7 @@ __device__ __forceinline__ HitRec trace_instances(const InstG* __restrict__ inst
     int i = 0;
     while (i < nn) {
         const TNode N = tn[i];
+        if (STATS) tc->tlas++;
         if (!(slab<true>(N.lo[0], N.lo[1], N.lo[2], N.hi[0], N.hi[1], N.hi[2], o, inv, e, bt) <
               __builtin_huge_valf())) {
             i = N.skip;
@@ -145,7 +147,7 @@ __device__ __forceinline__ HitRec trace_instances(const InstG* __restrict__ inst
         }
         const InstG& I = inst[N.inst];
         const f3 ol = xform_point(I.w2l, o), dl = xform_dir(I.w2l, d);
-        const HitRec h = traverse<W>(I.nodes, I.tris, ol, dl, bt, any, lds_ref, lds_dist, stride);
+        const HitRec h = traverse<W, STATS>(I.nodes, I.tris, ol, dl, bt, any, lds_ref, lds_dist, stride, tc);
         if (h.id != 0xffffffffu &&
             (best.id == 0xffffffffu || h.t < best.t || (h.t == best.t && (uint32_t)N.inst < hit_inst))) {
             best = h;

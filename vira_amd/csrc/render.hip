@@ -4,14 +4,16 @@
 // (cpu_path_tracer.ipp:29-511) on the GPU, in two schedules that share every
 // formula (shading.hpp) and produce the same numbers:
 //
-// * fused (scenes with at most one light, the default): one wave per 8x8
-//   pixel tile, one lane per pixel.  Each lane walks its pixel's samples and
-//   bounces as a small state machine; every loop iteration traces exactly one
-//   ray per lane (camera, shadow or bounce ray, with a per-lane any-hit flag),
-//   so lanes whose path ended regenerate the next sample instead of idling.
-//   Path throughput, path radiance and the Kahan sums stay in registers;
-//   only the first-hit passes go through memory.
-// * wavefront (any number of lights): queues of rays, closest-hit and any-hit
+// * fused (the default; any number of lights): one wave per 8x8 pixel tile,
+//   one lane per pixel.  Each lane walks its pixel's samples and bounces as a
+//   small state machine; every loop iteration traces exactly one ray per lane
+//   (camera, shadow or bounce ray, with a per-lane any-hit flag), so lanes
+//   whose path ended regenerate the next sample instead of idling.  Path
+//   throughput and path radiance stay in registers; the Kahan sums and the
+//   first-hit passes live in the tile-blocked accumulator block in global
+//   memory, read and written once per sample.
+// * wavefront (point and sphere lights, no adaptive sampling, ambient light or
+//   shown background): queues of rays, closest-hit and any-hit
 //   traversal kernels, shading into per-hit shadow slots, bounce rays
 //   compacted with a wave ballot and one atomic per wave.
 //
@@ -84,6 +86,40 @@ __global__ void __launch_bounds__(kFusedBS, kFusedWavesPerSimd) k_render_fused(T
     wave_add(&counters[1], n_shadow);
     wave_add(&counters[2], n_bounce);
     wave_add(&counters[3], n_hit);
+}
+
+// Counter words of the visit-counting launch: after the four ray counts, six
+// per ray kind (camera and bounce rays, then shadow rays): nodes, leaves,
+// triangles, spilled pushes, deepest stack, TLAS nodes.
+enum : int { kVisitBase = 4, kVisitWords = 6 };
+
+// The same render with the traversals' visit counts (opt-in; one light or
+// none, no background, adaptive sampling or material maps).
+template <int W, bool INST>
+__global__ void __launch_bounds__(kFusedBS, kFusedWavesPerSimd) k_render_fused_stats(TileJob J, float* __restrict__ acc,
+                                                         unsigned long long* __restrict__ counters) {
+    __shared__ uint32_t s_ref[kStackLds * kFusedBS];
+    __shared__ float s_dist[kStackLds * kFusedBS];
+    __shared__ float s_lds[fused_lds_floats<false>() * kFusedBS];
+    const uint32_t lane = threadIdx.x;
+    uint32_t n_prim = 0, n_shadow = 0, n_bounce = 0, n_hit = 0;
+    TravCount tc[2] = {};
+    render_tile<W, INST, false, false, true>(J, blockIdx.x, acc + (size_t)blockIdx.x * (A_FIELDS * 64), s_ref, s_dist,
+                                             s_lds, lane, n_prim, n_shadow, n_bounce, n_hit, tc);
+    wave_add(&counters[0], n_prim);
+    wave_add(&counters[1], n_shadow);
+    wave_add(&counters[2], n_bounce);
+    wave_add(&counters[3], n_hit);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        unsigned long long* c = counters + kVisitBase + k * kVisitWords;
+        wave_add64(&c[0], tc[k].nodes);
+        wave_add64(&c[1], tc[k].leaves);
+        wave_add64(&c[2], tc[k].tris);
+        wave_add64(&c[3], tc[k].spills);
+        wave_max(&c[4], tc[k].maxsp);
+        wave_add64(&c[5], tc[k].tlas);
+    }
 }
 
 // Camera-relative FP32 placement of every instance and light, computed in
@@ -499,11 +535,19 @@ void run_fused(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o
         {{VG_FUSED_PAIR(4, false, false), VG_FUSED_PAIR(4, false, true)},
          {VG_FUSED_PAIR(4, true, false), VG_FUSED_PAIR(4, true, true)}}};
 #undef VG_FUSED_PAIR
-    const K kern = table[w4 ? 1 : 0][inst ? 1 : 0][multi ? 1 : 0][feat ? 1 : 0];
+    K kern = table[w4 ? 1 : 0][inst ? 1 : 0][multi ? 1 : 0][feat ? 1 : 0];
+    if (o.count_visits) {
+        VG_CHECK(!multi && !feat, VIRA_GPU_ERR_UNSUPPORTED,
+                 "render: visit counting takes one light or none, no ambient light, background, adaptive "
+                 "sampling or material maps");
+        const K stats_table[2][2] = {{k_render_fused_stats<2, false>, k_render_fused_stats<2, true>},
+                                     {k_render_fused_stats<4, false>, k_render_fused_stats<4, true>}};
+        kern = stats_table[w4 ? 1 : 0][inst ? 1 : 0];
+    }
     kern<<<tiles, kFusedBS, 0, c.stream>>>(J, S.acc, S.counters);
     VG_HIP(hipGetLastError());
     VG_HIP(hipEventRecord(ev1, c.stream));
-    unsigned long long hc[4];
+    unsigned long long hc[kVisitBase + 2 * kVisitWords];
     VG_HIP(hipMemcpyAsync(hc, S.counters, sizeof(hc), hipMemcpyDeviceToHost, c.stream));
     VG_HIP(hipStreamSynchronize(c.stream));
     if (st) {
@@ -512,6 +556,19 @@ void run_fused(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o
         st->shadow_rays = hc[1];
         st->bounce_rays = hc[2];
         st->closest_hits = hc[3];
+        if (o.count_visits) {
+            st->visits_counted = 1;
+            vira_gpu_visit_counts* vc[2] = {&st->closest_visits, &st->shadow_visits};
+            for (int k = 0; k < 2; ++k) {
+                const unsigned long long* h = hc + kVisitBase + k * kVisitWords;
+                vc[k]->nodes = h[0];
+                vc[k]->leaves = h[1];
+                vc[k]->triangles = h[2];
+                vc[k]->stack_spills = h[3];
+                vc[k]->max_stack = (uint32_t)h[4];
+                vc[k]->tlas_nodes = h[5];
+            }
+        }
         float f = 0.f;
         VG_HIP(hipEventElapsedTime(&f, ev0, ev1));
         st->total_ms = f;
@@ -547,6 +604,8 @@ void render(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o, c
     const uint32_t nq = tiles * 64u;
     const bool fused = o.integrator != 2;
     VG_CHECK(!inst || fused, VIRA_GPU_ERR_UNSUPPORTED, "render: instanced scenes use the per-pixel integrator");
+    VG_CHECK(fused || !o.count_visits, VIRA_GPU_ERR_UNSUPPORTED,
+             "render: visit counting runs in the per-pixel integrator only");
     if (!fused) {
         VG_CHECK(!o.adaptive_sampling, VIRA_GPU_ERR_UNSUPPORTED,
                  "render: adaptive sampling runs in the per-pixel integrator only");

@@ -8,6 +8,12 @@
 // pushes and pops hit 64 consecutive dwords); deeper entries spill to private
 // memory.  Capacity: a binary tree of depth < 63 (the builder's bound) needs
 // at most 62 entries; its four-wide collapse at most 3 per level, 186.
+//
+// STATS = true counts what a traversal fetches (opt-in visit counters; the
+// reference's analogue is the Vulkan path tracer's debug counters,
+// ray_generation.glsl:274-276): internal nodes fetched, leaves entered,
+// triangles tested, stack entries moved out of LDS into private memory, and
+// the deepest stack.
 #pragma once
 
 #include "device_math.hpp"
@@ -21,6 +27,16 @@ struct HitRec {
     float t;
     uint32_t id;
     float b1, b2;
+};
+
+// Per-lane visit counts of the rays one lane traced (STATS traversals).
+struct TravCount {
+    uint32_t nodes;   // internal nodes fetched (BLAS)
+    uint32_t leaves;  // leaves entered
+    uint32_t tris;    // triangles tested
+    uint32_t spills;  // pushes that moved an entry from the LDS ring to private memory
+    uint32_t maxsp;   // deepest stack
+    uint32_t tlas;    // TLAS nodes whose box was tested (instanced scenes)
 };
 
 struct TravStack {
@@ -66,10 +82,10 @@ __device__ __forceinline__ void cswap(float& da, uint32_t& ra, float& db, uint32
 
 // any: occlusion query within [0, tmax] (returns on the first hit);
 // otherwise closest hit within [0, tmax], ties on t resolved to the lowest id.
-template <int W, bool SAFE>
+template <int W, bool SAFE, bool STATS>
 __device__ __forceinline__ HitRec traverse_impl(const void* __restrict__ nodes_v, const float4* __restrict__ tris,
                                                 f3 o, f3 d, float tmax, bool any, uint32_t* lds_ref, float* lds_dist,
-                                                int stride) {
+                                                int stride, TravCount* tc) {
     constexpr float kInfF = __builtin_huge_valf();
     HitRec h;
     h.t = kInfF;
@@ -85,10 +101,16 @@ __device__ __forceinline__ HitRec traverse_impl(const void* __restrict__ nodes_v
     st.lds_dist = lds_dist;
     st.stride = stride;
     st.sp = 0;
+    auto push = [&](uint32_t r, float dist) {
+        if (STATS && st.sp >= kStackLds) tc->spills++;
+        st.push(r, dist);
+        if (STATS && (uint32_t)st.sp > tc->maxsp) tc->maxsp = (uint32_t)st.sp;
+    };
     uint32_t cur = 0;  // root node
     while (true) {
         // ---- descend through internal nodes until a leaf (or nothing) is reached
         while (!ref_is_leaf(cur)) {
+            if (STATS) tc->nodes++;
             if (W == 2) {
                 const Node2 nd = reinterpret_cast<const Node2*>(nodes_v)[cur];
                 float d0 = nd.ref.x != kEmptyRef
@@ -106,7 +128,7 @@ __device__ __forceinline__ HitRec traverse_impl(const void* __restrict__ nodes_v
                         farr = nd.ref.x;
                         fard = d0;
                     }
-                    st.push(farr, fard);
+                    push(farr, fard);
                     cur = nearr;
                 } else if (h0) {
                     cur = nd.ref.x;
@@ -141,16 +163,18 @@ __device__ __forceinline__ HitRec traverse_impl(const void* __restrict__ nodes_v
                     cur = kEmptyRef;
                     break;
                 }
-                if (d3 < kInfF) st.push(r3, d3);
-                if (d2 < kInfF) st.push(r2, d2);
-                if (d1 < kInfF) st.push(r1, d1);
+                if (d3 < kInfF) push(r3, d3);
+                if (d2 < kInfF) push(r2, d2);
+                if (d1 < kInfF) push(r1, d1);
                 cur = r0;
             }
         }
         // ---- leaf
         if (cur != kEmptyRef) {
             uint32_t first = leaf_first(cur), cnt = leaf_count(cur);
+            if (STATS) tc->leaves++;
             for (uint32_t i = 0; i < cnt; ++i) {
+                if (STATS) tc->tris++;
                 const float4 a = tris[3 * (first + i)];
                 const float4 b = tris[3 * (first + i) + 1];
                 const float4 c = tris[3 * (first + i) + 2];
@@ -193,12 +217,14 @@ __device__ __forceinline__ HitRec traverse_impl(const void* __restrict__ nodes_v
 }
 
 // Rays with a (near-)zero direction component take the SAFE slab test.
-template <int W>
+// STATS: add this ray's visits to *tc.
+template <int W, bool STATS = false>
 __device__ __forceinline__ HitRec traverse(const void* __restrict__ nodes, const float4* __restrict__ tris, f3 o, f3 d,
-                                           float tmax, bool any, uint32_t* lds_ref, float* lds_dist, int stride) {
+                                           float tmax, bool any, uint32_t* lds_ref, float* lds_dist, int stride,
+                                           TravCount* tc = nullptr) {
     if (fabsf(d.x) < kTinyDir || fabsf(d.y) < kTinyDir || fabsf(d.z) < kTinyDir)
-        return traverse_impl<W, true>(nodes, tris, o, d, tmax, any, lds_ref, lds_dist, stride);
-    return traverse_impl<W, false>(nodes, tris, o, d, tmax, any, lds_ref, lds_dist, stride);
+        return traverse_impl<W, true, STATS>(nodes, tris, o, d, tmax, any, lds_ref, lds_dist, stride, tc);
+    return traverse_impl<W, false, STATS>(nodes, tris, o, d, tmax, any, lds_ref, lds_dist, stride, tc);
 }
 
 }  // namespace vg

@@ -236,6 +236,7 @@ class PathTracerOptions:
     samples_per_batch: int = 30
     sampling_tolerance: float = 0.05
     samples_to_detect_miss: int = 30
+    count_visits: bool = False  # BVH visit counters into the stats (per-pixel integrator)
 
     def to_c(self) -> capi.RenderOpts:
         o = capi.RenderOpts()
@@ -245,6 +246,7 @@ class PathTracerOptions:
         o.show_background, o.adaptive_sampling = int(self.show_background), int(self.adaptive_sampling)
         o.samples_per_batch, o.sampling_tolerance = int(self.samples_per_batch), float(self.sampling_tolerance)
         o.samples_to_detect_miss = int(self.samples_to_detect_miss)
+        o.count_visits = int(self.count_visits)
         return o
 
 
@@ -504,7 +506,7 @@ class Context:
                                                         passes.albedo, passes.depth, passes.normal_global)):
             raise ValueError("denoise needs the direct, indirect, albedo, depth and normal passes")
         self._call(self.lib.vira_gpu_render, self.h, C.byref(cam), C.byref(opt), C.byref(out), C.byref(st))
-        self.last_stats = {f: getattr(st, f) for f, _ in capi.RenderStats._fields_}
+        self.last_stats = capi.struct_dict(st)
         if options.denoise:  # cpu_path_tracer.ipp:124-134
             self.denoise(passes, options.denoiser)
         return self.last_stats
@@ -518,7 +520,7 @@ class Context:
         opt = options.to_c()
         out = passes.to_c()
         self._call(self.lib.vira_gpu_render_noalloc, self.h, C.byref(cam), C.byref(opt), C.byref(out), C.byref(st))
-        self.last_stats = {f: getattr(st, f) for f, _ in capi.RenderStats._fields_}
+        self.last_stats = capi.struct_dict(st)
         return self.last_stats
 
     def denoise(self, passes: RenderPasses, options: "DenoiseOptions" = None):
