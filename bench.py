@@ -178,7 +178,7 @@ def cpu_baseline_rows(built, passes) -> dict:
 def visits_leg(ctx, built, passes) -> dict:
     """One untimed render of the same frame by the visit-counting kernel: BVH
     nodes, leaves and triangles fetched per ray on the GPU's own tree, pushes
-    past the 16 LDS stack entries, and the deepest stack."""
+    past the 8 LDS stack entries, and the deepest stack."""
     from dataclasses import replace
     st = ctx.render(built.camera, replace(built.options, count_visits=True), passes)
     n_closest = st["primary_rays"] + st["bounce_rays"]

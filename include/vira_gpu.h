@@ -170,7 +170,7 @@ typedef struct {
     uint64_t nodes;         /* internal BVH nodes fetched (each tests all its child boxes) */
     uint64_t leaves;        /* leaves entered */
     uint64_t triangles;     /* ray-triangle tests */
-    uint64_t stack_spills;  /* pushes past the 16 stack entries kept in LDS (entry moved to private memory) */
+    uint64_t stack_spills;  /* pushes past the 8 stack entries kept in LDS (entry moved to private memory) */
     uint64_t tlas_nodes;    /* TLAS nodes whose box was tested (instanced scenes) */
     uint32_t max_stack;     /* deepest traversal stack of any ray */
 } vira_gpu_visit_counts;

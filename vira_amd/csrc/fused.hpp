@@ -7,8 +7,11 @@
 // bounces as a small state machine; every loop iteration traces exactly one
 // ray per lane (camera, shadow or bounce ray, with a per-lane any-hit flag),
 // so lanes whose path ended regenerate the next sample instead of idling.
-// Path throughput, path radiance and the Kahan sums' inputs stay in
-// registers; the per-pixel sums live in a tile-blocked accumulator block.
+// Path throughput and path radiance stay in registers, the radiance Kahan
+// sums in LDS; the other per-pixel sums live in a tile-blocked accumulator
+// block in global memory, the first-hit sums updated by atomic adds that
+// return nothing, so a lane finishing a sample never makes its wave wait on
+// global memory except for the direct pass's Kahan pair.
 #pragma once
 
 #include "instance.hpp"
@@ -26,7 +29,13 @@ constexpr int kFusedWavesPerSimd = 4;
 // Per-pixel accumulator fields.  Layout is tile-blocked SoA: field f of
 // pixel q lives at acc[((q >> 6) * A_FIELDS + f) * 64 + (q & 63)], so a wave
 // owning a tile reads or writes each field as 256 contiguous bytes from one
-// uniform base address.
+// uniform base address.  The per-pixel renderer keeps the radiance pair
+// (A_RAD_S, A_RAD_C) in LDS while it renders and stores it here at the end of
+// the tile; A_ALB..A_COUNT are only ever changed by relaxed device-scope
+// atomics (cleared by atomic stores, summed by atomic adds, read by atomic
+// loads).  One lane adds all of a pixel's samples, in sample order, and
+// accesses of one work-item to one address are coherence-ordered, so each sum
+// is the sequential FP32 sum.
 enum : int {
     A_RAD_S = 0,
     A_RAD_C = 8,
@@ -61,20 +70,39 @@ __device__ __forceinline__ const float& acc_ref(const float* base, int f, uint32
 }
 #define ACC(base, f, lane) acc_ref((base), (f), (lane))
 
-// First-hit passes at bounce 0 (cpu_path_tracer.ipp:352-381).
+// The atomically summed fields A_ALB..A_COUNT.
+__device__ __forceinline__ void acc_sum_add(float* base, int f, uint32_t lane, float v) {
+    __hip_atomic_fetch_add(&acc_ref(base, f, lane), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float acc_sum_get(const float* base, int f, uint32_t lane) {
+    return __hip_atomic_load(&acc_ref(base, f, lane), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void acc_sum_set(float* base, int f, uint32_t lane, float v) {
+    __hip_atomic_store(&acc_ref(base, f, lane), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// First-hit sums at bounce 0 (cpu_path_tracer.ipp:352-381): albedo, normal,
+// depth and the hit count.
+__device__ __forceinline__ void accumulate_first_hit_sums(float* __restrict__ a, uint32_t lane, float t,
+                                                          const Surface& S) {
+#pragma unroll
+    for (int k = 0; k < NB; ++k) acc_sum_add(a, A_ALB + k, lane, S.alb[k]);
+    acc_sum_add(a, A_NRM + 0, lane, S.N.x);
+    acc_sum_add(a, A_NRM + 1, lane, S.N.y);
+    acc_sum_add(a, A_NRM + 2, lane, S.N.z);
+    acc_sum_add(a, A_DEPTH, lane, t);
+    acc_sum_add(a, A_COUNT, lane, 1.0f);
+}
+
+// The same plus the first hit's triangle and instance, found by reading the
+// triangle field (the wavefront schedule, whose samples are separate launches).
 __device__ __forceinline__ void accumulate_first_hit(float* __restrict__ a, uint32_t lane, uint32_t id,
                                                      uint32_t inst, float t, const Surface& S) {
     if (__float_as_uint(ACC(a, A_TRI, lane)) == 0xffffffffu) {
         ACC(a, A_TRI, lane) = __uint_as_float(id);
         ACC(a, A_INST, lane) = __uint_as_float(inst);
     }
-#pragma unroll
-    for (int k = 0; k < NB; ++k) ACC(a, A_ALB + k, lane) = ACC(a, A_ALB + k, lane) + S.alb[k];
-    ACC(a, A_NRM + 0, lane) = ACC(a, A_NRM + 0, lane) + S.N.x;
-    ACC(a, A_NRM + 1, lane) = ACC(a, A_NRM + 1, lane) + S.N.y;
-    ACC(a, A_NRM + 2, lane) = ACC(a, A_NRM + 2, lane) + S.N.z;
-    ACC(a, A_DEPTH, lane) = ACC(a, A_DEPTH, lane) + t;
-    ACC(a, A_COUNT, lane) = ACC(a, A_COUNT, lane) + 1.0f;
+    accumulate_first_hit_sums(a, lane, t, S);
 }
 
 struct OutPtrs {
@@ -106,11 +134,11 @@ __device__ __forceinline__ void write_pixel(const OutPtrs& o, size_t p, int ns, 
         if (o.radiance) o.radiance[NB * p + k] = rad;
         if (o.direct) o.direct[NB * p + k] = dr;
         if (o.indirect) o.indirect[NB * p + k] = rad - dr;
-        if (o.albedo) o.albedo[NB * p + k] = __fdiv_rn(ACC(a, A_ALB + k, lane), sf);
+        if (o.albedo) o.albedo[NB * p + k] = __fdiv_rn(acc_sum_get(a, A_ALB + k, lane), sf);
     }
-    float cnt = ACC(a, A_COUNT, lane);
+    float cnt = acc_sum_get(a, A_COUNT, lane);
     if (o.normal || o.normal_camera) {
-        f3 n = mk3(ACC(a, A_NRM, lane), ACC(a, A_NRM + 1, lane), ACC(a, A_NRM + 2, lane));
+        f3 n = mk3(acc_sum_get(a, A_NRM, lane), acc_sum_get(a, A_NRM + 1, lane), acc_sum_get(a, A_NRM + 2, lane));
         if (cnt > 0.0f) n = normalize(n);
         if (o.normal) {
             o.normal[3 * p] = n.x;
@@ -128,7 +156,7 @@ __device__ __forceinline__ void write_pixel(const OutPtrs& o, size_t p, int ns, 
             o.normal_camera[3 * p + 2] = c.z;
         }
     }
-    if (o.depth) o.depth[p] = cnt > 0.0f ? __fdiv_rn(ACC(a, A_DEPTH, lane), cnt) : kInf;
+    if (o.depth) o.depth[p] = cnt > 0.0f ? __fdiv_rn(acc_sum_get(a, A_DEPTH, lane), cnt) : kInf;
     if (o.alpha) o.alpha[p] = __fdiv_rn(cnt, sf);
     const uint32_t first = __float_as_uint(ACC(a, A_TRI, lane));
     const uint32_t finst = __float_as_uint(ACC(a, A_INST, lane));
@@ -160,7 +188,22 @@ __device__ __forceinline__ void kahan_acc(float* __restrict__ a, uint32_t lane, 
 }
 
 __device__ __forceinline__ void acc_clear(float* __restrict__ a, uint32_t lane) {
-    for (int f = 0; f < A_FIELDS; ++f) ACC(a, f, lane) = f == A_TRI ? __uint_as_float(0xffffffffu) : 0.0f;
+    for (int f = 0; f < A_FIELDS; ++f) {
+        if (f >= A_ALB && f <= A_COUNT) acc_sum_set(a, f, lane, 0.0f);
+        else ACC(a, f, lane) = f == A_TRI ? __uint_as_float(0xffffffffu) : 0.0f;
+    }
+}
+
+// Kahan-compensated add of v[] into a lane's pair of NB sums and NB
+// compensations kept in LDS (column r, stride kFusedBS).
+__device__ __forceinline__ void kahan_lds(float* r, const float* v) {
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        float sv = r[k * kFusedBS], cv = r[(NB + k) * kFusedBS];
+        kahan_add(sv, cv, v[k]);
+        r[k * kFusedBS] = sv;
+        r[(NB + k) * kFusedBS] = cv;
+    }
 }
 
 enum : int { M_NEW = 0, M_CLOSEST = 1, M_SHADOW = 2, M_DONE = 3 };
@@ -207,18 +250,23 @@ struct TileJob {
 // Per-lane LDS of the several-light renderer: the vertex's throughput before
 // the bounce update, shading normal, albedo and its radiance so far.
 enum : int { ML_BETA = 0, ML_N = 8, ML_ALB = 11, ML_VRAD = 19, ML_FIELDS = 27 };
-// Per-lane LDS floats of the per-pixel renderers besides the stack.
+// Per-lane LDS floats of the per-pixel renderers besides the stack: the staged
+// light contribution (one light) or the vertex (several lights), then the
+// radiance Kahan pair (2 x NB) from fused_rad_lds on.
 template <bool MULTI>
-constexpr int fused_lds_floats() { return MULTI ? ML_FIELDS : NB; }
+constexpr int fused_rad_lds() { return MULTI ? ML_FIELDS : NB; }
+template <bool MULTI>
+constexpr int fused_lds_floats() { return fused_rad_lds<MULTI>() + 2 * NB; }
 
 // End of sample s of a lane's pixel: fold the path radiance into the Kahan
-// sums and, with adaptive sampling, apply the reference's stopping rules
-// (cpu_path_tracer.ipp:198-227).  Returns true when the pixel is finished
-// (s is then the divisor, as dataPayload.sample is in the reference).
+// sums (rk: the lane's LDS pair) and, with adaptive sampling, apply the
+// reference's stopping rules (cpu_path_tracer.ipp:198-227).  Returns true when
+// the pixel is finished (s is then the divisor, as dataPayload.sample is in
+// the reference).
 template <bool FEAT>
 __device__ __forceinline__ bool end_sample(float* __restrict__ a, uint32_t lane, const float path[NB],
-                                           const Adaptive& ad, uint32_t& s) {
-    kahan_acc(a, lane, A_RAD_S, A_RAD_C, path, false);
+                                           const Adaptive& ad, uint32_t& s, float* rk) {
+    kahan_lds(rk, path);
     if (FEAT && ad.on) {
         float sq = 0.0f;
 #pragma unroll
@@ -229,7 +277,7 @@ __device__ __forceinline__ bool end_sample(float* __restrict__ a, uint32_t lane,
         ACC(a, A_S1, lane) = s1;
         ACC(a, A_S2, lane) = s2;
         bool stop = false;
-        if (s % (uint32_t)ad.miss == 0 && s != 0 && ACC(a, A_DEPTH, lane) == 0.0f) stop = true;
+        if (s % (uint32_t)ad.miss == 0 && s != 0 && acc_sum_get(a, A_DEPTH, lane) == 0.0f) stop = true;
         if (!stop && s % (uint32_t)ad.batch == 0 && s != 0) {
             if (s1 == 0.0f) {
                 stop = true;
@@ -285,7 +333,13 @@ __device__ __forceinline__ void render_tile(const TileJob& J, uint32_t tile, flo
     tile_xy(q, tiles_x, x, y);
     const bool valid = x < (uint32_t)cam.W && y < (uint32_t)cam.H;
     const uint32_t pix = x + y * (uint32_t)cam.W;
-    if (valid) acc_clear(a, lane);
+    float* const rk = s_lds + lane + fused_rad_lds<MULTI>() * kFusedBS;  // radiance Kahan pair
+    bool first_seen = false;  // the pixel's first hit is recorded
+    if (valid) {
+        acc_clear(a, lane);
+#pragma unroll
+        for (int k = 0; k < 2 * NB; ++k) rk[k * kFusedBS] = 0.0f;
+    }
     float beta[NB], path[NB];
 #pragma unroll
     for (int k = 0; k < NB; ++k) beta[k] = path[k] = 0.0f;
@@ -372,12 +426,19 @@ __device__ __forceinline__ void render_tile(const TileJob& J, uint32_t tile, flo
                     for (int k = 0; k < NB; ++k) path[k] = path[k] + beta[k] * (bg[k] * 1.0f);
                 }
                 if (b == 0) kahan_acc(a, lane, A_DIR_S, A_DIR_C, path, true);
-                mode = end_sample<FEAT>(a, lane, path, J.ext->adapt, s) ? M_DONE : M_NEW;
+                mode = end_sample<FEAT>(a, lane, path, J.ext->adapt, s, rk) ? M_DONE : M_NEW;
             } else {
                 Surface S = INST ? surface_instance<FEAT>(J.inst[hinst], h.id, h.b1, h.b2)
                                  : surface_at<FEAT>(verts, idx, mat, h.id, h.b1, h.b2);
                 n_hit++;
-                if (b == 0) accumulate_first_hit(a, lane, h.id, hinst, h.t, S);
+                if (b == 0) {
+                    if (!first_seen) {
+                        ACC(a, A_TRI, lane) = __uint_as_float(h.id);
+                        ACC(a, A_INST, lane) = __uint_as_float(hinst);
+                        first_seen = true;
+                    }
+                    accumulate_first_hit_sums(a, lane, h.t, S);
+                }
                 o = S.org;
                 if (MULTI) {
                     // vertex radiance starts at 0 (+ ambient x albedo at bounce 0, cpu_path_tracer.ipp:389-396)
@@ -457,11 +518,18 @@ __device__ __forceinline__ void render_tile(const TileJob& J, uint32_t tile, flo
                 mode = M_CLOSEST;
                 n_bounce++;
             } else {
-                mode = end_sample<FEAT>(a, lane, path, J.ext->adapt, s) ? M_DONE : M_NEW;
+                mode = end_sample<FEAT>(a, lane, path, J.ext->adapt, s, rk) ? M_DONE : M_NEW;
             }
         }
     }
-    if (valid) write_pixel(J.ext->out, pix, (int)s, a, lane, J.ext->pass, INST ? (const void*)J.inst : nullptr);
+    if (valid) {
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            ACC(a, A_RAD_S + k, lane) = rk[k * kFusedBS];
+            ACC(a, A_RAD_C + k, lane) = rk[(NB + k) * kFusedBS];
+        }
+        write_pixel(J.ext->out, pix, (int)s, a, lane, J.ext->pass, INST ? (const void*)J.inst : nullptr);
+    }
 }
 
 }  // namespace vg

@@ -9,9 +9,10 @@
 //   small state machine; every loop iteration traces exactly one ray per lane
 //   (camera, shadow or bounce ray, with a per-lane any-hit flag), so lanes
 //   whose path ended regenerate the next sample instead of idling.  Path
-//   throughput and path radiance stay in registers; the Kahan sums and the
-//   first-hit passes live in the tile-blocked accumulator block in global
-//   memory, read and written once per sample.
+//   throughput and path radiance stay in registers and the radiance Kahan
+//   sums in LDS; the direct pass's Kahan sums and the first-hit passes live in
+//   the tile-blocked accumulator block in global memory (the first-hit sums
+//   updated by atomic adds that return nothing).
 // * wavefront (point and sphere lights, no adaptive sampling, ambient light or
 //   shown background): queues of rays, closest-hit and any-hit
 //   traversal kernels, shading into per-hit shadow slots, bounce rays
@@ -76,7 +77,8 @@ __global__ void __launch_bounds__(kFusedBS, kFusedWavesPerSimd) k_render_fused(T
                                                          unsigned long long* __restrict__ counters) {
     __shared__ uint32_t s_ref[kStackLds * kFusedBS];
     __shared__ float s_dist[kStackLds * kFusedBS];
-    // staged light contribution (one light) or the vertex state (several lights)
+    // staged light contribution (one light) or the vertex state (several lights), then the
+    // radiance Kahan pair
     __shared__ float s_lds[fused_lds_floats<MULTI>() * kFusedBS];
     const uint32_t lane = threadIdx.x;
     uint32_t n_prim = 0, n_shadow = 0, n_bounce = 0, n_hit = 0;

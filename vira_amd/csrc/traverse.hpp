@@ -4,10 +4,13 @@
 // ViraBLAS::intersect (vira_blas.ipp:58-108): all child boxes of a node are
 // tested, the nearest child is entered and the others are pushed far-to-near.
 // Two node layouts: Node2 (binary, 64 B) and Node4 (four-wide, 128 B).
-// The stack's top 16 entries live in LDS (one column per lane, so a wave's
+// The stack's top 8 entries live in LDS (one column per lane, so a wave's
 // pushes and pops hit 64 consecutive dwords); deeper entries spill to private
 // memory.  Capacity: a binary tree of depth < 63 (the builder's bound) needs
-// at most 62 entries; its four-wide collapse at most 3 per level, 186.
+// at most 62 entries; its four-wide collapse at most 3 per level, 186.  On
+// the 33.5M-triangle DEM frame a push past 8 entries happens on 4 in 10,000
+// camera and bounce rays (the visit counters below), and 16 LDS entries were
+// no faster, so the LDS they would take holds the radiance sums instead.
 //
 // STATS = true counts what a traversal fetches (opt-in visit counters; the
 // reference's analogue is the Vulkan path tracer's debug counters,
@@ -20,8 +23,8 @@
 
 namespace vg {
 
-constexpr int kStackLds = 16;
-constexpr int kStackSpill = 176;
+constexpr int kStackLds = 8;     // power of two (ring index = sp & (kStackLds - 1))
+constexpr int kStackSpill = 184;  // kStackLds + kStackSpill >= 186
 
 struct HitRec {
     float t;
