@@ -316,12 +316,35 @@ def instanced_leg(ctx, dem_built, steps: int, warmup: int) -> dict:
     seen = sorted(int(v) for v in torch.unique(inst[inst >= 0]).cpu())
     print(f"# instanced scene: {dt * 1e3:.1f} ms/frame, {rays / dt / 1e6:.1f} Mrays/s (primary+shadow), instances in view "
           f"{seen}", file=sys.stderr, flush=True)
+    # roofline of the per-pixel kernel: bytes the traversal fetches per ray on this layout (the visit
+    # counts of a counting render of the same frame), shaded hits and the per-sample accumulators,
+    # over the kernel's own HIP-event time (mean of the timed frames)
+    from dataclasses import replace
+    vst = ctx.render(isc.camera, replace(isc.options, count_visits=True), passes)
+    st0 = stats[0]
+    n_closest = st0["primary_rays"] + st0["bounce_rays"]
+    per_ray = lambda v, n: 48.0 + (128.0 * v["nodes"] + 48.0 * v["triangles"] + 32.0 * v["tlas_nodes"]) / max(1, n)
+    a_c, a_s = per_ray(vst["closest_visits"], n_closest), per_ray(vst["shadow_visits"], st0["shadow_rays"])
+    nbytes = n_closest * a_c + st0["shadow_rays"] * a_s + st0["closest_hits"] * BYTES_PER_HIT + \
+        st0["primary_rays"] * BYTES_PER_SAMPLE_ACC
+    kms = sum(s["fused_ms"] for s in stats) / len(stats)
+    achieved = nbytes / (kms / 1e3) / 1e9
+    visits = {k: {"nodes_per_ray": round(vst[f"{k}_visits"]["nodes"] / max(1, n), 3),
+                  "tlas_nodes_per_ray": round(vst[f"{k}_visits"]["tlas_nodes"] / max(1, n), 3),
+                  "triangles_per_ray": round(vst[f"{k}_visits"]["triangles"] / max(1, n), 3),
+                  "max_stack": vst[f"{k}_visits"]["max_stack"]}
+              for k, n in (("closest", n_closest), ("shadow", st0["shadow_rays"]))}
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "kernel": "k_render_fused<4, true, false, false>",
+            "kernel_ms": round(kms, 2), "bytes_per_closest_ray": round(a_c, 1), "bytes_per_shadow_ray": round(a_s, 1),
+            "bytes_per_hit": BYTES_PER_HIT, "bytes_per_sample_acc": BYTES_PER_SAMPLE_ACC, "visits": visits}
+    print(f"# instanced roofline: {json.dumps(roof)}", file=sys.stderr, flush=True)
     ctx.tlas_build([dem_built.mesh])  # back to the 4097^2 DEM scene
     ctx.lights_set(dem_built.lights)
     stars = star_leg(ctx, steps)
     return {"workload": "instanced_moon_smallbody_spacecraft_4096px_8spp_2b", "ms_per_frame": round(dt * 1e3, 2),
             "mrays_s": round(rays / dt / 1e6, 1), "bounce_rays": stats[0]["bounce_rays"], "instances_in_view": seen,
-            "stars": stars}
+            "roofline": roof, "stars": stars}
 
 
 def star_leg(ctx, steps: int) -> dict:
