@@ -405,14 +405,15 @@ vira_gpu_status vira_gpu_render(vira_gpu_ctx* ctx, const vira_gpu_camera* cam, c
 /* No counterpart in the reference: vira_gpu_render for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc).  It renders with the per-pixel (fused)
- * integrator, as vira_gpu_render does for scenes of at most one light, and
- * writes the same passes bit for bit; its scratch (tile accumulators, ray
- * counters, light and instance tables) is one fixed array per GPU that the
- * library holds for the life of the process (about 48 MiB), so no hipMalloc,
- * hipFree or other allocating call is made.  Limits: at most 4096 8x8 pixel
- * tiles (e.g. 512 x 512 pixels; larger images return VIRA_GPU_ERR_INVALID)
- * and 64 instances (VIRA_GPU_ERR_UNSUPPORTED for the wavefront integrator).  Every call on one GPU shares that
- * array: calls that overlap on one GPU (two contexts, two threads) race. */
+ * integrator, as vira_gpu_render does by default, and writes the same passes
+ * bit for bit; its scratch (tile accumulators, ray counters, light and
+ * instance tables) is one fixed array per GPU that the library holds for the
+ * life of the process (about 49 MiB), so no hipMalloc, hipFree or other
+ * allocating call is made.  Limits: at most 4096 8x8 pixel tiles (e.g. 512 x
+ * 512 pixels) and 64 instances (VIRA_GPU_ERR_INVALID beyond either); the
+ * wavefront integrator returns VIRA_GPU_ERR_UNSUPPORTED.  Every call on one
+ * GPU shares that array: calls that overlap on one GPU (two contexts, two
+ * threads) race. */
 vira_gpu_status vira_gpu_render_noalloc(vira_gpu_ctx* ctx, const vira_gpu_camera* cam,
                                         const vira_gpu_render_opts* opts, const vira_gpu_outputs* out,
                                         vira_gpu_render_stats* stats);

@@ -73,12 +73,21 @@ def test_dem4097_error_against_fp64(gpu_ctx, dem4097):
     flat = torch.full((64, 64), 0.11, dtype=torch.float32, device="cuda")
     for name, tex in (("flat albedo 0.11", flat), ("log-normal 4096^2 texture", built.texture)):
         gpu_ctx.material_set(built.mesh, albedo=1.0, texture=tex, smooth_shading=sc.smooth_shading)
-        rp = S.RenderPasses(sc.image, sc.image, which=("radiance", "tri_id"))
+        rp = S.RenderPasses(sc.image, sc.image, which=("radiance", "tri_id", "depth"))
         gpu_ctx.render(built.camera, opts, rp)
         o64 = om.render(cam, lights, O.material(1.0, tex.cpu().numpy(), sc.smooth_shading),
                         O.options(1, 0, False, 1, sc.render_seed), precision=64, pixels=pix)
         tri = _u32(rp.triangle_id)[pix]
         same = (tri == o64["tri_id"][pix]) & (tri != 0xFFFFFFFF)
+        # first hits that differ: triangles that meet (a shared edge, or the grid vertex where six
+        # meet), at the same depth: pixel centres of this grid-aligned image fall on edges and
+        # vertices of the mesh, where rounding picks the side
+        diff = np.nonzero((tri != o64["tri_id"][pix]) & (tri != 0xFFFFFFFF) & (o64["tri_id"][pix] != 0xFFFFFFFF))[0]
+        idx = om.idx
+        for i in diff:
+            assert len(set(idx[tri[i]]) & set(idx[o64["tri_id"][pix][i]])) >= 1, i
+        dg = _u32(rp.depth)[pix][diff].view(np.float32).astype(np.float64)
+        assert diff.size == 0 or np.max(np.abs(dg - o64["depth"][pix][diff]) / o64["depth"][pix][diff]) < 1e-5
         g = rp.total_radiance.cpu().numpy().reshape(-1, 8)[pix][same].astype(np.float64)
         r = o64["radiance"][pix][same].astype(np.float64)
         lit = r > 0

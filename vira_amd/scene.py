@@ -514,7 +514,7 @@ class Context:
     def render_noalloc(self, camera: Camera, options: PathTracerOptions, passes: RenderPasses):
         """vira_gpu_render_noalloc: the per-pixel render without any device
         allocation (scratch in the library's fixed per-GPU array; at most 4096
-        8x8 tiles, one light).  Same passes as render(); no denoise step."""
+        8x8 tiles, 64 instances).  Same passes as render(); no denoise step."""
         st = capi.RenderStats()
         cam = camera.to_c()
         opt = options.to_c()
