@@ -59,16 +59,28 @@ def traffic_bytes(kernel: str, workload: str):
     return int(t["traffic_bytes"])
 
 
+# Grid build bytes per triangle: index read and slot written (12 + 4), flag pass (4 + 4), scan
+# (4 + 4), order pass (slot and rank read, id written: 12), leaf triangle emitted (id 4 + index 12
+# + three 12 B positions + 48 B written), refit of the bottom level (48 B read) and the Node4s
+# written (128 B per four 4-triangle leaves, x 4/3 for the levels above: 11); a square
+# power-of-two grid has one slot per triangle
+GRID_BUILD_BYTES_PER_TRI = 12 + 4 + 8 + 8 + 12 + 100 + 48 + 11
+
+
 def bvh_build_row(stats: dict, triangles: int) -> dict:
-    """The BVH build against HBM: ~36 B per triangle (its box and id) read per
-    level of the binary tree, over the build's device time."""
+    """The BVH build against HBM over the build's device time.  Binned SAH: ~36
+    B per triangle (its box and id) read per level of the binary tree.  Grid
+    build: GRID_BUILD_BYTES_PER_TRI per triangle, one pass each."""
+    grid = stats.get("method") == 2
     levels = int(stats["max_depth"])
-    nbytes = 36.0 * triangles * levels
+    nbytes = (GRID_BUILD_BYTES_PER_TRI if grid else 36.0 * levels) * triangles
     a = nbytes / (stats["build_ms"] * 1e-3) / 1e9
-    return {"ms": round(stats["build_ms"], 1), "triangles": triangles, "levels": levels,
+    return {"ms": round(stats["build_ms"], 2), "method": "grid" if grid else "binned_sah", "triangles": triangles,
+            "levels": levels,
             "roofline": {"bound": "hbm", "achieved": round(a, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(a / HBM_PEAK_GBS, 4), "algorithmic_bytes": nbytes},
-            "sah_cost": round(stats["sah_cost"], 3), "wide_nodes": stats["num_wide_nodes"]}
+            "sah_cost": round(stats["sah_cost"], 3), "wide_sah_cost": round(stats["wide_sah_cost"], 3),
+            "wide_nodes": stats["num_wide_nodes"]}
 
 
 def cpu_baseline(built, ctx, stride: int, threads: int):
@@ -563,6 +575,13 @@ def main() -> int:
           f"{sum(s['trace_shadow_ms'] for s in stats):.1f} ms, shading {sum(s['shade_ms'] for s in stats):.1f} ms",
           file=sys.stderr, flush=True)
 
+    # the binned-SAH build of the same mesh, for comparison (the frames above used the default build)
+    bvh_sah = None
+    if built.bvh.get("method") == 2:
+        bvh_sah = bvh_build_row(ctx.blas_build(built.mesh, method="sah"), sc.triangles)
+        ctx.blas_build(built.mesh)
+        print(f"# BVH: grid build {built.bvh['build_ms']:.2f} ms, binned-SAH build {bvh_sah['ms']} ms",
+              file=sys.stderr, flush=True)
     visits = visits_leg(ctx, built, passes) if (fused and not args.no_visits) else None
     if visits is not None:
         print(f"# GPU visits (BVH{built.bvh.get('width', 4)}): {json.dumps(visits)}", file=sys.stderr, flush=True)
@@ -650,6 +669,7 @@ def main() -> int:
         "instanced": instanced,
         "gpu_visits": visits,
         "bvh_build": bvh_build_row(built.bvh, sc.triangles),
+        "bvh_build_binned_sah": bvh_sah,
         "sensor_tail": sensor,
         "quipu": quipu,
     }

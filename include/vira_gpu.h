@@ -68,6 +68,12 @@ typedef struct {
     int bins;       /* SAH bins per node, 4..32 (default 16)  */
     int width;      /* 2 (binary, 64 B nodes) or 4 (collapsed, 128 B nodes; default) */
     float traversal_cost; /* SAH cost of a node visit relative to a triangle test (0 = 1) */
+    int method;     /* 0 auto (grid for a mesh made from a DEM grid at width 4, else binned SAH);
+                       1 binned SAH; 2 grid: for a mesh made from a DEM grid (vira_gpu_dem_mesh,
+                       vira_gpu_qld_read) at width 4, the triangles in the Morton order of their grid
+                       cells, the largest power of two up to max_leaf (and 8) to a leaf, under a
+                       complete four-wide tree (each node a 2x2 block of its children on a square
+                       power-of-two grid), boxes reduced bottom up */
 } vira_gpu_bvh_opts;
 
 typedef struct {
@@ -81,6 +87,10 @@ typedef struct {
     double sah_cost;         /* sum over internal nodes of area/root area, plus leaf tris*area/root area */
     double build_ms;         /* device time of the build, HIP events   */
     uint64_t bytes;          /* device bytes of nodes + leaf triangles */
+    int32_t method;          /* the build that ran: 1 binned SAH, 2 grid */
+    double wide_sah_cost;    /* width 4: sum over Node4s of area/root area, plus leaf tris*area/root area
+                                (comparable between the two builds; sah_cost is the binary tree's, binned
+                                SAH only) */
 } vira_gpu_bvh_stats;
 
 /* Light: kind 0 = PointLight (point_light.ipp:10-38), value = spectral
@@ -355,7 +365,8 @@ vira_gpu_status vira_gpu_dem_update_refit(vira_gpu_ctx* ctx, int32_t mesh_id, co
 
 /* --- acceleration ---------------------------------------------------------- */
 /* Mesh::buildBVH -> EmbreeBLAS (mesh.ipp:396-418, embree_blas.ipp:14-71):
- * binned-SAH build on the GPU. opts may be NULL. */
+ * binned-SAH build on the GPU, or the grid build for DEM meshes (opts.method).
+ * opts may be NULL. */
 vira_gpu_status vira_gpu_blas_build(vira_gpu_ctx* ctx, int32_t mesh_id, const vira_gpu_bvh_opts* opts);
 vira_gpu_status vira_gpu_blas_stats(vira_gpu_ctx* ctx, int32_t mesh_id, vira_gpu_bvh_stats* stats);
 /* New (the reference rebuilds, embree_blas.ipp:14-71): a built BLAS refitted

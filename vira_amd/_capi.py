@@ -47,7 +47,8 @@ class Projection(C.Structure):
 
 
 class BVHOpts(C.Structure):
-    _fields_ = [("max_leaf", C.c_int), ("bins", C.c_int), ("width", C.c_int), ("traversal_cost", C.c_float)]
+    _fields_ = [("max_leaf", C.c_int), ("bins", C.c_int), ("width", C.c_int), ("traversal_cost", C.c_float),
+                ("method", C.c_int)]
 
 
 class BVHStats(C.Structure):
@@ -62,6 +63,8 @@ class BVHStats(C.Structure):
         ("sah_cost", C.c_double),
         ("build_ms", C.c_double),
         ("bytes", C.c_uint64),
+        ("method", C.c_int32),
+        ("wide_sah_cost", C.c_double),
     ]
 
 

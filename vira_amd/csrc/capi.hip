@@ -253,6 +253,7 @@ vira_gpu_status vira_gpu_blas_build(vira_gpu_ctx* ctx, int32_t mesh_id, const vi
         VG_CHECK(o.width == 2 || o.width == 4, VIRA_GPU_ERR_INVALID, "blas_build: width must be 2 or 4");
         VG_CHECK(o.traversal_cost >= 0.0f && o.traversal_cost <= 64.0f, VIRA_GPU_ERR_INVALID,
                  "blas_build: traversal_cost must be 0..64");
+        VG_CHECK(o.method >= 0 && o.method <= 2, VIRA_GPU_ERR_INVALID, "blas_build: method must be 0, 1 or 2");
         vg::MeshG& m = c.mesh(mesh_id);
         c.scene_gen++;
         vg::blas_build(c, m, o);

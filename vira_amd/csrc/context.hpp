@@ -149,6 +149,9 @@ struct MeshG {
     DBuf fnrm;    // float4[nt]: unit face normal from the FP64 positions, w = 0 where skipped
     Blas blas;
     vira_gpu_material mat{};
+    // grid provenance (dem_mesh, quipu levels): vertex v is sample (v % grid_w, v / grid_w) and the
+    // triangles are the grid's cells in imageToIndexBuffer order, holes left out
+    int grid_w = 0, grid_h = 0;
     // DEM provenance (dem_mesh): node v is grid sample (v % dem_w, v / dem_w)
     bool is_dem = false;
     int dem_w = 0, dem_h = 0;

@@ -318,6 +318,8 @@ void dem_grid_mesh(Ctx& c, MeshG& m, const float* heights, int W, int H,
     uint32_t nt = scan_exclusive_u32(c, flag.as<uint32_t>(), ids.as<uint32_t>(), ncand);
     m.nv = nv;
     m.nt = nt;
+    m.grid_w = W;
+    m.grid_h = H;
     m.verts.alloc(nv * sizeof(GVertex));
     m.idx.alloc(std::max<size_t>(1, (size_t)nt * 3 * 4));
     k_cand_write<<<blocks_for(ncand, 256), 256, 0, c.stream>>>(heights, W, H, ncand, flag.as<uint32_t>(),

@@ -147,11 +147,11 @@ class BuiltScene:
 
 
 def build(ctx: S.Context, sc: DemScene, bvh_max_leaf: int = 4, bvh_bins: int = 16, bvh_width: int = 4,
-          bvh_traversal_cost: float = 0.0) -> BuiltScene:
+          bvh_traversal_cost: float = 0.0, bvh_method: str = "auto") -> BuiltScene:
     heights = ctx.synth_fbm(sc.dem, sc.dem, sc.seed, sc.amplitude, sc.octaves, sc.base_cells)
     valb = ctx.synth_vertex_albedo(sc.dem * sc.dem, sc.seed) if sc.albedo == "vertex" else None
     mesh = ctx.dem_mesh(heights, make_projection(sc), valb)
-    bvh = ctx.blas_build(mesh, bvh_max_leaf, bvh_bins, bvh_width, bvh_traversal_cost)
+    bvh = ctx.blas_build(mesh, bvh_max_leaf, bvh_bins, bvh_width, bvh_traversal_cost, bvh_method)
     ctx.tlas_build([mesh])
     lights = make_lights(sc)
     ctx.lights_set(lights)
