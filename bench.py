@@ -291,6 +291,7 @@ def batch_leg(ctx, dem_built, steps: int, warmup: int) -> dict:
     st = {}
     batch_s = timed(lambda: st.update(ctx.batch_submit(reqs)), steps)
     rays = st["primary_rays"] + st["shadow_rays"]
+    ctx.blas_build(dem_built.mesh)  # the autotuner rebuilt the meshes at its width: back to the default tree
     print(f"# mixed batch: 64 requests, {st['tiles']} tiles, {rays} primary+shadow rays; per-request {plain_s * 1e3:.1f} ms, "
           f"batched {batch_s * 1e3:.1f} ms (kernel {st['ms']:.1f} ms, graph reused {st['graph_reused']}); "
           f"autotune {tune_s:.1f} s -> {cfg}; identical passes: {identical}", file=sys.stderr, flush=True)

@@ -72,15 +72,15 @@ struct TravStack {
     }
 };
 
+// Compare-exchange by selects (no branch: the sorting network stays straight-line code).
 __device__ __forceinline__ void cswap(float& da, uint32_t& ra, float& db, uint32_t& rb) {
-    if (db < da) {
-        float t = da;
-        da = db;
-        db = t;
-        uint32_t u = ra;
-        ra = rb;
-        rb = u;
-    }
+    const bool s = db < da;
+    const float lo = s ? db : da, hi = s ? da : db;
+    const uint32_t rl = s ? rb : ra, rh = s ? ra : rb;
+    da = lo;
+    db = hi;
+    ra = rl;
+    rb = rh;
 }
 
 // any: occlusion query within [0, tmax] (returns on the first hit);
@@ -114,14 +114,16 @@ __device__ __forceinline__ HitRec traverse_impl(const void* __restrict__ nodes_v
         // ---- descend through internal nodes until a leaf (or nothing) is reached
         while (!ref_is_leaf(cur)) {
             if (STATS) tc->nodes++;
+            // Every child's slab test runs, and an empty child's result is then replaced by a
+            // select: a test guarded by the child's reference made the compiler issue that
+            // child's bound loads only after the references arrived (two memory round trips
+            // per node) and branch around each test.
             if (W == 2) {
                 const Node2 nd = reinterpret_cast<const Node2*>(nodes_v)[cur];
-                float d0 = nd.ref.x != kEmptyRef
-                               ? slab<SAFE>(nd.x.x, nd.y.x, nd.z.x, nd.x.y, nd.y.y, nd.z.y, o, inv, e, best)
-                               : kInfF;
-                float d1 = nd.ref.y != kEmptyRef
-                               ? slab<SAFE>(nd.x.z, nd.y.z, nd.z.z, nd.x.w, nd.y.w, nd.z.w, o, inv, e, best)
-                               : kInfF;
+                float d0 = slab<SAFE>(nd.x.x, nd.y.x, nd.z.x, nd.x.y, nd.y.y, nd.z.y, o, inv, e, best);
+                float d1 = slab<SAFE>(nd.x.z, nd.y.z, nd.z.z, nd.x.w, nd.y.w, nd.z.w, o, inv, e, best);
+                d0 = nd.ref.x != kEmptyRef ? d0 : kInfF;
+                d1 = nd.ref.y != kEmptyRef ? d1 : kInfF;
                 bool h0 = d0 < kInfF, h1 = d1 < kInfF;
                 if (h0 && h1) {
                     uint32_t nearr = nd.ref.x, farr = nd.ref.y;
@@ -144,18 +146,14 @@ __device__ __forceinline__ HitRec traverse_impl(const void* __restrict__ nodes_v
             } else {
                 const Node4 nd = reinterpret_cast<const Node4*>(nodes_v)[cur];
                 uint32_t r0 = nd.ref.x, r1 = nd.ref.y, r2 = nd.ref.z, r3 = nd.ref.w;
-                float d0 = r0 != kEmptyRef
-                               ? slab<SAFE>(nd.lox.x, nd.loy.x, nd.loz.x, nd.hix.x, nd.hiy.x, nd.hiz.x, o, inv, e, best)
-                               : kInfF;
-                float d1 = r1 != kEmptyRef
-                               ? slab<SAFE>(nd.lox.y, nd.loy.y, nd.loz.y, nd.hix.y, nd.hiy.y, nd.hiz.y, o, inv, e, best)
-                               : kInfF;
-                float d2 = r2 != kEmptyRef
-                               ? slab<SAFE>(nd.lox.z, nd.loy.z, nd.loz.z, nd.hix.z, nd.hiy.z, nd.hiz.z, o, inv, e, best)
-                               : kInfF;
-                float d3 = r3 != kEmptyRef
-                               ? slab<SAFE>(nd.lox.w, nd.loy.w, nd.loz.w, nd.hix.w, nd.hiy.w, nd.hiz.w, o, inv, e, best)
-                               : kInfF;
+                float d0 = slab<SAFE>(nd.lox.x, nd.loy.x, nd.loz.x, nd.hix.x, nd.hiy.x, nd.hiz.x, o, inv, e, best);
+                float d1 = slab<SAFE>(nd.lox.y, nd.loy.y, nd.loz.y, nd.hix.y, nd.hiy.y, nd.hiz.y, o, inv, e, best);
+                float d2 = slab<SAFE>(nd.lox.z, nd.loy.z, nd.loz.z, nd.hix.z, nd.hiy.z, nd.hiz.z, o, inv, e, best);
+                float d3 = slab<SAFE>(nd.lox.w, nd.loy.w, nd.loz.w, nd.hix.w, nd.hiy.w, nd.hiz.w, o, inv, e, best);
+                d0 = r0 != kEmptyRef ? d0 : kInfF;
+                d1 = r1 != kEmptyRef ? d1 : kInfF;
+                d2 = r2 != kEmptyRef ? d2 : kInfF;
+                d3 = r3 != kEmptyRef ? d3 : kInfF;
                 // sorting network: d0 <= d1 <= d2 <= d3
                 cswap(d0, r0, d1, r1);
                 cswap(d2, r2, d3, r3);
