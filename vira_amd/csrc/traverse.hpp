@@ -23,6 +23,7 @@
 
 namespace vg {
 
+constexpr int kLeafBatch = 2;  // leaf triangles fetched together (4 spilled registers and measured slower)
 constexpr int kStackLds = 8;     // power of two (ring index = sp & (kStackLds - 1))
 constexpr int kStackSpill = 184;  // kStackLds + kStackSpill >= 186
 
@@ -170,33 +171,45 @@ __device__ __forceinline__ HitRec traverse_impl(const void* __restrict__ nodes_v
                 cur = r0;
             }
         }
-        // ---- leaf
+        // ---- leaf: its triangles are fetched kLeafBatch at a time, every load of a batch issued
+        // before the first test (one memory round trip per batch, not per triangle); slots past
+        // the leaf load its last triangle again, so every load is in bounds
         if (cur != kEmptyRef) {
-            uint32_t first = leaf_first(cur), cnt = leaf_count(cur);
+            const uint32_t first = leaf_first(cur), cnt = leaf_count(cur);
             if (STATS) tc->leaves++;
-            for (uint32_t i = 0; i < cnt; ++i) {
-                if (STATS) tc->tris++;
-                const float4 a = tris[3 * (first + i)];
-                const float4 b = tris[3 * (first + i) + 1];
-                const float4 c = tris[3 * (first + i) + 2];
-                float t, b1, b2;
-                if (!wt_intersect(o, w, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), t, b1, b2))
-                    continue;
-                uint32_t id = __float_as_uint(a.w);
-                if (any) {
-                    if (t <= tmax) {
+            for (uint32_t i0 = 0; i0 < cnt; i0 += kLeafBatch) {
+                float4 ta[kLeafBatch], tb[kLeafBatch], tcv[kLeafBatch];
+#pragma unroll
+                for (int k = 0; k < kLeafBatch; ++k) {
+                    const uint32_t i = first + min(i0 + (uint32_t)k, cnt - 1u);
+                    ta[k] = tris[3 * i];
+                    tb[k] = tris[3 * i + 1];
+                    tcv[k] = tris[3 * i + 2];
+                }
+#pragma unroll
+                for (int k = 0; k < kLeafBatch; ++k) {
+                    if (i0 + (uint32_t)k >= cnt) break;
+                    if (STATS) tc->tris++;
+                    const float4 a = ta[k], b = tb[k], c = tcv[k];
+                    float t, b1, b2;
+                    if (!wt_intersect(o, w, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), mk3(c.x, c.y, c.z), t, b1, b2))
+                        continue;
+                    uint32_t id = __float_as_uint(a.w);
+                    if (any) {
+                        if (t <= tmax) {
+                            h.t = t;
+                            h.id = id;
+                            h.b1 = b1;
+                            h.b2 = b2;
+                            return h;
+                        }
+                    } else if (t <= tmax && (t < h.t || (t == h.t && id < h.id))) {
                         h.t = t;
                         h.id = id;
                         h.b1 = b1;
                         h.b2 = b2;
-                        return h;
+                        best = t;
                     }
-                } else if (t <= tmax && (t < h.t || (t == h.t && id < h.id))) {
-                    h.t = t;
-                    h.id = id;
-                    h.b1 = b1;
-                    h.b2 = b2;
-                    best = t;
                 }
             }
         }
