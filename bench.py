@@ -203,6 +203,32 @@ def visits_leg(ctx, built, passes) -> dict:
     return out
 
 
+def compressed_leg(ctx, built, passes, steps: int) -> dict:
+    """The DEM frame with the 64 B quantised nodes (blas_build compress=True)
+    beside the 128 B nodes it is made from: kernel time per frame of each,
+    passes compared bit for bit."""
+    import torch
+
+    def frames(n):
+        ms = []
+        for _ in range(n + 1):
+            ms.append(ctx.render(built.camera, built.options, passes)["fused_ms"])
+        torch.cuda.synchronize()
+        return float(np.mean(ms[1:]))
+
+    plain_ms = frames(steps)
+    ref = passes.total_radiance.clone()
+    st = ctx.blas_build(built.mesh, compress=True)
+    q_ms = frames(steps)
+    same = bool(torch.equal(ref, passes.total_radiance))
+    ctx.blas_build(built.mesh)
+    out = {"node4_ms": round(plain_ms, 2), "node4q_ms": round(q_ms, 2), "identical_passes": same,
+           "node4_bytes": int(st["num_wide_nodes"]) * 128, "node4q_bytes": int(st["compressed_bytes"])}
+    print(f"# compressed nodes (DEM frame): Node4 {plain_ms:.2f} ms, Node4Q {q_ms:.2f} ms, identical passes {same}",
+          file=sys.stderr, flush=True)
+    return out
+
+
 def adjoint_leg(ctx, steps: int, warmup: int) -> dict:
     """Differentiable DI: 1025^2 DEM (2,097,152 triangles), 1024^2 image, fixed-visibility
     direct illumination.  Times the forward render (keeping hit records) and the
@@ -352,6 +378,20 @@ def instanced_leg(ctx, dem_built, steps: int, warmup: int) -> dict:
             "kernel_ms": round(kms, 2), "bytes_per_closest_ray": round(a_c, 1), "bytes_per_shadow_ray": round(a_s, 1),
             "bytes_per_hit": BYTES_PER_HIT, "bytes_per_sample_acc": BYTES_PER_SAMPLE_ACC, "visits": visits}
     print(f"# instanced roofline: {json.dumps(roof)}", file=sys.stderr, flush=True)
+    # the same frame with every BLAS's quantised node copy
+    for m in isc.meshes:
+        ctx.blas_build(m, compress=True)
+    ctx.tlas_build(isc.meshes, isc.transforms)
+    ref = passes.total_radiance.clone()
+    qms = [ctx.render(isc.camera, isc.options, passes)["fused_ms"] for _ in range(steps + 1)][1:]
+    torch.cuda.synchronize()
+    q_same = bool(torch.equal(ref, passes.total_radiance))
+    roof["node4q_kernel_ms"] = round(float(np.mean(qms)), 2)
+    roof["node4q_identical_passes"] = q_same
+    print(f"# instanced scene with Node4Q: {np.mean(qms):.2f} ms (Node4 {kms:.2f} ms), identical passes {q_same}",
+          file=sys.stderr, flush=True)
+    for m in isc.meshes:
+        ctx.blas_build(m)
     ctx.tlas_build([dem_built.mesh])  # back to the 4097^2 DEM scene
     ctx.lights_set(dem_built.lights)
     stars = star_leg(ctx, steps)
@@ -523,6 +563,7 @@ def main() -> int:
     ap.add_argument("--no-instanced", action="store_true", help="skip the instanced-scene leg")
     ap.add_argument("--no-quipu", action="store_true", help="skip the quipu pyramid read leg")
     ap.add_argument("--no-visits", action="store_true", help="skip the BVH visit-counting render")
+    ap.add_argument("--no-compressed", action="store_true", help="skip the quantised-node comparison")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -584,6 +625,8 @@ def main() -> int:
         print(f"# BVH: grid build {built.bvh['build_ms']:.2f} ms, binned-SAH build {bvh_sah['ms']} ms",
               file=sys.stderr, flush=True)
     visits = visits_leg(ctx, built, passes) if (fused and not args.no_visits) else None
+    compressed = compressed_leg(ctx, built, passes, max(2, min(5, args.steps))) if (fused and not args.no_compressed) \
+        else None
     if visits is not None:
         print(f"# GPU visits (BVH{built.bvh.get('width', 4)}): {json.dumps(visits)}", file=sys.stderr, flush=True)
     sensor = sensor_leg(ctx, built, passes, max(1, min(3, args.steps))) if args.scene == "dem4097" else None
@@ -669,6 +712,7 @@ def main() -> int:
         "adjoint": adjoint,
         "instanced": instanced,
         "gpu_visits": visits,
+        "bvh_compressed": compressed,
         "bvh_build": bvh_build_row(built.bvh, sc.triangles),
         "bvh_build_binned_sah": bvh_sah,
         "sensor_tail": sensor,

@@ -74,6 +74,9 @@ typedef struct {
                        cells, the largest power of two up to max_leaf (and 8) to a leaf, under a
                        complete four-wide tree (each node a 2x2 block of its children on a square
                        power-of-two grid), boxes reduced bottom up */
+    int compress;   /* width 4: also keep a copy of the nodes with 8-bit quantised child boxes (64 B per
+                       node, decoded outward-rounded), which vira_gpu_render's per-pixel integrator and
+                       vira_gpu_trace_rays traverse when every BLAS of the scene has one */
 } vira_gpu_bvh_opts;
 
 typedef struct {
@@ -91,6 +94,7 @@ typedef struct {
     double wide_sah_cost;    /* width 4: sum over Node4s of area/root area, plus leaf tris*area/root area
                                 (comparable between the two builds; sah_cost is the binary tree's, binned
                                 SAH only) */
+    uint64_t compressed_bytes; /* device bytes of the quantised node copy (0 without opts.compress) */
 } vira_gpu_bvh_stats;
 
 /* Light: kind 0 = PointLight (point_light.ipp:10-38), value = spectral

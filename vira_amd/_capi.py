@@ -48,7 +48,7 @@ class Projection(C.Structure):
 
 class BVHOpts(C.Structure):
     _fields_ = [("max_leaf", C.c_int), ("bins", C.c_int), ("width", C.c_int), ("traversal_cost", C.c_float),
-                ("method", C.c_int)]
+                ("method", C.c_int), ("compress", C.c_int)]
 
 
 class BVHStats(C.Structure):
@@ -65,6 +65,7 @@ class BVHStats(C.Structure):
         ("bytes", C.c_uint64),
         ("method", C.c_int32),
         ("wide_sah_cost", C.c_double),
+        ("compressed_bytes", C.c_uint64),
     ]
 
 

@@ -23,6 +23,7 @@
 namespace vg {
 
 void mesh_upload_pack(Ctx& c, MeshG& m, const float* pos, const float* alb, const float* uv, const float* nrm);
+void compress_blas(Ctx& c, Blas& B);
 
 namespace {
 
@@ -989,6 +990,7 @@ void blas_refit(Ctx& c, MeshG& m) {
     float ms = 0.f;
     VG_HIP(hipEventElapsedTime(&ms, e0, e1));
     B.stats.build_ms = ms;
+    if (B.compressed) compress_blas(c, B);  // the quantised copy follows the new boxes
 }
 
 // ---------------------------------------------------------------- grid build
@@ -1098,6 +1100,88 @@ __global__ void k_grid_topo(GridLevels L, uint32_t total, uint32_t nt, uint32_t 
     o.ref = make_uint4(r[0], r[1], r[2], r[3]);
     o.pad = make_uint4(0u, 0u, 0u, 0u);
     nodes[i] = o;
+}
+
+// ---------------------------------------------------------------- quantised copy
+// Node4 -> Node4Q.  Per node and axis the origin is the lowest child bound and
+// the scale the smallest power of two with which 255 steps reach the highest
+// (checked with the decoding fmaf itself); each child's lower byte is the
+// largest whose decoded value is at most its bound, the upper byte the
+// smallest whose decoded value is at least its bound.  err is set if a node
+// cannot be encoded (never for finite boxes).
+__global__ void k_compress4(const Node4* __restrict__ in, uint32_t n, Node4Q* __restrict__ out,
+                            uint32_t* __restrict__ err) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Node4 nd = in[i];
+    Node4Q q;
+    q.ref = nd.ref;
+    q.pad[0] = q.pad[1] = 0u;
+    q.ex = 0u;
+    const float4 L[3] = {nd.lox, nd.loy, nd.loz}, Hh[3] = {nd.hix, nd.hiy, nd.hiz};
+    float org[3];
+    for (int a = 0; a < 3; ++a) {
+        float mn = INFINITY, mx = -INFINITY;
+        for (int c = 0; c < 4; ++c) {
+            if (u4_get(nd.ref, c) == kEmptyRef) continue;
+            mn = fminf(mn, f4_get(L[a], c));
+            mx = fmaxf(mx, f4_get(Hh[a], c));
+        }
+        uint32_t qlo = 0u, qhi = 0u, ex = 127u;
+        if (!(mn <= mx)) {  // no child: nothing to bound
+            org[a] = 0.0f;
+        } else {
+            org[a] = mn;
+            const double ext = (double)mx - (double)mn;
+            int e = 1;
+            if (ext > 0.0) {
+                int k = 0;
+                (void)frexp(ext / 255.0, &k);  // 2^k >= ext / 255
+                e = max(1, k + 127);
+            }
+            while (e < 254 && __builtin_fmaf(255.0f, __uint_as_float((uint32_t)e << 23), mn) < mx) ++e;
+            if (__builtin_fmaf(255.0f, __uint_as_float((uint32_t)e << 23), mn) < mx) atomicOr(err, 1u);
+            ex = (uint32_t)e;
+            const float sc = __uint_as_float(ex << 23);
+            for (int c = 0; c < 4; ++c) {
+                if (u4_get(nd.ref, c) == kEmptyRef) continue;
+                const float lo = f4_get(L[a], c), hi = f4_get(Hh[a], c);
+                int ql = (int)fmin(255.0, fmax(0.0, floor(((double)lo - (double)mn) / (double)sc)));
+                while (ql > 0 && __builtin_fmaf((float)ql, sc, mn) > lo) --ql;
+                int qh = (int)fmin(255.0, fmax(0.0, ceil(((double)hi - (double)mn) / (double)sc)));
+                while (qh < 255 && __builtin_fmaf((float)qh, sc, mn) < hi) ++qh;
+                if (__builtin_fmaf((float)ql, sc, mn) > lo || __builtin_fmaf((float)qh, sc, mn) < hi) atomicOr(err, 1u);
+                qlo |= (uint32_t)ql << (8 * c);
+                qhi |= (uint32_t)qh << (8 * c);
+            }
+        }
+        q.qlo[a] = qlo;
+        q.qhi[a] = qhi;
+        q.ex |= ex << (8 * a);
+    }
+    q.ox = org[0];
+    q.oy = org[1];
+    q.oz = org[2];
+    out[i] = q;
+}
+
+// The quantised copy of a built four-wide tree (kept beside the Node4s).
+void compress_blas(Ctx& c, Blas& B) {
+    VG_CHECK(B.width == 4, VIRA_GPU_ERR_INVALID, "BVH compression needs a four-wide tree");
+    const uint32_t n = (uint32_t)B.num_nodes;
+    B.qnodes.alloc((size_t)n * sizeof(Node4Q));
+    DBuf err;
+    err.alloc(4);
+    VG_HIP(hipMemsetAsync(err.p, 0, 4, c.stream));
+    k_compress4<<<blocks_for(n, 256), 256, 0, c.stream>>>(B.nodes.as<Node4>(), n, B.qnodes.as<Node4Q>(),
+                                                          err.as<uint32_t>());
+    VG_HIP(hipGetLastError());
+    uint32_t herr = 0;
+    VG_HIP(hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, c.stream));
+    VG_HIP(hipStreamSynchronize(c.stream));
+    VG_CHECK(herr == 0, VIRA_GPU_ERR_HIP, "BVH compression: a node's boxes could not be encoded");
+    B.compressed = true;
+    B.stats.compressed_bytes = (uint64_t)n * sizeof(Node4Q);
 }
 
 // Statistics of a built Node4 tree (wide SAH, leaves, largest leaf).
@@ -1224,15 +1308,27 @@ void blas_build_grid(Ctx& c, MeshG& m, int max_leaf) {
     B.built = true;
 }
 
+// After any build: the quantised copy when asked for, else none.
+void finish_compression(Ctx& c, Blas& B, const vira_gpu_bvh_opts& opt) {
+    B.compressed = false;
+    B.qnodes.release();
+    B.stats.compressed_bytes = 0;
+    if (opt.compress && B.built && B.width == 4) compress_blas(c, B);
+}
+
 void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     const int max_leaf = std::min(15, std::max(1, opt.max_leaf > 0 ? opt.max_leaf : 4));
+    m.blas.compressed = false;
+    m.blas.qnodes.release();
     const bool grid_ok = m.grid_w >= 2 && m.grid_h >= 2 && opt.width != 2 && m.nt > 0;
     VG_CHECK(opt.method >= 0 && opt.method <= 2, VIRA_GPU_ERR_INVALID, "blas_build: method must be 0, 1 or 2");
     VG_CHECK(opt.method != 2 || grid_ok, VIRA_GPU_ERR_INVALID,
              "blas_build: the grid build takes a mesh made from a DEM grid, width 4");
+    VG_CHECK(!opt.compress || opt.width != 2, VIRA_GPU_ERR_INVALID, "blas_build: compression needs width 4");
     if (opt.method == 2 || (opt.method == 0 && grid_ok && kGridByDefault)) {
         m.blas.built = false;
         blas_build_grid(c, m, max_leaf);
+        finish_compression(c, m.blas, opt);
         return;
     }
     const int nb = std::min(kMaxBins, std::max(4, opt.bins > 0 ? opt.bins : 16));
@@ -1579,6 +1675,7 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     B.stats.method = 1;
     if (B.width == 4) B.stats.wide_sah_cost = wide_stats(c, B).sah;
     B.built = true;
+    finish_compression(c, B, opt);
 }
 
 }  // namespace vg

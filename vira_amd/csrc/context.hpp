@@ -139,6 +139,8 @@ struct Blas {
     vira_gpu_bvh_opts opts{4, 16, 4};  // options of the last build (dem_update rebuilds with them)
     float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {0.f, 0.f, 0.f};  // box of all triangles (empty: lo > hi)
     std::vector<uint32_t> level_start;  // Node4 levels: level l is [level_start[l], level_start[l + 1])
+    DBuf qnodes;              // Node4Q[num_nodes]: quantised copy (opts.compress), same indices
+    bool compressed = false;
 };
 
 struct MeshG {

@@ -199,6 +199,30 @@ struct __attribute__((aligned(16))) Node4 {
     uint4 pad;
 };
 static_assert(sizeof(Node4) == 128, "Node4 is two cache lines");
+// Four-wide node with quantised child boxes (64 B, one cache line; traversal
+// width code 5).  Per axis a the node keeps an FP32 origin o[a] and a scale
+// 2^(ex[a] - 127); child c's bounds on that axis are the bytes c of qlo[a] and
+// qhi[a], decoded as fmaf(q, scale, origin) in FP32.  The compressing kernel
+// chooses every byte with that same instruction so that the decoded box
+// contains the child's box, rounding outward, so no ray that reaches a child
+// is refused by its decoded box and the conservative slab test keeps its
+// guarantee.
+struct __attribute__((aligned(64))) Node4Q {
+    uint4 ref;
+    float ox, oy, oz;
+    uint32_t ex;       // byte a: biased exponent of axis a's scale
+    uint32_t qlo[3];   // per axis: byte c = child c's lower bound
+    uint32_t qhi[3];   // per axis: byte c = child c's upper bound
+    uint32_t pad[2];
+};
+static_assert(sizeof(Node4Q) == 64, "Node4Q is one cache line");
+constexpr int kWidthQ = 5;  // traversal width code of Node4Q trees
+
+__device__ __forceinline__ float q_scale(uint32_t ex, int a) { return __uint_as_float(((ex >> (8 * a)) & 0xffu) << 23); }
+__device__ __forceinline__ float q_decode(uint32_t q, int c, float scale, float origin) {
+    return __builtin_fmaf((float)((q >> (8 * c)) & 0xffu), scale, origin);
+}
+
 __device__ __host__ __forceinline__ bool ref_is_leaf(uint32_t r) { return (r & 0x80000000u) != 0; }
 __device__ __host__ __forceinline__ uint32_t leaf_first(uint32_t r) { return (r >> 4) & 0x7ffffffu; }
 __device__ __host__ __forceinline__ uint32_t leaf_count(uint32_t r) { return (r & 15u) + 1u; }

@@ -342,7 +342,10 @@ void trace_rays(Ctx& c, MeshG& m, const float* rays, size_t n, int any_hit, floa
     }
     EventPair ev;
     VG_HIP(hipEventRecord(ev.a, c.stream));
-    if (m.blas.width == 4)
+    if (m.blas.width == 4 && m.blas.compressed)
+        k_trace_rays<kWidthQ><<<blocks_for(n, kTraceBS), kTraceBS, 0, c.stream>>>(
+            m.blas.qnodes.p, m.blas.tris.as<float4>(), rays, n, any_hit, t, id, b1, b2);
+    else if (m.blas.width == 4)
         k_trace_rays<4><<<blocks_for(n, kTraceBS), kTraceBS, 0, c.stream>>>(m.blas.nodes.p, m.blas.tris.as<float4>(),
                                                                             rays, n, any_hit, t, id, b1, b2);
     else
@@ -446,6 +449,9 @@ void run_fused(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o
     }
     const Lights lf{nl, S.lights};
     const MatF mf = mat_params(m);
+    // the quantised nodes when every BLAS of the scene has them
+    bool wq = m.blas.width == 4 && m.blas.compressed;
+    for (int i = 0; i < ninst && wq; ++i) wq = c.mesh(c.tlas[i].mesh_id).blas.compressed;
     if (inst) {
         // instance table (pointers, materials) and FP64 inputs; k_rebase fills the FP32 placement.
         // The allocation-free call stages blocks of kStage instances on the stack; others take the
@@ -467,7 +473,7 @@ void run_fused(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o
             for (int i = 0; i < n; ++i) {
                 const MeshG& mi = c.mesh(c.tlas[i0 + i].mesh_id);
                 hi[i] = InstG{};
-                hi[i].nodes = mi.blas.nodes.p;
+                hi[i].nodes = wq ? mi.blas.qnodes.p : mi.blas.nodes.p;
                 hi[i].tris = mi.blas.tris.as<float4>();
                 hi[i].verts = mi.verts.as<GVertex>();
                 hi[i].idx = mi.idx.as<uint32_t>();
@@ -510,7 +516,7 @@ void run_fused(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o
     VG_HIP(hipEventRecord(ev0, c.stream));
     const bool w4 = m.blas.width == 4;
     TileJob J{};
-    J.nodes = m.blas.nodes.p;
+    J.nodes = wq ? m.blas.qnodes.p : m.blas.nodes.p;
     J.tris = m.blas.tris.as<float4>();
     J.verts = m.verts.as<GVertex>();
     J.idx = m.idx.as<uint32_t>();
@@ -531,20 +537,24 @@ void run_fused(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o
     const bool feat = fused_feat(c, o);
     using K = void (*)(TileJob, float*, unsigned long long*);
 #define VG_FUSED_PAIR(W_, I_, M_) {k_render_fused<W_, I_, M_, false>, k_render_fused<W_, I_, M_, true>}
-    const K table[2][2][2][2] = {
+    const K table[3][2][2][2] = {
         {{VG_FUSED_PAIR(2, false, false), VG_FUSED_PAIR(2, false, true)},
          {VG_FUSED_PAIR(2, true, false), VG_FUSED_PAIR(2, true, true)}},
         {{VG_FUSED_PAIR(4, false, false), VG_FUSED_PAIR(4, false, true)},
-         {VG_FUSED_PAIR(4, true, false), VG_FUSED_PAIR(4, true, true)}}};
+         {VG_FUSED_PAIR(4, true, false), VG_FUSED_PAIR(4, true, true)}},
+        {{VG_FUSED_PAIR(kWidthQ, false, false), VG_FUSED_PAIR(kWidthQ, false, true)},
+         {VG_FUSED_PAIR(kWidthQ, true, false), VG_FUSED_PAIR(kWidthQ, true, true)}}};
 #undef VG_FUSED_PAIR
-    K kern = table[w4 ? 1 : 0][inst ? 1 : 0][multi ? 1 : 0][feat ? 1 : 0];
+    const int wi = w4 ? (wq ? 2 : 1) : 0;
+    K kern = table[wi][inst ? 1 : 0][multi ? 1 : 0][feat ? 1 : 0];
     if (o.count_visits) {
         VG_CHECK(!multi && !feat, VIRA_GPU_ERR_UNSUPPORTED,
                  "render: visit counting takes one light or none, no ambient light, background, adaptive "
                  "sampling or material maps");
-        const K stats_table[2][2] = {{k_render_fused_stats<2, false>, k_render_fused_stats<2, true>},
-                                     {k_render_fused_stats<4, false>, k_render_fused_stats<4, true>}};
-        kern = stats_table[w4 ? 1 : 0][inst ? 1 : 0];
+        const K stats_table[3][2] = {{k_render_fused_stats<2, false>, k_render_fused_stats<2, true>},
+                                     {k_render_fused_stats<4, false>, k_render_fused_stats<4, true>},
+                                     {k_render_fused_stats<kWidthQ, false>, k_render_fused_stats<kWidthQ, true>}};
+        kern = stats_table[wi][inst ? 1 : 0];
     }
     kern<<<tiles, kFusedBS, 0, c.stream>>>(J, S.acc, S.counters);
     VG_HIP(hipGetLastError());

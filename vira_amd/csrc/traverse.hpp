@@ -3,7 +3,8 @@
 // Replaces rtcIntersect1 (embree_tlas.ipp:128-173) and follows the order of
 // ViraBLAS::intersect (vira_blas.ipp:58-108): all child boxes of a node are
 // tested, the nearest child is entered and the others are pushed far-to-near.
-// Two node layouts: Node2 (binary, 64 B) and Node4 (four-wide, 128 B).
+// Three node layouts: Node2 (binary, 64 B), Node4 (four-wide, 128 B) and
+// Node4Q (four-wide with quantised boxes, 64 B; width code kWidthQ).
 // The stack's top 8 entries live in LDS (one column per lane, so a wave's
 // pushes and pops hit 64 consecutive dwords); deeper entries spill to private
 // memory.  Capacity: a binary tree of depth < 63 (the builder's bound) needs
@@ -145,12 +146,36 @@ __device__ __forceinline__ HitRec traverse_impl(const void* __restrict__ nodes_v
                     break;
                 }
             } else {
-                const Node4 nd = reinterpret_cast<const Node4*>(nodes_v)[cur];
-                uint32_t r0 = nd.ref.x, r1 = nd.ref.y, r2 = nd.ref.z, r3 = nd.ref.w;
-                float d0 = slab<SAFE>(nd.lox.x, nd.loy.x, nd.loz.x, nd.hix.x, nd.hiy.x, nd.hiz.x, o, inv, e, best);
-                float d1 = slab<SAFE>(nd.lox.y, nd.loy.y, nd.loz.y, nd.hix.y, nd.hiy.y, nd.hiz.y, o, inv, e, best);
-                float d2 = slab<SAFE>(nd.lox.z, nd.loy.z, nd.loz.z, nd.hix.z, nd.hiy.z, nd.hiz.z, o, inv, e, best);
-                float d3 = slab<SAFE>(nd.lox.w, nd.loy.w, nd.loz.w, nd.hix.w, nd.hiy.w, nd.hiz.w, o, inv, e, best);
+                // the four children's boxes: FP32 (Node4) or decoded from bytes (Node4Q)
+                float lx[4], ly[4], lz[4], hx[4], hy[4], hz[4];
+                uint32_t r0, r1, r2, r3;
+                if (W == kWidthQ) {
+                    const Node4Q nd = reinterpret_cast<const Node4Q*>(nodes_v)[cur];
+                    r0 = nd.ref.x; r1 = nd.ref.y; r2 = nd.ref.z; r3 = nd.ref.w;
+                    const float sx = q_scale(nd.ex, 0), sy = q_scale(nd.ex, 1), sz = q_scale(nd.ex, 2);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        lx[k] = q_decode(nd.qlo[0], k, sx, nd.ox);
+                        hx[k] = q_decode(nd.qhi[0], k, sx, nd.ox);
+                        ly[k] = q_decode(nd.qlo[1], k, sy, nd.oy);
+                        hy[k] = q_decode(nd.qhi[1], k, sy, nd.oy);
+                        lz[k] = q_decode(nd.qlo[2], k, sz, nd.oz);
+                        hz[k] = q_decode(nd.qhi[2], k, sz, nd.oz);
+                    }
+                } else {
+                    const Node4 nd = reinterpret_cast<const Node4*>(nodes_v)[cur];
+                    r0 = nd.ref.x; r1 = nd.ref.y; r2 = nd.ref.z; r3 = nd.ref.w;
+                    lx[0] = nd.lox.x; lx[1] = nd.lox.y; lx[2] = nd.lox.z; lx[3] = nd.lox.w;
+                    ly[0] = nd.loy.x; ly[1] = nd.loy.y; ly[2] = nd.loy.z; ly[3] = nd.loy.w;
+                    lz[0] = nd.loz.x; lz[1] = nd.loz.y; lz[2] = nd.loz.z; lz[3] = nd.loz.w;
+                    hx[0] = nd.hix.x; hx[1] = nd.hix.y; hx[2] = nd.hix.z; hx[3] = nd.hix.w;
+                    hy[0] = nd.hiy.x; hy[1] = nd.hiy.y; hy[2] = nd.hiy.z; hy[3] = nd.hiy.w;
+                    hz[0] = nd.hiz.x; hz[1] = nd.hiz.y; hz[2] = nd.hiz.z; hz[3] = nd.hiz.w;
+                }
+                float d0 = slab<SAFE>(lx[0], ly[0], lz[0], hx[0], hy[0], hz[0], o, inv, e, best);
+                float d1 = slab<SAFE>(lx[1], ly[1], lz[1], hx[1], hy[1], hz[1], o, inv, e, best);
+                float d2 = slab<SAFE>(lx[2], ly[2], lz[2], hx[2], hy[2], hz[2], o, inv, e, best);
+                float d3 = slab<SAFE>(lx[3], ly[3], lz[3], hx[3], hy[3], hz[3], o, inv, e, best);
                 d0 = r0 != kEmptyRef ? d0 : kInfF;
                 d1 = r1 != kEmptyRef ? d1 : kInfF;
                 d2 = r2 != kEmptyRef ? d2 : kInfF;

@@ -428,9 +428,11 @@ class Context:
         self._call(self.lib.vira_gpu_mesh_release, self.h, mid)
 
     def blas_build(self, mid: int, max_leaf: int = 4, bins: int = 16, width: int = 4, traversal_cost: float = 0.0,
-                   method: str = "auto"):
-        """method: "auto", "sah" (binned SAH) or "grid" (meshes made from a DEM grid, width 4)."""
-        o = capi.BVHOpts(max_leaf, bins, width, traversal_cost, {"auto": 0, "sah": 1, "grid": 2}[method])
+                   method: str = "auto", compress: bool = False):
+        """method: "auto", "sah" (binned SAH) or "grid" (meshes made from a DEM grid, width 4).
+        compress: also keep the 64 B quantised node copy (width 4)."""
+        o = capi.BVHOpts(max_leaf, bins, width, traversal_cost, {"auto": 0, "sah": 1, "grid": 2}[method],
+                         int(compress))
         self._call(self.lib.vira_gpu_blas_build, self.h, mid, C.byref(o))
         s = capi.BVHStats()
         self._call(self.lib.vira_gpu_blas_stats, self.h, mid, C.byref(s))
