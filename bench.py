@@ -200,6 +200,10 @@ def visits_leg(ctx, built, passes) -> dict:
         out[kind] = {"rays": n, "nodes_per_ray": round(v["nodes"] / n, 3), "leaves_per_ray": round(v["leaves"] / n, 3),
                      "triangles_per_ray": round(v["triangles"] / n, 3),
                      "spills_per_ray": round(v["stack_spills"] / n, 4), "max_stack": v["max_stack"]}
+    # camera, bounce and shadow rays share one traversal call: the node loop's SIMD use over both kinds
+    cv, sv = st["closest_visits"], st["shadow_visits"]
+    steps = cv["wave_node_steps"] + sv["wave_node_steps"]
+    out["node_loop_simd_use"] = round((cv["nodes"] + sv["nodes"]) / max(1, 64 * steps), 3)
     return out
 
 

@@ -187,6 +187,8 @@ typedef struct {
     uint64_t stack_spills;  /* pushes past the 8 stack entries kept in LDS (entry moved to private memory) */
     uint64_t tlas_nodes;    /* TLAS nodes whose box was tested (instanced scenes) */
     uint32_t max_stack;     /* deepest traversal stack of any ray */
+    uint64_t wave_node_steps; /* iterations of the node loop summed over the waves (a wave steps until its
+                                 last lane is done): SIMD use of the loop = nodes / (64 x this) */
 } vira_gpu_visit_counts;
 
 typedef struct {

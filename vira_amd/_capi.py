@@ -147,6 +147,7 @@ class VisitCounts(C.Structure):
         ("stack_spills", C.c_uint64),
         ("tlas_nodes", C.c_uint64),
         ("max_stack", C.c_uint32),
+        ("wave_node_steps", C.c_uint64),
     ]
 
 

@@ -409,6 +409,7 @@ __device__ __forceinline__ void render_tile(const TileJob& J, uint32_t tile, flo
                     to.tris += tc.tris;
                     to.spills += tc.spills;
                     to.tlas += tc.tlas;
+                    to.wsteps += tc.wsteps;
                     to.maxsp = to.maxsp > tc.maxsp ? to.maxsp : tc.maxsp;
                 };
                 if (mode == M_SHADOW) add(tcount[1]);

@@ -90,10 +90,10 @@ __global__ void __launch_bounds__(kFusedBS, kFusedWavesPerSimd) k_render_fused(T
     wave_add(&counters[3], n_hit);
 }
 
-// Counter words of the visit-counting launch: after the four ray counts, six
+// Counter words of the visit-counting launch: after the four ray counts, seven
 // per ray kind (camera and bounce rays, then shadow rays): nodes, leaves,
-// triangles, spilled pushes, deepest stack, TLAS nodes.
-enum : int { kVisitBase = 4, kVisitWords = 6 };
+// triangles, spilled pushes, deepest stack, TLAS nodes, wave node steps.
+enum : int { kVisitBase = 4, kVisitWords = 7 };
 
 // The same render with the traversals' visit counts (opt-in; one light or
 // none, no background, adaptive sampling or material maps).
@@ -121,6 +121,7 @@ __global__ void __launch_bounds__(kFusedBS, kFusedWavesPerSimd) k_render_fused_s
         wave_add64(&c[3], tc[k].spills);
         wave_max(&c[4], tc[k].maxsp);
         wave_add64(&c[5], tc[k].tlas);
+        wave_add64(&c[6], tc[k].wsteps);
     }
 }
 
@@ -579,6 +580,7 @@ void run_fused(Ctx& c, const vira_gpu_camera& cam, const vira_gpu_render_opts& o
                 vc[k]->stack_spills = h[3];
                 vc[k]->max_stack = (uint32_t)h[4];
                 vc[k]->tlas_nodes = h[5];
+                vc[k]->wave_node_steps = h[6];
             }
         }
         float f = 0.f;

@@ -42,6 +42,7 @@ struct TravCount {
     uint32_t spills;  // pushes that moved an entry from the LDS ring to private memory
     uint32_t maxsp;   // deepest stack
     uint32_t tlas;    // TLAS nodes whose box was tested (instanced scenes)
+    uint32_t wsteps;  // node-loop iterations of the wave (counted once per iteration, by its lowest active lane)
 };
 
 struct TravStack {
@@ -115,7 +116,10 @@ __device__ __forceinline__ HitRec traverse_impl(const void* __restrict__ nodes_v
     while (true) {
         // ---- descend through internal nodes until a leaf (or nothing) is reached
         while (!ref_is_leaf(cur)) {
-            if (STATS) tc->nodes++;
+            if (STATS) {
+                tc->nodes++;
+                if (__lane_id() == (unsigned)(__ffsll((long long)__ballot(1)) - 1)) tc->wsteps++;
+            }
             // Every child's slab test runs, and an empty child's result is then replaced by a
             // select: a test guarded by the child's reference made the compiler issue that
             // child's bound loads only after the references arrived (two memory round trips
