@@ -353,6 +353,9 @@ void orc_atan2_f32(const float* y, const float* x, size_t n, float* out) {
     for (size_t i = 0; i < n; ++i) out[i] = vgm::atan2_f32(y[i], x[i]);
 }
 
+// Test instrument: plant one known mistake in the oracle (0 = none).
+void orc_plant(int k) { vo::g_plant = k; }
+
 // sin and cos of the kernels' FP32 routine (fp_trig.hpp), for the tests that pin it.
 void orc_sincos_f32(const float* x, size_t n, float* s, float* c) {
     for (size_t i = 0; i < n; ++i) vgm::sincos_f32(x[i], s[i], c[i]);

@@ -9,6 +9,8 @@
 
 namespace vo {
 
+int g_plant = PLANT_NONE;
+
 // ============================================================== BVH build
 // ViraBLAS::subdivide / findBestSplitPlane (vira_blas.ipp:132-254).  Subtrees
 // below the top levels are built on worker threads into private node arrays
@@ -233,12 +235,14 @@ inline void sincos_r(double x, double& s, double& c) { s = std::sin(x); c = std:
 template <class R>
 inline R power_heuristic(R fPdf, R gPdf) {  // cpu_path_tracer.ipp:472-478
     R f = fPdf, g = gPdf;
+    if (g_plant == PLANT_MIS_BALANCE) return f / (f + g);
     return (f * f) / (f * f + g * g);
 }
 
 // Image::sampleUVs + interpolatePixel (image.ipp:589-670), grey texture.
 template <class R>
 inline R tex_bilinear(const Material& mat, R u, R v) {
+    if (g_plant == PLANT_TEX_TRANSPOSED) std::swap(u, v);
     R rx = (R)mat.tex_w, ry = (R)mat.tex_h;
     R px = std::fmod(u * rx, rx), py = std::fmod(v * ry, ry);
     if (px < 0) px += rx;
@@ -246,6 +250,7 @@ inline R tex_bilinear(const Material& mat, R u, R v) {
     if (!(px >= 0)) px = 0;  // non-finite uv: keep the fetch in range
     if (!(py >= 0)) py = 0;
     int x0 = (int)std::floor(px), y0 = (int)std::floor(py);
+    if (g_plant == PLANT_TEX_OFF_BY_ONE) x0 = x0 + 1;
     x0 = std::min(x0, mat.tex_w - 1);
     y0 = std::min(y0, mat.tex_h - 1);
     int x1 = std::min(x0 + 1, mat.tex_w - 1), y1 = std::min(y0 + 1, mat.tex_h - 1);
@@ -291,7 +296,7 @@ inline void background_radiance(const Options& opt, const V3<R>& d, R bg[NB]) {
         for (int k = 0; k < NB; ++k) bg[k] = 0;
         return;
     }
-    const R u = atan2_r(d.y, d.x) / ((R)2 * Consts<R>::pi);
+    const R u = (g_plant == PLANT_BACKGROUND_SWAP ? atan2_r(d.x, d.y) : atan2_r(d.y, d.x)) / ((R)2 * Consts<R>::pi);
     const R v = atan2_r(std::sqrt(d.x * d.x + d.y * d.y), d.z) / Consts<R>::pi;
     image_bilinear<R>(opt.background, opt.bg_w, opt.bg_h, NB, u, v, bg);
 }
@@ -333,13 +338,13 @@ inline bool sample_light(const Light& L, const V3<R>& p, uint32_t seed, uint32_t
     R cc = radius / distance;
     R x = cc * cc;
     R q = x / ((R)1 + std::sqrt((R)1 - x));
-    R t = zeta * q;
+    R t = g_plant == PLANT_CONE_SQUARE ? (zeta * zeta) * q : zeta * q;
     R cosT = (R)1 - t, sinT = std::sqrt(t * ((R)2 - t));
     R sinP, cosP;
     sincos_r(phi, sinP, cosP);
     R a = cosP * sinT, bb = cosT, cz = sinP * sinT;
     dir = {(a * U.x + bb * W.x) + cz * Vv.x, (a * U.y + bb * W.y) + cz * Vv.y, (a * U.z + bb * W.z) + cz * Vv.z};
-    R Omega = ((R)2 * Consts<R>::pi) * q;
+    R Omega = (g_plant == PLANT_CONE_OMEGA ? Consts<R>::pi : (R)2 * Consts<R>::pi) * q;
     pdf = (R)1 / Omega;
     for (int k = 0; k < NB; ++k) rad[k] = (R)L.value[k];
     return true;
@@ -366,7 +371,7 @@ inline R light_pdf(const Light& L, const V3<R>& p, const V3<R>& dir) {
     R cosMax = (R)1 - q;
     R ca = dot(dir, tn);
     if (ca < cosMax) return 0;
-    R Omega = ((R)2 * Consts<R>::pi) * q;
+    R Omega = (g_plant == PLANT_CONE_OMEGA ? Consts<R>::pi : (R)2 * Consts<R>::pi) * q;
     return (R)1 / Omega;
 }
 
@@ -402,6 +407,7 @@ inline bool cosine_sample(uint32_t seed, uint32_t pix, uint32_t s, uint32_t boun
     }
     R lz = std::sqrt(std::max((R)0, ((R)1 - dx * dx) - dy * dy));
     mpdf = lz * Consts<R>::inv_pi;
+    if (g_plant == PLANT_BOUNCE_PDF && mpdf > (R)0) mpdf = Consts<R>::inv_pi / (R)2;
     if (!(mpdf > (R)0)) return false;
     nd = V3<R>{(tg.x * dx + bt.x * dy) + N.x * lz, (tg.y * dx + bt.y * dy) + N.y * lz,
                (tg.z * dx + bt.z * dy) + N.z * lz};
@@ -454,7 +460,7 @@ Placement<R> rebase_instance(const double* M, const double c[3]) {
         for (int k = 0; k < 3; ++k) {
             P.l2w[r * 4 + k] = (R)A[r][k];
             P.w2l[r * 4 + k] = (R)inv[r][k];
-            P.nm[r * 3 + k] = (R)inv[k][r];
+            P.nm[r * 3 + k] = g_plant == PLANT_NORMAL_MATRIX ? (R)inv[r][k] : (R)inv[k][r];
         }
         P.l2w[r * 4 + 3] = (R)t[r];
         P.w2l[r * 4 + 3] = (R)(-((inv[r][0] * t[0] + inv[r][1] * t[1]) + inv[r][2] * t[2]));
@@ -604,9 +610,10 @@ void render_impl(const InstanceO* inst, int ninst, const Camera& cam_in, const L
                         if (mat.nmap) {
                             R nt[3];
                             image_bilinear<R>(mat.nmap, mat.nmap_w, mat.nmap_h, 3, uvu, uvv, nt);
-                            N = normalize(V3<R>{(tg.x * nt[0] + bt.x * nt[1]) + N_frame.x * nt[2],
-                                                (tg.y * nt[0] + bt.y * nt[1]) + N_frame.y * nt[2],
-                                                (tg.z * nt[0] + bt.z * nt[1]) + N_frame.z * nt[2]});
+                            const V3<R> mapped{(tg.x * nt[0] + bt.x * nt[1]) + N_frame.x * nt[2],
+                                               (tg.y * nt[0] + bt.y * nt[1]) + N_frame.y * nt[2],
+                                               (tg.z * nt[0] + bt.z * nt[1]) + N_frame.z * nt[2]};
+                            N = g_plant == PLANT_NMAP_UNNORMALISED ? mapped : normalize(mapped);
                         }
                         // albedo: vertex albedo x Material::getAlbedo (material.ipp:114-116)
                         R mvs[NB];
@@ -648,7 +655,9 @@ void render_impl(const InstanceO* inst, int ninst, const Camera& cam_in, const L
                         (void)V;
                         R vrad[NB] = {0};
                         if (bounce == 0 && opt.has_ambient) {  // cpu_path_tracer.ipp:389-396, material.ipp:67
-                            for (int k = 0; k < NB; ++k) vrad[k] = vrad[k] + beta[k] * ((R)opt.ambient[k] * alb[k]);
+                            for (int k = 0; k < NB; ++k)
+                                vrad[k] = vrad[k] + beta[k] * ((R)opt.ambient[k] *
+                                                               (g_plant == PLANT_AMBIENT_NO_ALBEDO ? (R)1 : alb[k]));
                         }
                         for (int l = 0; l < nl; ++l) {
                             V3<R> ld;
@@ -664,7 +673,7 @@ void render_impl(const InstanceO* inst, int ninst, const Camera& cam_in, const L
                             if (rec && bounce == 0) rec[3] |= 1u << l;
                             R ndl = dot(N, ld);
                             R mx = std::max(ndl, (R)0);
-                            R cth = std::max((R)0, dot(ld, N));
+                            R cth = g_plant == PLANT_DROP_COSINE ? (R)1 : std::max((R)0, dot(ld, N));
                             R mpdf = (ndl <= (R)0) ? (R)0 : ndl * Consts<R>::inv_pi;
                             R wgt = power_heuristic(lpdf, mpdf);
                             for (int k = 0; k < NB; ++k) {
@@ -699,7 +708,8 @@ void render_impl(const InstanceO* inst, int ninst, const Camera& cam_in, const L
                             R q = std::min((R)1, qmax);
                             R ur = (R)rng_u01(opt.seed, (uint32_t)pix, (uint32_t)s, db | SLOT_RR);
                             if (ur >= q) break;
-                            for (int k = 0; k < NB; ++k) beta[k] = beta[k] / q;
+                            if (g_plant != PLANT_RR_NO_REWEIGHT)
+                                for (int k = 0; k < NB; ++k) beta[k] = beta[k] / q;
                         }
                         ray = RayT<R>{org, nd};
                         cnt[2]++;
@@ -726,7 +736,7 @@ void render_impl(const InstanceO* inst, int ninst, const Camera& cam_in, const L
                             }
                         }
                         if (stop) {
-                            nsamp = s;
+                            nsamp = g_plant == PLANT_ADAPTIVE_DIVISOR ? s + 1 : s;
                             break;
                         }
                     }

@@ -595,10 +595,36 @@ struct InstanceO {
     double M[16];
 };
 
+// Test instrument: g_plant != 0 plants one known mistake in the oracle, so
+// that the tests can show that each pin fails on it (tests/test_oracle_plants.py).
+// 0 everywhere else; set only from the tests, before a render starts.
+enum : int {
+    PLANT_NONE = 0,
+    PLANT_MIS_BALANCE = 1,        // balance heuristic instead of the power heuristic
+    PLANT_DROP_COSINE = 2,        // the light term loses cos(theta) at the surface
+    PLANT_CONE_OMEGA = 3,         // sphere-light solid angle pi (1 - cos) instead of 2 pi (1 - cos)
+    PLANT_CONE_SQUARE = 4,        // cone samples with 1 - cos = zeta^2 q (not uniform in cos)
+    PLANT_BOUNCE_PDF = 5,         // cosine-sampled bounce weighted with the uniform pdf 1/(2 pi)
+    PLANT_RR_NO_REWEIGHT = 6,     // a path that survives Russian roulette is not divided by q
+    PLANT_TEX_OFF_BY_ONE = 7,     // bilinear lookup one texel to the right
+    PLANT_TEX_TRANSPOSED = 8,     // bilinear lookup with u and v swapped
+    PLANT_KAHAN_OFF = 9,          // Kahan accumulation without the compensation
+    PLANT_NORMAL_MATRIX = 10,     // instances: normal matrix not transposed
+    PLANT_BACKGROUND_SWAP = 11,   // background u = atan2(x, y) / 2 pi
+    PLANT_ADAPTIVE_DIVISOR = 12,  // adaptive stop at sample s divides by s + 1
+    PLANT_NMAP_UNNORMALISED = 13, // normal-mapped normal not renormalised
+    PLANT_AMBIENT_NO_ALBEDO = 14, // ambient light not multiplied by the albedo
+};
+extern int g_plant;
+
 template <class R>
 struct Kahan {
     R s = 0, c = 0;
     void add(R x) {
+        if (g_plant == PLANT_KAHAN_OFF) {
+            s = s + x;
+            return;
+        }
         R y = x - c;
         R t = s + y;
         c = (t - s) - y;

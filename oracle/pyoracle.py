@@ -99,6 +99,7 @@ def lib() -> C.CDLL:
                                     P, P, P]),
         "orc_mesh_set_id": (None, [P, C.c_uint32]),
         "orc_atan2_f32": (None, [P, P, C.c_size_t, P]),
+        "orc_plant": (None, [C.c_int]),
         "orc_pixel_solid_angles": (None, [P, C.c_float, C.c_int, C.c_int, P]),
         "orc_psf_kernel": (None, [C.POINTER(PSF), C.c_int, P]),
         "orc_convolve": (None, [P, C.c_int, C.c_int, P, C.c_int, P]),
@@ -282,6 +283,17 @@ class Mesh:
                          "closest_internal": int(cnt[3]), "closest_tris": int(cnt[4]),
                          "any_internal": int(cnt[5]), "any_tris": int(cnt[6])}
         return out
+
+
+# Planted mistakes (the oracle's PLANT_* codes): test instrument for showing that each pin catches one.
+PLANTS = {"mis_balance": 1, "drop_cosine": 2, "cone_omega": 3, "cone_square": 4, "bounce_pdf": 5,
+          "rr_no_reweight": 6, "tex_off_by_one": 7, "tex_transposed": 8, "kahan_off": 9, "normal_matrix": 10,
+          "background_swap": 11, "adaptive_divisor": 12, "nmap_unnormalised": 13, "ambient_no_albedo": 14}
+
+
+def plant(name: str | None):
+    """Plants the named mistake in the oracle (None removes it)."""
+    lib().orc_plant(C.c_int(0 if name is None else PLANTS[name]))
 
 
 def atan2_f32(y, x):
