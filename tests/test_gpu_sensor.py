@@ -1,6 +1,7 @@
 """Received power (Camera::calculateReceivedPower, camera.ipp:1175-1178) and
 Monte Carlo acceptance of the path tracer against the FP64 oracle."""
 import numpy as np
+from scipy import stats
 import pytest
 
 import pyoracle as O
@@ -244,8 +245,10 @@ def test_sensor_filter_mosaic_bit_exact(gpu_ctx):
 def test_sensor_noise_statistics(gpu_ctx):
     """Poisson photon noise (multiplication method below 10, PTRS above), dark
     current (Poisson) and readout noise (normal): sample means and variances
-    over 2^17 pixels within 5 standard errors of the parameters, up to means of
-    1e6 photons (where an FP32 acceptance test would inflate the variance by ~4 %)."""
+    over 2^17 pixels within 5 standard errors of the parameters, and the photon
+    counts' histogram against scipy's Poisson cdf, up to means of 1e6 photons
+    (where an acceptance test that summed -lambda + k log(lambda) - lgamma(k + 1)
+    in FP32 would inflate the variance by ~4 %)."""
     import torch
     n = 1 << 17
     c0 = SN.SensorConfig()
@@ -261,6 +264,16 @@ def test_sensor_noise_statistics(gpu_ctx):
         assert np.all(ph[:, k] == np.round(ph[:, k])) and ph[:, k].min() >= 0
         assert abs(m - lam[k]) <= 5 * np.sqrt(lam[k] / n), (k, m, lam[k])
         assert abs(v - lam[k]) <= 5 * lam[k] * np.sqrt(2.0 / n) + 1e-3, (k, v, lam[k])
+        # the shape: Pearson chi-square over ~24 bins of near-equal Poisson
+        # probability (scipy's cdf), tails included; a distorted acceptance test
+        # in PTRS or a wrong product in the multiplication method moves mass
+        # between bins long before it moves the first two moments
+        edges = np.unique(stats.poisson.ppf(np.linspace(0.0, 1.0, 25)[1:-1], lam[k]))
+        cdf = stats.poisson.cdf(edges, lam[k])
+        prob = np.diff(np.concatenate([[0.0], cdf, [1.0]]))
+        obs = np.bincount(np.searchsorted(edges, ph[:, k], side="left"), minlength=prob.size)
+        chi2 = float(((obs - n * prob) ** 2 / (n * prob)).sum())
+        assert stats.chi2.sf(chi2, prob.size - 1) > 1e-5, (k, lam[k], chi2, prob.size)
     # dark current + readout noise, no light: image = gain (fp x (dark t + N(mu, sd))) / max_adu
     cfg = SN.SensorConfig(exposure_time=2.0, noise_model=True, dark_current=7.0, readout_noise_mean=20.0,
                           readout_noise_std=3.0, gain=1.0, bit_depth=16, seed=9)

@@ -273,9 +273,24 @@ constexpr uint32_t kDimPhoton = 64, kDimDark = 80, kDimReadout = 96;
 
 // Poisson(lambda): multiplication method below 10, PTRS (Hoermann 1993,
 // "The transformed rejection method for generating Poisson random
-// variables") at and above, in FP64 as std::poisson_distribution works
-// (camera.ipp:1306-1316); at most 64 proposals (the mean is returned in the
-// vanishingly rare case that none is accepted).
+// variables") at and above (camera.ipp:1306-1316 draws from
+// std::poisson_distribution); at most 64 proposals (the mean is returned in
+// the vanishingly rare case that none is accepted).
+//
+// The acceptance test compares log(V alpha / (a / us^2 + b)) with the
+// log-probability -lambda + k log(lambda) - lgamma(k + 1), whose three terms
+// reach ~1e7 at large means: summed in FP32 they keep only ~1 of their value.
+// For k >= 10 it is evaluated in the Stirling form
+//   -D - log(2 pi k) / 2 - 1 / (12 k) + 1 / (360 k^3)
+// (the series' next term is below 1e-9 for k >= 10), with the deviance
+//   D = k log1p(x) - (k - lambda) = lambda phi(x),  x = (k - lambda) / lambda,
+//   phi(x) = (1 + x) log1p(x) - x = sum_{n>=2} (-1)^n x^n / (n (n - 1)),
+// k - lambda exact (two floats within a factor of 2) and phi taken from its
+// series to x^9 for |x| < 0.1 (relative truncation below 1e-9), where the
+// closed form would cancel; FP32 then holds the log-probability to ~2e-6
+// within 3 standard deviations of any mean from 10 to 1e7 and ~1e-4 beyond.
+// Below 10 the terms are small and lgammaf is used.  The proposal's floor
+// takes lambda in FP64, where FP32's spacing (1/16 at 1e6) would move it.
 __device__ __forceinline__ float poisson(float lambda, uint32_t seed, uint32_t pix, uint32_t dim) {
     if (!(lambda > 0.0f)) return 0.0f;
     if (lambda < 10.0f) {
@@ -287,23 +302,43 @@ __device__ __forceinline__ float poisson(float lambda, uint32_t seed, uint32_t p
         }
         return lambda;
     }
-    // PTRS in FP64: at large means the log-acceptance terms reach ~1e7, where
-    // FP32's spacing (~1) would distort the accepted distribution
-    const double lam = (double)lambda;
-    const double slam = sqrt(lam), loglam = log(lam);
-    const double b = 0.931 + 2.53 * slam;
-    const double a = -0.059 + 0.02483 * b;
-    const double invalpha = 1.1239 + 1.1328 / (b - 3.4);
-    const double vr = 0.9277 - 3.6224 / (b - 2.0);
+    const float slam = sqrtf(lambda);
+    const float b = 0.931f + 2.53f * slam;
+    const float a = -0.059f + 0.02483f * b;
+    const float vr = 0.9277f - __fdiv_rn(3.6224f, b - 2.0f);
     for (uint32_t it = 0; it < 64u; ++it) {
-        const double U = (double)rng_u01(seed, pix, 2u * it, dim) - 0.5;
-        const double V = (double)rng_u01(seed, pix, 2u * it + 1u, dim);
-        const double us = 0.5 - fabs(U);
-        const double k = floor((2.0 * a / us + b) * U + lam + 0.43);
-        if (us >= 0.07 && V <= vr) return (float)k;
-        if (k < 0.0 || (us < 0.013 && V > us)) continue;
-        if ((log(V) + log(invalpha) - log(a / (us * us) + b)) <= (-lam + k * loglam - lgamma(k + 1.0)))
-            return (float)k;
+        const float U = rng_u01(seed, pix, 2u * it, dim) - 0.5f;
+        const float V = rng_u01(seed, pix, 2u * it + 1u, dim);
+        const float us = 0.5f - fabsf(U);
+        const float k = (float)floor((double)((__fdiv_rn(2.0f * a, us) + b) * U) + (double)lambda + 0.43);
+        if (us >= 0.07f && V <= vr) return k;
+        if (k < 0.0f || (us < 0.013f && V > us)) continue;
+        const float invalpha = 1.1239f + __fdiv_rn(1.1328f, b - 3.4f);
+        const float lhs = (logf(V) + logf(invalpha)) - logf(__fdiv_rn(a, us * us) + b);
+        float rhs;
+        if (k < 10.0f) {
+            rhs = (-lambda + k * logf(lambda)) - lgammaf(k + 1.0f);
+        } else {
+            const float dk = k - lambda;
+            const float x = __fdiv_rn(dk, lambda);
+            float D;
+            if (fabsf(x) < 0.1f) {
+                float p = -1.0f / 72.0f;
+                p = fmaf(p, x, 1.0f / 56.0f);
+                p = fmaf(p, x, -1.0f / 42.0f);
+                p = fmaf(p, x, 1.0f / 30.0f);
+                p = fmaf(p, x, -1.0f / 20.0f);
+                p = fmaf(p, x, 1.0f / 12.0f);
+                p = fmaf(p, x, -1.0f / 6.0f);
+                p = fmaf(p, x, 0.5f);
+                D = lambda * ((x * x) * p);
+            } else {
+                D = k * log1pf(x) - dk;
+            }
+            const float ik = __fdiv_rn(1.0f, k);
+            rhs = ((-D - 0.5f * logf(6.283185307179586f * k)) - ik * (1.0f / 12.0f)) + (ik * ik * ik) * (1.0f / 360.0f);
+        }
+        if (lhs <= rhs) return k;
     }
     return lambda;
 }
