@@ -9,8 +9,10 @@
 //     the GPU (SAH) and the host reads back only each node's decision (104 B)
 //     to lay out the next level; a stable chunked partition (block scan)
 //     moves the triangle ids.  Every step is deterministic.
-//   * small nodes are finished by one thread each (binned SAH, in-place
-//     partition), the per-thread "subdivide" of the reference.
+//   * small nodes are finished by one wave each from LDS: the wave splits
+//     nodes of more than 32 triangles together, then every lane splits the
+//     smaller ones alone (binned SAH, in-place partition), the "subdivide"
+//     of the reference.
 //   * the binary tree is then rewritten into 64-byte nodes that hold both
 //     children's boxes, and the triangles into leaf order (48 B each).
 #include <algorithm>
@@ -394,184 +396,486 @@ __global__ void k_chunk_bases(const Chunk* __restrict__ chunks, const TaskParam*
     bases[ci] = ChunkBase{tp.begin + lrun, tp.begin + sw[ch.task].nleft + rrun};
 }
 
-// One thread builds the whole subtree of a small node (the per-thread
-// "subdivide" of vira_blas.ipp:132-182).  Its nodes go to the task's own
-// scratch region (at most 2 (count - 1) of them, numbered from 0 within the
-// task), so no counter is shared between threads; k_compact_small then moves
-// every task's nodes behind the large-level nodes.  The smaller child is built
-// first and the larger one stacked, so the stack never holds more than
-// log2(count) + 1 entries.
+// Small subtrees (nodes of at most kSmall triangles; the reference's
+// recursive "subdivide", vira_blas.ipp:132-182), one wave per subtree.  The
+// wave stages its triangles' boxes in LDS once and works from LDS only:
+//   * a node of more than kCoop triangles is split by the whole wave: lane
+//     g * 8 + k sums the triangles of group g (items g, g + 8, ...) that fall
+//     into bin k, the 8 groups are merged across lanes, every lane sweeps the
+//     8 bins with the same arithmetic as a serial build, and a stable ballot
+//     partition reorders the node's items; the children's centroid bounds
+//     come from the bins;
+//   * smaller nodes are queued in LDS, and in each round every lane takes one
+//     queued node and splits it alone (bins in registers, the reference's
+//     in-place partition).
+// Child pairs are numbered by a prefix sum over the splitting lanes, so the
+// nodes of a task fill [0, used) of its scratch region without gaps, and
+// k_compact_small moves them behind the large-level nodes.  Every decision is
+// an exact function of a node's triangle set (integer counts, min/max bounds,
+// one fixed sweep), so the tree is the same as a serial build's whatever the
+// order of the items inside a node.
 constexpr uint32_t kTaskRoot = 0xffffffffu;
-constexpr int kSmallStack = 12;  // > log2(kSmall) + 1
+constexpr uint32_t kCoop = 32;              // nodes above this are split by the whole wave
+constexpr uint32_t kQueueCap = kSmall / 2;  // queued nodes hold >= 2 triangles each, disjoint
+constexpr int kCoopStack = 16;              // > log2(kSmall / kCoop) + 1 (smaller child first)
+static_assert(kCoop > 15, "wave-split nodes are never leaves (max_leaf <= 15)");
+static_assert(kSmallBins == 8, "the wave-split binning puts one bin on each lane of a group of 8");
 
-__global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, const uint32_t* order, uint32_t ntasks, uint32_t* prims, const PrimBox* pb,
-                              BNodeG* nodes, BNodeG* snodes, uint32_t* used, uint32_t* max_depth, uint32_t* err,
-                              int max_leaf, float trav_cost) {
-    // each thread's stack in its own LDS column (not in scratch memory)
-    __shared__ uint4 s_stack[kSmallStack][64];
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t md = 0;
-    if (gid < ntasks) {
-        // threads take the tasks largest first, so a wave's lanes build
-        // subtrees of about the same size; results stay indexed by task
-        const uint32_t ti = order[gid];
-        const SmallTask task = tasks[ti];
-        BNodeG* local = snodes + task.base;
-        struct Ent {
-            uint32_t node, begin, count, depth;
-        };
-        int sp = 0;
-        uint32_t next = 0;  // scratch nodes used
-        Ent e{kTaskRoot, task.begin, task.count, task.depth};
-        bool have = true;
-        while (have) {
-            md = max(md, e.depth);
-            BNodeG* slot = e.node == kTaskRoot ? &nodes[task.node] : &local[e.node];
-            BNodeG nd = *slot;
-            uint32_t n = e.count;
-            bool leaf = n <= 1;
-            int axis = 0, split = -1;
-            float cmin = 0.f, scale = 0.f;
-            float blo[kSmallBins][3], bhi[kSmallBins][3];
-            uint32_t bcnt[kSmallBins];
-            if (!leaf) {
-                float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
-                for (uint32_t i = 0; i < n; ++i) {
-                    const PrimBox p = pb[prims[e.begin + i]];
-                    for (int a = 0; a < 3; ++a) {
-                        const float c = centroid(p, a);
-                        clo[a] = fminf(clo[a], c);
-                        chi[a] = fmaxf(chi[a], c);
-                    }
+struct QEnt {  // queued node: slot (kTaskRoot or a scratch index), first item, items, depth, box
+    uint32_t slot, b, n, depth;
+    float lo[3], hi[3];
+};
+struct CEnt {  // wave-split node: the same plus its centroid bounds
+    uint32_t slot, b, n, depth;
+    float lo[3], hi[3], clo[3], chi[3];
+};
+
+// c ? a : b, field by field (a select of whole structs would go through scratch memory)
+__device__ __forceinline__ CEnt pick_ent(bool c, const CEnt& a, const CEnt& b) {
+    CEnt r;
+    r.slot = c ? a.slot : b.slot;
+    r.b = c ? a.b : b.b;
+    r.n = c ? a.n : b.n;
+    r.depth = c ? a.depth : b.depth;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        r.lo[k] = c ? a.lo[k] : b.lo[k];
+        r.hi[k] = c ? a.hi[k] : b.hi[k];
+        r.clo[k] = c ? a.clo[k] : b.clo[k];
+        r.chi[k] = c ? a.chi[k] : b.chi[k];
+    }
+    return r;
+}
+
+// number of set bits of m below this lane
+__device__ __forceinline__ uint32_t rank_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Centroid bounds of items [b, b + n) of the wave's list (every lane gets them).
+__device__ __forceinline__ void wave_centroid_bounds(const uint32_t* s_perm, const float4* s_lo,
+                                                     const float4* s_hi, uint32_t b, uint32_t n, uint32_t lane,
+                                                     float clo[3], float chi[3]) {
+    float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = lane; i < n; i += 64) {
+        const uint32_t j = s_perm[b + i];
+        const float4 lo = s_lo[j], hi = s_hi[j];
+        const float c[3] = {0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            l[k] = fminf(l[k], c[k]);
+            h[k] = fmaxf(h[k], c[k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        for (int off = 32; off > 0; off >>= 1) {
+            l[k] = fminf(l[k], __shfl_xor(l[k], off));
+            h[k] = fmaxf(h[k], __shfl_xor(h[k], off));
+        }
+        clo[k] = l[k];
+        chi[k] = h[k];
+    }
+}
+
+// SAH sweep over 8 bins (vira_blas.ipp:185-254 with bins): right-to-left
+// areas, then the cheapest left-to-right boundary; split = -1 when no
+// boundary has triangles on both sides.
+__device__ __forceinline__ void sweep8(const uint32_t bcnt[kSmallBins], const float blo[kSmallBins][3],
+                                       const float bhi[kSmallBins][3], int& split, float& best) {
+    float rarea[kSmallBins];
+    uint32_t rcnt[kSmallBins];
+    float rl[3] = {INFINITY, INFINITY, INFINITY}, rh[3] = {-INFINITY, -INFINITY, -INFINITY};
+    uint32_t rc = 0;
+#pragma unroll
+    for (int b = kSmallBins - 1; b > 0; --b) {
+        rc += bcnt[b];
+        for (int k = 0; k < 3; ++k) { rl[k] = fminf(rl[k], blo[b][k]); rh[k] = fmaxf(rh[k], bhi[b][k]); }
+        rarea[b] = rc ? half_area(rl, rh) : 0.f;
+        rcnt[b] = rc;
+    }
+    float ll[3] = {INFINITY, INFINITY, INFINITY}, lh[3] = {-INFINITY, -INFINITY, -INFINITY};
+    uint32_t lc = 0;
+    best = INFINITY;
+    split = -1;
+#pragma unroll
+    for (int b = 0; b < kSmallBins - 1; ++b) {
+        lc += bcnt[b];
+        for (int k = 0; k < 3; ++k) { ll[k] = fminf(ll[k], blo[b][k]); lh[k] = fmaxf(lh[k], bhi[b][k]); }
+        if (lc == 0 || rcnt[b + 1] == 0) continue;
+        const float cost = (float)lc * half_area(ll, lh) + (float)rcnt[b + 1] * rarea[b + 1];
+        if (cost < best) { best = cost; split = b; }
+    }
+}
+
+// The children's boxes (and counts) of a binned split: the union of the bins
+// on each side.
+__device__ __forceinline__ void split_boxes8(const uint32_t bcnt[kSmallBins], const float blo[kSmallBins][3],
+                                             const float bhi[kSmallBins][3], int split, uint32_t& nl, float Llo[3],
+                                             float Lhi[3], float Rlo[3], float Rhi[3]) {
+    nl = 0;
+    for (int k = 0; k < 3; ++k) { Llo[k] = Rlo[k] = INFINITY; Lhi[k] = Rhi[k] = -INFINITY; }
+#pragma unroll
+    for (int b = 0; b < kSmallBins; ++b) {
+        const bool left = b <= split;
+        nl += left ? bcnt[b] : 0u;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            Llo[k] = left ? fminf(Llo[k], blo[b][k]) : Llo[k];
+            Lhi[k] = left ? fmaxf(Lhi[k], bhi[b][k]) : Lhi[k];
+            Rlo[k] = left ? Rlo[k] : fminf(Rlo[k], blo[b][k]);
+            Rhi[k] = left ? Rhi[k] : fmaxf(Rhi[k], bhi[b][k]);
+        }
+    }
+}
+
+// One lane decides and splits queued node q alone (items in LDS).  Returns
+// false when q stays a leaf; else nl and the children's boxes are set and the
+// node's items are partitioned in place (vira_blas.ipp:149-159).
+__device__ __forceinline__ bool lane_split(const QEnt& q, uint32_t* s_perm, const float4* s_lo, const float4* s_hi,
+                                           int max_leaf, float trav_cost, uint32_t& nl, float Llo[3], float Lhi[3],
+                                           float Rlo[3], float Rhi[3]) {
+    const uint32_t n = q.n;
+    float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t j = s_perm[q.b + i];
+        const float4 lo = s_lo[j], hi = s_hi[j];
+        const float c[3] = {0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            clo[k] = fminf(clo[k], c[k]);
+            chi[k] = fmaxf(chi[k], c[k]);
+        }
+    }
+    const float ext[3] = {chi[0] - clo[0], chi[1] - clo[1], chi[2] - clo[2]};
+    int axis = (ext[1] > ext[0]) ? 1 : 0;
+    if (ext[2] > pick3(ext, axis)) axis = 2;
+    const float cmin = pick3(clo, axis);
+    const float eaxis = pick3(ext, axis);
+    const float scale = (float)kSmallBins / eaxis;
+    int split = -1;
+    if (!(eaxis > 0.f) || !isfinite(scale) || q.depth >= kMedianDepth) {
+        if (n <= (uint32_t)max_leaf) return false;
+    } else {
+        uint32_t bcnt[kSmallBins];
+        float blo[kSmallBins][3], bhi[kSmallBins][3];
+#pragma unroll
+        for (int b = 0; b < kSmallBins; ++b) {
+            bcnt[b] = 0;
+            blo[b][0] = blo[b][1] = blo[b][2] = INFINITY;
+            bhi[b][0] = bhi[b][1] = bhi[b][2] = -INFINITY;
+        }
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t j = s_perm[q.b + i];
+            const float4 lo = s_lo[j], hi = s_hi[j];
+            const float c = axis == 0 ? 0.5f * (lo.x + hi.x) : (axis == 1 ? 0.5f * (lo.y + hi.y) : 0.5f * (lo.z + hi.z));
+            const int bi = bin_of(c, cmin, scale, kSmallBins);
+            const float pl[3] = {lo.x, lo.y, lo.z}, ph[3] = {hi.x, hi.y, hi.z};
+#pragma unroll
+            for (int b = 0; b < kSmallBins; ++b) {
+                const bool m = bi == b;
+                bcnt[b] += m ? 1u : 0u;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    blo[b][a] = m ? fminf(blo[b][a], pl[a]) : blo[b][a];
+                    bhi[b][a] = m ? fmaxf(bhi[b][a], ph[a]) : bhi[b][a];
                 }
-                float ext[3] = {chi[0] - clo[0], chi[1] - clo[1], chi[2] - clo[2]};
-                axis = (ext[1] > ext[0]) ? 1 : 0;
-                if (ext[2] > pick3(ext, axis)) axis = 2;
-                cmin = pick3(clo, axis);
-                const float eaxis = pick3(ext, axis);
-                scale = (float)kSmallBins / eaxis;
-                bool degenerate = !(eaxis > 0.f) || !isfinite(scale) || e.depth >= kMedianDepth;
-                if (degenerate) {
-                    leaf = n <= (uint32_t)max_leaf;
-                } else {
-                    // the bins stay in registers: every bin is updated under a compare with its
-                    // compile-time index instead of being indexed by the run-time bin
-#pragma unroll
-                    for (int b = 0; b < kSmallBins; ++b) {
-                        bcnt[b] = 0;
-                        blo[b][0] = blo[b][1] = blo[b][2] = INFINITY;
-                        bhi[b][0] = bhi[b][1] = bhi[b][2] = -INFINITY;
-                    }
-                    for (uint32_t i = 0; i < n; ++i) {
-                        const PrimBox p = pb[prims[e.begin + i]];
-                        const int bi = bin_of(centroid_on(p, axis), cmin, scale, kSmallBins);
-#pragma unroll
-                        for (int b = 0; b < kSmallBins; ++b) {
-                            const bool m = bi == b;
-                            bcnt[b] += m ? 1u : 0u;
-#pragma unroll
-                            for (int a = 0; a < 3; ++a) {
-                                blo[b][a] = m ? fminf(blo[b][a], p.lo[a]) : blo[b][a];
-                                bhi[b][a] = m ? fmaxf(bhi[b][a], p.hi[a]) : bhi[b][a];
-                            }
-                        }
-                    }
-                    // SAH sweep: right-to-left areas, then left-to-right costs
-                    float rarea[kSmallBins];
-                    uint32_t rcnt[kSmallBins];
-                    float rl[3] = {INFINITY, INFINITY, INFINITY}, rh[3] = {-INFINITY, -INFINITY, -INFINITY};
-                    uint32_t rc = 0;
-#pragma unroll
-                    for (int b = kSmallBins - 1; b > 0; --b) {
-                        rc += bcnt[b];
-                        for (int k = 0; k < 3; ++k) { rl[k] = fminf(rl[k], blo[b][k]); rh[k] = fmaxf(rh[k], bhi[b][k]); }
-                        rarea[b] = rc ? half_area(rl, rh) : 0.f;
-                        rcnt[b] = rc;
-                    }
-                    float ll[3] = {INFINITY, INFINITY, INFINITY}, lh[3] = {-INFINITY, -INFINITY, -INFINITY};
-                    uint32_t lc = 0;
-                    float best = INFINITY;
-#pragma unroll
-                    for (int b = 0; b < kSmallBins - 1; ++b) {
-                        lc += bcnt[b];
-                        for (int k = 0; k < 3; ++k) { ll[k] = fminf(ll[k], blo[b][k]); lh[k] = fmaxf(lh[k], bhi[b][k]); }
-                        if (lc == 0 || rcnt[b + 1] == 0) continue;
-                        float cost = (float)lc * half_area(ll, lh) + (float)rcnt[b + 1] * rarea[b + 1];
-                        if (cost < best) { best = cost; split = b; }
-                    }
-                    float A = half_area(nd.lo, nd.hi);
-                    float leaf_cost = (float)n * A;
-                    if (split < 0) leaf = n <= (uint32_t)max_leaf;
-                    else if (n <= (uint32_t)max_leaf && leaf_cost <= trav_cost * A + best) leaf = true;
-                }
-            }
-            if (leaf) {
-                nd.left = e.begin;
-                nd.count = n;
-                *slot = nd;
-            } else {
-                uint32_t nl;
-                BNodeG L, R;
-                if (split >= 0) {
-                    // in-place partition (vira_blas.ipp:149-159)
-                    int64_t i = e.begin, j = (int64_t)e.begin + n - 1;
-                    while (i <= j) {
-                        float cc = centroid_on(pb[prims[i]], axis);
-                        if (bin_of(cc, cmin, scale, kSmallBins) <= split) {
-                            i++;
-                        } else {
-                            uint32_t t = prims[i];
-                            prims[i] = prims[j];
-                            prims[j] = t;
-                            j--;
-                        }
-                    }
-                    nl = (uint32_t)(i - e.begin);
-                    for (int k = 0; k < 3; ++k) { L.lo[k] = INFINITY; L.hi[k] = -INFINITY; R.lo[k] = INFINITY; R.hi[k] = -INFINITY; }
-#pragma unroll
-                    for (int b = 0; b < kSmallBins; ++b) {
-                        const bool left = b <= split;
-#pragma unroll
-                        for (int k = 0; k < 3; ++k) {
-                            L.lo[k] = left ? fminf(L.lo[k], blo[b][k]) : L.lo[k];
-                            L.hi[k] = left ? fmaxf(L.hi[k], bhi[b][k]) : L.hi[k];
-                            R.lo[k] = left ? R.lo[k] : fminf(R.lo[k], blo[b][k]);
-                            R.hi[k] = left ? R.hi[k] : fmaxf(R.hi[k], bhi[b][k]);
-                        }
-                    }
-                } else {
-                    // object median; children keep the parent box (conservative)
-                    nl = n / 2;
-                    for (int k = 0; k < 3; ++k) { L.lo[k] = R.lo[k] = nd.lo[k]; L.hi[k] = R.hi[k] = nd.hi[k]; }
-                }
-                const uint32_t c = next;  // at most 2 (count - 1) children in a binary tree of count leaves
-                next += 2;
-                L.left = e.begin; L.count = nl;
-                R.left = e.begin + nl; R.count = n - nl;
-                local[c] = L;
-                local[c + 1] = R;
-                nd.left = c;  // task-local until k_compact_small
-                nd.count = 0;
-                *slot = nd;
-                const Ent le{c, e.begin, nl, e.depth + 1}, re{c + 1, e.begin + nl, n - nl, e.depth + 1};
-                const bool left_small = nl <= n - nl;
-                const Ent& big = left_small ? re : le;
-                if (sp < kSmallStack) s_stack[sp++][threadIdx.x] = make_uint4(big.node, big.begin, big.count, big.depth);
-                else atomicOr(err, 2u);  // cannot happen: the stacked child is never the smaller one
-                e = left_small ? le : re;
-                continue;
-            }
-            have = sp > 0;
-            if (have) {
-                const uint4 t = s_stack[--sp][threadIdx.x];
-                e = Ent{t.x, t.y, t.z, t.w};
             }
         }
-        used[ti] = next;
+        float best;
+        sweep8(bcnt, blo, bhi, split, best);
+        const float A = half_area(q.lo, q.hi);
+        const float leaf_cost = (float)n * A;
+        if (split < 0) {
+            if (n <= (uint32_t)max_leaf) return false;
+        } else if (n <= (uint32_t)max_leaf && leaf_cost <= trav_cost * A + best) {
+            return false;
+        }
+        if (split >= 0) {
+            uint32_t i = q.b, j = q.b + n - 1;  // n >= 2: j >= i
+            while (i <= j) {
+                const uint32_t pi = s_perm[i];
+                const float4 lo = s_lo[pi], hi = s_hi[pi];
+                const float c =
+                    axis == 0 ? 0.5f * (lo.x + hi.x) : (axis == 1 ? 0.5f * (lo.y + hi.y) : 0.5f * (lo.z + hi.z));
+                if (bin_of(c, cmin, scale, kSmallBins) <= split) {
+                    i++;
+                } else {
+                    s_perm[i] = s_perm[j];
+                    s_perm[j] = pi;
+                    if (j == q.b) break;  // everything went right (cannot happen with a valid split)
+                    j--;
+                }
+            }
+            split_boxes8(bcnt, blo, bhi, split, nl, Llo, Lhi, Rlo, Rhi);
+            return true;
+        }
     }
-    // one atomic per wave for the depth
+    // object median; the children keep the parent box (conservative)
+    nl = n / 2;
+    for (int k = 0; k < 3; ++k) { Llo[k] = Rlo[k] = q.lo[k]; Lhi[k] = Rhi[k] = q.hi[k]; }
+    return true;
+}
+
+__global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, const uint32_t* order, uint32_t ntasks,
+                                                    uint32_t* prims, const PrimBox* pb, BNodeG* nodes, BNodeG* snodes,
+                                                    uint32_t* used, uint32_t* max_depth, uint32_t* err, int max_leaf,
+                                                    float trav_cost) {
+    __shared__ float4 s_lo[kSmall], s_hi[kSmall];
+    __shared__ uint32_t s_id[kSmall], s_perm[kSmall], s_tmp[kSmall];
+    __shared__ QEnt s_q[kQueueCap];
+    __shared__ CEnt s_stack[kCoopStack];
+    __shared__ float s_bins[kSmallBins][kBinVals];
+    const uint32_t lane = threadIdx.x;
+    if (blockIdx.x >= ntasks) return;
+    // blocks take the tasks largest first; results stay indexed by task
+    const uint32_t ti = order[blockIdx.x];
+    const SmallTask task = tasks[ti];
+    BNodeG* const local = snodes + task.base;
+    const uint32_t n0 = task.count;
+    for (uint32_t i = lane; i < n0; i += 64) {
+        const uint32_t id = prims[task.begin + i];
+        const PrimBox p = pb[id];
+        s_id[i] = id;
+        s_perm[i] = i;
+        s_lo[i] = make_float4(p.lo[0], p.lo[1], p.lo[2], 0.f);
+        s_hi[i] = make_float4(p.hi[0], p.hi[1], p.hi[2], 0.f);
+    }
+    const BNodeG root = nodes[task.node];
+    __syncthreads();
+    uint32_t next = 0;  // scratch nodes used
+    uint32_t qtail = 0, md = task.depth;
+    bool fail = false;
+    auto slot_ptr = [&](uint32_t s) -> BNodeG* { return s == kTaskRoot ? &nodes[task.node] : &local[s]; };
+    // ------------------------------------------------ nodes of more than kCoop triangles: the whole wave
+    CEnt e;
+    bool have = false;
+    int sp = 0;
+    if (n0 > kCoop) {
+        e.slot = kTaskRoot;
+        e.b = 0;
+        e.n = n0;
+        e.depth = task.depth;
+        for (int k = 0; k < 3; ++k) { e.lo[k] = root.lo[k]; e.hi[k] = root.hi[k]; }
+        wave_centroid_bounds(s_perm, s_lo, s_hi, 0, n0, lane, e.clo, e.chi);
+        have = true;
+    } else if (n0 >= 2) {
+        if (lane == 0) s_q[0] = QEnt{kTaskRoot, 0u, n0, task.depth, {root.lo[0], root.lo[1], root.lo[2]},
+                                     {root.hi[0], root.hi[1], root.hi[2]}};
+        qtail = 1;
+    }
+    while (have) {
+        md = max(md, e.depth);
+        const float ext[3] = {e.chi[0] - e.clo[0], e.chi[1] - e.clo[1], e.chi[2] - e.clo[2]};
+        int axis = (ext[1] > ext[0]) ? 1 : 0;
+        if (ext[2] > pick3(ext, axis)) axis = 2;
+        const float cmin = pick3(e.clo, axis);
+        const float eaxis = pick3(ext, axis);
+        const float scale = (float)kSmallBins / eaxis;
+        CEnt L, R;
+        uint32_t nl = 0;
+        int split = -1;
+        if (eaxis > 0.f && isfinite(scale) && e.depth < kMedianDepth) {
+            const int mybin = (int)(lane & 7);
+            uint32_t cnt = 0;
+            float v[12];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                v[k] = v[6 + k] = INFINITY;
+                v[3 + k] = v[9 + k] = -INFINITY;
+            }
+            for (uint32_t i = lane >> 3; i < e.n; i += 8) {
+                const uint32_t j = s_perm[e.b + i];
+                const float4 lo = s_lo[j], hi = s_hi[j];
+                const float c[3] = {0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
+                if (bin_of(pick3(c, axis), cmin, scale, kSmallBins) == mybin) {
+                    const float pl[3] = {lo.x, lo.y, lo.z}, ph[3] = {hi.x, hi.y, hi.z};
+                    cnt++;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        v[k] = fminf(v[k], pl[k]);
+                        v[3 + k] = fmaxf(v[3 + k], ph[k]);
+                        v[6 + k] = fminf(v[6 + k], c[k]);
+                        v[9 + k] = fmaxf(v[9 + k], c[k]);
+                    }
+                }
+            }
+            // merge the 8 groups: lanes of one bin differ in bits 3..5
+            for (int off = 8; off < 64; off <<= 1) {
+                cnt += (uint32_t)__shfl_xor((int)cnt, off);
+#pragma unroll
+                for (int k = 0; k < 12; ++k) {
+                    const float o = __shfl_xor(v[k], off);
+                    v[k] = (k % 6) < 3 ? fminf(v[k], o) : fmaxf(v[k], o);
+                }
+            }
+            if (lane < (uint32_t)kSmallBins) {
+                s_bins[lane][0] = __uint_as_float(cnt);
+#pragma unroll
+                for (int k = 0; k < 12; ++k) s_bins[lane][1 + k] = v[k];
+            }
+            __syncthreads();
+            uint32_t bcnt[kSmallBins];
+            float blo[kSmallBins][3], bhi[kSmallBins][3];
+#pragma unroll
+            for (int b = 0; b < kSmallBins; ++b) {
+                bcnt[b] = __float_as_uint(s_bins[b][0]);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    blo[b][k] = s_bins[b][1 + k];
+                    bhi[b][k] = s_bins[b][4 + k];
+                }
+            }
+            float best;
+            sweep8(bcnt, blo, bhi, split, best);
+            if (split >= 0) {
+                split_boxes8(bcnt, blo, bhi, split, nl, L.lo, L.hi, R.lo, R.hi);
+                for (int k = 0; k < 3; ++k) {
+                    L.clo[k] = R.clo[k] = INFINITY;
+                    L.chi[k] = R.chi[k] = -INFINITY;
+                }
+#pragma unroll
+                for (int b = 0; b < kSmallBins; ++b) {
+                    const bool left = b <= split;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const float cl = s_bins[b][7 + k], ch = s_bins[b][10 + k];
+                        L.clo[k] = left ? fminf(L.clo[k], cl) : L.clo[k];
+                        L.chi[k] = left ? fmaxf(L.chi[k], ch) : L.chi[k];
+                        R.clo[k] = left ? R.clo[k] : fminf(R.clo[k], cl);
+                        R.chi[k] = left ? R.chi[k] : fmaxf(R.chi[k], ch);
+                    }
+                }
+                // stable partition of the node's items through s_tmp
+                uint32_t lb = 0, rb = 0;
+                for (uint32_t k0 = 0; k0 < e.n; k0 += 64) {
+                    const bool valid = k0 + lane < e.n;
+                    uint32_t j = 0;
+                    bool left = false;
+                    if (valid) {
+                        j = s_perm[e.b + k0 + lane];
+                        const float4 lo = s_lo[j], hi = s_hi[j];
+                        const float c = axis == 0 ? 0.5f * (lo.x + hi.x)
+                                                  : (axis == 1 ? 0.5f * (lo.y + hi.y) : 0.5f * (lo.z + hi.z));
+                        left = bin_of(c, cmin, scale, kSmallBins) <= split;
+                    }
+                    const uint64_t lm = __ballot(valid && left), rm = __ballot(valid && !left);
+                    if (valid) s_tmp[e.b + (left ? lb + rank_below(lm) : nl + rb + rank_below(rm))] = j;
+                    lb += (uint32_t)__popcll(lm);
+                    rb += (uint32_t)__popcll(rm);
+                }
+                if (lb != nl) fail = true;
+                __syncthreads();
+                for (uint32_t i = lane; i < e.n; i += 64) s_perm[e.b + i] = s_tmp[e.b + i];
+                __syncthreads();
+            }
+        }
+        if (split < 0) {
+            // object median; the children keep the parent box (conservative) and get exact centroid bounds
+            nl = e.n / 2;
+            for (int k = 0; k < 3; ++k) {
+                L.lo[k] = R.lo[k] = e.lo[k];
+                L.hi[k] = R.hi[k] = e.hi[k];
+            }
+            wave_centroid_bounds(s_perm, s_lo, s_hi, e.b, nl, lane, L.clo, L.chi);
+            wave_centroid_bounds(s_perm, s_lo, s_hi, e.b + nl, e.n - nl, lane, R.clo, R.chi);
+        }
+        const uint32_t c = next;
+        next += 2;
+        L.slot = c;
+        L.b = e.b;
+        L.n = nl;
+        R.slot = c + 1;
+        R.b = e.b + nl;
+        R.n = e.n - nl;
+        L.depth = R.depth = e.depth + 1;
+        if (lane == 0) {
+            local[c] = BNodeG{{L.lo[0], L.lo[1], L.lo[2]}, {L.hi[0], L.hi[1], L.hi[2]}, task.begin + L.b, L.n};
+            local[c + 1] = BNodeG{{R.lo[0], R.lo[1], R.lo[2]}, {R.hi[0], R.hi[1], R.hi[2]}, task.begin + R.b, R.n};
+            *slot_ptr(e.slot) = BNodeG{{e.lo[0], e.lo[1], e.lo[2]}, {e.hi[0], e.hi[1], e.hi[2]}, c, 0u};
+        }
+        // children of up to kCoop triangles are queued for the lanes (single triangles are leaves as written)
+        auto route = [&](const CEnt& k) {
+            if (k.n > kCoop) return;
+            if (k.n >= 2) {
+                if (lane == 0)
+                    s_q[qtail % kQueueCap] = QEnt{k.slot, k.b, k.n, k.depth, {k.lo[0], k.lo[1], k.lo[2]},
+                                                  {k.hi[0], k.hi[1], k.hi[2]}};
+                qtail++;
+            } else {
+                md = max(md, k.depth);
+            }
+        };
+        route(L);
+        route(R);
+        const bool lbig = L.n > kCoop, rbig = R.n > kCoop;
+        if (lbig && rbig) {
+            const bool left_small = L.n <= R.n;
+            if (sp < kCoopStack) {
+                if (lane == 0) s_stack[sp] = pick_ent(left_small, R, L);
+                sp++;
+            } else {
+                fail = true;
+            }
+            e = pick_ent(left_small, L, R);
+        } else if (lbig || rbig) {
+            e = pick_ent(lbig, L, R);
+        } else if (sp > 0) {
+            __syncthreads();
+            e = s_stack[--sp];
+        } else {
+            have = false;
+        }
+        __syncthreads();
+    }
+    if (qtail > kQueueCap) fail = true;
+    // ------------------------------------------------ nodes of up to kCoop triangles: one lane each
+    uint32_t qhead = 0;
+    while (qhead < qtail && !fail) {
+        const uint32_t k = min(64u, qtail - qhead);
+        const bool act = lane < k;
+        QEnt q{};
+        if (act) q = s_q[(qhead + lane) % kQueueCap];
+        qhead += k;
+        __syncthreads();
+        uint32_t nl = 0;
+        float Llo[3], Lhi[3], Rlo[3], Rhi[3];
+        bool split = false;
+        if (act) {
+            md = max(md, q.depth);
+            split = lane_split(q, s_perm, s_lo, s_hi, max_leaf, trav_cost, nl, Llo, Lhi, Rlo, Rhi);
+        }
+        const uint64_t sm = __ballot(split);
+        const uint32_t c = next + 2 * rank_below(sm);
+        next += 2 * (uint32_t)__popcll(sm);
+        const uint32_t nr = q.n - nl;
+        if (split) {
+            local[c] = BNodeG{{Llo[0], Llo[1], Llo[2]}, {Lhi[0], Lhi[1], Lhi[2]}, task.begin + q.b, nl};
+            local[c + 1] = BNodeG{{Rlo[0], Rlo[1], Rlo[2]}, {Rhi[0], Rhi[1], Rhi[2]}, task.begin + q.b + nl, nr};
+            *slot_ptr(q.slot) = BNodeG{{q.lo[0], q.lo[1], q.lo[2]}, {q.hi[0], q.hi[1], q.hi[2]}, c, 0u};
+            if (nl == 1 || nr == 1) md = max(md, q.depth + 1);
+        }
+        const bool pl = split && nl >= 2, pr = split && nr >= 2;
+        const uint64_t ml = __ballot(pl), mr = __ballot(pr);
+        const uint32_t pos = qtail + rank_below(ml) + rank_below(mr);
+        if (pl) s_q[pos % kQueueCap] = QEnt{c, q.b, nl, q.depth + 1, {Llo[0], Llo[1], Llo[2]}, {Lhi[0], Lhi[1], Lhi[2]}};
+        if (pr)
+            s_q[(pos + (pl ? 1u : 0u)) % kQueueCap] =
+                QEnt{c + 1, q.b + nl, nr, q.depth + 1, {Rlo[0], Rlo[1], Rlo[2]}, {Rhi[0], Rhi[1], Rhi[2]}};
+        qtail += (uint32_t)(__popcll(ml) + __popcll(mr));
+        if (qtail - qhead > kQueueCap) fail = true;
+        __syncthreads();
+    }
+    __syncthreads();
+    // the task's items in their final order
+    for (uint32_t i = lane; i < n0; i += 64) prims[task.begin + i] = s_id[s_perm[i]];
     for (int off = 32; off > 0; off >>= 1) md = max(md, (uint32_t)__shfl_xor((int)md, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(max_depth, md);
+    const bool any_fail = __ballot(fail) != 0;
+    if (lane == 0) {
+        used[ti] = next;
+        atomicMax(max_depth, md);
+        if (any_fail) atomicOr(err, 2u);
+    }
 }
 
 // Moves each task's scratch nodes to their final place (first = the task's
@@ -1556,7 +1860,7 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
         d_used.alloc((size_t)ntasks * 4);
         d_first.alloc((size_t)ntasks * 4);
         VG_HIP(hipMemcpyAsync(d_small.p, st.data(), st.size() * sizeof(SmallTask), hipMemcpyHostToDevice, c.stream));
-        k_build_small<<<blocks_for(ntasks, 64), 64, 0, c.stream>>>(
+        k_build_small<<<ntasks, 64, 0, c.stream>>>(
             d_small.as<SmallTask>(), d_order.as<uint32_t>(), ntasks, prims.as<uint32_t>(), pbuf.as<PrimBox>(), nodes.as<BNodeG>(),
             snodes.as<BNodeG>(), d_used.as<uint32_t>(), d_counter.as<uint32_t>() + 1, d_counter.as<uint32_t>() + 2,
             max_leaf, opt.traversal_cost > 0.0f ? opt.traversal_cost : 1.0f);
@@ -1571,7 +1875,7 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     }
     VG_HIP(hipMemcpyAsync(hc, d_counter.p, 12, hipMemcpyDeviceToHost, c.stream));
     VG_HIP(hipStreamSynchronize(c.stream));
-    VG_CHECK(hc[2] == 0, VIRA_GPU_ERR_HIP, "BVH build: small-subtree stack overflow");
+    VG_CHECK(hc[2] == 0, VIRA_GPU_ERR_HIP, "BVH build: small-subtree builder failed (stack, queue or partition)");
     max_depth = hc[1];
     VG_CHECK(max_depth < 63, VIRA_GPU_ERR_HIP, "BVH deeper than the traversal stack");
     VG_CHECK(nn <= 2 * N, VIRA_GPU_ERR_HIP, "BVH node count overflow");
