@@ -139,166 +139,273 @@ __global__ void __launch_bounds__(256) k_reduce_bounds(size_t n, const PrimBox* 
     }
 }
 
-__global__ void k_init_bins(int* bins, size_t nvals) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nvals) return;
-    int v = (int)(i % kBinVals);
-    bins[i] = v == 0 ? 0 : (((v - 1) % 6) < 3 ? INT_MAX : INT_MIN);
+// ---------------------------------------------------------------- large nodes, level by level
+// Every level runs on the GPU from device-resident counts, so the host only
+// queues launches: a plan per node (axis, bins, chunks), the node's chunks,
+// binning, the SAH sweep, the stable partition, and the emission of the
+// children (2 per node, numbered base + 2 * node of the level) to the next
+// level or to the small-subtree list.  Grids are fixed and stride over the
+// level's nodes or chunks.
+struct LTask {  // a node of more than kSmall triangles awaiting its split
+    uint32_t node, begin, count, depth;
+    float clo[3], chi[3];
+};
+// Device counters of the level loop (one word each).
+enum : int {
+    LC_NT0 = 0,      // nodes of the even levels
+    LC_NT1 = 1,      // nodes of the odd levels
+    LC_NCH = 2,      // chunks of the current level
+    LC_NODE = 3,     // next free binary node
+    LC_SMALL = 4,    // small subtrees emitted so far
+    LC_LSMALL = 5,   // small subtrees emitted by the current level
+    LC_DEPTH = 6,    // deepest level node
+    LC_ERR = 7,      // 1: a binned node without a valid split
+    LC_WORDS = 8
+};
+constexpr int kLevelBlocks = 1024;  // grid of the per-node and per-chunk passes
+
+// Root node and root task from the reduced bounds (keys: box lo/hi, centroid lo/hi).
+__global__ void k_lv_root(const int* keys, uint32_t n, BNodeG* nodes, LTask* task, SmallTask* small,
+                          uint32_t* cnt) {
+    BNodeG r;
+    LTask t;
+    for (int k = 0; k < 3; ++k) {
+        r.lo[k] = key2f(keys[k]);
+        r.hi[k] = key2f(keys[3 + k]);
+        t.clo[k] = key2f(keys[6 + k]);
+        t.chi[k] = key2f(keys[9 + k]);
+    }
+    r.left = 0;
+    r.count = n;
+    nodes[0] = r;
+    t.node = 0;
+    t.begin = 0;
+    t.count = n;
+    t.depth = 0;
+    const bool large = n > kSmall;
+    if (large) task[0] = t;
+    else small[0] = SmallTask{0u, 0u, n, 0u, 0u, 0u};
+    for (int k = 0; k < LC_WORDS; ++k) cnt[k] = 0;
+    cnt[LC_NT0] = large ? 1u : 0u;
+    cnt[LC_NODE] = 1;
+    cnt[LC_SMALL] = large ? 0u : 1u;
 }
 
-// One workgroup per chunk of a large node: bin into LDS, merge into the node's bins.
-// Bins one chunk of a node in LDS and merges the chunk's bins into the
-// node's; the bin of every slot is kept for the partition.
-__global__ void __launch_bounds__(kChunkThreads) k_bin_large(const Chunk* chunks, const TaskParam* params, int nb,
-                                                             const uint32_t* prims, const PrimBox* pb, int* bins,
-                                                             uint32_t* chunk_counts, uint8_t* slot_bin) {
-    __shared__ int s[kMaxBins * kBinVals];
-    for (int i = threadIdx.x; i < nb * kBinVals; i += blockDim.x) {
-        int v = i % kBinVals;
-        s[i] = v == 0 ? 0 : (((v - 1) % 6) < 3 ? INT_MAX : INT_MIN);
+// Per node: axis, bin mapping and chunk count; nodes whose centroids coincide
+// (or that are too deep) get no chunks and are split at the object median.
+__global__ void k_lv_plan(const LTask* __restrict__ tasks, const uint32_t* __restrict__ d_nt, int nb,
+                          TaskParam* __restrict__ params, uint32_t* __restrict__ nchunk, uint32_t* cnt) {
+    const uint32_t nt = *d_nt;
+    uint32_t md = 0;
+    for (uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x; ti < nt; ti += gridDim.x * blockDim.x) {
+        const LTask t = tasks[ti];
+        md = max(md, t.depth);
+        const float ext[3] = {t.chi[0] - t.clo[0], t.chi[1] - t.clo[1], t.chi[2] - t.clo[2]};
+        int axis = (ext[1] > ext[0]) ? 1 : 0;
+        if (ext[2] > pick3(ext, axis)) axis = 2;
+        const float scale = (float)nb / pick3(ext, axis);
+        const bool median = !(pick3(ext, axis) > 0.f) || !isfinite(scale) || t.depth >= (uint32_t)kMedianDepth;
+        const uint32_t nch = median ? 0u : (t.count + kChunk - 1) / kChunk;
+        params[ti] = TaskParam{axis, pick3(t.clo, axis), scale, -1, t.begin, 0u, nch, 0u};
+        nchunk[ti] = nch;
     }
-    __syncthreads();
-    Chunk ch = chunks[blockIdx.x];
-    TaskParam tp = params[ch.task];
+    for (int off = 32; off > 0; off >>= 1) md = max(md, (uint32_t)__shfl_xor((int)md, off));
+    if ((threadIdx.x & 63) == 0 && md > 0) atomicMax(&cnt[LC_DEPTH], md);
+}
+
+// Each node's chunks (one wave per node): first chunk from the scan of the counts.
+__global__ void k_lv_chunks(const LTask* __restrict__ tasks, const uint32_t* __restrict__ d_nt,
+                            TaskParam* __restrict__ params, const uint32_t* __restrict__ chunk0,
+                            Chunk* __restrict__ chunks) {
+    const uint32_t nt = *d_nt, lane = threadIdx.x & 63;
+    const uint32_t nwaves = gridDim.x * (blockDim.x / 64);
+    for (uint32_t ti = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); ti < nt; ti += nwaves) {
+        const uint32_t c0 = chunk0[ti], nch = params[ti].nchunk;
+        const uint32_t b = tasks[ti].begin, e = b + tasks[ti].count;
+        if (lane == 0) params[ti].chunk0 = c0;
+        for (uint32_t k = lane; k < nch; k += 64)
+            chunks[c0 + k] = Chunk{ti, b + k * kChunk, min(b + (k + 1) * kChunk, e), 0u};
+    }
+}
+
+__global__ void k_init_bins(int* bins, const uint32_t* d_nt, int nb) {
+    const size_t nvals = (size_t)(*d_nt) * nb * kBinVals;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvals; i += (size_t)gridDim.x * blockDim.x) {
+        const int v = (int)(i % kBinVals);
+        bins[i] = v == 0 ? 0 : (((v - 1) % 6) < 3 ? INT_MAX : INT_MIN);
+    }
+}
+
+// One workgroup per chunk of a large node (striding over the level's chunks):
+// bin into LDS, merge into the node's global bins; the bin of every slot is
+// kept for the partition.
+__global__ void __launch_bounds__(kChunkThreads) k_bin_large(const Chunk* chunks, const uint32_t* d_nch,
+                                                             const TaskParam* params, int nb, const uint32_t* prims,
+                                                             const PrimBox* pb, int* bins, uint32_t* chunk_counts,
+                                                             uint8_t* slot_bin) {
+    __shared__ int s[kMaxBins * kBinVals];
+    const uint32_t nch = *d_nch;
     const int lane = threadIdx.x & 63;
-    // In a node of many chunks a bin spans a wide slab, neighbouring slots
-    // mostly fall into the same bin, and LDS atomics from a wave would mostly
-    // hit one address.  There the wave reduces the lanes of one bin at a time
-    // (the first pending lane's) with cross-lane min/max/add and one lane adds
-    // the result: as many rounds as distinct bins in the wave.  Small nodes'
-    // bins are narrow, their waves span many bins, and per-lane atomics win.
-    if (tp.nchunk < kReduceMinChunks) {
-        for (uint32_t slot = ch.begin + threadIdx.x; slot < ch.end; slot += blockDim.x) {
-            const PrimBox p = pb[prims[slot]];
-            const float c[3] = {centroid(p, 0), centroid(p, 1), centroid(p, 2)};
-            const int b = bin_of(c[tp.axis], tp.cmin, tp.scale, nb);
-            slot_bin[slot] = (uint8_t)b;
-            int* sb = &s[b * kBinVals];
-            atomicAdd(&sb[0], 1);
-            for (int k = 0; k < 3; ++k) {
-                atomicMin(&sb[1 + k], f2key(p.lo[k]));
-                atomicMax(&sb[4 + k], f2key(p.hi[k]));
-                atomicMin(&sb[7 + k], f2key(c[k]));
-                atomicMax(&sb[10 + k], f2key(c[k]));
-            }
+    for (uint32_t ci = blockIdx.x; ci < nch; ci += gridDim.x) {
+        for (int i = threadIdx.x; i < nb * kBinVals; i += blockDim.x) {
+            int v = i % kBinVals;
+            s[i] = v == 0 ? 0 : (((v - 1) % 6) < 3 ? INT_MAX : INT_MIN);
         }
-    } else for (uint32_t base = ch.begin; base < ch.end; base += blockDim.x) {
-        const uint32_t slot = base + threadIdx.x;
-        const bool valid = slot < ch.end;
-        int b = -1;
-        int v[kBinVals];
-        if (valid) {
-            const PrimBox p = pb[prims[slot]];
-            const float c[3] = {centroid(p, 0), centroid(p, 1), centroid(p, 2)};
-            b = bin_of(c[tp.axis], tp.cmin, tp.scale, nb);
-            slot_bin[slot] = (uint8_t)b;
-            v[0] = 1;
-            for (int k = 0; k < 3; ++k) {
-                v[1 + k] = f2key(p.lo[k]);
-                v[4 + k] = f2key(p.hi[k]);
-                v[7 + k] = f2key(c[k]);
-                v[10 + k] = f2key(c[k]);
+        __syncthreads();
+        const Chunk ch = chunks[ci];
+        const TaskParam tp = params[ch.task];
+        // In a node of many chunks a bin spans a wide slab, neighbouring slots
+        // mostly fall into the same bin, and LDS atomics from a wave would mostly
+        // hit one address.  There the wave reduces the lanes of one bin at a time
+        // (the first pending lane's) with cross-lane min/max/add and one lane adds
+        // the result: as many rounds as distinct bins in the wave.  Small nodes'
+        // bins are narrow, their waves span many bins, and per-lane atomics win.
+        if (tp.nchunk < kReduceMinChunks) {
+            for (uint32_t slot = ch.begin + threadIdx.x; slot < ch.end; slot += blockDim.x) {
+                const PrimBox p = pb[prims[slot]];
+                const float c[3] = {centroid(p, 0), centroid(p, 1), centroid(p, 2)};
+                const int b = bin_of(pick3(c, tp.axis), tp.cmin, tp.scale, nb);
+                slot_bin[slot] = (uint8_t)b;
+                int* sb = &s[b * kBinVals];
+                atomicAdd(&sb[0], 1);
+                for (int k = 0; k < 3; ++k) {
+                    atomicMin(&sb[1 + k], f2key(p.lo[k]));
+                    atomicMax(&sb[4 + k], f2key(p.hi[k]));
+                    atomicMin(&sb[7 + k], f2key(c[k]));
+                    atomicMax(&sb[10 + k], f2key(c[k]));
+                }
             }
-        }
-        uint64_t pending = __ballot(valid);
-        while (pending) {
-            const int lead = __builtin_ctzll(pending);
-            const int b0 = __shfl(b, lead, 64);
-            const bool mine = b == b0;
-            pending &= ~__ballot(mine);
-            int r[kBinVals];
-            r[0] = mine ? 1 : 0;
+        } else {
+            for (uint32_t base = ch.begin; base < ch.end; base += blockDim.x) {
+                const uint32_t slot = base + threadIdx.x;
+                const bool valid = slot < ch.end;
+                int b = -1;
+                int v[kBinVals];
+                if (valid) {
+                    const PrimBox p = pb[prims[slot]];
+                    const float c[3] = {centroid(p, 0), centroid(p, 1), centroid(p, 2)};
+                    b = bin_of(pick3(c, tp.axis), tp.cmin, tp.scale, nb);
+                    slot_bin[slot] = (uint8_t)b;
+                    v[0] = 1;
+                    for (int k = 0; k < 3; ++k) {
+                        v[1 + k] = f2key(p.lo[k]);
+                        v[4 + k] = f2key(p.hi[k]);
+                        v[7 + k] = f2key(c[k]);
+                        v[10 + k] = f2key(c[k]);
+                    }
+                }
+                uint64_t pending = __ballot(valid);
+                while (pending) {
+                    const int lead = __builtin_ctzll(pending);
+                    const int b0 = __shfl(b, lead, 64);
+                    const bool mine = b == b0;
+                    pending &= ~__ballot(mine);
+                    int r[kBinVals];
+                    r[0] = mine ? 1 : 0;
 #pragma unroll
-            for (int k = 1; k < kBinVals; ++k) r[k] = mine ? v[k] : (((k - 1) % 6) < 3 ? INT_MAX : INT_MIN);
-            // whole-wave reductions (DPP-based device library routines)
-            r[0] = __ockl_wfred_add_i32(r[0]);
+                    for (int k = 1; k < kBinVals; ++k) r[k] = mine ? v[k] : (((k - 1) % 6) < 3 ? INT_MAX : INT_MIN);
+                    // whole-wave reductions (DPP-based device library routines)
+                    r[0] = __ockl_wfred_add_i32(r[0]);
 #pragma unroll
-            for (int k = 1; k < kBinVals; ++k)
-                r[k] = ((k - 1) % 6) < 3 ? __ockl_wfred_min_i32(r[k]) : __ockl_wfred_max_i32(r[k]);
-            if (lane == lead) {
-                int* sb = &s[b0 * kBinVals];
-                atomicAdd(&sb[0], r[0]);
+                    for (int k = 1; k < kBinVals; ++k)
+                        r[k] = ((k - 1) % 6) < 3 ? __ockl_wfred_min_i32(r[k]) : __ockl_wfred_max_i32(r[k]);
+                    if (lane == lead) {
+                        int* sb = &s[b0 * kBinVals];
+                        atomicAdd(&sb[0], r[0]);
 #pragma unroll
-                for (int k = 1; k < kBinVals; ++k) {
-                    if (((k - 1) % 6) < 3) atomicMin(&sb[k], r[k]);
-                    else atomicMax(&sb[k], r[k]);
+                        for (int k = 1; k < kBinVals; ++k) {
+                            if (((k - 1) % 6) < 3) atomicMin(&sb[k], r[k]);
+                            else atomicMax(&sb[k], r[k]);
+                        }
+                    }
                 }
             }
         }
-    }
-    __syncthreads();
-    if (threadIdx.x < (unsigned)nb) {
-        int b = threadIdx.x;
-        const int* sb = &s[b * kBinVals];
-        chunk_counts[(size_t)blockIdx.x * nb + b] = (uint32_t)sb[0];
-        if (sb[0] > 0) {
-            int* gb = &bins[((size_t)ch.task * nb + b) * kBinVals];
-            atomicAdd(&gb[0], sb[0]);
-            for (int k = 1; k < kBinVals; ++k) {
-                if (((k - 1) % 6) < 3) atomicMin(&gb[k], sb[k]);
-                else atomicMax(&gb[k], sb[k]);
+        __syncthreads();
+        if (threadIdx.x < (unsigned)nb) {
+            int b = threadIdx.x;
+            const int* sb = &s[b * kBinVals];
+            chunk_counts[(size_t)ci * nb + b] = (uint32_t)sb[0];
+            if (sb[0] > 0) {
+                int* gb = &bins[((size_t)ch.task * nb + b) * kBinVals];
+                atomicAdd(&gb[0], sb[0]);
+                for (int k = 1; k < kBinVals; ++k) {
+                    if (((k - 1) % 6) < 3) atomicMin(&gb[k], sb[k]);
+                    else atomicMax(&gb[k], sb[k]);
+                }
             }
         }
+        __syncthreads();
     }
 }
 
-__global__ void __launch_bounds__(kChunkThreads) k_partition_large(const Chunk* chunks, const TaskParam* params,
-                                                                   const ChunkBase* bases, const uint32_t* prims,
-                                                                   const uint8_t* slot_bin, uint32_t* tmp) {
+// Stable partition of one chunk (striding over the level's chunks): lefts to
+// the chunk's left base, rights to its right base, in slot order.
+__global__ void __launch_bounds__(kChunkThreads) k_partition_large(const Chunk* chunks, const uint32_t* d_nch,
+                                                                   const TaskParam* params, const ChunkBase* bases,
+                                                                   const uint32_t* prims, const uint8_t* slot_bin,
+                                                                   uint32_t* tmp) {
     __shared__ uint32_t lds[kChunkThreads / 64 + 1];
-    Chunk ch = chunks[blockIdx.x];
-    TaskParam tp = params[ch.task];
-    ChunkBase cb = bases[blockIdx.x];
-    uint32_t ids[kChunkItems];
-    uint32_t flags = 0, nl = 0;
-    for (int k = 0; k < kChunkItems; ++k) {
-        uint32_t slot = ch.begin + threadIdx.x * kChunkItems + k;
-        ids[k] = 0;
-        if (slot < ch.end) {
-            ids[k] = prims[slot];
-            if ((int)slot_bin[slot] <= tp.split) {
-                flags |= 1u << k;
-                nl++;
+    const uint32_t nch = *d_nch;
+    for (uint32_t ci = blockIdx.x; ci < nch; ci += gridDim.x) {
+        const Chunk ch = chunks[ci];
+        const TaskParam tp = params[ch.task];
+        const ChunkBase cb = bases[ci];
+        uint32_t ids[kChunkItems];
+        uint32_t flags = 0, nl = 0;
+        for (int k = 0; k < kChunkItems; ++k) {
+            uint32_t slot = ch.begin + threadIdx.x * kChunkItems + k;
+            ids[k] = 0;
+            if (slot < ch.end) {
+                ids[k] = prims[slot];
+                if ((int)slot_bin[slot] <= tp.split) {
+                    flags |= 1u << k;
+                    nl++;
+                }
             }
         }
-    }
-    // block exclusive scan of nl
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    uint32_t x = nl;
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
-    }
-    if (lane == 63) lds[wid] = x;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t s = 0;
-        for (int w = 0; w < kChunkThreads / 64; ++w) {
-            uint32_t t = lds[w];
-            lds[w] = s;
-            s += t;
+        // block exclusive scan of nl
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+        uint32_t x = nl;
+        for (int off = 1; off < 64; off <<= 1) {
+            uint32_t y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
         }
-    }
-    __syncthreads();
-    uint32_t lbefore = x - nl + lds[wid];
-    uint32_t pos = threadIdx.x * kChunkItems;  // chunk-local index of the first item
-    for (int k = 0; k < kChunkItems; ++k) {
-        uint32_t slot = ch.begin + pos + k;
-        if (slot >= ch.end) break;
-        uint32_t li = pos + k;
-        if (flags & (1u << k)) {
-            tmp[cb.left + lbefore] = ids[k];
-            lbefore++;
-        } else {
-            uint32_t rbefore = li - lbefore;
-            tmp[cb.right + rbefore] = ids[k];
+        if (lane == 63) lds[wid] = x;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t s = 0;
+            for (int w = 0; w < kChunkThreads / 64; ++w) {
+                uint32_t t = lds[w];
+                lds[w] = s;
+                s += t;
+            }
         }
+        __syncthreads();
+        uint32_t lbefore = x - nl + lds[wid];
+        uint32_t pos = threadIdx.x * kChunkItems;  // chunk-local index of the first item
+        for (int k = 0; k < kChunkItems; ++k) {
+            uint32_t slot = ch.begin + pos + k;
+            if (slot >= ch.end) break;
+            uint32_t li = pos + k;
+            if (flags & (1u << k)) {
+                tmp[cb.left + lbefore] = ids[k];
+                lbefore++;
+            } else {
+                uint32_t rbefore = li - lbefore;
+                tmp[cb.right + rbefore] = ids[k];
+            }
+        }
+        __syncthreads();
     }
 }
 
-__global__ void k_copy_back(const Chunk* chunks, const uint32_t* tmp, uint32_t* prims) {
-    Chunk ch = chunks[blockIdx.x];
-    for (uint32_t s = ch.begin + threadIdx.x; s < ch.end; s += blockDim.x) prims[s] = tmp[s];
+__global__ void k_copy_back(const Chunk* chunks, const uint32_t* d_nch, const uint32_t* tmp, uint32_t* prims) {
+    const uint32_t nch = *d_nch;
+    for (uint32_t ci = blockIdx.x; ci < nch; ci += gridDim.x) {
+        const Chunk ch = chunks[ci];
+        for (uint32_t s = ch.begin + threadIdx.x; s < ch.end; s += blockDim.x) prims[s] = tmp[s];
+    }
 }
 
 __device__ __forceinline__ float half_area(const float lo[3], const float hi[3]) {
@@ -311,71 +418,111 @@ __device__ __forceinline__ float key2f(int k) { return __int_as_float(k >= 0 ? k
 // SAH sweep of one binned node's bins (vira_blas.ipp:185-254 with bins):
 // right-to-left areas, then the cheapest left-to-right boundary; the split is
 // written to the node's TaskParam for the partition, and the children's boxes
-// and centroid bounds are gathered from the bins.
-__global__ void k_sweep_large(const int* __restrict__ bins, int nb, uint32_t ntasks, TaskParam* params,
-                              SweepOut* out) {
-    const uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ti >= ntasks) return;
-    const int* tb = bins + (size_t)ti * nb * kBinVals;
-    float rarea[kMaxBins];
-    uint32_t rcnt[kMaxBins];
-    float rl[3] = {INFINITY, INFINITY, INFINITY}, rh[3] = {-INFINITY, -INFINITY, -INFINITY};
-    uint32_t rc = 0;
-    for (int b = nb - 1; b > 0; --b) {
-        const int* bb = tb + b * kBinVals;
-        rc += (uint32_t)bb[0];
-        if (bb[0] > 0)
-            for (int k = 0; k < 3; ++k) { rl[k] = fminf(rl[k], key2f(bb[1 + k])); rh[k] = fmaxf(rh[k], key2f(bb[4 + k])); }
-        rarea[b] = rc ? half_area(rl, rh) : 0.f;
-        rcnt[b] = rc;
-    }
-    float ll[3] = {INFINITY, INFINITY, INFINITY}, lh[3] = {-INFINITY, -INFINITY, -INFINITY};
-    uint32_t lc = 0;
-    float best = INFINITY;
-    int split = -1;
-    for (int b = 0; b < nb - 1; ++b) {
-        const int* bb = tb + b * kBinVals;
-        lc += (uint32_t)bb[0];
-        if (bb[0] > 0)
-            for (int k = 0; k < 3; ++k) { ll[k] = fminf(ll[k], key2f(bb[1 + k])); lh[k] = fmaxf(lh[k], key2f(bb[4 + k])); }
-        if (lc == 0 || rcnt[b + 1] == 0) continue;
-        const float cost = (float)lc * half_area(ll, lh) + (float)rcnt[b + 1] * rarea[b + 1];
-        if (cost < best) { best = cost; split = b; }
-    }
-    params[ti].split = split;
-    SweepOut o;
-    for (int h = 0; h < 2; ++h)
-        for (int k = 0; k < 3; ++k) {
-            o.lo[h][k] = o.clo[h][k] = INFINITY;
-            o.hi[h][k] = o.chi[h][k] = -INFINITY;
+// and centroid bounds are gathered from the bins.  A node without chunks is
+// split at the object median: both children keep its box and centroid bounds.
+// Also counts the node's children that stay large (nlarge) and that go to the
+// small-subtree builder (nsmall).
+__global__ void k_sweep_large(const int* __restrict__ bins, int nb, const LTask* __restrict__ tasks,
+                              const uint32_t* __restrict__ d_nt, const BNodeG* __restrict__ nodes, TaskParam* params,
+                              SweepOut* out, uint32_t* nlarge, uint32_t* nsmall, uint32_t* cnt) {
+    const uint32_t nt = *d_nt;
+    for (uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x; ti < nt; ti += gridDim.x * blockDim.x) {
+        const LTask t = tasks[ti];
+        SweepOut o;
+        if (params[ti].nchunk == 0) {
+            const BNodeG par = nodes[t.node];
+            o.split = -1;
+            o.nleft = t.count / 2;
+            for (int h = 0; h < 2; ++h)
+                for (int k = 0; k < 3; ++k) {
+                    o.lo[h][k] = par.lo[k];
+                    o.hi[h][k] = par.hi[k];
+                    o.clo[h][k] = t.clo[k];
+                    o.chi[h][k] = t.chi[k];
+                }
+        } else {
+            const int* tb = bins + (size_t)ti * nb * kBinVals;
+            float rarea[kMaxBins];
+            uint32_t rcnt[kMaxBins];
+            float rl[3] = {INFINITY, INFINITY, INFINITY}, rh[3] = {-INFINITY, -INFINITY, -INFINITY};
+            uint32_t rc = 0;
+            for (int b = nb - 1; b > 0; --b) {
+                const int* bb = tb + b * kBinVals;
+                rc += (uint32_t)bb[0];
+                if (bb[0] > 0)
+                    for (int k = 0; k < 3; ++k) {
+                        rl[k] = fminf(rl[k], key2f(bb[1 + k]));
+                        rh[k] = fmaxf(rh[k], key2f(bb[4 + k]));
+                    }
+                rarea[b] = rc ? half_area(rl, rh) : 0.f;
+                rcnt[b] = rc;
+            }
+            float ll[3] = {INFINITY, INFINITY, INFINITY}, lh[3] = {-INFINITY, -INFINITY, -INFINITY};
+            uint32_t lc = 0;
+            float best = INFINITY;
+            int split = -1;
+            for (int b = 0; b < nb - 1; ++b) {
+                const int* bb = tb + b * kBinVals;
+                lc += (uint32_t)bb[0];
+                if (bb[0] > 0)
+                    for (int k = 0; k < 3; ++k) {
+                        ll[k] = fminf(ll[k], key2f(bb[1 + k]));
+                        lh[k] = fmaxf(lh[k], key2f(bb[4 + k]));
+                    }
+                if (lc == 0 || rcnt[b + 1] == 0) continue;
+                const float cost = (float)lc * half_area(ll, lh) + (float)rcnt[b + 1] * rarea[b + 1];
+                if (cost < best) {
+                    best = cost;
+                    split = b;
+                }
+            }
+            params[ti].split = split;
+            for (int h = 0; h < 2; ++h)
+                for (int k = 0; k < 3; ++k) {
+                    o.lo[h][k] = o.clo[h][k] = INFINITY;
+                    o.hi[h][k] = o.chi[h][k] = -INFINITY;
+                }
+            uint32_t nl = 0;
+            for (int b = 0; b < nb; ++b) {
+                const int* bb = tb + b * kBinVals;
+                if (bb[0] == 0) continue;
+                const int h = b <= split ? 0 : 1;
+                if (h == 0) nl += (uint32_t)bb[0];
+                for (int k = 0; k < 3; ++k) {
+                    o.lo[h][k] = fminf(o.lo[h][k], key2f(bb[1 + k]));
+                    o.hi[h][k] = fmaxf(o.hi[h][k], key2f(bb[4 + k]));
+                    o.clo[h][k] = fminf(o.clo[h][k], key2f(bb[7 + k]));
+                    o.chi[h][k] = fmaxf(o.chi[h][k], key2f(bb[10 + k]));
+                }
+            }
+            o.split = split;
+            o.nleft = nl;
         }
-    uint32_t nl = 0;
-    for (int b = 0; b < nb; ++b) {
-        const int* bb = tb + b * kBinVals;
-        if (bb[0] == 0) continue;
-        const int h = b <= split ? 0 : 1;
-        if (h == 0) nl += (uint32_t)bb[0];
-        for (int k = 0; k < 3; ++k) {
-            o.lo[h][k] = fminf(o.lo[h][k], key2f(bb[1 + k]));
-            o.hi[h][k] = fmaxf(o.hi[h][k], key2f(bb[4 + k]));
-            o.clo[h][k] = fminf(o.clo[h][k], key2f(bb[7 + k]));
-            o.chi[h][k] = fmaxf(o.chi[h][k], key2f(bb[10 + k]));
+        out[ti] = o;
+        uint32_t nbig = 0, nsm = 0;
+        if (params[ti].nchunk != 0 && o.split < 0) {
+            atomicOr(&cnt[LC_ERR], 1u);  // no bin boundary separates the centroids: the node is dropped
+        } else {
+            const uint32_t nl = o.nleft, nr = t.count - o.nleft;
+            nbig = (nl > kSmall ? 1u : 0u) + (nr > kSmall ? 1u : 0u);
+            nsm = 2u - nbig;
         }
+        nlarge[ti] = nbig;
+        nsmall[ti] = nsm;
     }
-    o.split = split;
-    o.nleft = nl;
-    out[ti] = o;
 }
 
 // Per chunk: how many of its triangles go left (bins <= the node's split).
-__global__ void k_chunk_left(const Chunk* __restrict__ chunks, const TaskParam* __restrict__ params,
-                             const uint32_t* __restrict__ counts, int nb, uint32_t nchunks, uint32_t* left) {
-    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ci >= nchunks) return;
-    const int split = params[chunks[ci].task].split;
-    uint32_t cl = 0;
-    for (int b = 0; b <= split; ++b) cl += counts[(size_t)ci * nb + b];
-    left[ci] = cl;
+__global__ void k_chunk_left(const Chunk* __restrict__ chunks, const uint32_t* __restrict__ d_nch,
+                             const TaskParam* __restrict__ params, const uint32_t* __restrict__ counts, int nb,
+                             uint32_t* left) {
+    const uint32_t nch = *d_nch;
+    for (uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x; ci < nch; ci += gridDim.x * blockDim.x) {
+        const int split = params[chunks[ci].task].split;
+        uint32_t cl = 0;
+        for (int b = 0; b <= split; ++b) cl += counts[(size_t)ci * nb + b];
+        left[ci] = cl;
+    }
 }
 
 // Where each chunk writes its left and right triangles in the stable
@@ -384,16 +531,78 @@ __global__ void k_chunk_left(const Chunk* __restrict__ chunks, const TaskParam* 
 // chunks; a node's chunks are consecutive, so the lefts before chunk ci within
 // its node are lscan[ci] - lscan[first chunk], and the rights the remaining
 // slots before it.
-__global__ void k_chunk_bases(const Chunk* __restrict__ chunks, const TaskParam* __restrict__ params,
-                              const SweepOut* __restrict__ sw, const uint32_t* __restrict__ lscan, uint32_t nchunks,
-                              ChunkBase* bases) {
-    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ci >= nchunks) return;
-    const Chunk ch = chunks[ci];
-    const TaskParam tp = params[ch.task];
-    const uint32_t lrun = lscan[ci] - lscan[tp.chunk0];
-    const uint32_t rrun = (ch.begin - tp.begin) - lrun;
-    bases[ci] = ChunkBase{tp.begin + lrun, tp.begin + sw[ch.task].nleft + rrun};
+__global__ void k_chunk_bases(const Chunk* __restrict__ chunks, const uint32_t* __restrict__ d_nch,
+                              const TaskParam* __restrict__ params, const SweepOut* __restrict__ sw,
+                              const uint32_t* __restrict__ lscan, ChunkBase* bases) {
+    const uint32_t nch = *d_nch;
+    for (uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x; ci < nch; ci += gridDim.x * blockDim.x) {
+        const Chunk ch = chunks[ci];
+        const TaskParam tp = params[ch.task];
+        const uint32_t lrun = lscan[ci] - lscan[tp.chunk0];
+        const uint32_t rrun = (ch.begin - tp.begin) - lrun;
+        bases[ci] = ChunkBase{tp.begin + lrun, tp.begin + sw[ch.task].nleft + rrun};
+    }
+}
+
+// The children of every node of the level: binary nodes base + 2 ti and
+// base + 2 ti + 1 (boxes from the sweep, triangle ranges as leaves until
+// split), the parent turned internal, and each child queued as a large node of
+// the next level or as a small subtree (positions from the scans of nlarge and
+// nsmall).
+__global__ void k_lv_emit(const LTask* __restrict__ tasks, const uint32_t* __restrict__ d_nt,
+                          const SweepOut* __restrict__ sw, const uint32_t* __restrict__ nlarge,
+                          const uint32_t* __restrict__ nsmall, const uint32_t* __restrict__ loff,
+                          const uint32_t* __restrict__ soff, BNodeG* nodes,
+                          LTask* __restrict__ next, SmallTask* __restrict__ small, const uint32_t* __restrict__ cnt) {
+    const uint32_t nt = *d_nt, base = cnt[LC_NODE], sbase = cnt[LC_SMALL];
+    for (uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x; ti < nt; ti += gridDim.x * blockDim.x) {
+        const LTask t = tasks[ti];
+        const SweepOut o = sw[ti];
+        if (nlarge[ti] + nsmall[ti] == 0) continue;  // dropped by the sweep (error flagged)
+        const uint32_t c = base + 2 * ti;
+        const uint32_t nl = o.nleft, nr = t.count - o.nleft;
+        BNodeG L, R;
+        for (int k = 0; k < 3; ++k) {
+            L.lo[k] = o.lo[0][k];
+            L.hi[k] = o.hi[0][k];
+            R.lo[k] = o.lo[1][k];
+            R.hi[k] = o.hi[1][k];
+        }
+        L.left = t.begin;
+        L.count = nl;
+        R.left = t.begin + nl;
+        R.count = nr;
+        nodes[c] = L;
+        nodes[c + 1] = R;
+        nodes[t.node].left = c;
+        nodes[t.node].count = 0;
+        uint32_t lp = loff[ti], sp = sbase + soff[ti];
+        const uint32_t depth = t.depth + 1;
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t n = h == 0 ? nl : nr;
+            const uint32_t b = h == 0 ? t.begin : t.begin + nl;
+            if (n > kSmall) {
+                LTask x;
+                x.node = c + h;
+                x.begin = b;
+                x.count = n;
+                x.depth = depth;
+                for (int k = 0; k < 3; ++k) {
+                    x.clo[k] = o.clo[h][k];
+                    x.chi[k] = o.chi[h][k];
+                }
+                next[lp++] = x;
+            } else {
+                small[sp++] = SmallTask{c + h, b, n, depth, 0u, 0u};
+            }
+        }
+    }
+}
+
+// After a level: its nodes' children are numbered, its small subtrees counted.
+__global__ void k_lv_advance(uint32_t* cnt, int parity) {
+    cnt[LC_NODE] += 2 * cnt[parity ? LC_NT1 : LC_NT0];
+    cnt[LC_SMALL] += cnt[LC_LSMALL];
 }
 
 // Small subtrees (nodes of at most kSmall triangles; the reference's
@@ -445,6 +654,15 @@ __device__ __forceinline__ CEnt pick_ent(bool c, const CEnt& a, const CEnt& b) {
         r.chi[k] = c ? a.chi[k] : b.chi[k];
     }
     return r;
+}
+
+// The builder's workgroup is one wave, whose LDS accesses execute in program
+// order: ordering them needs only a compiler barrier, not __syncthreads (whose
+// fence would also wait for every outstanding global store).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // number of set bits of m below this lane
@@ -531,10 +749,13 @@ __device__ __forceinline__ void split_boxes8(const uint32_t bcnt[kSmallBins], co
 
 // One lane decides and splits queued node q alone (items in LDS).  Returns
 // false when q stays a leaf; else nl and the children's boxes are set and the
-// node's items are partitioned in place (vira_blas.ipp:149-159).
+// node's items are partitioned in place (vira_blas.ipp:149-159).  The bins
+// stay in registers (every bin updated under a compare with its compile-time
+// index, no indexing into registers); each item's bin is kept in s_binof for
+// the partition.
 __device__ __forceinline__ bool lane_split(const QEnt& q, uint32_t* s_perm, const float4* s_lo, const float4* s_hi,
-                                           int max_leaf, float trav_cost, uint32_t& nl, float Llo[3], float Lhi[3],
-                                           float Rlo[3], float Rhi[3]) {
+                                           uint8_t* s_binof, int max_leaf, float trav_cost, uint32_t& nl, float Llo[3],
+                                           float Lhi[3], float Rlo[3], float Rhi[3]) {
     const uint32_t n = q.n;
     float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (uint32_t i = 0; i < n; ++i) {
@@ -570,15 +791,16 @@ __device__ __forceinline__ bool lane_split(const QEnt& q, uint32_t* s_perm, cons
             const float4 lo = s_lo[j], hi = s_hi[j];
             const float c = axis == 0 ? 0.5f * (lo.x + hi.x) : (axis == 1 ? 0.5f * (lo.y + hi.y) : 0.5f * (lo.z + hi.z));
             const int bi = bin_of(c, cmin, scale, kSmallBins);
+            s_binof[j] = (uint8_t)bi;
             const float pl[3] = {lo.x, lo.y, lo.z}, ph[3] = {hi.x, hi.y, hi.z};
 #pragma unroll
             for (int b = 0; b < kSmallBins; ++b) {
                 const bool m = bi == b;
                 bcnt[b] += m ? 1u : 0u;
 #pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    blo[b][a] = m ? fminf(blo[b][a], pl[a]) : blo[b][a];
-                    bhi[b][a] = m ? fmaxf(bhi[b][a], ph[a]) : bhi[b][a];
+                for (int k = 0; k < 3; ++k) {
+                    blo[b][k] = m ? fminf(blo[b][k], pl[k]) : blo[b][k];
+                    bhi[b][k] = m ? fmaxf(bhi[b][k], ph[k]) : bhi[b][k];
                 }
             }
         }
@@ -595,10 +817,7 @@ __device__ __forceinline__ bool lane_split(const QEnt& q, uint32_t* s_perm, cons
             uint32_t i = q.b, j = q.b + n - 1;  // n >= 2: j >= i
             while (i <= j) {
                 const uint32_t pi = s_perm[i];
-                const float4 lo = s_lo[pi], hi = s_hi[pi];
-                const float c =
-                    axis == 0 ? 0.5f * (lo.x + hi.x) : (axis == 1 ? 0.5f * (lo.y + hi.y) : 0.5f * (lo.z + hi.z));
-                if (bin_of(c, cmin, scale, kSmallBins) <= split) {
+                if ((int)s_binof[pi] <= split) {
                     i++;
                 } else {
                     s_perm[i] = s_perm[j];
@@ -617,7 +836,7 @@ __device__ __forceinline__ bool lane_split(const QEnt& q, uint32_t* s_perm, cons
     return true;
 }
 
-__global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, const uint32_t* order, uint32_t ntasks,
+__global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, uint32_t ntasks,
                                                     uint32_t* prims, const PrimBox* pb, BNodeG* nodes, BNodeG* snodes,
                                                     uint32_t* used, uint32_t* max_depth, uint32_t* err, int max_leaf,
                                                     float trav_cost) {
@@ -625,24 +844,40 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
     __shared__ uint32_t s_id[kSmall], s_perm[kSmall], s_tmp[kSmall];
     __shared__ QEnt s_q[kQueueCap];
     __shared__ CEnt s_stack[kCoopStack];
-    __shared__ float s_bins[kSmallBins][kBinVals];
+    __shared__ int s_bk[kSmallBins * kBinVals];         // wave-split bins (keys)
+    __shared__ uint8_t s_binof[kSmall];                 // each item's bin in its current node
     const uint32_t lane = threadIdx.x;
     if (blockIdx.x >= ntasks) return;
-    // blocks take the tasks largest first; results stay indexed by task
-    const uint32_t ti = order[blockIdx.x];
+    const uint32_t ti = blockIdx.x;
     const SmallTask task = tasks[ti];
     BNodeG* const local = snodes + task.base;
     const uint32_t n0 = task.count;
-    for (uint32_t i = lane; i < n0; i += 64) {
-        const uint32_t id = prims[task.begin + i];
-        const PrimBox p = pb[id];
-        s_id[i] = id;
-        s_perm[i] = i;
-        s_lo[i] = make_float4(p.lo[0], p.lo[1], p.lo[2], 0.f);
-        s_hi[i] = make_float4(p.hi[0], p.hi[1], p.hi[2], 0.f);
+    {
+        // all ids, then all boxes: the gathers of a lane are in flight together
+        constexpr int kPer = kSmall / 64;
+        uint32_t ids[kPer];
+#pragma unroll
+        for (int r = 0; r < kPer; ++r) {
+            const uint32_t i = lane + 64u * r;
+            ids[r] = i < n0 ? prims[task.begin + i] : 0u;
+        }
+        PrimBox p[kPer];
+#pragma unroll
+        for (int r = 0; r < kPer; ++r)
+            if (lane + 64u * r < n0) p[r] = pb[ids[r]];
+#pragma unroll
+        for (int r = 0; r < kPer; ++r) {
+            const uint32_t i = lane + 64u * r;
+            if (i < n0) {
+                s_id[i] = ids[r];
+                s_perm[i] = i;
+                s_lo[i] = make_float4(p[r].lo[0], p[r].lo[1], p[r].lo[2], 0.f);
+                s_hi[i] = make_float4(p[r].hi[0], p[r].hi[1], p[r].hi[2], 0.f);
+            }
+        }
     }
     const BNodeG root = nodes[task.node];
-    __syncthreads();
+    wave_sync();
     uint32_t next = 0;  // scratch nodes used
     uint32_t qtail = 0, md = task.depth;
     bool fail = false;
@@ -676,54 +911,40 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
         uint32_t nl = 0;
         int split = -1;
         if (eaxis > 0.f && isfinite(scale) && e.depth < kMedianDepth) {
-            const int mybin = (int)(lane & 7);
-            uint32_t cnt = 0;
-            float v[12];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                v[k] = v[6 + k] = INFINITY;
-                v[3 + k] = v[9 + k] = -INFINITY;
+            // bins as order-preserving integer keys, filled by LDS atomics (one visit per item)
+            for (uint32_t t = lane; t < (uint32_t)(kSmallBins * kBinVals); t += 64) {
+                const int v = (int)(t % kBinVals);
+                s_bk[t] = v == 0 ? 0 : (((v - 1) % 6) < 3 ? INT_MAX : INT_MIN);
             }
-            for (uint32_t i = lane >> 3; i < e.n; i += 8) {
+            wave_sync();
+            for (uint32_t i = lane; i < e.n; i += 64) {
                 const uint32_t j = s_perm[e.b + i];
                 const float4 lo = s_lo[j], hi = s_hi[j];
                 const float c[3] = {0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
-                if (bin_of(pick3(c, axis), cmin, scale, kSmallBins) == mybin) {
-                    const float pl[3] = {lo.x, lo.y, lo.z}, ph[3] = {hi.x, hi.y, hi.z};
-                    cnt++;
+                const int bi = bin_of(pick3(c, axis), cmin, scale, kSmallBins);
+                s_binof[j] = (uint8_t)bi;
+                const float pl[3] = {lo.x, lo.y, lo.z}, ph[3] = {hi.x, hi.y, hi.z};
+                int* sb = &s_bk[bi * kBinVals];
+                atomicAdd(&sb[0], 1);
 #pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        v[k] = fminf(v[k], pl[k]);
-                        v[3 + k] = fmaxf(v[3 + k], ph[k]);
-                        v[6 + k] = fminf(v[6 + k], c[k]);
-                        v[9 + k] = fmaxf(v[9 + k], c[k]);
-                    }
+                for (int k = 0; k < 3; ++k) {
+                    atomicMin(&sb[1 + k], f2key(pl[k]));
+                    atomicMax(&sb[4 + k], f2key(ph[k]));
+                    atomicMin(&sb[7 + k], f2key(c[k]));
+                    atomicMax(&sb[10 + k], f2key(c[k]));
                 }
             }
-            // merge the 8 groups: lanes of one bin differ in bits 3..5
-            for (int off = 8; off < 64; off <<= 1) {
-                cnt += (uint32_t)__shfl_xor((int)cnt, off);
-#pragma unroll
-                for (int k = 0; k < 12; ++k) {
-                    const float o = __shfl_xor(v[k], off);
-                    v[k] = (k % 6) < 3 ? fminf(v[k], o) : fmaxf(v[k], o);
-                }
-            }
-            if (lane < (uint32_t)kSmallBins) {
-                s_bins[lane][0] = __uint_as_float(cnt);
-#pragma unroll
-                for (int k = 0; k < 12; ++k) s_bins[lane][1 + k] = v[k];
-            }
-            __syncthreads();
+            wave_sync();
             uint32_t bcnt[kSmallBins];
             float blo[kSmallBins][3], bhi[kSmallBins][3];
 #pragma unroll
             for (int b = 0; b < kSmallBins; ++b) {
-                bcnt[b] = __float_as_uint(s_bins[b][0]);
+                const int* sb = &s_bk[b * kBinVals];
+                bcnt[b] = (uint32_t)sb[0];
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    blo[b][k] = s_bins[b][1 + k];
-                    bhi[b][k] = s_bins[b][4 + k];
+                    blo[b][k] = bcnt[b] ? key2f(sb[1 + k]) : INFINITY;
+                    bhi[b][k] = bcnt[b] ? key2f(sb[4 + k]) : -INFINITY;
                 }
             }
             float best;
@@ -734,17 +955,28 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
                     L.clo[k] = R.clo[k] = INFINITY;
                     L.chi[k] = R.chi[k] = -INFINITY;
                 }
+                // the children's centroid bounds: unions of the bins' (in key order, then converted)
+                int kl[3] = {INT_MAX, INT_MAX, INT_MAX}, kh[3] = {INT_MIN, INT_MIN, INT_MIN};
+                int rl[3] = {INT_MAX, INT_MAX, INT_MAX}, rh[3] = {INT_MIN, INT_MIN, INT_MIN};
 #pragma unroll
                 for (int b = 0; b < kSmallBins; ++b) {
                     const bool left = b <= split;
+                    const int* sb = &s_bk[b * kBinVals];
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
-                        const float cl = s_bins[b][7 + k], ch = s_bins[b][10 + k];
-                        L.clo[k] = left ? fminf(L.clo[k], cl) : L.clo[k];
-                        L.chi[k] = left ? fmaxf(L.chi[k], ch) : L.chi[k];
-                        R.clo[k] = left ? R.clo[k] : fminf(R.clo[k], cl);
-                        R.chi[k] = left ? R.chi[k] : fmaxf(R.chi[k], ch);
+                        const int cl = sb[7 + k], ch = sb[10 + k];
+                        kl[k] = left ? min(kl[k], cl) : kl[k];
+                        kh[k] = left ? max(kh[k], ch) : kh[k];
+                        rl[k] = left ? rl[k] : min(rl[k], cl);
+                        rh[k] = left ? rh[k] : max(rh[k], ch);
                     }
+                }
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    L.clo[k] = key2f(kl[k]);
+                    L.chi[k] = key2f(kh[k]);
+                    R.clo[k] = key2f(rl[k]);
+                    R.chi[k] = key2f(rh[k]);
                 }
                 // stable partition of the node's items through s_tmp
                 uint32_t lb = 0, rb = 0;
@@ -754,10 +986,7 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
                     bool left = false;
                     if (valid) {
                         j = s_perm[e.b + k0 + lane];
-                        const float4 lo = s_lo[j], hi = s_hi[j];
-                        const float c = axis == 0 ? 0.5f * (lo.x + hi.x)
-                                                  : (axis == 1 ? 0.5f * (lo.y + hi.y) : 0.5f * (lo.z + hi.z));
-                        left = bin_of(c, cmin, scale, kSmallBins) <= split;
+                        left = (int)s_binof[j] <= split;
                     }
                     const uint64_t lm = __ballot(valid && left), rm = __ballot(valid && !left);
                     if (valid) s_tmp[e.b + (left ? lb + rank_below(lm) : nl + rb + rank_below(rm))] = j;
@@ -765,9 +994,9 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
                     rb += (uint32_t)__popcll(rm);
                 }
                 if (lb != nl) fail = true;
-                __syncthreads();
+                wave_sync();
                 for (uint32_t i = lane; i < e.n; i += 64) s_perm[e.b + i] = s_tmp[e.b + i];
-                __syncthreads();
+                wave_sync();
             }
         }
         if (split < 0) {
@@ -821,12 +1050,12 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
         } else if (lbig || rbig) {
             e = pick_ent(lbig, L, R);
         } else if (sp > 0) {
-            __syncthreads();
+            wave_sync();
             e = s_stack[--sp];
         } else {
             have = false;
         }
-        __syncthreads();
+        wave_sync();
     }
     if (qtail > kQueueCap) fail = true;
     // ------------------------------------------------ nodes of up to kCoop triangles: one lane each
@@ -837,13 +1066,13 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
         QEnt q{};
         if (act) q = s_q[(qhead + lane) % kQueueCap];
         qhead += k;
-        __syncthreads();
+        wave_sync();
         uint32_t nl = 0;
         float Llo[3], Lhi[3], Rlo[3], Rhi[3];
         bool split = false;
         if (act) {
             md = max(md, q.depth);
-            split = lane_split(q, s_perm, s_lo, s_hi, max_leaf, trav_cost, nl, Llo, Lhi, Rlo, Rhi);
+            split = lane_split(q, s_perm, s_lo, s_hi, s_binof, max_leaf, trav_cost, nl, Llo, Lhi, Rlo, Rhi);
         }
         const uint64_t sm = __ballot(split);
         const uint32_t c = next + 2 * rank_below(sm);
@@ -864,9 +1093,9 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
                 QEnt{c + 1, q.b + nl, nr, q.depth + 1, {Rlo[0], Rlo[1], Rlo[2]}, {Rhi[0], Rhi[1], Rhi[2]}};
         qtail += (uint32_t)(__popcll(ml) + __popcll(mr));
         if (qtail - qhead > kQueueCap) fail = true;
-        __syncthreads();
+        wave_sync();
     }
-    __syncthreads();
+    wave_sync();
     // the task's items in their final order
     for (uint32_t i = lane; i < n0; i += 64) prims[task.begin + i] = s_id[s_perm[i]];
     for (int off = 32; off > 0; off >>= 1) md = max(md, (uint32_t)__shfl_xor((int)md, off));
@@ -876,6 +1105,16 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
         atomicMax(max_depth, md);
         if (any_fail) atomicOr(err, 2u);
     }
+}
+
+// Scratch nodes of each small subtree: at most 2 (count - 1).
+__global__ void k_small_scratch(const SmallTask* tasks, uint32_t ntasks, uint32_t* sz) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ntasks) sz[i] = 2 * (tasks[i].count - 1);
+}
+__global__ void k_small_base(SmallTask* tasks, uint32_t ntasks, const uint32_t* first) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ntasks) tasks[i].base = first[i];
 }
 
 // Moves each task's scratch nodes to their final place (first = the task's
@@ -1254,10 +1493,6 @@ float key2f_h(int k) {
     return f;
 }
 
-struct HostTask {
-    uint32_t node, begin, count, depth;
-    float clo[3], chi[3];
-};
 
 }  // namespace
 
@@ -1677,191 +1912,152 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     k_reduce_bounds<<<std::min<unsigned>(1024, blocks_for(N, 256)), 256, 0, c.stream>>>(N, pbuf.as<PrimBox>(),
                                                                                       keys.as<int>());
     VG_HIP(hipGetLastError());
-    VG_HIP(hipMemcpyAsync(hk, keys.p, sizeof(hk), hipMemcpyDeviceToHost, c.stream));
-    VG_HIP(hipStreamSynchronize(c.stream));
 
-    std::vector<BNodeG> hnodes;  // nodes created by the large-node levels
-    hnodes.reserve(1 << 16);
-    BNodeG root{};
-    for (int k = 0; k < 3; ++k) { root.lo[k] = key2f_h(hk[k]); root.hi[k] = key2f_h(hk[3 + k]); }
-    for (int k = 0; k < 3; ++k) { B.lo[k] = root.lo[k]; B.hi[k] = root.hi[k]; }
-    root.left = 0;
-    root.count = (uint32_t)N;
-    hnodes.push_back(root);
-    const float root_area = host_area(root.lo, root.hi);
-
-    std::vector<HostTask> large, small_h;
-    HostTask rt{0, 0, (uint32_t)N, 0, {key2f_h(hk[6]), key2f_h(hk[7]), key2f_h(hk[8])},
-                {key2f_h(hk[9]), key2f_h(hk[10]), key2f_h(hk[11])}};
-    if (N > kSmall) large.push_back(rt);
-    else small_h.push_back(rt);
-
-    // level buffers sized once for the largest level: a binned node holds more than kSmall
+    // level buffers sized once for the largest level: a large node holds more than kSmall
     // triangles, and its chunks number at most count / kChunk + 1
     const size_t tasks_cap = N / (kSmall + 1) + 1, chunks_cap = N / kChunk + tasks_cap + 1;
-    DBuf d_chunks, d_params, d_bins, d_counts, d_bases, d_sweep, d_left, d_lscan;
-    d_chunks.alloc(chunks_cap * sizeof(Chunk));
+    DBuf d_task[2], d_params, d_nchunk, d_chunk0, d_chunks, d_bins, d_counts, d_bases, d_left, d_lscan, d_sweep,
+        d_nlarge, d_nsmall, d_loff, d_soff, d_small, d_cnt, d_ssum;
+    for (auto& t : d_task) t.alloc(tasks_cap * sizeof(LTask));
     d_params.alloc(tasks_cap * sizeof(TaskParam));
+    d_nchunk.alloc(tasks_cap * 4);
+    d_chunk0.alloc(tasks_cap * 4);
+    d_chunks.alloc(chunks_cap * sizeof(Chunk));
     d_bins.alloc(tasks_cap * nb * kBinVals * 4);
     d_counts.alloc(chunks_cap * nb * 4);
-    d_sweep.alloc(tasks_cap * sizeof(SweepOut));
     d_bases.alloc(chunks_cap * sizeof(ChunkBase));
     d_left.alloc(chunks_cap * 4);
     d_lscan.alloc(chunks_cap * 4);
-    c.bvh_up_chunks.alloc(chunks_cap * sizeof(Chunk));
-    c.bvh_up_params.alloc(tasks_cap * sizeof(TaskParam));
-    c.bvh_down.alloc(tasks_cap * sizeof(SweepOut));
-    struct Event {  // destroyed on every path out of the build
-        hipEvent_t e = nullptr;
-        ~Event() { if (e) (void)hipEventDestroy(e); }
-    } sweep_done;
-    VG_HIP(hipEventCreateWithFlags(&sweep_done.e, hipEventDisableTiming));
-    Chunk* chunks = c.bvh_up_chunks.as<Chunk>();
-    TaskParam* params = c.bvh_up_params.as<TaskParam>();
-    const SweepOut* hsweep = c.bvh_down.as<SweepOut>();
-    uint32_t max_depth = 0;
-    while (!large.empty()) {
-        std::vector<HostTask> next;
-        // tasks whose centroids coincide (or that are too deep) split at the median without binning
-        std::vector<HostTask> binned;
-        for (const HostTask& t : large) {
-            max_depth = std::max(max_depth, t.depth);
-            float ext[3] = {t.chi[0] - t.clo[0], t.chi[1] - t.clo[1], t.chi[2] - t.clo[2]};
-            int axis = (ext[1] > ext[0]) ? 1 : 0;
-            if (ext[2] > ext[axis]) axis = 2;
-            float scale = (float)nb / ext[axis];
-            if (!(ext[axis] > 0.f) || !std::isfinite(scale) || t.depth >= (uint32_t)kMedianDepth) {
-                uint32_t nl = t.count / 2;
-                uint32_t ci = (uint32_t)hnodes.size();
-                BNodeG par = hnodes[t.node];
-                BNodeG L = par, R = par;
-                L.left = t.begin; L.count = nl;
-                R.left = t.begin + nl; R.count = t.count - nl;
-                hnodes.push_back(L);
-                hnodes.push_back(R);
-                hnodes[t.node].left = ci;
-                hnodes[t.node].count = 0;
-                HostTask lt = t, rtk = t;
-                lt.node = ci; lt.count = nl; lt.depth = t.depth + 1;
-                rtk.node = ci + 1; rtk.begin = t.begin + nl; rtk.count = t.count - nl; rtk.depth = t.depth + 1;
-                (lt.count > kSmall ? next : small_h).push_back(lt);
-                (rtk.count > kSmall ? next : small_h).push_back(rtk);
-                continue;
-            }
-            binned.push_back(t);
+    d_sweep.alloc(tasks_cap * sizeof(SweepOut));
+    d_nlarge.alloc(tasks_cap * 4);
+    d_nsmall.alloc(tasks_cap * 4);
+    d_loff.alloc(tasks_cap * 4);
+    d_soff.alloc(tasks_cap * 4);
+    d_small.alloc(N * sizeof(SmallTask));  // disjoint subtrees of at least one triangle each
+    d_cnt.alloc((LC_WORDS + 1) * 4);        // + one word for scan totals nobody reads
+    d_ssum.alloc(scan_sums_words(std::max(tasks_cap, chunks_cap)) * 4);
+    uint32_t* cnt = d_cnt.as<uint32_t>();
+    uint32_t* ssum = d_ssum.as<uint32_t>();
+    k_lv_root<<<1, 1, 0, c.stream>>>(keys.as<int>(), (uint32_t)N, nodes.as<BNodeG>(), d_task[0].as<LTask>(),
+                                     d_small.as<SmallTask>(), cnt);
+    VG_HIP(hipGetLastError());
+
+    // The level loop is queued without reading anything back, except each
+    // level's count of large nodes for the next one, copied to pinned memory:
+    // the host waits for level L - 1's count after queueing level L, so the
+    // GPU always has the next level queued, and at most one empty level runs.
+    constexpr int kMaxLevels = 256;
+    c.bvh_counts.alloc((kMaxLevels + LC_WORDS + 12) * 4);
+    uint32_t* pin = c.bvh_counts.as<uint32_t>();
+    struct Events {  // destroyed on every path out of the build
+        std::vector<hipEvent_t> e;
+        ~Events() { for (hipEvent_t x : e) (void)hipEventDestroy(x); }
+    } evs;
+    const unsigned task_blocks = (unsigned)std::min<size_t>(kLevelBlocks, blocks_for(tasks_cap, 256));
+    const unsigned chunk_blocks = (unsigned)std::min<size_t>(4 * kLevelBlocks, chunks_cap);
+    const unsigned chunk_tblocks = (unsigned)std::min<size_t>(kLevelBlocks, blocks_for(chunks_cap, 256));
+    const unsigned sweep_blocks = (unsigned)std::min<size_t>(4 * kLevelBlocks, blocks_for(tasks_cap, 64));
+    for (int L = 0; N > kSmall; ++L) {
+        VG_CHECK(L < kMaxLevels, VIRA_GPU_ERR_HIP, "BVH build: too many large-node levels");
+        const int par = L & 1;
+        const uint32_t* d_nt = cnt + (par ? LC_NT1 : LC_NT0);
+        uint32_t* d_ntn = cnt + (par ? LC_NT0 : LC_NT1);
+        const LTask* cur = d_task[par].as<LTask>();
+        LTask* nxt = d_task[par ^ 1].as<LTask>();
+        k_lv_plan<<<task_blocks, 256, 0, c.stream>>>(cur, d_nt, nb, d_params.as<TaskParam>(),
+                                                     d_nchunk.as<uint32_t>(), cnt);
+        VG_HIP(hipGetLastError());
+        scan_exclusive_u32_dev(c, d_nchunk.as<uint32_t>(), d_chunk0.as<uint32_t>(), d_nt, tasks_cap, ssum,
+                               cnt + LC_NCH);
+        k_lv_chunks<<<kLevelBlocks, 256, 0, c.stream>>>(cur, d_nt, d_params.as<TaskParam>(), d_chunk0.as<uint32_t>(),
+                                                        d_chunks.as<Chunk>());
+        VG_HIP(hipGetLastError());
+        k_init_bins<<<kLevelBlocks, 256, 0, c.stream>>>(d_bins.as<int>(), d_nt, nb);
+        VG_HIP(hipGetLastError());
+        k_bin_large<<<chunk_blocks, kChunkThreads, 0, c.stream>>>(
+            d_chunks.as<Chunk>(), cnt + LC_NCH, d_params.as<TaskParam>(), nb, prims.as<uint32_t>(), pbuf.as<PrimBox>(),
+            d_bins.as<int>(), d_counts.as<uint32_t>(), slot_bin.as<uint8_t>());
+        VG_HIP(hipGetLastError());
+        k_sweep_large<<<sweep_blocks, 64, 0, c.stream>>>(d_bins.as<int>(), nb, cur, d_nt, nodes.as<BNodeG>(),
+                                                         d_params.as<TaskParam>(), d_sweep.as<SweepOut>(),
+                                                         d_nlarge.as<uint32_t>(), d_nsmall.as<uint32_t>(), cnt);
+        VG_HIP(hipGetLastError());
+        k_chunk_left<<<chunk_tblocks, 256, 0, c.stream>>>(d_chunks.as<Chunk>(), cnt + LC_NCH,
+                                                          d_params.as<TaskParam>(), d_counts.as<uint32_t>(), nb,
+                                                          d_left.as<uint32_t>());
+        VG_HIP(hipGetLastError());
+        scan_exclusive_u32_dev(c, d_left.as<uint32_t>(), d_lscan.as<uint32_t>(), cnt + LC_NCH, chunks_cap, ssum,
+                               cnt + LC_WORDS);
+        k_chunk_bases<<<chunk_tblocks, 256, 0, c.stream>>>(d_chunks.as<Chunk>(), cnt + LC_NCH,
+                                                           d_params.as<TaskParam>(), d_sweep.as<SweepOut>(),
+                                                           d_lscan.as<uint32_t>(), d_bases.as<ChunkBase>());
+        VG_HIP(hipGetLastError());
+        // a node without a valid split leaves its triangles where they are (split -1: all go right at
+        // offset nleft = 0, in order); the build is rejected below before they are used
+        k_partition_large<<<chunk_blocks, kChunkThreads, 0, c.stream>>>(
+            d_chunks.as<Chunk>(), cnt + LC_NCH, d_params.as<TaskParam>(), d_bases.as<ChunkBase>(), prims.as<uint32_t>(),
+            slot_bin.as<uint8_t>(), tmp.as<uint32_t>());
+        VG_HIP(hipGetLastError());
+        k_copy_back<<<chunk_blocks, 256, 0, c.stream>>>(d_chunks.as<Chunk>(), cnt + LC_NCH, tmp.as<uint32_t>(),
+                                                        prims.as<uint32_t>());
+        VG_HIP(hipGetLastError());
+        scan_exclusive_u32_dev(c, d_nlarge.as<uint32_t>(), d_loff.as<uint32_t>(), d_nt, tasks_cap, ssum, d_ntn);
+        scan_exclusive_u32_dev(c, d_nsmall.as<uint32_t>(), d_soff.as<uint32_t>(), d_nt, tasks_cap, ssum,
+                               cnt + LC_LSMALL);
+        k_lv_emit<<<task_blocks, 256, 0, c.stream>>>(cur, d_nt, d_sweep.as<SweepOut>(), d_nlarge.as<uint32_t>(),
+                                                     d_nsmall.as<uint32_t>(), d_loff.as<uint32_t>(),
+                                                     d_soff.as<uint32_t>(), nodes.as<BNodeG>(), nxt,
+                                                     d_small.as<SmallTask>(), cnt);
+        VG_HIP(hipGetLastError());
+        k_lv_advance<<<1, 1, 0, c.stream>>>(cnt, par);
+        VG_HIP(hipGetLastError());
+        hipEvent_t ev;
+        VG_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        evs.e.push_back(ev);
+        VG_HIP(hipMemcpyAsync(pin + L, d_ntn, 4, hipMemcpyDeviceToHost, c.stream));
+        VG_HIP(hipEventRecord(ev, c.stream));
+        if (L >= 1) {
+            VG_HIP(hipEventSynchronize(evs.e[L - 1]));
+            if (pin[L - 1] == 0) break;  // level L (queued) has no nodes
         }
-        if (!binned.empty()) {
-            const uint32_t nt = (uint32_t)binned.size();
-            VG_CHECK(nt <= tasks_cap, VIRA_GPU_ERR_HIP, "BVH build: level task overflow");
-            uint32_t nch = 0;
-            for (uint32_t ti = 0; ti < nt; ++ti) {
-                const HostTask& t = binned[ti];
-                float ext[3] = {t.chi[0] - t.clo[0], t.chi[1] - t.clo[1], t.chi[2] - t.clo[2]};
-                int axis = (ext[1] > ext[0]) ? 1 : 0;
-                if (ext[2] > ext[axis]) axis = 2;
-                const uint32_t chunk0 = nch;
-                for (uint32_t b = t.begin; b < t.begin + t.count; b += kChunk) {
-                    VG_CHECK(nch < chunks_cap, VIRA_GPU_ERR_HIP, "BVH build: level chunk overflow");
-                    chunks[nch++] = Chunk{ti, b, std::min(b + kChunk, t.begin + t.count), 0};
-                }
-                params[ti] = TaskParam{axis, t.clo[axis], (float)nb / ext[axis], -1, t.begin, chunk0, nch - chunk0, 0u};
-            }
-            size_t nbv = (size_t)nt * nb * kBinVals;
-            VG_HIP(hipMemcpyAsync(d_chunks.p, chunks, nch * sizeof(Chunk), hipMemcpyHostToDevice, c.stream));
-            VG_HIP(hipMemcpyAsync(d_params.p, params, nt * sizeof(TaskParam), hipMemcpyHostToDevice, c.stream));
-            k_init_bins<<<blocks_for(nbv, 256), 256, 0, c.stream>>>(d_bins.as<int>(), nbv);
-            VG_HIP(hipGetLastError());
-            k_bin_large<<<nch, kChunkThreads, 0, c.stream>>>(
-                d_chunks.as<Chunk>(), d_params.as<TaskParam>(), nb, prims.as<uint32_t>(), pbuf.as<PrimBox>(),
-                d_bins.as<int>(), d_counts.as<uint32_t>(), slot_bin.as<uint8_t>());
-            VG_HIP(hipGetLastError());
-            k_sweep_large<<<blocks_for(nt, 64), 64, 0, c.stream>>>(d_bins.as<int>(), nb, nt, d_params.as<TaskParam>(),
-                                                                   d_sweep.as<SweepOut>());
-            VG_HIP(hipGetLastError());
-            // the decisions go down now; the host builds the next level while the GPU partitions
-            VG_HIP(hipMemcpyAsync(c.bvh_down.p, d_sweep.p, (size_t)nt * sizeof(SweepOut), hipMemcpyDeviceToHost,
-                                  c.stream));
-            VG_HIP(hipEventRecord(sweep_done.e, c.stream));
-            k_chunk_left<<<blocks_for(nch, 256), 256, 0, c.stream>>>(d_chunks.as<Chunk>(), d_params.as<TaskParam>(),
-                                                                     d_counts.as<uint32_t>(), nb, nch,
-                                                                     d_left.as<uint32_t>());
-            VG_HIP(hipGetLastError());
-            scan_exclusive_u32(c, d_left.as<uint32_t>(), d_lscan.as<uint32_t>(), nch, false);
-            k_chunk_bases<<<blocks_for(nch, 256), 256, 0, c.stream>>>(d_chunks.as<Chunk>(), d_params.as<TaskParam>(),
-                                                                      d_sweep.as<SweepOut>(), d_lscan.as<uint32_t>(),
-                                                                      nch, d_bases.as<ChunkBase>());
-            VG_HIP(hipGetLastError());
-            // a node without a valid split leaves its triangles where they are (split -1: all go right at
-            // offset nleft = 0, in order); the build is rejected below before they are used
-            k_partition_large<<<nch, kChunkThreads, 0, c.stream>>>(
-                d_chunks.as<Chunk>(), d_params.as<TaskParam>(), d_bases.as<ChunkBase>(), prims.as<uint32_t>(),
-                slot_bin.as<uint8_t>(), tmp.as<uint32_t>());
-            VG_HIP(hipGetLastError());
-            k_copy_back<<<nch, 256, 0, c.stream>>>(d_chunks.as<Chunk>(), tmp.as<uint32_t>(),
-                                                                       prims.as<uint32_t>());
-            VG_HIP(hipGetLastError());
-            VG_HIP(hipEventSynchronize(sweep_done.e));
-            // children of each binned node from its sweep
-            for (uint32_t ti = 0; ti < nt; ++ti) {
-                const HostTask& t = binned[ti];
-                const SweepOut& o = hsweep[ti];
-                VG_CHECK(o.split >= 0, VIRA_GPU_ERR_HIP, "BVH build: no valid split for a binned node");
-                HostTask lt{}, rtk{};
-                BNodeG L{}, R{};
-                for (int k = 0; k < 3; ++k) {
-                    L.lo[k] = o.lo[0][k]; L.hi[k] = o.hi[0][k];
-                    R.lo[k] = o.lo[1][k]; R.hi[k] = o.hi[1][k];
-                    lt.clo[k] = o.clo[0][k]; lt.chi[k] = o.chi[0][k];
-                    rtk.clo[k] = o.clo[1][k]; rtk.chi[k] = o.chi[1][k];
-                }
-                const uint32_t nl = o.nleft;
-                uint32_t ci = (uint32_t)hnodes.size();
-                L.left = t.begin; L.count = nl;
-                R.left = t.begin + nl; R.count = t.count - nl;
-                hnodes.push_back(L);
-                hnodes.push_back(R);
-                hnodes[t.node].left = ci;
-                hnodes[t.node].count = 0;
-                lt.node = ci; lt.begin = t.begin; lt.count = nl; lt.depth = t.depth + 1;
-                rtk.node = ci + 1; rtk.begin = t.begin + nl; rtk.count = t.count - nl; rtk.depth = t.depth + 1;
-                (lt.count > kSmall ? next : small_h).push_back(lt);
-                (rtk.count > kSmall ? next : small_h).push_back(rtk);
-            }
-        }
-        large.swap(next);
     }
-    // upload the large-level nodes, then finish the small subtrees on the GPU
-    VG_HIP(hipMemcpyAsync(nodes.p, hnodes.data(), hnodes.size() * sizeof(BNodeG), hipMemcpyHostToDevice, c.stream));
+    // the loop's results: node and subtree counts, depth, error flag, root bounds
+    uint32_t* hcnt = pin + kMaxLevels;
+    int* hkeys = reinterpret_cast<int*>(pin + kMaxLevels + LC_WORDS);
+    VG_HIP(hipMemcpyAsync(hcnt, cnt, LC_WORDS * 4, hipMemcpyDeviceToHost, c.stream));
+    VG_HIP(hipMemcpyAsync(hkeys, keys.p, 12 * 4, hipMemcpyDeviceToHost, c.stream));
+    VG_HIP(hipStreamSynchronize(c.stream));
+    VG_CHECK(hcnt[LC_ERR] == 0, VIRA_GPU_ERR_HIP, "BVH build: no valid split for a binned node");
+    for (int k = 0; k < 3; ++k) {
+        B.lo[k] = key2f_h(hkeys[k]);
+        B.hi[k] = key2f_h(hkeys[3 + k]);
+    }
+    const float root_area = host_area(B.lo, B.hi);
+    uint32_t nn = hcnt[LC_NODE];
+    uint32_t max_depth = hcnt[LC_DEPTH];
+    const uint32_t ntasks = hcnt[LC_SMALL];
+    VG_CHECK(ntasks <= N && nn <= 2 * N, VIRA_GPU_ERR_HIP, "BVH build: level loop overflow");
+
+    // the small subtrees, one wave each
     // counter words: 0 unused, 1 max depth, 2 small-builder error flag (word 1 later becomes the emit error flag)
-    DBuf d_small, d_order, d_counter, snodes, d_used, d_first;
+    DBuf snodes, d_used, d_first, d_counter;
     d_counter.alloc(12);
     uint32_t hc[3] = {0u, max_depth, 0u};
     VG_HIP(hipMemcpyAsync(d_counter.p, hc, 12, hipMemcpyHostToDevice, c.stream));
-    uint32_t nn = (uint32_t)hnodes.size();
-    if (!small_h.empty()) {
-        const uint32_t ntasks = (uint32_t)small_h.size();
-        std::vector<SmallTask> st(ntasks);
-        size_t scratch = 0;
-        for (uint32_t i = 0; i < ntasks; ++i) {
-            st[i] = SmallTask{small_h[i].node, small_h[i].begin, small_h[i].count, small_h[i].depth,
-                              (uint32_t)scratch, 0u};
-            scratch += 2 * (size_t)(small_h[i].count - 1);
-        }
-        VG_CHECK(scratch < (1ull << 32), VIRA_GPU_ERR_INVALID, "BVH build: small-subtree scratch overflow");
-        // counting sort of the tasks by size, largest first (sizes 1..kSmall)
-        std::vector<uint32_t> start(kSmall + 2, 0u), order(ntasks);
-        for (uint32_t i = 0; i < ntasks; ++i) start[kSmall - std::min(st[i].count, kSmall) + 1]++;
-        for (uint32_t k = 1; k < kSmall + 2; ++k) start[k] += start[k - 1];
-        for (uint32_t i = 0; i < ntasks; ++i) order[start[kSmall - std::min(st[i].count, kSmall)]++] = i;
-        d_order.alloc((size_t)ntasks * 4);
-        VG_HIP(hipMemcpyAsync(d_order.p, order.data(), (size_t)ntasks * 4, hipMemcpyHostToDevice, c.stream));
-        d_small.alloc(st.size() * sizeof(SmallTask));
-        snodes.alloc(std::max<size_t>(1, scratch) * sizeof(BNodeG));
+    if (ntasks > 0) {
+        // each subtree's scratch region: 2 (count - 1) nodes, at the exclusive scan of the sizes
         d_used.alloc((size_t)ntasks * 4);
         d_first.alloc((size_t)ntasks * 4);
-        VG_HIP(hipMemcpyAsync(d_small.p, st.data(), st.size() * sizeof(SmallTask), hipMemcpyHostToDevice, c.stream));
+        k_small_scratch<<<blocks_for(ntasks, 256), 256, 0, c.stream>>>(d_small.as<SmallTask>(), ntasks,
+                                                                        d_used.as<uint32_t>());
+        VG_HIP(hipGetLastError());
+        const uint32_t scratch = scan_exclusive_u32(c, d_used.as<uint32_t>(), d_first.as<uint32_t>(), ntasks);
+        k_small_base<<<blocks_for(ntasks, 256), 256, 0, c.stream>>>(d_small.as<SmallTask>(), ntasks,
+                                                                     d_first.as<uint32_t>());
+        VG_HIP(hipGetLastError());
+        snodes.alloc(std::max<size_t>(1, scratch) * sizeof(BNodeG));
         k_build_small<<<ntasks, 64, 0, c.stream>>>(
-            d_small.as<SmallTask>(), d_order.as<uint32_t>(), ntasks, prims.as<uint32_t>(), pbuf.as<PrimBox>(), nodes.as<BNodeG>(),
+            d_small.as<SmallTask>(), ntasks, prims.as<uint32_t>(), pbuf.as<PrimBox>(), nodes.as<BNodeG>(),
             snodes.as<BNodeG>(), d_used.as<uint32_t>(), d_counter.as<uint32_t>() + 1, d_counter.as<uint32_t>() + 2,
             max_leaf, opt.traversal_cost > 0.0f ? opt.traversal_cost : 1.0f);
         VG_HIP(hipGetLastError());

@@ -205,10 +205,10 @@ struct Ctx {
     DBuf st_rec, st_count, st_start, st_fill, st_list;
     // denoiser work images: two per pass (direct, indirect) in ping-pong
     DBuf dn_work[4];
-    // scan scratch (block sums) and its pinned total; BVH build staging: the
-    // level's chunks and node parameters (up), the nodes' SAH decisions (down)
+    // scan scratch (block sums) and its pinned total; BVH build: each level's
+    // node count and the level loop's counters, read back in pinned memory
     DBuf scan_sums;
-    PinnedBuf scan_total, bvh_up_chunks, bvh_up_params, bvh_down;
+    PinnedBuf scan_total, bvh_counts;
     // pixel solid angles of the last camera geometry (width, height, z sign, K^-1)
     DBuf rp_omega;
     std::vector<float> rp_key;
@@ -295,5 +295,10 @@ size_t noalloc_max_tiles();
 // Exclusive scan of n uint32 values (in place allowed); returns the total after
 // a stream synchronisation, or 0 without waiting when total_wanted is false.
 uint32_t scan_exclusive_u32(Ctx& c, const uint32_t* in, uint32_t* out, size_t n, bool total_wanted = true);
+// The same with the element count in device memory (*dn <= n_max) and the total
+// written to device memory; sums: scan_sums_words(n_max) words of scratch.
+size_t scan_sums_words(size_t n_max);
+void scan_exclusive_u32_dev(Ctx& c, const uint32_t* in, uint32_t* out, const uint32_t* dn, size_t n_max,
+                            uint32_t* sums, uint32_t* total);
 
 }  // namespace vg

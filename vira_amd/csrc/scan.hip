@@ -1,6 +1,8 @@
 // scan.hip -- device-wide exclusive prefix sum of uint32 (reduce, scan the
 // block sums in one block, rescan and add).  Used by tessellation (triangle
-// ids with holes) and normal adjacency.
+// ids with holes), normal adjacency and the BVH build; the _dev form takes its
+// element count from device memory and leaves the total there, so a caller can
+// queue it without reading anything back.
 #include "context.hpp"
 
 namespace vg {
@@ -36,8 +38,11 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* l
     return excl;
 }
 
-__global__ void __launch_bounds__(kScanBlock) k_scan_reduce(const uint32_t* in, size_t n, uint32_t* sums) {
+// dn (optional): the element count in device memory, at most n (the grid is sized for n)
+__global__ void __launch_bounds__(kScanBlock) k_scan_reduce(const uint32_t* in, size_t n, uint32_t* sums,
+                                                            const uint32_t* dn) {
     __shared__ uint32_t lds[kScanBlock / 64 + 1];
+    if (dn) n = *dn;
     size_t base = (size_t)blockIdx.x * kScanTile;
     uint32_t s = 0;
     for (int k = 0; k < kScanItems; ++k) {
@@ -65,8 +70,9 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_sums(uint32_t* sums, size_t
 }
 
 __global__ void __launch_bounds__(kScanBlock) k_scan_apply(const uint32_t* in, uint32_t* out, size_t n,
-                                                           const uint32_t* sums) {
+                                                           const uint32_t* sums, const uint32_t* dn) {
     __shared__ uint32_t lds[kScanBlock / 64 + 1];
+    if (dn) n = *dn;
     size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
     uint32_t v[kScanItems];
     uint32_t s = 0;
@@ -95,17 +101,31 @@ uint32_t scan_exclusive_u32(Ctx& c, const uint32_t* in, uint32_t* out, size_t n,
     // k_scan_reduce sums each tile's elements in a strided order and
     // k_scan_apply scans them contiguously per thread: both cover the same
     // kScanTile elements of the tile, so the block totals agree.
-    k_scan_reduce<<<(unsigned)nb, kScanBlock, 0, c.stream>>>(in, n, s);
+    k_scan_reduce<<<(unsigned)nb, kScanBlock, 0, c.stream>>>(in, n, s, nullptr);
     VG_HIP(hipGetLastError());
     k_scan_sums<<<1, kScanBlock, 0, c.stream>>>(s, nb, s + nb);
     VG_HIP(hipGetLastError());
-    k_scan_apply<<<(unsigned)nb, kScanBlock, 0, c.stream>>>(in, out, n, s);
+    k_scan_apply<<<(unsigned)nb, kScanBlock, 0, c.stream>>>(in, out, n, s, nullptr);
     VG_HIP(hipGetLastError());
     if (!total_wanted) return 0;  // the scan stays queued on the stream
     uint32_t* total = c.scan_total.as<uint32_t>();
     VG_HIP(hipMemcpyAsync(total, s + nb, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
     VG_HIP(hipStreamSynchronize(c.stream));
     return *total;
+}
+
+size_t scan_sums_words(size_t n_max) { return (n_max + kScanTile - 1) / kScanTile + 1; }
+
+void scan_exclusive_u32_dev(Ctx& c, const uint32_t* in, uint32_t* out, const uint32_t* dn, size_t n_max,
+                            uint32_t* sums, uint32_t* total) {
+    if (n_max == 0) return;
+    const size_t nb = (n_max + kScanTile - 1) / kScanTile;
+    k_scan_reduce<<<(unsigned)nb, kScanBlock, 0, c.stream>>>(in, n_max, sums, dn);
+    VG_HIP(hipGetLastError());
+    k_scan_sums<<<1, kScanBlock, 0, c.stream>>>(sums, nb, total);
+    VG_HIP(hipGetLastError());
+    k_scan_apply<<<(unsigned)nb, kScanBlock, 0, c.stream>>>(in, out, n_max, sums, dn);
+    VG_HIP(hipGetLastError());
 }
 
 }  // namespace vg
