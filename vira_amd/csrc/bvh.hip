@@ -39,7 +39,6 @@ constexpr int kChunkItems = kChunk / kChunkThreads;
 constexpr uint32_t kSmall = 256;       // nodes at or below this size are built per thread
 constexpr int kMaxBins = 32;
 constexpr int kSmallBins = 8;
-constexpr uint32_t kReduceMinChunks = 4;  // nodes of >= 4 chunks bin by wave reductions
 constexpr int kBinVals = 13;           // count, box lo/hi (6), centroid lo/hi (6)
 constexpr int kMedianDepth = 48;       // deeper nodes are split at the object median
 
@@ -238,14 +237,28 @@ __global__ void k_init_bins(int* bins, const uint32_t* d_nt, int nb) {
 
 // One workgroup per chunk of a large node (striding over the level's chunks):
 // bin into LDS, merge into the node's global bins; the bin of every slot is
-// kept for the partition.
+// kept for the partition.  A thread takes kChunkItems consecutive slots, whose
+// triangles are mostly neighbours in the same bin: it accumulates each run of
+// equal bins in registers and adds a run to the LDS bins with one set of
+// atomics, so neither lanes of a wave nor slots of a thread contend for one
+// bin address slot by slot.
+__device__ __forceinline__ void bin_flush(int* s, int b, const int acc[kBinVals]) {
+    int* sb = &s[b * kBinVals];
+    atomicAdd(&sb[0], acc[0]);
+#pragma unroll
+    for (int k = 1; k < kBinVals; ++k) {
+        if (((k - 1) % 6) < 3) atomicMin(&sb[k], acc[k]);
+        else atomicMax(&sb[k], acc[k]);
+    }
+}
+
 __global__ void __launch_bounds__(kChunkThreads) k_bin_large(const Chunk* chunks, const uint32_t* d_nch,
                                                              const TaskParam* params, int nb, const uint32_t* prims,
                                                              const PrimBox* pb, int* bins, uint32_t* chunk_counts,
                                                              uint8_t* slot_bin) {
     __shared__ int s[kMaxBins * kBinVals];
+    constexpr int kBatch = 8;  // slots whose loads are in flight together
     const uint32_t nch = *d_nch;
-    const int lane = threadIdx.x & 63;
     for (uint32_t ci = blockIdx.x; ci < nch; ci += gridDim.x) {
         for (int i = threadIdx.x; i < nb * kBinVals; i += blockDim.x) {
             int v = i % kBinVals;
@@ -254,73 +267,53 @@ __global__ void __launch_bounds__(kChunkThreads) k_bin_large(const Chunk* chunks
         __syncthreads();
         const Chunk ch = chunks[ci];
         const TaskParam tp = params[ch.task];
-        // In a node of many chunks a bin spans a wide slab, neighbouring slots
-        // mostly fall into the same bin, and LDS atomics from a wave would mostly
-        // hit one address.  There the wave reduces the lanes of one bin at a time
-        // (the first pending lane's) with cross-lane min/max/add and one lane adds
-        // the result: as many rounds as distinct bins in the wave.  Small nodes'
-        // bins are narrow, their waves span many bins, and per-lane atomics win.
-        if (tp.nchunk < kReduceMinChunks) {
-            for (uint32_t slot = ch.begin + threadIdx.x; slot < ch.end; slot += blockDim.x) {
-                const PrimBox p = pb[prims[slot]];
-                const float c[3] = {centroid(p, 0), centroid(p, 1), centroid(p, 2)};
+        const uint32_t s0 = ch.begin + threadIdx.x * kChunkItems;
+        int rb = -1;  // bin of the run being accumulated
+        int acc[kBinVals];
+#pragma unroll
+        for (int k = 0; k < kBinVals; ++k) acc[k] = 0;
+#pragma unroll
+        for (int h = 0; h < kChunkItems; h += kBatch) {
+            uint32_t id[kBatch];
+#pragma unroll
+            for (int k = 0; k < kBatch; ++k) {
+                const uint32_t slot = s0 + h + k;
+                id[k] = slot < ch.end ? prims[slot] : 0u;
+            }
+            PrimBox p[kBatch];
+#pragma unroll
+            for (int k = 0; k < kBatch; ++k)
+                if (s0 + h + k < ch.end) p[k] = pb[id[k]];
+#pragma unroll
+            for (int k = 0; k < kBatch; ++k) {
+                const uint32_t slot = s0 + h + k;
+                if (slot >= ch.end) break;
+                const float c[3] = {centroid(p[k], 0), centroid(p[k], 1), centroid(p[k], 2)};
                 const int b = bin_of(pick3(c, tp.axis), tp.cmin, tp.scale, nb);
                 slot_bin[slot] = (uint8_t)b;
-                int* sb = &s[b * kBinVals];
-                atomicAdd(&sb[0], 1);
-                for (int k = 0; k < 3; ++k) {
-                    atomicMin(&sb[1 + k], f2key(p.lo[k]));
-                    atomicMax(&sb[4 + k], f2key(p.hi[k]));
-                    atomicMin(&sb[7 + k], f2key(c[k]));
-                    atomicMax(&sb[10 + k], f2key(c[k]));
-                }
-            }
-        } else {
-            for (uint32_t base = ch.begin; base < ch.end; base += blockDim.x) {
-                const uint32_t slot = base + threadIdx.x;
-                const bool valid = slot < ch.end;
-                int b = -1;
                 int v[kBinVals];
-                if (valid) {
-                    const PrimBox p = pb[prims[slot]];
-                    const float c[3] = {centroid(p, 0), centroid(p, 1), centroid(p, 2)};
-                    b = bin_of(pick3(c, tp.axis), tp.cmin, tp.scale, nb);
-                    slot_bin[slot] = (uint8_t)b;
-                    v[0] = 1;
-                    for (int k = 0; k < 3; ++k) {
-                        v[1 + k] = f2key(p.lo[k]);
-                        v[4 + k] = f2key(p.hi[k]);
-                        v[7 + k] = f2key(c[k]);
-                        v[10 + k] = f2key(c[k]);
-                    }
+                v[0] = 1;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    v[1 + q] = f2key(p[k].lo[q]);
+                    v[4 + q] = f2key(p[k].hi[q]);
+                    v[7 + q] = f2key(c[q]);
+                    v[10 + q] = f2key(c[q]);
                 }
-                uint64_t pending = __ballot(valid);
-                while (pending) {
-                    const int lead = __builtin_ctzll(pending);
-                    const int b0 = __shfl(b, lead, 64);
-                    const bool mine = b == b0;
-                    pending &= ~__ballot(mine);
-                    int r[kBinVals];
-                    r[0] = mine ? 1 : 0;
+                if (b != rb) {
+                    if (rb >= 0) bin_flush(s, rb, acc);
+                    rb = b;
 #pragma unroll
-                    for (int k = 1; k < kBinVals; ++k) r[k] = mine ? v[k] : (((k - 1) % 6) < 3 ? INT_MAX : INT_MIN);
-                    // whole-wave reductions (DPP-based device library routines)
-                    r[0] = __ockl_wfred_add_i32(r[0]);
+                    for (int q = 0; q < kBinVals; ++q) acc[q] = v[q];
+                } else {
+                    acc[0] += 1;
 #pragma unroll
-                    for (int k = 1; k < kBinVals; ++k)
-                        r[k] = ((k - 1) % 6) < 3 ? __ockl_wfred_min_i32(r[k]) : __ockl_wfred_max_i32(r[k]);
-                    if (lane == lead) {
-                        int* sb = &s[b0 * kBinVals];
-                        atomicAdd(&sb[0], r[0]);
-#pragma unroll
-                        for (int k = 1; k < kBinVals; ++k) {
-                            if (((k - 1) % 6) < 3) atomicMin(&sb[k], r[k]);
-                            else atomicMax(&sb[k], r[k]);
-                        }
-                    }
+                    for (int q = 1; q < kBinVals; ++q)
+                        acc[q] = ((q - 1) % 6) < 3 ? min(acc[q], v[q]) : max(acc[q], v[q]);
                 }
             }
         }
+        if (rb >= 0) bin_flush(s, rb, acc);
         __syncthreads();
         if (threadIdx.x < (unsigned)nb) {
             int b = threadIdx.x;
@@ -593,14 +586,17 @@ __global__ void k_lv_emit(const LTask* __restrict__ tasks, const uint32_t* __res
                 }
                 next[lp++] = x;
             } else {
-                small[sp++] = SmallTask{c + h, b, n, depth, 0u, 0u};
+                small[sp++] = SmallTask{c + h, b, n, depth, 2 * b, 0u};
             }
         }
     }
 }
 
-// After a level: its nodes' children are numbered, its small subtrees counted.
-__global__ void k_lv_advance(uint32_t* cnt, int parity) {
+// After a level: its nodes' children are numbered, its small subtrees counted
+// and their range recorded (rec[0] first, rec[1] count) for their builder.
+__global__ void k_lv_advance(uint32_t* cnt, int parity, uint32_t* rec) {
+    rec[0] = cnt[LC_SMALL];
+    rec[1] = cnt[LC_LSMALL];
     cnt[LC_NODE] += 2 * cnt[parity ? LC_NT1 : LC_NT0];
     cnt[LC_SMALL] += cnt[LC_LSMALL];
 }
@@ -836,7 +832,7 @@ __device__ __forceinline__ bool lane_split(const QEnt& q, uint32_t* s_perm, cons
     return true;
 }
 
-__global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, uint32_t ntasks,
+__global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, const uint32_t* range,
                                                     uint32_t* prims, const PrimBox* pb, BNodeG* nodes, BNodeG* snodes,
                                                     uint32_t* used, uint32_t* max_depth, uint32_t* err, int max_leaf,
                                                     float trav_cost) {
@@ -847,8 +843,9 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, uint
     __shared__ int s_bk[kSmallBins * kBinVals];         // wave-split bins (keys)
     __shared__ uint8_t s_binof[kSmall];                 // each item's bin in its current node
     const uint32_t lane = threadIdx.x;
-    if (blockIdx.x >= ntasks) return;
-    const uint32_t ti = blockIdx.x;
+    // the tasks range[0] .. range[0] + range[1] - 1 (the grid may be larger)
+    if (blockIdx.x >= range[1]) return;
+    const uint32_t ti = range[0] + blockIdx.x;
     const SmallTask task = tasks[ti];
     BNodeG* const local = snodes + task.base;
     const uint32_t n0 = task.count;
@@ -1105,16 +1102,6 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, uint
         atomicMax(max_depth, md);
         if (any_fail) atomicOr(err, 2u);
     }
-}
-
-// Scratch nodes of each small subtree: at most 2 (count - 1).
-__global__ void k_small_scratch(const SmallTask* tasks, uint32_t ntasks, uint32_t* sz) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < ntasks) sz[i] = 2 * (tasks[i].count - 1);
-}
-__global__ void k_small_base(SmallTask* tasks, uint32_t ntasks, const uint32_t* first) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < ntasks) tasks[i].base = first[i];
 }
 
 // Moves each task's scratch nodes to their final place (first = the task's
@@ -1938,25 +1925,68 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     d_ssum.alloc(scan_sums_words(std::max(tasks_cap, chunks_cap)) * 4);
     uint32_t* cnt = d_cnt.as<uint32_t>();
     uint32_t* ssum = d_ssum.as<uint32_t>();
+    // small subtrees: scratch nodes (region 2 begin of each), nodes used, and
+    // counter words 0 (unused), 1 (max depth), 2 (builder error flag; word 1
+    // later becomes the emit error flag)
+    constexpr int kMaxLevels = 256;
+    DBuf snodes, d_used, d_first, d_counter, d_rec;
+    snodes.alloc(std::max<size_t>(2, 2 * N) * sizeof(BNodeG));
+    d_used.alloc(N * 4);
+    d_first.alloc(N * 4);
+    d_counter.alloc(12);
+    d_rec.alloc((2 * kMaxLevels + 2) * 4);  // per level: first small subtree, count; last pair: the root's
+    VG_HIP(hipMemsetAsync(d_counter.p, 0, 12, c.stream));
     k_lv_root<<<1, 1, 0, c.stream>>>(keys.as<int>(), (uint32_t)N, nodes.as<BNodeG>(), d_task[0].as<LTask>(),
                                      d_small.as<SmallTask>(), cnt);
     VG_HIP(hipGetLastError());
+    const float trav_cost = opt.traversal_cost > 0.0f ? opt.traversal_cost : 1.0f;
+    uint32_t* rec = d_rec.as<uint32_t>();
+    auto build_small = [&](hipStream_t st, const uint32_t* range, uint32_t grid) {
+        if (grid == 0) return;
+        k_build_small<<<grid, 64, 0, st>>>(d_small.as<SmallTask>(), range, prims.as<uint32_t>(), pbuf.as<PrimBox>(),
+                                           nodes.as<BNodeG>(), snodes.as<BNodeG>(), d_used.as<uint32_t>(),
+                                           d_counter.as<uint32_t>() + 1, d_counter.as<uint32_t>() + 2, max_leaf,
+                                           trav_cost);
+        VG_HIP(hipGetLastError());
+    };
 
     // The level loop is queued without reading anything back, except each
     // level's count of large nodes for the next one, copied to pinned memory:
     // the host waits for level L - 1's count after queueing level L, so the
     // GPU always has the next level queued, and at most one empty level runs.
-    constexpr int kMaxLevels = 256;
+    // The small subtrees a level emits are built on the second stream while
+    // the next levels run (their triangles and nodes are disjoint from those
+    // of every later level); a level of n large nodes emits at most 2 n.
     c.bvh_counts.alloc((kMaxLevels + LC_WORDS + 12) * 4);
     uint32_t* pin = c.bvh_counts.as<uint32_t>();
+    if (!c.aux_stream) VG_HIP(hipStreamCreateWithFlags(&c.aux_stream, hipStreamNonBlocking));
     struct Events {  // destroyed on every path out of the build
         std::vector<hipEvent_t> e;
+        hipEvent_t make() {
+            hipEvent_t x;
+            VG_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+            e.push_back(x);
+            return x;
+        }
         ~Events() { for (hipEvent_t x : e) (void)hipEventDestroy(x); }
     } evs;
+    struct Drain {  // on every path out, both streams are idle before the buffers above are freed
+        hipStream_t a, b;
+        ~Drain() {
+            (void)hipStreamSynchronize(b);
+            (void)hipStreamSynchronize(a);
+        }
+    } drain{c.stream, c.aux_stream};
+    std::vector<hipEvent_t> level_ev;
     const unsigned task_blocks = (unsigned)std::min<size_t>(kLevelBlocks, blocks_for(tasks_cap, 256));
     const unsigned chunk_blocks = (unsigned)std::min<size_t>(4 * kLevelBlocks, chunks_cap);
     const unsigned chunk_tblocks = (unsigned)std::min<size_t>(kLevelBlocks, blocks_for(chunks_cap, 256));
     const unsigned sweep_blocks = (unsigned)std::min<size_t>(4 * kLevelBlocks, blocks_for(tasks_cap, 64));
+    if (N <= kSmall) {
+        const uint32_t root_range[2] = {0u, 1u};
+        VG_HIP(hipMemcpyAsync(rec + 2 * kMaxLevels, root_range, sizeof(root_range), hipMemcpyHostToDevice, c.stream));
+        build_small(c.stream, rec + 2 * kMaxLevels, 1);
+    }
     for (int L = 0; N > kSmall; ++L) {
         VG_CHECK(L < kMaxLevels, VIRA_GPU_ERR_HIP, "BVH build: too many large-node levels");
         const int par = L & 1;
@@ -2009,70 +2039,55 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
                                                      d_soff.as<uint32_t>(), nodes.as<BNodeG>(), nxt,
                                                      d_small.as<SmallTask>(), cnt);
         VG_HIP(hipGetLastError());
-        k_lv_advance<<<1, 1, 0, c.stream>>>(cnt, par);
+        k_lv_advance<<<1, 1, 0, c.stream>>>(cnt, par, rec + 2 * L);
         VG_HIP(hipGetLastError());
-        hipEvent_t ev;
-        VG_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-        evs.e.push_back(ev);
         VG_HIP(hipMemcpyAsync(pin + L, d_ntn, 4, hipMemcpyDeviceToHost, c.stream));
-        VG_HIP(hipEventRecord(ev, c.stream));
+        level_ev.push_back(evs.make());
+        VG_HIP(hipEventRecord(level_ev[L], c.stream));
+        // level L holds 1 node (the root) or pin[L - 1]; its small subtrees are built beside the next levels
+        uint32_t nodes_l = 1;
         if (L >= 1) {
-            VG_HIP(hipEventSynchronize(evs.e[L - 1]));
-            if (pin[L - 1] == 0) break;  // level L (queued) has no nodes
+            VG_HIP(hipEventSynchronize(level_ev[L - 1]));
+            nodes_l = pin[L - 1];
         }
+        if (nodes_l > 0) {
+            VG_HIP(hipStreamWaitEvent(c.aux_stream, level_ev[L], 0));
+            build_small(c.aux_stream, rec + 2 * L, 2 * nodes_l);
+        }
+        if (nodes_l == 0) break;  // level L (queued) has no nodes
     }
-    // the loop's results: node and subtree counts, depth, error flag, root bounds
+    // the main stream waits for the small subtrees, then the loop's results are read:
+    // node and subtree counts, depth, error flags, root bounds
+    {
+        hipEvent_t done = evs.make();
+        VG_HIP(hipEventRecord(done, c.aux_stream));
+        VG_HIP(hipStreamWaitEvent(c.stream, done, 0));
+    }
     uint32_t* hcnt = pin + kMaxLevels;
     int* hkeys = reinterpret_cast<int*>(pin + kMaxLevels + LC_WORDS);
+    uint32_t* hsmall = pin + kMaxLevels + LC_WORDS + 6;  // counter words 1 and 2 of the small builder
     VG_HIP(hipMemcpyAsync(hcnt, cnt, LC_WORDS * 4, hipMemcpyDeviceToHost, c.stream));
-    VG_HIP(hipMemcpyAsync(hkeys, keys.p, 12 * 4, hipMemcpyDeviceToHost, c.stream));
+    VG_HIP(hipMemcpyAsync(hkeys, keys.p, 6 * 4, hipMemcpyDeviceToHost, c.stream));
+    VG_HIP(hipMemcpyAsync(hsmall, d_counter.as<uint32_t>() + 1, 8, hipMemcpyDeviceToHost, c.stream));
     VG_HIP(hipStreamSynchronize(c.stream));
     VG_CHECK(hcnt[LC_ERR] == 0, VIRA_GPU_ERR_HIP, "BVH build: no valid split for a binned node");
+    VG_CHECK(hsmall[1] == 0, VIRA_GPU_ERR_HIP, "BVH build: small-subtree builder failed (stack, queue or partition)");
     for (int k = 0; k < 3; ++k) {
         B.lo[k] = key2f_h(hkeys[k]);
         B.hi[k] = key2f_h(hkeys[3 + k]);
     }
     const float root_area = host_area(B.lo, B.hi);
     uint32_t nn = hcnt[LC_NODE];
-    uint32_t max_depth = hcnt[LC_DEPTH];
+    const uint32_t max_depth = std::max(hcnt[LC_DEPTH], hsmall[0]);
     const uint32_t ntasks = hcnt[LC_SMALL];
-    VG_CHECK(ntasks <= N && nn <= 2 * N, VIRA_GPU_ERR_HIP, "BVH build: level loop overflow");
-
-    // the small subtrees, one wave each
-    // counter words: 0 unused, 1 max depth, 2 small-builder error flag (word 1 later becomes the emit error flag)
-    DBuf snodes, d_used, d_first, d_counter;
-    d_counter.alloc(12);
-    uint32_t hc[3] = {0u, max_depth, 0u};
-    VG_HIP(hipMemcpyAsync(d_counter.p, hc, 12, hipMemcpyHostToDevice, c.stream));
-    if (ntasks > 0) {
-        // each subtree's scratch region: 2 (count - 1) nodes, at the exclusive scan of the sizes
-        d_used.alloc((size_t)ntasks * 4);
-        d_first.alloc((size_t)ntasks * 4);
-        k_small_scratch<<<blocks_for(ntasks, 256), 256, 0, c.stream>>>(d_small.as<SmallTask>(), ntasks,
-                                                                        d_used.as<uint32_t>());
-        VG_HIP(hipGetLastError());
-        const uint32_t scratch = scan_exclusive_u32(c, d_used.as<uint32_t>(), d_first.as<uint32_t>(), ntasks);
-        k_small_base<<<blocks_for(ntasks, 256), 256, 0, c.stream>>>(d_small.as<SmallTask>(), ntasks,
-                                                                     d_first.as<uint32_t>());
-        VG_HIP(hipGetLastError());
-        snodes.alloc(std::max<size_t>(1, scratch) * sizeof(BNodeG));
-        k_build_small<<<ntasks, 64, 0, c.stream>>>(
-            d_small.as<SmallTask>(), ntasks, prims.as<uint32_t>(), pbuf.as<PrimBox>(), nodes.as<BNodeG>(),
-            snodes.as<BNodeG>(), d_used.as<uint32_t>(), d_counter.as<uint32_t>() + 1, d_counter.as<uint32_t>() + 2,
-            max_leaf, opt.traversal_cost > 0.0f ? opt.traversal_cost : 1.0f);
-        VG_HIP(hipGetLastError());
-        const uint32_t small_nodes = scan_exclusive_u32(c, d_used.as<uint32_t>(), d_first.as<uint32_t>(), ntasks);
-        VG_CHECK((size_t)nn + small_nodes <= 2 * N, VIRA_GPU_ERR_HIP, "BVH node count overflow");
-        k_compact_small<<<ntasks, 64, 0, c.stream>>>(d_small.as<SmallTask>(), ntasks, d_used.as<uint32_t>(),
-                                                     d_first.as<uint32_t>(), nn, snodes.as<BNodeG>(),
-                                                     nodes.as<BNodeG>());
-        VG_HIP(hipGetLastError());
-        nn += small_nodes;
-    }
-    VG_HIP(hipMemcpyAsync(hc, d_counter.p, 12, hipMemcpyDeviceToHost, c.stream));
-    VG_HIP(hipStreamSynchronize(c.stream));
-    VG_CHECK(hc[2] == 0, VIRA_GPU_ERR_HIP, "BVH build: small-subtree builder failed (stack, queue or partition)");
-    max_depth = hc[1];
+    VG_CHECK(ntasks >= 1 && ntasks <= N && nn <= 2 * N, VIRA_GPU_ERR_HIP, "BVH build: level loop overflow");
+    // the subtrees' nodes behind the large-level nodes, in subtree order
+    const uint32_t small_nodes = scan_exclusive_u32(c, d_used.as<uint32_t>(), d_first.as<uint32_t>(), ntasks);
+    VG_CHECK((size_t)nn + small_nodes <= 2 * N, VIRA_GPU_ERR_HIP, "BVH node count overflow");
+    k_compact_small<<<ntasks, 64, 0, c.stream>>>(d_small.as<SmallTask>(), ntasks, d_used.as<uint32_t>(),
+                                                 d_first.as<uint32_t>(), nn, snodes.as<BNodeG>(), nodes.as<BNodeG>());
+    VG_HIP(hipGetLastError());
+    nn += small_nodes;
     VG_CHECK(max_depth < 63, VIRA_GPU_ERR_HIP, "BVH deeper than the traversal stack");
     VG_CHECK(nn <= 2 * N, VIRA_GPU_ERR_HIP, "BVH node count overflow");
 

@@ -93,6 +93,10 @@ vira_gpu_status vira_gpu_destroy(vira_gpu_ctx* ctx) {
         vg::Ctx* c = reinterpret_cast<vg::Ctx*>(ctx);
         (void)hipSetDevice(c->device);
         (void)hipStreamSynchronize(c->stream);
+        if (c->aux_stream) {
+            (void)hipStreamSynchronize(c->aux_stream);
+            (void)hipStreamDestroy(c->aux_stream);
+        }
         if (c->own_stream) (void)hipStreamDestroy(c->stream);
         delete c;
     });

@@ -167,6 +167,10 @@ struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    // second stream of the BVH build (small subtrees beside the large levels),
+    // made on first use, default priority; it only ever waits on events of
+    // `stream`, and `stream` waits on it before the build returns
+    hipStream_t aux_stream = nullptr;
     std::vector<std::unique_ptr<MeshG>> meshes;
     int32_t scene_mesh = -1;  // the (first) instance's mesh of the current TLAS
     std::vector<vira_gpu_instance> tlas;
