@@ -622,7 +622,8 @@ __global__ void k_lv_advance(uint32_t* cnt, int parity, uint32_t* rec) {
 constexpr uint32_t kTaskRoot = 0xffffffffu;
 constexpr uint32_t kCoop = 32;              // nodes above this are split by the whole wave
 constexpr uint32_t kQueueCap = kSmall / 2;  // queued nodes hold >= 2 triangles each, disjoint
-constexpr int kCoopStack = 16;              // > log2(kSmall / kCoop) + 1 (smaller child first)
+constexpr int kCoopStack = 8;               // > log2(kSmall / kCoop) + 1 (smaller child first)
+static_assert(kSmall <= 256, "item numbers of a subtree are bytes");
 static_assert(kCoop > 15, "wave-split nodes are never leaves (max_leaf <= 15)");
 static_assert(kSmallBins == 8, "the wave-split binning puts one bin on each lane of a group of 8");
 
@@ -634,6 +635,28 @@ struct CEnt {  // wave-split node: the same plus its centroid bounds
     uint32_t slot, b, n, depth;
     float lo[3], hi[3], clo[3], chi[3];
 };
+
+// A queued node as kept in LDS (32 bytes): first item, items and depth packed in one word.
+struct QEntL {
+    uint32_t slot, bnd;  // bnd: b | n << 8 | depth << 17
+    float lo[3], hi[3];
+};
+__device__ __forceinline__ QEntL q_pack(const QEnt& q) {
+    return QEntL{q.slot, q.b | (q.n << 8) | (q.depth << 17), {q.lo[0], q.lo[1], q.lo[2]}, {q.hi[0], q.hi[1], q.hi[2]}};
+}
+__device__ __forceinline__ QEnt q_unpack(const QEntL& l) {
+    return QEnt{l.slot, l.bnd & 0xffu, (l.bnd >> 8) & 0x1ffu, l.bnd >> 17, {l.lo[0], l.lo[1], l.lo[2]},
+                {l.hi[0], l.hi[1], l.hi[2]}};
+}
+
+// Item j's box, kept in LDS as three float2 (24 bytes): (lo.x, lo.y), (lo.z, hi.x), (hi.y, hi.z).
+struct Box6 {
+    float lo[3], hi[3];
+};
+__device__ __forceinline__ Box6 box_at(const float2* s_box, uint32_t j) {
+    const float2 a = s_box[3 * j], b = s_box[3 * j + 1], c = s_box[3 * j + 2];
+    return Box6{{a.x, a.y, b.x}, {b.y, c.x, c.y}};
+}
 
 // c ? a : b, field by field (a select of whole structs would go through scratch memory)
 __device__ __forceinline__ CEnt pick_ent(bool c, const CEnt& a, const CEnt& b) {
@@ -667,18 +690,17 @@ __device__ __forceinline__ uint32_t rank_below(uint64_t m) {
 }
 
 // Centroid bounds of items [b, b + n) of the wave's list (every lane gets them).
-__device__ __forceinline__ void wave_centroid_bounds(const uint32_t* s_perm, const float4* s_lo,
-                                                     const float4* s_hi, uint32_t b, uint32_t n, uint32_t lane,
+__device__ __forceinline__ void wave_centroid_bounds(const uint8_t* s_perm, const float2* s_box, uint32_t b,
+                                                     uint32_t n, uint32_t lane,
                                                      float clo[3], float chi[3]) {
     float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (uint32_t i = lane; i < n; i += 64) {
-        const uint32_t j = s_perm[b + i];
-        const float4 lo = s_lo[j], hi = s_hi[j];
-        const float c[3] = {0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
+        const Box6 x = box_at(s_box, s_perm[b + i]);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            l[k] = fminf(l[k], c[k]);
-            h[k] = fmaxf(h[k], c[k]);
+            const float c = 0.5f * (x.lo[k] + x.hi[k]);
+            l[k] = fminf(l[k], c);
+            h[k] = fmaxf(h[k], c);
         }
     }
 #pragma unroll
@@ -749,19 +771,17 @@ __device__ __forceinline__ void split_boxes8(const uint32_t bcnt[kSmallBins], co
 // stay in registers (every bin updated under a compare with its compile-time
 // index, no indexing into registers); each item's bin is kept in s_binof for
 // the partition.
-__device__ __forceinline__ bool lane_split(const QEnt& q, uint32_t* s_perm, const float4* s_lo, const float4* s_hi,
-                                           uint8_t* s_binof, int max_leaf, float trav_cost, uint32_t& nl, float Llo[3],
+__device__ __forceinline__ bool lane_split(const QEnt& q, uint8_t* s_perm, const float2* s_box, uint8_t* s_binof, int max_leaf, float trav_cost, uint32_t& nl, float Llo[3],
                                            float Lhi[3], float Rlo[3], float Rhi[3]) {
     const uint32_t n = q.n;
     float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (uint32_t i = 0; i < n; ++i) {
-        const uint32_t j = s_perm[q.b + i];
-        const float4 lo = s_lo[j], hi = s_hi[j];
-        const float c[3] = {0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
+        const Box6 x = box_at(s_box, s_perm[q.b + i]);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            clo[k] = fminf(clo[k], c[k]);
-            chi[k] = fmaxf(chi[k], c[k]);
+            const float c = 0.5f * (x.lo[k] + x.hi[k]);
+            clo[k] = fminf(clo[k], c);
+            chi[k] = fmaxf(chi[k], c);
         }
     }
     const float ext[3] = {chi[0] - clo[0], chi[1] - clo[1], chi[2] - clo[2]};
@@ -784,11 +804,12 @@ __device__ __forceinline__ bool lane_split(const QEnt& q, uint32_t* s_perm, cons
         }
         for (uint32_t i = 0; i < n; ++i) {
             const uint32_t j = s_perm[q.b + i];
-            const float4 lo = s_lo[j], hi = s_hi[j];
-            const float c = axis == 0 ? 0.5f * (lo.x + hi.x) : (axis == 1 ? 0.5f * (lo.y + hi.y) : 0.5f * (lo.z + hi.z));
+            const Box6 x = box_at(s_box, j);
+            const float c = 0.5f * (pick3(x.lo, axis) + pick3(x.hi, axis));
             const int bi = bin_of(c, cmin, scale, kSmallBins);
             s_binof[j] = (uint8_t)bi;
-            const float pl[3] = {lo.x, lo.y, lo.z}, ph[3] = {hi.x, hi.y, hi.z};
+            const float* pl = x.lo;
+            const float* ph = x.hi;
 #pragma unroll
             for (int b = 0; b < kSmallBins; ++b) {
                 const bool m = bi == b;
@@ -812,7 +833,7 @@ __device__ __forceinline__ bool lane_split(const QEnt& q, uint32_t* s_perm, cons
         if (split >= 0) {
             uint32_t i = q.b, j = q.b + n - 1;  // n >= 2: j >= i
             while (i <= j) {
-                const uint32_t pi = s_perm[i];
+                const uint8_t pi = s_perm[i];
                 if ((int)s_binof[pi] <= split) {
                     i++;
                 } else {
@@ -836,9 +857,9 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
                                                     uint32_t* prims, const PrimBox* pb, BNodeG* nodes, BNodeG* snodes,
                                                     uint32_t* used, uint32_t* max_depth, uint32_t* err, int max_leaf,
                                                     float trav_cost) {
-    __shared__ float4 s_lo[kSmall], s_hi[kSmall];
-    __shared__ uint32_t s_id[kSmall], s_perm[kSmall], s_tmp[kSmall];
-    __shared__ QEnt s_q[kQueueCap];
+    __shared__ float2 s_box[3 * kSmall];
+    __shared__ uint8_t s_perm[kSmall], s_tmp[kSmall];  // the items in their current order
+    __shared__ QEntL s_q[kQueueCap];
     __shared__ CEnt s_stack[kCoopStack];
     __shared__ int s_bk[kSmallBins * kBinVals];         // wave-split bins (keys)
     __shared__ uint8_t s_binof[kSmall];                 // each item's bin in its current node
@@ -866,10 +887,10 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
         for (int r = 0; r < kPer; ++r) {
             const uint32_t i = lane + 64u * r;
             if (i < n0) {
-                s_id[i] = ids[r];
-                s_perm[i] = i;
-                s_lo[i] = make_float4(p[r].lo[0], p[r].lo[1], p[r].lo[2], 0.f);
-                s_hi[i] = make_float4(p[r].hi[0], p[r].hi[1], p[r].hi[2], 0.f);
+                s_perm[i] = (uint8_t)i;
+                s_box[3 * i] = make_float2(p[r].lo[0], p[r].lo[1]);
+                s_box[3 * i + 1] = make_float2(p[r].lo[2], p[r].hi[0]);
+                s_box[3 * i + 2] = make_float2(p[r].hi[1], p[r].hi[2]);
             }
         }
     }
@@ -889,11 +910,12 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
         e.n = n0;
         e.depth = task.depth;
         for (int k = 0; k < 3; ++k) { e.lo[k] = root.lo[k]; e.hi[k] = root.hi[k]; }
-        wave_centroid_bounds(s_perm, s_lo, s_hi, 0, n0, lane, e.clo, e.chi);
+        wave_centroid_bounds(s_perm, s_box, 0, n0, lane, e.clo, e.chi);
         have = true;
     } else if (n0 >= 2) {
-        if (lane == 0) s_q[0] = QEnt{kTaskRoot, 0u, n0, task.depth, {root.lo[0], root.lo[1], root.lo[2]},
-                                     {root.hi[0], root.hi[1], root.hi[2]}};
+        if (lane == 0)
+            s_q[0] = q_pack(QEnt{kTaskRoot, 0u, n0, task.depth, {root.lo[0], root.lo[1], root.lo[2]},
+                                 {root.hi[0], root.hi[1], root.hi[2]}});
         qtail = 1;
     }
     while (have) {
@@ -916,11 +938,12 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
             wave_sync();
             for (uint32_t i = lane; i < e.n; i += 64) {
                 const uint32_t j = s_perm[e.b + i];
-                const float4 lo = s_lo[j], hi = s_hi[j];
-                const float c[3] = {0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
+                const Box6 x = box_at(s_box, j);
+                const float c[3] = {0.5f * (x.lo[0] + x.hi[0]), 0.5f * (x.lo[1] + x.hi[1]), 0.5f * (x.lo[2] + x.hi[2])};
                 const int bi = bin_of(pick3(c, axis), cmin, scale, kSmallBins);
                 s_binof[j] = (uint8_t)bi;
-                const float pl[3] = {lo.x, lo.y, lo.z}, ph[3] = {hi.x, hi.y, hi.z};
+                const float* pl = x.lo;
+                const float* ph = x.hi;
                 int* sb = &s_bk[bi * kBinVals];
                 atomicAdd(&sb[0], 1);
 #pragma unroll
@@ -986,7 +1009,7 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
                         left = (int)s_binof[j] <= split;
                     }
                     const uint64_t lm = __ballot(valid && left), rm = __ballot(valid && !left);
-                    if (valid) s_tmp[e.b + (left ? lb + rank_below(lm) : nl + rb + rank_below(rm))] = j;
+                    if (valid) s_tmp[e.b + (left ? lb + rank_below(lm) : nl + rb + rank_below(rm))] = (uint8_t)j;
                     lb += (uint32_t)__popcll(lm);
                     rb += (uint32_t)__popcll(rm);
                 }
@@ -1003,8 +1026,8 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
                 L.lo[k] = R.lo[k] = e.lo[k];
                 L.hi[k] = R.hi[k] = e.hi[k];
             }
-            wave_centroid_bounds(s_perm, s_lo, s_hi, e.b, nl, lane, L.clo, L.chi);
-            wave_centroid_bounds(s_perm, s_lo, s_hi, e.b + nl, e.n - nl, lane, R.clo, R.chi);
+            wave_centroid_bounds(s_perm, s_box, e.b, nl, lane, L.clo, L.chi);
+            wave_centroid_bounds(s_perm, s_box, e.b + nl, e.n - nl, lane, R.clo, R.chi);
         }
         const uint32_t c = next;
         next += 2;
@@ -1025,8 +1048,8 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
             if (k.n > kCoop) return;
             if (k.n >= 2) {
                 if (lane == 0)
-                    s_q[qtail % kQueueCap] = QEnt{k.slot, k.b, k.n, k.depth, {k.lo[0], k.lo[1], k.lo[2]},
-                                                  {k.hi[0], k.hi[1], k.hi[2]}};
+                    s_q[qtail % kQueueCap] = q_pack(QEnt{k.slot, k.b, k.n, k.depth, {k.lo[0], k.lo[1], k.lo[2]},
+                                                         {k.hi[0], k.hi[1], k.hi[2]}});
                 qtail++;
             } else {
                 md = max(md, k.depth);
@@ -1061,7 +1084,7 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
         const uint32_t k = min(64u, qtail - qhead);
         const bool act = lane < k;
         QEnt q{};
-        if (act) q = s_q[(qhead + lane) % kQueueCap];
+        if (act) q = q_unpack(s_q[(qhead + lane) % kQueueCap]);
         qhead += k;
         wave_sync();
         uint32_t nl = 0;
@@ -1069,7 +1092,7 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
         bool split = false;
         if (act) {
             md = max(md, q.depth);
-            split = lane_split(q, s_perm, s_lo, s_hi, s_binof, max_leaf, trav_cost, nl, Llo, Lhi, Rlo, Rhi);
+            split = lane_split(q, s_perm, s_box, s_binof, max_leaf, trav_cost, nl, Llo, Lhi, Rlo, Rhi);
         }
         const uint64_t sm = __ballot(split);
         const uint32_t c = next + 2 * rank_below(sm);
@@ -1084,17 +1107,26 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
         const bool pl = split && nl >= 2, pr = split && nr >= 2;
         const uint64_t ml = __ballot(pl), mr = __ballot(pr);
         const uint32_t pos = qtail + rank_below(ml) + rank_below(mr);
-        if (pl) s_q[pos % kQueueCap] = QEnt{c, q.b, nl, q.depth + 1, {Llo[0], Llo[1], Llo[2]}, {Lhi[0], Lhi[1], Lhi[2]}};
+        if (pl)
+            s_q[pos % kQueueCap] =
+                q_pack(QEnt{c, q.b, nl, q.depth + 1, {Llo[0], Llo[1], Llo[2]}, {Lhi[0], Lhi[1], Lhi[2]}});
         if (pr)
             s_q[(pos + (pl ? 1u : 0u)) % kQueueCap] =
-                QEnt{c + 1, q.b + nl, nr, q.depth + 1, {Rlo[0], Rlo[1], Rlo[2]}, {Rhi[0], Rhi[1], Rhi[2]}};
+                q_pack(QEnt{c + 1, q.b + nl, nr, q.depth + 1, {Rlo[0], Rlo[1], Rlo[2]}, {Rhi[0], Rhi[1], Rhi[2]}});
         qtail += (uint32_t)(__popcll(ml) + __popcll(mr));
         if (qtail - qhead > kQueueCap) fail = true;
         wave_sync();
     }
     wave_sync();
     // the task's items in their final order
-    for (uint32_t i = lane; i < n0; i += 64) prims[task.begin + i] = s_id[s_perm[i]];
+    {
+        // the original ids (still in prims) through LDS (the boxes' space, no longer used): each
+        // store below depends on an LDS read made after every lane's load has landed in LDS
+        uint32_t* s_ids = reinterpret_cast<uint32_t*>(s_box);
+        for (uint32_t i = lane; i < n0; i += 64) s_ids[i] = prims[task.begin + i];
+        wave_sync();
+        for (uint32_t i = lane; i < n0; i += 64) prims[task.begin + i] = s_ids[s_perm[i]];
+    }
     for (int off = 32; off > 0; off >>= 1) md = max(md, (uint32_t)__shfl_xor((int)md, off));
     const bool any_fail = __ballot(fail) != 0;
     if (lane == 0) {
