@@ -406,8 +406,6 @@ __device__ __forceinline__ float half_area(const float lo[3], const float hi[3])
     return ex * ey + ey * ez + ez * ex;
 }
 
-__device__ __forceinline__ float key2f(int k) { return __int_as_float(k >= 0 ? k : (k ^ 0x7fffffff)); }
-
 // SAH sweep of one binned node's bins (vira_blas.ipp:185-254 with bins):
 // right-to-left areas, then the cheapest left-to-right boundary; the split is
 // written to the node's TaskParam for the partition, and the children's boxes
@@ -1954,7 +1952,8 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     d_soff.alloc(tasks_cap * 4);
     d_small.alloc(N * sizeof(SmallTask));  // disjoint subtrees of at least one triangle each
     d_cnt.alloc((LC_WORDS + 1) * 4);        // + one word for scan totals nobody reads
-    d_ssum.alloc(scan_sums_words(std::max(tasks_cap, chunks_cap)) * 4);
+    const size_t ssum_words = scan_sums_words(std::max(tasks_cap, chunks_cap));
+    d_ssum.alloc(ssum_words * 4);
     uint32_t* cnt = d_cnt.as<uint32_t>();
     uint32_t* ssum = d_ssum.as<uint32_t>();
     // small subtrees: scratch nodes (region 2 begin of each), nodes used, and
@@ -2030,7 +2029,7 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
                                                      d_nchunk.as<uint32_t>(), cnt);
         VG_HIP(hipGetLastError());
         scan_exclusive_u32_dev(c, d_nchunk.as<uint32_t>(), d_chunk0.as<uint32_t>(), d_nt, tasks_cap, ssum,
-                               cnt + LC_NCH);
+                               ssum_words, cnt + LC_NCH);
         k_lv_chunks<<<kLevelBlocks, 256, 0, c.stream>>>(cur, d_nt, d_params.as<TaskParam>(), d_chunk0.as<uint32_t>(),
                                                         d_chunks.as<Chunk>());
         VG_HIP(hipGetLastError());
@@ -2049,7 +2048,7 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
                                                           d_left.as<uint32_t>());
         VG_HIP(hipGetLastError());
         scan_exclusive_u32_dev(c, d_left.as<uint32_t>(), d_lscan.as<uint32_t>(), cnt + LC_NCH, chunks_cap, ssum,
-                               cnt + LC_WORDS);
+                               ssum_words, cnt + LC_WORDS);
         k_chunk_bases<<<chunk_tblocks, 256, 0, c.stream>>>(d_chunks.as<Chunk>(), cnt + LC_NCH,
                                                            d_params.as<TaskParam>(), d_sweep.as<SweepOut>(),
                                                            d_lscan.as<uint32_t>(), d_bases.as<ChunkBase>());
@@ -2063,9 +2062,10 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
         k_copy_back<<<chunk_blocks, 256, 0, c.stream>>>(d_chunks.as<Chunk>(), cnt + LC_NCH, tmp.as<uint32_t>(),
                                                         prims.as<uint32_t>());
         VG_HIP(hipGetLastError());
-        scan_exclusive_u32_dev(c, d_nlarge.as<uint32_t>(), d_loff.as<uint32_t>(), d_nt, tasks_cap, ssum, d_ntn);
+        scan_exclusive_u32_dev(c, d_nlarge.as<uint32_t>(), d_loff.as<uint32_t>(), d_nt, tasks_cap, ssum, ssum_words,
+                               d_ntn);
         scan_exclusive_u32_dev(c, d_nsmall.as<uint32_t>(), d_soff.as<uint32_t>(), d_nt, tasks_cap, ssum,
-                               cnt + LC_LSMALL);
+                               ssum_words, cnt + LC_LSMALL);
         k_lv_emit<<<task_blocks, 256, 0, c.stream>>>(cur, d_nt, d_sweep.as<SweepOut>(), d_nlarge.as<uint32_t>(),
                                                      d_nsmall.as<uint32_t>(), d_loff.as<uint32_t>(),
                                                      d_soff.as<uint32_t>(), nodes.as<BNodeG>(), nxt,

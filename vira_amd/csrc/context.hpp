@@ -300,9 +300,10 @@ size_t noalloc_max_tiles();
 // a stream synchronisation, or 0 without waiting when total_wanted is false.
 uint32_t scan_exclusive_u32(Ctx& c, const uint32_t* in, uint32_t* out, size_t n, bool total_wanted = true);
 // The same with the element count in device memory (*dn <= n_max) and the total
-// written to device memory; sums: scan_sums_words(n_max) words of scratch.
+// written to device memory; sums: scratch of sums_words >= scan_sums_words(n_max)
+// words (checked).
 size_t scan_sums_words(size_t n_max);
 void scan_exclusive_u32_dev(Ctx& c, const uint32_t* in, uint32_t* out, const uint32_t* dn, size_t n_max,
-                            uint32_t* sums, uint32_t* total);
+                            uint32_t* sums, size_t sums_words, uint32_t* total);
 
 }  // namespace vg

@@ -117,9 +117,10 @@ uint32_t scan_exclusive_u32(Ctx& c, const uint32_t* in, uint32_t* out, size_t n,
 size_t scan_sums_words(size_t n_max) { return (n_max + kScanTile - 1) / kScanTile + 1; }
 
 void scan_exclusive_u32_dev(Ctx& c, const uint32_t* in, uint32_t* out, const uint32_t* dn, size_t n_max,
-                            uint32_t* sums, uint32_t* total) {
+                            uint32_t* sums, size_t sums_words, uint32_t* total) {
     if (n_max == 0) return;
     const size_t nb = (n_max + kScanTile - 1) / kScanTile;
+    VG_CHECK(sums_words >= nb + 1, VIRA_GPU_ERR_INVALID, "scan: block-sum scratch smaller than the scan needs");
     k_scan_reduce<<<(unsigned)nb, kScanBlock, 0, c.stream>>>(in, n_max, sums, dn);
     VG_HIP(hipGetLastError());
     k_scan_sums<<<1, kScanBlock, 0, c.stream>>>(sums, nb, total);
