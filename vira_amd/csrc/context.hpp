@@ -213,6 +213,8 @@ struct Ctx {
     // node count and the level loop's counters, read back in pinned memory
     DBuf scan_sums;
     PinnedBuf scan_total, bvh_counts;
+    // quipu reads: page-locked staging of a level's decoded heights and albedo images
+    PinnedBuf qld_heights, qld_albedo;
     // pixel solid angles of the last camera geometry (width, height, z sign, K^-1)
     DBuf rp_omega;
     std::vector<float> rp_key;

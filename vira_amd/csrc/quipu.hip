@@ -22,7 +22,10 @@
 #include <cstddef>
 #include <cstdlib>
 #include <cstring>
-#include <fstream>
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
 #include <limits>
 #include <string>
 #include <thread>
@@ -38,26 +41,52 @@ enum : uint16_t {
     kIdFloat = 1, kIdDouble = 2, kIdU8 = 3, kIdU16 = 4, kIdU32 = 5, kIdU64 = 6, kIdDemPyramid = 2002,
 };
 
-// Bounds-checked little-endian reader over the file.
+// Bounds-checked little-endian reader over the file, mapped read-only: image
+// blocks are decoded straight from the mapping, with no copy of the file.
 struct Rd {
-    std::ifstream f;
     std::string path;
-    uint64_t size = 0;
-    explicit Rd(const std::string& p) : f(p, std::ios::binary), path(p) {
-        VG_CHECK(f.good(), VIRA_GPU_ERR_INVALID, "quipu: cannot open " + p);
-        f.seekg(0, std::ios::end);
-        size = (uint64_t)f.tellg();
-        f.seekg(0, std::ios::beg);
+    int fd = -1;
+    const uint8_t* base = nullptr;
+    uint64_t size = 0, at = 0;
+    explicit Rd(const std::string& p) : path(p) {
+        fd = ::open(p.c_str(), O_RDONLY);
+        VG_CHECK(fd >= 0, VIRA_GPU_ERR_INVALID, "quipu: cannot open " + p);
+        struct stat st;
+        if (::fstat(fd, &st) != 0) {
+            ::close(fd);
+            throw Error(VIRA_GPU_ERR_INVALID, "quipu: cannot stat " + p);
+        }
+        size = (uint64_t)st.st_size;
+        if (size > 0) {
+            void* m = ::mmap(nullptr, size, PROT_READ, MAP_PRIVATE | MAP_POPULATE, fd, 0);
+            if (m == MAP_FAILED) {
+                ::close(fd);
+                throw Error(VIRA_GPU_ERR_INVALID, "quipu: cannot map " + p);
+            }
+            base = static_cast<const uint8_t*>(m);
+        }
     }
-    uint64_t pos() { return (uint64_t)f.tellg(); }
-    void seek(uint64_t at) {
-        VG_CHECK(at <= size, VIRA_GPU_ERR_INVALID, "quipu: offset beyond the end of " + path);
-        f.seekg((std::streamoff)at, std::ios::beg);
+    Rd(const Rd&) = delete;
+    Rd& operator=(const Rd&) = delete;
+    ~Rd() {
+        if (base) ::munmap(const_cast<uint8_t*>(base), size);
+        if (fd >= 0) ::close(fd);
+    }
+    uint64_t pos() const { return at; }
+    void seek(uint64_t to) {
+        VG_CHECK(to <= size, VIRA_GPU_ERR_INVALID, "quipu: offset beyond the end of " + path);
+        at = to;
+    }
+    // the next n bytes of the file (checked), and the position moved past them
+    const uint8_t* take(uint64_t n) {
+        VG_CHECK(n <= size - at, VIRA_GPU_ERR_INVALID, "quipu: truncated file " + path);
+        const uint8_t* p = base + at;
+        at += n;
+        return p;
     }
     void bytes(void* dst, uint64_t n) {
-        VG_CHECK(n <= size - std::min(size, pos()), VIRA_GPU_ERR_INVALID, "quipu: truncated file " + path);
-        f.read(static_cast<char*>(dst), (std::streamsize)n);
-        VG_CHECK((uint64_t)f.gcount() == n, VIRA_GPU_ERR_INVALID, "quipu: truncated file " + path);
+        const uint8_t* p = take(n);
+        if (n) std::memcpy(dst, p, n);
     }
     template <class T>
     T get() {
@@ -159,15 +188,18 @@ Proj read_projection(Rd& r) {
 // One stored image (writeImage/readImage, quipu_io.ipp:744-811): class id,
 // resolution, compressed flag, then either size_t bytes + uint64 block size +
 // one LZ4 block of the pixels in memory order (i + j*W), or the raw pixels
-// written with i outer and j inner.  `raw` receives the pixels in memory
-// order, decoded.
+// written with i outer and j inner.  The pixels are decoded in memory order
+// into host memory that `dest(bytes)` returns (page-locked staging, so their
+// copy to the device is one DMA).
 struct Img {
     uint32_t w = 0, h = 0;
     int elem = 0;
     bool compressed = false;
-    std::vector<uint8_t> raw;
+    uint8_t* raw = nullptr;  // w * h * elem bytes, owned by the caller of read_image
+    size_t bytes() const { return (size_t)w * h * (size_t)elem; }
 };
-Img read_image(Rd& r, uint16_t expect_id, int elem, double& decode_ms) {
+template <class Dest>
+Img read_image(Rd& r, uint16_t expect_id, int elem, double& decode_ms, Dest&& dest) {
     Img im;
     const uint16_t id = r.get<uint16_t>();
     VG_CHECK(id == expect_id, VIRA_GPU_ERR_INVALID,
@@ -186,22 +218,20 @@ Img read_image(Rd& r, uint16_t expect_id, int elem, double& decode_ms) {
         VG_CHECK(cn <= left, VIRA_GPU_ERR_INVALID, "quipu: truncated LZ4 block");
         // an LZ4 block expands at most ~255-fold (a match length byte of 255 per input byte)
         VG_CHECK(n <= 255 * cn + 16, VIRA_GPU_ERR_INVALID, "quipu: image larger than its LZ4 block can hold");
-        im.raw.resize(n);
-        std::vector<uint8_t> comp(cn);
-        r.bytes(comp.data(), cn);
+        const uint8_t* comp = r.take(cn);
+        im.raw = dest(n);
         const auto t0 = std::chrono::steady_clock::now();
-        const long long got = lz4_decode_block(comp.data(), cn, im.raw.data(), n);
+        const long long got = lz4_decode_block(comp, cn, im.raw, n);
         decode_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         VG_CHECK(got == (long long)n, VIRA_GPU_ERR_INVALID, "quipu: LZ4 block failed to decompress");
     } else {
         VG_CHECK(n <= left, VIRA_GPU_ERR_INVALID, "quipu: image larger than the rest of its file");
-        im.raw.resize(n);
-        // stored with i outer, j inner: transpose into memory order
-        std::vector<uint8_t> tmp(n);
-        r.bytes(tmp.data(), n);
+        // stored with i outer, j inner: transposed into memory order
+        const uint8_t* src = r.take(n);
+        im.raw = dest(n);
         for (uint64_t i = 0; i < im.w; ++i)
             for (uint64_t j = 0; j < im.h; ++j)
-                std::memcpy(&im.raw[(i + j * im.w) * elem], &tmp[(i * im.h + j) * elem], elem);
+                std::memcpy(&im.raw[(i + j * im.w) * elem], &src[(i * im.h + j) * elem], elem);
     }
     return im;
 }
@@ -419,7 +449,10 @@ struct Level {
     std::vector<float> grey_mm;  // min/max per grey image
 };
 
-Level read_level(const vira_gpu_qld& q, uint64_t offset, bool read_albedos, double& decode_ms) {
+// The level's images go to the context's page-locked staging buffers (grown
+// as needed and kept): heights in one, albedo images one after another in the
+// other.  A read synchronises before it returns, so the next one may reuse them.
+Level read_level(Ctx& c, const vira_gpu_qld& q, uint64_t offset, bool read_albedos, double& decode_ms) {
     Rd r(q.path);
     r.seek(offset);
     Level L;
@@ -431,9 +464,19 @@ Level read_level(const vira_gpu_qld& q, uint64_t offset, bool read_albedos, doub
     VG_CHECK(prec == kIdU8 || prec == kIdU16 || prec == kIdFloat, VIRA_GPU_ERR_INVALID,
              "quipu: height precision class id " + std::to_string(prec));
     L.height_bits = prec == kIdU8 ? 8 : (prec == kIdU16 ? 16 : 32);
-    L.heights = read_image(r, prec, L.height_bits / 8, decode_ms);
+    L.heights = read_image(r, prec, L.height_bits / 8, decode_ms, [&](size_t n) {
+        c.qld_heights.alloc(n);
+        return c.qld_heights.as<uint8_t>();
+    });
     VG_CHECK(L.heights.w >= 2 && L.heights.h >= 2, VIRA_GPU_ERR_INVALID, "quipu: level smaller than 2x2");
-    // albedo (dem_quipu.ipp:176-209)
+    // albedo (dem_quipu.ipp:176-209): grey images of the heights' size, one byte per pixel
+    const size_t npix = (size_t)L.heights.w * L.heights.h;
+    auto grey_slot = [&](size_t k) {
+        return [&c, npix, k](size_t n) {
+            VG_CHECK(n == npix, VIRA_GPU_ERR_INVALID, "quipu: albedo resolution differs from the heights'");
+            return c.qld_albedo.as<uint8_t>() + k * npix;
+        };
+    };
     if (q.has_albedos && read_albedos) {
         const uint8_t type = r.get<uint8_t>();
         if (type == 0) {
@@ -441,19 +484,21 @@ Level read_level(const vira_gpu_qld& q, uint64_t offset, bool read_albedos, doub
             r.bytes(L.constant, sizeof(L.constant));
         } else if (type == 1) {
             L.albedo_kind = 2;
+            c.qld_albedo.alloc(npix);
             L.grey_mm.push_back(r.get<float>());
             L.grey_mm.push_back(r.get<float>());
-            L.grey.push_back(read_image(r, kIdU8, 1, decode_ms));
+            L.grey.push_back(read_image(r, kIdU8, 1, decode_ms, grey_slot(0)));
         } else if (type == 2) {
             L.albedo_kind = 3;
             const uint32_t nc = r.get<uint32_t>();
             VG_CHECK(nc == NB, VIRA_GPU_ERR_INVALID,
                      "quipu: spectral albedo with " + std::to_string(nc) + " channels (8 expected)");
+            c.qld_albedo.alloc(NB * npix);
             (void)r.get<uint8_t>();  // the channels' compressed flag, repeated per image
             for (uint32_t k = 0; k < nc; ++k) {
                 L.grey_mm.push_back(r.get<float>());
                 L.grey_mm.push_back(r.get<float>());
-                L.grey.push_back(read_image(r, kIdU8, 1, decode_ms));
+                L.grey.push_back(read_image(r, kIdU8, 1, decode_ms, grey_slot(k)));
             }
         } else {
             throw Error(VIRA_GPU_ERR_INVALID, "quipu: albedo type " + std::to_string(type));
@@ -525,15 +570,15 @@ void qld_read(Ctx& c, const vira_gpu_qld& q, int level, const vira_gpu_qld_read_
     VG_CHECK(level >= 0 && level < q.levels, VIRA_GPU_ERR_INVALID, "quipu: level out of range");
     const auto t0 = std::chrono::steady_clock::now();
     double decode_ms = 0.0;
-    Level L = read_level(q, q.offset[level], o.read_albedos != 0, decode_ms);
+    Level L = read_level(c, q, q.offset[level], o.read_albedos != 0, decode_ms);
     const auto t1 = std::chrono::steady_clock::now();
     const int W = (int)L.heights.w, H = (int)L.heights.h;
     const size_t n = (size_t)W * H;
     VG_CHECK(n < (1ull << 31), VIRA_GPU_ERR_INVALID, "quipu: level too large for 32-bit vertex ids");
     // quantised samples to the device, dequantised there
     DBuf hq, hbuf, abuf, aq;
-    hq.alloc(L.heights.raw.size());
-    VG_HIP(hipMemcpyAsync(hq.p, L.heights.raw.data(), L.heights.raw.size(), hipMemcpyHostToDevice, c.stream));
+    hq.alloc(L.heights.bytes());
+    VG_HIP(hipMemcpyAsync(hq.p, L.heights.raw, L.heights.bytes(), hipMemcpyHostToDevice, c.stream));
     float* heights = heights_dev;
     if (!heights) {
         hbuf.alloc(n * sizeof(float));
@@ -554,7 +599,7 @@ void qld_read(Ctx& c, const vira_gpu_qld& q, int level, const vira_gpu_qld_read_
         abuf.alloc(ng * n * sizeof(float));
         aq.alloc(ng * n);
         for (size_t g = 0; g < ng; ++g) {
-            VG_HIP(hipMemcpyAsync((uint8_t*)aq.p + g * n, L.grey[g].raw.data(), n, hipMemcpyHostToDevice, c.stream));
+            VG_HIP(hipMemcpyAsync((uint8_t*)aq.p + g * n, L.grey[g].raw, n, hipMemcpyHostToDevice, c.stream));
             launch_fixed<uint8_t>(c, (uint8_t*)aq.p + g * n, n, L.grey_mm[2 * g], L.grey_mm[2 * g + 1],
                                   (float*)abuf.p + g * n);
         }
