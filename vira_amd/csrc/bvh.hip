@@ -3,12 +3,13 @@
 // Replaces Mesh::buildBVH -> EmbreeBLAS (mesh.ipp:396-418,
 // embree_blas.ipp:14-71) and restates the reference's own binned SAH
 // (ViraBLAS, vira_blas.ipp:132-254) for a GPU:
-//   * large nodes (> kSmall triangles) are split level by level: each node is
-//     cut into 4096-slot chunks, one workgroup bins a chunk in LDS and merges
-//     it into the node's global bins; one block per node sweeps the bins on
-//     the GPU (SAH) and the host reads back only each node's decision (104 B)
-//     to lay out the next level; a stable chunked partition (block scan)
-//     moves the triangle ids.  Every step is deterministic.
+//   * large nodes (> kSmall triangles) are split level by level, every
+//     decision taken on the GPU: each node is cut into 4096-slot chunks, one
+//     workgroup bins a chunk in LDS and merges it into the node's global bins,
+//     one thread per node sweeps the bins (SAH), and a stable chunked
+//     partition moves the triangle ids from one id buffer to the other; the
+//     host waits only for each level's node count.  Every step is
+//     deterministic.
 //   * small nodes are finished by one wave each from LDS: the wave splits
 //     nodes of more than 32 triangles together, then every lane splits the
 //     smaller ones alone (binned SAH, in-place partition), the "subdivide"
@@ -61,7 +62,8 @@ __device__ __forceinline__ float centroid_on(const PrimBox& b, int a) {
 __device__ __forceinline__ float pick3(const float v[3], int a) { return a == 0 ? v[0] : (a == 1 ? v[1] : v[2]); }
 
 struct Chunk {
-    uint32_t task, begin, end, pad;
+    uint32_t task, begin, end;
+    uint32_t buf;  // the node's triangle ids are in buffer buf (0: prims, 1: the other buffer)
 };
 struct TaskParam {
     int axis;
@@ -87,7 +89,7 @@ struct ChunkBase {
 struct SmallTask {
     uint32_t node, begin, count, depth;
     uint32_t base;  // first node of the task's scratch region (2 (count - 1) nodes)
-    uint32_t pad;
+    uint32_t buf;   // buffer holding the task's triangle ids (0: prims, 1: the other buffer)
 };
 
 __device__ __forceinline__ int bin_of(float c, float cmin, float scale, int nb) {
@@ -148,6 +150,7 @@ __global__ void __launch_bounds__(256) k_reduce_bounds(size_t n, const PrimBox* 
 struct LTask {  // a node of more than kSmall triangles awaiting its split
     uint32_t node, begin, count, depth;
     float clo[3], chi[3];
+    uint32_t buf;  // buffer holding the node's triangle ids (0: prims, 1: the other buffer)
 };
 // Device counters of the level loop (one word each).
 enum : int {
@@ -181,6 +184,7 @@ __global__ void k_lv_root(const int* keys, uint32_t n, BNodeG* nodes, LTask* tas
     t.begin = 0;
     t.count = n;
     t.depth = 0;
+    t.buf = 0;
     const bool large = n > kSmall;
     if (large) task[0] = t;
     else small[0] = SmallTask{0u, 0u, n, 0u, 0u, 0u};
@@ -223,7 +227,7 @@ __global__ void k_lv_chunks(const LTask* __restrict__ tasks, const uint32_t* __r
         const uint32_t b = tasks[ti].begin, e = b + tasks[ti].count;
         if (lane == 0) params[ti].chunk0 = c0;
         for (uint32_t k = lane; k < nch; k += 64)
-            chunks[c0 + k] = Chunk{ti, b + k * kChunk, min(b + (k + 1) * kChunk, e), 0u};
+            chunks[c0 + k] = Chunk{ti, b + k * kChunk, min(b + (k + 1) * kChunk, e), tasks[ti].buf};
     }
 }
 
@@ -237,11 +241,21 @@ __global__ void k_init_bins(int* bins, const uint32_t* d_nt, int nb) {
 
 // One workgroup per chunk of a large node (striding over the level's chunks):
 // bin into LDS, merge into the node's global bins; the bin of every slot is
-// kept for the partition.  A thread takes kChunkItems consecutive slots, whose
-// triangles are mostly neighbours in the same bin: it accumulates each run of
-// equal bins in registers and adds a run to the LDS bins with one set of
-// atomics, so neither lanes of a wave nor slots of a thread contend for one
-// bin address slot by slot.
+// kept for the partition.  Each wave takes 1024 slots of the chunk in stages
+// of 256: the lanes load slots k*64 + lane (consecutive lanes, consecutive
+// slots, so the loads of the ids and of the mostly consecutive boxes
+// coalesce), bin them and stage the boxes in LDS; then every lane reads back
+// kStageItems consecutive slots, whose triangles are mostly neighbours in the
+// same bin, accumulates each run of equal bins in registers and adds a run to
+// the LDS bins with one set of atomics, so neither lanes of a wave nor slots
+// of a thread contend for one bin address slot by slot.
+constexpr int kStageItems = 4;                           // consecutive slots per lane and stage
+constexpr int kStageSlots = kStageItems * 64;            // slots per wave and stage
+constexpr int kStageWords = kStageSlots + kStageSlots / 32;
+// LDS word of staged slot p: one word of skew per 32 slots, so that lanes
+// reading slots lane * kStageItems + j hit distinct banks
+__device__ __forceinline__ int stage_word(int p) { return p + (p >> 5); }
+
 __device__ __forceinline__ void bin_flush(int* s, int b, const int acc[kBinVals]) {
     int* sb = &s[b * kBinVals];
     atomicAdd(&sb[0], acc[0]);
@@ -252,13 +266,26 @@ __device__ __forceinline__ void bin_flush(int* s, int b, const int acc[kBinVals]
     }
 }
 
+// A wave's LDS accesses execute in program order: ordering those of one wave
+// needs only a compiler barrier, not __syncthreads (whose fence would also
+// wait for every outstanding global store).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __global__ void __launch_bounds__(kChunkThreads) k_bin_large(const Chunk* chunks, const uint32_t* d_nch,
-                                                             const TaskParam* params, int nb, const uint32_t* prims,
-                                                             const PrimBox* pb, int* bins, uint32_t* chunk_counts,
-                                                             uint8_t* slot_bin) {
+                                                             const TaskParam* params, int nb, const uint32_t* ids0,
+                                                             const uint32_t* ids1, const PrimBox* pb, int* bins,
+                                                             uint32_t* chunk_counts, uint8_t* slot_bin) {
     __shared__ int s[kMaxBins * kBinVals];
-    constexpr int kBatch = 8;  // slots whose loads are in flight together
+    __shared__ float s_box[kChunkThreads / 64][6][kStageWords];  // per wave: lo x,y,z, hi x,y,z
+    __shared__ uint8_t s_bin[kChunkThreads / 64][kStageSlots];
     const uint32_t nch = *d_nch;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    float (*bx)[kStageWords] = s_box[w];
+    uint8_t* bn = s_bin[w];
     for (uint32_t ci = blockIdx.x; ci < nch; ci += gridDim.x) {
         for (int i = threadIdx.x; i < nb * kBinVals; i += blockDim.x) {
             int v = i % kBinVals;
@@ -267,51 +294,76 @@ __global__ void __launch_bounds__(kChunkThreads) k_bin_large(const Chunk* chunks
         __syncthreads();
         const Chunk ch = chunks[ci];
         const TaskParam tp = params[ch.task];
-        const uint32_t s0 = ch.begin + threadIdx.x * kChunkItems;
+        const uint32_t* prims = ch.buf ? ids1 : ids0;
         int rb = -1;  // bin of the run being accumulated
         int acc[kBinVals];
 #pragma unroll
         for (int k = 0; k < kBinVals; ++k) acc[k] = 0;
+        for (int st = 0; st < kChunkItems / kStageItems; ++st) {
+            const uint32_t wbase = ch.begin + (uint32_t)w * (kChunkItems * 64) + (uint32_t)st * kStageSlots;
+            if (wbase >= ch.end) break;  // the same for the whole wave
+            uint32_t id[kStageItems];
 #pragma unroll
-        for (int h = 0; h < kChunkItems; h += kBatch) {
-            uint32_t id[kBatch];
-#pragma unroll
-            for (int k = 0; k < kBatch; ++k) {
-                const uint32_t slot = s0 + h + k;
+            for (int k = 0; k < kStageItems; ++k) {
+                const uint32_t slot = wbase + k * 64 + lane;
                 id[k] = slot < ch.end ? prims[slot] : 0u;
             }
-            PrimBox p[kBatch];
+            PrimBox p[kStageItems];
 #pragma unroll
-            for (int k = 0; k < kBatch; ++k)
-                if (s0 + h + k < ch.end) p[k] = pb[id[k]];
+            for (int k = 0; k < kStageItems; ++k)
+                if (wbase + k * 64 + lane < ch.end) p[k] = pb[id[k]];
 #pragma unroll
-            for (int k = 0; k < kBatch; ++k) {
-                const uint32_t slot = s0 + h + k;
-                if (slot >= ch.end) break;
-                const float c[3] = {centroid(p[k], 0), centroid(p[k], 1), centroid(p[k], 2)};
-                const int b = bin_of(pick3(c, tp.axis), tp.cmin, tp.scale, nb);
-                slot_bin[slot] = (uint8_t)b;
+            for (int k = 0; k < kStageItems; ++k) {
+                const uint32_t slot = wbase + k * 64 + lane;
+                if (slot < ch.end) {
+                    const float c[3] = {centroid(p[k], 0), centroid(p[k], 1), centroid(p[k], 2)};
+                    const int b = bin_of(pick3(c, tp.axis), tp.cmin, tp.scale, nb);
+                    slot_bin[slot] = (uint8_t)b;
+                    const int q = stage_word(k * 64 + lane);
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        bx[a][q] = p[k].lo[a];
+                        bx[3 + a][q] = p[k].hi[a];
+                    }
+                    bn[k * 64 + lane] = (uint8_t)b;
+                }
+            }
+            wave_sync();
+#pragma unroll
+            for (int j = 0; j < kStageItems; ++j) {
+                const int pos = lane * kStageItems + j;
+                if (wbase + pos >= ch.end) break;
+                const int q = stage_word(pos);
+                PrimBox pj;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    pj.lo[a] = bx[a][q];
+                    pj.hi[a] = bx[3 + a][q];
+                }
+                const int b = bn[pos];
                 int v[kBinVals];
                 v[0] = 1;
 #pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    v[1 + q] = f2key(p[k].lo[q]);
-                    v[4 + q] = f2key(p[k].hi[q]);
-                    v[7 + q] = f2key(c[q]);
-                    v[10 + q] = f2key(c[q]);
+                for (int a = 0; a < 3; ++a) {
+                    const float c = centroid(pj, a);
+                    v[1 + a] = f2key(pj.lo[a]);
+                    v[4 + a] = f2key(pj.hi[a]);
+                    v[7 + a] = f2key(c);
+                    v[10 + a] = f2key(c);
                 }
                 if (b != rb) {
                     if (rb >= 0) bin_flush(s, rb, acc);
                     rb = b;
 #pragma unroll
-                    for (int q = 0; q < kBinVals; ++q) acc[q] = v[q];
+                    for (int q2 = 0; q2 < kBinVals; ++q2) acc[q2] = v[q2];
                 } else {
                     acc[0] += 1;
 #pragma unroll
-                    for (int q = 1; q < kBinVals; ++q)
-                        acc[q] = ((q - 1) % 6) < 3 ? min(acc[q], v[q]) : max(acc[q], v[q]);
+                    for (int q2 = 1; q2 < kBinVals; ++q2)
+                        acc[q2] = ((q2 - 1) % 6) < 3 ? min(acc[q2], v[q2]) : max(acc[q2], v[q2]);
                 }
             }
+            wave_sync();  // the stage is read before the next one overwrites it
         }
         if (rb >= 0) bin_flush(s, rb, acc);
         __syncthreads();
@@ -332,72 +384,62 @@ __global__ void __launch_bounds__(kChunkThreads) k_bin_large(const Chunk* chunks
     }
 }
 
-// Stable partition of one chunk (striding over the level's chunks): lefts to
-// the chunk's left base, rights to its right base, in slot order.
+// number of set bits of m below this lane
+__device__ __forceinline__ uint32_t rank_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Stable partition of one chunk (striding over the level's chunks) from the
+// node's buffer into the other one: lefts to the chunk's left base, rights to
+// its right base, in slot order.  Wave w
+// takes the chunk's slots w*1024 + k*64 + lane (coalesced); a slot's rank
+// among the lefts is the lefts of the waves before, of the steps before and
+// of the lanes below it in its step's ballot.
 __global__ void __launch_bounds__(kChunkThreads) k_partition_large(const Chunk* chunks, const uint32_t* d_nch,
                                                                    const TaskParam* params, const ChunkBase* bases,
-                                                                   const uint32_t* prims, const uint8_t* slot_bin,
-                                                                   uint32_t* tmp) {
-    __shared__ uint32_t lds[kChunkThreads / 64 + 1];
+                                                                   uint32_t* ids0, uint32_t* ids1,
+                                                                   const uint8_t* slot_bin) {
+    __shared__ uint32_t lds[kChunkThreads / 64];
     const uint32_t nch = *d_nch;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (uint32_t ci = blockIdx.x; ci < nch; ci += gridDim.x) {
         const Chunk ch = chunks[ci];
         const TaskParam tp = params[ch.task];
         const ChunkBase cb = bases[ci];
+        const uint32_t* prims = ch.buf ? ids1 : ids0;
+        uint32_t* tmp = ch.buf ? ids0 : ids1;
+        const uint32_t wfirst = (uint32_t)w * (kChunkItems * 64);  // chunk-local index of the wave's first slot
+        const uint32_t s0 = ch.begin + wfirst + lane;
         uint32_t ids[kChunkItems];
-        uint32_t flags = 0, nl = 0;
+        uint64_t lm[kChunkItems];  // lefts of each step
+        uint32_t nl = 0;
+#pragma unroll
         for (int k = 0; k < kChunkItems; ++k) {
-            uint32_t slot = ch.begin + threadIdx.x * kChunkItems + k;
+            const uint32_t slot = s0 + k * 64;
+            bool left = false;
             ids[k] = 0;
             if (slot < ch.end) {
                 ids[k] = prims[slot];
-                if ((int)slot_bin[slot] <= tp.split) {
-                    flags |= 1u << k;
-                    nl++;
-                }
+                left = (int)slot_bin[slot] <= tp.split;
             }
+            lm[k] = __ballot(left);
+            nl += __popcll(lm[k]);
         }
-        // block exclusive scan of nl
-        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-        uint32_t x = nl;
-        for (int off = 1; off < 64; off <<= 1) {
-            uint32_t y = __shfl_up(x, off, 64);
-            if (lane >= off) x += y;
-        }
-        if (lane == 63) lds[wid] = x;
+        if (lane == 0) lds[w] = nl;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t s = 0;
-            for (int w = 0; w < kChunkThreads / 64; ++w) {
-                uint32_t t = lds[w];
-                lds[w] = s;
-                s += t;
-            }
-        }
-        __syncthreads();
-        uint32_t lbefore = x - nl + lds[wid];
-        uint32_t pos = threadIdx.x * kChunkItems;  // chunk-local index of the first item
+        uint32_t lbefore = 0;
+        for (int q = 0; q < w; ++q) lbefore += lds[q];
+#pragma unroll
         for (int k = 0; k < kChunkItems; ++k) {
-            uint32_t slot = ch.begin + pos + k;
-            if (slot >= ch.end) break;
-            uint32_t li = pos + k;
-            if (flags & (1u << k)) {
-                tmp[cb.left + lbefore] = ids[k];
-                lbefore++;
-            } else {
-                uint32_t rbefore = li - lbefore;
-                tmp[cb.right + rbefore] = ids[k];
+            const uint32_t slot = s0 + k * 64;
+            if (slot < ch.end) {
+                const uint32_t l = lbefore + rank_below(lm[k]);
+                if ((lm[k] >> lane) & 1ull) tmp[cb.left + l] = ids[k];
+                else tmp[cb.right + (wfirst + k * 64 + lane - l)] = ids[k];
             }
+            lbefore += __popcll(lm[k]);
         }
-        __syncthreads();
-    }
-}
-
-__global__ void k_copy_back(const Chunk* chunks, const uint32_t* d_nch, const uint32_t* tmp, uint32_t* prims) {
-    const uint32_t nch = *d_nch;
-    for (uint32_t ci = blockIdx.x; ci < nch; ci += gridDim.x) {
-        const Chunk ch = chunks[ci];
-        for (uint32_t s = ch.begin + threadIdx.x; s < ch.end; s += blockDim.x) prims[s] = tmp[s];
+        __syncthreads();  // lds is rewritten by the next chunk
     }
 }
 
@@ -539,9 +581,11 @@ __global__ void k_chunk_bases(const Chunk* __restrict__ chunks, const uint32_t* 
 // base + 2 ti + 1 (boxes from the sweep, triangle ranges as leaves until
 // split), the parent turned internal, and each child queued as a large node of
 // the next level or as a small subtree (positions from the scans of nlarge and
-// nsmall).
+// nsmall).  The children's ids are in the other buffer after a binned split
+// (the partition moved them) and still in the node's after an object-median
+// split.
 __global__ void k_lv_emit(const LTask* __restrict__ tasks, const uint32_t* __restrict__ d_nt,
-                          const SweepOut* __restrict__ sw, const uint32_t* __restrict__ nlarge,
+                          const TaskParam* __restrict__ params, const SweepOut* __restrict__ sw, const uint32_t* __restrict__ nlarge,
                           const uint32_t* __restrict__ nsmall, const uint32_t* __restrict__ loff,
                           const uint32_t* __restrict__ soff, BNodeG* nodes,
                           LTask* __restrict__ next, SmallTask* __restrict__ small, const uint32_t* __restrict__ cnt) {
@@ -569,6 +613,7 @@ __global__ void k_lv_emit(const LTask* __restrict__ tasks, const uint32_t* __res
         nodes[t.node].count = 0;
         uint32_t lp = loff[ti], sp = sbase + soff[ti];
         const uint32_t depth = t.depth + 1;
+        const uint32_t buf = params[ti].nchunk > 0 ? t.buf ^ 1u : t.buf;
         for (int h = 0; h < 2; ++h) {
             const uint32_t n = h == 0 ? nl : nr;
             const uint32_t b = h == 0 ? t.begin : t.begin + nl;
@@ -578,13 +623,14 @@ __global__ void k_lv_emit(const LTask* __restrict__ tasks, const uint32_t* __res
                 x.begin = b;
                 x.count = n;
                 x.depth = depth;
+                x.buf = buf;
                 for (int k = 0; k < 3; ++k) {
                     x.clo[k] = o.clo[h][k];
                     x.chi[k] = o.chi[h][k];
                 }
                 next[lp++] = x;
             } else {
-                small[sp++] = SmallTask{c + h, b, n, depth, 2 * b, 0u};
+                small[sp++] = SmallTask{c + h, b, n, depth, 2 * b, buf};
             }
         }
     }
@@ -671,20 +717,6 @@ __device__ __forceinline__ CEnt pick_ent(bool c, const CEnt& a, const CEnt& b) {
         r.chi[k] = c ? a.chi[k] : b.chi[k];
     }
     return r;
-}
-
-// The builder's workgroup is one wave, whose LDS accesses execute in program
-// order: ordering them needs only a compiler barrier, not __syncthreads (whose
-// fence would also wait for every outstanding global store).
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// number of set bits of m below this lane
-__device__ __forceinline__ uint32_t rank_below(uint64_t m) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
 // Centroid bounds of items [b, b + n) of the wave's list (every lane gets them).
@@ -852,7 +884,7 @@ __device__ __forceinline__ bool lane_split(const QEnt& q, uint8_t* s_perm, const
 }
 
 __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, const uint32_t* range,
-                                                    uint32_t* prims, const PrimBox* pb, BNodeG* nodes, BNodeG* snodes,
+                                                    uint32_t* prims, const uint32_t* alt, const PrimBox* pb, BNodeG* nodes, BNodeG* snodes,
                                                     uint32_t* used, uint32_t* max_depth, uint32_t* err, int max_leaf,
                                                     float trav_cost) {
     __shared__ float2 s_box[3 * kSmall];
@@ -868,6 +900,7 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
     const SmallTask task = tasks[ti];
     BNodeG* const local = snodes + task.base;
     const uint32_t n0 = task.count;
+    const uint32_t* const src = task.buf ? alt : prims;  // the task's ids; its final order goes to prims
     {
         // all ids, then all boxes: the gathers of a lane are in flight together
         constexpr int kPer = kSmall / 64;
@@ -875,7 +908,7 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
 #pragma unroll
         for (int r = 0; r < kPer; ++r) {
             const uint32_t i = lane + 64u * r;
-            ids[r] = i < n0 ? prims[task.begin + i] : 0u;
+            ids[r] = i < n0 ? src[task.begin + i] : 0u;
         }
         PrimBox p[kPer];
 #pragma unroll
@@ -1118,10 +1151,10 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
     wave_sync();
     // the task's items in their final order
     {
-        // the original ids (still in prims) through LDS (the boxes' space, no longer used): each
-        // store below depends on an LDS read made after every lane's load has landed in LDS
+        // the original ids through LDS (the boxes' space, no longer used): each store below
+        // depends on an LDS read made after every lane's load has landed in LDS
         uint32_t* s_ids = reinterpret_cast<uint32_t*>(s_box);
-        for (uint32_t i = lane; i < n0; i += 64) s_ids[i] = prims[task.begin + i];
+        for (uint32_t i = lane; i < n0; i += 64) s_ids[i] = src[task.begin + i];
         wave_sync();
         for (uint32_t i = lane; i < n0; i += 64) prims[task.begin + i] = s_ids[s_perm[i]];
     }
@@ -1913,11 +1946,11 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
         B.built = true;
         return;
     }
-    DBuf pbuf, prims, tmp, nodes, keys, slot_bin;
+    DBuf pbuf, prims, prims2, nodes, keys, slot_bin;  // prims2: the second id buffer of the large levels
     pbuf.alloc(N * sizeof(PrimBox));
     slot_bin.alloc(N);
     prims.alloc(N * 4);
-    tmp.alloc(N * 4);
+    prims2.alloc(N * 4);
     nodes.alloc(2 * N * sizeof(BNodeG));
     keys.alloc(12 * 4);
     k_prim_boxes<<<blocks_for(N, 256), 256, 0, c.stream>>>(N, m.idx.as<uint32_t>(), m.verts.as<GVertex>(),
@@ -1974,8 +2007,9 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     uint32_t* rec = d_rec.as<uint32_t>();
     auto build_small = [&](hipStream_t st, const uint32_t* range, uint32_t grid) {
         if (grid == 0) return;
-        k_build_small<<<grid, 64, 0, st>>>(d_small.as<SmallTask>(), range, prims.as<uint32_t>(), pbuf.as<PrimBox>(),
-                                           nodes.as<BNodeG>(), snodes.as<BNodeG>(), d_used.as<uint32_t>(),
+        k_build_small<<<grid, 64, 0, st>>>(d_small.as<SmallTask>(), range, prims.as<uint32_t>(),
+                                           prims2.as<uint32_t>(), pbuf.as<PrimBox>(), nodes.as<BNodeG>(),
+                                           snodes.as<BNodeG>(), d_used.as<uint32_t>(),
                                            d_counter.as<uint32_t>() + 1, d_counter.as<uint32_t>() + 2, max_leaf,
                                            trav_cost);
         VG_HIP(hipGetLastError());
@@ -2036,8 +2070,9 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
         k_init_bins<<<kLevelBlocks, 256, 0, c.stream>>>(d_bins.as<int>(), d_nt, nb);
         VG_HIP(hipGetLastError());
         k_bin_large<<<chunk_blocks, kChunkThreads, 0, c.stream>>>(
-            d_chunks.as<Chunk>(), cnt + LC_NCH, d_params.as<TaskParam>(), nb, prims.as<uint32_t>(), pbuf.as<PrimBox>(),
-            d_bins.as<int>(), d_counts.as<uint32_t>(), slot_bin.as<uint8_t>());
+            d_chunks.as<Chunk>(), cnt + LC_NCH, d_params.as<TaskParam>(), nb, prims.as<uint32_t>(),
+            prims2.as<uint32_t>(), pbuf.as<PrimBox>(), d_bins.as<int>(), d_counts.as<uint32_t>(),
+            slot_bin.as<uint8_t>());
         VG_HIP(hipGetLastError());
         k_sweep_large<<<sweep_blocks, 64, 0, c.stream>>>(d_bins.as<int>(), nb, cur, d_nt, nodes.as<BNodeG>(),
                                                          d_params.as<TaskParam>(), d_sweep.as<SweepOut>(),
@@ -2057,17 +2092,15 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
         // offset nleft = 0, in order); the build is rejected below before they are used
         k_partition_large<<<chunk_blocks, kChunkThreads, 0, c.stream>>>(
             d_chunks.as<Chunk>(), cnt + LC_NCH, d_params.as<TaskParam>(), d_bases.as<ChunkBase>(), prims.as<uint32_t>(),
-            slot_bin.as<uint8_t>(), tmp.as<uint32_t>());
-        VG_HIP(hipGetLastError());
-        k_copy_back<<<chunk_blocks, 256, 0, c.stream>>>(d_chunks.as<Chunk>(), cnt + LC_NCH, tmp.as<uint32_t>(),
-                                                        prims.as<uint32_t>());
+            prims2.as<uint32_t>(), slot_bin.as<uint8_t>());
         VG_HIP(hipGetLastError());
         scan_exclusive_u32_dev(c, d_nlarge.as<uint32_t>(), d_loff.as<uint32_t>(), d_nt, tasks_cap, ssum, ssum_words,
                                d_ntn);
         scan_exclusive_u32_dev(c, d_nsmall.as<uint32_t>(), d_soff.as<uint32_t>(), d_nt, tasks_cap, ssum,
                                ssum_words, cnt + LC_LSMALL);
-        k_lv_emit<<<task_blocks, 256, 0, c.stream>>>(cur, d_nt, d_sweep.as<SweepOut>(), d_nlarge.as<uint32_t>(),
-                                                     d_nsmall.as<uint32_t>(), d_loff.as<uint32_t>(),
+        k_lv_emit<<<task_blocks, 256, 0, c.stream>>>(cur, d_nt, d_params.as<TaskParam>(), d_sweep.as<SweepOut>(),
+                                                     d_nlarge.as<uint32_t>(), d_nsmall.as<uint32_t>(),
+                                                     d_loff.as<uint32_t>(),
                                                      d_soff.as<uint32_t>(), nodes.as<BNodeG>(), nxt,
                                                      d_small.as<SmallTask>(), cnt);
         VG_HIP(hipGetLastError());
