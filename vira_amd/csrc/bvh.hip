@@ -239,9 +239,10 @@ __global__ void k_init_bins(int* bins, const uint32_t* d_nt, int nb) {
     }
 }
 
-// One workgroup per chunk of a large node (striding over the level's chunks):
-// bin into LDS, merge into the node's global bins; the bin of every slot is
-// kept for the partition.  Each wave takes 1024 slots of the chunk in stages
+// Binning of the level's chunks: a workgroup bins its chunks into LDS and
+// merges them into their nodes' global bins; the bin of every slot and each
+// chunk's count per bin are kept for the partition.  Each wave takes 1024
+// slots of a chunk in stages
 // of 256: the lanes load slots k*64 + lane (consecutive lanes, consecutive
 // slots, so the loads of the ids and of the mostly consecutive boxes
 // coalesce), bin them and stage the boxes in LDS; then every lane reads back
@@ -256,9 +257,11 @@ constexpr int kStageWords = kStageSlots + kStageSlots / 32;
 // reading slots lane * kStageItems + j hit distinct banks
 __device__ __forceinline__ int stage_word(int p) { return p + (p >> 5); }
 
-__device__ __forceinline__ void bin_flush(int* s, int b, const int acc[kBinVals]) {
+// A run of acc[0] slots of bin b: its count to the chunk's counts, its box and
+// centroid keys to the node's bins.
+__device__ __forceinline__ void bin_flush(int* s, int* s_cnt, int b, const int acc[kBinVals]) {
     int* sb = &s[b * kBinVals];
-    atomicAdd(&sb[0], acc[0]);
+    atomicAdd(&s_cnt[b], acc[0]);
 #pragma unroll
     for (int k = 1; k < kBinVals; ++k) {
         if (((k - 1) % 6) < 3) atomicMin(&sb[k], acc[k]);
@@ -266,33 +269,59 @@ __device__ __forceinline__ void bin_flush(int* s, int b, const int acc[kBinVals]
     }
 }
 
-// A wave's LDS accesses execute in program order: ordering those of one wave
-// needs only a compiler barrier, not __syncthreads (whose fence would also
-// wait for every outstanding global store).
+// The block's bins of one node into the node's global bins (after a barrier).
+__device__ __forceinline__ void bins_to_node(const int* s, int nb, int* node_bins) {
+    for (int i = threadIdx.x; i < nb * kBinVals; i += blockDim.x) {
+        const int v = i % kBinVals;
+        if (s[i - v] == 0) continue;  // empty bin
+        if (v == 0) atomicAdd(&node_bins[i], s[i]);
+        else if (((v - 1) % 6) < 3) atomicMin(&node_bins[i], s[i]);
+        else atomicMax(&node_bins[i], s[i]);
+    }
+}
+
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// A workgroup takes a contiguous range of the level's chunks, so the chunks
+// of a node of many chunks mostly share a workgroup: it keeps the node's bins
+// in LDS across them and merges them into the node's global bins once per
+// node (the top levels' few nodes would otherwise take one set of atomics per
+// chunk on the same addresses).
 __global__ void __launch_bounds__(kChunkThreads) k_bin_large(const Chunk* chunks, const uint32_t* d_nch,
                                                              const TaskParam* params, int nb, const uint32_t* ids0,
                                                              const uint32_t* ids1, const PrimBox* pb, int* bins,
                                                              uint32_t* chunk_counts, uint8_t* slot_bin) {
-    __shared__ int s[kMaxBins * kBinVals];
+    __shared__ int s[kMaxBins * kBinVals];  // the current node's bins
+    __shared__ int s_cnt[kMaxBins];         // the current chunk's counts
     __shared__ float s_box[kChunkThreads / 64][6][kStageWords];  // per wave: lo x,y,z, hi x,y,z
     __shared__ uint8_t s_bin[kChunkThreads / 64][kStageSlots];
     const uint32_t nch = *d_nch;
+    const uint32_t c_first = (uint32_t)(((uint64_t)nch * blockIdx.x) / gridDim.x);
+    const uint32_t c_last = (uint32_t)(((uint64_t)nch * (blockIdx.x + 1)) / gridDim.x);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     float (*bx)[kStageWords] = s_box[w];
     uint8_t* bn = s_bin[w];
-    for (uint32_t ci = blockIdx.x; ci < nch; ci += gridDim.x) {
-        for (int i = threadIdx.x; i < nb * kBinVals; i += blockDim.x) {
-            int v = i % kBinVals;
-            s[i] = v == 0 ? 0 : (((v - 1) % 6) < 3 ? INT_MAX : INT_MIN);
-        }
-        __syncthreads();
+    uint32_t node = 0xffffffffu;  // task whose bins s holds
+    for (uint32_t ci = c_first; ci < c_last; ++ci) {
         const Chunk ch = chunks[ci];
+        block_sync();  // the previous chunk's bins and counts are complete
+        if (ch.task != node) {  // the same for the whole block
+            if (node != 0xffffffffu) {
+                bins_to_node(s, nb, &bins[(size_t)node * nb * kBinVals]);
+                block_sync();
+            }
+            for (int i = threadIdx.x; i < nb * kBinVals; i += blockDim.x) {
+                const int v = i % kBinVals;
+                s[i] = v == 0 ? 0 : (((v - 1) % 6) < 3 ? INT_MAX : INT_MIN);
+            }
+            node = ch.task;
+        }
+        if (threadIdx.x < (unsigned)nb) s_cnt[threadIdx.x] = 0;
+        block_sync();
         const TaskParam tp = params[ch.task];
         const uint32_t* prims = ch.buf ? ids1 : ids0;
         int rb = -1;  // bin of the run being accumulated
@@ -352,7 +381,7 @@ __global__ void __launch_bounds__(kChunkThreads) k_bin_large(const Chunk* chunks
                     v[10 + a] = f2key(c);
                 }
                 if (b != rb) {
-                    if (rb >= 0) bin_flush(s, rb, acc);
+                    if (rb >= 0) bin_flush(s, s_cnt, rb, acc);
                     rb = b;
 #pragma unroll
                     for (int q2 = 0; q2 < kBinVals; ++q2) acc[q2] = v[q2];
@@ -365,23 +394,16 @@ __global__ void __launch_bounds__(kChunkThreads) k_bin_large(const Chunk* chunks
             }
             wave_sync();  // the stage is read before the next one overwrites it
         }
-        if (rb >= 0) bin_flush(s, rb, acc);
-        __syncthreads();
+        if (rb >= 0) bin_flush(s, s_cnt, rb, acc);
+        block_sync();
         if (threadIdx.x < (unsigned)nb) {
-            int b = threadIdx.x;
-            const int* sb = &s[b * kBinVals];
-            chunk_counts[(size_t)ci * nb + b] = (uint32_t)sb[0];
-            if (sb[0] > 0) {
-                int* gb = &bins[((size_t)ch.task * nb + b) * kBinVals];
-                atomicAdd(&gb[0], sb[0]);
-                for (int k = 1; k < kBinVals; ++k) {
-                    if (((k - 1) % 6) < 3) atomicMin(&gb[k], sb[k]);
-                    else atomicMax(&gb[k], sb[k]);
-                }
-            }
+            const int c = s_cnt[threadIdx.x];
+            chunk_counts[(size_t)ci * nb + threadIdx.x] = (uint32_t)c;
+            s[threadIdx.x * kBinVals] += c;
         }
-        __syncthreads();
     }
+    block_sync();
+    if (node != 0xffffffffu) bins_to_node(s, nb, &bins[(size_t)node * nb * kBinVals]);
 }
 
 // number of set bits of m below this lane
@@ -391,10 +413,10 @@ __device__ __forceinline__ uint32_t rank_below(uint64_t m) {
 
 // Stable partition of one chunk (striding over the level's chunks) from the
 // node's buffer into the other one: lefts to the chunk's left base, rights to
-// its right base, in slot order.  Wave w
-// takes the chunk's slots w*1024 + k*64 + lane (coalesced); a slot's rank
-// among the lefts is the lefts of the waves before, of the steps before and
-// of the lanes below it in its step's ballot.
+// its right base, in slot order.  Wave w takes the chunk's slots
+// w*1024 + k*64 + lane (coalesced); a slot's rank among the lefts is the
+// lefts of the waves before, of the steps before and of the lanes below it in
+// its step's ballot.
 __global__ void __launch_bounds__(kChunkThreads) k_partition_large(const Chunk* chunks, const uint32_t* d_nch,
                                                                    const TaskParam* params, const ChunkBase* bases,
                                                                    uint32_t* ids0, uint32_t* ids1,
@@ -2045,6 +2067,7 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     std::vector<hipEvent_t> level_ev;
     const unsigned task_blocks = (unsigned)std::min<size_t>(kLevelBlocks, blocks_for(tasks_cap, 256));
     const unsigned chunk_blocks = (unsigned)std::min<size_t>(4 * kLevelBlocks, chunks_cap);
+    const unsigned bin_blocks = (unsigned)std::min<size_t>(kLevelBlocks, chunks_cap);  // all resident at once
     const unsigned chunk_tblocks = (unsigned)std::min<size_t>(kLevelBlocks, blocks_for(chunks_cap, 256));
     const unsigned sweep_blocks = (unsigned)std::min<size_t>(4 * kLevelBlocks, blocks_for(tasks_cap, 64));
     if (N <= kSmall) {
@@ -2069,7 +2092,7 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
         VG_HIP(hipGetLastError());
         k_init_bins<<<kLevelBlocks, 256, 0, c.stream>>>(d_bins.as<int>(), d_nt, nb);
         VG_HIP(hipGetLastError());
-        k_bin_large<<<chunk_blocks, kChunkThreads, 0, c.stream>>>(
+        k_bin_large<<<bin_blocks, kChunkThreads, 0, c.stream>>>(
             d_chunks.as<Chunk>(), cnt + LC_NCH, d_params.as<TaskParam>(), nb, prims.as<uint32_t>(),
             prims2.as<uint32_t>(), pbuf.as<PrimBox>(), d_bins.as<int>(), d_counts.as<uint32_t>(),
             slot_bin.as<uint8_t>());

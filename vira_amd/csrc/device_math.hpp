@@ -14,6 +14,18 @@ namespace vg {
 
 constexpr int NB = 8;  // spectral bins
 
+// __syncthreads() that first waits for the calling wave's LDS operations.
+// For a workgroup barrier the compiler waits for LDS only where other memory
+// has to be ordered as well, and a wave's LDS store still in flight at the
+// barrier was seen on gfx950 to land after another wave's load issued past
+// it (a per-bin count stored by one wave just before the barrier was read
+// back stale by the others).  Used wherever one wave's LDS stores just
+// before a barrier are read by other waves after it.
+__device__ __forceinline__ void block_sync() {
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0); vmcnt and expcnt left alone
+    __syncthreads();
+}
+
 struct f3 {
     float x, y, z;
 };
