@@ -1968,13 +1968,51 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
         B.built = true;
         return;
     }
-    DBuf pbuf, prims, prims2, nodes, keys, slot_bin;  // prims2: the second id buffer of the large levels
-    pbuf.alloc(N * sizeof(PrimBox));
-    slot_bin.alloc(N);
-    prims.alloc(N * 4);
-    prims2.alloc(N * 4);
-    nodes.alloc(2 * N * sizeof(BNodeG));
-    keys.alloc(12 * 4);
+    // The build's scratch, in the context's arena.  Level buffers are sized once for
+    // the largest level: a large node holds more than kSmall triangles, and its chunks
+    // number at most count / kChunk + 1.  Small subtrees: scratch nodes (region
+    // 2 begin of each), nodes used, first node, and counter words 0 (unused), 1 (max
+    // depth), 2 (builder error flag; word 1 later becomes the emit error flag).
+    constexpr int kMaxLevels = 256;
+    const size_t tasks_cap = N / (kSmall + 1) + 1, chunks_cap = N / kChunk + tasks_cap + 1;
+    const size_t ssum_words = scan_sums_words(std::max(tasks_cap, chunks_cap));
+    Span pbuf, prims, prims2, nodes, keys, slot_bin;  // prims2: the second id buffer of the large levels
+    Span d_task[2], d_params, d_nchunk, d_chunk0, d_chunks, d_bins, d_counts, d_bases, d_left, d_lscan, d_sweep,
+        d_nlarge, d_nsmall, d_loff, d_soff, d_small, d_cnt, d_ssum;
+    Span snodes, d_used, d_first, d_counter, d_rec;
+    {
+        ScratchPlan plan;
+        plan.add(pbuf, N * sizeof(PrimBox));
+        plan.add(slot_bin, N);
+        plan.add(prims, N * 4);
+        plan.add(prims2, N * 4);
+        plan.add(nodes, 2 * N * sizeof(BNodeG));
+        plan.add(keys, 12 * 4);
+        for (auto& t : d_task) plan.add(t, tasks_cap * sizeof(LTask));
+        plan.add(d_params, tasks_cap * sizeof(TaskParam));
+        plan.add(d_nchunk, tasks_cap * 4);
+        plan.add(d_chunk0, tasks_cap * 4);
+        plan.add(d_chunks, chunks_cap * sizeof(Chunk));
+        plan.add(d_bins, tasks_cap * nb * kBinVals * 4);
+        plan.add(d_counts, chunks_cap * nb * 4);
+        plan.add(d_bases, chunks_cap * sizeof(ChunkBase));
+        plan.add(d_left, chunks_cap * 4);
+        plan.add(d_lscan, chunks_cap * 4);
+        plan.add(d_sweep, tasks_cap * sizeof(SweepOut));
+        plan.add(d_nlarge, tasks_cap * 4);
+        plan.add(d_nsmall, tasks_cap * 4);
+        plan.add(d_loff, tasks_cap * 4);
+        plan.add(d_soff, tasks_cap * 4);
+        plan.add(d_small, N * sizeof(SmallTask));  // disjoint subtrees of at least one triangle each
+        plan.add(d_cnt, (LC_WORDS + 1) * 4);        // + one word for scan totals nobody reads
+        plan.add(d_ssum, ssum_words * 4);
+        plan.add(snodes, std::max<size_t>(2, 2 * N) * sizeof(BNodeG));
+        plan.add(d_used, N * 4);
+        plan.add(d_first, N * 4);
+        plan.add(d_counter, 12);
+        plan.add(d_rec, (2 * kMaxLevels + 2) * 4);  // per level: first small subtree, count; last pair: the root's
+        plan.place(c.bvh_arena);
+    }
     k_prim_boxes<<<blocks_for(N, 256), 256, 0, c.stream>>>(N, m.idx.as<uint32_t>(), m.verts.as<GVertex>(),
                                                             pbuf.as<PrimBox>(), prims.as<uint32_t>());
     VG_HIP(hipGetLastError());
@@ -1985,42 +2023,8 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
                                                                                       keys.as<int>());
     VG_HIP(hipGetLastError());
 
-    // level buffers sized once for the largest level: a large node holds more than kSmall
-    // triangles, and its chunks number at most count / kChunk + 1
-    const size_t tasks_cap = N / (kSmall + 1) + 1, chunks_cap = N / kChunk + tasks_cap + 1;
-    DBuf d_task[2], d_params, d_nchunk, d_chunk0, d_chunks, d_bins, d_counts, d_bases, d_left, d_lscan, d_sweep,
-        d_nlarge, d_nsmall, d_loff, d_soff, d_small, d_cnt, d_ssum;
-    for (auto& t : d_task) t.alloc(tasks_cap * sizeof(LTask));
-    d_params.alloc(tasks_cap * sizeof(TaskParam));
-    d_nchunk.alloc(tasks_cap * 4);
-    d_chunk0.alloc(tasks_cap * 4);
-    d_chunks.alloc(chunks_cap * sizeof(Chunk));
-    d_bins.alloc(tasks_cap * nb * kBinVals * 4);
-    d_counts.alloc(chunks_cap * nb * 4);
-    d_bases.alloc(chunks_cap * sizeof(ChunkBase));
-    d_left.alloc(chunks_cap * 4);
-    d_lscan.alloc(chunks_cap * 4);
-    d_sweep.alloc(tasks_cap * sizeof(SweepOut));
-    d_nlarge.alloc(tasks_cap * 4);
-    d_nsmall.alloc(tasks_cap * 4);
-    d_loff.alloc(tasks_cap * 4);
-    d_soff.alloc(tasks_cap * 4);
-    d_small.alloc(N * sizeof(SmallTask));  // disjoint subtrees of at least one triangle each
-    d_cnt.alloc((LC_WORDS + 1) * 4);        // + one word for scan totals nobody reads
-    const size_t ssum_words = scan_sums_words(std::max(tasks_cap, chunks_cap));
-    d_ssum.alloc(ssum_words * 4);
     uint32_t* cnt = d_cnt.as<uint32_t>();
     uint32_t* ssum = d_ssum.as<uint32_t>();
-    // small subtrees: scratch nodes (region 2 begin of each), nodes used, and
-    // counter words 0 (unused), 1 (max depth), 2 (builder error flag; word 1
-    // later becomes the emit error flag)
-    constexpr int kMaxLevels = 256;
-    DBuf snodes, d_used, d_first, d_counter, d_rec;
-    snodes.alloc(std::max<size_t>(2, 2 * N) * sizeof(BNodeG));
-    d_used.alloc(N * 4);
-    d_first.alloc(N * 4);
-    d_counter.alloc(12);
-    d_rec.alloc((2 * kMaxLevels + 2) * 4);  // per level: first small subtree, count; last pair: the root's
     VG_HIP(hipMemsetAsync(d_counter.p, 0, 12, c.stream));
     k_lv_root<<<1, 1, 0, c.stream>>>(keys.as<int>(), (uint32_t)N, nodes.as<BNodeG>(), d_task[0].as<LTask>(),
                                      d_small.as<SmallTask>(), cnt);
@@ -2185,19 +2189,20 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     B.width = opt.width == 2 ? 2 : 4;
     uint32_t wide_depth = 0;
     if (B.width == 2) {
-        DBuf flag, iidx;
-        flag.alloc((size_t)nn * 4);
-        iidx.alloc((size_t)nn * 4);
-        k_flag_internal<<<blocks_for(nn, 256), 256, 0, c.stream>>>(nodes.as<BNodeG>(), nn, flag.as<uint32_t>());
+        // flags and internal-node indices in the boxes' space (2 nn words, nn < 2 N)
+        VG_CHECK((size_t)nn * 8 <= pbuf.bytes, VIRA_GPU_ERR_HIP, "BVH2 emission scratch");
+        uint32_t* flag = pbuf.as<uint32_t>();
+        uint32_t* iidx = flag + nn;
+        k_flag_internal<<<blocks_for(nn, 256), 256, 0, c.stream>>>(nodes.as<BNodeG>(), nn, flag);
         VG_HIP(hipGetLastError());
-        uint32_t ninternal = scan_exclusive_u32(c, flag.as<uint32_t>(), iidx.as<uint32_t>(), nn);
+        uint32_t ninternal = scan_exclusive_u32(c, flag, iidx, nn);
         B.num_nodes = std::max<uint32_t>(1, ninternal);
         B.nodes.alloc(B.num_nodes * sizeof(Node2));
         if (ninternal == 0) {
             k_emit_leaf_root<<<1, 1, 0, c.stream>>>(nodes.as<BNodeG>(), B.nodes.as<Node2>());
         } else {
-            k_emit_nodes<<<blocks_for(nn, 256), 256, 0, c.stream>>>(nodes.as<BNodeG>(), nn, iidx.as<uint32_t>(),
-                                                                    B.nodes.as<Node2>(), d_err);
+            k_emit_nodes<<<blocks_for(nn, 256), 256, 0, c.stream>>>(nodes.as<BNodeG>(), nn, iidx, B.nodes.as<Node2>(),
+                                                                    d_err);
         }
         VG_HIP(hipGetLastError());
         wide_depth = max_depth;
@@ -2211,25 +2216,27 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
             B.num_nodes = 1;
             B.level_start = {0u, 1u};
         } else {
-            DBuf qa, qb, qcnt, qoff;
-            qa.alloc(cap * sizeof(CItem));
-            qb.alloc(cap * sizeof(CItem));
-            qcnt.alloc(cap * 4);
-            qoff.alloc(cap * 4);
+            // the level queues, counts and offsets in the boxes' space (24 cap bytes, cap < N)
+            static_assert(2 * sizeof(CItem) + 8 <= sizeof(PrimBox), "collapse queues fit in the boxes' space");
+            VG_CHECK(cap * (2 * sizeof(CItem) + 8) <= pbuf.bytes, VIRA_GPU_ERR_HIP, "BVH4 collapse scratch");
+            CItem* qa = pbuf.as<CItem>();
+            CItem* qb = qa + cap;
+            uint32_t* qcnt = reinterpret_cast<uint32_t*>(qb + cap);
+            uint32_t* qoff = qcnt + cap;
             CItem root{0u, 0u};
-            VG_HIP(hipMemcpyAsync(qa.p, &root, sizeof(root), hipMemcpyHostToDevice, c.stream));
+            VG_HIP(hipMemcpyAsync(qa, &root, sizeof(root), hipMemcpyHostToDevice, c.stream));
             uint32_t total = 1, nin = 1;
             B.level_start = {0u, 1u};  // level 0 = the root; each pass below appends the end of the next level
-            CItem* qin = qa.as<CItem>();
-            CItem* qout = qb.as<CItem>();
+            CItem* qin = qa;
+            CItem* qout = qb;
             while (nin > 0) {
                 k_collapse4_count<<<blocks_for(nin, 256), 256, 0, c.stream>>>(nodes.as<BNodeG>(), qin, nin,
-                                                                              qcnt.as<uint32_t>());
+                                                                              qcnt);
                 VG_HIP(hipGetLastError());
-                const uint32_t nnext = scan_exclusive_u32(c, qcnt.as<uint32_t>(), qoff.as<uint32_t>(), nin);
+                const uint32_t nnext = scan_exclusive_u32(c, qcnt, qoff, nin);
                 VG_CHECK((size_t)total + nnext <= cap, VIRA_GPU_ERR_HIP, "BVH4 collapse overflow");
                 k_collapse4_emit<<<blocks_for(nin, 256), 256, 0, c.stream>>>(nodes.as<BNodeG>(), qin, nin,
-                                                                             qoff.as<uint32_t>(), total,
+                                                                             qoff, total,
                                                                              B.nodes.as<Node4>(), qout, d_err);
                 VG_HIP(hipGetLastError());
                 total += nnext;

@@ -3,10 +3,12 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/vira_gpu.h"
@@ -78,6 +80,32 @@ struct DBuf {
         bytes = n;
     }
     template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// A part of a scratch arena (not owning): placed by ScratchPlan.
+struct Span {
+    void* p = nullptr;
+    size_t bytes = 0;
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+// Lays out scratch buffers of one call in one device allocation kept by the
+// context: the arena grows to the largest call seen and is reused after that,
+// so repeated calls neither allocate nor free (hipFree would also wait for
+// the device).  Parts are 256-byte aligned.
+struct ScratchPlan {
+    std::vector<std::pair<Span*, size_t>> parts;
+    void add(Span& s, size_t n) { parts.emplace_back(&s, n); }
+    void place(DBuf& arena) {
+        size_t total = 0;
+        for (auto& pn : parts) total += (pn.second + 255) & ~size_t(255);
+        arena.alloc(std::max<size_t>(total, 256));
+        size_t off = 0;
+        for (auto& pn : parts) {
+            pn.first->p = arena.as<char>() + off;
+            pn.first->bytes = pn.second;
+            off += (pn.second + 255) & ~size_t(255);
+        }
+    }
 };
 
 // Owning page-locked host buffer (hipHostMalloc), for transfers that must not
@@ -213,6 +241,8 @@ struct Ctx {
     // node count and the level loop's counters, read back in pinned memory
     DBuf scan_sums;
     PinnedBuf scan_total, bvh_counts;
+    // binned-SAH build scratch (about 190 B per triangle), kept for the next build
+    DBuf bvh_arena;
     // quipu reads: page-locked staging of a level's decoded heights and albedo images
     PinnedBuf qld_heights, qld_albedo;
     // pixel solid angles of the last camera geometry (width, height, z sign, K^-1)
