@@ -1303,17 +1303,13 @@ __global__ void k_collapse4_count(const BNodeG* __restrict__ nodes, const CItem*
     cnt[i] = n;
 }
 
-__global__ void k_collapse4_emit(const BNodeG* __restrict__ nodes, const CItem* __restrict__ in, uint32_t nin,
-                                 const uint32_t* __restrict__ off, uint32_t next_base, Node4* __restrict__ out,
-                                 CItem* __restrict__ qout, uint32_t* __restrict__ err) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nin) return;
-    const CItem it = in[i];
-    uint32_t ch[4];
-    const int nc = collapse_children(nodes, it.bnode, ch);
+// The Node4 of item it (children ch[0..nc)) and its internal children queued at
+// qout[q..], numbered next_base + their queue position.
+__device__ __forceinline__ void collapse4_write(const BNodeG* __restrict__ nodes, const CItem it, const uint32_t ch[4],
+                                                int nc, uint32_t q, uint32_t next_base, Node4* __restrict__ out,
+                                                CItem* __restrict__ qout, uint32_t* __restrict__ err) {
     float lo[3][4], hi[3][4];
     uint32_t ref[4];
-    uint32_t q = off[i];
     for (int k = 0; k < 4; ++k) {
         if (k >= nc) {
             for (int a = 0; a < 3; ++a) lo[a][k] = hi[a][k] = 0.0f;
@@ -1348,6 +1344,79 @@ __global__ void k_collapse4_emit(const BNodeG* __restrict__ nodes, const CItem* 
     o.ref = make_uint4(ref[0], ref[1], ref[2], ref[3]);
     o.pad = make_uint4(0u, 0u, 0u, 0u);
     out[it.n4] = o;
+}
+
+__global__ void k_collapse4_emit(const BNodeG* __restrict__ nodes, const CItem* __restrict__ in, uint32_t nin,
+                                 const uint32_t* __restrict__ off, uint32_t next_base, Node4* __restrict__ out,
+                                 CItem* __restrict__ qout, uint32_t* __restrict__ err) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nin) return;
+    const CItem it = in[i];
+    uint32_t ch[4];
+    const int nc = collapse_children(nodes, it.bnode, ch);
+    collapse4_write(nodes, it, ch, nc, off[i], next_base, out, qout, err);
+}
+
+// The first levels of the collapse in one workgroup (one item per thread, while
+// a level has at most kTopItems items), with the same numbering as the
+// count / scan / emit passes.  info: [0] items of the next level (left to those
+// passes; 0 = done), [1] Node4s numbered (cap + 1: more than cap were needed),
+// [2] levels done, [3] queue of the next level (0: qa, 1: qb), [4 + l] Node4s
+// numbered after level l.
+constexpr int kTopItems = 1024;
+constexpr int kTopMaxLevels = 12;
+__global__ void __launch_bounds__(kTopItems) k_collapse4_top(const BNodeG* __restrict__ nodes, CItem* qa, CItem* qb,
+                                                             uint32_t cap, Node4* __restrict__ out, uint32_t* info,
+                                                             uint32_t* __restrict__ err) {
+    __shared__ uint32_t s_wsum[kTopItems / 64];
+    const uint32_t i = threadIdx.x, lane = i & 63, w = i >> 6;
+    uint32_t nin = 1, total = 1, lv = 0;
+    CItem* qin = qa;
+    CItem* qout = qb;
+    while (nin > 0 && nin <= (uint32_t)kTopItems && lv < (uint32_t)kTopMaxLevels) {
+        CItem it{0u, 0u};
+        uint32_t ch[4];
+        int nc = 0;
+        uint32_t n = 0;
+        if (i < nin) {
+            it = qin[i];
+            nc = collapse_children(nodes, it.bnode, ch);
+            for (int k = 0; k < nc; ++k) n += nodes[ch[k]].count == 0 ? 1u : 0u;
+        }
+        // exclusive scan of n over the workgroup: within the wave, then over the waves' sums
+        uint32_t x = n;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off, 64);
+            if (lane >= (uint32_t)off) x += y;
+        }
+        if (lane == 63) s_wsum[w] = x;
+        block_sync();
+        uint32_t before = 0, nnext = 0;
+        for (uint32_t q = 0; q < kTopItems / 64; ++q) {
+            const uint32_t v = s_wsum[q];
+            before += q < w ? v : 0u;
+            nnext += v;
+        }
+        if (total + nnext > cap) {  // the same for every thread; reported as total = cap + 1
+            total = cap + 1;
+            break;
+        }
+        if (i < nin) collapse4_write(nodes, it, ch, nc, before + x - n, total, out, qout, err);
+        total += nnext;
+        if (i == 0) info[4 + lv] = total;
+        lv++;
+        nin = nnext;
+        CItem* t = qin;
+        qin = qout;
+        qout = t;
+        block_sync_all();  // the next level's queue is complete (global memory) and s_wsum is read
+    }
+    if (i == 0) {
+        info[0] = nin;
+        info[1] = total;
+        info[2] = lv;
+        info[3] = qin == qa ? 0u : 1u;
+    }
 }
 
 // Refit (the tree kept, boxes recomputed from the current vertices): the
@@ -1979,7 +2048,7 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     Span pbuf, prims, prims2, nodes, keys, slot_bin;  // prims2: the second id buffer of the large levels
     Span d_task[2], d_params, d_nchunk, d_chunk0, d_chunks, d_bins, d_counts, d_bases, d_left, d_lscan, d_sweep,
         d_nlarge, d_nsmall, d_loff, d_soff, d_small, d_cnt, d_ssum;
-    Span snodes, d_used, d_first, d_counter, d_rec;
+    Span snodes, d_used, d_first, d_counter, d_rec, d_cinfo;
     {
         ScratchPlan plan;
         plan.add(pbuf, N * sizeof(PrimBox));
@@ -2011,6 +2080,7 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
         plan.add(d_first, N * 4);
         plan.add(d_counter, 12);
         plan.add(d_rec, (2 * kMaxLevels + 2) * 4);  // per level: first small subtree, count; last pair: the root's
+        plan.add(d_cinfo, (4 + kTopMaxLevels) * 4);  // the collapse's first levels (k_collapse4_top)
         plan.place(c.bvh_arena);
     }
     k_prim_boxes<<<blocks_for(N, 256), 256, 0, c.stream>>>(N, m.idx.as<uint32_t>(), m.verts.as<GVertex>(),
@@ -2048,7 +2118,7 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     // The small subtrees a level emits are built on the second stream while
     // the next levels run (their triangles and nodes are disjoint from those
     // of every later level); a level of n large nodes emits at most 2 n.
-    c.bvh_counts.alloc((kMaxLevels + LC_WORDS + 12) * 4);
+    c.bvh_counts.alloc((kMaxLevels + LC_WORDS + 12 + 4 + kTopMaxLevels) * 4);
     uint32_t* pin = c.bvh_counts.as<uint32_t>();
     if (!c.aux_stream) VG_HIP(hipStreamCreateWithFlags(&c.aux_stream, hipStreamNonBlocking));
     struct Events {  // destroyed on every path out of the build
@@ -2149,11 +2219,20 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
         if (nodes_l == 0) break;  // level L (queued) has no nodes
     }
     // the main stream waits for the small subtrees, then the loop's results are read:
-    // node and subtree counts, depth, error flags, root bounds
+    // node and subtree counts, depth, error flags, root bounds.  The triangles are
+    // written in leaf order on the second stream, beside the Node4 collapse.
+    hipEvent_t tris_done = evs.make();
     {
-        hipEvent_t done = evs.make();
+        hipEvent_t done = evs.make(), ready = evs.make();
         VG_HIP(hipEventRecord(done, c.aux_stream));
         VG_HIP(hipStreamWaitEvent(c.stream, done, 0));
+        VG_HIP(hipEventRecord(ready, c.stream));
+        VG_HIP(hipStreamWaitEvent(c.aux_stream, ready, 0));
+        B.tris.alloc(N * 3 * sizeof(float4));
+        k_emit_tris<<<blocks_for(N, 256), 256, 0, c.aux_stream>>>(N, prims.as<uint32_t>(), m.idx.as<uint32_t>(),
+                                                                  m.verts.as<GVertex>(), B.tris.as<float4>());
+        VG_HIP(hipGetLastError());
+        VG_HIP(hipEventRecord(tris_done, c.aux_stream));
     }
     uint32_t* hcnt = pin + kMaxLevels;
     int* hkeys = reinterpret_cast<int*>(pin + kMaxLevels + LC_WORDS);
@@ -2225,10 +2304,20 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
             uint32_t* qoff = qcnt + cap;
             CItem root{0u, 0u};
             VG_HIP(hipMemcpyAsync(qa, &root, sizeof(root), hipMemcpyHostToDevice, c.stream));
-            uint32_t total = 1, nin = 1;
-            B.level_start = {0u, 1u};  // level 0 = the root; each pass below appends the end of the next level
-            CItem* qin = qa;
-            CItem* qout = qb;
+            // the first levels in one workgroup, then one count / scan / emit round per level
+            k_collapse4_top<<<1, kTopItems, 0, c.stream>>>(nodes.as<BNodeG>(), qa, qb, (uint32_t)cap,
+                                                           B.nodes.as<Node4>(), d_cinfo.as<uint32_t>(), d_err);
+            VG_HIP(hipGetLastError());
+            uint32_t* hinfo = pin + kMaxLevels + LC_WORDS + 12;
+            VG_HIP(hipMemcpyAsync(hinfo, d_cinfo.p, (4 + kTopMaxLevels) * 4, hipMemcpyDeviceToHost, c.stream));
+            VG_HIP(hipStreamSynchronize(c.stream));
+            VG_CHECK(hinfo[1] <= cap && hinfo[2] <= (uint32_t)kTopMaxLevels, VIRA_GPU_ERR_HIP, "BVH4 collapse overflow");
+            B.level_start = {0u, 1u};  // level 0 = the root; each level appends its end
+            for (uint32_t l = 0; l < hinfo[2]; ++l) B.level_start.push_back(hinfo[4 + l]);
+            wide_depth += hinfo[2];
+            uint32_t total = hinfo[1], nin = hinfo[0];
+            CItem* qin = hinfo[3] ? qb : qa;
+            CItem* qout = hinfo[3] ? qa : qb;
             while (nin > 0) {
                 k_collapse4_count<<<blocks_for(nin, 256), 256, 0, c.stream>>>(nodes.as<BNodeG>(), qin, nin,
                                                                               qcnt);
@@ -2248,10 +2337,7 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
             B.num_nodes = total;
         }
     }
-    B.tris.alloc(N * 3 * sizeof(float4));
-    k_emit_tris<<<blocks_for(N, 256), 256, 0, c.stream>>>(N, prims.as<uint32_t>(), m.idx.as<uint32_t>(),
-                                                           m.verts.as<GVertex>(), B.tris.as<float4>());
-    VG_HIP(hipGetLastError());
+    VG_HIP(hipStreamWaitEvent(c.stream, tris_done, 0));
     VG_HIP(hipEventRecord(e1, c.stream));
     // statistics (outside the timed region)
     const uint32_t sblocks = std::min<uint32_t>(kStatsBlocks, blocks_for(nn, 256));

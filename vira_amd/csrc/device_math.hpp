@@ -25,6 +25,13 @@ __device__ __forceinline__ void block_sync() {
     __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0); vmcnt and expcnt left alone
     __syncthreads();
 }
+// The same for global memory written by one wave and read by the others after
+// the barrier: every memory operation of the wave completes first (the
+// compiler's workgroup barrier does not wait for global stores either).
+__device__ __forceinline__ void block_sync_all() {
+    __builtin_amdgcn_s_waitcnt(0);  // vmcnt(0) expcnt(0) lgkmcnt(0)
+    __syncthreads();
+}
 
 struct f3 {
     float x, y, z;
