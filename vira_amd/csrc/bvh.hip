@@ -249,7 +249,9 @@ __global__ void k_init_bins(int* bins, const uint32_t* d_nt, int nb) {
 // kStageItems consecutive slots, whose triangles are mostly neighbours in the
 // same bin, accumulates each run of equal bins in registers and adds a run to
 // the LDS bins with one set of atomics, so neither lanes of a wave nor slots
-// of a thread contend for one bin address slot by slot.
+// of a thread contend for one bin address slot by slot.  The stages are
+// pipelined: the next stage's boxes and the one after's ids are loaded while
+// a stage is accumulated.
 constexpr int kStageItems = 4;                           // consecutive slots per lane and stage
 constexpr int kStageSlots = kStageItems * 64;            // slots per wave and stage
 constexpr int kStageWords = kStageSlots + kStageSlots / 32;
@@ -328,19 +330,29 @@ __global__ void __launch_bounds__(kChunkThreads) k_bin_large(const Chunk* chunks
         int acc[kBinVals];
 #pragma unroll
         for (int k = 0; k < kBinVals; ++k) acc[k] = 0;
-        for (int st = 0; st < kChunkItems / kStageItems; ++st) {
-            const uint32_t wbase = ch.begin + (uint32_t)w * (kChunkItems * 64) + (uint32_t)st * kStageSlots;
-            if (wbase >= ch.end) break;  // the same for the whole wave
-            uint32_t id[kStageItems];
+        const uint32_t wfirst = ch.begin + (uint32_t)w * (kChunkItems * 64);
+        constexpr int kStages = kChunkItems / kStageItems;
+        uint32_t idn[kStageItems];
+        PrimBox pn[kStageItems];
 #pragma unroll
-            for (int k = 0; k < kStageItems; ++k) {
-                const uint32_t slot = wbase + k * 64 + lane;
-                id[k] = slot < ch.end ? prims[slot] : 0u;
-            }
+        for (int k = 0; k < kStageItems; ++k) {
+            const uint32_t slot = wfirst + k * 64 + lane;
+            idn[k] = slot < ch.end ? prims[slot] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < kStageItems; ++k)
+            if (wfirst + k * 64 + lane < ch.end) pn[k] = pb[idn[k]];
+#pragma unroll
+        for (int k = 0; k < kStageItems; ++k) {
+            const uint32_t slot = wfirst + kStageSlots + k * 64 + lane;
+            idn[k] = slot < ch.end ? prims[slot] : 0u;
+        }
+        for (int st = 0; st < kStages; ++st) {
+            const uint32_t wbase = wfirst + (uint32_t)st * kStageSlots;
+            if (wbase >= ch.end) break;  // the same for the whole wave
             PrimBox p[kStageItems];
 #pragma unroll
-            for (int k = 0; k < kStageItems; ++k)
-                if (wbase + k * 64 + lane < ch.end) p[k] = pb[id[k]];
+            for (int k = 0; k < kStageItems; ++k) p[k] = pn[k];
 #pragma unroll
             for (int k = 0; k < kStageItems; ++k) {
                 const uint32_t slot = wbase + k * 64 + lane;
@@ -355,6 +367,17 @@ __global__ void __launch_bounds__(kChunkThreads) k_bin_large(const Chunk* chunks
                         bx[3 + a][q] = p[k].hi[a];
                     }
                     bn[k * 64 + lane] = (uint8_t)b;
+                }
+            }
+            // the next stage's boxes and the one after's ids are in flight while this one is accumulated
+            if (st + 1 < kStages) {
+#pragma unroll
+                for (int k = 0; k < kStageItems; ++k)
+                    if (wbase + kStageSlots + k * 64 + lane < ch.end) pn[k] = pb[idn[k]];
+#pragma unroll
+                for (int k = 0; k < kStageItems; ++k) {
+                    const uint32_t slot = wbase + 2 * kStageSlots + k * 64 + lane;
+                    idn[k] = (st + 2 < kStages && slot < ch.end) ? prims[slot] : 0u;
                 }
             }
             wave_sync();
