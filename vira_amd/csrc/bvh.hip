@@ -2242,13 +2242,17 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
         if (nodes_l == 0) break;  // level L (queued) has no nodes
     }
     // the main stream waits for the small subtrees, then the loop's results are read:
-    // node and subtree counts, depth, error flags, root bounds.  The triangles are
-    // written in leaf order on the second stream, beside the Node4 collapse.
-    hipEvent_t tris_done = evs.make();
+    // node and subtree counts, depth, error flags, root bounds
     {
-        hipEvent_t done = evs.make(), ready = evs.make();
+        hipEvent_t done = evs.make();
         VG_HIP(hipEventRecord(done, c.aux_stream));
         VG_HIP(hipStreamWaitEvent(c.stream, done, 0));
+    }
+    // the triangles in leaf order, on the second stream beside the rest of the layout
+    // (queued once the main stream has reached the point where this is called)
+    hipEvent_t tris_done = evs.make();
+    auto emit_tris_aside = [&]() {
+        hipEvent_t ready = evs.make();
         VG_HIP(hipEventRecord(ready, c.stream));
         VG_HIP(hipStreamWaitEvent(c.aux_stream, ready, 0));
         B.tris.alloc(N * 3 * sizeof(float4));
@@ -2256,7 +2260,7 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
                                                                   m.verts.as<GVertex>(), B.tris.as<float4>());
         VG_HIP(hipGetLastError());
         VG_HIP(hipEventRecord(tris_done, c.aux_stream));
-    }
+    };
     uint32_t* hcnt = pin + kMaxLevels;
     int* hkeys = reinterpret_cast<int*>(pin + kMaxLevels + LC_WORDS);
     uint32_t* hsmall = pin + kMaxLevels + LC_WORDS + 6;  // counter words 1 and 2 of the small builder
@@ -2291,6 +2295,7 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     B.width = opt.width == 2 ? 2 : 4;
     uint32_t wide_depth = 0;
     if (B.width == 2) {
+        emit_tris_aside();
         // flags and internal-node indices in the boxes' space (2 nn words, nn < 2 N)
         VG_CHECK((size_t)nn * 8 <= pbuf.bytes, VIRA_GPU_ERR_HIP, "BVH2 emission scratch");
         uint32_t* flag = pbuf.as<uint32_t>();
@@ -2313,6 +2318,7 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
         const size_t cap = std::max<size_t>(1, (nn - 1) / 2);
         B.nodes.alloc(cap * sizeof(Node4));
         if (nn == 1) {
+            emit_tris_aside();
             k_emit_leaf_root4<<<1, 1, 0, c.stream>>>(nodes.as<BNodeG>(), B.nodes.as<Node4>());
             VG_HIP(hipGetLastError());
             B.num_nodes = 1;
@@ -2331,6 +2337,7 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
             k_collapse4_top<<<1, kTopItems, 0, c.stream>>>(nodes.as<BNodeG>(), qa, qb, (uint32_t)cap,
                                                            B.nodes.as<Node4>(), d_cinfo.as<uint32_t>(), d_err);
             VG_HIP(hipGetLastError());
+            emit_tris_aside();  // after the latency-bound first levels, not beside them
             uint32_t* hinfo = pin + kMaxLevels + LC_WORDS + 12;
             VG_HIP(hipMemcpyAsync(hinfo, d_cinfo.p, (4 + kTopMaxLevels) * 4, hipMemcpyDeviceToHost, c.stream));
             VG_HIP(hipStreamSynchronize(c.stream));
