@@ -711,6 +711,8 @@ __global__ void k_lv_advance(uint32_t* cnt, int parity, uint32_t* rec) {
 constexpr uint32_t kTaskRoot = 0xffffffffu;
 constexpr uint32_t kCoop = 32;              // nodes above this are split by the whole wave
 constexpr uint32_t kQueueCap = kSmall / 2;  // queued nodes hold >= 2 triangles each, disjoint
+constexpr uint32_t kGroupMin = 9;           // queued nodes of kGroupMin..kCoop triangles: 8 lanes each
+constexpr uint32_t kBigCap = kSmall / kGroupMin + 1;  // such nodes are disjoint
 constexpr int kCoopStack = 8;               // > log2(kSmall / kCoop) + 1 (smaller child first)
 static_assert(kSmall <= 256, "item numbers of a subtree are bytes");
 static_assert(kCoop > 15, "wave-split nodes are never leaves (max_leaf <= 15)");
@@ -928,13 +930,170 @@ __device__ __forceinline__ bool lane_split(const QEnt& q, uint8_t* s_perm, const
     return true;
 }
 
+// Eight lanes of a group (r = lane & 7 holds bin r) decide and split queued
+// node q together: the same decision, children and in-place partition as
+// lane_split, with the node's bins spread over the group instead of held by
+// one lane, so every lane reads each item once instead of updating eight bins
+// per item.  act: the group has a node.  Every lane of a group returns the
+// same values; the shuffles run on all 64 lanes.
+__device__ __forceinline__ bool group_split(const QEnt& q, bool act, uint32_t r, uint8_t* s_perm,
+                                            const float2* s_box, uint8_t* s_binof, int max_leaf, float trav_cost,
+                                            uint32_t& nl, float Llo[3], float Lhi[3], float Rlo[3], float Rhi[3]) {
+    const uint32_t n = act ? q.n : 0u;
+    // centroid bounds: lane r takes items r, r + 8, ...
+    float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = r; i < n; i += 8) {
+        const Box6 x = box_at(s_box, s_perm[q.b + i]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float c = 0.5f * (x.lo[k] + x.hi[k]);
+            clo[k] = fminf(clo[k], c);
+            chi[k] = fmaxf(chi[k], c);
+        }
+    }
+#pragma unroll
+    for (int off = 1; off < 8; off <<= 1)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            clo[k] = fminf(clo[k], __shfl_xor(clo[k], off, 64));
+            chi[k] = fmaxf(chi[k], __shfl_xor(chi[k], off, 64));
+        }
+    const float ext[3] = {chi[0] - clo[0], chi[1] - clo[1], chi[2] - clo[2]};
+    int axis = (ext[1] > ext[0]) ? 1 : 0;
+    if (ext[2] > pick3(ext, axis)) axis = 2;
+    const float cmin = pick3(clo, axis);
+    const float eaxis = pick3(ext, axis);
+    const float scale = (float)kSmallBins / eaxis;
+    const bool binned = act && eaxis > 0.f && isfinite(scale) && q.depth < kMedianDepth;
+    // bin r: every lane of the group reads every item; item i's bin is stored by lane i & 7
+    uint32_t cnt = 0;
+    float blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    if (binned) {
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t j = s_perm[q.b + i];
+            const Box6 x = box_at(s_box, j);
+            const float c = 0.5f * (pick3(x.lo, axis) + pick3(x.hi, axis));
+            const int bi = bin_of(c, cmin, scale, kSmallBins);
+            if ((i & 7u) == r) s_binof[j] = (uint8_t)bi;
+            if (bi == (int)r) {
+                cnt++;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    blo[k] = fminf(blo[k], x.lo[k]);
+                    bhi[k] = fmaxf(bhi[k], x.hi[k]);
+                }
+            }
+        }
+    }
+    wave_sync();  // the group's s_binof stores land before lane 0 of the group partitions
+    // left of split r: bins 0..r (inclusive prefix over the group); right: bins r+1..7
+    uint32_t lc = cnt, sc = cnt;
+    float ll[3], lh[3], sl[3], sh[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { ll[k] = sl[k] = blo[k]; lh[k] = sh[k] = bhi[k]; }
+#pragma unroll
+    for (int off = 1; off < 8; off <<= 1) {
+        const uint32_t oc = __shfl_up(lc, off, 8), os = __shfl_down(sc, off, 8);
+        const bool up = r >= (uint32_t)off, down = r + off < 8u;
+        lc += up ? oc : 0u;
+        sc += down ? os : 0u;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float a = __shfl_up(ll[k], off, 8), b = __shfl_up(lh[k], off, 8);
+            const float c = __shfl_down(sl[k], off, 8), d = __shfl_down(sh[k], off, 8);
+            ll[k] = up ? fminf(ll[k], a) : ll[k];
+            lh[k] = up ? fmaxf(lh[k], b) : lh[k];
+            sl[k] = down ? fminf(sl[k], c) : sl[k];
+            sh[k] = down ? fmaxf(sh[k], d) : sh[k];
+        }
+    }
+    uint32_t rc = __shfl_down(sc, 1, 8);
+    float rl[3], rh[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        rl[k] = __shfl_down(sl[k], 1, 8);
+        rh[k] = __shfl_down(sh[k], 1, 8);
+    }
+    if (r == 7u) rc = 0u;  // no split after the last bin
+    // the cheapest split: lowest cost, then lowest bin (the sweep's first strict minimum)
+    float cost = INFINITY;
+    if (lc != 0u && rc != 0u) cost = (float)lc * half_area(ll, lh) + (float)rc * half_area(rl, rh);
+    const bool cand = cost < INFINITY;
+    float bc = cand ? cost : INFINITY;
+    int bs = cand ? (int)r : kSmallBins;
+#pragma unroll
+    for (int off = 1; off < 8; off <<= 1) {
+        const float oc = __shfl_xor(bc, off, 64);
+        const int os = __shfl_xor(bs, off, 64);
+        if (oc < bc || (oc == bc && os < bs)) {
+            bc = oc;
+            bs = os;
+        }
+    }
+    const int split = bs < kSmallBins ? bs : -1;
+    const float best = bc;
+    // the children of split `split` as lane `split` of the group holds them
+    const uint32_t src = (threadIdx.x & ~7u) + (uint32_t)(split < 0 ? 0 : split);
+    const uint32_t snl = __shfl(lc, (int)src, 64);
+    float sLlo[3], sLhi[3], sRlo[3], sRhi[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        sLlo[k] = __shfl(ll[k], (int)src, 64);
+        sLhi[k] = __shfl(lh[k], (int)src, 64);
+        sRlo[k] = __shfl(rl[k], (int)src, 64);
+        sRhi[k] = __shfl(rh[k], (int)src, 64);
+    }
+    if (!act) return false;
+    if (!binned) {
+        if (n <= (uint32_t)max_leaf) return false;
+    } else {
+        const float A = half_area(q.lo, q.hi);
+        const float leaf_cost = (float)n * A;
+        if (split < 0) {
+            if (n <= (uint32_t)max_leaf) return false;
+        } else if (n <= (uint32_t)max_leaf && leaf_cost <= trav_cost * A + best) {
+            return false;
+        }
+        if (split >= 0) {
+            if (r == 0u) {  // the in-place partition of lane_split, by one lane
+                uint32_t i = q.b, j = q.b + n - 1;
+                while (i <= j) {
+                    const uint8_t pi = s_perm[i];
+                    if ((int)s_binof[pi] <= split) {
+                        i++;
+                    } else {
+                        s_perm[i] = s_perm[j];
+                        s_perm[j] = pi;
+                        if (j == q.b) break;
+                        j--;
+                    }
+                }
+            }
+            nl = snl;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                Llo[k] = sLlo[k];
+                Lhi[k] = sLhi[k];
+                Rlo[k] = sRlo[k];
+                Rhi[k] = sRhi[k];
+            }
+            return true;
+        }
+    }
+    // object median; the children keep the parent box (conservative)
+    nl = n / 2;
+    for (int k = 0; k < 3; ++k) { Llo[k] = Rlo[k] = q.lo[k]; Lhi[k] = Rhi[k] = q.hi[k]; }
+    return true;
+}
+
 __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, const uint32_t* range,
                                                     uint32_t* prims, const uint32_t* alt, const PrimBox* pb, BNodeG* nodes, BNodeG* snodes,
                                                     uint32_t* used, uint32_t* max_depth, uint32_t* err, int max_leaf,
                                                     float trav_cost) {
     __shared__ float2 s_box[3 * kSmall];
     __shared__ uint8_t s_perm[kSmall], s_tmp[kSmall];  // the items in their current order
-    __shared__ QEntL s_q[kQueueCap];
+    __shared__ QEntL s_q[kQueueCap];     // queued nodes of 2..kGroupMin-1 triangles
+    __shared__ QEntL s_bq[kBigCap];      // queued nodes of kGroupMin..kCoop triangles
     __shared__ CEnt s_stack[kCoopStack];
     __shared__ int s_bk[kSmallBins * kBinVals];         // wave-split bins (keys)
     __shared__ uint8_t s_binof[kSmall];                 // each item's bin in its current node
@@ -973,7 +1132,7 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
     const BNodeG root = nodes[task.node];
     wave_sync();
     uint32_t next = 0;  // scratch nodes used
-    uint32_t qtail = 0, md = task.depth;
+    uint32_t qtail = 0, btail = 0, md = task.depth;
     bool fail = false;
     auto slot_ptr = [&](uint32_t s) -> BNodeG* { return s == kTaskRoot ? &nodes[task.node] : &local[s]; };
     // ------------------------------------------------ nodes of more than kCoop triangles: the whole wave
@@ -989,10 +1148,15 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
         wave_centroid_bounds(s_perm, s_box, 0, n0, lane, e.clo, e.chi);
         have = true;
     } else if (n0 >= 2) {
-        if (lane == 0)
-            s_q[0] = q_pack(QEnt{kTaskRoot, 0u, n0, task.depth, {root.lo[0], root.lo[1], root.lo[2]},
-                                 {root.hi[0], root.hi[1], root.hi[2]}});
-        qtail = 1;
+        const QEntL r0 = q_pack(QEnt{kTaskRoot, 0u, n0, task.depth, {root.lo[0], root.lo[1], root.lo[2]},
+                                     {root.hi[0], root.hi[1], root.hi[2]}});
+        if (n0 >= kGroupMin) {
+            if (lane == 0) s_bq[0] = r0;
+            btail = 1;
+        } else {
+            if (lane == 0) s_q[0] = r0;
+            qtail = 1;
+        }
     }
     while (have) {
         md = max(md, e.depth);
@@ -1119,14 +1283,20 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
             local[c + 1] = BNodeG{{R.lo[0], R.lo[1], R.lo[2]}, {R.hi[0], R.hi[1], R.hi[2]}, task.begin + R.b, R.n};
             *slot_ptr(e.slot) = BNodeG{{e.lo[0], e.lo[1], e.lo[2]}, {e.hi[0], e.hi[1], e.hi[2]}, c, 0u};
         }
-        // children of up to kCoop triangles are queued for the lanes (single triangles are leaves as written)
+        // children of up to kCoop triangles are queued for the groups or the lanes (single triangles
+        // are leaves as written)
         auto route = [&](const CEnt& k) {
             if (k.n > kCoop) return;
             if (k.n >= 2) {
-                if (lane == 0)
-                    s_q[qtail % kQueueCap] = q_pack(QEnt{k.slot, k.b, k.n, k.depth, {k.lo[0], k.lo[1], k.lo[2]},
-                                                         {k.hi[0], k.hi[1], k.hi[2]}});
-                qtail++;
+                const QEntL e2 = q_pack(QEnt{k.slot, k.b, k.n, k.depth, {k.lo[0], k.lo[1], k.lo[2]},
+                                             {k.hi[0], k.hi[1], k.hi[2]}});
+                if (k.n >= kGroupMin) {
+                    if (lane == 0) s_bq[btail % kBigCap] = e2;
+                    btail++;
+                } else {
+                    if (lane == 0) s_q[qtail % kQueueCap] = e2;
+                    qtail++;
+                }
             } else {
                 md = max(md, k.depth);
             }
@@ -1153,8 +1323,54 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
         }
         wave_sync();
     }
-    if (qtail > kQueueCap) fail = true;
-    // ------------------------------------------------ nodes of up to kCoop triangles: one lane each
+    if (qtail > kQueueCap || btail > kBigCap) fail = true;
+    // ------------------------------------------------ nodes of kGroupMin..kCoop triangles: 8 lanes each
+    {
+        const uint32_t g = lane >> 3, r = lane & 7u;
+        uint32_t bhead = 0;
+        while (bhead < btail && !fail) {
+            const uint32_t k = min(8u, btail - bhead);
+            const bool act = g < k;
+            QEnt q{};
+            if (act) q = q_unpack(s_bq[(bhead + g) % kBigCap]);
+            bhead += k;
+            wave_sync();
+            uint32_t nl = 0;
+            float Llo[3], Lhi[3], Rlo[3], Rhi[3];
+            if (act) md = max(md, q.depth);
+            const bool split = group_split(q, act, r, s_perm, s_box, s_binof, max_leaf, trav_cost, nl, Llo, Lhi,
+                                           Rlo, Rhi);
+            const bool lead = act && r == 0u;
+            const uint64_t sm = __ballot(lead && split);
+            const uint32_t c = next + 2 * rank_below(sm);
+            next += 2 * (uint32_t)__popcll(sm);
+            const uint32_t nr = q.n - nl;
+            if (lead && split) {
+                local[c] = BNodeG{{Llo[0], Llo[1], Llo[2]}, {Lhi[0], Lhi[1], Lhi[2]}, task.begin + q.b, nl};
+                local[c + 1] = BNodeG{{Rlo[0], Rlo[1], Rlo[2]}, {Rhi[0], Rhi[1], Rhi[2]}, task.begin + q.b + nl, nr};
+                *slot_ptr(q.slot) = BNodeG{{q.lo[0], q.lo[1], q.lo[2]}, {q.hi[0], q.hi[1], q.hi[2]}, c, 0u};
+                if (nl == 1 || nr == 1) md = max(md, q.depth + 1);
+            }
+            // children: kGroupMin or more triangles to this queue, 2.. to the lanes' queue
+            const bool lb = lead && split && nl >= kGroupMin, rbg = lead && split && nr >= kGroupMin;
+            const bool ls = lead && split && nl >= 2 && nl < kGroupMin, rs = lead && split && nr >= 2 && nr < kGroupMin;
+            const uint64_t mlb = __ballot(lb), mrb = __ballot(rbg), mls = __ballot(ls), mrs = __ballot(rs);
+            const uint32_t pb2 = btail + rank_below(mlb) + rank_below(mrb);
+            const uint32_t ps = qtail + rank_below(mls) + rank_below(mrs);
+            const QEntL eL = q_pack(QEnt{c, q.b, nl, q.depth + 1, {Llo[0], Llo[1], Llo[2]}, {Lhi[0], Lhi[1], Lhi[2]}});
+            const QEntL eR =
+                q_pack(QEnt{c + 1, q.b + nl, nr, q.depth + 1, {Rlo[0], Rlo[1], Rlo[2]}, {Rhi[0], Rhi[1], Rhi[2]}});
+            if (lb) s_bq[pb2 % kBigCap] = eL;
+            if (rbg) s_bq[(pb2 + (lb ? 1u : 0u)) % kBigCap] = eR;
+            if (ls) s_q[ps % kQueueCap] = eL;
+            if (rs) s_q[(ps + (ls ? 1u : 0u)) % kQueueCap] = eR;
+            btail += (uint32_t)(__popcll(mlb) + __popcll(mrb));
+            qtail += (uint32_t)(__popcll(mls) + __popcll(mrs));
+            if (btail - bhead > kBigCap || qtail > kQueueCap) fail = true;
+            wave_sync();
+        }
+    }
+    // ------------------------------------------------ nodes of 2..kGroupMin-1 triangles: one lane each
     uint32_t qhead = 0;
     while (qhead < qtail && !fail) {
         const uint32_t k = min(64u, qtail - qhead);
