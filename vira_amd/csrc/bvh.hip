@@ -693,27 +693,33 @@ __global__ void k_lv_advance(uint32_t* cnt, int parity, uint32_t* rec) {
 // Small subtrees (nodes of at most kSmall triangles; the reference's
 // recursive "subdivide", vira_blas.ipp:132-182), one wave per subtree.  The
 // wave stages its triangles' boxes in LDS once and works from LDS only:
-//   * a node of more than kCoop triangles is split by the whole wave: lane
-//     g * 8 + k sums the triangles of group g (items g, g + 8, ...) that fall
-//     into bin k, the 8 groups are merged across lanes, every lane sweeps the
-//     8 bins with the same arithmetic as a serial build, and a stable ballot
-//     partition reorders the node's items; the children's centroid bounds
-//     come from the bins;
-//   * smaller nodes are queued in LDS, and in each round every lane takes one
-//     queued node and splits it alone (bins in registers, the reference's
-//     in-place partition).
-// Child pairs are numbered by a prefix sum over the splitting lanes, so the
-// nodes of a task fill [0, used) of its scratch region without gaps, and
+//   * nodes of more than kCoop triangles are split by the whole wave, up to
+//     kFront of them per round (breadth first): their items are binned into
+//     LDS with atomics (8 bins of order-preserving integer keys per node),
+//     lane f sweeps node f's bins with the same arithmetic as a serial build,
+//     a stable ballot partition reorders each node's items, and the
+//     children's centroid bounds come from the bins;
+//   * queued nodes of kGroupMin..kCoop triangles are split by groups of 8
+//     lanes, 8 nodes per round: lane r of a group holds bin r, every lane
+//     reads each item once, the sweep runs as prefix unions across the
+//     group, and one lane does the reference's in-place partition;
+//   * smaller queued nodes are split one per lane (bins in registers, the
+//     same in-place partition).
+// Child pairs are numbered by prefix sums over a round's splitting nodes, so
+// the nodes of a task fill [0, used) of its scratch region without gaps, and
 // k_compact_small moves them behind the large-level nodes.  Every decision is
 // an exact function of a node's triangle set (integer counts, min/max bounds,
 // one fixed sweep), so the tree is the same as a serial build's whatever the
-// order of the items inside a node.
+// order in which nodes are split; the item order inside a node, which the
+// object-median fallback and the leaves' triangle order follow, depends only
+// on which of the two partitions a node of its size takes.
 constexpr uint32_t kTaskRoot = 0xffffffffu;
 constexpr uint32_t kCoop = 32;              // nodes above this are split by the whole wave
 constexpr uint32_t kQueueCap = kSmall / 2;  // queued nodes hold >= 2 triangles each, disjoint
 constexpr uint32_t kGroupMin = 9;           // queued nodes of kGroupMin..kCoop triangles: 8 lanes each
 constexpr uint32_t kBigCap = kSmall / kGroupMin + 1;  // such nodes are disjoint
-constexpr int kCoopStack = 8;               // > log2(kSmall / kCoop) + 1 (smaller child first)
+constexpr int kFront = 4;                   // whole-wave nodes split per round
+constexpr uint32_t kFrontCap = kSmall / (kCoop + 1) + 1;  // whole-wave nodes are disjoint
 static_assert(kSmall <= 256, "item numbers of a subtree are bytes");
 static_assert(kCoop > 15, "wave-split nodes are never leaves (max_leaf <= 15)");
 static_assert(kSmallBins == 8, "the wave-split binning puts one bin on each lane of a group of 8");
@@ -749,24 +755,24 @@ __device__ __forceinline__ Box6 box_at(const float2* s_box, uint32_t j) {
     return Box6{{a.x, a.y, b.x}, {b.y, c.x, c.y}};
 }
 
-// c ? a : b, field by field (a select of whole structs would go through scratch memory)
-__device__ __forceinline__ CEnt pick_ent(bool c, const CEnt& a, const CEnt& b) {
-    CEnt r;
-    r.slot = c ? a.slot : b.slot;
-    r.b = c ? a.b : b.b;
-    r.n = c ? a.n : b.n;
-    r.depth = c ? a.depth : b.depth;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        r.lo[k] = c ? a.lo[k] : b.lo[k];
-        r.hi[k] = c ? a.hi[k] : b.hi[k];
-        r.clo[k] = c ? a.clo[k] : b.clo[k];
-        r.chi[k] = c ? a.chi[k] : b.chi[k];
-    }
-    return r;
+// Centroid bounds of items [b, b + n) of the wave's list (every lane gets them).
+// A whole-wave node's bin mapping from its centroid bounds: the axis of
+// largest extent, its low bound and bins per unit; not binned (object
+// median) when the centroids coincide on it or the node is too deep.
+struct WaveBinning {
+    bool binned;
+    int axis;
+    float cmin, scale;
+};
+__device__ __forceinline__ WaveBinning wave_binning(const float clo[3], const float chi[3], uint32_t depth) {
+    const float ext[3] = {chi[0] - clo[0], chi[1] - clo[1], chi[2] - clo[2]};
+    int axis = (ext[1] > ext[0]) ? 1 : 0;
+    if (ext[2] > pick3(ext, axis)) axis = 2;
+    const float eaxis = pick3(ext, axis);
+    const float scale = (float)kSmallBins / eaxis;
+    return WaveBinning{eaxis > 0.f && isfinite(scale) && depth < kMedianDepth, axis, pick3(clo, axis), scale};
 }
 
-// Centroid bounds of items [b, b + n) of the wave's list (every lane gets them).
 __device__ __forceinline__ void wave_centroid_bounds(const uint8_t* s_perm, const float2* s_box, uint32_t b,
                                                      uint32_t n, uint32_t lane,
                                                      float clo[3], float chi[3]) {
@@ -1094,8 +1100,9 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
     __shared__ uint8_t s_perm[kSmall], s_tmp[kSmall];  // the items in their current order
     __shared__ QEntL s_q[kQueueCap];     // queued nodes of 2..kGroupMin-1 triangles
     __shared__ QEntL s_bq[kBigCap];      // queued nodes of kGroupMin..kCoop triangles
-    __shared__ CEnt s_stack[kCoopStack];
-    __shared__ int s_bk[kSmallBins * kBinVals];         // wave-split bins (keys)
+    __shared__ CEnt s_fq[kFrontCap];     // nodes of more than kCoop triangles, breadth first
+    __shared__ int s_bk[kFront * kSmallBins * kBinVals];  // wave-split bins (keys) of a round's nodes
+    __shared__ int s_split[kFront];                       // their splits (-1: object median)
     __shared__ uint8_t s_binof[kSmall];                 // each item's bin in its current node
     const uint32_t lane = threadIdx.x;
     // the tasks range[0] .. range[0] + range[1] - 1 (the grid may be larger)
@@ -1135,18 +1142,19 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
     uint32_t qtail = 0, btail = 0, md = task.depth;
     bool fail = false;
     auto slot_ptr = [&](uint32_t s) -> BNodeG* { return s == kTaskRoot ? &nodes[task.node] : &local[s]; };
-    // ------------------------------------------------ nodes of more than kCoop triangles: the whole wave
-    CEnt e;
-    bool have = false;
-    int sp = 0;
+    // ------------------------------------------------ nodes of more than kCoop triangles: the whole wave,
+    // up to kFront of them at a time, breadth first (they are disjoint, so at most kFrontCap - 1 exist)
+    uint32_t fhead = 0, ftail = 0;
     if (n0 > kCoop) {
+        CEnt e;
         e.slot = kTaskRoot;
         e.b = 0;
         e.n = n0;
         e.depth = task.depth;
         for (int k = 0; k < 3; ++k) { e.lo[k] = root.lo[k]; e.hi[k] = root.hi[k]; }
         wave_centroid_bounds(s_perm, s_box, 0, n0, lane, e.clo, e.chi);
-        have = true;
+        if (lane == 0) s_fq[0] = e;
+        ftail = 1;
     } else if (n0 >= 2) {
         const QEntL r0 = q_pack(QEnt{kTaskRoot, 0u, n0, task.depth, {root.lo[0], root.lo[1], root.lo[2]},
                                      {root.hi[0], root.hi[1], root.hi[2]}});
@@ -1158,170 +1166,192 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
             qtail = 1;
         }
     }
-    while (have) {
-        md = max(md, e.depth);
-        const float ext[3] = {e.chi[0] - e.clo[0], e.chi[1] - e.clo[1], e.chi[2] - e.clo[2]};
-        int axis = (ext[1] > ext[0]) ? 1 : 0;
-        if (ext[2] > pick3(ext, axis)) axis = 2;
-        const float cmin = pick3(e.clo, axis);
-        const float eaxis = pick3(ext, axis);
-        const float scale = (float)kSmallBins / eaxis;
-        CEnt L, R;
-        uint32_t nl = 0;
-        int split = -1;
-        if (eaxis > 0.f && isfinite(scale) && e.depth < kMedianDepth) {
-            // bins as order-preserving integer keys, filled by LDS atomics (one visit per item)
-            for (uint32_t t = lane; t < (uint32_t)(kSmallBins * kBinVals); t += 64) {
-                const int v = (int)(t % kBinVals);
-                s_bk[t] = v == 0 ? 0 : (((v - 1) % 6) < 3 ? INT_MAX : INT_MIN);
-            }
-            wave_sync();
-            for (uint32_t i = lane; i < e.n; i += 64) {
-                const uint32_t j = s_perm[e.b + i];
+    while (fhead < ftail && !fail) {
+        const uint32_t nf = min((uint32_t)kFront, ftail - fhead);
+        wave_sync();
+        // bins as order-preserving integer keys, filled by LDS atomics (one visit per item)
+        for (uint32_t t = lane; t < nf * (uint32_t)(kSmallBins * kBinVals); t += 64) {
+            const int v = (int)(t % kBinVals);
+            s_bk[t] = v == 0 ? 0 : (((v - 1) % 6) < 3 ? INT_MAX : INT_MIN);
+        }
+        wave_sync();
+        for (uint32_t f = 0; f < nf; ++f) {
+            const CEnt& ef = s_fq[(fhead + f) % kFrontCap];
+            const uint32_t eb = ef.b, en = ef.n;
+            const WaveBinning wb = wave_binning(ef.clo, ef.chi, ef.depth);
+            if (!wb.binned) continue;  // the same for the whole wave
+            int* bk = &s_bk[f * kSmallBins * kBinVals];
+            for (uint32_t i = lane; i < en; i += 64) {
+                const uint32_t j = s_perm[eb + i];
                 const Box6 x = box_at(s_box, j);
                 const float c[3] = {0.5f * (x.lo[0] + x.hi[0]), 0.5f * (x.lo[1] + x.hi[1]), 0.5f * (x.lo[2] + x.hi[2])};
-                const int bi = bin_of(pick3(c, axis), cmin, scale, kSmallBins);
+                const int bi = bin_of(pick3(c, wb.axis), wb.cmin, wb.scale, kSmallBins);
                 s_binof[j] = (uint8_t)bi;
-                const float* pl = x.lo;
-                const float* ph = x.hi;
-                int* sb = &s_bk[bi * kBinVals];
+                int* sb = &bk[bi * kBinVals];
                 atomicAdd(&sb[0], 1);
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    atomicMin(&sb[1 + k], f2key(pl[k]));
-                    atomicMax(&sb[4 + k], f2key(ph[k]));
+                    atomicMin(&sb[1 + k], f2key(x.lo[k]));
+                    atomicMax(&sb[4 + k], f2key(x.hi[k]));
                     atomicMin(&sb[7 + k], f2key(c[k]));
                     atomicMax(&sb[10 + k], f2key(c[k]));
                 }
             }
-            wave_sync();
-            uint32_t bcnt[kSmallBins];
-            float blo[kSmallBins][3], bhi[kSmallBins][3];
-#pragma unroll
-            for (int b = 0; b < kSmallBins; ++b) {
-                const int* sb = &s_bk[b * kBinVals];
-                bcnt[b] = (uint32_t)sb[0];
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    blo[b][k] = bcnt[b] ? key2f(sb[1 + k]) : INFINITY;
-                    bhi[b][k] = bcnt[b] ? key2f(sb[4 + k]) : -INFINITY;
-                }
-            }
-            float best;
-            sweep8(bcnt, blo, bhi, split, best);
-            if (split >= 0) {
-                split_boxes8(bcnt, blo, bhi, split, nl, L.lo, L.hi, R.lo, R.hi);
-                for (int k = 0; k < 3; ++k) {
-                    L.clo[k] = R.clo[k] = INFINITY;
-                    L.chi[k] = R.chi[k] = -INFINITY;
-                }
-                // the children's centroid bounds: unions of the bins' (in key order, then converted)
-                int kl[3] = {INT_MAX, INT_MAX, INT_MAX}, kh[3] = {INT_MIN, INT_MIN, INT_MIN};
-                int rl[3] = {INT_MAX, INT_MAX, INT_MAX}, rh[3] = {INT_MIN, INT_MIN, INT_MIN};
+        }
+        wave_sync();
+        // lane f < nf: node f's SAH decision and its children (vira_blas.ipp:185-254 with bins)
+        CEnt e{}, L{}, R{};
+        uint32_t nl = 0;
+        int split = -1;
+        if (lane < nf) {
+            e = s_fq[(fhead + lane) % kFrontCap];
+            md = max(md, e.depth);
+            const WaveBinning wb = wave_binning(e.clo, e.chi, e.depth);
+            if (wb.binned) {
+                const int* bk = &s_bk[lane * kSmallBins * kBinVals];
+                uint32_t bcnt[kSmallBins];
+                float blo[kSmallBins][3], bhi[kSmallBins][3];
 #pragma unroll
                 for (int b = 0; b < kSmallBins; ++b) {
-                    const bool left = b <= split;
-                    const int* sb = &s_bk[b * kBinVals];
+                    const int* sb = &bk[b * kBinVals];
+                    bcnt[b] = (uint32_t)sb[0];
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
-                        const int cl = sb[7 + k], ch = sb[10 + k];
-                        kl[k] = left ? min(kl[k], cl) : kl[k];
-                        kh[k] = left ? max(kh[k], ch) : kh[k];
-                        rl[k] = left ? rl[k] : min(rl[k], cl);
-                        rh[k] = left ? rh[k] : max(rh[k], ch);
+                        blo[b][k] = bcnt[b] ? key2f(sb[1 + k]) : INFINITY;
+                        bhi[b][k] = bcnt[b] ? key2f(sb[4 + k]) : -INFINITY;
                     }
                 }
+                float best;
+                sweep8(bcnt, blo, bhi, split, best);
+                if (split >= 0) {
+                    split_boxes8(bcnt, blo, bhi, split, nl, L.lo, L.hi, R.lo, R.hi);
+                    // the children's centroid bounds: unions of the bins' (in key order, then converted)
+                    int kl[3] = {INT_MAX, INT_MAX, INT_MAX}, kh[3] = {INT_MIN, INT_MIN, INT_MIN};
+                    int rl[3] = {INT_MAX, INT_MAX, INT_MAX}, rh[3] = {INT_MIN, INT_MIN, INT_MIN};
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    L.clo[k] = key2f(kl[k]);
-                    L.chi[k] = key2f(kh[k]);
-                    R.clo[k] = key2f(rl[k]);
-                    R.chi[k] = key2f(rh[k]);
+                    for (int b = 0; b < kSmallBins; ++b) {
+                        const bool left = b <= split;
+                        const int* sb = &bk[b * kBinVals];
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            const int cl = sb[7 + k], ch = sb[10 + k];
+                            kl[k] = left ? min(kl[k], cl) : kl[k];
+                            kh[k] = left ? max(kh[k], ch) : kh[k];
+                            rl[k] = left ? rl[k] : min(rl[k], cl);
+                            rh[k] = left ? rh[k] : max(rh[k], ch);
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        L.clo[k] = key2f(kl[k]);
+                        L.chi[k] = key2f(kh[k]);
+                        R.clo[k] = key2f(rl[k]);
+                        R.chi[k] = key2f(rh[k]);
+                    }
                 }
-                // stable partition of the node's items through s_tmp
+            }
+            if (split < 0) {
+                // object median; the children keep the parent box (conservative) and get exact
+                // centroid bounds (below)
+                nl = e.n / 2;
+                for (int k = 0; k < 3; ++k) {
+                    L.lo[k] = R.lo[k] = e.lo[k];
+                    L.hi[k] = R.hi[k] = e.hi[k];
+                }
+            }
+            s_split[lane] = split;
+        }
+        wave_sync();
+        // each node's stable partition through s_tmp, or its median halves' centroid bounds
+        for (uint32_t f = 0; f < nf; ++f) {
+            const int sf = s_split[f];
+            const uint32_t eb = s_fq[(fhead + f) % kFrontCap].b, en = s_fq[(fhead + f) % kFrontCap].n;
+            const uint32_t nlf = __shfl(nl, (int)f, 64);
+            if (sf >= 0) {
                 uint32_t lb = 0, rb = 0;
-                for (uint32_t k0 = 0; k0 < e.n; k0 += 64) {
-                    const bool valid = k0 + lane < e.n;
+                for (uint32_t k0 = 0; k0 < en; k0 += 64) {
+                    const bool valid = k0 + lane < en;
                     uint32_t j = 0;
                     bool left = false;
                     if (valid) {
-                        j = s_perm[e.b + k0 + lane];
-                        left = (int)s_binof[j] <= split;
+                        j = s_perm[eb + k0 + lane];
+                        left = (int)s_binof[j] <= sf;
                     }
                     const uint64_t lm = __ballot(valid && left), rm = __ballot(valid && !left);
-                    if (valid) s_tmp[e.b + (left ? lb + rank_below(lm) : nl + rb + rank_below(rm))] = (uint8_t)j;
+                    if (valid) s_tmp[eb + (left ? lb + rank_below(lm) : nlf + rb + rank_below(rm))] = (uint8_t)j;
                     lb += (uint32_t)__popcll(lm);
                     rb += (uint32_t)__popcll(rm);
                 }
-                if (lb != nl) fail = true;
+                if (lb != nlf) fail = true;
                 wave_sync();
-                for (uint32_t i = lane; i < e.n; i += 64) s_perm[e.b + i] = s_tmp[e.b + i];
+                for (uint32_t i = lane; i < en; i += 64) s_perm[eb + i] = s_tmp[eb + i];
                 wave_sync();
-            }
-        }
-        if (split < 0) {
-            // object median; the children keep the parent box (conservative) and get exact centroid bounds
-            nl = e.n / 2;
-            for (int k = 0; k < 3; ++k) {
-                L.lo[k] = R.lo[k] = e.lo[k];
-                L.hi[k] = R.hi[k] = e.hi[k];
-            }
-            wave_centroid_bounds(s_perm, s_box, e.b, nl, lane, L.clo, L.chi);
-            wave_centroid_bounds(s_perm, s_box, e.b + nl, e.n - nl, lane, R.clo, R.chi);
-        }
-        const uint32_t c = next;
-        next += 2;
-        L.slot = c;
-        L.b = e.b;
-        L.n = nl;
-        R.slot = c + 1;
-        R.b = e.b + nl;
-        R.n = e.n - nl;
-        L.depth = R.depth = e.depth + 1;
-        if (lane == 0) {
-            local[c] = BNodeG{{L.lo[0], L.lo[1], L.lo[2]}, {L.hi[0], L.hi[1], L.hi[2]}, task.begin + L.b, L.n};
-            local[c + 1] = BNodeG{{R.lo[0], R.lo[1], R.lo[2]}, {R.hi[0], R.hi[1], R.hi[2]}, task.begin + R.b, R.n};
-            *slot_ptr(e.slot) = BNodeG{{e.lo[0], e.lo[1], e.lo[2]}, {e.hi[0], e.hi[1], e.hi[2]}, c, 0u};
-        }
-        // children of up to kCoop triangles are queued for the groups or the lanes (single triangles
-        // are leaves as written)
-        auto route = [&](const CEnt& k) {
-            if (k.n > kCoop) return;
-            if (k.n >= 2) {
-                const QEntL e2 = q_pack(QEnt{k.slot, k.b, k.n, k.depth, {k.lo[0], k.lo[1], k.lo[2]},
-                                             {k.hi[0], k.hi[1], k.hi[2]}});
-                if (k.n >= kGroupMin) {
-                    if (lane == 0) s_bq[btail % kBigCap] = e2;
-                    btail++;
-                } else {
-                    if (lane == 0) s_q[qtail % kQueueCap] = e2;
-                    qtail++;
-                }
             } else {
-                md = max(md, k.depth);
+                float a[3], bh[3], c[3], d[3];
+                wave_centroid_bounds(s_perm, s_box, eb, nlf, lane, a, bh);
+                wave_centroid_bounds(s_perm, s_box, eb + nlf, en - nlf, lane, c, d);
+                if (lane == f)
+                    for (int k = 0; k < 3; ++k) {
+                        L.clo[k] = a[k];
+                        L.chi[k] = bh[k];
+                        R.clo[k] = c[k];
+                        R.chi[k] = d[k];
+                    }
             }
-        };
-        route(L);
-        route(R);
-        const bool lbig = L.n > kCoop, rbig = R.n > kCoop;
-        if (lbig && rbig) {
-            const bool left_small = L.n <= R.n;
-            if (sp < kCoopStack) {
-                if (lane == 0) s_stack[sp] = pick_ent(left_small, R, L);
-                sp++;
-            } else {
-                fail = true;
-            }
-            e = pick_ent(left_small, L, R);
-        } else if (lbig || rbig) {
-            e = pick_ent(lbig, L, R);
-        } else if (sp > 0) {
-            wave_sync();
-            e = s_stack[--sp];
-        } else {
-            have = false;
         }
-        wave_sync();
+        wave_sync();  // every read of this round's entries is done before the children are queued
+        fhead += nf;
+        const bool act = lane < nf;
+        const uint32_t cnode = next + 2 * lane;
+        next += 2 * nf;  // every node here splits
+        if (act) {
+            L.slot = cnode;
+            L.b = e.b;
+            L.n = nl;
+            R.slot = cnode + 1;
+            R.b = e.b + nl;
+            R.n = e.n - nl;
+            L.depth = R.depth = e.depth + 1;
+            local[cnode] = BNodeG{{L.lo[0], L.lo[1], L.lo[2]}, {L.hi[0], L.hi[1], L.hi[2]}, task.begin + L.b, L.n};
+            local[cnode + 1] = BNodeG{{R.lo[0], R.lo[1], R.lo[2]}, {R.hi[0], R.hi[1], R.hi[2]}, task.begin + R.b, R.n};
+            *slot_ptr(e.slot) = BNodeG{{e.lo[0], e.lo[1], e.lo[2]}, {e.hi[0], e.hi[1], e.hi[2]}, cnode, 0u};
+        }
+        // children: more than kCoop triangles to the next rounds, kGroupMin.. to the groups' queue,
+        // 2.. to the lanes' queue; single triangles are leaves as written
+        const int cl = act ? (L.n > kCoop ? 0 : (L.n >= kGroupMin ? 1 : (L.n >= 2 ? 2 : 3))) : 4;
+        const int cr = act ? (R.n > kCoop ? 0 : (R.n >= kGroupMin ? 1 : (R.n >= 2 ? 2 : 3))) : 4;
+        {
+            const uint64_t ml = __ballot(cl == 0), mr = __ballot(cr == 0);
+            const uint32_t pos = ftail + rank_below(ml) + rank_below(mr);
+            if (cl == 0) s_fq[pos % kFrontCap] = L;
+            if (cr == 0) s_fq[(pos + (cl == 0 ? 1u : 0u)) % kFrontCap] = R;
+            ftail += (uint32_t)(__popcll(ml) + __popcll(mr));
+        }
+        {
+            const uint64_t ml = __ballot(cl == 1), mr = __ballot(cr == 1);
+            const uint32_t pos = btail + rank_below(ml) + rank_below(mr);
+            if (cl == 1) s_bq[pos % kBigCap] = q_pack(QEnt{L.slot, L.b, L.n, L.depth, {L.lo[0], L.lo[1], L.lo[2]},
+                                                          {L.hi[0], L.hi[1], L.hi[2]}});
+            if (cr == 1)
+                s_bq[(pos + (cl == 1 ? 1u : 0u)) % kBigCap] = q_pack(QEnt{R.slot, R.b, R.n, R.depth,
+                                                                         {R.lo[0], R.lo[1], R.lo[2]},
+                                                                         {R.hi[0], R.hi[1], R.hi[2]}});
+            btail += (uint32_t)(__popcll(ml) + __popcll(mr));
+        }
+        {
+            const uint64_t ml = __ballot(cl == 2), mr = __ballot(cr == 2);
+            const uint32_t pos = qtail + rank_below(ml) + rank_below(mr);
+            if (cl == 2) s_q[pos % kQueueCap] = q_pack(QEnt{L.slot, L.b, L.n, L.depth, {L.lo[0], L.lo[1], L.lo[2]},
+                                                           {L.hi[0], L.hi[1], L.hi[2]}});
+            if (cr == 2)
+                s_q[(pos + (cl == 2 ? 1u : 0u)) % kQueueCap] = q_pack(QEnt{R.slot, R.b, R.n, R.depth,
+                                                                          {R.lo[0], R.lo[1], R.lo[2]},
+                                                                          {R.hi[0], R.hi[1], R.hi[2]}});
+            qtail += (uint32_t)(__popcll(ml) + __popcll(mr));
+        }
+        if (cl == 3) md = max(md, L.depth);
+        if (cr == 3) md = max(md, R.depth);
+        if (ftail - fhead > kFrontCap || btail > kBigCap || qtail > kQueueCap) fail = true;
     }
     if (qtail > kQueueCap || btail > kBigCap) fail = true;
     // ------------------------------------------------ nodes of kGroupMin..kCoop triangles: 8 lanes each
