@@ -937,14 +937,16 @@ __device__ __forceinline__ bool lane_split(const QEnt& q, uint8_t* s_perm, const
 }
 
 // Eight lanes of a group (r = lane & 7 holds bin r) decide and split queued
-// node q together: the same decision, children and in-place partition as
-// lane_split, with the node's bins spread over the group instead of held by
-// one lane, so every lane reads each item once instead of updating eight bins
-// per item.  act: the group has a node.  Every lane of a group returns the
-// same values; the shuffles run on all 64 lanes.
-__device__ __forceinline__ bool group_split(const QEnt& q, bool act, uint32_t r, uint8_t* s_perm,
+// node q together: the same decision and children as lane_split, with the
+// node's bins spread over the group instead of held by one lane, so every lane
+// reads each item once instead of updating eight bins per item, and a stable
+// partition ranked by the group's ballots.  act: the group has a node.  Every
+// lane of a group returns the same values; the shuffles run on all 64 lanes.
+// bad is set when the partition's count disagrees with the bins'.
+__device__ __forceinline__ bool group_split(const QEnt& q, bool act, uint32_t r, uint8_t* s_perm, uint8_t* s_tmp,
                                             const float2* s_box, uint8_t* s_binof, int max_leaf, float trav_cost,
-                                            uint32_t& nl, float Llo[3], float Lhi[3], float Rlo[3], float Rhi[3]) {
+                                            uint32_t& nl, float Llo[3], float Lhi[3], float Rlo[3], float Rhi[3],
+                                            bool& bad) {
     const uint32_t n = act ? q.n : 0u;
     // centroid bounds: lane r takes items r, r + 8, ...
     float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -1061,20 +1063,27 @@ __device__ __forceinline__ bool group_split(const QEnt& q, bool act, uint32_t r,
             return false;
         }
         if (split >= 0) {
-            if (r == 0u) {  // the in-place partition of lane_split, by one lane
-                uint32_t i = q.b, j = q.b + n - 1;
-                while (i <= j) {
-                    const uint8_t pi = s_perm[i];
-                    if ((int)s_binof[pi] <= split) {
-                        i++;
-                    } else {
-                        s_perm[i] = s_perm[j];
-                        s_perm[j] = pi;
-                        if (j == q.b) break;
-                        j--;
-                    }
+            // stable partition through s_tmp: lane r takes items k0 + r, ranked by the group's ballots
+            const uint32_t gshift = threadIdx.x & ~7u, below = (1u << r) - 1u;
+            uint32_t lb = 0, rb = 0;
+            for (uint32_t k0 = 0; k0 < n; k0 += 8) {
+                const bool valid = k0 + r < n;
+                uint32_t j = 0;
+                bool left = false;
+                if (valid) {
+                    j = s_perm[q.b + k0 + r];
+                    left = (int)s_binof[j] <= split;
                 }
+                const uint32_t lm = (uint32_t)(__ballot(valid && left) >> gshift) & 0xffu;
+                const uint32_t rm = (uint32_t)(__ballot(valid && !left) >> gshift) & 0xffu;
+                if (valid)
+                    s_tmp[q.b + (left ? lb + __popc(lm & below) : snl + rb + __popc(rm & below))] = (uint8_t)j;
+                lb += (uint32_t)__popc(lm);
+                rb += (uint32_t)__popc(rm);
             }
+            if (lb != snl) bad = true;
+            wave_sync();
+            for (uint32_t i = r; i < n; i += 8) s_perm[q.b + i] = s_tmp[q.b + i];
             nl = snl;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
@@ -1112,10 +1121,11 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
     BNodeG* const local = snodes + task.base;
     const uint32_t n0 = task.count;
     const uint32_t* const src = task.buf ? alt : prims;  // the task's ids; its final order goes to prims
+    // all ids, then all boxes: the gathers of a lane are in flight together; the ids stay in
+    // registers for the final order
+    constexpr int kPer = kSmall / 64;
+    uint32_t ids[kPer];
     {
-        // all ids, then all boxes: the gathers of a lane are in flight together
-        constexpr int kPer = kSmall / 64;
-        uint32_t ids[kPer];
 #pragma unroll
         for (int r = 0; r < kPer; ++r) {
             const uint32_t i = lane + 64u * r;
@@ -1368,8 +1378,8 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
             uint32_t nl = 0;
             float Llo[3], Lhi[3], Rlo[3], Rhi[3];
             if (act) md = max(md, q.depth);
-            const bool split = group_split(q, act, r, s_perm, s_box, s_binof, max_leaf, trav_cost, nl, Llo, Lhi,
-                                           Rlo, Rhi);
+            const bool split = group_split(q, act, r, s_perm, s_tmp, s_box, s_binof, max_leaf, trav_cost, nl, Llo,
+                                           Lhi, Rlo, Rhi, fail);
             const bool lead = act && r == 0u;
             const uint64_t sm = __ballot(lead && split);
             const uint32_t c = next + 2 * rank_below(sm);
@@ -1442,10 +1452,11 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
     wave_sync();
     // the task's items in their final order
     {
-        // the original ids through LDS (the boxes' space, no longer used): each store below
-        // depends on an LDS read made after every lane's load has landed in LDS
+        // the original ids through LDS (the boxes' space, no longer used)
         uint32_t* s_ids = reinterpret_cast<uint32_t*>(s_box);
-        for (uint32_t i = lane; i < n0; i += 64) s_ids[i] = src[task.begin + i];
+#pragma unroll
+        for (int r = 0; r < kPer; ++r)
+            if (lane + 64u * r < n0) s_ids[lane + 64u * r] = ids[r];
         wave_sync();
         for (uint32_t i = lane; i < n0; i += 64) prims[task.begin + i] = s_ids[s_perm[i]];
     }
