@@ -702,7 +702,7 @@ __global__ void k_lv_advance(uint32_t* cnt, int parity, uint32_t* rec) {
 //   * queued nodes of kGroupMin..kCoop triangles are split by groups of 8
 //     lanes, 8 nodes per round: lane r of a group holds bin r, every lane
 //     reads each item once, the sweep runs as prefix unions across the
-//     group, and one lane does the reference's in-place partition;
+//     group, and the group partitions the items stably by its ballots;
 //   * smaller queued nodes are split one per lane (bins in registers, the
 //     same in-place partition).
 // Child pairs are numbered by prefix sums over a round's splitting nodes, so
@@ -712,7 +712,8 @@ __global__ void k_lv_advance(uint32_t* cnt, int parity, uint32_t* rec) {
 // one fixed sweep), so the tree is the same as a serial build's whatever the
 // order in which nodes are split; the item order inside a node, which the
 // object-median fallback and the leaves' triangle order follow, depends only
-// on which of the two partitions a node of its size takes.
+// on which partition (stable, or the lanes' in-place one) a node of its size
+// takes.
 constexpr uint32_t kTaskRoot = 0xffffffffu;
 constexpr uint32_t kCoop = 32;              // nodes above this are split by the whole wave
 constexpr uint32_t kQueueCap = kSmall / 2;  // queued nodes hold >= 2 triangles each, disjoint
