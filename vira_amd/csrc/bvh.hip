@@ -725,26 +725,53 @@ static_assert(kSmall <= 256, "item numbers of a subtree are bytes");
 static_assert(kCoop > 15, "wave-split nodes are never leaves (max_leaf <= 15)");
 static_assert(kSmallBins == 8, "the wave-split binning puts one bin on each lane of a group of 8");
 
-struct QEnt {  // queued node: slot (kTaskRoot or a scratch index), first item, items, depth, box
+// A node's box is not queued: it is the union of the boxes of the items in
+// its box range, which is the node's own item range for a child of a binned
+// split (its box is the union of its bins', i.e. of its items') and its
+// parent's box range for a child of an object-median split (whose box is its
+// parent's); root: the box is the task root's (from the large levels).
+struct BoxRef {
+    uint32_t b, n;
+    bool root;
+};
+// c ? a : b, field by field (a select of whole structs would go through scratch memory)
+__device__ __forceinline__ BoxRef pick_ref(bool c, const BoxRef& a, const BoxRef& b) {
+    return BoxRef{c ? a.b : b.b, c ? a.n : b.n, c ? a.root : b.root};
+}
+struct QEnt {  // queued node: slot (kTaskRoot or a scratch index), first item, items, depth, box range; the
+               // box, filled in from the box range by the node's splitter
     uint32_t slot, b, n, depth;
+    BoxRef ref;
     float lo[3], hi[3];
 };
-struct CEnt {  // wave-split node: the same plus its centroid bounds
+struct CEnt {  // wave-split node: slot, first item, items, depth, box range (packed), centroid bounds
     uint32_t slot, b, n, depth;
-    float lo[3], hi[3], clo[3], chi[3];
+    uint32_t ref;  // b | n << 8 | root << 17
+    float clo[3], chi[3];
 };
+__device__ __forceinline__ uint32_t ref_pack(const BoxRef& r) { return r.b | (r.n << 8) | (r.root ? 1u << 17 : 0u); }
+__device__ __forceinline__ BoxRef ref_unpack(uint32_t w) { return BoxRef{w & 0xffu, (w >> 8) & 0x1ffu, ((w >> 17) & 1u) != 0u}; }
 
-// A queued node as kept in LDS (32 bytes): first item, items and depth packed in one word.
+// A queued node as kept in LDS (8 bytes): slot (10 bits, the task root as all ones), first item,
+// items and the root flag in w0; depth (14 bits) and the box range in w1.
 struct QEntL {
-    uint32_t slot, bnd;  // bnd: b | n << 8 | depth << 17
-    float lo[3], hi[3];
+    uint32_t w0, w1;
 };
-__device__ __forceinline__ QEntL q_pack(const QEnt& q) {
-    return QEntL{q.slot, q.b | (q.n << 8) | (q.depth << 17), {q.lo[0], q.lo[1], q.lo[2]}, {q.hi[0], q.hi[1], q.hi[2]}};
+constexpr uint32_t kSlotRootPacked = 0x3ffu;
+__device__ __forceinline__ QEntL q_pack(uint32_t slot, uint32_t b, uint32_t n, uint32_t depth, const BoxRef& r) {
+    const uint32_t ps = slot == kTaskRoot ? kSlotRootPacked : slot;
+    return QEntL{ps | (b << 10) | (n << 18) | (r.root ? 1u << 27 : 0u), depth | (r.b << 14) | (r.n << 22)};
 }
 __device__ __forceinline__ QEnt q_unpack(const QEntL& l) {
-    return QEnt{l.slot, l.bnd & 0xffu, (l.bnd >> 8) & 0x1ffu, l.bnd >> 17, {l.lo[0], l.lo[1], l.lo[2]},
-                {l.hi[0], l.hi[1], l.hi[2]}};
+    QEnt q;
+    const uint32_t ps = l.w0 & 0x3ffu;
+    q.slot = ps == kSlotRootPacked ? kTaskRoot : ps;
+    q.b = (l.w0 >> 10) & 0xffu;
+    q.n = (l.w0 >> 18) & 0x1ffu;
+    q.depth = l.w1 & 0x3fffu;
+    q.ref = BoxRef{(l.w1 >> 14) & 0xffu, (l.w1 >> 22) & 0x1ffu, ((l.w0 >> 27) & 1u) != 0u};
+    for (int k = 0; k < 3; ++k) q.lo[k] = q.hi[k] = 0.0f;
+    return q;
 }
 
 // Item j's box, kept in LDS as three float2 (24 bytes): (lo.x, lo.y), (lo.z, hi.x), (hi.y, hi.z).
@@ -855,10 +882,15 @@ __device__ __forceinline__ void split_boxes8(const uint32_t bcnt[kSmallBins], co
 // stay in registers (every bin updated under a compare with its compile-time
 // index, no indexing into registers); each item's bin is kept in s_binof for
 // the partition.
-__device__ __forceinline__ bool lane_split(const QEnt& q, uint8_t* s_perm, const float2* s_box, uint8_t* s_binof, int max_leaf, float trav_cost, uint32_t& nl, float Llo[3],
-                                           float Lhi[3], float Rlo[3], float Rhi[3]) {
+__device__ __forceinline__ bool lane_split(QEnt& q, uint8_t* s_perm, const float2* s_box, uint8_t* s_binof,
+                                           const float root_lo[3], const float root_hi[3], int max_leaf,
+                                           float trav_cost, uint32_t& nl, float Llo[3], float Lhi[3], float Rlo[3],
+                                           float Rhi[3], bool& median) {
     const uint32_t n = q.n;
+    const bool own = !q.ref.root && q.ref.b == q.b && q.ref.n == q.n;  // the box is the items' union
     float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { q.lo[k] = INFINITY; q.hi[k] = -INFINITY; }
     for (uint32_t i = 0; i < n; ++i) {
         const Box6 x = box_at(s_box, s_perm[q.b + i]);
 #pragma unroll
@@ -866,8 +898,26 @@ __device__ __forceinline__ bool lane_split(const QEnt& q, uint8_t* s_perm, const
             const float c = 0.5f * (x.lo[k] + x.hi[k]);
             clo[k] = fminf(clo[k], c);
             chi[k] = fmaxf(chi[k], c);
+            if (own) {
+                q.lo[k] = fminf(q.lo[k], x.lo[k]);
+                q.hi[k] = fmaxf(q.hi[k], x.hi[k]);
+            }
         }
     }
+    if (q.ref.root) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { q.lo[k] = root_lo[k]; q.hi[k] = root_hi[k]; }
+    } else if (!own) {
+        for (uint32_t i = 0; i < q.ref.n; ++i) {
+            const Box6 x = box_at(s_box, s_perm[q.ref.b + i]);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                q.lo[k] = fminf(q.lo[k], x.lo[k]);
+                q.hi[k] = fmaxf(q.hi[k], x.hi[k]);
+            }
+        }
+    }
+    median = false;
     const float ext[3] = {chi[0] - clo[0], chi[1] - clo[1], chi[2] - clo[2]};
     int axis = (ext[1] > ext[0]) ? 1 : 0;
     if (ext[2] > pick3(ext, axis)) axis = 2;
@@ -933,6 +983,7 @@ __device__ __forceinline__ bool lane_split(const QEnt& q, uint8_t* s_perm, const
     }
     // object median; the children keep the parent box (conservative)
     nl = n / 2;
+    median = true;
     for (int k = 0; k < 3; ++k) { Llo[k] = Rlo[k] = q.lo[k]; Lhi[k] = Rhi[k] = q.hi[k]; }
     return true;
 }
@@ -944,13 +995,17 @@ __device__ __forceinline__ bool lane_split(const QEnt& q, uint8_t* s_perm, const
 // partition ranked by the group's ballots.  act: the group has a node.  Every
 // lane of a group returns the same values; the shuffles run on all 64 lanes.
 // bad is set when the partition's count disagrees with the bins'.
-__device__ __forceinline__ bool group_split(const QEnt& q, bool act, uint32_t r, uint8_t* s_perm, uint8_t* s_tmp,
-                                            const float2* s_box, uint8_t* s_binof, int max_leaf, float trav_cost,
-                                            uint32_t& nl, float Llo[3], float Lhi[3], float Rlo[3], float Rhi[3],
+__device__ __forceinline__ bool group_split(QEnt& q, bool act, uint32_t r, uint8_t* s_perm, uint8_t* s_tmp,
+                                            const float2* s_box, uint8_t* s_binof, const float root_lo[3],
+                                            const float root_hi[3], int max_leaf, float trav_cost, uint32_t& nl,
+                                            float Llo[3], float Lhi[3], float Rlo[3], float Rhi[3], bool& median,
                                             bool& bad) {
     const uint32_t n = act ? q.n : 0u;
-    // centroid bounds: lane r takes items r, r + 8, ...
+    // centroid bounds and the node's box: lane r takes items r, r + 8, ... (of the box range when it
+    // is not the node's own item range)
+    const bool own = !q.ref.root && q.ref.b == q.b && q.ref.n == q.n;
     float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    float nlo[3] = {INFINITY, INFINITY, INFINITY}, nhi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (uint32_t i = r; i < n; i += 8) {
         const Box6 x = box_at(s_box, s_perm[q.b + i]);
 #pragma unroll
@@ -958,6 +1013,20 @@ __device__ __forceinline__ bool group_split(const QEnt& q, bool act, uint32_t r,
             const float c = 0.5f * (x.lo[k] + x.hi[k]);
             clo[k] = fminf(clo[k], c);
             chi[k] = fmaxf(chi[k], c);
+            if (own) {
+                nlo[k] = fminf(nlo[k], x.lo[k]);
+                nhi[k] = fmaxf(nhi[k], x.hi[k]);
+            }
+        }
+    }
+    if (act && !own && !q.ref.root) {
+        for (uint32_t i = r; i < q.ref.n; i += 8) {
+            const Box6 x = box_at(s_box, s_perm[q.ref.b + i]);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                nlo[k] = fminf(nlo[k], x.lo[k]);
+                nhi[k] = fmaxf(nhi[k], x.hi[k]);
+            }
         }
     }
 #pragma unroll
@@ -966,7 +1035,15 @@ __device__ __forceinline__ bool group_split(const QEnt& q, bool act, uint32_t r,
         for (int k = 0; k < 3; ++k) {
             clo[k] = fminf(clo[k], __shfl_xor(clo[k], off, 64));
             chi[k] = fmaxf(chi[k], __shfl_xor(chi[k], off, 64));
+            nlo[k] = fminf(nlo[k], __shfl_xor(nlo[k], off, 64));
+            nhi[k] = fmaxf(nhi[k], __shfl_xor(nhi[k], off, 64));
         }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        q.lo[k] = q.ref.root ? root_lo[k] : nlo[k];
+        q.hi[k] = q.ref.root ? root_hi[k] : nhi[k];
+    }
+    median = false;
     const float ext[3] = {chi[0] - clo[0], chi[1] - clo[1], chi[2] - clo[2]};
     int axis = (ext[1] > ext[0]) ? 1 : 0;
     if (ext[2] > pick3(ext, axis)) axis = 2;
@@ -1098,11 +1175,12 @@ __device__ __forceinline__ bool group_split(const QEnt& q, bool act, uint32_t r,
     }
     // object median; the children keep the parent box (conservative)
     nl = n / 2;
+    median = true;
     for (int k = 0; k < 3; ++k) { Llo[k] = Rlo[k] = q.lo[k]; Lhi[k] = Rhi[k] = q.hi[k]; }
     return true;
 }
 
-__global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, const uint32_t* range,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_build_small(const SmallTask* tasks, const uint32_t* range,
                                                     uint32_t* prims, const uint32_t* alt, const PrimBox* pb, BNodeG* nodes, BNodeG* snodes,
                                                     uint32_t* used, uint32_t* max_depth, uint32_t* err, int max_leaf,
                                                     float trav_cost) {
@@ -1153,6 +1231,33 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
     uint32_t qtail = 0, btail = 0, md = task.depth;
     bool fail = false;
     auto slot_ptr = [&](uint32_t s) -> BNodeG* { return s == kTaskRoot ? &nodes[task.node] : &local[s]; };
+    // a node that splits: its record (box written when it was created) gets its first child
+    auto set_internal = [&](uint32_t s, uint32_t c) {
+        BNodeG* nd = slot_ptr(s);
+        nd->left = c;
+        nd->count = 0u;
+    };
+    // the box of a whole-wave node's box range, by the whole wave
+    auto wave_ref_box = [&](const BoxRef& r, float lo[3], float hi[3]) {
+        float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t i = lane; i < r.n; i += 64) {
+            const Box6 x = box_at(s_box, s_perm[r.b + i]);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                l[k] = fminf(l[k], x.lo[k]);
+                h[k] = fmaxf(h[k], x.hi[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            for (int off = 32; off > 0; off >>= 1) {
+                l[k] = fminf(l[k], __shfl_xor(l[k], off));
+                h[k] = fmaxf(h[k], __shfl_xor(h[k], off));
+            }
+            lo[k] = r.root ? root.lo[k] : l[k];
+            hi[k] = r.root ? root.hi[k] : h[k];
+        }
+    };
     // ------------------------------------------------ nodes of more than kCoop triangles: the whole wave,
     // up to kFront of them at a time, breadth first (they are disjoint, so at most kFrontCap - 1 exist)
     uint32_t fhead = 0, ftail = 0;
@@ -1162,13 +1267,12 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
         e.b = 0;
         e.n = n0;
         e.depth = task.depth;
-        for (int k = 0; k < 3; ++k) { e.lo[k] = root.lo[k]; e.hi[k] = root.hi[k]; }
+        e.ref = ref_pack(BoxRef{0u, n0, true});
         wave_centroid_bounds(s_perm, s_box, 0, n0, lane, e.clo, e.chi);
         if (lane == 0) s_fq[0] = e;
         ftail = 1;
     } else if (n0 >= 2) {
-        const QEntL r0 = q_pack(QEnt{kTaskRoot, 0u, n0, task.depth, {root.lo[0], root.lo[1], root.lo[2]},
-                                     {root.hi[0], root.hi[1], root.hi[2]}});
+        const QEntL r0 = q_pack(kTaskRoot, 0u, n0, task.depth, BoxRef{0u, n0, true});
         if (n0 >= kGroupMin) {
             if (lane == 0) s_bq[0] = r0;
             btail = 1;
@@ -1212,6 +1316,7 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
         wave_sync();
         // lane f < nf: node f's SAH decision and its children (vira_blas.ipp:185-254 with bins)
         CEnt e{}, L{}, R{};
+        float Llo[3] = {}, Lhi[3] = {}, Rlo[3] = {}, Rhi[3] = {};  // the children's boxes
         uint32_t nl = 0;
         int split = -1;
         if (lane < nf) {
@@ -1235,7 +1340,7 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
                 float best;
                 sweep8(bcnt, blo, bhi, split, best);
                 if (split >= 0) {
-                    split_boxes8(bcnt, blo, bhi, split, nl, L.lo, L.hi, R.lo, R.hi);
+                    split_boxes8(bcnt, blo, bhi, split, nl, Llo, Lhi, Rlo, Rhi);
                     // the children's centroid bounds: unions of the bins' (in key order, then converted)
                     int kl[3] = {INT_MAX, INT_MAX, INT_MAX}, kh[3] = {INT_MIN, INT_MIN, INT_MIN};
                     int rl[3] = {INT_MAX, INT_MAX, INT_MAX}, rh[3] = {INT_MIN, INT_MIN, INT_MIN};
@@ -1261,15 +1366,11 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
                     }
                 }
             }
-            if (split < 0) {
-                // object median; the children keep the parent box (conservative) and get exact
-                // centroid bounds (below)
-                nl = e.n / 2;
-                for (int k = 0; k < 3; ++k) {
-                    L.lo[k] = R.lo[k] = e.lo[k];
-                    L.hi[k] = R.hi[k] = e.hi[k];
-                }
-            }
+            // object median: the children keep the parent box (conservative, below) and its box
+            // range, and get exact centroid bounds (below)
+            if (split < 0) nl = e.n / 2;
+            L.ref = split < 0 ? e.ref : ref_pack(BoxRef{e.b, nl, false});
+            R.ref = split < 0 ? e.ref : ref_pack(BoxRef{e.b + nl, e.n - nl, false});
             s_split[lane] = split;
         }
         wave_sync();
@@ -1298,15 +1399,19 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
                 for (uint32_t i = lane; i < en; i += 64) s_perm[eb + i] = s_tmp[eb + i];
                 wave_sync();
             } else {
-                float a[3], bh[3], c[3], d[3];
+                float a[3], bh[3], c[3], d[3], plo[3], phi[3];
                 wave_centroid_bounds(s_perm, s_box, eb, nlf, lane, a, bh);
                 wave_centroid_bounds(s_perm, s_box, eb + nlf, en - nlf, lane, c, d);
+                const CEnt& ef = s_fq[(fhead + f) % kFrontCap];
+                wave_ref_box(ref_unpack(ef.ref), plo, phi);
                 if (lane == f)
                     for (int k = 0; k < 3; ++k) {
                         L.clo[k] = a[k];
                         L.chi[k] = bh[k];
                         R.clo[k] = c[k];
                         R.chi[k] = d[k];
+                        Llo[k] = Rlo[k] = plo[k];
+                        Lhi[k] = Rhi[k] = phi[k];
                     }
             }
         }
@@ -1323,9 +1428,9 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
             R.b = e.b + nl;
             R.n = e.n - nl;
             L.depth = R.depth = e.depth + 1;
-            local[cnode] = BNodeG{{L.lo[0], L.lo[1], L.lo[2]}, {L.hi[0], L.hi[1], L.hi[2]}, task.begin + L.b, L.n};
-            local[cnode + 1] = BNodeG{{R.lo[0], R.lo[1], R.lo[2]}, {R.hi[0], R.hi[1], R.hi[2]}, task.begin + R.b, R.n};
-            *slot_ptr(e.slot) = BNodeG{{e.lo[0], e.lo[1], e.lo[2]}, {e.hi[0], e.hi[1], e.hi[2]}, cnode, 0u};
+            local[cnode] = BNodeG{{Llo[0], Llo[1], Llo[2]}, {Lhi[0], Lhi[1], Lhi[2]}, task.begin + L.b, L.n};
+            local[cnode + 1] = BNodeG{{Rlo[0], Rlo[1], Rlo[2]}, {Rhi[0], Rhi[1], Rhi[2]}, task.begin + R.b, R.n};
+            set_internal(e.slot, cnode);
         }
         // children: more than kCoop triangles to the next rounds, kGroupMin.. to the groups' queue,
         // 2.. to the lanes' queue; single triangles are leaves as written
@@ -1341,23 +1446,17 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
         {
             const uint64_t ml = __ballot(cl == 1), mr = __ballot(cr == 1);
             const uint32_t pos = btail + rank_below(ml) + rank_below(mr);
-            if (cl == 1) s_bq[pos % kBigCap] = q_pack(QEnt{L.slot, L.b, L.n, L.depth, {L.lo[0], L.lo[1], L.lo[2]},
-                                                          {L.hi[0], L.hi[1], L.hi[2]}});
+            if (cl == 1) s_bq[pos % kBigCap] = q_pack(L.slot, L.b, L.n, L.depth, ref_unpack(L.ref));
             if (cr == 1)
-                s_bq[(pos + (cl == 1 ? 1u : 0u)) % kBigCap] = q_pack(QEnt{R.slot, R.b, R.n, R.depth,
-                                                                         {R.lo[0], R.lo[1], R.lo[2]},
-                                                                         {R.hi[0], R.hi[1], R.hi[2]}});
+                s_bq[(pos + (cl == 1 ? 1u : 0u)) % kBigCap] = q_pack(R.slot, R.b, R.n, R.depth, ref_unpack(R.ref));
             btail += (uint32_t)(__popcll(ml) + __popcll(mr));
         }
         {
             const uint64_t ml = __ballot(cl == 2), mr = __ballot(cr == 2);
             const uint32_t pos = qtail + rank_below(ml) + rank_below(mr);
-            if (cl == 2) s_q[pos % kQueueCap] = q_pack(QEnt{L.slot, L.b, L.n, L.depth, {L.lo[0], L.lo[1], L.lo[2]},
-                                                           {L.hi[0], L.hi[1], L.hi[2]}});
+            if (cl == 2) s_q[pos % kQueueCap] = q_pack(L.slot, L.b, L.n, L.depth, ref_unpack(L.ref));
             if (cr == 2)
-                s_q[(pos + (cl == 2 ? 1u : 0u)) % kQueueCap] = q_pack(QEnt{R.slot, R.b, R.n, R.depth,
-                                                                          {R.lo[0], R.lo[1], R.lo[2]},
-                                                                          {R.hi[0], R.hi[1], R.hi[2]}});
+                s_q[(pos + (cl == 2 ? 1u : 0u)) % kQueueCap] = q_pack(R.slot, R.b, R.n, R.depth, ref_unpack(R.ref));
             qtail += (uint32_t)(__popcll(ml) + __popcll(mr));
         }
         if (cl == 3) md = max(md, L.depth);
@@ -1378,9 +1477,10 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
             wave_sync();
             uint32_t nl = 0;
             float Llo[3], Lhi[3], Rlo[3], Rhi[3];
+            bool median = false;
             if (act) md = max(md, q.depth);
-            const bool split = group_split(q, act, r, s_perm, s_tmp, s_box, s_binof, max_leaf, trav_cost, nl, Llo,
-                                           Lhi, Rlo, Rhi, fail);
+            const bool split = group_split(q, act, r, s_perm, s_tmp, s_box, s_binof, root.lo, root.hi, max_leaf,
+                                           trav_cost, nl, Llo, Lhi, Rlo, Rhi, median, fail);
             const bool lead = act && r == 0u;
             const uint64_t sm = __ballot(lead && split);
             const uint32_t c = next + 2 * rank_below(sm);
@@ -1389,7 +1489,7 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
             if (lead && split) {
                 local[c] = BNodeG{{Llo[0], Llo[1], Llo[2]}, {Lhi[0], Lhi[1], Lhi[2]}, task.begin + q.b, nl};
                 local[c + 1] = BNodeG{{Rlo[0], Rlo[1], Rlo[2]}, {Rhi[0], Rhi[1], Rhi[2]}, task.begin + q.b + nl, nr};
-                *slot_ptr(q.slot) = BNodeG{{q.lo[0], q.lo[1], q.lo[2]}, {q.hi[0], q.hi[1], q.hi[2]}, c, 0u};
+                set_internal(q.slot, c);
                 if (nl == 1 || nr == 1) md = max(md, q.depth + 1);
             }
             // children: kGroupMin or more triangles to this queue, 2.. to the lanes' queue
@@ -1398,9 +1498,9 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
             const uint64_t mlb = __ballot(lb), mrb = __ballot(rbg), mls = __ballot(ls), mrs = __ballot(rs);
             const uint32_t pb2 = btail + rank_below(mlb) + rank_below(mrb);
             const uint32_t ps = qtail + rank_below(mls) + rank_below(mrs);
-            const QEntL eL = q_pack(QEnt{c, q.b, nl, q.depth + 1, {Llo[0], Llo[1], Llo[2]}, {Lhi[0], Lhi[1], Lhi[2]}});
+            const QEntL eL = q_pack(c, q.b, nl, q.depth + 1, pick_ref(median, q.ref, BoxRef{q.b, nl, false}));
             const QEntL eR =
-                q_pack(QEnt{c + 1, q.b + nl, nr, q.depth + 1, {Rlo[0], Rlo[1], Rlo[2]}, {Rhi[0], Rhi[1], Rhi[2]}});
+                q_pack(c + 1, q.b + nl, nr, q.depth + 1, pick_ref(median, q.ref, BoxRef{q.b + nl, nr, false}));
             if (lb) s_bq[pb2 % kBigCap] = eL;
             if (rbg) s_bq[(pb2 + (lb ? 1u : 0u)) % kBigCap] = eR;
             if (ls) s_q[ps % kQueueCap] = eL;
@@ -1422,10 +1522,11 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
         wave_sync();
         uint32_t nl = 0;
         float Llo[3], Lhi[3], Rlo[3], Rhi[3];
-        bool split = false;
+        bool split = false, median = false;
         if (act) {
             md = max(md, q.depth);
-            split = lane_split(q, s_perm, s_box, s_binof, max_leaf, trav_cost, nl, Llo, Lhi, Rlo, Rhi);
+            split = lane_split(q, s_perm, s_box, s_binof, root.lo, root.hi, max_leaf, trav_cost, nl, Llo, Lhi, Rlo,
+                               Rhi, median);
         }
         const uint64_t sm = __ballot(split);
         const uint32_t c = next + 2 * rank_below(sm);
@@ -1434,18 +1535,16 @@ __global__ void __launch_bounds__(64) k_build_small(const SmallTask* tasks, cons
         if (split) {
             local[c] = BNodeG{{Llo[0], Llo[1], Llo[2]}, {Lhi[0], Lhi[1], Lhi[2]}, task.begin + q.b, nl};
             local[c + 1] = BNodeG{{Rlo[0], Rlo[1], Rlo[2]}, {Rhi[0], Rhi[1], Rhi[2]}, task.begin + q.b + nl, nr};
-            *slot_ptr(q.slot) = BNodeG{{q.lo[0], q.lo[1], q.lo[2]}, {q.hi[0], q.hi[1], q.hi[2]}, c, 0u};
+            set_internal(q.slot, c);
             if (nl == 1 || nr == 1) md = max(md, q.depth + 1);
         }
         const bool pl = split && nl >= 2, pr = split && nr >= 2;
         const uint64_t ml = __ballot(pl), mr = __ballot(pr);
         const uint32_t pos = qtail + rank_below(ml) + rank_below(mr);
-        if (pl)
-            s_q[pos % kQueueCap] =
-                q_pack(QEnt{c, q.b, nl, q.depth + 1, {Llo[0], Llo[1], Llo[2]}, {Lhi[0], Lhi[1], Lhi[2]}});
+        if (pl) s_q[pos % kQueueCap] = q_pack(c, q.b, nl, q.depth + 1, pick_ref(median, q.ref, BoxRef{q.b, nl, false}));
         if (pr)
             s_q[(pos + (pl ? 1u : 0u)) % kQueueCap] =
-                q_pack(QEnt{c + 1, q.b + nl, nr, q.depth + 1, {Rlo[0], Rlo[1], Rlo[2]}, {Rhi[0], Rhi[1], Rhi[2]}});
+                q_pack(c + 1, q.b + nl, nr, q.depth + 1, pick_ref(median, q.ref, BoxRef{q.b + nl, nr, false}));
         qtail += (uint32_t)(__popcll(ml) + __popcll(mr));
         if (qtail - qhead > kQueueCap) fail = true;
         wave_sync();
