@@ -330,33 +330,37 @@ __global__ void __launch_bounds__(kChunkThreads) k_bin_large(const Chunk* chunks
         int acc[kBinVals];
 #pragma unroll
         for (int k = 0; k < kBinVals; ++k) acc[k] = 0;
-        const uint32_t wfirst = ch.begin + (uint32_t)w * (kChunkItems * 64);
+        // the chunk's slots split over the waves in whole stages, so a small chunk keeps every wave busy
+        const uint32_t per = (ch.end - ch.begin + (kChunkThreads / 64) * kStageSlots - 1) /
+                             ((kChunkThreads / 64) * kStageSlots) * kStageSlots;
+        const uint32_t wfirst = ch.begin + (uint32_t)w * per;
+        const uint32_t wend = min(wfirst + per, ch.end);
         constexpr int kStages = kChunkItems / kStageItems;
         uint32_t idn[kStageItems];
         PrimBox pn[kStageItems];
 #pragma unroll
         for (int k = 0; k < kStageItems; ++k) {
             const uint32_t slot = wfirst + k * 64 + lane;
-            idn[k] = slot < ch.end ? prims[slot] : 0u;
+            idn[k] = slot < wend ? prims[slot] : 0u;
         }
 #pragma unroll
         for (int k = 0; k < kStageItems; ++k)
-            if (wfirst + k * 64 + lane < ch.end) pn[k] = pb[idn[k]];
+            if (wfirst + k * 64 + lane < wend) pn[k] = pb[idn[k]];
 #pragma unroll
         for (int k = 0; k < kStageItems; ++k) {
             const uint32_t slot = wfirst + kStageSlots + k * 64 + lane;
-            idn[k] = slot < ch.end ? prims[slot] : 0u;
+            idn[k] = slot < wend ? prims[slot] : 0u;
         }
         for (int st = 0; st < kStages; ++st) {
             const uint32_t wbase = wfirst + (uint32_t)st * kStageSlots;
-            if (wbase >= ch.end) break;  // the same for the whole wave
+            if (wbase >= wend) break;  // the same for the whole wave
             PrimBox p[kStageItems];
 #pragma unroll
             for (int k = 0; k < kStageItems; ++k) p[k] = pn[k];
 #pragma unroll
             for (int k = 0; k < kStageItems; ++k) {
                 const uint32_t slot = wbase + k * 64 + lane;
-                if (slot < ch.end) {
+                if (slot < wend) {
                     const float c[3] = {centroid(p[k], 0), centroid(p[k], 1), centroid(p[k], 2)};
                     const int b = bin_of(pick3(c, tp.axis), tp.cmin, tp.scale, nb);
                     slot_bin[slot] = (uint8_t)b;
@@ -373,18 +377,18 @@ __global__ void __launch_bounds__(kChunkThreads) k_bin_large(const Chunk* chunks
             if (st + 1 < kStages) {
 #pragma unroll
                 for (int k = 0; k < kStageItems; ++k)
-                    if (wbase + kStageSlots + k * 64 + lane < ch.end) pn[k] = pb[idn[k]];
+                    if (wbase + kStageSlots + k * 64 + lane < wend) pn[k] = pb[idn[k]];
 #pragma unroll
                 for (int k = 0; k < kStageItems; ++k) {
                     const uint32_t slot = wbase + 2 * kStageSlots + k * 64 + lane;
-                    idn[k] = (st + 2 < kStages && slot < ch.end) ? prims[slot] : 0u;
+                    idn[k] = (st + 2 < kStages && slot < wend) ? prims[slot] : 0u;
                 }
             }
             wave_sync();
 #pragma unroll
             for (int j = 0; j < kStageItems; ++j) {
                 const int pos = lane * kStageItems + j;
-                if (wbase + pos >= ch.end) break;
+                if (wbase + pos >= wend) break;
                 const int q = stage_word(pos);
                 PrimBox pj;
 #pragma unroll
@@ -453,7 +457,11 @@ __global__ void __launch_bounds__(kChunkThreads) k_partition_large(const Chunk* 
         const ChunkBase cb = bases[ci];
         const uint32_t* prims = ch.buf ? ids1 : ids0;
         uint32_t* tmp = ch.buf ? ids0 : ids1;
-        const uint32_t wfirst = (uint32_t)w * (kChunkItems * 64);  // chunk-local index of the wave's first slot
+        // the chunk's slots split over the waves in whole steps of 64, so a small chunk keeps every
+        // wave busy; wfirst: chunk-local index of the wave's first slot
+        const uint32_t per = (ch.end - ch.begin + (kChunkThreads / 64) * 64 - 1) / ((kChunkThreads / 64) * 64) * 64;
+        const uint32_t wfirst = (uint32_t)w * per;
+        const uint32_t wend = min(ch.begin + wfirst + per, ch.end);
         const uint32_t s0 = ch.begin + wfirst + lane;
         uint32_t ids[kChunkItems];
         uint64_t lm[kChunkItems];  // lefts of each step
@@ -463,7 +471,7 @@ __global__ void __launch_bounds__(kChunkThreads) k_partition_large(const Chunk* 
             const uint32_t slot = s0 + k * 64;
             bool left = false;
             ids[k] = 0;
-            if (slot < ch.end) {
+            if (slot < wend) {
                 ids[k] = prims[slot];
                 left = (int)slot_bin[slot] <= tp.split;
             }
@@ -477,7 +485,7 @@ __global__ void __launch_bounds__(kChunkThreads) k_partition_large(const Chunk* 
 #pragma unroll
         for (int k = 0; k < kChunkItems; ++k) {
             const uint32_t slot = s0 + k * 64;
-            if (slot < ch.end) {
+            if (slot < wend) {
                 const uint32_t l = lbefore + rank_below(lm[k]);
                 if ((lm[k] >> lane) & 1ull) tmp[cb.left + l] = ids[k];
                 else tmp[cb.right + (wfirst + k * 64 + lane - l)] = ids[k];
