@@ -1828,42 +1828,54 @@ __global__ void k_refit_tris(size_t nt, const uint32_t* __restrict__ idx, const 
 __device__ __forceinline__ float f4_get(const float4& v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
 __device__ __forceinline__ uint32_t u4_get(const uint4& v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
 
-__global__ void k_refit4(Node4* __restrict__ nodes, uint32_t first, uint32_t count, const float4* __restrict__ tris) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// One level of the four-wide refit, one lane per child (four consecutive lanes
+// per node): a leaf child's box from its triangles, loaded up to four at a time
+// before the first min/max; an internal child's box as the union of its own
+// children's boxes.  Each lane writes only its child's six bounds, so the
+// references are not rewritten.  An empty child keeps a zero box.
+constexpr int kRefitThreads = 256;
+__global__ __launch_bounds__(kRefitThreads) void k_refit4(Node4* __restrict__ nodes, uint32_t first, uint32_t count,
+                                                         const float4* __restrict__ tris) {
+    const uint32_t g = blockIdx.x * kRefitThreads + threadIdx.x;
+    const uint32_t i = g >> 2, k = g & 3u;
     if (i >= count) return;
-    Node4 nd = nodes[first + i];
-    float lo[3][4], hi[3][4];
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t r = u4_get(nd.ref, k);
-        float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
-        if (r == kEmptyRef) {
-            for (int a = 0; a < 3; ++a) l[a] = h[a] = 0.0f;
-        } else if (ref_is_leaf(r)) {
-            const uint32_t f = leaf_first(r), n = leaf_count(r);
-            for (uint32_t t = f; t < f + n; ++t)
-                for (int v = 0; v < 3; ++v) {
-                    const float4 p = tris[3 * (size_t)t + v];
-                    l[0] = fminf(l[0], p.x); l[1] = fminf(l[1], p.y); l[2] = fminf(l[2], p.z);
-                    h[0] = fmaxf(h[0], p.x); h[1] = fmaxf(h[1], p.y); h[2] = fmaxf(h[2], p.z);
-                }
-        } else {
-            const Node4 cn = nodes[r];
+    Node4* nd = nodes + first + i;
+    const uint32_t r = reinterpret_cast<const uint32_t*>(&nd->ref)[k];
+    float l0 = INFINITY, l1 = INFINITY, l2 = INFINITY, h0 = -INFINITY, h1 = -INFINITY, h2 = -INFINITY;
+    if (r == kEmptyRef) {
+        l0 = l1 = l2 = h0 = h1 = h2 = 0.0f;
+    } else if (ref_is_leaf(r)) {
+        const uint32_t f = leaf_first(r), n = leaf_count(r);
+        for (uint32_t t0 = 0; t0 < n; t0 += 4) {
+            float4 p[12];
+#pragma unroll
             for (int q = 0; q < 4; ++q) {
-                if (u4_get(cn.ref, q) == kEmptyRef) continue;
-                l[0] = fminf(l[0], f4_get(cn.lox, q)); h[0] = fmaxf(h[0], f4_get(cn.hix, q));
-                l[1] = fminf(l[1], f4_get(cn.loy, q)); h[1] = fmaxf(h[1], f4_get(cn.hiy, q));
-                l[2] = fminf(l[2], f4_get(cn.loz, q)); h[2] = fmaxf(h[2], f4_get(cn.hiz, q));
+                const size_t t = f + min(t0 + (uint32_t)q, n - 1u);  // past the leaf: its last triangle again
+                p[3 * q] = tris[3 * t];
+                p[3 * q + 1] = tris[3 * t + 1];
+                p[3 * q + 2] = tris[3 * t + 2];
+            }
+#pragma unroll
+            for (int v = 0; v < 12; ++v) {
+                l0 = fminf(l0, p[v].x); l1 = fminf(l1, p[v].y); l2 = fminf(l2, p[v].z);
+                h0 = fmaxf(h0, p[v].x); h1 = fmaxf(h1, p[v].y); h2 = fmaxf(h2, p[v].z);
             }
         }
-        for (int a = 0; a < 3; ++a) { lo[a][k] = l[a]; hi[a][k] = h[a]; }
+    } else {
+        const Node4 cn = nodes[r];
+        for (int q = 0; q < 4; ++q) {
+            if (u4_get(cn.ref, q) == kEmptyRef) continue;
+            l0 = fminf(l0, f4_get(cn.lox, q)); h0 = fmaxf(h0, f4_get(cn.hix, q));
+            l1 = fminf(l1, f4_get(cn.loy, q)); h1 = fmaxf(h1, f4_get(cn.hiy, q));
+            l2 = fminf(l2, f4_get(cn.loz, q)); h2 = fmaxf(h2, f4_get(cn.hiz, q));
+        }
     }
-    nd.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
-    nd.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
-    nd.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
-    nd.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
-    nd.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
-    nd.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
-    nodes[first + i] = nd;
+    reinterpret_cast<float*>(&nd->lox)[k] = l0;
+    reinterpret_cast<float*>(&nd->loy)[k] = l1;
+    reinterpret_cast<float*>(&nd->loz)[k] = l2;
+    reinterpret_cast<float*>(&nd->hix)[k] = h0;
+    reinterpret_cast<float*>(&nd->hiy)[k] = h1;
+    reinterpret_cast<float*>(&nd->hiz)[k] = h2;
 }
 
 // The root's box (the union of its children) for the BLAS box kept on the host.
@@ -1893,16 +1905,29 @@ __global__ void k_emit_leaf_root4(const BNodeG* nodes, Node4* out) {
     out[0] = o;
 }
 
-__global__ void k_emit_tris(size_t nt, const uint32_t* prims, const uint32_t* idx, const GVertex* V, float4* tris) {
-    size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nt) return;
-    uint32_t p = prims[s];
-    const GVertex& a = V[idx[3 * (size_t)p]];
-    const GVertex& b = V[idx[3 * (size_t)p + 1]];
-    const GVertex& c = V[idx[3 * (size_t)p + 2]];
-    tris[3 * s] = make_float4(a.px, a.py, a.pz, __uint_as_float(p));
-    tris[3 * s + 1] = make_float4(b.px, b.py, b.pz, 0.f);
-    tris[3 * s + 2] = make_float4(c.px, c.py, c.pz, 0.f);
+// Triangles in leaf order, 48 B each (three float4 of vertex positions, the
+// first carrying the triangle id).  A block stages its kEmitThreads triangles
+// in LDS and writes them as one contiguous run, so each store instruction
+// covers 1 KB of consecutive addresses instead of 16 B pieces 48 B apart.
+constexpr int kEmitThreads = 256;
+__global__ __launch_bounds__(kEmitThreads) void k_emit_tris(size_t nt, const uint32_t* prims, const uint32_t* idx,
+                                                           const GVertex* V, float4* tris) {
+    __shared__ float4 st[3 * kEmitThreads];
+    const size_t base = (size_t)blockIdx.x * kEmitThreads;
+    const size_t s = base + threadIdx.x;
+    if (s < nt) {
+        const uint32_t p = prims[s];
+        const GVertex& a = V[idx[3 * (size_t)p]];
+        const GVertex& b = V[idx[3 * (size_t)p + 1]];
+        const GVertex& c = V[idx[3 * (size_t)p + 2]];
+        st[3 * threadIdx.x] = make_float4(a.px, a.py, a.pz, __uint_as_float(p));
+        st[3 * threadIdx.x + 1] = make_float4(b.px, b.py, b.pz, 0.f);
+        st[3 * threadIdx.x + 2] = make_float4(c.px, c.py, c.pz, 0.f);
+    }
+    block_sync();
+    const uint32_t n = 3u * (uint32_t)min((size_t)kEmitThreads, nt - base);
+    float4* out = tris + 3 * base;
+    for (uint32_t k = threadIdx.x; k < n; k += kEmitThreads) out[k] = st[k];
 }
 
 // Tree statistics (internal / leaf counts, largest leaf, SAH cost): a fixed
@@ -2042,7 +2067,7 @@ void blas_refit(Ctx& c, MeshG& m) {
     for (size_t l = B.level_start.size() - 1; l-- > 0;) {
         const uint32_t first = B.level_start[l], count = B.level_start[l + 1] - first;
         if (count == 0) continue;
-        k_refit4<<<blocks_for(count, 256), 256, 0, c.stream>>>(B.nodes.as<Node4>(), first, count,
+        k_refit4<<<blocks_for(4 * (size_t)count, kRefitThreads), kRefitThreads, 0, c.stream>>>(B.nodes.as<Node4>(), first, count,
                                                                B.tris.as<float4>());
         VG_HIP(hipGetLastError());
     }
@@ -2312,7 +2337,7 @@ void blas_build_grid(Ctx& c, MeshG& m, int max_leaf) {
                                                                 prims.as<uint32_t>());
     VG_HIP(hipGetLastError());
     B.tris.alloc(N * 3 * sizeof(float4));
-    k_emit_tris<<<blocks_for(N, 256), 256, 0, c.stream>>>(N, prims.as<uint32_t>(), m.idx.as<uint32_t>(),
+    k_emit_tris<<<blocks_for(N, kEmitThreads), kEmitThreads, 0, c.stream>>>(N, prims.as<uint32_t>(), m.idx.as<uint32_t>(),
                                                            m.verts.as<GVertex>(), B.tris.as<float4>());
     VG_HIP(hipGetLastError());
     // levels of the complete four-wide tree, bottom up, then numbered from the root
@@ -2344,7 +2369,7 @@ void blas_build_grid(Ctx& c, MeshG& m, int max_leaf) {
                                                               B.nodes.as<Node4>());
     VG_HIP(hipGetLastError());
     for (int l = L.n - 1; l >= 0; --l) {
-        k_refit4<<<blocks_for(L.count[l], 256), 256, 0, c.stream>>>(B.nodes.as<Node4>(), L.first[l], L.count[l],
+        k_refit4<<<blocks_for(4 * (size_t)L.count[l], kRefitThreads), kRefitThreads, 0, c.stream>>>(B.nodes.as<Node4>(), L.first[l], L.count[l],
                                                                     B.tris.as<float4>());
         VG_HIP(hipGetLastError());
     }
@@ -2432,7 +2457,7 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
     // depth), 2 (builder error flag; word 1 later becomes the emit error flag).
     constexpr int kMaxLevels = 256;
     const size_t tasks_cap = N / (kSmall + 1) + 1, chunks_cap = N / kChunk + tasks_cap + 1;
-    const size_t ssum_words = scan_sums_words(std::max(tasks_cap, chunks_cap));
+    const size_t ssum_words = 2 * scan_sums_words(std::max(tasks_cap, chunks_cap));  // room for a pair of scans
     Span pbuf, prims, prims2, nodes, keys, slot_bin;  // prims2: the second id buffer of the large levels
     Span d_task[2], d_params, d_nchunk, d_chunk0, d_chunks, d_bins, d_counts, d_bases, d_left, d_lscan, d_sweep,
         d_nlarge, d_nsmall, d_loff, d_soff, d_small, d_cnt, d_ssum;
@@ -2579,10 +2604,8 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
             d_chunks.as<Chunk>(), cnt + LC_NCH, d_params.as<TaskParam>(), d_bases.as<ChunkBase>(), prims.as<uint32_t>(),
             prims2.as<uint32_t>(), slot_bin.as<uint8_t>());
         VG_HIP(hipGetLastError());
-        scan_exclusive_u32_dev(c, d_nlarge.as<uint32_t>(), d_loff.as<uint32_t>(), d_nt, tasks_cap, ssum, ssum_words,
-                               d_ntn);
-        scan_exclusive_u32_dev(c, d_nsmall.as<uint32_t>(), d_soff.as<uint32_t>(), d_nt, tasks_cap, ssum,
-                               ssum_words, cnt + LC_LSMALL);
+        scan_exclusive_u32x2_dev(c, d_nlarge.as<uint32_t>(), d_loff.as<uint32_t>(), d_ntn, d_nsmall.as<uint32_t>(),
+                                 d_soff.as<uint32_t>(), cnt + LC_LSMALL, d_nt, tasks_cap, ssum, ssum_words);
         k_lv_emit<<<task_blocks, 256, 0, c.stream>>>(cur, d_nt, d_params.as<TaskParam>(), d_sweep.as<SweepOut>(),
                                                      d_nlarge.as<uint32_t>(), d_nsmall.as<uint32_t>(),
                                                      d_loff.as<uint32_t>(),
@@ -2621,7 +2644,7 @@ void blas_build(Ctx& c, MeshG& m, const vira_gpu_bvh_opts& opt) {
         VG_HIP(hipEventRecord(ready, c.stream));
         VG_HIP(hipStreamWaitEvent(c.aux_stream, ready, 0));
         B.tris.alloc(N * 3 * sizeof(float4));
-        k_emit_tris<<<blocks_for(N, 256), 256, 0, c.aux_stream>>>(N, prims.as<uint32_t>(), m.idx.as<uint32_t>(),
+        k_emit_tris<<<blocks_for(N, kEmitThreads), kEmitThreads, 0, c.aux_stream>>>(N, prims.as<uint32_t>(), m.idx.as<uint32_t>(),
                                                                   m.verts.as<GVertex>(), B.tris.as<float4>());
         VG_HIP(hipGetLastError());
         VG_HIP(hipEventRecord(tris_done, c.aux_stream));

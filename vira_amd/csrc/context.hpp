@@ -337,5 +337,10 @@ uint32_t scan_exclusive_u32(Ctx& c, const uint32_t* in, uint32_t* out, size_t n,
 size_t scan_sums_words(size_t n_max);
 void scan_exclusive_u32_dev(Ctx& c, const uint32_t* in, uint32_t* out, const uint32_t* dn, size_t n_max,
                             uint32_t* sums, size_t sums_words, uint32_t* total);
+// Two such scans over the same device count in one set of launches; sums:
+// sums_words >= 2 * scan_sums_words(n_max) (checked).
+void scan_exclusive_u32x2_dev(Ctx& c, const uint32_t* in0, uint32_t* out0, uint32_t* total0, const uint32_t* in1,
+                              uint32_t* out1, uint32_t* total1, const uint32_t* dn, size_t n_max, uint32_t* sums,
+                              size_t sums_words);
 
 }  // namespace vg
