@@ -2195,6 +2195,61 @@ __global__ void k_grid_topo(GridLevels L, uint32_t total, uint32_t nt, uint32_t 
     nodes[i] = o;
 }
 
+// The grid build's leaf-order triangles and its bottom level in one pass.  Leaf
+// c holds triangles [c * leaf, min((c + 1) * leaf, nt)) and bottom node k its
+// leaves 4k .. 4k + 3 (leaf divides kEmitThreads / 4, so a block covers whole
+// nodes).  The block stages its triangles in LDS as k_emit_tris does, writes
+// them as one run, and one thread per leaf then reduces the leaf's box from
+// LDS and writes that child of its node: bounds, reference (empty past the
+// last leaf, with a zero box, as k_grid_topo and k_refit4 leave one).
+__global__ __launch_bounds__(kEmitThreads) void k_grid_emit(size_t nt, const uint32_t* prims, const uint32_t* idx,
+                                                           const GVertex* V, float4* tris, Node4* __restrict__ bottom,
+                                                           uint32_t nbottom, uint32_t leaf, uint32_t nleaves) {
+    __shared__ float4 st[3 * kEmitThreads];
+    const size_t base = (size_t)blockIdx.x * kEmitThreads;
+    const size_t s = base + threadIdx.x;
+    if (s < nt) {
+        const uint32_t p = prims[s];
+        const GVertex& a = V[idx[3 * (size_t)p]];
+        const GVertex& b = V[idx[3 * (size_t)p + 1]];
+        const GVertex& c = V[idx[3 * (size_t)p + 2]];
+        st[3 * threadIdx.x] = make_float4(a.px, a.py, a.pz, __uint_as_float(p));
+        st[3 * threadIdx.x + 1] = make_float4(b.px, b.py, b.pz, 0.f);
+        st[3 * threadIdx.x + 2] = make_float4(c.px, c.py, c.pz, 0.f);
+    }
+    block_sync();
+    const uint32_t nblk = (uint32_t)min((size_t)kEmitThreads, nt - base);
+    float4* out = tris + 3 * base;
+    for (uint32_t k = threadIdx.x; k < 3u * nblk; k += kEmitThreads) out[k] = st[k];
+    if (threadIdx.x >= kEmitThreads / leaf) return;
+    const uint32_t c = (uint32_t)(base / leaf) + threadIdx.x;  // this thread's leaf
+    const uint32_t node = c >> 2, slot = c & 3u;
+    if (node >= nbottom) return;
+    Node4* nd = bottom + node;
+    float l0 = 0.f, l1 = 0.f, l2 = 0.f, h0 = 0.f, h1 = 0.f, h2 = 0.f;
+    uint32_t r = kEmptyRef;
+    if (c < nleaves) {
+        const uint32_t f = c * leaf, n = (uint32_t)min((size_t)leaf, nt - f);
+        l0 = l1 = l2 = INFINITY;
+        h0 = h1 = h2 = -INFINITY;
+        const float4* q = st + 3 * (f - (uint32_t)base);
+        for (uint32_t v = 0; v < 3u * n; ++v) {
+            const float4 x = q[v];
+            l0 = fminf(l0, x.x); l1 = fminf(l1, x.y); l2 = fminf(l2, x.z);
+            h0 = fmaxf(h0, x.x); h1 = fmaxf(h1, x.y); h2 = fmaxf(h2, x.z);
+        }
+        r = make_leaf(f, n);
+    }
+    reinterpret_cast<float*>(&nd->lox)[slot] = l0;
+    reinterpret_cast<float*>(&nd->loy)[slot] = l1;
+    reinterpret_cast<float*>(&nd->loz)[slot] = l2;
+    reinterpret_cast<float*>(&nd->hix)[slot] = h0;
+    reinterpret_cast<float*>(&nd->hiy)[slot] = h1;
+    reinterpret_cast<float*>(&nd->hiz)[slot] = h2;
+    reinterpret_cast<uint32_t*>(&nd->ref)[slot] = r;
+    reinterpret_cast<uint32_t*>(&nd->pad)[slot] = 0u;
+}
+
 // ---------------------------------------------------------------- quantised copy
 // Node4 -> Node4Q.  Per node and axis the origin is the lowest child bound and
 // the scale the smallest power of two with which 255 steps reach the highest
@@ -2336,10 +2391,6 @@ void blas_build_grid(Ctx& c, MeshG& m, int max_leaf) {
     k_grid_order<<<blocks_for(nslots, 256), 256, 0, c.stream>>>(nslots, slot.as<uint32_t>(), rank.as<uint32_t>(),
                                                                 prims.as<uint32_t>());
     VG_HIP(hipGetLastError());
-    B.tris.alloc(N * 3 * sizeof(float4));
-    k_emit_tris<<<blocks_for(N, kEmitThreads), kEmitThreads, 0, c.stream>>>(N, prims.as<uint32_t>(), m.idx.as<uint32_t>(),
-                                                           m.verts.as<GVertex>(), B.tris.as<float4>());
-    VG_HIP(hipGetLastError());
     // levels of the complete four-wide tree, bottom up, then numbered from the root
     uint32_t leaf = 1;
     while (leaf * 2u <= (uint32_t)max_leaf && leaf < 8u) leaf *= 2u;
@@ -2365,10 +2416,20 @@ void blas_build_grid(Ctx& c, MeshG& m, int max_leaf) {
     B.width = 4;
     B.num_nodes = total;
     B.nodes.alloc((size_t)total * sizeof(Node4));
-    k_grid_topo<<<blocks_for(total, 256), 256, 0, c.stream>>>(L, total, (uint32_t)N, leaf, nleaves,
-                                                              B.nodes.as<Node4>());
+    // the levels above the bottom get their references here; the bottom level, its leaves'
+    // boxes and the leaf-order triangles come from one pass over the triangles
+    const uint32_t upper = L.first[L.n - 1];
+    if (upper > 0) {
+        k_grid_topo<<<blocks_for(upper, 256), 256, 0, c.stream>>>(L, upper, (uint32_t)N, leaf, nleaves,
+                                                                  B.nodes.as<Node4>());
+        VG_HIP(hipGetLastError());
+    }
+    B.tris.alloc(N * 3 * sizeof(float4));
+    k_grid_emit<<<blocks_for(N, kEmitThreads), kEmitThreads, 0, c.stream>>>(
+        N, prims.as<uint32_t>(), m.idx.as<uint32_t>(), m.verts.as<GVertex>(), B.tris.as<float4>(),
+        B.nodes.as<Node4>() + upper, L.count[L.n - 1], leaf, nleaves);
     VG_HIP(hipGetLastError());
-    for (int l = L.n - 1; l >= 0; --l) {
+    for (int l = L.n - 2; l >= 0; --l) {
         k_refit4<<<blocks_for(4 * (size_t)L.count[l], kRefitThreads), kRefitThreads, 0, c.stream>>>(B.nodes.as<Node4>(), L.first[l], L.count[l],
                                                                     B.tris.as<float4>());
         VG_HIP(hipGetLastError());
