@@ -2092,12 +2092,14 @@ void blas_refit(Ctx& c, MeshG& m) {
 // (i, j+1), (i+1, j+1) (image_utils.ipp:364-412), so its cell and half follow
 // from two of its vertex ids.  The grid build orders the triangles by a
 // space-filling key of their cells (Morton over the low bits of both axes, the
-// longer axis's remaining bits above), cuts the sequence into leaves of
-// max_leaf triangles and puts a complete four-wide tree over the leaves: node
-// k of a level holds items 4k..4k+3 of the level below, so on a square
-// power-of-two grid every node is a 2x2 block of its children.  The boxes are
-// reduced bottom up by the refit kernels.  Every step is one pass over the
-// triangles or the nodes; nothing returns to the host between levels.
+// longer axis's remaining bits above): each triangle fills its key's slot and
+// one stream compaction of the slots gives the order.  The sequence is cut into
+// leaves of max_leaf triangles under a complete four-wide tree: node k of a
+// level holds items 4k..4k+3 of the level below, so on a square power-of-two
+// grid every node is a 2x2 block of its children.  One pass writes the
+// leaf-order triangles with the bottom level's boxes; the levels above are
+// reduced by the refit kernels.  Every step is one pass over the triangles,
+// the slots or the nodes; nothing returns to the host between levels.
 struct GridKey {
     uint32_t W, H;    // samples
     int kx, ky, kmin; // ceil(log2) of the cell counts, and their minimum
@@ -2148,17 +2150,6 @@ __global__ void k_grid_slots(size_t nt, const uint32_t* __restrict__ idx, GridKe
         return;
     }
     slot[p] = (uint32_t)t;
-}
-
-__global__ void k_grid_flags(uint64_t n, const uint32_t* __restrict__ slot, uint32_t* __restrict__ flag) {
-    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n) flag[p] = slot[p] != 0xffffffffu ? 1u : 0u;
-}
-
-__global__ void k_grid_order(uint64_t n, const uint32_t* __restrict__ slot, const uint32_t* __restrict__ rank,
-                             uint32_t* __restrict__ prims) {
-    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n && slot[p] != 0xffffffffu) prims[rank[p]] = slot[p];
 }
 
 // Levels of the complete four-wide tree, root level first: level l holds
@@ -2370,10 +2361,8 @@ void blas_build_grid(Ctx& c, MeshG& m, int max_leaf) {
     VG_CHECK(g.kx <= 16 && g.ky <= 16, VIRA_GPU_ERR_INVALID, "grid BVH: more than 65536 cells on an axis");
     const uint64_t nslots = 2ull << (g.kx + g.ky);
     VG_CHECK(nslots < (1ull << 31), VIRA_GPU_ERR_INVALID, "grid BVH: key space too large");
-    DBuf slot, flag, rank, prims, err;
+    DBuf slot, prims, err;
     slot.alloc(nslots * 4);
-    flag.alloc(nslots * 4);
-    rank.alloc(nslots * 4);
     prims.alloc(N * 4);
     err.alloc(4);
     VG_HIP(hipMemsetAsync(slot.p, 0xff, nslots * 4, c.stream));
@@ -2381,16 +2370,12 @@ void blas_build_grid(Ctx& c, MeshG& m, int max_leaf) {
     k_grid_slots<<<blocks_for(N, 256), 256, 0, c.stream>>>(N, m.idx.as<uint32_t>(), g, slot.as<uint32_t>(), nslots,
                                                            err.as<uint32_t>());
     VG_HIP(hipGetLastError());
-    k_grid_flags<<<blocks_for(nslots, 256), 256, 0, c.stream>>>(nslots, slot.as<uint32_t>(), flag.as<uint32_t>());
-    VG_HIP(hipGetLastError());
-    const uint32_t placed = scan_exclusive_u32(c, flag.as<uint32_t>(), rank.as<uint32_t>(), nslots);
+    // the placed triangles in key order (each triangle fills at most one slot, so at most N are placed)
+    const uint32_t placed = compact_u32(c, slot.as<uint32_t>(), prims.as<uint32_t>(), nslots);
     uint32_t herr = 0;
     VG_HIP(hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, c.stream));
     VG_HIP(hipStreamSynchronize(c.stream));
     VG_CHECK(herr == 0 && placed == N, VIRA_GPU_ERR_INVALID, "grid BVH: the mesh's triangles are not its grid's cells");
-    k_grid_order<<<blocks_for(nslots, 256), 256, 0, c.stream>>>(nslots, slot.as<uint32_t>(), rank.as<uint32_t>(),
-                                                                prims.as<uint32_t>());
-    VG_HIP(hipGetLastError());
     // levels of the complete four-wide tree, bottom up, then numbered from the root
     uint32_t leaf = 1;
     while (leaf * 2u <= (uint32_t)max_leaf && leaf < 8u) leaf *= 2u;

@@ -331,6 +331,9 @@ size_t noalloc_max_tiles();
 // Exclusive scan of n uint32 values (in place allowed); returns the total after
 // a stream synchronisation, or 0 without waiting when total_wanted is false.
 uint32_t scan_exclusive_u32(Ctx& c, const uint32_t* in, uint32_t* out, size_t n, bool total_wanted = true);
+// Stream compaction: the values of in[0..n) other than 0xffffffff, in order, to
+// out (not in place); returns their count after a stream synchronisation.
+uint32_t compact_u32(Ctx& c, const uint32_t* in, uint32_t* out, size_t n);
 // The same with the element count in device memory (*dn <= n_max) and the total
 // written to device memory; sums: scratch of sums_words >= scan_sums_words(n_max)
 // words (checked).

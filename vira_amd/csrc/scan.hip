@@ -47,7 +47,11 @@ struct ScanArgs {
     uint32_t* total[2];
 };
 
-// dn (optional): the element count in device memory, at most n (the grid is sized for n)
+constexpr uint32_t kNone = 0xffffffffu;  // COMPACT: the value that is dropped
+
+// dn (optional): the element count in device memory, at most n (the grid is sized for n).
+// COMPACT: the scanned value is 1 for an element other than kNone, else 0.
+template <bool COMPACT>
 __global__ void __launch_bounds__(kScanBlock) k_scan_reduce(ScanArgs a, size_t n, uint32_t* sums, size_t sstride,
                                                             const uint32_t* dn) {
     __shared__ uint32_t lds[kScanBlock / 64 + 1];
@@ -57,7 +61,7 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_reduce(ScanArgs a, size_t n
     uint32_t s = 0;
     for (int k = 0; k < kScanItems; ++k) {
         size_t i = base + (size_t)k * kScanBlock + threadIdx.x;
-        if (i < n) s += in[i];
+        if (i < n) s += COMPACT ? (in[i] != kNone ? 1u : 0u) : in[i];
     }
     uint32_t total;
     (void)block_exclusive_scan(s, lds, total);
@@ -88,6 +92,9 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_sums(ScanArgs a, uint32_t* 
     if (threadIdx.x == 0) *a.total[blockIdx.x] = carry;
 }
 
+// COMPACT: each element other than kNone is written to out[its rank] instead of
+// its prefix being written to out[i].
+template <bool COMPACT>
 __global__ void __launch_bounds__(kScanBlock) k_scan_apply(ScanArgs a, size_t n, const uint32_t* sums,
                                                            size_t sstride, const uint32_t* dn) {
     __shared__ uint32_t lds[kScanBlock / 64 + 1];
@@ -106,9 +113,20 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_apply(ScanArgs a, size_t n,
     } else {
         for (int k = 0; k < kScanItems; ++k) v[k] = base + k < n ? in[base + k] : 0;
     }
-    for (int k = 0; k < kScanItems; ++k) s += v[k];
+    uint32_t f[kScanItems];  // the scanned values
+    for (int k = 0; k < kScanItems; ++k) {
+        f[k] = COMPACT ? (base + k < n && v[k] != kNone ? 1u : 0u) : v[k];
+        s += f[k];
+    }
     uint32_t total;
     uint32_t e = block_exclusive_scan(s, lds, total) + sums[blockIdx.y * sstride + blockIdx.x];
+    if (COMPACT) {
+        for (int k = 0; k < kScanItems; ++k) {
+            if (f[k]) out[e] = v[k];
+            e += f[k];
+        }
+        return;
+    }
     uint32_t o[kScanItems];
     for (int k = 0; k < kScanItems; ++k) {
         o[k] = e;
@@ -124,14 +142,15 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_apply(ScanArgs a, size_t n,
 }
 
 // Queues the reduce / sums / apply launches for `arrays` scans of n elements.
+template <bool COMPACT = false>
 void scan_launch(Ctx& c, const ScanArgs& a, int arrays, size_t n, uint32_t* sums, size_t sstride,
                  const uint32_t* dn) {
     const size_t nb = (n + kScanTile - 1) / kScanTile;
-    k_scan_reduce<<<dim3((unsigned)nb, (unsigned)arrays), kScanBlock, 0, c.stream>>>(a, n, sums, sstride, dn);
+    k_scan_reduce<COMPACT><<<dim3((unsigned)nb, (unsigned)arrays), kScanBlock, 0, c.stream>>>(a, n, sums, sstride, dn);
     VG_HIP(hipGetLastError());
     k_scan_sums<<<(unsigned)arrays, kScanBlock, 0, c.stream>>>(a, sums, nb, sstride);
     VG_HIP(hipGetLastError());
-    k_scan_apply<<<dim3((unsigned)nb, (unsigned)arrays), kScanBlock, 0, c.stream>>>(a, n, sums, sstride, dn);
+    k_scan_apply<COMPACT><<<dim3((unsigned)nb, (unsigned)arrays), kScanBlock, 0, c.stream>>>(a, n, sums, sstride, dn);
     VG_HIP(hipGetLastError());
 }
 
@@ -149,6 +168,20 @@ uint32_t scan_exclusive_u32(Ctx& c, const uint32_t* in, uint32_t* out, size_t n,
     const ScanArgs a{{in, nullptr}, {out, nullptr}, {s + nb, nullptr}};
     scan_launch(c, a, 1, n, s, nb + 1, nullptr);
     if (!total_wanted) return 0;  // the scan stays queued on the stream
+    uint32_t* total = c.scan_total.as<uint32_t>();
+    VG_HIP(hipMemcpyAsync(total, s + nb, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+    VG_HIP(hipStreamSynchronize(c.stream));
+    return *total;
+}
+
+uint32_t compact_u32(Ctx& c, const uint32_t* in, uint32_t* out, size_t n) {
+    if (n == 0) return 0;
+    size_t nb = (n + kScanTile - 1) / kScanTile;
+    c.scan_sums.alloc((nb + 1) * sizeof(uint32_t));
+    c.scan_total.alloc(sizeof(uint32_t));
+    uint32_t* s = c.scan_sums.as<uint32_t>();
+    const ScanArgs a{{in, nullptr}, {out, nullptr}, {s + nb, nullptr}};
+    scan_launch<true>(c, a, 1, n, s, nb + 1, nullptr);
     uint32_t* total = c.scan_total.as<uint32_t>();
     VG_HIP(hipMemcpyAsync(total, s + nb, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
     VG_HIP(hipStreamSynchronize(c.stream));
